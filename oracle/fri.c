@@ -1,0 +1,512 @@
+/*
+ * fri.c -- oracle restatement of the SHA3 transcript, the Merkle channel and
+ * the DEEP-FRI / STIR prover with its ark-serialize wire format.
+ *
+ * Test infrastructure only (the oracle); the product path never calls it.
+ *
+ *   transcript : crates/transcript/src/lib.rs:13-17,162-212,338-344,427-446
+ *   merkle     : crates/merkle/src/lib.rs:95-113,185-238,377-512
+ *   fri        : crates/deep_ali/src/fri.rs:73-85 (safe_field_challenge),
+ *                :123-130 (tr_hash_fields_tagged), :338-391 (OOD coset),
+ *                :670-771 (coset coefficients, fold), :874-905 (ext coeffs),
+ *                :950-959 (merkle_depth), :1044-1103 (indices, tree configs),
+ *                :1105-1129 (statement binding), :1372-1393 (ext challenges),
+ *                :1398-1718 (transcript builder), :1813-1893 (STIR queries),
+ *                :1900-2055 (deep_fri_prove)
+ *   wire       : fri.rs:1235-1300 + ark-serialize 0.4.2 compressed
+ */
+#include "refcpu.h"
+#include "refcpu_int.h"
+#include <stdio.h>
+#include <stdlib.h>
+
+/* ------------------------------ transcript ------------------------------ */
+typedef struct { sha3_ctx h; } tscript;
+
+static void tr_init(tscript *t, const char *label) {
+    sha3_init(&t->h);
+    sha3_update(&t->h, "FSv1-TRANSCRIPT-INIT", 20);
+    sha3_update(&t->h, label, strlen(label));
+}
+static void tr_absorb(tscript *t, const void *b, size_t n) {
+    sha3_update(&t->h, "FSv1-ABSORB-BYTES", 17);
+    sha3_update(&t->h, b, n);
+}
+static void tr_absorb_field(tscript *t, uint64_t x) {
+    uint8_t le[8];
+    for (int i = 0; i < 8; i++) le[i] = (uint8_t)(x >> (8 * i));
+    tr_absorb(t, le, 8);
+}
+static void tr_absorb_fp6(tscript *t, fp6 v) { for (int i = 0; i < 6; i++) tr_absorb_field(t, v.c[i]); }
+/* challenge hashes a clone: the state is not modified */
+static void tr_challenge_bytes(const tscript *t, const void *label, size_t ln, uint8_t out[32]) {
+    sha3_ctx c = t->h;
+    sha3_update(&c, "FSv1-CHALLENGE", 14);
+    sha3_update(&c, label, ln);
+    sha3_final(&c, out);
+}
+static uint64_t safe_field_challenge(const tscript *t, const void *label, size_t ln) {
+    uint8_t d[32];
+    tr_challenge_bytes(t, label, ln, d);
+    /* rchunks(7): chunks taken from the end, the last one may be short */
+    uint64_t acc = 0;
+    int end = 32;
+    while (end > 0) {
+        int start = end - 7 < 0 ? 0 : end - 7;
+        int len = end - start;
+        uint64_t val = 0;
+        for (int i = 0; i < len; i++) val |= (uint64_t)d[start + i] << (8 * i);
+        acc = gl_add(gl_mul(acc, (uint64_t)1 << (8 * len)), val);
+        end = start;
+    }
+    return acc;
+}
+static fp6 challenge_ext(const tscript *t, const char *tag) {
+    fp6 r;
+    char sub[64];
+    for (int i = 0; i < 6; i++) {
+        int n = snprintf(sub, sizeof sub, "%s/c%d", tag, i);
+        r.c[i] = safe_field_challenge(t, sub, (size_t)n);
+    }
+    return r;
+}
+static uint64_t tr_hash_fields_tagged(const char *tag, const uint64_t *f, size_t n) {
+    tscript t;
+    tr_init(&t, "FRI/FS");
+    tr_absorb(&t, tag, strlen(tag));
+    for (size_t i = 0; i < n; i++) tr_absorb_field(&t, f[i]);
+    return safe_field_challenge(&t, "out", 3);
+}
+
+uint64_t ref_stdrng_first_u64(uint64_t seed) {
+    uint8_t key[32] = {0}, nonce[12] = {0};
+    uint32_t blk[16];
+    for (int i = 0; i < 8; i++) key[i] = (uint8_t)(seed >> (8 * i));
+    chacha_block(key, 0, nonce, 12, blk);
+    return (uint64_t)blk[0] | ((uint64_t)blk[1] << 32);
+}
+
+/* -------------------------------- merkle -------------------------------- */
+typedef struct {
+    size_t arity, depth;
+    uint64_t label;
+    size_t nlev;
+    uint8_t **lv;   /* lv[k]: digests of level k (after padding) */
+    size_t *len;
+} mtree;
+
+static void dslabel(uint8_t out[32], uint64_t arity, uint64_t level, uint64_t pos, uint64_t label) {
+    uint64_t v[4] = {arity, level, pos, label};
+    for (int k = 0; k < 4; k++)
+        for (int i = 0; i < 8; i++) out[8 * k + i] = (uint8_t)(v[k] >> (8 * i));
+}
+
+static void leaf_hash(size_t arity, uint64_t label, uint64_t pos, const uint64_t *vals, size_t nv, uint8_t out[32]) {
+    sha3_ctx c;
+    uint8_t ds[32];
+    dslabel(ds, arity, 0xFFFFFFFFULL, pos, label);
+    sha3_init(&c);
+    sha3_update(&c, ds, 32);
+    for (size_t i = 0; i < nv; i++) {
+        uint8_t le[8];
+        for (int b = 0; b < 8; b++) le[b] = (uint8_t)(vals[i] >> (8 * b));
+        sha3_update(&c, le, 8);
+    }
+    sha3_final(&c, out);
+}
+
+static void mt_init(mtree *t, size_t arity, size_t depth, uint64_t label, size_t n_leaves) {
+    t->arity = arity; t->depth = depth; t->label = label;
+    t->nlev = 1;
+    t->lv = (uint8_t **)calloc(depth + 2, sizeof(uint8_t *));
+    t->len = (size_t *)calloc(depth + 2, sizeof(size_t));
+    t->lv[0] = (uint8_t *)malloc(32 * (n_leaves + arity));
+    t->len[0] = n_leaves;
+}
+static void mt_free(mtree *t) {
+    for (size_t k = 0; k < t->nlev; k++) free(t->lv[k]);
+    free(t->lv); free(t->len);
+}
+/* finalize: levels padded by repeating the last node (merkle lib.rs:423-476) */
+static void mt_finalize(mtree *t, uint8_t root[32]) {
+    size_t level = 0;
+    while (t->len[level] > 1) {
+        if (level >= t->depth) { fprintf(stderr, "merkle: layer_arities exhausted\n"); abort(); }
+        size_t a = t->arity, cur = t->len[level];
+        if (cur % a) {
+            size_t pad = a - cur % a;
+            for (size_t i = 0; i < pad; i++) memcpy(t->lv[level] + 32 * (cur + i), t->lv[level] + 32 * (cur - 1), 32);
+            cur += pad;
+            t->len[level] = cur;
+        }
+        size_t np = cur / a;
+        t->lv[level + 1] = (uint8_t *)malloc(32 * (np + a));
+        for (size_t i = 0; i < np; i++) {
+            sha3_ctx c;
+            uint8_t ds[32];
+            dslabel(ds, a, level + 1, i, t->label);
+            sha3_init(&c);
+            sha3_update(&c, ds, 32);
+            sha3_update(&c, t->lv[level] + 32 * a * i, 32 * a);
+            sha3_final(&c, t->lv[level + 1] + 32 * i);
+        }
+        t->len[level + 1] = np;
+        t->nlev = level + 2;
+        level++;
+    }
+    memcpy(root, t->lv[t->nlev - 1], 32);
+}
+/* MerkleOpening serialized: leaf[32], path Vec<Vec<[32]>>, index u64 (merkle lib.rs:478-512) */
+static void mt_open_ser(const mtree *t, size_t index, bbuf *b) {
+    bb_put(b, t->lv[0] + 32 * index, 32);
+    bb_u64(b, t->nlev - 1);
+    size_t idx = index;
+    for (size_t level = 0; level + 1 < t->nlev; level++) {
+        size_t a = t->arity, gs = (idx / a) * a;
+        bb_u64(b, a - 1);
+        for (size_t i = 0; i < a; i++) {
+            size_t pos = gs + i;
+            if (pos == idx) continue;
+            size_t p2 = pos < t->len[level] ? pos : t->len[level] - 1;
+            bb_put(b, t->lv[level] + 32 * p2, 32);
+        }
+        idx /= a;
+    }
+    bb_u64(b, index);
+}
+
+static size_t pick_arity(size_t n, size_t m) {
+    static const size_t as[6] = {128, 64, 32, 16, 8, 4};
+    for (int i = 0; i < 6; i++)
+        if (m >= as[i] && n % as[i] == 0) return as[i];
+    return n % 2 == 0 ? 2 : 1;
+}
+static size_t merkle_depth(size_t leaves, size_t arity) {
+    size_t d = 1, cur = leaves;
+    while (cur > arity) { cur = (cur + arity - 1) / arity; d++; }
+    return d;
+}
+static size_t pick_arity2(size_t n, size_t m) { size_t a = pick_arity(n, m); return a < 2 ? 2 : a; }
+
+void ref_merkle_root(const uint64_t *leaf_vals, size_t n_leaves, size_t vpl, size_t arity, size_t depth, uint64_t label, uint8_t root[32]) {
+    mtree t;
+    mt_init(&t, arity, depth, label, n_leaves);
+    for (size_t i = 0; i < n_leaves; i++) leaf_hash(arity, label, i, leaf_vals + i * vpl, vpl, t.lv[0] + 32 * i);
+    mt_finalize(&t, root);
+    mt_free(&t);
+}
+
+/* ------------------------------ FRI pieces ------------------------------ */
+/* f(x) interpolated on <omega_n> (natural order), evaluated at m points
+ * zeta^j * z, zeta = omega^(n/m)  (fri.rs:338-391) */
+static void ood_coset_evals(const fp6 *f, size_t n, fp6 z, size_t m, fp6 *out) {
+    uint64_t *comp = (uint64_t *)malloc(sizeof(uint64_t) * n);
+    fp6 *coef = (fp6 *)malloc(sizeof(fp6) * n);
+    for (int c = 0; c < 6; c++) {
+        for (size_t i = 0; i < n; i++) comp[i] = f[i].c[c];
+        ref_ntt_inplace(comp, n, 1);
+        for (size_t i = 0; i < n; i++) coef[i].c[c] = comp[i];
+    }
+    uint64_t zeta = gl_pow(gl_root_of_unity(n), n / m);
+    fp6 zp = fp6_one();
+    for (size_t j = 0; j < m; j++) {
+        fp6 pt = fp6_mul(zp, z), acc = fp6_zero();
+        for (size_t k = n; k-- > 0;) acc = fp6_add(fp6_mul(acc, pt), coef[k]);
+        out[j] = acc;
+        zp = fp6_scale(zp, zeta);
+    }
+    free(comp); free(coef);
+}
+
+/* per coset b: monomial coefficients of the degree-(m-1) interpolant of
+ * f[b + j n/m] on the points omega^b zeta^j  (fri.rs:670-752) */
+static void coset_coeffs(const fp6 *f, size_t n, size_t m, fp6 *tuples /* (n/m) x m */) {
+    size_t nn = n / m;
+    uint64_t omega = gl_root_of_unity(n);
+    uint64_t zeta_inv = gl_inv(gl_pow(omega, nn));
+    uint64_t minv = gl_inv(m), omega_inv = gl_inv(omega);
+    uint64_t *zi = (uint64_t *)malloc(sizeof(uint64_t) * m);
+    zi[0] = 1;
+    for (size_t k = 1; k < m; k++) zi[k] = gl_mul(zi[k - 1], zeta_inv);
+    uint64_t obinv = 1; /* omega^{-b} */
+    for (size_t b = 0; b < nn; b++) {
+        uint64_t sh = minv; /* (1/m) * omega^{-bk} */
+        for (size_t k = 0; k < m; k++) {
+            fp6 acc = fp6_zero();
+            for (size_t j = 0; j < m; j++) acc = fp6_add(acc, fp6_scale(f[b + j * nn], zi[(j * k) % m]));
+            tuples[b * m + k] = fp6_scale(acc, sh);
+            sh = gl_mul(sh, obinv);
+        }
+        obinv = gl_mul(obinv, omega_inv);
+    }
+    free(zi);
+}
+
+static void fold_tuples(const fp6 *tuples, size_t nn, size_t m, fp6 alpha, fp6 *out) {
+    for (size_t b = 0; b < nn; b++) {
+        fp6 acc = fp6_zero(), ap = fp6_one();
+        for (size_t k = 0; k < m; k++) { acc = fp6_add(acc, fp6_mul(tuples[b * m + k], ap)); ap = fp6_mul(ap, alpha); }
+        out[b] = acc;
+    }
+}
+
+static void bind_statement(tscript *t, const ref_fri_params *p, size_t n0) {
+    if (p->public_inputs_hash) {
+        tr_absorb(t, "CAIRO-PUBLIC-INPUTS-V1", 22);
+        tr_absorb(t, p->public_inputs_hash, 32);
+    }
+    tr_absorb(t, "DEEP-FRI-STATEMENT-V2", 21);
+    tr_absorb_field(t, n0);
+    tr_absorb_field(t, p->n_layers);
+    for (size_t i = 0; i < p->n_layers; i++) tr_absorb_field(t, p->schedule[i]);
+    tr_absorb_field(t, gl_from_u64(p->seed_z));
+    tr_absorb_field(t, 6);
+    tr_absorb_field(t, p->coeff_commit_final ? 1 : 0);
+    tr_absorb_field(t, p->stir ? 1 : 0);
+}
+
+typedef struct {
+    size_t L, n0;
+    fp6 **f;            /* f[0..L]: layer values (f[0] = f0 lifted) */
+    size_t *sz;
+    uint8_t root_f0[32];
+    uint8_t (*roots)[32];
+    fp6 *fz;            /* L */
+    fp6 *coset_evals;   /* L x m_l, ragged: offsets */
+    size_t *ce_off;
+    fp6 *coeff_tuples;  /* final layer tuples */
+    size_t n_final_tuples, m_final;
+    uint8_t coeff_root[32];
+    int has_coeff;
+    fp6 final_coeffs[64];
+    size_t n_final_coeffs;
+    uint64_t query_seed;
+} fri_state;
+
+static void fs_free(fri_state *s) {
+    for (size_t i = 0; i <= s->L; i++) free(s->f[i]);
+    free(s->f); free(s->sz); free(s->roots); free(s->fz); free(s->coset_evals); free(s->ce_off); free(s->coeff_tuples);
+}
+
+static int fri_build(const uint64_t *f0, size_t n0, const ref_fri_params *p, fri_state *s) {
+    size_t L = p->n_layers;
+    if (!p->stir || L == 0) return -2; /* only the STIR path (production LdtMode::Stir) */
+    memset(s, 0, sizeof *s);
+    s->L = L; s->n0 = n0;
+    int use_cc = p->coeff_commit_final && L > 0;
+    size_t normal = use_cc ? L - 1 : L;
+    s->f = (fp6 **)calloc(L + 1, sizeof(fp6 *));
+    s->sz = (size_t *)calloc(L + 1, sizeof(size_t));
+    s->roots = (uint8_t(*)[32])calloc(L, 32);
+    s->fz = (fp6 *)calloc(L, sizeof(fp6));
+    s->ce_off = (size_t *)calloc(L + 1, sizeof(size_t));
+    size_t tot = 0, n = n0;
+    for (size_t l = 0; l < L; l++) {
+        if (p->schedule[l] < 2 || n % p->schedule[l]) return -1;
+        s->ce_off[l] = tot; tot += p->schedule[l]; n /= p->schedule[l];
+    }
+    s->ce_off[L] = tot;
+    s->coset_evals = (fp6 *)calloc(tot, sizeof(fp6));
+
+    tscript tr;
+    tr_init(&tr, "FRI/FS");
+    bind_statement(&tr, p, n0);
+
+    /* coset-packed f0 tree (fri.rs:1420-1431) */
+    size_t m0 = p->schedule[0], nn0 = n0 / m0;
+    {
+        size_t a = pick_arity2(nn0, 16), d = merkle_depth(nn0, a);
+        mtree t;
+        mt_init(&t, a, d, 0xFF, nn0);
+        uint64_t *fib = (uint64_t *)malloc(8 * m0);
+        for (size_t b = 0; b < nn0; b++) {
+            for (size_t j = 0; j < m0; j++) fib[j] = f0[b + j * nn0];
+            leaf_hash(a, 0xFF, b, fib, m0, t.lv[0] + 32 * b);
+        }
+        mt_finalize(&t, s->root_f0);
+        mt_free(&t); free(fib);
+    }
+    tr_absorb(&tr, s->root_f0, 32);
+    fp6 z = challenge_ext(&tr, "z_fp3");
+    uint8_t seed_bytes[32];
+    tr_challenge_bytes(&tr, "FRI/seed", 8, seed_bytes); /* trace_hash: unused by MerkleTreeChannel */
+
+    s->f[0] = (fp6 *)malloc(sizeof(fp6) * n0);
+    for (size_t i = 0; i < n0; i++) s->f[0][i] = fp6_from_base(f0[i]);
+    s->sz[0] = n0;
+    size_t cur = n0;
+    for (size_t ell = 0; ell < L; ell++) {
+        size_t m = p->schedule[ell], nn = cur / m;
+        int final_cc = use_cc && ell == L - 1;
+        fp6 alpha = fp6_zero();
+        if (!final_cc) alpha = challenge_ext(&tr, "alpha");
+        fp6 *ce = s->coset_evals + s->ce_off[ell];
+        ood_coset_evals(s->f[ell], cur, z, m, ce);
+        s->fz[ell] = ce[0];
+        z = fp6_pow(z, m);
+        fp6 *tuples = (fp6 *)malloc(sizeof(fp6) * cur);
+        coset_coeffs(s->f[ell], cur, m, tuples);
+        /* layer tree: STIR leaf = 6 components of f (fri.rs:1364-1366,1506-1516) */
+        {
+            size_t a = pick_arity2(cur, m), d = merkle_depth(cur, a);
+            mtree t;
+            mt_init(&t, a, d, ell, cur);
+            for (size_t i = 0; i < cur; i++) leaf_hash(a, ell, i, s->f[ell][i].c, 6, t.lv[0] + 32 * i);
+            mt_finalize(&t, s->roots[ell]);
+            mt_free(&t);
+        }
+        for (size_t j = 0; j < m; j++) tr_absorb_fp6(&tr, ce[j]);
+        tr_absorb(&tr, s->roots[ell], 32);
+        if (final_cc) {
+            /* coefficient-tuple tree, label 0xFE (fri.rs:860-868,1649-1664) */
+            size_t a = pick_arity2(nn, 16), d = merkle_depth(nn, a);
+            mtree t;
+            mt_init(&t, a, d, 0xFE, nn);
+            for (size_t b = 0; b < nn; b++) leaf_hash(a, 0xFE, b, (const uint64_t *)(tuples + b * m), 6 * m, t.lv[0] + 32 * b);
+            mt_finalize(&t, s->coeff_root);
+            mt_free(&t);
+            s->has_coeff = 1;
+            tr_absorb(&tr, s->coeff_root, 32);
+            alpha = challenge_ext(&tr, "alpha");
+            (void)challenge_ext(&tr, "beta_deg");
+        }
+        s->f[ell + 1] = (fp6 *)malloc(sizeof(fp6) * nn);
+        fold_tuples(tuples, nn, m, alpha, s->f[ell + 1]);
+        s->sz[ell + 1] = nn;
+        if (final_cc) { s->coeff_tuples = tuples; s->n_final_tuples = nn; s->m_final = m; }
+        else free(tuples);
+        cur = nn;
+    }
+    (void)normal;
+    /* final polynomial: first d_final coefficients of the last layer (fri.rs:1919-1937) */
+    size_t nf = s->sz[L];
+    {
+        fp6 *co = (fp6 *)malloc(sizeof(fp6) * nf);
+        uint64_t *comp = (uint64_t *)malloc(8 * nf);
+        for (int c = 0; c < 6; c++) {
+            for (size_t i = 0; i < nf; i++) comp[i] = s->f[L][i].c[c];
+            if (nf > 1) ref_ntt_inplace(comp, nf, 1);
+            for (size_t i = 0; i < nf; i++) co[i].c[c] = comp[i];
+        }
+        size_t d = p->d_final < nf ? p->d_final : nf;
+        if (d > 64) d = 64;
+        for (size_t i = 0; i < d; i++) s->final_coeffs[i] = co[i];
+        s->n_final_coeffs = d;
+        free(co); free(comp);
+    }
+    /* the replayed transcript equals this one: challenges never mutate it (fri.rs:1939-1993) */
+    for (size_t i = 0; i < s->n_final_coeffs; i++) tr_absorb_fp6(&tr, s->final_coeffs[i]);
+    s->query_seed = safe_field_challenge(&tr, "query_seed", 10);
+    return 0;
+}
+
+int ref_fri_layers(const uint64_t *f0, size_t n0, const ref_fri_params *p, uint64_t *layers_out, uint8_t *roots_out, uint8_t root_f0[32]) {
+    fri_state s;
+    int rc = fri_build(f0, n0, p, &s);
+    if (rc) return rc;
+    size_t off = 0;
+    for (size_t l = 1; l <= s.L; l++) {
+        memcpy(layers_out + off, s.f[l], sizeof(fp6) * s.sz[l]);
+        off += 6 * s.sz[l];
+    }
+    memcpy(roots_out, s.roots, 32 * s.L);
+    memcpy(root_f0, s.root_f0, 32);
+    fs_free(&s);
+    return 0;
+}
+
+int ref_deep_fri_prove(const uint64_t *f0, size_t n0, const ref_fri_params *p, uint8_t **out, size_t *out_len) {
+    fri_state s;
+    int rc = fri_build(f0, n0, p, &s);
+    if (rc) return rc;
+    size_t L = s.L;
+    bbuf b = {0};
+    bb_put(&b, s.root_f0, 32);
+    bb_u64(&b, L);
+    for (size_t l = 0; l < L; l++) bb_put(&b, s.roots[l], 32);
+    bb_u64(&b, 0); /* layer_proofs.layers */
+    bb_u64(&b, 0); /* f0_openings */
+    bb_u64(&b, 0); /* queries */
+    bb_u64(&b, L);
+    for (size_t l = 0; l < L; l++) bb_fp6(&b, s.fz[l]);
+    bb_u64(&b, s.n_final_coeffs);
+    for (size_t i = 0; i < s.n_final_coeffs; i++) bb_fp6(&b, s.final_coeffs[i]);
+    bb_u64(&b, n0);
+    bb_u64(&b, gl_root_of_unity(n0));
+    if (s.has_coeff) {
+        bb_u8(&b, 1);
+        bb_u64(&b, s.n_final_tuples);
+        for (size_t t = 0; t < s.n_final_tuples; t++) {
+            bb_u64(&b, s.m_final);
+            for (size_t k = 0; k < s.m_final; k++) bb_fp6(&b, s.coeff_tuples[t * s.m_final + k]);
+        }
+        bb_u8(&b, 1);
+        bb_put(&b, s.coeff_root, 32);
+    } else {
+        bb_u8(&b, 0);
+        bb_u8(&b, 0);
+    }
+    bb_u8(&b, 1);
+    bb_u64(&b, L);
+    for (size_t l = 0; l < L; l++) {
+        size_t m = p->schedule[l];
+        bb_u64(&b, m);
+        for (size_t j = 0; j < m; j++) bb_fp6(&b, s.coset_evals[s.ce_off[l] + j]);
+    }
+    /* STIR proximity queries (fri.rs:1813-1893): the f0 packed tree and the
+     * layer-1 tree are rebuilt exactly as committed. */
+    size_t m0 = p->schedule[0], nn0 = n0 / m0;
+    mtree t0, t1;
+    {
+        size_t a = pick_arity2(nn0, 16), d = merkle_depth(nn0, a);
+        uint8_t r[32];
+        mt_init(&t0, a, d, 0xFF, nn0);
+        uint64_t *fib = (uint64_t *)malloc(8 * m0);
+        for (size_t bb = 0; bb < nn0; bb++) {
+            for (size_t j = 0; j < m0; j++) fib[j] = f0[bb + j * nn0];
+            leaf_hash(a, 0xFF, bb, fib, m0, t0.lv[0] + 32 * bb);
+        }
+        mt_finalize(&t0, r);
+        free(fib);
+    }
+    if (L >= 2) {
+        size_t n1 = s.sz[1], a = pick_arity2(n1, p->schedule[1]), d = merkle_depth(n1, a);
+        uint8_t r[32];
+        mt_init(&t1, a, d, 1, n1);
+        for (size_t i = 0; i < n1; i++) leaf_hash(a, 1, i, s.f[1][i].c, 6, t1.lv[0] + 32 * i);
+        mt_finalize(&t1, r);
+    }
+    bb_u8(&b, 1);
+    bb_u64(&b, p->s0);
+    uint64_t npow2 = 1;
+    while (npow2 < n0) npow2 <<= 1;
+    for (size_t q = 0; q < p->s0; q++) {
+        uint64_t fl[3] = {s.query_seed, 0, gl_from_u64(q)};
+        uint64_t seed = tr_hash_fields_tagged("FRI/index", fl, 3);
+        uint64_t raw = (ref_stdrng_first_u64(seed) & (npow2 - 1)) % n0;
+        uint64_t base = raw % nn0;
+        bb_u64(&b, base);
+        bb_u64(&b, raw);
+        bb_u64(&b, m0);
+        for (size_t j = 0; j < m0; j++) bb_u64(&b, base + j * nn0);
+        bb_u64(&b, m0);
+        for (size_t j = 0; j < m0; j++) bb_u64(&b, f0[base + j * nn0]);
+        mt_open_ser(&t0, base, &b);
+        if (L >= 2) {
+            bb_fp6(&b, s.f[1][base]);
+            bb_u8(&b, 1);
+            mt_open_ser(&t1, base, &b);
+        } else {
+            bb_fp6(&b, fp6_zero());
+            bb_u8(&b, 0);
+        }
+    }
+    mt_free(&t0);
+    if (L >= 2) mt_free(&t1);
+    fs_free(&s);
+    *out = b.p;
+    *out_len = b.len;
+    return 0;
+}
+
+void ref_free(void *p) { free(p); }

@@ -1,0 +1,95 @@
+/*
+ * refcpu.h -- public interface of the CPU oracle ("refcpu").
+ *
+ * THE ORACLE IS TEST INFRASTRUCTURE. It is a plain-C restatement of the
+ * stark-swarm CPU prover's hot path, used only as the checker of the HIP
+ * product path by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg. Nothing in the shipped library links, loads or calls it.
+ *
+ * All field values cross this interface as canonical u64 in [0, p),
+ * p = 2^64 - 2^32 + 1. Fp6 values are 6 consecutive u64 in the reference's
+ * to_fp_components order (c0.c[0..3], c1.c[0..3]).
+ */
+#ifndef REFCPU_H
+#define REFCPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- field / primitive checks ---- */
+uint64_t ref_gl_mul(uint64_t a, uint64_t b);
+uint64_t ref_gl_inv(uint64_t a);
+uint64_t ref_gl_root_of_unity(size_t n);
+void ref_fp6_mul(const uint64_t *a, const uint64_t *b, uint64_t *out);
+void ref_fp6_inv(const uint64_t *a, uint64_t *out);
+void ref_ntt(uint64_t *a, size_t n, int inverse);
+void ref_sha3_256(const uint8_t *msg, size_t len, uint8_t out[32]);
+void ref_keccak_f1600(uint64_t st[25]);
+void ref_chacha_block(const uint8_t key[32], uint32_t counter, const uint8_t nonce[12], int rounds, uint32_t out[16]);
+/* StdRng(seed = LE64(seed) || 0^24).next_u64()  (fri.rs:1044-1051) */
+uint64_t ref_stdrng_first_u64(uint64_t seed);
+
+/* ---- LDE: trace_import.rs:251-281 (shift == 1); shift != 1 scales coefficient i by shift^i ---- */
+int ref_lde_columns(const uint64_t *cols, size_t w, size_t n_trace, size_t blowup, uint64_t shift, uint64_t *out);
+
+/* ---- HashRollup AIR: air_workloads.rs:571-599, merge lib.rs:343-437 ---- */
+void ref_build_hash_rollup_trace(size_t n_trace, const uint64_t *leaves, size_t n_leaves, uint64_t *trace /* 4 x n_trace */);
+/* phi_mode: 0 = always compute phi (non-parallel build), 1 = rayon with >1 thread
+ * (phi == 0 when n0 < 4096), 2 = rayon with 1 thread (phi always 0). */
+int ref_merge_hash_rollup(const uint64_t *lde, size_t n_trace, size_t blowup, int phi_mode, uint64_t *c_eval);
+
+/* ---- Merkle (crates/merkle/src/lib.rs) ---- */
+/* root of a tree with constant arity, leaves of `vals_per_leaf` base values */
+void ref_merkle_root(const uint64_t *leaf_vals, size_t n_leaves, size_t vals_per_leaf,
+                     size_t arity, size_t depth, uint64_t label, uint8_t root[32]);
+
+/* ---- DEEP-FRI / STIR prover (fri.rs:1398-2055) ---- */
+typedef struct {
+    const size_t *schedule;
+    size_t n_layers;
+    size_t r;
+    size_t s0;
+    uint64_t seed_z;
+    int coeff_commit_final;
+    size_t d_final;
+    int stir;
+    const uint8_t *public_inputs_hash; /* 32 bytes or NULL */
+} ref_fri_params;
+
+/* Writes the ark-serialize compressed DeepFriProof<SexticExt>; *out is malloc'd. */
+int ref_deep_fri_prove(const uint64_t *f0, size_t n0, const ref_fri_params *p, uint8_t **out, size_t *out_len);
+/* Fold trace of the prover for stage-by-stage checks: writes the layer values
+ * f_1 .. f_L (Fp6, concatenated) and the layer roots; sizes follow the schedule. */
+int ref_fri_layers(const uint64_t *f0, size_t n0, const ref_fri_params *p, uint64_t *layers_out, uint8_t *roots_out, uint8_t root_f0[32]);
+void ref_free(void *p);
+
+/* ---- swarm-dns prover (crates/swarm-dns/src/prover.rs, dns.rs) ---- */
+/* Synthetic zone record i (swarm-ctrl main.rs:1331-1357), leaf hash h2 (dns.rs:72-89). */
+void ref_zone_leaf_hashes(uint64_t first, size_t count, const uint8_t salt[16], uint8_t *out /* count x 32 */);
+/* Canonical bytes of synthetic record i; returns length (buffer >= 256 bytes). */
+size_t ref_zone_record_canonical(uint64_t i, uint8_t *out);
+void ref_dns_merkle_root(const uint8_t *leaves, size_t n, uint8_t root[32]);
+void ref_shard_fs_binding(const uint8_t pk_hash[32], const uint8_t job_id[32], uint32_t shard_id, const uint8_t nonce[32], uint8_t out[32]);
+/* prove_inner_shard over precomputed leaf hashes; returns proof bytes (malloc'd). */
+int ref_prove_inner_shard(const uint8_t salt[16], const uint8_t *leaf_hashes, size_t count,
+                          const uint8_t fs_binding[32], int phi_mode,
+                          uint8_t **proof, size_t *proof_len, uint8_t pi_hash[32], uint8_t merkle_root[32], uint8_t root_f0[32]);
+int ref_prove_outer_rollup(const uint8_t *pi_hashes, size_t count, const uint8_t pk_hash[32], int phi_mode,
+                           uint8_t **proof, size_t *proof_len, uint8_t root_f0[32]);
+
+/* ---- prover-core microbench (coset LDE -> row Merkle -> STIR prove) ---- */
+/* trace: w x n_trace column-major; returns proof bytes of deep_fri_prove over
+ * f0 = sum_c (c+1) * LDE_c with the row-tree root as public_inputs_hash, and
+ * the row root. lde_out (w x n0) may be NULL. */
+int ref_prover_core(const uint64_t *trace, size_t w, size_t n_trace, size_t blowup, uint64_t shift,
+                    const ref_fri_params *p, uint8_t row_root[32], uint64_t *f0_out,
+                    uint8_t **proof, size_t *proof_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
