@@ -415,7 +415,8 @@ int ref_fri_layers(const uint64_t *f0, size_t n0, const ref_fri_params *p, uint6
     return 0;
 }
 
-int ref_deep_fri_prove(const uint64_t *f0, size_t n0, const ref_fri_params *p, uint8_t **out, size_t *out_len) {
+int ref_deep_fri_prove_raw(const uint64_t *f0, size_t n0, const ref_fri_params *p, uint8_t **out, size_t *out_len,
+                           uint64_t *raw_out) {
     fri_state s;
     int rc = fri_build(f0, n0, p, &s);
     if (rc) return rc;
@@ -485,6 +486,7 @@ int ref_deep_fri_prove(const uint64_t *f0, size_t n0, const ref_fri_params *p, u
         uint64_t seed = tr_hash_fields_tagged("FRI/index", fl, 3);
         uint64_t raw = (ref_stdrng_first_u64(seed) & (npow2 - 1)) % n0;
         uint64_t base = raw % nn0;
+        if (raw_out) raw_out[q] = raw;
         bb_u64(&b, base);
         bb_u64(&b, raw);
         bb_u64(&b, m0);
@@ -509,4 +511,61 @@ int ref_deep_fri_prove(const uint64_t *f0, size_t n0, const ref_fri_params *p, u
     return 0;
 }
 
+int ref_deep_fri_prove(const uint64_t *f0, size_t n0, const ref_fri_params *p, uint8_t **out, size_t *out_len) {
+    return ref_deep_fri_prove_raw(f0, n0, p, out, out_len, NULL);
+}
+
 void ref_free(void *p) { free(p); }
+
+/* Prover-core microbench: coset LDE -> row tree (DsLabel arity 2, label 0,
+ * leaf = the w values of one LDE row) -> f0 = sum_c (c+1) LDE_c ->
+ * deep_fri_prove(f0) bound to the row root -> one row opening per STIR query
+ * at its raw query index. Output: proof || u64 s0 || s0 x (Vec<F> row, MerkleOpening). */
+int ref_prover_core(const uint64_t *trace, size_t w, size_t n_trace, size_t blowup, uint64_t shift,
+                    const ref_fri_params *p, uint8_t row_root[32], uint64_t *f0_out,
+                    uint8_t **proof, size_t *proof_len) {
+    size_t n0 = n_trace * blowup;
+    uint64_t *lde = (uint64_t *)malloc(8 * w * n0);
+    if (!lde) return -3;
+    int rc = ref_lde_columns(trace, w, n_trace, blowup, shift, lde);
+    if (rc) { free(lde); return rc; }
+    size_t depth = 0;
+    while (((size_t)1 << depth) < n0) depth++;
+    if (depth == 0) depth = 1;
+    mtree t;
+    mt_init(&t, 2, depth, 0, n0);
+    uint64_t *row = (uint64_t *)malloc(8 * w);
+    uint64_t *f0 = f0_out ? f0_out : (uint64_t *)malloc(8 * n0);
+    for (size_t i = 0; i < n0; i++) {
+        uint64_t acc = 0;
+        for (size_t c = 0; c < w; c++) {
+            row[c] = lde[c * n0 + i];
+            acc = gl_add(acc, gl_mul(row[c], c + 1));
+        }
+        f0[i] = acc;
+        leaf_hash(2, 0, i, row, w, t.lv[0] + 32 * i);
+    }
+    mt_finalize(&t, row_root);
+    ref_fri_params q = *p;
+    q.public_inputs_hash = row_root;
+    uint64_t *raw = (uint64_t *)malloc(8 * (p->s0 ? p->s0 : 1));
+    uint8_t *pb = NULL;
+    size_t pl = 0;
+    rc = ref_deep_fri_prove_raw(f0, n0, &q, &pb, &pl, raw);
+    if (rc == 0) {
+        bbuf b = {0};
+        bb_put(&b, pb, pl);
+        bb_u64(&b, p->s0);
+        for (size_t k = 0; k < p->s0; k++) {
+            bb_u64(&b, w);
+            for (size_t c = 0; c < w; c++) bb_u64(&b, lde[c * n0 + raw[k]]);
+            mt_open_ser(&t, raw[k], &b);
+        }
+        *proof = b.p;
+        *proof_len = b.len;
+    }
+    free(pb); free(raw); free(row); free(lde);
+    if (!f0_out) free(f0);
+    mt_free(&t);
+    return rc;
+}

@@ -1,0 +1,169 @@
+"""ctypes loader for the CPU oracle (oracle/librefcpu.so) -- the checker.
+
+Only tests, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+The library is built by __graft_entry__.build(); this loader builds it only
+when no built copy exists.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "librefcpu.so")
+P = 2**64 - 2**32 + 1
+
+u64p = ctypes.POINTER(ctypes.c_uint64)
+u8p = ctypes.POINTER(ctypes.c_uint8)
+sz = ctypes.c_size_t
+
+
+class FriParams(ctypes.Structure):
+    _fields_ = [("schedule", ctypes.POINTER(ctypes.c_size_t)), ("n_layers", sz), ("r", sz), ("s0", sz),
+                ("seed_z", ctypes.c_uint64), ("coeff_commit_final", ctypes.c_int), ("d_final", sz),
+                ("stir", ctypes.c_int), ("public_inputs_hash", ctypes.c_char_p)]
+
+
+def fri_params(schedule, r=54, s0=54, seed_z=0xDEEFBAAD, coeff_commit_final=True, d_final=1, stir=True, pi_hash=None):
+    arr = (ctypes.c_size_t * len(schedule))(*schedule)
+    p = FriParams(arr, len(schedule), r, s0, seed_z, int(coeff_commit_final), d_final, int(stir), pi_hash)
+    p._keep = arr
+    return p
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+        L = ctypes.CDLL(LIB)
+        L.ref_gl_mul.restype = ctypes.c_uint64
+        L.ref_gl_mul.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        L.ref_gl_inv.restype = ctypes.c_uint64
+        L.ref_gl_inv.argtypes = [ctypes.c_uint64]
+        L.ref_gl_root_of_unity.restype = ctypes.c_uint64
+        L.ref_gl_root_of_unity.argtypes = [sz]
+        L.ref_stdrng_first_u64.restype = ctypes.c_uint64
+        L.ref_stdrng_first_u64.argtypes = [ctypes.c_uint64]
+        L.ref_fp6_mul.argtypes = [u64p, u64p, u64p]
+        L.ref_fp6_inv.argtypes = [u64p, u64p]
+        L.ref_ntt.argtypes = [u64p, sz, ctypes.c_int]
+        L.ref_sha3_256.argtypes = [ctypes.c_char_p, sz, ctypes.c_char_p]
+        L.ref_keccak_f1600.argtypes = [u64p]
+        L.ref_chacha_block.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_uint32)]
+        L.ref_lde_columns.argtypes = [u64p, sz, sz, sz, ctypes.c_uint64, u64p]
+        L.ref_build_hash_rollup_trace.argtypes = [sz, u64p, sz, u64p]
+        L.ref_merge_hash_rollup.argtypes = [u64p, sz, sz, ctypes.c_int, u64p]
+        L.ref_merkle_root.argtypes = [u64p, sz, sz, sz, sz, ctypes.c_uint64, ctypes.c_char_p]
+        L.ref_deep_fri_prove.argtypes = [u64p, sz, ctypes.POINTER(FriParams), ctypes.POINTER(u8p),
+                                         ctypes.POINTER(sz)]
+        L.ref_fri_layers.argtypes = [u64p, sz, ctypes.POINTER(FriParams), u64p, ctypes.c_char_p, ctypes.c_char_p]
+        L.ref_free.argtypes = [ctypes.c_void_p]
+        L.ref_zone_leaf_hashes.argtypes = [ctypes.c_uint64, sz, ctypes.c_char_p, ctypes.c_char_p]
+        L.ref_zone_record_canonical.restype = sz
+        L.ref_zone_record_canonical.argtypes = [ctypes.c_uint64, ctypes.c_char_p]
+        L.ref_dns_merkle_root.argtypes = [ctypes.c_char_p, sz, ctypes.c_char_p]
+        L.ref_shard_fs_binding.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint32, ctypes.c_char_p,
+                                           ctypes.c_char_p]
+        L.ref_prove_inner_shard.argtypes = [ctypes.c_char_p, ctypes.c_char_p, sz, ctypes.c_char_p, ctypes.c_int,
+                                            ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.c_char_p,
+                                            ctypes.c_char_p, ctypes.c_char_p]
+        L.ref_prove_outer_rollup.argtypes = [ctypes.c_char_p, sz, ctypes.c_char_p, ctypes.c_int,
+                                             ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.c_char_p]
+        L.ref_prover_core.argtypes = [u64p, sz, sz, sz, ctypes.c_uint64, ctypes.POINTER(FriParams),
+                                      ctypes.c_char_p, u64p, ctypes.POINTER(u8p), ctypes.POINTER(sz)]
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    assert a.dtype == np.uint64 and a.flags.c_contiguous
+    return a.ctypes.data_as(u64p)
+
+
+def take_bytes(pp, pl):
+    b = ctypes.string_at(pp, pl.value)
+    lib().ref_free(pp)
+    return b
+
+
+def ntt(a, inverse=False):
+    a = np.ascontiguousarray(a, dtype=np.uint64).copy()
+    lib().ref_ntt(ptr(a), a.size, int(inverse))
+    return a
+
+
+def lde_columns(cols, blowup, shift=1):
+    cols = np.ascontiguousarray(cols, dtype=np.uint64)
+    w, n = cols.shape
+    out = np.empty((w, n * blowup), dtype=np.uint64)
+    rc = lib().ref_lde_columns(ptr(cols), w, n, blowup, shift, ptr(out))
+    if rc:
+        raise ValueError("lde_columns rc=%d" % rc)
+    return out
+
+
+def hash_rollup_trace(n_trace, leaves):
+    leaves = np.ascontiguousarray(leaves, dtype=np.uint64)
+    t = np.empty((4, n_trace), dtype=np.uint64)
+    lib().ref_build_hash_rollup_trace(n_trace, ptr(leaves), leaves.size, ptr(t))
+    return t
+
+
+def merge_hash_rollup(lde, n_trace, blowup, phi_mode=1):
+    lde = np.ascontiguousarray(lde, dtype=np.uint64)
+    out = np.empty(n_trace * blowup, dtype=np.uint64)
+    lib().ref_merge_hash_rollup(ptr(lde), n_trace, blowup, phi_mode, ptr(out))
+    return out
+
+
+def deep_fri_prove(f0, params):
+    f0 = np.ascontiguousarray(f0, dtype=np.uint64)
+    pp, pl = u8p(), sz()
+    rc = lib().ref_deep_fri_prove(ptr(f0), f0.size, ctypes.byref(params), ctypes.byref(pp), ctypes.byref(pl))
+    if rc:
+        raise ValueError("deep_fri_prove rc=%d" % rc)
+    return take_bytes(pp, pl)
+
+
+def zone_leaf_hashes(first, count, salt):
+    out = ctypes.create_string_buffer(32 * count)
+    lib().ref_zone_leaf_hashes(first, count, salt, out)
+    return out.raw
+
+
+def dns_merkle_root(leaves):
+    out = ctypes.create_string_buffer(32)
+    lib().ref_dns_merkle_root(leaves, len(leaves) // 32, out)
+    return out.raw
+
+
+def shard_fs_binding(pk_hash, job_id, shard_id, nonce):
+    out = ctypes.create_string_buffer(32)
+    lib().ref_shard_fs_binding(pk_hash, job_id, shard_id, nonce, out)
+    return out.raw
+
+
+def prove_inner_shard(salt, leaf_hashes, fs_binding, phi_mode=1):
+    pp, pl = u8p(), sz()
+    pi, mr, rf = (ctypes.create_string_buffer(32) for _ in range(3))
+    rc = lib().ref_prove_inner_shard(salt, leaf_hashes, len(leaf_hashes) // 32, fs_binding, phi_mode,
+                                     ctypes.byref(pp), ctypes.byref(pl), pi, mr, rf)
+    if rc:
+        raise ValueError("prove_inner_shard rc=%d" % rc)
+    return take_bytes(pp, pl), pi.raw, mr.raw, rf.raw
+
+
+def prove_outer_rollup(pi_hashes, pk_hash, phi_mode=1):
+    pp, pl = u8p(), sz()
+    rf = ctypes.create_string_buffer(32)
+    rc = lib().ref_prove_outer_rollup(pi_hashes, len(pi_hashes) // 32, pk_hash, phi_mode, ctypes.byref(pp),
+                                      ctypes.byref(pl), rf)
+    if rc:
+        raise ValueError("prove_outer_rollup rc=%d" % rc)
+    return take_bytes(pp, pl), rf.raw
