@@ -1,0 +1,164 @@
+/*
+ * stark_gpu.h -- C ABI of the MI355X (gfx950) proving backend for
+ * stark-swarm's data-parallel prover core.
+ *
+ * Conventions (a Rust / C caller binds these with plain pointers and sizes):
+ *  - every field value crosses as a canonical u64 in [0, p), p = 2^64-2^32+1;
+ *    an Fp6 value is 6 u64 in the reference's to_fp_components order;
+ *  - columns are column-major: column c occupies [c*n, (c+1)*n);
+ *  - status codes: 0 ok, STARK_E_* < 0 on error; stark_last_error() gives the
+ *    message; the library never aborts the process;
+ *  - output buffers are caller-owned, except proof bytes, which the library
+ *    allocates and the caller releases with stark_free();
+ *  - one context owns one HIP stream; calls on different contexts may run
+ *    from different threads; a single context is not re-entrant.
+ *
+ * Citations are to the reference workspace (saholmes/stark-swarm,
+ * stark-stir-swarm/); each entry point names the Rust function it replaces.
+ */
+#ifndef STARK_GPU_H
+#define STARK_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STARK_OK 0
+#define STARK_E_INVALID (-1)
+#define STARK_E_OOM (-2)
+#define STARK_E_DEVICE (-3)
+#define STARK_E_UNSUPPORTED (-4)
+#define STARK_E_INTERNAL (-5)
+
+/* phi_mode of the composition merge: the reference leaves phi identically
+ * zero when its rayon build runs with n < 4096 points or on one thread
+ * (crates/deep_ali/src/lib.rs:20-33, 374-397). */
+#define STARK_PHI_SEQUENTIAL 0 /* non-parallel build: phi always computed */
+#define STARK_PHI_RAYON 1      /* parallel build, >1 thread: phi = 0 iff n < 4096 */
+#define STARK_PHI_ONE_THREAD 2 /* parallel build, 1 thread: phi always 0 */
+
+#define STARK_LDT_STIR 0 /* LdtMode::Stir (crates/swarm-dns/src/prover.rs:37-41) */
+#define STARK_LDT_FRI 1  /* LdtMode::Fri: not implemented, returns STARK_E_UNSUPPORTED */
+
+#define STARK_AIR_HASH_ROLLUP 5 /* AirType::HashRollup (crates/deep_ali/src/air_workloads.rs:83) */
+
+typedef struct stark_ctx stark_ctx;
+
+/* DeepFriParams (crates/deep_ali/src/fri.rs:1303-1314) */
+typedef struct {
+    const size_t *schedule;
+    size_t n_layers;
+    size_t r;
+    size_t s0;
+    uint64_t seed_z;
+    int coeff_commit_final;
+    size_t d_final;
+    int stir;
+    const uint8_t *public_inputs_hash; /* 32 bytes, or NULL for None */
+} stark_fri_params;
+
+/* DnsRecord (crates/swarm-dns/src/dns.rs:24-31) */
+typedef struct {
+    const char *domain;
+    size_t domain_len;
+    uint16_t record_type;
+    uint32_t ttl;
+    const uint8_t *rdata;
+    size_t rdata_len;
+} stark_dns_record;
+
+/* InnerShardOutput (crates/swarm-dns/src/prover.rs:105-136) */
+typedef struct {
+    uint8_t pi_hash[32];
+    uint8_t merkle_root[32];
+    uint8_t root_f0[32];
+    size_t record_count;
+    size_t n_trace;
+    uint8_t *proof; /* ark-serialize compressed DeepFriProof<SexticExt>; stark_free() it */
+    size_t proof_len;
+} stark_inner_shard_out;
+
+/* ---- context ---- */
+int stark_ctx_create(int device, stark_ctx **out);
+void stark_ctx_destroy(stark_ctx *ctx);
+const char *stark_last_error(const stark_ctx *ctx);
+void stark_free(void *p);
+
+/* ---- host-buffer drop-ins ---- */
+
+/* lde_trace_columns (crates/deep_ali/src/trace_import.rs:251-281):
+ * cols is w x n_trace, out is w x (n_trace*blowup), natural order on <w_n0>. */
+int stark_lde_columns(stark_ctx *ctx, const uint64_t *cols, size_t w, size_t n_trace, size_t blowup, uint64_t *out);
+
+/* deep_ali_merge_general (crates/deep_ali/src/lib.rs:343-437) for
+ * air_id = STARK_AIR_HASH_ROLLUP; coeffs are the k combination coefficients
+ * (the reference passes lambda_j = j+1, prover.rs:100-102; only that choice
+ * is accepted). lde is w x n0, c_eval receives n0 values. */
+int stark_merge(stark_ctx *ctx, int air_id, const uint64_t *lde, size_t w, size_t n_trace, size_t blowup,
+                const uint64_t *coeffs, size_t k, int phi_mode, uint64_t *c_eval);
+
+/* deep_fri_prove::<SexticExt> (crates/deep_ali/src/fri.rs:1900-2055), STIR
+ * mode. *proof receives the ark-serialize compressed DeepFriProof bytes. */
+int stark_deep_fri_prove(stark_ctx *ctx, const uint64_t *f0, size_t n0, const stark_fri_params *params,
+                         uint8_t **proof, size_t *proof_len);
+
+/* MerkleTreeChannel::{new, push_leaves_parallel, finalize}
+ * (crates/merkle/src/lib.rs:365-476) with constant arity: leaf i hashes
+ * leaf_vals[i*vals_per_leaf .. (i+1)*vals_per_leaf]. */
+int stark_merkle_commit(stark_ctx *ctx, const uint64_t *leaf_vals, size_t n_leaves, size_t vals_per_leaf,
+                        size_t arity, uint64_t tree_label, uint8_t root[32]);
+
+/* prove_inner_shard (crates/swarm-dns/src/prover.rs:148-227). The local
+ * self-verify of the reference is not run (proof bytes are byte-identical). */
+int stark_prove_inner_shard(stark_ctx *ctx, const uint8_t salt[16], const stark_dns_record *records, size_t n,
+                            const uint8_t fs_binding[32], int ldt, int phi_mode, stark_inner_shard_out *out);
+
+/* prove_outer_rollup (crates/swarm-dns/src/prover.rs:253-304). */
+int stark_prove_outer_rollup(stark_ctx *ctx, const uint8_t *pi_hashes, size_t n, const uint8_t pk_hash[32], int ldt,
+                             int phi_mode, uint8_t **proof, size_t *proof_len, uint8_t root_f0[32]);
+
+/* ---- device-resident API (inputs already in HBM) ---- */
+int stark_dev_alloc(stark_ctx *ctx, size_t bytes, void **ptr);
+int stark_dev_free(stark_ctx *ctx, void *ptr);
+int stark_memcpy_h2d(stark_ctx *ctx, void *dst, const void *src, size_t bytes);
+int stark_memcpy_d2h(stark_ctx *ctx, void *dst, const void *src, size_t bytes);
+int stark_synchronize(stark_ctx *ctx);
+
+/* Radix2EvaluationDomain::{fft, ifft} on ncols columns of 2^log_n values
+ * (ark-poly 0.4.2, used at trace_import.rs:272-279, lib.rs:400-410). */
+int stark_ntt_dev(stark_ctx *ctx, const uint64_t *d_in, uint64_t *d_out, size_t log_n, size_t ncols, int inverse);
+
+/* lde_trace_columns on device buffers; shift != 1 evaluates on the coset
+ * shift * <w_n0> (coefficient i scaled by shift^i). */
+int stark_lde_columns_dev(stark_ctx *ctx, const uint64_t *d_cols, size_t w, size_t n_trace, size_t blowup,
+                          uint64_t shift, uint64_t *d_out);
+
+/* Prover core on device buffers: coset LDE (d_lde, w x n0) -> row Merkle tree
+ * (leaf = one LDE row, DsLabel arity 2, label 0) -> f0 = sum_c (c+1) LDE_c ->
+ * deep_fri_prove(f0) bound to the row root -> one row opening per STIR query.
+ * Output bytes: DeepFriProof || u64 s0 || s0 x (Vec<F> row, MerkleOpening). */
+int stark_prover_core_dev(stark_ctx *ctx, const uint64_t *d_trace, size_t w, size_t n_trace, size_t blowup,
+                          uint64_t shift, const stark_fri_params *params, uint64_t *d_lde, uint8_t row_root[32],
+                          uint8_t **proof, size_t *proof_len);
+
+/* ---- profiling: HIP events around every kernel launch on the context stream ---- */
+int stark_profile_enable(stark_ctx *ctx, int on);
+int stark_profile_reset(stark_ctx *ctx);
+/* Accumulated time (ms), launches and algorithmic bytes of kernels named `name`. */
+int stark_profile_get(stark_ctx *ctx, const char *name, double *ms, uint64_t *launches, double *bytes);
+/* Newline-separated list of kernel names seen; returns the length written. */
+size_t stark_profile_names(stark_ctx *ctx, char *buf, size_t cap);
+
+/* ---- host-side helpers (no device work; used by CPU tests of the host logic) ---- */
+void stark_host_sha3_256(const uint8_t *msg, size_t len, uint8_t out[32]);
+uint64_t stark_host_stdrng_first_u64(uint64_t seed);
+/* DnsRecord::leaf_hash (dns.rs:72-89) */
+void stark_host_dns_leaf_hash(const stark_dns_record *rec, const uint8_t salt[16], uint8_t out[32]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,51 @@
+"""stark_swarm_amd -- MI355X (gfx950) proving backend for stark-swarm's prover core.
+
+Host-side mirror of the reference's prover API for the data-parallel path,
+calling the hand-written HIP kernels in libstark_gpu.so through its C ABI
+(include/stark_gpu.h). Names, argument meaning and error behaviour follow the
+Rust functions they replace:
+
+  lde_trace_columns        crates/deep_ali/src/trace_import.rs:251-281
+  deep_ali_merge_general   crates/deep_ali/src/lib.rs:343-437
+  deep_fri_prove           crates/deep_ali/src/fri.rs:1900-2055
+  DeepFriParams, FriDomain crates/deep_ali/src/fri.rs:160-175,1303-1349
+  make_schedule, build_params, prove_inner_shard, prove_outer_rollup
+                           crates/swarm-dns/src/prover.rs:53-304
+  DnsRecord                crates/swarm-dns/src/dns.rs:24-89
+  shard_fs_binding         crates/swarm-proto/src/messages.rs:162-176
+
+There is no CPU fallback: importing works without a GPU, but every call that
+computes needs the built library and a visible gfx950 device, and raises
+otherwise.
+"""
+from .api import (  # noqa: F401
+    AirType,
+    CompositionInfo,
+    Context,
+    DeepFriParams,
+    DnsRecord,
+    FriDomain,
+    InnerShardOutput,
+    LdtMode,
+    OuterRollupOutput,
+    StarkError,
+    BLOWUP,
+    NUM_QUERIES,
+    SEED_Z,
+    PHI_RAYON,
+    PHI_ONE_THREAD,
+    PHI_SEQUENTIAL,
+    build_params,
+    deep_ali_merge_general,
+    deep_fri_prove,
+    default_context,
+    lde_trace_columns,
+    lib,
+    make_schedule,
+    merkle_commit,
+    ntt,
+    prove_inner_shard,
+    prove_outer_rollup,
+    prover_core,
+    shard_fs_binding,
+)

@@ -1,0 +1,482 @@
+"""ctypes binding of libstark_gpu.so and the reference-named host API."""
+import ctypes
+import enum
+import hashlib
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libstark_gpu.so")
+
+P = 2**64 - 2**32 + 1
+BLOWUP = 32          # crates/swarm-dns/src/prover.rs:32
+NUM_QUERIES = 54     # prover.rs:33
+SEED_Z = 0xDEEFBAAD  # prover.rs:34
+
+PHI_SEQUENTIAL, PHI_RAYON, PHI_ONE_THREAD = 0, 1, 2
+_LDT_STIR, _LDT_FRI = 0, 1
+_AIR_HASH_ROLLUP = 5
+
+u64p = ctypes.POINTER(ctypes.c_uint64)
+u8p = ctypes.POINTER(ctypes.c_uint8)
+sz = ctypes.c_size_t
+
+
+class StarkError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("stark_gpu error %d: %s" % (code, msg))
+        self.code = code
+
+
+class _FriParams(ctypes.Structure):
+    _fields_ = [("schedule", ctypes.POINTER(ctypes.c_size_t)), ("n_layers", sz), ("r", sz), ("s0", sz),
+                ("seed_z", ctypes.c_uint64), ("coeff_commit_final", ctypes.c_int), ("d_final", sz),
+                ("stir", ctypes.c_int), ("public_inputs_hash", ctypes.c_char_p)]
+
+
+class _DnsRecord(ctypes.Structure):
+    _fields_ = [("domain", ctypes.c_char_p), ("domain_len", sz), ("record_type", ctypes.c_uint16),
+                ("ttl", ctypes.c_uint32), ("rdata", ctypes.c_char_p), ("rdata_len", sz)]
+
+
+class _InnerOut(ctypes.Structure):
+    _fields_ = [("pi_hash", ctypes.c_uint8 * 32), ("merkle_root", ctypes.c_uint8 * 32),
+                ("root_f0", ctypes.c_uint8 * 32), ("record_count", sz), ("n_trace", sz),
+                ("proof", u8p), ("proof_len", sz)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libstark_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libstark_gpu.so is not built; run __graft_entry__.build() "
+                          "(make -C stark_swarm_amd)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    sig = {
+        "stark_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(vp)]),
+        "stark_ctx_destroy": (None, [vp]),
+        "stark_last_error": (ctypes.c_char_p, [vp]),
+        "stark_free": (None, [vp]),
+        "stark_lde_columns": (ctypes.c_int, [vp, u64p, sz, sz, sz, u64p]),
+        "stark_merge": (ctypes.c_int, [vp, ctypes.c_int, u64p, sz, sz, sz, u64p, sz, ctypes.c_int, u64p]),
+        "stark_deep_fri_prove": (ctypes.c_int, [vp, u64p, sz, ctypes.POINTER(_FriParams), ctypes.POINTER(u8p),
+                                                ctypes.POINTER(sz)]),
+        "stark_merkle_commit": (ctypes.c_int, [vp, u64p, sz, sz, sz, ctypes.c_uint64, ctypes.c_char_p]),
+        "stark_prove_inner_shard": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.POINTER(_DnsRecord), sz,
+                                                   ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+                                                   ctypes.POINTER(_InnerOut)]),
+        "stark_prove_outer_rollup": (ctypes.c_int, [vp, ctypes.c_char_p, sz, ctypes.c_char_p, ctypes.c_int,
+                                                    ctypes.c_int, ctypes.POINTER(u8p), ctypes.POINTER(sz),
+                                                    ctypes.c_char_p]),
+        "stark_dev_alloc": (ctypes.c_int, [vp, sz, ctypes.POINTER(vp)]),
+        "stark_dev_free": (ctypes.c_int, [vp, vp]),
+        "stark_memcpy_h2d": (ctypes.c_int, [vp, vp, vp, sz]),
+        "stark_memcpy_d2h": (ctypes.c_int, [vp, vp, vp, sz]),
+        "stark_synchronize": (ctypes.c_int, [vp]),
+        "stark_ntt_dev": (ctypes.c_int, [vp, vp, vp, sz, sz, ctypes.c_int]),
+        "stark_lde_columns_dev": (ctypes.c_int, [vp, vp, sz, sz, sz, ctypes.c_uint64, vp]),
+        "stark_prover_core_dev": (ctypes.c_int, [vp, vp, sz, sz, sz, ctypes.c_uint64, ctypes.POINTER(_FriParams),
+                                                 vp, ctypes.c_char_p, ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
+        "stark_profile_enable": (ctypes.c_int, [vp, ctypes.c_int]),
+        "stark_profile_reset": (ctypes.c_int, [vp]),
+        "stark_profile_get": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                                             ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)]),
+        "stark_profile_names": (sz, [vp, ctypes.c_char_p, sz]),
+        "stark_host_sha3_256": (None, [ctypes.c_char_p, sz, ctypes.c_char_p]),
+        "stark_host_stdrng_first_u64": (ctypes.c_uint64, [ctypes.c_uint64]),
+        "stark_host_dns_leaf_hash": (None, [ctypes.POINTER(_DnsRecord), ctypes.c_char_p, ctypes.c_char_p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _u64(a):
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    return a, a.ctypes.data_as(u64p)
+
+
+class DeviceBuffer:
+    """Device memory owned by a Context (caching allocator)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx, self.nbytes = ctx, nbytes
+        p = ctypes.c_void_p()
+        ctx._check(lib().stark_dev_alloc(ctx.h, nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        self.ctx._check(lib().stark_memcpy_h2d(self.ctx.h, self.ptr, arr.ctypes.data, arr.nbytes))
+        return self
+
+    def download(self, shape, dtype=np.uint64):
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        self.ctx._check(lib().stark_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr is not None:
+            lib().stark_dev_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One device context: a HIP stream, twiddle tables and an allocator."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        rc = lib().stark_ctx_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise StarkError(rc, "stark_ctx_create(device=%d) failed (no gfx950 device visible?)" % device)
+        self.h = h.value
+
+    def _check(self, rc):
+        if rc != 0:
+            raise StarkError(rc, lib().stark_last_error(self.h).decode(errors="replace"))
+
+    def close(self):
+        if self.h:
+            lib().stark_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def synchronize(self):
+        self._check(lib().stark_synchronize(self.h))
+
+    # profiling (HIP events around each launch on the context stream)
+    def profile(self, on=True):
+        self._check(lib().stark_profile_enable(self.h, int(on)))
+
+    def profile_reset(self):
+        self._check(lib().stark_profile_reset(self.h))
+
+    def profile_stats(self):
+        n = lib().stark_profile_names(self.h, None, 0)
+        buf = ctypes.create_string_buffer(n + 1)
+        lib().stark_profile_names(self.h, buf, n + 1)
+        out = {}
+        for name in buf.value.decode().split("\n"):
+            if not name:
+                continue
+            ms, la, by = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_double()
+            lib().stark_profile_get(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(la), ctypes.byref(by))
+            out[name] = {"ms": ms.value, "launches": la.value, "bytes": by.value}
+        return out
+
+
+_default = None
+
+
+def default_context():
+    global _default
+    if _default is None:
+        _default = Context(int(os.environ.get("STARK_GPU_DEVICE", "0")))
+    return _default
+
+
+def _take(ptr, n):
+    b = ctypes.string_at(ptr, n)
+    lib().stark_free(ptr)
+    return b
+
+
+# --------------------------------------------------------------------- LDE / NTT
+def lde_trace_columns(columns, trace_len, blowup, ctx=None):
+    """lde_trace_columns (trace_import.rs:251-281): w columns of trace_len values ->
+    w x (trace_len * blowup) evaluations on <w_n0>. Raises ValueError where the
+    reference returns Err(String)."""
+    if len(columns) == 0:
+        raise ValueError("no columns provided")
+    cols = np.asarray(columns, dtype=np.uint64) if not isinstance(columns, np.ndarray) else columns
+    if cols.ndim != 2 or cols.shape[1] != trace_len:
+        raise ValueError("expected %d rows per column, got shape %s" % (trace_len, cols.shape))
+    if trace_len <= 0 or trace_len & (trace_len - 1):
+        raise ValueError("trace_len %d must be a power of 2" % trace_len)
+    if blowup < 2 or blowup & (blowup - 1):
+        raise ValueError("blowup %d must be a power-of-2 >= 2" % blowup)
+    ctx = ctx or default_context()
+    cols, cp = _u64(cols)
+    out = np.empty((cols.shape[0], trace_len * blowup), dtype=np.uint64)
+    ctx._check(lib().stark_lde_columns(ctx.h, cp, cols.shape[0], trace_len, blowup, out.ctypes.data_as(u64p)))
+    return out
+
+
+def ntt(values, inverse=False, ctx=None):
+    """Radix2EvaluationDomain::{fft, ifft} over each row of a (ncols, 2^k) array."""
+    ctx = ctx or default_context()
+    a = np.ascontiguousarray(np.atleast_2d(values), dtype=np.uint64)
+    ncols, n = a.shape
+    lg = n.bit_length() - 1
+    if n != 1 << lg:
+        raise ValueError("size must be a power of two")
+    d = ctx.alloc(a.nbytes).upload(a)
+    o = ctx.alloc(a.nbytes)
+    ctx._check(lib().stark_ntt_dev(ctx.h, d.ptr, o.ptr, lg, ncols, int(inverse)))
+    r = o.download(a.shape)
+    d.free()
+    o.free()
+    return r.reshape(np.shape(values))
+
+
+# --------------------------------------------------------------------- merge
+class AirType(enum.Enum):
+    HashRollup = _AIR_HASH_ROLLUP  # air_workloads.rs:83 (w = 4, 3 constraints, degree 2)
+
+    def width(self):
+        return 4
+
+    def num_constraints(self):
+        return 3
+
+    def max_constraint_degree(self):
+        return 2
+
+
+@dataclass
+class CompositionInfo:
+    phi_degree_bound: int
+    quotient_degree_bound: int
+    rate: float
+    num_constraints: int
+    max_constraint_degree: int
+    trace_width: int
+
+
+def deep_ali_merge_general(trace_evals_on_lde, combination_coeffs, air, omega, n_trace, blowup,
+                           phi_mode=PHI_RAYON, ctx=None):
+    """deep_ali_merge_general (lib.rs:343-437). phi_mode selects the reference
+    build's behaviour: PHI_RAYON (the swarm-dns default build on a multi-core
+    host) leaves phi == 0 when n < 4096. Contract violations raise
+    AssertionError as the reference panics."""
+    air = AirType(air) if not isinstance(air, AirType) else air
+    w, k = air.width(), air.num_constraints()
+    n = n_trace * blowup
+    lde = np.asarray(trace_evals_on_lde, dtype=np.uint64)
+    assert lde.shape[0] == w, "trace width mismatch"
+    assert len(combination_coeffs) == k, \
+        "need one combination coefficient per constraint, got %d for %d constraints" % (len(combination_coeffs), k)
+    assert lde.shape[1] == n, "trace column length mismatch"
+    assert int(omega) == pow(7, (P - 1) // n, P), "omega must generate the n-point domain"
+    ctx = ctx or default_context()
+    lde, lp = _u64(lde)
+    co, cp = _u64(np.asarray(combination_coeffs, dtype=np.uint64))
+    out = np.empty(n, dtype=np.uint64)
+    ctx._check(lib().stark_merge(ctx.h, air.value, lp, w, n_trace, blowup, cp, k, phi_mode, out.ctypes.data_as(u64p)))
+    max_deg = air.max_constraint_degree()
+    phi_b = max_deg * n_trace
+    qb = phi_b - n_trace if phi_b > n_trace else 0
+    return out, CompositionInfo(phi_b, qb, qb / n, k, max_deg, w)
+
+
+# --------------------------------------------------------------------- FRI
+@dataclass
+class FriDomain:
+    """FriDomain::new_radix2 (fri.rs:160-175)."""
+    omega: int
+    size: int
+
+    @classmethod
+    def new_radix2(cls, size):
+        return cls(pow(7, (P - 1) // size, P), size)
+
+
+@dataclass
+class DeepFriParams:
+    """DeepFriParams (fri.rs:1303-1349)."""
+    schedule: List[int]
+    r: int
+    seed_z: int
+    coeff_commit_final: bool = False
+    d_final: int = 1
+    stir: bool = False
+    s0: Optional[int] = None
+    public_inputs_hash: Optional[bytes] = None
+
+    def _c(self):
+        arr = (ctypes.c_size_t * max(1, len(self.schedule)))(*self.schedule)
+        p = _FriParams(arr, len(self.schedule), self.r, self.s0 if self.s0 is not None else self.r, self.seed_z,
+                       int(self.coeff_commit_final), self.d_final, int(self.stir), self.public_inputs_hash)
+        p._keep = arr
+        return p
+
+
+def deep_fri_prove(f0, domain0, params, ctx=None):
+    """deep_fri_prove::<SexticExt> (fri.rs:1900-2055). Returns the proof's
+    ark-serialize compressed bytes (what prover.rs serializes and ships)."""
+    ctx = ctx or default_context()
+    f0, fp = _u64(f0)
+    assert f0.size == domain0.size
+    p = params._c()
+    pp, pl = u8p(), sz()
+    ctx._check(lib().stark_deep_fri_prove(ctx.h, fp, f0.size, ctypes.byref(p), ctypes.byref(pp), ctypes.byref(pl)))
+    return _take(pp, pl.value)
+
+
+def merkle_commit(leaf_values, arity, tree_label, ctx=None):
+    """MerkleTreeChannel push_leaves_parallel + finalize (merkle lib.rs:391-476) with
+    layer_arities = [arity; depth]; leaf_values is (n_leaves, vals_per_leaf)."""
+    ctx = ctx or default_context()
+    v = np.ascontiguousarray(np.atleast_2d(leaf_values), dtype=np.uint64)
+    root = ctypes.create_string_buffer(32)
+    ctx._check(lib().stark_merkle_commit(ctx.h, v.ctypes.data_as(u64p), v.shape[0], v.shape[1], arity, tree_label,
+                                         root))
+    return root.raw
+
+
+# --------------------------------------------------------------------- swarm-dns
+class LdtMode(enum.Enum):
+    Stir = _LDT_STIR
+    Fri = _LDT_FRI
+
+    def is_stir(self):
+        return self is LdtMode.Stir
+
+
+def make_schedule(n0, ldt=LdtMode.Stir):
+    """make_schedule (prover.rs:53-67)."""
+    assert n0 > 0 and n0 & (n0 - 1) == 0, "n0 must be a power of 2"
+    lg = n0.bit_length() - 1
+    if not ldt.is_stir():
+        return [2] * lg
+    s = [8] * (lg // 3)
+    if lg % 3:
+        s.append(1 << (lg % 3))
+    return s
+
+
+def build_params(n0, pk_hash_32, ldt=LdtMode.Stir):
+    """build_params (prover.rs:87-98)."""
+    return DeepFriParams(make_schedule(n0, ldt), NUM_QUERIES, SEED_Z, True, 1, ldt.is_stir(), NUM_QUERIES,
+                         bytes(pk_hash_32))
+
+
+@dataclass
+class DnsRecord:
+    """DnsRecord (dns.rs:24-31) with the constructors of dns.rs:33-41."""
+    domain: str
+    record_type: int
+    ttl: int
+    rdata: bytes
+
+    @classmethod
+    def a(cls, d, t, ip):
+        return cls(d, 1, t, bytes(ip))
+
+    @classmethod
+    def aaaa(cls, d, t, ip):
+        return cls(d, 28, t, bytes(ip))
+
+    @classmethod
+    def txt(cls, d, t, s):
+        return cls(d, 16, t, s.encode())
+
+    @classmethod
+    def mx(cls, d, t, prio, ex):
+        return cls(d, 15, t, prio.to_bytes(2, "big") + ex.encode())
+
+    def _c(self):
+        d = self.domain.encode()
+        return _DnsRecord(d, len(d), self.record_type, self.ttl, self.rdata, len(self.rdata))
+
+    def leaf_hash(self, salt):
+        out = ctypes.create_string_buffer(32)
+        r = self._c()
+        lib().stark_host_dns_leaf_hash(ctypes.byref(r), bytes(salt), out)
+        return out.raw
+
+
+@dataclass
+class InnerShardOutput:
+    pi_hash: bytes
+    merkle_root: bytes
+    record_count: int
+    n_trace: int
+    proof_bytes: int
+    root_f0: bytes
+    proof_blob: bytes
+
+
+@dataclass
+class OuterRollupOutput:
+    n_trace: int
+    proof_bytes: int
+    root_f0: bytes
+    proof_blob: bytes
+    n0: int
+
+
+def prove_inner_shard(salt, records: Sequence[DnsRecord], fs_binding_32, ldt=LdtMode.Stir, phi_mode=PHI_RAYON,
+                      ctx=None):
+    """prove_inner_shard (prover.rs:148-227)."""
+    ctx = ctx or default_context()
+    arr = (_DnsRecord * max(1, len(records)))(*[r._c() for r in records])
+    out = _InnerOut()
+    ctx._check(lib().stark_prove_inner_shard(ctx.h, bytes(salt), arr, len(records), bytes(fs_binding_32),
+                                             ldt.value, phi_mode, ctypes.byref(out)))
+    blob = _take(out.proof, out.proof_len)
+    return InnerShardOutput(bytes(out.pi_hash), bytes(out.merkle_root), out.record_count, out.n_trace, len(blob),
+                            bytes(out.root_f0), blob)
+
+
+def prove_outer_rollup(pi_hashes, pk_hash_32, ldt=LdtMode.Stir, phi_mode=PHI_RAYON, ctx=None):
+    """prove_outer_rollup (prover.rs:253-304)."""
+    ctx = ctx or default_context()
+    flat = b"".join(bytes(h) for h in pi_hashes)
+    pp, pl = u8p(), sz()
+    rf = ctypes.create_string_buffer(32)
+    ctx._check(lib().stark_prove_outer_rollup(ctx.h, flat, len(pi_hashes), bytes(pk_hash_32), ldt.value, phi_mode,
+                                              ctypes.byref(pp), ctypes.byref(pl), rf))
+    blob = _take(pp, pl.value)
+    n_trace = 8
+    while n_trace < 4 * len(pi_hashes):
+        n_trace *= 2
+    return OuterRollupOutput(n_trace, len(blob), rf.raw, blob, n_trace * BLOWUP)
+
+
+def shard_fs_binding(authority_pk_hash, job_id, shard_id, shard_nonce):
+    """shard_fs_binding (crates/swarm-proto/src/messages.rs:162-176); host SHA3."""
+    return hashlib.sha3_256(b"DNS-SWARM-SHARD-FS-V1" + bytes(authority_pk_hash) + bytes(job_id)
+                            + int(shard_id).to_bytes(4, "little") + bytes(shard_nonce)).digest()
+
+
+# --------------------------------------------------------------------- prover core
+def prover_core(d_trace, w, n_trace, blowup, shift, params, d_lde, ctx=None):
+    """Device-resident prover core: coset LDE -> row tree -> STIR prove -> row
+    openings (stark_prover_core_dev). d_trace / d_lde are DeviceBuffers.
+    Returns (row_root, proof bytes)."""
+    ctx = ctx or default_context()
+    p = params._c()
+    root = ctypes.create_string_buffer(32)
+    pp, pl = u8p(), sz()
+    ctx._check(lib().stark_prover_core_dev(ctx.h, d_trace.ptr, w, n_trace, blowup, shift, ctypes.byref(p), d_lde.ptr,
+                                           root, ctypes.byref(pp), ctypes.byref(pl)))
+    return root.raw, _take(pp, pl.value)

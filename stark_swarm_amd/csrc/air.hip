@@ -1,0 +1,108 @@
+// air.hip -- DEEP-ALI composition on gfx950 for the HashRollup AIR and the
+// Z_H quotient.
+//
+// deep_ali_merge_general (crates/deep_ali/src/lib.rs:343-437) with the
+// HashRollup evaluator (crates/deep_ali/src/air_workloads.rs:591-599):
+//   phi(i) = sum_j (j+1) C_j(row i, row (i+blowup) mod n),  lambda_j = j+1
+//            (crates/swarm-dns/src/prover.rs:100-102)
+//   C0 = idx' - idx - 1,  C1 = sq - st^2,  C2 = st' - sq - leaf
+// then c = NTT(pad(poly_div_zh(iNTT(phi), n_trace))) (lib.rs:190-228: the
+// quotient q[j] = sum_{t>=1, j+t*m<n} a[j+t*m], remainder dropped).
+// phi_mode reproduces the reference's thread-count quirk (lib.rs:20-33,
+// 374-397): 1 = rayon with >1 thread (phi == 0 when n < 4096), 2 = one
+// thread (phi always 0), 0 = the non-parallel build (phi always computed).
+#include "air.hpp"
+#include "gl.hpp"
+#include "ntt.hpp"
+
+namespace sg {
+
+namespace {
+
+constexpr int AT = 256;
+
+__global__ __launch_bounds__(AT) void compose_hash_rollup_k(const uint64_t *lde, uint64_t n, uint64_t blowup,
+                                                            uint64_t *phi) {
+    const uint64_t i = (uint64_t)blockIdx.x * AT + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t j = (i + blowup) & (n - 1);
+    const uint64_t *idx = lde, *leaf = lde + n, *st = lde + 2 * n, *sq = lde + 3 * n;
+    const uint64_t s = st[i], q = sq[i];
+    const uint64_t c0 = sub(sub(idx[j], idx[i]), 1);
+    const uint64_t c1 = sub(q, mul(s, s));
+    const uint64_t c2 = sub(sub(st[j], q), leaf[i]);
+    // 1*c0 + 2*c1 + 3*c2
+    phi[i] = add(add(c0, add(c1, c1)), add(add(c2, c2), c2));
+}
+
+// one residue class rho < m per lane: suffix sums along stride m
+__global__ __launch_bounds__(AT) void zh_div_k(const uint64_t *a, uint64_t n, uint64_t m, uint64_t *q) {
+    const uint64_t rho = (uint64_t)blockIdx.x * AT + threadIdx.x;
+    if (rho >= m) return;
+    const uint64_t B = n / m;
+    uint64_t acc = 0;
+    q[rho + (B - 1) * m] = 0;  // q has length n - m; the top block is zero padding
+    for (uint64_t t = B - 1; t >= 1; t--) {
+        acc = add(acc, a[rho + t * m]);
+        q[rho + (t - 1) * m] = acc;
+    }
+}
+
+__global__ void fill_k(uint64_t *p, uint64_t n, uint64_t v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+void fill_u64(Ctx *c, uint64_t *p, uint64_t n, uint64_t v) {
+    if (!n) return;
+    hipLaunchKernelGGL(fill_k, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, c->stream, p, n, v);
+    SG_HIP(hipGetLastError());
+}
+
+bool phi_computed(int phi_mode, uint64_t n) { return phi_mode == 0 || (phi_mode == 1 && n >= 4096); }
+
+void merge_hash_rollup_dev(Ctx *c, const uint64_t *lde, uint64_t n_trace, uint64_t blowup, int phi_mode,
+                           uint64_t *c_eval) {
+    SG_REQUIRE(is_pow2(n_trace) && is_pow2(blowup) && blowup >= 2, "merge: sizes must be powers of two");
+    const uint64_t n = n_trace * blowup;
+    SG_REQUIRE(n <= (1ULL << 24), "merge: domain above 2^24");
+    if (!phi_computed(phi_mode, n)) {
+        // phi == 0: iNTT, division and NTT of zero are zero
+        fill_u64(c, c_eval, n, 0);
+        return;
+    }
+    Scoped phi(c, 8 * n), coef(c, 8 * n), scr(c, 8 * n);
+    {
+        ProfScope ps(c, "compose_hash_rollup", 8.0 * 4 * n + 8.0 * n);
+        hipLaunchKernelGGL(compose_hash_rollup_k, dim3((uint32_t)((n + AT - 1) / AT)), dim3(AT), 0, c->stream, lde, n,
+                           blowup, phi.as<uint64_t>());
+        SG_HIP(hipGetLastError());
+    }
+    const uint32_t L = ilog2(n);
+    ntt_columns(c, phi.as<uint64_t>(), coef.as<uint64_t>(), scr.as<uint64_t>(), L, 1, n, n, true, nullptr);
+    {
+        ProfScope ps(c, "zh_div", 16.0 * n);
+        hipLaunchKernelGGL(zh_div_k, dim3((uint32_t)((n_trace + AT - 1) / AT)), dim3(AT), 0, c->stream,
+                           coef.as<uint64_t>(), n, n_trace, phi.as<uint64_t>());
+        SG_HIP(hipGetLastError());
+    }
+    ntt_columns(c, phi.as<uint64_t>(), c_eval, scr.as<uint64_t>(), L, 1, n, n, false, nullptr);
+}
+
+// HashRollup trace (air_workloads.rs:571-589): idx, leaf, state, state^2 with
+// state' = state^2 + leaf -- a sequential recurrence, built on the host.
+void build_hash_rollup_trace(uint64_t n_trace, const uint64_t *leaves, uint64_t n_leaves, uint64_t *tr) {
+    uint64_t s = 0;
+    for (uint64_t i = 0; i < n_trace; i++) {
+        uint64_t leaf = i < n_leaves ? canon(leaves[i]) : 0;
+        tr[i] = canon(i);
+        tr[n_trace + i] = leaf;
+        tr[2 * n_trace + i] = s;
+        tr[3 * n_trace + i] = mul(s, s);
+        s = add(tr[3 * n_trace + i], leaf);
+    }
+}
+
+}  // namespace sg

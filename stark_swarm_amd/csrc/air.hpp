@@ -1,0 +1,16 @@
+// air.hpp -- DEEP-ALI composition kernels (HashRollup) and helpers.
+#pragma once
+#include <stdint.h>
+
+#include "ctx.hpp"
+
+namespace sg {
+
+void fill_u64(Ctx *c, uint64_t *p, uint64_t n, uint64_t v);
+bool phi_computed(int phi_mode, uint64_t n);
+// c_eval (device, n_trace*blowup) from the 4-column HashRollup LDE (device, column-major)
+void merge_hash_rollup_dev(Ctx *c, const uint64_t *lde, uint64_t n_trace, uint64_t blowup, int phi_mode,
+                           uint64_t *c_eval);
+void build_hash_rollup_trace(uint64_t n_trace, const uint64_t *leaves, uint64_t n_leaves, uint64_t *tr);
+
+}  // namespace sg

@@ -1,0 +1,304 @@
+// capi.cpp -- the extern "C" boundary declared in include/stark_gpu.h.
+// Every entry point catches library errors and turns them into status codes.
+#include "../../include/stark_gpu.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "air.hpp"
+#include "ctx.hpp"
+#include "fri.hpp"
+#include "gl.hpp"
+#include "keccak.hpp"
+#include "ntt.hpp"
+#include "prover.hpp"
+
+struct stark_ctx {
+    sg::Ctx *c = nullptr;
+    std::string err;
+};
+
+namespace {
+
+template <class F>
+int guard(stark_ctx *ctx, F &&f) {
+    if (!ctx || !ctx->c) return STARK_E_INVALID;
+    try {
+        hipError_t e = hipSetDevice(ctx->c->device);
+        if (e != hipSuccess) throw sg::Error(sg::E_DEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e));
+        f(ctx->c);
+        if (ctx->c->profiling) ctx->c->prof_collect();
+        return STARK_OK;
+    } catch (const sg::Error &e) {
+        ctx->err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        ctx->err = "host allocation failed";
+        return STARK_E_OOM;
+    } catch (const std::exception &e) {
+        ctx->err = e.what();
+        return STARK_E_INTERNAL;
+    }
+}
+
+sg::FriParams to_params(const stark_fri_params *p) {
+    SG_REQUIRE(p != nullptr, "null fri params");
+    SG_REQUIRE(p->schedule != nullptr || p->n_layers == 0, "null schedule");
+    sg::FriParams q;
+    q.schedule.assign(p->schedule, p->schedule + p->n_layers);
+    q.r = p->r;
+    q.s0 = p->s0;
+    q.seed_z = p->seed_z;
+    q.coeff_commit_final = p->coeff_commit_final != 0;
+    q.d_final = p->d_final;
+    q.stir = p->stir != 0;
+    q.has_pi = p->public_inputs_hash != nullptr;
+    if (q.has_pi) memcpy(q.pi, p->public_inputs_hash, 32);
+    return q;
+}
+
+uint8_t *copy_out(const std::vector<uint8_t> &v, size_t *len) {
+    uint8_t *p = (uint8_t *)malloc(v.size() ? v.size() : 1);
+    if (!p) throw std::bad_alloc();
+    memcpy(p, v.data(), v.size());
+    *len = v.size();
+    return p;
+}
+
+void h2d(sg::Ctx *c, void *d, const void *h, size_t bytes) {
+    SG_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream));
+}
+void d2h(sg::Ctx *c, void *h, const void *d, size_t bytes) {
+    SG_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+int stark_ctx_create(int device, stark_ctx **out) {
+    if (!out) return STARK_E_INVALID;
+    *out = nullptr;
+    stark_ctx *x = new (std::nothrow) stark_ctx();
+    if (!x) return STARK_E_OOM;
+    try {
+        x->c = sg::ctx_create(device);
+    } catch (const sg::Error &e) {
+        delete x;
+        return e.code;
+    } catch (...) {
+        delete x;
+        return STARK_E_INTERNAL;
+    }
+    *out = x;
+    return STARK_OK;
+}
+
+void stark_ctx_destroy(stark_ctx *ctx) {
+    if (!ctx) return;
+    sg::ctx_destroy(ctx->c);
+    delete ctx;
+}
+
+const char *stark_last_error(const stark_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void stark_free(void *p) { free(p); }
+
+int stark_lde_columns(stark_ctx *ctx, const uint64_t *cols, size_t w, size_t n_trace, size_t blowup, uint64_t *out) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(cols && out && w > 0, "lde: null buffer or no columns");
+        SG_REQUIRE(sg::is_pow2(n_trace), "lde: trace_len must be a power of 2");
+        SG_REQUIRE(blowup >= 2 && sg::is_pow2(blowup), "lde: blowup must be a power-of-2 >= 2");
+        const size_t n0 = n_trace * blowup;
+        sg::Scoped din(c, 8 * w * n_trace), dout(c, 8 * w * n0);
+        h2d(c, din.p, cols, 8 * w * n_trace);
+        sg::lde_columns(c, din.as<uint64_t>(), dout.as<uint64_t>(), sg::ilog2(n_trace), sg::ilog2(blowup), w, n_trace,
+                        n0, 1);
+        d2h(c, out, dout.p, 8 * w * n0);
+    });
+}
+
+int stark_merge(stark_ctx *ctx, int air_id, const uint64_t *lde, size_t w, size_t n_trace, size_t blowup,
+                const uint64_t *coeffs, size_t k, int phi_mode, uint64_t *c_eval) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        if (air_id != STARK_AIR_HASH_ROLLUP) throw sg::Error(sg::E_UNSUPPORTED, "merge: only the HashRollup AIR");
+        SG_REQUIRE(w == 4 && k == 3 && lde && coeffs && c_eval, "merge: HashRollup needs w = 4, k = 3");
+        SG_REQUIRE(coeffs[0] == 1 && coeffs[1] == 2 && coeffs[2] == 3,
+                   "merge: only the reference combination coefficients (1, 2, 3)");
+        SG_REQUIRE(phi_mode >= 0 && phi_mode <= 2, "merge: bad phi_mode");
+        const size_t n0 = n_trace * blowup;
+        sg::Scoped dl(c, 8 * 4 * n0), dc(c, 8 * n0);
+        h2d(c, dl.p, lde, 8 * 4 * n0);
+        sg::merge_hash_rollup_dev(c, dl.as<uint64_t>(), n_trace, blowup, phi_mode, dc.as<uint64_t>());
+        d2h(c, c_eval, dc.p, 8 * n0);
+    });
+}
+
+int stark_deep_fri_prove(stark_ctx *ctx, const uint64_t *f0, size_t n0, const stark_fri_params *params,
+                         uint8_t **proof, size_t *proof_len) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(f0 && proof && proof_len, "deep_fri_prove: null buffer");
+        sg::FriParams p = to_params(params);
+        SG_REQUIRE(sg::is_pow2(n0), "deep_fri_prove: n0 must be a power of two");
+        sg::Scoped d(c, 8 * n0);
+        h2d(c, d.p, f0, 8 * n0);
+        std::vector<uint8_t> v = sg::deep_fri_prove_dev(c, d.as<uint64_t>(), n0, p, nullptr);
+        *proof = copy_out(v, proof_len);
+    });
+}
+
+int stark_merkle_commit(stark_ctx *ctx, const uint64_t *leaf_vals, size_t n_leaves, size_t vals_per_leaf,
+                        size_t arity, uint64_t tree_label, uint8_t root[32]) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(leaf_vals && root && n_leaves >= 1 && arity >= 2 && arity <= 128, "merkle_commit: bad arguments");
+        sg::Scoped d(c, 8 * n_leaves * vals_per_leaf + 8);
+        h2d(c, d.p, leaf_vals, 8 * n_leaves * vals_per_leaf);
+        sg::DevTree t;
+        sg::LeafSrc src{d.as<uint64_t>(), vals_per_leaf, 1, (uint32_t)vals_per_leaf, (uint32_t)vals_per_leaf};
+        sg::tree_build(c, t, src, n_leaves, (uint32_t)arity, sg::merkle_depth(n_leaves, (uint32_t)arity), tree_label);
+        memcpy(root, t.root, 32);
+        sg::tree_free(c, t);
+    });
+}
+
+int stark_prove_inner_shard(stark_ctx *ctx, const uint8_t salt[16], const stark_dns_record *records, size_t n,
+                            const uint8_t fs_binding[32], int ldt, int phi_mode, stark_inner_shard_out *out) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(salt && records && fs_binding && out, "prove_inner_shard: null argument");
+        if (ldt != STARK_LDT_STIR) throw sg::Error(sg::E_UNSUPPORTED, "only LdtMode::Stir is implemented");
+        std::vector<sg::DnsRec> recs(n);
+        for (size_t i = 0; i < n; i++)
+            recs[i] = {records[i].domain, records[i].domain_len, records[i].record_type, records[i].ttl,
+                       records[i].rdata, records[i].rdata_len};
+        sg::InnerShard s = sg::prove_inner_shard(c, salt, recs, fs_binding, phi_mode);
+        memcpy(out->pi_hash, s.pi_hash, 32);
+        memcpy(out->merkle_root, s.merkle_root, 32);
+        memcpy(out->root_f0, s.root_f0, 32);
+        out->record_count = s.record_count;
+        out->n_trace = s.n_trace;
+        out->proof = copy_out(s.proof, &out->proof_len);
+    });
+}
+
+int stark_prove_outer_rollup(stark_ctx *ctx, const uint8_t *pi_hashes, size_t n, const uint8_t pk_hash[32], int ldt,
+                             int phi_mode, uint8_t **proof, size_t *proof_len, uint8_t root_f0[32]) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE((pi_hashes || n == 0) && pk_hash && proof && proof_len && root_f0, "prove_outer_rollup: null argument");
+        if (ldt != STARK_LDT_STIR) throw sg::Error(sg::E_UNSUPPORTED, "only LdtMode::Stir is implemented");
+        std::vector<uint8_t> v = sg::prove_outer_rollup(c, pi_hashes, n, pk_hash, phi_mode, root_f0);
+        *proof = copy_out(v, proof_len);
+    });
+}
+
+int stark_dev_alloc(stark_ctx *ctx, size_t bytes, void **ptr) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(ptr, "dev_alloc: null out pointer");
+        *ptr = c->alloc(bytes);
+    });
+}
+
+int stark_dev_free(stark_ctx *ctx, void *ptr) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_HIP(hipStreamSynchronize(c->stream));
+        c->release(ptr);
+    });
+}
+
+int stark_memcpy_h2d(stark_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        h2d(c, dst, src, bytes);
+        SG_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int stark_memcpy_d2h(stark_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    return guard(ctx, [&](sg::Ctx *c) { d2h(c, dst, src, bytes); });
+}
+
+int stark_synchronize(stark_ctx *ctx) {
+    return guard(ctx, [&](sg::Ctx *c) { SG_HIP(hipStreamSynchronize(c->stream)); });
+}
+
+int stark_ntt_dev(stark_ctx *ctx, const uint64_t *d_in, uint64_t *d_out, size_t log_n, size_t ncols, int inverse) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(d_in && d_out && ncols >= 1 && log_n <= 24, "ntt: bad arguments");
+        const uint64_t n = 1ULL << log_n;
+        sg::Scoped scr(c, 8 * n * ncols);
+        sg::ntt_columns(c, d_in, d_out, scr.as<uint64_t>(), (uint32_t)log_n, ncols, n, n, inverse != 0, nullptr);
+        SG_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int stark_lde_columns_dev(stark_ctx *ctx, const uint64_t *d_cols, size_t w, size_t n_trace, size_t blowup,
+                          uint64_t shift, uint64_t *d_out) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(d_cols && d_out && w > 0, "lde: null buffer or no columns");
+        SG_REQUIRE(sg::is_pow2(n_trace), "lde: trace_len must be a power of 2");
+        SG_REQUIRE(blowup >= 2 && sg::is_pow2(blowup), "lde: blowup must be a power-of-2 >= 2");
+        SG_REQUIRE(shift != 0 && shift < sg::P, "lde: shift must be a nonzero canonical field element");
+        sg::lde_columns(c, d_cols, d_out, sg::ilog2(n_trace), sg::ilog2(blowup), w, n_trace, n_trace * blowup, shift);
+        SG_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int stark_prover_core_dev(stark_ctx *ctx, const uint64_t *d_trace, size_t w, size_t n_trace, size_t blowup,
+                          uint64_t shift, const stark_fri_params *params, uint64_t *d_lde, uint8_t row_root[32],
+                          uint8_t **proof, size_t *proof_len) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(d_trace && d_lde && row_root && proof && proof_len, "prover_core: null argument");
+        SG_REQUIRE(shift != 0 && shift < sg::P, "prover_core: shift must be a nonzero canonical field element");
+        sg::FriParams p = to_params(params);
+        std::vector<uint8_t> v = sg::prover_core_dev(c, d_trace, w, n_trace, blowup, shift, p, d_lde, row_root);
+        *proof = copy_out(v, proof_len);
+    });
+}
+
+int stark_profile_enable(stark_ctx *ctx, int on) {
+    if (!ctx || !ctx->c) return STARK_E_INVALID;
+    ctx->c->profiling = on != 0;
+    return STARK_OK;
+}
+
+int stark_profile_reset(stark_ctx *ctx) {
+    if (!ctx || !ctx->c) return STARK_E_INVALID;
+    ctx->c->stats.clear();
+    return STARK_OK;
+}
+
+int stark_profile_get(stark_ctx *ctx, const char *name, double *ms, uint64_t *launches, double *bytes) {
+    if (!ctx || !ctx->c || !name) return STARK_E_INVALID;
+    auto it = ctx->c->stats.find(name);
+    sg::KernelStat s;
+    if (it != ctx->c->stats.end()) s = it->second;
+    if (ms) *ms = s.ms;
+    if (launches) *launches = s.launches;
+    if (bytes) *bytes = s.bytes;
+    return STARK_OK;
+}
+
+size_t stark_profile_names(stark_ctx *ctx, char *buf, size_t cap) {
+    if (!ctx || !ctx->c) return 0;
+    std::string all;
+    for (auto &kv : ctx->c->stats) all += kv.first + "\n";
+    if (buf && cap) {
+        size_t n = all.size() < cap - 1 ? all.size() : cap - 1;
+        memcpy(buf, all.data(), n);
+        buf[n] = 0;
+    }
+    return all.size();
+}
+
+void stark_host_sha3_256(const uint8_t *msg, size_t len, uint8_t out[32]) { sg::sha3_256_host(msg, len, out); }
+
+uint64_t stark_host_stdrng_first_u64(uint64_t seed) { return sg::stdrng_first_u64(seed); }
+
+void stark_host_dns_leaf_hash(const stark_dns_record *r, const uint8_t salt[16], uint8_t out[32]) {
+    sg::DnsRec d{r->domain, r->domain_len, r->record_type, r->ttl, r->rdata, r->rdata_len};
+    sg::dns_leaf_hash(d, salt, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,121 @@
+// ctx.hpp -- per-context state of the gfx950 proving library: one HIP stream,
+// device-resident twiddle tables, a caching device allocator, optional
+// per-kernel event timing, and the error string the C ABI reports.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace sg {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+enum Status {
+    OK = 0,
+    E_INVALID = -1,
+    E_OOM = -2,
+    E_DEVICE = -3,
+    E_UNSUPPORTED = -4,
+    E_INTERNAL = -5,
+};
+
+#define SG_HIP(expr)                                                                           \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            throw ::sg::Error(_e == hipErrorOutOfMemory ? ::sg::E_OOM : ::sg::E_DEVICE,        \
+                              std::string(#expr) + ": " + hipGetErrorString(_e));             \
+    } while (0)
+
+#define SG_REQUIRE(cond, msg)                                     \
+    do {                                                          \
+        if (!(cond)) throw ::sg::Error(::sg::E_INVALID, (msg));  \
+    } while (0)
+
+// Device buffer owned by a context's allocator.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct KernelStat {
+    double ms = 0;
+    uint64_t launches = 0;
+    double bytes = 0;  // algorithmic bytes credited to the launches
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+
+    // root tables: w26 = omega_{2^26}; rt_lo[e & 8191] * rt_hi[e >> 13] = w26^e
+    uint64_t *rt_lo = nullptr, *rt_hi = nullptr;
+    // tw_f[(1 << k) + t] = omega_{2^k}^t, tw_i = inverse, k <= 12
+    uint64_t *tw_f = nullptr, *tw_i = nullptr;
+
+    // caching allocator: freed blocks are kept by size for reuse
+    std::multimap<size_t, void *> free_blocks;
+    std::map<void *, size_t> live;
+    size_t cached_bytes = 0;
+
+    // profiling
+    bool profiling = false;
+    struct Pending {
+        std::string name;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    std::vector<Pending> pending;
+    std::map<std::string, KernelStat> stats;
+
+    void *alloc(size_t bytes);
+    void release(void *p);
+    void trim();
+    template <class T>
+    T *alloc_n(size_t n) { return (T *)alloc(n * sizeof(T)); }
+
+    // profiling helpers: begin/end around one launch on `stream`
+    int prof_begin(const char *name, double bytes);
+    void prof_end(int token);
+    void prof_collect();  // synchronizes and folds pending events into stats
+};
+
+// RAII holder for a context allocation
+struct Scoped {
+    Ctx *c;
+    void *p;
+    Scoped(Ctx *c_, size_t bytes) : c(c_), p(c_->alloc(bytes)) {}
+    ~Scoped() { if (p) c->release(p); }
+    Scoped(const Scoped &) = delete;
+    Scoped &operator=(const Scoped &) = delete;
+    template <class T>
+    T *as() const { return (T *)p; }
+};
+
+// RAII profiler scope
+struct ProfScope {
+    Ctx *c;
+    int tok;
+    ProfScope(Ctx *c_, const char *name, double bytes) : c(c_), tok(c_->profiling ? c_->prof_begin(name, bytes) : -1) {}
+    ~ProfScope() { if (tok >= 0) c->prof_end(tok); }
+};
+
+Ctx *ctx_create(int device);
+void ctx_destroy(Ctx *c);
+
+inline uint32_t ilog2(uint64_t n) {
+    uint32_t l = 0;
+    while ((1ULL << l) < n) l++;
+    return l;
+}
+inline bool is_pow2(uint64_t n) { return n && !(n & (n - 1)); }
+
+}  // namespace sg

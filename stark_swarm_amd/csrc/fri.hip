@@ -1,0 +1,550 @@
+// fri.hip -- STIR DEEP-FRI prover for gfx950.
+//
+// Device work per layer l (size n, fold arity m), crates/deep_ali/src/fri.rs:
+//   OOD coset   f_l(zeta^j z) for j < m (fri.rs:338-391). With c = iNTT(f_l)
+//               and zeta = w_n^(n/m) a primitive m-th root,
+//               f_l(zeta^j z) = sum_rho zeta^(j rho) S_rho,
+//               S_rho = sum_{k = rho mod m} c_k z^k,
+//               so one streaming pass forms the m residue sums and an m-point
+//               DFT finishes (`ood_sums`, `ood_finish`).
+//   commit      SHA3 leaves of the 6 components, arity pick_arity(n, m).
+//   fold        per coset b: interpolant coefficients of the fibre
+//               f[b + j n/m] on w^b zeta^j (m-point inverse DFT, 1/m,
+//               w^(-bk)), then f_{l+1}[b] = sum_k coeff_k alpha^k
+//               (fri.rs:670-771, `coset_fold`).
+// The transcript, challenges and query indices are host work (fri.rs:1398-1718,
+// 1813-1893); values cross PCIe only as roots, m coset evaluations, the final
+// layer and the query gathers.
+//
+// Layout: an Fp6 layer is 6 planes of n u64 (component-major), so every
+// kernel reads unit-stride streams and the NTT engine runs on the planes.
+#include "fri.hpp"
+#include "gl.hpp"
+#include "keccak.hpp"
+#include "ntt.hpp"
+
+#include <string.h>
+
+#include <array>
+
+namespace sg {
+
+namespace {
+
+constexpr int FT = 256;
+
+struct F6Arg {
+    uint64_t c[6];
+};
+static F6Arg f6arg(const fp6 &v) {
+    F6Arg a;
+    memcpy(a.c, v.c, 48);
+    return a;
+}
+
+// zpow[t] = z^t for t < CH (host-computed, device copy) ; block partial sums
+// of S_rho over chunks of CH coefficients per thread.
+template <int M>
+__global__ __launch_bounds__(FT) void ood_sums_k(const uint64_t *coef, uint64_t n, int ncomp, const uint64_t *zpow,
+                                                 uint32_t ch, F6Arg zch, uint64_t *partials) {
+    // thread chunk [k0, k0+ch); acc_rho = z^k0 * sum_t c_{k0+t} z^t, t = rho mod M (ch multiple of M, k0 too)
+    const uint64_t tid = (uint64_t)blockIdx.x * FT + threadIdx.x;
+    const uint64_t k0 = tid * ch;
+    fp6 acc[M];
+#pragma unroll
+    for (int r = 0; r < M; r++) acc[r] = f6_zero();
+    if (k0 < n) {
+        for (uint32_t t = 0; t < ch; t += M) {
+#pragma unroll
+            for (int r = 0; r < M; r++) {
+                const uint64_t k = k0 + t + r;
+                fp6 zp;
+                memcpy(zp.c, zpow + 6 * (t + r), 48);
+                if (ncomp == 1) {
+                    acc[r] = f6_add(acc[r], f6_scale(zp, coef[k]));
+                } else {
+                    fp6 cv;
+#pragma unroll
+                    for (int q = 0; q < 6; q++) cv.c[q] = coef[q * n + k];
+                    acc[r] = f6_add(acc[r], f6_mul(cv, zp));
+                }
+            }
+        }
+        // multiply by z^k0 = (z^ch)^tid
+        fp6 zk = f6_pow(*(const fp6 *)&zch, tid);
+#pragma unroll
+        for (int r = 0; r < M; r++) acc[r] = f6_mul(acc[r], zk);
+    }
+    // block reduction through LDS, one residue at a time
+    __shared__ uint64_t red[FT * 6];
+    for (int r = 0; r < M; r++) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) red[q * FT + threadIdx.x] = acc[r].c[q];
+        __syncthreads();
+        for (int s = FT / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) {
+#pragma unroll
+                for (int q = 0; q < 6; q++)
+                    red[q * FT + threadIdx.x] = add(red[q * FT + threadIdx.x], red[q * FT + threadIdx.x + s]);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x < 6) partials[((uint64_t)blockIdx.x * M + r) * 6 + threadIdx.x] = red[threadIdx.x * FT];
+        __syncthreads();
+    }
+}
+
+// Single block: sum block partials, then evals_j = sum_rho zeta^(j rho) S_rho.
+__global__ void ood_finish_k(const uint64_t *partials, uint32_t nblocks, uint32_t m, uint64_t zeta, uint64_t *evals) {
+    __shared__ uint64_t S[16 * 6];
+    const uint32_t t = threadIdx.x;  // t < 6*m : (rho, comp)
+    if (t < 6 * m) {
+        uint32_t rho = t / 6, q = t % 6;
+        uint64_t s = 0;
+        for (uint32_t b = 0; b < nblocks; b++) s = add(s, partials[((uint64_t)b * m + rho) * 6 + q]);
+        S[t] = s;
+    }
+    __syncthreads();
+    if (t < 6 * m) {
+        uint32_t j = t / 6, q = t % 6;
+        uint64_t zj = pow(zeta, j), zp = 1, s = 0;
+        for (uint32_t rho = 0; rho < m; rho++) {
+            s = add(s, mul(S[rho * 6 + q], zp));
+            zp = mul(zp, zj);
+        }
+        evals[j * 6 + q] = s;
+    }
+}
+
+constexpr uint64_t cmulc(uint64_t a, uint64_t b) { return (uint64_t)(((unsigned __int128)a * b) % P); }
+constexpr uint64_t cpowc(uint64_t a, uint64_t e) {
+    uint64_t r = 1;
+    while (e) {
+        if (e & 1) r = cmulc(r, a);
+        a = cmulc(a, a);
+        e >>= 1;
+    }
+    return r;
+}
+struct RootI16 {
+    uint64_t i[16];
+};
+constexpr RootI16 make_rooti16() {
+    RootI16 t{};
+    uint64_t wi = cpowc(cpowc(7, (P - 1) / 16), 15), a = 1;
+    for (int k = 0; k < 16; k++) {
+        t.i[k] = a;
+        a = cmulc(a, wi);
+    }
+    return t;
+}
+__device__ constexpr RootI16 ROOTI16 = make_rooti16();
+
+// inverse DFT (unscaled) of size M = 2^LM on one component: v[k] <- sum_j v[j] zeta^(-jk)
+template <int LM>
+__device__ __forceinline__ void idft_small(uint64_t *v) {
+    constexpr int M = 1 << LM;
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+        int j = 0;
+#pragma unroll
+        for (int b = 0; b < LM; b++) j |= ((i >> b) & 1) << (LM - 1 - b);
+        if (i < j) {
+            uint64_t t = v[i];
+            v[i] = v[j];
+            v[j] = t;
+        }
+    }
+#pragma unroll
+    for (int lh = 0; lh < LM; lh++) {
+        const int half = 1 << lh;
+#pragma unroll
+        for (int base = 0; base < M; base += 2 * half) {
+#pragma unroll
+            for (int j = 0; j < half; j++) {
+                uint64_t t = v[base + j + half];
+                if (j) t = mul(t, ROOTI16.i[j * (16 >> (lh + 1))]);
+                uint64_t u = v[base + j];
+                v[base + j] = add(u, t);
+                v[base + j + half] = sub(u, t);
+            }
+        }
+    }
+}
+
+// f_next[b] = sum_k alpha^k * (1/m) w^(-bk) * IDFT_m(f[b + j nn])_k ; optional tuple planes
+template <int LM>
+__global__ __launch_bounds__(FT) void coset_fold_k(const uint64_t *f, uint64_t n, int ncomp, F6Arg alpha,
+                                                   uint64_t minv, const uint64_t *rt_lo, const uint64_t *rt_hi,
+                                                   uint32_t logn, uint64_t *fnext, uint64_t *tuples) {
+    constexpr int M = 1 << LM;
+    const uint64_t nn = n >> LM;
+    const uint64_t b = (uint64_t)blockIdx.x * FT + threadIdx.x;
+    if (b >= nn) return;
+    uint64_t v[6][M];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+#pragma unroll
+        for (int j = 0; j < M; j++) v[q][j] = (q < ncomp) ? f[q * n + b + j * nn] : 0;
+        if (q < ncomp) idft_small<LM>(v[q]);
+    }
+    // w^(-b) from the 2^26 root table
+    uint64_t e26 = (b << (26 - logn)) & ((1ULL << 26) - 1);
+    e26 = (0 - e26) & ((1ULL << 26) - 1);
+    const uint64_t wb = mul(rt_lo[e26 & 8191], rt_hi[e26 >> 13]);
+    uint64_t s = minv;  // (1/m) w^(-bk)
+    fp6 y[M];
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) y[k].c[q] = (q < ncomp) ? mul(v[q][k], s) : 0;
+        s = mul(s, wb);
+    }
+    if (tuples) {
+#pragma unroll
+        for (int k = 0; k < M; k++)
+#pragma unroll
+            for (int q = 0; q < 6; q++) tuples[(uint64_t)(k * 6 + q) * nn + b] = y[k].c[q];
+    }
+    const fp6 a = *(const fp6 *)&alpha;
+    fp6 acc = y[M - 1];
+#pragma unroll
+    for (int k = M - 2; k >= 0; k--) acc = f6_add(f6_mul(acc, a), y[k]);
+#pragma unroll
+    for (int q = 0; q < 6; q++) fnext[q * nn + b] = acc.c[q];
+}
+
+template <int LM>
+void launch_fold(Ctx *c, const uint64_t *f, uint64_t n, int ncomp, const fp6 &alpha, uint64_t *fnext,
+                 uint64_t *tuples) {
+    const uint64_t nn = n >> LM;
+    const uint64_t minv = inv(1ULL << LM);
+    ProfScope ps(c, "coset_fold", 8.0 * ncomp * n + 48.0 * nn + (tuples ? 48.0 * n : 0));
+    hipLaunchKernelGGL(coset_fold_k<LM>, dim3((uint32_t)((nn + FT - 1) / FT)), dim3(FT), 0, c->stream, f, n, ncomp,
+                       f6arg(alpha), minv, c->rt_lo, c->rt_hi, ilog2(n), fnext, tuples);
+    SG_HIP(hipGetLastError());
+}
+
+void fold(Ctx *c, const uint64_t *f, uint64_t n, int ncomp, uint64_t m, const fp6 &alpha, uint64_t *fnext,
+          uint64_t *tuples) {
+    switch (m) {
+    case 2: launch_fold<1>(c, f, n, ncomp, alpha, fnext, tuples); break;
+    case 4: launch_fold<2>(c, f, n, ncomp, alpha, fnext, tuples); break;
+    case 8: launch_fold<3>(c, f, n, ncomp, alpha, fnext, tuples); break;
+    case 16: launch_fold<4>(c, f, n, ncomp, alpha, fnext, tuples); break;
+    default: throw Error(E_UNSUPPORTED, "fold arity must be 2, 4, 8 or 16");
+    }
+}
+
+template <int M>
+void launch_ood(Ctx *c, const uint64_t *coef, uint64_t n, int ncomp, const fp6 &z, uint64_t zeta, uint64_t *evals_host) {
+    // chunk per thread: a multiple of M, small enough to keep the grid wide
+    uint32_t ch = 64;
+    while (ch > (uint32_t)M && (n / ch) < 4096) ch >>= 1;
+    if (ch < (uint32_t)M) ch = M;
+    const uint64_t threads = (n + ch - 1) / ch;
+    const uint32_t nblocks = (uint32_t)((threads + FT - 1) / FT);
+    std::vector<uint64_t> zp(6 * ch);
+    fp6 a = f6_one();
+    for (uint32_t t = 0; t < ch; t++) {
+        memcpy(&zp[6 * t], a.c, 48);
+        a = f6_mul(a, z);
+    }
+    fp6 zch = a;  // z^ch
+    Scoped d_zp(c, 48 * ch), d_part(c, 48ULL * M * nblocks), d_ev(c, 48 * M);
+    SG_HIP(hipMemcpyAsync(d_zp.p, zp.data(), 48 * ch, hipMemcpyHostToDevice, c->stream));
+    {
+        ProfScope ps(c, "ood_sums", 8.0 * ncomp * n);
+        hipLaunchKernelGGL(ood_sums_k<M>, dim3(nblocks), dim3(FT), 0, c->stream, coef, n, ncomp, d_zp.as<uint64_t>(), ch,
+                           f6arg(zch), d_part.as<uint64_t>());
+        SG_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(ood_finish_k, dim3(1), dim3(128), 0, c->stream, d_part.as<uint64_t>(), nblocks, (uint32_t)M, zeta,
+                       d_ev.as<uint64_t>());
+    SG_HIP(hipGetLastError());
+    SG_HIP(hipMemcpyAsync(evals_host, d_ev.p, 48 * M, hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+}
+
+void ood(Ctx *c, const uint64_t *coef, uint64_t n, int ncomp, uint64_t m, const fp6 &z, uint64_t zeta,
+         uint64_t *evals_host) {
+    switch (m) {
+    case 2: launch_ood<2>(c, coef, n, ncomp, z, zeta, evals_host); break;
+    case 4: launch_ood<4>(c, coef, n, ncomp, z, zeta, evals_host); break;
+    case 8: launch_ood<8>(c, coef, n, ncomp, z, zeta, evals_host); break;
+    case 16: launch_ood<16>(c, coef, n, ncomp, z, zeta, evals_host); break;
+    default: throw Error(E_UNSUPPORTED, "fold arity must be 2, 4, 8 or 16");
+    }
+}
+
+struct Bytes {
+    std::vector<uint8_t> v;
+    void put(const void *d, size_t n) { v.insert(v.end(), (const uint8_t *)d, (const uint8_t *)d + n); }
+    void u8(uint8_t x) { v.push_back(x); }
+    void u64(uint64_t x) {
+        for (int i = 0; i < 8; i++) v.push_back((uint8_t)(x >> (8 * i)));
+    }
+    void f6(const uint64_t *c) {
+        for (int i = 0; i < 6; i++) u64(c[i]);
+    }
+};
+
+}  // namespace
+
+std::vector<uint64_t> stir_schedule(uint64_t n0) {
+    uint32_t lg = ilog2(n0);
+    std::vector<uint64_t> s(lg / 3, 8);
+    if (lg % 3) s.push_back(1ULL << (lg % 3));
+    return s;
+}
+
+std::vector<uint8_t> deep_fri_prove_dev(Ctx *c, const uint64_t *d_f0, uint64_t n0, const FriParams &p,
+                                        std::vector<uint64_t> *raw_out) {
+    const size_t L = p.schedule.size();
+    if (!p.stir || L == 0) throw Error(E_UNSUPPORTED, "only the STIR low-degree test is implemented");
+    SG_REQUIRE(is_pow2(n0) && n0 >= 2 && n0 <= (1ULL << 24), "deep_fri_prove: n0 must be a power of two in [2, 2^24]");
+    {
+        uint64_t n = n0;
+        for (uint64_t m : p.schedule) {
+            SG_REQUIRE(m >= 2 && m <= 16 && is_pow2(m) && n % m == 0, "deep_fri_prove: bad fold schedule");
+            n /= m;
+        }
+    }
+    const bool use_cc = p.coeff_commit_final;
+
+    Transcript tr("FRI/FS");
+    if (p.has_pi) {
+        tr.absorb("CAIRO-PUBLIC-INPUTS-V1");
+        tr.absorb(p.pi, 32);
+    }
+    tr.absorb("DEEP-FRI-STATEMENT-V2");
+    tr.absorb_field(n0);
+    tr.absorb_field(L);
+    for (uint64_t m : p.schedule) tr.absorb_field(m);
+    tr.absorb_field(canon(p.seed_z));
+    tr.absorb_field(6);
+    tr.absorb_field(p.coeff_commit_final ? 1 : 0);
+    tr.absorb_field(p.stir ? 1 : 0);
+
+    // coset-packed f0 tree: leaf b = fibre f0[b + j n0/m0] (fri.rs:1420-1431)
+    const uint64_t m0 = p.schedule[0], nn0 = n0 / m0;
+    DevTree t0;
+    {
+        uint32_t a = pick_arity(nn0, 16);
+        LeafSrc src{d_f0, 1, nn0, (uint32_t)m0, (uint32_t)m0};
+        tree_build(c, t0, src, nn0, a, merkle_depth(nn0, a), 0xFF);
+    }
+    tr.absorb(t0.root, 32);
+    fp6 z;
+    tr.challenge_ext("z_fp3", z.c);
+
+    // layers
+    std::vector<std::vector<uint64_t>> coset_evals(L);
+    std::vector<fp6> fz(L);
+    std::vector<std::array<uint8_t, 32>> roots(L);
+    std::vector<uint64_t> tuples_h;
+    uint8_t coeff_root[32] = {0};
+    DevTree t1;
+    bool have_t1 = false;
+
+    uint64_t cur = n0;
+    const uint64_t *f = d_f0;  // layer values, planes
+    int ncomp = 1;
+    uint64_t *owned = nullptr;      // current layer planes we allocated
+    uint64_t *layer1 = nullptr;     // f_1 planes, kept for the queries
+    for (size_t ell = 0; ell < L; ell++) {
+        const uint64_t m = p.schedule[ell], nn = cur / m;
+        const bool final_cc = use_cc && ell == L - 1;
+        fp6 alpha = f6_zero();
+        if (!final_cc) tr.challenge_ext("alpha", alpha.c);
+        // OOD coset evaluations
+        {
+            Scoped coef(c, 8 * ncomp * cur), scr(c, 8 * ncomp * cur);
+            ntt_columns(c, f, coef.as<uint64_t>(), scr.as<uint64_t>(), ilog2(cur), ncomp, cur, cur, true, nullptr);
+            coset_evals[ell].resize(6 * m);
+            uint64_t zeta = pow(root_of_unity(cur), nn);
+            ood(c, coef.as<uint64_t>(), cur, ncomp, m, z, zeta, coset_evals[ell].data());
+        }
+        memcpy(fz[ell].c, coset_evals[ell].data(), 48);
+        z = f6_pow(z, m);
+        // layer commitment: leaf = 6 components (fri.rs:1364-1366)
+        DevTree tl;
+        {
+            uint32_t a = pick_arity(cur, m);
+            // a base-field layer (f0) hashes components 1..5 as zeros
+            LeafSrc src{f, 1, cur, 6, (uint32_t)ncomp};
+            tree_build(c, tl, src, cur, a, merkle_depth(cur, a), ell);
+        }
+        memcpy(roots[ell].data(), tl.root, 32);
+        for (uint64_t j = 0; j < m; j++) {
+            for (int q = 0; q < 6; q++) tr.absorb_field(coset_evals[ell][6 * j + q]);
+        }
+        tr.absorb(tl.root, 32);
+        if (ell == 1) {
+            t1 = tl;
+            have_t1 = true;
+        } else {
+            tree_free(c, tl);
+        }
+        uint64_t *next = c->alloc_n<uint64_t>(6 * nn);
+        if (final_cc) {
+            Scoped tup(c, 8 * 6 * m * nn);
+            fold(c, f, cur, ncomp, m, f6_zero(), next, tup.as<uint64_t>());
+            const uint32_t a = pick_arity(nn, 16);
+            DevTree tc;
+            LeafSrc src{tup.as<uint64_t>(), 1, nn, (uint32_t)(6 * m), (uint32_t)(6 * m)};
+            tree_build(c, tc, src, nn, a, merkle_depth(nn, a), 0xFE);
+            memcpy(coeff_root, tc.root, 32);
+            tree_free(c, tc);
+            std::vector<uint64_t> planes(6 * m * nn);
+            SG_HIP(hipMemcpyAsync(planes.data(), tup.p, 8 * planes.size(), hipMemcpyDeviceToHost, c->stream));
+            SG_HIP(hipStreamSynchronize(c->stream));
+            tuples_h.resize(6 * m * nn);  // [b][k][q]
+            for (uint64_t b = 0; b < nn; b++)
+                for (uint64_t kq = 0; kq < 6 * m; kq++) tuples_h[b * 6 * m + kq] = planes[kq * nn + b];
+            tr.absorb(coeff_root, 32);
+            tr.challenge_ext("alpha", alpha.c);
+            fp6 beta;
+            tr.challenge_ext("beta_deg", beta.c);
+            // fold of the final layer with the post-commit alpha
+            fold(c, f, cur, ncomp, m, alpha, next, nullptr);
+        } else {
+            fold(c, f, cur, ncomp, m, alpha, next, nullptr);
+        }
+        if (ell == 0) {
+            layer1 = next;
+        } else if (owned && owned != layer1) {
+            c->release(owned);
+        }
+        owned = next;
+        f = next;
+        ncomp = 6;
+        cur = nn;
+    }
+
+    // final polynomial: first d_final coefficients of the last layer (fri.rs:1919-1937)
+    const uint64_t nf = cur;
+    std::vector<uint64_t> fin(6 * nf);
+    {
+        Scoped coef(c, 8 * 6 * nf), scr(c, 8 * 6 * nf);
+        const uint64_t *src = f;
+        if (nf > 1) {
+            ntt_columns(c, f, coef.as<uint64_t>(), scr.as<uint64_t>(), ilog2(nf), 6, nf, nf, true, nullptr);
+            src = coef.as<uint64_t>();
+        }
+        SG_HIP(hipMemcpyAsync(fin.data(), src, 8 * 6 * nf, hipMemcpyDeviceToHost, c->stream));
+        SG_HIP(hipStreamSynchronize(c->stream));
+    }
+    const uint64_t dfin = p.d_final < nf ? p.d_final : nf;
+    std::vector<fp6> final_coeffs(dfin);
+    for (uint64_t k = 0; k < dfin; k++)
+        for (int q = 0; q < 6; q++) final_coeffs[k].c[q] = fin[q * nf + k];
+    for (auto &fc : final_coeffs)
+        for (int q = 0; q < 6; q++) tr.absorb_field(fc.c[q]);
+    const uint64_t query_seed = tr.challenge_field("query_seed");
+
+    // STIR proximity queries (fri.rs:1813-1893)
+    uint64_t npow2 = 1;
+    while (npow2 < n0) npow2 <<= 1;
+    std::vector<uint64_t> raw(p.s0), base(p.s0);
+    for (uint64_t q = 0; q < p.s0; q++) {
+        uint64_t fl[3] = {query_seed, 0, canon(q)};
+        uint64_t seed = hash_fields_tagged("FRI/index", fl, 3);
+        raw[q] = (stdrng_first_u64(seed) & (npow2 - 1)) % n0;
+        base[q] = raw[q] % nn0;
+    }
+    if (raw_out) *raw_out = raw;
+    // gathers: fibre values of f0 and f_1[base]
+    std::vector<uint64_t> gidx;
+    for (uint64_t q = 0; q < p.s0; q++)
+        for (uint64_t j = 0; j < m0; j++) gidx.push_back(base[q] + j * nn0);
+    const bool l2 = L >= 2;
+    std::vector<uint64_t> fib(gidx.size()), f1v(6 * p.s0);
+    {
+        Scoped di(c, 8 * (gidx.size() + 1)), dv(c, 8 * (gidx.size() + 1));
+        SG_HIP(hipMemcpyAsync(di.p, gidx.data(), 8 * gidx.size(), hipMemcpyHostToDevice, c->stream));
+        gather_u64(c, d_f0, di.as<uint64_t>(), gidx.size(), dv.as<uint64_t>());
+        SG_HIP(hipMemcpyAsync(fib.data(), dv.p, 8 * gidx.size(), hipMemcpyDeviceToHost, c->stream));
+        SG_HIP(hipStreamSynchronize(c->stream));
+    }
+    if (l2) {
+        const uint64_t n1 = n0 / m0;
+        std::vector<uint64_t> gi;
+        for (int q6 = 0; q6 < 6; q6++)
+            for (uint64_t q = 0; q < p.s0; q++) gi.push_back(q6 * n1 + base[q]);
+        Scoped di(c, 8 * gi.size()), dv(c, 8 * gi.size());
+        SG_HIP(hipMemcpyAsync(di.p, gi.data(), 8 * gi.size(), hipMemcpyHostToDevice, c->stream));
+        gather_u64(c, layer1, di.as<uint64_t>(), gi.size(), dv.as<uint64_t>());
+        std::vector<uint64_t> tmp(gi.size());
+        SG_HIP(hipMemcpyAsync(tmp.data(), dv.p, 8 * gi.size(), hipMemcpyDeviceToHost, c->stream));
+        SG_HIP(hipStreamSynchronize(c->stream));
+        for (uint64_t q = 0; q < p.s0; q++)
+            for (int q6 = 0; q6 < 6; q6++) f1v[6 * q + q6] = tmp[q6 * p.s0 + q];
+    }
+    std::vector<std::vector<uint8_t>> op0, op1;
+    tree_open_batch(c, t0, base, op0);
+    if (l2) tree_open_batch(c, t1, base, op1);
+
+    // serialize DeepFriProof<SexticExt> (fri.rs:1286-1300, ark-serialize compressed)
+    Bytes b;
+    b.put(t0.root, 32);
+    b.u64(L);
+    for (auto &r : roots) b.put(r.data(), 32);
+    b.u64(0);  // layer_proofs.layers
+    b.u64(0);  // f0_openings
+    b.u64(0);  // queries
+    b.u64(L);
+    for (auto &v : fz) b.f6(v.c);
+    b.u64(final_coeffs.size());
+    for (auto &v : final_coeffs) b.f6(v.c);
+    b.u64(n0);
+    b.u64(root_of_unity(n0));
+    if (use_cc) {
+        const uint64_t m = p.schedule[L - 1];
+        const uint64_t nt = tuples_h.size() / (6 * m);
+        b.u8(1);
+        b.u64(nt);
+        for (uint64_t t = 0; t < nt; t++) {
+            b.u64(m);
+            for (uint64_t k = 0; k < m; k++) b.f6(&tuples_h[(t * m + k) * 6]);
+        }
+        b.u8(1);
+        b.put(coeff_root, 32);
+    } else {
+        b.u8(0);
+        b.u8(0);
+    }
+    b.u8(1);
+    b.u64(L);
+    for (size_t l = 0; l < L; l++) {
+        b.u64(p.schedule[l]);
+        for (uint64_t j = 0; j < p.schedule[l]; j++) b.f6(&coset_evals[l][6 * j]);
+    }
+    b.u8(1);
+    b.u64(p.s0);
+    for (uint64_t q = 0; q < p.s0; q++) {
+        b.u64(base[q]);
+        b.u64(raw[q]);
+        b.u64(m0);
+        for (uint64_t j = 0; j < m0; j++) b.u64(base[q] + j * nn0);
+        b.u64(m0);
+        for (uint64_t j = 0; j < m0; j++) b.u64(fib[q * m0 + j]);
+        b.put(op0[q].data(), op0[q].size());
+        if (l2) {
+            b.f6(&f1v[6 * q]);
+            b.u8(1);
+            b.put(op1[q].data(), op1[q].size());
+        } else {
+            uint64_t zero6[6] = {0, 0, 0, 0, 0, 0};
+            b.f6(zero6);
+            b.u8(0);
+        }
+    }
+    tree_free(c, t0);
+    if (have_t1) tree_free(c, t1);
+    if (owned && owned != layer1) c->release(owned);
+    if (layer1) c->release(layer1);
+    return std::move(b.v);
+}
+
+}  // namespace sg

@@ -1,0 +1,396 @@
+// keccak.hip -- Keccak-f[1600] / SHA3-256 Merkle commitments on gfx950 and the
+// host-side SHA3 transcript used by the proof orchestration.
+//
+// Tree semantics follow crates/merkle/src/lib.rs:95-238,377-512: a leaf is
+// SHA3-256(DsLabel(arity, u32::MAX, position, label) || LE64 values), a node is
+// SHA3-256(DsLabel(arity, level, position, label) || children), and a level
+// whose length is not a multiple of the arity is padded by repeating its last
+// node. Every digest is 4 u64 lanes (the first 32 bytes of the state).
+//
+// One leaf or node per lane: the 25-lane state lives in VGPRs, message lanes
+// are read straight from HBM (consecutive lanes read consecutive leaves, so
+// each value column is a coalesced stream).
+#include "gl.hpp"
+#include "keccak.hpp"
+
+namespace sg {
+
+#define KECCAK_RC_LIST                                                                                         \
+    0x0000000000000001ULL, 0x0000000000008082ULL, 0x800000000000808aULL, 0x8000000080008000ULL,              \
+        0x000000000000808bULL, 0x0000000080000001ULL, 0x8000000080008081ULL, 0x8000000000008009ULL,          \
+        0x000000000000008aULL, 0x0000000000000088ULL, 0x0000000080008009ULL, 0x000000008000000aULL,          \
+        0x000000008000808bULL, 0x800000000000008bULL, 0x8000000000008089ULL, 0x8000000000008003ULL,          \
+        0x8000000000008002ULL, 0x8000000000000080ULL, 0x000000000000800aULL, 0x800000008000000aULL,          \
+        0x8000000080008081ULL, 0x8000000000008080ULL, 0x0000000080000001ULL, 0x8000000080008008ULL
+
+__device__ constexpr uint64_t KRC_D[24] = {KECCAK_RC_LIST};
+static const uint64_t KRC_H[24] = {KECCAK_RC_LIST};
+
+SG_HD uint64_t rotl64(uint64_t x, int n) { return n == 0 ? x : (x << n) | (x >> (64 - n)); }
+
+// One round on 25 named lanes: theta, rho+pi, chi, iota.
+SG_HD void keccak_round(uint64_t *a, uint64_t rc) {
+    uint64_t c0 = a[0] ^ a[5] ^ a[10] ^ a[15] ^ a[20];
+    uint64_t c1 = a[1] ^ a[6] ^ a[11] ^ a[16] ^ a[21];
+    uint64_t c2 = a[2] ^ a[7] ^ a[12] ^ a[17] ^ a[22];
+    uint64_t c3 = a[3] ^ a[8] ^ a[13] ^ a[18] ^ a[23];
+    uint64_t c4 = a[4] ^ a[9] ^ a[14] ^ a[19] ^ a[24];
+    uint64_t d0 = c4 ^ rotl64(c1, 1), d1 = c0 ^ rotl64(c2, 1), d2 = c1 ^ rotl64(c3, 1);
+    uint64_t d3 = c2 ^ rotl64(c4, 1), d4 = c3 ^ rotl64(c0, 1);
+    // rho + pi: b[y + 5*((2x+3y)%5)] = rotl(a[x + 5y] ^ d[x], r[x][y])
+    uint64_t b[25];
+    b[0] = a[0] ^ d0;
+    b[10] = rotl64(a[1] ^ d1, 1);
+    b[20] = rotl64(a[2] ^ d2, 62);
+    b[5] = rotl64(a[3] ^ d3, 28);
+    b[15] = rotl64(a[4] ^ d4, 27);
+    b[16] = rotl64(a[5] ^ d0, 36);
+    b[1] = rotl64(a[6] ^ d1, 44);
+    b[11] = rotl64(a[7] ^ d2, 6);
+    b[21] = rotl64(a[8] ^ d3, 55);
+    b[6] = rotl64(a[9] ^ d4, 20);
+    b[7] = rotl64(a[10] ^ d0, 3);
+    b[17] = rotl64(a[11] ^ d1, 10);
+    b[2] = rotl64(a[12] ^ d2, 43);
+    b[12] = rotl64(a[13] ^ d3, 25);
+    b[22] = rotl64(a[14] ^ d4, 39);
+    b[23] = rotl64(a[15] ^ d0, 41);
+    b[8] = rotl64(a[16] ^ d1, 45);
+    b[18] = rotl64(a[17] ^ d2, 15);
+    b[3] = rotl64(a[18] ^ d3, 21);
+    b[13] = rotl64(a[19] ^ d4, 8);
+    b[14] = rotl64(a[20] ^ d0, 18);
+    b[24] = rotl64(a[21] ^ d1, 2);
+    b[9] = rotl64(a[22] ^ d2, 61);
+    b[19] = rotl64(a[23] ^ d3, 56);
+    b[4] = rotl64(a[24] ^ d4, 14);
+#pragma unroll
+    for (int y = 0; y < 25; y += 5) {
+        uint64_t e0 = b[y], e1 = b[y + 1], e2 = b[y + 2], e3 = b[y + 3], e4 = b[y + 4];
+        a[y] = e0 ^ (~e1 & e2);
+        a[y + 1] = e1 ^ (~e2 & e3);
+        a[y + 2] = e2 ^ (~e3 & e4);
+        a[y + 3] = e3 ^ (~e4 & e0);
+        a[y + 4] = e4 ^ (~e0 & e1);
+    }
+    a[0] ^= rc;
+}
+
+__device__ __forceinline__ void keccak_f_dev(uint64_t *a) {
+#pragma unroll 4
+    for (int r = 0; r < 24; r++) keccak_round(a, KRC_D[r]);
+}
+
+void keccak_f1600_host(uint64_t s[25]) {
+    for (int r = 0; r < 24; r++) keccak_round(s, KRC_H[r]);
+}
+
+// ---------------------------------------------------------------- host SHA3
+void Sha3::update(const void *d, size_t n) {
+    const uint8_t *p = (const uint8_t *)d;
+    for (size_t i = 0; i < n; i++) {
+        s[pos >> 3] ^= (uint64_t)p[i] << (8 * (pos & 7));
+        if (++pos == 136) {
+            keccak_f1600_host(s);
+            pos = 0;
+        }
+    }
+}
+
+void Sha3::final(uint8_t out[32]) const {
+    uint64_t t[25];
+    memcpy(t, s, sizeof t);
+    t[pos >> 3] ^= 0x06ULL << (8 * (pos & 7));
+    t[16] ^= 0x8000000000000000ULL;
+    keccak_f1600_host(t);
+    for (int i = 0; i < 32; i++) out[i] = (uint8_t)(t[i >> 3] >> (8 * (i & 7)));
+}
+
+void sha3_256_host(const void *d, size_t n, uint8_t out[32]) {
+    Sha3 h;
+    h.update(d, n);
+    h.final(out);
+}
+
+Transcript::Transcript(const std::string &label) {
+    h.update("FSv1-TRANSCRIPT-INIT");
+    h.update(label);
+}
+void Transcript::absorb(const void *b, size_t n) {
+    h.update("FSv1-ABSORB-BYTES");
+    h.update(b, n);
+}
+void Transcript::absorb_field(uint64_t x) {
+    uint8_t le[8];
+    for (int i = 0; i < 8; i++) le[i] = (uint8_t)(x >> (8 * i));
+    absorb(le, 8);
+}
+void Transcript::challenge_bytes(const std::string &label, uint8_t out[32]) const {
+    Sha3 c = h;
+    c.update("FSv1-CHALLENGE");
+    c.update(label);
+    c.final(out);
+}
+uint64_t Transcript::challenge_field(const std::string &label) const {
+    uint8_t d[32];
+    challenge_bytes(label, d);
+    // chunks of 7 bytes taken from the end (the first 4 bytes form the last chunk);
+    // acc = acc * 2^(8 len) + LE(chunk), in the field
+    uint64_t acc = 0;
+    for (int end = 32; end > 0;) {
+        int start = end >= 7 ? end - 7 : 0, len = end - start;
+        uint64_t v = 0;
+        for (int i = 0; i < len; i++) v |= (uint64_t)d[start + i] << (8 * i);
+        acc = add(mul(acc, 1ULL << (8 * len)), v);
+        end = start;
+    }
+    return acc;
+}
+void Transcript::challenge_ext(const std::string &tag, uint64_t out[6]) const {
+    for (int i = 0; i < 6; i++) out[i] = challenge_field(tag + "/c" + std::to_string(i));
+}
+
+uint64_t hash_fields_tagged(const std::string &tag, const uint64_t *f, size_t n) {
+    Transcript t("FRI/FS");
+    t.absorb(tag);
+    for (size_t i = 0; i < n; i++) t.absorb_field(f[i]);
+    return t.challenge_field("out");
+}
+
+uint64_t stdrng_first_u64(uint64_t seed) {
+    // ChaCha12 block 0: constants, key = LE64(seed) || 0^24, counter 0, nonce 0
+    uint32_t in[16] = {0x61707865, 0x3320646e, 0x79622d32, 0x6b206574, (uint32_t)seed, (uint32_t)(seed >> 32),
+                       0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t x[16];
+    memcpy(x, in, sizeof x);
+    auto qr = [&](int a, int b, int c, int d) {
+        auto rl = [](uint32_t v, int n) { return (v << n) | (v >> (32 - n)); };
+        x[a] += x[b]; x[d] = rl(x[d] ^ x[a], 16);
+        x[c] += x[d]; x[b] = rl(x[b] ^ x[c], 12);
+        x[a] += x[b]; x[d] = rl(x[d] ^ x[a], 8);
+        x[c] += x[d]; x[b] = rl(x[b] ^ x[c], 7);
+    };
+    for (int r = 0; r < 6; r++) {
+        qr(0, 4, 8, 12); qr(1, 5, 9, 13); qr(2, 6, 10, 14); qr(3, 7, 11, 15);
+        qr(0, 5, 10, 15); qr(1, 6, 11, 12); qr(2, 7, 8, 13); qr(3, 4, 9, 14);
+    }
+    uint32_t w0 = x[0] + in[0], w1 = x[1] + in[1];
+    return (uint64_t)w0 | ((uint64_t)w1 << 32);
+}
+
+// ---------------------------------------------------------------- kernels
+namespace {
+
+constexpr int KT = 256;
+
+// absorb `total` message lanes produced by lane(t), SHA3-256 padding, digest out
+template <class LaneFn>
+__device__ __forceinline__ void sha3_lanes(uint32_t total, LaneFn lane, uint64_t *dig) {
+    uint64_t st[25];
+#pragma unroll
+    for (int j = 0; j < 25; j++) st[j] = 0;
+    uint32_t t0 = 0;
+    while (total - t0 >= 17) {
+#pragma unroll
+        for (int j = 0; j < 17; j++) st[j] ^= lane(t0 + j);
+        keccak_f_dev(st);
+        t0 += 17;
+    }
+    const uint32_t rem = total - t0;
+#pragma unroll
+    for (int j = 0; j < 17; j++) {
+        if ((uint32_t)j < rem) st[j] ^= lane(t0 + j);
+        if ((uint32_t)j == rem) st[j] ^= 0x06ULL;
+    }
+    st[16] ^= 0x8000000000000000ULL;
+    keccak_f_dev(st);
+    dig[0] = st[0];
+    dig[1] = st[1];
+    dig[2] = st[2];
+    dig[3] = st[3];
+}
+
+__global__ __launch_bounds__(KT) void merkle_leaves_k(LeafSrc src, uint64_t n, uint32_t arity, uint64_t label,
+                                                      uint64_t *out) {
+    uint64_t i = (uint64_t)blockIdx.x * KT + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t *base = src.vals + i * src.leaf_stride;
+    const uint64_t cs = src.cs;
+    // sum_c (c+1) v_c = w * T_w - sum_{k<w} T_k with T_k = sum_{c<k} v_c
+    uint64_t T = 0, U = 0;
+    const bool combo = src.combo_out != nullptr;
+    auto lane = [&](uint32_t t) -> uint64_t {
+        if (t >= 4) {
+            uint64_t v = (t - 4 < src.nz) ? base[(uint64_t)(t - 4) * cs] : 0;
+            if (combo) {
+                U = add(U, T);
+                T = add(T, v);
+            }
+            return v;
+        }
+        return t == 0 ? (uint64_t)arity : t == 1 ? 0xFFFFFFFFULL : t == 2 ? i : label;
+    };
+    uint64_t d[4];
+    sha3_lanes(4 + src.w, lane, d);
+    if (combo) src.combo_out[i] = sub(mul(T, src.w), U);
+    uint64_t *o = out + 4 * i;
+    o[0] = d[0];
+    o[1] = d[1];
+    o[2] = d[2];
+    o[3] = d[3];
+}
+
+__global__ __launch_bounds__(KT) void merkle_nodes_k(const uint64_t *child, uint64_t n_child, uint64_t n_parent,
+                                                     uint32_t arity, uint32_t level, uint64_t label, uint64_t *out) {
+    uint64_t i = (uint64_t)blockIdx.x * KT + threadIdx.x;
+    if (i >= n_parent) return;
+    auto lane = [&](uint32_t t) -> uint64_t {
+        if (t >= 4) {
+            uint32_t j = (t - 4) >> 2, q = (t - 4) & 3;
+            uint64_t ci = i * arity + j;
+            if (ci >= n_child) ci = n_child - 1;
+            return child[4 * ci + q];
+        }
+        return t == 0 ? (uint64_t)arity : t == 1 ? (uint64_t)level : t == 2 ? i : label;
+    };
+    uint64_t d[4];
+    sha3_lanes(4 + 4 * arity, lane, d);
+    uint64_t *o = out + 4 * i;
+    o[0] = d[0];
+    o[1] = d[1];
+    o[2] = d[2];
+    o[3] = d[3];
+}
+
+__global__ void gather_k(const uint64_t *src, const uint64_t *idx, uint64_t n, uint32_t width, uint64_t *out) {
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n * width) return;
+    uint64_t q = r / width, l = r - q * width;
+    out[r] = src[idx[q] * width + l];
+}
+
+}  // namespace
+
+uint32_t merkle_depth(uint64_t leaves, uint32_t arity) {
+    uint32_t d = 1;
+    uint64_t cur = leaves;
+    while (cur > arity) {
+        cur = (cur + arity - 1) / arity;
+        d++;
+    }
+    return d;
+}
+
+uint32_t pick_arity(uint64_t n, uint64_t m) {
+    static const uint64_t as[6] = {128, 64, 32, 16, 8, 4};
+    for (uint64_t a : as)
+        if (m >= a && n % a == 0) return (uint32_t)a;
+    return 2;  // n % 2 == 0 gives 2; the reference's 1 is raised to 2 by .max(2)
+}
+
+void tree_build(Ctx *c, DevTree &t, const LeafSrc &src, uint64_t n_leaves, uint32_t arity, uint32_t depth,
+                uint64_t label) {
+    SG_REQUIRE(n_leaves >= 1 && arity >= 2, "merkle: empty tree or arity < 2");
+    t.arity = arity;
+    t.label = label;
+    t.len.clear();
+    t.real.clear();
+    t.off.clear();
+    // level sizes
+    uint64_t cur = n_leaves, total = 0;
+    for (;;) {
+        uint64_t padded = cur > 1 ? (cur + arity - 1) / arity * arity : cur;
+        t.real.push_back(cur);
+        t.len.push_back(padded);
+        t.off.push_back(total);
+        total += cur;
+        if (cur <= 1) break;
+        SG_REQUIRE(t.real.size() <= depth, "merkle: layer arities exhausted");
+        cur = padded / arity;
+    }
+    t.buf = c->alloc_n<uint64_t>(4 * total);
+    {
+        ProfScope ps(c, "merkle_leaves", 8.0 * src.w * n_leaves + 32.0 * n_leaves);
+        hipLaunchKernelGGL(merkle_leaves_k, dim3((uint32_t)((n_leaves + KT - 1) / KT)), dim3(KT), 0, c->stream, src,
+                           n_leaves, arity, label, t.buf);
+        SG_HIP(hipGetLastError());
+    }
+    for (size_t l = 1; l < t.real.size(); l++) {
+        uint64_t np = t.real[l];
+        ProfScope ps(c, "merkle_nodes", 32.0 * t.real[l - 1] + 32.0 * np);
+        hipLaunchKernelGGL(merkle_nodes_k, dim3((uint32_t)((np + KT - 1) / KT)), dim3(KT), 0, c->stream,
+                           t.buf + 4 * t.off[l - 1], t.real[l - 1], np, arity, (uint32_t)l, label,
+                           t.buf + 4 * t.off[l]);
+        SG_HIP(hipGetLastError());
+    }
+    uint64_t r[4];
+    SG_HIP(hipMemcpyAsync(r, t.buf + 4 * t.off.back(), 32, hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    memcpy(t.root, r, 32);
+}
+
+void tree_free(Ctx *c, DevTree &t) {
+    if (t.buf) c->release(t.buf);
+    t.buf = nullptr;
+}
+
+void gather_u64(Ctx *c, const uint64_t *src, const uint64_t *d_idx, uint64_t n, uint64_t *out) {
+    if (!n) return;
+    hipLaunchKernelGGL(gather_k, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, c->stream, src, d_idx, n, 1u, out);
+    SG_HIP(hipGetLastError());
+}
+
+void tree_open_batch(Ctx *c, const DevTree &t, const std::vector<uint64_t> &idx,
+                     std::vector<std::vector<uint8_t>> &out) {
+    const uint32_t a = t.arity;
+    const size_t nlev = t.real.size();
+    // digest indices: per query, the leaf then every level's full group
+    std::vector<uint64_t> want;
+    for (uint64_t q : idx) {
+        want.push_back(t.off[0] + q);
+        uint64_t x = q;
+        for (size_t l = 0; l + 1 < nlev; l++) {
+            uint64_t gs = x / a * a;
+            for (uint32_t i = 0; i < a; i++) {
+                uint64_t pos = gs + i;
+                if (pos >= t.real[l]) pos = t.real[l] - 1;
+                want.push_back(t.off[l] + pos);
+            }
+            x /= a;
+        }
+    }
+    Scoped d_idx(c, 8 * want.size()), d_out(c, 32 * want.size());
+    SG_HIP(hipMemcpyAsync(d_idx.p, want.data(), 8 * want.size(), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(gather_k, dim3((uint32_t)((4 * want.size() + 255) / 256)), dim3(256), 0, c->stream, t.buf,
+                       d_idx.as<uint64_t>(), (uint64_t)want.size(), 4u, d_out.as<uint64_t>());
+    SG_HIP(hipGetLastError());
+    std::vector<uint64_t> h(4 * want.size());
+    SG_HIP(hipMemcpyAsync(h.data(), d_out.p, 32 * want.size(), hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    out.clear();
+    size_t k = 0;
+    auto put64 = [](std::vector<uint8_t> &v, uint64_t x) {
+        for (int i = 0; i < 8; i++) v.push_back((uint8_t)(x >> (8 * i)));
+    };
+    for (uint64_t q : idx) {
+        std::vector<uint8_t> o;
+        const uint8_t *leaf = (const uint8_t *)&h[4 * k++];
+        o.insert(o.end(), leaf, leaf + 32);
+        put64(o, nlev - 1);
+        uint64_t x = q;
+        for (size_t l = 0; l + 1 < nlev; l++) {
+            put64(o, a - 1);
+            uint64_t gs = x / a * a;
+            for (uint32_t i = 0; i < a; i++, k++) {
+                if (gs + i == x) continue;
+                const uint8_t *d = (const uint8_t *)&h[4 * k];
+                o.insert(o.end(), d, d + 32);
+            }
+            x /= a;
+        }
+        put64(o, q);
+        out.push_back(std::move(o));
+    }
+}
+
+}  // namespace sg

@@ -1,0 +1,85 @@
+// keccak.hpp -- SHA3-256 Merkle trees on the device, host SHA3 / transcript /
+// ChaCha12 used by the proof orchestration.
+#pragma once
+#include <stdint.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "ctx.hpp"
+
+namespace sg {
+
+// ---------------------------------------------------------------- host SHA3
+struct Sha3 {
+    uint64_t s[25];
+    size_t pos;
+    Sha3() { reset(); }
+    void reset() {
+        memset(s, 0, sizeof s);
+        pos = 0;
+    }
+    void update(const void *d, size_t n);
+    void update(const std::string &str) { update(str.data(), str.size()); }
+    void final(uint8_t out[32]) const;  // leaves the state untouched
+};
+void keccak_f1600_host(uint64_t s[25]);
+void sha3_256_host(const void *d, size_t n, uint8_t out[32]);
+
+// Fiat-Shamir transcript over SHA3-256 (crates/transcript/src/lib.rs:162-212):
+// init "FSv1-TRANSCRIPT-INIT"||label, absorb "FSv1-ABSORB-BYTES"||bytes,
+// challenges hash a clone with "FSv1-CHALLENGE"||label and leave the state as is.
+struct Transcript {
+    Sha3 h;
+    explicit Transcript(const std::string &label);
+    void absorb(const void *b, size_t n);
+    void absorb(const std::string &s) { absorb(s.data(), s.size()); }
+    void absorb_field(uint64_t x);
+    void challenge_bytes(const std::string &label, uint8_t out[32]) const;
+    uint64_t challenge_field(const std::string &label) const;  // fri.rs:73-85 (rchunks(7))
+    void challenge_ext(const std::string &tag, uint64_t out[6]) const;  // fri.rs:1372-1386
+};
+uint64_t hash_fields_tagged(const std::string &tag, const uint64_t *f, size_t n);  // fri.rs:123-130
+// rand 0.8 StdRng (ChaCha12, key = LE64(seed) || 0^24): first next_u64()
+uint64_t stdrng_first_u64(uint64_t seed);
+
+// ---------------------------------------------------------------- device trees
+// Levels of one tree, resident on the device. Digests are 4 u64 (LE lanes).
+struct DevTree {
+    uint32_t arity = 2;
+    uint64_t label = 0;
+    std::vector<uint64_t> len;  // nodes per level after padding to a multiple of arity
+    std::vector<uint64_t> real; // nodes per level before padding
+    std::vector<uint64_t> off;  // digest offset (in digests) of each level
+    uint64_t *buf = nullptr;    // all digests
+    uint8_t root[32];
+};
+
+// Leaf i hashes DsLabel(arity, u32::MAX, i, label) || LE64(vals[base + i*leaf_stride + c*cs]) for c < w.
+// Values c >= nz are zero (a base-field layer hashed as 6 components).
+// combo_out (optional): per leaf, sum_c (c+1) * value_c written alongside the
+// digest -- the prover-core f0 fused into the row-leaf pass.
+struct LeafSrc {
+    const uint64_t *vals;
+    uint64_t leaf_stride;
+    uint64_t cs;
+    uint32_t w;
+    uint32_t nz;
+    uint64_t *combo_out = nullptr;
+};
+
+// Build the whole tree with constant arity (crates/merkle/src/lib.rs:377-476).
+void tree_build(Ctx *c, DevTree &t, const LeafSrc &src, uint64_t n_leaves, uint32_t arity, uint32_t depth,
+                uint64_t label);
+void tree_free(Ctx *c, DevTree &t);
+// Serialized MerkleOpenings (leaf[32], path Vec<Vec<[32]>>, index u64) for a batch of indices.
+void tree_open_batch(Ctx *c, const DevTree &t, const std::vector<uint64_t> &idx, std::vector<std::vector<uint8_t>> &out);
+
+// gather out[r] = src[idx[r]] for u64 elements (device index list)
+void gather_u64(Ctx *c, const uint64_t *src, const uint64_t *d_idx, uint64_t n, uint64_t *out);
+
+uint32_t merkle_depth(uint64_t leaves, uint32_t arity);    // fri.rs:950-959
+uint32_t pick_arity(uint64_t n, uint64_t requested);       // fri.rs:1094-1103 (.max(2) applied)
+
+}  // namespace sg

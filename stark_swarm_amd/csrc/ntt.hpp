@@ -1,0 +1,44 @@
+// ntt.hpp -- batched natural-order Goldilocks NTT passes and LDE plans.
+#pragma once
+#include <stdint.h>
+
+#include "ctx.hpp"
+
+namespace sg {
+
+// One strided pass: for batch b < nb and group g < ng, a DFT of size 2^log_s
+// over index i of in[b*ib + g*ig + i*ii], written to out[b*ob + g*og + k*ok].
+struct NttPass {
+    const uint64_t *in;
+    uint64_t *out;
+    int64_t ib, ig, ii;
+    int64_t ob, og, ok;
+    uint64_t ng;
+    uint32_t nb;
+    uint32_t log_s, log_g;
+    uint32_t inv;
+    // input pre-multiply by w_{2^pre_log}^(g*pre_g + i*pre_i + b*pre_b) (0 = off)
+    uint32_t pre_log;
+    uint64_t pre_g, pre_i, pre_b;
+    // output multiply by w_{2^post_log}^(g*k), inverse root when inv (0 = off)
+    uint32_t post_log;
+    uint64_t scale;
+    // output multiply by s^(g*sp_g + k*sp_k) from a two-level power table
+    const uint64_t *sp_lo, *sp_hi;
+    uint64_t sp_g, sp_k;
+    const uint64_t *rt_lo, *rt_hi, *tw;
+};
+
+struct PowTable {
+    uint64_t *buf = nullptr;
+    const uint64_t *lo = nullptr, *hi = nullptr;
+};
+
+PowTable make_pow_table(Ctx *c, uint64_t s);
+void launch_pass(Ctx *c, NttPass p, const char *name);
+void ntt_columns(Ctx *c, const uint64_t *in, uint64_t *out, uint64_t *scratch, uint32_t L, uint64_t ncols,
+                 uint64_t in_cs, uint64_t out_cs, bool inv, const PowTable *spow);
+void lde_columns(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint32_t LB, uint64_t ncols,
+                 uint64_t in_cs, uint64_t out_cs, uint64_t shift);
+
+}  // namespace sg

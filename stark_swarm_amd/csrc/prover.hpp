@@ -1,0 +1,38 @@
+// prover.hpp -- whole-proof flows (swarm-dns HashRollup provers, prover core).
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+#include "ctx.hpp"
+#include "fri.hpp"
+
+namespace sg {
+
+struct DnsRec {
+    const char *domain;
+    size_t domain_len;
+    uint16_t record_type;
+    uint32_t ttl;
+    const uint8_t *rdata;
+    size_t rdata_len;
+};
+
+struct InnerShard {
+    uint8_t pi_hash[32], merkle_root[32], root_f0[32];
+    size_t record_count = 0, n_trace = 0;
+    std::vector<uint8_t> proof;
+};
+
+void dns_leaf_hash(const DnsRec &r, const uint8_t salt[16], uint8_t out[32]);
+void dns_merkle_root(const std::vector<uint8_t> &leaves, uint8_t root[32]);
+std::vector<uint8_t> rollup_prove(Ctx *c, const std::vector<uint64_t> &leaves, const uint8_t binding[32],
+                                  int phi_mode, uint8_t root_f0[32]);
+InnerShard prove_inner_shard(Ctx *c, const uint8_t salt[16], const std::vector<DnsRec> &recs,
+                             const uint8_t fs_binding[32], int phi_mode);
+std::vector<uint8_t> prove_outer_rollup(Ctx *c, const uint8_t *pi_hashes, size_t n, const uint8_t pk_hash[32],
+                                        int phi_mode, uint8_t root_f0[32]);
+std::vector<uint8_t> prover_core_dev(Ctx *c, const uint64_t *d_trace, uint64_t w, uint64_t n_trace, uint64_t blowup,
+                                     uint64_t shift, const FriParams &p, uint64_t *d_lde, uint8_t row_root[32]);
+
+}  // namespace sg

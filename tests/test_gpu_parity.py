@@ -1,0 +1,198 @@
+"""GPU parity: every HIP stage against the oracle on the same seeded inputs,
+bit-exact, and whole proofs against the reference's golden bytes.
+
+Marked gpu: needs an MI355X and the built libstark_gpu.so (no CPU fallback).
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from tests import _oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+P = O.P
+
+
+@pytest.fixture(scope="module")
+def S():
+    import stark_swarm_amd as S
+    S.default_context()
+    return S
+
+
+def rnd(seed, shape):
+    return np.random.default_rng(seed).integers(0, P, size=shape, dtype=np.uint64)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+# ---------------------------------------------------------------- NTT
+@pytest.mark.parametrize("lg,ncols", [(0, 3), (1, 2), (3, 5), (4, 1), (7, 9), (10, 4), (12, 2), (13, 3), (16, 2),
+                                      (20, 1)])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_ntt_matches_oracle(S, lg, ncols, inverse):
+    say("ntt lg=%d ncols=%d inverse=%s" % (lg, ncols, inverse))
+    a = rnd(100 + lg, (ncols, 1 << lg))
+    got = S.ntt(a, inverse=inverse)
+    for c in range(ncols):
+        assert np.array_equal(got[c], O.ntt(a[c], inverse=inverse)), "column %d" % c
+
+
+def test_ntt_edge_values(S):
+    n = 1 << 14
+    for v in [0, 1, P - 1, P - 2, 2**32, 2**32 - 1]:
+        a = np.full((1, n), v, dtype=np.uint64)
+        assert np.array_equal(S.ntt(a)[0], O.ntt(a[0]))
+        assert np.array_equal(S.ntt(a, inverse=True)[0], O.ntt(a[0], inverse=True))
+
+
+# ---------------------------------------------------------------- LDE
+@pytest.mark.parametrize("w,lg,blowup", [(1, 3, 2), (4, 4, 32), (3, 8, 8), (5, 12, 4), (2, 13, 8), (4, 16, 32),
+                                         (2, 18, 8)])
+def test_lde_matches_oracle(S, w, lg, blowup):
+    say("lde w=%d n=2^%d blowup=%d" % (w, lg, blowup))
+    cols = rnd(200 + lg, (w, 1 << lg))
+    got = S.lde_trace_columns(cols, 1 << lg, blowup)
+    assert np.array_equal(got, O.lde_columns(cols, blowup))
+
+
+@pytest.mark.parametrize("lg,blowup", [(5, 8), (12, 8), (14, 8), (16, 4)])
+def test_coset_lde_matches_oracle(S, lg, blowup):
+    say("coset lde n=2^%d blowup=%d shift=7" % (lg, blowup))
+    ctx = S.default_context()
+    w, n = 3, 1 << lg
+    cols = rnd(300 + lg, (w, n))
+    d = ctx.alloc(cols.nbytes).upload(cols)
+    o = ctx.alloc(cols.nbytes * blowup)
+    ctx._check(S.lib().stark_lde_columns_dev(ctx.h, d.ptr, w, n, blowup, 7, o.ptr))
+    assert np.array_equal(o.download((w, n * blowup)), O.lde_columns(cols, blowup, shift=7))
+
+
+def test_lde_errors_like_reference(S):
+    with pytest.raises(ValueError):
+        S.lde_trace_columns([], 8, 4)
+    with pytest.raises(ValueError):
+        S.lde_trace_columns(rnd(1, (2, 12)), 12, 4)
+    with pytest.raises(ValueError):
+        S.lde_trace_columns(rnd(1, (2, 16)), 16, 3)
+    with pytest.raises(ValueError):
+        S.lde_trace_columns(rnd(1, (2, 16)), 8, 4)
+
+
+# ---------------------------------------------------------------- merge
+@pytest.mark.parametrize("lg,blowup,phi_mode", [(3, 32, 1), (7, 32, 1), (7, 32, 0), (8, 8, 0), (12, 32, 1),
+                                                (10, 8, 2)])
+def test_merge_matches_oracle(S, lg, blowup, phi_mode):
+    say("merge n_trace=2^%d blowup=%d phi_mode=%d" % (lg, blowup, phi_mode))
+    n = 1 << lg
+    leaves = np.random.default_rng(lg).integers(0, 2**64, size=n, dtype=np.uint64) % np.uint64(P)
+    tr = O.hash_rollup_trace(n, leaves)
+    lde = O.lde_columns(tr, blowup)
+    got, info = S.deep_ali_merge_general(lde, [1, 2, 3], S.AirType.HashRollup, pow(7, (P - 1) // (n * blowup), P),
+                                         n, blowup, phi_mode=phi_mode)
+    assert np.array_equal(got, O.merge_hash_rollup(lde, n, blowup, phi_mode=phi_mode))
+    assert info.num_constraints == 3 and info.trace_width == 4
+
+
+# ---------------------------------------------------------------- Merkle
+@pytest.mark.parametrize("n,vpl,arity,label", [(1, 3, 2, 0), (7, 1, 2, 3), (64, 6, 8, 1), (100, 8, 16, 0xFF),
+                                               (1000, 13, 4, 7), (4096, 6, 8, 2), (33, 48, 16, 0xFE)])
+def test_merkle_matches_oracle(S, n, vpl, arity, label):
+    vals = rnd(n + vpl, (n, vpl))
+    import ctypes
+    root = ctypes.create_string_buffer(32)
+    depth = 1
+    cur = n
+    while cur > arity:
+        cur = (cur + arity - 1) // arity
+        depth += 1
+    O.lib().ref_merkle_root(O.ptr(vals), n, vpl, arity, depth, label, root)
+    assert S.merkle_commit(vals, arity, label) == root.raw
+
+
+# ---------------------------------------------------------------- FRI / STIR
+@pytest.mark.parametrize("lg,schedule,s0,cc", [(6, [8, 8], 5, True), (9, [8, 8, 8], 12, True),
+                                               (10, [8, 8, 8, 2], 20, True), (11, [4, 8, 4, 8, 2], 9, False),
+                                               (13, [16, 8, 8, 8], 30, True), (17, [8, 8, 8, 8, 8, 4], 54, True)])
+def test_deep_fri_prove_matches_oracle(S, lg, schedule, s0, cc):
+    say("deep_fri_prove n0=2^%d schedule=%s" % (lg, schedule))
+    f0 = rnd(400 + lg, 1 << lg)
+    pi = hashlib.sha3_256(b"pi%d" % lg).digest()
+    ref = O.deep_fri_prove(f0, O.fri_params(schedule, r=s0, s0=s0, coeff_commit_final=cc, pi_hash=pi))
+    got = S.deep_fri_prove(f0, S.FriDomain.new_radix2(1 << lg),
+                           S.DeepFriParams(schedule, s0, S.SEED_Z, cc, 1, True, s0, pi))
+    assert got == ref
+
+
+# ---------------------------------------------------------------- golden proofs
+def _gold():
+    return json.load(open(os.path.join(GOLD, "primary_state.json")))
+
+
+@pytest.mark.parametrize("idx", [0, 1, 2])
+def test_inner_shard_golden_on_gpu(S, idx):
+    from stark_swarm_amd.zone import build_synthetic_zone, split_evenly
+    d = _gold()
+    w = d["shards"][idx]
+    say("golden inner shard %d" % w["shard_id"])
+    salt = bytes.fromhex(d["zone_salt_hex"])
+    recs = split_evenly(build_synthetic_zone(d["record_count"]), 3)[idx]
+    fsb = S.shard_fs_binding(bytes.fromhex(d["authority_pk_hash_hex"]), bytes.fromhex(d["job_id_hex"]),
+                             w["shard_id"], bytes.fromhex(w["shard_nonce_hex"]))
+    out = S.prove_inner_shard(salt, recs, fsb, S.LdtMode.Stir)
+    assert out.merkle_root.hex() == w["merkle_root_hex"]
+    assert out.pi_hash.hex() == w["pi_hash_hex"]
+    assert out.n_trace == w["n_trace"]
+    assert len(out.proof_blob) == d["blobs"][w["proof"]]["len"]
+    assert hashlib.sha3_256(out.proof_blob).hexdigest() == d["blobs"][w["proof"]]["sha3_256"]
+    if idx == 0:
+        assert out.proof_blob == open(os.path.join(GOLD, w["proof"]), "rb").read()
+
+
+def test_outer_rollup_golden_on_gpu(S):
+    d = _gold()
+    say("golden outer rollup")
+    pis = [bytes.fromhex(h) for h in d["inner_pi_hashes_hex"]]
+    out = S.prove_outer_rollup(pis, bytes.fromhex(d["authority_pk_hash_hex"]), S.LdtMode.Stir)
+    assert out.proof_blob == open(os.path.join(GOLD, "outer-rollup.bin"), "rb").read()
+    assert out.root_f0.hex() == d["outer_root_f0_hex"]
+    assert out.n0 == d["outer_n0"] and out.n_trace == d["outer_n_trace"]
+
+
+def test_fri_ldt_unsupported_is_an_error(S):
+    with pytest.raises(S.StarkError):
+        S.prove_outer_rollup([b"\x00" * 32], b"\x01" * 32, S.LdtMode.Fri)
+
+
+# ---------------------------------------------------------------- prover core
+@pytest.mark.parametrize("w,lg,blowup,schedule,s0", [(8, 10, 8, [8, 8, 8, 4], 10), (16, 11, 8, [8, 8, 8, 8], 12),
+                                                     (3, 8, 4, [8, 8, 2], 6)])
+def test_prover_core_matches_oracle(S, w, lg, blowup, schedule, s0):
+    say("prover core w=%d n=2^%d blowup=%d" % (w, lg, blowup))
+    import ctypes
+    ctx = S.default_context()
+    n = 1 << lg
+    tr = rnd(500 + lg, (w, n))
+    params = S.DeepFriParams(schedule, s0, S.SEED_Z, True, 1, True, s0, None)
+    d_tr = ctx.alloc(tr.nbytes).upload(tr)
+    d_lde = ctx.alloc(tr.nbytes * blowup)
+    root, blob = S.prover_core(d_tr, w, n, blowup, 7, params, d_lde)
+    rr = ctypes.create_string_buffer(32)
+    pp, pl = O.u8p(), O.sz()
+    f0 = np.empty(n * blowup, dtype=np.uint64)
+    rc = O.lib().ref_prover_core(O.ptr(tr), w, n, blowup, 7,
+                                 O.fri_params(schedule, r=s0, s0=s0), rr, O.ptr(f0), pp, pl)
+    assert rc == 0
+    ref = O.take_bytes(pp, pl)
+    assert root == rr.raw
+    assert blob == ref
+    assert np.array_equal(d_lde.download((w, n * blowup)), O.lde_columns(tr, blowup, shift=7))
