@@ -113,6 +113,7 @@ def main():
     ap.add_argument("--cols", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", action="store_true")
+    ap.add_argument("--streams", type=int, default=4, help="proofs in flight for the batched measurement")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the prover core runs on a single MI355X and this build has no "
@@ -199,10 +200,71 @@ def main():
                      "bytes_per_launch": per_launch_bytes, "avg_launch_ms": round(avg_ms, 4)},
         "kernels": kernels,
     }
+    if args.streams > 1:
+        bres = batched(S, args, w, n, blowup, params, ref_blob, ms_step)
+        res.update(bres)
+        # headline value: batched throughput (K proofs in flight); the plain
+        # one-proof-at-a-time rate stays in batched.plain_proofs_per_s
+        b = bres["batched"]
+        res["value"] = b["proofs_per_s"]
+        res["ms_per_step"] = round(1e3 * b["seconds"] / args.steps, 3)
+        res["config"]["proofs_in_flight"] = args.streams
+        res["config"]["step"] = "%d concurrent prover-core proofs (one per HIP stream)" % args.streams
     if not args.no_cpu:
         log("cpu baseline: timing the oracle on a bounded sample")
         res["cpu_baseline"] = cpu_baseline(w, lg, blowup, schedule, s0)
     print(json.dumps(res))
+
+
+def batched(S, args, w, n, blowup, params, ref_blob, plain_ms):
+    """K independent prover-core proofs in flight at once: one context (HIP
+    stream) and one host thread per proof slot, each over its own resident
+    trace (the same seeded trace, so every proof must equal the plain one)."""
+    import threading
+    K = args.streams
+    log("batched: %d streams" % K)
+    trace = splitmix_field(0x5EED0002, w * n).reshape(w, n)
+    slots = []
+    for k in range(K):
+        ctx = S.Context(0)
+        slots.append((ctx, ctx.alloc(trace.nbytes).upload(trace), ctx.alloc(trace.nbytes * blowup)))
+    del trace
+    per = max(1, args.steps)
+    errors = []
+
+    def worker(k, count):
+        ctx, d_tr, d_lde = slots[k]
+        try:
+            for _ in range(count):
+                _, blob = S.prover_core(d_tr, w, n, blowup, 7, params, d_lde, ctx=ctx)
+                if blob != ref_blob:
+                    errors.append("slot %d: proof differs from the plain proof" % k)
+        except Exception as e:  # noqa: BLE001
+            errors.append("slot %d: %r" % (k, e))
+
+    # warm each slot once (allocator, code objects)
+    ths = [threading.Thread(target=worker, args=(k, 1)) for k in range(K)]
+    [t.start() for t in ths]
+    [t.join() for t in ths]
+    if errors:
+        raise SystemExit("; ".join(errors))
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=worker, args=(k, per)) for k in range(K)]
+    [t.start() for t in ths]
+    [t.join() for t in ths]
+    dt = time.perf_counter() - t0
+    if errors:
+        raise SystemExit("; ".join(errors))
+    for ctx, a, b in slots:
+        a.free()
+        b.free()
+        ctx.close()
+    total = K * per
+    log("batched: %d proofs in %.3f s" % (total, dt))
+    return {"batched": {"streams": K, "proofs": total, "seconds": round(dt, 4),
+                        "proofs_per_s": round(total / dt, 4), "plain_proofs_per_s": round(1e3 / plain_ms, 4),
+                        "speedup_vs_plain": round((total / dt) / (1e3 / plain_ms), 3),
+                        "bit_exact_vs_plain": True}}
 
 
 if __name__ == "__main__":

@@ -29,13 +29,29 @@ SG_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
 }
 
 SG_HD uint64_t reduce128(uint64_t lo, uint64_t hi) {
-    uint64_t hh = hi >> 32, hl = hi & EPS;
+    const uint64_t hh = hi >> 32;
+    const uint32_t hl = (uint32_t)hi;
     uint64_t t0 = lo - hh;
     if (lo < hh) t0 -= EPS;
-    uint64_t t1 = (hl << 32) - hl;  // hl * (2^32 - 1)
+    // hl * (2^32 - 1) = (hl - 1) * 2^32 + (2^32 - hl) for hl > 0, built from
+    // 32-bit pieces so it stays off the quarter-rate multiplier
+    const uint32_t t1_hi = hl ? hl - 1 : 0;
+    const uint32_t t1_lo = 0u - hl;
+    const uint64_t t1 = ((uint64_t)t1_hi << 32) | t1_lo;
     uint64_t r = t0 + t1;
     if (r < t1) r += EPS;
     return r >= P ? r - P : r;
+}
+
+// full 64x64 -> 128 product from four 32x32 -> 64 multiply-adds
+SG_HD void mul128(uint64_t a, uint64_t b, uint64_t &lo, uint64_t &hi) {
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+    const uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+    const uint64_t p00 = (uint64_t)a0 * b0;
+    const uint64_t mid = (uint64_t)a0 * b1 + (p00 >> 32);
+    const uint64_t mid2 = (uint64_t)a1 * b0 + (uint32_t)mid;
+    hi = (uint64_t)a1 * b1 + (mid >> 32) + (mid2 >> 32);
+    lo = (mid2 << 32) | (uint32_t)p00;
 }
 
 SG_HD uint64_t add(uint64_t a, uint64_t b) {
@@ -45,7 +61,11 @@ SG_HD uint64_t add(uint64_t a, uint64_t b) {
 }
 SG_HD uint64_t sub(uint64_t a, uint64_t b) { return a >= b ? a - b : a + (P - b); }
 SG_HD uint64_t neg(uint64_t a) { return a ? P - a : 0; }
-SG_HD uint64_t mul(uint64_t a, uint64_t b) { return reduce128(a * b, mulhi64(a, b)); }
+SG_HD uint64_t mul(uint64_t a, uint64_t b) {
+    uint64_t lo, hi;
+    mul128(a, b, lo, hi);
+    return reduce128(lo, hi);
+}
 SG_HD uint64_t canon(uint64_t x) { return x >= P ? x - P : x; }
 
 SG_HD uint64_t pow(uint64_t a, uint64_t e) {

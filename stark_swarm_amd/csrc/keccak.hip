@@ -76,9 +76,77 @@ SG_HD void keccak_round(uint64_t *a, uint64_t rc) {
     a[0] ^= rc;
 }
 
+// Device permutation on 32-bit halves: a 64-bit rotate is two v_alignbit_b32,
+// the column parities use three-input XORs, and the rho/pi lane permutation
+// is a renaming of registers.
+__device__ constexpr int KROT_D[25] = {0, 1, 62, 28, 27, 36, 44, 6, 55, 20, 3, 10, 43,
+                                       25, 39, 41, 45, 15, 21, 8, 18, 2, 61, 56, 14};
+
+__device__ __forceinline__ void rotl2(uint32_t lo, uint32_t hi, int n, uint32_t &olo, uint32_t &ohi) {
+    if (n == 0) {
+        olo = lo;
+        ohi = hi;
+    } else if (n < 32) {
+        ohi = __builtin_amdgcn_alignbit(hi, lo, 32 - n);
+        olo = __builtin_amdgcn_alignbit(lo, hi, 32 - n);
+    } else if (n == 32) {
+        olo = hi;
+        ohi = lo;
+    } else {
+        const int m = n - 32;
+        ohi = __builtin_amdgcn_alignbit(lo, hi, 32 - m);
+        olo = __builtin_amdgcn_alignbit(hi, lo, 32 - m);
+    }
+}
+
+__device__ __forceinline__ void keccak_round32(uint32_t *lo, uint32_t *hi, uint64_t rc) {
+    uint32_t clo[5], chi[5];
+#pragma unroll
+    for (int x = 0; x < 5; x++) {
+        // 0x96: three-input XOR truth table
+        clo[x] = __builtin_amdgcn_bitop3_b32(__builtin_amdgcn_bitop3_b32(lo[x], lo[x + 5], lo[x + 10], 0x96),
+                                             lo[x + 15], lo[x + 20], 0x96);
+        chi[x] = __builtin_amdgcn_bitop3_b32(__builtin_amdgcn_bitop3_b32(hi[x], hi[x + 5], hi[x + 10], 0x96),
+                                             hi[x + 15], hi[x + 20], 0x96);
+    }
+    uint32_t dlo[5], dhi[5];
+#pragma unroll
+    for (int x = 0; x < 5; x++) {
+        uint32_t rl, rh;
+        rotl2(clo[(x + 1) % 5], chi[(x + 1) % 5], 1, rl, rh);
+        dlo[x] = clo[(x + 4) % 5] ^ rl;
+        dhi[x] = chi[(x + 4) % 5] ^ rh;
+    }
+    uint32_t blo[25], bhi[25];
+#pragma unroll
+    for (int x = 0; x < 5; x++)
+#pragma unroll
+        for (int y = 0; y < 5; y++) {
+            const int src = x + 5 * y, dst = y + 5 * ((2 * x + 3 * y) % 5);
+            rotl2(lo[src] ^ dlo[x], hi[src] ^ dhi[x], KROT_D[src], blo[dst], bhi[dst]);
+        }
+#pragma unroll
+    for (int y = 0; y < 25; y += 5)
+#pragma unroll
+        for (int x = 0; x < 5; x++) {
+            lo[y + x] = blo[y + x] ^ (~blo[y + (x + 1) % 5] & blo[y + (x + 2) % 5]);
+            hi[y + x] = bhi[y + x] ^ (~bhi[y + (x + 1) % 5] & bhi[y + (x + 2) % 5]);
+        }
+    lo[0] ^= (uint32_t)rc;
+    hi[0] ^= (uint32_t)(rc >> 32);
+}
+
 __device__ __forceinline__ void keccak_f_dev(uint64_t *a) {
-#pragma unroll 4
-    for (int r = 0; r < 24; r++) keccak_round(a, KRC_D[r]);
+    uint32_t lo[25], hi[25];
+#pragma unroll
+    for (int i = 0; i < 25; i++) {
+        lo[i] = (uint32_t)a[i];
+        hi[i] = (uint32_t)(a[i] >> 32);
+    }
+#pragma unroll 2
+    for (int r = 0; r < 24; r++) keccak_round32(lo, hi, KRC_D[r]);
+#pragma unroll
+    for (int i = 0; i < 25; i++) a[i] = ((uint64_t)hi[i] << 32) | lo[i];
 }
 
 void keccak_f1600_host(uint64_t s[25]) {

@@ -6,14 +6,27 @@
 // order in and out, forward transform X[k] = sum_j x[j] w^(jk) with
 // w = 7^((p-1)/n), inverse scaled by 1/n.
 //
-// Structure: a DFT of size n = n1*n2 is two strided passes (four-step):
+// A DFT of size n = n1*n2 is two strided passes (four-step):
 //   pass A  Z[k1*n2+i2] = w_n^(i2*k1) * DFT_n1( x[i1*n2+i2] )     (columns)
 //   pass B  X[k1+n1*k2] = DFT_n2( Z[k1*n2+i2] )                    (rows)
-// Each pass is one launch of `ntt_pass`: a workgroup owns a tile of G
-// sub-DFTs of size S (S*G = 8192 u64 = 64 KiB of LDS), loads it with the
-// tile's contiguous dimension on consecutive lanes, runs radix-16/8/4/2
-// Stockham steps between registers and LDS, and writes it back, fusing
-// input pre-twiddles, the four-step twiddle and output scaling.
+// Each pass is one launch of `ntt_tile<LOG_S, INV>`: a 256-lane workgroup owns
+// a tile of G sub-DFTs of size S = 2^LOG_S (S*G = 8192 u64, 68 KiB of padded
+// LDS, two workgroups per CU). The tile is loaded with its contiguous
+// dimension on consecutive lanes (coalesced runs of G u64 or whole rows), the
+// sub-DFT runs as 2-3 Stockham steps of radix <= 32 held in VGPRs (32
+// elements per lane), and the result is stored the same way, fusing input
+// pre-twiddles, the four-step twiddle and output scaling.
+//
+// Inside a register DFT every twiddle is a power of w_32 = 2^78 (mod p), so
+// it is a shift plus one 128->64 reduction instead of a full 64x64 product.
+// The LDS index is padded by one u64 per 16 to break the bank conflicts of
+// the strided Stockham writes.
+//
+// The LDE of a column with blowup B = 2^LB uses
+//   LDE[B*k + r] = DFT_n( c_j * w_{Bn}^(r*j) )[k],  r < B,
+// so pass A reads each coefficient tile once and emits the B pre-twiddled
+// column DFTs (`nr` repetitions), and pass B scatters row DFTs of every r
+// into the interleaved output.
 #include "gl.hpp"
 #include "ctx.hpp"
 #include "ntt.hpp"
@@ -25,34 +38,64 @@ namespace {
 constexpr int TILE_LOG = 13;
 constexpr int TILE = 1 << TILE_LOG;
 constexpr int NT = 256;
-constexpr int EPT = TILE / NT;  // elements held per thread during a step
+constexpr int EPT = TILE / NT;  // elements held per lane during a step
+constexpr int MAX_LR = 5;       // register DFTs of at most 2^MAX_LR points
+constexpr int PAD_SHIFT = 4;
+constexpr int LDS_ELEMS = TILE + (TILE >> PAD_SHIFT);
 
-constexpr uint64_t cmul(uint64_t a, uint64_t b) { return (uint64_t)(((unsigned __int128)a * b) % P); }
-constexpr uint64_t cpow(uint64_t a, uint64_t e) {
-    uint64_t r = 1;
-    while (e) {
-        if (e & 1) r = cmul(r, a);
-        a = cmul(a, a);
-        e >>= 1;
+__device__ __forceinline__ uint32_t padi(uint32_t e) { return e + (e >> PAD_SHIFT); }
+
+// x * 2^k mod p for a compile-time-foldable k in [0, 192): 2^96 = -1.
+// x may be lazily reduced (any u64); the result is canonical.
+__device__ __forceinline__ uint64_t mul_pow2(uint64_t x, int k) {
+    const bool negate = k >= 96;
+    const int s = k % 96;
+    uint64_t y;
+    if (s == 0) {
+        y = canon(x);
+    } else if (s < 32) {
+        // x * 2^s = lo + 2^64 * h, h < 2^32:  lo + h * (2^32 - 1)
+        const uint64_t lo = x << s;
+        const uint32_t h = (uint32_t)(x >> (64 - s));
+        const uint64_t t1 = ((uint64_t)(h ? h - 1 : 0) << 32) | (uint32_t)(0u - h);
+        uint64_t r = lo + t1;
+        if (r < t1) r += EPS;
+        y = canon(r);
+    } else if (s < 64) {
+        y = reduce128(x << s, x >> (64 - s));
+    } else {
+        // s in [64, 96): 2^s = -2^(s-96), so x * 2^s = -(x / 2^t), t = 96 - s.
+        // With x = a 2^t + b: x / 2^t = a - b 2^s and b 2^s = c (2^32 - 1),
+        // c = b << (s - 64) < 2^32; hence x * 2^s = c (2^32 - 1) - a.
+        const int t = 96 - s;
+        const uint64_t a = x >> t;
+        const uint32_t c = (uint32_t)(x << (s - 64));
+        const uint64_t ce = ((uint64_t)(c ? c - 1 : 0) << 32) | (uint32_t)(0u - c);
+        uint64_t r = ce - a;
+        if (ce < a) r -= EPS;  // + p (mod 2^64); ce - a + p > 0 since a < 2^63
+        y = canon(r);
     }
-    return r;
+    return negate ? neg(y) : y;
 }
-struct Root16 {
-    uint64_t f[16], i[16];
-};
-constexpr Root16 make_root16() {
-    Root16 t{};
-    uint64_t w = cpow(7, (P - 1) / 16), wi = cpow(w, 15);
-    uint64_t a = 1, b = 1;
-    for (int k = 0; k < 16; k++) {
-        t.f[k] = a;
-        t.i[k] = b;
-        a = cmul(a, w);
-        b = cmul(b, wi);
-    }
-    return t;
+
+// lazily reduced add / sub: a is any u64, b is canonical; results are any u64
+// congruent to a +- b (2^64 = 2^32 - 1 mod p)
+__device__ __forceinline__ uint64_t lz_add(uint64_t a, uint64_t b) {
+    uint64_t r;
+    const bool c = __builtin_add_overflow(a, b, &r);
+    return r + (uint64_t)(c ? 0xFFFFFFFFu : 0u);
 }
-__device__ constexpr Root16 ROOT16 = make_root16();
+__device__ __forceinline__ uint64_t lz_sub(uint64_t a, uint64_t b) {
+    uint64_t r;
+    const bool c = __builtin_sub_overflow(a, b, &r);
+    return r - (uint64_t)(c ? 0xFFFFFFFFu : 0u);
+}
+
+// w_32^e = 2^(78 e mod 192); inverse: 2^(-78 e mod 192)
+__device__ __forceinline__ int w32_exp(int e, bool inv) {
+    int k = (78 * e) % 192;
+    return inv ? (192 - k) % 192 : k;
+}
 
 template <int LR>
 __device__ __forceinline__ constexpr int brev_c(int x) {
@@ -61,13 +104,15 @@ __device__ __forceinline__ constexpr int brev_c(int x) {
     return r;
 }
 
-// in-register DFT of size R = 2^LR: v[k] <- sum_r v[r] w_R^(rk)
+// in-register DFT of size R = 2^LR <= 32: v[k] <- sum_r v[r] w_R^(+-rk).
+// Values between butterflies are lazily reduced u64 (LDS holds them that way
+// too); every product and every store to global memory is canonical.
 template <int LR, bool INV>
 __device__ __forceinline__ void dft_reg(uint64_t *v) {
     constexpr int R = 1 << LR;
 #pragma unroll
     for (int i = 0; i < R; i++) {
-        int j = brev_c<LR>(i);
+        const int j = brev_c<LR>(i);
         if (i < j) {
             uint64_t t = v[i];
             v[i] = v[j];
@@ -81,140 +126,238 @@ __device__ __forceinline__ void dft_reg(uint64_t *v) {
         for (int base = 0; base < R; base += 2 * half) {
 #pragma unroll
             for (int j = 0; j < half; j++) {
-                // w_{2 half}^j = w_16^(j * 16 / (2 half))
-                const int e = j * (16 >> (lh + 1));
+                // w_{2 half}^j = w_32^(j * 32 / (2 half))
+                // t canonical (twiddled, or reduced once), u lazy
                 uint64_t t = v[base + j + half];
-                if (j != 0) t = mul(t, INV ? ROOT16.i[e] : ROOT16.f[e]);
-                uint64_t u = v[base + j];
-                v[base + j] = add(u, t);
-                v[base + j + half] = sub(u, t);
+                t = (j != 0) ? mul_pow2(t, w32_exp(j * (32 >> (lh + 1)), INV)) : canon(t);
+                const uint64_t u = v[base + j];
+                v[base + j] = lz_add(u, t);
+                v[base + j + half] = lz_sub(u, t);
             }
         }
     }
 }
 
-__device__ __forceinline__ uint64_t root_pow(const NttPass &p, uint32_t lg, uint64_t e, bool inv) {
+__device__ __forceinline__ uint64_t root_pow(const uint64_t *rt_lo, const uint64_t *rt_hi, uint32_t lg, uint64_t e,
+                                             bool inv) {
     uint64_t e26 = (e & ((1ULL << lg) - 1)) << (26 - lg);
     if (inv) e26 = (0 - e26) & ((1ULL << 26) - 1);
-    return mul(p.rt_lo[e26 & 8191], p.rt_hi[e26 >> 13]);
+    return mul(rt_lo[e26 & 8191], rt_hi[e26 >> 13]);
 }
 
-template <int LR, bool INV>
-__device__ __forceinline__ void stockham_step(uint64_t *lds, uint32_t log_s, uint32_t tile, uint32_t log_ns,
-                                              const uint64_t *tw) {
-    constexpr int R = 1 << LR;
-    constexpr int MAXG = EPT / R;
-    const uint32_t sub_log = log_s - LR;
-    const uint32_t sub = 1u << sub_log;
-    const uint32_t groups = tile >> LR;
-    const uint32_t Ns = 1u << log_ns;
-    uint64_t v[MAXG][R];
+// radix of the next Stockham step: ceil(remaining / steps_left), steps of <= MAX_LR bits
+constexpr int next_lr(int left) {
+    const int steps = (left + MAX_LR - 1) / MAX_LR;
+    return (left + steps - 1) / steps;
+}
+
+template <int LOG_S, int LOG_NS, bool INV>
+__device__ __forceinline__ void stockham_steps(uint64_t *lds, uint32_t tile, const uint64_t *tw) {
+    if constexpr (LOG_NS < LOG_S) {
+        constexpr int LR = next_lr(LOG_S - LOG_NS);
+        constexpr int R = 1 << LR;
+        constexpr int MAXG = EPT / R;
+        constexpr int SUB_LOG = LOG_S - LR;
+        constexpr uint32_t SUB = 1u << SUB_LOG;
+        constexpr uint32_t NS = 1u << LOG_NS;
+        const uint32_t groups = tile >> LR;
+        uint64_t v[MAXG][R];
 #pragma unroll
-    for (int u = 0; u < MAXG; u++) {
-        uint32_t q = threadIdx.x + u * NT;
-        if (q < groups) {
-            uint32_t g = q >> sub_log, j = q & (sub - 1);
-            const uint64_t *src = lds + (g << log_s) + j;
+        for (int u = 0; u < MAXG; u++) {
+            const uint32_t q = threadIdx.x + u * NT;
+            if (q < groups) {
+                const uint32_t g = q >> SUB_LOG, j = q & (SUB - 1);
+                const uint32_t base = (g << LOG_S) + j;
 #pragma unroll
-            for (int r = 0; r < R; r++) v[u][r] = src[r * sub];
-            if (log_ns) {
-                uint32_t jm = j & (Ns - 1);
-                const uint64_t *t = tw + (Ns << LR);
+                for (int r = 0; r < R; r++) v[u][r] = lds[padi(base + r * SUB)];
+                if constexpr (LOG_NS > 0) {
+                    const uint32_t jm = j & (NS - 1);
+                    const uint64_t *t = tw + (NS << LR);
 #pragma unroll
-                for (int r = 1; r < R; r++) v[u][r] = mul(v[u][r], t[jm * r]);
+                    for (int r = 1; r < R; r++) v[u][r] = mul(v[u][r], t[jm * r]);
+                }
+                dft_reg<LR, INV>(v[u]);
             }
-            dft_reg<LR, INV>(v[u]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < MAXG; u++) {
+            const uint32_t q = threadIdx.x + u * NT;
+            if (q < groups) {
+                const uint32_t g = q >> SUB_LOG, j = q & (SUB - 1);
+                const uint32_t jm = j & (NS - 1);
+                const uint32_t base = (g << LOG_S) + ((j >> LOG_NS) << (LOG_NS + LR)) + jm;
+#pragma unroll
+                for (int r = 0; r < R; r++) lds[padi(base + r * NS)] = v[u][r];
+            }
+        }
+        __syncthreads();
+        stockham_steps<LOG_S, LOG_NS + LR, INV>(lds, tile, tw);
+    }
+}
+
+template <int LOG_S, bool INV, bool HOLD>
+__global__ __launch_bounds__(NT, 2) void ntt_tile(NttPass p) {
+    __shared__ uint64_t lds[LDS_ELEMS];
+    constexpr uint32_t S = 1u << LOG_S;
+    const uint32_t log_g = p.log_g, G = 1u << log_g;
+    const uint32_t tile = S << log_g;
+    const uint64_t tiles_per_batch = (p.ng + G - 1) >> log_g;
+    // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a
+    // contiguous range of tiles so tiles that share 128-byte lines (runs of
+    // G u64 per row) meet in the same L2.
+    uint64_t blk = blockIdx.x;
+    if ((gridDim.x & 7) == 0) blk = (blk & 7) * (gridDim.x >> 3) + (blk >> 3);
+    const uint64_t b = blk / tiles_per_batch;
+    const uint64_t g0 = (blk - b * tiles_per_batch) << log_g;
+    const uint64_t b_hi = b / p.nb2, b_lo = b - b_hi * p.nb2;
+    const uint64_t *in = p.in + b_hi * p.ib + b_lo * p.ib2 + g0 * p.ig;
+    const bool in_gfast = log_g > 0 && p.ig < p.ii;
+    const bool out_gfast = log_g > 0 && p.og < p.ok;
+    // g-fast lane mapping: element u of a lane is (g = gl, i = il + u*di), so
+    // the lane's twiddles form geometric sequences in u (no table gathers)
+    const bool inc_in = in_gfast && (1u << log_g) <= (uint32_t)NT;
+    const bool inc_out = out_gfast && (1u << log_g) <= (uint32_t)NT;
+    const uint32_t gl = threadIdx.x & (G - 1);
+    const uint32_t il = threadIdx.x >> log_g;
+    const uint32_t di = NT >> log_g;
+    const bool gok = g0 + gl < p.ng;
+
+    // HOLD: the tile stays in VGPRs across the repetitions (LDE cosets)
+    uint64_t xs[HOLD ? EPT : 1];
+    if constexpr (HOLD) {
+#pragma unroll
+        for (int u = 0; u < EPT; u++) {
+            const uint32_t i = il + u * di;
+            xs[u] = (gok && i < S) ? in[(int64_t)gl * p.ig + (int64_t)i * p.ii] : 0;
         }
     }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < MAXG; u++) {
-        uint32_t q = threadIdx.x + u * NT;
-        if (q < groups) {
-            uint32_t g = q >> sub_log, j = q & (sub - 1);
-            uint32_t jm = j & (Ns - 1);
-            uint32_t idx = ((j >> log_ns) << (log_ns + LR)) + jm;
-            uint64_t *dst = lds + (g << log_s) + idx;
-#pragma unroll
-            for (int r = 0; r < R; r++) dst[r * Ns] = v[u][r];
+    uint64_t *out0 = p.out + b_hi * p.ob + b_lo * p.ob2 + g0 * p.og;
+    for (uint32_t rep = 0; rep < p.nr; rep++) {
+        // pre-twiddle w^(r * (g*pre_g + i*pre_i + b*pre_b)), r = r0 + rep
+        const uint64_t rr = p.r0 + rep;
+        const bool pre = p.pre_log && rr;
+        if (HOLD || inc_in) {
+            uint64_t w = 1, st = 1;
+            if (pre) {
+                const uint64_t e0 = (g0 + gl) * p.pre_g + (uint64_t)il * p.pre_i + b * p.pre_b;
+                w = root_pow(p.rt_lo, p.rt_hi, p.pre_log, rr * e0, false);
+                st = root_pow(p.rt_lo, p.rt_hi, p.pre_log, rr * di * p.pre_i, false);
+            }
+#pragma unroll 2
+            for (int u = 0; u < EPT; u++) {
+                const uint32_t i = il + u * di;
+                if (i < S) {
+                    uint64_t x;
+                    if constexpr (HOLD)
+                        x = xs[u];
+                    else
+                        x = gok ? in[(int64_t)gl * p.ig + (int64_t)i * p.ii] : 0;
+                    if (pre) x = mul(x, w);
+                    lds[padi((gl << LOG_S) + i)] = x;
+                }
+                if (pre) w = mul(w, st);
+            }
+        } else {
+#pragma unroll 2
+            for (int u = 0; u < EPT; u++) {
+                const uint32_t e = threadIdx.x + u * NT;
+                if (e >= tile) continue;
+                uint32_t g, i;
+                if (in_gfast) {
+                    g = e & (G - 1);
+                    i = e >> log_g;
+                } else {
+                    i = e & (S - 1);
+                    g = e >> LOG_S;
+                }
+                uint64_t x = 0;
+                if (g0 + g < p.ng) {
+                    x = in[(int64_t)g * p.ig + (int64_t)i * p.ii];
+                    if (pre) {
+                        const uint64_t ee = (g0 + g) * p.pre_g + (uint64_t)i * p.pre_i + b * p.pre_b;
+                        x = mul(x, root_pow(p.rt_lo, p.rt_hi, p.pre_log, ee * rr, false));
+                    }
+                }
+                lds[padi((g << LOG_S) + i)] = x;
+            }
         }
+        __syncthreads();
+        stockham_steps<LOG_S, 0, INV>(lds, tile, p.tw);
+        uint64_t *out = out0 + (uint64_t)rep * p.out_rs;
+        if (inc_out) {
+            // post twiddle w^(+-g*k) and output scale * s^(g*sp_g + k*sp_k)
+            uint64_t pw = 1, pst = 1, sw = p.scale, sst = 1;
+            if (p.post_log) {
+                pw = root_pow(p.rt_lo, p.rt_hi, p.post_log, (g0 + gl) * il, p.inv);
+                pst = root_pow(p.rt_lo, p.rt_hi, p.post_log, (g0 + gl) * di, p.inv);
+            }
+            if (p.sp_lo) {
+                const uint64_t e0 = (g0 + gl) * p.sp_g + (uint64_t)il * p.sp_k, ed = (uint64_t)di * p.sp_k;
+                sw = mul(sw, mul(p.sp_lo[e0 & 8191], p.sp_hi[e0 >> 13]));
+                sst = mul(p.sp_lo[ed & 8191], p.sp_hi[ed >> 13]);
+            }
+            const bool sc = p.sp_lo || p.scale != 1;
+#pragma unroll 2
+            for (int u = 0; u < EPT; u++) {
+                const uint32_t k = il + u * di;
+                if (k < S && gok) {
+                    uint64_t x = lds[padi((gl << LOG_S) + k)];
+                    if (p.post_log) x = mul(x, pw);
+                    if (sc) x = mul(x, sw);
+                    if (!p.post_log && !sc) x = canon(x);
+                    out[(int64_t)gl * p.og + (int64_t)k * p.ok] = x;
+                }
+                if (p.post_log) pw = mul(pw, pst);
+                if (p.sp_lo) sw = mul(sw, sst);
+            }
+        } else {
+#pragma unroll 2
+            for (int u = 0; u < EPT; u++) {
+                const uint32_t e = threadIdx.x + u * NT;
+                if (e >= tile) continue;
+                uint32_t g, k;
+                if (out_gfast) {
+                    g = e & (G - 1);
+                    k = e >> log_g;
+                } else {
+                    k = e & (S - 1);
+                    g = e >> LOG_S;
+                }
+                if (g0 + g >= p.ng) continue;
+                uint64_t x = lds[padi((g << LOG_S) + k)];
+                if (p.post_log) x = mul(x, root_pow(p.rt_lo, p.rt_hi, p.post_log, (g0 + g) * k, p.inv));
+                if (p.scale != 1) x = mul(x, p.scale);
+                if (p.sp_lo) {
+                    const uint64_t ee = (g0 + g) * p.sp_g + (uint64_t)k * p.sp_k;
+                    x = mul(x, mul(p.sp_lo[ee & 8191], p.sp_hi[ee >> 13]));
+                }
+                if (!p.post_log && p.scale == 1 && !p.sp_lo) x = canon(x);
+                out[(int64_t)g * p.og + (int64_t)k * p.ok] = x;
+            }
+        }
+        __syncthreads();
     }
-    __syncthreads();
 }
 
 template <bool INV>
-__device__ void dft_tile(uint64_t *lds, uint32_t log_s, uint32_t tile, const uint64_t *tw) {
-    uint32_t log_ns = 0;
-    while (log_ns < log_s) {
-        uint32_t left = log_s - log_ns;
-        if (left >= 4) {
-            stockham_step<4, INV>(lds, log_s, tile, log_ns, tw);
-            log_ns += 4;
-        } else if (left == 3) {
-            stockham_step<3, INV>(lds, log_s, tile, log_ns, tw);
-            log_ns += 3;
-        } else if (left == 2) {
-            stockham_step<2, INV>(lds, log_s, tile, log_ns, tw);
-            log_ns += 2;
-        } else {
-            stockham_step<1, INV>(lds, log_s, tile, log_ns, tw);
-            log_ns += 1;
+void launch_tile(uint32_t log_s, bool hold, dim3 grid, hipStream_t st, const NttPass &p) {
+    if (hold) {
+        switch (log_s) {
+#define SG_CASE(L) \
+    case L: hipLaunchKernelGGL((ntt_tile<L, INV, true>), grid, dim3(NT), 0, st, p); return;
+            SG_CASE(5) SG_CASE(6) SG_CASE(7) SG_CASE(8) SG_CASE(9) SG_CASE(10) SG_CASE(11) SG_CASE(12)
+#undef SG_CASE
+        default: break;
         }
     }
-}
-
-__global__ __launch_bounds__(NT) void ntt_pass(NttPass p) {
-    __shared__ uint64_t lds[TILE];
-    const uint32_t log_s = p.log_s, log_g = p.log_g;
-    const uint32_t S = 1u << log_s, G = 1u << log_g;
-    const uint32_t tile = S << log_g;
-    const uint64_t tiles_per_batch = (p.ng + G - 1) >> log_g;
-    const uint64_t blk = blockIdx.x;
-    const uint64_t b = blk / tiles_per_batch;
-    const uint64_t g0 = (blk - b * tiles_per_batch) << log_g;
-    const uint64_t *in = p.in + b * p.ib + g0 * p.ig;
-    const bool in_gfast = log_g > 0 && p.ig < p.ii;
-    for (uint32_t e = threadIdx.x; e < tile; e += NT) {
-        uint32_t g, i;
-        if (in_gfast) {
-            g = e & (G - 1);
-            i = e >> log_g;
-        } else {
-            i = e & (S - 1);
-            g = e >> log_s;
-        }
-        uint64_t x = 0;
-        if (g0 + g < p.ng) {
-            x = in[(int64_t)g * p.ig + (int64_t)i * p.ii];
-            if (p.pre_log) x = mul(x, root_pow(p, p.pre_log, (g0 + g) * p.pre_g + i * p.pre_i + b * p.pre_b, false));
-        }
-        lds[(g << log_s) + i] = x;
-    }
-    __syncthreads();
-    if (p.inv)
-        dft_tile<true>(lds, log_s, tile, p.tw);
-    else
-        dft_tile<false>(lds, log_s, tile, p.tw);
-    uint64_t *out = p.out + b * p.ob + g0 * p.og;
-    const bool out_gfast = log_g > 0 && p.og < p.ok;
-    for (uint32_t e = threadIdx.x; e < tile; e += NT) {
-        uint32_t g, k;
-        if (out_gfast) {
-            g = e & (G - 1);
-            k = e >> log_g;
-        } else {
-            k = e & (S - 1);
-            g = e >> log_s;
-        }
-        if (g0 + g >= p.ng) continue;
-        uint64_t x = lds[(g << log_s) + k];
-        if (p.post_log) x = mul(x, root_pow(p, p.post_log, (g0 + g) * k, p.inv));
-        if (p.scale != 1) x = mul(x, p.scale);
-        if (p.sp_lo) {
-            uint64_t ee = (g0 + g) * p.sp_g + (uint64_t)k * p.sp_k;
-            x = mul(x, mul(p.sp_lo[ee & 8191], p.sp_hi[ee >> 13]));
-        }
-        out[(int64_t)g * p.og + (int64_t)k * p.ok] = x;
+    switch (log_s) {
+#define SG_CASE(L) \
+    case L: hipLaunchKernelGGL((ntt_tile<L, INV, false>), grid, dim3(NT), 0, st, p); break;
+        SG_CASE(0) SG_CASE(1) SG_CASE(2) SG_CASE(3) SG_CASE(4) SG_CASE(5) SG_CASE(6)
+        SG_CASE(7) SG_CASE(8) SG_CASE(9) SG_CASE(10) SG_CASE(11) SG_CASE(12)
+#undef SG_CASE
+    default: throw Error(E_INVALID, "ntt pass: sub-DFT larger than 4096");
     }
 }
 
@@ -222,17 +365,25 @@ __global__ __launch_bounds__(NT) void ntt_pass(NttPass p) {
 
 void launch_pass(Ctx *c, NttPass p, const char *name) {
     SG_REQUIRE(p.log_s <= 12, "ntt pass: sub-DFT larger than 4096");
-    uint32_t max_g = TILE_LOG - p.log_s;
+    const uint32_t max_g = TILE_LOG - p.log_s;
     if (p.log_g > max_g) p.log_g = max_g;
+    if (p.nr == 0) p.nr = 1;
+    if (p.nb2 == 0) p.nb2 = 1;
+    SG_REQUIRE(p.pre_log <= 26 && p.post_log <= 26, "ntt pass: twiddle order above 2^26");
     p.rt_lo = c->rt_lo;
     p.rt_hi = c->rt_hi;
     p.tw = p.inv ? c->tw_i : c->tw_f;
-    uint64_t G = 1ULL << p.log_g;
-    uint64_t tiles = ((p.ng + G - 1) / G) * p.nb;
+    const uint64_t G = 1ULL << p.log_g;
+    const uint64_t tiles = ((p.ng + G - 1) / G) * p.nb;
     SG_REQUIRE(tiles < (1ULL << 31), "ntt pass: grid too large");
-    double bytes = 16.0 * (double)p.ng * (double)(1u << p.log_s) * p.nb;
+    const double bytes = 8.0 * (double)p.ng * (double)(1u << p.log_s) * p.nb * (1.0 + p.nr);
     ProfScope ps(c, name, bytes);
-    hipLaunchKernelGGL(ntt_pass, dim3((uint32_t)tiles), dim3(NT), 0, c->stream, p);
+    // hold the tile in VGPRs across repetitions when the input is g-fast
+    const bool hold = p.nr > 1 && p.log_g > 0 && (1u << p.log_g) <= (uint32_t)NT && p.ig < p.ii && p.log_s >= 5;
+    if (p.inv)
+        launch_tile<true>(p.log_s, hold, dim3((uint32_t)tiles), c->stream, p);
+    else
+        launch_tile<false>(p.log_s, hold, dim3((uint32_t)tiles), c->stream, p);
     SG_HIP(hipGetLastError());
 }
 
@@ -245,16 +396,15 @@ static uint32_t pick_log_g(uint64_t ng, uint32_t log_s) {
 // Power table s^e, e < 2^26, two levels of 2^13 entries, built on the host.
 PowTable make_pow_table(Ctx *c, uint64_t s) {
     PowTable t;
-    std::vector<uint64_t> lo(8192), hi(8192);
-    uint64_t s13 = pow(s, 8192);
-    lo[0] = hi[0] = 1;
+    std::vector<uint64_t> tab(16384);
+    const uint64_t s13 = pow(s, 8192);
+    tab[0] = tab[8192] = 1;
     for (int i = 1; i < 8192; i++) {
-        lo[i] = mul(lo[i - 1], s);
-        hi[i] = mul(hi[i - 1], s13);
+        tab[i] = mul(tab[i - 1], s);
+        tab[8192 + i] = mul(tab[8192 + i - 1], s13);
     }
     t.buf = c->alloc_n<uint64_t>(16384);
-    SG_HIP(hipMemcpyAsync(t.buf, lo.data(), 8 * 8192, hipMemcpyHostToDevice, c->stream));
-    SG_HIP(hipMemcpyAsync(t.buf + 8192, hi.data(), 8 * 8192, hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipMemcpyAsync(t.buf, tab.data(), 8 * 16384, hipMemcpyHostToDevice, c->stream));
     SG_HIP(hipStreamSynchronize(c->stream));
     t.lo = t.buf;
     t.hi = t.buf + 8192;
@@ -264,6 +414,8 @@ PowTable make_pow_table(Ctx *c, uint64_t s) {
 static NttPass base_pass() {
     NttPass p{};
     p.scale = 1;
+    p.nr = 1;
+    p.nb2 = 1;
     return p;
 }
 
@@ -272,7 +424,7 @@ void ntt_columns(Ctx *c, const uint64_t *in, uint64_t *out, uint64_t *scratch, u
                  uint64_t in_cs, uint64_t out_cs, bool inv, const PowTable *spow) {
     SG_REQUIRE(L <= 24, "ntt: size above 2^24");
     const uint64_t n = 1ULL << L;
-    uint64_t scale = inv ? sg::inv(n % P) : 1;
+    const uint64_t scale = inv ? sg::inv(n % P) : 1;
     if (L <= 12) {
         NttPass p = base_pass();
         p.in = in;
@@ -345,63 +497,74 @@ void lde_columns(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint32
     SG_REQUIRE(L + LB <= 26, "lde: domain above 2^26");
     SG_REQUIRE(L <= 24, "lde: trace above 2^24");
     const uint64_t n = 1ULL << L, B = 1ULL << LB;
-    Scoped coef(c, 8 * n * ncols), scr(c, 8 * n * ncols);
+    Scoped coef(c, 8 * n * ncols);
     PowTable sp{};
-    bool use_shift = shift != 1;
+    const bool use_shift = shift != 1;
     if (use_shift) sp = make_pow_table(c, shift);
-    ntt_columns(c, cols, coef.as<uint64_t>(), scr.as<uint64_t>(), L, ncols, in_cs, n, true, use_shift ? &sp : nullptr);
+    {
+        Scoped scr(c, 8 * n * ncols);
+        ntt_columns(c, cols, coef.as<uint64_t>(), scr.as<uint64_t>(), L, ncols, in_cs, n, true,
+                    use_shift ? &sp : nullptr);
+    }
     const uint32_t LN = L + LB;
-    for (uint64_t r = 0; r < B; r++) {
-        if (L <= 12) {
-            NttPass p = base_pass();
-            p.in = coef.as<uint64_t>();
-            p.out = out + r;
-            p.nb = 1;
-            p.ng = ncols;
-            p.ig = n;
-            p.ii = 1;
-            p.og = out_cs;
-            p.ok = B;
-            p.log_s = L;
-            p.log_g = pick_log_g(ncols, L);
-            p.pre_log = LN;
-            p.pre_i = r;
-            launch_pass(c, p, "lde_small");
-            continue;
-        }
+    if (L <= 12) {
+        // one pass per column tile, the B cosets as repetitions
+        NttPass p = base_pass();
+        p.in = coef.as<uint64_t>();
+        p.out = out;
+        p.nb = 1;
+        p.ng = ncols;
+        p.ig = n;
+        p.ii = 1;
+        p.og = out_cs;
+        p.ok = B;
+        p.log_s = L;
+        p.log_g = pick_log_g(ncols, L);
+        p.pre_log = LN;
+        p.pre_i = 1;
+        p.nr = (uint32_t)B;
+        p.out_rs = 1;
+        launch_pass(c, p, "lde_small");
+    } else {
         const uint32_t L1 = (L + 1) / 2, L2 = L - L1;
         const uint64_t n1 = 1ULL << L1, n2 = 1ULL << L2;
+        // scratch [col][k1][r][i2]: row q = k1*B + r of pass B is the k1-th
+        // column DFT of coset r, and LDE[B*(k1 + n1*k2) + r] = out[q + B*n1*k2],
+        // so pass B writes runs of consecutive q
+        Scoped scr(c, 8 * n * B * ncols);
         NttPass a = base_pass();
         a.in = coef.as<uint64_t>();
         a.out = scr.as<uint64_t>();
         a.nb = (uint32_t)ncols;
         a.ib = n;
-        a.ob = n;
+        a.ob = B * n;
         a.ng = n2;
         a.ig = 1;
         a.ii = n2;
         a.og = 1;
-        a.ok = n2;
+        a.ok = B * n2;
         a.log_s = L1;
         a.log_g = pick_log_g(n2, L1);
         a.post_log = L;
-        a.pre_log = LN;
-        a.pre_g = r;
-        a.pre_i = r * n2;
+        a.pre_log = LN;  // repetition r multiplies element j = i*n2 + g by w_{Bn}^(r j)
+        a.pre_g = 1;
+        a.pre_i = n2;
+        a.nr = (uint32_t)B;
+        a.out_rs = n2;
         launch_pass(c, a, "lde_passA");
         NttPass b = base_pass();
         b.in = scr.as<uint64_t>();
-        b.out = out + r;
+        b.out = out;
         b.nb = (uint32_t)ncols;
-        b.ib = n;
+        b.ib = B * n;
         b.ob = out_cs;
-        b.ng = n1;
+        b.ng = B * n1;
         b.ig = n2;
         b.ii = 1;
-        b.og = B;
+        b.og = 1;
         b.ok = B * n1;
         b.log_s = L2;
-        b.log_g = pick_log_g(n1, L2);
+        b.log_g = pick_log_g(B * n1, L2);
         launch_pass(c, b, "lde_passB");
     }
     if (use_shift) c->release(sp.buf);

@@ -15,9 +15,16 @@ struct NttPass {
     int64_t ob, og, ok;
     uint64_t ng;
     uint32_t nb;
+    // batch b splits into (b / nb2, b % nb2) with strides (ib, ib2) / (ob, ob2)
+    uint32_t nb2;
+    int64_t ib2, ob2;
+    // nr repetitions of the pass over the same input, repetition q using
+    // pre-twiddle multiplier r = r0 + q and writing at out + q * out_rs
+    uint32_t nr, r0;
+    int64_t out_rs;
     uint32_t log_s, log_g;
     uint32_t inv;
-    // input pre-multiply by w_{2^pre_log}^(g*pre_g + i*pre_i + b*pre_b) (0 = off)
+    // input pre-multiply by w_{2^pre_log}^(r * (g*pre_g + i*pre_i + b*pre_b)) (0 = off)
     uint32_t pre_log;
     uint64_t pre_g, pre_i, pre_b;
     // output multiply by w_{2^post_log}^(g*k), inverse root when inv (0 = off)
