@@ -114,6 +114,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", action="store_true")
     ap.add_argument("--streams", type=int, default=4, help="proofs in flight for the batched measurement")
+    ap.add_argument("--shards", type=int, default=32, help="inner-shard proofs for the plain-vs-batched line (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the prover core runs on a single MI355X and this build has no "
@@ -210,10 +211,71 @@ def main():
         res["ms_per_step"] = round(1e3 * b["seconds"] / args.steps, 3)
         res["config"]["proofs_in_flight"] = args.streams
         res["config"]["step"] = "%d concurrent prover-core proofs (one per HIP stream)" % args.streams
+    if args.shards:
+        log("inner shards: plain vs batched")
+        res.update(shard_batch(S, args.streams, args.shards))
     if not args.no_cpu:
         log("cpu baseline: timing the oracle on a bounded sample")
         res["cpu_baseline"] = cpu_baseline(w, lg, blowup, schedule, s0)
     print(json.dumps(res))
+
+
+def shard_batch(S, K, nproofs, records_per_shard=1024):
+    """prove_inner_shard (HashRollup, n0 = 2^17 at 1024 records) on distinct
+    synthetic shards: one proof at a time on one stream (plain) vs K streams
+    with one host thread each (batched). Every batched proof must equal the
+    plain proof of the same shard."""
+    import threading
+    from stark_swarm_amd.zone import build_synthetic_zone
+    salt = b"swarm-test-zone1"
+    zone = build_synthetic_zone(records_per_shard * 4)
+    jobs = []
+    for i in range(nproofs):
+        off = (i * 97) % (len(zone) - records_per_shard)
+        fsb = S.shard_fs_binding(b"\x11" * 32, b"\x22" * 32, i + 1, i.to_bytes(32, "little"))
+        jobs.append((zone[off:off + records_per_shard], fsb))
+    ctx0 = S.default_context()
+    S.prove_inner_shard(salt, jobs[0][0], jobs[0][1], ctx=ctx0)  # warm
+    t0 = time.perf_counter()
+    lat = []
+    plain = []
+    for recs, fsb in jobs:
+        t = time.perf_counter()
+        plain.append(S.prove_inner_shard(salt, recs, fsb, ctx=ctx0).proof_blob)
+        lat.append(time.perf_counter() - t)
+    plain_dt = time.perf_counter() - t0
+    ctxs = [S.Context(0) for _ in range(K)]
+    for c in ctxs:
+        S.prove_inner_shard(salt, jobs[0][0], jobs[0][1], ctx=c)  # warm
+    out = [None] * nproofs
+    errs = []
+
+    def worker(k):
+        try:
+            for i in range(k, nproofs, K):
+                out[i] = S.prove_inner_shard(salt, jobs[i][0], jobs[i][1], ctx=ctxs[k]).proof_blob
+        except Exception as e:  # noqa: BLE001
+            errs.append(repr(e))
+
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=worker, args=(k,)) for k in range(K)]
+    [t.start() for t in ths]
+    [t.join() for t in ths]
+    bdt = time.perf_counter() - t0
+    if errs:
+        raise SystemExit("; ".join(errs))
+    exact = all(out[i] == plain[i] for i in range(nproofs))
+    if not exact:
+        raise SystemExit("batched inner-shard proofs differ from the plain proofs")
+    for c in ctxs:
+        c.close()
+    lat.sort()
+    return {"inner_shard": {"records": records_per_shard, "n0": 1 << 17, "proofs": nproofs, "streams": K,
+                            "plain_proofs_per_s": round(nproofs / plain_dt, 2),
+                            "batched_proofs_per_s": round(nproofs / bdt, 2),
+                            "speedup": round(plain_dt / bdt, 3),
+                            "plain_latency_ms_p50": round(1e3 * lat[len(lat) // 2], 3),
+                            "bit_exact_vs_plain": exact}}
 
 
 def batched(S, args, w, n, blowup, params, ref_blob, plain_ms):
