@@ -222,10 +222,9 @@ def main():
 
 def shard_batch(S, K, nproofs, records_per_shard=1024):
     """prove_inner_shard (HashRollup, n0 = 2^17 at 1024 records) on distinct
-    synthetic shards: one proof at a time on one stream (plain) vs K streams
-    with one host thread each (batched). Every batched proof must equal the
-    plain proof of the same shard."""
-    import threading
+    synthetic shards: one proof at a time (plain) vs all of them as one
+    lockstep device batch. Every batched proof must equal the plain proof of
+    the same shard."""
     from stark_swarm_amd.zone import build_synthetic_zone
     salt = b"swarm-test-zone1"
     zone = build_synthetic_zone(records_per_shard * 4)
@@ -244,36 +243,22 @@ def shard_batch(S, K, nproofs, records_per_shard=1024):
         plain.append(S.prove_inner_shard(salt, recs, fsb, ctx=ctx0).proof_blob)
         lat.append(time.perf_counter() - t)
     plain_dt = time.perf_counter() - t0
-    ctxs = [S.Context(0) for _ in range(K)]
-    for c in ctxs:
-        S.prove_inner_shard(salt, jobs[0][0], jobs[0][1], ctx=c)  # warm
-    out = [None] * nproofs
-    errs = []
-
-    def worker(k):
-        try:
-            for i in range(k, nproofs, K):
-                out[i] = S.prove_inner_shard(salt, jobs[i][0], jobs[i][1], ctx=ctxs[k]).proof_blob
-        except Exception as e:  # noqa: BLE001
-            errs.append(repr(e))
-
+    # lockstep batch: all shards in one stark_prove_inner_shard_batch call
+    shards = [j[0] for j in jobs]
+    fsbs = [j[1] for j in jobs]
+    S.prove_inner_shard_batch(salt, shards[:K], fsbs[:K], ctx=ctx0)  # warm
     t0 = time.perf_counter()
-    ths = [threading.Thread(target=worker, args=(k,)) for k in range(K)]
-    [t.start() for t in ths]
-    [t.join() for t in ths]
-    bdt = time.perf_counter() - t0
-    if errs:
-        raise SystemExit("; ".join(errs))
-    exact = all(out[i] == plain[i] for i in range(nproofs))
+    outs = S.prove_inner_shard_batch(salt, shards, fsbs, ctx=ctx0)
+    ldt = time.perf_counter() - t0
+    exact = all(outs[i].proof_blob == plain[i] for i in range(nproofs))
     if not exact:
-        raise SystemExit("batched inner-shard proofs differ from the plain proofs")
-    for c in ctxs:
-        c.close()
+        raise SystemExit("lockstep-batched inner-shard proofs differ from the plain proofs")
     lat.sort()
-    return {"inner_shard": {"records": records_per_shard, "n0": 1 << 17, "proofs": nproofs, "streams": K,
+    return {"inner_shard": {"records": records_per_shard, "n0": 1 << 17, "proofs": nproofs,
                             "plain_proofs_per_s": round(nproofs / plain_dt, 2),
-                            "batched_proofs_per_s": round(nproofs / bdt, 2),
-                            "speedup": round(plain_dt / bdt, 3),
+                            "batched_proofs_per_s": round(nproofs / ldt, 2),
+                            "speedup": round(plain_dt / ldt, 3),
+                            "batch": "one lockstep device batch (stark_prove_inner_shard_batch)",
                             "plain_latency_ms_p50": round(1e3 * lat[len(lat) // 2], 3),
                             "bit_exact_vs_plain": exact}}
 

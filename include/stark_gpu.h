@@ -116,6 +116,25 @@ int stark_merkle_commit(stark_ctx *ctx, const uint64_t *leaf_vals, size_t n_leav
 int stark_prove_inner_shard(stark_ctx *ctx, const uint8_t salt[16], const stark_dns_record *records, size_t n,
                             const uint8_t fs_binding[32], int ldt, int phi_mode, stark_inner_shard_out *out);
 
+/* Batched prove_inner_shard: n_shards independent shards proven together
+ * (one lockstep device batch per trace size, host hashing on a thread pool).
+ * Shard s owns records [sum(shard_sizes[0..s]), +shard_sizes[s]) of
+ * `records` and fs_bindings[32*s .. 32*s+32); outs[s] receives exactly what
+ * stark_prove_inner_shard returns for that shard alone. No counterpart in
+ * the reference: it replaces a loop of prove_inner_shard calls
+ * (crates/swarm-dns/src/prover.rs:148) over the shards a worker holds. */
+int stark_prove_inner_shard_batch(stark_ctx *ctx, const uint8_t salt[16], const stark_dns_record *records,
+                                  const size_t *shard_sizes, size_t n_shards, const uint8_t *fs_bindings, int ldt,
+                                  int phi_mode, stark_inner_shard_out *outs);
+
+/* Batched deep_fri_prove (crates/deep_ali/src/fri.rs:1900): n_proofs f0 vectors
+ * of n0 values each, back to back; public_inputs_hashes (32 bytes per proof,
+ * or NULL to use params->public_inputs_hash for all). proofs[k] / proof_lens[k]
+ * receive library-allocated bytes equal to the single-proof result. */
+int stark_deep_fri_prove_batch(stark_ctx *ctx, const uint64_t *f0s, size_t n_proofs, size_t n0,
+                               const stark_fri_params *params, const uint8_t *public_inputs_hashes,
+                               uint8_t **proofs, size_t *proof_lens);
+
 /* prove_outer_rollup (crates/swarm-dns/src/prover.rs:253-304). */
 int stark_prove_outer_rollup(stark_ctx *ctx, const uint8_t *pi_hashes, size_t n, const uint8_t pk_hash[32], int ldt,
                              int phi_mode, uint8_t **proof, size_t *proof_len, uint8_t root_f0[32]);

@@ -38,6 +38,7 @@ from .api import (  # noqa: F401
     build_params,
     deep_ali_merge_general,
     deep_fri_prove,
+    deep_fri_prove_batch,
     default_context,
     lde_trace_columns,
     lib,
@@ -45,6 +46,7 @@ from .api import (  # noqa: F401
     merkle_commit,
     ntt,
     prove_inner_shard,
+    prove_inner_shard_batch,
     prove_outer_rollup,
     prover_core,
     shard_fs_binding,

@@ -74,6 +74,11 @@ def lib():
         "stark_prove_inner_shard": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.POINTER(_DnsRecord), sz,
                                                    ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
                                                    ctypes.POINTER(_InnerOut)]),
+        "stark_prove_inner_shard_batch": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.POINTER(_DnsRecord),
+                                                         ctypes.POINTER(sz), sz, ctypes.c_char_p, ctypes.c_int,
+                                                         ctypes.c_int, ctypes.POINTER(_InnerOut)]),
+        "stark_deep_fri_prove_batch": (ctypes.c_int, [vp, u64p, sz, sz, ctypes.POINTER(_FriParams), ctypes.c_char_p,
+                                                      ctypes.POINTER(u8p), ctypes.POINTER(sz)]),
         "stark_prove_outer_rollup": (ctypes.c_int, [vp, ctypes.c_char_p, sz, ctypes.c_char_p, ctypes.c_int,
                                                     ctypes.c_int, ctypes.POINTER(u8p), ctypes.POINTER(sz),
                                                     ctypes.c_char_p]),
@@ -445,6 +450,42 @@ def prove_inner_shard(salt, records: Sequence[DnsRecord], fs_binding_32, ldt=Ldt
     blob = _take(out.proof, out.proof_len)
     return InnerShardOutput(bytes(out.pi_hash), bytes(out.merkle_root), out.record_count, out.n_trace, len(blob),
                             bytes(out.root_f0), blob)
+
+
+def prove_inner_shard_batch(salt, shards, fs_bindings, ldt=LdtMode.Stir, phi_mode=PHI_RAYON, ctx=None):
+    """Many prove_inner_shard calls as one lockstep device batch
+    (stark_prove_inner_shard_batch); result k equals prove_inner_shard on shard k."""
+    ctx = ctx or default_context()
+    assert len(shards) == len(fs_bindings)
+    flat = [r._c() for s in shards for r in s]
+    arr = (_DnsRecord * max(1, len(flat)))(*flat)
+    sizes = (ctypes.c_size_t * max(1, len(shards)))(*[len(s) for s in shards])
+    outs = (_InnerOut * max(1, len(shards)))()
+    fsb = b"".join(bytes(f) for f in fs_bindings)
+    ctx._check(lib().stark_prove_inner_shard_batch(ctx.h, bytes(salt), arr, sizes, len(shards), fsb, ldt.value,
+                                                   phi_mode, outs))
+    res = []
+    for k in range(len(shards)):
+        o = outs[k]
+        blob = _take(o.proof, o.proof_len)
+        res.append(InnerShardOutput(bytes(o.pi_hash), bytes(o.merkle_root), o.record_count, o.n_trace, len(blob),
+                                    bytes(o.root_f0), blob))
+    return res
+
+
+def deep_fri_prove_batch(f0s, domain0, params, public_inputs_hashes=None, ctx=None):
+    """deep_fri_prove over many f0 vectors of one domain in lockstep; result k
+    equals deep_fri_prove(f0s[k]) with public_inputs_hash = public_inputs_hashes[k]."""
+    ctx = ctx or default_context()
+    f0s = np.ascontiguousarray(f0s, dtype=np.uint64)
+    K, n0 = f0s.shape
+    assert n0 == domain0.size
+    p = params._c()
+    pp = (u8p * K)()
+    pl = (sz * K)()
+    pis = b"".join(bytes(h) for h in public_inputs_hashes) if public_inputs_hashes is not None else None
+    ctx._check(lib().stark_deep_fri_prove_batch(ctx.h, f0s.ctypes.data_as(u64p), K, n0, ctypes.byref(p), pis, pp, pl))
+    return [_take(pp[k], pl[k]) for k in range(K)]
 
 
 def prove_outer_rollup(pi_hashes, pk_hash_32, ldt=LdtMode.Stir, phi_mode=PHI_RAYON, ctx=None):

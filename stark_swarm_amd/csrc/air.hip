@@ -21,10 +21,13 @@ namespace {
 
 constexpr int AT = 256;
 
+// grid.y = proof: lde of proof k at lde + 4 k n, phi at phi + k n
 __global__ __launch_bounds__(AT) void compose_hash_rollup_k(const uint64_t *lde, uint64_t n, uint64_t blowup,
                                                             uint64_t *phi) {
     const uint64_t i = (uint64_t)blockIdx.x * AT + threadIdx.x;
     if (i >= n) return;
+    lde += 4 * n * blockIdx.y;
+    phi += n * blockIdx.y;
     const uint64_t j = (i + blowup) & (n - 1);
     const uint64_t *idx = lde, *leaf = lde + n, *st = lde + 2 * n, *sq = lde + 3 * n;
     const uint64_t s = st[i], q = sq[i];
@@ -39,6 +42,8 @@ __global__ __launch_bounds__(AT) void compose_hash_rollup_k(const uint64_t *lde,
 __global__ __launch_bounds__(AT) void zh_div_k(const uint64_t *a, uint64_t n, uint64_t m, uint64_t *q) {
     const uint64_t rho = (uint64_t)blockIdx.x * AT + threadIdx.x;
     if (rho >= m) return;
+    a += n * blockIdx.y;
+    q += n * blockIdx.y;
     const uint64_t B = n / m;
     uint64_t acc = 0;
     q[rho + (B - 1) * m] = 0;  // q has length n - m; the top block is zero padding
@@ -63,32 +68,38 @@ void fill_u64(Ctx *c, uint64_t *p, uint64_t n, uint64_t v) {
 
 bool phi_computed(int phi_mode, uint64_t n) { return phi_mode == 0 || (phi_mode == 1 && n >= 4096); }
 
-void merge_hash_rollup_dev(Ctx *c, const uint64_t *lde, uint64_t n_trace, uint64_t blowup, int phi_mode,
-                           uint64_t *c_eval) {
+void merge_hash_rollup_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup,
+                             int phi_mode, uint64_t *c_eval) {
     SG_REQUIRE(is_pow2(n_trace) && is_pow2(blowup) && blowup >= 2, "merge: sizes must be powers of two");
+    SG_REQUIRE(K >= 1 && K <= 65535, "merge: batch size out of range");
     const uint64_t n = n_trace * blowup;
     SG_REQUIRE(n <= (1ULL << 24), "merge: domain above 2^24");
     if (!phi_computed(phi_mode, n)) {
         // phi == 0: iNTT, division and NTT of zero are zero
-        fill_u64(c, c_eval, n, 0);
+        fill_u64(c, c_eval, n * K, 0);
         return;
     }
-    Scoped phi(c, 8 * n), coef(c, 8 * n), scr(c, 8 * n);
+    Scoped phi(c, 8 * n * K), coef(c, 8 * n * K), scr(c, 8 * n * K);
     {
-        ProfScope ps(c, "compose_hash_rollup", 8.0 * 4 * n + 8.0 * n);
-        hipLaunchKernelGGL(compose_hash_rollup_k, dim3((uint32_t)((n + AT - 1) / AT)), dim3(AT), 0, c->stream, lde, n,
-                           blowup, phi.as<uint64_t>());
+        ProfScope ps(c, "compose_hash_rollup", (8.0 * 4 * n + 8.0 * n) * K);
+        hipLaunchKernelGGL(compose_hash_rollup_k, dim3((uint32_t)((n + AT - 1) / AT), K), dim3(AT), 0, c->stream, lde,
+                           n, blowup, phi.as<uint64_t>());
         SG_HIP(hipGetLastError());
     }
     const uint32_t L = ilog2(n);
-    ntt_columns(c, phi.as<uint64_t>(), coef.as<uint64_t>(), scr.as<uint64_t>(), L, 1, n, n, true, nullptr);
+    ntt_columns(c, phi.as<uint64_t>(), coef.as<uint64_t>(), scr.as<uint64_t>(), L, K, n, n, true, nullptr);
     {
-        ProfScope ps(c, "zh_div", 16.0 * n);
-        hipLaunchKernelGGL(zh_div_k, dim3((uint32_t)((n_trace + AT - 1) / AT)), dim3(AT), 0, c->stream,
+        ProfScope ps(c, "zh_div", 16.0 * n * K);
+        hipLaunchKernelGGL(zh_div_k, dim3((uint32_t)((n_trace + AT - 1) / AT), K), dim3(AT), 0, c->stream,
                            coef.as<uint64_t>(), n, n_trace, phi.as<uint64_t>());
         SG_HIP(hipGetLastError());
     }
-    ntt_columns(c, phi.as<uint64_t>(), c_eval, scr.as<uint64_t>(), L, 1, n, n, false, nullptr);
+    ntt_columns(c, phi.as<uint64_t>(), c_eval, scr.as<uint64_t>(), L, K, n, n, false, nullptr);
+}
+
+void merge_hash_rollup_dev(Ctx *c, const uint64_t *lde, uint64_t n_trace, uint64_t blowup, int phi_mode,
+                           uint64_t *c_eval) {
+    merge_hash_rollup_batch(c, lde, 1, n_trace, blowup, phi_mode, c_eval);
 }
 
 // HashRollup trace (air_workloads.rs:571-589): idx, leaf, state, state^2 with

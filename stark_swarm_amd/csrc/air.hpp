@@ -11,6 +11,9 @@ bool phi_computed(int phi_mode, uint64_t n);
 // c_eval (device, n_trace*blowup) from the 4-column HashRollup LDE (device, column-major)
 void merge_hash_rollup_dev(Ctx *c, const uint64_t *lde, uint64_t n_trace, uint64_t blowup, int phi_mode,
                            uint64_t *c_eval);
+// K proofs: lde of proof k at lde + 4*k*n0, c_eval at c_eval + k*n0
+void merge_hash_rollup_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup,
+                             int phi_mode, uint64_t *c_eval);
 void build_hash_rollup_trace(uint64_t n_trace, const uint64_t *leaves, uint64_t n_leaves, uint64_t *tr);
 
 }  // namespace sg

@@ -184,6 +184,59 @@ int stark_prove_inner_shard(stark_ctx *ctx, const uint8_t salt[16], const stark_
     });
 }
 
+int stark_prove_inner_shard_batch(stark_ctx *ctx, const uint8_t salt[16], const stark_dns_record *records,
+                                  const size_t *shard_sizes, size_t n_shards, const uint8_t *fs_bindings, int ldt,
+                                  int phi_mode, stark_inner_shard_out *outs) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(salt && shard_sizes && fs_bindings && outs && (records || n_shards == 0),
+                   "prove_inner_shard_batch: null argument");
+        if (ldt != STARK_LDT_STIR) throw sg::Error(sg::E_UNSUPPORTED, "only LdtMode::Stir is implemented");
+        std::vector<std::vector<sg::DnsRec>> shards(n_shards);
+        std::vector<std::array<uint8_t, 32>> fsb(n_shards);
+        size_t off = 0;
+        for (size_t s = 0; s < n_shards; s++) {
+            for (size_t i = 0; i < shard_sizes[s]; i++) {
+                const stark_dns_record &r = records[off + i];
+                shards[s].push_back({r.domain, r.domain_len, r.record_type, r.ttl, r.rdata, r.rdata_len});
+            }
+            off += shard_sizes[s];
+            memcpy(fsb[s].data(), fs_bindings + 32 * s, 32);
+        }
+        std::vector<sg::InnerShard> res = sg::prove_inner_shard_batch(c, salt, shards, fsb, phi_mode);
+        for (size_t s = 0; s < n_shards; s++) outs[s].proof = nullptr;
+        for (size_t s = 0; s < n_shards; s++) {
+            memcpy(outs[s].pi_hash, res[s].pi_hash, 32);
+            memcpy(outs[s].merkle_root, res[s].merkle_root, 32);
+            memcpy(outs[s].root_f0, res[s].root_f0, 32);
+            outs[s].record_count = res[s].record_count;
+            outs[s].n_trace = res[s].n_trace;
+            outs[s].proof = copy_out(res[s].proof, &outs[s].proof_len);
+        }
+    });
+}
+
+int stark_deep_fri_prove_batch(stark_ctx *ctx, const uint64_t *f0s, size_t n_proofs, size_t n0,
+                               const stark_fri_params *params, const uint8_t *public_inputs_hashes,
+                               uint8_t **proofs, size_t *proof_lens) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(f0s && proofs && proof_lens && n_proofs >= 1, "deep_fri_prove_batch: null argument");
+        sg::FriParams p = to_params(params);
+        SG_REQUIRE(sg::is_pow2(n0), "deep_fri_prove_batch: n0 must be a power of two");
+        std::vector<std::array<uint8_t, 32>> pis;
+        if (public_inputs_hashes) {
+            pis.resize(n_proofs);
+            for (size_t k = 0; k < n_proofs; k++) memcpy(pis[k].data(), public_inputs_hashes + 32 * k, 32);
+        }
+        sg::Scoped d(c, 8 * n0 * n_proofs);
+        h2d(c, d.p, f0s, 8 * n0 * n_proofs);
+        std::vector<std::vector<uint8_t>> v =
+            sg::deep_fri_prove_batch(c, d.as<uint64_t>(), (uint32_t)n_proofs, n0, p, public_inputs_hashes ? &pis : nullptr,
+                                     nullptr);
+        for (size_t k = 0; k < n_proofs; k++) proofs[k] = nullptr;
+        for (size_t k = 0; k < n_proofs; k++) proofs[k] = copy_out(v[k], &proof_lens[k]);
+    });
+}
+
 int stark_prove_outer_rollup(stark_ctx *ctx, const uint8_t *pi_hashes, size_t n, const uint8_t pk_hash[32], int ldt,
                              int phi_mode, uint8_t **proof, size_t *proof_len, uint8_t root_f0[32]) {
     return guard(ctx, [&](sg::Ctx *c) {

@@ -5,9 +5,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace sg {
@@ -110,6 +112,27 @@ struct ProfScope {
 
 Ctx *ctx_create(int device);
 void ctx_destroy(Ctx *c);
+
+// Run fn(i) for i < n on up to 16 host threads (the per-proof host work of a
+// batch: transcripts, record hashing, serialization).
+template <class F>
+void parallel_for(size_t n, F fn) {
+    size_t nt = std::thread::hardware_concurrency();
+    if (nt == 0) nt = 4;
+    if (nt > 16) nt = 16;
+    if (nt > n) nt = n;
+    if (nt <= 1) {
+        for (size_t i = 0; i < n; i++) fn(i);
+        return;
+    }
+    std::atomic<size_t> next(0);
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < nt; t++)
+        th.emplace_back([&]() {
+            for (size_t i; (i = next.fetch_add(1)) < n;) fn(i);
+        });
+    for (auto &x : th) x.join();
+}
 
 inline uint32_t ilog2(uint64_t n) {
     uint32_t l = 0;

@@ -1,4 +1,4 @@
-// fri.hip -- STIR DEEP-FRI prover for gfx950.
+// fri.hip -- STIR DEEP-FRI prover for gfx950, batched over independent proofs.
 //
 // Device work per layer l (size n, fold arity m), crates/deep_ali/src/fri.rs:
 //   OOD coset   f_l(zeta^j z) for j < m (fri.rs:338-391). With c = iNTT(f_l)
@@ -16,8 +16,13 @@
 // 1813-1893); values cross PCIe only as roots, m coset evaluations, the final
 // layer and the query gathers.
 //
-// Layout: an Fp6 layer is 6 planes of n u64 (component-major), so every
-// kernel reads unit-stride streams and the NTT engine runs on the planes.
+// Batching: K proofs of the same size run in lockstep. Every launch covers
+// all K (grid.y = proof) and each stage synchronizes once for the batch; the
+// K transcripts are independent host objects.
+//
+// Layout: an Fp6 layer is 6 planes of n u64 (component-major), proofs
+// back to back ([proof][component][n]), so every kernel reads unit-stride
+// streams and the NTT engine sees K*6 columns.
 #include "fri.hpp"
 #include "gl.hpp"
 #include "keccak.hpp"
@@ -33,21 +38,15 @@ namespace {
 
 constexpr int FT = 256;
 
-struct F6Arg {
-    uint64_t c[6];
-};
-static F6Arg f6arg(const fp6 &v) {
-    F6Arg a;
-    memcpy(a.c, v.c, 48);
-    return a;
-}
-
-// zpow[t] = z^t for t < CH (host-computed, device copy) ; block partial sums
-// of S_rho over chunks of CH coefficients per thread.
+// Per thread: chunk [k0, k0+ch) of proof blockIdx.y,
+// acc_rho = z^k0 * sum_{t = rho mod M} c_{k0+t} z^t (ch and k0 multiples of M).
+// zpow: per proof, z^t for t < ch; zch: per proof, z^ch.
 template <int M>
 __global__ __launch_bounds__(FT) void ood_sums_k(const uint64_t *coef, uint64_t n, int ncomp, const uint64_t *zpow,
-                                                 uint32_t ch, F6Arg zch, uint64_t *partials) {
-    // thread chunk [k0, k0+ch); acc_rho = z^k0 * sum_t c_{k0+t} z^t, t = rho mod M (ch multiple of M, k0 too)
+                                                 uint32_t ch, const uint64_t *zch, uint64_t *partials) {
+    const uint64_t pk = blockIdx.y;
+    coef += pk * ncomp * n;
+    zpow += pk * 6 * ch;
     const uint64_t tid = (uint64_t)blockIdx.x * FT + threadIdx.x;
     const uint64_t k0 = tid * ch;
     fp6 acc[M];
@@ -70,13 +69,15 @@ __global__ __launch_bounds__(FT) void ood_sums_k(const uint64_t *coef, uint64_t 
                 }
             }
         }
-        // multiply by z^k0 = (z^ch)^tid
-        fp6 zk = f6_pow(*(const fp6 *)&zch, tid);
+        fp6 z_ch;
+        memcpy(z_ch.c, zch + 6 * pk, 48);
+        const fp6 zk = f6_pow(z_ch, tid);
 #pragma unroll
         for (int r = 0; r < M; r++) acc[r] = f6_mul(acc[r], zk);
     }
     // block reduction through LDS, one residue at a time
     __shared__ uint64_t red[FT * 6];
+    uint64_t *part = partials + (pk * gridDim.x + blockIdx.x) * M * 6;
     for (int r = 0; r < M; r++) {
 #pragma unroll
         for (int q = 0; q < 6; q++) red[q * FT + threadIdx.x] = acc[r].c[q];
@@ -89,25 +90,29 @@ __global__ __launch_bounds__(FT) void ood_sums_k(const uint64_t *coef, uint64_t 
             }
             __syncthreads();
         }
-        if (threadIdx.x < 6) partials[((uint64_t)blockIdx.x * M + r) * 6 + threadIdx.x] = red[threadIdx.x * FT];
+        if (threadIdx.x < 6) part[r * 6 + threadIdx.x] = red[threadIdx.x * FT];
         __syncthreads();
     }
 }
 
-// Single block: sum block partials, then evals_j = sum_rho zeta^(j rho) S_rho.
+// One block per proof: sum the block partials, then evals_j = sum_rho zeta^(j rho) S_rho.
 __global__ void ood_finish_k(const uint64_t *partials, uint32_t nblocks, uint32_t m, uint64_t zeta, uint64_t *evals) {
     __shared__ uint64_t S[16 * 6];
+    const uint64_t pk = blockIdx.x;
+    partials += pk * nblocks * m * 6;
+    evals += pk * m * 6;
     const uint32_t t = threadIdx.x;  // t < 6*m : (rho, comp)
     if (t < 6 * m) {
-        uint32_t rho = t / 6, q = t % 6;
+        const uint32_t rho = t / 6, q = t % 6;
         uint64_t s = 0;
         for (uint32_t b = 0; b < nblocks; b++) s = add(s, partials[((uint64_t)b * m + rho) * 6 + q]);
         S[t] = s;
     }
     __syncthreads();
     if (t < 6 * m) {
-        uint32_t j = t / 6, q = t % 6;
-        uint64_t zj = pow(zeta, j), zp = 1, s = 0;
+        const uint32_t j = t / 6, q = t % 6;
+        const uint64_t zj = pow(zeta, j);
+        uint64_t zp = 1, s = 0;
         for (uint32_t rho = 0; rho < m; rho++) {
             s = add(s, mul(S[rho * 6 + q], zp));
             zp = mul(zp, zj);
@@ -172,15 +177,20 @@ __device__ __forceinline__ void idft_small(uint64_t *v) {
     }
 }
 
-// f_next[b] = sum_k alpha^k * (1/m) w^(-bk) * IDFT_m(f[b + j nn])_k ; optional tuple planes
+// f_next[b] = sum_k alpha^k * (1/m) w^(-bk) * IDFT_m(f[b + j nn])_k, proof blockIdx.y;
+// optional coefficient-tuple planes
 template <int LM>
-__global__ __launch_bounds__(FT) void coset_fold_k(const uint64_t *f, uint64_t n, int ncomp, F6Arg alpha,
+__global__ __launch_bounds__(FT) void coset_fold_k(const uint64_t *f, uint64_t n, int ncomp, const uint64_t *alphas,
                                                    uint64_t minv, const uint64_t *rt_lo, const uint64_t *rt_hi,
                                                    uint32_t logn, uint64_t *fnext, uint64_t *tuples) {
     constexpr int M = 1 << LM;
     const uint64_t nn = n >> LM;
     const uint64_t b = (uint64_t)blockIdx.x * FT + threadIdx.x;
     if (b >= nn) return;
+    const uint64_t pk = blockIdx.y;
+    f += pk * ncomp * n;
+    fnext += pk * 6 * nn;
+    if (tuples) tuples += pk * 6 * M * nn;
     uint64_t v[6][M];
 #pragma unroll
     for (int q = 0; q < 6; q++) {
@@ -206,7 +216,8 @@ __global__ __launch_bounds__(FT) void coset_fold_k(const uint64_t *f, uint64_t n
 #pragma unroll
             for (int q = 0; q < 6; q++) tuples[(uint64_t)(k * 6 + q) * nn + b] = y[k].c[q];
     }
-    const fp6 a = *(const fp6 *)&alpha;
+    fp6 a;
+    memcpy(a.c, alphas + 6 * pk, 48);
     fp6 acc = y[M - 1];
 #pragma unroll
     for (int k = M - 2; k >= 0; k--) acc = f6_add(f6_mul(acc, a), y[k]);
@@ -215,64 +226,70 @@ __global__ __launch_bounds__(FT) void coset_fold_k(const uint64_t *f, uint64_t n
 }
 
 template <int LM>
-void launch_fold(Ctx *c, const uint64_t *f, uint64_t n, int ncomp, const fp6 &alpha, uint64_t *fnext,
-                 uint64_t *tuples) {
+void launch_fold(Ctx *c, const uint64_t *f, uint64_t n, int ncomp, uint32_t K, const uint64_t *d_alphas,
+                 uint64_t *fnext, uint64_t *tuples) {
     const uint64_t nn = n >> LM;
     const uint64_t minv = inv(1ULL << LM);
-    ProfScope ps(c, "coset_fold", 8.0 * ncomp * n + 48.0 * nn + (tuples ? 48.0 * n : 0));
-    hipLaunchKernelGGL(coset_fold_k<LM>, dim3((uint32_t)((nn + FT - 1) / FT)), dim3(FT), 0, c->stream, f, n, ncomp,
-                       f6arg(alpha), minv, c->rt_lo, c->rt_hi, ilog2(n), fnext, tuples);
+    ProfScope ps(c, "coset_fold", (8.0 * ncomp * n + 48.0 * nn + (tuples ? 48.0 * n : 0)) * K);
+    hipLaunchKernelGGL(coset_fold_k<LM>, dim3((uint32_t)((nn + FT - 1) / FT), K), dim3(FT), 0, c->stream, f, n, ncomp,
+                       d_alphas, minv, c->rt_lo, c->rt_hi, ilog2(n), fnext, tuples);
     SG_HIP(hipGetLastError());
 }
 
-void fold(Ctx *c, const uint64_t *f, uint64_t n, int ncomp, uint64_t m, const fp6 &alpha, uint64_t *fnext,
-          uint64_t *tuples) {
+void fold(Ctx *c, const uint64_t *f, uint64_t n, int ncomp, uint64_t m, uint32_t K, const uint64_t *d_alphas,
+          uint64_t *fnext, uint64_t *tuples) {
     switch (m) {
-    case 2: launch_fold<1>(c, f, n, ncomp, alpha, fnext, tuples); break;
-    case 4: launch_fold<2>(c, f, n, ncomp, alpha, fnext, tuples); break;
-    case 8: launch_fold<3>(c, f, n, ncomp, alpha, fnext, tuples); break;
-    case 16: launch_fold<4>(c, f, n, ncomp, alpha, fnext, tuples); break;
+    case 2: launch_fold<1>(c, f, n, ncomp, K, d_alphas, fnext, tuples); break;
+    case 4: launch_fold<2>(c, f, n, ncomp, K, d_alphas, fnext, tuples); break;
+    case 8: launch_fold<3>(c, f, n, ncomp, K, d_alphas, fnext, tuples); break;
+    case 16: launch_fold<4>(c, f, n, ncomp, K, d_alphas, fnext, tuples); break;
     default: throw Error(E_UNSUPPORTED, "fold arity must be 2, 4, 8 or 16");
     }
 }
 
+// coset evaluations of K proofs' layers at z_k zeta^j; evals_host: K x m x 6
 template <int M>
-void launch_ood(Ctx *c, const uint64_t *coef, uint64_t n, int ncomp, const fp6 &z, uint64_t zeta, uint64_t *evals_host) {
+void launch_ood(Ctx *c, const uint64_t *coef, uint64_t n, int ncomp, uint32_t K, const std::vector<fp6> &z,
+                uint64_t zeta, uint64_t *evals_host) {
     // chunk per thread: a multiple of M, small enough to keep the grid wide
     uint32_t ch = 64;
-    while (ch > (uint32_t)M && (n / ch) < 4096) ch >>= 1;
+    while (ch > (uint32_t)M && (n / ch) * K < 4096) ch >>= 1;
     if (ch < (uint32_t)M) ch = M;
     const uint64_t threads = (n + ch - 1) / ch;
     const uint32_t nblocks = (uint32_t)((threads + FT - 1) / FT);
-    std::vector<uint64_t> zp(6 * ch);
-    fp6 a = f6_one();
-    for (uint32_t t = 0; t < ch; t++) {
-        memcpy(&zp[6 * t], a.c, 48);
-        a = f6_mul(a, z);
+    std::vector<uint64_t> zp(6 * (size_t)ch * K), zc(6 * (size_t)K);
+    for (uint32_t k = 0; k < K; k++) {
+        fp6 a = f6_one();
+        for (uint32_t t = 0; t < ch; t++) {
+            memcpy(&zp[6 * ((size_t)k * ch + t)], a.c, 48);
+            a = f6_mul(a, z[k]);
+        }
+        memcpy(&zc[6 * k], a.c, 48);  // z^ch
     }
-    fp6 zch = a;  // z^ch
-    Scoped d_zp(c, 48 * ch), d_part(c, 48ULL * M * nblocks), d_ev(c, 48 * M);
-    SG_HIP(hipMemcpyAsync(d_zp.p, zp.data(), 48 * ch, hipMemcpyHostToDevice, c->stream));
+    Scoped d_zp(c, 8 * zp.size()), d_zc(c, 8 * zc.size()), d_part(c, 48ULL * M * nblocks * K),
+        d_ev(c, 48ULL * M * K);
+    SG_HIP(hipMemcpyAsync(d_zp.p, zp.data(), 8 * zp.size(), hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipMemcpyAsync(d_zc.p, zc.data(), 8 * zc.size(), hipMemcpyHostToDevice, c->stream));
     {
-        ProfScope ps(c, "ood_sums", 8.0 * ncomp * n);
-        hipLaunchKernelGGL(ood_sums_k<M>, dim3(nblocks), dim3(FT), 0, c->stream, coef, n, ncomp, d_zp.as<uint64_t>(), ch,
-                           f6arg(zch), d_part.as<uint64_t>());
+        ProfScope ps(c, "ood_sums", 8.0 * ncomp * n * K);
+        hipLaunchKernelGGL(ood_sums_k<M>, dim3(nblocks, K), dim3(FT), 0, c->stream, coef, n, ncomp,
+                           d_zp.as<uint64_t>(), ch, d_zc.as<uint64_t>(), d_part.as<uint64_t>());
         SG_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(ood_finish_k, dim3(1), dim3(128), 0, c->stream, d_part.as<uint64_t>(), nblocks, (uint32_t)M, zeta,
-                       d_ev.as<uint64_t>());
+    hipLaunchKernelGGL(ood_finish_k, dim3(K), dim3(128), 0, c->stream, d_part.as<uint64_t>(), nblocks, (uint32_t)M,
+                       zeta, d_ev.as<uint64_t>());
     SG_HIP(hipGetLastError());
-    SG_HIP(hipMemcpyAsync(evals_host, d_ev.p, 48 * M, hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipMemcpyAsync(evals_host, d_ev.p, 48ULL * M * K, hipMemcpyDeviceToHost, c->stream));
     SG_HIP(hipStreamSynchronize(c->stream));
 }
 
-void ood(Ctx *c, const uint64_t *coef, uint64_t n, int ncomp, uint64_t m, const fp6 &z, uint64_t zeta,
-         uint64_t *evals_host) {
+void ood(Ctx *c, const uint64_t *coef, uint64_t n, int ncomp, uint64_t m, uint32_t K, const std::vector<fp6> &z,
+         uint64_t zeta, uint64_t *evals_host) {
     switch (m) {
-    case 2: launch_ood<2>(c, coef, n, ncomp, z, zeta, evals_host); break;
-    case 4: launch_ood<4>(c, coef, n, ncomp, z, zeta, evals_host); break;
-    case 8: launch_ood<8>(c, coef, n, ncomp, z, zeta, evals_host); break;
-    case 16: launch_ood<16>(c, coef, n, ncomp, z, zeta, evals_host); break;
+    case 2: launch_ood<2>(c, coef, n, ncomp, K, z, zeta, evals_host); break;
+    case 4: launch_ood<4>(c, coef, n, ncomp, K, z, zeta, evals_host); break;
+    case 8: launch_ood<8>(c, coef, n, ncomp, K, z, zeta, evals_host); break;
+    case 16: launch_ood<16>(c, coef, n, ncomp, K, z, zeta, evals_host); break;
     default: throw Error(E_UNSUPPORTED, "fold arity must be 2, 4, 8 or 16");
     }
 }
@@ -289,20 +306,27 @@ struct Bytes {
     }
 };
 
+void upload_f6(Ctx *c, const std::vector<fp6> &v, uint64_t *d) {
+    SG_HIP(hipMemcpyAsync(d, v.data(), 48 * v.size(), hipMemcpyHostToDevice, c->stream));
+}
+
 }  // namespace
 
 std::vector<uint64_t> stir_schedule(uint64_t n0) {
-    uint32_t lg = ilog2(n0);
+    const uint32_t lg = ilog2(n0);
     std::vector<uint64_t> s(lg / 3, 8);
     if (lg % 3) s.push_back(1ULL << (lg % 3));
     return s;
 }
 
-std::vector<uint8_t> deep_fri_prove_dev(Ctx *c, const uint64_t *d_f0, uint64_t n0, const FriParams &p,
-                                        std::vector<uint64_t> *raw_out) {
+std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d_f0, uint32_t K, uint64_t n0,
+                                                       const FriParams &p, const std::vector<std::array<uint8_t, 32>> *pis,
+                                                       std::vector<std::vector<uint64_t>> *raw_out) {
     const size_t L = p.schedule.size();
     if (!p.stir || L == 0) throw Error(E_UNSUPPORTED, "only the STIR low-degree test is implemented");
+    SG_REQUIRE(K >= 1 && K <= 65535, "deep_fri_prove: batch size out of range");
     SG_REQUIRE(is_pow2(n0) && n0 >= 2 && n0 <= (1ULL << 24), "deep_fri_prove: n0 must be a power of two in [2, 2^24]");
+    SG_REQUIRE(!pis || pis->size() == K, "deep_fri_prove: one public-inputs hash per proof");
     {
         uint64_t n = n0;
         for (uint64_t m : p.schedule) {
@@ -312,105 +336,115 @@ std::vector<uint8_t> deep_fri_prove_dev(Ctx *c, const uint64_t *d_f0, uint64_t n
     }
     const bool use_cc = p.coeff_commit_final;
 
-    Transcript tr("FRI/FS");
-    if (p.has_pi) {
-        tr.absorb("CAIRO-PUBLIC-INPUTS-V1");
-        tr.absorb(p.pi, 32);
+    std::vector<Transcript> tr;
+    tr.reserve(K);
+    for (uint32_t k = 0; k < K; k++) {
+        tr.emplace_back("FRI/FS");
+        Transcript &t = tr.back();
+        if (pis || p.has_pi) {
+            t.absorb("CAIRO-PUBLIC-INPUTS-V1");
+            t.absorb(pis ? (*pis)[k].data() : p.pi, 32);
+        }
+        t.absorb("DEEP-FRI-STATEMENT-V2");
+        t.absorb_field(n0);
+        t.absorb_field(L);
+        for (uint64_t m : p.schedule) t.absorb_field(m);
+        t.absorb_field(canon(p.seed_z));
+        t.absorb_field(6);
+        t.absorb_field(p.coeff_commit_final ? 1 : 0);
+        t.absorb_field(p.stir ? 1 : 0);
     }
-    tr.absorb("DEEP-FRI-STATEMENT-V2");
-    tr.absorb_field(n0);
-    tr.absorb_field(L);
-    for (uint64_t m : p.schedule) tr.absorb_field(m);
-    tr.absorb_field(canon(p.seed_z));
-    tr.absorb_field(6);
-    tr.absorb_field(p.coeff_commit_final ? 1 : 0);
-    tr.absorb_field(p.stir ? 1 : 0);
 
-    // coset-packed f0 tree: leaf b = fibre f0[b + j n0/m0] (fri.rs:1420-1431)
+    // coset-packed f0 trees: leaf b = fibre f0[b + j n0/m0] (fri.rs:1420-1431)
     const uint64_t m0 = p.schedule[0], nn0 = n0 / m0;
     DevTree t0;
     {
-        uint32_t a = pick_arity(nn0, 16);
-        LeafSrc src{d_f0, 1, nn0, (uint32_t)m0, (uint32_t)m0};
-        tree_build(c, t0, src, nn0, a, merkle_depth(nn0, a), 0xFF);
+        const uint32_t a = pick_arity(nn0, 16);
+        LeafSrc src{d_f0, 1, nn0, (uint32_t)m0, (uint32_t)m0, nullptr, n0};
+        tree_build(c, t0, src, nn0, a, merkle_depth(nn0, a), 0xFF, K);
     }
-    tr.absorb(t0.root, 32);
-    fp6 z;
-    tr.challenge_ext("z_fp3", z.c);
+    std::vector<fp6> z(K), alpha(K);
+    parallel_for(K, [&](size_t k) {
+        tr[k].absorb(t0.roots[k].data(), 32);
+        tr[k].challenge_ext("z_fp3", z[k].c);
+    });
 
     // layers
-    std::vector<std::vector<uint64_t>> coset_evals(L);
-    std::vector<fp6> fz(L);
-    std::vector<std::array<uint8_t, 32>> roots(L);
-    std::vector<uint64_t> tuples_h;
-    uint8_t coeff_root[32] = {0};
+    std::vector<std::vector<uint64_t>> coset_evals(L);  // [l][k * m * 6]
+    std::vector<std::array<uint8_t, 32>> roots((size_t)L * K);
+    std::vector<uint64_t> tuples_planes;  // final tuples, [k][6m][nn]
+    std::vector<std::array<uint8_t, 32>> coeff_roots(K);
+    uint64_t n_final_tuples = 0, m_final = 0;
     DevTree t1;
     bool have_t1 = false;
+    Scoped d_alpha(c, 48ULL * K);
 
     uint64_t cur = n0;
-    const uint64_t *f = d_f0;  // layer values, planes
+    const uint64_t *f = d_f0;  // layer planes of all proofs
     int ncomp = 1;
-    uint64_t *owned = nullptr;      // current layer planes we allocated
-    uint64_t *layer1 = nullptr;     // f_1 planes, kept for the queries
+    uint64_t *owned = nullptr;  // current layer planes we allocated
+    uint64_t *layer1 = nullptr; // f_1 planes, kept for the queries
     for (size_t ell = 0; ell < L; ell++) {
         const uint64_t m = p.schedule[ell], nn = cur / m;
         const bool final_cc = use_cc && ell == L - 1;
-        fp6 alpha = f6_zero();
-        if (!final_cc) tr.challenge_ext("alpha", alpha.c);
+        if (!final_cc) parallel_for(K, [&](size_t k) { tr[k].challenge_ext("alpha", alpha[k].c); });
         // OOD coset evaluations
+        coset_evals[ell].resize((size_t)K * m * 6);
         {
-            Scoped coef(c, 8 * ncomp * cur), scr(c, 8 * ncomp * cur);
-            ntt_columns(c, f, coef.as<uint64_t>(), scr.as<uint64_t>(), ilog2(cur), ncomp, cur, cur, true, nullptr);
-            coset_evals[ell].resize(6 * m);
-            uint64_t zeta = pow(root_of_unity(cur), nn);
-            ood(c, coef.as<uint64_t>(), cur, ncomp, m, z, zeta, coset_evals[ell].data());
+            Scoped coef(c, 8ULL * K * ncomp * cur), scr(c, 8ULL * K * ncomp * cur);
+            ntt_columns(c, f, coef.as<uint64_t>(), scr.as<uint64_t>(), ilog2(cur), (uint64_t)K * ncomp, cur, cur, true,
+                        nullptr);
+            const uint64_t zeta = pow(root_of_unity(cur), nn);
+            ood(c, coef.as<uint64_t>(), cur, ncomp, m, K, z, zeta, coset_evals[ell].data());
         }
-        memcpy(fz[ell].c, coset_evals[ell].data(), 48);
-        z = f6_pow(z, m);
-        // layer commitment: leaf = 6 components (fri.rs:1364-1366)
+        for (uint32_t k = 0; k < K; k++) z[k] = f6_pow(z[k], m);
+        // layer commitment: leaf = 6 components (fri.rs:1364-1366); a base-field
+        // layer (f0) hashes components 1..5 as zeros
         DevTree tl;
         {
-            uint32_t a = pick_arity(cur, m);
-            // a base-field layer (f0) hashes components 1..5 as zeros
-            LeafSrc src{f, 1, cur, 6, (uint32_t)ncomp};
-            tree_build(c, tl, src, cur, a, merkle_depth(cur, a), ell);
+            const uint32_t a = pick_arity(cur, m);
+            LeafSrc src{f, 1, cur, 6, (uint32_t)ncomp, nullptr, (uint64_t)ncomp * cur};
+            tree_build(c, tl, src, cur, a, merkle_depth(cur, a), ell, K);
         }
-        memcpy(roots[ell].data(), tl.root, 32);
-        for (uint64_t j = 0; j < m; j++) {
-            for (int q = 0; q < 6; q++) tr.absorb_field(coset_evals[ell][6 * j + q]);
-        }
-        tr.absorb(tl.root, 32);
+        parallel_for(K, [&](size_t k) {
+            roots[ell * K + k] = tl.roots[k];
+            for (uint64_t j = 0; j < 6 * m; j++) tr[k].absorb_field(coset_evals[ell][k * m * 6 + j]);
+            tr[k].absorb(tl.roots[k].data(), 32);
+        });
         if (ell == 1) {
             t1 = tl;
             have_t1 = true;
         } else {
             tree_free(c, tl);
         }
-        uint64_t *next = c->alloc_n<uint64_t>(6 * nn);
+        uint64_t *next = c->alloc_n<uint64_t>(6ULL * nn * K);
         if (final_cc) {
-            Scoped tup(c, 8 * 6 * m * nn);
-            fold(c, f, cur, ncomp, m, f6_zero(), next, tup.as<uint64_t>());
+            Scoped tup(c, 8ULL * 6 * m * nn * K);
+            std::vector<fp6> zero(K, f6_zero());
+            upload_f6(c, zero, d_alpha.as<uint64_t>());
+            fold(c, f, cur, ncomp, m, K, d_alpha.as<uint64_t>(), next, tup.as<uint64_t>());
             const uint32_t a = pick_arity(nn, 16);
             DevTree tc;
-            LeafSrc src{tup.as<uint64_t>(), 1, nn, (uint32_t)(6 * m), (uint32_t)(6 * m)};
-            tree_build(c, tc, src, nn, a, merkle_depth(nn, a), 0xFE);
-            memcpy(coeff_root, tc.root, 32);
-            tree_free(c, tc);
-            std::vector<uint64_t> planes(6 * m * nn);
-            SG_HIP(hipMemcpyAsync(planes.data(), tup.p, 8 * planes.size(), hipMemcpyDeviceToHost, c->stream));
+            LeafSrc src{tup.as<uint64_t>(), 1, nn, (uint32_t)(6 * m), (uint32_t)(6 * m), nullptr, 6 * m * nn};
+            tree_build(c, tc, src, nn, a, merkle_depth(nn, a), 0xFE, K);
+            tuples_planes.resize((size_t)6 * m * nn * K);
+            SG_HIP(hipMemcpyAsync(tuples_planes.data(), tup.p, 8 * tuples_planes.size(), hipMemcpyDeviceToHost,
+                                  c->stream));
             SG_HIP(hipStreamSynchronize(c->stream));
-            tuples_h.resize(6 * m * nn);  // [b][k][q]
-            for (uint64_t b = 0; b < nn; b++)
-                for (uint64_t kq = 0; kq < 6 * m; kq++) tuples_h[b * 6 * m + kq] = planes[kq * nn + b];
-            tr.absorb(coeff_root, 32);
-            tr.challenge_ext("alpha", alpha.c);
-            fp6 beta;
-            tr.challenge_ext("beta_deg", beta.c);
-            // fold of the final layer with the post-commit alpha
-            fold(c, f, cur, ncomp, m, alpha, next, nullptr);
-        } else {
-            fold(c, f, cur, ncomp, m, alpha, next, nullptr);
+            n_final_tuples = nn;
+            m_final = m;
+            parallel_for(K, [&](size_t k) {
+                coeff_roots[k] = tc.roots[k];
+                tr[k].absorb(tc.roots[k].data(), 32);
+                tr[k].challenge_ext("alpha", alpha[k].c);
+                fp6 beta;
+                tr[k].challenge_ext("beta_deg", beta.c);
+            });
+            tree_free(c, tc);
         }
+        // fold with this layer's alpha (the final layer's is drawn after its coefficient commitment)
+        upload_f6(c, alpha, d_alpha.as<uint64_t>());
+        fold(c, f, cur, ncomp, m, K, d_alpha.as<uint64_t>(), next, nullptr);
         if (ell == 0) {
             layer1 = next;
         } else if (owned && owned != layer1) {
@@ -424,127 +458,137 @@ std::vector<uint8_t> deep_fri_prove_dev(Ctx *c, const uint64_t *d_f0, uint64_t n
 
     // final polynomial: first d_final coefficients of the last layer (fri.rs:1919-1937)
     const uint64_t nf = cur;
-    std::vector<uint64_t> fin(6 * nf);
+    std::vector<uint64_t> fin(6 * nf * K);
     {
-        Scoped coef(c, 8 * 6 * nf), scr(c, 8 * 6 * nf);
+        Scoped coef(c, 8 * 6 * nf * K), scr(c, 8 * 6 * nf * K);
         const uint64_t *src = f;
         if (nf > 1) {
-            ntt_columns(c, f, coef.as<uint64_t>(), scr.as<uint64_t>(), ilog2(nf), 6, nf, nf, true, nullptr);
+            ntt_columns(c, f, coef.as<uint64_t>(), scr.as<uint64_t>(), ilog2(nf), 6ULL * K, nf, nf, true, nullptr);
             src = coef.as<uint64_t>();
         }
-        SG_HIP(hipMemcpyAsync(fin.data(), src, 8 * 6 * nf, hipMemcpyDeviceToHost, c->stream));
+        SG_HIP(hipMemcpyAsync(fin.data(), src, 8 * fin.size(), hipMemcpyDeviceToHost, c->stream));
         SG_HIP(hipStreamSynchronize(c->stream));
     }
     const uint64_t dfin = p.d_final < nf ? p.d_final : nf;
-    std::vector<fp6> final_coeffs(dfin);
-    for (uint64_t k = 0; k < dfin; k++)
-        for (int q = 0; q < 6; q++) final_coeffs[k].c[q] = fin[q * nf + k];
-    for (auto &fc : final_coeffs)
-        for (int q = 0; q < 6; q++) tr.absorb_field(fc.c[q]);
-    const uint64_t query_seed = tr.challenge_field("query_seed");
-
-    // STIR proximity queries (fri.rs:1813-1893)
+    std::vector<std::vector<fp6>> final_coeffs(K, std::vector<fp6>(dfin));
+    std::vector<uint64_t> query_seed(K);
     uint64_t npow2 = 1;
     while (npow2 < n0) npow2 <<= 1;
-    std::vector<uint64_t> raw(p.s0), base(p.s0);
-    for (uint64_t q = 0; q < p.s0; q++) {
-        uint64_t fl[3] = {query_seed, 0, canon(q)};
-        uint64_t seed = hash_fields_tagged("FRI/index", fl, 3);
-        raw[q] = (stdrng_first_u64(seed) & (npow2 - 1)) % n0;
-        base[q] = raw[q] % nn0;
-    }
+    std::vector<std::vector<uint64_t>> raw(K, std::vector<uint64_t>(p.s0)), base(K, std::vector<uint64_t>(p.s0));
+    parallel_for(K, [&](size_t k) {
+        for (uint64_t i = 0; i < dfin; i++)
+            for (int q = 0; q < 6; q++) final_coeffs[k][i].c[q] = fin[(k * 6 + q) * nf + i];
+        for (auto &fc : final_coeffs[k])
+            for (int q = 0; q < 6; q++) tr[k].absorb_field(fc.c[q]);
+        query_seed[k] = tr[k].challenge_field("query_seed");
+        // STIR proximity query indices (fri.rs:1813-1893)
+        for (uint64_t q = 0; q < p.s0; q++) {
+            const uint64_t fl[3] = {query_seed[k], 0, canon(q)};
+            const uint64_t seed = hash_fields_tagged("FRI/index", fl, 3);
+            raw[k][q] = (stdrng_first_u64(seed) & (npow2 - 1)) % n0;
+            base[k][q] = raw[k][q] % nn0;
+        }
+    });
     if (raw_out) *raw_out = raw;
-    // gathers: fibre values of f0 and f_1[base]
-    std::vector<uint64_t> gidx;
-    for (uint64_t q = 0; q < p.s0; q++)
-        for (uint64_t j = 0; j < m0; j++) gidx.push_back(base[q] + j * nn0);
     const bool l2 = L >= 2;
-    std::vector<uint64_t> fib(gidx.size()), f1v(6 * p.s0);
-    {
-        Scoped di(c, 8 * (gidx.size() + 1)), dv(c, 8 * (gidx.size() + 1));
-        SG_HIP(hipMemcpyAsync(di.p, gidx.data(), 8 * gidx.size(), hipMemcpyHostToDevice, c->stream));
-        gather_u64(c, d_f0, di.as<uint64_t>(), gidx.size(), dv.as<uint64_t>());
-        SG_HIP(hipMemcpyAsync(fib.data(), dv.p, 8 * gidx.size(), hipMemcpyDeviceToHost, c->stream));
-        SG_HIP(hipStreamSynchronize(c->stream));
-    }
-    if (l2) {
-        const uint64_t n1 = n0 / m0;
-        std::vector<uint64_t> gi;
-        for (int q6 = 0; q6 < 6; q6++)
-            for (uint64_t q = 0; q < p.s0; q++) gi.push_back(q6 * n1 + base[q]);
-        Scoped di(c, 8 * gi.size()), dv(c, 8 * gi.size());
-        SG_HIP(hipMemcpyAsync(di.p, gi.data(), 8 * gi.size(), hipMemcpyHostToDevice, c->stream));
-        gather_u64(c, layer1, di.as<uint64_t>(), gi.size(), dv.as<uint64_t>());
-        std::vector<uint64_t> tmp(gi.size());
-        SG_HIP(hipMemcpyAsync(tmp.data(), dv.p, 8 * gi.size(), hipMemcpyDeviceToHost, c->stream));
-        SG_HIP(hipStreamSynchronize(c->stream));
+    const uint64_t n1 = n0 / m0;
+    // gathers: fibre values of f0 and, per component, f_1[base]
+    std::vector<uint64_t> gidx;
+    for (uint32_t k = 0; k < K; k++)
         for (uint64_t q = 0; q < p.s0; q++)
-            for (int q6 = 0; q6 < 6; q6++) f1v[6 * q + q6] = tmp[q6 * p.s0 + q];
+            for (uint64_t j = 0; j < m0; j++) gidx.push_back(k * n0 + base[k][q] + j * nn0);
+    const size_t nfib = gidx.size();
+    if (l2)
+        for (uint32_t k = 0; k < K; k++)
+            for (uint64_t q = 0; q < p.s0; q++)
+                for (int q6 = 0; q6 < 6; q6++) gidx.push_back(k * 6 * n1 + q6 * n1 + base[k][q]);
+    std::vector<uint64_t> gv(gidx.size());
+    if (!gidx.empty()) {
+        Scoped di(c, 8 * gidx.size()), dv(c, 8 * gidx.size());
+        SG_HIP(hipMemcpyAsync(di.p, gidx.data(), 8 * gidx.size(), hipMemcpyHostToDevice, c->stream));
+        gather_u64(c, d_f0, di.as<uint64_t>(), nfib, dv.as<uint64_t>());
+        if (l2) gather_u64(c, layer1, di.as<uint64_t>() + nfib, gidx.size() - nfib, dv.as<uint64_t>() + nfib);
+        SG_HIP(hipMemcpyAsync(gv.data(), dv.p, 8 * gidx.size(), hipMemcpyDeviceToHost, c->stream));
+        SG_HIP(hipStreamSynchronize(c->stream));
     }
-    std::vector<std::vector<uint8_t>> op0, op1;
-    tree_open_batch(c, t0, base, op0);
-    if (l2) tree_open_batch(c, t1, base, op1);
+    std::vector<std::vector<std::vector<uint8_t>>> op0, op1;
+    tree_open_multi(c, t0, base, op0);
+    if (l2) tree_open_multi(c, t1, base, op1);
 
     // serialize DeepFriProof<SexticExt> (fri.rs:1286-1300, ark-serialize compressed)
-    Bytes b;
-    b.put(t0.root, 32);
-    b.u64(L);
-    for (auto &r : roots) b.put(r.data(), 32);
-    b.u64(0);  // layer_proofs.layers
-    b.u64(0);  // f0_openings
-    b.u64(0);  // queries
-    b.u64(L);
-    for (auto &v : fz) b.f6(v.c);
-    b.u64(final_coeffs.size());
-    for (auto &v : final_coeffs) b.f6(v.c);
-    b.u64(n0);
-    b.u64(root_of_unity(n0));
-    if (use_cc) {
-        const uint64_t m = p.schedule[L - 1];
-        const uint64_t nt = tuples_h.size() / (6 * m);
-        b.u8(1);
-        b.u64(nt);
-        for (uint64_t t = 0; t < nt; t++) {
-            b.u64(m);
-            for (uint64_t k = 0; k < m; k++) b.f6(&tuples_h[(t * m + k) * 6]);
-        }
-        b.u8(1);
-        b.put(coeff_root, 32);
-    } else {
-        b.u8(0);
-        b.u8(0);
-    }
-    b.u8(1);
-    b.u64(L);
-    for (size_t l = 0; l < L; l++) {
-        b.u64(p.schedule[l]);
-        for (uint64_t j = 0; j < p.schedule[l]; j++) b.f6(&coset_evals[l][6 * j]);
-    }
-    b.u8(1);
-    b.u64(p.s0);
-    for (uint64_t q = 0; q < p.s0; q++) {
-        b.u64(base[q]);
-        b.u64(raw[q]);
-        b.u64(m0);
-        for (uint64_t j = 0; j < m0; j++) b.u64(base[q] + j * nn0);
-        b.u64(m0);
-        for (uint64_t j = 0; j < m0; j++) b.u64(fib[q * m0 + j]);
-        b.put(op0[q].data(), op0[q].size());
-        if (l2) {
-            b.f6(&f1v[6 * q]);
+    std::vector<std::vector<uint8_t>> proofs(K);
+    const uint64_t omega0 = root_of_unity(n0);
+    parallel_for(K, [&](size_t k) {
+        Bytes b;
+        b.put(t0.roots[k].data(), 32);
+        b.u64(L);
+        for (size_t l = 0; l < L; l++) b.put(roots[l * K + k].data(), 32);
+        b.u64(0);  // layer_proofs.layers
+        b.u64(0);  // f0_openings
+        b.u64(0);  // queries
+        b.u64(L);
+        for (size_t l = 0; l < L; l++) b.f6(&coset_evals[l][(size_t)k * p.schedule[l] * 6]);  // fz = first coset eval
+        b.u64(dfin);
+        for (auto &v : final_coeffs[k]) b.f6(v.c);
+        b.u64(n0);
+        b.u64(omega0);
+        if (use_cc) {
+            const uint64_t nt = n_final_tuples, m = m_final;
+            const uint64_t *tp = &tuples_planes[(size_t)k * 6 * m * nt];
             b.u8(1);
-            b.put(op1[q].data(), op1[q].size());
+            b.u64(nt);
+            for (uint64_t t = 0; t < nt; t++) {
+                b.u64(m);
+                for (uint64_t kq = 0; kq < 6 * m; kq++) b.u64(tp[kq * nt + t]);
+            }
+            b.u8(1);
+            b.put(coeff_roots[k].data(), 32);
         } else {
-            uint64_t zero6[6] = {0, 0, 0, 0, 0, 0};
-            b.f6(zero6);
+            b.u8(0);
             b.u8(0);
         }
-    }
+        b.u8(1);
+        b.u64(L);
+        for (size_t l = 0; l < L; l++) {
+            const uint64_t m = p.schedule[l];
+            b.u64(m);
+            for (uint64_t j = 0; j < m; j++) b.f6(&coset_evals[l][((size_t)k * m + j) * 6]);
+        }
+        b.u8(1);
+        b.u64(p.s0);
+        for (uint64_t q = 0; q < p.s0; q++) {
+            b.u64(base[k][q]);
+            b.u64(raw[k][q]);
+            b.u64(m0);
+            for (uint64_t j = 0; j < m0; j++) b.u64(base[k][q] + j * nn0);
+            b.u64(m0);
+            for (uint64_t j = 0; j < m0; j++) b.u64(gv[((size_t)k * p.s0 + q) * m0 + j]);
+            b.put(op0[k][q].data(), op0[k][q].size());
+            if (l2) {
+                b.f6(&gv[nfib + ((size_t)k * p.s0 + q) * 6]);
+                b.u8(1);
+                b.put(op1[k][q].data(), op1[k][q].size());
+            } else {
+                const uint64_t zero6[6] = {0, 0, 0, 0, 0, 0};
+                b.f6(zero6);
+                b.u8(0);
+            }
+        }
+        proofs[k] = std::move(b.v);
+    });
     tree_free(c, t0);
     if (have_t1) tree_free(c, t1);
     if (owned && owned != layer1) c->release(owned);
     if (layer1) c->release(layer1);
-    return std::move(b.v);
+    return proofs;
+}
+
+std::vector<uint8_t> deep_fri_prove_dev(Ctx *c, const uint64_t *d_f0, uint64_t n0, const FriParams &p,
+                                        std::vector<uint64_t> *raw_out) {
+    std::vector<std::vector<uint64_t>> raw;
+    std::vector<std::vector<uint8_t>> v = deep_fri_prove_batch(c, d_f0, 1, n0, p, nullptr, raw_out ? &raw : nullptr);
+    if (raw_out) *raw_out = raw[0];
+    return std::move(v[0]);
 }
 
 }  // namespace sg

@@ -278,11 +278,13 @@ __device__ __forceinline__ void sha3_lanes(uint32_t total, LaneFn lane, uint64_t
     dig[3] = st[3];
 }
 
+// grid.y = tree of the batch; leaf i of tree k is written at out[k * n + i]
 __global__ __launch_bounds__(KT) void merkle_leaves_k(LeafSrc src, uint64_t n, uint32_t arity, uint64_t label,
                                                       uint64_t *out) {
-    uint64_t i = (uint64_t)blockIdx.x * KT + threadIdx.x;
+    const uint64_t i = (uint64_t)blockIdx.x * KT + threadIdx.x;
     if (i >= n) return;
-    const uint64_t *base = src.vals + i * src.leaf_stride;
+    const uint64_t k = blockIdx.y;
+    const uint64_t *base = src.vals + k * src.proof_stride + i * src.leaf_stride;
     const uint64_t cs = src.cs;
     // sum_c (c+1) v_c = w * T_w - sum_{k<w} T_k with T_k = sum_{c<k} v_c
     uint64_t T = 0, U = 0;
@@ -300,18 +302,22 @@ __global__ __launch_bounds__(KT) void merkle_leaves_k(LeafSrc src, uint64_t n, u
     };
     uint64_t d[4];
     sha3_lanes(4 + src.w, lane, d);
-    if (combo) src.combo_out[i] = sub(mul(T, src.w), U);
-    uint64_t *o = out + 4 * i;
+    if (combo) src.combo_out[k * n + i] = sub(mul(T, src.w), U);
+    uint64_t *o = out + 4 * (k * n + i);
     o[0] = d[0];
     o[1] = d[1];
     o[2] = d[2];
     o[3] = d[3];
 }
 
+// grid.y = tree of the batch; child/out point at the level arrays of all trees
 __global__ __launch_bounds__(KT) void merkle_nodes_k(const uint64_t *child, uint64_t n_child, uint64_t n_parent,
                                                      uint32_t arity, uint32_t level, uint64_t label, uint64_t *out) {
-    uint64_t i = (uint64_t)blockIdx.x * KT + threadIdx.x;
+    const uint64_t i = (uint64_t)blockIdx.x * KT + threadIdx.x;
     if (i >= n_parent) return;
+    const uint64_t k = blockIdx.y;
+    child += 4 * k * n_child;
+    out += 4 * k * n_parent;
     auto lane = [&](uint32_t t) -> uint64_t {
         if (t >= 4) {
             uint32_t j = (t - 4) >> 2, q = (t - 4) & 3;
@@ -337,7 +343,174 @@ __global__ void gather_k(const uint64_t *src, const uint64_t *idx, uint64_t n, u
     out[r] = src[idx[q] * width + l];
 }
 
+// ---- DNS record commitments (crates/swarm-dns/src/dns.rs:43-118) ----
+// h1 = SHA3("DNS-LEAF-V1" || salt || canonical(record)) from host-padded
+// message lanes (nblk[r] blocks of 17 lanes at msg + 17 * blk_off[r]);
+// h2 = SHA3("DNS-LEAF-DOUBLE-V1" || salt || h1): 66 bytes, a fixed 34-byte
+// prefix (4 full lanes + 2 bytes) then h1 at byte offset 34.
+__global__ __launch_bounds__(KT) void dns_leaf_k(const uint64_t *msg, const uint64_t *blk_off, const uint32_t *nblk,
+                                                 uint64_t n, const uint64_t *prefix2, uint64_t *out) {
+    const uint64_t r = (uint64_t)blockIdx.x * KT + threadIdx.x;
+    if (r >= n) return;
+    uint64_t st[25];
+#pragma unroll
+    for (int j = 0; j < 25; j++) st[j] = 0;
+    const uint64_t *m = msg + 17 * blk_off[r];
+    for (uint32_t b = 0; b < nblk[r]; b++) {
+#pragma unroll
+        for (int j = 0; j < 17; j++) st[j] ^= m[17 * b + j];
+        keccak_f_dev(st);
+    }
+    const uint64_t h0 = st[0], h1 = st[1], h2 = st[2], h3 = st[3];
+#pragma unroll
+    for (int j = 0; j < 25; j++) st[j] = 0;
+    st[0] = prefix2[0];
+    st[1] = prefix2[1];
+    st[2] = prefix2[2];
+    st[3] = prefix2[3];
+    st[4] = prefix2[4] | (h0 << 16);  // prefix2[4]: the last 2 prefix bytes
+    st[5] = (h0 >> 48) | (h1 << 16);
+    st[6] = (h1 >> 48) | (h2 << 16);
+    st[7] = (h2 >> 48) | (h3 << 16);
+    st[8] = (h3 >> 48) | (0x06ULL << 16);
+    st[16] = 0x8000000000000000ULL;
+    keccak_f_dev(st);
+    uint64_t *o = out + 4 * r;
+    o[0] = st[0];
+    o[1] = st[1];
+    o[2] = st[2];
+    o[3] = st[3];
+}
+
+// parent i = SHA3("DNS-NODE-V1" || left || right), right = left when the level is odd.
+// 75-byte message: the 64 child bytes start at byte 11. grid.y = tree of a batch.
+__global__ __launch_bounds__(KT) void dns_node_k(const uint64_t *child, uint64_t n_child, uint64_t n_parent,
+                                                 uint64_t child_stride, uint64_t parent_stride, uint64_t *out) {
+    const uint64_t i = (uint64_t)blockIdx.x * KT + threadIdx.x;
+    if (i >= n_parent) return;
+    child += blockIdx.y * child_stride;
+    out += blockIdx.y * parent_stride;
+    const uint64_t li = 2 * i, ri = (2 * i + 1 < n_child) ? 2 * i + 1 : 2 * i;
+    uint64_t w[8];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        w[q] = child[4 * li + q];
+        w[4 + q] = child[4 * ri + q];
+    }
+    // "DNS-NODE-V1" = 44 4E 53 2D 4E 4F 44 45 | 2D 56 31
+    const uint64_t tag0 = 0x45444F4E2D534E44ULL, tag1 = 0x31562DULL;
+    uint64_t st[25];
+#pragma unroll
+    for (int j = 0; j < 25; j++) st[j] = 0;
+    st[0] = tag0;
+    st[1] = tag1 | (w[0] << 24);
+#pragma unroll
+    for (int j = 2; j <= 8; j++) st[j] = (w[j - 2] >> 40) | (w[j - 1] << 24);
+    st[9] = (w[7] >> 40) | (0x06ULL << 24);
+    st[16] = 0x8000000000000000ULL;
+    keccak_f_dev(st);
+    uint64_t *o = out + 4 * i;
+    o[0] = st[0];
+    o[1] = st[1];
+    o[2] = st[2];
+    o[3] = st[3];
+}
+
 }  // namespace
+
+// Leaf hashes (h2) and Merkle roots of K DNS shards on the device.
+// recs_per_shard[k] records each; the host lays out padded message lanes.
+void dns_commit_batch(Ctx *c, const std::vector<std::vector<std::vector<uint8_t>>> &canon, const uint8_t salt[16],
+                      std::vector<std::vector<uint8_t>> &leaf_hashes, std::vector<std::array<uint8_t, 32>> &roots) {
+    const size_t K = canon.size();
+    std::vector<uint64_t> blk_off;
+    std::vector<uint32_t> nblk;
+    std::vector<size_t> first(K + 1, 0);
+    uint64_t total_blocks = 0;
+    for (size_t k = 0; k < K; k++) {
+        first[k + 1] = first[k] + canon[k].size();
+        for (auto &cb : canon[k]) {
+            const size_t len = 11 + 16 + cb.size();
+            const uint32_t nb = (uint32_t)(len / 136 + 1);
+            blk_off.push_back(total_blocks);
+            nblk.push_back(nb);
+            total_blocks += nb;
+        }
+    }
+    const size_t nrec = first[K];
+    SG_REQUIRE(nrec >= 1, "dns commit: no records");
+    std::vector<uint64_t> msg(17 * total_blocks, 0);
+    parallel_for(K, [&](size_t k) {
+        for (size_t i = 0; i < canon[k].size(); i++) {
+            const size_t r = first[k] + i;
+            uint8_t *p = (uint8_t *)&msg[17 * blk_off[r]];
+            size_t o = 0;
+            memcpy(p + o, "DNS-LEAF-V1", 11);
+            o += 11;
+            memcpy(p + o, salt, 16);
+            o += 16;
+            memcpy(p + o, canon[k][i].data(), canon[k][i].size());
+            o += canon[k][i].size();
+            p[o] ^= 0x06;
+            p[136 * nblk[r] - 1] ^= 0x80;
+        }
+    });
+    uint64_t prefix2[5] = {0, 0, 0, 0, 0};
+    {
+        uint8_t pb[40] = {0};
+        memcpy(pb, "DNS-LEAF-DOUBLE-V1", 18);
+        memcpy(pb + 18, salt, 16);
+        memcpy(prefix2, pb, 40);
+        prefix2[4] &= 0xFFFF;  // bytes 32..33
+    }
+    // level sizes of one shard's tree; all trees of a batch share nothing but the launch
+    Scoped d_msg(c, 8 * msg.size()), d_off(c, 8 * nrec), d_nb(c, 4 * nrec), d_pre(c, 40), d_leaf(c, 32 * nrec);
+    SG_HIP(hipMemcpyAsync(d_msg.p, msg.data(), 8 * msg.size(), hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipMemcpyAsync(d_off.p, blk_off.data(), 8 * nrec, hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipMemcpyAsync(d_nb.p, nblk.data(), 4 * nrec, hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipMemcpyAsync(d_pre.p, prefix2, 40, hipMemcpyHostToDevice, c->stream));
+    {
+        ProfScope ps(c, "dns_leaf", 8.0 * msg.size() + 32.0 * nrec);
+        hipLaunchKernelGGL(dns_leaf_k, dim3((uint32_t)((nrec + KT - 1) / KT)), dim3(KT), 0, c->stream,
+                           d_msg.as<uint64_t>(), d_off.as<uint64_t>(), d_nb.as<uint32_t>(), (uint64_t)nrec,
+                           d_pre.as<uint64_t>(), d_leaf.as<uint64_t>());
+        SG_HIP(hipGetLastError());
+    }
+    leaf_hashes.assign(K, {});
+    std::vector<uint8_t> all(32 * nrec);
+    SG_HIP(hipMemcpyAsync(all.data(), d_leaf.p, 32 * nrec, hipMemcpyDeviceToHost, c->stream));
+    // trees: shards of equal size go up level by level in one launch per level
+    roots.assign(K, {});
+    std::map<size_t, std::vector<size_t>> by_size;
+    for (size_t k = 0; k < K; k++) by_size[canon[k].size()].push_back(k);
+    for (auto &g : by_size) {
+        const size_t n = g.first, G = g.second.size();
+        std::vector<uint64_t> idx;
+        for (size_t k : g.second)
+            for (size_t i = 0; i < n; i++) idx.push_back(first[k] + i);
+        Scoped d_idx(c, 8 * idx.size()), a(c, 32 * idx.size()), b(c, 32 * idx.size());
+        SG_HIP(hipMemcpyAsync(d_idx.p, idx.data(), 8 * idx.size(), hipMemcpyHostToDevice, c->stream));
+        gather_u64_w(c, d_leaf.as<uint64_t>(), d_idx.as<uint64_t>(), idx.size(), 4, a.as<uint64_t>());
+        uint64_t *cur = a.as<uint64_t>(), *nxt = b.as<uint64_t>();
+        size_t len = n;
+        while (len > 1) {
+            const size_t np = (len + 1) / 2;
+            ProfScope ps(c, "dns_node", 32.0 * (len + np) * G);
+            hipLaunchKernelGGL(dns_node_k, dim3((uint32_t)((np + KT - 1) / KT), (uint32_t)G), dim3(KT), 0, c->stream,
+                               cur, (uint64_t)len, (uint64_t)np, (uint64_t)4 * n, (uint64_t)4 * n, nxt);
+            SG_HIP(hipGetLastError());
+            std::swap(cur, nxt);
+            len = np;
+        }
+        std::vector<uint64_t> rt(4 * n * G);
+        SG_HIP(hipMemcpyAsync(rt.data(), cur, 32 * n * G, hipMemcpyDeviceToHost, c->stream));
+        SG_HIP(hipStreamSynchronize(c->stream));
+        for (size_t j = 0; j < G; j++) memcpy(roots[g.second[j]].data(), &rt[4 * n * j], 32);
+    }
+    SG_HIP(hipStreamSynchronize(c->stream));
+    for (size_t k = 0; k < K; k++)
+        leaf_hashes[k].assign(all.begin() + 32 * first[k], all.begin() + 32 * first[k + 1]);
+}
 
 uint32_t merkle_depth(uint64_t leaves, uint32_t arity) {
     uint32_t d = 1;
@@ -357,44 +530,50 @@ uint32_t pick_arity(uint64_t n, uint64_t m) {
 }
 
 void tree_build(Ctx *c, DevTree &t, const LeafSrc &src, uint64_t n_leaves, uint32_t arity, uint32_t depth,
-                uint64_t label) {
+                uint64_t label, uint32_t batch) {
     SG_REQUIRE(n_leaves >= 1 && arity >= 2, "merkle: empty tree or arity < 2");
+    SG_REQUIRE(batch >= 1 && batch <= 65535, "merkle: batch out of range");
+    SG_REQUIRE(n_leaves <= (1ULL << 32) * KT, "merkle: too many leaves");
     t.arity = arity;
     t.label = label;
+    t.batch = batch;
     t.len.clear();
     t.real.clear();
     t.off.clear();
-    // level sizes
+    // level sizes (one tree); level arrays of the whole batch are contiguous
     uint64_t cur = n_leaves, total = 0;
     for (;;) {
-        uint64_t padded = cur > 1 ? (cur + arity - 1) / arity * arity : cur;
+        const uint64_t padded = cur > 1 ? (cur + arity - 1) / arity * arity : cur;
         t.real.push_back(cur);
         t.len.push_back(padded);
         t.off.push_back(total);
-        total += cur;
+        total += cur * batch;
         if (cur <= 1) break;
         SG_REQUIRE(t.real.size() <= depth, "merkle: layer arities exhausted");
         cur = padded / arity;
     }
     t.buf = c->alloc_n<uint64_t>(4 * total);
     {
-        ProfScope ps(c, "merkle_leaves", 8.0 * src.w * n_leaves + 32.0 * n_leaves);
-        hipLaunchKernelGGL(merkle_leaves_k, dim3((uint32_t)((n_leaves + KT - 1) / KT)), dim3(KT), 0, c->stream, src,
-                           n_leaves, arity, label, t.buf);
+        ProfScope ps(c, "merkle_leaves", (8.0 * src.w * n_leaves + 32.0 * n_leaves) * batch);
+        hipLaunchKernelGGL(merkle_leaves_k, dim3((uint32_t)((n_leaves + KT - 1) / KT), batch), dim3(KT), 0, c->stream,
+                           src, n_leaves, arity, label, t.buf);
         SG_HIP(hipGetLastError());
     }
     for (size_t l = 1; l < t.real.size(); l++) {
-        uint64_t np = t.real[l];
-        ProfScope ps(c, "merkle_nodes", 32.0 * t.real[l - 1] + 32.0 * np);
-        hipLaunchKernelGGL(merkle_nodes_k, dim3((uint32_t)((np + KT - 1) / KT)), dim3(KT), 0, c->stream,
+        const uint64_t np = t.real[l];
+        ProfScope ps(c, "merkle_nodes", (32.0 * t.real[l - 1] + 32.0 * np) * batch);
+        hipLaunchKernelGGL(merkle_nodes_k, dim3((uint32_t)((np + KT - 1) / KT), batch), dim3(KT), 0, c->stream,
                            t.buf + 4 * t.off[l - 1], t.real[l - 1], np, arity, (uint32_t)l, label,
                            t.buf + 4 * t.off[l]);
         SG_HIP(hipGetLastError());
     }
-    uint64_t r[4];
-    SG_HIP(hipMemcpyAsync(r, t.buf + 4 * t.off.back(), 32, hipMemcpyDeviceToHost, c->stream));
+    // the top level holds one root per tree
+    std::vector<uint64_t> r(4 * (size_t)batch);
+    SG_HIP(hipMemcpyAsync(r.data(), t.buf + 4 * t.off.back(), 32 * (size_t)batch, hipMemcpyDeviceToHost, c->stream));
     SG_HIP(hipStreamSynchronize(c->stream));
-    memcpy(t.root, r, 32);
+    t.roots.resize(batch);
+    for (uint32_t k = 0; k < batch; k++) memcpy(t.roots[k].data(), &r[4 * k], 32);
+    memcpy(t.root, t.roots[0].data(), 32);
 }
 
 void tree_free(Ctx *c, DevTree &t) {
@@ -403,30 +582,41 @@ void tree_free(Ctx *c, DevTree &t) {
 }
 
 void gather_u64(Ctx *c, const uint64_t *src, const uint64_t *d_idx, uint64_t n, uint64_t *out) {
+    gather_u64_w(c, src, d_idx, n, 1, out);
+}
+
+void gather_u64_w(Ctx *c, const uint64_t *src, const uint64_t *d_idx, uint64_t n, uint32_t width, uint64_t *out) {
     if (!n) return;
-    hipLaunchKernelGGL(gather_k, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, c->stream, src, d_idx, n, 1u, out);
+    hipLaunchKernelGGL(gather_k, dim3((uint32_t)((n * width + 255) / 256)), dim3(256), 0, c->stream, src, d_idx, n,
+                       width, out);
     SG_HIP(hipGetLastError());
 }
 
-void tree_open_batch(Ctx *c, const DevTree &t, const std::vector<uint64_t> &idx,
-                     std::vector<std::vector<uint8_t>> &out) {
+void tree_open_multi(Ctx *c, const DevTree &t, const std::vector<std::vector<uint64_t>> &idx,
+                     std::vector<std::vector<std::vector<uint8_t>>> &out) {
     const uint32_t a = t.arity;
     const size_t nlev = t.real.size();
+    SG_REQUIRE(idx.size() <= t.batch, "merkle open: more index lists than trees");
     // digest indices: per query, the leaf then every level's full group
     std::vector<uint64_t> want;
-    for (uint64_t q : idx) {
-        want.push_back(t.off[0] + q);
-        uint64_t x = q;
-        for (size_t l = 0; l + 1 < nlev; l++) {
-            uint64_t gs = x / a * a;
-            for (uint32_t i = 0; i < a; i++) {
-                uint64_t pos = gs + i;
-                if (pos >= t.real[l]) pos = t.real[l] - 1;
-                want.push_back(t.off[l] + pos);
+    for (size_t k = 0; k < idx.size(); k++) {
+        for (uint64_t q : idx[k]) {
+            SG_REQUIRE(q < t.real[0], "merkle open: leaf index out of range");
+            want.push_back(t.off[0] + k * t.real[0] + q);
+            uint64_t x = q;
+            for (size_t l = 0; l + 1 < nlev; l++) {
+                const uint64_t gs = x / a * a;
+                for (uint32_t i = 0; i < a; i++) {
+                    uint64_t pos = gs + i;
+                    if (pos >= t.real[l]) pos = t.real[l] - 1;
+                    want.push_back(t.off[l] + k * t.real[l] + pos);
+                }
+                x /= a;
             }
-            x /= a;
         }
     }
+    out.assign(idx.size(), {});
+    if (want.empty()) return;
     Scoped d_idx(c, 8 * want.size()), d_out(c, 32 * want.size());
     SG_HIP(hipMemcpyAsync(d_idx.p, want.data(), 8 * want.size(), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(gather_k, dim3((uint32_t)((4 * want.size() + 255) / 256)), dim3(256), 0, c->stream, t.buf,
@@ -435,30 +625,38 @@ void tree_open_batch(Ctx *c, const DevTree &t, const std::vector<uint64_t> &idx,
     std::vector<uint64_t> h(4 * want.size());
     SG_HIP(hipMemcpyAsync(h.data(), d_out.p, 32 * want.size(), hipMemcpyDeviceToHost, c->stream));
     SG_HIP(hipStreamSynchronize(c->stream));
-    out.clear();
-    size_t k = 0;
+    size_t w = 0;
     auto put64 = [](std::vector<uint8_t> &v, uint64_t x) {
         for (int i = 0; i < 8; i++) v.push_back((uint8_t)(x >> (8 * i)));
     };
-    for (uint64_t q : idx) {
-        std::vector<uint8_t> o;
-        const uint8_t *leaf = (const uint8_t *)&h[4 * k++];
-        o.insert(o.end(), leaf, leaf + 32);
-        put64(o, nlev - 1);
-        uint64_t x = q;
-        for (size_t l = 0; l + 1 < nlev; l++) {
-            put64(o, a - 1);
-            uint64_t gs = x / a * a;
-            for (uint32_t i = 0; i < a; i++, k++) {
-                if (gs + i == x) continue;
-                const uint8_t *d = (const uint8_t *)&h[4 * k];
-                o.insert(o.end(), d, d + 32);
+    for (size_t k = 0; k < idx.size(); k++) {
+        for (uint64_t q : idx[k]) {
+            std::vector<uint8_t> o;
+            const uint8_t *leaf = (const uint8_t *)&h[4 * w++];
+            o.insert(o.end(), leaf, leaf + 32);
+            put64(o, nlev - 1);
+            uint64_t x = q;
+            for (size_t l = 0; l + 1 < nlev; l++) {
+                put64(o, a - 1);
+                const uint64_t gs = x / a * a;
+                for (uint32_t i = 0; i < a; i++, w++) {
+                    if (gs + i == x) continue;
+                    const uint8_t *d = (const uint8_t *)&h[4 * w];
+                    o.insert(o.end(), d, d + 32);
+                }
+                x /= a;
             }
-            x /= a;
+            put64(o, q);
+            out[k].push_back(std::move(o));
         }
-        put64(o, q);
-        out.push_back(std::move(o));
     }
+}
+
+void tree_open_batch(Ctx *c, const DevTree &t, const std::vector<uint64_t> &idx,
+                     std::vector<std::vector<uint8_t>> &out) {
+    std::vector<std::vector<std::vector<uint8_t>>> m;
+    tree_open_multi(c, t, {idx}, m);
+    out = std::move(m[0]);
 }
 
 }  // namespace sg

@@ -4,6 +4,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <array>
 #include <string>
 #include <vector>
 
@@ -46,14 +47,18 @@ uint64_t stdrng_first_u64(uint64_t seed);
 
 // ---------------------------------------------------------------- device trees
 // Levels of one tree, resident on the device. Digests are 4 u64 (LE lanes).
+// `batch` trees of identical shape, built together. Level l of proof k holds
+// real[l] digests at buf[4 * (off[l] + k * real[l])].
 struct DevTree {
     uint32_t arity = 2;
     uint64_t label = 0;
-    std::vector<uint64_t> len;  // nodes per level after padding to a multiple of arity
-    std::vector<uint64_t> real; // nodes per level before padding
-    std::vector<uint64_t> off;  // digest offset (in digests) of each level
+    uint32_t batch = 1;
+    std::vector<uint64_t> len;  // nodes per level (one tree) after padding to a multiple of arity
+    std::vector<uint64_t> real; // nodes per level (one tree) before padding
+    std::vector<uint64_t> off;  // digest offset of each level (all trees)
     uint64_t *buf = nullptr;    // all digests
-    uint8_t root[32];
+    std::vector<std::array<uint8_t, 32>> roots;  // per tree
+    uint8_t root[32];                            // tree 0
 };
 
 // Leaf i hashes DsLabel(arity, u32::MAX, i, label) || LE64(vals[base + i*leaf_stride + c*cs]) for c < w.
@@ -67,17 +72,31 @@ struct LeafSrc {
     uint32_t w;
     uint32_t nz;
     uint64_t *combo_out = nullptr;
+    uint64_t proof_stride = 0;  // value offset between the trees of a batch
 };
 
-// Build the whole tree with constant arity (crates/merkle/src/lib.rs:377-476).
+// Build `batch` trees of n_leaves leaves each, constant arity
+// (crates/merkle/src/lib.rs:377-476); tree k reads its values at
+// src.vals + k * src.proof_stride.
 void tree_build(Ctx *c, DevTree &t, const LeafSrc &src, uint64_t n_leaves, uint32_t arity, uint32_t depth,
-                uint64_t label);
+                uint64_t label, uint32_t batch = 1);
 void tree_free(Ctx *c, DevTree &t);
-// Serialized MerkleOpenings (leaf[32], path Vec<Vec<[32]>>, index u64) for a batch of indices.
+// Serialized MerkleOpenings (leaf[32], path Vec<Vec<[32]>>, index u64): idx[k]
+// lists the leaf indices to open in tree k; out[k][q] is one opening.
+void tree_open_multi(Ctx *c, const DevTree &t, const std::vector<std::vector<uint64_t>> &idx,
+                     std::vector<std::vector<std::vector<uint8_t>>> &out);
 void tree_open_batch(Ctx *c, const DevTree &t, const std::vector<uint64_t> &idx, std::vector<std::vector<uint8_t>> &out);
 
 // gather out[r] = src[idx[r]] for u64 elements (device index list)
 void gather_u64(Ctx *c, const uint64_t *src, const uint64_t *d_idx, uint64_t n, uint64_t *out);
+// gather rows of `width` u64: out[r*width + l] = src[idx[r]*width + l]
+void gather_u64_w(Ctx *c, const uint64_t *src, const uint64_t *d_idx, uint64_t n, uint32_t width, uint64_t *out);
+
+// DNS record commitments of K shards on the device (crates/swarm-dns/src/dns.rs:60-118):
+// canon[k][i] is DnsRecord::canonical_bytes of record i of shard k; returns the
+// doubly salted leaf hashes (32 bytes each) and each shard's DNS-NODE-V1 root.
+void dns_commit_batch(Ctx *c, const std::vector<std::vector<std::vector<uint8_t>>> &canon, const uint8_t salt[16],
+                      std::vector<std::vector<uint8_t>> &leaf_hashes, std::vector<std::array<uint8_t, 32>> &roots);
 
 uint32_t merkle_depth(uint64_t leaves, uint32_t arity);    // fri.rs:950-959
 uint32_t pick_arity(uint64_t n, uint64_t requested);       // fri.rs:1094-1103 (.max(2) applied)

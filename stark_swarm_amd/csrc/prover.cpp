@@ -12,6 +12,10 @@
 
 #include <string.h>
 
+#include <atomic>
+#include <map>
+#include <thread>
+
 #include "air.hpp"
 #include "fri.hpp"
 #include "gl.hpp"
@@ -24,8 +28,10 @@ static void put_le(std::vector<uint8_t> &v, uint64_t x, int n) {
     for (int i = 0; i < n; i++) v.push_back((uint8_t)(x >> (8 * i)));
 }
 
-void dns_leaf_hash(const DnsRec &r, const uint8_t salt[16], uint8_t out[32]) {
+// DnsRecord::canonical_bytes (dns.rs:60-70)
+std::vector<uint8_t> dns_canonical(const DnsRec &r) {
     std::vector<uint8_t> canon;
+    canon.reserve(27 + r.domain_len + r.rdata_len);
     const char *tag = "DNS-RECORD-V1";
     canon.insert(canon.end(), tag, tag + 13);
     put_le(canon, r.domain_len, 4);
@@ -34,6 +40,11 @@ void dns_leaf_hash(const DnsRec &r, const uint8_t salt[16], uint8_t out[32]) {
     put_le(canon, r.ttl, 4);
     put_le(canon, r.rdata_len, 4);
     canon.insert(canon.end(), r.rdata, r.rdata + r.rdata_len);
+    return canon;
+}
+
+void dns_leaf_hash(const DnsRec &r, const uint8_t salt[16], uint8_t out[32]) {
+    const std::vector<uint8_t> canon = dns_canonical(r);
     Sha3 h;
     uint8_t h1[32];
     h.update("DNS-LEAF-V1");
@@ -103,32 +114,110 @@ static void pack_hash(const uint8_t *h, std::vector<uint64_t> &out) {
     }
 }
 
+// One proof at a time: the batch path with a batch of one.
 InnerShard prove_inner_shard(Ctx *c, const uint8_t salt[16], const std::vector<DnsRec> &recs,
                              const uint8_t fs_binding[32], int phi_mode) {
-    SG_REQUIRE(!recs.empty(), "prove_inner_shard: merkle tree needs >= 1 leaf");
-    InnerShard o;
-    std::vector<uint8_t> lh(32 * recs.size());
-    for (size_t i = 0; i < recs.size(); i++) dns_leaf_hash(recs[i], salt, &lh[32 * i]);
-    dns_merkle_root(lh, o.merkle_root);
-    std::vector<uint64_t> leaves;
-    for (size_t i = 0; i < recs.size(); i++) pack_hash(&lh[32 * i], leaves);
+    std::array<uint8_t, 32> f;
+    memcpy(f.data(), fs_binding, 32);
+    std::vector<InnerShard> v = prove_inner_shard_batch(c, salt, {recs}, {f}, phi_mode);
+    return std::move(v[0]);
+}
+
+std::vector<std::vector<uint8_t>> rollup_prove_batch(Ctx *c, const std::vector<std::vector<uint64_t>> &leaves,
+                                                     const std::vector<std::array<uint8_t, 32>> &bindings,
+                                                     int phi_mode, std::vector<std::array<uint8_t, 32>> &root_f0) {
+    const uint32_t K = (uint32_t)leaves.size();
+    SG_REQUIRE(K >= 1 && bindings.size() == K, "rollup batch: one binding per proof");
     uint64_t n_trace = 8;
-    while (n_trace < leaves.size()) n_trace <<= 1;
-    o.n_trace = n_trace;
-    o.record_count = recs.size();
-    o.proof = rollup_prove(c, leaves, fs_binding, phi_mode, o.root_f0);
-    // pi_hash = SHA3("DNS-SHARD-PIHASH-V1" || salt || LE64(count) || root || root_f0 || fs_binding)
-    Sha3 h;
-    uint8_t cnt[8];
-    for (int i = 0; i < 8; i++) cnt[i] = (uint8_t)((uint64_t)recs.size() >> (8 * i));
-    h.update("DNS-SHARD-PIHASH-V1");
-    h.update(salt, 16);
-    h.update(cnt, 8);
-    h.update(o.merkle_root, 32);
-    h.update(o.root_f0, 32);
-    h.update(fs_binding, 32);
-    h.final(o.pi_hash);
-    return o;
+    while (n_trace < leaves[0].size()) n_trace <<= 1;
+    for (auto &l : leaves) {
+        uint64_t nt = 8;
+        while (nt < l.size()) nt <<= 1;
+        SG_REQUIRE(nt == n_trace, "rollup batch: proofs of one batch must share n_trace");
+    }
+    const uint64_t n0 = n_trace * SWARM_BLOWUP;
+    std::vector<uint64_t> trace(4 * n_trace * K);
+    parallel_for(K, [&](size_t k) {
+        build_hash_rollup_trace(n_trace, leaves[k].data(), leaves[k].size(), &trace[4 * n_trace * k]);
+    });
+    Scoped d_tr(c, 8 * trace.size()), d_lde(c, 8ULL * 4 * n0 * K), d_c(c, 8ULL * n0 * K);
+    SG_HIP(hipMemcpyAsync(d_tr.p, trace.data(), 8 * trace.size(), hipMemcpyHostToDevice, c->stream));
+    lde_columns(c, d_tr.as<uint64_t>(), d_lde.as<uint64_t>(), ilog2(n_trace), ilog2(SWARM_BLOWUP), 4ULL * K, n_trace,
+                n0, 1);
+    merge_hash_rollup_batch(c, d_lde.as<uint64_t>(), K, n_trace, SWARM_BLOWUP, phi_mode, d_c.as<uint64_t>());
+    FriParams p;
+    p.schedule = stir_schedule(n0);
+    p.r = p.s0 = SWARM_QUERIES;
+    p.seed_z = SWARM_SEED_Z;
+    p.coeff_commit_final = true;
+    p.d_final = 1;
+    p.stir = true;
+    p.has_pi = true;
+    std::vector<std::vector<uint8_t>> proofs = deep_fri_prove_batch(c, d_c.as<uint64_t>(), K, n0, p, &bindings, nullptr);
+    root_f0.resize(K);
+    for (uint32_t k = 0; k < K; k++) memcpy(root_f0[k].data(), proofs[k].data(), 32);
+    return proofs;
+}
+
+std::vector<InnerShard> prove_inner_shard_batch(Ctx *c, const uint8_t salt[16],
+                                                const std::vector<std::vector<DnsRec>> &shards,
+                                                const std::vector<std::array<uint8_t, 32>> &fs_bindings,
+                                                int phi_mode) {
+    const size_t K = shards.size();
+    SG_REQUIRE(fs_bindings.size() == K, "prove_inner_shard_batch: one fs_binding per shard");
+    std::vector<InnerShard> out(K);
+    std::vector<std::vector<uint64_t>> leaves(K);
+    for (auto &s : shards) SG_REQUIRE(!s.empty(), "prove_inner_shard: merkle tree needs >= 1 leaf");
+    // record commitments (the step before the path): canonical bytes on the
+    // host, leaf hashes and DNS trees on the device
+    std::vector<std::vector<std::vector<uint8_t>>> canon(K);
+    parallel_for(K, [&](size_t k) {
+        canon[k].resize(shards[k].size());
+        for (size_t i = 0; i < shards[k].size(); i++) canon[k][i] = dns_canonical(shards[k][i]);
+    });
+    std::vector<std::vector<uint8_t>> lhs;
+    std::vector<std::array<uint8_t, 32>> droots;
+    dns_commit_batch(c, canon, salt, lhs, droots);
+    parallel_for(K, [&](size_t k) {
+        memcpy(out[k].merkle_root, droots[k].data(), 32);
+        for (size_t i = 0; i < shards[k].size(); i++) pack_hash(&lhs[k][32 * i], leaves[k]);
+        uint64_t nt = 8;
+        while (nt < leaves[k].size()) nt <<= 1;
+        out[k].n_trace = nt;
+        out[k].record_count = shards[k].size();
+    });
+    // device: one lockstep batch per trace size
+    std::map<uint64_t, std::vector<size_t>> groups;
+    for (size_t k = 0; k < K; k++) groups[out[k].n_trace].push_back(k);
+    for (auto &g : groups) {
+        std::vector<std::vector<uint64_t>> lv;
+        std::vector<std::array<uint8_t, 32>> bd;
+        for (size_t k : g.second) {
+            lv.push_back(std::move(leaves[k]));
+            bd.push_back(fs_bindings[k]);
+        }
+        std::vector<std::array<uint8_t, 32>> rf;
+        std::vector<std::vector<uint8_t>> proofs = rollup_prove_batch(c, lv, bd, phi_mode, rf);
+        for (size_t j = 0; j < g.second.size(); j++) {
+            InnerShard &o = out[g.second[j]];
+            o.proof = std::move(proofs[j]);
+            memcpy(o.root_f0, rf[j].data(), 32);
+        }
+    }
+    for (size_t k = 0; k < K; k++) {
+        InnerShard &o = out[k];
+        Sha3 h;
+        uint8_t cnt[8];
+        for (int i = 0; i < 8; i++) cnt[i] = (uint8_t)((uint64_t)o.record_count >> (8 * i));
+        h.update("DNS-SHARD-PIHASH-V1");
+        h.update(salt, 16);
+        h.update(cnt, 8);
+        h.update(o.merkle_root, 32);
+        h.update(o.root_f0, 32);
+        h.update(fs_bindings[k].data(), 32);
+        h.final(o.pi_hash);
+    }
+    return out;
 }
 
 std::vector<uint8_t> prove_outer_rollup(Ctx *c, const uint8_t *pi_hashes, size_t n, const uint8_t pk_hash[32],

@@ -2,6 +2,7 @@
 #pragma once
 #include <stdint.h>
 
+#include <array>
 #include <vector>
 
 #include "ctx.hpp"
@@ -24,12 +25,21 @@ struct InnerShard {
     std::vector<uint8_t> proof;
 };
 
+std::vector<uint8_t> dns_canonical(const DnsRec &r);
 void dns_leaf_hash(const DnsRec &r, const uint8_t salt[16], uint8_t out[32]);
 void dns_merkle_root(const std::vector<uint8_t> &leaves, uint8_t root[32]);
 std::vector<uint8_t> rollup_prove(Ctx *c, const std::vector<uint64_t> &leaves, const uint8_t binding[32],
                                   int phi_mode, uint8_t root_f0[32]);
 InnerShard prove_inner_shard(Ctx *c, const uint8_t salt[16], const std::vector<DnsRec> &recs,
                              const uint8_t fs_binding[32], int phi_mode);
+// K shards in one lockstep device batch per trace size; host hashing on a thread pool
+std::vector<InnerShard> prove_inner_shard_batch(Ctx *c, const uint8_t salt[16],
+                                                const std::vector<std::vector<DnsRec>> &shards,
+                                                const std::vector<std::array<uint8_t, 32>> &fs_bindings,
+                                                int phi_mode);
+std::vector<std::vector<uint8_t>> rollup_prove_batch(Ctx *c, const std::vector<std::vector<uint64_t>> &leaves,
+                                                     const std::vector<std::array<uint8_t, 32>> &bindings,
+                                                     int phi_mode, std::vector<std::array<uint8_t, 32>> &root_f0);
 std::vector<uint8_t> prove_outer_rollup(Ctx *c, const uint8_t *pi_hashes, size_t n, const uint8_t pk_hash[32],
                                         int phi_mode, uint8_t root_f0[32]);
 std::vector<uint8_t> prover_core_dev(Ctx *c, const uint64_t *d_trace, uint64_t w, uint64_t n_trace, uint64_t blowup,

@@ -196,3 +196,49 @@ def test_prover_core_matches_oracle(S, w, lg, blowup, schedule, s0):
     assert root == rr.raw
     assert blob == ref
     assert np.array_equal(d_lde.download((w, n * blowup)), O.lde_columns(tr, blowup, shift=7))
+
+
+# ---------------------------------------------------------------- lockstep batches
+def test_golden_shards_as_one_batch(S):
+    from stark_swarm_amd.zone import build_synthetic_zone, split_evenly
+    d = _gold()
+    say("golden shards 1-3 as one lockstep batch")
+    salt = bytes.fromhex(d["zone_salt_hex"])
+    shards = split_evenly(build_synthetic_zone(d["record_count"]), 3)
+    fsbs = [S.shard_fs_binding(bytes.fromhex(d["authority_pk_hash_hex"]), bytes.fromhex(d["job_id_hex"]),
+                               w["shard_id"], bytes.fromhex(w["shard_nonce_hex"])) for w in d["shards"]]
+    outs = S.prove_inner_shard_batch(salt, shards, fsbs)
+    for w, o in zip(d["shards"], outs):
+        assert o.pi_hash.hex() == w["pi_hash_hex"]
+        assert o.merkle_root.hex() == w["merkle_root_hex"]
+        assert hashlib.sha3_256(o.proof_blob).hexdigest() == d["blobs"][w["proof"]]["sha3_256"]
+
+
+def test_mixed_size_shard_batch_equals_singles(S):
+    from stark_swarm_amd.zone import build_synthetic_zone
+    say("mixed-size shard batch vs single proofs")
+    zone = build_synthetic_zone(3000)
+    salt = b"0123456789abcdef"
+    sizes = [1024, 700, 1, 2048, 1024, 33, 300]
+    shards, fsbs, off = [], [], 0
+    for i, s in enumerate(sizes):
+        shards.append(zone[off:off + s])
+        off = (off + 131 * (i + 1)) % (3000 - 2048)
+        fsbs.append(hashlib.sha3_256(b"fsb%d" % i).digest())
+    outs = S.prove_inner_shard_batch(salt, shards, fsbs)
+    for sh, f, o in zip(shards, fsbs, outs):
+        single = S.prove_inner_shard(salt, sh, f)
+        assert o.proof_blob == single.proof_blob
+        assert o.pi_hash == single.pi_hash and o.n_trace == single.n_trace
+
+
+@pytest.mark.parametrize("K,lg,schedule", [(5, 9, [8, 8, 8]), (3, 13, [8, 8, 8, 8, 2]), (16, 8, [4, 4, 4, 4])])
+def test_fri_batch_matches_oracle(S, K, lg, schedule):
+    say("deep_fri_prove_batch K=%d n0=2^%d" % (K, lg))
+    f0s = rnd(600 + K + lg, (K, 1 << lg))
+    pis = [hashlib.sha3_256(b"pi%d" % k).digest() for k in range(K)]
+    got = S.deep_fri_prove_batch(f0s, S.FriDomain.new_radix2(1 << lg),
+                                 S.DeepFriParams(schedule, 7, S.SEED_Z, True, 1, True, 7, None), pis)
+    for k in range(K):
+        ref = O.deep_fri_prove(f0s[k], O.fri_params(schedule, r=7, s0=7, pi_hash=pis[k]))
+        assert got[k] == ref, "proof %d" % k
