@@ -28,6 +28,7 @@ from .api import (  # noqa: F401
     InnerShardOutput,
     LdtMode,
     OuterRollupOutput,
+    PackedShards,
     StarkError,
     BLOWUP,
     NUM_QUERIES,

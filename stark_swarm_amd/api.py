@@ -452,14 +452,29 @@ def prove_inner_shard(salt, records: Sequence[DnsRecord], fs_binding_32, ldt=Ldt
                             bytes(out.root_f0), blob)
 
 
+class PackedShards:
+    """Records of many shards packed once into the C record array that
+    stark_prove_inner_shard_batch reads; pass it in place of the list of
+    shards to prove the same records again without re-marshaling them."""
+
+    def __init__(self, shards):
+        flat = [r._c() for s in shards for r in s]
+        self.arr = (_DnsRecord * max(1, len(flat)))(*flat)
+        self.counts = [len(s) for s in shards]
+        self.sizes = (ctypes.c_size_t * max(1, len(shards)))(*self.counts)
+
+    def __len__(self):
+        return len(self.counts)
+
+
 def prove_inner_shard_batch(salt, shards, fs_bindings, ldt=LdtMode.Stir, phi_mode=PHI_RAYON, ctx=None):
     """Many prove_inner_shard calls as one lockstep device batch
-    (stark_prove_inner_shard_batch); result k equals prove_inner_shard on shard k."""
+    (stark_prove_inner_shard_batch); result k equals prove_inner_shard on shard k.
+    `shards` is a list of record lists or a PackedShards."""
     ctx = ctx or default_context()
-    assert len(shards) == len(fs_bindings)
-    flat = [r._c() for s in shards for r in s]
-    arr = (_DnsRecord * max(1, len(flat)))(*flat)
-    sizes = (ctypes.c_size_t * max(1, len(shards)))(*[len(s) for s in shards])
+    packed = shards if isinstance(shards, PackedShards) else PackedShards(shards)
+    assert len(packed) == len(fs_bindings)
+    arr, sizes = packed.arr, packed.sizes
     outs = (_InnerOut * max(1, len(shards)))()
     fsb = b"".join(bytes(f) for f in fs_bindings)
     ctx._check(lib().stark_prove_inner_shard_batch(ctx.h, bytes(salt), arr, sizes, len(shards), fsb, ldt.value,

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <atomic>
+#include <chrono>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -108,6 +109,21 @@ struct ProfScope {
     int tok;
     ProfScope(Ctx *c_, const char *name, double bytes) : c(c_), tok(c_->profiling ? c_->prof_begin(name, bytes) : -1) {}
     ~ProfScope() { if (tok >= 0) c->prof_end(tok); }
+};
+
+// Host wall time of a stage, kept beside the kernel stats as "host:<name>"
+// (launches counts the calls). Only for host code on the calling thread.
+struct HostScope {
+    Ctx *c;
+    const char *name;
+    std::chrono::steady_clock::time_point t0;
+    HostScope(Ctx *c_, const char *n) : c(c_), name(n), t0(std::chrono::steady_clock::now()) {}
+    ~HostScope() {
+        if (!c->profiling) return;
+        auto &s = c->stats[std::string("host:") + name];
+        s.ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        s.launches += 1;
+    }
 };
 
 Ctx *ctx_create(int device);

@@ -38,60 +38,86 @@ namespace {
 
 constexpr int FT = 256;
 
-// Per thread: chunk [k0, k0+ch) of proof blockIdx.y,
-// acc_rho = z^k0 * sum_{t = rho mod M} c_{k0+t} z^t (ch and k0 multiples of M).
-// zpow: per proof, z^t for t < ch; zch: per proof, z^ch.
+// Residue sums S_rho = sum_{k = rho mod M} c_k z^k of proof blockIdx.y.
+// T: per proof, T[i] = z^(2^i) (24 entries). Block b covers [k0, k0 + FT*ch),
+// k0 = b*FT*ch; thread tid reads k = k0 + t*FT + tid (coalesced), all of
+// residue tid mod M, and keeps one accumulator A = sum_t c_k z^(t*FT). The
+// z^tid factors come in as a Horner tree: lanes at distance s combine as
+// v_l + z^s v_{l+s} for s = M..32 inside a wave, then across the FT/64 waves
+// with z^64; lane rho < M is left with z^-(k0+rho) S_rho(block).
 template <int M>
-__global__ __launch_bounds__(FT) void ood_sums_k(const uint64_t *coef, uint64_t n, int ncomp, const uint64_t *zpow,
-                                                 uint32_t ch, const uint64_t *zch, uint64_t *partials) {
+__global__ __launch_bounds__(FT) void ood_sums_k(const uint64_t *coef, uint64_t n, int ncomp, const uint64_t *T,
+                                                 uint32_t tstride, uint32_t ch, uint64_t *partials) {
+    constexpr int NW = FT / 64;
+    __shared__ uint64_t zt[64 * 6];  // z^(t*FT), t < ch <= 64
+    __shared__ uint64_t red[NW][M * 6];
     const uint64_t pk = blockIdx.y;
     coef += pk * ncomp * n;
-    zpow += pk * 6 * ch;
-    const uint64_t tid = (uint64_t)blockIdx.x * FT + threadIdx.x;
-    const uint64_t k0 = tid * ch;
-    fp6 acc[M];
+    T += pk * 6 * tstride;
+    auto tget = [&](int i) {
+        fp6 r;
 #pragma unroll
-    for (int r = 0; r < M; r++) acc[r] = f6_zero();
-    if (k0 < n) {
-        for (uint32_t t = 0; t < ch; t += M) {
+        for (int q = 0; q < 6; q++) r.c[q] = T[6 * i + q];
+        return r;
+    };
+    const uint32_t tid = threadIdx.x;
+    if (tid < ch) {
+        fp6 w = f6_one();
+        for (int bit = 0; (1u << bit) <= tid; bit++)
+            if (tid >> bit & 1) w = f6_mul(w, tget(8 + bit));  // FT = 2^8
 #pragma unroll
-            for (int r = 0; r < M; r++) {
-                const uint64_t k = k0 + t + r;
-                fp6 zp;
-                memcpy(zp.c, zpow + 6 * (t + r), 48);
-                if (ncomp == 1) {
-                    acc[r] = f6_add(acc[r], f6_scale(zp, coef[k]));
-                } else {
-                    fp6 cv;
-#pragma unroll
-                    for (int q = 0; q < 6; q++) cv.c[q] = coef[q * n + k];
-                    acc[r] = f6_add(acc[r], f6_mul(cv, zp));
-                }
-            }
-        }
-        fp6 z_ch;
-        memcpy(z_ch.c, zch + 6 * pk, 48);
-        const fp6 zk = f6_pow(z_ch, tid);
-#pragma unroll
-        for (int r = 0; r < M; r++) acc[r] = f6_mul(acc[r], zk);
+        for (int q = 0; q < 6; q++) zt[6 * tid + q] = w.c[q];
     }
-    // block reduction through LDS, one residue at a time
-    __shared__ uint64_t red[FT * 6];
-    uint64_t *part = partials + (pk * gridDim.x + blockIdx.x) * M * 6;
-    for (int r = 0; r < M; r++) {
+    __syncthreads();
+    const uint64_t k0 = (uint64_t)blockIdx.x * FT * ch;
+    fp6 acc = f6_zero();
+    for (uint32_t t = 0; t < ch; t++) {
+        const uint64_t k = k0 + (uint64_t)t * FT + tid;
+        if (k >= n) break;
+        fp6 zp;
 #pragma unroll
-        for (int q = 0; q < 6; q++) red[q * FT + threadIdx.x] = acc[r].c[q];
-        __syncthreads();
-        for (int s = FT / 2; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) {
+        for (int q = 0; q < 6; q++) zp.c[q] = zt[6 * t + q];
+        if (ncomp == 1) {
+            acc = f6_add(acc, f6_scale(zp, coef[k]));
+        } else {
+            fp6 cv;
 #pragma unroll
-                for (int q = 0; q < 6; q++)
-                    red[q * FT + threadIdx.x] = add(red[q * FT + threadIdx.x], red[q * FT + threadIdx.x + s]);
-            }
-            __syncthreads();
+            for (int q = 0; q < 6; q++) cv.c[q] = coef[q * n + k];
+            acc = f6_add(acc, f6_mul(cv, zp));
         }
-        if (threadIdx.x < 6) part[r * 6 + threadIdx.x] = red[threadIdx.x * FT];
-        __syncthreads();
+    }
+    const uint32_t lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+    for (int s = M, bit = __builtin_ctz(M); s < 64; s <<= 1, bit++) {
+        fp6 o;
+#pragma unroll
+        for (int q = 0; q < 6; q++) o.c[q] = __shfl_down(acc.c[q], s, 64);
+        if ((lane & s) == 0) acc = f6_add(acc, f6_mul(o, tget(bit)));
+    }
+    if (lane < (uint32_t)M) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) red[wv][lane * 6 + q] = acc.c[q];
+    }
+    __syncthreads();
+    if (tid < (uint32_t)M) {
+        fp6 v;
+#pragma unroll
+        for (int q = 0; q < 6; q++) v.c[q] = red[NW - 1][tid * 6 + q];
+        const fp6 z64 = tget(6);
+#pragma unroll
+        for (int w = NW - 2; w >= 0; w--) {
+            fp6 x;
+#pragma unroll
+            for (int q = 0; q < 6; q++) x.c[q] = red[w][tid * 6 + q];
+            v = f6_add(f6_mul(v, z64), x);
+        }
+        // times z^(k0 + rho)
+        const uint64_t e = k0 + tid;
+        for (int bit = 0; bit < 24; bit++)
+            if (e >> bit & 1) v = f6_mul(v, tget(bit));
+        uint64_t *part = partials + (pk * gridDim.x + blockIdx.x) * M * 6;
+#pragma unroll
+        for (int q = 0; q < 6; q++) part[tid * 6 + q] = v.c[q];
     }
 }
 
@@ -247,33 +273,21 @@ void fold(Ctx *c, const uint64_t *f, uint64_t n, int ncomp, uint64_t m, uint32_t
     }
 }
 
-// coset evaluations of K proofs' layers at z_k zeta^j; evals_host: K x m x 6
+// coset evaluations of K proofs' layers at z_k zeta^j; evals_host: K x m x 6.
+// d_T: per proof tstride entries z0^(2^i); this layer's z = z0^(2^toff).
 template <int M>
-void launch_ood(Ctx *c, const uint64_t *coef, uint64_t n, int ncomp, uint32_t K, const std::vector<fp6> &z,
-                uint64_t zeta, uint64_t *evals_host) {
-    // chunk per thread: a multiple of M, small enough to keep the grid wide
+void launch_ood(Ctx *c, const uint64_t *coef, uint64_t n, int ncomp, uint32_t K, const uint64_t *d_T,
+                uint32_t tstride, uint32_t toff, uint64_t zeta, uint64_t *evals_host) {
+    SG_REQUIRE(n <= (1ULL << 24) && toff + 24 <= tstride, "ood: table too short");
+    // elements per thread: enough blocks to fill the chip, at most 64
     uint32_t ch = 64;
-    while (ch > (uint32_t)M && (n / ch) * K < 4096) ch >>= 1;
-    if (ch < (uint32_t)M) ch = M;
-    const uint64_t threads = (n + ch - 1) / ch;
-    const uint32_t nblocks = (uint32_t)((threads + FT - 1) / FT);
-    std::vector<uint64_t> zp(6 * (size_t)ch * K), zc(6 * (size_t)K);
-    for (uint32_t k = 0; k < K; k++) {
-        fp6 a = f6_one();
-        for (uint32_t t = 0; t < ch; t++) {
-            memcpy(&zp[6 * ((size_t)k * ch + t)], a.c, 48);
-            a = f6_mul(a, z[k]);
-        }
-        memcpy(&zc[6 * k], a.c, 48);  // z^ch
-    }
-    Scoped d_zp(c, 8 * zp.size()), d_zc(c, 8 * zc.size()), d_part(c, 48ULL * M * nblocks * K),
-        d_ev(c, 48ULL * M * K);
-    SG_HIP(hipMemcpyAsync(d_zp.p, zp.data(), 8 * zp.size(), hipMemcpyHostToDevice, c->stream));
-    SG_HIP(hipMemcpyAsync(d_zc.p, zc.data(), 8 * zc.size(), hipMemcpyHostToDevice, c->stream));
+    while (ch > 1 && (n / ((uint64_t)FT * ch)) * K < 2048) ch >>= 1;
+    const uint32_t nblocks = (uint32_t)((n + (uint64_t)FT * ch - 1) / ((uint64_t)FT * ch));
+    Scoped d_part(c, 48ULL * M * nblocks * K), d_ev(c, 48ULL * M * K);
     {
         ProfScope ps(c, "ood_sums", 8.0 * ncomp * n * K);
-        hipLaunchKernelGGL(ood_sums_k<M>, dim3(nblocks, K), dim3(FT), 0, c->stream, coef, n, ncomp,
-                           d_zp.as<uint64_t>(), ch, d_zc.as<uint64_t>(), d_part.as<uint64_t>());
+        hipLaunchKernelGGL(ood_sums_k<M>, dim3(nblocks, K), dim3(FT), 0, c->stream, coef, n, ncomp, d_T + 6 * toff,
+                           tstride, ch, d_part.as<uint64_t>());
         SG_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(ood_finish_k, dim3(K), dim3(128), 0, c->stream, d_part.as<uint64_t>(), nblocks, (uint32_t)M,
@@ -283,13 +297,13 @@ void launch_ood(Ctx *c, const uint64_t *coef, uint64_t n, int ncomp, uint32_t K,
     SG_HIP(hipStreamSynchronize(c->stream));
 }
 
-void ood(Ctx *c, const uint64_t *coef, uint64_t n, int ncomp, uint64_t m, uint32_t K, const std::vector<fp6> &z,
-         uint64_t zeta, uint64_t *evals_host) {
+void ood(Ctx *c, const uint64_t *coef, uint64_t n, int ncomp, uint64_t m, uint32_t K, const uint64_t *d_T,
+         uint32_t tstride, uint32_t toff, uint64_t zeta, uint64_t *evals_host) {
     switch (m) {
-    case 2: launch_ood<2>(c, coef, n, ncomp, K, z, zeta, evals_host); break;
-    case 4: launch_ood<4>(c, coef, n, ncomp, K, z, zeta, evals_host); break;
-    case 8: launch_ood<8>(c, coef, n, ncomp, K, z, zeta, evals_host); break;
-    case 16: launch_ood<16>(c, coef, n, ncomp, K, z, zeta, evals_host); break;
+    case 2: launch_ood<2>(c, coef, n, ncomp, K, d_T, tstride, toff, zeta, evals_host); break;
+    case 4: launch_ood<4>(c, coef, n, ncomp, K, d_T, tstride, toff, zeta, evals_host); break;
+    case 8: launch_ood<8>(c, coef, n, ncomp, K, d_T, tstride, toff, zeta, evals_host); break;
+    case 16: launch_ood<16>(c, coef, n, ncomp, K, d_T, tstride, toff, zeta, evals_host); break;
     default: throw Error(E_UNSUPPORTED, "fold arity must be 2, 4, 8 or 16");
     }
 }
@@ -364,10 +378,22 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
         tree_build(c, t0, src, nn0, a, merkle_depth(nn0, a), 0xFF, K);
     }
     std::vector<fp6> z(K), alpha(K);
+    // z_l = z^(m_0 ... m_{l-1}): one table of squares z^(2^i) serves every layer
+    uint32_t tstride = 24;
+    for (uint64_t m : p.schedule) tstride += ilog2(m);
+    std::vector<uint64_t> ztab((size_t)K * tstride * 6);
     parallel_for(K, [&](size_t k) {
         tr[k].absorb(t0.roots[k].data(), 32);
         tr[k].challenge_ext("z_fp3", z[k].c);
+        fp6 w = z[k];
+        for (uint32_t i = 0; i < tstride; i++) {
+            memcpy(&ztab[((size_t)k * tstride + i) * 6], w.c, 48);
+            w = f6_mul(w, w);
+        }
     });
+    Scoped d_ztab(c, 8 * ztab.size());
+    SG_HIP(hipMemcpyAsync(d_ztab.p, ztab.data(), 8 * ztab.size(), hipMemcpyHostToDevice, c->stream));
+    uint32_t toff = 0;
 
     // layers
     std::vector<std::vector<uint64_t>> coset_evals(L);  // [l][k * m * 6]
@@ -395,9 +421,10 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
             ntt_columns(c, f, coef.as<uint64_t>(), scr.as<uint64_t>(), ilog2(cur), (uint64_t)K * ncomp, cur, cur, true,
                         nullptr);
             const uint64_t zeta = pow(root_of_unity(cur), nn);
-            ood(c, coef.as<uint64_t>(), cur, ncomp, m, K, z, zeta, coset_evals[ell].data());
+            ood(c, coef.as<uint64_t>(), cur, ncomp, m, K, d_ztab.as<uint64_t>(), tstride, toff,
+                zeta, coset_evals[ell].data());
         }
-        for (uint32_t k = 0; k < K; k++) z[k] = f6_pow(z[k], m);
+        toff += ilog2(m);
         // layer commitment: leaf = 6 components (fri.rs:1364-1366); a base-field
         // layer (f0) hashes components 1..5 as zeros
         DevTree tl;
@@ -472,23 +499,18 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
     const uint64_t dfin = p.d_final < nf ? p.d_final : nf;
     std::vector<std::vector<fp6>> final_coeffs(K, std::vector<fp6>(dfin));
     std::vector<uint64_t> query_seed(K);
-    uint64_t npow2 = 1;
-    while (npow2 < n0) npow2 <<= 1;
-    std::vector<std::vector<uint64_t>> raw(K, std::vector<uint64_t>(p.s0)), base(K, std::vector<uint64_t>(p.s0));
+    std::vector<std::vector<uint64_t>> raw, base(K, std::vector<uint64_t>(p.s0));
     parallel_for(K, [&](size_t k) {
         for (uint64_t i = 0; i < dfin; i++)
             for (int q = 0; q < 6; q++) final_coeffs[k][i].c[q] = fin[(k * 6 + q) * nf + i];
         for (auto &fc : final_coeffs[k])
             for (int q = 0; q < 6; q++) tr[k].absorb_field(fc.c[q]);
         query_seed[k] = tr[k].challenge_field("query_seed");
-        // STIR proximity query indices (fri.rs:1813-1893)
-        for (uint64_t q = 0; q < p.s0; q++) {
-            const uint64_t fl[3] = {query_seed[k], 0, canon(q)};
-            const uint64_t seed = hash_fields_tagged("FRI/index", fl, 3);
-            raw[k][q] = (stdrng_first_u64(seed) & (npow2 - 1)) % n0;
-            base[k][q] = raw[k][q] % nn0;
-        }
     });
+    // STIR proximity query indices (fri.rs:1813-1893)
+    fri_query_indices(c, query_seed, p.s0, n0, raw);
+    for (uint32_t k = 0; k < K; k++)
+        for (uint64_t q = 0; q < p.s0; q++) base[k][q] = raw[k][q] % nn0;
     if (raw_out) *raw_out = raw;
     const bool l2 = L >= 2;
     const uint64_t n1 = n0 / m0;
@@ -512,8 +534,12 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
         SG_HIP(hipStreamSynchronize(c->stream));
     }
     std::vector<std::vector<std::vector<uint8_t>>> op0, op1;
-    tree_open_multi(c, t0, base, op0);
-    if (l2) tree_open_multi(c, t1, base, op1);
+    {
+        HostScope ho(c, "fri_openings");
+        tree_open_multi(c, t0, base, op0);
+        if (l2) tree_open_multi(c, t1, base, op1);
+    }
+    HostScope hs(c, "fri_serialize");
 
     // serialize DeepFriProof<SexticExt> (fri.rs:1286-1300, ark-serialize compressed)
     std::vector<std::vector<uint8_t>> proofs(K);

@@ -343,6 +343,76 @@ __global__ void gather_k(const uint64_t *src, const uint64_t *idx, uint64_t n, u
     out[r] = src[idx[q] * width + l];
 }
 
+// ---- STIR query indices (crates/fri/src/fri.rs:1813-1893) ----
+// Bytes [start, start + len) of a 32-byte digest held as 4 LE lanes, LE value.
+__device__ __forceinline__ uint64_t digest_chunk(const uint64_t *d, int start, int len) {
+    const int i = start >> 3, sh = (start & 7) * 8;
+    uint64_t v = d[i] >> sh;
+    if (sh && 64 - sh < 8 * len) v |= d[i + 1] << (64 - sh);
+    return len == 8 ? v : v & ((1ULL << (8 * len)) - 1);
+}
+
+// One thread per (proof k, query q): seed = hash_fields_tagged("FRI/index",
+// [query_seed_k, 0, q]) as a two-block SHA3 over a fixed message with the two
+// variable fields XORed in at byte offsets seed_off / q_off, the digest
+// folded to a field element by 7-byte chunks from the end, then the first
+// u64 of StdRng(seed) (ChaCha12 block 0) gives raw = (r & (npow2-1)) % n0.
+constexpr int QI_BLOCKS = 2;
+__global__ __launch_bounds__(KT) void query_indices_k(const uint64_t *tmpl, const uint64_t *seeds, uint32_t K,
+                                                      uint32_t s0, uint32_t seed_off, uint32_t q_off, uint64_t npow2,
+                                                      uint64_t n0, uint64_t *raw) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= K * s0) return;
+    const uint32_t k = i / s0, q = i - k * s0;
+    uint64_t m[17 * QI_BLOCKS];
+#pragma unroll
+    for (int j = 0; j < 17 * QI_BLOCKS; j++) m[j] = tmpl[j];
+    auto put = [&](uint32_t off, uint64_t x) {
+        const uint32_t l = off >> 3, sh = (off & 7) * 8;
+#pragma unroll
+        for (int j = 0; j < 17 * QI_BLOCKS; j++) {
+            if ((uint32_t)j == l) m[j] ^= x << sh;
+            if (sh && (uint32_t)j == l + 1) m[j] ^= x >> (64 - sh);
+        }
+    };
+    put(seed_off, seeds[k]);
+    put(q_off, (uint64_t)q);
+    uint64_t st[25];
+#pragma unroll
+    for (int j = 0; j < 25; j++) st[j] = 0;
+#pragma unroll
+    for (int b = 0; b < QI_BLOCKS; b++) {
+#pragma unroll
+        for (int j = 0; j < 17; j++) st[j] ^= m[17 * b + j];
+        keccak_f_dev(st);
+    }
+    uint64_t acc = 0;
+    for (int end = 32; end > 0;) {
+        const int start = end >= 7 ? end - 7 : 0, len = end - start;
+        acc = add(mul(acc, 1ULL << (8 * len)), digest_chunk(st, start, len));
+        end = start;
+    }
+    // ChaCha12 block 0 with key LE64(acc) || 0^24, counter 0, nonce 0
+    const uint32_t in[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u, (uint32_t)acc, (uint32_t)(acc >> 32),
+                             0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t x[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) x[j] = in[j];
+    auto qr = [&](int a, int b, int c, int d) {
+        x[a] += x[b]; x[d] = __builtin_rotateleft32(x[d] ^ x[a], 16);
+        x[c] += x[d]; x[b] = __builtin_rotateleft32(x[b] ^ x[c], 12);
+        x[a] += x[b]; x[d] = __builtin_rotateleft32(x[d] ^ x[a], 8);
+        x[c] += x[d]; x[b] = __builtin_rotateleft32(x[b] ^ x[c], 7);
+    };
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+        qr(0, 4, 8, 12); qr(1, 5, 9, 13); qr(2, 6, 10, 14); qr(3, 7, 11, 15);
+        qr(0, 5, 10, 15); qr(1, 6, 11, 12); qr(2, 7, 8, 13); qr(3, 4, 9, 14);
+    }
+    const uint64_t r64 = (uint64_t)(x[0] + in[0]) | ((uint64_t)(x[1] + in[1]) << 32);
+    raw[i] = (r64 & (npow2 - 1)) % n0;
+}
+
 // ---- DNS record commitments (crates/swarm-dns/src/dns.rs:43-118) ----
 // h1 = SHA3("DNS-LEAF-V1" || salt || canonical(record)) from host-padded
 // message lanes (nblk[r] blocks of 17 lanes at msg + 17 * blk_off[r]);
@@ -418,43 +488,46 @@ __global__ __launch_bounds__(KT) void dns_node_k(const uint64_t *child, uint64_t
 
 }  // namespace
 
-// Leaf hashes (h2) and Merkle roots of K DNS shards on the device.
-// recs_per_shard[k] records each; the host lays out padded message lanes.
-void dns_commit_batch(Ctx *c, const std::vector<std::vector<std::vector<uint8_t>>> &canon, const uint8_t salt[16],
+// Leaf hashes (h2) and Merkle roots of K DNS shards on the device. The host
+// writes each record's first SHA3 message, padded, straight into the lanes.
+void dns_commit_batch(Ctx *c, const RecordSource &src, const uint8_t salt[16],
                       std::vector<std::vector<uint8_t>> &leaf_hashes, std::vector<std::array<uint8_t, 32>> &roots) {
-    const size_t K = canon.size();
-    std::vector<uint64_t> blk_off;
-    std::vector<uint32_t> nblk;
+    const size_t K = src.counts.size();
     std::vector<size_t> first(K + 1, 0);
-    uint64_t total_blocks = 0;
     for (size_t k = 0; k < K; k++) {
-        first[k + 1] = first[k] + canon[k].size();
-        for (auto &cb : canon[k]) {
-            const size_t len = 11 + 16 + cb.size();
-            const uint32_t nb = (uint32_t)(len / 136 + 1);
-            blk_off.push_back(total_blocks);
-            nblk.push_back(nb);
-            total_blocks += nb;
-        }
+        // dns.rs merkle_root unwraps the top level: an empty shard has no root
+        SG_REQUIRE(src.counts[k] >= 1, "dns commit: a shard needs at least one record");
+        first[k + 1] = first[k] + src.counts[k];
     }
     const size_t nrec = first[K];
-    SG_REQUIRE(nrec >= 1, "dns commit: no records");
-    std::vector<uint64_t> msg(17 * total_blocks, 0);
+    std::vector<uint64_t> blk_off(nrec);
+    std::vector<uint32_t> nblk(nrec);
     parallel_for(K, [&](size_t k) {
-        for (size_t i = 0; i < canon[k].size(); i++) {
-            const size_t r = first[k] + i;
-            uint8_t *p = (uint8_t *)&msg[17 * blk_off[r]];
-            size_t o = 0;
-            memcpy(p + o, "DNS-LEAF-V1", 11);
-            o += 11;
-            memcpy(p + o, salt, 16);
-            o += 16;
-            memcpy(p + o, canon[k][i].data(), canon[k][i].size());
-            o += canon[k][i].size();
-            p[o] ^= 0x06;
-            p[136 * nblk[r] - 1] ^= 0x80;
+        for (size_t i = 0; i < src.counts[k]; i++) {
+            const size_t len = 11 + 16 + src.size(k, i);
+            nblk[first[k] + i] = (uint32_t)(len / 136 + 1);
         }
     });
+    uint64_t total_blocks = 0;
+    for (size_t r = 0; r < nrec; r++) {
+        blk_off[r] = total_blocks;
+        total_blocks += nblk[r];
+    }
+    std::vector<uint64_t> msg(17 * total_blocks, 0);
+    {
+        HostScope hm(c, "dns_messages");
+        parallel_for(K, [&](size_t k) {
+            for (size_t i = 0; i < src.counts[k]; i++) {
+                const size_t r = first[k] + i;
+                uint8_t *p = (uint8_t *)&msg[17 * blk_off[r]];
+                memcpy(p, "DNS-LEAF-V1", 11);
+                memcpy(p + 11, salt, 16);
+                const size_t cl = src.write(k, i, p + 27);
+                p[27 + cl] ^= 0x06;
+                p[136 * nblk[r] - 1] ^= 0x80;
+            }
+        });
+    }
     uint64_t prefix2[5] = {0, 0, 0, 0, 0};
     {
         uint8_t pb[40] = {0};
@@ -482,7 +555,7 @@ void dns_commit_batch(Ctx *c, const std::vector<std::vector<std::vector<uint8_t>
     // trees: shards of equal size go up level by level in one launch per level
     roots.assign(K, {});
     std::map<size_t, std::vector<size_t>> by_size;
-    for (size_t k = 0; k < K; k++) by_size[canon[k].size()].push_back(k);
+    for (size_t k = 0; k < K; k++) by_size[src.counts[k]].push_back(k);
     for (auto &g : by_size) {
         const size_t n = g.first, G = g.second.size();
         std::vector<uint64_t> idx;
@@ -595,28 +668,34 @@ void gather_u64_w(Ctx *c, const uint64_t *src, const uint64_t *d_idx, uint64_t n
 void tree_open_multi(Ctx *c, const DevTree &t, const std::vector<std::vector<uint64_t>> &idx,
                      std::vector<std::vector<std::vector<uint8_t>>> &out) {
     const uint32_t a = t.arity;
-    const size_t nlev = t.real.size();
-    SG_REQUIRE(idx.size() <= t.batch, "merkle open: more index lists than trees");
+    const size_t nlev = t.real.size(), K = idx.size();
+    SG_REQUIRE(K <= t.batch, "merkle open: more index lists than trees");
     // digest indices: per query, the leaf then every level's full group
-    std::vector<uint64_t> want;
-    for (size_t k = 0; k < idx.size(); k++) {
+    const size_t per = 1 + (nlev - 1) * a;
+    std::vector<size_t> qoff(K + 1, 0);
+    for (size_t k = 0; k < K; k++) {
+        for (uint64_t q : idx[k]) SG_REQUIRE(q < t.real[0], "merkle open: leaf index out of range");
+        qoff[k + 1] = qoff[k] + idx[k].size();
+    }
+    out.assign(K, {});
+    if (!qoff[K]) return;
+    std::vector<uint64_t> want(qoff[K] * per);
+    parallel_for(K, [&](size_t k) {
+        uint64_t *w = &want[qoff[k] * per];
         for (uint64_t q : idx[k]) {
-            SG_REQUIRE(q < t.real[0], "merkle open: leaf index out of range");
-            want.push_back(t.off[0] + k * t.real[0] + q);
+            *w++ = t.off[0] + k * t.real[0] + q;
             uint64_t x = q;
             for (size_t l = 0; l + 1 < nlev; l++) {
                 const uint64_t gs = x / a * a;
                 for (uint32_t i = 0; i < a; i++) {
                     uint64_t pos = gs + i;
                     if (pos >= t.real[l]) pos = t.real[l] - 1;
-                    want.push_back(t.off[l] + k * t.real[l] + pos);
+                    *w++ = t.off[l] + k * t.real[l] + pos;
                 }
                 x /= a;
             }
         }
-    }
-    out.assign(idx.size(), {});
-    if (want.empty()) return;
+    });
     Scoped d_idx(c, 8 * want.size()), d_out(c, 32 * want.size());
     SG_HIP(hipMemcpyAsync(d_idx.p, want.data(), 8 * want.size(), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(gather_k, dim3((uint32_t)((4 * want.size() + 255) / 256)), dim3(256), 0, c->stream, t.buf,
@@ -625,31 +704,86 @@ void tree_open_multi(Ctx *c, const DevTree &t, const std::vector<std::vector<uin
     std::vector<uint64_t> h(4 * want.size());
     SG_HIP(hipMemcpyAsync(h.data(), d_out.p, 32 * want.size(), hipMemcpyDeviceToHost, c->stream));
     SG_HIP(hipStreamSynchronize(c->stream));
-    size_t w = 0;
-    auto put64 = [](std::vector<uint8_t> &v, uint64_t x) {
-        for (int i = 0; i < 8; i++) v.push_back((uint8_t)(x >> (8 * i)));
+    // MerkleOpening: leaf[32] || LE64(levels) || per level (LE64(a-1) || a-1 siblings) || LE64(index)
+    const size_t osz = 32 + 8 + (nlev - 1) * (8 + 32 * (size_t)(a - 1)) + 8;
+    auto put64 = [](uint8_t *p, uint64_t x) {
+        for (int i = 0; i < 8; i++) p[i] = (uint8_t)(x >> (8 * i));
     };
-    for (size_t k = 0; k < idx.size(); k++) {
-        for (uint64_t q : idx[k]) {
-            std::vector<uint8_t> o;
-            const uint8_t *leaf = (const uint8_t *)&h[4 * w++];
-            o.insert(o.end(), leaf, leaf + 32);
-            put64(o, nlev - 1);
+    parallel_for(K, [&](size_t k) {
+        out[k].resize(idx[k].size());
+        const uint64_t *hw = &h[4 * qoff[k] * per];
+        for (size_t j = 0; j < idx[k].size(); j++) {
+            const uint64_t q = idx[k][j];
+            std::vector<uint8_t> &o = out[k][j];
+            o.resize(osz);
+            uint8_t *p = o.data();
+            memcpy(p, hw, 32);
+            hw += 4;
+            put64(p + 32, nlev - 1);
+            p += 40;
             uint64_t x = q;
             for (size_t l = 0; l + 1 < nlev; l++) {
-                put64(o, a - 1);
+                put64(p, a - 1);
+                p += 8;
                 const uint64_t gs = x / a * a;
-                for (uint32_t i = 0; i < a; i++, w++) {
+                for (uint32_t i = 0; i < a; i++, hw += 4) {
                     if (gs + i == x) continue;
-                    const uint8_t *d = (const uint8_t *)&h[4 * w];
-                    o.insert(o.end(), d, d + 32);
+                    memcpy(p, hw, 32);
+                    p += 32;
                 }
                 x /= a;
             }
-            put64(o, q);
-            out[k].push_back(std::move(o));
+            put64(p, q);
         }
+    });
+}
+
+void fri_query_indices(Ctx *c, const std::vector<uint64_t> &seeds, uint64_t s0, uint64_t n0,
+                       std::vector<std::vector<uint64_t>> &raw) {
+    const size_t K = seeds.size();
+    raw.assign(K, std::vector<uint64_t>(s0));
+    if (!K || !s0) return;
+    SG_REQUIRE(K * s0 < (1ULL << 31), "query indices: too many queries");
+    // the byte stream of hash_fields_tagged("FRI/index", [seed, 0, q]) with seed = q = 0
+    std::vector<uint8_t> b;
+    auto app = [&](const char *str) { b.insert(b.end(), str, str + strlen(str)); };
+    auto le8 = [&]() { b.insert(b.end(), 8, 0); };
+    app("FSv1-TRANSCRIPT-INIT");
+    app("FRI/FS");
+    app("FSv1-ABSORB-BYTES");
+    app("FRI/index");
+    app("FSv1-ABSORB-BYTES");
+    const uint32_t seed_off = (uint32_t)b.size();
+    le8();
+    app("FSv1-ABSORB-BYTES");
+    le8();
+    app("FSv1-ABSORB-BYTES");
+    const uint32_t q_off = (uint32_t)b.size();
+    le8();
+    app("FSv1-CHALLENGE");
+    app("out");
+    const size_t nb = b.size() / 136 + 1;
+    SG_REQUIRE(nb == (size_t)QI_BLOCKS, "query indices: message block count");
+    std::vector<uint8_t> pad(136 * nb, 0);
+    memcpy(pad.data(), b.data(), b.size());
+    pad[b.size()] ^= 0x06;
+    pad[pad.size() - 1] ^= 0x80;
+    uint64_t npow2 = 1;
+    while (npow2 < n0) npow2 <<= 1;
+    Scoped d_t(c, pad.size()), d_s(c, 8 * K), d_r(c, 8 * K * s0);
+    SG_HIP(hipMemcpyAsync(d_t.p, pad.data(), pad.size(), hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipMemcpyAsync(d_s.p, seeds.data(), 8 * K, hipMemcpyHostToDevice, c->stream));
+    {
+        ProfScope ps(c, "query_indices", 8.0 * K * s0);
+        hipLaunchKernelGGL(query_indices_k, dim3((uint32_t)((K * s0 + KT - 1) / KT)), dim3(KT), 0, c->stream,
+                           d_t.as<uint64_t>(), d_s.as<uint64_t>(), (uint32_t)K, (uint32_t)s0, seed_off, q_off, npow2,
+                           n0, d_r.as<uint64_t>());
+        SG_HIP(hipGetLastError());
     }
+    std::vector<uint64_t> all(K * s0);
+    SG_HIP(hipMemcpyAsync(all.data(), d_r.p, 8 * K * s0, hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    for (size_t k = 0; k < K; k++) std::copy(all.begin() + k * s0, all.begin() + (k + 1) * s0, raw[k].begin());
 }
 
 void tree_open_batch(Ctx *c, const DevTree &t, const std::vector<uint64_t> &idx,

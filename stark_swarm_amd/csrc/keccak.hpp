@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <array>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -92,11 +93,22 @@ void gather_u64(Ctx *c, const uint64_t *src, const uint64_t *d_idx, uint64_t n, 
 // gather rows of `width` u64: out[r*width + l] = src[idx[r]*width + l]
 void gather_u64_w(Ctx *c, const uint64_t *src, const uint64_t *d_idx, uint64_t n, uint32_t width, uint64_t *out);
 
+// Records of K shards: counts[k] records in shard k; size(k, i) is the length
+// of DnsRecord::canonical_bytes of record i and write(k, i, dst) writes them.
+struct RecordSource {
+    std::vector<size_t> counts;
+    std::function<size_t(size_t, size_t)> size;
+    std::function<size_t(size_t, size_t, uint8_t *)> write;
+};
 // DNS record commitments of K shards on the device (crates/swarm-dns/src/dns.rs:60-118):
-// canon[k][i] is DnsRecord::canonical_bytes of record i of shard k; returns the
-// doubly salted leaf hashes (32 bytes each) and each shard's DNS-NODE-V1 root.
-void dns_commit_batch(Ctx *c, const std::vector<std::vector<std::vector<uint8_t>>> &canon, const uint8_t salt[16],
+// the doubly salted leaf hashes (32 bytes each) and each shard's DNS-NODE-V1 root.
+void dns_commit_batch(Ctx *c, const RecordSource &src, const uint8_t salt[16],
                       std::vector<std::vector<uint8_t>> &leaf_hashes, std::vector<std::array<uint8_t, 32>> &roots);
+
+// STIR query indices of K proofs on the device (fri.rs:1813-1893): raw[k][q] =
+// (StdRng(hash_fields_tagged("FRI/index", [seeds[k], 0, q])).next_u64() & (npow2-1)) % n0
+void fri_query_indices(Ctx *c, const std::vector<uint64_t> &seeds, uint64_t s0, uint64_t n0,
+                       std::vector<std::vector<uint64_t>> &raw);
 
 uint32_t merkle_depth(uint64_t leaves, uint32_t arity);    // fri.rs:950-959
 uint32_t pick_arity(uint64_t n, uint64_t requested);       // fri.rs:1094-1103 (.max(2) applied)

@@ -232,10 +232,11 @@ __global__ __launch_bounds__(NT, 2) void ntt_tile(NttPass p) {
             xs[u] = (gok && i < S) ? in[(int64_t)gl * p.ig + (int64_t)i * p.ii] : 0;
         }
     }
-    uint64_t *out0 = p.out + b_hi * p.ob + b_lo * p.ob2 + g0 * p.og;
+    const uint32_t rbase = blockIdx.y * p.nr;
+    uint64_t *out0 = p.out + b_hi * p.ob + b_lo * p.ob2 + g0 * p.og + (uint64_t)rbase * p.out_rs;
     for (uint32_t rep = 0; rep < p.nr; rep++) {
-        // pre-twiddle w^(r * (g*pre_g + i*pre_i + b*pre_b)), r = r0 + rep
-        const uint64_t rr = p.r0 + rep;
+        // pre-twiddle w^(r * (g*pre_g + i*pre_i + b*pre_b)), r = r0 + rbase + rep
+        const uint64_t rr = p.r0 + rbase + rep;
         const bool pre = p.pre_log && rr;
         if (HOLD || inc_in) {
             uint64_t w = 1, st = 1;
@@ -244,6 +245,8 @@ __global__ __launch_bounds__(NT, 2) void ntt_tile(NttPass p) {
                 w = root_pow(p.rt_lo, p.rt_hi, p.pre_log, rr * e0, false);
                 st = root_pow(p.rt_lo, p.rt_hi, p.pre_log, rr * di * p.pre_i, false);
             }
+            // xs[] of HOLD lives in scratch at this unroll depth; unrolling
+            // fully would keep it in VGPRs but spills more and runs slower
 #pragma unroll 2
             for (int u = 0; u < EPT; u++) {
                 const uint32_t i = il + u * di;
@@ -378,12 +381,18 @@ void launch_pass(Ctx *c, NttPass p, const char *name) {
     SG_REQUIRE(tiles < (1ULL << 31), "ntt pass: grid too large");
     const double bytes = 8.0 * (double)p.ng * (double)(1u << p.log_s) * p.nb * (1.0 + p.nr);
     ProfScope ps(c, name, bytes);
+    // few tiles (small LDEs): deal the repetitions to grid.y so the launch
+    // still fills the chip; block y runs repetitions [y*nr, (y+1)*nr)
+    uint32_t ry = 1;
+    while (p.nr % (2 * ry) == 0 && tiles * ry < 2048) ry *= 2;
+    p.nr /= ry;
     // hold the tile in VGPRs across repetitions when the input is g-fast
     const bool hold = p.nr > 1 && p.log_g > 0 && (1u << p.log_g) <= (uint32_t)NT && p.ig < p.ii && p.log_s >= 5;
+    const dim3 grid((uint32_t)tiles, ry);
     if (p.inv)
-        launch_tile<true>(p.log_s, hold, dim3((uint32_t)tiles), c->stream, p);
+        launch_tile<true>(p.log_s, hold, grid, c->stream, p);
     else
-        launch_tile<false>(p.log_s, hold, dim3((uint32_t)tiles), c->stream, p);
+        launch_tile<false>(p.log_s, hold, grid, c->stream, p);
     SG_HIP(hipGetLastError());
 }
 

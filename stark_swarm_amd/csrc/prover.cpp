@@ -28,7 +28,21 @@ static void put_le(std::vector<uint8_t> &v, uint64_t x, int n) {
     for (int i = 0; i < n; i++) v.push_back((uint8_t)(x >> (8 * i)));
 }
 
-// DnsRecord::canonical_bytes (dns.rs:60-70)
+// DnsRecord::canonical_bytes (dns.rs:60-70) written to dst; returns the length
+size_t dns_canonical_to(const DnsRec &r, uint8_t *dst) {
+    size_t o = 0;
+    memcpy(dst, "DNS-RECORD-V1", 13);
+    o += 13;
+    for (int i = 0; i < 4; i++) dst[o++] = (uint8_t)(r.domain_len >> (8 * i));
+    memcpy(dst + o, r.domain, r.domain_len);
+    o += r.domain_len;
+    for (int i = 0; i < 2; i++) dst[o++] = (uint8_t)(r.record_type >> (8 * i));
+    for (int i = 0; i < 4; i++) dst[o++] = (uint8_t)(r.ttl >> (8 * i));
+    for (int i = 0; i < 4; i++) dst[o++] = (uint8_t)(r.rdata_len >> (8 * i));
+    memcpy(dst + o, r.rdata, r.rdata_len);
+    return o + r.rdata_len;
+}
+
 std::vector<uint8_t> dns_canonical(const DnsRec &r) {
     std::vector<uint8_t> canon;
     canon.reserve(27 + r.domain_len + r.rdata_len);
@@ -137,9 +151,12 @@ std::vector<std::vector<uint8_t>> rollup_prove_batch(Ctx *c, const std::vector<s
     }
     const uint64_t n0 = n_trace * SWARM_BLOWUP;
     std::vector<uint64_t> trace(4 * n_trace * K);
-    parallel_for(K, [&](size_t k) {
-        build_hash_rollup_trace(n_trace, leaves[k].data(), leaves[k].size(), &trace[4 * n_trace * k]);
-    });
+    {
+        HostScope h(c, "rollup_trace");
+        parallel_for(K, [&](size_t k) {
+            build_hash_rollup_trace(n_trace, leaves[k].data(), leaves[k].size(), &trace[4 * n_trace * k]);
+        });
+    }
     Scoped d_tr(c, 8 * trace.size()), d_lde(c, 8ULL * 4 * n0 * K), d_c(c, 8ULL * n0 * K);
     SG_HIP(hipMemcpyAsync(d_tr.p, trace.data(), 8 * trace.size(), hipMemcpyHostToDevice, c->stream));
     lde_columns(c, d_tr.as<uint64_t>(), d_lde.as<uint64_t>(), ilog2(n_trace), ilog2(SWARM_BLOWUP), 4ULL * K, n_trace,
@@ -153,6 +170,7 @@ std::vector<std::vector<uint8_t>> rollup_prove_batch(Ctx *c, const std::vector<s
     p.d_final = 1;
     p.stir = true;
     p.has_pi = true;
+    HostScope h(c, "deep_fri_prove_batch");
     std::vector<std::vector<uint8_t>> proofs = deep_fri_prove_batch(c, d_c.as<uint64_t>(), K, n0, p, &bindings, nullptr);
     root_f0.resize(K);
     for (uint32_t k = 0; k < K; k++) memcpy(root_f0[k].data(), proofs[k].data(), 32);
@@ -170,14 +188,16 @@ std::vector<InnerShard> prove_inner_shard_batch(Ctx *c, const uint8_t salt[16],
     for (auto &s : shards) SG_REQUIRE(!s.empty(), "prove_inner_shard: merkle tree needs >= 1 leaf");
     // record commitments (the step before the path): canonical bytes on the
     // host, leaf hashes and DNS trees on the device
-    std::vector<std::vector<std::vector<uint8_t>>> canon(K);
-    parallel_for(K, [&](size_t k) {
-        canon[k].resize(shards[k].size());
-        for (size_t i = 0; i < shards[k].size(); i++) canon[k][i] = dns_canonical(shards[k][i]);
-    });
+    RecordSource rs;
+    for (auto &s : shards) rs.counts.push_back(s.size());
+    rs.size = [&](size_t k, size_t i) { return 27 + shards[k][i].domain_len + shards[k][i].rdata_len; };
+    rs.write = [&](size_t k, size_t i, uint8_t *dst) { return dns_canonical_to(shards[k][i], dst); };
     std::vector<std::vector<uint8_t>> lhs;
     std::vector<std::array<uint8_t, 32>> droots;
-    dns_commit_batch(c, canon, salt, lhs, droots);
+    {
+        HostScope h(c, "dns_commit");
+        dns_commit_batch(c, rs, salt, lhs, droots);
+    }
     parallel_for(K, [&](size_t k) {
         memcpy(out[k].merkle_root, droots[k].data(), 32);
         for (size_t i = 0; i < shards[k].size(); i++) pack_hash(&lhs[k][32 * i], leaves[k]);
@@ -197,6 +217,7 @@ std::vector<InnerShard> prove_inner_shard_batch(Ctx *c, const uint8_t salt[16],
             bd.push_back(fs_bindings[k]);
         }
         std::vector<std::array<uint8_t, 32>> rf;
+        HostScope h(c, "rollup_prove_batch");
         std::vector<std::vector<uint8_t>> proofs = rollup_prove_batch(c, lv, bd, phi_mode, rf);
         for (size_t j = 0; j < g.second.size(); j++) {
             InnerShard &o = out[g.second[j]];

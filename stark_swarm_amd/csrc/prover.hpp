@@ -26,6 +26,7 @@ struct InnerShard {
 };
 
 std::vector<uint8_t> dns_canonical(const DnsRec &r);
+size_t dns_canonical_to(const DnsRec &r, uint8_t *dst);  // returns the length written
 void dns_leaf_hash(const DnsRec &r, const uint8_t salt[16], uint8_t out[32]);
 void dns_merkle_root(const std::vector<uint8_t> &leaves, uint8_t root[32]);
 std::vector<uint8_t> rollup_prove(Ctx *c, const std::vector<uint64_t> &leaves, const uint8_t binding[32],

@@ -242,3 +242,44 @@ def test_fri_batch_matches_oracle(S, K, lg, schedule):
     for k in range(K):
         ref = O.deep_fri_prove(f0s[k], O.fri_params(schedule, r=7, s0=7, pi_hash=pis[k]))
         assert got[k] == ref, "proof %d" % k
+
+
+def test_dns_commitments_with_long_records(S):
+    """Device leaf hashes / DNS trees against the host restatement and the
+    oracle's tree, including records whose message spans several SHA3 blocks."""
+    say("dns commitments, long records")
+    salt = b"zone-salt-16byte"
+    recs = []
+    for i in range(37):
+        ln = [0, 1, 60, 80, 81, 82, 100, 108, 109, 110, 200, 244, 245, 300, 500][i % 15]
+        recs.append(S.DnsRecord.txt("t%d.example.org" % i, 60 + i, "x" * ln))
+    recs.append(S.DnsRecord.aaaa("v6.example.org", 300, bytes(range(16))))
+    leaves = b"".join(r.leaf_hash(salt) for r in recs)
+    ref_root = O.dns_merkle_root(leaves)
+    fsb = hashlib.sha3_256(b"long").digest()
+    outs = S.prove_inner_shard_batch(salt, [recs, recs[:5], recs[7:8]], [fsb, fsb, fsb])
+    assert outs[0].merkle_root == ref_root
+    assert outs[1].merkle_root == O.dns_merkle_root(leaves[:5 * 32])
+    assert outs[2].merkle_root == leaves[7 * 32:8 * 32]  # a one-leaf tree is its leaf
+    # the proof itself against the oracle on the same leaf hashes
+    proof, pi, root, rf = O.prove_inner_shard(salt, leaves, fsb, phi_mode=1)
+    assert outs[0].proof_blob == proof and outs[0].pi_hash == pi
+
+
+def test_packed_shards_and_empty_shard(S):
+    """A PackedShards batch proves the same bytes as the record lists, twice in
+    a row; a shard without records is an error (dns.rs merkle_root unwraps)."""
+    from stark_swarm_amd.zone import build_synthetic_zone
+    say("packed shards, empty shard")
+    salt = b"swarm-test-zone1"
+    zone = build_synthetic_zone(300)
+    shards = [zone[:40], zone[40:41], zone[41:300]]
+    fsbs = [hashlib.sha3_256(bytes([i])).digest() for i in range(3)]
+    ref = [o.proof_blob for o in S.prove_inner_shard_batch(salt, shards, fsbs)]
+    packed = S.PackedShards(shards)
+    for _ in range(2):
+        assert [o.proof_blob for o in S.prove_inner_shard_batch(salt, packed, fsbs)] == ref
+    with pytest.raises(S.StarkError):
+        S.prove_inner_shard_batch(salt, [zone[:3], []], fsbs[:2])
+    with pytest.raises(S.StarkError):
+        S.prove_inner_shard(salt, [], fsbs[0])
