@@ -9,12 +9,12 @@
 // A DFT of size n = n1*n2 is two strided passes (four-step):
 //   pass A  Z[k1*n2+i2] = w_n^(i2*k1) * DFT_n1( x[i1*n2+i2] )     (columns)
 //   pass B  X[k1+n1*k2] = DFT_n2( Z[k1*n2+i2] )                    (rows)
-// Each pass is one launch of `ntt_tile<LOG_S, INV>`: a 256-lane workgroup owns
+// Each pass is one launch of `ntt_tile<LOG_S, INV>`: a 512-lane workgroup owns
 // a tile of G sub-DFTs of size S = 2^LOG_S (S*G = 8192 u64, 68 KiB of padded
 // LDS, two workgroups per CU). The tile is loaded with its contiguous
 // dimension on consecutive lanes (coalesced runs of G u64 or whole rows), the
-// sub-DFT runs as 2-3 Stockham steps of radix <= 32 held in VGPRs (32
-// elements per lane), and the result is stored the same way, fusing input
+// sub-DFT runs as Stockham steps of radix <= 16 held in VGPRs (16 elements
+// per lane), and the result is stored the same way, fusing input
 // pre-twiddles, the four-step twiddle and output scaling.
 //
 // Inside a register DFT every twiddle is a power of w_32 = 2^78 (mod p), so
@@ -37,9 +37,9 @@ namespace {
 
 constexpr int TILE_LOG = 13;
 constexpr int TILE = 1 << TILE_LOG;
-constexpr int NT = 256;
+constexpr int NT = 512;
 constexpr int EPT = TILE / NT;  // elements held per lane during a step
-constexpr int MAX_LR = 5;       // register DFTs of at most 2^MAX_LR points
+constexpr int MAX_LR = 4;       // register DFTs of at most 2^MAX_LR points
 constexpr int PAD_SHIFT = 4;
 constexpr int LDS_ELEMS = TILE + (TILE >> PAD_SHIFT);
 
@@ -196,8 +196,11 @@ __device__ __forceinline__ void stockham_steps(uint64_t *lds, uint32_t tile, con
     }
 }
 
-template <int LOG_S, bool INV, bool HOLD>
-__global__ __launch_bounds__(NT, 2) void ntt_tile(NttPass p) {
+template <int LOG_S, bool INV>
+// 512 lanes x 16 elements, at most 128 VGPRs: two blocks (4 waves per SIMD)
+// share a CU, which hides more of the 64-bit multiply latency than 256 x 32
+// at 2 waves per SIMD (about 5 % faster on the 2^20 x 128 LDE)
+__global__ __launch_bounds__(NT, 4) void ntt_tile(NttPass p) {
     __shared__ uint64_t lds[LDS_ELEMS];
     constexpr uint32_t S = 1u << LOG_S;
     const uint32_t log_g = p.log_g, G = 1u << log_g;
@@ -208,6 +211,10 @@ __global__ __launch_bounds__(NT, 2) void ntt_tile(NttPass p) {
     // G u64 per row) meet in the same L2.
     uint64_t blk = blockIdx.x;
     if ((gridDim.x & 7) == 0) blk = (blk & 7) * (gridDim.x >> 3) + (blk >> 3);
+    // the repetition chunks of one tile are neighbours on one XCD, so all but
+    // the first read the tile's input from L2
+    const uint32_t rbase = (uint32_t)(blk & ((1u << p.rs_log) - 1)) * p.nr;
+    blk >>= p.rs_log;
     const uint64_t b = blk / tiles_per_batch;
     const uint64_t g0 = (blk - b * tiles_per_batch) << log_g;
     const uint64_t b_hi = b / p.nb2, b_lo = b - b_hi * p.nb2;
@@ -223,39 +230,23 @@ __global__ __launch_bounds__(NT, 2) void ntt_tile(NttPass p) {
     const uint32_t di = NT >> log_g;
     const bool gok = g0 + gl < p.ng;
 
-    // HOLD: the tile stays in VGPRs across the repetitions (LDE cosets)
-    uint64_t xs[HOLD ? EPT : 1];
-    if constexpr (HOLD) {
-#pragma unroll
-        for (int u = 0; u < EPT; u++) {
-            const uint32_t i = il + u * di;
-            xs[u] = (gok && i < S) ? in[(int64_t)gl * p.ig + (int64_t)i * p.ii] : 0;
-        }
-    }
-    const uint32_t rbase = blockIdx.y * p.nr;
     uint64_t *out0 = p.out + b_hi * p.ob + b_lo * p.ob2 + g0 * p.og + (uint64_t)rbase * p.out_rs;
     for (uint32_t rep = 0; rep < p.nr; rep++) {
         // pre-twiddle w^(r * (g*pre_g + i*pre_i + b*pre_b)), r = r0 + rbase + rep
         const uint64_t rr = p.r0 + rbase + rep;
         const bool pre = p.pre_log && rr;
-        if (HOLD || inc_in) {
+        if (inc_in) {
             uint64_t w = 1, st = 1;
             if (pre) {
                 const uint64_t e0 = (g0 + gl) * p.pre_g + (uint64_t)il * p.pre_i + b * p.pre_b;
                 w = root_pow(p.rt_lo, p.rt_hi, p.pre_log, rr * e0, false);
                 st = root_pow(p.rt_lo, p.rt_hi, p.pre_log, rr * di * p.pre_i, false);
             }
-            // xs[] of HOLD lives in scratch at this unroll depth; unrolling
-            // fully would keep it in VGPRs but spills more and runs slower
 #pragma unroll 2
             for (int u = 0; u < EPT; u++) {
                 const uint32_t i = il + u * di;
                 if (i < S) {
-                    uint64_t x;
-                    if constexpr (HOLD)
-                        x = xs[u];
-                    else
-                        x = gok ? in[(int64_t)gl * p.ig + (int64_t)i * p.ii] : 0;
+                    uint64_t x = gok ? in[(int64_t)gl * p.ig + (int64_t)i * p.ii] : 0;
                     if (pre) x = mul(x, w);
                     lds[padi((gl << LOG_S) + i)] = x;
                 }
@@ -344,19 +335,10 @@ __global__ __launch_bounds__(NT, 2) void ntt_tile(NttPass p) {
 }
 
 template <bool INV>
-void launch_tile(uint32_t log_s, bool hold, dim3 grid, hipStream_t st, const NttPass &p) {
-    if (hold) {
-        switch (log_s) {
-#define SG_CASE(L) \
-    case L: hipLaunchKernelGGL((ntt_tile<L, INV, true>), grid, dim3(NT), 0, st, p); return;
-            SG_CASE(5) SG_CASE(6) SG_CASE(7) SG_CASE(8) SG_CASE(9) SG_CASE(10) SG_CASE(11) SG_CASE(12)
-#undef SG_CASE
-        default: break;
-        }
-    }
+void launch_tile(uint32_t log_s, dim3 grid, hipStream_t st, const NttPass &p) {
     switch (log_s) {
 #define SG_CASE(L) \
-    case L: hipLaunchKernelGGL((ntt_tile<L, INV, false>), grid, dim3(NT), 0, st, p); break;
+    case L: hipLaunchKernelGGL((ntt_tile<L, INV>), grid, dim3(NT), 0, st, p); break;
         SG_CASE(0) SG_CASE(1) SG_CASE(2) SG_CASE(3) SG_CASE(4) SG_CASE(5) SG_CASE(6)
         SG_CASE(7) SG_CASE(8) SG_CASE(9) SG_CASE(10) SG_CASE(11) SG_CASE(12)
 #undef SG_CASE
@@ -381,18 +363,18 @@ void launch_pass(Ctx *c, NttPass p, const char *name) {
     SG_REQUIRE(tiles < (1ULL << 31), "ntt pass: grid too large");
     const double bytes = 8.0 * (double)p.ng * (double)(1u << p.log_s) * p.nb * (1.0 + p.nr);
     ProfScope ps(c, name, bytes);
-    // few tiles (small LDEs): deal the repetitions to grid.y so the launch
-    // still fills the chip; block y runs repetitions [y*nr, (y+1)*nr)
-    uint32_t ry = 1;
-    while (p.nr % (2 * ry) == 0 && tiles * ry < 2048) ry *= 2;
-    p.nr /= ry;
-    // hold the tile in VGPRs across repetitions when the input is g-fast
-    const bool hold = p.nr > 1 && p.log_g > 0 && (1u << p.log_g) <= (uint32_t)NT && p.ig < p.ii && p.log_s >= 5;
-    const dim3 grid((uint32_t)tiles, ry);
+    // one repetition per workgroup: the B coset DFTs of a tile run side by
+    // side on one XCD instead of one after another, which fills the chip for
+    // small LDEs and reads the tile from HBM about once
+    uint32_t rs_log = 0;
+    while (p.nr % (2u << rs_log) == 0 && (tiles << (rs_log + 1)) < (1ULL << 31)) rs_log++;
+    p.rs_log = rs_log;
+    p.nr >>= rs_log;
+    const dim3 grid((uint32_t)(tiles << rs_log));
     if (p.inv)
-        launch_tile<true>(p.log_s, hold, grid, c->stream, p);
+        launch_tile<true>(p.log_s, grid, c->stream, p);
     else
-        launch_tile<false>(p.log_s, hold, grid, c->stream, p);
+        launch_tile<false>(p.log_s, grid, c->stream, p);
     SG_HIP(hipGetLastError());
 }
 

@@ -22,6 +22,9 @@ struct NttPass {
     // pre-twiddle multiplier r = r0 + q and writing at out + q * out_rs
     uint32_t nr, r0;
     int64_t out_rs;
+    // repetitions dealt to 2^rs_log workgroups per tile (set by launch_pass):
+    // blockIdx.x = tile * 2^rs_log + chunk, chunk c runs repetitions [c nr, (c+1) nr)
+    uint32_t rs_log;
     uint32_t log_s, log_g;
     uint32_t inv;
     // input pre-multiply by w_{2^pre_log}^(r * (g*pre_g + i*pre_i + b*pre_b)) (0 = off)
