@@ -115,6 +115,8 @@ def main():
     ap.add_argument("--calib-copy", action="store_true")
     ap.add_argument("--streams", type=int, default=4, help="proofs in flight for the batched measurement")
     ap.add_argument("--shards", type=int, default=32, help="inner-shard proofs for the plain-vs-batched line (0 = skip)")
+    ap.add_argument("--shards-only", action="store_true",
+                    help="run only the inner-shard plain-vs-batched measurement (for profiling that path)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the prover core runs on a single MI355X and this build has no "
@@ -125,6 +127,9 @@ def main():
         return
 
     import stark_swarm_amd as S
+    if args.shards_only:
+        print(json.dumps(shard_batch(S, args.streams, max(1, args.shards))))
+        return
     ctx = S.default_context()
     w, lg, blowup = args.cols, args.log_rows, 8
     n = 1 << lg

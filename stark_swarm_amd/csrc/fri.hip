@@ -122,28 +122,39 @@ __global__ __launch_bounds__(FT) void ood_sums_k(const uint64_t *coef, uint64_t 
 }
 
 // One block per proof: sum the block partials, then evals_j = sum_rho zeta^(j rho) S_rho.
-__global__ void ood_finish_k(const uint64_t *partials, uint32_t nblocks, uint32_t m, uint64_t zeta, uint64_t *evals) {
+// One block of FIN_T threads per proof: sum the block partials (thread
+// (c, r) adds blocks b = r mod R of column c = (rho, comp), then the R row
+// sums meet in LDS), then evals_j = sum_rho zeta^(j rho) S_rho.
+constexpr int FIN_T = 1024;
+__global__ __launch_bounds__(FIN_T) void ood_finish_k(const uint64_t *partials, uint32_t nblocks, uint32_t m,
+                                                      uint64_t zeta, uint64_t *evals) {
+    __shared__ uint64_t red[FIN_T];
     __shared__ uint64_t S[16 * 6];
     const uint64_t pk = blockIdx.x;
     partials += pk * nblocks * m * 6;
     evals += pk * m * 6;
-    const uint32_t t = threadIdx.x;  // t < 6*m : (rho, comp)
-    if (t < 6 * m) {
-        const uint32_t rho = t / 6, q = t % 6;
-        uint64_t s = 0;
-        for (uint32_t b = 0; b < nblocks; b++) s = add(s, partials[((uint64_t)b * m + rho) * 6 + q]);
-        S[t] = s;
+    const uint32_t C = 6 * m, R = FIN_T / C, t = threadIdx.x;
+    const uint32_t c = t % C, r = t / C;
+    uint64_t s = 0;
+    if (r < R)
+        for (uint32_t b = r; b < nblocks; b += R) s = add(s, partials[(uint64_t)b * C + c]);
+    red[t] = s;
+    __syncthreads();
+    if (t < C) {
+        uint64_t v = 0;
+        for (uint32_t i = 0; i < R; i++) v = add(v, red[i * C + t]);
+        S[t] = v;
     }
     __syncthreads();
-    if (t < 6 * m) {
+    if (t < C) {
         const uint32_t j = t / 6, q = t % 6;
         const uint64_t zj = pow(zeta, j);
-        uint64_t zp = 1, s = 0;
+        uint64_t zp = 1, acc = 0;
         for (uint32_t rho = 0; rho < m; rho++) {
-            s = add(s, mul(S[rho * 6 + q], zp));
+            acc = add(acc, mul(S[rho * 6 + q], zp));
             zp = mul(zp, zj);
         }
-        evals[j * 6 + q] = s;
+        evals[j * 6 + q] = acc;
     }
 }
 
@@ -203,10 +214,29 @@ __device__ __forceinline__ void idft_small(uint64_t *v) {
     }
 }
 
-// f_next[b] = sum_k alpha^k * (1/m) w^(-bk) * IDFT_m(f[b + j nn])_k, proof blockIdx.y;
-// optional coefficient-tuple planes
+// x * u^E in Fp6 = Fp[u]/(u^6 - 7), components in memory order
+// (c0.0, c0.1, c0.2, c1.0, c1.1, c1.2) = coefficients of (1, u^2, u^4, u, u^3, u^5)
+template <int E>
+__device__ __forceinline__ void f6_add_mul_upow(fp6 &acc, const fp6 &x) {
+    constexpr int upow[6] = {0, 2, 4, 1, 3, 5};
+    constexpr int slot[6] = {0, 3, 1, 4, 2, 5};  // memory slot of u^i
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const int e = upow[q] + E;
+        const uint64_t v = e >= 6 ? mul(x.c[q], 7) : x.c[q];
+        const int d = slot[e % 6];
+        acc.c[d] = add(acc.c[d], v);
+    }
+}
+
+// f_next[b] = sum_k alpha^k * (1/m) w^(-bk) * IDFT_m(f[b + j nn])_k, proof blockIdx.y.
+// One component at a time: with y_k = sum_q y_{k,q} E_q (E_q the basis
+// element of memory slot q), sum_k alpha^k y_k = sum_q E_q sum_k y_{k,q} alpha^k,
+// so each component costs a scalar times Fp6 per k (apow: alpha^k, k < M, per
+// proof) and only one component's M values are live at a time.
+// Optional coefficient-tuple planes: tuples[(k*6 + q) * nn + b] = y_{k,q}.
 template <int LM>
-__global__ __launch_bounds__(FT) void coset_fold_k(const uint64_t *f, uint64_t n, int ncomp, const uint64_t *alphas,
+__global__ __launch_bounds__(FT) void coset_fold_k(const uint64_t *f, uint64_t n, int ncomp, const uint64_t *apow,
                                                    uint64_t minv, const uint64_t *rt_lo, const uint64_t *rt_hi,
                                                    uint32_t logn, uint64_t *fnext, uint64_t *tuples) {
     constexpr int M = 1 << LM;
@@ -216,59 +246,69 @@ __global__ __launch_bounds__(FT) void coset_fold_k(const uint64_t *f, uint64_t n
     const uint64_t pk = blockIdx.y;
     f += pk * ncomp * n;
     fnext += pk * 6 * nn;
+    apow += pk * 6 * M;
     if (tuples) tuples += pk * 6 * M * nn;
-    uint64_t v[6][M];
-#pragma unroll
-    for (int q = 0; q < 6; q++) {
-#pragma unroll
-        for (int j = 0; j < M; j++) v[q][j] = (q < ncomp) ? f[q * n + b + j * nn] : 0;
-        if (q < ncomp) idft_small<LM>(v[q]);
-    }
     // w^(-b) from the 2^26 root table
     uint64_t e26 = (b << (26 - logn)) & ((1ULL << 26) - 1);
     e26 = (0 - e26) & ((1ULL << 26) - 1);
     const uint64_t wb = mul(rt_lo[e26 & 8191], rt_hi[e26 >> 13]);
-    uint64_t s = minv;  // (1/m) w^(-bk)
-    fp6 y[M];
+    fp6 acc = f6_zero();
+    auto component = [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if (q >= ncomp) {
+            if (tuples)
 #pragma unroll
-    for (int k = 0; k < M; k++) {
+                for (int k = 0; k < M; k++) tuples[(uint64_t)(k * 6 + q) * nn + b] = 0;
+            return;
+        }
+        uint64_t v[M];
 #pragma unroll
-        for (int q = 0; q < 6; q++) y[k].c[q] = (q < ncomp) ? mul(v[q][k], s) : 0;
-        s = mul(s, wb);
-    }
-    if (tuples) {
+        for (int j = 0; j < M; j++) v[j] = f[q * n + b + j * nn];
+        idft_small<LM>(v);
+        uint64_t s = minv;  // (1/m) w^(-bk)
+        fp6 pq = f6_zero();
 #pragma unroll
-        for (int k = 0; k < M; k++)
+        for (int k = 0; k < M; k++) {
+            const uint64_t y = mul(v[k], s);
+            if (tuples) tuples[(uint64_t)(k * 6 + q) * nn + b] = y;
+            fp6 ak;
 #pragma unroll
-            for (int q = 0; q < 6; q++) tuples[(uint64_t)(k * 6 + q) * nn + b] = y[k].c[q];
-    }
-    fp6 a;
-    memcpy(a.c, alphas + 6 * pk, 48);
-    fp6 acc = y[M - 1];
-#pragma unroll
-    for (int k = M - 2; k >= 0; k--) acc = f6_add(f6_mul(acc, a), y[k]);
+            for (int i = 0; i < 6; i++) ak.c[i] = apow[6 * k + i];
+            pq = f6_add(pq, f6_scale(ak, y));
+            s = mul(s, wb);
+        }
+        constexpr int upow[6] = {0, 2, 4, 1, 3, 5};
+        f6_add_mul_upow<upow[q]>(acc, pq);
+    };
+    component(std::integral_constant<int, 0>{});
+    component(std::integral_constant<int, 1>{});
+    component(std::integral_constant<int, 2>{});
+    component(std::integral_constant<int, 3>{});
+    component(std::integral_constant<int, 4>{});
+    component(std::integral_constant<int, 5>{});
 #pragma unroll
     for (int q = 0; q < 6; q++) fnext[q * nn + b] = acc.c[q];
 }
 
 template <int LM>
-void launch_fold(Ctx *c, const uint64_t *f, uint64_t n, int ncomp, uint32_t K, const uint64_t *d_alphas,
+void launch_fold(Ctx *c, const uint64_t *f, uint64_t n, int ncomp, uint32_t K, const uint64_t *d_apow,
                  uint64_t *fnext, uint64_t *tuples) {
     const uint64_t nn = n >> LM;
     const uint64_t minv = inv(1ULL << LM);
     ProfScope ps(c, "coset_fold", (8.0 * ncomp * n + 48.0 * nn + (tuples ? 48.0 * n : 0)) * K);
     hipLaunchKernelGGL(coset_fold_k<LM>, dim3((uint32_t)((nn + FT - 1) / FT), K), dim3(FT), 0, c->stream, f, n, ncomp,
-                       d_alphas, minv, c->rt_lo, c->rt_hi, ilog2(n), fnext, tuples);
+                       d_apow, minv, c->rt_lo, c->rt_hi, ilog2(n), fnext, tuples);
     SG_HIP(hipGetLastError());
 }
 
-void fold(Ctx *c, const uint64_t *f, uint64_t n, int ncomp, uint64_t m, uint32_t K, const uint64_t *d_alphas,
+// d_apow: per proof alpha^k, k < m (m Fp6 values)
+void fold(Ctx *c, const uint64_t *f, uint64_t n, int ncomp, uint64_t m, uint32_t K, const uint64_t *d_apow,
           uint64_t *fnext, uint64_t *tuples) {
     switch (m) {
-    case 2: launch_fold<1>(c, f, n, ncomp, K, d_alphas, fnext, tuples); break;
-    case 4: launch_fold<2>(c, f, n, ncomp, K, d_alphas, fnext, tuples); break;
-    case 8: launch_fold<3>(c, f, n, ncomp, K, d_alphas, fnext, tuples); break;
-    case 16: launch_fold<4>(c, f, n, ncomp, K, d_alphas, fnext, tuples); break;
+    case 2: launch_fold<1>(c, f, n, ncomp, K, d_apow, fnext, tuples); break;
+    case 4: launch_fold<2>(c, f, n, ncomp, K, d_apow, fnext, tuples); break;
+    case 8: launch_fold<3>(c, f, n, ncomp, K, d_apow, fnext, tuples); break;
+    case 16: launch_fold<4>(c, f, n, ncomp, K, d_apow, fnext, tuples); break;
     default: throw Error(E_UNSUPPORTED, "fold arity must be 2, 4, 8 or 16");
     }
 }
@@ -290,7 +330,7 @@ void launch_ood(Ctx *c, const uint64_t *coef, uint64_t n, int ncomp, uint32_t K,
                            tstride, ch, d_part.as<uint64_t>());
         SG_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(ood_finish_k, dim3(K), dim3(128), 0, c->stream, d_part.as<uint64_t>(), nblocks, (uint32_t)M,
+    hipLaunchKernelGGL(ood_finish_k, dim3(K), dim3(FIN_T), 0, c->stream, d_part.as<uint64_t>(), nblocks, (uint32_t)M,
                        zeta, d_ev.as<uint64_t>());
     SG_HIP(hipGetLastError());
     SG_HIP(hipMemcpyAsync(evals_host, d_ev.p, 48ULL * M * K, hipMemcpyDeviceToHost, c->stream));
@@ -320,8 +360,17 @@ struct Bytes {
     }
 };
 
-void upload_f6(Ctx *c, const std::vector<fp6> &v, uint64_t *d) {
-    SG_HIP(hipMemcpyAsync(d, v.data(), 48 * v.size(), hipMemcpyHostToDevice, c->stream));
+// alpha_k^j for j < m, proof-major, to d (K * m Fp6)
+void upload_powers(Ctx *c, const std::vector<fp6> &alpha, uint64_t m, uint64_t *d) {
+    std::vector<fp6> pw(alpha.size() * m);
+    for (size_t k = 0; k < alpha.size(); k++) {
+        fp6 x = f6_one();
+        for (uint64_t j = 0; j < m; j++) {
+            pw[k * m + j] = x;
+            x = f6_mul(x, alpha[k]);
+        }
+    }
+    SG_HIP(hipMemcpyAsync(d, pw.data(), 48 * pw.size(), hipMemcpyHostToDevice, c->stream));
 }
 
 }  // namespace
@@ -403,7 +452,7 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
     uint64_t n_final_tuples = 0, m_final = 0;
     DevTree t1;
     bool have_t1 = false;
-    Scoped d_alpha(c, 48ULL * K);
+    Scoped d_alpha(c, 48ULL * 16 * K);  // alpha powers, m <= 16
 
     uint64_t cur = n0;
     const uint64_t *f = d_f0;  // layer planes of all proofs
@@ -448,7 +497,7 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
         if (final_cc) {
             Scoped tup(c, 8ULL * 6 * m * nn * K);
             std::vector<fp6> zero(K, f6_zero());
-            upload_f6(c, zero, d_alpha.as<uint64_t>());
+            upload_powers(c, zero, m, d_alpha.as<uint64_t>());
             fold(c, f, cur, ncomp, m, K, d_alpha.as<uint64_t>(), next, tup.as<uint64_t>());
             const uint32_t a = pick_arity(nn, 16);
             DevTree tc;
@@ -470,7 +519,7 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
             tree_free(c, tc);
         }
         // fold with this layer's alpha (the final layer's is drawn after its coefficient commitment)
-        upload_f6(c, alpha, d_alpha.as<uint64_t>());
+        upload_powers(c, alpha, m, d_alpha.as<uint64_t>());
         fold(c, f, cur, ncomp, m, K, d_alpha.as<uint64_t>(), next, nullptr);
         if (ell == 0) {
             layer1 = next;
