@@ -91,6 +91,20 @@ def _cpu_model():
     return "unknown"
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch (read + write) of `kernel` on the 2^20 x 128 workload
+    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary, or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic__prover_core_2e20x128.md")
+    try:
+        for line in open(path):
+            f = line.split()
+            if len(f) == 4 and f[0] == "TRAFFIC" and f[1] == kernel:
+                return float(f[2]) + float(f[3])
+    except OSError:
+        pass
+    return None
+
+
 def calib_copy(nbytes=1 << 30):
     """A plain device copy of known size, for scaling FETCH_SIZE / WRITE_SIZE."""
     import torch
@@ -204,7 +218,10 @@ def main():
         "merkle_ms": round(merkle_ms, 3),
         "proof_bytes": len(ref_blob),
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(dname) if (lg, w) == (20, 128) else None,
+                     "traffic_source": "profiles/hbm_traffic__prover_core_2e20x128.md (FETCH_SIZE x2 + WRITE_SIZE x1, "
+                                       "calibrated on a 1 GiB copy)",
                      "bytes_per_launch": per_launch_bytes, "avg_launch_ms": round(avg_ms, 4)},
         "kernels": kernels,
     }
