@@ -81,6 +81,17 @@ typedef struct {
     size_t proof_len;
 } stark_inner_shard_out;
 
+/* DsKskOutput (crates/swarm-dns/src/prover.rs:464-490) */
+typedef struct {
+    uint8_t pi_hash[32];
+    uint8_t asserted_digest[32]; /* SHA-256(dnskey) as read from the trace */
+    uint8_t root_f0[32];
+    size_t n_blocks;
+    size_t n_trace;
+    uint8_t *proof; /* ark-serialize compressed DeepFriProof<SexticExt>; stark_free() it */
+    size_t proof_len;
+} stark_ds_ksk_out;
+
 /* ---- context ---- */
 int stark_ctx_create(int device, stark_ctx **out);
 void stark_ctx_destroy(stark_ctx *ctx);
@@ -135,6 +146,29 @@ int stark_deep_fri_prove_batch(stark_ctx *ctx, const uint64_t *f0s, size_t n_pro
                                const stark_fri_params *params, const uint8_t *public_inputs_hashes,
                                uint8_t **proofs, size_t *proof_lens);
 
+/* deep_ali_merge_sha256 (crates/deep_ali/src/lib.rs:453-545): lde is the
+ * 756 x n0 SHA-256 AIR LDE, coeffs the 766 combination coefficients (only
+ * lambda_j = j+1, prover.rs:100-102, is accepted). */
+int stark_merge_sha256(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_blocks,
+                       const uint64_t *coeffs, size_t k, int phi_mode, uint64_t *c_eval);
+
+/* prove_ds_ksk_binding (crates/swarm-dns/src/prover.rs:512-585): SHA-256
+ * trace of dnskey -> LDE x32 -> SHA-256 merge -> STIR proof bound to
+ * fs_binding -> ds_ksk_pi_hash. The reference's local self-verify is not
+ * run (the proof bytes are the reference's). */
+int stark_prove_ds_ksk_binding(stark_ctx *ctx, const uint8_t *dnskey, size_t len, const uint8_t parent_ds_hash[32],
+                               const uint8_t fs_binding[32], int ldt, int phi_mode, stark_ds_ksk_out *out);
+
+/* Batched prove_ds_ksk_binding: key i is dnskeys[offsets[i] .. offsets[i] +
+ * lens[i]), with parent_ds_hashes / fs_bindings 32 bytes per key; keys of one
+ * trace height are proven as one lockstep device batch. outs[i] receives
+ * exactly what stark_prove_ds_ksk_binding returns for key i alone. No
+ * counterpart in the reference: it replaces a loop of prove_ds_ksk_binding
+ * calls (crates/swarm-dns/src/prover.rs:512). */
+int stark_prove_ds_ksk_binding_batch(stark_ctx *ctx, const uint8_t *dnskeys, const size_t *offsets, const size_t *lens,
+                                     size_t n_keys, const uint8_t *parent_ds_hashes, const uint8_t *fs_bindings,
+                                     int ldt, int phi_mode, stark_ds_ksk_out *outs);
+
 /* prove_outer_rollup (crates/swarm-dns/src/prover.rs:253-304). */
 int stark_prove_outer_rollup(stark_ctx *ctx, const uint8_t *pi_hashes, size_t n, const uint8_t pk_hash[32], int ldt,
                              int phi_mode, uint8_t **proof, size_t *proof_len, uint8_t root_f0[32]);
@@ -175,6 +209,11 @@ size_t stark_profile_names(stark_ctx *ctx, char *buf, size_t cap);
 void stark_host_sha3_256(const uint8_t *msg, size_t len, uint8_t out[32]);
 uint64_t stark_host_stdrng_first_u64(uint64_t seed);
 /* DnsRecord::leaf_hash (dns.rs:72-89) */
+/* build_sha256_trace_multi (crates/deep_ali/src/sha256_air.rs:429-479):
+ * trace (756 x stark_host_sha256_trace_height(len), column-major) of msg;
+ * returns n_blocks and the digest read from the trace (prover.rs:529-538). */
+size_t stark_host_sha256_trace_height(size_t len);
+size_t stark_host_build_sha256_trace(const uint8_t *msg, size_t len, uint64_t *trace, uint8_t digest[32]);
 void stark_host_dns_leaf_hash(const stark_dns_record *rec, const uint8_t salt[16], uint8_t out[32]);
 
 #ifdef __cplusplus

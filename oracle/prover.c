@@ -75,7 +75,7 @@ void ref_build_hash_rollup_trace(size_t n_trace, const uint64_t *leaves, size_t 
 }
 
 /* c = FFT(pad(poly_div_zh(iFFT(phi), n_trace)))  -- remainder dropped */
-static void quotient_from_phi(uint64_t *phi, size_t n, size_t n_trace) {
+void ref_quotient_from_phi(uint64_t *phi, size_t n, size_t n_trace) {
     ref_ntt_inplace(phi, n, 1);
     size_t m = n_trace;
     if (n <= m) {
@@ -109,7 +109,7 @@ int ref_merge_hash_rollup(const uint64_t *lde, size_t n_trace, size_t blowup, in
     } else {
         memset(c_eval, 0, 8 * n);
     }
-    quotient_from_phi(c_eval, n, n_trace);
+    ref_quotient_from_phi(c_eval, n, n_trace);
     return 0;
 }
 
@@ -219,7 +219,7 @@ void ref_shard_fs_binding(const uint8_t pk_hash[32], const uint8_t job_id[32], u
 #define SWARM_QUERIES 54
 #define SWARM_SEED_Z 0xDEEFBAADULL
 
-static size_t make_schedule(size_t n0, size_t *s) {
+size_t ref_make_schedule(size_t n0, size_t *s) {
     int lg = 0;
     while (((size_t)1 << lg) < n0) lg++;
     size_t k = 0;
@@ -243,7 +243,7 @@ static int rollup_prove(const uint64_t *leaves, size_t n_active, const uint8_t b
     size_t sched[24];
     ref_fri_params p;
     p.schedule = sched;
-    p.n_layers = make_schedule(n0, sched);
+    p.n_layers = ref_make_schedule(n0, sched);
     p.r = SWARM_QUERIES; p.s0 = SWARM_QUERIES; p.seed_z = SWARM_SEED_Z;
     p.coeff_commit_final = 1; p.d_final = 1; p.stir = 1;
     p.public_inputs_hash = binding;

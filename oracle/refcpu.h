@@ -81,6 +81,26 @@ int ref_prove_inner_shard(const uint8_t salt[16], const uint8_t *leaf_hashes, si
 int ref_prove_outer_rollup(const uint8_t *pi_hashes, size_t count, const uint8_t pk_hash[32], int phi_mode,
                            uint8_t **proof, size_t *proof_len, uint8_t root_f0[32]);
 
+/* c <- FFT(pad(poly_div_zh(iFFT(c), n_trace))), remainder dropped (lib.rs:190-228, 421-429) */
+void ref_quotient_from_phi(uint64_t *phi, size_t n, size_t n_trace);
+/* make_schedule(n0, Stir): 8s then the 2^(log n0 mod 3) residual; returns the layer count */
+size_t ref_make_schedule(size_t n0, size_t *s);
+
+/* ---- SHA-256 AIR and prove_ds_ksk_binding (sha256_air.rs, lib.rs:453-545, prover.rs:446-585) ---- */
+size_t ref_sha256_n_blocks(size_t len);
+size_t ref_sha256_trace_height(size_t len);
+/* 756 x height column-major trace of the padded message; returns n_blocks and
+ * the digest read from the trace (H-state at block row 65 of the last block). */
+int ref_build_sha256_trace(const uint8_t *msg, size_t len, uint64_t *tr, uint8_t digest[32]);
+/* the 766 constraint values at one row (cur, nxt: 756 values each) */
+void ref_sha256_constraints(const uint64_t *cur, const uint64_t *nxt, size_t row, size_t n_blocks, uint64_t *out);
+int ref_merge_sha256(const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_blocks, int phi_mode,
+                     uint64_t *c_eval);
+int ref_prove_ds_ksk_binding(const uint8_t *dnskey, size_t len, const uint8_t parent_ds_hash[32],
+                             const uint8_t fs_binding[32], int phi_mode, uint8_t **proof, size_t *proof_len,
+                             uint8_t pi_hash[32], uint8_t asserted_digest[32], uint8_t root_f0[32],
+                             size_t *n_blocks_out, size_t *n_trace_out);
+
 /* ---- prover-core microbench (coset LDE -> row Merkle -> STIR prove) ---- */
 /* trace: w x n_trace column-major; returns proof bytes of deep_fri_prove over
  * f0 = sum_c (c+1) * LDE_c with the row-tree root as public_inputs_hash, and

@@ -11,6 +11,9 @@ Rust functions they replace:
   DeepFriParams, FriDomain crates/deep_ali/src/fri.rs:160-175,1303-1349
   make_schedule, build_params, prove_inner_shard, prove_outer_rollup
                            crates/swarm-dns/src/prover.rs:53-304
+  prove_ds_ksk_binding     crates/swarm-dns/src/prover.rs:446-585
+  build_sha256_trace_multi crates/deep_ali/src/sha256_air.rs:429-479
+  deep_ali_merge_sha256    crates/deep_ali/src/lib.rs:453-545
   DnsRecord                crates/swarm-dns/src/dns.rs:24-89
   shard_fs_binding         crates/swarm-proto/src/messages.rs:162-176
 
@@ -29,6 +32,7 @@ from .api import (  # noqa: F401
     LdtMode,
     OuterRollupOutput,
     PackedShards,
+    DsKskOutput,
     StarkError,
     BLOWUP,
     NUM_QUERIES,
@@ -38,6 +42,8 @@ from .api import (  # noqa: F401
     PHI_SEQUENTIAL,
     build_params,
     deep_ali_merge_general,
+    deep_ali_merge_sha256,
+    build_sha256_trace_multi,
     deep_fri_prove,
     deep_fri_prove_batch,
     default_context,
@@ -49,6 +55,8 @@ from .api import (  # noqa: F401
     prove_inner_shard,
     prove_inner_shard_batch,
     prove_outer_rollup,
+    prove_ds_ksk_binding,
+    prove_ds_ksk_binding_batch,
     prover_core,
     shard_fs_binding,
 )

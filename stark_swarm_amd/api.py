@@ -48,6 +48,12 @@ class _InnerOut(ctypes.Structure):
                 ("proof", u8p), ("proof_len", sz)]
 
 
+class _DsKskOut(ctypes.Structure):
+    _fields_ = [("pi_hash", ctypes.c_uint8 * 32), ("asserted_digest", ctypes.c_uint8 * 32),
+                ("root_f0", ctypes.c_uint8 * 32), ("n_blocks", sz), ("n_trace", sz),
+                ("proof", u8p), ("proof_len", sz)]
+
+
 _lib = None
 
 
@@ -99,6 +105,15 @@ def lib():
         "stark_host_sha3_256": (None, [ctypes.c_char_p, sz, ctypes.c_char_p]),
         "stark_host_stdrng_first_u64": (ctypes.c_uint64, [ctypes.c_uint64]),
         "stark_host_dns_leaf_hash": (None, [ctypes.POINTER(_DnsRecord), ctypes.c_char_p, ctypes.c_char_p]),
+        "stark_merge_sha256": (ctypes.c_int, [vp, u64p, sz, sz, sz, u64p, sz, ctypes.c_int, u64p]),
+        "stark_prove_ds_ksk_binding": (ctypes.c_int, [vp, ctypes.c_char_p, sz, ctypes.c_char_p, ctypes.c_char_p,
+                                                      ctypes.c_int, ctypes.c_int, ctypes.POINTER(_DsKskOut)]),
+        "stark_prove_ds_ksk_binding_batch": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.POINTER(sz),
+                                                            ctypes.POINTER(sz), sz, ctypes.c_char_p,
+                                                            ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+                                                            ctypes.POINTER(_DsKskOut)]),
+        "stark_host_sha256_trace_height": (sz, [sz]),
+        "stark_host_build_sha256_trace": (sz, [ctypes.c_char_p, sz, u64p, ctypes.c_char_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -302,6 +317,42 @@ def deep_ali_merge_general(trace_evals_on_lde, combination_coeffs, air, omega, n
     return out, CompositionInfo(phi_b, qb, qb / n, k, max_deg, w)
 
 
+SHA256_WIDTH, SHA256_CONSTRAINTS = 756, 766
+
+
+def build_sha256_trace_multi(message):
+    """build_sha256_trace_multi (sha256_air.rs:429-479), host side: returns
+    (trace as a 756 x n_trace uint64 array, n_blocks)."""
+    message = bytes(message)
+    h = lib().stark_host_sha256_trace_height(len(message))
+    tr = np.zeros((SHA256_WIDTH, h), dtype=np.uint64)
+    dg = ctypes.create_string_buffer(32)
+    nb = lib().stark_host_build_sha256_trace(message, len(message), tr.ctypes.data_as(u64p), dg)
+    return tr, nb
+
+
+def deep_ali_merge_sha256(trace_evals_on_lde, combination_coeffs, omega, n_trace, blowup, n_blocks,
+                          phi_mode=PHI_RAYON, ctx=None):
+    """deep_ali_merge_sha256 (lib.rs:453-545)."""
+    n = n_trace * blowup
+    lde = np.asarray(trace_evals_on_lde, dtype=np.uint64)
+    assert lde.shape[0] == SHA256_WIDTH, "trace width mismatch"
+    assert len(combination_coeffs) == SHA256_CONSTRAINTS, \
+        "need one combination coefficient per constraint, got %d for %d constraints" % (
+            len(combination_coeffs), SHA256_CONSTRAINTS)
+    assert lde.shape[1] == n, "trace column length mismatch"
+    del omega  # unused by the reference too (lib.rs:466)
+    ctx = ctx or default_context()
+    lde, lp = _u64(lde)
+    co, cp = _u64(np.asarray(combination_coeffs, dtype=np.uint64))
+    out = np.empty(n, dtype=np.uint64)
+    ctx._check(lib().stark_merge_sha256(ctx.h, lp, n_trace, blowup, n_blocks, cp, SHA256_CONSTRAINTS, phi_mode,
+                                        out.ctypes.data_as(u64p)))
+    phi_b = 2 * n_trace
+    qb = phi_b - n_trace
+    return out, CompositionInfo(phi_b, qb, qb / n, SHA256_CONSTRAINTS, 2, SHA256_WIDTH)
+
+
 # --------------------------------------------------------------------- FRI
 @dataclass
 class FriDomain:
@@ -450,6 +501,55 @@ def prove_inner_shard(salt, records: Sequence[DnsRecord], fs_binding_32, ldt=Ldt
     blob = _take(out.proof, out.proof_len)
     return InnerShardOutput(bytes(out.pi_hash), bytes(out.merkle_root), out.record_count, out.n_trace, len(blob),
                             bytes(out.root_f0), blob)
+
+
+@dataclass
+class DsKskOutput:
+    """DsKskOutput (prover.rs:464-490); prove_ms / local_verify_ms are not kept."""
+    pi_hash: bytes
+    asserted_digest: bytes
+    n_blocks: int
+    n_trace: int
+    proof_bytes: int
+    root_f0: bytes
+    proof_blob: bytes
+
+
+def _ds_ksk_out(o):
+    blob = _take(o.proof, o.proof_len)
+    return DsKskOutput(bytes(o.pi_hash), bytes(o.asserted_digest), o.n_blocks, o.n_trace, len(blob),
+                       bytes(o.root_f0), blob)
+
+
+def prove_ds_ksk_binding(dnskey_bytes, parent_ds_hash, fs_binding_32, ldt=LdtMode.Stir, phi_mode=PHI_RAYON,
+                         ctx=None):
+    """prove_ds_ksk_binding (prover.rs:512-585)."""
+    ctx = ctx or default_context()
+    out = _DsKskOut()
+    key = bytes(dnskey_bytes)
+    ctx._check(lib().stark_prove_ds_ksk_binding(ctx.h, key, len(key), bytes(parent_ds_hash), bytes(fs_binding_32),
+                                                ldt.value, phi_mode, ctypes.byref(out)))
+    return _ds_ksk_out(out)
+
+
+def prove_ds_ksk_binding_batch(dnskeys, parent_ds_hashes, fs_bindings, ldt=LdtMode.Stir, phi_mode=PHI_RAYON,
+                               ctx=None):
+    """Many prove_ds_ksk_binding calls as lockstep device batches
+    (stark_prove_ds_ksk_binding_batch); result k equals the single call on key k."""
+    ctx = ctx or default_context()
+    keys = [bytes(k) for k in dnskeys]
+    n = len(keys)
+    assert len(parent_ds_hashes) == n and len(fs_bindings) == n
+    offs, acc = [], 0
+    for k in keys:
+        offs.append(acc)
+        acc += len(k)
+    outs = (_DsKskOut * max(1, n))()
+    ctx._check(lib().stark_prove_ds_ksk_binding_batch(
+        ctx.h, b"".join(keys), (sz * max(1, n))(*offs), (sz * max(1, n))(*[len(k) for k in keys]), n,
+        b"".join(bytes(p) for p in parent_ds_hashes), b"".join(bytes(f) for f in fs_bindings), ldt.value, phi_mode,
+        outs))
+    return [_ds_ksk_out(outs[k]) for k in range(n)]
 
 
 class PackedShards:

@@ -79,22 +79,27 @@ void merge_hash_rollup_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n
         fill_u64(c, c_eval, n * K, 0);
         return;
     }
-    Scoped phi(c, 8 * n * K), coef(c, 8 * n * K), scr(c, 8 * n * K);
+    Scoped phi(c, 8 * n * K);
     {
         ProfScope ps(c, "compose_hash_rollup", (8.0 * 4 * n + 8.0 * n) * K);
         hipLaunchKernelGGL(compose_hash_rollup_k, dim3((uint32_t)((n + AT - 1) / AT), K), dim3(AT), 0, c->stream, lde,
                            n, blowup, phi.as<uint64_t>());
         SG_HIP(hipGetLastError());
     }
+    quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
+}
+
+void quotient_batch(Ctx *c, uint64_t *phi, uint32_t K, uint64_t n_trace, uint64_t n, uint64_t *c_eval) {
+    Scoped coef(c, 8 * n * K), scr(c, 8 * n * K);
     const uint32_t L = ilog2(n);
-    ntt_columns(c, phi.as<uint64_t>(), coef.as<uint64_t>(), scr.as<uint64_t>(), L, K, n, n, true, nullptr);
+    ntt_columns(c, phi, coef.as<uint64_t>(), scr.as<uint64_t>(), L, K, n, n, true, nullptr);
     {
         ProfScope ps(c, "zh_div", 16.0 * n * K);
         hipLaunchKernelGGL(zh_div_k, dim3((uint32_t)((n_trace + AT - 1) / AT), K), dim3(AT), 0, c->stream,
-                           coef.as<uint64_t>(), n, n_trace, phi.as<uint64_t>());
+                           coef.as<uint64_t>(), n, n_trace, phi);
         SG_HIP(hipGetLastError());
     }
-    ntt_columns(c, phi.as<uint64_t>(), c_eval, scr.as<uint64_t>(), L, K, n, n, false, nullptr);
+    ntt_columns(c, phi, c_eval, scr.as<uint64_t>(), L, K, n, n, false, nullptr);
 }
 
 void merge_hash_rollup_dev(Ctx *c, const uint64_t *lde, uint64_t n_trace, uint64_t blowup, int phi_mode,

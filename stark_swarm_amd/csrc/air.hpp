@@ -8,6 +8,8 @@ namespace sg {
 
 void fill_u64(Ctx *c, uint64_t *p, uint64_t n, uint64_t v);
 bool phi_computed(int phi_mode, uint64_t n);
+// K proofs' phi (device, n values each, overwritten) -> c_eval = NTT(pad(poly_div_zh(iNTT(phi), n_trace)))
+void quotient_batch(Ctx *c, uint64_t *phi, uint32_t K, uint64_t n_trace, uint64_t n, uint64_t *c_eval);
 // c_eval (device, n_trace*blowup) from the 4-column HashRollup LDE (device, column-major)
 void merge_hash_rollup_dev(Ctx *c, const uint64_t *lde, uint64_t n_trace, uint64_t blowup, int phi_mode,
                            uint64_t *c_eval);

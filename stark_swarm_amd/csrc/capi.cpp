@@ -15,6 +15,7 @@
 #include "keccak.hpp"
 #include "ntt.hpp"
 #include "prover.hpp"
+#include "sha256_air.hpp"
 
 struct stark_ctx {
     sg::Ctx *c = nullptr;
@@ -215,6 +216,67 @@ int stark_prove_inner_shard_batch(stark_ctx *ctx, const uint8_t salt[16], const 
     });
 }
 
+int stark_merge_sha256(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_blocks,
+                       const uint64_t *coeffs, size_t k, int phi_mode, uint64_t *c_eval) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(lde && coeffs && c_eval, "merge_sha256: null argument");
+        SG_REQUIRE(k == sg::SHA256_K, "merge_sha256: need one combination coefficient per constraint (766)");
+        for (size_t j = 0; j < k; j++)
+            SG_REQUIRE(coeffs[j] == j + 1, "merge_sha256: only the reference combination coefficients (j + 1)");
+        SG_REQUIRE(phi_mode >= 0 && phi_mode <= 2, "merge_sha256: bad phi_mode");
+        SG_REQUIRE(n_blocks >= 1 && n_blocks <= 0xFFFFFFFFu, "merge_sha256: bad n_blocks");
+        const size_t n0 = n_trace * blowup;
+        sg::Scoped dl(c, 8ULL * sg::SHA256_W * n0), dc(c, 8 * n0);
+        h2d(c, dl.p, lde, 8ULL * sg::SHA256_W * n0);
+        sg::merge_sha256_batch(c, dl.as<uint64_t>(), 1, n_trace, blowup, {(uint32_t)n_blocks}, phi_mode,
+                               dc.as<uint64_t>());
+        d2h(c, c_eval, dc.p, 8 * n0);
+    });
+}
+
+static void fill_ds_ksk(const sg::DsKsk &r, stark_ds_ksk_out *o) {
+    memcpy(o->pi_hash, r.pi_hash, 32);
+    memcpy(o->asserted_digest, r.asserted_digest, 32);
+    memcpy(o->root_f0, r.root_f0, 32);
+    o->n_blocks = r.n_blocks;
+    o->n_trace = r.n_trace;
+    o->proof = copy_out(r.proof, &o->proof_len);
+}
+
+int stark_prove_ds_ksk_binding(stark_ctx *ctx, const uint8_t *dnskey, size_t len, const uint8_t parent_ds_hash[32],
+                               const uint8_t fs_binding[32], int ldt, int phi_mode, stark_ds_ksk_out *out) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE((dnskey || len == 0) && parent_ds_hash && fs_binding && out, "prove_ds_ksk_binding: null argument");
+        if (ldt != STARK_LDT_STIR) throw sg::Error(sg::E_UNSUPPORTED, "only LdtMode::Stir is implemented");
+        out->proof = nullptr;
+        std::array<uint8_t, 32> par, fsb;
+        memcpy(par.data(), parent_ds_hash, 32);
+        memcpy(fsb.data(), fs_binding, 32);
+        std::vector<sg::DsKsk> r = sg::prove_ds_ksk_binding_batch(c, {{dnskey, len}}, {par}, {fsb}, phi_mode);
+        fill_ds_ksk(r[0], out);
+    });
+}
+
+int stark_prove_ds_ksk_binding_batch(stark_ctx *ctx, const uint8_t *dnskeys, const size_t *offsets, const size_t *lens,
+                                     size_t n_keys, const uint8_t *parent_ds_hashes, const uint8_t *fs_bindings,
+                                     int ldt, int phi_mode, stark_ds_ksk_out *outs) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(offsets && lens && parent_ds_hashes && fs_bindings && outs, "prove_ds_ksk_binding_batch: null argument");
+        if (ldt != STARK_LDT_STIR) throw sg::Error(sg::E_UNSUPPORTED, "only LdtMode::Stir is implemented");
+        std::vector<sg::ByteSpan> keys(n_keys);
+        std::vector<std::array<uint8_t, 32>> par(n_keys), fsb(n_keys);
+        for (size_t i = 0; i < n_keys; i++) {
+            SG_REQUIRE(dnskeys || lens[i] == 0, "prove_ds_ksk_binding_batch: null keys");
+            keys[i] = {dnskeys ? dnskeys + offsets[i] : nullptr, lens[i]};
+            memcpy(par[i].data(), parent_ds_hashes + 32 * i, 32);
+            memcpy(fsb[i].data(), fs_bindings + 32 * i, 32);
+            outs[i].proof = nullptr;
+        }
+        std::vector<sg::DsKsk> r = sg::prove_ds_ksk_binding_batch(c, keys, par, fsb, phi_mode);
+        for (size_t i = 0; i < n_keys; i++) fill_ds_ksk(r[i], &outs[i]);
+    });
+}
+
 int stark_deep_fri_prove_batch(stark_ctx *ctx, const uint64_t *f0s, size_t n_proofs, size_t n0,
                                const stark_fri_params *params, const uint8_t *public_inputs_hashes,
                                uint8_t **proofs, size_t *proof_lens) {
@@ -348,6 +410,12 @@ size_t stark_profile_names(stark_ctx *ctx, char *buf, size_t cap) {
 void stark_host_sha3_256(const uint8_t *msg, size_t len, uint8_t out[32]) { sg::sha3_256_host(msg, len, out); }
 
 uint64_t stark_host_stdrng_first_u64(uint64_t seed) { return sg::stdrng_first_u64(seed); }
+
+size_t stark_host_sha256_trace_height(size_t len) { return sg::sha256_trace_height(len); }
+
+size_t stark_host_build_sha256_trace(const uint8_t *msg, size_t len, uint64_t *trace, uint8_t digest[32]) {
+    return sg::build_sha256_trace(msg, len, trace, digest);
+}
 
 void stark_host_dns_leaf_hash(const stark_dns_record *r, const uint8_t salt[16], uint8_t out[32]) {
     sg::DnsRec d{r->domain, r->domain_len, r->record_type, r->ttl, r->rdata, r->rdata_len};

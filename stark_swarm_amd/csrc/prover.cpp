@@ -12,6 +12,7 @@
 
 #include <string.h>
 
+#include <algorithm>
 #include <atomic>
 #include <map>
 #include <thread>
@@ -21,6 +22,7 @@
 #include "gl.hpp"
 #include "keccak.hpp"
 #include "ntt.hpp"
+#include "sha256_air.hpp"
 
 namespace sg {
 
@@ -237,6 +239,84 @@ std::vector<InnerShard> prove_inner_shard_batch(Ctx *c, const uint8_t salt[16],
         h.update(o.root_f0, 32);
         h.update(fs_bindings[k].data(), 32);
         h.final(o.pi_hash);
+    }
+    return out;
+}
+
+// ds_ksk_pi_hash (prover.rs:446-462)
+static void ds_ksk_pi_hash(const ByteSpan &key, const uint8_t parent[32], const uint8_t digest[32],
+                           const uint8_t root_f0[32], const uint8_t fsb[32], uint8_t out[32]) {
+    Sha3 h;
+    uint8_t le[8];
+    for (int i = 0; i < 8; i++) le[i] = (uint8_t)((uint64_t)key.n >> (8 * i));
+    h.update("DS-KSK-PIHASH-V1");
+    h.update(le, 8);
+    h.update(key.p, key.n);
+    h.update(parent, 32);
+    h.update(digest, 32);
+    h.update(root_f0, 32);
+    h.update(fsb, 32);
+    h.final(out);
+}
+
+std::vector<DsKsk> prove_ds_ksk_binding_batch(Ctx *c, const std::vector<ByteSpan> &keys,
+                                              const std::vector<std::array<uint8_t, 32>> &parents,
+                                              const std::vector<std::array<uint8_t, 32>> &fsbs, int phi_mode) {
+    const size_t K = keys.size();
+    SG_REQUIRE(parents.size() == K && fsbs.size() == K, "prove_ds_ksk_binding: one parent hash and binding per key");
+    std::vector<DsKsk> out(K);
+    std::map<uint64_t, std::vector<size_t>> groups;  // trace height -> keys
+    for (size_t k = 0; k < K; k++) {
+        SG_REQUIRE(keys[k].p || keys[k].n == 0, "prove_ds_ksk_binding: null key");
+        const uint64_t h = sha256_trace_height(keys[k].n);
+        SG_REQUIRE(h * SWARM_BLOWUP <= (1ULL << 24), "prove_ds_ksk_binding: key too long (domain above 2^24)");
+        groups[h].push_back(k);
+    }
+    for (auto &g : groups) {
+        const uint64_t h = g.first, n0 = h * SWARM_BLOWUP;
+        // lockstep chunks bounded by HBM: trace + LDE + merge buffers per proof
+        const uint64_t per = 8ULL * SHA256_W * (h + n0) + 8ULL * 8 * n0;
+        const size_t chunk = (size_t)std::max<uint64_t>(1, std::min<uint64_t>(4096, (24ULL << 30) / per));
+        for (size_t c0 = 0; c0 < g.second.size(); c0 += chunk) {
+            const std::vector<size_t> ids(g.second.begin() + c0,
+                                          g.second.begin() + std::min(g.second.size(), c0 + chunk));
+            const uint32_t G = (uint32_t)ids.size();
+            std::vector<uint64_t> tr((size_t)SHA256_W * h * G);
+            std::vector<uint32_t> nb(G);
+            {
+                HostScope hs(c, "sha256_trace");
+                parallel_for(G, [&](size_t j) {
+                    DsKsk &o = out[ids[j]];
+                    nb[j] = (uint32_t)build_sha256_trace(keys[ids[j]].p, keys[ids[j]].n,
+                                                         &tr[(size_t)SHA256_W * h * j], o.asserted_digest);
+                    o.n_blocks = nb[j];
+                    o.n_trace = h;
+                });
+            }
+            Scoped d_tr(c, 8 * tr.size()), d_lde(c, 8ULL * SHA256_W * n0 * G), d_c(c, 8ULL * n0 * G);
+            SG_HIP(hipMemcpyAsync(d_tr.p, tr.data(), 8 * tr.size(), hipMemcpyHostToDevice, c->stream));
+            lde_columns(c, d_tr.as<uint64_t>(), d_lde.as<uint64_t>(), ilog2(h), ilog2(SWARM_BLOWUP),
+                        (uint64_t)SHA256_W * G, h, n0, 1);
+            merge_sha256_batch(c, d_lde.as<uint64_t>(), G, h, SWARM_BLOWUP, nb, phi_mode, d_c.as<uint64_t>());
+            FriParams p;
+            p.schedule = stir_schedule(n0);
+            p.r = p.s0 = SWARM_QUERIES;
+            p.seed_z = SWARM_SEED_Z;
+            p.coeff_commit_final = true;
+            p.d_final = 1;
+            p.stir = true;
+            p.has_pi = true;
+            std::vector<std::array<uint8_t, 32>> pis(G);
+            for (uint32_t j = 0; j < G; j++) pis[j] = fsbs[ids[j]];
+            std::vector<std::vector<uint8_t>> proofs = deep_fri_prove_batch(c, d_c.as<uint64_t>(), G, n0, p, &pis, nullptr);
+            for (uint32_t j = 0; j < G; j++) {
+                DsKsk &o = out[ids[j]];
+                o.proof = std::move(proofs[j]);
+                memcpy(o.root_f0, o.proof.data(), 32);
+                ds_ksk_pi_hash(keys[ids[j]], parents[ids[j]].data(), o.asserted_digest, o.root_f0,
+                               fsbs[ids[j]].data(), o.pi_hash);
+            }
+        }
     }
     return out;
 }

@@ -25,6 +25,17 @@ struct InnerShard {
     std::vector<uint8_t> proof;
 };
 
+// DsKskOutput (crates/swarm-dns/src/prover.rs:464-490)
+struct DsKsk {
+    uint8_t pi_hash[32], asserted_digest[32], root_f0[32];
+    size_t n_blocks = 0, n_trace = 0;
+    std::vector<uint8_t> proof;
+};
+struct ByteSpan {
+    const uint8_t *p;
+    size_t n;
+};
+
 std::vector<uint8_t> dns_canonical(const DnsRec &r);
 size_t dns_canonical_to(const DnsRec &r, uint8_t *dst);  // returns the length written
 void dns_leaf_hash(const DnsRec &r, const uint8_t salt[16], uint8_t out[32]);
@@ -41,6 +52,11 @@ std::vector<InnerShard> prove_inner_shard_batch(Ctx *c, const uint8_t salt[16],
 std::vector<std::vector<uint8_t>> rollup_prove_batch(Ctx *c, const std::vector<std::vector<uint64_t>> &leaves,
                                                      const std::vector<std::array<uint8_t, 32>> &bindings,
                                                      int phi_mode, std::vector<std::array<uint8_t, 32>> &root_f0);
+// prove_ds_ksk_binding (prover.rs:512-585) for K keys: lockstep device
+// batches per trace height; result k equals the single-key proof
+std::vector<DsKsk> prove_ds_ksk_binding_batch(Ctx *c, const std::vector<ByteSpan> &dnskeys,
+                                              const std::vector<std::array<uint8_t, 32>> &parent_ds_hashes,
+                                              const std::vector<std::array<uint8_t, 32>> &fs_bindings, int phi_mode);
 std::vector<uint8_t> prove_outer_rollup(Ctx *c, const uint8_t *pi_hashes, size_t n, const uint8_t pk_hash[32],
                                         int phi_mode, uint8_t root_f0[32]);
 std::vector<uint8_t> prover_core_dev(Ctx *c, const uint64_t *d_trace, uint64_t w, uint64_t n_trace, uint64_t blowup,

@@ -1,0 +1,361 @@
+// sha256_air.hip -- the SHA-256 AIR (756 columns, 766 transition
+// constraints) for prove_ds_ksk_binding: the host trace builder and the
+// composition kernel on gfx950.
+//
+//   column layout          crates/deep_ali/src/sha256_air.rs:140-253
+//   trace builder          sha256_air.rs:392-479, 648-898
+//   constraint evaluator   sha256_air.rs:956-1294
+//   merge                  crates/deep_ali/src/lib.rs:453-545
+//
+// The composition runs one LDE point per lane: the lane reads its row and
+// the row `blowup` points ahead straight from the column-major LDE (adjacent
+// lanes read adjacent addresses of every column) and folds the 766
+// constraints into phi = sum_j (j+1) C_j as it goes; constraints a row's
+// phase switches off are skipped, not evaluated.
+#include <string.h>
+
+#include "air.hpp"
+#include "gl.hpp"
+#include "ntt.hpp"
+#include "sha256_air.hpp"
+
+namespace sg {
+
+namespace sha {
+
+enum : uint32_t {
+    A_A = 0, A_D = 3, A_E = 4, A_H = 7,
+    BIT_A = 8, BIT_B = 40, BIT_C = 72, BIT_E = 104, BIT_F = 136, BIT_G = 168,
+    S0_T = 200, S0_BIT = 232, S0_W = 264,
+    S1_T = 265, S1_BIT = 297, S1_W = 329,
+    CH_BIT = 330, CH_W = 362,
+    MAJ_AB = 363, MAJ_AC = 395, MAJ_BC = 427, MAJ_U = 459, MAJ_BIT = 491, MAJ_W = 523,
+    COL_W = 524,
+    T1_LO = 525, T1_C0 = 526, T1_C1 = 527, T1_C2 = 528, T1_CW = 529,
+    T2_LO = 530, T2_C0 = 531, T2_C1 = 532, T2_CW = 533,
+    H0 = 534, WW0 = 542, BIT_WW14 = 557, BIT_WW1 = 589,
+    SS0_T = 621, SS0_BIT = 653, SS0_W = 685,
+    SS1_T = 686, SS1_BIT = 718, SS1_W = 750,
+    WS_C0 = 751, WS_C1 = 752, WS_CW = 753, NA_C = 754, NE_C = 755,
+};
+
+#define SG_SHA_K                                                                                                 \
+    0x428a2f98u, 0x71374491u, 0xb5c0fbcfu, 0xe9b5dba5u, 0x3956c25bu, 0x59f111f1u, 0x923f82a4u, 0xab1c5ed5u,      \
+        0xd807aa98u, 0x12835b01u, 0x243185beu, 0x550c7dc3u, 0x72be5d74u, 0x80deb1feu, 0x9bdc06a7u, 0xc19bf174u,  \
+        0xe49b69c1u, 0xefbe4786u, 0x0fc19dc6u, 0x240ca1ccu, 0x2de92c6fu, 0x4a7484aau, 0x5cb0a9dcu, 0x76f988dau,  \
+        0x983e5152u, 0xa831c66du, 0xb00327c8u, 0xbf597fc7u, 0xc6e00bf3u, 0xd5a79147u, 0x06ca6351u, 0x14292967u,  \
+        0x27b70a85u, 0x2e1b2138u, 0x4d2c6dfcu, 0x53380d13u, 0x650a7354u, 0x766a0abbu, 0x81c2c92eu, 0x92722c85u,  \
+        0xa2bfe8a1u, 0xa81a664bu, 0xc24b8b70u, 0xc76c51a3u, 0xd192e819u, 0xd6990624u, 0xf40e3585u, 0x106aa070u,  \
+        0x19a4c116u, 0x1e376c08u, 0x2748774cu, 0x34b0bcb5u, 0x391c0cb3u, 0x4ed8aa4au, 0x5b9cca4fu, 0x682e6ff3u,  \
+        0x748f82eeu, 0x78a5636fu, 0x84c87814u, 0x8cc70208u, 0x90befffau, 0xa4506cebu, 0xbef9a3f7u, 0xc67178f2u
+
+static const uint32_t KH[64] = {SG_SHA_K};     // FIPS 180-4 section 4.2.2
+__constant__ uint32_t KD[64] = {SG_SHA_K};
+static const uint32_t IV[8] = {0x6a09e667u, 0xbb67ae85u, 0x3c6ef372u, 0xa54ff53au,
+                               0x510e527fu, 0x9b05688cu, 0x1f83d9abu, 0x5be0cd19u};
+
+inline uint32_t rotr(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+inline uint32_t big0(uint32_t x) { return rotr(x, 2) ^ rotr(x, 13) ^ rotr(x, 22); }
+inline uint32_t big1(uint32_t x) { return rotr(x, 6) ^ rotr(x, 11) ^ rotr(x, 25); }
+inline uint32_t small0(uint32_t x) { return rotr(x, 7) ^ rotr(x, 18) ^ (x >> 3); }
+inline uint32_t small1(uint32_t x) { return rotr(x, 17) ^ rotr(x, 19) ^ (x >> 10); }
+inline uint32_t chf(uint32_t e, uint32_t f, uint32_t g) { return (e & f) ^ (~e & g); }
+inline uint32_t majf(uint32_t a, uint32_t b, uint32_t c) { return (a & b) ^ (a & c) ^ (b & c); }
+
+// One 128-row block segment (sha256_air.rs:648-898): rows 0..63 are the
+// rounds (state before round r), row 64 the post-compression state, rows
+// 65..127 idle with the state held; H-state is the incoming IV through row
+// 64 and the block digest from row 65. Returns the post-compression state.
+std::array<uint32_t, 8> fill_block(uint64_t *tr, uint64_t h, uint64_t base, const uint8_t *blk,
+                                   const uint32_t iv[8]) {
+    uint32_t w[64];
+    for (int i = 0; i < 16; i++)
+        w[i] = (uint32_t)blk[4 * i] << 24 | (uint32_t)blk[4 * i + 1] << 16 | (uint32_t)blk[4 * i + 2] << 8 |
+               blk[4 * i + 3];
+    for (int t = 16; t < 64; t++) w[t] = small1(w[t - 2]) + w[t - 7] + small0(w[t - 15]) + w[t - 16];
+    auto wat = [&](int t) { return (t >= 0 && t < 64) ? w[t] : 0u; };
+    std::array<std::array<uint32_t, 8>, 65> st;
+    for (int k = 0; k < 8; k++) st[0][k] = iv[k];
+    for (int t = 0; t < 64; t++) {
+        const auto &s = st[t];
+        const uint32_t t1 = s[7] + big1(s[4]) + chf(s[4], s[5], s[6]) + KH[t] + w[t];
+        const uint32_t t2 = big0(s[0]) + majf(s[0], s[1], s[2]);
+        st[t + 1] = {t1 + t2, s[0], s[1], s[2], s[3] + t1, s[4], s[5], s[6]};
+    }
+    uint32_t dg[8];
+    for (int k = 0; k < 8; k++) dg[k] = iv[k] + st[64][k];
+    for (int br = 0; br < 128; br++) {
+        uint64_t *row = tr + base + br;  // column c of this row at row[c * h]
+        auto put = [&](uint32_t col, uint64_t v) { row[(uint64_t)col * h] = v; };
+        auto put_bits = [&](uint32_t col, uint32_t x) {
+            for (int i = 0; i < 32; i++) put(col + i, (x >> i) & 1);
+        };
+        const auto &s = st[br <= 64 ? br : 64];
+        const uint32_t a = s[0], b = s[1], c = s[2], d = s[3], e = s[4], f = s[5], g = s[6], hh = s[7];
+        for (int k = 0; k < 8; k++) put(A_A + k, s[k]);
+        put_bits(BIT_A, a);
+        put_bits(BIT_B, b);
+        put_bits(BIT_C, c);
+        put_bits(BIT_E, e);
+        put_bits(BIT_F, f);
+        put_bits(BIT_G, g);
+        // rotate-xor sub-blocks: t_i = x_(i+r1) ^ x_(i+r2), s_i = t_i ^ third_i
+        auto sigma = [&](uint32_t tcol, uint32_t bcol, uint32_t wcol, uint32_t x, int r1, int r2, int r3, bool shr) {
+            uint32_t t = rotr(x, r1) ^ rotr(x, r2), third = shr ? (x >> r3) : rotr(x, r3);
+            put_bits(tcol, t);
+            put_bits(bcol, t ^ third);
+            put(wcol, t ^ third);
+        };
+        sigma(S0_T, S0_BIT, S0_W, a, 2, 13, 22, false);
+        sigma(S1_T, S1_BIT, S1_W, e, 6, 11, 25, false);
+        const uint32_t chw = chf(e, f, g), mjw = majf(a, b, c);
+        put_bits(CH_BIT, chw);
+        put(CH_W, chw);
+        put_bits(MAJ_AB, a & b);
+        put_bits(MAJ_AC, a & c);
+        put_bits(MAJ_BC, b & c);
+        put_bits(MAJ_U, (a & b) ^ (a & c));
+        put_bits(MAJ_BIT, mjw);
+        put(MAJ_W, mjw);
+        const uint32_t wt = br < 64 ? w[br] : 0, kt = br < 64 ? KH[br] : 0;
+        put(COL_W, wt);
+        const uint64_t t1 = (uint64_t)hh + big1(e) + chw + kt + wt, t2 = (uint64_t)big0(a) + mjw;
+        const uint32_t c1 = (uint32_t)(t1 >> 32), c2 = (uint32_t)(t2 >> 32);
+        put(T1_LO, (uint32_t)t1);
+        put(T1_C0, c1 & 1);
+        put(T1_C1, (c1 >> 1) & 1);
+        put(T1_C2, (c1 >> 2) & 1);
+        put(T1_CW, c1);
+        put(T2_LO, (uint32_t)t2);
+        put(T2_C0, c2 & 1);
+        put(T2_C1, (c2 >> 1) & 1);
+        put(T2_CW, c2);
+        for (int k = 0; k < 8; k++) put(H0 + k, br <= 64 ? iv[k] : dg[k]);
+        for (int k = 0; k < 15; k++) put(WW0 + k, wat(br - 15 + k));
+        const uint32_t ww14 = wat(br - 1), ww1 = wat(br - 14);
+        put_bits(BIT_WW14, ww14);
+        put_bits(BIT_WW1, ww1);
+        sigma(SS0_T, SS0_BIT, SS0_W, ww1, 7, 18, 3, true);
+        sigma(SS1_T, SS1_BIT, SS1_W, ww14, 17, 19, 10, true);
+        const uint64_t sum = (uint64_t)wat(br - 15) + small0(ww1) + wat(br - 6) + small1(ww14);
+        const uint32_t sc = (uint32_t)(sum >> 32);
+        put(WS_C0, sc & 1);
+        put(WS_C1, (sc >> 1) & 1);
+        put(WS_CW, sc);
+        put(NA_C, br < 64 ? (((uint64_t)(uint32_t)t1 + (uint32_t)t2) >> 32) : 0);
+        put(NE_C, br < 64 ? (((uint64_t)d + (uint32_t)t1) >> 32) : 0);
+    }
+    std::array<uint32_t, 8> post;
+    for (int k = 0; k < 8; k++) post[k] = st[64][k];
+    return post;
+}
+
+// ---- constraints (sha256_air.rs:956-1294), one LDE point per lane ----
+struct Phi {
+    uint64_t acc = 0, w = 1;  // w = j + 1 of the next constraint
+    __device__ __forceinline__ void put(uint64_t v) {
+        acc = add(acc, mul(w, v));
+        w++;
+    }
+    __device__ __forceinline__ void skip(int k) { w += k; }
+};
+
+__device__ __forceinline__ uint64_t xorp(uint64_t x, uint64_t y) {
+    const uint64_t p = mul(x, y);
+    return sub(add(x, y), add(p, p));
+}
+__device__ __forceinline__ uint64_t boolc(uint64_t b) { return mul(b, sub(1, b)); }
+
+constexpr int AT = 128;
+
+// grid.y = proof: the LDE of proof k is 756 columns of n at lde + 756 k n;
+// n_blocks[k] its SHA-256 block count
+__global__ __launch_bounds__(AT) void compose_sha256_k(const uint64_t *lde, uint64_t n, uint64_t blowup,
+                                                       const uint32_t *n_blocks, uint64_t *phi) {
+    const uint64_t i = (uint64_t)blockIdx.x * AT + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t *L = lde + (uint64_t)SHA256_W * n * blockIdx.y;
+    const uint64_t ni = (i + blowup) & (n - 1);
+    auto cur = [&](uint32_t col) { return L[(uint64_t)col * n + i]; };
+    auto nxt = [&](uint32_t col) { return L[(uint64_t)col * n + ni]; };
+    const uint64_t row = i / blowup, blk = row >> 7, br = row & 127;
+    const bool past = blk >= n_blocks[blockIdx.y];
+    const bool comp = !past && br < 64, fin = !past && br == 64, sched = !past && br >= 15 && br <= 62;
+    const uint64_t TWO32 = 1ULL << 32;
+    Phi f;
+    // booleanity of the 8 bit blocks
+    const uint32_t boff[8] = {BIT_A, BIT_B, BIT_C, BIT_E, BIT_F, BIT_G, BIT_WW14, BIT_WW1};
+#pragma unroll 1
+    for (int o = 0; o < 8; o++)
+#pragma unroll 4
+        for (int b = 0; b < 32; b++) f.put(boolc(cur(boff[o] + b)));
+    auto pack = [&](uint32_t off) {
+        uint64_t s = 0;
+#pragma unroll 8
+        for (int b = 0; b < 32; b++) s = add(s, mul(1ULL << b, cur(off + b)));
+        return s;
+    };
+    const uint32_t wcol[8] = {A_A, A_A + 1, A_A + 2, A_E, A_E + 1, A_E + 2, WW0 + 14, WW0 + 1};
+#pragma unroll 1
+    for (int o = 0; o < 8; o++) f.put(sub(cur(wcol[o]), pack(boff[o])));
+    // Sigma0(a), Sigma1(e), sigma0(W_win[1]), sigma1(W_win[14])
+    struct Sg {
+        uint32_t src, t, b, w;
+        int r1, r2, r3;
+        bool shr;
+    };
+    const Sg sg[4] = {{BIT_A, S0_T, S0_BIT, S0_W, 2, 13, 22, false},
+                      {BIT_E, S1_T, S1_BIT, S1_W, 6, 11, 25, false},
+                      {BIT_WW1, SS0_T, SS0_BIT, SS0_W, 7, 18, 3, true},
+                      {BIT_WW14, SS1_T, SS1_BIT, SS1_W, 17, 19, 10, true}};
+#pragma unroll 1
+    for (int s = 0; s < 4; s++) {
+        const Sg &g = sg[s];
+#pragma unroll 2
+        for (int b = 0; b < 32; b++)
+            f.put(sub(cur(g.t + b), xorp(cur(g.src + (b + g.r1) % 32), cur(g.src + (b + g.r2) % 32))));
+#pragma unroll 2
+        for (int b = 0; b < 32; b++) {
+            const uint64_t third = g.shr ? (b + g.r3 < 32 ? cur(g.src + b + g.r3) : 0) : cur(g.src + (b + g.r3) % 32);
+            f.put(sub(cur(g.b + b), xorp(cur(g.t + b), third)));
+        }
+        f.put(sub(cur(g.w), pack(g.b)));
+    }
+    // Ch(e, f, g)_i = e f + g - e g
+#pragma unroll 2
+    for (int b = 0; b < 32; b++) {
+        const uint64_t e = cur(BIT_E + b), ff = cur(BIT_F + b), g = cur(BIT_G + b);
+        f.put(sub(cur(CH_BIT + b), sub(add(mul(e, ff), g), mul(e, g))));
+    }
+    f.put(sub(cur(CH_W), pack(CH_BIT)));
+    // Maj(a, b, c) with the ab/ac/bc/u auxiliaries
+#pragma unroll 2
+    for (int b = 0; b < 32; b++) {
+        const uint64_t a = cur(BIT_A + b), bb = cur(BIT_B + b), c = cur(BIT_C + b);
+        f.put(sub(cur(MAJ_AB + b), mul(a, bb)));
+        f.put(sub(cur(MAJ_AC + b), mul(a, c)));
+        f.put(sub(cur(MAJ_BC + b), mul(bb, c)));
+    }
+#pragma unroll 2
+    for (int b = 0; b < 32; b++) f.put(sub(cur(MAJ_U + b), xorp(cur(MAJ_AB + b), cur(MAJ_AC + b))));
+#pragma unroll 2
+    for (int b = 0; b < 32; b++) f.put(sub(cur(MAJ_BIT + b), xorp(cur(MAJ_U + b), cur(MAJ_BC + b))));
+    f.put(sub(cur(MAJ_W), pack(MAJ_BIT)));
+    // T1 = h + Sigma1 + Ch + K[t] + W[t] = T1_lo + 2^32 T1_cw; T2 = Sigma0 + Maj
+    const uint64_t krow = comp ? KD[br] : 0;
+    const uint64_t t1s = add(add(add(add(cur(A_H), cur(S1_W)), cur(CH_W)), krow), cur(COL_W));
+    f.put(sub(add(cur(T1_LO), mul(TWO32, cur(T1_CW))), t1s));
+    f.put(boolc(cur(T1_C0)));
+    f.put(boolc(cur(T1_C1)));
+    f.put(boolc(cur(T1_C2)));
+    f.put(sub(sub(sub(cur(T1_CW), cur(T1_C0)), mul(2, cur(T1_C1))), mul(4, cur(T1_C2))));
+    f.put(sub(add(cur(T2_LO), mul(TWO32, cur(T2_CW))), add(cur(S0_W), cur(MAJ_W))));
+    f.put(boolc(cur(T2_C0)));
+    f.put(boolc(cur(T2_C1)));
+    f.put(sub(sub(cur(T2_CW), cur(T2_C0)), mul(2, cur(T2_C1))));
+    if (comp) {
+        // register shifts b<-a, c<-b, d<-c, f<-e, g<-f, h<-g
+        f.put(sub(nxt(A_A + 1), cur(A_A)));
+        f.put(sub(nxt(A_A + 2), cur(A_A + 1)));
+        f.put(sub(nxt(A_D), cur(A_A + 2)));
+        f.put(sub(nxt(A_E + 1), cur(A_E)));
+        f.put(sub(nxt(A_E + 2), cur(A_E + 1)));
+        f.put(sub(nxt(A_H), cur(A_E + 2)));
+        // new_a, new_e with one carry bit each (two zero slots follow)
+        f.put(sub(sub(add(nxt(A_A), mul(TWO32, cur(NA_C))), cur(T1_LO)), cur(T2_LO)));
+        f.put(boolc(cur(NA_C)));
+        f.put(sub(sub(add(nxt(A_E), mul(TWO32, cur(NE_C))), cur(A_D)), cur(T1_LO)));
+        f.put(boolc(cur(NE_C)));
+        f.skip(2);
+    } else {
+        f.skip(12);
+    }
+    // H-state: add-with-carry at finalisation, hold otherwise
+#pragma unroll 1
+    for (int k = 0; k < 8; k++) {
+        if (fin) {
+            const uint64_t dl = sub(add(cur(H0 + k), cur(A_A + k)), nxt(H0 + k));
+            f.put(mul(dl, sub(dl, TWO32)));
+        } else {
+            f.put(sub(nxt(H0 + k), cur(H0 + k)));
+        }
+    }
+    // W window shift, then W_win[14]' = W
+#pragma unroll 2
+    for (int k = 0; k < 14; k++) f.put(sub(nxt(WW0 + k), cur(WW0 + k + 1)));
+    f.put(sub(nxt(WW0 + 14), cur(COL_W)));
+    if (sched) {
+        const uint64_t sum = add(add(add(cur(WW0), cur(SS0_W)), cur(WW0 + 9)), cur(SS1_W));
+        f.put(sub(add(nxt(COL_W), mul(TWO32, cur(WS_CW))), sum));
+        f.put(boolc(cur(WS_C0)));
+        f.put(boolc(cur(WS_C1)));
+        f.put(sub(sub(cur(WS_CW), cur(WS_C0)), mul(2, cur(WS_C1))));
+    }
+    phi[n * blockIdx.y + i] = f.acc;
+}
+
+}  // namespace sha
+
+uint64_t sha256_n_blocks(uint64_t len) { return (len + 9 + 63) / 64; }
+
+uint64_t sha256_trace_height(uint64_t len) {
+    const uint64_t useful = 128 * sha256_n_blocks(len);
+    uint64_t h = 128;
+    while (h < useful) h <<= 1;
+    return h;
+}
+
+// build_sha256_trace_multi (sha256_air.rs:392-479): FIPS 180-4 padding,
+// one 128-row segment per block with the IV chained, rows past the last
+// block copy the last useful row. Also returns the digest as
+// prove_ds_ksk_binding reads it (prover.rs:529-538): H-state at block row 65
+// of the last block, big-endian words.
+uint64_t build_sha256_trace(const uint8_t *msg, uint64_t len, uint64_t *tr, uint8_t digest[32]) {
+    const uint64_t nb = sha256_n_blocks(len), h = sha256_trace_height(len);
+    std::vector<uint8_t> m(64 * nb, 0);
+    if (len) memcpy(m.data(), msg, len);
+    m[len] = 0x80;
+    const uint64_t bits = len * 8;
+    for (int i = 0; i < 8; i++) m[64 * nb - 1 - i] = (uint8_t)(bits >> (8 * i));
+    memset(tr, 0, 8 * SHA256_W * h);
+    uint32_t iv[8];
+    memcpy(iv, sha::IV, 32);
+    for (uint64_t b = 0; b < nb; b++) {
+        const std::array<uint32_t, 8> post = sha::fill_block(tr, h, 128 * b, &m[64 * b], iv);
+        for (int k = 0; k < 8; k++) iv[k] += post[k];
+    }
+    const uint64_t useful = 128 * nb;
+    for (uint64_t c = 0; c < SHA256_W; c++)
+        for (uint64_t r = useful; r < h; r++) tr[c * h + r] = tr[c * h + useful - 1];
+    const uint64_t drow = 128 * (nb - 1) + 65;
+    for (int k = 0; k < 8; k++) {
+        const uint32_t wv = (uint32_t)tr[(sha::H0 + k) * h + drow];
+        for (int i = 0; i < 4; i++) digest[4 * k + i] = (uint8_t)(wv >> (24 - 8 * i));
+    }
+    return nb;
+}
+
+void merge_sha256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup,
+                        const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval) {
+    SG_REQUIRE(is_pow2(n_trace) && n_trace >= 128 && is_pow2(blowup) && blowup >= 2,
+               "merge_sha256: n_trace must be a power of two >= 128");
+    SG_REQUIRE(K >= 1 && K <= 65535 && n_blocks.size() == K, "merge_sha256: one n_blocks per proof");
+    const uint64_t n = n_trace * blowup;
+    SG_REQUIRE(n <= (1ULL << 24), "merge_sha256: domain above 2^24");
+    for (uint32_t nb : n_blocks) SG_REQUIRE(nb >= 1 && 128 * (uint64_t)nb <= n_trace, "merge_sha256: bad n_blocks");
+    if (!phi_computed(phi_mode, n)) {
+        fill_u64(c, c_eval, n * K, 0);
+        return;
+    }
+    Scoped phi(c, 8 * n * K), d_nb(c, 4 * K);
+    SG_HIP(hipMemcpyAsync(d_nb.p, n_blocks.data(), 4 * K, hipMemcpyHostToDevice, c->stream));
+    {
+        ProfScope ps(c, "compose_sha256", (8.0 * SHA256_W * n + 8.0 * n) * K);
+        hipLaunchKernelGGL(sha::compose_sha256_k, dim3((uint32_t)((n + sha::AT - 1) / sha::AT), K), dim3(sha::AT), 0,
+                           c->stream, lde, n, blowup, d_nb.as<uint32_t>(), phi.as<uint64_t>());
+        SG_HIP(hipGetLastError());
+    }
+    quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
+}
+
+}  // namespace sg

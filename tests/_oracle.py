@@ -77,6 +77,18 @@ def lib():
                                              ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.c_char_p]
         L.ref_prover_core.argtypes = [u64p, sz, sz, sz, ctypes.c_uint64, ctypes.POINTER(FriParams),
                                       ctypes.c_char_p, u64p, ctypes.POINTER(u8p), ctypes.POINTER(sz)]
+        L.ref_sha256_n_blocks.restype = sz
+        L.ref_sha256_n_blocks.argtypes = [sz]
+        L.ref_sha256_trace_height.restype = sz
+        L.ref_sha256_trace_height.argtypes = [sz]
+        L.ref_build_sha256_trace.restype = ctypes.c_int
+        L.ref_build_sha256_trace.argtypes = [ctypes.c_char_p, sz, u64p, ctypes.c_char_p]
+        L.ref_sha256_constraints.argtypes = [u64p, u64p, sz, sz, u64p]
+        L.ref_merge_sha256.argtypes = [u64p, sz, sz, sz, ctypes.c_int, u64p]
+        L.ref_prove_ds_ksk_binding.argtypes = [ctypes.c_char_p, sz, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                               ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.c_char_p,
+                                               ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(sz),
+                                               ctypes.POINTER(sz)]
         _lib = L
     return _lib
 
@@ -167,3 +179,42 @@ def prove_outer_rollup(pi_hashes, pk_hash, phi_mode=1):
     if rc:
         raise ValueError("prove_outer_rollup rc=%d" % rc)
     return take_bytes(pp, pl), rf.raw
+
+
+SHA256_W, SHA256_K = 756, 766
+
+
+def sha256_trace(msg):
+    """(trace 756 x h, n_blocks, digest) of build_sha256_trace_multi."""
+    h = lib().ref_sha256_trace_height(len(msg))
+    tr = np.zeros((SHA256_W, h), dtype=np.uint64)
+    dg = ctypes.create_string_buffer(32)
+    nb = lib().ref_build_sha256_trace(bytes(msg), len(msg), ptr(tr), dg)
+    return tr, nb, dg.raw
+
+
+def sha256_constraints(cur, nxt, row, n_blocks):
+    cur = np.ascontiguousarray(cur, dtype=np.uint64)
+    nxt = np.ascontiguousarray(nxt, dtype=np.uint64)
+    out = np.zeros(SHA256_K, dtype=np.uint64)
+    lib().ref_sha256_constraints(ptr(cur), ptr(nxt), row, n_blocks, ptr(out))
+    return out
+
+
+def merge_sha256(lde, n_trace, blowup, n_blocks, phi_mode=1):
+    lde = np.ascontiguousarray(lde, dtype=np.uint64)
+    out = np.zeros(n_trace * blowup, dtype=np.uint64)
+    lib().ref_merge_sha256(ptr(lde), n_trace, blowup, n_blocks, phi_mode, ptr(out))
+    return out
+
+
+def prove_ds_ksk_binding(dnskey, parent_ds_hash, fs_binding, phi_mode=1):
+    """(proof, pi_hash, asserted_digest, root_f0, n_blocks, n_trace)"""
+    pp, pl = u8p(), sz()
+    pi, dg, rf = (ctypes.create_string_buffer(32) for _ in range(3))
+    nb, nt = sz(), sz()
+    rc = lib().ref_prove_ds_ksk_binding(bytes(dnskey), len(dnskey), bytes(parent_ds_hash), bytes(fs_binding),
+                                        phi_mode, ctypes.byref(pp), ctypes.byref(pl), pi, dg, rf,
+                                        ctypes.byref(nb), ctypes.byref(nt))
+    assert rc == 0
+    return take_bytes(pp, pl), pi.raw, dg.raw, rf.raw, nb.value, nt.value

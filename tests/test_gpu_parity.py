@@ -283,3 +283,47 @@ def test_packed_shards_and_empty_shard(S):
         S.prove_inner_shard_batch(salt, [zone[:3], []], fsbs[:2])
     with pytest.raises(S.StarkError):
         S.prove_inner_shard(salt, [], fsbs[0])
+
+
+@pytest.mark.parametrize("msg,phi_mode", [(b"abc", 0), (b"abc", 1), (bytes(range(68)), 1), (b"r" * 130, 0)],
+                         ids=["1blk-phi0", "1blk-rayon", "2blk-rayon", "3blk-phi0"])
+def test_merge_sha256_matches_oracle(S, msg, phi_mode):
+    """Device SHA-256 composition + quotient against the oracle's merge."""
+    say("merge_sha256 len %d phi %d" % (len(msg), phi_mode))
+    tr, nb, _ = O.sha256_trace(msg)
+    h = tr.shape[1]
+    lde = O.lde_columns(tr, 32)
+    want = O.merge_sha256(lde, h, 32, nb, phi_mode)
+    got, info = S.deep_ali_merge_sha256(lde, list(range(1, 767)), pow(7, (O.P - 1) // (32 * h), O.P), h, 32, nb,
+                                        phi_mode=phi_mode)
+    assert np.array_equal(got, want)
+    assert info.num_constraints == 766 and info.trace_width == 756
+
+
+@pytest.mark.parametrize("key,fsb", [(b"\x42" * 44, b"\xAA" * 32), (bytes(range(68)), b"\xBB" * 32)],
+                         ids=["ed25519-44B", "p256-68B"])
+def test_ds_ksk_binding_matches_oracle(S, key, fsb):
+    """The reference's own DS->KSK cases (prover.rs:2946-2975): proof bytes,
+    pi_hash and the asserted digest (= SHA-256 of the key)."""
+    say("ds_ksk %d bytes" % len(key))
+    parent = hashlib.sha256(key).digest()
+    out = S.prove_ds_ksk_binding(key, parent, fsb)
+    proof, pi, dg, rf, nb, nt = O.prove_ds_ksk_binding(key, parent, fsb)
+    assert out.asserted_digest == parent == dg
+    assert (out.n_blocks, out.n_trace) == (nb, nt)
+    assert out.root_f0 == rf and out.pi_hash == pi
+    assert out.proof_blob == proof
+
+
+def test_ds_ksk_batch_matches_singles(S):
+    """Keys of 1, 2 and 3 blocks (three trace heights) in one batch call."""
+    say("ds_ksk batch")
+    keys = [b"\x42" * 44, bytes(range(68)), b"k" * 10, b"m" * 130, b"n" * 60]
+    parents = [hashlib.sha256(k).digest() for k in keys]
+    fsbs = [bytes([i]) * 32 for i in range(len(keys))]
+    outs = S.prove_ds_ksk_binding_batch(keys, parents, fsbs)
+    for k, o in zip(keys, outs):
+        single = S.prove_ds_ksk_binding(k, parents[keys.index(k)], fsbs[keys.index(k)])
+        assert o.proof_blob == single.proof_blob and o.pi_hash == single.pi_hash
+    proof, pi, dg, rf, nb, nt = O.prove_ds_ksk_binding(keys[3], parents[3], fsbs[3])
+    assert outs[3].proof_blob == proof and outs[3].pi_hash == pi and nb == 3

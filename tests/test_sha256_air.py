@@ -1,0 +1,106 @@
+"""SHA-256 AIR (crates/deep_ali/src/sha256_air.rs) on the CPU: the oracle's
+trace builder and constraint evaluator, and the library's host trace builder.
+
+No stored proof exists for this AIR, so the oracle is pinned by what the
+mathematics and the reference's own tests fix:
+  - the digest read from the trace equals FIPS 180-4 SHA-256 (hashlib), for
+    the reference's "abc" known answer (sha256_air.rs:1346-1355) and for
+    lengths around every padding boundary;
+  - every transition constraint vanishes on a valid trace except the
+    documented wrap row (sha256_air.rs:447-466: H-state at the digest row
+    differs from the IV at row 0), where exactly the 8 H-state slots fire;
+  - a planted mistake in one cell fires exactly the constraint slots the
+    reference's constraint budget (sha256_air.rs:49-71) places on that cell,
+    which pins the order of the 766 slots (the weights lambda_j = j+1 depend
+    on it).
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+from tests import _oracle as O
+
+P = O.P
+# slot offsets from the budget table (sha256_air.rs:49-71)
+BOOL, PACK, SIG0, SIG1, SSIG0, SSIG1 = 0, 256, 264, 329, 394, 459
+CH, MAJ, T1, T2, SHIFT, NEWAE, HST, WWIN, SCHED = 524, 557, 718, 723, 727, 733, 739, 747, 762
+# columns (sha256_air.rs:140-253)
+BIT_A, COL_W = 8, 524
+
+MSGS = [b"abc", b"", b"\x42" * 44, bytes(range(68)), b"x" * 55, b"x" * 56, b"y" * 64, b"z" * 119, b"q" * 120,
+        b"r" * 300]
+
+
+def nonzero_slots(tr, nb, row):
+    h = tr.shape[1]
+    c = O.sha256_constraints(tr[:, row], tr[:, (row + 1) % h], row, nb)
+    return set(int(i) for i in np.nonzero(c)[0])
+
+
+@pytest.mark.parametrize("msg", MSGS, ids=lambda m: "len%d" % len(m))
+def test_trace_digest_is_sha256(msg):
+    tr, nb, dg = O.sha256_trace(msg)
+    assert dg == hashlib.sha256(msg).digest()
+    assert nb == (len(msg) + 9 + 63) // 64
+    h = tr.shape[1]
+    assert h >= 128 * nb and h & (h - 1) == 0
+    assert tr.max() < 2**32
+
+
+def test_abc_known_answer():
+    # sha256_air.rs:1346-1355: "abc" -> ba7816bf...
+    _, _, dg = O.sha256_trace(b"abc")
+    assert dg.hex() == "ba7816bf8f01cfea414140de5dae2223b00361a396177a9cb410ff61f20015ad"
+
+
+@pytest.mark.parametrize("msg", [b"abc", bytes(range(68)), b"r" * 300], ids=lambda m: "len%d" % len(m))
+def test_constraints_vanish_except_wrap(msg):
+    tr, nb, _ = O.sha256_trace(msg)
+    h = tr.shape[1]
+    for r in range(h - 1):
+        assert nonzero_slots(tr, nb, r) == set(), "row %d" % r
+    assert nonzero_slots(tr, nb, h - 1) == set(range(HST, HST + 8))
+
+
+@pytest.mark.parametrize("r", [3, 10, 40, 90])
+def test_planted_bit_fires_its_slots(r):
+    """a_5 := 2: booleanity 5, the pack of a, Sigma0 t_3 and t_24 (inputs
+    a_(i+2), a_(i+13)), Sigma0 bit 15 (input a_(i+22)); Maj ab_5 = a_5 b_5 and
+    ac_5 = a_5 c_5 fire where b_5 resp. c_5 is 1 (2 b differs from the stored
+    product only then)."""
+    tr, nb, _ = O.sha256_trace(b"abc")
+    b5, c5 = int(tr[BIT_A + 32 + 5, r]), int(tr[BIT_A + 64 + 5, r])
+    tr[BIT_A + 5, r] = 2
+    want = {BOOL + 5, PACK + 0, SIG0 + 3, SIG0 + 24, SIG0 + 32 + 15}
+    if b5:
+        want.add(MAJ + 3 * 5)
+    if c5:
+        want.add(MAJ + 3 * 5 + 1)
+    assert nonzero_slots(tr, nb, r) == want
+    assert nonzero_slots(tr, nb, r - 1) == set()
+
+
+def test_planted_word_fires_its_slots():
+    """W[20] + 1: row 20's T1 sum and the W-window feed; row 19's schedule sum."""
+    tr, nb, _ = O.sha256_trace(b"abc")
+    r = 20
+    tr[COL_W, r] = (int(tr[COL_W, r]) + 1) % P
+    assert nonzero_slots(tr, nb, r) == {T1, WWIN + 14}
+    assert nonzero_slots(tr, nb, r - 1) == {SCHED}
+
+
+def test_phi_quirk_zero():
+    tr, nb, _ = O.sha256_trace(b"abc")
+    lde = O.lde_columns(tr, 32)
+    assert not O.merge_sha256(lde, 128, 32, nb, phi_mode=2).any()
+    assert O.merge_sha256(lde, 128, 32, nb, phi_mode=1).any()
+
+
+@pytest.mark.parametrize("msg", MSGS, ids=lambda m: "len%d" % len(m))
+def test_library_host_trace_matches_oracle(msg):
+    """The library's host trace builder (the step before the device path)."""
+    import stark_swarm_amd as S
+    tr, nb = S.build_sha256_trace_multi(msg)
+    tr2, nb2, _ = O.sha256_trace(msg)
+    assert nb == nb2 and np.array_equal(tr, tr2)
