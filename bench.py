@@ -129,6 +129,7 @@ def main():
     ap.add_argument("--calib-copy", action="store_true")
     ap.add_argument("--streams", type=int, default=4, help="proofs in flight for the batched measurement")
     ap.add_argument("--shards", type=int, default=32, help="inner-shard proofs for the plain-vs-batched line (0 = skip)")
+    ap.add_argument("--dsksk", type=int, default=32, help="DS-KSK (SHA-256 AIR) proofs for the plain-vs-batched line (0 = skip)")
     ap.add_argument("--shards-only", action="store_true",
                     help="run only the inner-shard plain-vs-batched measurement (for profiling that path)")
     args = ap.parse_args()
@@ -238,6 +239,9 @@ def main():
     if args.shards:
         log("inner shards: plain vs batched")
         res.update(shard_batch(S, args.streams, args.shards))
+    if args.dsksk:
+        log("ds-ksk (SHA-256 AIR): plain vs batched")
+        res.update(ds_ksk_batch(S, args.dsksk))
     if not args.no_cpu:
         log("cpu baseline: timing the oracle on a bounded sample")
         res["cpu_baseline"] = cpu_baseline(w, lg, blowup, schedule, s0)
@@ -286,7 +290,44 @@ def shard_batch(S, K, nproofs, records_per_shard=1024):
                             "speedup": round(plain_dt / ldt, 3),
                             "batch": "one lockstep device batch (stark_prove_inner_shard_batch)",
                             "plain_latency_ms_p50": round(1e3 * lat[len(lat) // 2], 3),
+                            "plain_latency_ms_p99": round(1e3 * lat[min(len(lat) - 1, (99 * len(lat)) // 100)], 3),
                             "bit_exact_vs_plain": exact}}
+
+
+def ds_ksk_batch(S, nproofs):
+    """prove_ds_ksk_binding at C1: 64-byte messages (2 SHA-256 blocks, n_trace
+    256, n0 = 8192, 756 columns, 766 constraints) from SplitMix64 seed
+    0x5EED0001, fs_binding = [0xAA; 32]: one proof per call vs one batch call.
+    Every batched proof must equal the single-call proof of the same key."""
+    import hashlib
+    raw = splitmix_field(0x5EED0001, 8 * nproofs).astype("<u8").tobytes()
+    keys = [raw[64 * i:64 * (i + 1)] for i in range(nproofs)]
+    parents = [hashlib.sha256(k).digest() for k in keys]
+    fsb = [b"\xAA" * 32] * nproofs
+    S.prove_ds_ksk_binding(keys[0], parents[0], fsb[0])  # warm
+    lat, plain = [], []
+    t0 = time.perf_counter()
+    for k, pa, f in zip(keys, parents, fsb):
+        t = time.perf_counter()
+        plain.append(S.prove_ds_ksk_binding(k, pa, f).proof_blob)
+        lat.append(time.perf_counter() - t)
+    plain_dt = time.perf_counter() - t0
+    S.prove_ds_ksk_binding_batch(keys[:4], parents[:4], fsb[:4])  # warm
+    t0 = time.perf_counter()
+    outs = S.prove_ds_ksk_binding_batch(keys, parents, fsb)
+    bdt = time.perf_counter() - t0
+    exact = all(o.proof_blob == p for o, p in zip(outs, plain)) and all(o.asserted_digest == pa for o, pa in
+                                                                        zip(outs, parents))
+    if not exact:
+        raise SystemExit("batched DS-KSK proofs differ from the single-call proofs")
+    lat.sort()
+    return {"ds_ksk": {"message_bytes": 64, "n_blocks": outs[0].n_blocks, "n_trace": outs[0].n_trace,
+                       "n0": outs[0].n_trace * 32, "proofs": nproofs,
+                       "plain_proofs_per_s": round(nproofs / plain_dt, 2),
+                       "batched_proofs_per_s": round(nproofs / bdt, 2), "speedup": round(plain_dt / bdt, 3),
+                       "plain_latency_ms_p50": round(1e3 * lat[len(lat) // 2], 3),
+                       "plain_latency_ms_p99": round(1e3 * lat[min(len(lat) - 1, (99 * len(lat)) // 100)], 3),
+                       "bit_exact_vs_plain": exact}}
 
 
 def batched(S, args, w, n, blowup, params, ref_blob, plain_ms):

@@ -62,10 +62,25 @@ inline uint32_t small1(uint32_t x) { return rotr(x, 17) ^ rotr(x, 19) ^ (x >> 10
 inline uint32_t chf(uint32_t e, uint32_t f, uint32_t g) { return (e & f) ^ (~e & g); }
 inline uint32_t majf(uint32_t a, uint32_t b, uint32_t c) { return (a & b) ^ (a & c) ^ (b & c); }
 
+// The words that determine one trace row; every cell of the row is one of
+// them or one of their bits.
+struct RowWords {
+    uint32_t st[8];           // a..h
+    uint32_t s0t, s1t;        // rotr-xor halves of Sigma0(a), Sigma1(e)
+    uint32_t chw, mjw, wt;    // Ch, Maj, W[t]
+    uint64_t t1, t2;          // full T1 and T2 sums
+    uint32_t hs[8];           // H-state
+    uint32_t ww[15];          // W window W[t-15..t-1]
+    uint32_t ss0t, ss1t;      // rotr-xor halves of sigma0(W[t-14]), sigma1(W[t-1])
+    uint32_t sc, na, ne;      // schedule carry, new_a / new_e carries
+};
+
 // One 128-row block segment (sha256_air.rs:648-898): rows 0..63 are the
 // rounds (state before round r), row 64 the post-compression state, rows
 // 65..127 idle with the state held; H-state is the incoming IV through row
-// 64 and the block digest from row 65. Returns the post-compression state.
+// 64 and the block digest from row 65. The row words come first, then the
+// trace is written column by column so every store stream is contiguous.
+// Returns the post-compression state.
 std::array<uint32_t, 8> fill_block(uint64_t *tr, uint64_t h, uint64_t base, const uint8_t *blk,
                                    const uint32_t iv[8]) {
     uint32_t w[64];
@@ -74,77 +89,91 @@ std::array<uint32_t, 8> fill_block(uint64_t *tr, uint64_t h, uint64_t base, cons
                blk[4 * i + 3];
     for (int t = 16; t < 64; t++) w[t] = small1(w[t - 2]) + w[t - 7] + small0(w[t - 15]) + w[t - 16];
     auto wat = [&](int t) { return (t >= 0 && t < 64) ? w[t] : 0u; };
-    std::array<std::array<uint32_t, 8>, 65> st;
+    uint32_t st[65][8];
     for (int k = 0; k < 8; k++) st[0][k] = iv[k];
     for (int t = 0; t < 64; t++) {
-        const auto &s = st[t];
-        const uint32_t t1 = s[7] + big1(s[4]) + chf(s[4], s[5], s[6]) + KH[t] + w[t];
-        const uint32_t t2 = big0(s[0]) + majf(s[0], s[1], s[2]);
-        st[t + 1] = {t1 + t2, s[0], s[1], s[2], s[3] + t1, s[4], s[5], s[6]};
+        const uint32_t *q = st[t];
+        const uint32_t t1 = q[7] + big1(q[4]) + chf(q[4], q[5], q[6]) + KH[t] + w[t];
+        const uint32_t t2 = big0(q[0]) + majf(q[0], q[1], q[2]);
+        const uint32_t nx[8] = {t1 + t2, q[0], q[1], q[2], q[3] + t1, q[4], q[5], q[6]};
+        memcpy(st[t + 1], nx, 32);
     }
     uint32_t dg[8];
     for (int k = 0; k < 8; k++) dg[k] = iv[k] + st[64][k];
+    static thread_local RowWords R[128];
     for (int br = 0; br < 128; br++) {
-        uint64_t *row = tr + base + br;  // column c of this row at row[c * h]
-        auto put = [&](uint32_t col, uint64_t v) { row[(uint64_t)col * h] = v; };
-        auto put_bits = [&](uint32_t col, uint32_t x) {
-            for (int i = 0; i < 32; i++) put(col + i, (x >> i) & 1);
-        };
-        const auto &s = st[br <= 64 ? br : 64];
-        const uint32_t a = s[0], b = s[1], c = s[2], d = s[3], e = s[4], f = s[5], g = s[6], hh = s[7];
-        for (int k = 0; k < 8; k++) put(A_A + k, s[k]);
-        put_bits(BIT_A, a);
-        put_bits(BIT_B, b);
-        put_bits(BIT_C, c);
-        put_bits(BIT_E, e);
-        put_bits(BIT_F, f);
-        put_bits(BIT_G, g);
-        // rotate-xor sub-blocks: t_i = x_(i+r1) ^ x_(i+r2), s_i = t_i ^ third_i
-        auto sigma = [&](uint32_t tcol, uint32_t bcol, uint32_t wcol, uint32_t x, int r1, int r2, int r3, bool shr) {
-            uint32_t t = rotr(x, r1) ^ rotr(x, r2), third = shr ? (x >> r3) : rotr(x, r3);
-            put_bits(tcol, t);
-            put_bits(bcol, t ^ third);
-            put(wcol, t ^ third);
-        };
-        sigma(S0_T, S0_BIT, S0_W, a, 2, 13, 22, false);
-        sigma(S1_T, S1_BIT, S1_W, e, 6, 11, 25, false);
-        const uint32_t chw = chf(e, f, g), mjw = majf(a, b, c);
-        put_bits(CH_BIT, chw);
-        put(CH_W, chw);
-        put_bits(MAJ_AB, a & b);
-        put_bits(MAJ_AC, a & c);
-        put_bits(MAJ_BC, b & c);
-        put_bits(MAJ_U, (a & b) ^ (a & c));
-        put_bits(MAJ_BIT, mjw);
-        put(MAJ_W, mjw);
-        const uint32_t wt = br < 64 ? w[br] : 0, kt = br < 64 ? KH[br] : 0;
-        put(COL_W, wt);
-        const uint64_t t1 = (uint64_t)hh + big1(e) + chw + kt + wt, t2 = (uint64_t)big0(a) + mjw;
-        const uint32_t c1 = (uint32_t)(t1 >> 32), c2 = (uint32_t)(t2 >> 32);
-        put(T1_LO, (uint32_t)t1);
-        put(T1_C0, c1 & 1);
-        put(T1_C1, (c1 >> 1) & 1);
-        put(T1_C2, (c1 >> 2) & 1);
-        put(T1_CW, c1);
-        put(T2_LO, (uint32_t)t2);
-        put(T2_C0, c2 & 1);
-        put(T2_C1, (c2 >> 1) & 1);
-        put(T2_CW, c2);
-        for (int k = 0; k < 8; k++) put(H0 + k, br <= 64 ? iv[k] : dg[k]);
-        for (int k = 0; k < 15; k++) put(WW0 + k, wat(br - 15 + k));
-        const uint32_t ww14 = wat(br - 1), ww1 = wat(br - 14);
-        put_bits(BIT_WW14, ww14);
-        put_bits(BIT_WW1, ww1);
-        sigma(SS0_T, SS0_BIT, SS0_W, ww1, 7, 18, 3, true);
-        sigma(SS1_T, SS1_BIT, SS1_W, ww14, 17, 19, 10, true);
-        const uint64_t sum = (uint64_t)wat(br - 15) + small0(ww1) + wat(br - 6) + small1(ww14);
-        const uint32_t sc = (uint32_t)(sum >> 32);
-        put(WS_C0, sc & 1);
-        put(WS_C1, (sc >> 1) & 1);
-        put(WS_CW, sc);
-        put(NA_C, br < 64 ? (((uint64_t)(uint32_t)t1 + (uint32_t)t2) >> 32) : 0);
-        put(NE_C, br < 64 ? (((uint64_t)d + (uint32_t)t1) >> 32) : 0);
+        RowWords &r = R[br];
+        memcpy(r.st, st[br <= 64 ? br : 64], 32);
+        const uint32_t a = r.st[0], b = r.st[1], c = r.st[2], d = r.st[3], e = r.st[4], f = r.st[5], g = r.st[6];
+        r.s0t = rotr(a, 2) ^ rotr(a, 13);
+        r.s1t = rotr(e, 6) ^ rotr(e, 11);
+        r.chw = chf(e, f, g);
+        r.mjw = majf(a, b, c);
+        r.wt = br < 64 ? w[br] : 0;
+        r.t1 = (uint64_t)r.st[7] + big1(e) + r.chw + (br < 64 ? KH[br] : 0) + r.wt;
+        r.t2 = (uint64_t)big0(a) + r.mjw;
+        memcpy(r.hs, br <= 64 ? iv : dg, 32);
+        for (int k = 0; k < 15; k++) r.ww[k] = wat(br - 15 + k);
+        const uint32_t ww14 = r.ww[14], ww1 = r.ww[1];
+        r.ss0t = rotr(ww1, 7) ^ rotr(ww1, 18);
+        r.ss1t = rotr(ww14, 17) ^ rotr(ww14, 19);
+        r.sc = (uint32_t)(((uint64_t)r.ww[0] + small0(ww1) + wat(br - 6) + small1(ww14)) >> 32);
+        r.na = br < 64 ? (uint32_t)(((uint64_t)(uint32_t)r.t1 + (uint32_t)r.t2) >> 32) : 0;
+        r.ne = br < 64 ? (uint32_t)(((uint64_t)d + (uint32_t)r.t1) >> 32) : 0;
     }
+    // column writers: col(c, f) stores f(row words) down column c
+    auto col = [&](uint32_t c, auto f) {
+        uint64_t *o = tr + (uint64_t)c * h + base;
+        for (int br = 0; br < 128; br++) o[br] = f(R[br]);
+    };
+    auto bits = [&](uint32_t c0, auto word) {
+        for (int i = 0; i < 32; i++) col(c0 + i, [&](const RowWords &r) { return (uint64_t)((word(r) >> i) & 1); });
+    };
+    for (uint32_t k = 0; k < 8; k++) col(A_A + k, [&](const RowWords &r) { return (uint64_t)r.st[k]; });
+    bits(BIT_A, [](const RowWords &r) { return r.st[0]; });
+    bits(BIT_B, [](const RowWords &r) { return r.st[1]; });
+    bits(BIT_C, [](const RowWords &r) { return r.st[2]; });
+    bits(BIT_E, [](const RowWords &r) { return r.st[4]; });
+    bits(BIT_F, [](const RowWords &r) { return r.st[5]; });
+    bits(BIT_G, [](const RowWords &r) { return r.st[6]; });
+    // Sigma0(a) = t ^ rotr(a, 22), Sigma1(e) = t ^ rotr(e, 25)
+    bits(S0_T, [](const RowWords &r) { return r.s0t; });
+    bits(S0_BIT, [](const RowWords &r) { return r.s0t ^ rotr(r.st[0], 22); });
+    col(S0_W, [](const RowWords &r) { return (uint64_t)(r.s0t ^ rotr(r.st[0], 22)); });
+    bits(S1_T, [](const RowWords &r) { return r.s1t; });
+    bits(S1_BIT, [](const RowWords &r) { return r.s1t ^ rotr(r.st[4], 25); });
+    col(S1_W, [](const RowWords &r) { return (uint64_t)(r.s1t ^ rotr(r.st[4], 25)); });
+    bits(CH_BIT, [](const RowWords &r) { return r.chw; });
+    col(CH_W, [](const RowWords &r) { return (uint64_t)r.chw; });
+    bits(MAJ_AB, [](const RowWords &r) { return r.st[0] & r.st[1]; });
+    bits(MAJ_AC, [](const RowWords &r) { return r.st[0] & r.st[2]; });
+    bits(MAJ_BC, [](const RowWords &r) { return r.st[1] & r.st[2]; });
+    bits(MAJ_U, [](const RowWords &r) { return (r.st[0] & r.st[1]) ^ (r.st[0] & r.st[2]); });
+    bits(MAJ_BIT, [](const RowWords &r) { return r.mjw; });
+    col(MAJ_W, [](const RowWords &r) { return (uint64_t)r.mjw; });
+    col(COL_W, [](const RowWords &r) { return (uint64_t)r.wt; });
+    col(T1_LO, [](const RowWords &r) { return r.t1 & 0xFFFFFFFFu; });
+    for (uint32_t k = 0; k < 3; k++) col(T1_C0 + k, [&](const RowWords &r) { return (r.t1 >> (32 + k)) & 1; });
+    col(T1_CW, [](const RowWords &r) { return r.t1 >> 32; });
+    col(T2_LO, [](const RowWords &r) { return r.t2 & 0xFFFFFFFFu; });
+    for (uint32_t k = 0; k < 2; k++) col(T2_C0 + k, [&](const RowWords &r) { return (r.t2 >> (32 + k)) & 1; });
+    col(T2_CW, [](const RowWords &r) { return r.t2 >> 32; });
+    for (uint32_t k = 0; k < 8; k++) col(H0 + k, [&](const RowWords &r) { return (uint64_t)r.hs[k]; });
+    for (uint32_t k = 0; k < 15; k++) col(WW0 + k, [&](const RowWords &r) { return (uint64_t)r.ww[k]; });
+    bits(BIT_WW14, [](const RowWords &r) { return r.ww[14]; });
+    bits(BIT_WW1, [](const RowWords &r) { return r.ww[1]; });
+    // sigma0(x) = t ^ (x >> 3), sigma1(x) = t ^ (x >> 10)
+    bits(SS0_T, [](const RowWords &r) { return r.ss0t; });
+    bits(SS0_BIT, [](const RowWords &r) { return r.ss0t ^ (r.ww[1] >> 3); });
+    col(SS0_W, [](const RowWords &r) { return (uint64_t)(r.ss0t ^ (r.ww[1] >> 3)); });
+    bits(SS1_T, [](const RowWords &r) { return r.ss1t; });
+    bits(SS1_BIT, [](const RowWords &r) { return r.ss1t ^ (r.ww[14] >> 10); });
+    col(SS1_W, [](const RowWords &r) { return (uint64_t)(r.ss1t ^ (r.ww[14] >> 10)); });
+    col(WS_C0, [](const RowWords &r) { return (uint64_t)(r.sc & 1); });
+    col(WS_C1, [](const RowWords &r) { return (uint64_t)((r.sc >> 1) & 1); });
+    col(WS_CW, [](const RowWords &r) { return (uint64_t)r.sc; });
+    col(NA_C, [](const RowWords &r) { return (uint64_t)r.na; });
+    col(NE_C, [](const RowWords &r) { return (uint64_t)r.ne; });
     std::array<uint32_t, 8> post;
     for (int k = 0; k < 8; k++) post[k] = st[64][k];
     return post;
@@ -317,7 +346,6 @@ uint64_t build_sha256_trace(const uint8_t *msg, uint64_t len, uint64_t *tr, uint
     m[len] = 0x80;
     const uint64_t bits = len * 8;
     for (int i = 0; i < 8; i++) m[64 * nb - 1 - i] = (uint8_t)(bits >> (8 * i));
-    memset(tr, 0, 8 * SHA256_W * h);
     uint32_t iv[8];
     memcpy(iv, sha::IV, 32);
     for (uint64_t b = 0; b < nb; b++) {

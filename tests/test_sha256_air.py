@@ -104,3 +104,18 @@ def test_library_host_trace_matches_oracle(msg):
     tr, nb = S.build_sha256_trace_multi(msg)
     tr2, nb2, _ = O.sha256_trace(msg)
     assert nb == nb2 and np.array_equal(tr, tr2)
+
+
+def test_library_host_trace_writes_every_cell():
+    """The column-wise builder must write all 756 x h cells (no reliance on a
+    zeroed buffer)."""
+    import ctypes
+
+    import stark_swarm_amd.api as A
+    for msg in [b"abc", bytes(range(68)), b"r" * 300]:
+        h = A.lib().stark_host_sha256_trace_height(len(msg))
+        tr = np.full((756, h), 0xDEADBEEFDEADBEEF, dtype=np.uint64)
+        dg = ctypes.create_string_buffer(32)
+        A.lib().stark_host_build_sha256_trace(msg, len(msg), tr.ctypes.data_as(A.u64p), dg)
+        tr2, _, dg2 = O.sha256_trace(msg)
+        assert np.array_equal(tr, tr2) and dg.raw == dg2
