@@ -154,8 +154,8 @@ int stark_merge_sha256(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, size
 
 /* prove_ds_ksk_binding (crates/swarm-dns/src/prover.rs:512-585): SHA-256
  * trace of dnskey -> LDE x32 -> SHA-256 merge -> STIR proof bound to
- * fs_binding -> ds_ksk_pi_hash. The reference's local self-verify is not
- * run (the proof bytes are the reference's). */
+ * fs_binding -> ds_ksk_pi_hash. The reference's local self-verify
+ * (prover.rs:563-566) is left to the caller. */
 int stark_prove_ds_ksk_binding(stark_ctx *ctx, const uint8_t *dnskey, size_t len, const uint8_t parent_ds_hash[32],
                                const uint8_t fs_binding[32], int ldt, int phi_mode, stark_ds_ksk_out *out);
 
