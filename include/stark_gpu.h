@@ -92,6 +92,16 @@ typedef struct {
     size_t proof_len;
 } stark_ds_ksk_out;
 
+/* One SHA-256 / SHA-512 AIR proof of stark_prove_sha_batch */
+typedef struct {
+    uint8_t digest[64]; /* SHA-256 fills the first 32 bytes */
+    uint8_t root_f0[32];
+    size_t n_blocks;
+    size_t n_trace;
+    uint8_t *proof; /* ark-serialize compressed DeepFriProof<SexticExt>; stark_free() it */
+    size_t proof_len;
+} stark_sha_out;
+
 /* ---- context ---- */
 int stark_ctx_create(int device, stark_ctx **out);
 void stark_ctx_destroy(stark_ctx *ctx);
@@ -169,6 +179,24 @@ int stark_prove_ds_ksk_binding_batch(stark_ctx *ctx, const uint8_t *dnskeys, con
                                      size_t n_keys, const uint8_t *parent_ds_hashes, const uint8_t *fs_bindings,
                                      int ldt, int phi_mode, stark_ds_ksk_out *outs);
 
+/* deep_ali_merge_sha512 (crates/deep_ali/src/lib.rs:561-653): lde is the
+ * 1510 x n0 SHA-512 AIR LDE, coeffs the 1526 coefficients (lambda_j = j+1). */
+int stark_merge_sha512(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_blocks,
+                       const uint64_t *coeffs, size_t k, int phi_mode, uint64_t *c_eval);
+
+/* SHA-256 (alg = 256) or SHA-512 (alg = 512) AIR proofs over n messages with
+ * the prove_ds_ksk_binding pipeline (trace -> LDE x32 -> merge -> STIR with
+ * 54 queries bound to fs_bindings[32 i ..]); message i is msgs[offsets[i] ..
+ * + lens[i]) and its trace has heights[i] rows (a power of two >= the
+ * message's minimum; padding rows repeat the last useful row, the trace
+ * builder's own rule, sha256_air.rs:441-479 / sha512_air.rs:551-573), or the
+ * minimum when heights is NULL. Proven in lockstep device batches per
+ * height, in waves sized to free HBM. No single counterpart in the
+ * reference: the batched SHA-256/512 workload of the swarm's hash AIRs. */
+int stark_prove_sha_batch(stark_ctx *ctx, int alg, const uint8_t *msgs, const size_t *offsets, const size_t *lens,
+                          const size_t *heights, size_t n, const uint8_t *fs_bindings, int phi_mode,
+                          stark_sha_out *outs);
+
 /* prove_outer_rollup (crates/swarm-dns/src/prover.rs:253-304). */
 int stark_prove_outer_rollup(stark_ctx *ctx, const uint8_t *pi_hashes, size_t n, const uint8_t pk_hash[32], int ldt,
                              int phi_mode, uint8_t **proof, size_t *proof_len, uint8_t root_f0[32]);
@@ -214,6 +242,10 @@ uint64_t stark_host_stdrng_first_u64(uint64_t seed);
  * returns n_blocks and the digest read from the trace (prover.rs:529-538). */
 size_t stark_host_sha256_trace_height(size_t len);
 size_t stark_host_build_sha256_trace(const uint8_t *msg, size_t len, uint64_t *trace, uint8_t digest[32]);
+/* build_sha512_trace_multi (crates/deep_ali/src/sha512_air.rs:551-573):
+ * 1510 x stark_host_sha512_trace_height(len) trace; digest = 64 bytes. */
+size_t stark_host_sha512_trace_height(size_t len);
+size_t stark_host_build_sha512_trace(const uint8_t *msg, size_t len, uint64_t *trace, uint8_t digest[64]);
 void stark_host_dns_leaf_hash(const stark_dns_record *rec, const uint8_t salt[16], uint8_t out[32]);
 
 #ifdef __cplusplus

@@ -101,6 +101,15 @@ int ref_prove_ds_ksk_binding(const uint8_t *dnskey, size_t len, const uint8_t pa
                              uint8_t pi_hash[32], uint8_t asserted_digest[32], uint8_t root_f0[32],
                              size_t *n_blocks_out, size_t *n_trace_out);
 
+/* ---- SHA-512 AIR (sha512_air.rs, lib.rs:561-653) ---- */
+size_t ref_sha512_n_blocks(size_t len);
+size_t ref_sha512_trace_height(size_t len);
+/* 1510 x height trace; returns n_blocks, digest = the 64-byte digest read from the trace */
+int ref_build_sha512_trace(const uint8_t *msg, size_t len, uint64_t *tr, uint8_t digest[64]);
+void ref_sha512_constraints(const uint64_t *cur, const uint64_t *nxt, size_t row, size_t n_blocks, uint64_t *out);
+int ref_merge_sha512(const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_blocks, int phi_mode,
+                     uint64_t *c_eval);
+
 /* ---- prover-core microbench (coset LDE -> row Merkle -> STIR prove) ---- */
 /* trace: w x n_trace column-major; returns proof bytes of deep_fri_prove over
  * f0 = sum_c (c+1) * LDE_c with the row-tree root as public_inputs_hash, and

@@ -14,6 +14,8 @@ Rust functions they replace:
   prove_ds_ksk_binding     crates/swarm-dns/src/prover.rs:446-585
   build_sha256_trace_multi crates/deep_ali/src/sha256_air.rs:429-479
   deep_ali_merge_sha256    crates/deep_ali/src/lib.rs:453-545
+  build_sha512_trace_multi crates/deep_ali/src/sha512_air.rs:551-573
+  deep_ali_merge_sha512    crates/deep_ali/src/lib.rs:561-653
   DnsRecord                crates/swarm-dns/src/dns.rs:24-89
   shard_fs_binding         crates/swarm-proto/src/messages.rs:162-176
 
@@ -44,6 +46,9 @@ from .api import (  # noqa: F401
     deep_ali_merge_general,
     deep_ali_merge_sha256,
     build_sha256_trace_multi,
+    deep_ali_merge_sha512,
+    build_sha512_trace_multi,
+    prove_sha_batch,
     deep_fri_prove,
     deep_fri_prove_batch,
     default_context,

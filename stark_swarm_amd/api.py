@@ -48,6 +48,11 @@ class _InnerOut(ctypes.Structure):
                 ("proof", u8p), ("proof_len", sz)]
 
 
+class _ShaOut(ctypes.Structure):
+    _fields_ = [("digest", ctypes.c_uint8 * 64), ("root_f0", ctypes.c_uint8 * 32), ("n_blocks", sz),
+                ("n_trace", sz), ("proof", u8p), ("proof_len", sz)]
+
+
 class _DsKskOut(ctypes.Structure):
     _fields_ = [("pi_hash", ctypes.c_uint8 * 32), ("asserted_digest", ctypes.c_uint8 * 32),
                 ("root_f0", ctypes.c_uint8 * 32), ("n_blocks", sz), ("n_trace", sz),
@@ -113,6 +118,12 @@ def lib():
                                                             ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
                                                             ctypes.POINTER(_DsKskOut)]),
         "stark_host_sha256_trace_height": (sz, [sz]),
+        "stark_host_sha512_trace_height": (sz, [sz]),
+        "stark_host_build_sha512_trace": (sz, [ctypes.c_char_p, sz, u64p, ctypes.c_char_p]),
+        "stark_merge_sha512": (ctypes.c_int, [vp, u64p, sz, sz, sz, u64p, sz, ctypes.c_int, u64p]),
+        "stark_prove_sha_batch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(sz),
+                                                 ctypes.POINTER(sz), ctypes.POINTER(sz), sz, ctypes.c_char_p,
+                                                 ctypes.c_int, ctypes.POINTER(_ShaOut)]),
         "stark_host_build_sha256_trace": (sz, [ctypes.c_char_p, sz, u64p, ctypes.c_char_p]),
     }
     for name, (res, args) in sig.items():
@@ -351,6 +362,74 @@ def deep_ali_merge_sha256(trace_evals_on_lde, combination_coeffs, omega, n_trace
     phi_b = 2 * n_trace
     qb = phi_b - n_trace
     return out, CompositionInfo(phi_b, qb, qb / n, SHA256_CONSTRAINTS, 2, SHA256_WIDTH)
+
+
+SHA512_WIDTH, SHA512_CONSTRAINTS = 1510, 1526
+
+
+def build_sha512_trace_multi(message):
+    """build_sha512_trace_multi (sha512_air.rs:551-573), host side: returns
+    (trace as a 1510 x n_trace uint64 array, n_blocks)."""
+    message = bytes(message)
+    h = lib().stark_host_sha512_trace_height(len(message))
+    tr = np.zeros((SHA512_WIDTH, h), dtype=np.uint64)
+    dg = ctypes.create_string_buffer(64)
+    nb = lib().stark_host_build_sha512_trace(message, len(message), tr.ctypes.data_as(u64p), dg)
+    return tr, nb
+
+
+def deep_ali_merge_sha512(trace_evals_on_lde, combination_coeffs, omega, n_trace, blowup, n_blocks,
+                          phi_mode=PHI_RAYON, ctx=None):
+    """deep_ali_merge_sha512 (lib.rs:561-653)."""
+    n = n_trace * blowup
+    lde = np.asarray(trace_evals_on_lde, dtype=np.uint64)
+    assert lde.shape[0] == SHA512_WIDTH, "trace width mismatch"
+    assert len(combination_coeffs) == SHA512_CONSTRAINTS, \
+        "need one combination coefficient per constraint, got %d for %d constraints" % (
+            len(combination_coeffs), SHA512_CONSTRAINTS)
+    assert lde.shape[1] == n, "trace column length mismatch"
+    del omega
+    ctx = ctx or default_context()
+    lde, lp = _u64(lde)
+    co, cp = _u64(np.asarray(combination_coeffs, dtype=np.uint64))
+    out = np.empty(n, dtype=np.uint64)
+    ctx._check(lib().stark_merge_sha512(ctx.h, lp, n_trace, blowup, n_blocks, cp, SHA512_CONSTRAINTS, phi_mode,
+                                        out.ctypes.data_as(u64p)))
+    return out, CompositionInfo(2 * n_trace, n_trace, n_trace / n, SHA512_CONSTRAINTS, 2, SHA512_WIDTH)
+
+
+@dataclass
+class ShaProof:
+    digest: bytes
+    root_f0: bytes
+    n_blocks: int
+    n_trace: int
+    proof_blob: bytes
+
+
+def prove_sha_batch(alg, messages, fs_bindings, heights=None, phi_mode=PHI_RAYON, ctx=None):
+    """SHA-256 (alg=256) / SHA-512 (alg=512) AIR proofs with the
+    prove_ds_ksk_binding pipeline (stark_prove_sha_batch); heights[i] is the
+    trace height of message i (None: the minimum)."""
+    ctx = ctx or default_context()
+    msgs = [bytes(m) for m in messages]
+    n = len(msgs)
+    assert len(fs_bindings) == n and (heights is None or len(heights) == n)
+    offs, acc = [], 0
+    for m in msgs:
+        offs.append(acc)
+        acc += len(m)
+    outs = (_ShaOut * max(1, n))()
+    hs = (sz * max(1, n))(*heights) if heights is not None else None
+    ctx._check(lib().stark_prove_sha_batch(ctx.h, alg, b"".join(msgs), (sz * max(1, n))(*offs),
+                                           (sz * max(1, n))(*[len(m) for m in msgs]), hs, n,
+                                           b"".join(bytes(f) for f in fs_bindings), phi_mode, outs))
+    dl = 32 if alg == 256 else 64
+    res = []
+    for k in range(n):
+        o = outs[k]
+        res.append(ShaProof(bytes(o.digest)[:dl], bytes(o.root_f0), o.n_blocks, o.n_trace, _take(o.proof, o.proof_len)))
+    return res
 
 
 # --------------------------------------------------------------------- FRI

@@ -234,6 +234,51 @@ int stark_merge_sha256(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, size
     });
 }
 
+int stark_merge_sha512(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_blocks,
+                       const uint64_t *coeffs, size_t k, int phi_mode, uint64_t *c_eval) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(lde && coeffs && c_eval, "merge_sha512: null argument");
+        SG_REQUIRE(k == sg::SHA512_K, "merge_sha512: need one combination coefficient per constraint (1526)");
+        for (size_t j = 0; j < k; j++)
+            SG_REQUIRE(coeffs[j] == j + 1, "merge_sha512: only the reference combination coefficients (j + 1)");
+        SG_REQUIRE(phi_mode >= 0 && phi_mode <= 2, "merge_sha512: bad phi_mode");
+        SG_REQUIRE(n_blocks >= 1 && n_blocks <= 0xFFFFFFFFu, "merge_sha512: bad n_blocks");
+        const size_t n0 = n_trace * blowup;
+        sg::Scoped dl(c, 8ULL * sg::SHA512_W * n0), dc(c, 8 * n0);
+        h2d(c, dl.p, lde, 8ULL * sg::SHA512_W * n0);
+        sg::merge_sha512_batch(c, dl.as<uint64_t>(), 1, n_trace, blowup, {(uint32_t)n_blocks}, phi_mode,
+                               dc.as<uint64_t>());
+        d2h(c, c_eval, dc.p, 8 * n0);
+    });
+}
+
+int stark_prove_sha_batch(stark_ctx *ctx, int alg, const uint8_t *msgs, const size_t *offsets, const size_t *lens,
+                          const size_t *heights, size_t n, const uint8_t *fs_bindings, int phi_mode,
+                          stark_sha_out *outs) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(offsets && lens && fs_bindings && outs, "prove_sha_batch: null argument");
+        SG_REQUIRE(phi_mode >= 0 && phi_mode <= 2, "prove_sha_batch: bad phi_mode");
+        std::vector<sg::ByteSpan> m(n);
+        std::vector<uint64_t> h(n, 0);
+        std::vector<std::array<uint8_t, 32>> fsb(n);
+        for (size_t i = 0; i < n; i++) {
+            SG_REQUIRE(msgs || lens[i] == 0, "prove_sha_batch: null messages");
+            m[i] = {msgs ? msgs + offsets[i] : nullptr, lens[i]};
+            if (heights) h[i] = heights[i];
+            memcpy(fsb[i].data(), fs_bindings + 32 * i, 32);
+            outs[i].proof = nullptr;
+        }
+        std::vector<sg::ShaProof> r = sg::prove_sha_batch(c, alg, m, h, fsb, phi_mode);
+        for (size_t i = 0; i < n; i++) {
+            memcpy(outs[i].digest, r[i].digest, 64);
+            memcpy(outs[i].root_f0, r[i].root_f0, 32);
+            outs[i].n_blocks = r[i].n_blocks;
+            outs[i].n_trace = r[i].n_trace;
+            outs[i].proof = copy_out(r[i].proof, &outs[i].proof_len);
+        }
+    });
+}
+
 static void fill_ds_ksk(const sg::DsKsk &r, stark_ds_ksk_out *o) {
     memcpy(o->pi_hash, r.pi_hash, 32);
     memcpy(o->asserted_digest, r.asserted_digest, 32);
@@ -415,6 +460,12 @@ size_t stark_host_sha256_trace_height(size_t len) { return sg::sha256_trace_heig
 
 size_t stark_host_build_sha256_trace(const uint8_t *msg, size_t len, uint64_t *trace, uint8_t digest[32]) {
     return sg::build_sha256_trace(msg, len, trace, digest);
+}
+
+size_t stark_host_sha512_trace_height(size_t len) { return sg::sha512_trace_height(len); }
+
+size_t stark_host_build_sha512_trace(const uint8_t *msg, size_t len, uint64_t *trace, uint8_t digest[64]) {
+    return sg::build_sha512_trace(msg, len, trace, digest);
 }
 
 void stark_host_dns_leaf_hash(const stark_dns_record *r, const uint8_t salt[16], uint8_t out[32]) {

@@ -259,45 +259,66 @@ static void ds_ksk_pi_hash(const ByteSpan &key, const uint8_t parent[32], const 
     h.final(out);
 }
 
-std::vector<DsKsk> prove_ds_ksk_binding_batch(Ctx *c, const std::vector<ByteSpan> &keys,
-                                              const std::vector<std::array<uint8_t, 32>> &parents,
-                                              const std::vector<std::array<uint8_t, 32>> &fsbs, int phi_mode) {
-    const size_t K = keys.size();
-    SG_REQUIRE(parents.size() == K && fsbs.size() == K, "prove_ds_ksk_binding: one parent hash and binding per key");
-    std::vector<DsKsk> out(K);
-    std::map<uint64_t, std::vector<size_t>> groups;  // trace height -> keys
+std::vector<ShaProof> prove_sha_batch(Ctx *c, int alg, const std::vector<ByteSpan> &msgs,
+                                      const std::vector<uint64_t> &heights,
+                                      const std::vector<std::array<uint8_t, 32>> &fsbs, int phi_mode) {
+    SG_REQUIRE(alg == 256 || alg == 512, "prove_sha: alg must be 256 or 512");
+    const size_t K = msgs.size();
+    SG_REQUIRE(heights.size() == K && fsbs.size() == K, "prove_sha: one height and binding per message");
+    const uint64_t W = alg == 256 ? SHA256_W : SHA512_W;
+    std::vector<ShaProof> out(K);
+    std::map<uint64_t, std::vector<size_t>> groups;  // trace height -> inputs
     for (size_t k = 0; k < K; k++) {
-        SG_REQUIRE(keys[k].p || keys[k].n == 0, "prove_ds_ksk_binding: null key");
-        const uint64_t h = sha256_trace_height(keys[k].n);
-        SG_REQUIRE(h * SWARM_BLOWUP <= (1ULL << 24), "prove_ds_ksk_binding: key too long (domain above 2^24)");
+        SG_REQUIRE(msgs[k].p || msgs[k].n == 0, "prove_sha: null message");
+        const uint64_t hmin = alg == 256 ? sha256_trace_height(msgs[k].n) : sha512_trace_height(msgs[k].n);
+        const uint64_t h = heights[k] ? heights[k] : hmin;
+        SG_REQUIRE(is_pow2(h) && h >= hmin, "prove_sha: trace height below the message's minimum");
+        SG_REQUIRE(h * SWARM_BLOWUP <= (1ULL << 24), "prove_sha: domain above 2^24");
         groups[h].push_back(k);
     }
+    size_t free_b = 0, total_b = 0;
+    SG_HIP(hipMemGetInfo(&free_b, &total_b));
+    const uint64_t budget = (uint64_t)((free_b + c->cached_bytes) * 0.6);
     for (auto &g : groups) {
         const uint64_t h = g.first, n0 = h * SWARM_BLOWUP;
-        // lockstep chunks bounded by HBM: trace + LDE + merge buffers per proof
-        const uint64_t per = 8ULL * SHA256_W * (h + n0) + 8ULL * 8 * n0;
-        const size_t chunk = (size_t)std::max<uint64_t>(1, std::min<uint64_t>(4096, (24ULL << 30) / per));
+        // per proof: trace + its coefficients, LDE + the LDE pass scratch, merge and FRI buffers
+        const uint64_t per = 8ULL * W * (2 * h + 2 * n0) + 8ULL * 16 * n0;
+        SG_REQUIRE(per <= budget, "prove_sha: one proof does not fit in device memory");
+        const size_t chunk = (size_t)std::min<uint64_t>(4096, budget / per);
         for (size_t c0 = 0; c0 < g.second.size(); c0 += chunk) {
             const std::vector<size_t> ids(g.second.begin() + c0,
                                           g.second.begin() + std::min(g.second.size(), c0 + chunk));
             const uint32_t G = (uint32_t)ids.size();
-            std::vector<uint64_t> tr((size_t)SHA256_W * h * G);
+            std::vector<uint64_t> tr((size_t)W * h * G);
             std::vector<uint32_t> nb(G);
             {
-                HostScope hs(c, "sha256_trace");
+                HostScope hs(c, "sha_trace");
                 parallel_for(G, [&](size_t j) {
-                    DsKsk &o = out[ids[j]];
-                    nb[j] = (uint32_t)build_sha256_trace(keys[ids[j]].p, keys[ids[j]].n,
-                                                         &tr[(size_t)SHA256_W * h * j], o.asserted_digest);
+                    ShaProof &o = out[ids[j]];
+                    const ByteSpan &m = msgs[ids[j]];
+                    nb[j] = (uint32_t)(alg == 256 ? build_sha256_trace(m.p, m.n, &tr[(size_t)W * h * j], o.digest, h)
+                                                  : build_sha512_trace(m.p, m.n, &tr[(size_t)W * h * j], o.digest, h));
                     o.n_blocks = nb[j];
                     o.n_trace = h;
                 });
             }
-            Scoped d_tr(c, 8 * tr.size()), d_lde(c, 8ULL * SHA256_W * n0 * G), d_c(c, 8ULL * n0 * G);
-            SG_HIP(hipMemcpyAsync(d_tr.p, tr.data(), 8 * tr.size(), hipMemcpyHostToDevice, c->stream));
-            lde_columns(c, d_tr.as<uint64_t>(), d_lde.as<uint64_t>(), ilog2(h), ilog2(SWARM_BLOWUP),
-                        (uint64_t)SHA256_W * G, h, n0, 1);
-            merge_sha256_batch(c, d_lde.as<uint64_t>(), G, h, SWARM_BLOWUP, nb, phi_mode, d_c.as<uint64_t>());
+            Scoped d_c(c, 8ULL * n0 * G);
+            {
+                Scoped d_lde(c, 8ULL * W * n0 * G);
+                {
+                    Scoped d_tr(c, 8 * tr.size());
+                    SG_HIP(hipMemcpyAsync(d_tr.p, tr.data(), 8 * tr.size(), hipMemcpyHostToDevice, c->stream));
+                    lde_columns(c, d_tr.as<uint64_t>(), d_lde.as<uint64_t>(), ilog2(h), ilog2(SWARM_BLOWUP), W * G,
+                                h, n0, 1);
+                }
+                if (alg == 256)
+                    merge_sha256_batch(c, d_lde.as<uint64_t>(), G, h, SWARM_BLOWUP, nb, phi_mode, d_c.as<uint64_t>());
+                else
+                    merge_sha512_batch(c, d_lde.as<uint64_t>(), G, h, SWARM_BLOWUP, nb, phi_mode, d_c.as<uint64_t>());
+            }
+            // a wave that took much of HBM hands its cached blocks back, so the
+            // next wave's different sizes do not pile up beside them
+            if (per * G > budget / 2) c->trim();
             FriParams p;
             p.schedule = stir_schedule(n0);
             p.r = p.s0 = SWARM_QUERIES;
@@ -310,13 +331,30 @@ std::vector<DsKsk> prove_ds_ksk_binding_batch(Ctx *c, const std::vector<ByteSpan
             for (uint32_t j = 0; j < G; j++) pis[j] = fsbs[ids[j]];
             std::vector<std::vector<uint8_t>> proofs = deep_fri_prove_batch(c, d_c.as<uint64_t>(), G, n0, p, &pis, nullptr);
             for (uint32_t j = 0; j < G; j++) {
-                DsKsk &o = out[ids[j]];
+                ShaProof &o = out[ids[j]];
                 o.proof = std::move(proofs[j]);
                 memcpy(o.root_f0, o.proof.data(), 32);
-                ds_ksk_pi_hash(keys[ids[j]], parents[ids[j]].data(), o.asserted_digest, o.root_f0,
-                               fsbs[ids[j]].data(), o.pi_hash);
             }
         }
+    }
+    return out;
+}
+
+std::vector<DsKsk> prove_ds_ksk_binding_batch(Ctx *c, const std::vector<ByteSpan> &keys,
+                                              const std::vector<std::array<uint8_t, 32>> &parents,
+                                              const std::vector<std::array<uint8_t, 32>> &fsbs, int phi_mode) {
+    const size_t K = keys.size();
+    SG_REQUIRE(parents.size() == K && fsbs.size() == K, "prove_ds_ksk_binding: one parent hash and binding per key");
+    std::vector<ShaProof> sp = prove_sha_batch(c, 256, keys, std::vector<uint64_t>(K, 0), fsbs, phi_mode);
+    std::vector<DsKsk> out(K);
+    for (size_t k = 0; k < K; k++) {
+        DsKsk &o = out[k];
+        memcpy(o.asserted_digest, sp[k].digest, 32);
+        memcpy(o.root_f0, sp[k].root_f0, 32);
+        o.n_blocks = sp[k].n_blocks;
+        o.n_trace = sp[k].n_trace;
+        o.proof = std::move(sp[k].proof);
+        ds_ksk_pi_hash(keys[k], parents[k].data(), o.asserted_digest, o.root_f0, fsbs[k].data(), o.pi_hash);
     }
     return out;
 }

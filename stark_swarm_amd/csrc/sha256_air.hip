@@ -339,8 +339,9 @@ uint64_t sha256_trace_height(uint64_t len) {
 // block copy the last useful row. Also returns the digest as
 // prove_ds_ksk_binding reads it (prover.rs:529-538): H-state at block row 65
 // of the last block, big-endian words.
-uint64_t build_sha256_trace(const uint8_t *msg, uint64_t len, uint64_t *tr, uint8_t digest[32]) {
-    const uint64_t nb = sha256_n_blocks(len), h = sha256_trace_height(len);
+uint64_t build_sha256_trace(const uint8_t *msg, uint64_t len, uint64_t *tr, uint8_t digest[32], uint64_t height) {
+    const uint64_t nb = sha256_n_blocks(len), h = height ? height : sha256_trace_height(len);
+    SG_REQUIRE(is_pow2(h) && h >= 128 * nb, "sha256 trace: height must be a power of two >= 128 * n_blocks");
     std::vector<uint8_t> m(64 * nb, 0);
     if (len) memcpy(m.data(), msg, len);
     m[len] = 0x80;

@@ -13,14 +13,26 @@ namespace sg {
 constexpr uint32_t SHA256_W = 756;  // trace columns (sha256_air.rs:248)
 constexpr uint32_t SHA256_K = 766;  // transition constraints (sha256_air.rs:288)
 
+constexpr uint32_t SHA512_W = 1510;  // sha512_air.rs:478
+constexpr uint32_t SHA512_K = 1526;  // sha512_air.rs:477
+
 uint64_t sha256_n_blocks(uint64_t len);
 uint64_t sha256_trace_height(uint64_t len);
 // 756 x height column-major trace of msg; returns n_blocks, digest = the
-// SHA-256 of msg as read from the trace's H-state
-uint64_t build_sha256_trace(const uint8_t *msg, uint64_t len, uint64_t *tr, uint8_t digest[32]);
+// SHA-256 of msg as read from the trace's H-state. height 0 = the minimum
+// (next_pow2(128 n_blocks)); a larger power of two pads by repeating the
+// last useful row, the builder's own padding rule.
+uint64_t build_sha256_trace(const uint8_t *msg, uint64_t len, uint64_t *tr, uint8_t digest[32], uint64_t height = 0);
+uint64_t sha512_n_blocks(uint64_t len);
+uint64_t sha512_trace_height(uint64_t len);
+// 1510 x height trace (two 32-bit limbs per word); digest = SHA-512(msg)
+uint64_t build_sha512_trace(const uint8_t *msg, uint64_t len, uint64_t *tr, uint8_t digest[64], uint64_t height = 0);
 // deep_ali_merge_sha256 (lib.rs:453-545) for K proofs of one n_trace: lde of
 // proof k at lde + 756*k*n0, c_eval at c_eval + k*n0
 void merge_sha256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup,
+                        const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval);
+// deep_ali_merge_sha512 (lib.rs:561-653), same batching
+void merge_sha512_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup,
                         const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval);
 
 }  // namespace sg

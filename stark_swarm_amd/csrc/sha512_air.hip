@@ -1,0 +1,436 @@
+// sha512_air.hip -- the SHA-512 AIR (1510 columns, 1526 transition
+// constraints, every 64-bit word as two 32-bit limbs): the host trace builder
+// and the composition kernel on gfx950.
+//
+//   column layout          crates/deep_ali/src/sha512_air.rs:313-481
+//   trace builder          sha512_air.rs:218-236, 551-573, 585-898
+//   constraint evaluator   sha512_air.rs:909-1282
+//   merge                  crates/deep_ali/src/lib.rs:561-653
+//
+// Same structure as the SHA-256 path: one LDE point per lane, the
+// constraints folded into phi = sum_j (j+1) C_j as they are produced.
+#include <string.h>
+
+#include "air.hpp"
+#include "gl.hpp"
+#include "ntt.hpp"
+#include "sha256_air.hpp"
+
+namespace sg {
+
+namespace sha5 {
+
+enum : uint32_t {
+    A0 = 0,
+    BIT_A = 16, BIT_B = 80, BIT_C = 144, BIT_E = 208, BIT_F = 272, BIT_G = 336,
+    S0_T = 400, S0_BIT = 464, S0_LO = 528,
+    S1_T = 530, S1_BIT = 594, S1_LO = 658,
+    CH_BIT = 660, CH_LO = 724,
+    MAJ_AB = 726, MAJ_AC = 790, MAJ_BC = 854, MAJ_U = 918, MAJ_BIT = 982, MAJ_LO = 1046,
+    W_LO = 1048,
+    T1L_LO = 1050, T1L_C0 = 1051, T1L_CW = 1054,
+    T1H_LO = 1055, T1H_C0 = 1056, T1H_CW = 1059,
+    T2L_LO = 1060, T2L_C0 = 1061, T2L_CW = 1062, T2H_LO = 1063, T2H_C0 = 1064, T2H_CW = 1065,
+    H0 = 1066, WW0 = 1082, BIT_WW14 = 1112, BIT_WW1 = 1176,
+    SS0_T = 1240, SS0_BIT = 1304, SS0_LO = 1368,
+    SS1_T = 1370, SS1_BIT = 1434, SS1_LO = 1498,
+    WSL_C0 = 1500, WSL_CW = 1502, WSH_C0 = 1503, WSH_CW = 1505,
+    NA_LO = 1506, NA_HI = 1507, NE_LO = 1508, NE_HI = 1509,
+};
+
+#define SG_SHA512_K 0x428a2f98d728ae22ull, 0x7137449123ef65cdull, 0xb5c0fbcfec4d3b2full, 0xe9b5dba58189dbbcull, \
+    0x3956c25bf348b538ull, 0x59f111f1b605d019ull, 0x923f82a4af194f9bull, 0xab1c5ed5da6d8118ull, \
+    0xd807aa98a3030242ull, 0x12835b0145706fbeull, 0x243185be4ee4b28cull, 0x550c7dc3d5ffb4e2ull, \
+    0x72be5d74f27b896full, 0x80deb1fe3b1696b1ull, 0x9bdc06a725c71235ull, 0xc19bf174cf692694ull, \
+    0xe49b69c19ef14ad2ull, 0xefbe4786384f25e3ull, 0x0fc19dc68b8cd5b5ull, 0x240ca1cc77ac9c65ull, \
+    0x2de92c6f592b0275ull, 0x4a7484aa6ea6e483ull, 0x5cb0a9dcbd41fbd4ull, 0x76f988da831153b5ull, \
+    0x983e5152ee66dfabull, 0xa831c66d2db43210ull, 0xb00327c898fb213full, 0xbf597fc7beef0ee4ull, \
+    0xc6e00bf33da88fc2ull, 0xd5a79147930aa725ull, 0x06ca6351e003826full, 0x142929670a0e6e70ull, \
+    0x27b70a8546d22ffcull, 0x2e1b21385c26c926ull, 0x4d2c6dfc5ac42aedull, 0x53380d139d95b3dfull, \
+    0x650a73548baf63deull, 0x766a0abb3c77b2a8ull, 0x81c2c92e47edaee6ull, 0x92722c851482353bull, \
+    0xa2bfe8a14cf10364ull, 0xa81a664bbc423001ull, 0xc24b8b70d0f89791ull, 0xc76c51a30654be30ull, \
+    0xd192e819d6ef5218ull, 0xd69906245565a910ull, 0xf40e35855771202aull, 0x106aa07032bbd1b8ull, \
+    0x19a4c116b8d2d0c8ull, 0x1e376c085141ab53ull, 0x2748774cdf8eeb99ull, 0x34b0bcb5e19b48a8ull, \
+    0x391c0cb3c5c95a63ull, 0x4ed8aa4ae3418acbull, 0x5b9cca4f7763e373ull, 0x682e6ff3d6b2b8a3ull, \
+    0x748f82ee5defb2fcull, 0x78a5636f43172f60ull, 0x84c87814a1f0ab72ull, 0x8cc702081a6439ecull, \
+    0x90befffa23631e28ull, 0xa4506cebde82bde9ull, 0xbef9a3f7b2c67915ull, 0xc67178f2e372532bull, \
+    0xca273eceea26619cull, 0xd186b8c721c0c207ull, 0xeada7dd6cde0eb1eull, 0xf57d4f7fee6ed178ull, \
+    0x06f067aa72176fbaull, 0x0a637dc5a2c898a6ull, 0x113f9804bef90daeull, 0x1b710b35131c471bull, \
+    0x28db77f523047d84ull, 0x32caab7b40c72493ull, 0x3c9ebe0a15c9bebcull, 0x431d67c49c100d4cull, \
+    0x4cc5d4becb3e42b6ull, 0x597f299cfc657e2aull, 0x5fcb6fab3ad6faecull, 0x6c44198c4a475817ull
+
+static const uint64_t KH[80] = {SG_SHA512_K};  // FIPS 180-4 section 4.2.3
+__constant__ uint64_t KD[80] = {SG_SHA512_K};
+static const uint64_t IV[8] = {0x6a09e667f3bcc908ull, 0xbb67ae8584caa73bull, 0x3c6ef372fe94f82bull, 0xa54ff53a5f1d36f1ull, 0x510e527fade682d1ull, 0x9b05688c2b3e6c1full, 0x1f83d9abfb41bd6bull, 0x5be0cd19137e2179ull};  // section 5.3.5
+
+inline uint64_t rotr(uint64_t x, int n) { return (x >> n) | (x << (64 - n)); }
+inline uint64_t big0(uint64_t x) { return rotr(x, 28) ^ rotr(x, 34) ^ rotr(x, 39); }
+inline uint64_t big1(uint64_t x) { return rotr(x, 14) ^ rotr(x, 18) ^ rotr(x, 41); }
+inline uint64_t small0(uint64_t x) { return rotr(x, 1) ^ rotr(x, 8) ^ (x >> 7); }
+inline uint64_t small1(uint64_t x) { return rotr(x, 19) ^ rotr(x, 61) ^ (x >> 6); }
+inline uint64_t chf(uint64_t e, uint64_t f, uint64_t g) { return (e & f) ^ (~e & g); }
+inline uint64_t majf(uint64_t a, uint64_t b, uint64_t c) { return (a & b) ^ (a & c) ^ (b & c); }
+inline uint64_t lo(uint64_t x) { return x & 0xFFFFFFFFu; }
+inline uint64_t hi(uint64_t x) { return x >> 32; }
+
+struct RowWords {
+    uint64_t st[8];                   // a..h
+    uint64_t s0t, s1t, chw, mjw, wt;  // rotr-xor halves of Sigma0/Sigma1, Ch, Maj, W[t]
+    uint64_t t1l, t1h, t2l, t2h;      // per-limb T1 / T2 sums (lo carry folded into hi)
+    uint64_t hs[8], ww[15];           // H-state, W window
+    uint64_t ss0t, ss1t;              // rotr-xor halves of sigma0(W[t-14]), sigma1(W[t-1])
+    uint64_t sl, sh;                  // per-limb schedule sums
+    uint64_t na_l, na_h, ne_l, ne_h;  // new_a / new_e limb carries
+};
+
+// One 128-row block (sha512_air.rs:585-898): rows 0..79 rounds, row 80 the
+// post-compression state, rows 81..127 idle; H-state = incoming IV through
+// row 80, the block digest from row 81. Written column by column.
+std::array<uint64_t, 8> fill_block(uint64_t *tr, uint64_t h, uint64_t base, const uint8_t *blk,
+                                   const uint64_t iv[8]) {
+    uint64_t w[80];
+    for (int i = 0; i < 16; i++) {
+        uint64_t v = 0;
+        for (int b = 0; b < 8; b++) v = v << 8 | blk[8 * i + b];
+        w[i] = v;
+    }
+    for (int t = 16; t < 80; t++) w[t] = small1(w[t - 2]) + w[t - 7] + small0(w[t - 15]) + w[t - 16];
+    auto wat = [&](int t) { return (t >= 0 && t < 80) ? w[t] : 0ull; };
+    uint64_t st[81][8];
+    for (int k = 0; k < 8; k++) st[0][k] = iv[k];
+    for (int t = 0; t < 80; t++) {
+        const uint64_t *q = st[t];
+        const uint64_t t1 = q[7] + big1(q[4]) + chf(q[4], q[5], q[6]) + KH[t] + w[t];
+        const uint64_t t2 = big0(q[0]) + majf(q[0], q[1], q[2]);
+        const uint64_t nx[8] = {t1 + t2, q[0], q[1], q[2], q[3] + t1, q[4], q[5], q[6]};
+        memcpy(st[t + 1], nx, 64);
+    }
+    uint64_t dg[8];
+    for (int k = 0; k < 8; k++) dg[k] = iv[k] + st[80][k];
+    static thread_local RowWords R[128];
+    for (int br = 0; br < 128; br++) {
+        RowWords &r = R[br];
+        memcpy(r.st, st[br <= 80 ? br : 80], 64);
+        const uint64_t a = r.st[0], b = r.st[1], c = r.st[2], d = r.st[3], e = r.st[4], f = r.st[5], g = r.st[6];
+        r.s0t = rotr(a, 28) ^ rotr(a, 34);
+        r.s1t = rotr(e, 14) ^ rotr(e, 18);
+        r.chw = chf(e, f, g);
+        r.mjw = majf(a, b, c);
+        r.wt = br < 80 ? w[br] : 0;
+        const uint64_t kt = br < 80 ? KH[br] : 0, s0 = big0(a), s1 = big1(e);
+        r.t1l = lo(r.st[7]) + lo(s1) + lo(r.chw) + lo(kt) + lo(r.wt);
+        r.t1h = hi(r.st[7]) + hi(s1) + hi(r.chw) + hi(kt) + hi(r.wt) + (r.t1l >> 32);
+        r.t2l = lo(s0) + lo(r.mjw);
+        r.t2h = hi(s0) + hi(r.mjw) + (r.t2l >> 32);
+        memcpy(r.hs, br <= 80 ? iv : dg, 64);
+        for (int k = 0; k < 15; k++) r.ww[k] = wat(br - 15 + k);
+        const uint64_t ww14 = r.ww[14], ww1 = r.ww[1], ss0 = small0(ww1), ss1 = small1(ww14), w9 = wat(br - 6);
+        r.ss0t = rotr(ww1, 1) ^ rotr(ww1, 8);
+        r.ss1t = rotr(ww14, 19) ^ rotr(ww14, 61);
+        r.sl = lo(r.ww[0]) + lo(ss0) + lo(w9) + lo(ss1);
+        r.sh = hi(r.ww[0]) + hi(ss0) + hi(w9) + hi(ss1) + (r.sl >> 32);
+        if (br < 80) {
+            const uint64_t nal = lo(r.t1l) + lo(r.t2l), nah = lo(r.t1h) + lo(r.t2h) + (nal >> 32);
+            const uint64_t nel = lo(d) + lo(r.t1l), neh = hi(d) + lo(r.t1h) + (nel >> 32);
+            r.na_l = nal >> 32;
+            r.na_h = nah >> 32;
+            r.ne_l = nel >> 32;
+            r.ne_h = neh >> 32;
+        } else {
+            r.na_l = r.na_h = r.ne_l = r.ne_h = 0;
+        }
+    }
+    auto col = [&](uint32_t cc, auto fn) {
+        uint64_t *o = tr + (uint64_t)cc * h + base;
+        for (int br = 0; br < 128; br++) o[br] = fn(R[br]);
+    };
+    auto bits = [&](uint32_t c0, auto word) {
+        for (int i = 0; i < 64; i++) col(c0 + i, [&](const RowWords &r) { return (word(r) >> i) & 1; });
+    };
+    auto limbs = [&](uint32_t c0, auto word) {
+        col(c0, [&](const RowWords &r) { return lo(word(r)); });
+        col(c0 + 1, [&](const RowWords &r) { return hi(word(r)); });
+    };
+    auto carry = [&](uint32_t c0, int nbits, auto sum) {  // bits of sum >> 32, then the packed carry
+        for (int k = 0; k < nbits; k++) col(c0 + k, [&](const RowWords &r) { return (sum(r) >> (32 + k)) & 1; });
+        col(c0 + nbits, [&](const RowWords &r) { return sum(r) >> 32; });
+    };
+    for (uint32_t k = 0; k < 8; k++) limbs(A0 + 2 * k, [&](const RowWords &r) { return r.st[k]; });
+    bits(BIT_A, [](const RowWords &r) { return r.st[0]; });
+    bits(BIT_B, [](const RowWords &r) { return r.st[1]; });
+    bits(BIT_C, [](const RowWords &r) { return r.st[2]; });
+    bits(BIT_E, [](const RowWords &r) { return r.st[4]; });
+    bits(BIT_F, [](const RowWords &r) { return r.st[5]; });
+    bits(BIT_G, [](const RowWords &r) { return r.st[6]; });
+    bits(S0_T, [](const RowWords &r) { return r.s0t; });
+    bits(S0_BIT, [](const RowWords &r) { return r.s0t ^ rotr(r.st[0], 39); });
+    limbs(S0_LO, [](const RowWords &r) { return r.s0t ^ rotr(r.st[0], 39); });
+    bits(S1_T, [](const RowWords &r) { return r.s1t; });
+    bits(S1_BIT, [](const RowWords &r) { return r.s1t ^ rotr(r.st[4], 41); });
+    limbs(S1_LO, [](const RowWords &r) { return r.s1t ^ rotr(r.st[4], 41); });
+    bits(CH_BIT, [](const RowWords &r) { return r.chw; });
+    limbs(CH_LO, [](const RowWords &r) { return r.chw; });
+    bits(MAJ_AB, [](const RowWords &r) { return r.st[0] & r.st[1]; });
+    bits(MAJ_AC, [](const RowWords &r) { return r.st[0] & r.st[2]; });
+    bits(MAJ_BC, [](const RowWords &r) { return r.st[1] & r.st[2]; });
+    bits(MAJ_U, [](const RowWords &r) { return (r.st[0] & r.st[1]) ^ (r.st[0] & r.st[2]); });
+    bits(MAJ_BIT, [](const RowWords &r) { return r.mjw; });
+    limbs(MAJ_LO, [](const RowWords &r) { return r.mjw; });
+    limbs(W_LO, [](const RowWords &r) { return r.wt; });
+    col(T1L_LO, [](const RowWords &r) { return lo(r.t1l); });
+    carry(T1L_C0, 3, [](const RowWords &r) { return r.t1l; });
+    col(T1H_LO, [](const RowWords &r) { return lo(r.t1h); });
+    carry(T1H_C0, 3, [](const RowWords &r) { return r.t1h; });
+    col(T2L_LO, [](const RowWords &r) { return lo(r.t2l); });
+    carry(T2L_C0, 1, [](const RowWords &r) { return r.t2l; });
+    col(T2H_LO, [](const RowWords &r) { return lo(r.t2h); });
+    carry(T2H_C0, 1, [](const RowWords &r) { return r.t2h; });
+    for (uint32_t k = 0; k < 8; k++) limbs(H0 + 2 * k, [&](const RowWords &r) { return r.hs[k]; });
+    for (uint32_t k = 0; k < 15; k++) limbs(WW0 + 2 * k, [&](const RowWords &r) { return r.ww[k]; });
+    bits(BIT_WW14, [](const RowWords &r) { return r.ww[14]; });
+    bits(BIT_WW1, [](const RowWords &r) { return r.ww[1]; });
+    bits(SS0_T, [](const RowWords &r) { return r.ss0t; });
+    bits(SS0_BIT, [](const RowWords &r) { return r.ss0t ^ (r.ww[1] >> 7); });
+    limbs(SS0_LO, [](const RowWords &r) { return r.ss0t ^ (r.ww[1] >> 7); });
+    bits(SS1_T, [](const RowWords &r) { return r.ss1t; });
+    bits(SS1_BIT, [](const RowWords &r) { return r.ss1t ^ (r.ww[14] >> 6); });
+    limbs(SS1_LO, [](const RowWords &r) { return r.ss1t ^ (r.ww[14] >> 6); });
+    carry(WSL_C0, 2, [](const RowWords &r) { return r.sl; });
+    carry(WSH_C0, 2, [](const RowWords &r) { return r.sh; });
+    col(NA_LO, [](const RowWords &r) { return r.na_l; });
+    col(NA_HI, [](const RowWords &r) { return r.na_h; });
+    col(NE_LO, [](const RowWords &r) { return r.ne_l; });
+    col(NE_HI, [](const RowWords &r) { return r.ne_h; });
+    std::array<uint64_t, 8> post;
+    for (int k = 0; k < 8; k++) post[k] = st[80][k];
+    return post;
+}
+
+// ---- constraints (sha512_air.rs:909-1282), one LDE point per lane ----
+struct Phi {
+    uint64_t acc = 0, w = 1;
+    __device__ __forceinline__ void put(uint64_t v) {
+        acc = add(acc, mul(w, v));
+        w++;
+    }
+    __device__ __forceinline__ void skip(int k) { w += k; }
+};
+__device__ __forceinline__ uint64_t xorp(uint64_t x, uint64_t y) {
+    const uint64_t p = mul(x, y);
+    return sub(add(x, y), add(p, p));
+}
+__device__ __forceinline__ uint64_t boolc(uint64_t b) { return mul(b, sub(1, b)); }
+
+constexpr int AT = 128;
+
+__global__ __launch_bounds__(AT) void compose_sha512_k(const uint64_t *lde, uint64_t n, uint64_t blowup,
+                                                       const uint32_t *n_blocks, uint64_t *phi) {
+    const uint64_t i = (uint64_t)blockIdx.x * AT + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t *L = lde + (uint64_t)SHA512_W * n * blockIdx.y;
+    const uint64_t ni = (i + blowup) & (n - 1);
+    auto cur = [&](uint32_t c) { return L[(uint64_t)c * n + i]; };
+    auto nxt = [&](uint32_t c) { return L[(uint64_t)c * n + ni]; };
+    const uint64_t row = i / blowup, blk = row >> 7, br = row & 127;
+    const bool past = blk >= n_blocks[blockIdx.y];
+    const bool comp = !past && br < 80, fin = !past && br == 80, sched = !past && br >= 15 && br <= 78;
+    const uint64_t TWO32 = 1ULL << 32;
+    Phi f;
+    const uint32_t boff[8] = {BIT_A, BIT_B, BIT_C, BIT_E, BIT_F, BIT_G, BIT_WW14, BIT_WW1};
+#pragma unroll 1
+    for (int o = 0; o < 8; o++)
+#pragma unroll 4
+        for (int b = 0; b < 64; b++) f.put(boolc(cur(boff[o] + b)));
+    auto pack = [&](uint32_t off) {  // sum_{b<32} 2^b cell(off + b)
+        uint64_t s = 0;
+#pragma unroll 8
+        for (int b = 0; b < 32; b++) s = add(s, mul(1ULL << b, cur(off + b)));
+        return s;
+    };
+    const uint32_t wlo[8] = {A0 + 0, A0 + 2, A0 + 4, A0 + 8, A0 + 10, A0 + 12, WW0 + 28, WW0 + 2};
+#pragma unroll 1
+    for (int o = 0; o < 8; o++) {
+        f.put(sub(cur(wlo[o]), pack(boff[o])));
+        f.put(sub(cur(wlo[o] + 1), pack(boff[o] + 32)));
+    }
+    struct Sg {
+        uint32_t src, t, b, lo;
+        int r1, r2, r3;
+        bool shr;
+    };
+    const Sg sg[4] = {{BIT_A, S0_T, S0_BIT, S0_LO, 28, 34, 39, false},
+                      {BIT_E, S1_T, S1_BIT, S1_LO, 14, 18, 41, false},
+                      {BIT_WW1, SS0_T, SS0_BIT, SS0_LO, 1, 8, 7, true},
+                      {BIT_WW14, SS1_T, SS1_BIT, SS1_LO, 19, 61, 6, true}};
+#pragma unroll 1
+    for (int s = 0; s < 4; s++) {
+        const Sg &g = sg[s];
+#pragma unroll 2
+        for (int b = 0; b < 64; b++)
+            f.put(sub(cur(g.t + b), xorp(cur(g.src + (b + g.r1) % 64), cur(g.src + (b + g.r2) % 64))));
+#pragma unroll 2
+        for (int b = 0; b < 64; b++) {
+            const uint64_t third = g.shr ? (b + g.r3 < 64 ? cur(g.src + b + g.r3) : 0) : cur(g.src + (b + g.r3) % 64);
+            f.put(sub(cur(g.b + b), xorp(cur(g.t + b), third)));
+        }
+        f.put(sub(cur(g.lo), pack(g.b)));
+        f.put(sub(cur(g.lo + 1), pack(g.b + 32)));
+    }
+#pragma unroll 2
+    for (int b = 0; b < 64; b++) {
+        const uint64_t e = cur(BIT_E + b), ff = cur(BIT_F + b), g = cur(BIT_G + b);
+        f.put(sub(cur(CH_BIT + b), sub(add(mul(e, ff), g), mul(e, g))));
+    }
+    f.put(sub(cur(CH_LO), pack(CH_BIT)));
+    f.put(sub(cur(CH_LO + 1), pack(CH_BIT + 32)));
+#pragma unroll 2
+    for (int b = 0; b < 64; b++) {
+        const uint64_t a = cur(BIT_A + b), bb = cur(BIT_B + b), c = cur(BIT_C + b);
+        f.put(sub(cur(MAJ_AB + b), mul(a, bb)));
+        f.put(sub(cur(MAJ_AC + b), mul(a, c)));
+        f.put(sub(cur(MAJ_BC + b), mul(bb, c)));
+    }
+#pragma unroll 2
+    for (int b = 0; b < 64; b++) f.put(sub(cur(MAJ_U + b), xorp(cur(MAJ_AB + b), cur(MAJ_AC + b))));
+#pragma unroll 2
+    for (int b = 0; b < 64; b++) f.put(sub(cur(MAJ_BIT + b), xorp(cur(MAJ_U + b), cur(MAJ_BC + b))));
+    f.put(sub(cur(MAJ_LO), pack(MAJ_BIT)));
+    f.put(sub(cur(MAJ_LO + 1), pack(MAJ_BIT + 32)));
+    // T1 per limb (carry-in from the lo limb), T2 per limb
+    const uint64_t kt = comp ? KD[br] : 0;
+    auto carry3 = [&](uint32_t c0, uint32_t cw) {
+        f.put(boolc(cur(c0)));
+        f.put(boolc(cur(c0 + 1)));
+        f.put(boolc(cur(c0 + 2)));
+        f.put(sub(sub(sub(cur(cw), cur(c0)), mul(2, cur(c0 + 1))), mul(4, cur(c0 + 2))));
+    };
+    f.put(sub(add(cur(T1L_LO), mul(TWO32, cur(T1L_CW))),
+              add(add(add(add(cur(A0 + 14), cur(S1_LO)), cur(CH_LO)), kt & 0xFFFFFFFFu), cur(W_LO))));
+    carry3(T1L_C0, T1L_CW);
+    f.put(sub(add(cur(T1H_LO), mul(TWO32, cur(T1H_CW))),
+              add(add(add(add(add(cur(A0 + 15), cur(S1_LO + 1)), cur(CH_LO + 1)), kt >> 32), cur(W_LO + 1)),
+                  cur(T1L_CW))));
+    carry3(T1H_C0, T1H_CW);
+    f.put(sub(add(cur(T2L_LO), mul(TWO32, cur(T2L_CW))), add(cur(S0_LO), cur(MAJ_LO))));
+    f.put(boolc(cur(T2L_C0)));
+    f.put(sub(cur(T2L_CW), cur(T2L_C0)));
+    f.put(sub(add(cur(T2H_LO), mul(TWO32, cur(T2H_CW))), add(add(cur(S0_LO + 1), cur(MAJ_LO + 1)), cur(T2L_CW))));
+    f.put(boolc(cur(T2H_C0)));
+    f.put(sub(cur(T2H_CW), cur(T2H_C0)));
+    if (comp) {
+        // shifts b<-a, c<-b, d<-c, f<-e, g<-f, h<-g, both limbs
+        const uint32_t dst[6] = {1, 2, 3, 5, 6, 7};
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            f.put(sub(nxt(A0 + 2 * dst[k]), cur(A0 + 2 * (dst[k] - 1))));
+            f.put(sub(nxt(A0 + 2 * dst[k] + 1), cur(A0 + 2 * (dst[k] - 1) + 1)));
+        }
+        f.put(sub(sub(add(nxt(A0), mul(TWO32, cur(NA_LO))), cur(T1L_LO)), cur(T2L_LO)));
+        f.put(boolc(cur(NA_LO)));
+        f.put(sub(sub(sub(add(nxt(A0 + 1), mul(TWO32, cur(NA_HI))), cur(T1H_LO)), cur(T2H_LO)), cur(NA_LO)));
+        f.put(boolc(cur(NA_HI)));
+        f.put(sub(sub(add(nxt(A0 + 8), mul(TWO32, cur(NE_LO))), cur(A0 + 6)), cur(T1L_LO)));
+        f.put(boolc(cur(NE_LO)));
+        f.put(sub(sub(sub(add(nxt(A0 + 9), mul(TWO32, cur(NE_HI))), cur(A0 + 7)), cur(T1H_LO)), cur(NE_LO)));
+        f.put(boolc(cur(NE_HI)));
+    } else {
+        f.skip(20);
+    }
+    // H-state: finalisation with the lo carry delta_lo * 2^-32 folded into hi
+    constexpr uint64_t INV32 = 0xFFFFFFFE00000002ull;  // 2^-32 mod p
+    static_assert((unsigned __int128)INV32 * (1ull << 32) % P == 1, "2^-32");
+#pragma unroll 1
+    for (int k = 0; k < 8; k++) {
+        if (fin) {
+            const uint64_t dl = sub(add(cur(H0 + 2 * k), cur(A0 + 2 * k)), nxt(H0 + 2 * k));
+            const uint64_t dh = sub(add(add(cur(H0 + 2 * k + 1), cur(A0 + 2 * k + 1)), mul(dl, INV32)),
+                                    nxt(H0 + 2 * k + 1));
+            f.put(mul(dl, sub(dl, TWO32)));
+            f.put(mul(dh, sub(dh, TWO32)));
+        } else {
+            f.put(sub(nxt(H0 + 2 * k), cur(H0 + 2 * k)));
+            f.put(sub(nxt(H0 + 2 * k + 1), cur(H0 + 2 * k + 1)));
+        }
+    }
+#pragma unroll 2
+    for (int k = 0; k < 14; k++) {
+        f.put(sub(nxt(WW0 + 2 * k), cur(WW0 + 2 * k + 2)));
+        f.put(sub(nxt(WW0 + 2 * k + 1), cur(WW0 + 2 * k + 3)));
+    }
+    f.put(sub(nxt(WW0 + 28), cur(W_LO)));
+    f.put(sub(nxt(WW0 + 29), cur(W_LO + 1)));
+    if (sched) {
+        const uint64_t sl = add(add(add(cur(WW0), cur(SS0_LO)), cur(WW0 + 18)), cur(SS1_LO));
+        f.put(sub(add(nxt(W_LO), mul(TWO32, cur(WSL_CW))), sl));
+        f.put(boolc(cur(WSL_C0)));
+        f.put(boolc(cur(WSL_C0 + 1)));
+        f.put(sub(sub(cur(WSL_CW), cur(WSL_C0)), mul(2, cur(WSL_C0 + 1))));
+        const uint64_t sh = add(add(add(add(cur(WW0 + 1), cur(SS0_LO + 1)), cur(WW0 + 19)), cur(SS1_LO + 1)), cur(WSL_CW));
+        f.put(sub(add(nxt(W_LO + 1), mul(TWO32, cur(WSH_CW))), sh));
+        f.put(boolc(cur(WSH_C0)));
+        f.put(boolc(cur(WSH_C0 + 1)));
+        f.put(sub(sub(cur(WSH_CW), cur(WSH_C0)), mul(2, cur(WSH_C0 + 1))));
+    }
+    phi[n * blockIdx.y + i] = f.acc;
+}
+
+}  // namespace sha5
+
+uint64_t sha512_n_blocks(uint64_t len) { return (len + 17 + 127) / 128; }
+
+uint64_t sha512_trace_height(uint64_t len) {
+    const uint64_t useful = 128 * sha512_n_blocks(len);
+    uint64_t h = 128;
+    while (h < useful) h <<= 1;
+    return h;
+}
+
+uint64_t build_sha512_trace(const uint8_t *msg, uint64_t len, uint64_t *tr, uint8_t digest[64], uint64_t height) {
+    const uint64_t nb = sha512_n_blocks(len), h = height ? height : sha512_trace_height(len);
+    SG_REQUIRE(is_pow2(h) && h >= 128 * nb, "sha512 trace: height must be a power of two >= 128 * n_blocks");
+    std::vector<uint8_t> m(128 * nb, 0);
+    if (len) memcpy(m.data(), msg, len);
+    m[len] = 0x80;
+    const uint64_t bits = len * 8;  // the upper half of the 128-bit length is zero
+    for (int i = 0; i < 8; i++) m[128 * nb - 1 - i] = (uint8_t)(bits >> (8 * i));
+    uint64_t iv[8];
+    memcpy(iv, sha5::IV, 64);
+    for (uint64_t b = 0; b < nb; b++) {
+        const std::array<uint64_t, 8> post = sha5::fill_block(tr, h, 128 * b, &m[128 * b], iv);
+        for (int k = 0; k < 8; k++) iv[k] += post[k];
+    }
+    const uint64_t useful = 128 * nb;
+    for (uint64_t c = 0; c < SHA512_W; c++)
+        for (uint64_t r = useful; r < h; r++) tr[c * h + r] = tr[c * h + useful - 1];
+    const uint64_t drow = 128 * (nb - 1) + 81;
+    for (int k = 0; k < 8; k++) {
+        const uint64_t v = tr[(sha5::H0 + 2 * k) * h + drow] | tr[(sha5::H0 + 2 * k + 1) * h + drow] << 32;
+        for (int i = 0; i < 8; i++) digest[8 * k + i] = (uint8_t)(v >> (56 - 8 * i));
+    }
+    return nb;
+}
+
+void merge_sha512_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup,
+                        const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval) {
+    SG_REQUIRE(is_pow2(n_trace) && n_trace >= 128 && is_pow2(blowup) && blowup >= 2,
+               "merge_sha512: n_trace must be a power of two >= 128");
+    SG_REQUIRE(K >= 1 && K <= 65535 && n_blocks.size() == K, "merge_sha512: one n_blocks per proof");
+    const uint64_t n = n_trace * blowup;
+    SG_REQUIRE(n <= (1ULL << 24), "merge_sha512: domain above 2^24");
+    for (uint32_t nb : n_blocks) SG_REQUIRE(nb >= 1 && 128 * (uint64_t)nb <= n_trace, "merge_sha512: bad n_blocks");
+    if (!phi_computed(phi_mode, n)) {
+        fill_u64(c, c_eval, n * K, 0);
+        return;
+    }
+    Scoped phi(c, 8 * n * K), d_nb(c, 4 * K);
+    SG_HIP(hipMemcpyAsync(d_nb.p, n_blocks.data(), 4 * K, hipMemcpyHostToDevice, c->stream));
+    {
+        ProfScope ps(c, "compose_sha512", (8.0 * SHA512_W * n + 8.0 * n) * K);
+        hipLaunchKernelGGL(sha5::compose_sha512_k, dim3((uint32_t)((n + sha5::AT - 1) / sha5::AT), K), dim3(sha5::AT),
+                           0, c->stream, lde, n, blowup, d_nb.as<uint32_t>(), phi.as<uint64_t>());
+        SG_HIP(hipGetLastError());
+    }
+    quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
+}
+
+}  // namespace sg

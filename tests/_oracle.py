@@ -89,6 +89,14 @@ def lib():
                                                ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.c_char_p,
                                                ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(sz),
                                                ctypes.POINTER(sz)]
+        L.ref_sha512_n_blocks.restype = sz
+        L.ref_sha512_n_blocks.argtypes = [sz]
+        L.ref_sha512_trace_height.restype = sz
+        L.ref_sha512_trace_height.argtypes = [sz]
+        L.ref_build_sha512_trace.restype = ctypes.c_int
+        L.ref_build_sha512_trace.argtypes = [ctypes.c_char_p, sz, u64p, ctypes.c_char_p]
+        L.ref_sha512_constraints.argtypes = [u64p, u64p, sz, sz, u64p]
+        L.ref_merge_sha512.argtypes = [u64p, sz, sz, sz, ctypes.c_int, u64p]
         _lib = L
     return _lib
 
@@ -218,3 +226,30 @@ def prove_ds_ksk_binding(dnskey, parent_ds_hash, fs_binding, phi_mode=1):
                                         ctypes.byref(nb), ctypes.byref(nt))
     assert rc == 0
     return take_bytes(pp, pl), pi.raw, dg.raw, rf.raw, nb.value, nt.value
+
+
+SHA512_W, SHA512_K = 1510, 1526
+
+
+def sha512_trace(msg):
+    """(trace 1510 x h, n_blocks, digest) of build_sha512_trace_multi."""
+    h = lib().ref_sha512_trace_height(len(msg))
+    tr = np.zeros((SHA512_W, h), dtype=np.uint64)
+    dg = ctypes.create_string_buffer(64)
+    nb = lib().ref_build_sha512_trace(bytes(msg), len(msg), ptr(tr), dg)
+    return tr, nb, dg.raw
+
+
+def sha512_constraints(cur, nxt, row, n_blocks):
+    cur = np.ascontiguousarray(cur, dtype=np.uint64)
+    nxt = np.ascontiguousarray(nxt, dtype=np.uint64)
+    out = np.zeros(SHA512_K, dtype=np.uint64)
+    lib().ref_sha512_constraints(ptr(cur), ptr(nxt), row, n_blocks, ptr(out))
+    return out
+
+
+def merge_sha512(lde, n_trace, blowup, n_blocks, phi_mode=1):
+    lde = np.ascontiguousarray(lde, dtype=np.uint64)
+    out = np.zeros(n_trace * blowup, dtype=np.uint64)
+    lib().ref_merge_sha512(ptr(lde), n_trace, blowup, n_blocks, phi_mode, ptr(out))
+    return out

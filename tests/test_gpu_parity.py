@@ -327,3 +327,51 @@ def test_ds_ksk_batch_matches_singles(S):
         assert o.proof_blob == single.proof_blob and o.pi_hash == single.pi_hash
     proof, pi, dg, rf, nb, nt = O.prove_ds_ksk_binding(keys[3], parents[3], fsbs[3])
     assert outs[3].proof_blob == proof and outs[3].pi_hash == pi and nb == 3
+
+
+@pytest.mark.parametrize("msg,phi_mode", [(b"abc", 0), (b"x" * 112, 1)], ids=["1blk-phi0", "2blk-rayon"])
+def test_merge_sha512_matches_oracle(S, msg, phi_mode):
+    say("merge_sha512 len %d phi %d" % (len(msg), phi_mode))
+    tr, nb, _ = O.sha512_trace(msg)
+    h = tr.shape[1]
+    lde = O.lde_columns(tr, 32)
+    want = O.merge_sha512(lde, h, 32, nb, phi_mode)
+    got, info = S.deep_ali_merge_sha512(lde, list(range(1, 1527)), 0, h, 32, nb, phi_mode=phi_mode)
+    assert np.array_equal(got, want)
+
+
+def _oracle_sha_proof(alg, msg, fsb, height=None):
+    """The pipeline of prove_ds_ksk_binding restated with oracle pieces: trace
+    (padded to `height` by repeating the last useful row) -> LDE x32 -> merge
+    -> STIR proof with 54 queries bound to fsb."""
+    tr, nb, dg = (O.sha256_trace if alg == 256 else O.sha512_trace)(msg)
+    useful = 128 * nb
+    h = height or tr.shape[1]
+    if h > tr.shape[1]:
+        pad = np.repeat(tr[:, useful - 1:useful], h - tr.shape[1], axis=1)
+        tr = np.concatenate([tr, pad], axis=1)
+    lde = O.lde_columns(tr, 32)
+    c = (O.merge_sha256 if alg == 256 else O.merge_sha512)(lde, h, 32, nb, 1)
+    n0 = 32 * h
+    lg = n0.bit_length() - 1
+    sched = [8] * (lg // 3) + ([1 << (lg % 3)] if lg % 3 else [])
+    return O.deep_fri_prove(c, O.fri_params(sched, r=54, s0=54, pi_hash=fsb)), dg, nb
+
+
+def test_sha_batch_matches_oracle(S):
+    """prove_sha_batch: SHA-512 and SHA-256 proofs, one with a trace taller
+    than its minimum (padding rows), against the oracle pipeline."""
+    say("prove_sha_batch")
+    msgs512 = [b"abc", b"x" * 112]
+    fsb = [hashlib.sha3_256(m).digest() for m in msgs512]
+    outs = S.prove_sha_batch(512, msgs512, fsb)
+    for m, f, o in zip(msgs512, fsb, outs):
+        proof, dg, nb = _oracle_sha_proof(512, m, f)
+        assert o.digest == hashlib.sha512(m).digest() == dg and o.n_blocks == nb
+        assert o.proof_blob == proof
+    msgs256 = [b"k" * 10, b"m" * 70]
+    fsb2 = [b"\x01" * 32, b"\x02" * 32]
+    outs = S.prove_sha_batch(256, msgs256, fsb2, heights=[256, 256])
+    for m, f, o in zip(msgs256, fsb2, outs):
+        proof, dg, nb = _oracle_sha_proof(256, m, f, height=256)
+        assert o.n_trace == 256 and o.digest == dg and o.proof_blob == proof
