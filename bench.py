@@ -130,6 +130,8 @@ def main():
     ap.add_argument("--streams", type=int, default=4, help="proofs in flight for the batched measurement")
     ap.add_argument("--shards", type=int, default=32, help="inner-shard proofs for the plain-vs-batched line (0 = skip)")
     ap.add_argument("--dsksk", type=int, default=32, help="DS-KSK (SHA-256 AIR) proofs for the plain-vs-batched line (0 = skip)")
+    ap.add_argument("--serve", default="250,1000,4000", help="serving: Poisson arrival rates (requests/s), '' = skip")
+    ap.add_argument("--serve-requests", type=int, default=400, help="serving: requests per rate")
     ap.add_argument("--shards-only", action="store_true",
                     help="run only the inner-shard plain-vs-batched measurement (for profiling that path)")
     args = ap.parse_args()
@@ -242,6 +244,9 @@ def main():
     if args.dsksk:
         log("ds-ksk (SHA-256 AIR): plain vs batched")
         res.update(ds_ksk_batch(S, args.dsksk))
+    if args.serve:
+        log("serving: Poisson arrivals")
+        res.update(serve(S, [int(r) for r in args.serve.split(",")], args.serve_requests))
     if not args.no_cpu:
         log("cpu baseline: timing the oracle on a bounded sample")
         res["cpu_baseline"] = cpu_baseline(w, lg, blowup, schedule, s0)
@@ -328,6 +333,83 @@ def ds_ksk_batch(S, nproofs):
                        "plain_latency_ms_p50": round(1e3 * lat[len(lat) // 2], 3),
                        "plain_latency_ms_p99": round(1e3 * lat[min(len(lat) - 1, (99 * len(lat)) // 100)], 3),
                        "bit_exact_vs_plain": exact}}
+
+
+def serve(S, rates, per_rate, workers=4, max_batch=32):
+    """Serving: RRset proof requests arrive as a Poisson process at each rate
+    (requests/s) and go through the proving server (HBM-budget admission,
+    continuous batching over `workers` streams). Request mix: 70 % inner-shard
+    proofs of an RRset group of 8-64 synthetic zone records (HashRollup), 30 %
+    DS->KSK SHA-256 bindings of 44-132 byte DNSKEYs. Latency = completion -
+    arrival. A sample of results is checked byte-equal to the same request
+    proven alone."""
+    import hashlib
+    import threading
+    from stark_swarm_amd.server import ProvingServer
+    from stark_swarm_amd.zone import build_synthetic_zone
+    rng = np.random.default_rng(0x5EED0005)
+    zone = build_synthetic_zone(10000)
+    salt = b"swarm-test-zone1"
+    out = []
+    with ProvingServer(workers=workers, hbm_budget=64 << 30, max_batch=max_batch) as srv:
+        # warm every worker's context once
+        for rid in [srv.submit_ds_ksk(b"w" * 44, b"\0" * 32, b"\0" * 32) for _ in range(workers)]:
+            srv.wait(rid)
+        for rate in rates:
+            n = per_rate
+            reqs = []
+            for i in range(n):
+                fsb = rng.bytes(32)
+                if rng.random() < 0.7:
+                    k = int(rng.integers(8, 65))
+                    o = int(rng.integers(0, len(zone) - k))
+                    reqs.append(("shard", S.PackedShards([zone[o:o + k]]), fsb, (o, k)))
+                else:
+                    key = rng.bytes(int(rng.integers(44, 133)))
+                    reqs.append(("ksk", key, fsb, hashlib.sha256(key).digest()))
+            gaps = rng.exponential(1.0 / rate, size=n)
+            ids = [None] * n
+            res = [None] * n
+
+            def collect(lo, hi):
+                for j in range(lo, hi):
+                    while ids[j] is None:
+                        time.sleep(0.0005)
+                    res[j] = srv.wait(ids[j], timeout=600)
+
+            waiters = [threading.Thread(target=collect, args=(q * n // 4, (q + 1) * n // 4)) for q in range(4)]
+            for t in waiters:
+                t.start()
+            t0 = time.perf_counter()
+            due = t0
+            for j, (kind, x, fsb, aux) in enumerate(reqs):
+                due += gaps[j]
+                d = due - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                ids[j] = (srv.submit_inner_shard(salt, x, fsb) if kind == "shard"
+                          else srv.submit_ds_ksk(x, aux, fsb))
+            for t in waiters:
+                t.join()
+            wall = time.perf_counter() - t0
+            lat = sorted(r.latency for r in res)
+            out.append({"rate_per_s": rate, "requests": n, "proofs_per_s": round(n / wall, 1),
+                        "latency_ms_p50": round(1e3 * lat[n // 2], 2),
+                        "latency_ms_p99": round(1e3 * lat[min(n - 1, (99 * n) // 100)], 2),
+                        "mean_batch": round(sum(r.batch_size for r in res) / n, 2)})
+            log("serve rate %d/s: %s" % (rate, out[-1]))
+            # byte-equality of a sample against the request proven alone
+            for j in range(0, n, max(1, n // 6)):
+                kind, x, fsb, aux = reqs[j]
+                alone = (S.prove_inner_shard_batch(salt, x, [fsb])[0] if kind == "shard"
+                         else S.prove_ds_ksk_binding(x, aux, fsb))
+                if alone.proof_blob != res[j].proof_blob or alone.pi_hash != res[j].pi_hash:
+                    raise SystemExit("served proof %d differs from the proof made alone" % j)
+        st = srv.stats()
+    return {"serve": {"workers": workers, "max_batch": max_batch, "hbm_budget_GiB": 64,
+                      "peak_admitted_GiB": round(st["peak_bytes_admitted"] / 2**30, 3),
+                      "mix": "70% inner-shard RRset groups (8-64 records), 30% DS-KSK SHA-256 (44-132 B keys)",
+                      "rates": out, "sample_bit_exact_vs_alone": True}}
 
 
 def batched(S, args, w, n, blowup, params, ref_blob, plain_ms):

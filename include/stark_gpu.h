@@ -102,6 +102,40 @@ typedef struct {
     size_t proof_len;
 } stark_sha_out;
 
+/* ---- proving server (one GPU, continuous batching) ---- */
+typedef struct stark_server stark_server;
+
+#define STARK_SERVE_DS_KSK 0      /* prove_ds_ksk_binding request */
+#define STARK_SERVE_INNER_SHARD 1 /* prove_inner_shard request */
+#define STARK_SERVER_DRY_RUN 1    /* flag: schedule only, no device work (admission tests) */
+#define STARK_E_TIMEOUT (-6)
+
+typedef struct {
+    int kind;
+    const uint8_t *dnskey; /* DS-KSK */
+    size_t dnskey_len;
+    const uint8_t *parent_ds_hash; /* DS-KSK, 32 bytes */
+    const uint8_t *salt;           /* inner shard, 16 bytes */
+    const stark_dns_record *records;
+    size_t n_records;
+    const uint8_t *fs_binding; /* 32 bytes */
+} stark_serve_request;
+
+typedef struct {
+    uint64_t id;
+    int status; /* 0 or a STARK_E_* code of the proof */
+    int kind;
+    uint8_t pi_hash[32];
+    uint8_t root_f0[32];
+    uint8_t aux[32]; /* inner shard: merkle_root; DS-KSK: asserted_digest */
+    size_t n_trace;
+    size_t batch_size; /* requests proven in the same lockstep batch */
+    uint64_t batch_seq; /* admission order of that batch (1, 2, ...) */
+    uint8_t *proof;    /* stark_free() it */
+    size_t proof_len;
+    double t_submit, t_start, t_done; /* seconds since the server started; t_start = admission */
+} stark_serve_result;
+
 /* ---- context ---- */
 int stark_ctx_create(int device, stark_ctx **out);
 void stark_ctx_destroy(stark_ctx *ctx);
@@ -200,6 +234,23 @@ int stark_prove_sha_batch(stark_ctx *ctx, int alg, const uint8_t *msgs, const si
 /* prove_outer_rollup (crates/swarm-dns/src/prover.rs:253-304). */
 int stark_prove_outer_rollup(stark_ctx *ctx, const uint8_t *pi_hashes, size_t n, const uint8_t pk_hash[32], int ldt,
                              int phi_mode, uint8_t **proof, size_t *proof_len, uint8_t root_f0[32]);
+
+/* ---- proving server ----
+ * No counterpart in the reference: it is the device side of the swarm
+ * worker's request loop (crates/swarm-worker/src/main.rs:220-335). `workers`
+ * threads own one context (HIP stream) each; a free worker admits the head
+ * request and every queued request of the same kind (and salt) that fits the
+ * HBM budget, up to max_batch, and proves them as one lockstep batch while
+ * other workers' batches run on their own streams. Each result equals the
+ * single-call result for that request (stark_prove_ds_ksk_binding /
+ * stark_prove_inner_shard). submit copies the request's inputs. */
+int stark_server_create(int device, int workers, size_t hbm_budget, size_t max_batch, int flags, stark_server **out);
+void stark_server_destroy(stark_server *s); /* finishes queued requests first */
+int stark_server_submit(stark_server *s, const stark_serve_request *req, uint64_t *id);
+/* STARK_E_TIMEOUT if the result is not ready within timeout_s */
+int stark_server_wait(stark_server *s, uint64_t id, double timeout_s, stark_serve_result *out);
+int stark_server_stats(stark_server *s, size_t *completed, size_t *peak_bytes_admitted, size_t *budget);
+const char *stark_server_last_error(const stark_server *s);
 
 /* ---- device-resident API (inputs already in HBM) ---- */
 int stark_dev_alloc(stark_ctx *ctx, size_t bytes, void **ptr);

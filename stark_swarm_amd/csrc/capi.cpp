@@ -15,10 +15,16 @@
 #include "keccak.hpp"
 #include "ntt.hpp"
 #include "prover.hpp"
+#include "server.hpp"
 #include "sha256_air.hpp"
 
 struct stark_ctx {
     sg::Ctx *c = nullptr;
+    std::string err;
+};
+
+struct stark_server {
+    sg::Server *s = nullptr;
     std::string err;
 };
 
@@ -461,6 +467,94 @@ size_t stark_host_sha256_trace_height(size_t len) { return sg::sha256_trace_heig
 size_t stark_host_build_sha256_trace(const uint8_t *msg, size_t len, uint64_t *trace, uint8_t digest[32]) {
     return sg::build_sha256_trace(msg, len, trace, digest);
 }
+
+int stark_server_create(int device, int workers, size_t hbm_budget, size_t max_batch, int flags, stark_server **out) {
+    if (!out) return STARK_E_INVALID;
+    *out = nullptr;
+    stark_server *x = new (std::nothrow) stark_server();
+    if (!x) return STARK_E_OOM;
+    try {
+        x->s = new sg::Server(device, workers, hbm_budget, (uint32_t)max_batch, (flags & STARK_SERVER_DRY_RUN) != 0);
+    } catch (const sg::Error &e) {
+        delete x;
+        return e.code;
+    } catch (...) {
+        delete x;
+        return STARK_E_INTERNAL;
+    }
+    *out = x;
+    return STARK_OK;
+}
+
+void stark_server_destroy(stark_server *s) {
+    if (!s) return;
+    delete s->s;
+    delete s;
+}
+
+int stark_server_submit(stark_server *s, const stark_serve_request *q, uint64_t *id) {
+    if (!s || !s->s || !q || !id) return STARK_E_INVALID;
+    try {
+        sg::ServeRequest r;
+        r.kind = q->kind;
+        SG_REQUIRE(q->fs_binding, "server submit: null fs_binding");
+        memcpy(r.fs_binding.data(), q->fs_binding, 32);
+        if (q->kind == STARK_SERVE_DS_KSK) {
+            SG_REQUIRE((q->dnskey || q->dnskey_len == 0) && q->parent_ds_hash, "server submit: null DS-KSK input");
+            if (q->dnskey_len) r.key.assign(q->dnskey, q->dnskey + q->dnskey_len);
+            memcpy(r.parent.data(), q->parent_ds_hash, 32);
+        } else {
+            SG_REQUIRE(q->salt && (q->records || q->n_records == 0), "server submit: null shard input");
+            memcpy(r.salt.data(), q->salt, 16);
+            for (size_t i = 0; i < q->n_records; i++) {
+                const stark_dns_record &d = q->records[i];
+                r.domains.emplace_back(d.domain, d.domain_len);
+                r.rdata.emplace_back(d.rdata, d.rdata + d.rdata_len);
+                r.rtype.push_back(d.record_type);
+                r.ttl.push_back(d.ttl);
+            }
+        }
+        *id = s->s->submit(std::move(r));
+        return STARK_OK;
+    } catch (const sg::Error &e) {
+        s->err = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        s->err = e.what();
+        return STARK_E_INTERNAL;
+    }
+}
+
+int stark_server_wait(stark_server *s, uint64_t id, double timeout_s, stark_serve_result *out) {
+    if (!s || !s->s || !out) return STARK_E_INVALID;
+    sg::ServeResult r;
+    if (!s->s->wait(id, r, timeout_s)) return STARK_E_TIMEOUT;
+    out->id = r.id;
+    out->status = r.status;
+    out->kind = r.kind;
+    memcpy(out->pi_hash, r.pi_hash, 32);
+    memcpy(out->root_f0, r.root_f0, 32);
+    memcpy(out->aux, r.aux, 32);
+    out->n_trace = r.n_trace;
+    out->batch_size = r.batch_size;
+    out->batch_seq = r.batch_seq;
+    out->t_submit = r.t_submit;
+    out->t_start = r.t_start;
+    out->t_done = r.t_done;
+    out->proof = copy_out(r.proof, &out->proof_len);
+    if (r.status) s->err = r.error;
+    return STARK_OK;
+}
+
+int stark_server_stats(stark_server *s, size_t *completed, size_t *peak, size_t *budget) {
+    if (!s || !s->s) return STARK_E_INVALID;
+    if (completed) *completed = s->s->completed();
+    if (peak) *peak = s->s->peak_in_use();
+    if (budget) *budget = s->s->budget();
+    return STARK_OK;
+}
+
+const char *stark_server_last_error(const stark_server *s) { return s ? s->err.c_str() : "null server"; }
 
 size_t stark_host_sha512_trace_height(size_t len) { return sg::sha512_trace_height(len); }
 

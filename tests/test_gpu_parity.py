@@ -375,3 +375,46 @@ def test_sha_batch_matches_oracle(S):
     for m, f, o in zip(msgs256, fsb2, outs):
         proof, dg, nb = _oracle_sha_proof(256, m, f, height=256)
         assert o.n_trace == 256 and o.digest == dg and o.proof_blob == proof
+
+
+def test_server_results_equal_standalone(S):
+    """Requests admitted by the proving server in a mix (several workers,
+    continuous batching, submitters on two threads) give the same bytes as
+    each request proven alone."""
+    import threading
+
+    from stark_swarm_amd.server import ProvingServer
+    from stark_swarm_amd.zone import build_synthetic_zone
+    say("proving server mix")
+    zone = build_synthetic_zone(1200)
+    salt = b"swarm-test-zone1"
+    reqs = []
+    for i in range(14):
+        fsb = hashlib.sha3_256(b"req%d" % i).digest()
+        if i % 3 == 0:
+            reqs.append(("shard", zone[80 * i: 80 * i + 20 + 7 * i], fsb))
+        else:
+            key = bytes([(i * 13 + j) & 255 for j in range(20 + 9 * i)])
+            reqs.append(("ksk", key, fsb))
+    ids = [None] * len(reqs)
+    with ProvingServer(workers=3, hbm_budget=16 << 30, max_batch=6) as srv:
+        def go(part):
+            for j in part:
+                kind, x, fsb = reqs[j]
+                ids[j] = (srv.submit_inner_shard(salt, x, fsb) if kind == "shard"
+                          else srv.submit_ds_ksk(x, hashlib.sha256(x).digest(), fsb))
+        th = [threading.Thread(target=go, args=(range(t, len(reqs), 2),)) for t in range(2)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        res = [srv.wait(i, timeout=300) for i in ids]
+    assert max(r.batch_size for r in res) > 1, "the mix must have produced shared batches"
+    for (kind, x, fsb), r in zip(reqs, res):
+        if kind == "shard":
+            alone = S.prove_inner_shard(salt, x, fsb)
+            assert r.aux == alone.merkle_root
+        else:
+            alone = S.prove_ds_ksk_binding(x, hashlib.sha256(x).digest(), fsb)
+            assert r.aux == alone.asserted_digest
+        assert r.proof_blob == alone.proof_blob and r.pi_hash == alone.pi_hash
