@@ -132,6 +132,9 @@ def main():
     ap.add_argument("--dsksk", type=int, default=32, help="DS-KSK (SHA-256 AIR) proofs for the plain-vs-batched line (0 = skip)")
     ap.add_argument("--serve", default="250,1000,4000", help="serving: Poisson arrival rates (requests/s), '' = skip")
     ap.add_argument("--serve-requests", type=int, default=400, help="serving: requests per rate")
+    ap.add_argument("--sha-batch", default="64,10,13",
+                    help="batched uneven SHA-256/512 proofs: count,log2 min rows,log2 max rows ('' = skip); "
+                         "the full workload is 256,10,17")
     ap.add_argument("--shards-only", action="store_true",
                     help="run only the inner-shard plain-vs-batched measurement (for profiling that path)")
     args = ap.parse_args()
@@ -244,6 +247,10 @@ def main():
     if args.dsksk:
         log("ds-ksk (SHA-256 AIR): plain vs batched")
         res.update(ds_ksk_batch(S, args.dsksk))
+    if args.sha_batch:
+        cnt, lo, hi = (int(x) for x in args.sha_batch.split(","))
+        log("sha batch: %d uneven SHA-256/512 proofs, 2^%d..2^%d rows" % (cnt, lo, hi))
+        res.update(c4_batch(S, cnt, lo, hi))
     if args.serve:
         log("serving: Poisson arrivals")
         res.update(serve(S, [int(r) for r in args.serve.split(",")], args.serve_requests))
@@ -333,6 +340,49 @@ def ds_ksk_batch(S, nproofs):
                        "plain_latency_ms_p50": round(1e3 * lat[len(lat) // 2], 3),
                        "plain_latency_ms_p99": round(1e3 * lat[min(len(lat) - 1, (99 * len(lat)) // 100)], 3),
                        "bit_exact_vs_plain": exact}}
+
+
+def c4_batch(S, nproofs, log_lo, log_hi):
+    """Batched uneven SHA-256/512 proofs: nproofs alternating SHA-256 and
+    SHA-512 AIR proofs (the prove_ds_ksk_binding pipeline: LDE x32, merge,
+    STIR with 54 queries), messages of seeded uniform 64-4096 bytes, trace
+    heights log-uniform in [2^log_lo, 2^log_hi] raised to each message's
+    minimum (padding rows repeat the last useful row). One proof per call
+    (plain) vs one stark_prove_sha_batch call (lockstep per alg and height,
+    HBM-sized waves); every batched proof must equal its single-call proof."""
+    rng = np.random.default_rng(0x5EED0004)
+    algs, msgs, hs = [], [], []
+    for i in range(nproofs):
+        alg = 256 if i % 2 == 0 else 512
+        m = rng.bytes(int(rng.integers(64, 4097)))
+        blocks = (len(m) + 9 + 63) // 64 if alg == 256 else (len(m) + 17 + 127) // 128
+        hmin = 1 << max(7, (128 * blocks - 1).bit_length())
+        h = max(hmin, 1 << int(rng.integers(log_lo, log_hi + 1)))
+        algs.append(alg)
+        msgs.append(m)
+        hs.append(h)
+    fsb = [rng.bytes(32) for _ in range(nproofs)]
+    lde_bytes = sum(8 * (756 if a == 256 else 1510) * 32 * h for a, h in zip(algs, hs))
+    S.prove_sha_batch([256, 512], [b"w", b"w"], fsb[:2])  # warm
+    t0 = time.perf_counter()
+    plain = []
+    for i in range(nproofs):
+        plain.append(S.prove_sha_batch(algs[i], [msgs[i]], [fsb[i]], heights=[hs[i]])[0].proof_blob)
+        if (i + 1) % 16 == 0:
+            log("sha batch: %d/%d plain proofs, %.1f s" % (i + 1, nproofs, time.perf_counter() - t0))
+    plain_dt = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    outs = S.prove_sha_batch(algs, msgs, fsb, heights=hs)
+    bdt = time.perf_counter() - t0
+    exact = all(o.proof_blob == p for o, p in zip(outs, plain))
+    if not exact:
+        raise SystemExit("batched SHA proofs differ from the single-call proofs")
+    return {"sha_batch": {"proofs": nproofs, "algs": "alternating SHA-256 / SHA-512",
+                          "trace_rows_log2": [log_lo, log_hi], "message_bytes": [64, 4096],
+                          "total_lde_GB": round(lde_bytes / 1e9, 2),
+                          "plain_proofs_per_s": round(nproofs / plain_dt, 2),
+                          "batched_proofs_per_s": round(nproofs / bdt, 2), "speedup": round(plain_dt / bdt, 3),
+                          "batched_lde_GBps": round(lde_bytes / bdt / 1e9, 1), "bit_exact_vs_plain": exact}}
 
 
 def serve(S, rates, per_rate, workers=4, max_batch=32):

@@ -218,16 +218,16 @@ int stark_prove_ds_ksk_binding_batch(stark_ctx *ctx, const uint8_t *dnskeys, con
 int stark_merge_sha512(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_blocks,
                        const uint64_t *coeffs, size_t k, int phi_mode, uint64_t *c_eval);
 
-/* SHA-256 (alg = 256) or SHA-512 (alg = 512) AIR proofs over n messages with
+/* SHA-256 (algs[i] = 256) or SHA-512 (512) AIR proofs over n messages with
  * the prove_ds_ksk_binding pipeline (trace -> LDE x32 -> merge -> STIR with
  * 54 queries bound to fs_bindings[32 i ..]); message i is msgs[offsets[i] ..
  * + lens[i]) and its trace has heights[i] rows (a power of two >= the
  * message's minimum; padding rows repeat the last useful row, the trace
  * builder's own rule, sha256_air.rs:441-479 / sha512_air.rs:551-573), or the
  * minimum when heights is NULL. Proven in lockstep device batches per
- * height, in waves sized to free HBM. No single counterpart in the
+ * (alg, height), in waves sized to free HBM. No single counterpart in the
  * reference: the batched SHA-256/512 workload of the swarm's hash AIRs. */
-int stark_prove_sha_batch(stark_ctx *ctx, int alg, const uint8_t *msgs, const size_t *offsets, const size_t *lens,
+int stark_prove_sha_batch(stark_ctx *ctx, const int *algs, const uint8_t *msgs, const size_t *offsets, const size_t *lens,
                           const size_t *heights, size_t n, const uint8_t *fs_bindings, int phi_mode,
                           stark_sha_out *outs);
 

@@ -121,7 +121,7 @@ def lib():
         "stark_host_sha512_trace_height": (sz, [sz]),
         "stark_host_build_sha512_trace": (sz, [ctypes.c_char_p, sz, u64p, ctypes.c_char_p]),
         "stark_merge_sha512": (ctypes.c_int, [vp, u64p, sz, sz, sz, u64p, sz, ctypes.c_int, u64p]),
-        "stark_prove_sha_batch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(sz),
+        "stark_prove_sha_batch": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.POINTER(sz),
                                                  ctypes.POINTER(sz), ctypes.POINTER(sz), sz, ctypes.c_char_p,
                                                  ctypes.c_int, ctypes.POINTER(_ShaOut)]),
         "stark_host_build_sha256_trace": (sz, [ctypes.c_char_p, sz, u64p, ctypes.c_char_p]),
@@ -409,11 +409,14 @@ class ShaProof:
 
 def prove_sha_batch(alg, messages, fs_bindings, heights=None, phi_mode=PHI_RAYON, ctx=None):
     """SHA-256 (alg=256) / SHA-512 (alg=512) AIR proofs with the
-    prove_ds_ksk_binding pipeline (stark_prove_sha_batch); heights[i] is the
-    trace height of message i (None: the minimum)."""
+    prove_ds_ksk_binding pipeline (stark_prove_sha_batch); `alg` is one value
+    or one per message; heights[i] is the trace height of message i (None:
+    the minimum)."""
     ctx = ctx or default_context()
     msgs = [bytes(m) for m in messages]
     n = len(msgs)
+    algs = [alg] * n if isinstance(alg, int) else list(alg)
+    assert len(algs) == n
     assert len(fs_bindings) == n and (heights is None or len(heights) == n)
     offs, acc = [], 0
     for m in msgs:
@@ -421,13 +424,14 @@ def prove_sha_batch(alg, messages, fs_bindings, heights=None, phi_mode=PHI_RAYON
         acc += len(m)
     outs = (_ShaOut * max(1, n))()
     hs = (sz * max(1, n))(*heights) if heights is not None else None
-    ctx._check(lib().stark_prove_sha_batch(ctx.h, alg, b"".join(msgs), (sz * max(1, n))(*offs),
+    ctx._check(lib().stark_prove_sha_batch(ctx.h, (ctypes.c_int * max(1, n))(*algs), b"".join(msgs),
+                                           (sz * max(1, n))(*offs),
                                            (sz * max(1, n))(*[len(m) for m in msgs]), hs, n,
                                            b"".join(bytes(f) for f in fs_bindings), phi_mode, outs))
-    dl = 32 if alg == 256 else 64
     res = []
     for k in range(n):
         o = outs[k]
+        dl = 32 if algs[k] == 256 else 64
         res.append(ShaProof(bytes(o.digest)[:dl], bytes(o.root_f0), o.n_blocks, o.n_trace, _take(o.proof, o.proof_len)))
     return res
 

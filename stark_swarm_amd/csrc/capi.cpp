@@ -258,11 +258,11 @@ int stark_merge_sha512(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, size
     });
 }
 
-int stark_prove_sha_batch(stark_ctx *ctx, int alg, const uint8_t *msgs, const size_t *offsets, const size_t *lens,
+int stark_prove_sha_batch(stark_ctx *ctx, const int *algs, const uint8_t *msgs, const size_t *offsets, const size_t *lens,
                           const size_t *heights, size_t n, const uint8_t *fs_bindings, int phi_mode,
                           stark_sha_out *outs) {
     return guard(ctx, [&](sg::Ctx *c) {
-        SG_REQUIRE(offsets && lens && fs_bindings && outs, "prove_sha_batch: null argument");
+        SG_REQUIRE(algs && offsets && lens && fs_bindings && outs, "prove_sha_batch: null argument");
         SG_REQUIRE(phi_mode >= 0 && phi_mode <= 2, "prove_sha_batch: bad phi_mode");
         std::vector<sg::ByteSpan> m(n);
         std::vector<uint64_t> h(n, 0);
@@ -274,7 +274,7 @@ int stark_prove_sha_batch(stark_ctx *ctx, int alg, const uint8_t *msgs, const si
             memcpy(fsb[i].data(), fs_bindings + 32 * i, 32);
             outs[i].proof = nullptr;
         }
-        std::vector<sg::ShaProof> r = sg::prove_sha_batch(c, alg, m, h, fsb, phi_mode);
+        std::vector<sg::ShaProof> r = sg::prove_sha_batch(c, std::vector<int>(algs, algs + n), m, h, fsb, phi_mode);
         for (size_t i = 0; i < n; i++) {
             memcpy(outs[i].digest, r[i].digest, 64);
             memcpy(outs[i].root_f0, r[i].root_f0, 32);

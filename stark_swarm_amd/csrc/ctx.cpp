@@ -9,8 +9,10 @@ void *Ctx::alloc(size_t bytes) {
     if (bytes == 0) bytes = 256;
     bytes = (bytes + 255) & ~size_t(255);
     auto it = free_blocks.lower_bound(bytes);
-    // reuse a cached block when it is not much larger than the request
-    if (it != free_blocks.end() && it->first <= bytes + bytes / 4) {
+    // reuse the smallest cached block that holds the request when it is at
+    // most twice as large: hipMalloc of fresh memory costs time in proportion
+    // to its size (tens of GB for an uneven batch), reuse costs nothing
+    if (it != free_blocks.end() && it->first <= 2 * bytes) {
         void *p = it->second;
         live[p] = it->first;
         cached_bytes -= it->first;
@@ -19,9 +21,14 @@ void *Ctx::alloc(size_t bytes) {
     }
     void *p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);
-    if (e == hipErrorOutOfMemory) {
+    // out of memory: hand cached blocks back, largest first, until it fits
+    while (e == hipErrorOutOfMemory && !free_blocks.empty()) {
         (void)hipGetLastError();
-        trim();
+        (void)hipStreamSynchronize(stream);
+        auto big = std::prev(free_blocks.end());
+        (void)hipFree(big->second);
+        cached_bytes -= big->first;
+        free_blocks.erase(big);
         e = hipMalloc(&p, bytes);
     }
     if (e != hipSuccess) {

@@ -7,6 +7,7 @@
 
 #include <atomic>
 #include <chrono>
+#include <iterator>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -89,6 +90,12 @@ struct Ctx {
     int prof_begin(const char *name, double bytes);
     void prof_end(int token);
     void prof_collect();  // synchronizes and folds pending events into stats
+};
+
+// A borrowed byte string.
+struct ByteSpan {
+    const uint8_t *p;
+    size_t n;
 };
 
 // RAII holder for a context allocation

@@ -259,55 +259,69 @@ static void ds_ksk_pi_hash(const ByteSpan &key, const uint8_t parent[32], const 
     h.final(out);
 }
 
-std::vector<ShaProof> prove_sha_batch(Ctx *c, int alg, const std::vector<ByteSpan> &msgs,
+std::vector<ShaProof> prove_sha_batch(Ctx *c, const std::vector<int> &algs, const std::vector<ByteSpan> &msgs,
                                       const std::vector<uint64_t> &heights,
                                       const std::vector<std::array<uint8_t, 32>> &fsbs, int phi_mode) {
-    SG_REQUIRE(alg == 256 || alg == 512, "prove_sha: alg must be 256 or 512");
     const size_t K = msgs.size();
-    SG_REQUIRE(heights.size() == K && fsbs.size() == K, "prove_sha: one height and binding per message");
-    const uint64_t W = alg == 256 ? SHA256_W : SHA512_W;
+    SG_REQUIRE(algs.size() == K && heights.size() == K && fsbs.size() == K,
+               "prove_sha: one alg, height and binding per message");
     std::vector<ShaProof> out(K);
-    std::map<uint64_t, std::vector<size_t>> groups;  // trace height -> inputs
+    std::map<std::pair<int, uint64_t>, std::vector<size_t>> groups;  // (alg, trace height) -> inputs
     for (size_t k = 0; k < K; k++) {
+        SG_REQUIRE(algs[k] == 256 || algs[k] == 512, "prove_sha: alg must be 256 or 512");
         SG_REQUIRE(msgs[k].p || msgs[k].n == 0, "prove_sha: null message");
-        const uint64_t hmin = alg == 256 ? sha256_trace_height(msgs[k].n) : sha512_trace_height(msgs[k].n);
+        const uint64_t hmin = algs[k] == 256 ? sha256_trace_height(msgs[k].n) : sha512_trace_height(msgs[k].n);
         const uint64_t h = heights[k] ? heights[k] : hmin;
         SG_REQUIRE(is_pow2(h) && h >= hmin, "prove_sha: trace height below the message's minimum");
         SG_REQUIRE(h * SWARM_BLOWUP <= (1ULL << 24), "prove_sha: domain above 2^24");
-        groups[h].push_back(k);
+        groups[{algs[k], h}].push_back(k);
     }
     size_t free_b = 0, total_b = 0;
     SG_HIP(hipMemGetInfo(&free_b, &total_b));
     const uint64_t budget = (uint64_t)((free_b + c->cached_bytes) * 0.6);
-    for (auto &g : groups) {
-        const uint64_t h = g.first, n0 = h * SWARM_BLOWUP;
+    // largest groups first, so the buffers of later waves fit in blocks the
+    // caching allocator already holds
+    std::vector<std::pair<std::pair<int, uint64_t>, std::vector<size_t>>> order(groups.begin(), groups.end());
+    auto wbytes = [](const std::pair<int, uint64_t> &k) { return (k.first == 256 ? SHA256_W : SHA512_W) * k.second; };
+    std::stable_sort(order.begin(), order.end(),
+                     [&](const auto &a, const auto &b) { return wbytes(a.first) > wbytes(b.first); });
+    for (auto &g : order) {
+        const int alg = g.first.first;
+        const uint64_t h = g.first.second, n0 = h * SWARM_BLOWUP, W = alg == 256 ? SHA256_W : SHA512_W;
         // per proof: trace + its coefficients, LDE + the LDE pass scratch, merge and FRI buffers
         const uint64_t per = 8ULL * W * (2 * h + 2 * n0) + 8ULL * 16 * n0;
         SG_REQUIRE(per <= budget, "prove_sha: one proof does not fit in device memory");
-        const size_t chunk = (size_t)std::min<uint64_t>(4096, budget / per);
+        // equal waves: every wave of a group reuses the same buffer sizes
+        const size_t cap = (size_t)std::min<uint64_t>(4096, budget / per);
+        const size_t nw = (g.second.size() + cap - 1) / cap;
+        const size_t chunk = (g.second.size() + nw - 1) / nw;
         for (size_t c0 = 0; c0 < g.second.size(); c0 += chunk) {
             const std::vector<size_t> ids(g.second.begin() + c0,
                                           g.second.begin() + std::min(g.second.size(), c0 + chunk));
             const uint32_t G = (uint32_t)ids.size();
-            std::vector<uint64_t> tr((size_t)W * h * G);
-            std::vector<uint32_t> nb(G);
-            {
-                HostScope hs(c, "sha_trace");
-                parallel_for(G, [&](size_t j) {
-                    ShaProof &o = out[ids[j]];
-                    const ByteSpan &m = msgs[ids[j]];
-                    nb[j] = (uint32_t)(alg == 256 ? build_sha256_trace(m.p, m.n, &tr[(size_t)W * h * j], o.digest, h)
-                                                  : build_sha512_trace(m.p, m.n, &tr[(size_t)W * h * j], o.digest, h));
-                    o.n_blocks = nb[j];
-                    o.n_trace = h;
-                });
-            }
+            std::vector<ByteSpan> gm(G);
+            for (uint32_t j = 0; j < G; j++) gm[j] = msgs[ids[j]];
+            std::vector<uint32_t> nb;
+            std::vector<std::array<uint8_t, 64>> dgs;
             Scoped d_c(c, 8ULL * n0 * G);
             {
                 Scoped d_lde(c, 8ULL * W * n0 * G);
                 {
-                    Scoped d_tr(c, 8 * tr.size());
-                    SG_HIP(hipMemcpyAsync(d_tr.p, tr.data(), 8 * tr.size(), hipMemcpyHostToDevice, c->stream));
+                    // traces expanded on the device from the host's row words
+                    Scoped d_tr(c, 8ULL * W * h * G);
+                    {
+                        HostScope hs(c, "sha_trace");
+                        if (alg == 256)
+                            build_sha256_traces_dev(c, gm, h, d_tr.as<uint64_t>(), nb, dgs);
+                        else
+                            build_sha512_traces_dev(c, gm, h, d_tr.as<uint64_t>(), nb, dgs);
+                    }
+                    for (uint32_t j = 0; j < G; j++) {
+                        ShaProof &o = out[ids[j]];
+                        memcpy(o.digest, dgs[j].data(), 64);
+                        o.n_blocks = nb[j];
+                        o.n_trace = h;
+                    }
                     lde_columns(c, d_tr.as<uint64_t>(), d_lde.as<uint64_t>(), ilog2(h), ilog2(SWARM_BLOWUP), W * G,
                                 h, n0, 1);
                 }
@@ -316,9 +330,6 @@ std::vector<ShaProof> prove_sha_batch(Ctx *c, int alg, const std::vector<ByteSpa
                 else
                     merge_sha512_batch(c, d_lde.as<uint64_t>(), G, h, SWARM_BLOWUP, nb, phi_mode, d_c.as<uint64_t>());
             }
-            // a wave that took much of HBM hands its cached blocks back, so the
-            // next wave's different sizes do not pile up beside them
-            if (per * G > budget / 2) c->trim();
             FriParams p;
             p.schedule = stir_schedule(n0);
             p.r = p.s0 = SWARM_QUERIES;
@@ -345,7 +356,8 @@ std::vector<DsKsk> prove_ds_ksk_binding_batch(Ctx *c, const std::vector<ByteSpan
                                               const std::vector<std::array<uint8_t, 32>> &fsbs, int phi_mode) {
     const size_t K = keys.size();
     SG_REQUIRE(parents.size() == K && fsbs.size() == K, "prove_ds_ksk_binding: one parent hash and binding per key");
-    std::vector<ShaProof> sp = prove_sha_batch(c, 256, keys, std::vector<uint64_t>(K, 0), fsbs, phi_mode);
+    std::vector<ShaProof> sp =
+        prove_sha_batch(c, std::vector<int>(K, 256), keys, std::vector<uint64_t>(K, 0), fsbs, phi_mode);
     std::vector<DsKsk> out(K);
     for (size_t k = 0; k < K; k++) {
         DsKsk &o = out[k];

@@ -31,10 +31,6 @@ struct DsKsk {
     size_t n_blocks = 0, n_trace = 0;
     std::vector<uint8_t> proof;
 };
-struct ByteSpan {
-    const uint8_t *p;
-    size_t n;
-};
 // One SHA-256 / SHA-512 AIR proof (the prove_ds_ksk_binding pipeline).
 struct ShaProof {
     uint8_t digest[64];  // SHA-256 uses the first 32 bytes
@@ -59,11 +55,11 @@ std::vector<InnerShard> prove_inner_shard_batch(Ctx *c, const uint8_t salt[16],
 std::vector<std::vector<uint8_t>> rollup_prove_batch(Ctx *c, const std::vector<std::vector<uint64_t>> &leaves,
                                                      const std::vector<std::array<uint8_t, 32>> &bindings,
                                                      int phi_mode, std::vector<std::array<uint8_t, 32>> &root_f0);
-// SHA-256 (alg 256) or SHA-512 (alg 512) AIR proofs: trace of msgs[k] with
+// SHA-256 (algs[k] = 256) or SHA-512 (512) AIR proofs: trace of msgs[k] with
 // height heights[k] (0 = minimum) -> LDE x32 -> merge -> STIR (54 queries,
 // bound to fs_bindings[k]); lockstep batches per (alg, height) in HBM-sized
 // waves; result k equals the proof of input k alone.
-std::vector<ShaProof> prove_sha_batch(Ctx *c, int alg, const std::vector<ByteSpan> &msgs,
+std::vector<ShaProof> prove_sha_batch(Ctx *c, const std::vector<int> &algs, const std::vector<ByteSpan> &msgs,
                                       const std::vector<uint64_t> &heights,
                                       const std::vector<std::array<uint8_t, 32>> &fs_bindings, int phi_mode);
 // prove_ds_ksk_binding (prover.rs:512-585) for K keys: lockstep device

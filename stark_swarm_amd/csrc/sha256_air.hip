@@ -54,35 +54,63 @@ __constant__ uint32_t KD[64] = {SG_SHA_K};
 static const uint32_t IV[8] = {0x6a09e667u, 0xbb67ae85u, 0x3c6ef372u, 0xa54ff53au,
                                0x510e527fu, 0x9b05688cu, 0x1f83d9abu, 0x5be0cd19u};
 
-inline uint32_t rotr(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
-inline uint32_t big0(uint32_t x) { return rotr(x, 2) ^ rotr(x, 13) ^ rotr(x, 22); }
-inline uint32_t big1(uint32_t x) { return rotr(x, 6) ^ rotr(x, 11) ^ rotr(x, 25); }
-inline uint32_t small0(uint32_t x) { return rotr(x, 7) ^ rotr(x, 18) ^ (x >> 3); }
-inline uint32_t small1(uint32_t x) { return rotr(x, 17) ^ rotr(x, 19) ^ (x >> 10); }
+SG_HD uint32_t rotr(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+SG_HD uint32_t big0(uint32_t x) { return rotr(x, 2) ^ rotr(x, 13) ^ rotr(x, 22); }
+SG_HD uint32_t big1(uint32_t x) { return rotr(x, 6) ^ rotr(x, 11) ^ rotr(x, 25); }
+SG_HD uint32_t small0(uint32_t x) { return rotr(x, 7) ^ rotr(x, 18) ^ (x >> 3); }
+SG_HD uint32_t small1(uint32_t x) { return rotr(x, 17) ^ rotr(x, 19) ^ (x >> 10); }
 inline uint32_t chf(uint32_t e, uint32_t f, uint32_t g) { return (e & f) ^ (~e & g); }
 inline uint32_t majf(uint32_t a, uint32_t b, uint32_t c) { return (a & b) ^ (a & c) ^ (b & c); }
 
-// The words that determine one trace row; every cell of the row is one of
-// them or one of their bits.
+// Every cell of a trace row is one of these words or one of their bits
+// (44 u32 per row against 756 cells).
 struct RowWords {
     uint32_t st[8];           // a..h
     uint32_t s0t, s1t;        // rotr-xor halves of Sigma0(a), Sigma1(e)
     uint32_t chw, mjw, wt;    // Ch, Maj, W[t]
-    uint64_t t1, t2;          // full T1 and T2 sums
+    uint32_t t1lo, t1c, t2lo, t2c;  // T1 / T2 mod 2^32 and their carries
     uint32_t hs[8];           // H-state
     uint32_t ww[15];          // W window W[t-15..t-1]
     uint32_t ss0t, ss1t;      // rotr-xor halves of sigma0(W[t-14]), sigma1(W[t-1])
     uint32_t sc, na, ne;      // schedule carry, new_a / new_e carries
 };
 
-// One 128-row block segment (sha256_air.rs:648-898): rows 0..63 are the
+// The 756 cells of one row in column order: put(column, value). Shared by the
+// host builder and the device expansion, so both lay out the same trace.
+template <class Put>
+SG_HD void emit_row(const RowWords &r, Put put) {
+    uint32_t c = 0;
+    auto word = [&](uint32_t v) { put(c++, (uint64_t)v); };
+    auto bits = [&](uint32_t v) {
+        for (int i = 0; i < 32; i++) put(c++, (uint64_t)((v >> i) & 1));
+    };
+    for (int k = 0; k < 8; k++) word(r.st[k]);                          // A
+    bits(r.st[0]), bits(r.st[1]), bits(r.st[2]);                         // B
+    bits(r.st[4]), bits(r.st[5]), bits(r.st[6]);
+    const uint32_t s0 = r.s0t ^ rotr(r.st[0], 22), s1 = r.s1t ^ rotr(r.st[4], 25);
+    bits(r.s0t), bits(s0), word(s0);                                     // C0
+    bits(r.s1t), bits(s1), word(s1);                                     // C1
+    bits(r.chw), word(r.chw);                                            // D
+    const uint32_t ab = r.st[0] & r.st[1], ac = r.st[0] & r.st[2], bc = r.st[1] & r.st[2];
+    bits(ab), bits(ac), bits(bc), bits(ab ^ ac), bits(r.mjw), word(r.mjw);  // E
+    word(r.wt);                                                          // F
+    word(r.t1lo), word(r.t1c & 1), word((r.t1c >> 1) & 1), word((r.t1c >> 2) & 1), word(r.t1c);  // G
+    word(r.t2lo), word(r.t2c & 1), word((r.t2c >> 1) & 1), word(r.t2c);
+    for (int k = 0; k < 8; k++) word(r.hs[k]);                          // H
+    for (int k = 0; k < 15; k++) word(r.ww[k]);                         // I
+    bits(r.ww[14]), bits(r.ww[1]);                                       // J, K
+    const uint32_t ss0 = r.ss0t ^ (r.ww[1] >> 3), ss1 = r.ss1t ^ (r.ww[14] >> 10);
+    bits(r.ss0t), bits(ss0), word(ss0);                                  // L
+    bits(r.ss1t), bits(ss1), word(ss1);                                  // M
+    word(r.sc & 1), word((r.sc >> 1) & 1), word(r.sc);                   // N
+    word(r.na), word(r.ne);                                              // O
+}
+
+// Row words of one 128-row block (sha256_air.rs:648-898): rows 0..63 are the
 // rounds (state before round r), row 64 the post-compression state, rows
 // 65..127 idle with the state held; H-state is the incoming IV through row
-// 64 and the block digest from row 65. The row words come first, then the
-// trace is written column by column so every store stream is contiguous.
-// Returns the post-compression state.
-std::array<uint32_t, 8> fill_block(uint64_t *tr, uint64_t h, uint64_t base, const uint8_t *blk,
-                                   const uint32_t iv[8]) {
+// 64 and the block digest from row 65. Returns the post-compression state.
+std::array<uint32_t, 8> block_words(const uint8_t *blk, const uint32_t iv[8], RowWords *R) {
     uint32_t w[64];
     for (int i = 0; i < 16; i++)
         w[i] = (uint32_t)blk[4 * i] << 24 | (uint32_t)blk[4 * i + 1] << 16 | (uint32_t)blk[4 * i + 2] << 8 |
@@ -100,7 +128,6 @@ std::array<uint32_t, 8> fill_block(uint64_t *tr, uint64_t h, uint64_t base, cons
     }
     uint32_t dg[8];
     for (int k = 0; k < 8; k++) dg[k] = iv[k] + st[64][k];
-    static thread_local RowWords R[128];
     for (int br = 0; br < 128; br++) {
         RowWords &r = R[br];
         memcpy(r.st, st[br <= 64 ? br : 64], 32);
@@ -110,73 +137,39 @@ std::array<uint32_t, 8> fill_block(uint64_t *tr, uint64_t h, uint64_t base, cons
         r.chw = chf(e, f, g);
         r.mjw = majf(a, b, c);
         r.wt = br < 64 ? w[br] : 0;
-        r.t1 = (uint64_t)r.st[7] + big1(e) + r.chw + (br < 64 ? KH[br] : 0) + r.wt;
-        r.t2 = (uint64_t)big0(a) + r.mjw;
+        const uint64_t t1 = (uint64_t)r.st[7] + big1(e) + r.chw + (br < 64 ? KH[br] : 0) + r.wt;
+        const uint64_t t2 = (uint64_t)big0(a) + r.mjw;
+        r.t1lo = (uint32_t)t1;
+        r.t1c = (uint32_t)(t1 >> 32);
+        r.t2lo = (uint32_t)t2;
+        r.t2c = (uint32_t)(t2 >> 32);
         memcpy(r.hs, br <= 64 ? iv : dg, 32);
         for (int k = 0; k < 15; k++) r.ww[k] = wat(br - 15 + k);
         const uint32_t ww14 = r.ww[14], ww1 = r.ww[1];
         r.ss0t = rotr(ww1, 7) ^ rotr(ww1, 18);
         r.ss1t = rotr(ww14, 17) ^ rotr(ww14, 19);
         r.sc = (uint32_t)(((uint64_t)r.ww[0] + small0(ww1) + wat(br - 6) + small1(ww14)) >> 32);
-        r.na = br < 64 ? (uint32_t)(((uint64_t)(uint32_t)r.t1 + (uint32_t)r.t2) >> 32) : 0;
-        r.ne = br < 64 ? (uint32_t)(((uint64_t)d + (uint32_t)r.t1) >> 32) : 0;
+        r.na = br < 64 ? (uint32_t)(((uint64_t)r.t1lo + r.t2lo) >> 32) : 0;
+        r.ne = br < 64 ? (uint32_t)(((uint64_t)d + r.t1lo) >> 32) : 0;
     }
-    // column writers: col(c, f) stores f(row words) down column c
-    auto col = [&](uint32_t c, auto f) {
-        uint64_t *o = tr + (uint64_t)c * h + base;
-        for (int br = 0; br < 128; br++) o[br] = f(R[br]);
-    };
-    auto bits = [&](uint32_t c0, auto word) {
-        for (int i = 0; i < 32; i++) col(c0 + i, [&](const RowWords &r) { return (uint64_t)((word(r) >> i) & 1); });
-    };
-    for (uint32_t k = 0; k < 8; k++) col(A_A + k, [&](const RowWords &r) { return (uint64_t)r.st[k]; });
-    bits(BIT_A, [](const RowWords &r) { return r.st[0]; });
-    bits(BIT_B, [](const RowWords &r) { return r.st[1]; });
-    bits(BIT_C, [](const RowWords &r) { return r.st[2]; });
-    bits(BIT_E, [](const RowWords &r) { return r.st[4]; });
-    bits(BIT_F, [](const RowWords &r) { return r.st[5]; });
-    bits(BIT_G, [](const RowWords &r) { return r.st[6]; });
-    // Sigma0(a) = t ^ rotr(a, 22), Sigma1(e) = t ^ rotr(e, 25)
-    bits(S0_T, [](const RowWords &r) { return r.s0t; });
-    bits(S0_BIT, [](const RowWords &r) { return r.s0t ^ rotr(r.st[0], 22); });
-    col(S0_W, [](const RowWords &r) { return (uint64_t)(r.s0t ^ rotr(r.st[0], 22)); });
-    bits(S1_T, [](const RowWords &r) { return r.s1t; });
-    bits(S1_BIT, [](const RowWords &r) { return r.s1t ^ rotr(r.st[4], 25); });
-    col(S1_W, [](const RowWords &r) { return (uint64_t)(r.s1t ^ rotr(r.st[4], 25)); });
-    bits(CH_BIT, [](const RowWords &r) { return r.chw; });
-    col(CH_W, [](const RowWords &r) { return (uint64_t)r.chw; });
-    bits(MAJ_AB, [](const RowWords &r) { return r.st[0] & r.st[1]; });
-    bits(MAJ_AC, [](const RowWords &r) { return r.st[0] & r.st[2]; });
-    bits(MAJ_BC, [](const RowWords &r) { return r.st[1] & r.st[2]; });
-    bits(MAJ_U, [](const RowWords &r) { return (r.st[0] & r.st[1]) ^ (r.st[0] & r.st[2]); });
-    bits(MAJ_BIT, [](const RowWords &r) { return r.mjw; });
-    col(MAJ_W, [](const RowWords &r) { return (uint64_t)r.mjw; });
-    col(COL_W, [](const RowWords &r) { return (uint64_t)r.wt; });
-    col(T1_LO, [](const RowWords &r) { return r.t1 & 0xFFFFFFFFu; });
-    for (uint32_t k = 0; k < 3; k++) col(T1_C0 + k, [&](const RowWords &r) { return (r.t1 >> (32 + k)) & 1; });
-    col(T1_CW, [](const RowWords &r) { return r.t1 >> 32; });
-    col(T2_LO, [](const RowWords &r) { return r.t2 & 0xFFFFFFFFu; });
-    for (uint32_t k = 0; k < 2; k++) col(T2_C0 + k, [&](const RowWords &r) { return (r.t2 >> (32 + k)) & 1; });
-    col(T2_CW, [](const RowWords &r) { return r.t2 >> 32; });
-    for (uint32_t k = 0; k < 8; k++) col(H0 + k, [&](const RowWords &r) { return (uint64_t)r.hs[k]; });
-    for (uint32_t k = 0; k < 15; k++) col(WW0 + k, [&](const RowWords &r) { return (uint64_t)r.ww[k]; });
-    bits(BIT_WW14, [](const RowWords &r) { return r.ww[14]; });
-    bits(BIT_WW1, [](const RowWords &r) { return r.ww[1]; });
-    // sigma0(x) = t ^ (x >> 3), sigma1(x) = t ^ (x >> 10)
-    bits(SS0_T, [](const RowWords &r) { return r.ss0t; });
-    bits(SS0_BIT, [](const RowWords &r) { return r.ss0t ^ (r.ww[1] >> 3); });
-    col(SS0_W, [](const RowWords &r) { return (uint64_t)(r.ss0t ^ (r.ww[1] >> 3)); });
-    bits(SS1_T, [](const RowWords &r) { return r.ss1t; });
-    bits(SS1_BIT, [](const RowWords &r) { return r.ss1t ^ (r.ww[14] >> 10); });
-    col(SS1_W, [](const RowWords &r) { return (uint64_t)(r.ss1t ^ (r.ww[14] >> 10)); });
-    col(WS_C0, [](const RowWords &r) { return (uint64_t)(r.sc & 1); });
-    col(WS_C1, [](const RowWords &r) { return (uint64_t)((r.sc >> 1) & 1); });
-    col(WS_CW, [](const RowWords &r) { return (uint64_t)r.sc; });
-    col(NA_C, [](const RowWords &r) { return (uint64_t)r.na; });
-    col(NE_C, [](const RowWords &r) { return (uint64_t)r.ne; });
     std::array<uint32_t, 8> post;
     for (int k = 0; k < 8; k++) post[k] = st[64][k];
     return post;
+}
+
+// Device trace expansion: row r of proof k (grid.y) is built from row words
+// words[off[k] + min(r, useful[k] - 1)] (rows past the last block repeat the
+// last useful row); lanes hold consecutive rows, so every column store is
+// coalesced.
+__global__ __launch_bounds__(256) void expand_sha256_k(const RowWords *words, const uint64_t *off,
+                                                       const uint32_t *useful, uint64_t h, uint64_t *tr) {
+    const uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= h) return;
+    const uint32_t k = blockIdx.y;
+    const uint64_t src = off[k] + (r < useful[k] ? r : useful[k] - 1);
+    const RowWords rw = words[src];
+    uint64_t *o = tr + (uint64_t)k * SHA256_W * h + r;
+    emit_row(rw, [&](uint32_t c, uint64_t v) { o[(uint64_t)c * h] = v; });
 }
 
 // ---- constraints (sha256_air.rs:956-1294), one LDE point per lane ----
@@ -339,29 +332,70 @@ uint64_t sha256_trace_height(uint64_t len) {
 // block copy the last useful row. Also returns the digest as
 // prove_ds_ksk_binding reads it (prover.rs:529-538): H-state at block row 65
 // of the last block, big-endian words.
-uint64_t build_sha256_trace(const uint8_t *msg, uint64_t len, uint64_t *tr, uint8_t digest[32], uint64_t height) {
-    const uint64_t nb = sha256_n_blocks(len), h = height ? height : sha256_trace_height(len);
-    SG_REQUIRE(is_pow2(h) && h >= 128 * nb, "sha256 trace: height must be a power of two >= 128 * n_blocks");
+// row words of the padded message, block by block with the IV chained
+// (build_sha256_trace_multi, sha256_air.rs:429-479); returns n_blocks and the
+// digest as prove_ds_ksk_binding reads it (prover.rs:529-538): H-state at
+// block row 65 of the last block, big-endian words.
+static uint64_t sha256_words(const uint8_t *msg, uint64_t len, std::vector<sha::RowWords> &rows, uint8_t digest[32]) {
+    const uint64_t nb = sha256_n_blocks(len);
     std::vector<uint8_t> m(64 * nb, 0);
     if (len) memcpy(m.data(), msg, len);
     m[len] = 0x80;
     const uint64_t bits = len * 8;
     for (int i = 0; i < 8; i++) m[64 * nb - 1 - i] = (uint8_t)(bits >> (8 * i));
+    rows.resize(128 * nb);
     uint32_t iv[8];
     memcpy(iv, sha::IV, 32);
     for (uint64_t b = 0; b < nb; b++) {
-        const std::array<uint32_t, 8> post = sha::fill_block(tr, h, 128 * b, &m[64 * b], iv);
+        const std::array<uint32_t, 8> post = sha::block_words(&m[64 * b], iv, &rows[128 * b]);
         for (int k = 0; k < 8; k++) iv[k] += post[k];
     }
-    const uint64_t useful = 128 * nb;
-    for (uint64_t c = 0; c < SHA256_W; c++)
-        for (uint64_t r = useful; r < h; r++) tr[c * h + r] = tr[c * h + useful - 1];
-    const uint64_t drow = 128 * (nb - 1) + 65;
-    for (int k = 0; k < 8; k++) {
-        const uint32_t wv = (uint32_t)tr[(sha::H0 + k) * h + drow];
-        for (int i = 0; i < 4; i++) digest[4 * k + i] = (uint8_t)(wv >> (24 - 8 * i));
+    const sha::RowWords &d = rows[128 * (nb - 1) + 65];
+    for (int k = 0; k < 8; k++)
+        for (int i = 0; i < 4; i++) digest[4 * k + i] = (uint8_t)(d.hs[k] >> (24 - 8 * i));
+    return nb;
+}
+
+uint64_t build_sha256_trace(const uint8_t *msg, uint64_t len, uint64_t *tr, uint8_t digest[32], uint64_t height) {
+    std::vector<sha::RowWords> rows;
+    const uint64_t nb = sha256_words(msg, len, rows, digest), h = height ? height : sha256_trace_height(len);
+    SG_REQUIRE(is_pow2(h) && h >= 128 * nb, "sha256 trace: height must be a power of two >= 128 * n_blocks");
+    for (uint64_t r = 0; r < h; r++) {
+        const sha::RowWords &w = rows[r < rows.size() ? r : rows.size() - 1];
+        sha::emit_row(w, [&](uint32_t c, uint64_t v) { tr[(uint64_t)c * h + r] = v; });
     }
     return nb;
+}
+
+void build_sha256_traces_dev(Ctx *c, const std::vector<ByteSpan> &msgs, uint64_t h, uint64_t *d_tr,
+                             std::vector<uint32_t> &n_blocks, std::vector<std::array<uint8_t, 64>> &digests) {
+    const size_t K = msgs.size();
+    std::vector<std::vector<sha::RowWords>> rows(K);
+    n_blocks.assign(K, 0);
+    digests.assign(K, {});
+    parallel_for(K, [&](size_t k) {
+        n_blocks[k] = (uint32_t)sha256_words(msgs[k].p, msgs[k].n, rows[k], digests[k].data());
+    });
+    std::vector<uint64_t> off(K);
+    std::vector<uint32_t> useful(K);
+    uint64_t total = 0;
+    for (size_t k = 0; k < K; k++) {
+        SG_REQUIRE(128ULL * n_blocks[k] <= h, "sha256 traces: height below a message's minimum");
+        off[k] = total;
+        useful[k] = (uint32_t)rows[k].size();
+        total += rows[k].size();
+    }
+    std::vector<sha::RowWords> all(total);
+    parallel_for(K, [&](size_t k) { std::copy(rows[k].begin(), rows[k].end(), all.begin() + off[k]); });
+    Scoped d_w(c, sizeof(sha::RowWords) * total), d_off(c, 8 * K), d_u(c, 4 * K);
+    SG_HIP(hipMemcpyAsync(d_w.p, all.data(), sizeof(sha::RowWords) * total, hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipMemcpyAsync(d_off.p, off.data(), 8 * K, hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipMemcpyAsync(d_u.p, useful.data(), 4 * K, hipMemcpyHostToDevice, c->stream));
+    ProfScope ps(c, "expand_sha256", 8.0 * SHA256_W * h * K);
+    hipLaunchKernelGGL(sha::expand_sha256_k, dim3((uint32_t)((h + 255) / 256), (uint32_t)K), dim3(256), 0, c->stream,
+                       d_w.as<sha::RowWords>(), d_off.as<uint64_t>(), d_u.as<uint32_t>(), h, d_tr);
+    SG_HIP(hipGetLastError());
+    SG_HIP(hipStreamSynchronize(c->stream));  // the host word buffers go out of scope
 }
 
 void merge_sha256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup,

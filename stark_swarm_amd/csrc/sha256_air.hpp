@@ -23,10 +23,18 @@ uint64_t sha256_trace_height(uint64_t len);
 // (next_pow2(128 n_blocks)); a larger power of two pads by repeating the
 // last useful row, the builder's own padding rule.
 uint64_t build_sha256_trace(const uint8_t *msg, uint64_t len, uint64_t *tr, uint8_t digest[32], uint64_t height = 0);
+// K traces of height h built on the device (d_tr: K x 756 x h): the host
+// derives each row's 44 words (the sequential rounds), the device expands
+// them into cells and repeats the last useful row; returns n_blocks and the
+// 32-byte digests (first half of each 64-byte entry)
+void build_sha256_traces_dev(Ctx *c, const std::vector<ByteSpan> &msgs, uint64_t h, uint64_t *d_tr,
+                             std::vector<uint32_t> &n_blocks, std::vector<std::array<uint8_t, 64>> &digests);
 uint64_t sha512_n_blocks(uint64_t len);
 uint64_t sha512_trace_height(uint64_t len);
 // 1510 x height trace (two 32-bit limbs per word); digest = SHA-512(msg)
 uint64_t build_sha512_trace(const uint8_t *msg, uint64_t len, uint64_t *tr, uint8_t digest[64], uint64_t height = 0);
+void build_sha512_traces_dev(Ctx *c, const std::vector<ByteSpan> &msgs, uint64_t h, uint64_t *d_tr,
+                             std::vector<uint32_t> &n_blocks, std::vector<std::array<uint8_t, 64>> &digests);
 // deep_ali_merge_sha256 (lib.rs:453-545) for K proofs of one n_trace: lde of
 // proof k at lde + 756*k*n0, c_eval at c_eval + k*n0
 void merge_sha256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup,

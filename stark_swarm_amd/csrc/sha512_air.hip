@@ -63,16 +63,18 @@ static const uint64_t KH[80] = {SG_SHA512_K};  // FIPS 180-4 section 4.2.3
 __constant__ uint64_t KD[80] = {SG_SHA512_K};
 static const uint64_t IV[8] = {0x6a09e667f3bcc908ull, 0xbb67ae8584caa73bull, 0x3c6ef372fe94f82bull, 0xa54ff53a5f1d36f1ull, 0x510e527fade682d1ull, 0x9b05688c2b3e6c1full, 0x1f83d9abfb41bd6bull, 0x5be0cd19137e2179ull};  // section 5.3.5
 
-inline uint64_t rotr(uint64_t x, int n) { return (x >> n) | (x << (64 - n)); }
+SG_HD uint64_t rotr(uint64_t x, int n) { return (x >> n) | (x << (64 - n)); }
 inline uint64_t big0(uint64_t x) { return rotr(x, 28) ^ rotr(x, 34) ^ rotr(x, 39); }
 inline uint64_t big1(uint64_t x) { return rotr(x, 14) ^ rotr(x, 18) ^ rotr(x, 41); }
 inline uint64_t small0(uint64_t x) { return rotr(x, 1) ^ rotr(x, 8) ^ (x >> 7); }
 inline uint64_t small1(uint64_t x) { return rotr(x, 19) ^ rotr(x, 61) ^ (x >> 6); }
 inline uint64_t chf(uint64_t e, uint64_t f, uint64_t g) { return (e & f) ^ (~e & g); }
 inline uint64_t majf(uint64_t a, uint64_t b, uint64_t c) { return (a & b) ^ (a & c) ^ (b & c); }
-inline uint64_t lo(uint64_t x) { return x & 0xFFFFFFFFu; }
-inline uint64_t hi(uint64_t x) { return x >> 32; }
+SG_HD uint64_t lo(uint64_t x) { return x & 0xFFFFFFFFu; }
+SG_HD uint64_t hi(uint64_t x) { return x >> 32; }
 
+// Every cell of a trace row is a 32-bit limb of one of these words, one of
+// their bits, or a carry (46 u64 per row against 1510 cells).
 struct RowWords {
     uint64_t st[8];                   // a..h
     uint64_t s0t, s1t, chw, mjw, wt;  // rotr-xor halves of Sigma0/Sigma1, Ch, Maj, W[t]
@@ -80,14 +82,49 @@ struct RowWords {
     uint64_t hs[8], ww[15];           // H-state, W window
     uint64_t ss0t, ss1t;              // rotr-xor halves of sigma0(W[t-14]), sigma1(W[t-1])
     uint64_t sl, sh;                  // per-limb schedule sums
-    uint64_t na_l, na_h, ne_l, ne_h;  // new_a / new_e limb carries
+    uint32_t na_l, na_h, ne_l, ne_h;  // new_a / new_e limb carries
 };
 
-// One 128-row block (sha512_air.rs:585-898): rows 0..79 rounds, row 80 the
-// post-compression state, rows 81..127 idle; H-state = incoming IV through
-// row 80, the block digest from row 81. Written column by column.
-std::array<uint64_t, 8> fill_block(uint64_t *tr, uint64_t h, uint64_t base, const uint8_t *blk,
-                                   const uint64_t iv[8]) {
+// The 1510 cells of one row in column order (host builder and device expansion).
+template <class Put>
+SG_HD void emit_row(const RowWords &r, Put put) {
+    uint32_t c = 0;
+    auto cell = [&](uint64_t v) { put(c++, v); };
+    auto limbs = [&](uint64_t v) { cell(lo(v)), cell(hi(v)); };
+    auto bits = [&](uint64_t v) {
+        for (int i = 0; i < 64; i++) put(c++, (v >> i) & 1);
+    };
+    auto carry = [&](uint64_t sum, int nbits) {  // bits of sum >> 32, then the packed carry
+        for (int k = 0; k < nbits; k++) cell((sum >> (32 + k)) & 1);
+        cell(sum >> 32);
+    };
+    for (int k = 0; k < 8; k++) limbs(r.st[k]);                            // A
+    bits(r.st[0]), bits(r.st[1]), bits(r.st[2]);                           // B
+    bits(r.st[4]), bits(r.st[5]), bits(r.st[6]);
+    const uint64_t s0 = r.s0t ^ rotr(r.st[0], 39), s1 = r.s1t ^ rotr(r.st[4], 41);
+    bits(r.s0t), bits(s0), limbs(s0);                                      // C0
+    bits(r.s1t), bits(s1), limbs(s1);                                      // C1
+    bits(r.chw), limbs(r.chw);                                             // D
+    const uint64_t ab = r.st[0] & r.st[1], ac = r.st[0] & r.st[2], bc = r.st[1] & r.st[2];
+    bits(ab), bits(ac), bits(bc), bits(ab ^ ac), bits(r.mjw), limbs(r.mjw);  // E
+    limbs(r.wt);                                                           // F
+    cell(lo(r.t1l)), carry(r.t1l, 3), cell(lo(r.t1h)), carry(r.t1h, 3);    // G
+    cell(lo(r.t2l)), carry(r.t2l, 1), cell(lo(r.t2h)), carry(r.t2h, 1);
+    for (int k = 0; k < 8; k++) limbs(r.hs[k]);                           // H
+    for (int k = 0; k < 15; k++) limbs(r.ww[k]);                          // I
+    bits(r.ww[14]), bits(r.ww[1]);                                         // J, K
+    const uint64_t ss0 = r.ss0t ^ (r.ww[1] >> 7), ss1 = r.ss1t ^ (r.ww[14] >> 6);
+    bits(r.ss0t), bits(ss0), limbs(ss0);                                   // L
+    bits(r.ss1t), bits(ss1), limbs(ss1);                                   // M
+    carry(r.sl, 2), carry(r.sh, 2);                                        // N
+    cell(r.na_l), cell(r.na_h), cell(r.ne_l), cell(r.ne_h);                // O
+}
+
+// Row words of one 128-row block (sha512_air.rs:585-898): rows 0..79 rounds,
+// row 80 the post-compression state, rows 81..127 idle; H-state = incoming
+// IV through row 80, the block digest from row 81. Returns the
+// post-compression state.
+std::array<uint64_t, 8> block_words(const uint8_t *blk, const uint64_t iv[8], RowWords *R) {
     uint64_t w[80];
     for (int i = 0; i < 16; i++) {
         uint64_t v = 0;
@@ -107,7 +144,6 @@ std::array<uint64_t, 8> fill_block(uint64_t *tr, uint64_t h, uint64_t base, cons
     }
     uint64_t dg[8];
     for (int k = 0; k < 8; k++) dg[k] = iv[k] + st[80][k];
-    static thread_local RowWords R[128];
     for (int br = 0; br < 128; br++) {
         RowWords &r = R[br];
         memcpy(r.st, st[br <= 80 ? br : 80], 64);
@@ -132,78 +168,30 @@ std::array<uint64_t, 8> fill_block(uint64_t *tr, uint64_t h, uint64_t base, cons
         if (br < 80) {
             const uint64_t nal = lo(r.t1l) + lo(r.t2l), nah = lo(r.t1h) + lo(r.t2h) + (nal >> 32);
             const uint64_t nel = lo(d) + lo(r.t1l), neh = hi(d) + lo(r.t1h) + (nel >> 32);
-            r.na_l = nal >> 32;
-            r.na_h = nah >> 32;
-            r.ne_l = nel >> 32;
-            r.ne_h = neh >> 32;
+            r.na_l = (uint32_t)(nal >> 32);
+            r.na_h = (uint32_t)(nah >> 32);
+            r.ne_l = (uint32_t)(nel >> 32);
+            r.ne_h = (uint32_t)(neh >> 32);
         } else {
             r.na_l = r.na_h = r.ne_l = r.ne_h = 0;
         }
     }
-    auto col = [&](uint32_t cc, auto fn) {
-        uint64_t *o = tr + (uint64_t)cc * h + base;
-        for (int br = 0; br < 128; br++) o[br] = fn(R[br]);
-    };
-    auto bits = [&](uint32_t c0, auto word) {
-        for (int i = 0; i < 64; i++) col(c0 + i, [&](const RowWords &r) { return (word(r) >> i) & 1; });
-    };
-    auto limbs = [&](uint32_t c0, auto word) {
-        col(c0, [&](const RowWords &r) { return lo(word(r)); });
-        col(c0 + 1, [&](const RowWords &r) { return hi(word(r)); });
-    };
-    auto carry = [&](uint32_t c0, int nbits, auto sum) {  // bits of sum >> 32, then the packed carry
-        for (int k = 0; k < nbits; k++) col(c0 + k, [&](const RowWords &r) { return (sum(r) >> (32 + k)) & 1; });
-        col(c0 + nbits, [&](const RowWords &r) { return sum(r) >> 32; });
-    };
-    for (uint32_t k = 0; k < 8; k++) limbs(A0 + 2 * k, [&](const RowWords &r) { return r.st[k]; });
-    bits(BIT_A, [](const RowWords &r) { return r.st[0]; });
-    bits(BIT_B, [](const RowWords &r) { return r.st[1]; });
-    bits(BIT_C, [](const RowWords &r) { return r.st[2]; });
-    bits(BIT_E, [](const RowWords &r) { return r.st[4]; });
-    bits(BIT_F, [](const RowWords &r) { return r.st[5]; });
-    bits(BIT_G, [](const RowWords &r) { return r.st[6]; });
-    bits(S0_T, [](const RowWords &r) { return r.s0t; });
-    bits(S0_BIT, [](const RowWords &r) { return r.s0t ^ rotr(r.st[0], 39); });
-    limbs(S0_LO, [](const RowWords &r) { return r.s0t ^ rotr(r.st[0], 39); });
-    bits(S1_T, [](const RowWords &r) { return r.s1t; });
-    bits(S1_BIT, [](const RowWords &r) { return r.s1t ^ rotr(r.st[4], 41); });
-    limbs(S1_LO, [](const RowWords &r) { return r.s1t ^ rotr(r.st[4], 41); });
-    bits(CH_BIT, [](const RowWords &r) { return r.chw; });
-    limbs(CH_LO, [](const RowWords &r) { return r.chw; });
-    bits(MAJ_AB, [](const RowWords &r) { return r.st[0] & r.st[1]; });
-    bits(MAJ_AC, [](const RowWords &r) { return r.st[0] & r.st[2]; });
-    bits(MAJ_BC, [](const RowWords &r) { return r.st[1] & r.st[2]; });
-    bits(MAJ_U, [](const RowWords &r) { return (r.st[0] & r.st[1]) ^ (r.st[0] & r.st[2]); });
-    bits(MAJ_BIT, [](const RowWords &r) { return r.mjw; });
-    limbs(MAJ_LO, [](const RowWords &r) { return r.mjw; });
-    limbs(W_LO, [](const RowWords &r) { return r.wt; });
-    col(T1L_LO, [](const RowWords &r) { return lo(r.t1l); });
-    carry(T1L_C0, 3, [](const RowWords &r) { return r.t1l; });
-    col(T1H_LO, [](const RowWords &r) { return lo(r.t1h); });
-    carry(T1H_C0, 3, [](const RowWords &r) { return r.t1h; });
-    col(T2L_LO, [](const RowWords &r) { return lo(r.t2l); });
-    carry(T2L_C0, 1, [](const RowWords &r) { return r.t2l; });
-    col(T2H_LO, [](const RowWords &r) { return lo(r.t2h); });
-    carry(T2H_C0, 1, [](const RowWords &r) { return r.t2h; });
-    for (uint32_t k = 0; k < 8; k++) limbs(H0 + 2 * k, [&](const RowWords &r) { return r.hs[k]; });
-    for (uint32_t k = 0; k < 15; k++) limbs(WW0 + 2 * k, [&](const RowWords &r) { return r.ww[k]; });
-    bits(BIT_WW14, [](const RowWords &r) { return r.ww[14]; });
-    bits(BIT_WW1, [](const RowWords &r) { return r.ww[1]; });
-    bits(SS0_T, [](const RowWords &r) { return r.ss0t; });
-    bits(SS0_BIT, [](const RowWords &r) { return r.ss0t ^ (r.ww[1] >> 7); });
-    limbs(SS0_LO, [](const RowWords &r) { return r.ss0t ^ (r.ww[1] >> 7); });
-    bits(SS1_T, [](const RowWords &r) { return r.ss1t; });
-    bits(SS1_BIT, [](const RowWords &r) { return r.ss1t ^ (r.ww[14] >> 6); });
-    limbs(SS1_LO, [](const RowWords &r) { return r.ss1t ^ (r.ww[14] >> 6); });
-    carry(WSL_C0, 2, [](const RowWords &r) { return r.sl; });
-    carry(WSH_C0, 2, [](const RowWords &r) { return r.sh; });
-    col(NA_LO, [](const RowWords &r) { return r.na_l; });
-    col(NA_HI, [](const RowWords &r) { return r.na_h; });
-    col(NE_LO, [](const RowWords &r) { return r.ne_l; });
-    col(NE_HI, [](const RowWords &r) { return r.ne_h; });
     std::array<uint64_t, 8> post;
     for (int k = 0; k < 8; k++) post[k] = st[80][k];
     return post;
+}
+
+// Device expansion (as for SHA-256): lane = row, rows past the last block
+// repeat the last useful row.
+__global__ __launch_bounds__(256) void expand_sha512_k(const RowWords *words, const uint64_t *off,
+                                                       const uint32_t *useful, uint64_t h, uint64_t *tr) {
+    const uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= h) return;
+    const uint32_t k = blockIdx.y;
+    const uint64_t src = off[k] + (r < useful[k] ? r : useful[k] - 1);
+    const RowWords rw = words[src];
+    uint64_t *o = tr + (uint64_t)k * SHA512_W * h + r;
+    emit_row(rw, [&](uint32_t c, uint64_t v) { o[(uint64_t)c * h] = v; });
 }
 
 // ---- constraints (sha512_air.rs:909-1282), one LDE point per lane ----
@@ -385,29 +373,69 @@ uint64_t sha512_trace_height(uint64_t len) {
     return h;
 }
 
-uint64_t build_sha512_trace(const uint8_t *msg, uint64_t len, uint64_t *tr, uint8_t digest[64], uint64_t height) {
-    const uint64_t nb = sha512_n_blocks(len), h = height ? height : sha512_trace_height(len);
-    SG_REQUIRE(is_pow2(h) && h >= 128 * nb, "sha512 trace: height must be a power of two >= 128 * n_blocks");
+// row words of the padded message (build_sha512_trace_multi,
+// sha512_air.rs:551-573); digest from the H-state at block row 81 of the
+// last block (read_digest_from_trace, :1285-1297)
+static uint64_t sha512_words(const uint8_t *msg, uint64_t len, std::vector<sha5::RowWords> &rows, uint8_t digest[64]) {
+    const uint64_t nb = sha512_n_blocks(len);
     std::vector<uint8_t> m(128 * nb, 0);
     if (len) memcpy(m.data(), msg, len);
     m[len] = 0x80;
     const uint64_t bits = len * 8;  // the upper half of the 128-bit length is zero
     for (int i = 0; i < 8; i++) m[128 * nb - 1 - i] = (uint8_t)(bits >> (8 * i));
+    rows.resize(128 * nb);
     uint64_t iv[8];
     memcpy(iv, sha5::IV, 64);
     for (uint64_t b = 0; b < nb; b++) {
-        const std::array<uint64_t, 8> post = sha5::fill_block(tr, h, 128 * b, &m[128 * b], iv);
+        const std::array<uint64_t, 8> post = sha5::block_words(&m[128 * b], iv, &rows[128 * b]);
         for (int k = 0; k < 8; k++) iv[k] += post[k];
     }
-    const uint64_t useful = 128 * nb;
-    for (uint64_t c = 0; c < SHA512_W; c++)
-        for (uint64_t r = useful; r < h; r++) tr[c * h + r] = tr[c * h + useful - 1];
-    const uint64_t drow = 128 * (nb - 1) + 81;
-    for (int k = 0; k < 8; k++) {
-        const uint64_t v = tr[(sha5::H0 + 2 * k) * h + drow] | tr[(sha5::H0 + 2 * k + 1) * h + drow] << 32;
-        for (int i = 0; i < 8; i++) digest[8 * k + i] = (uint8_t)(v >> (56 - 8 * i));
+    const sha5::RowWords &d = rows[128 * (nb - 1) + 81];
+    for (int k = 0; k < 8; k++)
+        for (int i = 0; i < 8; i++) digest[8 * k + i] = (uint8_t)(d.hs[k] >> (56 - 8 * i));
+    return nb;
+}
+
+uint64_t build_sha512_trace(const uint8_t *msg, uint64_t len, uint64_t *tr, uint8_t digest[64], uint64_t height) {
+    std::vector<sha5::RowWords> rows;
+    const uint64_t nb = sha512_words(msg, len, rows, digest), h = height ? height : sha512_trace_height(len);
+    SG_REQUIRE(is_pow2(h) && h >= 128 * nb, "sha512 trace: height must be a power of two >= 128 * n_blocks");
+    for (uint64_t r = 0; r < h; r++) {
+        const sha5::RowWords &w = rows[r < rows.size() ? r : rows.size() - 1];
+        sha5::emit_row(w, [&](uint32_t c, uint64_t v) { tr[(uint64_t)c * h + r] = v; });
     }
     return nb;
+}
+
+void build_sha512_traces_dev(Ctx *c, const std::vector<ByteSpan> &msgs, uint64_t h, uint64_t *d_tr,
+                             std::vector<uint32_t> &n_blocks, std::vector<std::array<uint8_t, 64>> &digests) {
+    const size_t K = msgs.size();
+    std::vector<std::vector<sha5::RowWords>> rows(K);
+    n_blocks.assign(K, 0);
+    digests.assign(K, {});
+    parallel_for(K, [&](size_t k) {
+        n_blocks[k] = (uint32_t)sha512_words(msgs[k].p, msgs[k].n, rows[k], digests[k].data());
+    });
+    std::vector<uint64_t> off(K);
+    std::vector<uint32_t> useful(K);
+    uint64_t total = 0;
+    for (size_t k = 0; k < K; k++) {
+        SG_REQUIRE(128ULL * n_blocks[k] <= h, "sha512 traces: height below a message's minimum");
+        off[k] = total;
+        useful[k] = (uint32_t)rows[k].size();
+        total += rows[k].size();
+    }
+    std::vector<sha5::RowWords> all(total);
+    parallel_for(K, [&](size_t k) { std::copy(rows[k].begin(), rows[k].end(), all.begin() + off[k]); });
+    Scoped d_w(c, sizeof(sha5::RowWords) * total), d_off(c, 8 * K), d_u(c, 4 * K);
+    SG_HIP(hipMemcpyAsync(d_w.p, all.data(), sizeof(sha5::RowWords) * total, hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipMemcpyAsync(d_off.p, off.data(), 8 * K, hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipMemcpyAsync(d_u.p, useful.data(), 4 * K, hipMemcpyHostToDevice, c->stream));
+    ProfScope ps(c, "expand_sha512", 8.0 * SHA512_W * h * K);
+    hipLaunchKernelGGL(sha5::expand_sha512_k, dim3((uint32_t)((h + 255) / 256), (uint32_t)K), dim3(256), 0, c->stream,
+                       d_w.as<sha5::RowWords>(), d_off.as<uint64_t>(), d_u.as<uint32_t>(), h, d_tr);
+    SG_HIP(hipGetLastError());
+    SG_HIP(hipStreamSynchronize(c->stream));
 }
 
 void merge_sha512_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup,

@@ -375,6 +375,15 @@ def test_sha_batch_matches_oracle(S):
     for m, f, o in zip(msgs256, fsb2, outs):
         proof, dg, nb = _oracle_sha_proof(256, m, f, height=256)
         assert o.n_trace == 256 and o.digest == dg and o.proof_blob == proof
+    # both hash AIRs and several heights in one call, against the single calls
+    algs = [256, 512, 256, 512]
+    msgs = [b"a" * 5, b"b" * 200, b"c" * 100, b"d"]
+    hs = [512, 256, 256, 128]
+    fsb3 = [bytes([i]) * 32 for i in range(4)]
+    mixed = S.prove_sha_batch(algs, msgs, fsb3, heights=hs)
+    for i in range(4):
+        one = S.prove_sha_batch(algs[i], [msgs[i]], [fsb3[i]], heights=[hs[i]])[0]
+        assert mixed[i].proof_blob == one.proof_blob and mixed[i].n_trace == hs[i]
 
 
 def test_server_results_equal_standalone(S):
