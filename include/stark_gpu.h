@@ -102,6 +102,15 @@ typedef struct {
     size_t proof_len;
 } stark_sha_out;
 
+/* One RSA-2048 exponentiation proof of stark_prove_rsa_exp_batch */
+typedef struct {
+    uint8_t pi_hash[32]; /* SHA3-256(public_inputs_hash || root_f0) */
+    uint8_t root_f0[32];
+    size_t n_trace;
+    uint8_t *proof; /* ark-serialize compressed DeepFriProof<SexticExt>; stark_free() it */
+    size_t proof_len;
+} stark_rsa_out;
+
 /* ---- proving server (one GPU, continuous batching) ---- */
 typedef struct stark_server stark_server;
 
@@ -231,6 +240,27 @@ int stark_prove_sha_batch(stark_ctx *ctx, const int *algs, const uint8_t *msgs, 
                           const size_t *heights, size_t n, const uint8_t *fs_bindings, int phi_mode,
                           stark_sha_out *outs);
 
+/* deep_ali_merge_rsa_exp_multirow_streaming (crates/deep_ali/src/lib.rs:
+ * 1827-1944; n_records = 1) and deep_ali_merge_rsa_stacked_streaming
+ * (lib.rs:1949-2049; n_records stacked RSA-2048 chains): lde is the
+ * (11041 n_records) x n0 LDE, coeffs the 11280 n_records coefficients
+ * (lambda_j = j+1 only). The streaming merges always evaluate phi. */
+int stark_merge_rsa_exp(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_records,
+                        const uint64_t *coeffs, size_t k, uint64_t *c_eval);
+
+/* prove_record_rsa (crates/swarm-dns/examples/mixed_algorithm_zone_rollup.rs:
+ * 349-395) from the trace on, for n_proofs proofs of records_per_proof
+ * stacked RSA-2048 records each: record j of proof i is (n, s, em) at byte
+ * offset 256 (i records_per_proof + j) of n_be, s_be, em_be (big-endian,
+ * s < n; em as the chain's expected s^65537 mod n, the caller's PKCS#1
+ * encoding). Trace of n_trace rows (a power of two >= 32; the reference
+ * uses 32) -> LDE x32 -> merge -> STIR with 54 queries bound to
+ * public_inputs_hashes[32 i ..]. Lockstep device batches in HBM-sized
+ * waves; result i equals the proof of its records alone. */
+int stark_prove_rsa_exp_batch(stark_ctx *ctx, const uint8_t *n_be, const uint8_t *s_be, const uint8_t *em_be,
+                              size_t n_proofs, size_t records_per_proof, size_t n_trace,
+                              const uint8_t *public_inputs_hashes, stark_rsa_out *outs);
+
 /* prove_outer_rollup (crates/swarm-dns/src/prover.rs:253-304). */
 int stark_prove_outer_rollup(stark_ctx *ctx, const uint8_t *pi_hashes, size_t n, const uint8_t pk_hash[32], int ldt,
                              int phi_mode, uint8_t **proof, size_t *proof_len, uint8_t root_f0[32]);
@@ -298,6 +328,15 @@ size_t stark_host_build_sha256_trace(const uint8_t *msg, size_t len, uint64_t *t
 size_t stark_host_sha512_trace_height(size_t len);
 size_t stark_host_build_sha512_trace(const uint8_t *msg, size_t len, uint64_t *trace, uint8_t digest[64]);
 void stark_host_dns_leaf_hash(const stark_dns_record *rec, const uint8_t salt[16], uint8_t out[32]);
+/* fill_rsa_exp_multirow (crates/deep_ali/src/rsa2048_exp_air.rs:148-202) /
+ * fill_rsa_stacked (rsa2048_stacked_air.rs:64-78): column-major trace of
+ * 11041 n_records columns x n_trace rows (n_trace a power of two >= 32) for
+ * the records (n, s, em) at 256-byte strides. 0, or STARK_E_INVALID for
+ * s >= n, n = 0 or a bad height. */
+int stark_host_build_rsa_exp_trace(const uint8_t *n_be, const uint8_t *s_be, const uint8_t *em_be, size_t n_records,
+                                   size_t n_trace, uint64_t *trace);
+/* emsa_pkcs1_v1_5_encode_sha256 (crates/deep_ali/src/rsa2048.rs) for k = 256 */
+void stark_host_rsa_pkcs1_em_sha256(const uint8_t digest[32], uint8_t em[256]);
 
 #ifdef __cplusplus
 }

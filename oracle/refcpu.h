@@ -110,6 +110,23 @@ void ref_sha512_constraints(const uint64_t *cur, const uint64_t *nxt, size_t row
 int ref_merge_sha512(const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_blocks, int phi_mode,
                      uint64_t *c_eval);
 
+/* ---- RSA-2048 exponentiation AIR (s^65537 mod n; rsa2048_exp_air.rs) ---- */
+size_t ref_rsa_exp_width(void);                  /* 11 041 columns per record */
+size_t ref_rsa_exp_constraints_per_record(void); /* 11 280 */
+/* record at columns [base, base + 11041) of a column-major trace of ht rows;
+ * n, s, em: 256-byte big-endian, s < n. 0 or a negative code. */
+int ref_fill_rsa_exp(const uint8_t n_be[256], const uint8_t s_be[256], const uint8_t em_be[256], size_t ht,
+                     size_t base, uint64_t *tr);
+void ref_rsa_exp_constraints(const uint64_t *cur, const uint64_t *nxt, size_t trace_row, size_t n_trace,
+                             uint64_t *out);
+/* n_records stacked records; coeffs NULL = lambda_j = j + 1 */
+int ref_merge_rsa(const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_records, const uint64_t *coeffs,
+                  uint64_t *c_eval);
+void ref_rsa_pkcs1_em_sha256(const uint8_t digest[32], uint8_t em[256]);
+int ref_prove_rsa_exp(const uint8_t n_be[256], const uint8_t s_be[256], const uint8_t em_be[256],
+                      const uint8_t pis[32], uint8_t **proof, size_t *proof_len, uint8_t root_f0[32],
+                      uint8_t pi_hash[32]);
+
 /* ---- prover-core microbench (coset LDE -> row Merkle -> STIR prove) ---- */
 /* trace: w x n_trace column-major; returns proof bytes of deep_fri_prove over
  * f0 = sum_c (c+1) * LDE_c with the row-tree root as public_inputs_hash, and

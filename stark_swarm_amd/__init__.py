@@ -16,6 +16,14 @@ Rust functions they replace:
   deep_ali_merge_sha256    crates/deep_ali/src/lib.rs:453-545
   build_sha512_trace_multi crates/deep_ali/src/sha512_air.rs:551-573
   deep_ali_merge_sha512    crates/deep_ali/src/lib.rs:561-653
+  fill_rsa_exp_multirow, fill_rsa_stacked
+                           crates/deep_ali/src/rsa2048_exp_air.rs:148-202,
+                           rsa2048_stacked_air.rs:64-78
+  deep_ali_merge_rsa_exp_multirow_streaming, deep_ali_merge_rsa_stacked_streaming
+                           crates/deep_ali/src/lib.rs:1827-2049
+  emsa_pkcs1_v1_5_encode_sha256
+                           crates/deep_ali/src/rsa2048.rs
+  prove_rsa_exp_batch      crates/swarm-dns/examples/mixed_algorithm_zone_rollup.rs:324-395
   DnsRecord                crates/swarm-dns/src/dns.rs:24-89
   shard_fs_binding         crates/swarm-proto/src/messages.rs:162-176
 
@@ -49,6 +57,15 @@ from .api import (  # noqa: F401
     deep_ali_merge_sha512,
     build_sha512_trace_multi,
     prove_sha_batch,
+    RsaStackedRecord,
+    RsaProof,
+    fill_rsa_exp_multirow,
+    fill_rsa_stacked,
+    deep_ali_merge_rsa_exp_multirow_streaming,
+    deep_ali_merge_rsa_stacked_streaming,
+    emsa_pkcs1_v1_5_encode_sha256,
+    rsa_record_pk_hash,
+    prove_rsa_exp_batch,
     deep_fri_prove,
     deep_fri_prove_batch,
     default_context,

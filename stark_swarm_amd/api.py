@@ -53,6 +53,11 @@ class _ShaOut(ctypes.Structure):
                 ("n_trace", sz), ("proof", u8p), ("proof_len", sz)]
 
 
+class _RsaOut(ctypes.Structure):
+    _fields_ = [("pi_hash", ctypes.c_uint8 * 32), ("root_f0", ctypes.c_uint8 * 32), ("n_trace", sz),
+                ("proof", u8p), ("proof_len", sz)]
+
+
 class _DsKskOut(ctypes.Structure):
     _fields_ = [("pi_hash", ctypes.c_uint8 * 32), ("asserted_digest", ctypes.c_uint8 * 32),
                 ("root_f0", ctypes.c_uint8 * 32), ("n_blocks", sz), ("n_trace", sz),
@@ -125,6 +130,12 @@ def lib():
                                                  ctypes.POINTER(sz), ctypes.POINTER(sz), sz, ctypes.c_char_p,
                                                  ctypes.c_int, ctypes.POINTER(_ShaOut)]),
         "stark_host_build_sha256_trace": (sz, [ctypes.c_char_p, sz, u64p, ctypes.c_char_p]),
+        "stark_merge_rsa_exp": (ctypes.c_int, [vp, u64p, sz, sz, sz, u64p, sz, u64p]),
+        "stark_prove_rsa_exp_batch": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz, sz,
+                                                     sz, ctypes.c_char_p, ctypes.POINTER(_RsaOut)]),
+        "stark_host_build_rsa_exp_trace": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz,
+                                                          sz, u64p]),
+        "stark_host_rsa_pkcs1_em_sha256": (None, [ctypes.c_char_p, ctypes.c_char_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -434,6 +445,112 @@ def prove_sha_batch(alg, messages, fs_bindings, heights=None, phi_mode=PHI_RAYON
         dl = 32 if algs[k] == 256 else 64
         res.append(ShaProof(bytes(o.digest)[:dl], bytes(o.root_f0), o.n_blocks, o.n_trace, _take(o.proof, o.proof_len)))
     return res
+
+
+# ------------------------------------------------------------ RSA-2048 AIR
+RSA_EXP_WIDTH, RSA_EXP_CONSTRAINTS, RSA_EXP_N_TRACE = 11041, 11280, 32
+
+
+@dataclass
+class RsaStackedRecord:
+    """RsaStackedRecord (rsa2048_stacked_air.rs:58-62): modulus n, signature
+    value s < n and the encoded message em = s^65537 mod n, as integers."""
+    n: int
+    s: int
+    em: int
+
+
+def _rsa_bytes(records):
+    def be(x):
+        return int(x).to_bytes(256, "big")
+    return (b"".join(be(r.n) for r in records), b"".join(be(r.s) for r in records),
+            b"".join(be(r.em) for r in records))
+
+
+def fill_rsa_stacked(records, n_trace=RSA_EXP_N_TRACE):
+    """fill_rsa_stacked (rsa2048_stacked_air.rs:64-78); one record is
+    fill_rsa_exp_multirow (rsa2048_exp_air.rs:148-202). Returns the
+    (11041 * len(records)) x n_trace trace."""
+    records = list(records)
+    tr = np.zeros((RSA_EXP_WIDTH * len(records), n_trace), dtype=np.uint64)
+    nb, sb, eb = _rsa_bytes(records)
+    rc = lib().stark_host_build_rsa_exp_trace(nb, sb, eb, len(records), n_trace, tr.ctypes.data_as(u64p))
+    if rc != 0:
+        raise StarkError(rc, "fill_rsa_stacked: need 0 < n, s < n and n_trace a power of two >= 32")
+    return tr
+
+
+def fill_rsa_exp_multirow(n, s, em, n_trace=RSA_EXP_N_TRACE):
+    return fill_rsa_stacked([RsaStackedRecord(n, s, em)], n_trace)
+
+
+def deep_ali_merge_rsa_stacked_streaming(trace_evals_on_lde, combination_coeffs, n_records, omega, n_trace, blowup,
+                                         ctx=None):
+    """deep_ali_merge_rsa_stacked_streaming (lib.rs:1949-2049); with one
+    record it is deep_ali_merge_rsa_exp_multirow_streaming (lib.rs:1827-1944)."""
+    n = n_trace * blowup
+    k = RSA_EXP_CONSTRAINTS * n_records
+    lde = np.asarray(trace_evals_on_lde, dtype=np.uint64)
+    assert lde.shape == (RSA_EXP_WIDTH * n_records, n), "trace shape mismatch"
+    assert len(combination_coeffs) == k, \
+        "need one combination coefficient per constraint, got %d for %d constraints" % (len(combination_coeffs), k)
+    del omega  # unused by the reference too (lib.rs:1838)
+    ctx = ctx or default_context()
+    lde, lp = _u64(lde)
+    co, cp = _u64(np.asarray(combination_coeffs, dtype=np.uint64))
+    out = np.empty(n, dtype=np.uint64)
+    ctx._check(lib().stark_merge_rsa_exp(ctx.h, lp, n_trace, blowup, n_records, cp, k, out.ctypes.data_as(u64p)))
+    return out, CompositionInfo(2 * n_trace, n_trace, n_trace / n, k, 2, RSA_EXP_WIDTH * n_records)
+
+
+def deep_ali_merge_rsa_exp_multirow_streaming(trace_evals_on_lde, combination_coeffs, omega, n_trace, blowup,
+                                              ctx=None):
+    return deep_ali_merge_rsa_stacked_streaming(trace_evals_on_lde, combination_coeffs, 1, omega, n_trace, blowup,
+                                                ctx)
+
+
+def emsa_pkcs1_v1_5_encode_sha256(digest, k=256):
+    """emsa_pkcs1_v1_5_encode_sha256 (rsa2048.rs) for k = 256."""
+    assert k == 256 and len(digest) == 32
+    em = ctypes.create_string_buffer(256)
+    lib().stark_host_rsa_pkcs1_em_sha256(bytes(digest), em)
+    return em.raw
+
+
+def rsa_record_pk_hash(n_be, signature, message, em_bytes):
+    """The per-record public-input hash of the mixed-algorithm zone example
+    (mixed_algorithm_zone_rollup.rs:370-378): SHA3-256("RSA2048-MIXEDZONE"
+    || n || signature || message || em)."""
+    import hashlib
+    h = hashlib.sha3_256()
+    for part in (b"RSA2048-MIXEDZONE", n_be, signature, message, em_bytes):
+        h.update(bytes(part))
+    return h.digest()
+
+
+@dataclass
+class RsaProof:
+    pi_hash: bytes
+    root_f0: bytes
+    n_trace: int
+    proof_blob: bytes
+
+
+def prove_rsa_exp_batch(records, public_inputs_hashes, records_per_proof=1, n_trace=RSA_EXP_N_TRACE, ctx=None):
+    """RSA-2048 exponentiation proofs (stark_prove_rsa_exp_batch): the
+    prove_record_rsa pipeline (mixed_algorithm_zone_rollup.rs:349-395) from
+    the trace on, for len(records) / records_per_proof proofs of that many
+    stacked records each; pi_hash = SHA3-256(public_inputs_hash || root_f0)."""
+    ctx = ctx or default_context()
+    records = list(records)
+    assert records and len(records) % records_per_proof == 0
+    n = len(records) // records_per_proof
+    assert len(public_inputs_hashes) == n
+    nb, sb, eb = _rsa_bytes(records)
+    outs = (_RsaOut * n)()
+    ctx._check(lib().stark_prove_rsa_exp_batch(ctx.h, nb, sb, eb, n, records_per_proof, n_trace,
+                                               b"".join(bytes(x) for x in public_inputs_hashes), outs))
+    return [RsaProof(bytes(o.pi_hash), bytes(o.root_f0), o.n_trace, _take(o.proof, o.proof_len)) for o in outs]
 
 
 # --------------------------------------------------------------------- FRI

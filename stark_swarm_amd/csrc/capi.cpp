@@ -15,6 +15,7 @@
 #include "keccak.hpp"
 #include "ntt.hpp"
 #include "prover.hpp"
+#include "rsa_air.hpp"
 #include "server.hpp"
 #include "sha256_air.hpp"
 
@@ -279,6 +280,58 @@ int stark_prove_sha_batch(stark_ctx *ctx, const int *algs, const uint8_t *msgs, 
             memcpy(outs[i].digest, r[i].digest, 64);
             memcpy(outs[i].root_f0, r[i].root_f0, 32);
             outs[i].n_blocks = r[i].n_blocks;
+            outs[i].n_trace = r[i].n_trace;
+            outs[i].proof = copy_out(r[i].proof, &outs[i].proof_len);
+        }
+    });
+}
+
+static std::vector<sg::RsaRecord> rsa_records(const uint8_t *n_be, const uint8_t *s_be, const uint8_t *em_be,
+                                               size_t count) {
+    SG_REQUIRE(n_be && s_be && em_be, "rsa: null record bytes");
+    SG_REQUIRE(count >= 1 && count <= 65535, "rsa: 1 to 65535 records");
+    std::vector<sg::RsaRecord> r(count);
+    for (size_t i = 0; i < count; i++) {
+        memcpy(r[i].n.data(), n_be + 256 * i, 256);
+        memcpy(r[i].s.data(), s_be + 256 * i, 256);
+        memcpy(r[i].em.data(), em_be + 256 * i, 256);
+    }
+    return r;
+}
+
+int stark_merge_rsa_exp(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_records,
+                        const uint64_t *coeffs, size_t k, uint64_t *c_eval) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(lde && coeffs && c_eval, "merge_rsa_exp: null argument");
+        SG_REQUIRE(n_records >= 1 && n_records <= sg::RSA_MAX_RECORDS, "merge_rsa_exp: 1 to 4096 records");
+        SG_REQUIRE(k == sg::RSA_K * n_records, "merge_rsa_exp: need one coefficient per constraint (11280 per record)");
+        for (size_t j = 0; j < k; j++)
+            SG_REQUIRE(coeffs[j] == j + 1, "merge_rsa_exp: only the reference combination coefficients (j + 1)");
+        SG_REQUIRE(n_trace <= (1u << 20) && blowup <= (1u << 20), "merge_rsa_exp: sizes out of range");
+        const size_t n0 = n_trace * blowup, w = (size_t)sg::RSA_W * n_records;
+        sg::Scoped dl(c, 8ULL * w * n0), dc(c, 8 * n0);
+        h2d(c, dl.p, lde, 8ULL * w * n0);
+        sg::merge_rsa_batch(c, dl.as<uint64_t>(), 1, (uint32_t)n_records, n_trace, blowup, dc.as<uint64_t>());
+        d2h(c, c_eval, dc.p, 8 * n0);
+    });
+}
+
+int stark_prove_rsa_exp_batch(stark_ctx *ctx, const uint8_t *n_be, const uint8_t *s_be, const uint8_t *em_be,
+                              size_t n_proofs, size_t records_per_proof, size_t n_trace,
+                              const uint8_t *public_inputs_hashes, stark_rsa_out *outs) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(public_inputs_hashes && outs, "prove_rsa_exp_batch: null argument");
+        SG_REQUIRE(records_per_proof >= 1 && records_per_proof <= sg::RSA_MAX_RECORDS,
+                   "prove_rsa_exp_batch: 1 to 4096 records per proof");
+        SG_REQUIRE(n_proofs >= 1 && n_proofs * records_per_proof <= 65535, "prove_rsa_exp_batch: 1 to 65535 records");
+        for (size_t i = 0; i < n_proofs; i++) outs[i].proof = nullptr;
+        std::vector<sg::RsaRecord> recs = rsa_records(n_be, s_be, em_be, n_proofs * records_per_proof);
+        std::vector<std::array<uint8_t, 32>> pis(n_proofs);
+        for (size_t i = 0; i < n_proofs; i++) memcpy(pis[i].data(), public_inputs_hashes + 32 * i, 32);
+        std::vector<sg::RsaProof> r = sg::prove_rsa_batch(c, recs, (uint32_t)records_per_proof, n_trace, pis);
+        for (size_t i = 0; i < n_proofs; i++) {
+            memcpy(outs[i].pi_hash, r[i].pi_hash, 32);
+            memcpy(outs[i].root_f0, r[i].root_f0, 32);
             outs[i].n_trace = r[i].n_trace;
             outs[i].proof = copy_out(r[i].proof, &outs[i].proof_len);
         }
@@ -561,6 +614,23 @@ size_t stark_host_sha512_trace_height(size_t len) { return sg::sha512_trace_heig
 size_t stark_host_build_sha512_trace(const uint8_t *msg, size_t len, uint64_t *trace, uint8_t digest[64]) {
     return sg::build_sha512_trace(msg, len, trace, digest);
 }
+
+int stark_host_build_rsa_exp_trace(const uint8_t *n_be, const uint8_t *s_be, const uint8_t *em_be, size_t n_records,
+                                   size_t n_trace, uint64_t *trace) {
+    try {
+        SG_REQUIRE(trace, "rsa trace: null output");
+        sg::build_rsa_trace_host(rsa_records(n_be, s_be, em_be, n_records), n_trace, trace);
+        return STARK_OK;
+    } catch (const sg::Error &e) {
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        return STARK_E_OOM;
+    } catch (const std::exception &) {
+        return STARK_E_INTERNAL;
+    }
+}
+
+void stark_host_rsa_pkcs1_em_sha256(const uint8_t digest[32], uint8_t em[256]) { sg::rsa_pkcs1_em_sha256(digest, em); }
 
 void stark_host_dns_leaf_hash(const stark_dns_record *r, const uint8_t salt[16], uint8_t out[32]) {
     sg::DnsRec d{r->domain, r->domain_len, r->record_type, r->ttl, r->rdata, r->rdata_len};

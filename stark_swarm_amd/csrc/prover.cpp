@@ -22,6 +22,7 @@
 #include "gl.hpp"
 #include "keccak.hpp"
 #include "ntt.hpp"
+#include "rsa_air.hpp"
 #include "sha256_air.hpp"
 
 namespace sg {
@@ -367,6 +368,63 @@ std::vector<DsKsk> prove_ds_ksk_binding_batch(Ctx *c, const std::vector<ByteSpan
         o.n_trace = sp[k].n_trace;
         o.proof = std::move(sp[k].proof);
         ds_ksk_pi_hash(keys[k], parents[k].data(), o.asserted_digest, o.root_f0, fsbs[k].data(), o.pi_hash);
+    }
+    return out;
+}
+
+std::vector<RsaProof> prove_rsa_batch(Ctx *c, const std::vector<RsaRecord> &records, uint32_t R, uint64_t h,
+                                      const std::vector<std::array<uint8_t, 32>> &pis) {
+    SG_REQUIRE(R >= 1 && R <= RSA_MAX_RECORDS, "prove_rsa: 1 to 4096 records per proof");
+    SG_REQUIRE(!records.empty() && records.size() % R == 0, "prove_rsa: records must fill whole proofs");
+    const size_t K = records.size() / R;
+    SG_REQUIRE(pis.size() == K, "prove_rsa: one public-input hash per proof");
+    SG_REQUIRE(is_pow2(h) && h >= 32 && h * SWARM_BLOWUP <= (1ULL << 24), "prove_rsa: n_trace a power of two >= 32");
+    const uint64_t n0 = h * SWARM_BLOWUP, W = (uint64_t)RSA_W * R;
+    size_t free_b = 0, total_b = 0;
+    SG_HIP(hipMemGetInfo(&free_b, &total_b));
+    const uint64_t budget = (uint64_t)((free_b + c->cached_bytes) * 0.6);
+    const uint64_t per = 8ULL * W * (2 * h + 2 * n0) + 8ULL * 16 * n0;
+    SG_REQUIRE(per <= budget, "prove_rsa: one proof does not fit in device memory");
+    const size_t cap = (size_t)std::min<uint64_t>(std::min<uint64_t>(4096, 65535 / R), budget / per);
+    const size_t nw = (K + cap - 1) / cap, chunk = (K + nw - 1) / nw;
+    std::vector<RsaProof> out(K);
+    for (size_t k0 = 0; k0 < K; k0 += chunk) {
+        const uint32_t G = (uint32_t)std::min(chunk, K - k0);
+        Scoped d_c(c, 8ULL * n0 * G);
+        {
+            Scoped d_lde(c, 8ULL * W * n0 * G);
+            {
+                Scoped d_tr(c, 8ULL * W * h * G);
+                {
+                    HostScope hs(c, "rsa_trace");
+                    const std::vector<RsaRecord> part(records.begin() + k0 * R, records.begin() + (k0 + G) * R);
+                    build_rsa_traces_dev(c, part, R, h, d_tr.as<uint64_t>());
+                }
+                lde_columns(c, d_tr.as<uint64_t>(), d_lde.as<uint64_t>(), ilog2(h), ilog2(SWARM_BLOWUP), W * G, h,
+                            n0, 1);
+            }
+            merge_rsa_batch(c, d_lde.as<uint64_t>(), G, R, h, SWARM_BLOWUP, d_c.as<uint64_t>());
+        }
+        FriParams p;
+        p.schedule = stir_schedule(n0);
+        p.r = p.s0 = SWARM_QUERIES;
+        p.seed_z = SWARM_SEED_Z;
+        p.coeff_commit_final = true;
+        p.d_final = 1;
+        p.stir = true;
+        p.has_pi = true;
+        std::vector<std::array<uint8_t, 32>> gp(pis.begin() + k0, pis.begin() + k0 + G);
+        std::vector<std::vector<uint8_t>> proofs = deep_fri_prove_batch(c, d_c.as<uint64_t>(), G, n0, p, &gp, nullptr);
+        for (uint32_t j = 0; j < G; j++) {
+            RsaProof &o = out[k0 + j];
+            o.proof = std::move(proofs[j]);
+            o.n_trace = h;
+            memcpy(o.root_f0, o.proof.data(), 32);
+            Sha3 s;
+            s.update(gp[j].data(), 32);
+            s.update(o.root_f0, 32);
+            s.final(o.pi_hash);
+        }
     }
     return out;
 }

@@ -7,6 +7,7 @@
 
 #include "ctx.hpp"
 #include "fri.hpp"
+#include "rsa_air.hpp"
 
 namespace sg {
 
@@ -67,6 +68,21 @@ std::vector<ShaProof> prove_sha_batch(Ctx *c, const std::vector<int> &algs, cons
 std::vector<DsKsk> prove_ds_ksk_binding_batch(Ctx *c, const std::vector<ByteSpan> &dnskeys,
                                               const std::vector<std::array<uint8_t, 32>> &parent_ds_hashes,
                                               const std::vector<std::array<uint8_t, 32>> &fs_bindings, int phi_mode);
+// One RSA-2048 exponentiation proof (one record, or R stacked records).
+struct RsaProof {
+    uint8_t pi_hash[32], root_f0[32];
+    size_t n_trace = 0;
+    std::vector<uint8_t> proof;
+};
+// prove_record_rsa (crates/swarm-dns/examples/mixed_algorithm_zone_rollup.rs:
+// 349-395) from the trace on, for records.size() / R proofs of R stacked
+// records each (R = 1: the exponentiation AIR; R > 1: the stacked AIR of
+// rsa2048_stacked_air.rs): trace of n_trace rows -> LDE x32 -> merge
+// (lambda_j = j + 1) -> STIR (54 queries) bound to pis[k]; pi_hash =
+// SHA3-256(pis[k] || root_f0). Lockstep in HBM-sized waves; result k equals
+// the proof of its records alone.
+std::vector<RsaProof> prove_rsa_batch(Ctx *c, const std::vector<RsaRecord> &records, uint32_t R,
+                                      uint64_t n_trace, const std::vector<std::array<uint8_t, 32>> &pis);
 std::vector<uint8_t> prove_outer_rollup(Ctx *c, const uint8_t *pi_hashes, size_t n, const uint8_t pk_hash[32],
                                         int phi_mode, uint8_t root_f0[32]);
 std::vector<uint8_t> prover_core_dev(Ctx *c, const uint64_t *d_trace, uint64_t w, uint64_t n_trace, uint64_t blowup,

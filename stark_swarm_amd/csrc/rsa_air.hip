@@ -1,0 +1,498 @@
+// rsa_air.hip -- RSA-2048 exponentiation AIR for gfx950.
+//
+// Trace (crates/deep_ali/src/rsa2048_exp_air.rs:97-202,
+// rsa2048_field_air.rs:129-162,244-344): each row holds one product
+// c = acc * op2 mod n with its witness quotient q, 26-bit limbs (80 per
+// value), the bits of c and q and the 40-bit biased carries of the
+// limb-by-limb identity a*b - q*n - c = 0. Row 16 multiplies by s, every
+// other row squares, so row 16 outputs s^65537 mod n. The host computes each
+// row's limbs, quotient and carries (big-integer division is sequential
+// work); `expand_rsa_k` turns them into the 11 041 cells per row.
+//
+// Composition (eval_rsa_exp_multirow_per_row, rsa2048_exp_air.rs:210-285,
+// eval_rsa_mul_gadget, rsa2048_field_air.rs:356-432) with lambda_j = j + 1:
+// phi is linear in the slot weights, and field arithmetic is exact, so the
+// 11 280 weighted constraints are summed in regrouped form. The 159
+// position identities weigh sum_{i+j=k} a_i b_j by lambda_k, and lambda is
+// affine in k, so
+//   sum_k lambda_k sum_{i+j=k} a_i b_j = sum_i a_i ((L0 + i) B + B1),
+//   B = sum_j b_j, B1 = sum_j j b_j,  L0 = lambda of position slot 0,
+// and the carry terms collapse to sum_m sc_m (lambda_{m+1} - 2^26 lambda_m).
+// That leaves one square per bit cell (booleanity) as the bulk of the work.
+// Weighted terms with small weights are summed lazily in 128 bits and
+// reduced once.
+//
+// Launch: a workgroup of 8 waves covers 64 consecutive LDE points; the lane
+// is the point (every column read is a coalesced 512-byte run) and the wave
+// is a slice of the limbs (i = slice mod 8) and carry positions; the slices
+// exchange B, B1, N, N1 and their phi parts through LDS. 8 waves rather than
+// 16 because 16-wave workgroups cap a lane at 128 VGPRs, which spills.
+#include <string.h>
+
+#include "air.hpp"
+#include "gl.hpp"
+#include "rsa_air.hpp"
+
+namespace sg {
+
+namespace rsa {
+
+constexpr int NL = 80, LB = 26, CARRY_BITS = 40, CARRY_POS = 158, CROSS = 159, NW = 64;
+constexpr uint64_t CARRY_OFF = 1ULL << 39;
+constexpr uint32_t OUT_ROW = 16;
+// columns of one record
+enum : uint32_t {
+    C_ACC = 0, C_S = 80, C_N = 160, C_PHASE = 240, C_OP2 = 241, C_EM = 321,
+    C_C = 401, C_Q = 481, C_QBITS = 561, C_CBITS = 2641, C_CARRY = 4721
+};
+// constraint slots of one record
+enum : uint32_t {
+    K_PHASE = 0, K_SEL = 1, K_CBOOL = 81, K_CPACK = 2161, K_QBOOL = 2241, K_QPACK = 4321,
+    K_POS = 4401, K_CARRY = 4560, K_EM = 10880, K_TRANS = 10960
+};
+static_assert(C_CARRY + CARRY_BITS * CARRY_POS == RSA_W, "column layout");
+static_assert(K_TRANS + 4 * NL == RSA_K, "slot layout");
+
+// Everything one row's cells are made of.
+struct RowWords {
+    uint32_t acc[NL], op2[NL], c[NL], q[NL];  // 26-bit limbs
+    uint64_t carry[CARRY_POS];                 // carries biased by 2^39 (< 2^40)
+    uint32_t phase, pad;
+};
+struct RecWords {
+    uint32_t s[NL], n[NL], em[NL];
+};
+
+SG_HD uint64_t cell(uint32_t col, const RowWords &w, const RecWords &r) {
+    if (col < C_S) return w.acc[col];
+    if (col < C_N) return r.s[col - C_S];
+    if (col < C_PHASE) return r.n[col - C_N];
+    if (col == C_PHASE) return w.phase;
+    if (col < C_EM) return w.op2[col - C_OP2];
+    if (col < C_C) return r.em[col - C_EM];
+    if (col < C_Q) return w.c[col - C_C];
+    if (col < C_QBITS) return w.q[col - C_Q];
+    if (col < C_CBITS) {
+        const uint32_t j = col - C_QBITS;
+        return (w.q[j / LB] >> (j % LB)) & 1;
+    }
+    if (col < C_CARRY) {
+        const uint32_t j = col - C_CBITS;
+        return (w.c[j / LB] >> (j % LB)) & 1;
+    }
+    const uint32_t j = col - C_CARRY;
+    return (w.carry[j / CARRY_BITS] >> (j % CARRY_BITS)) & 1;
+}
+
+// ---------------------------------------------------------------- host rows
+
+// little-endian 32-bit words of a 256-byte big-endian value
+static void words_from_be(const uint8_t *be, uint32_t *w) {
+    for (int i = 0; i < NW; i++)
+        w[i] = (uint32_t)be[255 - 4 * i] | (uint32_t)be[254 - 4 * i] << 8 | (uint32_t)be[253 - 4 * i] << 16 |
+               (uint32_t)be[252 - 4 * i] << 24;
+}
+
+// 26-bit limbs of a value held in nw words (bits above 2080 are not
+// representable and must be zero, which a < n < 2^2048 guarantees)
+static void to_limbs(const uint32_t *w, int nw, uint32_t *limbs) {
+    for (int i = 0; i < NL; i++) {
+        const int bit = LB * i, k = bit / 32, sh = bit % 32;
+        const uint64_t lo = k < nw ? w[k] : 0, hi = k + 1 < nw ? w[k + 1] : 0;
+        limbs[i] = (uint32_t)(((lo | hi << 32) >> sh) & ((1u << LB) - 1));
+    }
+}
+
+static int top_words(const uint32_t *w, int nw) {
+    while (nw > 0 && w[nw - 1] == 0) nw--;
+    return nw;
+}
+
+// u (ul words) = q * v + r for v of vl significant words (v[vl-1] != 0):
+// long division with 32-bit digits and a normalised divisor (Knuth, TAOCP
+// vol. 2, 4.3.1, algorithm D). q gets ul - vl + 1 words, r gets vl words.
+static void divmod(const uint32_t *u, int ul, const uint32_t *v, int vl, uint32_t *q, uint32_t *r) {
+    if (ul < vl) {
+        memset(q, 0, 4);
+        for (int i = 0; i < vl; i++) r[i] = i < ul ? u[i] : 0;
+        return;
+    }
+    if (vl == 1) {
+        uint64_t rem = 0;
+        for (int j = ul - 1; j >= 0; j--) {
+            const uint64_t cur = rem << 32 | u[j];
+            q[j] = (uint32_t)(cur / v[0]);
+            rem = cur % v[0];
+        }
+        r[0] = (uint32_t)rem;
+        return;
+    }
+    const int s = __builtin_clz(v[vl - 1]);
+    uint32_t vn[NW], un[2 * NW + 1];
+    for (int i = vl - 1; i > 0; i--) vn[i] = (uint32_t)(((uint64_t)v[i] << s) | ((uint64_t)v[i - 1] >> (32 - s)));
+    vn[0] = v[0] << s;
+    un[ul] = (uint32_t)((uint64_t)u[ul - 1] >> (32 - s));
+    for (int i = ul - 1; i > 0; i--) un[i] = (uint32_t)(((uint64_t)u[i] << s) | ((uint64_t)u[i - 1] >> (32 - s)));
+    un[0] = u[0] << s;
+    const uint64_t B = 1ULL << 32;
+    for (int j = ul - vl; j >= 0; j--) {
+        const uint64_t num = (uint64_t)un[j + vl] << 32 | un[j + vl - 1];
+        uint64_t qh = num / vn[vl - 1], rh = num % vn[vl - 1];
+        while (qh >= B || qh * vn[vl - 2] > (rh << 32 | un[j + vl - 2])) {
+            qh--;
+            rh += vn[vl - 1];
+            if (rh >= B) break;
+        }
+        int64_t borrow = 0, t;
+        for (int i = 0; i < vl; i++) {
+            const uint64_t p = qh * vn[i];
+            t = (int64_t)un[i + j] - borrow - (int64_t)(p & 0xFFFFFFFFu);
+            un[i + j] = (uint32_t)t;
+            borrow = (int64_t)(p >> 32) - (t >> 32);
+        }
+        t = (int64_t)un[j + vl] - borrow;
+        un[j + vl] = (uint32_t)t;
+        q[j] = (uint32_t)qh;
+        if (t < 0) {  // qh was one too large: add v back
+            q[j]--;
+            uint64_t carry = 0;
+            for (int i = 0; i < vl; i++) {
+                const uint64_t sum = (uint64_t)un[i + j] + vn[i] + carry;
+                un[i + j] = (uint32_t)sum;
+                carry = sum >> 32;
+            }
+            un[j + vl] += (uint32_t)carry;
+        }
+    }
+    for (int i = 0; i < vl; i++) r[i] = (uint32_t)(((uint64_t)un[i] >> s) | ((uint64_t)un[i + 1] << (32 - s)));
+}
+
+// The words of one product row: c = a*b mod n, q = (a*b - c)/n, carries.
+// Returns false if the limb identity does not close (cannot happen for
+// a, b < n; checked so a bad input never yields an unsatisfiable trace).
+static bool product_row(const uint32_t *a, const uint32_t *b, const uint32_t *n, int nl, RowWords &w,
+                        uint32_t *c_words) {
+    uint32_t prod[2 * NW] = {};
+    for (int i = 0; i < NW; i++) {
+        uint64_t carry = 0;
+        for (int j = 0; j < NW; j++) {
+            const uint64_t t = (uint64_t)a[i] * b[j] + prod[i + j] + carry;
+            prod[i + j] = (uint32_t)t;
+            carry = t >> 32;
+        }
+        prod[i + NW] = (uint32_t)carry;
+    }
+    uint32_t q[2 * NW + 1] = {}, r[NW] = {};
+    const int pl = top_words(prod, 2 * NW);
+    divmod(prod, pl ? pl : 1, n, nl, q, r);
+    for (int i = pl - nl + 1; i < 2 * NW; i++)
+        if (i >= NW && q[i]) return false;
+    memcpy(c_words, r, 4 * nl);
+    for (int i = nl; i < NW; i++) c_words[i] = 0;
+    to_limbs(a, NW, w.acc);
+    to_limbs(b, NW, w.op2);
+    to_limbs(c_words, NW, w.c);
+    to_limbs(q, NW, w.q);
+    uint32_t nlimb[NL];
+    to_limbs(n, NW, nlimb);
+    int64_t prev = 0;
+    for (int k = 0; k < CROSS; k++) {
+        int64_t acc = prev - (k < NL ? (int64_t)w.c[k] : 0);
+        const int lo = k > NL - 1 ? k - (NL - 1) : 0, hi = k < NL - 1 ? k : NL - 1;
+        for (int i = lo; i <= hi; i++)
+            acc += (int64_t)w.acc[i] * w.op2[k - i] - (int64_t)w.q[i] * nlimb[k - i];
+        if (k == CROSS - 1) return acc == 0;
+        if (acc & ((1 << LB) - 1)) return false;
+        const int64_t cy = acc >> LB;  // exact multiple: the shift is the division
+        if (cy >= (int64_t)CARRY_OFF || cy < -(int64_t)CARRY_OFF) return false;
+        w.carry[k] = (uint64_t)(cy + (int64_t)CARRY_OFF);
+        prev = cy;
+    }
+    return true;
+}
+
+// All h rows of one record; 0, or E_INVALID for s >= n or n = 0, or
+// E_INTERNAL if a product row does not close.
+static int record_rows(const RsaRecord &in, uint64_t h, RecWords &rec, RowWords *rows) {
+    uint32_t n[NW], s[NW], em[NW], acc[NW], c[NW];
+    words_from_be(in.n.data(), n);
+    words_from_be(in.s.data(), s);
+    words_from_be(in.em.data(), em);
+    const int nl = top_words(n, NW);
+    if (nl == 0) return E_INVALID;
+    bool s_below_n = false;
+    for (int i = NW - 1; i >= 0; i--)
+        if (s[i] != n[i]) {
+            s_below_n = s[i] < n[i];
+            break;
+        }
+    if (!s_below_n) return E_INVALID;
+    to_limbs(s, NW, rec.s);
+    to_limbs(n, NW, rec.n);
+    to_limbs(em, NW, rec.em);
+    memcpy(acc, s, sizeof acc);
+    for (uint64_t r = 0; r < h; r++) {
+        RowWords &w = rows[r];
+        w.phase = r == OUT_ROW;
+        w.pad = 0;
+        if (!product_row(acc, w.phase ? s : acc, n, nl, w, c)) return E_INTERNAL;
+        memcpy(acc, c, sizeof acc);
+    }
+    return 0;
+}
+
+static void all_rows(const std::vector<RsaRecord> &records, uint64_t h, std::vector<RecWords> &recs,
+                     std::vector<RowWords> &rows) {
+    SG_REQUIRE(is_pow2(h) && h >= 32 && h <= (1u << 20), "rsa trace: height must be a power of two >= 32");
+    const size_t G = records.size();
+    recs.resize(G);
+    rows.resize(G * h);
+    std::vector<int> rc(G, 0);
+    parallel_for(G, [&](size_t g) { rc[g] = record_rows(records[g], h, recs[g], rows.data() + g * h); });
+    for (size_t g = 0; g < G; g++) {
+        SG_REQUIRE(rc[g] != E_INVALID, "rsa trace: need 0 < n and s < n");
+        if (rc[g]) throw Error(E_INTERNAL, "rsa trace: product identity does not close");
+    }
+}
+
+// ---------------------------------------------------------------- device
+
+// one cell per lane, rows fastest: tr_g[col * h + row], g = blockIdx.y
+__global__ __launch_bounds__(256) void expand_rsa_k(const RowWords *rows, const RecWords *recs, uint64_t h,
+                                                    uint64_t *tr) {
+    const uint64_t g = blockIdx.y;
+    const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= h * RSA_W) return;
+    const uint64_t row = t & (h - 1), col = t / h;
+    tr[g * RSA_W * h + t] = cell((uint32_t)col, rows[g * h + row], recs[g]);
+}
+
+// 128-bit lazy sum of field terms
+struct Acc {
+    uint64_t lo, hi;
+};
+__device__ __forceinline__ void acc_add(Acc &a, uint64_t x) {
+    a.lo += x;
+    a.hi += a.lo < x;
+}
+// a += x * k, k < 2^32
+__device__ __forceinline__ void acc_mad(Acc &a, uint64_t x, uint32_t k) {
+    const uint64_t p0 = (uint64_t)(uint32_t)x * k, p1 = (x >> 32) * (uint64_t)k;
+    const uint64_t lo = p0 + (p1 << 32);
+    const uint64_t hi = (p1 >> 32) + (lo < p0);
+    a.lo += lo;
+    a.hi += hi + (a.lo < lo);
+}
+// a += x << b, b < 64 (compile-time in the unrolled loops)
+__device__ __forceinline__ void acc_shl(Acc &a, uint64_t x, int b) {
+    const uint64_t lo = x << b, hi = b ? x >> (64 - b) : 0;
+    a.lo += lo;
+    a.hi += hi + (a.lo < lo);
+}
+__device__ __forceinline__ uint64_t acc_val(const Acc &a) { return reduce128(a.lo, a.hi); }
+__device__ __forceinline__ uint64_t mul_small(uint64_t x, uint32_t k) {
+    Acc a{0, 0};
+    acc_mad(a, x, k);
+    return acc_val(a);
+}
+
+constexpr int SL = 8;  // slices (waves) per workgroup
+constexpr int PT = 64;  // points per workgroup
+
+// phi of K proofs (grid.y) of R stacked records at n0 points
+__global__ __launch_bounds__(SL * PT) void compose_rsa_k(const uint64_t *lde, uint64_t n0, uint64_t blowup,
+                                                        uint32_t R, uint64_t *phi) {
+    __shared__ uint64_t red[SL][4][PT];
+    const uint32_t lane = threadIdx.x & (PT - 1), s = threadIdx.x / PT;
+    const uint64_t pt = (uint64_t)blockIdx.x * PT + lane, np = (pt + blowup) & (n0 - 1);
+    const uint64_t n_trace = n0 / blowup, trow = pt / blowup;
+    const bool out_row = trow == OUT_ROW, trans = trow + 1 < n_trace;
+    uint64_t phi_lane = 0;
+    for (uint32_t rec = 0; rec < R; rec++) {
+        const uint64_t *col = lde + ((uint64_t)blockIdx.y * R + rec) * RSA_W * n0;
+        const uint32_t O = rec * RSA_K + 1;  // lambda(slot) = O + slot
+#define AT(c) col[(uint64_t)(c) * n0 + pt]
+#define NX(c) col[(uint64_t)(c) * n0 + np]
+        // B, B1 (op2) and N, N1 (modulus) over every limb
+        {
+            uint64_t b0 = 0, b1 = 0, m0 = 0, m1 = 0;
+#pragma unroll
+            for (int u = 0; u < NL / SL; u++) {
+                const uint32_t i = s + SL * u;
+                const uint64_t op = AT(C_OP2 + i), nn = AT(C_N + i);
+                b0 = add(b0, op);
+                b1 = add(b1, mul_small(op, i));
+                m0 = add(m0, nn);
+                m1 = add(m1, mul_small(nn, i));
+            }
+            red[s][0][lane] = b0;
+            red[s][1][lane] = b1;
+            red[s][2][lane] = m0;
+            red[s][3][lane] = m1;
+        }
+        __syncthreads();
+        uint64_t B = 0, B1 = 0, N = 0, N1 = 0;
+#pragma unroll
+        for (int t = 0; t < SL; t++) {
+            B = add(B, red[t][0][lane]);
+            B1 = add(B1, red[t][1][lane]);
+            N = add(N, red[t][2][lane]);
+            N1 = add(N1, red[t][3][lane]);
+        }
+        __syncthreads();  // red is written again for the next record
+        Acc pos{0, 0}, neg{0, 0}, sel{0, 0};
+        const uint32_t L0 = O + K_POS;
+#pragma unroll 1
+        for (int u = 0; u < NL / SL; u++) {
+            const uint32_t i = s + SL * u;
+            const uint64_t a = AT(C_ACC + i), op = AT(C_OP2 + i), sv = AT(C_S + i);
+            const uint64_t cc = AT(C_C + i), qq = AT(C_Q + i);
+            // select: op - (ph (s - a) + a); the ph part is applied once below
+            acc_mad(pos, op, O + K_SEL + i);
+            acc_mad(neg, a, O + K_SEL + i);
+            acc_mad(sel, sub(sv, a), O + K_SEL + i);
+            // position identities, regrouped by the a and q limbs
+            acc_add(pos, mul(a, add(mul_small(B, L0 + i), B1)));
+            acc_add(neg, mul(qq, add(mul_small(N, L0 + i), N1)));
+            acc_mad(neg, cc, L0 + i);
+            // range checks of c and q: booleanity and limb packing
+            Acc pk{0, 0};
+#pragma unroll
+            for (int b = 0; b < LB; b++) {
+                const uint64_t x = AT(C_CBITS + LB * i + b);
+                acc_mad(pos, x, O + K_CBOOL + LB * i + b);
+                acc_mad(neg, mul(x, x), O + K_CBOOL + LB * i + b);
+                acc_shl(pk, x, b);
+            }
+            acc_mad(pos, cc, O + K_CPACK + i);
+            acc_mad(neg, acc_val(pk), O + K_CPACK + i);
+            pk = Acc{0, 0};
+#pragma unroll
+            for (int b = 0; b < LB; b++) {
+                const uint64_t x = AT(C_QBITS + LB * i + b);
+                acc_mad(pos, x, O + K_QBOOL + LB * i + b);
+                acc_mad(neg, mul(x, x), O + K_QBOOL + LB * i + b);
+                acc_shl(pk, x, b);
+            }
+            acc_mad(pos, qq, O + K_QPACK + i);
+            acc_mad(neg, acc_val(pk), O + K_QPACK + i);
+            if (out_row) {
+                acc_mad(pos, cc, O + K_EM + i);
+                acc_mad(neg, AT(C_EM + i), O + K_EM + i);
+            }
+        }
+        // transitions into the next row (a loop of its own keeps the bit
+        // loops' registers free)
+        if (trans) {
+#pragma unroll 1
+            for (int u = 0; u < NL / SL; u++) {
+                const uint32_t i = s + SL * u;
+                acc_mad(pos, NX(C_ACC + i), O + K_TRANS + i);
+                acc_mad(neg, AT(C_C + i), O + K_TRANS + i);
+                acc_mad(pos, NX(C_S + i), O + K_TRANS + NL + i);
+                acc_mad(neg, AT(C_S + i), O + K_TRANS + NL + i);
+                acc_mad(pos, NX(C_N + i), O + K_TRANS + 2 * NL + i);
+                acc_mad(neg, AT(C_N + i), O + K_TRANS + 2 * NL + i);
+                acc_mad(pos, NX(C_EM + i), O + K_TRANS + 3 * NL + i);
+                acc_mad(neg, AT(C_EM + i), O + K_TRANS + 3 * NL + i);
+            }
+        }
+        // carries: booleanity, and sc_m (lambda_{m+1} - 2^26 lambda_m) from the
+        // position identities
+#pragma unroll 1
+        for (uint32_t m = s; m < CARRY_POS; m += SL) {
+            Acc sc{0, 0};
+#pragma unroll
+            for (int b = 0; b < CARRY_BITS; b++) {
+                const uint64_t y = AT(C_CARRY + CARRY_BITS * m + b);
+                acc_mad(pos, y, O + K_CARRY + CARRY_BITS * m + b);
+                acc_mad(neg, mul(y, y), O + K_CARRY + CARRY_BITS * m + b);
+                acc_shl(sc, y, b);
+            }
+            const uint64_t gamma = sub(L0 + m + 1, (uint64_t)(L0 + m) << LB);
+            acc_add(pos, mul(sub(acc_val(sc), CARRY_OFF), gamma));
+        }
+        const uint64_t ph = AT(C_PHASE);
+        if (s == 0) {
+            acc_mad(pos, ph, O + K_PHASE);
+            acc_mad(neg, mul(ph, ph), O + K_PHASE);
+        }
+#undef AT
+#undef NX
+        phi_lane = add(phi_lane, sub(sub(acc_val(pos), acc_val(neg)), mul(ph, acc_val(sel))));
+    }
+    red[s][0][lane] = phi_lane;
+    __syncthreads();
+    if (s == 0) {
+        uint64_t v = 0;
+#pragma unroll
+        for (int t = 0; t < SL; t++) v = add(v, red[t][0][lane]);
+        phi[(uint64_t)blockIdx.y * n0 + pt] = v;
+    }
+}
+
+}  // namespace rsa
+
+void build_rsa_trace_host(const std::vector<RsaRecord> &records, uint64_t h, uint64_t *tr) {
+    std::vector<rsa::RecWords> recs;
+    std::vector<rsa::RowWords> rows;
+    rsa::all_rows(records, h, recs, rows);
+    const size_t G = records.size();
+    parallel_for(G, [&](size_t g) {
+        uint64_t *t = tr + g * RSA_W * h;
+        for (uint32_t col = 0; col < RSA_W; col++)
+            for (uint64_t r = 0; r < h; r++) t[col * h + r] = rsa::cell(col, rows[g * h + r], recs[g]);
+    });
+}
+
+void build_rsa_traces_dev(Ctx *c, const std::vector<RsaRecord> &records, uint32_t R, uint64_t h, uint64_t *d_tr) {
+    const size_t G = records.size();
+    SG_REQUIRE(G >= 1 && R >= 1 && G % R == 0 && G <= 65535, "rsa traces: records must fill whole proofs");
+    std::vector<rsa::RecWords> recs;
+    std::vector<rsa::RowWords> rows;
+    rsa::all_rows(records, h, recs, rows);
+    Scoped d_rows(c, sizeof(rsa::RowWords) * rows.size()), d_recs(c, sizeof(rsa::RecWords) * G);
+    SG_HIP(hipMemcpyAsync(d_rows.p, rows.data(), sizeof(rsa::RowWords) * rows.size(), hipMemcpyHostToDevice,
+                          c->stream));
+    SG_HIP(hipMemcpyAsync(d_recs.p, recs.data(), sizeof(rsa::RecWords) * G, hipMemcpyHostToDevice, c->stream));
+    {
+        ProfScope ps(c, "expand_rsa", 8.0 * RSA_W * h * G);
+        hipLaunchKernelGGL(rsa::expand_rsa_k, dim3((uint32_t)((h * RSA_W + 255) / 256), (uint32_t)G), dim3(256), 0,
+                           c->stream, d_rows.as<rsa::RowWords>(), d_recs.as<rsa::RecWords>(), h, d_tr);
+        SG_HIP(hipGetLastError());
+    }
+    SG_HIP(hipStreamSynchronize(c->stream));  // the host word buffers go out of scope
+}
+
+void merge_rsa_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint32_t R, uint64_t n_trace, uint64_t blowup,
+                     uint64_t *c_eval) {
+    SG_REQUIRE(is_pow2(n_trace) && n_trace >= 32 && is_pow2(blowup) && blowup >= 2,
+               "merge_rsa: n_trace must be a power of two >= 32 and blowup >= 2");
+    SG_REQUIRE(K >= 1 && K <= 65535, "merge_rsa: 1 to 65535 proofs");
+    SG_REQUIRE(R >= 1 && R <= RSA_MAX_RECORDS, "merge_rsa: 1 to 4096 records per proof");
+    const uint64_t n = n_trace * blowup;
+    SG_REQUIRE(n <= (1ULL << 24), "merge_rsa: domain above 2^24");
+    Scoped phi(c, 8 * n * K);
+    {
+        ProfScope ps(c, "compose_rsa", (8.0 * RSA_W * n * R + 8.0 * n) * K);
+        hipLaunchKernelGGL(rsa::compose_rsa_k, dim3((uint32_t)(n / rsa::PT), K), dim3(rsa::SL * rsa::PT), 0,
+                           c->stream, lde, n, blowup, R, phi.as<uint64_t>());
+        SG_HIP(hipGetLastError());
+    }
+    quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
+}
+
+void rsa_pkcs1_em_sha256(const uint8_t digest[32], uint8_t em[256]) {
+    // RFC 8017 9.2: 00 01 PS(FF..) 00 || DigestInfo(SHA-256) || H
+    static const uint8_t info[19] = {0x30, 0x31, 0x30, 0x0D, 0x06, 0x09, 0x60, 0x86, 0x48, 0x01,
+                                     0x65, 0x03, 0x04, 0x02, 0x01, 0x05, 0x00, 0x04, 0x20};
+    const size_t ps = 256 - 3 - 19 - 32;
+    em[0] = 0x00;
+    em[1] = 0x01;
+    memset(em + 2, 0xFF, ps);
+    em[2 + ps] = 0x00;
+    memcpy(em + 3 + ps, info, 19);
+    memcpy(em + 256 - 32, digest, 32);
+}
+
+}  // namespace sg

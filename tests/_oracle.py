@@ -97,6 +97,15 @@ def lib():
         L.ref_build_sha512_trace.argtypes = [ctypes.c_char_p, sz, u64p, ctypes.c_char_p]
         L.ref_sha512_constraints.argtypes = [u64p, u64p, sz, sz, u64p]
         L.ref_merge_sha512.argtypes = [u64p, sz, sz, sz, ctypes.c_int, u64p]
+        L.ref_rsa_exp_width.restype = sz
+        L.ref_rsa_exp_constraints_per_record.restype = sz
+        L.ref_fill_rsa_exp.restype = ctypes.c_int
+        L.ref_fill_rsa_exp.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz, sz, u64p]
+        L.ref_rsa_exp_constraints.argtypes = [u64p, u64p, sz, sz, u64p]
+        L.ref_merge_rsa.argtypes = [u64p, sz, sz, sz, u64p, u64p]
+        L.ref_rsa_pkcs1_em_sha256.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+        L.ref_prove_rsa_exp.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                        ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.c_char_p, ctypes.c_char_p]
         _lib = L
     return _lib
 
@@ -253,3 +262,56 @@ def merge_sha512(lde, n_trace, blowup, n_blocks, phi_mode=1):
     out = np.zeros(n_trace * blowup, dtype=np.uint64)
     lib().ref_merge_sha512(ptr(lde), n_trace, blowup, n_blocks, phi_mode, ptr(out))
     return out
+
+
+RSA_W, RSA_K = 11041, 11280
+
+
+def _be256(x):
+    return int(x).to_bytes(256, "big")
+
+
+def rsa_exp_trace(records, n_trace=32):
+    """(w x n_trace) trace of fill_rsa_stacked / fill_rsa_exp_multirow for
+    records [(n, s, em), ...] given as Python ints; raises on a bad input."""
+    tr = np.zeros((RSA_W * len(records), n_trace), dtype=np.uint64)
+    for r, (n, s, em) in enumerate(records):
+        rc = lib().ref_fill_rsa_exp(_be256(n), _be256(s), _be256(em), n_trace, RSA_W * r, ptr(tr))
+        if rc:
+            raise ValueError("ref_fill_rsa_exp: %d" % rc)
+    return tr
+
+
+def rsa_exp_constraints(cur, nxt, row, n_trace):
+    cur = np.ascontiguousarray(cur, dtype=np.uint64)
+    nxt = np.ascontiguousarray(nxt, dtype=np.uint64)
+    out = np.zeros(RSA_K, dtype=np.uint64)
+    lib().ref_rsa_exp_constraints(ptr(cur), ptr(nxt), row, n_trace, ptr(out))
+    return out
+
+
+def merge_rsa(lde, n_trace, blowup, n_records, coeffs=None):
+    lde = np.ascontiguousarray(lde, dtype=np.uint64)
+    out = np.zeros(n_trace * blowup, dtype=np.uint64)
+    cp = None
+    if coeffs is not None:
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.uint64)
+        cp = ptr(coeffs)
+    lib().ref_merge_rsa(ptr(lde), n_trace, blowup, n_records, cp, ptr(out))
+    return out
+
+
+def rsa_pkcs1_em_sha256(digest):
+    em = ctypes.create_string_buffer(256)
+    lib().ref_rsa_pkcs1_em_sha256(bytes(digest), em)
+    return em.raw
+
+
+def prove_rsa_exp(n, s, em, pis):
+    """(proof, root_f0, pi_hash) of the per-record RSA-2048 prover"""
+    pp, pl = u8p(), sz()
+    rf, pi = ctypes.create_string_buffer(32), ctypes.create_string_buffer(32)
+    rc = lib().ref_prove_rsa_exp(_be256(n), _be256(s), _be256(em), bytes(pis), ctypes.byref(pp), ctypes.byref(pl),
+                                 rf, pi)
+    assert rc == 0, rc
+    return take_bytes(pp, pl), rf.raw, pi.raw

@@ -427,3 +427,81 @@ def test_server_results_equal_standalone(S):
             alone = S.prove_ds_ksk_binding(x, hashlib.sha256(x).digest(), fsb)
             assert r.aux == alone.asserted_digest
         assert r.proof_blob == alone.proof_blob and r.pi_hash == alone.pi_hash
+
+
+# ---- RSA-2048 exponentiation AIR ----
+
+def _rsa_rec(seed, bits=2047):
+    import random
+    rng = random.Random(seed)
+    n = (rng.getrandbits(bits - 1) << 1) | 1 | (1 << (bits - 1))
+    s = rng.randrange(n)
+    return n, s, pow(s, 65537, n)
+
+
+def _oracle_rsa_proof(recs, pis, n_trace=32):
+    """trace of the stacked records -> LDE x32 -> merge -> STIR (54 queries)
+    bound to pis, with oracle pieces"""
+    tr = O.rsa_exp_trace(recs, n_trace)
+    lde = O.lde_columns(tr, 32)
+    c = O.merge_rsa(lde, n_trace, 32, len(recs))
+    lg = (32 * n_trace).bit_length() - 1
+    sched = [8] * (lg // 3) + ([1 << (lg % 3)] if lg % 3 else [])
+    return O.deep_fri_prove(c, O.fri_params(sched, r=54, s0=54, pi_hash=pis))
+
+
+@pytest.mark.parametrize("nrec", [1, 2])
+def test_merge_rsa_matches_oracle(S, nrec):
+    say("merge_rsa records %d" % nrec)
+    recs = [_rsa_rec(0x700 + i, bits=2047 + i) for i in range(nrec)]
+    tr = O.rsa_exp_trace(recs)
+    lde = O.lde_columns(tr, 32)
+    want = O.merge_rsa(lde, 32, 32, nrec)
+    got, info = S.deep_ali_merge_rsa_stacked_streaming(lde, list(range(1, 11280 * nrec + 1)), nrec, 0, 32, 32)
+    assert np.array_equal(got, want)
+    # a trace that is not valid (one carry bit planted) composes the same way
+    tr[4721 + 40 * 7 + 2, 5] = 2
+    lde = O.lde_columns(tr, 32)
+    got, _ = S.deep_ali_merge_rsa_stacked_streaming(lde, list(range(1, 11280 * nrec + 1)), nrec, 0, 32, 32)
+    assert np.array_equal(got, O.merge_rsa(lde, 32, 32, nrec))
+
+
+def test_prove_rsa_matches_oracle(S):
+    """single-record proofs (the per-record prover of the mixed-algorithm zone
+    example) and a two-record stacked proof, byte for byte against the oracle;
+    a three-proof batch against the single calls"""
+    say("prove_rsa")
+    recs = [_rsa_rec(0x800 + i) for i in range(3)]
+    pis = [hashlib.sha3_256(bytes([i])).digest() for i in range(3)]
+    outs = S.prove_rsa_exp_batch([S.RsaStackedRecord(*r) for r in recs], pis)
+    for r, p, o in zip(recs, pis, outs):
+        proof, rf, pi = O.prove_rsa_exp(*r, p)
+        assert o.proof_blob == proof and o.root_f0 == rf and o.pi_hash == pi and o.n_trace == 32
+    assert outs[0].proof_blob == _oracle_rsa_proof(recs[:1], pis[0])
+    one = S.prove_rsa_exp_batch([S.RsaStackedRecord(*recs[1])], [pis[1]])[0]
+    assert one.proof_blob == outs[1].proof_blob
+    say("prove_rsa stacked")
+    st = S.prove_rsa_exp_batch([S.RsaStackedRecord(*r) for r in recs[:2]], [pis[2]], records_per_proof=2)[0]
+    assert st.proof_blob == _oracle_rsa_proof(recs[:2], pis[2])
+    assert st.pi_hash == hashlib.sha3_256(pis[2] + st.root_f0).digest()
+
+
+def test_prove_rsa_real_signature(S):
+    """an RSASSA-PKCS1-v1_5 SHA-256 signature from a fresh 2048-bit key, bound
+    with the example's public-input recipe, at a taller trace (64 rows)"""
+    from tests.test_rsa_air import rsa_key
+    say("prove_rsa real signature")
+    n, d = rsa_key(0xA1)
+    msg = b"DNSSEC-RRSIG-V0|legacy.example.com.|A|10.0.0.3|epoch-0|alg-8"
+    em_b = S.emsa_pkcs1_v1_5_encode_sha256(hashlib.sha256(msg).digest())
+    em = int.from_bytes(em_b, "big")
+    s = pow(em, d, n)
+    pk = S.rsa_record_pk_hash(n.to_bytes(256, "big"), s.to_bytes(256, "big"), msg, em_b)
+    o = S.prove_rsa_exp_batch([S.RsaStackedRecord(n, s, em)], [pk], n_trace=64)[0]
+    assert o.n_trace == 64 and o.proof_blob == _oracle_rsa_proof([(n, s, em)], pk, n_trace=64)
+
+
+def test_prove_rsa_rejects_bad_record(S):
+    n, s, em = _rsa_rec(5)
+    with pytest.raises(S.StarkError):
+        S.prove_rsa_exp_batch([S.RsaStackedRecord(n, n + 1, em)], [bytes(32)])
