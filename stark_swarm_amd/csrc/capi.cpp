@@ -80,7 +80,7 @@ void h2d(sg::Ctx *c, void *d, const void *h, size_t bytes) {
     SG_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream));
 }
 void d2h(sg::Ctx *c, void *h, const void *d, size_t bytes) {
-    SG_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, c->stream));
+    c->d2h(h, d, bytes);
     SG_HIP(hipStreamSynchronize(c->stream));
 }
 

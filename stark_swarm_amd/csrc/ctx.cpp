@@ -1,6 +1,12 @@
 // ctx.cpp -- context creation, root tables, caching allocator, event timing.
 #include "ctx.hpp"
 
+#include <string.h>
+
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+
 #include "gl.hpp"
 
 namespace sg {
@@ -38,6 +44,26 @@ void *Ctx::alloc(size_t bytes) {
     }
     live[p] = bytes;
     return p;
+}
+
+void Ctx::d2h(void *dst, const void *src, size_t n) {
+    constexpr size_t CHUNK = 64ULL << 20;
+    if (n == 0) return;
+    const size_t want = n < CHUNK ? n : CHUNK;
+    if (pin_bytes < want) {
+        if (pin) (void)hipHostFree(pin);
+        pin = nullptr;
+        pin_bytes = 0;
+        const size_t sz = want < (1ULL << 20) ? (1ULL << 20) : want;
+        SG_HIP(hipHostMalloc(&pin, sz, hipHostMallocDefault));
+        pin_bytes = sz;
+    }
+    for (size_t off = 0; off < n; off += CHUNK) {
+        const size_t m = n - off < CHUNK ? n - off : CHUNK;
+        SG_HIP(hipMemcpyAsync(pin, (const uint8_t *)src + off, m, hipMemcpyDeviceToHost, stream));
+        SG_HIP(hipStreamSynchronize(stream));
+        memcpy((uint8_t *)dst + off, pin, m);
+    }
 }
 
 void Ctx::release(void *p) {
@@ -123,6 +149,96 @@ Ctx *ctx_create(int device) {
     return c;
 }
 
+namespace {
+
+// Worker pool behind parallel_for. A job is a loop over [0, n) that the
+// caller and any idle workers share through an atomic index; the caller
+// waits until every index has run and no worker still holds the job.
+struct PoolJob {
+    void (*call)(void *, size_t);
+    void *arg;
+    size_t n;
+    std::atomic<size_t> next{0}, done{0};
+    int users = 0;  // workers inside the job (guarded by Pool::mu)
+};
+
+struct Pool {
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<PoolJob *> jobs;
+    std::vector<std::thread> threads;
+    bool stop = false;
+
+    Pool() {
+        size_t nt = std::thread::hardware_concurrency();
+        if (nt == 0) nt = 4;
+        if (nt > 16) nt = 16;
+        for (size_t t = 1; t < nt; t++) threads.emplace_back([this] { work(); });
+    }
+    ~Pool() {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            stop = true;
+        }
+        cv_work.notify_all();
+        for (auto &t : threads) t.join();
+    }
+    static void run_indices(PoolJob *j) {
+        for (size_t i; (i = j->next.fetch_add(1)) < j->n;) {
+            j->call(j->arg, i);
+            j->done.fetch_add(1);
+        }
+    }
+    void retire(PoolJob *j) {  // with mu held: no new worker picks j up
+        for (auto it = jobs.begin(); it != jobs.end(); ++it)
+            if (*it == j) {
+                jobs.erase(it);
+                break;
+            }
+    }
+    void work() {
+        std::unique_lock<std::mutex> lk(mu);
+        while (true) {
+            cv_work.wait(lk, [&] { return stop || !jobs.empty(); });
+            if (stop) return;
+            PoolJob *j = jobs.front();
+            j->users++;
+            lk.unlock();
+            run_indices(j);
+            lk.lock();
+            j->users--;
+            retire(j);
+            cv_done.notify_all();
+        }
+    }
+    void run(PoolJob &j) {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            jobs.push_back(&j);
+        }
+        cv_work.notify_all();
+        run_indices(&j);
+        std::unique_lock<std::mutex> lk(mu);
+        retire(&j);
+        cv_done.wait(lk, [&] { return j.done.load() == j.n && j.users == 0; });
+    }
+};
+
+Pool &pool() {
+    static Pool p;
+    return p;
+}
+
+}  // namespace
+
+void parallel_run(size_t n, void (*call)(void *, size_t), void *arg) {
+    PoolJob j;
+    j.call = call;
+    j.arg = arg;
+    j.n = n;
+    pool().run(j);
+}
+
 void ctx_destroy(Ctx *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -132,6 +248,7 @@ void ctx_destroy(Ctx *c) {
     }
     for (auto &kv : c->live) (void)hipFree(kv.first);
     for (auto &kv : c->free_blocks) (void)hipFree(kv.second);
+    if (c->pin) (void)hipHostFree(c->pin);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -80,6 +80,14 @@ struct Ctx {
     std::vector<Pending> pending;
     std::map<std::string, KernelStat> stats;
 
+    // pinned staging for device-to-host reads: a copy into pageable memory
+    // has cost milliseconds for tens of KB, one into pinned memory does not
+    void *pin = nullptr;
+    size_t pin_bytes = 0;
+    // copy n bytes of device memory to host memory after the stream's
+    // earlier work; returns once dst holds them
+    void d2h(void *dst, const void *src, size_t n);
+
     void *alloc(size_t bytes);
     void release(void *p);
     void trim();
@@ -136,25 +144,17 @@ struct HostScope {
 Ctx *ctx_create(int device);
 void ctx_destroy(Ctx *c);
 
-// Run fn(i) for i < n on up to 16 host threads (the per-proof host work of a
-// batch: transcripts, record hashing, serialization).
+// Run fn(i) for i < n on a process-wide pool of up to 16 host threads (the
+// per-proof host work of a batch: transcripts, record hashing,
+// serialization); the caller works on its own loop too. fn must not throw.
+void parallel_run(size_t n, void (*call)(void *, size_t), void *arg);
 template <class F>
 void parallel_for(size_t n, F fn) {
-    size_t nt = std::thread::hardware_concurrency();
-    if (nt == 0) nt = 4;
-    if (nt > 16) nt = 16;
-    if (nt > n) nt = n;
-    if (nt <= 1) {
+    if (n <= 1) {
         for (size_t i = 0; i < n; i++) fn(i);
         return;
     }
-    std::atomic<size_t> next(0);
-    std::vector<std::thread> th;
-    for (size_t t = 0; t < nt; t++)
-        th.emplace_back([&]() {
-            for (size_t i; (i = next.fetch_add(1)) < n;) fn(i);
-        });
-    for (auto &x : th) x.join();
+    parallel_run(n, [](void *a, size_t i) { (*static_cast<F *>(a))(i); }, &fn);
 }
 
 inline uint32_t ilog2(uint64_t n) {

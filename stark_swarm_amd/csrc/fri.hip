@@ -333,7 +333,7 @@ void launch_ood(Ctx *c, const uint64_t *coef, uint64_t n, int ncomp, uint32_t K,
     hipLaunchKernelGGL(ood_finish_k, dim3(K), dim3(FIN_T), 0, c->stream, d_part.as<uint64_t>(), nblocks, (uint32_t)M,
                        zeta, d_ev.as<uint64_t>());
     SG_HIP(hipGetLastError());
-    SG_HIP(hipMemcpyAsync(evals_host, d_ev.p, 48ULL * M * K, hipMemcpyDeviceToHost, c->stream));
+    c->d2h(evals_host, d_ev.p, 48ULL * M * K);
     SG_HIP(hipStreamSynchronize(c->stream));
 }
 
@@ -422,6 +422,7 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
     const uint64_t m0 = p.schedule[0], nn0 = n0 / m0;
     DevTree t0;
     {
+        HostScope hs(c, "fri_f0_trees");
         const uint32_t a = pick_arity(nn0, 16);
         LeafSrc src{d_f0, 1, nn0, (uint32_t)m0, (uint32_t)m0, nullptr, n0};
         tree_build(c, t0, src, nn0, a, merkle_depth(nn0, a), 0xFF, K);
@@ -466,6 +467,7 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
         // OOD coset evaluations
         coset_evals[ell].resize((size_t)K * m * 6);
         {
+            HostScope hs(c, "fri_ood");
             Scoped coef(c, 8ULL * K * ncomp * cur), scr(c, 8ULL * K * ncomp * cur);
             ntt_columns(c, f, coef.as<uint64_t>(), scr.as<uint64_t>(), ilog2(cur), (uint64_t)K * ncomp, cur, cur, true,
                         nullptr);
@@ -478,6 +480,7 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
         // layer (f0) hashes components 1..5 as zeros
         DevTree tl;
         {
+            HostScope hs(c, "fri_layer_tree");
             const uint32_t a = pick_arity(cur, m);
             LeafSrc src{f, 1, cur, 6, (uint32_t)ncomp, nullptr, (uint64_t)ncomp * cur};
             tree_build(c, tl, src, cur, a, merkle_depth(cur, a), ell, K);
@@ -504,8 +507,7 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
             LeafSrc src{tup.as<uint64_t>(), 1, nn, (uint32_t)(6 * m), (uint32_t)(6 * m), nullptr, 6 * m * nn};
             tree_build(c, tc, src, nn, a, merkle_depth(nn, a), 0xFE, K);
             tuples_planes.resize((size_t)6 * m * nn * K);
-            SG_HIP(hipMemcpyAsync(tuples_planes.data(), tup.p, 8 * tuples_planes.size(), hipMemcpyDeviceToHost,
-                                  c->stream));
+            c->d2h(tuples_planes.data(), tup.p, 8 * tuples_planes.size());
             SG_HIP(hipStreamSynchronize(c->stream));
             n_final_tuples = nn;
             m_final = m;
@@ -519,6 +521,7 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
             tree_free(c, tc);
         }
         // fold with this layer's alpha (the final layer's is drawn after its coefficient commitment)
+        HostScope hfold(c, "fri_fold");
         upload_powers(c, alpha, m, d_alpha.as<uint64_t>());
         fold(c, f, cur, ncomp, m, K, d_alpha.as<uint64_t>(), next, nullptr);
         if (ell == 0) {
@@ -542,7 +545,7 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
             ntt_columns(c, f, coef.as<uint64_t>(), scr.as<uint64_t>(), ilog2(nf), 6ULL * K, nf, nf, true, nullptr);
             src = coef.as<uint64_t>();
         }
-        SG_HIP(hipMemcpyAsync(fin.data(), src, 8 * fin.size(), hipMemcpyDeviceToHost, c->stream));
+        c->d2h(fin.data(), src, 8 * fin.size());
         SG_HIP(hipStreamSynchronize(c->stream));
     }
     const uint64_t dfin = p.d_final < nf ? p.d_final : nf;
@@ -575,11 +578,12 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
                 for (int q6 = 0; q6 < 6; q6++) gidx.push_back(k * 6 * n1 + q6 * n1 + base[k][q]);
     std::vector<uint64_t> gv(gidx.size());
     if (!gidx.empty()) {
+        HostScope hs(c, "fri_gather");
         Scoped di(c, 8 * gidx.size()), dv(c, 8 * gidx.size());
         SG_HIP(hipMemcpyAsync(di.p, gidx.data(), 8 * gidx.size(), hipMemcpyHostToDevice, c->stream));
         gather_u64(c, d_f0, di.as<uint64_t>(), nfib, dv.as<uint64_t>());
         if (l2) gather_u64(c, layer1, di.as<uint64_t>() + nfib, gidx.size() - nfib, dv.as<uint64_t>() + nfib);
-        SG_HIP(hipMemcpyAsync(gv.data(), dv.p, 8 * gidx.size(), hipMemcpyDeviceToHost, c->stream));
+        c->d2h(gv.data(), dv.p, 8 * gidx.size());
         SG_HIP(hipStreamSynchronize(c->stream));
     }
     std::vector<std::vector<std::vector<uint8_t>>> op0, op1;

@@ -551,7 +551,7 @@ void dns_commit_batch(Ctx *c, const RecordSource &src, const uint8_t salt[16],
     }
     leaf_hashes.assign(K, {});
     std::vector<uint8_t> all(32 * nrec);
-    SG_HIP(hipMemcpyAsync(all.data(), d_leaf.p, 32 * nrec, hipMemcpyDeviceToHost, c->stream));
+    c->d2h(all.data(), d_leaf.p, 32 * nrec);
     // trees: shards of equal size go up level by level in one launch per level
     roots.assign(K, {});
     std::map<size_t, std::vector<size_t>> by_size;
@@ -576,7 +576,7 @@ void dns_commit_batch(Ctx *c, const RecordSource &src, const uint8_t salt[16],
             len = np;
         }
         std::vector<uint64_t> rt(4 * n * G);
-        SG_HIP(hipMemcpyAsync(rt.data(), cur, 32 * n * G, hipMemcpyDeviceToHost, c->stream));
+        c->d2h(rt.data(), cur, 32 * n * G);
         SG_HIP(hipStreamSynchronize(c->stream));
         for (size_t j = 0; j < G; j++) memcpy(roots[g.second[j]].data(), &rt[4 * n * j], 32);
     }
@@ -642,7 +642,7 @@ void tree_build(Ctx *c, DevTree &t, const LeafSrc &src, uint64_t n_leaves, uint3
     }
     // the top level holds one root per tree
     std::vector<uint64_t> r(4 * (size_t)batch);
-    SG_HIP(hipMemcpyAsync(r.data(), t.buf + 4 * t.off.back(), 32 * (size_t)batch, hipMemcpyDeviceToHost, c->stream));
+    c->d2h(r.data(), t.buf + 4 * t.off.back(), 32 * (size_t)batch);
     SG_HIP(hipStreamSynchronize(c->stream));
     t.roots.resize(batch);
     for (uint32_t k = 0; k < batch; k++) memcpy(t.roots[k].data(), &r[4 * k], 32);
@@ -702,7 +702,7 @@ void tree_open_multi(Ctx *c, const DevTree &t, const std::vector<std::vector<uin
                        d_idx.as<uint64_t>(), (uint64_t)want.size(), 4u, d_out.as<uint64_t>());
     SG_HIP(hipGetLastError());
     std::vector<uint64_t> h(4 * want.size());
-    SG_HIP(hipMemcpyAsync(h.data(), d_out.p, 32 * want.size(), hipMemcpyDeviceToHost, c->stream));
+    c->d2h(h.data(), d_out.p, 32 * want.size());
     SG_HIP(hipStreamSynchronize(c->stream));
     // MerkleOpening: leaf[32] || LE64(levels) || per level (LE64(a-1) || a-1 siblings) || LE64(index)
     const size_t osz = 32 + 8 + (nlev - 1) * (8 + 32 * (size_t)(a - 1)) + 8;
@@ -781,7 +781,7 @@ void fri_query_indices(Ctx *c, const std::vector<uint64_t> &seeds, uint64_t s0, 
         SG_HIP(hipGetLastError());
     }
     std::vector<uint64_t> all(K * s0);
-    SG_HIP(hipMemcpyAsync(all.data(), d_r.p, 8 * K * s0, hipMemcpyDeviceToHost, c->stream));
+    c->d2h(all.data(), d_r.p, 8 * K * s0);
     SG_HIP(hipStreamSynchronize(c->stream));
     for (size_t k = 0; k < K; k++) std::copy(all.begin() + k * s0, all.begin() + (k + 1) * s0, raw[k].begin());
 }

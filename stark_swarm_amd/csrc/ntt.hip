@@ -334,6 +334,58 @@ __global__ __launch_bounds__(NT, 4) void ntt_tile(NttPass p) {
     }
 }
 
+// Small LDE (B * 2^LOG_S <= 8192 points per column): one workgroup holds
+// all B cosets of 8192 / (B * 2^LOG_S) columns, sub-DFT g = (column, r) with
+// r fastest, so the interleaved output LDE[B*k + r] of a column leaves in
+// runs of B consecutive u64 (the repetition form of ntt_tile writes every
+// B-th element instead). Input c_i of coset r is pre-twiddled by
+// w_{B n}^(r i); coefficients already carry 1/n and any shift^i.
+template <int LOG_S>
+__global__ __launch_bounds__(NT, 4) void lde_small_tile(const uint64_t *coef, uint64_t *out, uint64_t ncols,
+                                                       uint32_t log_b, uint64_t in_cs, uint64_t out_cs,
+                                                       const uint64_t *rt_lo, const uint64_t *rt_hi,
+                                                       const uint64_t *tw) {
+    __shared__ uint64_t lds[LDS_ELEMS];
+    constexpr uint32_t S = 1u << LOG_S;
+    constexpr uint32_t LOG_G = TILE_LOG - LOG_S, G = 1u << LOG_G;
+    static_assert(G <= (uint32_t)NT, "one sub-DFT per lane at least");
+    constexpr uint32_t DI = NT >> LOG_G;  // lanes per sub-DFT
+    uint64_t blk = blockIdx.x;
+    if ((gridDim.x & 7) == 0) blk = (blk & 7) * (gridDim.x >> 3) + (blk >> 3);
+    const uint32_t gl = threadIdx.x & (G - 1), il = threadIdx.x >> LOG_G;
+    const uint64_t col = (blk << (LOG_G - log_b)) + (gl >> log_b);
+    const uint32_t r = gl & ((1u << log_b) - 1);
+    const bool ok = col < ncols;
+    const uint32_t log_n0 = LOG_S + log_b;
+    {
+        uint64_t w = 1, st = 1;
+        if (r) {
+            w = root_pow(rt_lo, rt_hi, log_n0, (uint64_t)r * il, false);
+            st = root_pow(rt_lo, rt_hi, log_n0, (uint64_t)r * DI, false);
+        }
+        const uint64_t *src = coef + col * in_cs;
+#pragma unroll 2
+        for (uint32_t u = 0; u < EPT; u++) {
+            const uint32_t i = il + u * DI;
+            if (i < S) {
+                uint64_t x = ok ? src[i] : 0;
+                if (r) x = mul(x, w);
+                lds[padi((gl << LOG_S) + i)] = x;
+            }
+            if (r) w = mul(w, st);
+        }
+    }
+    __syncthreads();
+    stockham_steps<LOG_S, 0, false>(lds, TILE, tw);
+    if (!ok) return;
+    uint64_t *dst = out + col * out_cs + r;
+#pragma unroll 2
+    for (uint32_t u = 0; u < EPT; u++) {
+        const uint32_t k = il + u * DI;
+        if (k < S) dst[(uint64_t)k << log_b] = canon(lds[padi((gl << LOG_S) + k)]);
+    }
+}
+
 template <bool INV>
 void launch_tile(uint32_t log_s, dim3 grid, hipStream_t st, const NttPass &p) {
     switch (log_s) {
@@ -498,7 +550,24 @@ void lde_columns(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint32
                     use_shift ? &sp : nullptr);
     }
     const uint32_t LN = L + LB;
-    if (L <= 12) {
+    if (L >= 4 && L <= 9 && LN <= (uint32_t)TILE_LOG) {
+        const uint64_t cpt = (uint64_t)TILE >> LN;  // columns per workgroup
+        const uint64_t blocks = (ncols + cpt - 1) / cpt;
+        SG_REQUIRE(blocks < (1ULL << 31), "lde: grid too large");
+        ProfScope ps(c, "lde_small", 8.0 * ncols * (n + (n << LB)));
+        const dim3 grid((uint32_t)blocks);
+        switch (L) {
+#define SG_CASE(LS)                                                                                          \
+    case LS:                                                                                                 \
+        hipLaunchKernelGGL((lde_small_tile<LS>), grid, dim3(NT), 0, c->stream, coef.as<uint64_t>(), out, ncols, \
+                           LB, n, out_cs, c->rt_lo, c->rt_hi, c->tw_f);                                       \
+        break;
+            SG_CASE(4) SG_CASE(5) SG_CASE(6) SG_CASE(7) SG_CASE(8) SG_CASE(9)
+#undef SG_CASE
+        default: throw Error(E_INTERNAL, "lde: small-LDE case out of range");
+        }
+        SG_HIP(hipGetLastError());
+    } else if (L <= 12) {
         // one pass per column tile, the B cosets as repetitions
         NttPass p = base_pass();
         p.in = coef.as<uint64_t>();

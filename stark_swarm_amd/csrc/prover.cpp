@@ -400,11 +400,14 @@ std::vector<RsaProof> prove_rsa_batch(Ctx *c, const std::vector<RsaRecord> &reco
                     const std::vector<RsaRecord> part(records.begin() + k0 * R, records.begin() + (k0 + G) * R);
                     build_rsa_traces_dev(c, part, R, h, d_tr.as<uint64_t>());
                 }
+                HostScope hs(c, "rsa_lde_merge");
                 lde_columns(c, d_tr.as<uint64_t>(), d_lde.as<uint64_t>(), ilog2(h), ilog2(SWARM_BLOWUP), W * G, h,
                             n0, 1);
+                merge_rsa_batch(c, d_lde.as<uint64_t>(), G, R, h, SWARM_BLOWUP, d_c.as<uint64_t>());
+                if (c->profiling) SG_HIP(hipStreamSynchronize(c->stream));
             }
-            merge_rsa_batch(c, d_lde.as<uint64_t>(), G, R, h, SWARM_BLOWUP, d_c.as<uint64_t>());
         }
+        HostScope hf(c, "rsa_fri");
         FriParams p;
         p.schedule = stir_schedule(n0);
         p.r = p.s0 = SWARM_QUERIES;
@@ -464,7 +467,7 @@ std::vector<uint8_t> prover_core_dev(Ctx *c, const uint64_t *d_trace, uint64_t w
         Scoped di(c, 8 * gi.size()), dv(c, 8 * gi.size());
         SG_HIP(hipMemcpyAsync(di.p, gi.data(), 8 * gi.size(), hipMemcpyHostToDevice, c->stream));
         gather_u64(c, d_lde, di.as<uint64_t>(), gi.size(), dv.as<uint64_t>());
-        SG_HIP(hipMemcpyAsync(vals.data(), dv.p, 8 * gi.size(), hipMemcpyDeviceToHost, c->stream));
+        c->d2h(vals.data(), dv.p, 8 * gi.size());
         SG_HIP(hipStreamSynchronize(c->stream));
     }
     std::vector<std::vector<uint8_t>> ops;
