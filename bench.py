@@ -130,6 +130,7 @@ def main():
     ap.add_argument("--streams", type=int, default=4, help="proofs in flight for the batched measurement")
     ap.add_argument("--shards", type=int, default=32, help="inner-shard proofs for the plain-vs-batched line (0 = skip)")
     ap.add_argument("--dsksk", type=int, default=32, help="DS-KSK (SHA-256 AIR) proofs for the plain-vs-batched line (0 = skip)")
+    ap.add_argument("--rsa", type=int, default=32, help="RSA-2048 record proofs for the plain-vs-batched line (0 = skip)")
     ap.add_argument("--serve", default="250,1000,4000", help="serving: Poisson arrival rates (requests/s), '' = skip")
     ap.add_argument("--serve-requests", type=int, default=400, help="serving: requests per rate")
     ap.add_argument("--sha-batch", default="64,10,13",
@@ -247,6 +248,9 @@ def main():
     if args.dsksk:
         log("ds-ksk (SHA-256 AIR): plain vs batched")
         res.update(ds_ksk_batch(S, args.dsksk))
+    if args.rsa:
+        log("rsa-2048 records: plain vs batched")
+        res.update(rsa_batch(S, args.rsa))
     if args.sha_batch:
         cnt, lo, hi = (int(x) for x in args.sha_batch.split(","))
         log("sha batch: %d uneven SHA-256/512 proofs, 2^%d..2^%d rows" % (cnt, lo, hi))
@@ -288,7 +292,7 @@ def shard_batch(S, K, nproofs, records_per_shard=1024):
     # lockstep batch: all shards in one stark_prove_inner_shard_batch call
     packed = S.PackedShards([j[0] for j in jobs])
     fsbs = [j[1] for j in jobs]
-    S.prove_inner_shard_batch(salt, S.PackedShards([j[0] for j in jobs[:K]]), fsbs[:K], ctx=ctx0)  # warm
+    S.prove_inner_shard_batch(salt, packed, fsbs, ctx=ctx0)  # warm: the batch's buffers, once
     t0 = time.perf_counter()
     outs = S.prove_inner_shard_batch(salt, packed, fsbs, ctx=ctx0)
     ldt = time.perf_counter() - t0
@@ -324,7 +328,7 @@ def ds_ksk_batch(S, nproofs):
         plain.append(S.prove_ds_ksk_binding(k, pa, f).proof_blob)
         lat.append(time.perf_counter() - t)
     plain_dt = time.perf_counter() - t0
-    S.prove_ds_ksk_binding_batch(keys[:4], parents[:4], fsb[:4])  # warm
+    S.prove_ds_ksk_binding_batch(keys, parents, fsb)  # warm: the batch's buffers, once
     t0 = time.perf_counter()
     outs = S.prove_ds_ksk_binding_batch(keys, parents, fsb)
     bdt = time.perf_counter() - t0
@@ -340,6 +344,52 @@ def ds_ksk_batch(S, nproofs):
                        "plain_latency_ms_p50": round(1e3 * lat[len(lat) // 2], 3),
                        "plain_latency_ms_p99": round(1e3 * lat[min(len(lat) - 1, (99 * len(lat)) // 100)], 3),
                        "bit_exact_vs_plain": exact}}
+
+
+def rsa_records(S, n, seed):
+    """n RSA-2048 records: odd 2048-bit moduli with the top bit set,
+    signatures s < n and em = s^65537 mod n, from a seeded generator (the
+    proving cost does not depend on n being a product of two primes)."""
+    import random
+    rng = random.Random(seed)
+    out = []
+    for _ in range(n):
+        m = rng.getrandbits(2048) | 1 | (1 << 2047)
+        sig = rng.randrange(m)
+        out.append(S.RsaStackedRecord(m, sig, pow(sig, 65537, m)))
+    return out
+
+
+def rsa_batch(S, nproofs):
+    """RSA-2048 signature records (the per-record prover of the
+    mixed-algorithm zone example: 11 041 columns, 11 280 constraints,
+    n_trace 32, n0 = 1024): one proof per call vs one batch call, every
+    batched proof equal to its single-call proof."""
+    import hashlib
+    recs = rsa_records(S, nproofs, 0x5EED0006)
+    pis = [hashlib.sha3_256(b"rsa-pis-%d" % i).digest() for i in range(nproofs)]
+    S.prove_rsa_exp_batch(recs[:1], pis[:1])  # warm
+    lat, plain = [], []
+    t0 = time.perf_counter()
+    for r, p in zip(recs, pis):
+        t = time.perf_counter()
+        plain.append(S.prove_rsa_exp_batch([r], [p])[0].proof_blob)
+        lat.append(time.perf_counter() - t)
+    plain_dt = time.perf_counter() - t0
+    S.prove_rsa_exp_batch(recs, pis)  # warm: the batch's buffers, once
+    t0 = time.perf_counter()
+    outs = S.prove_rsa_exp_batch(recs, pis)
+    bdt = time.perf_counter() - t0
+    exact = all(o.proof_blob == p for o, p in zip(outs, plain))
+    if not exact:
+        raise SystemExit("batched RSA proofs differ from the single-call proofs")
+    lat.sort()
+    return {"rsa": {"records_per_proof": 1, "n_trace": 32, "n0": 1024, "columns": 11041, "constraints": 11280,
+                    "proofs": nproofs, "plain_proofs_per_s": round(nproofs / plain_dt, 2),
+                    "batched_proofs_per_s": round(nproofs / bdt, 2), "speedup": round(plain_dt / bdt, 3),
+                    "plain_latency_ms_p50": round(1e3 * lat[len(lat) // 2], 3),
+                    "plain_latency_ms_p99": round(1e3 * lat[min(len(lat) - 1, (99 * len(lat)) // 100)], 3),
+                    "bit_exact_vs_plain": exact}}
 
 
 def c4_batch(S, nproofs, log_lo, log_hi):
@@ -388,32 +438,38 @@ def c4_batch(S, nproofs, log_lo, log_hi):
 def serve(S, rates, per_rate, workers=4, max_batch=32):
     """Serving: RRset proof requests arrive as a Poisson process at each rate
     (requests/s) and go through the proving server (HBM-budget admission,
-    continuous batching over `workers` streams). Request mix: 70 % inner-shard
-    proofs of an RRset group of 8-64 synthetic zone records (HashRollup), 30 %
-    DS->KSK SHA-256 bindings of 44-132 byte DNSKEYs. Latency = completion -
-    arrival. A sample of results is checked byte-equal to the same request
-    proven alone."""
+    continuous batching over `workers` streams). Request mix: 60 % inner-shard
+    proofs of an RRset group of 8-64 synthetic zone records (HashRollup), 20 %
+    DS->KSK SHA-256 bindings of 44-132 byte DNSKEYs, 20 % RSA-2048 signature
+    records. Latency = completion - arrival. A sample of results is checked
+    byte-equal to the same request proven alone."""
     import hashlib
     import threading
     from stark_swarm_amd.server import ProvingServer
     from stark_swarm_amd.zone import build_synthetic_zone
     rng = np.random.default_rng(0x5EED0005)
     zone = build_synthetic_zone(10000)
+    rsa_pool = rsa_records(S, 64, 0x5EED0007)
     salt = b"swarm-test-zone1"
     out = []
     with ProvingServer(workers=workers, hbm_budget=64 << 30, max_batch=max_batch) as srv:
         # warm every worker's context once
         for rid in [srv.submit_ds_ksk(b"w" * 44, b"\0" * 32, b"\0" * 32) for _ in range(workers)]:
             srv.wait(rid)
+        for rid in [srv.submit_rsa(rsa_pool[0], b"\0" * 32) for _ in range(workers)]:
+            srv.wait(rid)
         for rate in rates:
             n = per_rate
             reqs = []
             for i in range(n):
                 fsb = rng.bytes(32)
-                if rng.random() < 0.7:
+                u = rng.random()
+                if u < 0.6:
                     k = int(rng.integers(8, 65))
                     o = int(rng.integers(0, len(zone) - k))
                     reqs.append(("shard", S.PackedShards([zone[o:o + k]]), fsb, (o, k)))
+                elif u < 0.8:
+                    reqs.append(("rsa", rsa_pool[int(rng.integers(0, len(rsa_pool)))], fsb, None))
                 else:
                     key = rng.bytes(int(rng.integers(44, 133)))
                     reqs.append(("ksk", key, fsb, hashlib.sha256(key).digest()))
@@ -437,8 +493,12 @@ def serve(S, rates, per_rate, workers=4, max_batch=32):
                 d = due - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
-                ids[j] = (srv.submit_inner_shard(salt, x, fsb) if kind == "shard"
-                          else srv.submit_ds_ksk(x, aux, fsb))
+                if kind == "shard":
+                    ids[j] = srv.submit_inner_shard(salt, x, fsb)
+                elif kind == "rsa":
+                    ids[j] = srv.submit_rsa(x, fsb)
+                else:
+                    ids[j] = srv.submit_ds_ksk(x, aux, fsb)
             for t in waiters:
                 t.join()
             wall = time.perf_counter() - t0
@@ -451,14 +511,19 @@ def serve(S, rates, per_rate, workers=4, max_batch=32):
             # byte-equality of a sample against the request proven alone
             for j in range(0, n, max(1, n // 6)):
                 kind, x, fsb, aux = reqs[j]
-                alone = (S.prove_inner_shard_batch(salt, x, [fsb])[0] if kind == "shard"
-                         else S.prove_ds_ksk_binding(x, aux, fsb))
+                if kind == "shard":
+                    alone = S.prove_inner_shard_batch(salt, x, [fsb])[0]
+                elif kind == "rsa":
+                    alone = S.prove_rsa_exp_batch([x], [fsb])[0]
+                else:
+                    alone = S.prove_ds_ksk_binding(x, aux, fsb)
                 if alone.proof_blob != res[j].proof_blob or alone.pi_hash != res[j].pi_hash:
                     raise SystemExit("served proof %d differs from the proof made alone" % j)
         st = srv.stats()
     return {"serve": {"workers": workers, "max_batch": max_batch, "hbm_budget_GiB": 64,
                       "peak_admitted_GiB": round(st["peak_bytes_admitted"] / 2**30, 3),
-                      "mix": "70% inner-shard RRset groups (8-64 records), 30% DS-KSK SHA-256 (44-132 B keys)",
+                      "mix": "60% inner-shard RRset groups (8-64 records), 20% DS-KSK SHA-256 (44-132 B keys), "
+                             "20% RSA-2048 signature records",
                       "rates": out, "sample_bit_exact_vs_alone": True}}
 
 

@@ -556,6 +556,11 @@ int stark_server_submit(stark_server *s, const stark_serve_request *q, uint64_t 
             SG_REQUIRE((q->dnskey || q->dnskey_len == 0) && q->parent_ds_hash, "server submit: null DS-KSK input");
             if (q->dnskey_len) r.key.assign(q->dnskey, q->dnskey + q->dnskey_len);
             memcpy(r.parent.data(), q->parent_ds_hash, 32);
+        } else if (q->kind == STARK_SERVE_RSA) {
+            SG_REQUIRE(q->rsa_n && q->rsa_s && q->rsa_em, "server submit: null RSA input");
+            memcpy(r.rsa.n.data(), q->rsa_n, 256);
+            memcpy(r.rsa.s.data(), q->rsa_s, 256);
+            memcpy(r.rsa.em.data(), q->rsa_em, 256);
         } else {
             SG_REQUIRE(q->salt && (q->records || q->n_records == 0), "server submit: null shard input");
             memcpy(r.salt.data(), q->salt, 16);

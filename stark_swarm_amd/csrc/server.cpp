@@ -13,6 +13,10 @@ namespace sg {
 static const uint64_t BLOWUP = 32;
 
 uint64_t Server::need_bytes(const ServeRequest &r) {
+    if (r.kind == SERVE_RSA) {
+        const uint64_t h = 32, n0 = h * BLOWUP;
+        return 8ULL * RSA_W * (2 * h + 2 * n0) + 8ULL * 16 * n0;
+    }
     if (r.kind == SERVE_DS_KSK) {
         const uint64_t h = sha256_trace_height(r.key.size()), n0 = h * BLOWUP;
         return 8ULL * SHA256_W * (2 * h + 2 * n0) + 8ULL * 16 * n0;
@@ -59,7 +63,8 @@ double Server::now() const {
 }
 
 uint64_t Server::submit(ServeRequest &&r) {
-    SG_REQUIRE(r.kind == SERVE_DS_KSK || r.kind == SERVE_INNER_SHARD, "server: unknown request kind");
+    SG_REQUIRE(r.kind == SERVE_DS_KSK || r.kind == SERVE_INNER_SHARD || r.kind == SERVE_RSA,
+               "server: unknown request kind");
     if (r.kind == SERVE_INNER_SHARD) {
         SG_REQUIRE(!r.domains.empty(), "server: an inner shard needs at least one record");
         SG_REQUIRE(r.rdata.size() == r.domains.size() && r.rtype.size() == r.domains.size() &&
@@ -178,7 +183,21 @@ void Server::run_batch(Ctx *c, std::vector<Item> &batch, std::vector<ServeResult
         return;
     }
     try {
-        if (batch[0].req.kind == SERVE_DS_KSK) {
+        if (batch[0].req.kind == SERVE_RSA) {
+            std::vector<RsaRecord> recs(K);
+            std::vector<std::array<uint8_t, 32>> pis(K);
+            for (size_t i = 0; i < K; i++) {
+                recs[i] = batch[i].req.rsa;
+                pis[i] = batch[i].req.fs_binding;
+            }
+            std::vector<RsaProof> out = prove_rsa_batch(c, recs, 1, 32, pis);
+            for (size_t i = 0; i < K; i++) {
+                memcpy(res[i].pi_hash, out[i].pi_hash, 32);
+                memcpy(res[i].root_f0, out[i].root_f0, 32);
+                res[i].n_trace = out[i].n_trace;
+                res[i].proof = std::move(out[i].proof);
+            }
+        } else if (batch[0].req.kind == SERVE_DS_KSK) {
             std::vector<ByteSpan> keys(K);
             std::vector<std::array<uint8_t, 32>> par(K), fsb(K);
             for (size_t i = 0; i < K; i++) {

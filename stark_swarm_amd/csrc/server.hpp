@@ -7,7 +7,8 @@
 //
 // It stands in for the swarm worker's request loop
 // (crates/swarm-worker/src/main.rs:220-335, which calls prove_inner_shard
-// from tokio spawn_blocking threads) on the device side.
+// from tokio spawn_blocking threads) on the device side; besides inner
+// shards it proves DS->KSK bindings and RSA-2048 signature records.
 #pragma once
 #include <stdint.h>
 
@@ -26,7 +27,7 @@
 
 namespace sg {
 
-enum ServeKind : int { SERVE_DS_KSK = 0, SERVE_INNER_SHARD = 1 };
+enum ServeKind : int { SERVE_DS_KSK = 0, SERVE_INNER_SHARD = 1, SERVE_RSA = 2 };
 
 struct ServeRequest {
     int kind = SERVE_DS_KSK;
@@ -37,6 +38,7 @@ struct ServeRequest {
     std::vector<std::vector<uint8_t>> rdata;
     std::vector<uint16_t> rtype;
     std::vector<uint32_t> ttl;
+    RsaRecord rsa{};                    // RSA-2048: n, s, em (fs_binding = its public-input hash)
     std::array<uint8_t, 32> fs_binding{};
 };
 
@@ -45,7 +47,7 @@ struct ServeResult {
     int status = 0;  // 0 or an E_* code
     std::string error;
     int kind = 0;
-    uint8_t pi_hash[32] = {}, root_f0[32] = {}, aux[32] = {};  // aux: merkle root / asserted digest
+    uint8_t pi_hash[32] = {}, root_f0[32] = {}, aux[32] = {};  // aux: merkle root / asserted digest / zero
     size_t n_trace = 0, batch_size = 0;
     uint64_t batch_seq = 0;  // admission order of the batch the request ran in
     std::vector<uint8_t> proof;

@@ -2,13 +2,14 @@
 (crates/swarm-worker/src/main.rs:220-335) on one GPU: requests are admitted
 by an HBM budget and proven with continuous batching (stark_server_* in
 include/stark_gpu.h). Each result equals the single-call result of
-prove_ds_ksk_binding / prove_inner_shard for the same request."""
+prove_ds_ksk_binding / prove_inner_shard / the RSA-2048 record prover for
+the same request."""
 import ctypes
 from dataclasses import dataclass
 
 from . import api as A
 
-SERVE_DS_KSK, SERVE_INNER_SHARD = 0, 1
+SERVE_DS_KSK, SERVE_INNER_SHARD, SERVE_RSA = 0, 1, 2
 DRY_RUN = 1
 E_TIMEOUT = -6
 
@@ -16,7 +17,8 @@ E_TIMEOUT = -6
 class _Req(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("dnskey", ctypes.c_char_p), ("dnskey_len", A.sz),
                 ("parent_ds_hash", ctypes.c_char_p), ("salt", ctypes.c_char_p),
-                ("records", ctypes.POINTER(A._DnsRecord)), ("n_records", A.sz), ("fs_binding", ctypes.c_char_p)]
+                ("records", ctypes.POINTER(A._DnsRecord)), ("n_records", A.sz), ("fs_binding", ctypes.c_char_p),
+                ("rsa_n", ctypes.c_char_p), ("rsa_s", ctypes.c_char_p), ("rsa_em", ctypes.c_char_p)]
 
 
 class _Res(ctypes.Structure):
@@ -34,7 +36,7 @@ class ServeResult:
     kind: int
     pi_hash: bytes
     root_f0: bytes
-    aux: bytes  # merkle_root (inner shard) or asserted_digest (DS-KSK)
+    aux: bytes  # merkle_root (inner shard), asserted_digest (DS-KSK), zero (RSA)
     n_trace: int
     batch_size: int
     batch_seq: int
@@ -100,6 +102,13 @@ class ProvingServer:
         assert len(recs) == 1
         return self._submit(_Req(SERVE_INNER_SHARD, None, 0, None, bytes(salt), recs.arr, recs.counts[0],
                                  bytes(fs_binding)))
+
+    def submit_rsa(self, record, public_inputs_hash):
+        """one RSA-2048 record (an api.RsaStackedRecord: n, s < n, em)"""
+        def be(x):
+            return int(x).to_bytes(256, "big")
+        return self._submit(_Req(SERVE_RSA, None, 0, None, None, None, 0, bytes(public_inputs_hash),
+                                 be(record.n), be(record.s), be(record.em)))
 
     def wait(self, rid, timeout=600.0):
         r = _Res()

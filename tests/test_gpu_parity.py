@@ -395,13 +395,15 @@ def test_server_results_equal_standalone(S):
     from stark_swarm_amd.server import ProvingServer
     from stark_swarm_amd.zone import build_synthetic_zone
     say("proving server mix")
-    zone = build_synthetic_zone(1200)
+    zone = build_synthetic_zone(1600)
     salt = b"swarm-test-zone1"
     reqs = []
-    for i in range(14):
+    for i in range(18):
         fsb = hashlib.sha3_256(b"req%d" % i).digest()
         if i % 3 == 0:
             reqs.append(("shard", zone[80 * i: 80 * i + 20 + 7 * i], fsb))
+        elif i % 4 == 1:
+            reqs.append(("rsa", S.RsaStackedRecord(*_rsa_rec(0x900 + i)), fsb))
         else:
             key = bytes([(i * 13 + j) & 255 for j in range(20 + 9 * i)])
             reqs.append(("ksk", key, fsb))
@@ -410,8 +412,12 @@ def test_server_results_equal_standalone(S):
         def go(part):
             for j in part:
                 kind, x, fsb = reqs[j]
-                ids[j] = (srv.submit_inner_shard(salt, x, fsb) if kind == "shard"
-                          else srv.submit_ds_ksk(x, hashlib.sha256(x).digest(), fsb))
+                if kind == "shard":
+                    ids[j] = srv.submit_inner_shard(salt, x, fsb)
+                elif kind == "rsa":
+                    ids[j] = srv.submit_rsa(x, fsb)
+                else:
+                    ids[j] = srv.submit_ds_ksk(x, hashlib.sha256(x).digest(), fsb)
         th = [threading.Thread(target=go, args=(range(t, len(reqs), 2),)) for t in range(2)]
         for t in th:
             t.start()
@@ -423,6 +429,9 @@ def test_server_results_equal_standalone(S):
         if kind == "shard":
             alone = S.prove_inner_shard(salt, x, fsb)
             assert r.aux == alone.merkle_root
+        elif kind == "rsa":
+            alone = S.prove_rsa_exp_batch([x], [fsb])[0]
+            assert r.aux == bytes(32) and r.kind == 2
         else:
             alone = S.prove_ds_ksk_binding(x, hashlib.sha256(x).digest(), fsb)
             assert r.aux == alone.asserted_digest

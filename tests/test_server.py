@@ -57,13 +57,15 @@ def test_oversized_request_runs_alone():
 
 
 def test_mixed_kinds_and_concurrent_submitters():
-    zone = build_synthetic_zone(400)
+    zone = build_synthetic_zone(500)
     out = {}
 
     def submitter(srv, base):
-        for j in range(10):
-            if j % 2:
+        for j in range(12):
+            if j % 3 == 1:
                 rid = srv.submit_inner_shard(b"swarm-test-zone1", zone[base + 10 * j: base + 10 * j + 10], b"\2" * 32)
+            elif j % 3 == 2:
+                rid = srv.submit_rsa(S.RsaStackedRecord((1 << 2047) | 1, 5 + j, 7), b"\4" * 32)
             else:
                 rid = srv.submit_ds_ksk(_keys(1, base + j)[0], b"\0" * 32, b"\3" * 32)
             out[rid] = srv.wait(rid, timeout=60)
@@ -74,9 +76,14 @@ def test_mixed_kinds_and_concurrent_submitters():
             t.start()
         for t in th:
             t.join()
-        assert srv.stats()["completed"] == 40
+        assert srv.stats()["completed"] == 48
     kinds = {r.kind for r in out.values()}
-    assert kinds == {0, 1}
+    assert kinds == {0, 1, 2}
+    # a batch holds one kind only
+    by_batch = {}
+    for r in out.values():
+        by_batch.setdefault(r.batch_seq, set()).add(r.kind)
+    assert all(len(k) == 1 for k in by_batch.values())
 
 
 def test_wait_timeout_and_bad_requests():
