@@ -78,7 +78,31 @@ def cpu_baseline(w_full, lg_full, blowup, schedule_full, s0):
             "sample": "oracle ref_prover_core (single thread) on 2^%d x %d, blowup %d, schedule %s, %d queries: "
                       "%.2f s; scaled x%.0f by rows*columns to the 2^%d x %d workload" %
                       (lg, w, blowup, sched, s0, dt, scale, lg_full, w_full),
-            "cpu": _cpu_model()}
+            "cpu": _cpu_model(), "small_proofs": cpu_small_proofs(O)}
+
+
+def cpu_small_proofs(O):
+    """One whole proof of each small-proof line with the C oracle (single
+    thread), as proofs/s: the CPU side of the plain-vs-batched lines."""
+    import hashlib
+    import random
+    out = {}
+    key = splitmix_field(0x5EED0001, 8).astype("<u8").tobytes()
+    t = time.perf_counter()
+    O.prove_ds_ksk_binding(key, hashlib.sha256(key).digest(), b"\xAA" * 32)
+    out["ds_ksk"] = round(1.0 / (time.perf_counter() - t), 3)
+    rng = random.Random(0x5EED0006)
+    m = rng.getrandbits(2048) | 1 | (1 << 2047)
+    sig = rng.randrange(m)
+    t = time.perf_counter()
+    O.prove_rsa_exp(m, sig, pow(sig, 65537, m), bytes(32))
+    out["rsa"] = round(1.0 / (time.perf_counter() - t), 3)
+    leaves = O.zone_leaf_hashes(0, 1024, b"swarm-test-zone1")
+    t = time.perf_counter()
+    O.prove_inner_shard(b"swarm-test-zone1", leaves, bytes(32))
+    out["inner_shard"] = round(1.0 / (time.perf_counter() - t), 3)
+    out["unit"] = "proofs/s, one thread, one proof each (64-byte DS-KSK key, one RSA-2048 record, 1024-record shard)"
+    return out
 
 
 def _cpu_model():
@@ -136,6 +160,8 @@ def main():
     ap.add_argument("--sha-batch", default="64,10,13",
                     help="batched uneven SHA-256/512 proofs: count,log2 min rows,log2 max rows ('' = skip); "
                          "the full workload is 256,10,17")
+    ap.add_argument("--only", choices=["shards", "ds_ksk", "rsa", "sha_batch", "serve"],
+                    help="run only this plain-vs-batched line (for profiling it) and print its JSON")
     ap.add_argument("--shards-only", action="store_true",
                     help="run only the inner-shard plain-vs-batched measurement (for profiling that path)")
     args = ap.parse_args()
@@ -149,7 +175,20 @@ def main():
 
     import stark_swarm_amd as S
     if args.shards_only:
-        print(json.dumps(shard_batch(S, args.streams, max(1, args.shards))))
+        args.only = "shards"
+    if args.only:
+        if args.only == "shards":
+            line = shard_batch(S, args.streams, max(1, args.shards))
+        elif args.only == "ds_ksk":
+            line = ds_ksk_batch(S, max(1, args.dsksk))
+        elif args.only == "rsa":
+            line = rsa_batch(S, max(1, args.rsa))
+        elif args.only == "sha_batch":
+            cnt, lo, hi = (int(x) for x in args.sha_batch.split(","))
+            line = c4_batch(S, cnt, lo, hi)
+        else:
+            line = serve(S, [int(r) for r in args.serve.split(",")], args.serve_requests)
+        print(json.dumps(line))
         return
     ctx = S.default_context()
     w, lg, blowup = args.cols, args.log_rows, 8
