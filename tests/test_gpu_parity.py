@@ -65,6 +65,18 @@ def test_lde_matches_oracle(S, w, lg, blowup):
     assert np.array_equal(got, O.lde_columns(cols, blowup))
 
 
+@pytest.mark.parametrize("w,lg,blowup", [(33, 4, 2), (7, 5, 32), (33, 6, 4), (9, 7, 16), (3, 8, 32), (40, 9, 2),
+                                         (5, 9, 16), (2, 9, 32)])
+def test_small_lde_sizes(S, w, lg, blowup):
+    """the one-pass small LDE (blowup x 2^lg <= 8192: all cosets of a column
+    in one workgroup), with column counts that leave the last workgroup
+    partly empty, and the first size past it (2^9 x 32)"""
+    say("small lde w=%d n=2^%d blowup=%d" % (w, lg, blowup))
+    cols = rnd(400 + lg + w, (w, 1 << lg))
+    got = S.lde_trace_columns(cols, 1 << lg, blowup)
+    assert np.array_equal(got, O.lde_columns(cols, blowup))
+
+
 @pytest.mark.parametrize("lg,blowup", [(5, 8), (12, 8), (14, 8), (16, 4)])
 def test_coset_lde_matches_oracle(S, lg, blowup):
     say("coset lde n=2^%d blowup=%d shift=7" % (lg, blowup))
@@ -514,3 +526,21 @@ def test_prove_rsa_rejects_bad_record(S):
     n, s, em = _rsa_rec(5)
     with pytest.raises(S.StarkError):
         S.prove_rsa_exp_batch([S.RsaStackedRecord(n, n + 1, em)], [bytes(32)])
+
+
+def test_prove_rsa_stacked_three_and_batch_of_stacked(S):
+    """stacked_rsa_n3 records in one proof (rsa2048_stacked_air.rs:125-157
+    shape), and two stacked proofs of two records in one batch call"""
+    say("prove_rsa stacked 3")
+    recs = [_rsa_rec(0xA00 + i, bits=2046 + i) for i in range(3)]
+    pis = hashlib.sha3_256(b"stacked3").digest()
+    o = S.prove_rsa_exp_batch([S.RsaStackedRecord(*r) for r in recs], [pis], records_per_proof=3)[0]
+    assert o.proof_blob == _oracle_rsa_proof(recs, pis)
+    say("prove_rsa batch of stacked")
+    recs4 = [_rsa_rec(0xB00 + i) for i in range(4)]
+    pis2 = [bytes([7]) * 32, bytes([9]) * 32]
+    two = S.prove_rsa_exp_batch([S.RsaStackedRecord(*r) for r in recs4], pis2, records_per_proof=2)
+    for k in range(2):
+        one = S.prove_rsa_exp_batch([S.RsaStackedRecord(*r) for r in recs4[2 * k:2 * k + 2]], [pis2[k]],
+                                    records_per_proof=2)[0]
+        assert two[k].proof_blob == one.proof_blob
