@@ -59,7 +59,7 @@ def cpu_baseline(w_full, lg_full, blowup, schedule_full, s0):
     """Time the C oracle (single thread) on a bounded sample of the pipeline and
     scale to the full workload by rows x columns."""
     from tests import _oracle as O
-    lg, w = 14, 128
+    lg, w = 15, 128
     n = 1 << lg
     trace = splitmix_field(0x5EED0002, w * n).reshape(w, n)
     lg0 = lg + blowup.bit_length() - 1
@@ -115,18 +115,30 @@ def _cpu_model():
     return "unknown"
 
 
-def profiled_traffic(kernel):
-    """HBM bytes per launch (read + write) of `kernel` on the 2^20 x 128 workload
-    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary, or None."""
+def _profile_line(tag, kernel):
     path = os.path.join(ROOT, "profiles", "hbm_traffic__prover_core_2e20x128.md")
     try:
         for line in open(path):
             f = line.split()
-            if len(f) == 4 and f[0] == "TRAFFIC" and f[1] == kernel:
-                return float(f[2]) + float(f[3])
+            if len(f) >= 3 and f[0] == tag and f[1] == kernel:
+                return [float(x) for x in f[2:]]
     except OSError:
         pass
     return None
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch (read + write) of `kernel` on the 2^20 x 128 workload
+    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary, or None."""
+    v = _profile_line("TRAFFIC", kernel)
+    return v[0] + v[1] if v else None
+
+
+def profiled_valu_load(kernel):
+    """VALU issue load of `kernel` (SQ_INSTS_VALU x 4 cycles over the SIMD
+    cycles of its launch) from the same committed summary, or None."""
+    v = _profile_line("VALU_LOAD", kernel)
+    return v[0] if v else None
 
 
 def calib_copy(nbytes=1 << 30):
@@ -268,7 +280,10 @@ def main():
                      "traffic": profiled_traffic(dname) if (lg, w) == (20, 128) else None,
                      "traffic_source": "profiles/hbm_traffic__prover_core_2e20x128.md (FETCH_SIZE x2 + WRITE_SIZE x1, "
                                        "calibrated on a 1 GiB copy)",
-                     "bytes_per_launch": per_launch_bytes, "avg_launch_ms": round(avg_ms, 4)},
+                     "bytes_per_launch": per_launch_bytes, "avg_launch_ms": round(avg_ms, 4),
+                     "valu_issue_load": profiled_valu_load(dname) if (lg, w) == (20, 128) else None,
+                     "note": "integer-VALU bound (64-bit modular products on 32-bit lanes): the launch issues "
+                             "VALU instructions in valu_issue_load of the SIMD cycles, well before HBM limits"},
         "kernels": kernels,
     }
     if args.streams > 1:
