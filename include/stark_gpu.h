@@ -43,7 +43,9 @@ extern "C" {
 #define STARK_LDT_STIR 0 /* LdtMode::Stir (crates/swarm-dns/src/prover.rs:37-41) */
 #define STARK_LDT_FRI 1  /* LdtMode::Fri: not implemented, returns STARK_E_UNSUPPORTED */
 
-#define STARK_AIR_HASH_ROLLUP 5 /* AirType::HashRollup (crates/deep_ali/src/air_workloads.rs:83) */
+/* AirType ids: the variant's position (1-based) in crates/deep_ali/src/air_workloads.rs:20-140 */
+#define STARK_AIR_HASH_ROLLUP 5 /* AirType::HashRollup (air_workloads.rs:83) */
+#define STARK_AIR_NSEC3_CHAIN 6 /* AirType::Nsec3Chain (air_workloads.rs:117) */
 
 typedef struct stark_ctx stark_ctx;
 
@@ -101,6 +103,17 @@ typedef struct {
     uint8_t *proof; /* ark-serialize compressed DeepFriProof<SexticExt>; stark_free() it */
     size_t proof_len;
 } stark_sha_out;
+
+/* Nsec3Output (crates/swarm-dns/src/prover.rs:327-338) */
+typedef struct {
+    uint8_t pi_hash[32];
+    uint8_t chain_root[32];
+    uint8_t root_f0[32];
+    size_t record_count;
+    size_t n_trace;
+    uint8_t *proof; /* stark_free() it */
+    size_t proof_len;
+} stark_nsec3_out;
 
 /* One RSA-2048 exponentiation proof of stark_prove_rsa_exp_batch */
 typedef struct {
@@ -160,7 +173,7 @@ void stark_free(void *p);
 int stark_lde_columns(stark_ctx *ctx, const uint64_t *cols, size_t w, size_t n_trace, size_t blowup, uint64_t *out);
 
 /* deep_ali_merge_general (crates/deep_ali/src/lib.rs:343-437) for
- * air_id = STARK_AIR_HASH_ROLLUP; coeffs are the k combination coefficients
+ * air_id = STARK_AIR_HASH_ROLLUP (w 4, k 3) or STARK_AIR_NSEC3_CHAIN (w 8, k 4); coeffs are the k combination coefficients
  * (the reference passes lambda_j = j+1, prover.rs:100-102; only that choice
  * is accepted). lde is w x n0, c_eval receives n0 values. */
 int stark_merge(stark_ctx *ctx, int air_id, const uint64_t *lde, size_t w, size_t n_trace, size_t blowup,
@@ -262,6 +275,19 @@ int stark_merge_rsa_exp(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, siz
 int stark_prove_rsa_exp_batch(stark_ctx *ctx, const uint8_t *n_be, const uint8_t *s_be, const uint8_t *em_be,
                               size_t n_proofs, size_t records_per_proof, size_t n_trace,
                               const uint8_t *public_inputs_hashes, stark_rsa_out *outs);
+
+/* prove_nsec3_completeness (crates/swarm-dns/src/prover.rs:348-446): records
+ * = n x (owner_hash[32] || next_hash[32]), a closed chain (records[n-1]'s
+ * next_hash equals records[0]'s owner_hash, else STARK_E_INVALID, where the
+ * reference panics). */
+int stark_prove_nsec3_completeness(stark_ctx *ctx, const uint8_t *records, size_t n, const uint8_t salt[16],
+                                   const uint8_t fs_binding[32], int ldt, int phi_mode, stark_nsec3_out *out);
+/* n_chains chains in lockstep device batches per trace height: chain i is
+ * counts[i] records starting at record sum(counts[0..i)) of records; no
+ * single counterpart in the reference (a loop of prove_nsec3_completeness). */
+int stark_prove_nsec3_completeness_batch(stark_ctx *ctx, const uint8_t *records, const size_t *counts,
+                                         size_t n_chains, const uint8_t salt[16], const uint8_t *fs_bindings,
+                                         int ldt, int phi_mode, stark_nsec3_out *outs);
 
 /* prove_outer_rollup (crates/swarm-dns/src/prover.rs:253-304). */
 int stark_prove_outer_rollup(stark_ctx *ctx, const uint8_t *pi_hashes, size_t n, const uint8_t pk_hash[32], int ldt,

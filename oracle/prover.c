@@ -13,6 +13,8 @@
  *   shard_fs_binding        crates/swarm-proto/src/messages.rs:162-176
  *   prove_inner_shard       crates/swarm-dns/src/prover.rs:53-67,87-102,148-227
  *   prove_outer_rollup      crates/swarm-dns/src/prover.rs:253-304
+ *   NSEC3 chain AIR         crates/deep_ali/src/air_workloads.rs:789-870
+ *   prove_nsec3_completeness crates/swarm-dns/src/prover.rs:348-446
  */
 #include "refcpu.h"
 #include "refcpu_int.h"
@@ -292,4 +294,92 @@ int ref_prove_outer_rollup(const uint8_t *pi_hashes, size_t count, const uint8_t
     int rc = rollup_prove(leaves, 4 * count, pk_hash, phi_mode, proof, proof_len, root_f0);
     free(leaves);
     return rc;
+}
+
+/* ---------------------------- NSEC3 chain ---------------------------- */
+/* build_nsec3_chain_trace (crates/deep_ali/src/air_workloads.rs:811-860):
+ * columns 0..3 owner limbs, 4..7 next limbs (raw LE u64, reduced as
+ * F::from); rows past the chain repeat the last next limbs as owner and
+ * next, and the last row's next is forced to row 0's owner */
+void ref_build_nsec3_trace(size_t n_trace, const uint64_t *chain, size_t n, uint64_t *tr) {
+    memset(tr, 0, 8 * 8 * n_trace);
+    const size_t m = n < n_trace ? n : n_trace;
+    for (size_t i = 0; i < m; i++)
+        for (int k = 0; k < 8; k++) tr[k * n_trace + i] = gl_from_u64(chain[8 * i + k]);
+    if (n < n_trace) {
+        uint64_t prev[4] = {0, 0, 0, 0};
+        if (n)
+            for (int k = 0; k < 4; k++) prev[k] = chain[8 * (n - 1) + 4 + k];
+        for (size_t i = n; i < n_trace; i++)
+            for (int k = 0; k < 4; k++) tr[k * n_trace + i] = tr[(4 + k) * n_trace + i] = gl_from_u64(prev[k]);
+        for (int k = 0; k < 4; k++) tr[(4 + k) * n_trace + n_trace - 1] = tr[k * n_trace];
+    }
+}
+
+/* deep_ali_merge_general (lib.rs:343-437) for AirType::Nsec3Chain:
+ * C_k = nxt[k] - cur[4 + k] (eval_nsec3_chain_constraints,
+ * air_workloads.rs:862-870), lambda_k = k + 1, same phi quirk */
+int ref_merge_nsec3(const uint64_t *lde, size_t n_trace, size_t blowup, int phi_mode, uint64_t *c_eval) {
+    const size_t n = n_trace * blowup;
+    const int compute = phi_mode == 0 || (phi_mode == 1 && n >= 4096);
+    for (size_t i = 0; i < n; i++) {
+        uint64_t acc = 0;
+        if (compute) {
+            const size_t j = (i + blowup) % n;
+            for (int k = 0; k < 4; k++)
+                acc = gl_add(acc, gl_mul((uint64_t)(k + 1), gl_sub(lde[k * n + j], lde[(4 + k) * n + i])));
+        }
+        c_eval[i] = acc;
+    }
+    ref_quotient_from_phi(c_eval, n, n_trace);
+    return 0;
+}
+
+/* prove_nsec3_completeness (crates/swarm-dns/src/prover.rs:348-446):
+ * records = n x (owner_hash[32] || next_hash[32]), a closed chain */
+int ref_prove_nsec3_completeness(const uint8_t *records, size_t n, const uint8_t salt[16], const uint8_t fs_binding[32],
+                                 int phi_mode, uint8_t **proof, size_t *proof_len, uint8_t pi_hash[32],
+                                 uint8_t chain_root[32], uint8_t root_f0[32], size_t *n_trace_out) {
+    if (n == 0) return -1;
+    if (memcmp(records + 64 * (n - 1) + 32, records, 32) != 0) return -2; /* not closed */
+    uint8_t cnt[8];
+    put_le(cnt, n, 8);
+    sha3_ctx c;
+    sha3_init(&c);
+    sha3_update(&c, "DNS-NSEC3-CHAIN-ROOT-V1", 23);
+    sha3_update(&c, salt, 16);
+    sha3_update(&c, cnt, 8);
+    sha3_update(&c, records, 64 * n);
+    sha3_final(&c, chain_root);
+    size_t n_trace = 8;
+    while (n_trace < n) n_trace <<= 1;
+    const size_t n0 = n_trace * SWARM_BLOWUP;
+    uint64_t *chain = (uint64_t *)malloc(8 * 8 * n);
+    pack_leaves(records, 2 * n, chain);
+    uint64_t *tr = (uint64_t *)malloc(8 * 8 * n_trace), *lde = (uint64_t *)malloc(8 * 8 * n0);
+    uint64_t *ce = (uint64_t *)malloc(8 * n0);
+    ref_build_nsec3_trace(n_trace, chain, n, tr);
+    ref_lde_columns(tr, 8, n_trace, SWARM_BLOWUP, 1, lde);
+    ref_merge_nsec3(lde, n_trace, SWARM_BLOWUP, phi_mode, ce);
+    size_t sched[24];
+    ref_fri_params p;
+    p.schedule = sched;
+    p.n_layers = ref_make_schedule(n0, sched);
+    p.r = SWARM_QUERIES; p.s0 = SWARM_QUERIES; p.seed_z = SWARM_SEED_Z;
+    p.coeff_commit_final = 1; p.d_final = 1; p.stir = 1;
+    p.public_inputs_hash = fs_binding;
+    int rc = ref_deep_fri_prove(ce, n0, &p, proof, proof_len);
+    free(chain); free(tr); free(lde); free(ce);
+    if (rc) return rc;
+    memcpy(root_f0, *proof, 32);
+    sha3_init(&c);
+    sha3_update(&c, "DNS-NSEC3-PIHASH-V1", 19);
+    sha3_update(&c, salt, 16);
+    sha3_update(&c, cnt, 8);
+    sha3_update(&c, chain_root, 32);
+    sha3_update(&c, root_f0, 32);
+    sha3_update(&c, fs_binding, 32);
+    sha3_final(&c, pi_hash);
+    *n_trace_out = n_trace;
+    return 0;
 }

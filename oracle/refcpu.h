@@ -110,6 +110,15 @@ void ref_sha512_constraints(const uint64_t *cur, const uint64_t *nxt, size_t row
 int ref_merge_sha512(const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_blocks, int phi_mode,
                      uint64_t *c_eval);
 
+/* ---- NSEC3 chain (AirType::Nsec3Chain) and prove_nsec3_completeness ---- */
+/* chain: n records x 8 raw u64 (owner limbs, next limbs) */
+void ref_build_nsec3_trace(size_t n_trace, const uint64_t *chain, size_t n, uint64_t *tr);
+int ref_merge_nsec3(const uint64_t *lde, size_t n_trace, size_t blowup, int phi_mode, uint64_t *c_eval);
+/* records: n x (owner_hash[32] || next_hash[32]); -2 if not a closed chain */
+int ref_prove_nsec3_completeness(const uint8_t *records, size_t n, const uint8_t salt[16], const uint8_t fs_binding[32],
+                                 int phi_mode, uint8_t **proof, size_t *proof_len, uint8_t pi_hash[32],
+                                 uint8_t chain_root[32], uint8_t root_f0[32], size_t *n_trace_out);
+
 /* ---- RSA-2048 exponentiation AIR (s^65537 mod n; rsa2048_exp_air.rs) ---- */
 size_t ref_rsa_exp_width(void);                  /* 11 041 columns per record */
 size_t ref_rsa_exp_constraints_per_record(void); /* 11 280 */

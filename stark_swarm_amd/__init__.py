@@ -12,6 +12,8 @@ Rust functions they replace:
   make_schedule, build_params, prove_inner_shard, prove_outer_rollup
                            crates/swarm-dns/src/prover.rs:53-304
   prove_ds_ksk_binding     crates/swarm-dns/src/prover.rs:446-585
+  prove_nsec3_completeness, Nsec3Record, Nsec3Output
+                           crates/swarm-dns/src/prover.rs:321-446
   build_sha256_trace_multi crates/deep_ali/src/sha256_air.rs:429-479
   deep_ali_merge_sha256    crates/deep_ali/src/lib.rs:453-545
   build_sha512_trace_multi crates/deep_ali/src/sha512_air.rs:551-573
@@ -57,6 +59,10 @@ from .api import (  # noqa: F401
     deep_ali_merge_sha512,
     build_sha512_trace_multi,
     prove_sha_batch,
+    Nsec3Record,
+    Nsec3Output,
+    prove_nsec3_completeness,
+    prove_nsec3_completeness_batch,
     RsaStackedRecord,
     RsaProof,
     fill_rsa_exp_multirow,

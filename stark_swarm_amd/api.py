@@ -19,6 +19,7 @@ SEED_Z = 0xDEEFBAAD  # prover.rs:34
 PHI_SEQUENTIAL, PHI_RAYON, PHI_ONE_THREAD = 0, 1, 2
 _LDT_STIR, _LDT_FRI = 0, 1
 _AIR_HASH_ROLLUP = 5
+_AIR_NSEC3_CHAIN = 6
 
 u64p = ctypes.POINTER(ctypes.c_uint64)
 u8p = ctypes.POINTER(ctypes.c_uint8)
@@ -56,6 +57,11 @@ class _ShaOut(ctypes.Structure):
 class _RsaOut(ctypes.Structure):
     _fields_ = [("pi_hash", ctypes.c_uint8 * 32), ("root_f0", ctypes.c_uint8 * 32), ("n_trace", sz),
                 ("proof", u8p), ("proof_len", sz)]
+
+
+class _Nsec3Out(ctypes.Structure):
+    _fields_ = [("pi_hash", ctypes.c_uint8 * 32), ("chain_root", ctypes.c_uint8 * 32), ("root_f0", ctypes.c_uint8 * 32),
+                ("record_count", sz), ("n_trace", sz), ("proof", u8p), ("proof_len", sz)]
 
 
 class _DsKskOut(ctypes.Structure):
@@ -130,6 +136,11 @@ def lib():
                                                  ctypes.POINTER(sz), ctypes.POINTER(sz), sz, ctypes.c_char_p,
                                                  ctypes.c_int, ctypes.POINTER(_ShaOut)]),
         "stark_host_build_sha256_trace": (sz, [ctypes.c_char_p, sz, u64p, ctypes.c_char_p]),
+        "stark_prove_nsec3_completeness": (ctypes.c_int, [vp, ctypes.c_char_p, sz, ctypes.c_char_p, ctypes.c_char_p,
+                                                          ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Nsec3Out)]),
+        "stark_prove_nsec3_completeness_batch": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.POINTER(sz), sz,
+                                                                ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                                                ctypes.c_int, ctypes.POINTER(_Nsec3Out)]),
         "stark_merge_rsa_exp": (ctypes.c_int, [vp, u64p, sz, sz, sz, u64p, sz, u64p]),
         "stark_prove_rsa_exp_batch": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz, sz,
                                                      sz, ctypes.c_char_p, ctypes.POINTER(_RsaOut)]),
@@ -292,15 +303,16 @@ def ntt(values, inverse=False, ctx=None):
 # --------------------------------------------------------------------- merge
 class AirType(enum.Enum):
     HashRollup = _AIR_HASH_ROLLUP  # air_workloads.rs:83 (w = 4, 3 constraints, degree 2)
+    Nsec3Chain = _AIR_NSEC3_CHAIN  # air_workloads.rs:117 (w = 8, 4 constraints, degree 1)
 
     def width(self):
-        return 4
+        return 4 if self is AirType.HashRollup else 8
 
     def num_constraints(self):
-        return 3
+        return 3 if self is AirType.HashRollup else 4
 
     def max_constraint_degree(self):
-        return 2
+        return 2 if self is AirType.HashRollup else 1
 
 
 @dataclass
@@ -701,6 +713,64 @@ def prove_inner_shard(salt, records: Sequence[DnsRecord], fs_binding_32, ldt=Ldt
     blob = _take(out.proof, out.proof_len)
     return InnerShardOutput(bytes(out.pi_hash), bytes(out.merkle_root), out.record_count, out.n_trace, len(blob),
                             bytes(out.root_f0), blob)
+
+
+@dataclass
+class Nsec3Record:
+    """Nsec3Record (prover.rs:321-325)"""
+    owner_hash: bytes
+    next_hash: bytes
+
+
+@dataclass
+class Nsec3Output:
+    """Nsec3Output (prover.rs:327-338); prove_ms / local_verify_ms are not kept."""
+    pi_hash: bytes
+    chain_root: bytes
+    record_count: int
+    n_trace: int
+    proof_bytes: int
+    root_f0: bytes
+    proof_blob: bytes
+
+
+def _nsec3_out(o):
+    blob = _take(o.proof, o.proof_len)
+    return Nsec3Output(bytes(o.pi_hash), bytes(o.chain_root), o.record_count, o.n_trace, len(blob), bytes(o.root_f0),
+                       blob)
+
+
+def _nsec3_bytes(records):
+    out = []
+    for r in records:
+        assert len(r.owner_hash) == 32 and len(r.next_hash) == 32
+        out.append(bytes(r.owner_hash) + bytes(r.next_hash))
+    return b"".join(out)
+
+
+def prove_nsec3_completeness(records: Sequence[Nsec3Record], salt, fs_binding_32, ldt=LdtMode.Stir,
+                             phi_mode=PHI_RAYON, ctx=None):
+    """prove_nsec3_completeness (prover.rs:348-446); an empty or open chain
+    raises StarkError where the reference panics."""
+    ctx = ctx or default_context()
+    out = _Nsec3Out()
+    ctx._check(lib().stark_prove_nsec3_completeness(ctx.h, _nsec3_bytes(records), len(records), bytes(salt),
+                                                    bytes(fs_binding_32), ldt.value, phi_mode, ctypes.byref(out)))
+    return _nsec3_out(out)
+
+
+def prove_nsec3_completeness_batch(chains, salt, fs_bindings, ldt=LdtMode.Stir, phi_mode=PHI_RAYON, ctx=None):
+    """prove_nsec3_completeness for several chains in lockstep device batches
+    (stark_prove_nsec3_completeness_batch); result i equals the single call."""
+    ctx = ctx or default_context()
+    chains = [list(c) for c in chains]
+    n = len(chains)
+    assert len(fs_bindings) == n
+    outs = (_Nsec3Out * max(1, n))()
+    ctx._check(lib().stark_prove_nsec3_completeness_batch(
+        ctx.h, b"".join(_nsec3_bytes(c) for c in chains), (sz * max(1, n))(*[len(c) for c in chains]), n, bytes(salt),
+        b"".join(bytes(f) for f in fs_bindings), ldt.value, phi_mode, outs))
+    return [_nsec3_out(outs[i]) for i in range(n)]
 
 
 @dataclass

@@ -38,6 +38,24 @@ __global__ __launch_bounds__(AT) void compose_hash_rollup_k(const uint64_t *lde,
     phi[i] = add(add(c0, add(c1, c1)), add(add(c2, c2), c2));
 }
 
+// AirType::Nsec3Chain (air_workloads.rs:789-870): 8 columns (owner limbs,
+// next limbs), C_k = nxt[k] - cur[4 + k], phi = sum (k + 1) C_k
+__global__ __launch_bounds__(AT) void compose_nsec3_k(const uint64_t *lde, uint64_t n, uint64_t blowup,
+                                                     uint64_t *phi) {
+    const uint64_t i = (uint64_t)blockIdx.x * AT + threadIdx.x;
+    if (i >= n) return;
+    lde += 8 * n * blockIdx.y;
+    phi += n * blockIdx.y;
+    const uint64_t j = (i + blowup) & (n - 1);
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint64_t d = sub(lde[k * n + j], lde[(4 + k) * n + i]);
+        for (int t = 0; t <= k; t++) acc = add(acc, d);
+    }
+    phi[i] = acc;
+}
+
 // one residue class rho < m per lane: suffix sums along stride m
 __global__ __launch_bounds__(AT) void zh_div_k(const uint64_t *a, uint64_t n, uint64_t m, uint64_t *q) {
     const uint64_t rho = (uint64_t)blockIdx.x * AT + threadIdx.x;
@@ -87,6 +105,42 @@ void merge_hash_rollup_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n
         SG_HIP(hipGetLastError());
     }
     quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
+}
+
+void merge_nsec3_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup, int phi_mode,
+                       uint64_t *c_eval) {
+    SG_REQUIRE(is_pow2(n_trace) && is_pow2(blowup) && blowup >= 2, "merge: sizes must be powers of two");
+    SG_REQUIRE(K >= 1 && K <= 65535, "merge: batch size out of range");
+    const uint64_t n = n_trace * blowup;
+    SG_REQUIRE(n <= (1ULL << 24), "merge: domain above 2^24");
+    if (!phi_computed(phi_mode, n)) {
+        fill_u64(c, c_eval, n * K, 0);
+        return;
+    }
+    Scoped phi(c, 8 * n * K);
+    {
+        ProfScope ps(c, "compose_nsec3", (8.0 * 8 * n + 8.0 * n) * K);
+        hipLaunchKernelGGL(compose_nsec3_k, dim3((uint32_t)((n + AT - 1) / AT), K), dim3(AT), 0, c->stream, lde, n,
+                           blowup, phi.as<uint64_t>());
+        SG_HIP(hipGetLastError());
+    }
+    quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
+}
+
+// build_nsec3_chain_trace (air_workloads.rs:811-860): chain = n records of
+// 8 raw LE u64 limbs (owner, next), reduced into the field; rows past the
+// chain repeat the last next limbs as owner and next, and the last row's
+// next becomes row 0's owner.
+void build_nsec3_trace(uint64_t n_trace, const uint64_t *chain, uint64_t n, uint64_t *tr) {
+    const uint64_t m = n < n_trace ? n : n_trace;
+    for (uint64_t i = 0; i < m; i++)
+        for (int k = 0; k < 8; k++) tr[k * n_trace + i] = canon(chain[8 * i + k]);
+    if (n < n_trace) {
+        for (uint64_t i = n; i < n_trace; i++)
+            for (int k = 0; k < 4; k++)
+                tr[k * n_trace + i] = tr[(4 + k) * n_trace + i] = n ? canon(chain[8 * (n - 1) + 4 + k]) : 0;
+        for (int k = 0; k < 4; k++) tr[(4 + k) * n_trace + n_trace - 1] = tr[k * n_trace];
+    }
 }
 
 void quotient_batch(Ctx *c, uint64_t *phi, uint32_t K, uint64_t n_trace, uint64_t n, uint64_t *c_eval) {

@@ -17,5 +17,9 @@ void merge_hash_rollup_dev(Ctx *c, const uint64_t *lde, uint64_t n_trace, uint64
 void merge_hash_rollup_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup,
                              int phi_mode, uint64_t *c_eval);
 void build_hash_rollup_trace(uint64_t n_trace, const uint64_t *leaves, uint64_t n_leaves, uint64_t *tr);
+// AirType::Nsec3Chain: K proofs, lde of proof k at lde + 8*k*n0
+void merge_nsec3_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup, int phi_mode,
+                       uint64_t *c_eval);
+void build_nsec3_trace(uint64_t n_trace, const uint64_t *chain, uint64_t n, uint64_t *tr);
 
 }  // namespace sg

@@ -133,15 +133,20 @@ int stark_lde_columns(stark_ctx *ctx, const uint64_t *cols, size_t w, size_t n_t
 int stark_merge(stark_ctx *ctx, int air_id, const uint64_t *lde, size_t w, size_t n_trace, size_t blowup,
                 const uint64_t *coeffs, size_t k, int phi_mode, uint64_t *c_eval) {
     return guard(ctx, [&](sg::Ctx *c) {
-        if (air_id != STARK_AIR_HASH_ROLLUP) throw sg::Error(sg::E_UNSUPPORTED, "merge: only the HashRollup AIR");
-        SG_REQUIRE(w == 4 && k == 3 && lde && coeffs && c_eval, "merge: HashRollup needs w = 4, k = 3");
-        SG_REQUIRE(coeffs[0] == 1 && coeffs[1] == 2 && coeffs[2] == 3,
-                   "merge: only the reference combination coefficients (1, 2, 3)");
+        if (air_id != STARK_AIR_HASH_ROLLUP && air_id != STARK_AIR_NSEC3_CHAIN)
+            throw sg::Error(sg::E_UNSUPPORTED, "merge: only the HashRollup and Nsec3Chain AIRs");
+        const size_t ww = air_id == STARK_AIR_HASH_ROLLUP ? 4 : 8, kk = air_id == STARK_AIR_HASH_ROLLUP ? 3 : 4;
+        SG_REQUIRE(w == ww && k == kk && lde && coeffs && c_eval, "merge: wrong trace width or constraint count");
+        for (size_t j = 0; j < k; j++)
+            SG_REQUIRE(coeffs[j] == j + 1, "merge: only the reference combination coefficients (j + 1)");
         SG_REQUIRE(phi_mode >= 0 && phi_mode <= 2, "merge: bad phi_mode");
         const size_t n0 = n_trace * blowup;
-        sg::Scoped dl(c, 8 * 4 * n0), dc(c, 8 * n0);
-        h2d(c, dl.p, lde, 8 * 4 * n0);
-        sg::merge_hash_rollup_dev(c, dl.as<uint64_t>(), n_trace, blowup, phi_mode, dc.as<uint64_t>());
+        sg::Scoped dl(c, 8 * ww * n0), dc(c, 8 * n0);
+        h2d(c, dl.p, lde, 8 * ww * n0);
+        if (air_id == STARK_AIR_HASH_ROLLUP)
+            sg::merge_hash_rollup_dev(c, dl.as<uint64_t>(), n_trace, blowup, phi_mode, dc.as<uint64_t>());
+        else
+            sg::merge_nsec3_batch(c, dl.as<uint64_t>(), 1, n_trace, blowup, phi_mode, dc.as<uint64_t>());
         d2h(c, c_eval, dc.p, 8 * n0);
     });
 }
@@ -335,6 +340,43 @@ int stark_prove_rsa_exp_batch(stark_ctx *ctx, const uint8_t *n_be, const uint8_t
             outs[i].n_trace = r[i].n_trace;
             outs[i].proof = copy_out(r[i].proof, &outs[i].proof_len);
         }
+    });
+}
+
+static void fill_nsec3(const sg::Nsec3Proof &r, stark_nsec3_out *o) {
+    memcpy(o->pi_hash, r.pi_hash, 32);
+    memcpy(o->chain_root, r.chain_root, 32);
+    memcpy(o->root_f0, r.root_f0, 32);
+    o->record_count = r.record_count;
+    o->n_trace = r.n_trace;
+    o->proof = copy_out(r.proof, &o->proof_len);
+}
+
+int stark_prove_nsec3_completeness(stark_ctx *ctx, const uint8_t *records, size_t n, const uint8_t salt[16],
+                                   const uint8_t fs_binding[32], int ldt, int phi_mode, stark_nsec3_out *out) {
+    size_t counts[1] = {n};
+    return stark_prove_nsec3_completeness_batch(ctx, records, counts, 1, salt, fs_binding, ldt, phi_mode, out);
+}
+
+int stark_prove_nsec3_completeness_batch(stark_ctx *ctx, const uint8_t *records, const size_t *counts,
+                                         size_t n_chains, const uint8_t salt[16], const uint8_t *fs_bindings,
+                                         int ldt, int phi_mode, stark_nsec3_out *outs) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(records && counts && salt && fs_bindings && outs, "prove_nsec3_completeness: null argument");
+        if (ldt != STARK_LDT_STIR) throw sg::Error(sg::E_UNSUPPORTED, "only LdtMode::Stir is implemented");
+        SG_REQUIRE(phi_mode >= 0 && phi_mode <= 2, "prove_nsec3_completeness: bad phi_mode");
+        SG_REQUIRE(n_chains >= 1 && n_chains <= 65535, "prove_nsec3_completeness: 1 to 65535 chains");
+        std::vector<sg::ByteSpan> ch(n_chains);
+        std::vector<std::array<uint8_t, 32>> fsb(n_chains);
+        size_t off = 0;
+        for (size_t i = 0; i < n_chains; i++) {
+            ch[i] = {records + 64 * off, 64 * counts[i]};
+            off += counts[i];
+            memcpy(fsb[i].data(), fs_bindings + 32 * i, 32);
+            outs[i].proof = nullptr;
+        }
+        std::vector<sg::Nsec3Proof> r = sg::prove_nsec3_batch(c, ch, salt, fsb, phi_mode);
+        for (size_t i = 0; i < n_chains; i++) fill_nsec3(r[i], &outs[i]);
     });
 }
 

@@ -372,6 +372,90 @@ std::vector<DsKsk> prove_ds_ksk_binding_batch(Ctx *c, const std::vector<ByteSpan
     return out;
 }
 
+std::vector<Nsec3Proof> prove_nsec3_batch(Ctx *c, const std::vector<ByteSpan> &chains, const uint8_t salt[16],
+                                          const std::vector<std::array<uint8_t, 32>> &fs_bindings, int phi_mode) {
+    const size_t K = chains.size();
+    SG_REQUIRE(K >= 1 && fs_bindings.size() == K, "prove_nsec3: one fs_binding per chain");
+    std::vector<Nsec3Proof> out(K);
+    std::map<uint64_t, std::vector<size_t>> groups;  // n_trace -> chains
+    for (size_t k = 0; k < K; k++) {
+        const ByteSpan &ch = chains[k];
+        SG_REQUIRE(ch.p && ch.n >= 64 && ch.n % 64 == 0, "prove_nsec3: NSEC3 chain must be non-empty");
+        const size_t n = ch.n / 64;
+        SG_REQUIRE(memcmp(ch.p + 64 * (n - 1) + 32, ch.p, 32) == 0,
+                   "prove_nsec3: NSEC3 chain must be closed: records[n-1].next_hash != records[0].owner_hash");
+        uint64_t nt = 8;
+        while (nt < n) nt <<= 1;
+        SG_REQUIRE(nt * SWARM_BLOWUP <= (1ULL << 24), "prove_nsec3: chain above 2^19 records");
+        groups[nt].push_back(k);
+        // chain root (prover.rs:372-381)
+        Nsec3Proof &o = out[k];
+        o.record_count = n;
+        o.n_trace = nt;
+        uint8_t cnt[8];
+        for (int b = 0; b < 8; b++) cnt[b] = (uint8_t)((uint64_t)n >> (8 * b));
+        Sha3 h;
+        h.update("DNS-NSEC3-CHAIN-ROOT-V1", 23);
+        h.update(salt, 16);
+        h.update(cnt, 8);
+        h.update(ch.p, ch.n);
+        h.final(o.chain_root);
+    }
+    for (auto &g : groups) {
+        const uint64_t nt = g.first, n0 = nt * SWARM_BLOWUP;
+        const uint32_t G = (uint32_t)g.second.size();
+        std::vector<uint64_t> trace(8 * nt * G);
+        parallel_for(G, [&](size_t j) {
+            const ByteSpan &ch = chains[g.second[j]];
+            const size_t n = ch.n / 64;
+            std::vector<uint64_t> limbs(8 * n);
+            for (size_t q = 0; q < 8 * n; q++) {
+                uint64_t v = 0;
+                for (int b = 0; b < 8; b++) v |= (uint64_t)ch.p[8 * q + b] << (8 * b);
+                limbs[q] = v;
+            }
+            build_nsec3_trace(nt, limbs.data(), n, &trace[8 * nt * j]);
+        });
+        Scoped d_tr(c, 8 * trace.size()), d_c(c, 8ULL * n0 * G);
+        {
+            Scoped d_lde(c, 8ULL * 8 * n0 * G);
+            SG_HIP(hipMemcpyAsync(d_tr.p, trace.data(), 8 * trace.size(), hipMemcpyHostToDevice, c->stream));
+            lde_columns(c, d_tr.as<uint64_t>(), d_lde.as<uint64_t>(), ilog2(nt), ilog2(SWARM_BLOWUP), 8ULL * G, nt,
+                        n0, 1);
+            merge_nsec3_batch(c, d_lde.as<uint64_t>(), G, nt, SWARM_BLOWUP, phi_mode, d_c.as<uint64_t>());
+            SG_HIP(hipStreamSynchronize(c->stream));  // the host trace goes out of scope
+        }
+        FriParams p;
+        p.schedule = stir_schedule(n0);
+        p.r = p.s0 = SWARM_QUERIES;
+        p.seed_z = SWARM_SEED_Z;
+        p.coeff_commit_final = true;
+        p.d_final = 1;
+        p.stir = true;
+        p.has_pi = true;
+        std::vector<std::array<uint8_t, 32>> pis(G);
+        for (uint32_t j = 0; j < G; j++) pis[j] = fs_bindings[g.second[j]];
+        std::vector<std::vector<uint8_t>> proofs = deep_fri_prove_batch(c, d_c.as<uint64_t>(), G, n0, p, &pis, nullptr);
+        for (uint32_t j = 0; j < G; j++) {
+            Nsec3Proof &o = out[g.second[j]];
+            o.proof = std::move(proofs[j]);
+            memcpy(o.root_f0, o.proof.data(), 32);
+            // pi_hash (prover.rs:431-438)
+            uint8_t cnt[8];
+            for (int b = 0; b < 8; b++) cnt[b] = (uint8_t)((uint64_t)o.record_count >> (8 * b));
+            Sha3 h;
+            h.update("DNS-NSEC3-PIHASH-V1", 19);
+            h.update(salt, 16);
+            h.update(cnt, 8);
+            h.update(o.chain_root, 32);
+            h.update(o.root_f0, 32);
+            h.update(pis[j].data(), 32);
+            h.final(o.pi_hash);
+        }
+    }
+    return out;
+}
+
 std::vector<RsaProof> prove_rsa_batch(Ctx *c, const std::vector<RsaRecord> &records, uint32_t R, uint64_t h,
                                       const std::vector<std::array<uint8_t, 32>> &pis) {
     SG_REQUIRE(R >= 1 && R <= RSA_MAX_RECORDS, "prove_rsa: 1 to 4096 records per proof");

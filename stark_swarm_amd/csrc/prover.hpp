@@ -68,6 +68,19 @@ std::vector<ShaProof> prove_sha_batch(Ctx *c, const std::vector<int> &algs, cons
 std::vector<DsKsk> prove_ds_ksk_binding_batch(Ctx *c, const std::vector<ByteSpan> &dnskeys,
                                               const std::vector<std::array<uint8_t, 32>> &parent_ds_hashes,
                                               const std::vector<std::array<uint8_t, 32>> &fs_bindings, int phi_mode);
+// prove_nsec3_completeness output (crates/swarm-dns/src/prover.rs:327-338)
+struct Nsec3Proof {
+    uint8_t pi_hash[32], chain_root[32], root_f0[32];
+    size_t record_count = 0, n_trace = 0;
+    std::vector<uint8_t> proof;
+};
+// prove_nsec3_completeness (prover.rs:348-446) for K closed NSEC3 chains
+// (chain k = counts[k] records of owner_hash || next_hash, 64 bytes each):
+// lockstep device batches per trace height; result k equals the proof of
+// chain k alone. Throws E_INVALID for an empty or open chain.
+std::vector<Nsec3Proof> prove_nsec3_batch(Ctx *c, const std::vector<ByteSpan> &chains, const uint8_t salt[16],
+                                          const std::vector<std::array<uint8_t, 32>> &fs_bindings, int phi_mode);
+
 // One RSA-2048 exponentiation proof (one record, or R stacked records).
 struct RsaProof {
     uint8_t pi_hash[32], root_f0[32];

@@ -97,6 +97,12 @@ def lib():
         L.ref_build_sha512_trace.argtypes = [ctypes.c_char_p, sz, u64p, ctypes.c_char_p]
         L.ref_sha512_constraints.argtypes = [u64p, u64p, sz, sz, u64p]
         L.ref_merge_sha512.argtypes = [u64p, sz, sz, sz, ctypes.c_int, u64p]
+        L.ref_build_nsec3_trace.argtypes = [sz, u64p, sz, u64p]
+        L.ref_merge_nsec3.argtypes = [u64p, sz, sz, ctypes.c_int, u64p]
+        L.ref_prove_nsec3_completeness.restype = ctypes.c_int
+        L.ref_prove_nsec3_completeness.argtypes = [ctypes.c_char_p, sz, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                                   ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.c_char_p,
+                                                   ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(sz)]
         L.ref_rsa_exp_width.restype = sz
         L.ref_rsa_exp_constraints_per_record.restype = sz
         L.ref_fill_rsa_exp.restype = ctypes.c_int
@@ -315,3 +321,31 @@ def prove_rsa_exp(n, s, em, pis):
                                  rf, pi)
     assert rc == 0, rc
     return take_bytes(pp, pl), rf.raw, pi.raw
+
+
+def nsec3_trace(n_trace, chain):
+    """build_nsec3_chain_trace; chain = [(owner limbs[4], next limbs[4]), ...] as u64"""
+    ch = np.array([list(o) + list(x) for o, x in chain], dtype=np.uint64).reshape(-1) if chain else \
+        np.zeros(8, dtype=np.uint64)
+    tr = np.zeros((8, n_trace), dtype=np.uint64)
+    lib().ref_build_nsec3_trace(n_trace, ptr(ch), len(chain), ptr(tr))
+    return tr
+
+
+def merge_nsec3(lde, n_trace, blowup, phi_mode=1):
+    lde = np.ascontiguousarray(lde, dtype=np.uint64)
+    out = np.zeros(n_trace * blowup, dtype=np.uint64)
+    lib().ref_merge_nsec3(ptr(lde), n_trace, blowup, phi_mode, ptr(out))
+    return out
+
+
+def prove_nsec3_completeness(records, salt, fs_binding, phi_mode=1):
+    """records = [(owner 32 B, next 32 B)]; (proof, pi_hash, chain_root, root_f0, n_trace) or rc < 0"""
+    blob = b"".join(bytes(o) + bytes(x) for o, x in records)
+    pp, pl, nt = u8p(), sz(), sz()
+    pi, cr, rf = (ctypes.create_string_buffer(32) for _ in range(3))
+    rc = lib().ref_prove_nsec3_completeness(blob, len(records), bytes(salt), bytes(fs_binding), phi_mode,
+                                            ctypes.byref(pp), ctypes.byref(pl), pi, cr, rf, ctypes.byref(nt))
+    if rc:
+        return rc
+    return take_bytes(pp, pl), pi.raw, cr.raw, rf.raw, nt.value

@@ -544,3 +544,46 @@ def test_prove_rsa_stacked_three_and_batch_of_stacked(S):
         one = S.prove_rsa_exp_batch([S.RsaStackedRecord(*r) for r in recs4[2 * k:2 * k + 2]], [pis2[k]],
                                     records_per_proof=2)[0]
         assert two[k].proof_blob == one.proof_blob
+
+
+# ---- NSEC3 chain completeness ----
+
+def _nsec3_chain(n, tag=b"NSEC3-DEMO-OWNER"):
+    owners = sorted(hashlib.sha3_256(tag + i.to_bytes(8, "little")).digest() for i in range(n))
+    return [(owners[i], owners[(i + 1) % n]) for i in range(n)]
+
+
+@pytest.mark.parametrize("n,phi_mode", [(1, 1), (5, 1), (200, 1), (200, 0), (64, 2)])
+def test_prove_nsec3_matches_oracle(S, n, phi_mode):
+    say("prove_nsec3 n=%d phi %d" % (n, phi_mode))
+    recs = _nsec3_chain(n)
+    salt, fsb = b"swarm-test-zone1", hashlib.sha3_256(b"nsec3-%d" % n).digest()
+    proof, pi, cr, rf, nt = O.prove_nsec3_completeness(recs, salt, fsb, phi_mode)
+    o = S.prove_nsec3_completeness([S.Nsec3Record(a, b) for a, b in recs], salt, fsb, phi_mode=phi_mode)
+    assert (o.proof_blob, o.pi_hash, o.chain_root, o.root_f0, o.n_trace, o.record_count) == (proof, pi, cr, rf, nt, n)
+
+
+def test_prove_nsec3_batch_and_errors(S):
+    say("prove_nsec3 batch")
+    chains = [_nsec3_chain(n, b"T%d" % n) for n in (3, 40, 9, 300)]
+    fsbs = [bytes([i]) * 32 for i in range(4)]
+    salt = b"swarm-test-zone2"
+    outs = S.prove_nsec3_completeness_batch([[S.Nsec3Record(a, b) for a, b in c] for c in chains], salt, fsbs)
+    for c, f, o in zip(chains, fsbs, outs):
+        proof, pi, cr, rf, nt = O.prove_nsec3_completeness(c, salt, f)
+        assert o.proof_blob == proof and o.pi_hash == pi and o.chain_root == cr
+    open_chain = _nsec3_chain(6)
+    open_chain[-1] = (open_chain[-1][0], bytes(32))
+    with pytest.raises(S.StarkError):
+        S.prove_nsec3_completeness([S.Nsec3Record(a, b) for a, b in open_chain], salt, fsbs[0])
+    with pytest.raises(S.StarkError):
+        S.prove_nsec3_completeness([], salt, fsbs[0])
+
+
+def test_merge_nsec3_matches_oracle(S):
+    say("merge nsec3")
+    n = 256
+    tr = rnd(77, (8, n))
+    lde = O.lde_columns(tr, 32)
+    got, info = S.deep_ali_merge_general(lde, [1, 2, 3, 4], S.AirType.Nsec3Chain, pow(7, (P - 1) // (32 * n), P), n, 32)
+    assert np.array_equal(got, O.merge_nsec3(lde, n, 32)) and info.max_constraint_degree == 1
