@@ -313,17 +313,18 @@ void fold(Ctx *c, const uint64_t *f, uint64_t n, int ncomp, uint64_t m, uint32_t
     }
 }
 
-// coset evaluations of K proofs' layers at z_k zeta^j; evals_host: K x m x 6.
-// d_T: per proof tstride entries z0^(2^i); this layer's z = z0^(2^toff).
+// coset evaluations of K proofs' layers at z_k zeta^j into d_ev (device,
+// K x m x 6), queued on the stream; d_T: per proof tstride entries
+// z0^(2^i); this layer's z = z0^(2^toff).
 template <int M>
 void launch_ood(Ctx *c, const uint64_t *coef, uint64_t n, int ncomp, uint32_t K, const uint64_t *d_T,
-                uint32_t tstride, uint32_t toff, uint64_t zeta, uint64_t *evals_host) {
+                uint32_t tstride, uint32_t toff, uint64_t zeta, uint64_t *d_ev) {
     SG_REQUIRE(n <= (1ULL << 24) && toff + 24 <= tstride, "ood: table too short");
     // elements per thread: enough blocks to fill the chip, at most 64
     uint32_t ch = 64;
     while (ch > 1 && (n / ((uint64_t)FT * ch)) * K < 2048) ch >>= 1;
     const uint32_t nblocks = (uint32_t)((n + (uint64_t)FT * ch - 1) / ((uint64_t)FT * ch));
-    Scoped d_part(c, 48ULL * M * nblocks * K), d_ev(c, 48ULL * M * K);
+    Scoped d_part(c, 48ULL * M * nblocks * K);
     {
         ProfScope ps(c, "ood_sums", 8.0 * ncomp * n * K);
         hipLaunchKernelGGL(ood_sums_k<M>, dim3(nblocks, K), dim3(FT), 0, c->stream, coef, n, ncomp, d_T + 6 * toff,
@@ -331,19 +332,17 @@ void launch_ood(Ctx *c, const uint64_t *coef, uint64_t n, int ncomp, uint32_t K,
         SG_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(ood_finish_k, dim3(K), dim3(FIN_T), 0, c->stream, d_part.as<uint64_t>(), nblocks, (uint32_t)M,
-                       zeta, d_ev.as<uint64_t>());
+                       zeta, d_ev);
     SG_HIP(hipGetLastError());
-    c->d2h(evals_host, d_ev.p, 48ULL * M * K);
-    SG_HIP(hipStreamSynchronize(c->stream));
 }
 
 void ood(Ctx *c, const uint64_t *coef, uint64_t n, int ncomp, uint64_t m, uint32_t K, const uint64_t *d_T,
-         uint32_t tstride, uint32_t toff, uint64_t zeta, uint64_t *evals_host) {
+         uint32_t tstride, uint32_t toff, uint64_t zeta, uint64_t *d_ev) {
     switch (m) {
-    case 2: launch_ood<2>(c, coef, n, ncomp, K, d_T, tstride, toff, zeta, evals_host); break;
-    case 4: launch_ood<4>(c, coef, n, ncomp, K, d_T, tstride, toff, zeta, evals_host); break;
-    case 8: launch_ood<8>(c, coef, n, ncomp, K, d_T, tstride, toff, zeta, evals_host); break;
-    case 16: launch_ood<16>(c, coef, n, ncomp, K, d_T, tstride, toff, zeta, evals_host); break;
+    case 2: launch_ood<2>(c, coef, n, ncomp, K, d_T, tstride, toff, zeta, d_ev); break;
+    case 4: launch_ood<4>(c, coef, n, ncomp, K, d_T, tstride, toff, zeta, d_ev); break;
+    case 8: launch_ood<8>(c, coef, n, ncomp, K, d_T, tstride, toff, zeta, d_ev); break;
+    case 16: launch_ood<16>(c, coef, n, ncomp, K, d_T, tstride, toff, zeta, d_ev); break;
     default: throw Error(E_UNSUPPORTED, "fold arity must be 2, 4, 8 or 16");
     }
 }
@@ -466,24 +465,27 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
         if (!final_cc) parallel_for(K, [&](size_t k) { tr[k].challenge_ext("alpha", alpha[k].c); });
         // OOD coset evaluations
         coset_evals[ell].resize((size_t)K * m * 6);
+        Scoped d_ev(c, 48ULL * m * K);
         {
-            HostScope hs(c, "fri_ood");
             Scoped coef(c, 8ULL * K * ncomp * cur), scr(c, 8ULL * K * ncomp * cur);
             ntt_columns(c, f, coef.as<uint64_t>(), scr.as<uint64_t>(), ilog2(cur), (uint64_t)K * ncomp, cur, cur, true,
                         nullptr);
             const uint64_t zeta = pow(root_of_unity(cur), nn);
-            ood(c, coef.as<uint64_t>(), cur, ncomp, m, K, d_ztab.as<uint64_t>(), tstride, toff,
-                zeta, coset_evals[ell].data());
+            ood(c, coef.as<uint64_t>(), cur, ncomp, m, K, d_ztab.as<uint64_t>(), tstride, toff, zeta,
+                d_ev.as<uint64_t>());
         }
         toff += ilog2(m);
         // layer commitment: leaf = 6 components (fri.rs:1364-1366); a base-field
-        // layer (f0) hashes components 1..5 as zeros
+        // layer (f0) hashes components 1..5 as zeros. Queued behind the OOD
+        // kernels, so one wait returns both the coset evaluations and the roots.
         DevTree tl;
         {
-            HostScope hs(c, "fri_layer_tree");
+            HostScope hs(c, "fri_ood_and_layer_tree");
             const uint32_t a = pick_arity(cur, m);
             LeafSrc src{f, 1, cur, 6, (uint32_t)ncomp, nullptr, (uint64_t)ncomp * cur};
-            tree_build(c, tl, src, cur, a, merkle_depth(cur, a), ell, K);
+            tree_build(c, tl, src, cur, a, merkle_depth(cur, a), ell, K, false);
+            c->d2h(coset_evals[ell].data(), d_ev.p, 48ULL * m * K);
+            tree_read_roots(c, tl);
         }
         parallel_for(K, [&](size_t k) {
             roots[ell * K + k] = tl.roots[k];
@@ -505,10 +507,10 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
             const uint32_t a = pick_arity(nn, 16);
             DevTree tc;
             LeafSrc src{tup.as<uint64_t>(), 1, nn, (uint32_t)(6 * m), (uint32_t)(6 * m), nullptr, 6 * m * nn};
-            tree_build(c, tc, src, nn, a, merkle_depth(nn, a), 0xFE, K);
+            tree_build(c, tc, src, nn, a, merkle_depth(nn, a), 0xFE, K, false);
             tuples_planes.resize((size_t)6 * m * nn * K);
             c->d2h(tuples_planes.data(), tup.p, 8 * tuples_planes.size());
-            SG_HIP(hipStreamSynchronize(c->stream));
+            tree_read_roots(c, tc);
             n_final_tuples = nn;
             m_final = m;
             parallel_for(K, [&](size_t k) {

@@ -603,7 +603,7 @@ uint32_t pick_arity(uint64_t n, uint64_t m) {
 }
 
 void tree_build(Ctx *c, DevTree &t, const LeafSrc &src, uint64_t n_leaves, uint32_t arity, uint32_t depth,
-                uint64_t label, uint32_t batch) {
+                uint64_t label, uint32_t batch, bool read_roots) {
     SG_REQUIRE(n_leaves >= 1 && arity >= 2, "merkle: empty tree or arity < 2");
     SG_REQUIRE(batch >= 1 && batch <= 65535, "merkle: batch out of range");
     SG_REQUIRE(n_leaves <= (1ULL << 32) * KT, "merkle: too many leaves");
@@ -640,12 +640,15 @@ void tree_build(Ctx *c, DevTree &t, const LeafSrc &src, uint64_t n_leaves, uint3
                            t.buf + 4 * t.off[l]);
         SG_HIP(hipGetLastError());
     }
+    if (read_roots) tree_read_roots(c, t);
+}
+
+void tree_read_roots(Ctx *c, DevTree &t) {
     // the top level holds one root per tree
-    std::vector<uint64_t> r(4 * (size_t)batch);
-    c->d2h(r.data(), t.buf + 4 * t.off.back(), 32 * (size_t)batch);
-    SG_HIP(hipStreamSynchronize(c->stream));
-    t.roots.resize(batch);
-    for (uint32_t k = 0; k < batch; k++) memcpy(t.roots[k].data(), &r[4 * k], 32);
+    std::vector<uint64_t> r(4 * (size_t)t.batch);
+    c->d2h(r.data(), t.buf + 4 * t.off.back(), 32 * (size_t)t.batch);
+    t.roots.resize(t.batch);
+    for (uint32_t k = 0; k < t.batch; k++) memcpy(t.roots[k].data(), &r[4 * k], 32);
     memcpy(t.root, t.roots[0].data(), 32);
 }
 

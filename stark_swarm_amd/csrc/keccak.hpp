@@ -78,9 +78,12 @@ struct LeafSrc {
 
 // Build `batch` trees of n_leaves leaves each, constant arity
 // (crates/merkle/src/lib.rs:377-476); tree k reads its values at
-// src.vals + k * src.proof_stride.
+// src.vals + k * src.proof_stride. With read_roots false the kernels are only
+// queued and tree_read_roots fills t.roots later (one wait for several
+// results).
 void tree_build(Ctx *c, DevTree &t, const LeafSrc &src, uint64_t n_leaves, uint32_t arity, uint32_t depth,
-                uint64_t label, uint32_t batch = 1);
+                uint64_t label, uint32_t batch = 1, bool read_roots = true);
+void tree_read_roots(Ctx *c, DevTree &t);
 void tree_free(Ctx *c, DevTree &t);
 // Serialized MerkleOpenings (leaf[32], path Vec<Vec<[32]>>, index u64): idx[k]
 // lists the leaf indices to open in tree k; out[k][q] is one opening.
