@@ -164,9 +164,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", action="store_true")
     ap.add_argument("--streams", type=int, default=4, help="proofs in flight for the batched measurement")
-    ap.add_argument("--shards", type=int, default=32, help="inner-shard proofs for the plain-vs-batched line (0 = skip)")
-    ap.add_argument("--dsksk", type=int, default=32, help="DS-KSK (SHA-256 AIR) proofs for the plain-vs-batched line (0 = skip)")
-    ap.add_argument("--rsa", type=int, default=32, help="RSA-2048 record proofs for the plain-vs-batched line (0 = skip)")
+    ap.add_argument("--shards", type=int, default=128, help="inner-shard proofs for the plain-vs-batched line (0 = skip)")
+    ap.add_argument("--dsksk", type=int, default=64, help="DS-KSK (SHA-256 AIR) proofs for the plain-vs-batched line (0 = skip)")
+    ap.add_argument("--rsa", type=int, default=64, help="RSA-2048 record proofs for the plain-vs-batched line (0 = skip)")
     ap.add_argument("--serve", default="250,1000,4000", help="serving: Poisson arrival rates (requests/s), '' = skip")
     ap.add_argument("--serve-requests", type=int, default=400, help="serving: requests per rate")
     ap.add_argument("--sha-batch", default="64,10,13",
