@@ -136,6 +136,32 @@ int ref_prove_rsa_exp(const uint8_t n_be[256], const uint8_t s_be[256], const ui
                       const uint8_t pis[32], uint8_t **proof, size_t *proof_len, uint8_t root_f0[32],
                       uint8_t pi_hash[32]);
 
+/* ---- double-chain ECDSA-P256 AIR (p256_ecdsa_double_multirow_air.rs) ---- */
+size_t ref_p256_width(void);       /* 151 334 columns */
+size_t ref_p256_n_constraints(void); /* 155 774 */
+/* column bases: [0] bit A, [1] bit B, [2..7] select outputs of A then B,
+ * [8..13] r_proj, [14..19] acc of A then B, [20..25] base of A then B */
+void ref_p256_columns(size_t out[26]);
+/* sub-gadgets of chain ch (0 = A, 1 = B) in constraint order: kinds (0 mul,
+ * 1 add, 2 sub, 3 freeze, 4 select) and 6 column bases each (c, c bits,
+ * q / diff limbs, q / diff bits, carries / pos, neg); returns the count */
+int ref_p256_gadgets(int ch, int *kinds, size_t *cols);
+/* fill_ecdsa_double_multirow: pts = 12 x 32-byte BE values (a_initial xyz,
+ * a_base xyz, b_initial xyz, b_base xyz), rproj = 6 x 32 bytes, bits 0/1 */
+int ref_fill_p256(const uint8_t *pts, const uint8_t *rproj, const uint8_t *a_bits, size_t k_a, const uint8_t *b_bits,
+                  size_t k_b, size_t ht, uint64_t *tr);
+void ref_p256_constraints(const uint64_t *cur, const uint64_t *nxt, size_t trace_row, size_t n_trace, uint64_t *out);
+int ref_merge_p256(const uint64_t *lde, size_t n_trace, size_t blowup, const uint64_t *coeffs, uint64_t *c_eval);
+/* ECDSA record (qx, qy, r, s, digest: 32-byte BE): MSB-first bits of
+ * u1 = e/s and u2 = r/s mod n; -30 bad signature, -31 bad key */
+int ref_p256_ecdsa_bits(const uint8_t qx[32], const uint8_t qy[32], const uint8_t r[32], const uint8_t s[32],
+                        const uint8_t digest[32], uint8_t u1_bits[256], uint8_t u2_bits[256]);
+int ref_p256_ecdsa_trace(const uint8_t qx[32], const uint8_t qy[32], const uint8_t r[32], const uint8_t s[32],
+                         const uint8_t digest[32], size_t ht, uint64_t *tr, uint8_t rproj[6 * 32]);
+int ref_prove_p256_ecdsa(const uint8_t qx[32], const uint8_t qy[32], const uint8_t r[32], const uint8_t s[32],
+                         const uint8_t digest[32], uint8_t **proof, size_t *proof_len, uint8_t root_f0[32],
+                         uint8_t pi_hash[32]);
+
 /* ---- prover-core microbench (coset LDE -> row Merkle -> STIR prove) ---- */
 /* trace: w x n_trace column-major; returns proof bytes of deep_fri_prove over
  * f0 = sum_c (c+1) * LDE_c with the row-tree root as public_inputs_hash, and

@@ -112,6 +112,23 @@ def lib():
         L.ref_rsa_pkcs1_em_sha256.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
         L.ref_prove_rsa_exp.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
                                         ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.c_char_p, ctypes.c_char_p]
+        L.ref_p256_width.restype = sz
+        L.ref_p256_n_constraints.restype = sz
+        L.ref_p256_columns.argtypes = [ctypes.POINTER(sz)]
+        L.ref_p256_gadgets.restype = ctypes.c_int
+        L.ref_p256_gadgets.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(sz)]
+        L.ref_fill_p256.restype = ctypes.c_int
+        L.ref_fill_p256.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz, ctypes.c_char_p, sz, sz,
+                                    u64p]
+        L.ref_p256_constraints.argtypes = [u64p, u64p, sz, sz, u64p]
+        L.ref_merge_p256.argtypes = [u64p, sz, sz, u64p, u64p]
+        L.ref_p256_ecdsa_bits.restype = ctypes.c_int
+        L.ref_p256_ecdsa_bits.argtypes = [ctypes.c_char_p] * 7
+        L.ref_p256_ecdsa_trace.restype = ctypes.c_int
+        L.ref_p256_ecdsa_trace.argtypes = [ctypes.c_char_p] * 5 + [sz, u64p, ctypes.c_char_p]
+        L.ref_prove_p256_ecdsa.restype = ctypes.c_int
+        L.ref_prove_p256_ecdsa.argtypes = [ctypes.c_char_p] * 5 + [ctypes.POINTER(u8p), ctypes.POINTER(sz),
+                                                                    ctypes.c_char_p, ctypes.c_char_p]
         _lib = L
     return _lib
 
@@ -349,3 +366,92 @@ def prove_nsec3_completeness(records, salt, fs_binding, phi_mode=1):
     if rc:
         return rc
     return take_bytes(pp, pl), pi.raw, cr.raw, rf.raw, nt.value
+
+
+def _be32(x):
+    return int(x).to_bytes(32, "big")
+
+
+def p256_width():
+    return lib().ref_p256_width()
+
+
+def p256_n_constraints():
+    return lib().ref_p256_n_constraints()
+
+
+def p256_columns():
+    """dict of the column bases the tests name (see ref_p256_columns)"""
+    a = (sz * 26)()
+    lib().ref_p256_columns(a)
+    v = list(a)
+    return {"bit": v[0:2], "sel": [v[2:5], v[5:8]], "rproj": v[8:14], "acc": [v[14:17], v[17:20]],
+            "base": [v[20:23], v[23:26]]}
+
+
+def p256_gadgets(ch):
+    """[(kind, (c, cb, q/dl, qb/db, carries/pos, neg)), ...] of chain ch in constraint order;
+    kind 0 mul, 1 add, 2 sub, 3 freeze, 4 select"""
+    kinds = (ctypes.c_int * 256)()
+    cols = (sz * (6 * 256))()
+    n = lib().ref_p256_gadgets(ch, kinds, cols)
+    return [(kinds[i], tuple(cols[6 * i:6 * i + 6])) for i in range(n)]
+
+
+def p256_trace(points, rproj, a_bits, b_bits, n_trace):
+    """fill_ecdsa_double_multirow: points = 12 ints (a_initial xyz, a_base xyz,
+    b_initial xyz, b_base xyz), rproj = 6 ints, bits lists of 0/1"""
+    tr = np.zeros((p256_width(), n_trace), dtype=np.uint64)
+    rc = lib().ref_fill_p256(b"".join(_be32(v) for v in points), b"".join(_be32(v) for v in rproj),
+                             bytes(a_bits), len(a_bits), bytes(b_bits), len(b_bits), n_trace, ptr(tr))
+    if rc:
+        raise ValueError("ref_fill_p256: %d" % rc)
+    return tr
+
+
+def p256_constraints(cur, nxt, row, n_trace):
+    cur = np.ascontiguousarray(cur, dtype=np.uint64)
+    nxt = np.ascontiguousarray(nxt, dtype=np.uint64)
+    out = np.zeros(p256_n_constraints(), dtype=np.uint64)
+    lib().ref_p256_constraints(ptr(cur), ptr(nxt), row, n_trace, ptr(out))
+    return out
+
+
+def merge_p256(lde, n_trace, blowup, coeffs=None):
+    lde = np.ascontiguousarray(lde, dtype=np.uint64)
+    out = np.zeros(n_trace * blowup, dtype=np.uint64)
+    cp = None
+    if coeffs is not None:
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.uint64)
+        cp = ptr(coeffs)
+    lib().ref_merge_p256(ptr(lde), n_trace, blowup, cp, ptr(out))
+    return out
+
+
+def p256_ecdsa_bits(qx, qy, r, s, digest):
+    """(u1 bits, u2 bits) MSB first, or the negative code for a bad record"""
+    u1, u2 = ctypes.create_string_buffer(256), ctypes.create_string_buffer(256)
+    rc = lib().ref_p256_ecdsa_bits(_be32(qx), _be32(qy), _be32(r), _be32(s), bytes(digest), u1, u2)
+    if rc:
+        return rc
+    return list(u1.raw), list(u2.raw)
+
+
+def p256_ecdsa_trace(qx, qy, r, s, digest):
+    """(trace, r_proj ints) of the per-record ECDSA prover's 256-row trace"""
+    tr = np.zeros((p256_width(), 256), dtype=np.uint64)
+    rp = ctypes.create_string_buffer(6 * 32)
+    rc = lib().ref_p256_ecdsa_trace(_be32(qx), _be32(qy), _be32(r), _be32(s), bytes(digest), 256, ptr(tr), rp)
+    if rc:
+        raise ValueError("ref_p256_ecdsa_trace: %d" % rc)
+    return tr, [int.from_bytes(rp.raw[32 * i:32 * i + 32], "big") for i in range(6)]
+
+
+def prove_p256_ecdsa(qx, qy, r, s, digest):
+    """(proof, root_f0, pi_hash) of the per-record ECDSA-P256 prover"""
+    pp, pl = u8p(), sz()
+    rf, pi = ctypes.create_string_buffer(32), ctypes.create_string_buffer(32)
+    rc = lib().ref_prove_p256_ecdsa(_be32(qx), _be32(qy), _be32(r), _be32(s), bytes(digest), ctypes.byref(pp),
+                                    ctypes.byref(pl), rf, pi)
+    assert rc == 0, rc
+    return take_bytes(pp, pl), rf.raw, pi.raw
