@@ -124,6 +124,29 @@ typedef struct {
     size_t proof_len;
 } stark_rsa_out;
 
+/* Inputs of fill_ecdsa_double_multirow (crates/deep_ali/src/
+ * p256_ecdsa_double_multirow_air.rs:134-222) for one double-chain trace:
+ * 32-byte big-endian field elements below p. */
+typedef struct {
+    uint8_t pts[12][32];  /* chain A initial x, y, z, base x, y, z; then chain B the same */
+    uint8_t rproj[6][32]; /* r_a x, y, z, r_b x, y, z (used when bind_outputs == 0) */
+    const uint8_t *a_bits; /* k_a scalar bits of chain A (0/1), row order; rows past them take 0 */
+    size_t k_a;
+    const uint8_t *b_bits;
+    size_t k_b;
+    int bind_outputs; /* 1: r_proj = the chain outputs at rows k_a - 1 and k_b - 1 (the callers' two-pass fill) */
+} stark_p256_chains;
+
+/* One double-chain ECDSA-P256 proof */
+typedef struct {
+    uint8_t pi_hash[32]; /* SHA3-256(public_inputs_hash || root_f0) */
+    uint8_t root_f0[32];
+    uint8_t rproj[6][32]; /* the r_proj values bound in the trace */
+    size_t n_trace;
+    uint8_t *proof; /* ark-serialize compressed DeepFriProof<SexticExt>; stark_free() it */
+    size_t proof_len;
+} stark_p256_out;
+
 /* ---- proving server (one GPU, continuous batching) ---- */
 typedef struct stark_server stark_server;
 
@@ -276,6 +299,31 @@ int stark_prove_rsa_exp_batch(stark_ctx *ctx, const uint8_t *n_be, const uint8_t
                               size_t n_proofs, size_t records_per_proof, size_t n_trace,
                               const uint8_t *public_inputs_hashes, stark_rsa_out *outs);
 
+/* deep_ali_merge_ecdsa_double_multirow_streaming (crates/deep_ali/src/lib.rs:
+ * 1421-1540): lde is the 151334 x n0 LDE of a double-chain trace, coeffs the
+ * 155774 coefficients (lambda_j = j+1 only). Phi is evaluated at every point. */
+int stark_merge_ecdsa_p256(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, size_t blowup, const uint64_t *coeffs,
+                           size_t k, uint64_t *c_eval);
+
+/* run_double_chain_streaming (crates/swarm-dns/examples/
+ * p256_full_ecdsa_stark_bench.rs:133-240) for n_proofs chain inputs: trace of
+ * n_trace rows (a power of two >= max(k_a, k_b)) -> LDE x32 -> merge -> STIR
+ * with 54 queries bound to public_inputs_hashes[32 i ..]. Lockstep device
+ * batches in HBM-sized waves; result i equals the proof of input i alone. */
+int stark_prove_p256_chains_batch(stark_ctx *ctx, const stark_p256_chains *in, size_t n_proofs, size_t n_trace,
+                                  const uint8_t *public_inputs_hashes, stark_p256_out *outs);
+
+/* prove_record_ecdsa (crates/swarm-dns/examples/mixed_algorithm_zone_rollup.rs:
+ * 193-322) from the signature on, for n_records records: record i is
+ * (qx, qy, r, s, digest) at byte offset 32 i of each array (big-endian).
+ * u1 = e/s, u2 = r/s mod n; chain A from G over the MSB-first bits of u1,
+ * chain B from Q over those of u2, n_trace 256, r_proj bound to the outputs;
+ * STIR bound to SHA3-256("ECDSAP256-MIXEDZONE" || qx || qy || r || s ||
+ * digest). STARK_E_INVALID if a key is not a curve point or a signature does
+ * not verify (where the example asserts). */
+int stark_prove_ecdsa_p256_batch(stark_ctx *ctx, const uint8_t *qx, const uint8_t *qy, const uint8_t *r,
+                                 const uint8_t *s, const uint8_t *digest, size_t n_records, stark_p256_out *outs);
+
 /* prove_nsec3_completeness (crates/swarm-dns/src/prover.rs:348-446): records
  * = n x (owner_hash[32] || next_hash[32]), a closed chain (records[n-1]'s
  * next_hash equals records[0]'s owner_hash, else STARK_E_INVALID, where the
@@ -356,6 +404,12 @@ size_t stark_host_build_sha256_trace(const uint8_t *msg, size_t len, uint64_t *t
 size_t stark_host_sha512_trace_height(size_t len);
 size_t stark_host_build_sha512_trace(const uint8_t *msg, size_t len, uint64_t *trace, uint8_t digest[64]);
 void stark_host_dns_leaf_hash(const stark_dns_record *rec, const uint8_t salt[16], uint8_t out[32]);
+/* fill_ecdsa_double_multirow (crates/deep_ali/src/
+ * p256_ecdsa_double_multirow_air.rs:134-222): column-major 151334 x n_trace
+ * trace; in->rproj is overwritten with the bound values when
+ * in->bind_outputs. */
+int stark_host_build_ecdsa_p256_trace(stark_p256_chains *in, size_t n_trace, uint64_t *trace);
+
 /* fill_rsa_exp_multirow (crates/deep_ali/src/rsa2048_exp_air.rs:148-202) /
  * fill_rsa_stacked (rsa2048_stacked_air.rs:64-78): column-major trace of
  * 11041 n_records columns x n_trace rows (n_trace a power of two >= 32) for

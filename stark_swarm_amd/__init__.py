@@ -26,6 +26,13 @@ Rust functions they replace:
   emsa_pkcs1_v1_5_encode_sha256
                            crates/deep_ali/src/rsa2048.rs
   prove_rsa_exp_batch      crates/swarm-dns/examples/mixed_algorithm_zone_rollup.rs:324-395
+  fill_ecdsa_double_multirow, P256Chains
+                           crates/deep_ali/src/p256_ecdsa_double_multirow_air.rs:134-222
+  deep_ali_merge_ecdsa_double_multirow_streaming
+                           crates/deep_ali/src/lib.rs:1421-1540
+  prove_p256_chains_batch  crates/swarm-dns/examples/p256_full_ecdsa_stark_bench.rs:133-240
+  prove_record_ecdsa_batch, EcdsaRecord
+                           crates/swarm-dns/examples/mixed_algorithm_zone_rollup.rs:193-322
   DnsRecord                crates/swarm-dns/src/dns.rs:24-89
   shard_fs_binding         crates/swarm-proto/src/messages.rs:162-176
 
@@ -72,6 +79,14 @@ from .api import (  # noqa: F401
     emsa_pkcs1_v1_5_encode_sha256,
     rsa_record_pk_hash,
     prove_rsa_exp_batch,
+    P256Chains,
+    P256Proof,
+    EcdsaRecord,
+    fill_ecdsa_double_multirow,
+    deep_ali_merge_ecdsa_double_multirow_streaming,
+    prove_p256_chains_batch,
+    prove_record_ecdsa_batch,
+    ecdsa_record_pk_hash,
     deep_fri_prove,
     deep_fri_prove_batch,
     default_context,

@@ -59,6 +59,16 @@ class _RsaOut(ctypes.Structure):
                 ("proof", u8p), ("proof_len", sz)]
 
 
+class _P256Chains(ctypes.Structure):
+    _fields_ = [("pts", (ctypes.c_uint8 * 32) * 12), ("rproj", (ctypes.c_uint8 * 32) * 6), ("a_bits", u8p),
+                ("k_a", sz), ("b_bits", u8p), ("k_b", sz), ("bind_outputs", ctypes.c_int)]
+
+
+class _P256Out(ctypes.Structure):
+    _fields_ = [("pi_hash", ctypes.c_uint8 * 32), ("root_f0", ctypes.c_uint8 * 32), ("rproj", (ctypes.c_uint8 * 32) * 6),
+                ("n_trace", sz), ("proof", u8p), ("proof_len", sz)]
+
+
 class _Nsec3Out(ctypes.Structure):
     _fields_ = [("pi_hash", ctypes.c_uint8 * 32), ("chain_root", ctypes.c_uint8 * 32), ("root_f0", ctypes.c_uint8 * 32),
                 ("record_count", sz), ("n_trace", sz), ("proof", u8p), ("proof_len", sz)]
@@ -147,6 +157,13 @@ def lib():
         "stark_host_build_rsa_exp_trace": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz,
                                                           sz, u64p]),
         "stark_host_rsa_pkcs1_em_sha256": (None, [ctypes.c_char_p, ctypes.c_char_p]),
+        "stark_merge_ecdsa_p256": (ctypes.c_int, [vp, u64p, sz, sz, u64p, sz, u64p]),
+        "stark_prove_p256_chains_batch": (ctypes.c_int, [vp, ctypes.POINTER(_P256Chains), sz, sz, ctypes.c_char_p,
+                                                         ctypes.POINTER(_P256Out)]),
+        "stark_prove_ecdsa_p256_batch": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                                        ctypes.c_char_p, ctypes.c_char_p, sz,
+                                                        ctypes.POINTER(_P256Out)]),
+        "stark_host_build_ecdsa_p256_trace": (ctypes.c_int, [ctypes.POINTER(_P256Chains), sz, u64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -563,6 +580,144 @@ def prove_rsa_exp_batch(records, public_inputs_hashes, records_per_proof=1, n_tr
     ctx._check(lib().stark_prove_rsa_exp_batch(ctx.h, nb, sb, eb, n, records_per_proof, n_trace,
                                                b"".join(bytes(x) for x in public_inputs_hashes), outs))
     return [RsaProof(bytes(o.pi_hash), bytes(o.root_f0), o.n_trace, _take(o.proof, o.proof_len)) for o in outs]
+
+
+# ------------------------------------------------------ ECDSA-P256 AIR
+P256_WIDTH, P256_CONSTRAINTS, P256_ECDSA_N_TRACE = 151334, 155774, 256
+
+
+@dataclass
+class P256Chains:
+    """The inputs of fill_ecdsa_double_multirow
+    (p256_ecdsa_double_multirow_air.rs:134-222), as integers below p: for
+    chain A then chain B the initial accumulator and the base point
+    (projective x, y, z), the scalar bits (0/1, one per row), and the six
+    r_proj values; bind_outputs takes r_proj from the chain outputs at the
+    chains' last bits (the callers' two-pass fill)."""
+    a_initial: tuple
+    a_base: tuple
+    b_initial: tuple
+    b_base: tuple
+    a_bits: list
+    b_bits: list
+    rproj: tuple = (0, 0, 0, 0, 0, 0)
+    bind_outputs: bool = True
+
+
+def _p256_chains(c):
+    x = _P256Chains()
+    pts = list(c.a_initial) + list(c.a_base) + list(c.b_initial) + list(c.b_base)
+    assert len(pts) == 12 and len(c.rproj) == 6
+    for i, v in enumerate(pts):
+        x.pts[i][:] = int(v).to_bytes(32, "big")
+    for i, v in enumerate(c.rproj):
+        x.rproj[i][:] = int(v).to_bytes(32, "big")
+    a = (ctypes.c_uint8 * max(1, len(c.a_bits)))(*[int(b) for b in c.a_bits])
+    b = (ctypes.c_uint8 * max(1, len(c.b_bits)))(*[int(v) for v in c.b_bits])
+    x.a_bits, x.k_a = ctypes.cast(a, u8p), len(c.a_bits)
+    x.b_bits, x.k_b = ctypes.cast(b, u8p), len(c.b_bits)
+    x.bind_outputs = int(bool(c.bind_outputs))
+    x._keep = (a, b)
+    return x
+
+
+def fill_ecdsa_double_multirow(chains: P256Chains, n_trace):
+    """fill_ecdsa_double_multirow (p256_ecdsa_double_multirow_air.rs:134-222):
+    the 151334 x n_trace trace and the r_proj values it binds."""
+    x = _p256_chains(chains)
+    tr = np.zeros((P256_WIDTH, n_trace), dtype=np.uint64)
+    rc = lib().stark_host_build_ecdsa_p256_trace(ctypes.byref(x), n_trace, tr.ctypes.data_as(u64p))
+    if rc != 0:
+        raise StarkError(rc, "fill_ecdsa_double_multirow: need values below p, n_trace a power of two >= the bits")
+    return tr, tuple(int.from_bytes(bytes(x.rproj[i]), "big") for i in range(6))
+
+
+def deep_ali_merge_ecdsa_double_multirow_streaming(trace_evals_on_lde, combination_coeffs, omega, n_trace, blowup,
+                                                   ctx=None):
+    """deep_ali_merge_ecdsa_double_multirow_streaming (lib.rs:1421-1540)."""
+    n = n_trace * blowup
+    lde = np.asarray(trace_evals_on_lde, dtype=np.uint64)
+    assert lde.shape == (P256_WIDTH, n), "trace shape mismatch"
+    assert len(combination_coeffs) == P256_CONSTRAINTS, "need one combination coefficient per constraint"
+    del omega  # unused by the reference too (lib.rs:1430)
+    ctx = ctx or default_context()
+    lde, lp = _u64(lde)
+    co, cp = _u64(np.asarray(combination_coeffs, dtype=np.uint64))
+    out = np.empty(n, dtype=np.uint64)
+    ctx._check(lib().stark_merge_ecdsa_p256(ctx.h, lp, n_trace, blowup, cp, P256_CONSTRAINTS,
+                                            out.ctypes.data_as(u64p)))
+    return out, CompositionInfo(2 * n_trace, n_trace, n_trace / n, P256_CONSTRAINTS, 2, P256_WIDTH)
+
+
+@dataclass
+class P256Proof:
+    pi_hash: bytes
+    root_f0: bytes
+    rproj: tuple
+    n_trace: int
+    proof_blob: bytes
+
+
+def _p256_outs(outs):
+    return [P256Proof(bytes(o.pi_hash), bytes(o.root_f0),
+                      tuple(int.from_bytes(bytes(o.rproj[i]), "big") for i in range(6)), o.n_trace,
+                      _take(o.proof, o.proof_len)) for o in outs]
+
+
+def prove_p256_chains_batch(chains, n_trace, public_inputs_hashes, ctx=None):
+    """Double-chain proofs (stark_prove_p256_chains_batch): the
+    run_double_chain_streaming pipeline (p256_full_ecdsa_stark_bench.rs:
+    133-240) per chain input; pi_hash = SHA3-256(public_inputs_hash || root_f0)."""
+    ctx = ctx or default_context()
+    chains = list(chains)
+    assert chains and len(public_inputs_hashes) == len(chains)
+    xs = [_p256_chains(c) for c in chains]
+    arr = (_P256Chains * len(xs))(*xs)
+    outs = (_P256Out * len(xs))()
+    ctx._check(lib().stark_prove_p256_chains_batch(ctx.h, arr, len(xs), n_trace,
+                                                   b"".join(bytes(x) for x in public_inputs_hashes), outs))
+    return _p256_outs(outs)
+
+
+@dataclass
+class EcdsaRecord:
+    """An ECDSA-P256 signature record: public key (qx, qy), signature (r, s)
+    as integers, and the 32-byte SHA-256 digest of the signed message."""
+    qx: int
+    qy: int
+    r: int
+    s: int
+    digest: bytes
+
+
+def ecdsa_record_pk_hash(rec: EcdsaRecord):
+    """pk_hash_pre of prove_record_ecdsa (mixed_algorithm_zone_rollup.rs:
+    293-302): SHA3-256("ECDSAP256-MIXEDZONE" || qx || qy || r || s || digest)."""
+    import hashlib
+    h = hashlib.sha3_256(b"ECDSAP256-MIXEDZONE")
+    for v in (rec.qx, rec.qy, rec.r, rec.s):
+        h.update(int(v).to_bytes(32, "big"))
+    h.update(bytes(rec.digest))
+    return h.digest()
+
+
+def prove_record_ecdsa_batch(records, ctx=None):
+    """prove_record_ecdsa (mixed_algorithm_zone_rollup.rs:193-322) from the
+    signature on, one proof per record (stark_prove_ecdsa_p256_batch):
+    raises StarkError for a key off the curve or a signature that does not
+    verify, where the example asserts."""
+    ctx = ctx or default_context()
+    records = list(records)
+    assert records
+
+    def cat(f):
+        return b"".join(f(r) for r in records)
+    outs = (_P256Out * len(records))()
+    ctx._check(lib().stark_prove_ecdsa_p256_batch(
+        ctx.h, cat(lambda r: int(r.qx).to_bytes(32, "big")), cat(lambda r: int(r.qy).to_bytes(32, "big")),
+        cat(lambda r: int(r.r).to_bytes(32, "big")), cat(lambda r: int(r.s).to_bytes(32, "big")),
+        cat(lambda r: bytes(r.digest)), len(records), outs))
+    return _p256_outs(outs)
 
 
 # --------------------------------------------------------------------- FRI

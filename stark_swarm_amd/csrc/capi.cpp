@@ -14,6 +14,7 @@
 #include "gl.hpp"
 #include "keccak.hpp"
 #include "ntt.hpp"
+#include "p256_air.hpp"
 #include "prover.hpp"
 #include "rsa_air.hpp"
 #include "server.hpp"
@@ -318,6 +319,78 @@ int stark_merge_rsa_exp(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, siz
         h2d(c, dl.p, lde, 8ULL * w * n0);
         sg::merge_rsa_batch(c, dl.as<uint64_t>(), 1, (uint32_t)n_records, n_trace, blowup, dc.as<uint64_t>());
         d2h(c, c_eval, dc.p, 8 * n0);
+    });
+}
+
+static sg::P256Chains p256_chains(const stark_p256_chains &x) {
+    sg::P256Chains c;
+    SG_REQUIRE((x.a_bits || !x.k_a) && (x.b_bits || !x.k_b), "p256: null bits");
+    SG_REQUIRE(x.k_a <= (1u << 16) && x.k_b <= (1u << 16), "p256: too many bits");
+    for (int i = 0; i < 12; i++) memcpy(c.pts[i].data(), x.pts[i], 32);
+    for (int i = 0; i < 6; i++) memcpy(c.rproj[i].data(), x.rproj[i], 32);
+    c.a_bits.assign(x.a_bits, x.a_bits + x.k_a);
+    c.b_bits.assign(x.b_bits, x.b_bits + x.k_b);
+    c.bind_outputs = x.bind_outputs != 0;
+    return c;
+}
+
+static void p256_outs(const std::vector<sg::P256Proof> &r, stark_p256_out *outs) {
+    for (size_t i = 0; i < r.size(); i++) {
+        memcpy(outs[i].pi_hash, r[i].pi_hash, 32);
+        memcpy(outs[i].root_f0, r[i].root_f0, 32);
+        for (int j = 0; j < 6; j++) memcpy(outs[i].rproj[j], r[i].rproj[j].data(), 32);
+        outs[i].n_trace = r[i].n_trace;
+        outs[i].proof = copy_out(r[i].proof, &outs[i].proof_len);
+    }
+}
+
+int stark_merge_ecdsa_p256(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, size_t blowup, const uint64_t *coeffs,
+                           size_t k, uint64_t *c_eval) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(lde && coeffs && c_eval, "merge_ecdsa_p256: null argument");
+        SG_REQUIRE(k == sg::P256_K, "merge_ecdsa_p256: need one coefficient per constraint (155774)");
+        for (size_t j = 0; j < k; j++)
+            SG_REQUIRE(coeffs[j] == j + 1, "merge_ecdsa_p256: only the reference combination coefficients (j + 1)");
+        SG_REQUIRE(n_trace <= (1u << 16) && blowup <= (1u << 10), "merge_ecdsa_p256: sizes out of range");
+        const size_t n0 = n_trace * blowup;
+        sg::Scoped dl(c, 8ULL * sg::P256_W * n0), dc(c, 8 * n0);
+        h2d(c, dl.p, lde, 8ULL * sg::P256_W * n0);
+        sg::merge_p256_batch(c, dl.as<uint64_t>(), 1, n_trace, blowup, dc.as<uint64_t>());
+        d2h(c, c_eval, dc.p, 8 * n0);
+    });
+}
+
+int stark_prove_p256_chains_batch(stark_ctx *ctx, const stark_p256_chains *in, size_t n_proofs, size_t n_trace,
+                                  const uint8_t *public_inputs_hashes, stark_p256_out *outs) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(in && public_inputs_hashes && outs, "prove_p256_chains_batch: null argument");
+        SG_REQUIRE(n_proofs >= 1 && n_proofs <= 65535, "prove_p256_chains_batch: 1 to 65535 proofs");
+        for (size_t i = 0; i < n_proofs; i++) outs[i].proof = nullptr;
+        std::vector<sg::P256Chains> ch(n_proofs);
+        std::vector<std::array<uint8_t, 32>> pis(n_proofs);
+        for (size_t i = 0; i < n_proofs; i++) {
+            ch[i] = p256_chains(in[i]);
+            memcpy(pis[i].data(), public_inputs_hashes + 32 * i, 32);
+        }
+        p256_outs(sg::prove_p256_batch(c, ch, n_trace, pis), outs);
+    });
+}
+
+int stark_prove_ecdsa_p256_batch(stark_ctx *ctx, const uint8_t *qx, const uint8_t *qy, const uint8_t *r,
+                                 const uint8_t *s, const uint8_t *digest, size_t n_records, stark_p256_out *outs) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(qx && qy && r && s && digest && outs, "prove_ecdsa_p256_batch: null argument");
+        SG_REQUIRE(n_records >= 1 && n_records <= 65535, "prove_ecdsa_p256_batch: 1 to 65535 records");
+        for (size_t i = 0; i < n_records; i++) outs[i].proof = nullptr;
+        std::vector<sg::P256Record> recs(n_records);
+        for (size_t i = 0; i < n_records; i++) {
+            memcpy(recs[i].qx.data(), qx + 32 * i, 32);
+            memcpy(recs[i].qy.data(), qy + 32 * i, 32);
+            memcpy(recs[i].r.data(), r + 32 * i, 32);
+            memcpy(recs[i].s.data(), s + 32 * i, 32);
+            memcpy(recs[i].digest.data(), digest + 32 * i, 32);
+        }
+        p256_outs(sg::prove_ecdsa_p256_batch(c, recs), outs);
     });
 }
 
@@ -678,6 +751,22 @@ int stark_host_build_rsa_exp_trace(const uint8_t *n_be, const uint8_t *s_be, con
 }
 
 void stark_host_rsa_pkcs1_em_sha256(const uint8_t digest[32], uint8_t em[256]) { sg::rsa_pkcs1_em_sha256(digest, em); }
+
+int stark_host_build_ecdsa_p256_trace(stark_p256_chains *in, size_t n_trace, uint64_t *trace) {
+    try {
+        SG_REQUIRE(in && trace, "p256 trace: null argument");
+        std::array<sg::Be32, 6> rp;
+        sg::build_p256_trace_host(p256_chains(*in), n_trace, trace, &rp);
+        for (int j = 0; j < 6; j++) memcpy(in->rproj[j], rp[j].data(), 32);
+        return STARK_OK;
+    } catch (const sg::Error &e) {
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        return STARK_E_OOM;
+    } catch (const std::exception &) {
+        return STARK_E_INTERNAL;
+    }
+}
 
 void stark_host_dns_leaf_hash(const stark_dns_record *r, const uint8_t salt[16], uint8_t out[32]) {
     sg::DnsRec d{r->domain, r->domain_len, r->record_type, r->ttl, r->rdata, r->rdata_len};

@@ -81,6 +81,36 @@ SG_HD uint64_t inv(uint64_t a) { return pow(a, P - 2); }
 // omega_n = 7^((p-1)/n), the generator ark-poly's radix-2 domain uses
 inline uint64_t root_of_unity(uint64_t n) { return pow(7, (P - 1) / n); }
 
+// 128-bit lazy sum of field terms, reduced once (a sum of up to 2^46
+// terms below 2^82 cannot wrap)
+struct Acc {
+    uint64_t lo, hi;
+};
+SG_HD void acc_add(Acc &a, uint64_t x) {
+    a.lo += x;
+    a.hi += a.lo < x;
+}
+// a += x * k, k < 2^32
+SG_HD void acc_mad(Acc &a, uint64_t x, uint32_t k) {
+    const uint64_t p0 = (uint64_t)(uint32_t)x * k, p1 = (x >> 32) * (uint64_t)k;
+    const uint64_t lo = p0 + (p1 << 32);
+    const uint64_t hi = (p1 >> 32) + (lo < p0);
+    a.lo += lo;
+    a.hi += hi + (a.lo < lo);
+}
+// a += x << b, b < 64 (compile-time in unrolled loops)
+SG_HD void acc_shl(Acc &a, uint64_t x, int b) {
+    const uint64_t lo = x << b, hi = b ? x >> (64 - b) : 0;
+    a.lo += lo;
+    a.hi += hi + (a.lo < lo);
+}
+SG_HD uint64_t acc_val(const Acc &a) { return reduce128(a.lo, a.hi); }
+SG_HD uint64_t mul_small(uint64_t x, uint32_t k) {
+    Acc a{0, 0};
+    acc_mad(a, x, k);
+    return acc_val(a);
+}
+
 struct fp3 { uint64_t c0, c1, c2; };
 struct fp6 { uint64_t c[6]; };
 

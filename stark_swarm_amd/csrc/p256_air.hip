@@ -1,0 +1,1063 @@
+// p256_air.hip -- double-chain ECDSA-P256 AIR for gfx950.
+//
+// Trace (crates/deep_ali/src/p256_ecdsa_double_multirow_air.rs:62-222 over
+// p256_scalar_mul_air.rs:74-220, p256_group_air.rs:111-1046 and
+// p256_field_air.rs:125-1531): each row holds, for each of two chains, one
+// step acc' = bit ? 2 acc + base : 2 acc as an RCB-2016 doubling (13 mul,
+// 15 add, 6 sub gadgets) and addition (14 mul, 20 add, 9 sub), every add and
+// sub followed by a freeze to [0, p), and three selects. A field element is
+// 10 limbs of 26 bits; most cells are the bits of those limbs and of the
+// 36-bit biased carries of the mul gadgets' limb identities
+// a b - q p - c = 0. The host computes the limbs, quotients and carries of
+// every gadget (the "witness words", about 8.6 k per row against 151 k
+// cells); `expand_p256_k` turns them into cells through a per-column
+// descriptor (word, and which bit / value / carry sign of it).
+//
+// Composition (eval_ecdsa_double_multirow_per_row, :225-307) with
+// lambda_j = j + 1: one wave per (64 LDE points, sub-gadget); the lane is the
+// point, so every column read is a coalesced 512-byte run. Each wave sums its
+// gadget's weighted constraints lazily in 128 bits: linear terms as
+// lambda * cell, booleanity as lambda * (x - x^2); the 19 position identities
+// of a mul gadget are regrouped with lambda affine in the position k,
+//   sum_k lambda_k sum_{i+j=k} a_i b_j = sum_i a_i ((L0 + i) B + B1),
+//   B = sum_j b_j, B1 = sum_j j b_j,
+// the q p products likewise with the constant limbs of p, and the carry
+// terms as sum_m sc_m (lambda_{m+1} - 2^26 lambda_m). Per-gadget partial
+// sums are added per point by `p256_sum_k`, then the Z_H quotient.
+#include <string.h>
+
+#include "air.hpp"
+#include "bigint.hpp"
+#include "gl.hpp"
+#include "keccak.hpp"
+#include "p256_air.hpp"
+
+namespace sg {
+
+namespace p256 {
+
+constexpr int NL = 10, LB = 26, EC = 270, MCB = 36, MCP = 18, MSP = 19;
+constexpr int64_t MC_OFF = 1LL << 35;
+// p in 26-bit limbs (P_LIMBS_TIGHT, p256_field.rs:224-235)
+constexpr int64_t PL[NL] = {0x3FFFFFF, 0x3FFFFFF, 0x3FFFFFF, 0x003FFFF, 0, 0, 0, 0x0000400, 0x3FF0000, 0x03FFFFF};
+constexpr uint64_t SP = 0x3FFFFFF * 3 + 0x003FFFF + 0x0000400 + 0x3FF0000 + 0x03FFFFF;  // sum_j p_j
+constexpr uint64_t SP1 = 0x3FFFFFF * (0 + 1 + 2) + 0x003FFFF * 3 + 0x0000400 * 7 + 0x3FF0000 * 8 +
+                         0x03FFFFF * 9;  // sum_j j p_j
+constexpr uint32_t P_W[8] = {0xFFFFFFFF, 0xFFFFFFFF, 0xFFFFFFFF, 0, 0, 0, 1, 0xFFFFFFFF};
+constexpr uint32_t N_W[8] = {0xFC632551, 0xF3B9CAC2, 0xA7179E84, 0xBCE6FAAD, 0xFFFFFFFF, 0xFFFFFFFF, 0, 0xFFFFFFFF};
+constexpr uint8_t B_BE[32] = {0x5a, 0xc6, 0x35, 0xd8, 0xaa, 0x3a, 0x93, 0xe7, 0xb3, 0xeb, 0xbd,
+                              0x55, 0x76, 0x98, 0x86, 0xbc, 0x65, 0x1d, 0x06, 0xb0, 0xcc, 0x53,
+                              0xb0, 0xf6, 0x3b, 0xce, 0x3c, 0x3e, 0x27, 0xd2, 0x60, 0x4b};
+constexpr uint8_t GX_BE[32] = {0x6b, 0x17, 0xd1, 0xf2, 0xe1, 0x2c, 0x42, 0x47, 0xf8, 0xbc, 0xe6,
+                               0xe5, 0x63, 0xa4, 0x40, 0xf2, 0x77, 0x03, 0x7d, 0x81, 0x2d, 0xeb,
+                               0x33, 0xa0, 0xf4, 0xa1, 0x39, 0x45, 0xd8, 0x98, 0xc2, 0x96};
+constexpr uint8_t GY_BE[32] = {0x4f, 0xe3, 0x42, 0xe2, 0xfe, 0x1a, 0x7f, 0x9b, 0x8e, 0xe7, 0xeb,
+                               0x4a, 0x7c, 0x0f, 0x9e, 0x16, 0x2b, 0xce, 0x33, 0x57, 0x6b, 0x31,
+                               0x5e, 0xce, 0xcb, 0xb6, 0x40, 0x68, 0x37, 0xbf, 0x51, 0xf5};
+
+enum : uint32_t { GM, GA, GS, GF, GSEL, GTAIL };
+// cell descriptor: word << 7 | code; code < 64 is a bit of the word
+enum : uint32_t { D_VAL = 64, D_POS = 65, D_NEG = 66 };
+
+// device gadget table entry (constraint order); for a freeze q / qb are the
+// diff limbs / bits, for an add pos holds the carries
+struct GDesc {
+    uint32_t kind, s, a, b, c, cb, q, qb, cy, pos, neg, sel;
+};
+// boundary and transitions (p256_ecdsa_double_multirow_air.rs:243-304)
+struct TailDesc {
+    uint32_t s, selc[6], rproj[6], acc[6];
+};
+
+// ---------------------------------------------------------------- layout
+
+// RCB-2016 programs as written in p256_group_air.rs: op, destination
+// register, two source registers; every add and sub is followed by a freeze
+// whose output the destination names.
+struct Op {
+    uint8_t op, d, a, b;
+};
+// doubling (alg. 6, p256_group_air.rs:364-438): X Y Z = 0..2, b = 3,
+// t0..t3 = 4..7, x3 = 8, y3 = 9, z3 = 10
+constexpr Op DBL[34] = {
+    {GM, 4, 0, 0},   {GM, 5, 1, 1},    {GM, 6, 2, 2},    {GM, 7, 0, 1},    {GA, 7, 7, 7},    {GM, 10, 0, 2},
+    {GA, 10, 10, 10}, {GM, 9, 3, 6},   {GS, 9, 9, 10},   {GA, 8, 9, 9},    {GA, 9, 8, 9},    {GS, 8, 5, 9},
+    {GA, 9, 5, 9},   {GM, 9, 8, 9},    {GM, 8, 8, 7},    {GA, 7, 6, 6},    {GA, 6, 6, 7},    {GM, 10, 3, 10},
+    {GS, 10, 10, 6}, {GS, 10, 10, 4},  {GA, 7, 10, 10},  {GA, 10, 10, 7},  {GA, 7, 4, 4},    {GA, 4, 7, 4},
+    {GS, 4, 4, 6},   {GM, 4, 4, 10},   {GA, 9, 9, 4},    {GM, 4, 1, 2},    {GA, 4, 4, 4},    {GM, 10, 4, 10},
+    {GS, 8, 8, 10},  {GM, 10, 4, 5},   {GA, 10, 10, 10}, {GA, 10, 10, 10}};
+// addition (alg. 4, p256_group_air.rs:745-850): X1 Y1 Z1 = 0..2,
+// X2 Y2 Z2 = 3..5, b = 6, t0..t4 = 7..11, x3 = 12, y3 = 13, z3 = 14
+constexpr Op ADD[43] = {
+    {GM, 7, 0, 3},    {GM, 8, 1, 4},    {GM, 9, 2, 5},    {GA, 10, 0, 1},   {GA, 11, 3, 4},   {GM, 10, 10, 11},
+    {GA, 11, 7, 8},   {GS, 10, 10, 11}, {GA, 11, 1, 2},   {GA, 12, 4, 5},   {GM, 11, 11, 12}, {GA, 12, 8, 9},
+    {GS, 11, 11, 12}, {GA, 12, 0, 2},   {GA, 13, 3, 5},   {GM, 12, 12, 13}, {GA, 13, 7, 9},   {GS, 13, 12, 13},
+    {GM, 14, 6, 9},   {GS, 12, 13, 14}, {GA, 14, 12, 12}, {GA, 12, 12, 14}, {GS, 14, 8, 12},  {GA, 12, 8, 12},
+    {GM, 13, 6, 13},  {GA, 8, 9, 9},    {GA, 9, 8, 9},    {GS, 13, 13, 9},  {GS, 13, 13, 7},  {GA, 8, 13, 13},
+    {GA, 13, 8, 13},  {GA, 8, 7, 7},    {GA, 7, 8, 7},    {GS, 7, 7, 9},    {GM, 8, 11, 13},  {GM, 9, 7, 13},
+    {GM, 13, 12, 14}, {GA, 13, 13, 9},  {GM, 12, 10, 12}, {GS, 12, 12, 8},  {GM, 14, 11, 14}, {GM, 8, 10, 7},
+    {GA, 14, 14, 8}};
+
+struct Gad {
+    uint32_t kind, a, b, c, cb, q, qb, cy, pos, neg, sel, w0;  // columns; w0 = first witness word
+};
+struct Group {
+    const Op *prog = nullptr;
+    int n_ops = 0, n_in = 0;
+    std::vector<Gad> g;       // allocation order (an add / sub, then its freeze)
+    std::vector<int> op_gad;  // first gadget of each op
+    uint32_t reg[16] = {};    // register columns after the layout pass
+};
+struct Step {
+    uint32_t acc[3], base[3], bit, w_acc, w_base, w_bit;
+    Group dbl, add;
+    Gad sel[3];
+};
+struct Layout {
+    Step ch[2];
+    uint32_t rproj[6], w_rproj, w_b, n_words;
+    std::vector<uint32_t> desc;  // one per column
+    std::vector<GDesc> gd;       // both chains, constraint order
+    TailDesc tail;
+};
+
+struct Builder {
+    Layout &L;
+    uint32_t words = 0;
+    uint32_t col() const { return (uint32_t)L.desc.size(); }
+    void val(uint32_t w) { L.desc.push_back(w << 7 | D_VAL); }
+    // an element block: 10 limb cells, then the 26 bits of each limb
+    uint32_t elem(uint32_t w) {
+        const uint32_t c0 = col();
+        for (int i = 0; i < NL; i++) val(w + i);
+        for (int i = 0; i < NL; i++)
+            for (int b = 0; b < LB; b++) L.desc.push_back((w + i) << 7 | b);
+        return c0;
+    }
+    void signed_cells(uint32_t w, Gad &g) {
+        g.pos = col();
+        for (int k = 0; k < NL; k++) L.desc.push_back((w + k) << 7 | D_POS);
+        g.neg = col();
+        for (int k = 0; k < NL; k++) L.desc.push_back((w + k) << 7 | D_NEG);
+    }
+    // alloc_mul_layout (p256_group_air.rs:111-137): c, q, carry bits
+    Gad mul(uint32_t a, uint32_t b) {
+        Gad g{GM, a, b};
+        g.w0 = words;
+        words += 2 * NL + MCP;
+        g.c = elem(g.w0);
+        g.cb = g.c + NL;
+        g.q = elem(g.w0 + NL);
+        g.qb = g.q + NL;
+        g.cy = col();
+        for (int k = 0; k < MCP; k++)
+            for (int b = 0; b < MCB; b++) L.desc.push_back((g.w0 + 2 * NL + k) << 7 | b);
+        return g;
+    }
+    // alloc_add_layout (:139-161): c, carries
+    Gad addg(uint32_t a, uint32_t b) {
+        Gad g{GA, a, b};
+        g.w0 = words;
+        words += 2 * NL;
+        g.c = elem(g.w0);
+        g.cb = g.c + NL;
+        g.pos = col();
+        for (int k = 0; k < NL; k++) val(g.w0 + NL + k);
+        return g;
+    }
+    // alloc_sub_layout (:163-187): c, positive and negative carries
+    Gad subg(uint32_t a, uint32_t b) {
+        Gad g{GS, a, b};
+        g.w0 = words;
+        words += 2 * NL;
+        g.c = elem(g.w0);
+        g.cb = g.c + NL;
+        signed_cells(g.w0 + NL, g);
+        return g;
+    }
+    // alloc_freeze_layout (:189-216): diff, c, positive and negative carries
+    Gad freeze(uint32_t a) {
+        Gad g{GF, a, 0};
+        g.w0 = words;
+        words += 3 * NL;
+        g.q = elem(g.w0);
+        g.qb = g.q + NL;
+        g.c = elem(g.w0 + NL);
+        g.cb = g.c + NL;
+        signed_cells(g.w0 + 2 * NL, g);
+        return g;
+    }
+    // build_group_double_layout / build_group_add_layout: the b block, then
+    // the program's gadgets in order
+    void group(Group &G, const Op *prog, int n_ops, const uint32_t *in, int n_in) {
+        G.prog = prog;
+        G.n_ops = n_ops;
+        G.n_in = n_in;
+        for (int i = 0; i < n_in; i++) G.reg[i] = in[i];
+        G.reg[n_in] = elem(L.w_b);
+        for (int k = 0; k < n_ops; k++) {
+            const Op o = prog[k];
+            G.op_gad.push_back((int)G.g.size());
+            if (o.op == GM) {
+                G.g.push_back(mul(G.reg[o.a], G.reg[o.b]));
+                G.reg[o.d] = G.g.back().c;
+            } else {
+                G.g.push_back(o.op == GA ? addg(G.reg[o.a], G.reg[o.b]) : subg(G.reg[o.a], G.reg[o.b]));
+                G.g.push_back(freeze(G.g.back().c));
+                G.reg[o.d] = G.g.back().c;
+            }
+        }
+    }
+    // build_ecdsa_double_multirow_layout (p256_ecdsa_double_multirow_air.rs:
+    // 62-117) / build_scalar_mul_step_layout (p256_scalar_mul_air.rs:74-155)
+    void step(Step &S) {
+        S.w_acc = words;
+        words += 3 * NL;
+        S.w_base = words;
+        words += 3 * NL;
+        S.w_bit = words++;
+        for (int i = 0; i < 3; i++) {
+            S.acc[i] = col();
+            for (int l = 0; l < NL; l++) val(S.w_acc + NL * i + l);
+        }
+        for (int i = 0; i < 3; i++) {
+            S.base[i] = col();
+            for (int l = 0; l < NL; l++) val(S.w_base + NL * i + l);
+        }
+        S.bit = col();
+        val(S.w_bit);
+        group(S.dbl, DBL, 34, S.acc, 3);
+        const uint32_t two[3] = {S.dbl.reg[8], S.dbl.reg[9], S.dbl.reg[10]};
+        const uint32_t in[6] = {two[0], two[1], two[2], S.base[0], S.base[1], S.base[2]};
+        group(S.add, ADD, 43, in, 6);
+        for (int i = 0; i < 3; i++) {
+            Gad g{GSEL, S.add.reg[12 + i], two[i]};
+            g.w0 = words;
+            words += NL;
+            g.c = elem(g.w0);
+            g.cb = g.c + NL;
+            g.sel = S.bit;
+            S.sel[i] = g;
+        }
+    }
+};
+
+uint32_t gad_constraints(uint32_t kind) {
+    switch (kind) {
+    case GM: return 2 * EC + MSP + MCP * MCB;
+    case GA: return EC + 2 * NL;
+    case GS: return EC + 4 * NL + 1;
+    case GF: return 2 * EC + 5 * NL + 1;
+    default: return EC + NL + 1;
+    }
+}
+
+// eval_group_*_gadget order: muls, adds, subs, freezes after adds, freezes
+// after subs
+void group_order(const Group &G, std::vector<const Gad *> &out) {
+    for (int kind = 0; kind < 5; kind++)
+        for (size_t i = 0; i < G.g.size(); i++) {
+            const Gad &g = G.g[i];
+            const bool take = kind < 3 ? g.kind == (kind == 0 ? GM : kind == 1 ? GA : GS)
+                                       : g.kind == GF && G.g[i - 1].kind == (kind == 3 ? GA : GS);
+            if (take) out.push_back(&g);
+        }
+}
+
+const Layout &layout() {
+    static const Layout *L = [] {
+        Layout *l = new Layout();
+        l->desc.reserve(P256_W);
+        Builder b{*l};
+        l->w_b = b.words;
+        b.words += NL;
+        b.step(l->ch[0]);
+        b.step(l->ch[1]);
+        l->w_rproj = b.words;
+        b.words += 6 * NL;
+        for (int i = 0; i < 6; i++) {
+            l->rproj[i] = b.col();
+            for (int k = 0; k < NL; k++) b.val(l->w_rproj + NL * i + k);
+        }
+        l->n_words = b.words;
+        uint32_t s = 0;
+        for (int c = 0; c < 2; c++) {
+            std::vector<const Gad *> ord;
+            group_order(l->ch[c].dbl, ord);
+            group_order(l->ch[c].add, ord);
+            for (int i = 0; i < 3; i++) ord.push_back(&l->ch[c].sel[i]);
+            for (const Gad *g : ord) {
+                l->gd.push_back(GDesc{g->kind, s, g->a, g->b, g->c, g->cb, g->q, g->qb, g->cy, g->pos, g->neg, g->sel});
+                s += gad_constraints(g->kind);
+            }
+        }
+        l->tail.s = s;
+        for (int c = 0; c < 2; c++)
+            for (int i = 0; i < 3; i++) {
+                l->tail.selc[3 * c + i] = l->ch[c].sel[i].c;
+                l->tail.acc[3 * c + i] = l->ch[c].acc[i];
+            }
+        for (int i = 0; i < 6; i++) l->tail.rproj[i] = l->rproj[i];
+        if (l->desc.size() != P256_W || s + 180 != P256_K || l->gd.size() != 260)
+            throw Error(E_INTERNAL, "p256 layout: sizes differ from the reference");
+        return l;
+    }();
+    return *L;
+}
+
+// ---------------------------------------------------------------- host rows
+
+using Limbs = std::array<int64_t, NL>;
+
+void be_to_words(const uint8_t *be, uint32_t w[8]) {
+    for (int i = 0; i < 8; i++)
+        w[i] = (uint32_t)be[31 - 4 * i] | (uint32_t)be[30 - 4 * i] << 8 | (uint32_t)be[29 - 4 * i] << 16 |
+               (uint32_t)be[28 - 4 * i] << 24;
+}
+void words_to_be(const uint32_t w[8], uint8_t *be) {
+    for (int i = 0; i < 8; i++)
+        for (int b = 0; b < 4; b++) be[31 - 4 * i - b] = (uint8_t)(w[i] >> (8 * b));
+}
+// 26-bit limbs of a value held in nw words (bits above 260 dropped)
+Limbs to_limbs(const uint32_t *w, int nw) {
+    Limbs l;
+    for (int i = 0; i < NL; i++) {
+        const int bit = LB * i, k = bit / 32, sh = bit % 32;
+        const uint64_t lo = k < nw ? w[k] : 0, hi = k + 1 < nw ? w[k + 1] : 0;
+        l[i] = (int64_t)(((lo | hi << 32) >> sh) & ((1u << LB) - 1));
+    }
+    return l;
+}
+void from_limbs(const Limbs &l, uint32_t w[9]) {
+    memset(w, 0, 36);
+    for (int i = 0; i < NL; i++) {
+        const int bit = LB * i, k = bit / 32, sh = bit % 32;
+        const uint64_t v = (uint64_t)l[i] << sh;
+        w[k] |= (uint32_t)v;
+        if (k + 1 < 9) w[k + 1] |= (uint32_t)(v >> 32);
+    }
+}
+Limbs be_limbs(const uint8_t *be) {
+    uint32_t w[8];
+    be_to_words(be, w);
+    return to_limbs(w, 8);
+}
+
+// fill_mul_gadget (p256_field_air.rs:953-1042): c = a b mod p, q = (a b - c) / p
+// and the biased carries of a b - q p - c = 0; false if the identity fails
+bool row_mul(const Gad &g, const Limbs &a, const Limbs &b, uint64_t *W, Limbs &c) {
+    uint32_t aw[9], bw[9];
+    from_limbs(a, aw);
+    from_limbs(b, bw);
+    if (aw[8] || bw[8]) return false;  // operands are below p in every program
+    uint32_t prod[16] = {};
+    for (int i = 0; i < 8; i++) {
+        uint64_t carry = 0;
+        for (int j = 0; j < 8; j++) {
+            const uint64_t t = (uint64_t)aw[i] * bw[j] + prod[i + j] + carry;
+            prod[i + j] = (uint32_t)t;
+            carry = t >> 32;
+        }
+        prod[i + 8] = (uint32_t)carry;
+    }
+    int pl = 16;
+    while (pl > 1 && prod[pl - 1] == 0) pl--;
+    uint32_t q[17] = {}, r[8] = {};
+    divmod_words<8>(prod, pl, P_W, 8, q, r);
+    c = to_limbs(r, 8);
+    const Limbs ql = to_limbs(q, 9);
+    for (int i = 0; i < NL; i++) {
+        W[g.w0 + i] = (uint64_t)c[i];
+        W[g.w0 + NL + i] = (uint64_t)ql[i];
+    }
+    int64_t prev = 0;
+    for (int k = 0; k < MSP; k++) {
+        int64_t acc = prev - (k < NL ? c[k] : 0);
+        const int lo = k > NL - 1 ? k - (NL - 1) : 0, hi = k < NL - 1 ? k : NL - 1;
+        for (int i = lo; i <= hi; i++) acc += a[i] * b[k - i] - ql[i] * PL[k - i];
+        if (k == MSP - 1) return acc == 0;
+        if (acc & ((1 << LB) - 1)) return false;
+        const int64_t cy = acc >> LB;  // exact multiple: the shift is the division
+        if (cy >= MC_OFF || cy < -MC_OFF) return false;
+        W[g.w0 + 2 * NL + k] = (uint64_t)(cy + MC_OFF);
+        prev = cy;
+    }
+    return true;
+}
+// fill_add_gadget (:344-402): c = a + b limb by limb, carries 0 / 1
+void row_add(const Gad &g, const Limbs &a, const Limbs &b, uint64_t *W, Limbs &c) {
+    int64_t prev = 0;
+    for (int k = 0; k < NL; k++) {
+        const int64_t s = a[k] + b[k] + prev;
+        c[k] = s & ((1 << LB) - 1);
+        prev = s >> LB;
+        W[g.w0 + k] = (uint64_t)c[k];
+        W[g.w0 + NL + k] = (uint64_t)prev;
+    }
+}
+// Euclidean division by 2^26
+int64_t split26(int64_t x, int64_t &rem) {
+    rem = x & ((1 << LB) - 1);
+    return x >> LB;  // arithmetic shift: the floor
+}
+// fill_sub_gadget (:570-634): c = a + p - b, carries -1 / 0 / 1
+void row_sub(const Gad &g, const Limbs &a, const Limbs &b, uint64_t *W, Limbs &c) {
+    int64_t prev = 0;
+    for (int k = 0; k < NL; k++) {
+        prev = split26(a[k] + PL[k] - b[k] + prev, c[k]);
+        W[g.w0 + k] = (uint64_t)c[k];
+        W[g.w0 + NL + k] = (uint64_t)prev;
+    }
+}
+// fill_freeze_gadget (:1374-1447): diff = a - p; c = a when the last carry is
+// -1 (a < p), else diff
+void row_freeze(const Gad &g, const Limbs &a, uint64_t *W, Limbs &c) {
+    int64_t prev = 0, cy[NL];
+    Limbs d;
+    for (int k = 0; k < NL; k++) {
+        cy[k] = prev = split26(a[k] - PL[k] + prev, d[k]);
+    }
+    for (int k = 0; k < NL; k++) {
+        c[k] = cy[NL - 1] == -1 ? a[k] : d[k];
+        W[g.w0 + k] = (uint64_t)d[k];
+        W[g.w0 + NL + k] = (uint64_t)c[k];
+        W[g.w0 + 2 * NL + k] = (uint64_t)cy[k];
+    }
+}
+// fill_group_double_gadget / fill_group_add_gadget: run the program; reg
+// holds every register afterwards (results x3, y3, z3 at 8..10 for the
+// doubling, 12..14 for the addition)
+bool row_group(const Group &G, const Limbs *in, const Limbs &bconst, uint64_t *W, Limbs *reg) {
+    for (int i = 0; i < G.n_in; i++) reg[i] = in[i];
+    reg[G.n_in] = bconst;
+    for (int k = 0; k < G.n_ops; k++) {
+        const Op o = G.prog[k];
+        const Gad &g = G.g[G.op_gad[k]];
+        Limbs c;
+        if (o.op == GM) {
+            if (!row_mul(g, reg[o.a], reg[o.b], W, c)) return false;
+        } else {
+            Limbs raw;
+            if (o.op == GA) row_add(g, reg[o.a], reg[o.b], W, raw);
+            else row_sub(g, reg[o.a], reg[o.b], W, raw);
+            row_freeze(G.g[G.op_gad[k] + 1], raw, W, c);
+        }
+        reg[o.d] = c;
+    }
+    return true;
+}
+// fill_scalar_mul_step_gadget (p256_scalar_mul_air.rs:174-220); acc <- acc'
+bool row_step(const Step &S, Limbs *acc, const Limbs *base, int bit, const Limbs &bconst, uint64_t *W) {
+    for (int i = 0; i < 3; i++)
+        for (int l = 0; l < NL; l++) {
+            W[S.w_acc + NL * i + l] = (uint64_t)acc[i][l];
+            W[S.w_base + NL * i + l] = (uint64_t)base[i][l];
+        }
+    W[S.w_bit] = (uint64_t)bit;
+    Limbs rd[16], ra[16], in[6];
+    if (!row_group(S.dbl, acc, bconst, W, rd)) return false;
+    for (int i = 0; i < 3; i++) {
+        in[i] = rd[8 + i];
+        in[3 + i] = base[i];
+    }
+    if (!row_group(S.add, in, bconst, W, ra)) return false;
+    for (int i = 0; i < 3; i++) {
+        acc[i] = bit ? ra[12 + i] : rd[8 + i];
+        for (int l = 0; l < NL; l++) W[S.sel[i].w0 + l] = (uint64_t)acc[i][l];
+    }
+    return true;
+}
+
+// All h rows of one proof's chains: words[row * n_words + w]; the r_proj
+// words are the chain outputs (bind_outputs) or the given values.
+void chain_words(const P256Chains &in, uint64_t h, uint64_t *words, std::array<Be32, 6> &rproj) {
+    const Layout &L = layout();
+    const uint64_t nw = L.n_words;
+    const Limbs bconst = be_limbs(B_BE);
+    Limbs out[2][3];
+    for (int c = 0; c < 2; c++) {
+        Limbs acc[3], base[3];
+        for (int i = 0; i < 3; i++) {
+            acc[i] = be_limbs(in.pts[6 * c + i].data());
+            base[i] = be_limbs(in.pts[6 * c + 3 + i].data());
+        }
+        const std::vector<uint8_t> &bits = c ? in.b_bits : in.a_bits;
+        for (uint64_t r = 0; r < h; r++) {
+            const int bit = r < bits.size() ? (bits[r] != 0) : 0;
+            if (!row_step(L.ch[c], acc, base, bit, bconst, words + r * nw))
+                throw Error(E_INVALID, "p256 trace: a field product does not close (inputs must be below p)");
+            if (r + 1 == bits.size())
+                for (int i = 0; i < 3; i++) out[c][i] = acc[i];
+        }
+    }
+    Limbs rp[6];
+    for (int i = 0; i < 6; i++) {
+        if (in.bind_outputs) {
+            rp[i] = out[i / 3][i % 3];
+            uint32_t w[9];
+            from_limbs(rp[i], w);
+            words_to_be(w, rproj[i].data());
+        } else {
+            rp[i] = be_limbs(in.rproj[i].data());
+            rproj[i] = in.rproj[i];
+        }
+    }
+    for (uint64_t r = 0; r < h; r++) {
+        uint64_t *W = words + r * nw;
+        for (int l = 0; l < NL; l++) W[L.w_b + l] = (uint64_t)bconst[l];
+        for (int i = 0; i < 6; i++)
+            for (int l = 0; l < NL; l++) W[L.w_rproj + NL * i + l] = (uint64_t)rp[i][l];
+    }
+}
+
+void check_inputs(const std::vector<P256Chains> &in, uint64_t h) {
+    SG_REQUIRE(is_pow2(h) && h >= 2 && h <= (1u << 16), "p256 trace: height must be a power of two in [2, 2^16]");
+    uint32_t pw[8];
+    for (const P256Chains &x : in) {
+        SG_REQUIRE(x.a_bits.size() <= h && x.b_bits.size() <= h, "p256 trace: more bits than rows");
+        SG_REQUIRE(!x.bind_outputs || (!x.a_bits.empty() && !x.b_bits.empty()),
+                   "p256 trace: binding r_proj to the outputs needs at least one bit per chain");
+        for (const Be32 &v : x.pts) {
+            be_to_words(v.data(), pw);
+            bool below = false;
+            for (int i = 7; i >= 0; i--)
+                if (pw[i] != P_W[i]) {
+                    below = pw[i] < P_W[i];
+                    break;
+                }
+            SG_REQUIRE(below, "p256 trace: point coordinates must be below p");
+        }
+    }
+}
+
+uint64_t cell(uint32_t d, const uint64_t *W) {
+    const uint64_t v = W[d >> 7];
+    const uint32_t code = d & 127;
+    if (code < 64) return (v >> code) & 1;
+    if (code == D_VAL) return v;
+    return code == D_POS ? (uint64_t)(v == 1) : (uint64_t)(v == ~0ULL);
+}
+
+// ---------------------------------------------------------------- device
+
+// one cell per lane, rows fastest: tr_g[col * h + row], g = blockIdx.y
+__global__ __launch_bounds__(256) void expand_p256_k(const uint64_t *words, const uint32_t *desc, uint32_t nw,
+                                                     uint32_t log_h, uint64_t *tr) {
+    const uint64_t g = blockIdx.y, h = 1ULL << log_h;
+    const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= h * P256_W) return;
+    const uint64_t row = t & (h - 1), col = t >> log_h;
+    const uint32_t d = desc[col];
+    const uint64_t v = words[(g * h + row) * nw + (d >> 7)];
+    const uint32_t code = d & 127;
+    uint64_t x;
+    if (code < 64) x = (v >> code) & 1;
+    else if (code == D_VAL) x = v;
+    else x = code == D_POS ? (uint64_t)(v == 1) : (uint64_t)(v == ~0ULL);
+    tr[g * P256_W * h + t] = x;
+}
+
+struct Sums {
+    Acc pos{0, 0}, neg{0, 0}, rpos{0, 0}, rneg{0, 0};  // value = pos - neg - 2^26 (rpos - rneg)
+    __device__ uint64_t value() const {
+        return sub(sub(acc_val(pos), acc_val(neg)), mul(sub(acc_val(rpos), acc_val(rneg)), 1ULL << LB));
+    }
+};
+
+// lambda (x - x^2) for one boolean-constrained cell
+__device__ __forceinline__ void boolean(Sums &S, uint64_t x, uint32_t lam) {
+    acc_mad(S.pos, x, lam);
+    acc_mad(S.neg, mul(x, x), lam);
+}
+
+// range check of one element block at slot s: 260 booleanity slots, then
+// 10 limb packings limb_i - sum_b 2^b bit_ib
+__device__ __forceinline__ void range(Sums &S, const uint64_t *col, uint64_t n0, uint64_t pt, uint32_t lb,
+                                      uint32_t bb, uint32_t s) {
+#pragma unroll 1
+    for (int i = 0; i < NL; i++) {
+        Acc pk{0, 0};
+#pragma unroll
+        for (int b = 0; b < LB; b++) {
+            const uint64_t x = col[(uint64_t)(bb + LB * i + b) * n0 + pt];
+            boolean(S, x, s + LB * i + b + 1);
+            acc_shl(pk, x, b);
+        }
+        acc_mad(S.pos, col[(uint64_t)(lb + i) * n0 + pt], s + EC - NL + i + 1);
+        acc_mad(S.neg, acc_val(pk), s + EC - NL + i + 1);
+    }
+}
+
+// signed carries (pos_k, neg_k) of a sub / freeze chain whose identities sit
+// at slots s0 + k: net_m = pos_m - neg_m enters as carry-in of m + 1 and as
+// -2^26 carry-out of m; then booleanity of pos and neg, exclusion, at s1
+__device__ __forceinline__ void signed_chain(Sums &S, const uint64_t *col, uint64_t n0, uint64_t pt, uint32_t pos,
+                                             uint32_t neg, uint32_t s0, uint32_t s1) {
+#pragma unroll 1
+    for (int m = 0; m < NL; m++) {
+        const uint64_t cp = col[(uint64_t)(pos + m) * n0 + pt], cn = col[(uint64_t)(neg + m) * n0 + pt];
+        if (m + 1 < NL) {
+            acc_mad(S.pos, cp, s0 + m + 2);
+            acc_mad(S.neg, cn, s0 + m + 2);
+        }
+        acc_mad(S.rpos, cp, s0 + m + 1);
+        acc_mad(S.rneg, cn, s0 + m + 1);
+        boolean(S, cp, s1 + m + 1);
+        boolean(S, cn, s1 + NL + m + 1);
+        acc_mad(S.pos, mul(cp, cn), s1 + 2 * NL + m + 1);
+    }
+}
+
+// phi partial sums of one sub-gadget (blockIdx.y < ng) or of the boundary and
+// transitions (blockIdx.y == ng) at 64 points, proof blockIdx.z:
+// part[(k * (ng + 1) + g) * n0 + pt]
+__global__ __launch_bounds__(64) void compose_p256_k(const uint64_t *lde, uint64_t n0, uint64_t blowup,
+                                                    const GDesc *gd, uint32_t ng, TailDesc td, uint64_t *part) {
+    const uint32_t g = blockIdx.y;
+    const uint64_t k = blockIdx.z;
+    const uint64_t pt = (uint64_t)blockIdx.x * 64 + threadIdx.x;
+    const uint64_t *col = lde + k * (uint64_t)P256_W * n0;
+#define AT(c) col[(uint64_t)(c) * n0 + pt]
+    Sums S;
+    if (g == ng) {
+        const uint64_t np = (pt + blowup) & (n0 - 1), n_trace = n0 / blowup, trow = pt / blowup;
+        if (trow + 1 == n_trace) {
+#pragma unroll 1
+            for (int j = 0; j < 6; j++)
+                for (int l = 0; l < NL; l++) {
+                    const uint32_t lam = td.s + NL * j + l + 1;
+                    acc_mad(S.pos, AT(td.selc[j] + l), lam);
+                    acc_mad(S.neg, AT(td.rproj[j] + l), lam);
+                }
+        } else {
+#pragma unroll 1
+            for (int j = 0; j < 6; j++)
+                for (int l = 0; l < NL; l++) {
+                    const uint32_t lam = td.s + 6 * NL + NL * j + l + 1, lam2 = lam + 6 * NL;
+                    acc_mad(S.pos, col[(uint64_t)(td.acc[j] + l) * n0 + np], lam);
+                    acc_mad(S.neg, AT(td.selc[j] + l), lam);
+                    acc_mad(S.pos, col[(uint64_t)(td.rproj[j] + l) * n0 + np], lam2);
+                    acc_mad(S.neg, AT(td.rproj[j] + l), lam2);
+                }
+        }
+    } else {
+        const GDesc d = gd[g];
+        const uint32_t s = d.s;
+        switch (d.kind) {
+        case GM: {  // eval_mul_gadget (p256_field_air.rs:1044-1123)
+            range(S, col, n0, pt, d.c, d.cb, s);
+            range(S, col, n0, pt, d.q, d.qb, s + EC);
+            const uint32_t L0 = s + 2 * EC + 1;  // lambda of position 0
+            uint64_t B = 0, B1 = 0;
+#pragma unroll
+            for (int j = 0; j < NL; j++) {
+                const uint64_t bj = AT(d.b + j);
+                B = add(B, bj);
+                B1 = add(B1, mul_small(bj, j));
+            }
+#pragma unroll 1
+            for (int i = 0; i < NL; i++) {
+                acc_add(S.pos, mul(AT(d.a + i), add(mul_small(B, L0 + i), B1)));
+                // sum_j lambda_{i+j} q_i p_j = q_i ((L0 + i) SP + SP1), below 2^50
+                acc_add(S.neg, mul(AT(d.q + i), (uint64_t)(L0 + i) * SP + SP1));
+                acc_mad(S.neg, AT(d.c + i), L0 + i);
+            }
+#pragma unroll 1
+            for (int m = 0; m < MCP; m++) {
+                Acc sc{0, 0};
+#pragma unroll
+                for (int b = 0; b < MCB; b++) {
+                    const uint64_t y = AT(d.cy + MCB * m + b);
+                    boolean(S, y, s + 2 * EC + MSP + MCB * m + b + 1);
+                    acc_shl(sc, y, b);
+                }
+                const uint64_t gamma = sub(L0 + m + 1, mul((uint64_t)(L0 + m), 1ULL << LB));
+                acc_add(S.pos, mul(sub(acc_val(sc), (uint64_t)MC_OFF), gamma));
+            }
+            break;
+        }
+        case GA: {  // eval_add_gadget (:404-455)
+            range(S, col, n0, pt, d.c, d.cb, s);
+#pragma unroll 1
+            for (int kk = 0; kk < NL; kk++) {
+                const uint32_t lam = s + EC + kk + 1;
+                const uint64_t cy = AT(d.pos + kk);
+                acc_mad(S.pos, AT(d.a + kk), lam);
+                acc_mad(S.pos, AT(d.b + kk), lam);
+                acc_mad(S.neg, AT(d.c + kk), lam);
+                if (kk + 1 < NL) {
+                    acc_mad(S.pos, cy, lam + 1);             // carry-in of kk + 1
+                    boolean(S, cy, s + EC + NL + kk + 1);     // booleanity, kk < 9
+                } else {
+                    acc_mad(S.pos, cy, s + EC + 2 * NL);      // carry[9] = 0
+                }
+                acc_mad(S.rpos, cy, lam);
+            }
+            break;
+        }
+        case GS: {  // eval_sub_gadget (:636-698)
+            range(S, col, n0, pt, d.c, d.cb, s);
+            const uint32_t s0 = s + EC;
+            acc_add(S.pos, (uint64_t)(s0 + 1) * SP + SP1);
+#pragma unroll 1
+            for (int kk = 0; kk < NL; kk++) {
+                acc_mad(S.pos, AT(d.a + kk), s0 + kk + 1);
+                acc_mad(S.neg, AT(d.b + kk), s0 + kk + 1);
+                acc_mad(S.neg, AT(d.c + kk), s0 + kk + 1);
+            }
+            signed_chain(S, col, n0, pt, d.pos, d.neg, s0, s0 + NL);
+            const uint32_t lam = s + EC + 4 * NL + 1;
+            acc_mad(S.pos, AT(d.pos + NL - 1), lam);
+            acc_mad(S.pos, AT(d.neg + NL - 1), lam);
+            break;
+        }
+        case GF: {  // eval_freeze_gadget (:1449-1531)
+            range(S, col, n0, pt, d.q, d.qb, s);
+            range(S, col, n0, pt, d.c, d.cb, s + EC);
+            const uint32_t s0 = s + 2 * EC;
+            acc_add(S.neg, (uint64_t)(s0 + 1) * SP + SP1);
+            Acc M{0, 0};
+            const uint32_t sm = s0 + 4 * NL + 1;  // mux slots
+#pragma unroll 1
+            for (int kk = 0; kk < NL; kk++) {
+                const uint64_t a = AT(d.a + kk), dk = AT(d.q + kk);
+                acc_mad(S.pos, a, s0 + kk + 1);
+                acc_mad(S.neg, dk, s0 + kk + 1);
+                acc_mad(S.pos, AT(d.c + kk), sm + kk + 1);
+                acc_mad(S.neg, dk, sm + kk + 1);
+                acc_mad(M, sub(a, dk), sm + kk + 1);
+            }
+            signed_chain(S, col, n0, pt, d.pos, d.neg, s0, s0 + NL);
+            acc_mad(S.pos, AT(d.pos + NL - 1), s0 + 4 * NL + 1);
+            acc_add(S.neg, mul(AT(d.neg + NL - 1), acc_val(M)));
+            break;
+        }
+        default: {  // eval_select_gadget (:1243-1276)
+            range(S, col, n0, pt, d.c, d.cb, s);
+            Acc M{0, 0};
+#pragma unroll 1
+            for (int kk = 0; kk < NL; kk++) {
+                const uint32_t lam = s + EC + kk + 1;
+                const uint64_t bk = AT(d.b + kk);
+                acc_mad(S.pos, AT(d.c + kk), lam);
+                acc_mad(S.neg, bk, lam);
+                acc_mad(M, sub(AT(d.a + kk), bk), lam);
+            }
+            const uint64_t sel = AT(d.sel);
+            acc_add(S.neg, mul(sel, acc_val(M)));
+            boolean(S, sel, s + EC + NL + 1);
+        }
+        }
+    }
+#undef AT
+    part[(k * (ng + 1) + g) * n0 + pt] = S.value();
+}
+
+// phi[k][pt] = sum over the ng + 1 partial sums
+__global__ __launch_bounds__(256) void p256_sum_k(const uint64_t *part, uint32_t nparts, uint64_t n0,
+                                                 uint64_t *phi) {
+    const uint64_t k = blockIdx.y, pt = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (pt >= n0) return;
+    const uint64_t *p = part + k * nparts * n0 + pt;
+    Acc a{0, 0};
+    for (uint32_t g = 0; g < nparts; g++) acc_add(a, p[(uint64_t)g * n0]);
+    phi[k * n0 + pt] = acc_val(a);
+}
+
+// ---------------------------------------------------------------- ECDSA (host)
+
+// F_p / Z_n arithmetic on 8 little-endian words
+void mulmod(const uint32_t *a, const uint32_t *b, const uint32_t *m, uint32_t *r) {
+    uint32_t prod[16] = {};
+    for (int i = 0; i < 8; i++) {
+        uint64_t carry = 0;
+        for (int j = 0; j < 8; j++) {
+            const uint64_t t = (uint64_t)a[i] * b[j] + prod[i + j] + carry;
+            prod[i + j] = (uint32_t)t;
+            carry = t >> 32;
+        }
+        prod[i + 8] = (uint32_t)carry;
+    }
+    int pl = 16;
+    while (pl > 1 && prod[pl - 1] == 0) pl--;
+    uint32_t q[17], rr[8] = {};
+    divmod_words<8>(prod, pl, m, 8, q, rr);
+    memcpy(r, rr, 32);
+}
+int cmp8(const uint32_t *a, const uint32_t *b) {
+    for (int i = 7; i >= 0; i--)
+        if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
+    return 0;
+}
+bool zero8(const uint32_t *a) {
+    for (int i = 0; i < 8; i++)
+        if (a[i]) return false;
+    return true;
+}
+void addmod(const uint32_t *a, const uint32_t *b, const uint32_t *m, uint32_t *r) {
+    uint32_t t[8];
+    uint64_t c = 0;
+    for (int i = 0; i < 8; i++) {
+        c += (uint64_t)a[i] + b[i];
+        t[i] = (uint32_t)c;
+        c >>= 32;
+    }
+    if (c || cmp8(t, m) >= 0) {
+        uint64_t br = 0;
+        for (int i = 0; i < 8; i++) {
+            const uint64_t d = (uint64_t)t[i] - m[i] - br;
+            t[i] = (uint32_t)d;
+            br = (d >> 63) & 1;
+        }
+    }
+    memcpy(r, t, 32);
+}
+void submod(const uint32_t *a, const uint32_t *b, const uint32_t *m, uint32_t *r) {
+    uint32_t nb[8];
+    uint64_t br = 0;
+    for (int i = 0; i < 8; i++) {
+        const uint64_t d = (uint64_t)m[i] - b[i] - br;
+        nb[i] = (uint32_t)d;
+        br = (d >> 63) & 1;
+    }
+    if (zero8(b)) memset(nb, 0, 32);
+    addmod(a, nb, m, r);
+}
+// a^(m-2) mod m, m prime
+void invmod(const uint32_t *a, const uint32_t *m, uint32_t *r) {
+    uint32_t e[8], acc[8] = {1};
+    memcpy(e, m, 32);
+    e[0] -= 2;
+    for (int bit = 255; bit >= 0; bit--) {
+        mulmod(acc, acc, m, acc);
+        if ((e[bit / 32] >> (bit % 32)) & 1) mulmod(acc, a, m, acc);
+    }
+    memcpy(r, acc, 32);
+}
+
+struct Pt {
+    uint32_t x[8], y[8], z[8];
+};
+// complete projective addition (Renes-Costello-Batina 2016, alg. 4, a = -3);
+// also doubles
+void pt_add(const Pt &P, const Pt &Q, const uint32_t *bc, Pt &R) {
+    const uint32_t *m = P_W;
+    uint32_t t0[8], t1[8], t2[8], t3[8], t4[8], x3[8], y3[8], z3[8];
+    mulmod(P.x, Q.x, m, t0);
+    mulmod(P.y, Q.y, m, t1);
+    mulmod(P.z, Q.z, m, t2);
+    addmod(P.x, P.y, m, t3);
+    addmod(Q.x, Q.y, m, t4);
+    mulmod(t3, t4, m, t3);
+    addmod(t0, t1, m, t4);
+    submod(t3, t4, m, t3);
+    addmod(P.y, P.z, m, t4);
+    addmod(Q.y, Q.z, m, x3);
+    mulmod(t4, x3, m, t4);
+    addmod(t1, t2, m, x3);
+    submod(t4, x3, m, t4);
+    addmod(P.x, P.z, m, x3);
+    addmod(Q.x, Q.z, m, y3);
+    mulmod(x3, y3, m, x3);
+    addmod(t0, t2, m, y3);
+    submod(x3, y3, m, y3);
+    mulmod(bc, t2, m, z3);
+    submod(y3, z3, m, x3);
+    addmod(x3, x3, m, z3);
+    addmod(x3, z3, m, x3);
+    submod(t1, x3, m, z3);
+    addmod(t1, x3, m, x3);
+    mulmod(bc, y3, m, y3);
+    addmod(t2, t2, m, t1);
+    addmod(t1, t2, m, t2);
+    submod(y3, t2, m, y3);
+    submod(y3, t0, m, y3);
+    addmod(y3, y3, m, t1);
+    addmod(t1, y3, m, y3);
+    addmod(t0, t0, m, t1);
+    addmod(t1, t0, m, t0);
+    submod(t0, t2, m, t0);
+    mulmod(t4, y3, m, t1);
+    mulmod(t0, y3, m, t2);
+    mulmod(x3, z3, m, y3);
+    addmod(y3, t2, m, y3);
+    mulmod(t3, x3, m, x3);
+    submod(x3, t1, m, x3);
+    mulmod(t4, z3, m, z3);
+    mulmod(t3, t0, m, t1);
+    addmod(z3, t1, m, z3);
+    memcpy(R.x, x3, 32);
+    memcpy(R.y, y3, 32);
+    memcpy(R.z, z3, 32);
+}
+void pt_mul(const Pt &P, const uint32_t *k, const uint32_t *bc, Pt &R) {
+    Pt acc{};
+    acc.y[0] = 1;  // the identity (0 : 1 : 0)
+    for (int bit = 255; bit >= 0; bit--) {
+        pt_add(acc, acc, bc, acc);
+        if ((k[bit / 32] >> (bit % 32)) & 1) pt_add(acc, P, bc, acc);
+    }
+    R = acc;
+}
+
+}  // namespace p256
+
+using namespace p256;
+
+void p256_pk_hash_pre(const P256Record &rec, uint8_t out[32]) {
+    Sha3 h;
+    h.update("ECDSAP256-MIXEDZONE", 19);
+    h.update(rec.qx.data(), 32);
+    h.update(rec.qy.data(), 32);
+    h.update(rec.r.data(), 32);
+    h.update(rec.s.data(), 32);
+    h.update(rec.digest.data(), 32);
+    h.final(out);
+}
+
+P256Chains p256_ecdsa_chains(const P256Record &rec) {
+    uint32_t r[8], s[8], qx[8], qy[8], e[8], bc[8];
+    be_to_words(rec.r.data(), r);
+    be_to_words(rec.s.data(), s);
+    be_to_words(rec.qx.data(), qx);
+    be_to_words(rec.qy.data(), qy);
+    be_to_words(B_BE, bc);
+    SG_REQUIRE(!zero8(r) && !zero8(s) && cmp8(r, N_W) < 0 && cmp8(s, N_W) < 0, "ecdsa: r and s must be in [1, n)");
+    SG_REQUIRE(cmp8(qx, P_W) < 0 && cmp8(qy, P_W) < 0, "ecdsa: public key coordinates must be below p");
+    {  // Q on y^2 = x^3 - 3x + b
+        uint32_t l[8], t[8];
+        mulmod(qy, qy, P_W, l);
+        mulmod(qx, qx, P_W, t);
+        mulmod(t, qx, P_W, t);
+        for (int i = 0; i < 3; i++) submod(t, qx, P_W, t);
+        addmod(t, bc, P_W, t);
+        SG_REQUIRE(cmp8(l, t) == 0, "ecdsa: public key is not a curve point");
+    }
+    {  // e = digest mod n (one subtraction suffices: 2^256 < 2n)
+        be_to_words(rec.digest.data(), e);
+        if (cmp8(e, N_W) >= 0) {
+            uint64_t br = 0;
+            for (int i = 0; i < 8; i++) {
+                const uint64_t d = (uint64_t)e[i] - N_W[i] - br;
+                e[i] = (uint32_t)d;
+                br = (d >> 63) & 1;
+            }
+        }
+    }
+    uint32_t w[8], u1[8], u2[8];
+    invmod(s, N_W, w);
+    mulmod(e, w, N_W, u1);
+    mulmod(r, w, N_W, u2);
+    Pt G{}, Q{}, A, B, R;
+    be_to_words(GX_BE, G.x);
+    be_to_words(GY_BE, G.y);
+    G.z[0] = 1;
+    memcpy(Q.x, qx, 32);
+    memcpy(Q.y, qy, 32);
+    Q.z[0] = 1;
+    pt_mul(G, u1, bc, A);
+    pt_mul(Q, u2, bc, B);
+    pt_add(A, B, bc, R);
+    SG_REQUIRE(!zero8(R.z), "ecdsa: signature does not verify");
+    uint32_t zi[8], x[8];
+    invmod(R.z, P_W, zi);
+    mulmod(R.x, zi, P_W, x);
+    if (cmp8(x, N_W) >= 0) submod(x, N_W, P_W, x);  // x < p < 2n
+    SG_REQUIRE(cmp8(x, r) == 0, "ecdsa: signature does not verify");
+    P256Chains c;
+    Be32 one{};
+    one[31] = 1;
+    for (int h = 0; h < 2; h++) {
+        memcpy(c.pts[3 * h].data(), GX_BE, 32);
+        memcpy(c.pts[3 * h + 1].data(), GY_BE, 32);
+        c.pts[3 * h + 2] = one;
+        c.pts[6 + 3 * h] = rec.qx;
+        c.pts[6 + 3 * h + 1] = rec.qy;
+        c.pts[6 + 3 * h + 2] = one;
+    }
+    uint8_t b1[32], b2[32];
+    words_to_be(u1, b1);
+    words_to_be(u2, b2);
+    c.a_bits.resize(256);
+    c.b_bits.resize(256);
+    for (int i = 0; i < 256; i++) {
+        c.a_bits[i] = (b1[i / 8] >> (7 - i % 8)) & 1;
+        c.b_bits[i] = (b2[i / 8] >> (7 - i % 8)) & 1;
+    }
+    c.bind_outputs = true;
+    return c;
+}
+
+void build_p256_trace_host(const P256Chains &in, uint64_t h, uint64_t *tr, std::array<Be32, 6> *rproj_out) {
+    check_inputs({in}, h);
+    const Layout &L = layout();
+    std::vector<uint64_t> words((size_t)h * L.n_words, 0);
+    std::array<Be32, 6> rp;
+    chain_words(in, h, words.data(), rp);
+    if (rproj_out) *rproj_out = rp;
+    parallel_for(16, [&](size_t part) {
+        for (uint64_t col = part; col < P256_W; col += 16)
+            for (uint64_t r = 0; r < h; r++) tr[col * h + r] = cell(L.desc[col], words.data() + r * L.n_words);
+    });
+}
+
+void build_p256_traces_dev(Ctx *c, const std::vector<P256Chains> &in, uint64_t h, uint64_t *d_tr,
+                           std::vector<std::array<Be32, 6>> *rproj_out) {
+    const size_t K = in.size();
+    SG_REQUIRE(K >= 1 && K <= 65535, "p256 traces: 1 to 65535 proofs");
+    check_inputs(in, h);
+    const Layout &L = layout();
+    Scoped d_desc(c, 4ULL * P256_W);
+    SG_HIP(hipMemcpyAsync(d_desc.p, L.desc.data(), 4ULL * P256_W, hipMemcpyHostToDevice, c->stream));
+    const uint64_t nw = L.n_words;
+    std::vector<uint64_t> words(K * h * nw, 0);
+    std::vector<std::array<Be32, 6>> rp(K);
+    std::vector<int> bad(K, 0);
+    parallel_for(K, [&](size_t k) {
+        try {
+            chain_words(in[k], h, words.data() + k * h * nw, rp[k]);
+        } catch (...) {
+            bad[k] = 1;
+        }
+    });
+    for (size_t k = 0; k < K; k++)
+        SG_REQUIRE(!bad[k], "p256 trace: a field product does not close (inputs must be below p)");
+    if (rproj_out) *rproj_out = rp;
+    Scoped d_words(c, 8 * words.size());
+    SG_HIP(hipMemcpyAsync(d_words.p, words.data(), 8 * words.size(), hipMemcpyHostToDevice, c->stream));
+    {
+        ProfScope ps(c, "expand_p256", 8.0 * P256_W * h * K);
+        hipLaunchKernelGGL(expand_p256_k, dim3((uint32_t)((h * P256_W + 255) / 256), (uint32_t)K), dim3(256), 0,
+                           c->stream, d_words.as<uint64_t>(), d_desc.as<uint32_t>(), (uint32_t)nw, ilog2(h), d_tr);
+        SG_HIP(hipGetLastError());
+    }
+    SG_HIP(hipStreamSynchronize(c->stream));  // the host word buffer goes out of scope
+}
+
+void merge_p256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup, uint64_t *c_eval) {
+    SG_REQUIRE(is_pow2(n_trace) && n_trace >= 2 && is_pow2(blowup) && blowup >= 2,
+               "merge_p256: n_trace and blowup must be powers of two >= 2");
+    SG_REQUIRE(K >= 1 && K <= 65535, "merge_p256: 1 to 65535 proofs");
+    const uint64_t n = n_trace * blowup;
+    SG_REQUIRE(n >= 64 && n <= (1ULL << 20), "merge_p256: domain must be in [64, 2^20]");
+    const Layout &L = layout();
+    const uint32_t ng = (uint32_t)L.gd.size();
+    Scoped d_gd(c, sizeof(GDesc) * ng);
+    SG_HIP(hipMemcpyAsync(d_gd.p, L.gd.data(), sizeof(GDesc) * ng, hipMemcpyHostToDevice, c->stream));
+    Scoped part(c, 8ULL * (ng + 1) * n * K), phi(c, 8 * n * K);
+    {
+        ProfScope ps(c, "compose_p256", (8.0 * P256_W * n + 8.0 * (ng + 1) * n) * K);
+        hipLaunchKernelGGL(compose_p256_k, dim3((uint32_t)(n / 64), ng + 1, K), dim3(64), 0, c->stream, lde, n,
+                           blowup, d_gd.as<GDesc>(), ng, L.tail, part.as<uint64_t>());
+        SG_HIP(hipGetLastError());
+    }
+    {
+        ProfScope ps(c, "p256_sum", 8.0 * (ng + 2) * n * K);
+        hipLaunchKernelGGL(p256_sum_k, dim3((uint32_t)((n + 255) / 256), K), dim3(256), 0, c->stream,
+                           part.as<uint64_t>(), ng + 1, n, phi.as<uint64_t>());
+        SG_HIP(hipGetLastError());
+    }
+    quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
+}
+
+}  // namespace sg

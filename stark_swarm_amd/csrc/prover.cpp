@@ -516,6 +516,90 @@ std::vector<RsaProof> prove_rsa_batch(Ctx *c, const std::vector<RsaRecord> &reco
     return out;
 }
 
+std::vector<P256Proof> prove_p256_batch(Ctx *c, const std::vector<P256Chains> &in, uint64_t h,
+                                        const std::vector<std::array<uint8_t, 32>> &pis) {
+    const size_t K = in.size();
+    SG_REQUIRE(K >= 1 && K <= 65535, "prove_p256: 1 to 65535 proofs");
+    SG_REQUIRE(pis.size() == K, "prove_p256: one public-input hash per proof");
+    SG_REQUIRE(is_pow2(h) && h >= 2 && h * SWARM_BLOWUP <= (1ULL << 20), "prove_p256: n_trace a power of two >= 2");
+    const uint64_t n0 = h * SWARM_BLOWUP, W = P256_W;
+    size_t free_b = 0, total_b = 0;
+    SG_HIP(hipMemGetInfo(&free_b, &total_b));
+    const uint64_t budget = (uint64_t)((free_b + c->cached_bytes) * 0.6);
+    // trace, LDE input and coefficients, LDE, merge partial sums, FRI
+    const uint64_t per = 8ULL * W * (3 * h + n0) + 8ULL * 300 * n0 + 8ULL * 16 * n0;
+    SG_REQUIRE(per <= budget, "prove_p256: one proof does not fit in device memory");
+    const size_t cap = (size_t)std::min<uint64_t>(4096, budget / per);
+    const size_t nw = (K + cap - 1) / cap, chunk = (K + nw - 1) / nw;
+    std::vector<P256Proof> out(K);
+    for (size_t k0 = 0; k0 < K; k0 += chunk) {
+        const uint32_t G = (uint32_t)std::min(chunk, K - k0);
+        Scoped d_c(c, 8ULL * n0 * G);
+        {
+            std::vector<std::array<Be32, 6>> rp;
+            Scoped d_lde(c, 8ULL * W * n0 * G);
+            {
+                Scoped d_tr(c, 8ULL * W * h * G);
+                {
+                    HostScope hs(c, "p256_trace");
+                    const std::vector<P256Chains> part(in.begin() + k0, in.begin() + k0 + G);
+                    build_p256_traces_dev(c, part, h, d_tr.as<uint64_t>(), &rp);
+                }
+                HostScope hs(c, "p256_lde_merge");
+                lde_columns(c, d_tr.as<uint64_t>(), d_lde.as<uint64_t>(), ilog2(h), ilog2(SWARM_BLOWUP), W * G, h,
+                            n0, 1);
+            }
+            merge_p256_batch(c, d_lde.as<uint64_t>(), G, h, SWARM_BLOWUP, d_c.as<uint64_t>());
+            if (c->profiling) SG_HIP(hipStreamSynchronize(c->stream));
+            for (uint32_t j = 0; j < G; j++) out[k0 + j].rproj = rp[j];
+        }
+        HostScope hf(c, "p256_fri");
+        FriParams p;
+        p.schedule = stir_schedule(n0);
+        p.r = p.s0 = SWARM_QUERIES;
+        p.seed_z = SWARM_SEED_Z;
+        p.coeff_commit_final = true;
+        p.d_final = 1;
+        p.stir = true;
+        p.has_pi = true;
+        std::vector<std::array<uint8_t, 32>> gp(pis.begin() + k0, pis.begin() + k0 + G);
+        std::vector<std::vector<uint8_t>> proofs = deep_fri_prove_batch(c, d_c.as<uint64_t>(), G, n0, p, &gp, nullptr);
+        for (uint32_t j = 0; j < G; j++) {
+            P256Proof &o = out[k0 + j];
+            o.proof = std::move(proofs[j]);
+            o.n_trace = h;
+            memcpy(o.root_f0, o.proof.data(), 32);
+            Sha3 s;
+            s.update(gp[j].data(), 32);
+            s.update(o.root_f0, 32);
+            s.final(o.pi_hash);
+        }
+    }
+    return out;
+}
+
+std::vector<P256Proof> prove_ecdsa_p256_batch(Ctx *c, const std::vector<P256Record> &recs) {
+    const size_t K = recs.size();
+    SG_REQUIRE(K >= 1 && K <= 65535, "prove_ecdsa_p256: 1 to 65535 records");
+    std::vector<P256Chains> in(K);
+    std::vector<std::array<uint8_t, 32>> pis(K);
+    std::vector<std::string> err(K);
+    {
+        HostScope hs(c, "p256_ecdsa_verify");
+        parallel_for(K, [&](size_t k) {
+            try {
+                in[k] = p256_ecdsa_chains(recs[k]);
+            } catch (const std::exception &e) {
+                err[k] = e.what();
+            }
+            p256_pk_hash_pre(recs[k], pis[k].data());
+        });
+    }
+    for (size_t k = 0; k < K; k++)
+        if (!err[k].empty()) throw Error(E_INVALID, "prove_ecdsa_p256: record " + std::to_string(k) + ": " + err[k]);
+    return prove_p256_batch(c, in, P256_ECDSA_ROWS, pis);
+}
+
 std::vector<uint8_t> prove_outer_rollup(Ctx *c, const uint8_t *pi_hashes, size_t n, const uint8_t pk_hash[32],
                                         int phi_mode, uint8_t root_f0[32]) {
     std::vector<uint64_t> leaves;

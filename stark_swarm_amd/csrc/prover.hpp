@@ -7,6 +7,7 @@
 
 #include "ctx.hpp"
 #include "fri.hpp"
+#include "p256_air.hpp"
 #include "rsa_air.hpp"
 
 namespace sg {
@@ -96,6 +97,26 @@ struct RsaProof {
 // the proof of its records alone.
 std::vector<RsaProof> prove_rsa_batch(Ctx *c, const std::vector<RsaRecord> &records, uint32_t R,
                                       uint64_t n_trace, const std::vector<std::array<uint8_t, 32>> &pis);
+// One double-chain ECDSA-P256 proof.
+struct P256Proof {
+    uint8_t pi_hash[32], root_f0[32];
+    std::array<Be32, 6> rproj;  // the r_proj values in the trace
+    size_t n_trace = 0;
+    std::vector<uint8_t> proof;
+};
+// The double-chain proof of run_double_chain_streaming
+// (crates/swarm-dns/examples/p256_full_ecdsa_stark_bench.rs:133-240) for each
+// chain input: trace of n_trace rows -> LDE x32 -> merge (lambda_j = j + 1)
+// -> STIR (54 queries) bound to pis[k]; pi_hash = SHA3-256(pis[k] ||
+// root_f0). Lockstep in HBM-sized waves; result k equals the proof of input
+// k alone.
+std::vector<P256Proof> prove_p256_batch(Ctx *c, const std::vector<P256Chains> &in, uint64_t n_trace,
+                                        const std::vector<std::array<uint8_t, 32>> &pis);
+// prove_record_ecdsa (crates/swarm-dns/examples/mixed_algorithm_zone_rollup.rs:
+// 193-322) for each record: the chains of p256_ecdsa_chains at n_trace 256,
+// bound to the record's pk_hash_pre. Throws E_INVALID for a record whose
+// signature does not verify.
+std::vector<P256Proof> prove_ecdsa_p256_batch(Ctx *c, const std::vector<P256Record> &recs);
 std::vector<uint8_t> prove_outer_rollup(Ctx *c, const uint8_t *pi_hashes, size_t n, const uint8_t pk_hash[32],
                                         int phi_mode, uint8_t root_f0[32]);
 std::vector<uint8_t> prover_core_dev(Ctx *c, const uint64_t *d_trace, uint64_t w, uint64_t n_trace, uint64_t blowup,

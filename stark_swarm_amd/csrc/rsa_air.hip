@@ -30,6 +30,7 @@
 #include <string.h>
 
 #include "air.hpp"
+#include "bigint.hpp"
 #include "gl.hpp"
 #include "rsa_air.hpp"
 
@@ -108,63 +109,8 @@ static int top_words(const uint32_t *w, int nw) {
     return nw;
 }
 
-// u (ul words) = q * v + r for v of vl significant words (v[vl-1] != 0):
-// long division with 32-bit digits and a normalised divisor (Knuth, TAOCP
-// vol. 2, 4.3.1, algorithm D). q gets ul - vl + 1 words, r gets vl words.
 static void divmod(const uint32_t *u, int ul, const uint32_t *v, int vl, uint32_t *q, uint32_t *r) {
-    if (ul < vl) {
-        memset(q, 0, 4);
-        for (int i = 0; i < vl; i++) r[i] = i < ul ? u[i] : 0;
-        return;
-    }
-    if (vl == 1) {
-        uint64_t rem = 0;
-        for (int j = ul - 1; j >= 0; j--) {
-            const uint64_t cur = rem << 32 | u[j];
-            q[j] = (uint32_t)(cur / v[0]);
-            rem = cur % v[0];
-        }
-        r[0] = (uint32_t)rem;
-        return;
-    }
-    const int s = __builtin_clz(v[vl - 1]);
-    uint32_t vn[NW], un[2 * NW + 1];
-    for (int i = vl - 1; i > 0; i--) vn[i] = (uint32_t)(((uint64_t)v[i] << s) | ((uint64_t)v[i - 1] >> (32 - s)));
-    vn[0] = v[0] << s;
-    un[ul] = (uint32_t)((uint64_t)u[ul - 1] >> (32 - s));
-    for (int i = ul - 1; i > 0; i--) un[i] = (uint32_t)(((uint64_t)u[i] << s) | ((uint64_t)u[i - 1] >> (32 - s)));
-    un[0] = u[0] << s;
-    const uint64_t B = 1ULL << 32;
-    for (int j = ul - vl; j >= 0; j--) {
-        const uint64_t num = (uint64_t)un[j + vl] << 32 | un[j + vl - 1];
-        uint64_t qh = num / vn[vl - 1], rh = num % vn[vl - 1];
-        while (qh >= B || qh * vn[vl - 2] > (rh << 32 | un[j + vl - 2])) {
-            qh--;
-            rh += vn[vl - 1];
-            if (rh >= B) break;
-        }
-        int64_t borrow = 0, t;
-        for (int i = 0; i < vl; i++) {
-            const uint64_t p = qh * vn[i];
-            t = (int64_t)un[i + j] - borrow - (int64_t)(p & 0xFFFFFFFFu);
-            un[i + j] = (uint32_t)t;
-            borrow = (int64_t)(p >> 32) - (t >> 32);
-        }
-        t = (int64_t)un[j + vl] - borrow;
-        un[j + vl] = (uint32_t)t;
-        q[j] = (uint32_t)qh;
-        if (t < 0) {  // qh was one too large: add v back
-            q[j]--;
-            uint64_t carry = 0;
-            for (int i = 0; i < vl; i++) {
-                const uint64_t sum = (uint64_t)un[i + j] + vn[i] + carry;
-                un[i + j] = (uint32_t)sum;
-                carry = sum >> 32;
-            }
-            un[j + vl] += (uint32_t)carry;
-        }
-    }
-    for (int i = 0; i < vl; i++) r[i] = (uint32_t)(((uint64_t)un[i] >> s) | ((uint64_t)un[i + 1] << (32 - s)));
+    divmod_words<NW>(u, ul, v, vl, q, r);
 }
 
 // The words of one product row: c = a*b mod n, q = (a*b - c)/n, carries.
@@ -265,35 +211,6 @@ __global__ __launch_bounds__(256) void expand_rsa_k(const RowWords *rows, const 
     if (t >= h * RSA_W) return;
     const uint64_t row = t & (h - 1), col = t / h;
     tr[g * RSA_W * h + t] = cell((uint32_t)col, rows[g * h + row], recs[g]);
-}
-
-// 128-bit lazy sum of field terms
-struct Acc {
-    uint64_t lo, hi;
-};
-__device__ __forceinline__ void acc_add(Acc &a, uint64_t x) {
-    a.lo += x;
-    a.hi += a.lo < x;
-}
-// a += x * k, k < 2^32
-__device__ __forceinline__ void acc_mad(Acc &a, uint64_t x, uint32_t k) {
-    const uint64_t p0 = (uint64_t)(uint32_t)x * k, p1 = (x >> 32) * (uint64_t)k;
-    const uint64_t lo = p0 + (p1 << 32);
-    const uint64_t hi = (p1 >> 32) + (lo < p0);
-    a.lo += lo;
-    a.hi += hi + (a.lo < lo);
-}
-// a += x << b, b < 64 (compile-time in the unrolled loops)
-__device__ __forceinline__ void acc_shl(Acc &a, uint64_t x, int b) {
-    const uint64_t lo = x << b, hi = b ? x >> (64 - b) : 0;
-    a.lo += lo;
-    a.hi += hi + (a.lo < lo);
-}
-__device__ __forceinline__ uint64_t acc_val(const Acc &a) { return reduce128(a.lo, a.hi); }
-__device__ __forceinline__ uint64_t mul_small(uint64_t x, uint32_t k) {
-    Acc a{0, 0};
-    acc_mad(a, x, k);
-    return acc_val(a);
 }
 
 constexpr int SL = 8;  // slices (waves) per workgroup
