@@ -167,12 +167,13 @@ def main():
     ap.add_argument("--shards", type=int, default=128, help="inner-shard proofs for the plain-vs-batched line (0 = skip)")
     ap.add_argument("--dsksk", type=int, default=64, help="DS-KSK (SHA-256 AIR) proofs for the plain-vs-batched line (0 = skip)")
     ap.add_argument("--rsa", type=int, default=64, help="RSA-2048 record proofs for the plain-vs-batched line (0 = skip)")
-    ap.add_argument("--serve", default="250,1000,4000", help="serving: Poisson arrival rates (requests/s), '' = skip")
+    ap.add_argument("--p256", type=int, default=16, help="ECDSA-P256 record proofs for the plain-vs-batched line (0 = skip)")
+    ap.add_argument("--serve", default="200,600,2000", help="serving: Poisson arrival rates (requests/s), '' = skip")
     ap.add_argument("--serve-requests", type=int, default=400, help="serving: requests per rate")
     ap.add_argument("--sha-batch", default="64,10,13",
                     help="batched uneven SHA-256/512 proofs: count,log2 min rows,log2 max rows ('' = skip); "
                          "the full workload is 256,10,17")
-    ap.add_argument("--only", choices=["shards", "ds_ksk", "rsa", "sha_batch", "serve"],
+    ap.add_argument("--only", choices=["shards", "ds_ksk", "rsa", "p256", "sha_batch", "serve"],
                     help="run only this plain-vs-batched line (for profiling it) and print its JSON")
     ap.add_argument("--shards-only", action="store_true",
                     help="run only the inner-shard plain-vs-batched measurement (for profiling that path)")
@@ -195,6 +196,8 @@ def main():
             line = ds_ksk_batch(S, max(1, args.dsksk))
         elif args.only == "rsa":
             line = rsa_batch(S, max(1, args.rsa))
+        elif args.only == "p256":
+            line = p256_batch(S, max(1, args.p256))
         elif args.only == "sha_batch":
             cnt, lo, hi = (int(x) for x in args.sha_batch.split(","))
             line = c4_batch(S, cnt, lo, hi)
@@ -305,6 +308,9 @@ def main():
     if args.rsa:
         log("rsa-2048 records: plain vs batched")
         res.update(rsa_batch(S, args.rsa))
+    if args.p256:
+        log("ecdsa-p256 records: plain vs batched")
+        res.update(p256_batch(S, args.p256))
     if args.sha_batch:
         cnt, lo, hi = (int(x) for x in args.sha_batch.split(","))
         log("sha batch: %d uneven SHA-256/512 proofs, 2^%d..2^%d rows" % (cnt, lo, hi))
@@ -446,6 +452,39 @@ def rsa_batch(S, nproofs):
                     "bit_exact_vs_plain": exact}}
 
 
+def p256_batch(S, nproofs):
+    """ECDSA-P256 signature records (the per-record prover of the
+    mixed-algorithm zone example: two 256-step scalar-multiplication chains,
+    151 334 columns, 155 774 constraints, n_trace 256, n0 = 8192): one proof
+    per call vs one batch call, every batched proof equal to its single-call
+    proof."""
+    from stark_swarm_amd.zone import synthetic_ecdsa_records
+    recs = synthetic_ecdsa_records(nproofs, 0x5EED0008)
+    S.prove_record_ecdsa_batch(recs[:1])  # warm
+    lat, plain = [], []
+    t0 = time.perf_counter()
+    for r in recs:
+        t = time.perf_counter()
+        plain.append(S.prove_record_ecdsa_batch([r])[0].proof_blob)
+        lat.append(time.perf_counter() - t)
+    plain_dt = time.perf_counter() - t0
+    S.prove_record_ecdsa_batch(recs)  # warm: the batch's buffers, once
+    t0 = time.perf_counter()
+    outs = S.prove_record_ecdsa_batch(recs)
+    bdt = time.perf_counter() - t0
+    exact = all(o.proof_blob == p for o, p in zip(outs, plain))
+    if not exact:
+        raise SystemExit("batched ECDSA-P256 proofs differ from the single-call proofs")
+    lat.sort()
+    lde_bytes = 8.0 * 151334 * (256 + 8192)
+    return {"p256": {"n_trace": 256, "n0": 8192, "columns": 151334, "constraints": 155774,
+                     "proofs": nproofs, "plain_proofs_per_s": round(nproofs / plain_dt, 2),
+                     "batched_proofs_per_s": round(nproofs / bdt, 2), "speedup": round(plain_dt / bdt, 3),
+                     "plain_latency_ms_p50": round(1e3 * lat[len(lat) // 2], 3),
+                     "plain_latency_ms_p99": round(1e3 * lat[min(len(lat) - 1, (99 * len(lat)) // 100)], 3),
+                     "lde_bytes_per_proof": lde_bytes, "bit_exact_vs_plain": exact}}
+
+
 def c4_batch(S, nproofs, log_lo, log_hi):
     """Batched uneven SHA-256/512 proofs: nproofs alternating SHA-256 and
     SHA-512 AIR proofs (the prove_ds_ksk_binding pipeline: LDE x32, merge,
@@ -492,11 +531,11 @@ def c4_batch(S, nproofs, log_lo, log_hi):
 def serve(S, rates, per_rate, workers=4, max_batch=32):
     """Serving: RRset proof requests arrive as a Poisson process at each rate
     (requests/s) and go through the proving server (HBM-budget admission,
-    continuous batching over `workers` streams). Request mix: 60 % inner-shard
-    proofs of an RRset group of 8-64 synthetic zone records (HashRollup), 20 %
-    DS->KSK SHA-256 bindings of 44-132 byte DNSKEYs, 20 % RSA-2048 signature
-    records. Latency = completion - arrival. A sample of results is checked
-    byte-equal to the same request proven alone."""
+    continuous batching over `workers` streams). Request mix: 40 % inner-shard
+    proofs of an RRset group of 8-64 synthetic zone records (HashRollup), 10 %
+    DS->KSK SHA-256 bindings of 44-132 byte DNSKEYs, 20 % RSA-2048 and 30 %
+    ECDSA-P256 signature records. Latency = completion - arrival. A sample of
+    results is checked byte-equal to the same request proven alone."""
     import hashlib
     import threading
     from stark_swarm_amd.server import ProvingServer
@@ -504,6 +543,8 @@ def serve(S, rates, per_rate, workers=4, max_batch=32):
     rng = np.random.default_rng(0x5EED0005)
     zone = build_synthetic_zone(10000)
     rsa_pool = rsa_records(S, 64, 0x5EED0007)
+    from stark_swarm_amd.zone import synthetic_ecdsa_records
+    p256_pool = synthetic_ecdsa_records(64, 0x5EED000A)
     salt = b"swarm-test-zone1"
     out = []
     with ProvingServer(workers=workers, hbm_budget=64 << 30, max_batch=max_batch) as srv:
@@ -512,18 +553,22 @@ def serve(S, rates, per_rate, workers=4, max_batch=32):
             srv.wait(rid)
         for rid in [srv.submit_rsa(rsa_pool[0], b"\0" * 32) for _ in range(workers)]:
             srv.wait(rid)
+        for rid in [srv.submit_p256(p256_pool[0]) for _ in range(workers)]:
+            srv.wait(rid)
         for rate in rates:
             n = per_rate
             reqs = []
             for i in range(n):
                 fsb = rng.bytes(32)
                 u = rng.random()
-                if u < 0.6:
+                if u < 0.4:
                     k = int(rng.integers(8, 65))
                     o = int(rng.integers(0, len(zone) - k))
                     reqs.append(("shard", S.PackedShards([zone[o:o + k]]), fsb, (o, k)))
-                elif u < 0.8:
+                elif u < 0.6:
                     reqs.append(("rsa", rsa_pool[int(rng.integers(0, len(rsa_pool)))], fsb, None))
+                elif u < 0.9:
+                    reqs.append(("p256", p256_pool[int(rng.integers(0, len(p256_pool)))], None, None))
                 else:
                     key = rng.bytes(int(rng.integers(44, 133)))
                     reqs.append(("ksk", key, fsb, hashlib.sha256(key).digest()))
@@ -551,6 +596,8 @@ def serve(S, rates, per_rate, workers=4, max_batch=32):
                     ids[j] = srv.submit_inner_shard(salt, x, fsb)
                 elif kind == "rsa":
                     ids[j] = srv.submit_rsa(x, fsb)
+                elif kind == "p256":
+                    ids[j] = srv.submit_p256(x)
                 else:
                     ids[j] = srv.submit_ds_ksk(x, aux, fsb)
             for t in waiters:
@@ -569,6 +616,8 @@ def serve(S, rates, per_rate, workers=4, max_batch=32):
                     alone = S.prove_inner_shard_batch(salt, x, [fsb])[0]
                 elif kind == "rsa":
                     alone = S.prove_rsa_exp_batch([x], [fsb])[0]
+                elif kind == "p256":
+                    alone = S.prove_record_ecdsa_batch([x])[0]
                 else:
                     alone = S.prove_ds_ksk_binding(x, aux, fsb)
                 if alone.proof_blob != res[j].proof_blob or alone.pi_hash != res[j].pi_hash:
@@ -576,8 +625,8 @@ def serve(S, rates, per_rate, workers=4, max_batch=32):
         st = srv.stats()
     return {"serve": {"workers": workers, "max_batch": max_batch, "hbm_budget_GiB": 64,
                       "peak_admitted_GiB": round(st["peak_bytes_admitted"] / 2**30, 3),
-                      "mix": "60% inner-shard RRset groups (8-64 records), 20% DS-KSK SHA-256 (44-132 B keys), "
-                             "20% RSA-2048 signature records",
+                      "mix": "40% inner-shard RRset groups (8-64 records), 10% DS-KSK SHA-256 (44-132 B keys), "
+                             "20% RSA-2048 and 30% ECDSA-P256 signature records",
                       "rates": out, "sample_bit_exact_vs_alone": True}}
 
 

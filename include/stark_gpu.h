@@ -153,6 +153,7 @@ typedef struct stark_server stark_server;
 #define STARK_SERVE_DS_KSK 0      /* prove_ds_ksk_binding request */
 #define STARK_SERVE_INNER_SHARD 1 /* prove_inner_shard request */
 #define STARK_SERVE_RSA 2         /* RSA-2048 record (stark_prove_rsa_exp_batch with one record) */
+#define STARK_SERVE_P256 3        /* ECDSA-P256 record (stark_prove_ecdsa_p256_batch) */
 #define STARK_SERVER_DRY_RUN 1    /* flag: schedule only, no device work (admission tests) */
 #define STARK_E_TIMEOUT (-6)
 
@@ -166,6 +167,7 @@ typedef struct {
     size_t n_records;
     const uint8_t *fs_binding; /* 32 bytes; RSA: the public-inputs hash */
     const uint8_t *rsa_n, *rsa_s, *rsa_em; /* RSA: 256 bytes each, big-endian, s < n */
+    const uint8_t *p256_qx, *p256_qy, *p256_r, *p256_s, *p256_digest; /* ECDSA-P256: 32 bytes each, big-endian */
 } stark_serve_request;
 
 typedef struct {
@@ -174,7 +176,7 @@ typedef struct {
     int kind;
     uint8_t pi_hash[32];
     uint8_t root_f0[32];
-    uint8_t aux[32]; /* inner shard: merkle_root; DS-KSK: asserted_digest; RSA: zero */
+    uint8_t aux[32]; /* inner shard: merkle_root; DS-KSK: asserted_digest; RSA, P-256: zero */
     size_t n_trace;
     size_t batch_size; /* requests proven in the same lockstep batch */
     uint64_t batch_seq; /* admission order of that batch (1, 2, ...) */

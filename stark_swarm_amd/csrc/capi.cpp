@@ -665,9 +665,17 @@ int stark_server_submit(stark_server *s, const stark_serve_request *q, uint64_t 
     try {
         sg::ServeRequest r;
         r.kind = q->kind;
-        SG_REQUIRE(q->fs_binding, "server submit: null fs_binding");
-        memcpy(r.fs_binding.data(), q->fs_binding, 32);
-        if (q->kind == STARK_SERVE_DS_KSK) {
+        SG_REQUIRE(q->fs_binding || q->kind == STARK_SERVE_P256, "server submit: null fs_binding");
+        if (q->fs_binding) memcpy(r.fs_binding.data(), q->fs_binding, 32);
+        if (q->kind == STARK_SERVE_P256) {
+            SG_REQUIRE(q->p256_qx && q->p256_qy && q->p256_r && q->p256_s && q->p256_digest,
+                       "server submit: null ECDSA-P256 input");
+            memcpy(r.p256.qx.data(), q->p256_qx, 32);
+            memcpy(r.p256.qy.data(), q->p256_qy, 32);
+            memcpy(r.p256.r.data(), q->p256_r, 32);
+            memcpy(r.p256.s.data(), q->p256_s, 32);
+            memcpy(r.p256.digest.data(), q->p256_digest, 32);
+        } else if (q->kind == STARK_SERVE_DS_KSK) {
             SG_REQUIRE((q->dnskey || q->dnskey_len == 0) && q->parent_ds_hash, "server submit: null DS-KSK input");
             if (q->dnskey_len) r.key.assign(q->dnskey, q->dnskey + q->dnskey_len);
             memcpy(r.parent.data(), q->parent_ds_hash, 32);

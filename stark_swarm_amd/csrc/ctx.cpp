@@ -66,6 +66,18 @@ void Ctx::d2h(void *dst, const void *src, size_t n) {
     }
 }
 
+void *Ctx::upload_staging(size_t n) {
+    if (up_bytes < n) {
+        if (up) (void)hipHostFree(up);
+        up = nullptr;
+        up_bytes = 0;
+        const size_t sz = n < (16ULL << 20) ? (16ULL << 20) : n + n / 4;
+        SG_HIP(hipHostMalloc(&up, sz, hipHostMallocDefault));
+        up_bytes = sz;
+    }
+    return up;
+}
+
 void Ctx::release(void *p) {
     auto it = live.find(p);
     if (it == live.end()) return;
@@ -249,6 +261,7 @@ void ctx_destroy(Ctx *c) {
     for (auto &kv : c->live) (void)hipFree(kv.first);
     for (auto &kv : c->free_blocks) (void)hipFree(kv.second);
     if (c->pin) (void)hipHostFree(c->pin);
+    if (c->up) (void)hipHostFree(c->up);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -84,6 +84,12 @@ struct Ctx {
     // has cost milliseconds for tens of KB, one into pinned memory does not
     void *pin = nullptr;
     size_t pin_bytes = 0;
+    // pinned host memory that host-built inputs are written into before an
+    // upload (the SHA / RSA / P-256 witness words); reused across calls, so a
+    // caller must have waited for its upload before it returns
+    void *up = nullptr;
+    size_t up_bytes = 0;
+    void *upload_staging(size_t n);
     // copy n bytes of device memory to host memory after the stream's
     // earlier work; returns once dst holds them
     void d2h(void *dst, const void *src, size_t n);

@@ -30,6 +30,7 @@
 #include "bigint.hpp"
 #include "gl.hpp"
 #include "keccak.hpp"
+#include "ntt.hpp"
 #include "p256_air.hpp"
 
 namespace sg {
@@ -468,28 +469,34 @@ bool row_step(const Step &S, Limbs *acc, const Limbs *base, int bit, const Limbs
     return true;
 }
 
-// All h rows of one proof's chains: words[row * n_words + w]; the r_proj
-// words are the chain outputs (bind_outputs) or the given values.
-void chain_words(const P256Chains &in, uint64_t h, uint64_t *words, std::array<Be32, 6> &rproj) {
+// The h rows of one chain (c = 0: A, 1: B) of one proof:
+// words[row * n_words + w] for the chain's words; out gets the chain's
+// accumulator after its last bit.
+bool chain_rows(const P256Chains &in, int c, uint64_t h, uint64_t *words, Limbs *out) {
     const Layout &L = layout();
     const uint64_t nw = L.n_words;
     const Limbs bconst = be_limbs(B_BE);
-    Limbs out[2][3];
-    for (int c = 0; c < 2; c++) {
-        Limbs acc[3], base[3];
-        for (int i = 0; i < 3; i++) {
-            acc[i] = be_limbs(in.pts[6 * c + i].data());
-            base[i] = be_limbs(in.pts[6 * c + 3 + i].data());
-        }
-        const std::vector<uint8_t> &bits = c ? in.b_bits : in.a_bits;
-        for (uint64_t r = 0; r < h; r++) {
-            const int bit = r < bits.size() ? (bits[r] != 0) : 0;
-            if (!row_step(L.ch[c], acc, base, bit, bconst, words + r * nw))
-                throw Error(E_INVALID, "p256 trace: a field product does not close (inputs must be below p)");
-            if (r + 1 == bits.size())
-                for (int i = 0; i < 3; i++) out[c][i] = acc[i];
-        }
+    Limbs acc[3], base[3];
+    for (int i = 0; i < 3; i++) {
+        acc[i] = be_limbs(in.pts[6 * c + i].data());
+        base[i] = be_limbs(in.pts[6 * c + 3 + i].data());
     }
+    const std::vector<uint8_t> &bits = c ? in.b_bits : in.a_bits;
+    for (uint64_t r = 0; r < h; r++) {
+        const int bit = r < bits.size() ? (bits[r] != 0) : 0;
+        if (!row_step(L.ch[c], acc, base, bit, bconst, words + r * nw)) return false;
+        if (r + 1 == bits.size())
+            for (int i = 0; i < 3; i++) out[i] = acc[i];
+    }
+    return true;
+}
+
+// The words shared by both chains (the curve constant b and r_proj, bound to
+// the chain outputs or given) in every row.
+void shared_rows(const P256Chains &in, uint64_t h, const Limbs (*out)[3], uint64_t *words, std::array<Be32, 6> &rproj) {
+    const Layout &L = layout();
+    const uint64_t nw = L.n_words;
+    const Limbs bconst = be_limbs(B_BE);
     Limbs rp[6];
     for (int i = 0; i < 6; i++) {
         if (in.bind_outputs) {
@@ -508,6 +515,15 @@ void chain_words(const P256Chains &in, uint64_t h, uint64_t *words, std::array<B
         for (int i = 0; i < 6; i++)
             for (int l = 0; l < NL; l++) W[L.w_rproj + NL * i + l] = (uint64_t)rp[i][l];
     }
+}
+
+// All h rows of one proof: every word of every row is written.
+void chain_words(const P256Chains &in, uint64_t h, uint64_t *words, std::array<Be32, 6> &rproj) {
+    Limbs out[2][3];
+    for (int c = 0; c < 2; c++)
+        if (!chain_rows(in, c, h, words, out[c]))
+            throw Error(E_INVALID, "p256 trace: a field product does not close (inputs must be below p)");
+    shared_rows(in, h, out, words, rproj);
 }
 
 void check_inputs(const std::vector<P256Chains> &in, uint64_t h) {
@@ -577,7 +593,7 @@ __device__ __forceinline__ void range(Sums &S, const uint64_t *col, uint64_t n0,
 #pragma unroll 1
     for (int i = 0; i < NL; i++) {
         Acc pk{0, 0};
-#pragma unroll
+#pragma unroll 8
         for (int b = 0; b < LB; b++) {
             const uint64_t x = col[(uint64_t)(bb + LB * i + b) * n0 + pt];
             boolean(S, x, s + LB * i + b + 1);
@@ -609,14 +625,16 @@ __device__ __forceinline__ void signed_chain(Sums &S, const uint64_t *col, uint6
 }
 
 // phi partial sums of one sub-gadget (blockIdx.y < ng) or of the boundary and
-// transitions (blockIdx.y == ng) at 64 points, proof blockIdx.z:
-// part[(k * (ng + 1) + g) * n0 + pt]
-__global__ __launch_bounds__(64) void compose_p256_k(const uint64_t *lde, uint64_t n0, uint64_t blowup,
-                                                    const GDesc *gd, uint32_t ng, TailDesc td, uint64_t *part) {
+// transitions (blockIdx.y == ng, launched when grid.y = ng + 1) at 64 points,
+// proof blockIdx.z whose wstride columns start at lde + k * wstride * n0:
+// part[(k * gridDim.y + g) * n0 + pt]
+__global__ __launch_bounds__(64) void compose_p256_k(const uint64_t *lde, uint64_t wstride, uint64_t n0,
+                                                    uint64_t blowup, const GDesc *gd, uint32_t ng, TailDesc td,
+                                                    uint64_t *part) {
     const uint32_t g = blockIdx.y;
     const uint64_t k = blockIdx.z;
     const uint64_t pt = (uint64_t)blockIdx.x * 64 + threadIdx.x;
-    const uint64_t *col = lde + k * (uint64_t)P256_W * n0;
+    const uint64_t *col = lde + k * wstride * n0;
 #define AT(c) col[(uint64_t)(c) * n0 + pt]
     Sums S;
     if (g == ng) {
@@ -665,7 +683,7 @@ __global__ __launch_bounds__(64) void compose_p256_k(const uint64_t *lde, uint64
 #pragma unroll 1
             for (int m = 0; m < MCP; m++) {
                 Acc sc{0, 0};
-#pragma unroll
+#pragma unroll 8
                 for (int b = 0; b < MCB; b++) {
                     const uint64_t y = AT(d.cy + MCB * m + b);
                     boolean(S, y, s + 2 * EC + MSP + MCB * m + b + 1);
@@ -750,7 +768,7 @@ __global__ __launch_bounds__(64) void compose_p256_k(const uint64_t *lde, uint64
         }
     }
 #undef AT
-    part[(k * (ng + 1) + g) * n0 + pt] = S.value();
+    part[(k * gridDim.y + g) * n0 + pt] = S.value();
 }
 
 // phi[k][pt] = sum over the ng + 1 partial sums
@@ -1009,29 +1027,129 @@ void build_p256_traces_dev(Ctx *c, const std::vector<P256Chains> &in, uint64_t h
     const Layout &L = layout();
     Scoped d_desc(c, 4ULL * P256_W);
     SG_HIP(hipMemcpyAsync(d_desc.p, L.desc.data(), 4ULL * P256_W, hipMemcpyHostToDevice, c->stream));
-    const uint64_t nw = L.n_words;
-    std::vector<uint64_t> words(K * h * nw, 0);
+    const uint64_t nw = L.n_words, per = h * nw;
+    // every word is written below, straight into pinned memory
+    uint64_t *words = (uint64_t *)c->upload_staging(8 * K * per);
     std::vector<std::array<Be32, 6>> rp(K);
-    std::vector<int> bad(K, 0);
-    parallel_for(K, [&](size_t k) {
-        try {
-            chain_words(in[k], h, words.data() + k * h * nw, rp[k]);
-        } catch (...) {
-            bad[k] = 1;
-        }
+    std::vector<std::array<std::array<Limbs, 3>, 2>> outs(K);
+    std::vector<int> bad(2 * K, 0);
+    parallel_for(2 * K, [&](size_t t) {
+        const size_t k = t / 2;
+        const int ch = (int)(t % 2);
+        bad[t] = !chain_rows(in[k], ch, h, words + k * per, outs[k][ch].data());
     });
-    for (size_t k = 0; k < K; k++)
-        SG_REQUIRE(!bad[k], "p256 trace: a field product does not close (inputs must be below p)");
+    for (size_t t = 0; t < 2 * K; t++)
+        SG_REQUIRE(!bad[t], "p256 trace: a field product does not close (inputs must be below p)");
+    parallel_for(K, [&](size_t k) {
+        const Limbs out[2][3] = {{outs[k][0][0], outs[k][0][1], outs[k][0][2]},
+                                 {outs[k][1][0], outs[k][1][1], outs[k][1][2]}};
+        shared_rows(in[k], h, out, words + k * per, rp[k]);
+    });
     if (rproj_out) *rproj_out = rp;
-    Scoped d_words(c, 8 * words.size());
-    SG_HIP(hipMemcpyAsync(d_words.p, words.data(), 8 * words.size(), hipMemcpyHostToDevice, c->stream));
+    Scoped d_words(c, 8 * K * per);
+    SG_HIP(hipMemcpyAsync(d_words.p, words, 8 * K * per, hipMemcpyHostToDevice, c->stream));
     {
         ProfScope ps(c, "expand_p256", 8.0 * P256_W * h * K);
         hipLaunchKernelGGL(expand_p256_k, dim3((uint32_t)((h * P256_W + 255) / 256), (uint32_t)K), dim3(256), 0,
                            c->stream, d_words.as<uint64_t>(), d_desc.as<uint32_t>(), (uint32_t)nw, ilog2(h), d_tr);
         SG_HIP(hipGetLastError());
     }
-    SG_HIP(hipStreamSynchronize(c->stream));  // the host word buffer goes out of scope
+    SG_HIP(hipStreamSynchronize(c->stream));  // the staging buffer is reused by the next upload
+}
+
+namespace p256 {
+
+// the 180 columns of the boundary and transitions, in the order of a compact
+// tail trace: select outputs (6 x 10), r_proj (6 x 10), accumulators (6 x 10)
+void tail_columns(const Layout &L, uint32_t *cols, TailDesc &compact) {
+    compact.s = L.tail.s;
+    for (int j = 0; j < 6; j++) {
+        compact.selc[j] = NL * j;
+        compact.rproj[j] = 6 * NL + NL * j;
+        compact.acc[j] = 12 * NL + NL * j;
+        for (int l = 0; l < NL; l++) {
+            cols[NL * j + l] = L.tail.selc[j] + l;
+            cols[6 * NL + NL * j + l] = L.tail.rproj[j] + l;
+            cols[12 * NL + NL * j + l] = L.tail.acc[j] + l;
+        }
+    }
+}
+
+// out[k][t][r] = tr[k][cols[t]][r] for the K traces of h rows
+__global__ __launch_bounds__(256) void gather_cols_k(const uint64_t *tr, const uint32_t *cols, uint32_t nc,
+                                                     uint32_t log_h, uint64_t *out) {
+    const uint64_t k = blockIdx.y, h = 1ULL << log_h;
+    const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (uint64_t)nc * h) return;
+    const uint64_t c = t >> log_h, r = t & (h - 1);
+    out[k * nc * h + t] = tr[(k * P256_W + cols[c]) * h + r];
+}
+
+__global__ __launch_bounds__(256) void add_into_k(uint64_t *a, const uint64_t *b, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) a[i] = add(a[i], b[i]);
+}
+
+}  // namespace p256
+
+void merge_p256_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint64_t n_trace, uint64_t blowup,
+                                 uint64_t *c_eval) {
+    SG_REQUIRE(is_pow2(n_trace) && n_trace >= 32 && is_pow2(blowup) && blowup >= 2,
+               "merge_p256: n_trace must be a power of two >= 32 and blowup >= 2");
+    SG_REQUIRE(K >= 1 && K <= 65535, "merge_p256: 1 to 65535 proofs");
+    const uint64_t n = n_trace * blowup, n2 = 2 * n_trace;
+    SG_REQUIRE(n <= (1ULL << 20), "merge_p256: domain above 2^20");
+    const Layout &L = layout();
+    const uint32_t ng = (uint32_t)L.gd.size(), NT = 18 * NL;
+    Scoped phi(c, 8 * n * K);
+    {
+        // the gadget constraints at the 2 n_trace points of <w_{2 n_trace}>
+        // (LDE points i = (blowup / 2) t), then interpolated to all n points
+        Scoped d_gd(c, sizeof(GDesc) * ng), phi2(c, 8 * n2 * K);
+        SG_HIP(hipMemcpyAsync(d_gd.p, L.gd.data(), sizeof(GDesc) * ng, hipMemcpyHostToDevice, c->stream));
+        {
+            Scoped lde2(c, 8ULL * P256_W * n2 * K), part(c, 8ULL * ng * n2 * K);
+            lde_columns(c, d_tr, lde2.as<uint64_t>(), ilog2(n_trace), 1, (uint64_t)P256_W * K, n_trace, n2, 1);
+            {
+                ProfScope ps(c, "compose_p256", (8.0 * P256_W * n2 + 8.0 * ng * n2) * K);
+                hipLaunchKernelGGL(compose_p256_k, dim3((uint32_t)(n2 / 64), ng, K), dim3(64), 0, c->stream,
+                                   lde2.as<uint64_t>(), (uint64_t)P256_W, n2, (uint64_t)2, d_gd.as<GDesc>(), ng,
+                                   L.tail, part.as<uint64_t>());
+                SG_HIP(hipGetLastError());
+            }
+            {
+                ProfScope ps(c, "p256_sum", 8.0 * (ng + 1) * n2 * K);
+                hipLaunchKernelGGL(p256_sum_k, dim3((uint32_t)((n2 + 255) / 256), K), dim3(256), 0, c->stream,
+                                   part.as<uint64_t>(), ng, n2, phi2.as<uint64_t>());
+                SG_HIP(hipGetLastError());
+            }
+        }
+        lde_columns(c, phi2.as<uint64_t>(), phi.as<uint64_t>(), ilog2(n2), ilog2(blowup / 2), K, n2, n, 1);
+    }
+    {
+        // boundary and transitions at every point, from the LDE of their 180 columns
+        uint32_t cols[18 * NL];
+        TailDesc td;
+        tail_columns(L, cols, td);
+        Scoped d_cols(c, 4 * NT), ttr(c, 8ULL * NT * n_trace * K), tlde(c, 8ULL * NT * n * K), tphi(c, 8 * n * K);
+        SG_HIP(hipMemcpyAsync(d_cols.p, cols, 4 * NT, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(gather_cols_k, dim3((uint32_t)((NT * n_trace + 255) / 256), K), dim3(256), 0, c->stream,
+                           d_tr, d_cols.as<uint32_t>(), NT, ilog2(n_trace), ttr.as<uint64_t>());
+        SG_HIP(hipGetLastError());
+        lde_columns(c, ttr.as<uint64_t>(), tlde.as<uint64_t>(), ilog2(n_trace), ilog2(blowup), (uint64_t)NT * K,
+                    n_trace, n, 1);
+        {
+            ProfScope ps(c, "compose_p256_tail", (8.0 * NT * n + 8.0 * n) * K);
+            hipLaunchKernelGGL(compose_p256_k, dim3((uint32_t)(n / 64), 1, K), dim3(64), 0, c->stream,
+                               tlde.as<uint64_t>(), (uint64_t)NT, n, blowup, (const GDesc *)nullptr, 0u, td,
+                               tphi.as<uint64_t>());
+            SG_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(add_into_k, dim3((uint32_t)((n * K + 255) / 256)), dim3(256), 0, c->stream,
+                           phi.as<uint64_t>(), tphi.as<uint64_t>(), n * K);
+        SG_HIP(hipGetLastError());
+    }
+    quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
 }
 
 void merge_p256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup, uint64_t *c_eval) {
@@ -1047,8 +1165,8 @@ void merge_p256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace,
     Scoped part(c, 8ULL * (ng + 1) * n * K), phi(c, 8 * n * K);
     {
         ProfScope ps(c, "compose_p256", (8.0 * P256_W * n + 8.0 * (ng + 1) * n) * K);
-        hipLaunchKernelGGL(compose_p256_k, dim3((uint32_t)(n / 64), ng + 1, K), dim3(64), 0, c->stream, lde, n,
-                           blowup, d_gd.as<GDesc>(), ng, L.tail, part.as<uint64_t>());
+        hipLaunchKernelGGL(compose_p256_k, dim3((uint32_t)(n / 64), ng + 1, K), dim3(64), 0, c->stream, lde,
+                           (uint64_t)P256_W, n, blowup, d_gd.as<GDesc>(), ng, L.tail, part.as<uint64_t>());
         SG_HIP(hipGetLastError());
     }
     {

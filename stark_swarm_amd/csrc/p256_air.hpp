@@ -55,6 +55,16 @@ void build_p256_traces_dev(Ctx *c, const std::vector<P256Chains> &in, uint64_t h
 // lde + P256_W * k * n0, c_eval at c_eval + k * n0. Phi is evaluated at every
 // point (the streaming merge has no thread-count quirk).
 void merge_p256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup, uint64_t *c_eval);
+// The same c_eval from the K traces themselves (d_tr proof-major, P256_W x
+// n_trace each), without the full LDE: every gadget constraint has degree
+// <= 2 in trace polynomials of degree < n_trace, so their weighted sum is a
+// polynomial of degree <= 2 n_trace - 2, fixed by its values at the
+// 2 n_trace points of <w_{2 n_trace}> (a blowup-2 LDE of all columns) and
+// interpolated from there; only the 180 columns of the row-gated boundary and
+// transition constraints are extended to all n points. Exact arithmetic
+// makes the result equal to merge_p256_batch on the full LDE.
+void merge_p256_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint64_t n_trace, uint64_t blowup,
+                                 uint64_t *c_eval);
 // The chain inputs of an ECDSA record: u1 = e / s and u2 = r / s mod n
 // (e = digest mod n), chain A from G with the MSB-first bits of u1, chain B
 // from Q with those of u2, 256 steps each, r_proj bound to the outputs.

@@ -526,8 +526,10 @@ std::vector<P256Proof> prove_p256_batch(Ctx *c, const std::vector<P256Chains> &i
     size_t free_b = 0, total_b = 0;
     SG_HIP(hipMemGetInfo(&free_b, &total_b));
     const uint64_t budget = (uint64_t)((free_b + c->cached_bytes) * 0.6);
-    // trace, LDE input and coefficients, LDE, merge partial sums, FRI
-    const uint64_t per = 8ULL * W * (3 * h + n0) + 8ULL * 300 * n0 + 8ULL * 16 * n0;
+    // trace, its LDE coefficients and blowup-2 LDE, the gadget partial sums,
+    // the tail LDE, phi, FRI
+    const uint64_t per = h >= 32 ? 8ULL * W * 4 * h + 8ULL * 300 * 2 * h + 8ULL * 200 * n0 + 8ULL * 16 * n0
+                                 : 8ULL * W * (3 * h + n0) + 8ULL * 300 * n0 + 8ULL * 16 * n0;
     SG_REQUIRE(per <= budget, "prove_p256: one proof does not fit in device memory");
     const size_t cap = (size_t)std::min<uint64_t>(4096, budget / per);
     const size_t nw = (K + cap - 1) / cap, chunk = (K + nw - 1) / nw;
@@ -537,19 +539,21 @@ std::vector<P256Proof> prove_p256_batch(Ctx *c, const std::vector<P256Chains> &i
         Scoped d_c(c, 8ULL * n0 * G);
         {
             std::vector<std::array<Be32, 6>> rp;
-            Scoped d_lde(c, 8ULL * W * n0 * G);
+            Scoped d_tr(c, 8ULL * W * h * G);
             {
-                Scoped d_tr(c, 8ULL * W * h * G);
-                {
-                    HostScope hs(c, "p256_trace");
-                    const std::vector<P256Chains> part(in.begin() + k0, in.begin() + k0 + G);
-                    build_p256_traces_dev(c, part, h, d_tr.as<uint64_t>(), &rp);
-                }
-                HostScope hs(c, "p256_lde_merge");
+                HostScope hs(c, "p256_trace");
+                const std::vector<P256Chains> part(in.begin() + k0, in.begin() + k0 + G);
+                build_p256_traces_dev(c, part, h, d_tr.as<uint64_t>(), &rp);
+            }
+            HostScope hs(c, "p256_lde_merge");
+            if (h >= 32) {
+                merge_p256_from_trace_batch(c, d_tr.as<uint64_t>(), G, h, SWARM_BLOWUP, d_c.as<uint64_t>());
+            } else {  // below 64 points of <w_{2 n_trace}>: compose on the full LDE
+                Scoped d_lde(c, 8ULL * W * n0 * G);
                 lde_columns(c, d_tr.as<uint64_t>(), d_lde.as<uint64_t>(), ilog2(h), ilog2(SWARM_BLOWUP), W * G, h,
                             n0, 1);
+                merge_p256_batch(c, d_lde.as<uint64_t>(), G, h, SWARM_BLOWUP, d_c.as<uint64_t>());
             }
-            merge_p256_batch(c, d_lde.as<uint64_t>(), G, h, SWARM_BLOWUP, d_c.as<uint64_t>());
             if (c->profiling) SG_HIP(hipStreamSynchronize(c->stream));
             for (uint32_t j = 0; j < G; j++) out[k0 + j].rproj = rp[j];
         }

@@ -17,6 +17,10 @@ uint64_t Server::need_bytes(const ServeRequest &r) {
         const uint64_t h = 32, n0 = h * BLOWUP;
         return 8ULL * RSA_W * (2 * h + 2 * n0) + 8ULL * 16 * n0;
     }
+    if (r.kind == SERVE_P256) {  // trace, its blowup-2 LDE and partial sums, the tail LDE, FRI
+        const uint64_t h = P256_ECDSA_ROWS, n0 = h * BLOWUP;
+        return 8ULL * P256_W * 4 * h + 8ULL * 300 * 2 * h + 8ULL * 200 * n0 + 8ULL * 16 * n0;
+    }
     if (r.kind == SERVE_DS_KSK) {
         const uint64_t h = sha256_trace_height(r.key.size()), n0 = h * BLOWUP;
         return 8ULL * SHA256_W * (2 * h + 2 * n0) + 8ULL * 16 * n0;
@@ -63,7 +67,7 @@ double Server::now() const {
 }
 
 uint64_t Server::submit(ServeRequest &&r) {
-    SG_REQUIRE(r.kind == SERVE_DS_KSK || r.kind == SERVE_INNER_SHARD || r.kind == SERVE_RSA,
+    SG_REQUIRE(r.kind == SERVE_DS_KSK || r.kind == SERVE_INNER_SHARD || r.kind == SERVE_RSA || r.kind == SERVE_P256,
                "server: unknown request kind");
     if (r.kind == SERVE_INNER_SHARD) {
         SG_REQUIRE(!r.domains.empty(), "server: an inner shard needs at least one record");
@@ -191,6 +195,16 @@ void Server::run_batch(Ctx *c, std::vector<Item> &batch, std::vector<ServeResult
                 pis[i] = batch[i].req.fs_binding;
             }
             std::vector<RsaProof> out = prove_rsa_batch(c, recs, 1, 32, pis);
+            for (size_t i = 0; i < K; i++) {
+                memcpy(res[i].pi_hash, out[i].pi_hash, 32);
+                memcpy(res[i].root_f0, out[i].root_f0, 32);
+                res[i].n_trace = out[i].n_trace;
+                res[i].proof = std::move(out[i].proof);
+            }
+        } else if (batch[0].req.kind == SERVE_P256) {
+            std::vector<P256Record> recs(K);
+            for (size_t i = 0; i < K; i++) recs[i] = batch[i].req.p256;
+            std::vector<P256Proof> out = prove_ecdsa_p256_batch(c, recs);
             for (size_t i = 0; i < K; i++) {
                 memcpy(res[i].pi_hash, out[i].pi_hash, 32);
                 memcpy(res[i].root_f0, out[i].root_f0, 32);

@@ -8,7 +8,8 @@
 // It stands in for the swarm worker's request loop
 // (crates/swarm-worker/src/main.rs:220-335, which calls prove_inner_shard
 // from tokio spawn_blocking threads) on the device side; besides inner
-// shards it proves DS->KSK bindings and RSA-2048 signature records.
+// shards it proves DS->KSK bindings and RSA-2048 and ECDSA-P256 signature
+// records.
 #pragma once
 #include <stdint.h>
 
@@ -27,7 +28,7 @@
 
 namespace sg {
 
-enum ServeKind : int { SERVE_DS_KSK = 0, SERVE_INNER_SHARD = 1, SERVE_RSA = 2 };
+enum ServeKind : int { SERVE_DS_KSK = 0, SERVE_INNER_SHARD = 1, SERVE_RSA = 2, SERVE_P256 = 3 };
 
 struct ServeRequest {
     int kind = SERVE_DS_KSK;
@@ -39,6 +40,7 @@ struct ServeRequest {
     std::vector<uint16_t> rtype;
     std::vector<uint32_t> ttl;
     RsaRecord rsa{};                    // RSA-2048: n, s, em (fs_binding = its public-input hash)
+    P256Record p256{};                  // ECDSA-P256: qx, qy, r, s, digest
     std::array<uint8_t, 32> fs_binding{};
 };
 

@@ -2,14 +2,14 @@
 (crates/swarm-worker/src/main.rs:220-335) on one GPU: requests are admitted
 by an HBM budget and proven with continuous batching (stark_server_* in
 include/stark_gpu.h). Each result equals the single-call result of
-prove_ds_ksk_binding / prove_inner_shard / the RSA-2048 record prover for
-the same request."""
+prove_ds_ksk_binding / prove_inner_shard / the RSA-2048 or ECDSA-P256 record
+prover for the same request."""
 import ctypes
 from dataclasses import dataclass
 
 from . import api as A
 
-SERVE_DS_KSK, SERVE_INNER_SHARD, SERVE_RSA = 0, 1, 2
+SERVE_DS_KSK, SERVE_INNER_SHARD, SERVE_RSA, SERVE_P256 = 0, 1, 2, 3
 DRY_RUN = 1
 E_TIMEOUT = -6
 
@@ -18,7 +18,9 @@ class _Req(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("dnskey", ctypes.c_char_p), ("dnskey_len", A.sz),
                 ("parent_ds_hash", ctypes.c_char_p), ("salt", ctypes.c_char_p),
                 ("records", ctypes.POINTER(A._DnsRecord)), ("n_records", A.sz), ("fs_binding", ctypes.c_char_p),
-                ("rsa_n", ctypes.c_char_p), ("rsa_s", ctypes.c_char_p), ("rsa_em", ctypes.c_char_p)]
+                ("rsa_n", ctypes.c_char_p), ("rsa_s", ctypes.c_char_p), ("rsa_em", ctypes.c_char_p),
+                ("p256_qx", ctypes.c_char_p), ("p256_qy", ctypes.c_char_p), ("p256_r", ctypes.c_char_p),
+                ("p256_s", ctypes.c_char_p), ("p256_digest", ctypes.c_char_p)]
 
 
 class _Res(ctypes.Structure):
@@ -36,7 +38,7 @@ class ServeResult:
     kind: int
     pi_hash: bytes
     root_f0: bytes
-    aux: bytes  # merkle_root (inner shard), asserted_digest (DS-KSK), zero (RSA)
+    aux: bytes  # merkle_root (inner shard), asserted_digest (DS-KSK), zero (RSA, P-256)
     n_trace: int
     batch_size: int
     batch_seq: int
@@ -109,6 +111,14 @@ class ProvingServer:
             return int(x).to_bytes(256, "big")
         return self._submit(_Req(SERVE_RSA, None, 0, None, None, None, 0, bytes(public_inputs_hash),
                                  be(record.n), be(record.s), be(record.em)))
+
+    def submit_p256(self, record):
+        """one ECDSA-P256 record (an api.EcdsaRecord); its public-input hash is
+        the record's pk_hash_pre"""
+        def be(x):
+            return int(x).to_bytes(32, "big")
+        return self._submit(_Req(SERVE_P256, None, 0, None, None, None, 0, None, None, None, None,
+                                 be(record.qx), be(record.qy), be(record.r), be(record.s), bytes(record.digest)))
 
     def wait(self, rid, timeout=600.0):
         r = _Res()

@@ -140,3 +140,28 @@ def test_prove_ecdsa_rejects_bad_records(S):
         S.prove_record_ecdsa_batch([S.EcdsaRecord(qx, qy, r, (s + 1) % N, dg)])
     with pytest.raises(S.StarkError):
         S.prove_record_ecdsa_batch([S.EcdsaRecord(qx, (qy + 1) % FP, r, s, dg)])
+
+
+def test_prove_p256_chains_low_degree_path(S):
+    """n_trace 32: the prover composes the gadget constraints on the
+    blowup-2 LDE and interpolates, the tail on the full LDE; byte for byte
+    against the oracle's direct evaluation at all 1024 points (uneven chain
+    lengths; a second input with r_proj values whose boundary fails)"""
+    say("prove_p256_chains n_trace 32")
+    g, q = (GX, GY), ec_mul(3, (GX, GY))
+    pts = [g[0], g[1], 1, g[0], g[1], 1, q[0], q[1], 1, q[0], q[1], 1]
+    a = [int((i * 7) % 3 == 0) for i in range(20)]
+    b = [int((i * 5) % 4 == 1) for i in range(27)]
+    tr = O.p256_trace(pts, [0] * 6, a, b, 32)
+    c = O.p256_columns()
+    rp = [sum(int(tr[c["sel"][ch][i] + l, (19, 26)[ch]]) << (26 * l) for l in range(10))
+          for ch in range(2) for i in range(3)]
+    bad = [v ^ 5 for v in rp]
+    pis = [hashlib.sha3_256(b"low-degree %d" % i).digest() for i in range(2)]
+    chains = [S.P256Chains(tuple(pts[0:3]), tuple(pts[3:6]), tuple(pts[6:9]), tuple(pts[9:12]), a, b),
+              S.P256Chains(tuple(pts[0:3]), tuple(pts[3:6]), tuple(pts[6:9]), tuple(pts[9:12]), a, b,
+                           rproj=tuple(bad), bind_outputs=False)]
+    outs = S.prove_p256_chains_batch(chains, 32, pis)
+    assert list(outs[0].rproj) == rp
+    assert outs[0].proof_blob == _oracle_chain_proof(pts, rp, a, b, 32, pis[0])
+    assert outs[1].proof_blob == _oracle_chain_proof(pts, bad, a, b, 32, pis[1])

@@ -410,12 +410,16 @@ def test_server_results_equal_standalone(S):
     zone = build_synthetic_zone(1600)
     salt = b"swarm-test-zone1"
     reqs = []
+    from stark_swarm_amd.zone import synthetic_ecdsa_records
+    ecdsa = synthetic_ecdsa_records(4, 0x5EED0009)
     for i in range(18):
         fsb = hashlib.sha3_256(b"req%d" % i).digest()
         if i % 3 == 0:
             reqs.append(("shard", zone[80 * i: 80 * i + 20 + 7 * i], fsb))
         elif i % 4 == 1:
             reqs.append(("rsa", S.RsaStackedRecord(*_rsa_rec(0x900 + i)), fsb))
+        elif i % 8 == 2:
+            reqs.append(("p256", ecdsa[i // 8], None))
         else:
             key = bytes([(i * 13 + j) & 255 for j in range(20 + 9 * i)])
             reqs.append(("ksk", key, fsb))
@@ -428,6 +432,8 @@ def test_server_results_equal_standalone(S):
                     ids[j] = srv.submit_inner_shard(salt, x, fsb)
                 elif kind == "rsa":
                     ids[j] = srv.submit_rsa(x, fsb)
+                elif kind == "p256":
+                    ids[j] = srv.submit_p256(x)
                 else:
                     ids[j] = srv.submit_ds_ksk(x, hashlib.sha256(x).digest(), fsb)
         th = [threading.Thread(target=go, args=(range(t, len(reqs), 2),)) for t in range(2)]
@@ -444,6 +450,9 @@ def test_server_results_equal_standalone(S):
         elif kind == "rsa":
             alone = S.prove_rsa_exp_batch([x], [fsb])[0]
             assert r.aux == bytes(32) and r.kind == 2
+        elif kind == "p256":
+            alone = S.prove_record_ecdsa_batch([x])[0]
+            assert r.kind == 3 and r.n_trace == 256
         else:
             alone = S.prove_ds_ksk_binding(x, hashlib.sha256(x).digest(), fsb)
             assert r.aux == alone.asserted_digest

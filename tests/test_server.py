@@ -62,10 +62,12 @@ def test_mixed_kinds_and_concurrent_submitters():
 
     def submitter(srv, base):
         for j in range(12):
-            if j % 3 == 1:
+            if j % 4 == 1:
                 rid = srv.submit_inner_shard(b"swarm-test-zone1", zone[base + 10 * j: base + 10 * j + 10], b"\2" * 32)
-            elif j % 3 == 2:
+            elif j % 4 == 2:
                 rid = srv.submit_rsa(S.RsaStackedRecord((1 << 2047) | 1, 5 + j, 7), b"\4" * 32)
+            elif j % 4 == 3:  # dry run: the record is scheduled, never proven
+                rid = srv.submit_p256(S.EcdsaRecord(1, 2, 3, 4, bytes(32)))
             else:
                 rid = srv.submit_ds_ksk(_keys(1, base + j)[0], b"\0" * 32, b"\3" * 32)
             out[rid] = srv.wait(rid, timeout=60)
@@ -78,7 +80,7 @@ def test_mixed_kinds_and_concurrent_submitters():
             t.join()
         assert srv.stats()["completed"] == 48
     kinds = {r.kind for r in out.values()}
-    assert kinds == {0, 1, 2}
+    assert kinds == {0, 1, 2, 3}
     # a batch holds one kind only
     by_batch = {}
     for r in out.values():
