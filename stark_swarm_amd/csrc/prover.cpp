@@ -289,8 +289,9 @@ std::vector<ShaProof> prove_sha_batch(Ctx *c, const std::vector<int> &algs, cons
     for (auto &g : order) {
         const int alg = g.first.first;
         const uint64_t h = g.first.second, n0 = h * SWARM_BLOWUP, W = alg == 256 ? SHA256_W : SHA512_W;
-        // per proof: trace + its coefficients, LDE + the LDE pass scratch, merge and FRI buffers
-        const uint64_t per = 8ULL * W * (2 * h + 2 * n0) + 8ULL * 16 * n0;
+        // per proof: trace + its coefficients + its blowup-2 LDE, the gate-group
+        // polynomials at 2 h and n0 points, phi, FRI buffers
+        const uint64_t per = 8ULL * W * 4 * h + 8ULL * 5 * 2 * h + 8ULL * 24 * n0;
         SG_REQUIRE(per <= budget, "prove_sha: one proof does not fit in device memory");
         // equal waves: every wave of a group reuses the same buffer sizes
         const size_t cap = (size_t)std::min<uint64_t>(4096, budget / per);
@@ -306,30 +307,29 @@ std::vector<ShaProof> prove_sha_batch(Ctx *c, const std::vector<int> &algs, cons
             std::vector<std::array<uint8_t, 64>> dgs;
             Scoped d_c(c, 8ULL * n0 * G);
             {
-                Scoped d_lde(c, 8ULL * W * n0 * G);
+                // traces expanded on the device from the host's row words
+                Scoped d_tr(c, 8ULL * W * h * G);
                 {
-                    // traces expanded on the device from the host's row words
-                    Scoped d_tr(c, 8ULL * W * h * G);
-                    {
-                        HostScope hs(c, "sha_trace");
-                        if (alg == 256)
-                            build_sha256_traces_dev(c, gm, h, d_tr.as<uint64_t>(), nb, dgs);
-                        else
-                            build_sha512_traces_dev(c, gm, h, d_tr.as<uint64_t>(), nb, dgs);
-                    }
-                    for (uint32_t j = 0; j < G; j++) {
-                        ShaProof &o = out[ids[j]];
-                        memcpy(o.digest, dgs[j].data(), 64);
-                        o.n_blocks = nb[j];
-                        o.n_trace = h;
-                    }
-                    lde_columns(c, d_tr.as<uint64_t>(), d_lde.as<uint64_t>(), ilog2(h), ilog2(SWARM_BLOWUP), W * G,
-                                h, n0, 1);
+                    HostScope hs(c, "sha_trace");
+                    if (alg == 256)
+                        build_sha256_traces_dev(c, gm, h, d_tr.as<uint64_t>(), nb, dgs);
+                    else
+                        build_sha512_traces_dev(c, gm, h, d_tr.as<uint64_t>(), nb, dgs);
                 }
+                for (uint32_t j = 0; j < G; j++) {
+                    ShaProof &o = out[ids[j]];
+                    memcpy(o.digest, dgs[j].data(), 64);
+                    o.n_blocks = nb[j];
+                    o.n_trace = h;
+                }
+                // the merge from the traces: gate-group polynomials on a
+                // blowup-2 LDE, interpolated (no blowup-32 LDE is built)
                 if (alg == 256)
-                    merge_sha256_batch(c, d_lde.as<uint64_t>(), G, h, SWARM_BLOWUP, nb, phi_mode, d_c.as<uint64_t>());
+                    merge_sha256_from_trace_batch(c, d_tr.as<uint64_t>(), G, h, SWARM_BLOWUP, nb, phi_mode,
+                                                  d_c.as<uint64_t>());
                 else
-                    merge_sha512_batch(c, d_lde.as<uint64_t>(), G, h, SWARM_BLOWUP, nb, phi_mode, d_c.as<uint64_t>());
+                    merge_sha512_from_trace_batch(c, d_tr.as<uint64_t>(), G, h, SWARM_BLOWUP, nb, phi_mode,
+                                                  d_c.as<uint64_t>());
             }
             FriParams p;
             p.schedule = stir_schedule(n0);

@@ -190,10 +190,19 @@ __device__ __forceinline__ uint64_t boolc(uint64_t b) { return mul(b, sub(1, b))
 
 constexpr int AT = 128;
 
+// Gate groups of the constraints: ungated, compression rows, the
+// finalisation row, every other row, message-schedule rows.
+enum { G_ALL, G_COMP, G_FIN, G_NOTFIN, G_SCHED, NG };
+
 // grid.y = proof: the LDE of proof k is 756 columns of n at lde + 756 k n;
-// n_blocks[k] its SHA-256 block count
+// n_blocks[k] its SHA-256 block count.
+// GROUPS = false: phi at every point, gates from the point's trace row.
+// GROUPS = true: the five gate groups' weighted sums as polynomials of the
+// point, gates ignored and the row constant K[t] left out, written to
+// phi[(k * NG + g) * n + i]; t1_lam gets the weight of the T1 identity.
+template <bool GROUPS>
 __global__ __launch_bounds__(AT) void compose_sha256_k(const uint64_t *lde, uint64_t n, uint64_t blowup,
-                                                       const uint32_t *n_blocks, uint64_t *phi) {
+                                                       const uint32_t *n_blocks, uint64_t *phi, uint64_t *t1_lam) {
     const uint64_t i = (uint64_t)blockIdx.x * AT + threadIdx.x;
     if (i >= n) return;
     const uint64_t *L = lde + (uint64_t)SHA256_W * n * blockIdx.y;
@@ -201,10 +210,12 @@ __global__ __launch_bounds__(AT) void compose_sha256_k(const uint64_t *lde, uint
     auto cur = [&](uint32_t col) { return L[(uint64_t)col * n + i]; };
     auto nxt = [&](uint32_t col) { return L[(uint64_t)col * n + ni]; };
     const uint64_t row = i / blowup, blk = row >> 7, br = row & 127;
-    const bool past = blk >= n_blocks[blockIdx.y];
-    const bool comp = !past && br < 64, fin = !past && br == 64, sched = !past && br >= 15 && br <= 62;
+    const bool past = GROUPS || blk >= n_blocks[blockIdx.y];
+    const bool comp = GROUPS || (!past && br < 64), fin = !past && br == 64;
+    const bool sched = GROUPS || (!past && br >= 15 && br <= 62);
     const uint64_t TWO32 = 1ULL << 32;
-    Phi f;
+    Phi f, fc, ff, fn, fs;  // ungated; compression, finalisation, not finalisation, schedule (GROUPS mode)
+    Phi &gc = GROUPS ? fc : f, &gs = GROUPS ? fs : f;
     // booleanity of the 8 bit blocks
     const uint32_t boff[8] = {BIT_A, BIT_B, BIT_C, BIT_E, BIT_F, BIT_G, BIT_WW14, BIT_WW1};
 #pragma unroll 1
@@ -264,8 +275,9 @@ __global__ __launch_bounds__(AT) void compose_sha256_k(const uint64_t *lde, uint
     for (int b = 0; b < 32; b++) f.put(sub(cur(MAJ_BIT + b), xorp(cur(MAJ_U + b), cur(MAJ_BC + b))));
     f.put(sub(cur(MAJ_W), pack(MAJ_BIT)));
     // T1 = h + Sigma1 + Ch + K[t] + W[t] = T1_lo + 2^32 T1_cw; T2 = Sigma0 + Maj
-    const uint64_t krow = comp ? KD[br] : 0;
+    const uint64_t krow = GROUPS ? 0 : comp ? KD[br] : 0;
     const uint64_t t1s = add(add(add(add(cur(A_H), cur(S1_W)), cur(CH_W)), krow), cur(COL_W));
+    if (GROUPS && i == 0 && blockIdx.y == 0) *t1_lam = f.w;
     f.put(sub(add(cur(T1_LO), mul(TWO32, cur(T1_CW))), t1s));
     f.put(boolc(cur(T1_C0)));
     f.put(boolc(cur(T1_C1)));
@@ -276,26 +288,35 @@ __global__ __launch_bounds__(AT) void compose_sha256_k(const uint64_t *lde, uint
     f.put(boolc(cur(T2_C1)));
     f.put(sub(sub(cur(T2_CW), cur(T2_C0)), mul(2, cur(T2_C1))));
     if (comp) {
+        gc.w = f.w;
         // register shifts b<-a, c<-b, d<-c, f<-e, g<-f, h<-g
-        f.put(sub(nxt(A_A + 1), cur(A_A)));
-        f.put(sub(nxt(A_A + 2), cur(A_A + 1)));
-        f.put(sub(nxt(A_D), cur(A_A + 2)));
-        f.put(sub(nxt(A_E + 1), cur(A_E)));
-        f.put(sub(nxt(A_E + 2), cur(A_E + 1)));
-        f.put(sub(nxt(A_H), cur(A_E + 2)));
+        gc.put(sub(nxt(A_A + 1), cur(A_A)));
+        gc.put(sub(nxt(A_A + 2), cur(A_A + 1)));
+        gc.put(sub(nxt(A_D), cur(A_A + 2)));
+        gc.put(sub(nxt(A_E + 1), cur(A_E)));
+        gc.put(sub(nxt(A_E + 2), cur(A_E + 1)));
+        gc.put(sub(nxt(A_H), cur(A_E + 2)));
         // new_a, new_e with one carry bit each (two zero slots follow)
-        f.put(sub(sub(add(nxt(A_A), mul(TWO32, cur(NA_C))), cur(T1_LO)), cur(T2_LO)));
-        f.put(boolc(cur(NA_C)));
-        f.put(sub(sub(add(nxt(A_E), mul(TWO32, cur(NE_C))), cur(A_D)), cur(T1_LO)));
-        f.put(boolc(cur(NE_C)));
-        f.skip(2);
+        gc.put(sub(sub(add(nxt(A_A), mul(TWO32, cur(NA_C))), cur(T1_LO)), cur(T2_LO)));
+        gc.put(boolc(cur(NA_C)));
+        gc.put(sub(sub(add(nxt(A_E), mul(TWO32, cur(NE_C))), cur(A_D)), cur(T1_LO)));
+        gc.put(boolc(cur(NE_C)));
+        gc.skip(2);
+        f.w = gc.w;
     } else {
         f.skip(12);
     }
-    // H-state: add-with-carry at finalisation, hold otherwise
+    // H-state: add-with-carry at finalisation, hold otherwise (GROUPS: both
+    // forms at the same slot, to the finalisation and "not finalisation" sums)
 #pragma unroll 1
     for (int k = 0; k < 8; k++) {
-        if (fin) {
+        if (GROUPS) {
+            const uint64_t dl = sub(add(cur(H0 + k), cur(A_A + k)), nxt(H0 + k));
+            ff.w = fn.w = f.w;
+            ff.put(mul(dl, sub(dl, TWO32)));
+            fn.put(sub(nxt(H0 + k), cur(H0 + k)));
+            f.skip(1);
+        } else if (fin) {
             const uint64_t dl = sub(add(cur(H0 + k), cur(A_A + k)), nxt(H0 + k));
             f.put(mul(dl, sub(dl, TWO32)));
         } else {
@@ -307,13 +328,42 @@ __global__ __launch_bounds__(AT) void compose_sha256_k(const uint64_t *lde, uint
     for (int k = 0; k < 14; k++) f.put(sub(nxt(WW0 + k), cur(WW0 + k + 1)));
     f.put(sub(nxt(WW0 + 14), cur(COL_W)));
     if (sched) {
+        gs.w = f.w;
         const uint64_t sum = add(add(add(cur(WW0), cur(SS0_W)), cur(WW0 + 9)), cur(SS1_W));
-        f.put(sub(add(nxt(COL_W), mul(TWO32, cur(WS_CW))), sum));
-        f.put(boolc(cur(WS_C0)));
-        f.put(boolc(cur(WS_C1)));
-        f.put(sub(sub(cur(WS_CW), cur(WS_C0)), mul(2, cur(WS_C1))));
+        gs.put(sub(add(nxt(COL_W), mul(TWO32, cur(WS_CW))), sum));
+        gs.put(boolc(cur(WS_C0)));
+        gs.put(boolc(cur(WS_C1)));
+        gs.put(sub(sub(cur(WS_CW), cur(WS_C0)), mul(2, cur(WS_C1))));
+        f.w = gs.w;
     }
-    phi[n * blockIdx.y + i] = f.acc;
+    if (GROUPS) {
+        uint64_t *o = phi + (uint64_t)blockIdx.y * NG * n + i;
+        o[G_ALL * n] = f.acc;
+        o[G_COMP * n] = fc.acc;
+        o[G_FIN * n] = ff.acc;
+        o[G_NOTFIN * n] = fn.acc;
+        o[G_SCHED * n] = fs.acc;
+    } else {
+        phi[n * blockIdx.y + i] = f.acc;
+    }
+}
+
+// phi at every point from the interpolated gate-group polynomials
+// pg[(k * NG + g) * n + i]: the groups whose gate holds at the point's trace
+// row, less lambda_T1 K[t] on compression rows (the T1 identity's constant)
+__global__ __launch_bounds__(256) void combine_sha256_k(const uint64_t *pg, uint64_t n, uint64_t blowup,
+                                                        const uint32_t *n_blocks, const uint64_t *t1_lam,
+                                                        uint64_t *phi) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t row = i / blowup, blk = row >> 7, br = row & 127;
+    const bool past = blk >= n_blocks[blockIdx.y];
+    const bool comp = !past && br < 64, fin = !past && br == 64, sched = !past && br >= 15 && br <= 62;
+    const uint64_t *q = pg + (uint64_t)blockIdx.y * NG * n + i;
+    uint64_t v = add(q[G_ALL * n], fin ? q[G_FIN * n] : q[G_NOTFIN * n]);
+    if (comp) v = sub(add(v, q[G_COMP * n]), mul(*t1_lam, KD[br]));
+    if (sched) v = add(v, q[G_SCHED * n]);
+    phi[n * blockIdx.y + i] = v;
 }
 
 }  // namespace sha
@@ -414,8 +464,45 @@ void merge_sha256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trac
     SG_HIP(hipMemcpyAsync(d_nb.p, n_blocks.data(), 4 * K, hipMemcpyHostToDevice, c->stream));
     {
         ProfScope ps(c, "compose_sha256", (8.0 * SHA256_W * n + 8.0 * n) * K);
-        hipLaunchKernelGGL(sha::compose_sha256_k, dim3((uint32_t)((n + sha::AT - 1) / sha::AT), K), dim3(sha::AT), 0,
-                           c->stream, lde, n, blowup, d_nb.as<uint32_t>(), phi.as<uint64_t>());
+        hipLaunchKernelGGL(sha::compose_sha256_k<false>, dim3((uint32_t)((n + sha::AT - 1) / sha::AT), K),
+                           dim3(sha::AT), 0, c->stream, lde, n, blowup, d_nb.as<uint32_t>(), phi.as<uint64_t>(),
+                           (uint64_t *)nullptr);
+        SG_HIP(hipGetLastError());
+    }
+    quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
+}
+
+void merge_sha256_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint64_t n_trace, uint64_t blowup,
+                                   const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval) {
+    SG_REQUIRE(is_pow2(n_trace) && n_trace >= 128 && is_pow2(blowup) && blowup >= 4,
+               "merge_sha256: n_trace must be a power of two >= 128, blowup >= 4");
+    SG_REQUIRE(K >= 1 && K <= 65535 && n_blocks.size() == K, "merge_sha256: one n_blocks per proof");
+    const uint64_t n = n_trace * blowup, n2 = 2 * n_trace;
+    SG_REQUIRE(n <= (1ULL << 24), "merge_sha256: domain above 2^24");
+    for (uint32_t nb : n_blocks) SG_REQUIRE(nb >= 1 && 128 * (uint64_t)nb <= n_trace, "merge_sha256: bad n_blocks");
+    if (!phi_computed(phi_mode, n)) {
+        fill_u64(c, c_eval, n * K, 0);
+        return;
+    }
+    Scoped phi(c, 8 * n * K), d_nb(c, 4 * K), t1(c, 8);
+    SG_HIP(hipMemcpyAsync(d_nb.p, n_blocks.data(), 4 * K, hipMemcpyHostToDevice, c->stream));
+    {
+        Scoped groups(c, 8ULL * sha::NG * n2 * K);
+        {
+            Scoped lde2(c, 8ULL * SHA256_W * n2 * K);
+            lde_columns(c, d_tr, lde2.as<uint64_t>(), ilog2(n_trace), 1, (uint64_t)SHA256_W * K, n_trace, n2, 1);
+            ProfScope ps(c, "compose_sha256", (8.0 * SHA256_W * n2 + 8.0 * sha::NG * n2) * K);
+            hipLaunchKernelGGL(sha::compose_sha256_k<true>, dim3((uint32_t)((n2 + sha::AT - 1) / sha::AT), K),
+                               dim3(sha::AT), 0, c->stream, lde2.as<uint64_t>(), n2, (uint64_t)2,
+                               d_nb.as<uint32_t>(), groups.as<uint64_t>(), t1.as<uint64_t>());
+            SG_HIP(hipGetLastError());
+        }
+        Scoped full(c, 8ULL * sha::NG * n * K);
+        lde_columns(c, groups.as<uint64_t>(), full.as<uint64_t>(), ilog2(n2), ilog2(blowup / 2),
+                    (uint64_t)sha::NG * K, n2, n, 1);
+        ProfScope ps(c, "combine_sha256", 8.0 * (sha::NG + 1) * n * K);
+        hipLaunchKernelGGL(sha::combine_sha256_k, dim3((uint32_t)((n + 255) / 256), K), dim3(256), 0, c->stream,
+                           full.as<uint64_t>(), n, blowup, d_nb.as<uint32_t>(), t1.as<uint64_t>(), phi.as<uint64_t>());
         SG_HIP(hipGetLastError());
     }
     quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);

@@ -42,5 +42,20 @@ void merge_sha256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trac
 // deep_ali_merge_sha512 (lib.rs:561-653), same batching
 void merge_sha512_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup,
                         const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval);
+// The same c_eval from the K traces themselves (d_tr: K x W x n_trace),
+// without the blowup-x LDE. Every constraint has degree <= 2 in the cells of
+// the current and next row, and the next row's polynomial is T(w_n x), so
+// each gate group's weighted sum (ungated; compression rows; finalisation
+// row; other rows; schedule rows) is a polynomial of degree <= 2 n_trace - 2
+// in the point: it is evaluated on the 2 n_trace points of a blowup-2 LDE
+// with the gates ignored and interpolated to all n points, where the gates of
+// each point's trace row select the groups and the round constants K[t]
+// (the one row-dependent term, in the T1 identity) are subtracted. Exact
+// arithmetic makes the result equal to merge_sha256_batch / _sha512_batch
+// on the full LDE. blowup >= 4.
+void merge_sha256_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint64_t n_trace, uint64_t blowup,
+                                   const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval);
+void merge_sha512_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint64_t n_trace, uint64_t blowup,
+                                   const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval);
 
 }  // namespace sg

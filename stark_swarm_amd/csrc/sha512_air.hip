@@ -211,8 +211,17 @@ __device__ __forceinline__ uint64_t boolc(uint64_t b) { return mul(b, sub(1, b))
 
 constexpr int AT = 128;
 
+// gate groups, as for SHA-256: ungated, compression rows, the finalisation
+// row, every other row, message-schedule rows
+enum { G_ALL, G_COMP, G_FIN, G_NOTFIN, G_SCHED, NG };
+
+// GROUPS = false: phi at every point. GROUPS = true: the gate groups'
+// weighted sums with the gates ignored and the round constant left out,
+// phi[(k * NG + g) * n + i]; t1_lam[0..1] get the weights of the two T1
+// limb identities.
+template <bool GROUPS>
 __global__ __launch_bounds__(AT) void compose_sha512_k(const uint64_t *lde, uint64_t n, uint64_t blowup,
-                                                       const uint32_t *n_blocks, uint64_t *phi) {
+                                                       const uint32_t *n_blocks, uint64_t *phi, uint64_t *t1_lam) {
     const uint64_t i = (uint64_t)blockIdx.x * AT + threadIdx.x;
     if (i >= n) return;
     const uint64_t *L = lde + (uint64_t)SHA512_W * n * blockIdx.y;
@@ -220,10 +229,12 @@ __global__ __launch_bounds__(AT) void compose_sha512_k(const uint64_t *lde, uint
     auto cur = [&](uint32_t c) { return L[(uint64_t)c * n + i]; };
     auto nxt = [&](uint32_t c) { return L[(uint64_t)c * n + ni]; };
     const uint64_t row = i / blowup, blk = row >> 7, br = row & 127;
-    const bool past = blk >= n_blocks[blockIdx.y];
-    const bool comp = !past && br < 80, fin = !past && br == 80, sched = !past && br >= 15 && br <= 78;
+    const bool past = GROUPS || blk >= n_blocks[blockIdx.y];
+    const bool comp = GROUPS || (!past && br < 80), fin = !past && br == 80;
+    const bool sched = GROUPS || (!past && br >= 15 && br <= 78);
     const uint64_t TWO32 = 1ULL << 32;
-    Phi f;
+    Phi f, fc, ff, fn, fs;  // ungated; compression, finalisation, not finalisation, schedule (GROUPS mode)
+    Phi &gc = GROUPS ? fc : f, &gs = GROUPS ? fs : f;
     const uint32_t boff[8] = {BIT_A, BIT_B, BIT_C, BIT_E, BIT_F, BIT_G, BIT_WW14, BIT_WW1};
 #pragma unroll 1
     for (int o = 0; o < 8; o++)
@@ -285,16 +296,18 @@ __global__ __launch_bounds__(AT) void compose_sha512_k(const uint64_t *lde, uint
     f.put(sub(cur(MAJ_LO), pack(MAJ_BIT)));
     f.put(sub(cur(MAJ_LO + 1), pack(MAJ_BIT + 32)));
     // T1 per limb (carry-in from the lo limb), T2 per limb
-    const uint64_t kt = comp ? KD[br] : 0;
+    const uint64_t kt = GROUPS ? 0 : comp ? KD[br] : 0;
     auto carry3 = [&](uint32_t c0, uint32_t cw) {
         f.put(boolc(cur(c0)));
         f.put(boolc(cur(c0 + 1)));
         f.put(boolc(cur(c0 + 2)));
         f.put(sub(sub(sub(cur(cw), cur(c0)), mul(2, cur(c0 + 1))), mul(4, cur(c0 + 2))));
     };
+    if (GROUPS && i == 0 && blockIdx.y == 0) t1_lam[0] = f.w;
     f.put(sub(add(cur(T1L_LO), mul(TWO32, cur(T1L_CW))),
               add(add(add(add(cur(A0 + 14), cur(S1_LO)), cur(CH_LO)), kt & 0xFFFFFFFFu), cur(W_LO))));
     carry3(T1L_C0, T1L_CW);
+    if (GROUPS && i == 0 && blockIdx.y == 0) t1_lam[1] = f.w;
     f.put(sub(add(cur(T1H_LO), mul(TWO32, cur(T1H_CW))),
               add(add(add(add(add(cur(A0 + 15), cur(S1_LO + 1)), cur(CH_LO + 1)), kt >> 32), cur(W_LO + 1)),
                   cur(T1L_CW))));
@@ -306,21 +319,23 @@ __global__ __launch_bounds__(AT) void compose_sha512_k(const uint64_t *lde, uint
     f.put(boolc(cur(T2H_C0)));
     f.put(sub(cur(T2H_CW), cur(T2H_C0)));
     if (comp) {
+        gc.w = f.w;
         // shifts b<-a, c<-b, d<-c, f<-e, g<-f, h<-g, both limbs
         const uint32_t dst[6] = {1, 2, 3, 5, 6, 7};
 #pragma unroll
         for (int k = 0; k < 6; k++) {
-            f.put(sub(nxt(A0 + 2 * dst[k]), cur(A0 + 2 * (dst[k] - 1))));
-            f.put(sub(nxt(A0 + 2 * dst[k] + 1), cur(A0 + 2 * (dst[k] - 1) + 1)));
+            gc.put(sub(nxt(A0 + 2 * dst[k]), cur(A0 + 2 * (dst[k] - 1))));
+            gc.put(sub(nxt(A0 + 2 * dst[k] + 1), cur(A0 + 2 * (dst[k] - 1) + 1)));
         }
-        f.put(sub(sub(add(nxt(A0), mul(TWO32, cur(NA_LO))), cur(T1L_LO)), cur(T2L_LO)));
-        f.put(boolc(cur(NA_LO)));
-        f.put(sub(sub(sub(add(nxt(A0 + 1), mul(TWO32, cur(NA_HI))), cur(T1H_LO)), cur(T2H_LO)), cur(NA_LO)));
-        f.put(boolc(cur(NA_HI)));
-        f.put(sub(sub(add(nxt(A0 + 8), mul(TWO32, cur(NE_LO))), cur(A0 + 6)), cur(T1L_LO)));
-        f.put(boolc(cur(NE_LO)));
-        f.put(sub(sub(sub(add(nxt(A0 + 9), mul(TWO32, cur(NE_HI))), cur(A0 + 7)), cur(T1H_LO)), cur(NE_LO)));
-        f.put(boolc(cur(NE_HI)));
+        gc.put(sub(sub(add(nxt(A0), mul(TWO32, cur(NA_LO))), cur(T1L_LO)), cur(T2L_LO)));
+        gc.put(boolc(cur(NA_LO)));
+        gc.put(sub(sub(sub(add(nxt(A0 + 1), mul(TWO32, cur(NA_HI))), cur(T1H_LO)), cur(T2H_LO)), cur(NA_LO)));
+        gc.put(boolc(cur(NA_HI)));
+        gc.put(sub(sub(add(nxt(A0 + 8), mul(TWO32, cur(NE_LO))), cur(A0 + 6)), cur(T1L_LO)));
+        gc.put(boolc(cur(NE_LO)));
+        gc.put(sub(sub(sub(add(nxt(A0 + 9), mul(TWO32, cur(NE_HI))), cur(A0 + 7)), cur(T1H_LO)), cur(NE_LO)));
+        gc.put(boolc(cur(NE_HI)));
+        f.w = gc.w;
     } else {
         f.skip(20);
     }
@@ -329,16 +344,22 @@ __global__ __launch_bounds__(AT) void compose_sha512_k(const uint64_t *lde, uint
     static_assert((unsigned __int128)INV32 * (1ull << 32) % P == 1, "2^-32");
 #pragma unroll 1
     for (int k = 0; k < 8; k++) {
-        if (fin) {
+        if (GROUPS || fin) {
             const uint64_t dl = sub(add(cur(H0 + 2 * k), cur(A0 + 2 * k)), nxt(H0 + 2 * k));
             const uint64_t dh = sub(add(add(cur(H0 + 2 * k + 1), cur(A0 + 2 * k + 1)), mul(dl, INV32)),
                                     nxt(H0 + 2 * k + 1));
-            f.put(mul(dl, sub(dl, TWO32)));
-            f.put(mul(dh, sub(dh, TWO32)));
-        } else {
-            f.put(sub(nxt(H0 + 2 * k), cur(H0 + 2 * k)));
-            f.put(sub(nxt(H0 + 2 * k + 1), cur(H0 + 2 * k + 1)));
+            Phi &g = GROUPS ? ff : f;
+            g.w = f.w;
+            g.put(mul(dl, sub(dl, TWO32)));
+            g.put(mul(dh, sub(dh, TWO32)));
         }
+        if (GROUPS || !fin) {
+            Phi &g = GROUPS ? fn : f;
+            if (GROUPS) g.w = f.w;
+            g.put(sub(nxt(H0 + 2 * k), cur(H0 + 2 * k)));
+            g.put(sub(nxt(H0 + 2 * k + 1), cur(H0 + 2 * k + 1)));
+        }
+        if (GROUPS) f.skip(2);
     }
 #pragma unroll 2
     for (int k = 0; k < 14; k++) {
@@ -348,18 +369,50 @@ __global__ __launch_bounds__(AT) void compose_sha512_k(const uint64_t *lde, uint
     f.put(sub(nxt(WW0 + 28), cur(W_LO)));
     f.put(sub(nxt(WW0 + 29), cur(W_LO + 1)));
     if (sched) {
+        gs.w = f.w;
         const uint64_t sl = add(add(add(cur(WW0), cur(SS0_LO)), cur(WW0 + 18)), cur(SS1_LO));
-        f.put(sub(add(nxt(W_LO), mul(TWO32, cur(WSL_CW))), sl));
-        f.put(boolc(cur(WSL_C0)));
-        f.put(boolc(cur(WSL_C0 + 1)));
-        f.put(sub(sub(cur(WSL_CW), cur(WSL_C0)), mul(2, cur(WSL_C0 + 1))));
+        gs.put(sub(add(nxt(W_LO), mul(TWO32, cur(WSL_CW))), sl));
+        gs.put(boolc(cur(WSL_C0)));
+        gs.put(boolc(cur(WSL_C0 + 1)));
+        gs.put(sub(sub(cur(WSL_CW), cur(WSL_C0)), mul(2, cur(WSL_C0 + 1))));
         const uint64_t sh = add(add(add(add(cur(WW0 + 1), cur(SS0_LO + 1)), cur(WW0 + 19)), cur(SS1_LO + 1)), cur(WSL_CW));
-        f.put(sub(add(nxt(W_LO + 1), mul(TWO32, cur(WSH_CW))), sh));
-        f.put(boolc(cur(WSH_C0)));
-        f.put(boolc(cur(WSH_C0 + 1)));
-        f.put(sub(sub(cur(WSH_CW), cur(WSH_C0)), mul(2, cur(WSH_C0 + 1))));
+        gs.put(sub(add(nxt(W_LO + 1), mul(TWO32, cur(WSH_CW))), sh));
+        gs.put(boolc(cur(WSH_C0)));
+        gs.put(boolc(cur(WSH_C0 + 1)));
+        gs.put(sub(sub(cur(WSH_CW), cur(WSH_C0)), mul(2, cur(WSH_C0 + 1))));
+        f.w = gs.w;
     }
-    phi[n * blockIdx.y + i] = f.acc;
+    if (GROUPS) {
+        uint64_t *o = phi + (uint64_t)blockIdx.y * NG * n + i;
+        o[G_ALL * n] = f.acc;
+        o[G_COMP * n] = fc.acc;
+        o[G_FIN * n] = ff.acc;
+        o[G_NOTFIN * n] = fn.acc;
+        o[G_SCHED * n] = fs.acc;
+    } else {
+        phi[n * blockIdx.y + i] = f.acc;
+    }
+}
+
+// phi at every point from the interpolated gate-group polynomials, less the
+// round constant's two 32-bit halves in the T1 limb identities on
+// compression rows
+__global__ __launch_bounds__(256) void combine_sha512_k(const uint64_t *pg, uint64_t n, uint64_t blowup,
+                                                        const uint32_t *n_blocks, const uint64_t *t1_lam,
+                                                        uint64_t *phi) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t row = i / blowup, blk = row >> 7, br = row & 127;
+    const bool past = blk >= n_blocks[blockIdx.y];
+    const bool comp = !past && br < 80, fin = !past && br == 80, sched = !past && br >= 15 && br <= 78;
+    const uint64_t *q = pg + (uint64_t)blockIdx.y * NG * n + i;
+    uint64_t v = add(q[G_ALL * n], fin ? q[G_FIN * n] : q[G_NOTFIN * n]);
+    if (comp) {
+        const uint64_t kt = KD[br];
+        v = sub(sub(add(v, q[G_COMP * n]), mul(t1_lam[0], kt & 0xFFFFFFFFu)), mul(t1_lam[1], kt >> 32));
+    }
+    if (sched) v = add(v, q[G_SCHED * n]);
+    phi[n * blockIdx.y + i] = v;
 }
 
 }  // namespace sha5
@@ -454,8 +507,45 @@ void merge_sha512_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trac
     SG_HIP(hipMemcpyAsync(d_nb.p, n_blocks.data(), 4 * K, hipMemcpyHostToDevice, c->stream));
     {
         ProfScope ps(c, "compose_sha512", (8.0 * SHA512_W * n + 8.0 * n) * K);
-        hipLaunchKernelGGL(sha5::compose_sha512_k, dim3((uint32_t)((n + sha5::AT - 1) / sha5::AT), K), dim3(sha5::AT),
-                           0, c->stream, lde, n, blowup, d_nb.as<uint32_t>(), phi.as<uint64_t>());
+        hipLaunchKernelGGL(sha5::compose_sha512_k<false>, dim3((uint32_t)((n + sha5::AT - 1) / sha5::AT), K),
+                           dim3(sha5::AT), 0, c->stream, lde, n, blowup, d_nb.as<uint32_t>(), phi.as<uint64_t>(),
+                           (uint64_t *)nullptr);
+        SG_HIP(hipGetLastError());
+    }
+    quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
+}
+
+void merge_sha512_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint64_t n_trace, uint64_t blowup,
+                                   const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval) {
+    SG_REQUIRE(is_pow2(n_trace) && n_trace >= 128 && is_pow2(blowup) && blowup >= 4,
+               "merge_sha512: n_trace must be a power of two >= 128, blowup >= 4");
+    SG_REQUIRE(K >= 1 && K <= 65535 && n_blocks.size() == K, "merge_sha512: one n_blocks per proof");
+    const uint64_t n = n_trace * blowup, n2 = 2 * n_trace;
+    SG_REQUIRE(n <= (1ULL << 24), "merge_sha512: domain above 2^24");
+    for (uint32_t nb : n_blocks) SG_REQUIRE(nb >= 1 && 128 * (uint64_t)nb <= n_trace, "merge_sha512: bad n_blocks");
+    if (!phi_computed(phi_mode, n)) {
+        fill_u64(c, c_eval, n * K, 0);
+        return;
+    }
+    Scoped phi(c, 8 * n * K), d_nb(c, 4 * K), t1(c, 16);
+    SG_HIP(hipMemcpyAsync(d_nb.p, n_blocks.data(), 4 * K, hipMemcpyHostToDevice, c->stream));
+    {
+        Scoped groups(c, 8ULL * sha5::NG * n2 * K);
+        {
+            Scoped lde2(c, 8ULL * SHA512_W * n2 * K);
+            lde_columns(c, d_tr, lde2.as<uint64_t>(), ilog2(n_trace), 1, (uint64_t)SHA512_W * K, n_trace, n2, 1);
+            ProfScope ps(c, "compose_sha512", (8.0 * SHA512_W * n2 + 8.0 * sha5::NG * n2) * K);
+            hipLaunchKernelGGL(sha5::compose_sha512_k<true>, dim3((uint32_t)((n2 + sha5::AT - 1) / sha5::AT), K),
+                               dim3(sha5::AT), 0, c->stream, lde2.as<uint64_t>(), n2, (uint64_t)2,
+                               d_nb.as<uint32_t>(), groups.as<uint64_t>(), t1.as<uint64_t>());
+            SG_HIP(hipGetLastError());
+        }
+        Scoped full(c, 8ULL * sha5::NG * n * K);
+        lde_columns(c, groups.as<uint64_t>(), full.as<uint64_t>(), ilog2(n2), ilog2(blowup / 2),
+                    (uint64_t)sha5::NG * K, n2, n, 1);
+        ProfScope ps(c, "combine_sha512", 8.0 * (sha5::NG + 1) * n * K);
+        hipLaunchKernelGGL(sha5::combine_sha512_k, dim3((uint32_t)((n + 255) / 256), K), dim3(256), 0, c->stream,
+                           full.as<uint64_t>(), n, blowup, d_nb.as<uint32_t>(), t1.as<uint64_t>(), phi.as<uint64_t>());
         SG_HIP(hipGetLastError());
     }
     quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
