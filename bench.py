@@ -478,11 +478,13 @@ def p256_batch(S, nproofs):
     lat.sort()
     lde_bytes = 8.0 * 151334 * (256 + 8192)
     return {"p256": {"n_trace": 256, "n0": 8192, "columns": 151334, "constraints": 155774,
+                     "note": "reference_lde_bytes_per_proof is the blowup-32 LDE the reference reads in its merge; "
+                             "this library composes the gadget constraints on a blowup-2 LDE and interpolates",
                      "proofs": nproofs, "plain_proofs_per_s": round(nproofs / plain_dt, 2),
                      "batched_proofs_per_s": round(nproofs / bdt, 2), "speedup": round(plain_dt / bdt, 3),
                      "plain_latency_ms_p50": round(1e3 * lat[len(lat) // 2], 3),
                      "plain_latency_ms_p99": round(1e3 * lat[min(len(lat) - 1, (99 * len(lat)) // 100)], 3),
-                     "lde_bytes_per_proof": lde_bytes, "bit_exact_vs_plain": exact}}
+                     "reference_lde_bytes_per_proof": lde_bytes, "bit_exact_vs_plain": exact}}
 
 
 def c4_batch(S, nproofs, log_lo, log_hi):
@@ -514,6 +516,7 @@ def c4_batch(S, nproofs, log_lo, log_hi):
         if (i + 1) % 16 == 0:
             log("sha batch: %d/%d plain proofs, %.1f s" % (i + 1, nproofs, time.perf_counter() - t0))
     plain_dt = time.perf_counter() - t0
+    S.prove_sha_batch(algs, msgs, fsb, heights=hs)  # warm: the batch's buffers, once
     t0 = time.perf_counter()
     outs = S.prove_sha_batch(algs, msgs, fsb, heights=hs)
     bdt = time.perf_counter() - t0
@@ -522,10 +525,13 @@ def c4_batch(S, nproofs, log_lo, log_hi):
         raise SystemExit("batched SHA proofs differ from the single-call proofs")
     return {"sha_batch": {"proofs": nproofs, "algs": "alternating SHA-256 / SHA-512",
                           "trace_rows_log2": [log_lo, log_hi], "message_bytes": [64, 4096],
-                          "total_lde_GB": round(lde_bytes / 1e9, 2),
+                          "reference_lde_GB": round(lde_bytes / 1e9, 2),
                           "plain_proofs_per_s": round(nproofs / plain_dt, 2),
                           "batched_proofs_per_s": round(nproofs / bdt, 2), "speedup": round(plain_dt / bdt, 3),
-                          "batched_lde_GBps": round(lde_bytes / bdt / 1e9, 1), "bit_exact_vs_plain": exact}}
+                          "bit_exact_vs_plain": exact,
+                          "note": "reference_lde_GB is the blowup-32 LDE the reference writes for these proofs; "
+                                  "this library composes the gate-group polynomials on a blowup-2 LDE and "
+                                  "interpolates them, so it never builds that LDE"}}
 
 
 def serve(S, rates, per_rate, workers=4, max_batch=32):

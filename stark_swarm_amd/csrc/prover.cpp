@@ -467,7 +467,8 @@ std::vector<RsaProof> prove_rsa_batch(Ctx *c, const std::vector<RsaRecord> &reco
     size_t free_b = 0, total_b = 0;
     SG_HIP(hipMemGetInfo(&free_b, &total_b));
     const uint64_t budget = (uint64_t)((free_b + c->cached_bytes) * 0.6);
-    const uint64_t per = 8ULL * W * (2 * h + 2 * n0) + 8ULL * 16 * n0;
+    // trace, its coefficients and blowup-2 LDE, the group polynomials, phi, FRI
+    const uint64_t per = 8ULL * W * 4 * h + 8ULL * 3 * 2 * h + 8ULL * 24 * n0;
     SG_REQUIRE(per <= budget, "prove_rsa: one proof does not fit in device memory");
     const size_t cap = (size_t)std::min<uint64_t>(std::min<uint64_t>(4096, 65535 / R), budget / per);
     const size_t nw = (K + cap - 1) / cap, chunk = (K + nw - 1) / nw;
@@ -476,20 +477,17 @@ std::vector<RsaProof> prove_rsa_batch(Ctx *c, const std::vector<RsaRecord> &reco
         const uint32_t G = (uint32_t)std::min(chunk, K - k0);
         Scoped d_c(c, 8ULL * n0 * G);
         {
-            Scoped d_lde(c, 8ULL * W * n0 * G);
+            Scoped d_tr(c, 8ULL * W * h * G);
             {
-                Scoped d_tr(c, 8ULL * W * h * G);
-                {
-                    HostScope hs(c, "rsa_trace");
-                    const std::vector<RsaRecord> part(records.begin() + k0 * R, records.begin() + (k0 + G) * R);
-                    build_rsa_traces_dev(c, part, R, h, d_tr.as<uint64_t>());
-                }
-                HostScope hs(c, "rsa_lde_merge");
-                lde_columns(c, d_tr.as<uint64_t>(), d_lde.as<uint64_t>(), ilog2(h), ilog2(SWARM_BLOWUP), W * G, h,
-                            n0, 1);
-                merge_rsa_batch(c, d_lde.as<uint64_t>(), G, R, h, SWARM_BLOWUP, d_c.as<uint64_t>());
-                if (c->profiling) SG_HIP(hipStreamSynchronize(c->stream));
+                HostScope hs(c, "rsa_trace");
+                const std::vector<RsaRecord> part(records.begin() + k0 * R, records.begin() + (k0 + G) * R);
+                build_rsa_traces_dev(c, part, R, h, d_tr.as<uint64_t>());
             }
+            // the merge from the traces: gate-group polynomials on a blowup-2
+            // LDE, interpolated (no blowup-32 LDE is built)
+            HostScope hs(c, "rsa_lde_merge");
+            merge_rsa_from_trace_batch(c, d_tr.as<uint64_t>(), G, R, h, SWARM_BLOWUP, d_c.as<uint64_t>());
+            if (c->profiling) SG_HIP(hipStreamSynchronize(c->stream));
         }
         HostScope hf(c, "rsa_fri");
         FriParams p;

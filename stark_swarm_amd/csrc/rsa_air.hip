@@ -32,6 +32,7 @@
 #include "air.hpp"
 #include "bigint.hpp"
 #include "gl.hpp"
+#include "ntt.hpp"
 #include "rsa_air.hpp"
 
 namespace sg {
@@ -217,13 +218,20 @@ constexpr int SL = 8;  // slices (waves) per workgroup
 constexpr int PT = 64;  // points per workgroup
 
 // phi of K proofs (grid.y) of R stacked records at n0 points
+// gate groups: ungated, the output row's EM check, the transitions
+enum { G_ALL, G_OUT, G_TRANS, NG };
+
+// GROUPS = false: phi at every point. GROUPS = true: the three gate groups'
+// weighted sums with the gates ignored, phi[(k * NG + g) * n0 + pt].
+template <bool GROUPS>
 __global__ __launch_bounds__(SL * PT) void compose_rsa_k(const uint64_t *lde, uint64_t n0, uint64_t blowup,
                                                         uint32_t R, uint64_t *phi) {
     __shared__ uint64_t red[SL][4][PT];
     const uint32_t lane = threadIdx.x & (PT - 1), s = threadIdx.x / PT;
     const uint64_t pt = (uint64_t)blockIdx.x * PT + lane, np = (pt + blowup) & (n0 - 1);
     const uint64_t n_trace = n0 / blowup, trow = pt / blowup;
-    const bool out_row = trow == OUT_ROW, trans = trow + 1 < n_trace;
+    const bool out_row = GROUPS || trow == OUT_ROW, trans = GROUPS || trow + 1 < n_trace;
+    uint64_t out_lane = 0, trans_lane = 0;  // GROUPS: the EM and transition sums
     uint64_t phi_lane = 0;
     for (uint32_t rec = 0; rec < R; rec++) {
         const uint64_t *col = lde + ((uint64_t)blockIdx.y * R + rec) * RSA_W * n0;
@@ -258,6 +266,8 @@ __global__ __launch_bounds__(SL * PT) void compose_rsa_k(const uint64_t *lde, ui
         }
         __syncthreads();  // red is written again for the next record
         Acc pos{0, 0}, neg{0, 0}, sel{0, 0};
+        Acc epos{0, 0}, eneg{0, 0}, tpos{0, 0}, tneg{0, 0};  // GROUPS: EM and transition sums
+        Acc &ep = GROUPS ? epos : pos, &en = GROUPS ? eneg : neg, &tp = GROUPS ? tpos : pos, &tn = GROUPS ? tneg : neg;
         const uint32_t L0 = O + K_POS;
 #pragma unroll 1
         for (int u = 0; u < NL / SL; u++) {
@@ -294,8 +304,8 @@ __global__ __launch_bounds__(SL * PT) void compose_rsa_k(const uint64_t *lde, ui
             acc_mad(pos, qq, O + K_QPACK + i);
             acc_mad(neg, acc_val(pk), O + K_QPACK + i);
             if (out_row) {
-                acc_mad(pos, cc, O + K_EM + i);
-                acc_mad(neg, AT(C_EM + i), O + K_EM + i);
+                acc_mad(ep, cc, O + K_EM + i);
+                acc_mad(en, AT(C_EM + i), O + K_EM + i);
             }
         }
         // transitions into the next row (a loop of its own keeps the bit
@@ -304,14 +314,14 @@ __global__ __launch_bounds__(SL * PT) void compose_rsa_k(const uint64_t *lde, ui
 #pragma unroll 1
             for (int u = 0; u < NL / SL; u++) {
                 const uint32_t i = s + SL * u;
-                acc_mad(pos, NX(C_ACC + i), O + K_TRANS + i);
-                acc_mad(neg, AT(C_C + i), O + K_TRANS + i);
-                acc_mad(pos, NX(C_S + i), O + K_TRANS + NL + i);
-                acc_mad(neg, AT(C_S + i), O + K_TRANS + NL + i);
-                acc_mad(pos, NX(C_N + i), O + K_TRANS + 2 * NL + i);
-                acc_mad(neg, AT(C_N + i), O + K_TRANS + 2 * NL + i);
-                acc_mad(pos, NX(C_EM + i), O + K_TRANS + 3 * NL + i);
-                acc_mad(neg, AT(C_EM + i), O + K_TRANS + 3 * NL + i);
+                acc_mad(tp, NX(C_ACC + i), O + K_TRANS + i);
+                acc_mad(tn, AT(C_C + i), O + K_TRANS + i);
+                acc_mad(tp, NX(C_S + i), O + K_TRANS + NL + i);
+                acc_mad(tn, AT(C_S + i), O + K_TRANS + NL + i);
+                acc_mad(tp, NX(C_N + i), O + K_TRANS + 2 * NL + i);
+                acc_mad(tn, AT(C_N + i), O + K_TRANS + 2 * NL + i);
+                acc_mad(tp, NX(C_EM + i), O + K_TRANS + 3 * NL + i);
+                acc_mad(tn, AT(C_EM + i), O + K_TRANS + 3 * NL + i);
             }
         }
         // carries: booleanity, and sc_m (lambda_{m+1} - 2^26 lambda_m) from the
@@ -337,15 +347,47 @@ __global__ __launch_bounds__(SL * PT) void compose_rsa_k(const uint64_t *lde, ui
 #undef AT
 #undef NX
         phi_lane = add(phi_lane, sub(sub(acc_val(pos), acc_val(neg)), mul(ph, acc_val(sel))));
+        if (GROUPS) {
+            out_lane = add(out_lane, sub(acc_val(epos), acc_val(eneg)));
+            trans_lane = add(trans_lane, sub(acc_val(tpos), acc_val(tneg)));
+        }
     }
     red[s][0][lane] = phi_lane;
+    red[s][1][lane] = out_lane;
+    red[s][2][lane] = trans_lane;
     __syncthreads();
     if (s == 0) {
-        uint64_t v = 0;
+        uint64_t v = 0, vo = 0, vt = 0;
 #pragma unroll
-        for (int t = 0; t < SL; t++) v = add(v, red[t][0][lane]);
-        phi[(uint64_t)blockIdx.y * n0 + pt] = v;
+        for (int t = 0; t < SL; t++) {
+            v = add(v, red[t][0][lane]);
+            vo = add(vo, red[t][1][lane]);
+            vt = add(vt, red[t][2][lane]);
+        }
+        if (GROUPS) {
+            uint64_t *o = phi + (uint64_t)blockIdx.y * NG * n0 + pt;
+            o[G_ALL * n0] = v;
+            o[G_OUT * n0] = vo;
+            o[G_TRANS * n0] = vt;
+        } else {
+            phi[(uint64_t)blockIdx.y * n0 + pt] = v;
+        }
     }
+}
+
+// phi at every point from the interpolated gate-group polynomials
+// pg[(k * NG + g) * n + i]: the EM check on the output row, the transitions
+// on every row but the last
+__global__ __launch_bounds__(256) void combine_rsa_k(const uint64_t *pg, uint64_t n, uint64_t blowup,
+                                                     uint64_t *phi) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t trow = i / blowup, n_trace = n / blowup;
+    const uint64_t *q = pg + (uint64_t)blockIdx.y * NG * n + i;
+    uint64_t v = q[G_ALL * n];
+    if (trow == OUT_ROW) v = add(v, q[G_OUT * n]);
+    if (trow + 1 < n_trace) v = add(v, q[G_TRANS * n]);
+    phi[n * blockIdx.y + i] = v;
 }
 
 }  // namespace rsa
@@ -392,8 +434,38 @@ void merge_rsa_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint32_t R, uint64
     Scoped phi(c, 8 * n * K);
     {
         ProfScope ps(c, "compose_rsa", (8.0 * RSA_W * n * R + 8.0 * n) * K);
-        hipLaunchKernelGGL(rsa::compose_rsa_k, dim3((uint32_t)(n / rsa::PT), K), dim3(rsa::SL * rsa::PT), 0,
+        hipLaunchKernelGGL(rsa::compose_rsa_k<false>, dim3((uint32_t)(n / rsa::PT), K), dim3(rsa::SL * rsa::PT), 0,
                            c->stream, lde, n, blowup, R, phi.as<uint64_t>());
+        SG_HIP(hipGetLastError());
+    }
+    quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
+}
+
+void merge_rsa_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint32_t R, uint64_t n_trace,
+                                uint64_t blowup, uint64_t *c_eval) {
+    SG_REQUIRE(is_pow2(n_trace) && n_trace >= 32 && is_pow2(blowup) && blowup >= 4,
+               "merge_rsa: n_trace must be a power of two >= 32 and blowup >= 4");
+    SG_REQUIRE(K >= 1 && K <= 65535, "merge_rsa: 1 to 65535 proofs");
+    SG_REQUIRE(R >= 1 && R <= RSA_MAX_RECORDS, "merge_rsa: 1 to 4096 records per proof");
+    const uint64_t n = n_trace * blowup, n2 = 2 * n_trace;
+    SG_REQUIRE(n <= (1ULL << 24), "merge_rsa: domain above 2^24");
+    Scoped phi(c, 8 * n * K);
+    {
+        Scoped groups(c, 8ULL * rsa::NG * n2 * K);
+        {
+            Scoped lde2(c, 8ULL * RSA_W * R * n2 * K);
+            lde_columns(c, d_tr, lde2.as<uint64_t>(), ilog2(n_trace), 1, (uint64_t)RSA_W * R * K, n_trace, n2, 1);
+            ProfScope ps(c, "compose_rsa", (8.0 * RSA_W * n2 * R + 8.0 * rsa::NG * n2) * K);
+            hipLaunchKernelGGL(rsa::compose_rsa_k<true>, dim3((uint32_t)(n2 / rsa::PT), K), dim3(rsa::SL * rsa::PT), 0,
+                               c->stream, lde2.as<uint64_t>(), n2, (uint64_t)2, R, groups.as<uint64_t>());
+            SG_HIP(hipGetLastError());
+        }
+        Scoped full(c, 8ULL * rsa::NG * n * K);
+        lde_columns(c, groups.as<uint64_t>(), full.as<uint64_t>(), ilog2(n2), ilog2(blowup / 2),
+                    (uint64_t)rsa::NG * K, n2, n, 1);
+        ProfScope ps(c, "combine_rsa", 8.0 * (rsa::NG + 1) * n * K);
+        hipLaunchKernelGGL(rsa::combine_rsa_k, dim3((uint32_t)((n + 255) / 256), K), dim3(256), 0, c->stream,
+                           full.as<uint64_t>(), n, blowup, phi.as<uint64_t>());
         SG_HIP(hipGetLastError());
     }
     quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);

@@ -38,6 +38,15 @@ void build_rsa_traces_dev(Ctx *c, const std::vector<RsaRecord> &records, uint32_
 // c_eval at c_eval + k*n0. The streaming merges always evaluate phi.
 void merge_rsa_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint32_t R, uint64_t n_trace, uint64_t blowup,
                      uint64_t *c_eval);
+// The same c_eval from the K traces themselves (d_tr: K x RSA_W R x
+// n_trace), without the blowup-x LDE: every constraint has degree <= 2 in the
+// cells of the current and next row, so each gate group's weighted sum
+// (ungated; the EM check of the output row; the transitions) is a
+// polynomial of degree <= 2 n_trace - 2, evaluated on a blowup-2 LDE and
+// interpolated; the gates of each point's trace row then select the groups.
+// Exact arithmetic makes the result equal to merge_rsa_batch. blowup >= 4.
+void merge_rsa_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint32_t R, uint64_t n_trace,
+                                uint64_t blowup, uint64_t *c_eval);
 // emsa_pkcs1_v1_5_encode_sha256 (crates/deep_ali/src/rsa2048.rs) for k = 256
 void rsa_pkcs1_em_sha256(const uint8_t digest[32], uint8_t em[256]);
 

@@ -15,7 +15,7 @@ static const uint64_t BLOWUP = 32;
 uint64_t Server::need_bytes(const ServeRequest &r) {
     if (r.kind == SERVE_RSA) {
         const uint64_t h = 32, n0 = h * BLOWUP;
-        return 8ULL * RSA_W * (2 * h + 2 * n0) + 8ULL * 16 * n0;
+        return 8ULL * RSA_W * 4 * h + 8ULL * 3 * 2 * h + 8ULL * 24 * n0;
     }
     if (r.kind == SERVE_P256) {  // trace, its blowup-2 LDE and partial sums, the tail LDE, FRI
         const uint64_t h = P256_ECDSA_ROWS, n0 = h * BLOWUP;
@@ -23,7 +23,7 @@ uint64_t Server::need_bytes(const ServeRequest &r) {
     }
     if (r.kind == SERVE_DS_KSK) {
         const uint64_t h = sha256_trace_height(r.key.size()), n0 = h * BLOWUP;
-        return 8ULL * SHA256_W * (2 * h + 2 * n0) + 8ULL * 16 * n0;
+        return 8ULL * SHA256_W * 4 * h + 8ULL * 5 * 2 * h + 8ULL * 24 * n0;
     }
     uint64_t nt = 8;
     while (nt < 4 * r.domains.size()) nt <<= 1;

@@ -12,7 +12,7 @@ import stark_swarm_amd as S
 from stark_swarm_amd.server import ProvingServer
 from stark_swarm_amd.zone import build_synthetic_zone
 
-KSK_NEED_1BLK = 8 * 756 * (2 * 128 + 2 * 4096) + 8 * 16 * 4096  # Server::need_bytes for a one-block key
+KSK_NEED_1BLK = 8 * 756 * 4 * 128 + 8 * 5 * 2 * 128 + 8 * 24 * 4096  # Server::need_bytes for a one-block key
 
 
 def _keys(n, start=0):
