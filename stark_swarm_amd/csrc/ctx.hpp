@@ -89,6 +89,9 @@ struct Ctx {
     // caller must have waited for its upload before it returns
     void *up = nullptr;
     size_t up_bytes = 0;
+    // shift power tables (s^e for e < 2^26, two levels of 8192) by shift,
+    // built once per context
+    std::map<uint64_t, uint64_t *> pow_tables;
     void *upload_staging(size_t n);
     // copy n bytes of device memory to host memory after the stream's
     // earlier work; returns once dst holds them

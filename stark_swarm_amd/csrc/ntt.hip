@@ -439,6 +439,13 @@ static uint32_t pick_log_g(uint64_t ng, uint32_t log_s) {
 // Power table s^e, e < 2^26, two levels of 2^13 entries, built on the host.
 PowTable make_pow_table(Ctx *c, uint64_t s) {
     PowTable t;
+    auto it = c->pow_tables.find(s);
+    if (it != c->pow_tables.end()) {
+        t.buf = it->second;
+        t.lo = t.buf;
+        t.hi = t.buf + 8192;
+        return t;
+    }
     std::vector<uint64_t> tab(16384);
     const uint64_t s13 = pow(s, 8192);
     tab[0] = tab[8192] = 1;
@@ -451,6 +458,7 @@ PowTable make_pow_table(Ctx *c, uint64_t s) {
     SG_HIP(hipStreamSynchronize(c->stream));
     t.lo = t.buf;
     t.hi = t.buf + 8192;
+    c->pow_tables[s] = t.buf;  // kept (as a live block) until the context is destroyed
     return t;
 }
 
@@ -627,7 +635,11 @@ void lde_columns(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint32
         b.log_g = pick_log_g(B * n1, L2);
         launch_pass(c, b, "lde_passB");
     }
-    if (use_shift) c->release(sp.buf);
+}
+
+void lde_odd_coset(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint64_t ncols) {
+    const uint64_t n = 1ULL << L;
+    lde_columns(c, cols, out, L, 0, ncols, n, n, root_of_unity(2 * n));
 }
 
 }  // namespace sg

@@ -48,6 +48,10 @@ PowTable make_pow_table(Ctx *c, uint64_t s);
 void launch_pass(Ctx *c, NttPass p, const char *name);
 void ntt_columns(Ctx *c, const uint64_t *in, uint64_t *out, uint64_t *scratch, uint32_t L, uint64_t ncols,
                  uint64_t in_cs, uint64_t out_cs, bool inv, const PowTable *spow);
+// The odd half of a blowup-2 LDE: out[c][j] = T_c(w_{2n} w_n^j), the points
+// w_{2n}^(2j+1) of <w_{2n}> (the even points w_{2n}^(2j) = w_n^j carry the
+// trace values themselves); cols and out are ncols x n, n = 2^L.
+void lde_odd_coset(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint64_t ncols);
 void lde_columns(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint32_t LB, uint64_t ncols,
                  uint64_t in_cs, uint64_t out_cs, uint64_t shift);
 

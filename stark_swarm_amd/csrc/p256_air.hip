@@ -627,15 +627,18 @@ __device__ __forceinline__ void signed_chain(Sums &S, const uint64_t *col, uint6
 // phi partial sums of one sub-gadget (blockIdx.y < ng) or of the boundary and
 // transitions (blockIdx.y == ng, launched when grid.y = ng + 1) at 64 points,
 // proof blockIdx.z whose wstride columns start at lde + k * wstride * n0:
-// part[(k * gridDim.y + g) * n0 + pt]
-__global__ __launch_bounds__(64) void compose_p256_k(const uint64_t *lde, uint64_t wstride, uint64_t n0,
-                                                    uint64_t blowup, const GDesc *gd, uint32_t ng, TailDesc td,
-                                                    uint64_t *part) {
+// part[(k * gridDim.y + g) * n0 + pt]. With odd set (the gadget sums on a
+// blowup-2 LDE) the even points are rows pt / 2 of lde (the traces, n0 / 2
+// rows) and the odd points rows of odd.
+__global__ __launch_bounds__(64) void compose_p256_k(const uint64_t *lde, const uint64_t *odd, uint64_t wstride,
+                                                    uint64_t n0, uint64_t blowup, const GDesc *gd, uint32_t ng,
+                                                    TailDesc td, uint64_t *part) {
     const uint32_t g = blockIdx.y;
     const uint64_t k = blockIdx.z;
     const uint64_t pt = (uint64_t)blockIdx.x * 64 + threadIdx.x;
-    const uint64_t *col = lde + k * wstride * n0;
-#define AT(c) col[(uint64_t)(c) * n0 + pt]
+    const uint64_t rows = odd ? n0 / 2 : n0, at = odd ? pt >> 1 : pt;
+    const uint64_t *col = (odd && (pt & 1) ? odd : lde) + k * wstride * rows;
+#define AT(c) col[(uint64_t)(c) * rows + at]
     Sums S;
     if (g == ng) {
         const uint64_t np = (pt + blowup) & (n0 - 1), n_trace = n0 / blowup, trow = pt / blowup;
@@ -663,8 +666,8 @@ __global__ __launch_bounds__(64) void compose_p256_k(const uint64_t *lde, uint64
         const uint32_t s = d.s;
         switch (d.kind) {
         case GM: {  // eval_mul_gadget (p256_field_air.rs:1044-1123)
-            range(S, col, n0, pt, d.c, d.cb, s);
-            range(S, col, n0, pt, d.q, d.qb, s + EC);
+            range(S, col, rows, at, d.c, d.cb, s);
+            range(S, col, rows, at, d.q, d.qb, s + EC);
             const uint32_t L0 = s + 2 * EC + 1;  // lambda of position 0
             uint64_t B = 0, B1 = 0;
 #pragma unroll
@@ -695,7 +698,7 @@ __global__ __launch_bounds__(64) void compose_p256_k(const uint64_t *lde, uint64
             break;
         }
         case GA: {  // eval_add_gadget (:404-455)
-            range(S, col, n0, pt, d.c, d.cb, s);
+            range(S, col, rows, at, d.c, d.cb, s);
 #pragma unroll 1
             for (int kk = 0; kk < NL; kk++) {
                 const uint32_t lam = s + EC + kk + 1;
@@ -714,7 +717,7 @@ __global__ __launch_bounds__(64) void compose_p256_k(const uint64_t *lde, uint64
             break;
         }
         case GS: {  // eval_sub_gadget (:636-698)
-            range(S, col, n0, pt, d.c, d.cb, s);
+            range(S, col, rows, at, d.c, d.cb, s);
             const uint32_t s0 = s + EC;
             acc_add(S.pos, (uint64_t)(s0 + 1) * SP + SP1);
 #pragma unroll 1
@@ -723,15 +726,15 @@ __global__ __launch_bounds__(64) void compose_p256_k(const uint64_t *lde, uint64
                 acc_mad(S.neg, AT(d.b + kk), s0 + kk + 1);
                 acc_mad(S.neg, AT(d.c + kk), s0 + kk + 1);
             }
-            signed_chain(S, col, n0, pt, d.pos, d.neg, s0, s0 + NL);
+            signed_chain(S, col, rows, at, d.pos, d.neg, s0, s0 + NL);
             const uint32_t lam = s + EC + 4 * NL + 1;
             acc_mad(S.pos, AT(d.pos + NL - 1), lam);
             acc_mad(S.pos, AT(d.neg + NL - 1), lam);
             break;
         }
         case GF: {  // eval_freeze_gadget (:1449-1531)
-            range(S, col, n0, pt, d.q, d.qb, s);
-            range(S, col, n0, pt, d.c, d.cb, s + EC);
+            range(S, col, rows, at, d.q, d.qb, s);
+            range(S, col, rows, at, d.c, d.cb, s + EC);
             const uint32_t s0 = s + 2 * EC;
             acc_add(S.neg, (uint64_t)(s0 + 1) * SP + SP1);
             Acc M{0, 0};
@@ -745,13 +748,13 @@ __global__ __launch_bounds__(64) void compose_p256_k(const uint64_t *lde, uint64
                 acc_mad(S.neg, dk, sm + kk + 1);
                 acc_mad(M, sub(a, dk), sm + kk + 1);
             }
-            signed_chain(S, col, n0, pt, d.pos, d.neg, s0, s0 + NL);
+            signed_chain(S, col, rows, at, d.pos, d.neg, s0, s0 + NL);
             acc_mad(S.pos, AT(d.pos + NL - 1), s0 + 4 * NL + 1);
             acc_add(S.neg, mul(AT(d.neg + NL - 1), acc_val(M)));
             break;
         }
         default: {  // eval_select_gadget (:1243-1276)
-            range(S, col, n0, pt, d.c, d.cb, s);
+            range(S, col, rows, at, d.c, d.cb, s);
             Acc M{0, 0};
 #pragma unroll 1
             for (int kk = 0; kk < NL; kk++) {
@@ -1108,12 +1111,13 @@ void merge_p256_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint6
         Scoped d_gd(c, sizeof(GDesc) * ng), phi2(c, 8 * n2 * K);
         SG_HIP(hipMemcpyAsync(d_gd.p, L.gd.data(), sizeof(GDesc) * ng, hipMemcpyHostToDevice, c->stream));
         {
-            Scoped lde2(c, 8ULL * P256_W * n2 * K), part(c, 8ULL * ng * n2 * K);
-            lde_columns(c, d_tr, lde2.as<uint64_t>(), ilog2(n_trace), 1, (uint64_t)P256_W * K, n_trace, n2, 1);
+            // the blowup-2 LDE as the traces (its even points) and the odd coset
+            Scoped odd(c, 8ULL * P256_W * n_trace * K), part(c, 8ULL * ng * n2 * K);
+            lde_odd_coset(c, d_tr, odd.as<uint64_t>(), ilog2(n_trace), (uint64_t)P256_W * K);
             {
                 ProfScope ps(c, "compose_p256", (8.0 * P256_W * n2 + 8.0 * ng * n2) * K);
-                hipLaunchKernelGGL(compose_p256_k, dim3((uint32_t)(n2 / 64), ng, K), dim3(64), 0, c->stream,
-                                   lde2.as<uint64_t>(), (uint64_t)P256_W, n2, (uint64_t)2, d_gd.as<GDesc>(), ng,
+                hipLaunchKernelGGL(compose_p256_k, dim3((uint32_t)(n2 / 64), ng, K), dim3(64), 0, c->stream, d_tr,
+                                   odd.as<uint64_t>(), (uint64_t)P256_W, n2, (uint64_t)2, d_gd.as<GDesc>(), ng,
                                    L.tail, part.as<uint64_t>());
                 SG_HIP(hipGetLastError());
             }
@@ -1141,8 +1145,8 @@ void merge_p256_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint6
         {
             ProfScope ps(c, "compose_p256_tail", (8.0 * NT * n + 8.0 * n) * K);
             hipLaunchKernelGGL(compose_p256_k, dim3((uint32_t)(n / 64), 1, K), dim3(64), 0, c->stream,
-                               tlde.as<uint64_t>(), (uint64_t)NT, n, blowup, (const GDesc *)nullptr, 0u, td,
-                               tphi.as<uint64_t>());
+                               tlde.as<uint64_t>(), (const uint64_t *)nullptr, (uint64_t)NT, n, blowup,
+                               (const GDesc *)nullptr, 0u, td, tphi.as<uint64_t>());
             SG_HIP(hipGetLastError());
         }
         hipLaunchKernelGGL(add_into_k, dim3((uint32_t)((n * K + 255) / 256)), dim3(256), 0, c->stream,
@@ -1166,7 +1170,8 @@ void merge_p256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace,
     {
         ProfScope ps(c, "compose_p256", (8.0 * P256_W * n + 8.0 * (ng + 1) * n) * K);
         hipLaunchKernelGGL(compose_p256_k, dim3((uint32_t)(n / 64), ng + 1, K), dim3(64), 0, c->stream, lde,
-                           (uint64_t)P256_W, n, blowup, d_gd.as<GDesc>(), ng, L.tail, part.as<uint64_t>());
+                           (const uint64_t *)nullptr, (uint64_t)P256_W, n, blowup, d_gd.as<GDesc>(), ng, L.tail,
+                           part.as<uint64_t>());
         SG_HIP(hipGetLastError());
     }
     {

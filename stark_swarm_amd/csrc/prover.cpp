@@ -526,7 +526,7 @@ std::vector<P256Proof> prove_p256_batch(Ctx *c, const std::vector<P256Chains> &i
     const uint64_t budget = (uint64_t)((free_b + c->cached_bytes) * 0.6);
     // trace, its LDE coefficients and blowup-2 LDE, the gadget partial sums,
     // the tail LDE, phi, FRI
-    const uint64_t per = h >= 32 ? 8ULL * W * 4 * h + 8ULL * 300 * 2 * h + 8ULL * 200 * n0 + 8ULL * 16 * n0
+    const uint64_t per = h >= 32 ? 8ULL * W * 3 * h + 8ULL * 300 * 2 * h + 8ULL * 200 * n0 + 8ULL * 16 * n0
                                  : 8ULL * W * (3 * h + n0) + 8ULL * 300 * n0 + 8ULL * 16 * n0;
     SG_REQUIRE(per <= budget, "prove_p256: one proof does not fit in device memory");
     const size_t cap = (size_t)std::min<uint64_t>(4096, budget / per);

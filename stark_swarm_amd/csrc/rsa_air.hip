@@ -222,22 +222,28 @@ constexpr int PT = 64;  // points per workgroup
 enum { G_ALL, G_OUT, G_TRANS, NG };
 
 // GROUPS = false: phi at every point. GROUPS = true: the three gate groups'
-// weighted sums with the gates ignored, phi[(k * NG + g) * n0 + pt].
+// weighted sums with the gates ignored, phi[(k * NG + g) * n0 + pt], on the
+// n0 points of a blowup-2 LDE held as its even points (lde: the traces,
+// n0 / 2 rows) and its odd points (odd, the same layout).
 template <bool GROUPS>
-__global__ __launch_bounds__(SL * PT) void compose_rsa_k(const uint64_t *lde, uint64_t n0, uint64_t blowup,
-                                                        uint32_t R, uint64_t *phi) {
+__global__ __launch_bounds__(SL * PT) void compose_rsa_k(const uint64_t *lde, const uint64_t *odd, uint64_t n0,
+                                                        uint64_t blowup, uint32_t R, uint64_t *phi) {
     __shared__ uint64_t red[SL][4][PT];
     const uint32_t lane = threadIdx.x & (PT - 1), s = threadIdx.x / PT;
-    const uint64_t pt = (uint64_t)blockIdx.x * PT + lane, np = (pt + blowup) & (n0 - 1);
+    const uint64_t pt = (uint64_t)blockIdx.x * PT + lane;
+    // GROUPS: point pt is row pt / 2 of the traces (even pt) or of the odd coset
+    const uint64_t rows = GROUPS ? n0 / 2 : n0, at = GROUPS ? pt >> 1 : pt;
+    const uint64_t np = GROUPS ? (at + 1) & (rows - 1) : (pt + blowup) & (n0 - 1);
+    const uint64_t *base = GROUPS && (pt & 1) ? odd : lde;
     const uint64_t n_trace = n0 / blowup, trow = pt / blowup;
     const bool out_row = GROUPS || trow == OUT_ROW, trans = GROUPS || trow + 1 < n_trace;
     uint64_t out_lane = 0, trans_lane = 0;  // GROUPS: the EM and transition sums
     uint64_t phi_lane = 0;
     for (uint32_t rec = 0; rec < R; rec++) {
-        const uint64_t *col = lde + ((uint64_t)blockIdx.y * R + rec) * RSA_W * n0;
+        const uint64_t *col = base + ((uint64_t)blockIdx.y * R + rec) * RSA_W * rows;
         const uint32_t O = rec * RSA_K + 1;  // lambda(slot) = O + slot
-#define AT(c) col[(uint64_t)(c) * n0 + pt]
-#define NX(c) col[(uint64_t)(c) * n0 + np]
+#define AT(c) col[(uint64_t)(c) * rows + at]
+#define NX(c) col[(uint64_t)(c) * rows + np]
         // B, B1 (op2) and N, N1 (modulus) over every limb
         {
             uint64_t b0 = 0, b1 = 0, m0 = 0, m1 = 0;
@@ -435,7 +441,7 @@ void merge_rsa_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint32_t R, uint64
     {
         ProfScope ps(c, "compose_rsa", (8.0 * RSA_W * n * R + 8.0 * n) * K);
         hipLaunchKernelGGL(rsa::compose_rsa_k<false>, dim3((uint32_t)(n / rsa::PT), K), dim3(rsa::SL * rsa::PT), 0,
-                           c->stream, lde, n, blowup, R, phi.as<uint64_t>());
+                           c->stream, lde, (const uint64_t *)nullptr, n, blowup, R, phi.as<uint64_t>());
         SG_HIP(hipGetLastError());
     }
     quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
@@ -453,11 +459,12 @@ void merge_rsa_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint32
     {
         Scoped groups(c, 8ULL * rsa::NG * n2 * K);
         {
-            Scoped lde2(c, 8ULL * RSA_W * R * n2 * K);
-            lde_columns(c, d_tr, lde2.as<uint64_t>(), ilog2(n_trace), 1, (uint64_t)RSA_W * R * K, n_trace, n2, 1);
+            // the blowup-2 LDE as the traces (its even points) and the odd coset
+            Scoped odd(c, 8ULL * RSA_W * R * n_trace * K);
+            lde_odd_coset(c, d_tr, odd.as<uint64_t>(), ilog2(n_trace), (uint64_t)RSA_W * R * K);
             ProfScope ps(c, "compose_rsa", (8.0 * RSA_W * n2 * R + 8.0 * rsa::NG * n2) * K);
             hipLaunchKernelGGL(rsa::compose_rsa_k<true>, dim3((uint32_t)(n2 / rsa::PT), K), dim3(rsa::SL * rsa::PT), 0,
-                               c->stream, lde2.as<uint64_t>(), n2, (uint64_t)2, R, groups.as<uint64_t>());
+                               c->stream, d_tr, odd.as<uint64_t>(), n2, (uint64_t)2, R, groups.as<uint64_t>());
             SG_HIP(hipGetLastError());
         }
         Scoped full(c, 8ULL * rsa::NG * n * K);

@@ -199,16 +199,21 @@ enum { G_ALL, G_COMP, G_FIN, G_NOTFIN, G_SCHED, NG };
 // GROUPS = false: phi at every point, gates from the point's trace row.
 // GROUPS = true: the five gate groups' weighted sums as polynomials of the
 // point, gates ignored and the row constant K[t] left out, written to
-// phi[(k * NG + g) * n + i]; t1_lam gets the weight of the T1 identity.
+// phi[(k * NG + g) * n + i], on the n points of a blowup-2 LDE held as its
+// even points (lde: the traces, n / 2 rows) and its odd points (odd, the
+// same layout); t1_lam gets the weight of the T1 identity.
 template <bool GROUPS>
-__global__ __launch_bounds__(AT) void compose_sha256_k(const uint64_t *lde, uint64_t n, uint64_t blowup,
-                                                       const uint32_t *n_blocks, uint64_t *phi, uint64_t *t1_lam) {
+__global__ __launch_bounds__(AT) void compose_sha256_k(const uint64_t *lde, const uint64_t *odd, uint64_t n,
+                                                       uint64_t blowup, const uint32_t *n_blocks, uint64_t *phi,
+                                                       uint64_t *t1_lam) {
     const uint64_t i = (uint64_t)blockIdx.x * AT + threadIdx.x;
     if (i >= n) return;
-    const uint64_t *L = lde + (uint64_t)SHA256_W * n * blockIdx.y;
-    const uint64_t ni = (i + blowup) & (n - 1);
-    auto cur = [&](uint32_t col) { return L[(uint64_t)col * n + i]; };
-    auto nxt = [&](uint32_t col) { return L[(uint64_t)col * n + ni]; };
+    // GROUPS: point i is row i / 2 of the traces (even i) or of the odd coset
+    const uint64_t rows = GROUPS ? n / 2 : n, at = GROUPS ? i >> 1 : i;
+    const uint64_t *L = (GROUPS && (i & 1) ? odd : lde) + (uint64_t)SHA256_W * rows * blockIdx.y;
+    const uint64_t ni = GROUPS ? (at + 1) & (rows - 1) : (i + blowup) & (n - 1);
+    auto cur = [&](uint32_t col) { return L[(uint64_t)col * rows + at]; };
+    auto nxt = [&](uint32_t col) { return L[(uint64_t)col * rows + ni]; };
     const uint64_t row = i / blowup, blk = row >> 7, br = row & 127;
     const bool past = GROUPS || blk >= n_blocks[blockIdx.y];
     const bool comp = GROUPS || (!past && br < 64), fin = !past && br == 64;
@@ -465,8 +470,8 @@ void merge_sha256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trac
     {
         ProfScope ps(c, "compose_sha256", (8.0 * SHA256_W * n + 8.0 * n) * K);
         hipLaunchKernelGGL(sha::compose_sha256_k<false>, dim3((uint32_t)((n + sha::AT - 1) / sha::AT), K),
-                           dim3(sha::AT), 0, c->stream, lde, n, blowup, d_nb.as<uint32_t>(), phi.as<uint64_t>(),
-                           (uint64_t *)nullptr);
+                           dim3(sha::AT), 0, c->stream, lde, (const uint64_t *)nullptr, n, blowup,
+                           d_nb.as<uint32_t>(), phi.as<uint64_t>(), (uint64_t *)nullptr);
         SG_HIP(hipGetLastError());
     }
     quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
@@ -489,11 +494,12 @@ void merge_sha256_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uin
     {
         Scoped groups(c, 8ULL * sha::NG * n2 * K);
         {
-            Scoped lde2(c, 8ULL * SHA256_W * n2 * K);
-            lde_columns(c, d_tr, lde2.as<uint64_t>(), ilog2(n_trace), 1, (uint64_t)SHA256_W * K, n_trace, n2, 1);
+            // the blowup-2 LDE as the traces (its even points) and the odd coset
+            Scoped odd(c, 8ULL * SHA256_W * n_trace * K);
+            lde_odd_coset(c, d_tr, odd.as<uint64_t>(), ilog2(n_trace), (uint64_t)SHA256_W * K);
             ProfScope ps(c, "compose_sha256", (8.0 * SHA256_W * n2 + 8.0 * sha::NG * n2) * K);
             hipLaunchKernelGGL(sha::compose_sha256_k<true>, dim3((uint32_t)((n2 + sha::AT - 1) / sha::AT), K),
-                               dim3(sha::AT), 0, c->stream, lde2.as<uint64_t>(), n2, (uint64_t)2,
+                               dim3(sha::AT), 0, c->stream, d_tr, odd.as<uint64_t>(), n2, (uint64_t)2,
                                d_nb.as<uint32_t>(), groups.as<uint64_t>(), t1.as<uint64_t>());
             SG_HIP(hipGetLastError());
         }
