@@ -398,6 +398,29 @@ def test_sha_batch_matches_oracle(S):
         assert mixed[i].proof_blob == one.proof_blob and mixed[i].n_trace == hs[i]
 
 
+def test_sha_tall_traces_match_oracle(S):
+    """the provers' merge from the traces at heights whose blowup-2 LDE and
+    interpolation take the two-pass transforms (SHA-256 at 2^12 rows, 11
+    blocks then padding rows; SHA-512 at 2^11 rows), against the oracle's
+    direct evaluation on the blowup-32 LDE"""
+    say("prove_sha_batch tall traces")
+    m256, m512 = b"q" * 700, b"z" * 300
+    f256, f512 = b"\x05" * 32, b"\x06" * 32
+    o256 = S.prove_sha_batch(256, [m256], [f256], heights=[4096])[0]
+    assert o256.proof_blob == _oracle_sha_proof(256, m256, f256, height=4096)[0]
+    o512 = S.prove_sha_batch(512, [m512], [f512], heights=[2048])[0]
+    assert o512.proof_blob == _oracle_sha_proof(512, m512, f512, height=2048)[0]
+
+
+def test_sha256_two_pass_odd_coset_matches_oracle(S):
+    """SHA-256 at 2^13 rows: the provers' odd-coset transform and the
+    interpolation both take the two-pass NTT, against the oracle"""
+    say("prove_sha_batch 2^13 rows")
+    m, f = b"q" * 700, b"\x07" * 32
+    o = S.prove_sha_batch(256, [m], [f], heights=[8192])[0]
+    assert o.proof_blob == _oracle_sha_proof(256, m, f, height=8192)[0]
+
+
 def test_server_results_equal_standalone(S):
     """Requests admitted by the proving server in a mix (several workers,
     continuous batching, submitters on two threads) give the same bytes as
