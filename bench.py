@@ -211,6 +211,9 @@ def main():
                     help="run only this plain-vs-batched line (for profiling it) and print its JSON")
     ap.add_argument("--shards-only", action="store_true",
                     help="run only the inner-shard plain-vs-batched measurement (for profiling that path)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1: contexts time the pass splits of each transform shape on first use (in the warm-ups) "
+                         "and keep the fastest; 0: default splits")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the prover core runs on a single MI355X and this build has no "
@@ -221,6 +224,7 @@ def main():
         return
 
     import stark_swarm_amd as S
+    S.set_autotune_default(bool(args.autotune))
     if args.shards_only:
         args.only = "shards"
     if args.only:
@@ -273,6 +277,11 @@ def main():
     dt = time.perf_counter() - t0
     ctx.profile(False)
     stats = ctx.profile_stats()
+    # the plans the warm-up tuned for this workload's transforms
+    plans = {"%s 2^%d x%d, 2^%d cols" % (k[0], k[1], 1 << k[2], k[3]):
+             {"split": [best, k[1] - best] if best < k[1] else [best],
+              "ms": {str(s): round(v, 4) for s, v in sorted(ms.items())}}
+             for k, (best, ms) in sorted(S.autotune_report().items())}
 
     ms_step = 1e3 * dt / args.steps
     # stage accounting
@@ -322,6 +331,9 @@ def main():
                      "note": "integer-VALU bound (64-bit modular products on 32-bit lanes): the launch issues "
                              "VALU instructions in valu_issue_load of the SIMD cycles, well before HBM limits"},
         "kernels": kernels,
+        "autotune": {"enabled": bool(args.autotune), "plans": plans,
+                     "note": "split = log2 sizes of the two passes (one entry: one pass with the cosets as "
+                             "repetitions); ms = one timed run per candidate split in the warm-up"},
     }
     if args.streams > 1:
         bres = batched(S, args, w, n, blowup, params, ref_blob, ms_step)
@@ -543,6 +555,13 @@ def c4_batch(S, nproofs, log_lo, log_hi):
     fsb = [rng.bytes(32) for _ in range(nproofs)]
     lde_bytes = sum(8 * (756 if a == 256 else 1510) * 32 * h for a, h in zip(algs, hs))
     S.prove_sha_batch([256, 512], [b"w", b"w"], fsb[:2])  # warm
+    # one untimed single proof per (alg, height): the first call of a shape
+    # tunes its transform plans when autotune is on
+    shapes = {}
+    for i in range(nproofs):
+        shapes.setdefault((algs[i], hs[i]), i)
+    for i in shapes.values():
+        S.prove_sha_batch(algs[i], [msgs[i]], [fsb[i]], heights=[hs[i]])
     t0 = time.perf_counter()
     plain = []
     for i in range(nproofs):
@@ -587,6 +606,9 @@ def serve(S, rates, per_rate, workers=4, max_batch=32):
     p256_pool = synthetic_ecdsa_records(64, 0x5EED000A)
     salt = b"swarm-test-zone1"
     out = []
+    # the workers take the plans tuned so far but time none themselves:
+    # tuning a new batch shape mid-stream would stall the requests behind it
+    S.set_autotune_default(False)
     with ProvingServer(workers=workers, hbm_budget=64 << 30, max_batch=max_batch) as srv:
         # warm every worker's context once
         for rid in [srv.submit_ds_ksk(b"w" * 44, b"\0" * 32, b"\0" * 32) for _ in range(workers)]:

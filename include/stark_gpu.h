@@ -392,6 +392,24 @@ int stark_profile_get(stark_ctx *ctx, const char *name, double *ms, uint64_t *la
 /* Newline-separated list of kernel names seen; returns the length written. */
 size_t stark_profile_names(stark_ctx *ctx, char *buf, size_t cap);
 
+/* ---- launch-plan autotuner (no counterpart in the reference, whose NTTs are
+ * ark-poly's; the pass split of a GPU transform has no effect on its output) ----
+ * With autotune on, the first NTT / iNTT / LDE of each shape (kind, log2
+ * points, log2 blowup, log2 columns) times every admissible split of its two
+ * passes on the context stream and the process keeps the fastest for all
+ * later calls. ctx = NULL sets the default for contexts created afterwards
+ * (the proving server's workers included); STARK_AUTOTUNE=1 in the
+ * environment makes it on from the start. A context with it off times
+ * nothing: it takes the plans already tuned in the process, else the fixed
+ * default splits. */
+int stark_set_autotune(stark_ctx *ctx, int on);
+/* Pin the first pass of every two-pass transform that admits it to
+ * 2^log_first_pass points (3..12; 0 = no pin); used to check each split. */
+int stark_set_ntt_split(stark_ctx *ctx, unsigned log_first_pass);
+/* The tuned plans, one line per shape: "<ntt|intt|lde> L LB log_cols -> L1 |
+ * L1:ms ...". Returns the full length; writes at most cap - 1 bytes + NUL. */
+size_t stark_autotune_report(char *buf, size_t cap);
+
 /* ---- host-side helpers (no device work; used by CPU tests of the host logic) ---- */
 void stark_host_sha3_256(const uint8_t *msg, size_t len, uint8_t out[32]);
 uint64_t stark_host_stdrng_first_u64(uint64_t seed);

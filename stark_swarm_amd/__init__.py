@@ -60,6 +60,8 @@ from .api import (  # noqa: F401
     PHI_ONE_THREAD,
     PHI_SEQUENTIAL,
     build_params,
+    set_autotune_default,
+    autotune_report,
     deep_ali_merge_general,
     deep_ali_merge_sha256,
     build_sha256_trace_multi,

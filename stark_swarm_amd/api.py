@@ -128,6 +128,9 @@ def lib():
         "stark_profile_get": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                              ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)]),
         "stark_profile_names": (sz, [vp, ctypes.c_char_p, sz]),
+        "stark_set_autotune": (ctypes.c_int, [vp, ctypes.c_int]),
+        "stark_set_ntt_split": (ctypes.c_int, [vp, ctypes.c_uint]),
+        "stark_autotune_report": (sz, [ctypes.c_char_p, sz]),
         "stark_host_sha3_256": (None, [ctypes.c_char_p, sz, ctypes.c_char_p]),
         "stark_host_stdrng_first_u64": (ctypes.c_uint64, [ctypes.c_uint64]),
         "stark_host_dns_leaf_hash": (None, [ctypes.POINTER(_DnsRecord), ctypes.c_char_p, ctypes.c_char_p]),
@@ -261,6 +264,41 @@ class Context:
             lib().stark_profile_get(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(la), ctypes.byref(by))
             out[name] = {"ms": ms.value, "launches": la.value, "bytes": by.value}
         return out
+
+    # launch-plan autotuner (the split of a transform's two passes)
+    def autotune(self, on=True):
+        """Time the admissible pass splits on the first NTT / LDE of each
+        shape and keep the fastest (process-wide); off: default splits."""
+        self._check(lib().stark_set_autotune(self.h, int(on)))
+
+    def ntt_split(self, log_first_pass):
+        """Pin the first pass of every two-pass transform that admits it to
+        2^log_first_pass points (0 = no pin)."""
+        self._check(lib().stark_set_ntt_split(self.h, int(log_first_pass)))
+
+
+def set_autotune_default(on=True):
+    """Autotune for every context created afterwards (the proving server's
+    worker contexts included)."""
+    rc = lib().stark_set_autotune(None, int(on))
+    if rc != 0:
+        raise StarkError(rc, "stark_set_autotune")
+
+
+def autotune_report():
+    """{(kind, log_points, log_blowup, log_cols): (chosen L1, {L1: ms})} of the
+    shapes tuned so far in this process."""
+    n = lib().stark_autotune_report(None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    lib().stark_autotune_report(buf, n + 1)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        head, times = line.split("|")
+        shape, best = head.split("->")
+        kind, L, LB, lc = shape.split()
+        ms = {int(a): float(b) for a, b in (t.split(":") for t in times.split())}
+        out[(kind, int(L), int(LB), int(lc))] = (int(best), ms)
+    return out
 
 
 _default = None

@@ -626,6 +626,32 @@ size_t stark_profile_names(stark_ctx *ctx, char *buf, size_t cap) {
     return all.size();
 }
 
+int stark_set_autotune(stark_ctx *ctx, int on) {
+    if (!ctx) {
+        sg::set_autotune_default(on != 0);
+        return STARK_OK;
+    }
+    if (!ctx->c) return STARK_E_INVALID;
+    ctx->c->autotune = on != 0;
+    return STARK_OK;
+}
+
+int stark_set_ntt_split(stark_ctx *ctx, unsigned log_first_pass) {
+    if (!ctx || !ctx->c || log_first_pass > 12) return STARK_E_INVALID;
+    ctx->c->force_split = log_first_pass;
+    return STARK_OK;
+}
+
+size_t stark_autotune_report(char *buf, size_t cap) {
+    const std::string all = sg::autotune_report();
+    if (buf && cap) {
+        size_t n = all.size() < cap - 1 ? all.size() : cap - 1;
+        memcpy(buf, all.data(), n);
+        buf[n] = 0;
+    }
+    return all.size();
+}
+
 void stark_host_sha3_256(const uint8_t *msg, size_t len, uint8_t out[32]) { sg::sha3_256_host(msg, len, out); }
 
 uint64_t stark_host_stdrng_first_u64(uint64_t seed) { return sg::stdrng_first_u64(seed); }

@@ -1,6 +1,7 @@
 // ctx.cpp -- context creation, root tables, caching allocator, event timing.
 #include "ctx.hpp"
 
+#include <stdlib.h>
 #include <string.h>
 
 #include <condition_variable>
@@ -121,9 +122,27 @@ void Ctx::prof_collect() {
     pending.clear();
 }
 
+namespace {
+std::atomic<int> autotune_dflt{-1};  // -1: not read from the environment yet
+}
+
+bool autotune_default() {
+    int v = autotune_dflt.load();
+    if (v < 0) {
+        const char *e = getenv("STARK_AUTOTUNE");
+        int want = (e && e[0] == '1') ? 1 : 0;
+        autotune_dflt.compare_exchange_strong(v, want);
+        v = autotune_dflt.load();
+    }
+    return v == 1;
+}
+
+void set_autotune_default(bool on) { autotune_dflt.store(on ? 1 : 0); }
+
 Ctx *ctx_create(int device) {
     Ctx *c = new Ctx();
     c->device = device;
+    c->autotune = autotune_default();
     try {
         SG_HIP(hipSetDevice(device));
         SG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));

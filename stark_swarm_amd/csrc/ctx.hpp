@@ -92,6 +92,12 @@ struct Ctx {
     // shift power tables (s^e for e < 2^26, two levels of 8192) by shift,
     // built once per context
     std::map<uint64_t, uint64_t *> pow_tables;
+    // NTT / LDE launch plans: with autotune on, the first transform of each
+    // shape times the admissible pass splits and the process keeps the
+    // fastest; force_split (log2 of the first pass, 0 = off) pins one split
+    // for every two-pass transform it admits.
+    bool autotune = false;
+    uint32_t force_split = 0;
     void *upload_staging(size_t n);
     // copy n bytes of device memory to host memory after the stream's
     // earlier work; returns once dst holds them
@@ -152,6 +158,11 @@ struct HostScope {
 
 Ctx *ctx_create(int device);
 void ctx_destroy(Ctx *c);
+// Whether contexts created from now on tune their NTT / LDE plans (the
+// proving server's worker contexts included). Starts as STARK_AUTOTUNE=1 in
+// the environment, else off.
+bool autotune_default();
+void set_autotune_default(bool on);
 
 // Run fn(i) for i < n on a process-wide pool of up to 16 host threads (the
 // per-proof host work of a batch: transcripts, record hashing,

@@ -27,6 +27,9 @@
 // so pass A reads each coefficient tile once and emits the B pre-twiddled
 // column DFTs (`nr` repetitions), and pass B scatters row DFTs of every r
 // into the interleaved output.
+#include <algorithm>
+#include <mutex>
+
 #include "gl.hpp"
 #include "ctx.hpp"
 #include "ntt.hpp"
@@ -470,36 +473,125 @@ static NttPass base_pass() {
     return p;
 }
 
-// out[col][k] = DFT(in[col])[k] (natural order), optional 1/n and s^k output scaling.
-void ntt_columns(Ctx *c, const uint64_t *in, uint64_t *out, uint64_t *scratch, uint32_t L, uint64_t ncols,
-                 uint64_t in_cs, uint64_t out_cs, bool inv, const PowTable *spow) {
-    SG_REQUIRE(L <= 24, "ntt: size above 2^24");
-    const uint64_t n = 1ULL << L;
-    const uint64_t scale = inv ? sg::inv(n % P) : 1;
-    if (L <= 12) {
-        NttPass p = base_pass();
-        p.in = in;
-        p.out = out;
-        p.nb = 1;
-        p.ng = ncols;
-        p.ig = in_cs;
-        p.ii = 1;
-        p.og = out_cs;
-        p.ok = 1;
-        p.log_s = L;
-        p.log_g = pick_log_g(ncols, L);
-        p.inv = inv;
-        p.scale = scale;
-        if (spow) {
-            p.sp_lo = spow->lo;
-            p.sp_hi = spow->hi;
-            p.sp_k = 1;
-        }
-        launch_pass(c, p, inv ? "intt_small" : "ntt_small");
-        return;
+// ---- launch plans ----
+
+namespace {
+
+std::mutex plan_mu;
+struct PlanEntry {
+    uint32_t best = 0;  // 0 while a context is still timing the candidates
+    std::vector<std::pair<uint32_t, float>> ms;
+};
+std::map<uint32_t, PlanEntry> plan_table;
+
+uint32_t plan_key(uint32_t kind, uint32_t L, uint32_t LB, uint64_t ncols) {
+    uint32_t lc = 0;
+    while (lc < 40 && (2ULL << lc) <= ncols) lc++;
+    return (kind << 24) | (L << 16) | (LB << 8) | lc;
+}
+
+// two-pass L1 window: within 2 of ceil(L/2), both passes of 2^3..2^12 points
+void two_pass_window(uint32_t L, std::vector<uint32_t> &out) {
+    const int mid = (int)(L + 1) / 2;
+    const int lo = std::max({mid - 2, (int)L - 12, 3}), hi = std::min({mid + 2, 12, (int)L - 3});
+    for (int s = lo; s <= hi; s++) out.push_back((uint32_t)s);
+}
+
+// The split for this call: the forced one if it is admissible, else the
+// process's tuned one for the shape, else (with tuning on) the fastest
+// candidate timed now with `run`, else the default. Contexts with tuning
+// off still take plans other contexts have tuned, but never time any.
+template <class Run>
+uint32_t choose_split(Ctx *c, uint32_t kind, uint32_t L, uint32_t LB, uint64_t ncols, const PlanSet &ps, Run run) {
+    if (c->force_split) {
+        for (uint32_t s : ps.cand)
+            if (s == c->force_split) return s;
+        return ps.dflt;
     }
-    const uint32_t L1 = (L + 1) / 2, L2 = L - L1;
-    const uint64_t n1 = 1ULL << L1, n2 = 1ULL << L2;
+    if (ps.cand.size() < 2) return ps.dflt;
+    const uint32_t key = plan_key(kind, L, LB, ncols);
+    {
+        std::lock_guard<std::mutex> g(plan_mu);
+        auto it = plan_table.find(key);
+        if (it != plan_table.end()) return it->second.best ? it->second.best : ps.dflt;
+        if (!c->autotune) return ps.dflt;
+        plan_table[key] = PlanEntry{};  // claimed: concurrent callers take the default meanwhile
+    }
+    PlanEntry e;
+    hipEvent_t a, b;
+    SG_HIP(hipEventCreate(&a));
+    SG_HIP(hipEventCreate(&b));
+    float best_ms = 0;
+    for (uint32_t s : ps.cand) {
+        run(s);  // first launch of a kernel shape (code object, caches) stays out of the timing
+        SG_HIP(hipEventRecord(a, c->stream));
+        run(s);
+        SG_HIP(hipEventRecord(b, c->stream));
+        SG_HIP(hipEventSynchronize(b));
+        float ms = 0;
+        SG_HIP(hipEventElapsedTime(&ms, a, b));
+        e.ms.push_back({s, ms});
+        if (!e.best || ms < best_ms) {
+            e.best = s;
+            best_ms = ms;
+        }
+    }
+    SG_HIP(hipEventDestroy(a));
+    SG_HIP(hipEventDestroy(b));
+    std::lock_guard<std::mutex> g(plan_mu);
+    plan_table[key] = e;
+    return e.best;
+}
+
+}  // namespace
+
+PlanSet ntt_plans(uint32_t L) {
+    PlanSet ps;
+    ps.dflt = L <= 12 ? L : (L + 1) / 2;
+    if (L > 12) two_pass_window(L, ps.cand);
+    return ps;
+}
+
+PlanSet lde_plans(uint32_t L, uint32_t LB) {
+    PlanSet ps;
+    if (L >= 4 && L <= 9 && L + LB <= (uint32_t)TILE_LOG) {
+        ps.dflt = L;  // all cosets of a column in one workgroup: nothing to choose
+        return ps;
+    }
+    ps.dflt = L <= 12 ? L : (L + 1) / 2;
+    if (L <= 12) ps.cand.push_back(L);
+    two_pass_window(L, ps.cand);
+    return ps;
+}
+
+std::string autotune_report() {
+    std::lock_guard<std::mutex> g(plan_mu);
+    static const char *kinds[] = {"ntt", "intt", "lde"};
+    std::string out;
+    char line[96];
+    for (auto &kv : plan_table) {
+        if (!kv.second.best) continue;
+        const uint32_t k = kv.first;
+        snprintf(line, sizeof line, "%s %u %u %u -> %u |", kinds[(k >> 24) % 3], (k >> 16) & 255, (k >> 8) & 255,
+                 k & 255, kv.second.best);
+        out += line;
+        for (auto &m : kv.second.ms) {
+            snprintf(line, sizeof line, " %u:%.4f", m.first, m.second);
+            out += line;
+        }
+        out += "\n";
+    }
+    return out;
+}
+
+namespace {
+
+// DFT of 2^L points per column as two strided passes of 2^L1 and 2^(L - L1)
+// points (four-step), through `scratch`
+void ntt_two_pass(Ctx *c, const uint64_t *in, uint64_t *out, uint64_t *scratch, uint32_t L, uint32_t L1,
+                  uint64_t ncols, uint64_t in_cs, uint64_t out_cs, bool inv, uint64_t scale, const PowTable *spow) {
+    const uint32_t L2 = L - L1;
+    const uint64_t n = 1ULL << L, n1 = 1ULL << L1, n2 = 1ULL << L2;
     NttPass a = base_pass();
     a.in = in;
     a.out = scratch;
@@ -540,6 +632,51 @@ void ntt_columns(Ctx *c, const uint64_t *in, uint64_t *out, uint64_t *scratch, u
     launch_pass(c, b, inv ? "intt_passB" : "ntt_passB");
 }
 
+}  // namespace
+
+// out[col][k] = DFT(in[col])[k] (natural order), optional 1/n and s^k output scaling.
+// A two-pass transform with in == out runs exactly once (no tuning), since a
+// second run would read its own output.
+void ntt_columns(Ctx *c, const uint64_t *in, uint64_t *out, uint64_t *scratch, uint32_t L, uint64_t ncols,
+                 uint64_t in_cs, uint64_t out_cs, bool inv, const PowTable *spow) {
+    SG_REQUIRE(L <= 24, "ntt: size above 2^24");
+    const uint64_t n = 1ULL << L;
+    const uint64_t scale = inv ? sg::inv(n % P) : 1;
+    if (L <= 12) {
+        NttPass p = base_pass();
+        p.in = in;
+        p.out = out;
+        p.nb = 1;
+        p.ng = ncols;
+        p.ig = in_cs;
+        p.ii = 1;
+        p.og = out_cs;
+        p.ok = 1;
+        p.log_s = L;
+        p.log_g = pick_log_g(ncols, L);
+        p.inv = inv;
+        p.scale = scale;
+        if (spow) {
+            p.sp_lo = spow->lo;
+            p.sp_hi = spow->hi;
+            p.sp_k = 1;
+        }
+        launch_pass(c, p, inv ? "intt_small" : "ntt_small");
+        return;
+    }
+    PlanSet ps = ntt_plans(L);
+    if (in == out) ps.cand.clear();
+    const uint32_t L1 = choose_split(c, inv ? 1 : 0, L, 0, ncols, ps, [&](uint32_t s) {
+        ntt_two_pass(c, in, out, scratch, L, s, ncols, in_cs, out_cs, inv, scale, spow);
+    });
+    ntt_two_pass(c, in, out, scratch, L, L1, ncols, in_cs, out_cs, inv, scale, spow);
+}
+
+namespace {
+void lde_forward(Ctx *c, const uint64_t *coef, uint64_t *out, uint32_t L, uint32_t LB, uint32_t L1, uint64_t ncols,
+                 uint64_t out_cs);
+}  // namespace
+
 // Coset LDE of `ncols` columns: coefficients c = iDFT(col) scaled by shift^j,
 // then out[col][blowup*k + r] = DFT_n( c_j * w_{blowup*n}^(r j) )[k] for
 // r < blowup -- the evaluations on shift * <w_{blowup*n}> in natural order.
@@ -547,7 +684,7 @@ void lde_columns(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint32
                  uint64_t in_cs, uint64_t out_cs, uint64_t shift) {
     SG_REQUIRE(L + LB <= 26, "lde: domain above 2^26");
     SG_REQUIRE(L <= 24, "lde: trace above 2^24");
-    const uint64_t n = 1ULL << L, B = 1ULL << LB;
+    const uint64_t n = 1ULL << L;
     Scoped coef(c, 8 * n * ncols);
     PowTable sp{};
     const bool use_shift = shift != 1;
@@ -575,10 +712,29 @@ void lde_columns(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint32
         default: throw Error(E_INTERNAL, "lde: small-LDE case out of range");
         }
         SG_HIP(hipGetLastError());
-    } else if (L <= 12) {
-        // one pass per column tile, the B cosets as repetitions
+    } else {
+        const PlanSet ps = lde_plans(L, LB);
+        const uint64_t *cf = coef.as<uint64_t>();
+        const uint32_t L1 = choose_split(c, 2, L, LB, ncols, ps, [&](uint32_t s) {
+            lde_forward(c, cf, out, L, LB, s, ncols, out_cs);
+        });
+        lde_forward(c, cf, out, L, LB, L1, ncols, out_cs);
+    }
+}
+
+namespace {
+
+// The forward half of a coset LDE from its (shift-scaled) coefficients coef
+// (ncols x 2^L): L1 = L is one pass per column tile with the B cosets as
+// repetitions (L <= 12); otherwise two passes of 2^L1 and 2^(L - L1) points.
+void lde_forward(Ctx *c, const uint64_t *coef, uint64_t *out, uint32_t L, uint32_t LB, uint32_t L1, uint64_t ncols,
+                 uint64_t out_cs) {
+    const uint64_t n = 1ULL << L, B = 1ULL << LB;
+    const uint32_t LN = L + LB;
+    if (L1 == L) {
+        SG_REQUIRE(L <= 12, "lde: one-pass form above 2^12 points");
         NttPass p = base_pass();
-        p.in = coef.as<uint64_t>();
+        p.in = coef;
         p.out = out;
         p.nb = 1;
         p.ng = ncols;
@@ -594,14 +750,15 @@ void lde_columns(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint32
         p.out_rs = 1;
         launch_pass(c, p, "lde_small");
     } else {
-        const uint32_t L1 = (L + 1) / 2, L2 = L - L1;
+        SG_REQUIRE(L1 < L && L - L1 <= 12 && L1 <= 12, "lde: pass split out of range");
+        const uint32_t L2 = L - L1;
         const uint64_t n1 = 1ULL << L1, n2 = 1ULL << L2;
         // scratch [col][k1][r][i2]: row q = k1*B + r of pass B is the k1-th
         // column DFT of coset r, and LDE[B*(k1 + n1*k2) + r] = out[q + B*n1*k2],
         // so pass B writes runs of consecutive q
         Scoped scr(c, 8 * n * B * ncols);
         NttPass a = base_pass();
-        a.in = coef.as<uint64_t>();
+        a.in = coef;
         a.out = scr.as<uint64_t>();
         a.nb = (uint32_t)ncols;
         a.ib = n;
@@ -636,6 +793,8 @@ void lde_columns(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint32
         launch_pass(c, b, "lde_passB");
     }
 }
+
+}  // namespace
 
 void lde_odd_coset(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint64_t ncols) {
     const uint64_t n = 1ULL << L;

@@ -55,4 +55,23 @@ void lde_odd_coset(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint
 void lde_columns(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint32_t LB, uint64_t ncols,
                  uint64_t in_cs, uint64_t out_cs, uint64_t shift);
 
+// Autotuned launch plans. A plan is the log2 size L1 of the first pass of a
+// two-pass transform of 2^L points (the second pass has 2^(L - L1)); for an
+// LDE of 2^L <= 4096 points L1 = L is the one-pass form with the cosets as
+// repetitions. Two-pass candidates: L1 within 2 of ceil(L/2), both passes of
+// 2^3 to 2^12 points. The fastest split depends on the shape (sub-DFTs per
+// 8K tile, the length of coalesced runs, twiddle layers), so it is measured
+// rather than modelled: on the first call per (kind, L, LB, log2 columns)
+// with autotune on, each candidate runs once untimed and once between HIP
+// events on the context stream, and the process-wide table keeps the
+// fastest. Every candidate computes the same exact transform.
+struct PlanSet {
+    std::vector<uint32_t> cand;  // admissible L1 (empty: nothing to choose)
+    uint32_t dflt;               // the untuned choice
+};
+PlanSet ntt_plans(uint32_t L);
+PlanSet lde_plans(uint32_t L, uint32_t LB);
+// one "kind L LB log_cols -> L1 | L1:ms ..." line per tuned shape
+std::string autotune_report();
+
 }  // namespace sg
