@@ -26,7 +26,14 @@
 //   LDE[B*k + r] = DFT_n( c_j * w_{Bn}^(r*j) )[k],  r < B,
 // so pass A reads each coefficient tile once and emits the B pre-twiddled
 // column DFTs (`nr` repetitions), and pass B scatters row DFTs of every r
-// into the interleaved output.
+// into the interleaved output. (Taking the four-step twiddle at pass B's
+// input instead, with pass A's pre-twiddle reduced to w_{B n1}^(r*i1),
+// measured slower on the 2^20 x 128 LDE: pass B's extra products and
+// register spills cost more than pass A saved.)
+//
+// Passes without any twiddle or scale (the LDE's and the forward NTT's
+// second pass) run a PLAIN instantiation with those paths compiled out:
+// fewer live registers than the general kernel at the same 128-VGPR budget.
 #include <algorithm>
 #include <mutex>
 
@@ -148,6 +155,11 @@ __device__ __forceinline__ uint64_t root_pow(const uint64_t *rt_lo, const uint64
     return mul(rt_lo[e26 & 8191], rt_hi[e26 >> 13]);
 }
 
+// w_{2^pre_log}^e for pre_log <= 13: rt_hi[j] = w_{2^26}^(8192 j) = w_{2^13}^j
+__device__ __forceinline__ uint64_t direct_root(const NttPass &p, uint32_t e) {
+    return p.rt_hi[(e & ((1u << p.pre_log) - 1)) << (13 - p.pre_log)];
+}
+
 // radix of the next Stockham step: ceil(remaining / steps_left), steps of <= MAX_LR bits
 constexpr int next_lr(int left) {
     const int steps = (left + MAX_LR - 1) / MAX_LR;
@@ -199,11 +211,28 @@ __device__ __forceinline__ void stockham_steps(uint64_t *lds, uint32_t tile, con
     }
 }
 
-template <int LOG_S, bool INV>
+// Compile-time forms of the pass: MODE_PLAIN has no pre-twiddle, post
+// twiddle or scaling, so it needs fewer live registers than the general
+// kernel at the same 128-VGPR budget (the 1024-point pass spills 16 bytes a
+// lane instead of 72: lde_passB 10.3 -> 9.7 ms on the 2^20 x 128 LDE). A
+// third form with the gather / scatter paths compiled out for passes that
+// read and write g-fast spilled 64 bytes and ran no faster, so it is not
+// kept.
+constexpr int MODE_GENERAL = 0, MODE_PLAIN = 1;
+
+template <int LOG_S, bool INV, int MODE>
 // 512 lanes x 16 elements, at most 128 VGPRs: two blocks (4 waves per SIMD)
 // share a CU, which hides more of the 64-bit multiply latency than 256 x 32
 // at 2 waves per SIMD (about 5 % faster on the 2^20 x 128 LDE)
 __global__ __launch_bounds__(NT, 4) void ntt_tile(NttPass p) {
+    if constexpr (MODE == MODE_PLAIN) {
+        // no pre-twiddle, post twiddle or scaling: constants the compiler
+        // folds, so those paths and their registers drop out
+        p.pre_log = 0;
+        p.post_log = 0;
+        p.scale = 1;
+        p.sp_lo = nullptr;
+    }
     __shared__ uint64_t lds[LDS_ELEMS];
     constexpr uint32_t S = 1u << LOG_S;
     const uint32_t log_g = p.log_g, G = 1u << log_g;
@@ -239,21 +268,32 @@ __global__ __launch_bounds__(NT, 4) void ntt_tile(NttPass p) {
         const uint64_t rr = p.r0 + rbase + rep;
         const bool pre = p.pre_log && rr;
         if (inc_in) {
+            // pre factors along the lane's elements: a geometric sequence
+            // (one product per step), or with pre_direct one table read per
+            // element at exponent ed (mod 2^pre_log, kept in 32 bits)
+            const bool pre_inc = pre && !p.pre_direct;
             uint64_t w = 1, st = 1;
+            uint32_t ed = 0, es = 0;
             if (pre) {
                 const uint64_t e0 = (g0 + gl) * p.pre_g + (uint64_t)il * p.pre_i + b * p.pre_b;
-                w = root_pow(p.rt_lo, p.rt_hi, p.pre_log, rr * e0, false);
-                st = root_pow(p.rt_lo, p.rt_hi, p.pre_log, rr * di * p.pre_i, false);
+                if (p.pre_direct) {
+                    ed = (uint32_t)(rr * e0);
+                    es = (uint32_t)(rr * di * p.pre_i);
+                } else {
+                    w = root_pow(p.rt_lo, p.rt_hi, p.pre_log, rr * e0, false);
+                    st = root_pow(p.rt_lo, p.rt_hi, p.pre_log, rr * di * p.pre_i, false);
+                }
             }
 #pragma unroll 2
             for (int u = 0; u < EPT; u++) {
                 const uint32_t i = il + u * di;
                 if (i < S) {
                     uint64_t x = gok ? in[(int64_t)gl * p.ig + (int64_t)i * p.ii] : 0;
-                    if (pre) x = mul(x, w);
+                    if (pre) x = mul(x, p.pre_direct ? direct_root(p, ed) : w);
                     lds[padi((gl << LOG_S) + i)] = x;
                 }
-                if (pre) w = mul(w, st);
+                if (pre_inc) w = mul(w, st);
+                ed += es;
             }
         } else {
 #pragma unroll 2
@@ -273,7 +313,8 @@ __global__ __launch_bounds__(NT, 4) void ntt_tile(NttPass p) {
                     x = in[(int64_t)g * p.ig + (int64_t)i * p.ii];
                     if (pre) {
                         const uint64_t ee = (g0 + g) * p.pre_g + (uint64_t)i * p.pre_i + b * p.pre_b;
-                        x = mul(x, root_pow(p.rt_lo, p.rt_hi, p.pre_log, ee * rr, false));
+                        x = mul(x, p.pre_direct ? direct_root(p, (uint32_t)(ee * rr))
+                                                : root_pow(p.rt_lo, p.rt_hi, p.pre_log, ee * rr, false));
                     }
                 }
                 lds[padi((g << LOG_S) + i)] = x;
@@ -389,11 +430,11 @@ __global__ __launch_bounds__(NT, 4) void lde_small_tile(const uint64_t *coef, ui
     }
 }
 
-template <bool INV>
+template <bool INV, int MODE>
 void launch_tile(uint32_t log_s, dim3 grid, hipStream_t st, const NttPass &p) {
     switch (log_s) {
 #define SG_CASE(L) \
-    case L: hipLaunchKernelGGL((ntt_tile<L, INV>), grid, dim3(NT), 0, st, p); break;
+    case L: hipLaunchKernelGGL((ntt_tile<L, INV, MODE>), grid, dim3(NT), 0, st, p); break;
         SG_CASE(0) SG_CASE(1) SG_CASE(2) SG_CASE(3) SG_CASE(4) SG_CASE(5) SG_CASE(6)
         SG_CASE(7) SG_CASE(8) SG_CASE(9) SG_CASE(10) SG_CASE(11) SG_CASE(12)
 #undef SG_CASE
@@ -410,6 +451,7 @@ void launch_pass(Ctx *c, NttPass p, const char *name) {
     if (p.nr == 0) p.nr = 1;
     if (p.nb2 == 0) p.nb2 = 1;
     SG_REQUIRE(p.pre_log <= 26 && p.post_log <= 26, "ntt pass: twiddle order above 2^26");
+    p.pre_direct = p.pre_log != 0 && p.pre_log <= 13;
     p.rt_lo = c->rt_lo;
     p.rt_hi = c->rt_hi;
     p.tw = p.inv ? c->tw_i : c->tw_f;
@@ -426,10 +468,13 @@ void launch_pass(Ctx *c, NttPass p, const char *name) {
     p.rs_log = rs_log;
     p.nr >>= rs_log;
     const dim3 grid((uint32_t)(tiles << rs_log));
+    const bool plain = p.pre_log == 0 && p.post_log == 0 && p.scale == 1 && p.sp_lo == nullptr;
     if (p.inv)
-        launch_tile<true>(p.log_s, grid, c->stream, p);
+        launch_tile<true, MODE_GENERAL>(p.log_s, grid, c->stream, p);
+    else if (plain)
+        launch_tile<false, MODE_PLAIN>(p.log_s, grid, c->stream, p);
     else
-        launch_tile<false>(p.log_s, grid, c->stream, p);
+        launch_tile<false, MODE_GENERAL>(p.log_s, grid, c->stream, p);
     SG_HIP(hipGetLastError());
 }
 

@@ -27,8 +27,10 @@ struct NttPass {
     uint32_t rs_log;
     uint32_t log_s, log_g;
     uint32_t inv;
-    // input pre-multiply by w_{2^pre_log}^(r * (g*pre_g + i*pre_i + b*pre_b)) (0 = off)
-    uint32_t pre_log;
+    // input pre-multiply by w_{2^pre_log}^(r * (g*pre_g + i*pre_i + b*pre_b)) (0 = off);
+    // pre_direct (set by launch_pass when pre_log <= 13): each factor is one
+    // read of the w_{2^13} table instead of a product
+    uint32_t pre_log, pre_direct;
     uint64_t pre_g, pre_i, pre_b;
     // output multiply by w_{2^post_log}^(g*k), inverse root when inv (0 = off)
     uint32_t post_log;
