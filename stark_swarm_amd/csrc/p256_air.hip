@@ -344,6 +344,104 @@ Limbs be_limbs(const uint8_t *be) {
     return to_limbs(w, 8);
 }
 
+int cmp8(const uint32_t *a, const uint32_t *b);
+
+// r = c mod p for c < 2^512 (16 little-endian words): the NIST fast
+// reduction for p = 2^256 - 2^224 + 2^192 + 2^96 - 1 (FIPS 186-4 D.2.3,
+// Solinas 1999), s1 + 2 s2 + 2 s3 + s4 + s5 - d1 - d2 - d3 - d4 summed word
+// by word in signed 64 bits, then p added or subtracted until r is in [0, p).
+void p256_reduce(const uint32_t *c, uint32_t r[8]) {
+    const int64_t c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3], c4 = c[4], c5 = c[5], c6 = c[6], c7 = c[7];
+    const int64_t c8 = c[8], c9 = c[9], c10 = c[10], c11 = c[11], c12 = c[12], c13 = c[13], c14 = c[14],
+                  c15 = c[15];
+    int64_t t[8];
+    t[0] = c0 + c8 + c9 - c11 - c12 - c13 - c14;
+    t[1] = c1 + c9 + c10 - c12 - c13 - c14 - c15;
+    t[2] = c2 + c10 + c11 - c13 - c14 - c15;
+    t[3] = c3 + 2 * c11 + 2 * c12 + c13 - c15 - c8 - c9;
+    t[4] = c4 + 2 * c12 + 2 * c13 + c14 - c9 - c10;
+    t[5] = c5 + 2 * c13 + 2 * c14 + c15 - c10 - c11;
+    t[6] = c6 + 3 * c14 + 2 * c15 + c13 - c8 - c9;
+    t[7] = c7 + 3 * c15 + c8 - c10 - c11 - c12 - c13;
+    // value = sum t_j 2^(32 j): carry-propagate to 8 words plus a signed top k
+    int64_t k = 0;
+    for (int j = 0; j < 8; j++) {
+        const int64_t v = t[j] + k;
+        r[j] = (uint32_t)v;
+        k = v >> 32;  // arithmetic shift: floor division by 2^32
+    }
+    // (r, k) is a signed multiple-word value congruent to c; |k| <= 6
+    while (k != 0 || cmp8(r, P_W) >= 0) {
+        int64_t br = 0;
+        if (k < 0) {  // add p
+            for (int j = 0; j < 8; j++) {
+                const int64_t v = (int64_t)r[j] + P_W[j] + br;
+                r[j] = (uint32_t)v;
+                br = v >> 32;
+            }
+        } else {  // subtract p
+            for (int j = 0; j < 8; j++) {
+                const int64_t v = (int64_t)r[j] - (int64_t)P_W[j] + br;
+                r[j] = (uint32_t)v;
+                br = v >> 32;
+            }
+        }
+        k += br;
+    }
+}
+
+// x * y mod 2^256 (8 words each)
+void mul_lo8(const uint32_t *x, const uint32_t *y, uint32_t out[8]) {
+    uint32_t o[8] = {};
+    for (int i = 0; i < 8; i++) {
+        uint64_t carry = 0;
+        for (int j = 0; i + j < 8; j++) {
+            const uint64_t t = (uint64_t)x[i] * y[j] + o[i + j] + carry;
+            o[i + j] = (uint32_t)t;
+            carry = t >> 32;
+        }
+    }
+    memcpy(out, o, 32);
+}
+
+// p^-1 mod 2^256 by Newton's iteration x <- x (2 - p x), from p^-1 = -1 mod 2^32
+const uint32_t *p_inv256() {
+    static const std::array<uint32_t, 8> inv = [] {
+        std::array<uint32_t, 8> x{};
+        x[0] = 0xFFFFFFFF;
+        for (int it = 0; it < 3; it++) {
+            uint32_t px[8], two_m[8];
+            mul_lo8(P_W, x.data(), px);
+            // 2 - p x mod 2^256
+            uint64_t br = 0;
+            for (int j = 0; j < 8; j++) {
+                const uint64_t v = (uint64_t)(j == 0 ? 2 : 0) - px[j] - br;
+                two_m[j] = (uint32_t)v;
+                br = (v >> 63) & 1;
+            }
+            uint32_t nx[8];
+            mul_lo8(x.data(), two_m, nx);
+            memcpy(x.data(), nx, 32);
+        }
+        return x;
+    }();
+    return inv.data();
+}
+
+// c = prod mod p and q = (prod - c) / p for prod = a b < p^2: the division
+// is exact, so q = (prod - c) p^-1 mod 2^256 (q < p < 2^256)
+void p256_divmod(const uint32_t prod[16], uint32_t q[8], uint32_t c[8]) {
+    p256_reduce(prod, c);
+    uint32_t d[8];
+    uint64_t br = 0;
+    for (int j = 0; j < 8; j++) {
+        const uint64_t v = (uint64_t)prod[j] - c[j] - br;
+        d[j] = (uint32_t)v;
+        br = (v >> 63) & 1;
+    }
+    mul_lo8(d, p_inv256(), q);
+}
+
 // fill_mul_gadget (p256_field_air.rs:953-1042): c = a b mod p, q = (a b - c) / p
 // and the biased carries of a b - q p - c = 0; false if the identity fails
 bool row_mul(const Gad &g, const Limbs &a, const Limbs &b, uint64_t *W, Limbs &c) {
@@ -361,12 +459,10 @@ bool row_mul(const Gad &g, const Limbs &a, const Limbs &b, uint64_t *W, Limbs &c
         }
         prod[i + 8] = (uint32_t)carry;
     }
-    int pl = 16;
-    while (pl > 1 && prod[pl - 1] == 0) pl--;
-    uint32_t q[17] = {}, r[8] = {};
-    divmod_words<8>(prod, pl, P_W, 8, q, r);
+    uint32_t q[8], r[8];
+    p256_divmod(prod, q, r);
     c = to_limbs(r, 8);
-    const Limbs ql = to_limbs(q, 9);
+    const Limbs ql = to_limbs(q, 8);
     for (int i = 0; i < NL; i++) {
         W[g.w0 + i] = (uint64_t)c[i];
         W[g.w0 + NL + i] = (uint64_t)ql[i];
@@ -799,6 +895,10 @@ void mulmod(const uint32_t *a, const uint32_t *b, const uint32_t *m, uint32_t *r
         }
         prod[i + 8] = (uint32_t)carry;
     }
+    if (m == P_W) {
+        p256_reduce(prod, r);
+        return;
+    }
     int pl = 16;
     while (pl > 1 && prod[pl - 1] == 0) pl--;
     uint32_t q[17], rr[8] = {};
@@ -911,12 +1011,23 @@ void pt_add(const Pt &P, const Pt &Q, const uint32_t *bc, Pt &R) {
     memcpy(R.y, y3, 32);
     memcpy(R.z, z3, 32);
 }
-void pt_mul(const Pt &P, const uint32_t *k, const uint32_t *bc, Pt &R) {
+// R = k1 P1 + k2 P2 by one double-and-add ladder over both scalars
+// (Shamir's trick: 256 doublings and an addition of P1, P2 or P1 + P2 per
+// nonzero bit pair); the complete formula covers every case
+void pt_mul2(const Pt &P1, const uint32_t *k1, const Pt &P2, const uint32_t *k2, const uint32_t *bc, Pt &R) {
+    Pt both;
+    pt_add(P1, P2, bc, both);
     Pt acc{};
     acc.y[0] = 1;  // the identity (0 : 1 : 0)
     for (int bit = 255; bit >= 0; bit--) {
         pt_add(acc, acc, bc, acc);
-        if ((k[bit / 32] >> (bit % 32)) & 1) pt_add(acc, P, bc, acc);
+        const int b1 = (k1[bit / 32] >> (bit % 32)) & 1, b2 = (k2[bit / 32] >> (bit % 32)) & 1;
+        if (b1 && b2)
+            pt_add(acc, both, bc, acc);
+        else if (b1)
+            pt_add(acc, P1, bc, acc);
+        else if (b2)
+            pt_add(acc, P2, bc, acc);
     }
     R = acc;
 }
@@ -969,16 +1080,14 @@ P256Chains p256_ecdsa_chains(const P256Record &rec) {
     invmod(s, N_W, w);
     mulmod(e, w, N_W, u1);
     mulmod(r, w, N_W, u2);
-    Pt G{}, Q{}, A, B, R;
+    Pt G{}, Q{}, R;
     be_to_words(GX_BE, G.x);
     be_to_words(GY_BE, G.y);
     G.z[0] = 1;
     memcpy(Q.x, qx, 32);
     memcpy(Q.y, qy, 32);
     Q.z[0] = 1;
-    pt_mul(G, u1, bc, A);
-    pt_mul(Q, u2, bc, B);
-    pt_add(A, B, bc, R);
+    pt_mul2(G, u1, Q, u2, bc, R);
     SG_REQUIRE(!zero8(R.z), "ecdsa: signature does not verify");
     uint32_t zi[8], x[8];
     invmod(R.z, P_W, zi);
