@@ -92,6 +92,8 @@ struct Ctx {
     // shift power tables (s^e for e < 2^26, two levels of 8192) by shift,
     // built once per context
     std::map<uint64_t, uint64_t *> pow_tables;
+    // odd-coset twist tables w_{2n}^i / n (i < n) by log2 n, built once per context
+    std::map<uint32_t, uint64_t *> odd_twists;
     // NTT / LDE launch plans: with autotune on, the first transform of each
     // shape times the admissible pass splits and the process keeps the
     // fastest; force_split (log2 of the first pass, 0 = off) pins one split

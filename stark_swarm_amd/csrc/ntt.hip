@@ -433,13 +433,12 @@ __global__ __launch_bounds__(NT, 4) void lde_small_tile(const uint64_t *coef, ui
 // Odd coset of G = 8192 / 2^LOG_S whole columns per workgroup in one pass:
 // out[c][j] = T_c(w_{2n} w_n^j) = DFT_n( iDFT_n(col_c)[i] * w_{2n}^i )[j].
 // The column is read once, inverse-transformed in LDS, twisted by
-// w_{2n}^i / n (w_{2n}^i = rt_hi[i << (12 - LOG_S)], since 2n <= 2^13),
-// transformed forward in LDS and written once: half the HBM traffic and
-// launches of an iNTT pass followed by a one-pass LDE.
+// twist[i] = w_{2n}^i / n, transformed forward in LDS and written once: half
+// the HBM traffic and launches of an iNTT pass followed by a one-pass LDE.
 template <int LOG_S>
 __global__ __launch_bounds__(NT, 4) void odd_coset_tile(const uint64_t *in, uint64_t *out, uint64_t ncols,
-                                                       uint64_t in_cs, uint64_t out_cs, const uint64_t *rt_hi,
-                                                       const uint64_t *tw_f, const uint64_t *tw_i, uint64_t ninv) {
+                                                       uint64_t in_cs, uint64_t out_cs, const uint64_t *twist,
+                                                       const uint64_t *tw_f, const uint64_t *tw_i) {
     __shared__ uint64_t lds[LDS_ELEMS];
     constexpr uint32_t S = 1u << LOG_S;
     constexpr uint32_t LOG_G = TILE_LOG - LOG_S;
@@ -457,7 +456,7 @@ __global__ __launch_bounds__(NT, 4) void odd_coset_tile(const uint64_t *in, uint
 #pragma unroll 2
     for (uint32_t u = 0; u < EPT; u++) {
         const uint32_t e = threadIdx.x + u * NT;
-        lds[padi(e)] = mul(mul(lds[padi(e)], ninv), rt_hi[(e & (S - 1)) << (12 - LOG_S)]);
+        lds[padi(e)] = mul(lds[padi(e)], twist[e & (S - 1)]);
     }
     __syncthreads();
     stockham_steps<LOG_S, 0, false>(lds, TILE, tw_f);
@@ -888,14 +887,25 @@ void lde_odd_coset(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint
     }
     const uint64_t blocks = (ncols + (TILE >> L) - 1) >> (TILE_LOG - L);
     SG_REQUIRE(blocks < (1ULL << 31), "odd coset: grid too large");
+    // twist[i] = w_{2n}^i / n, i < n, built once per context and size
+    uint64_t *&twist = c->odd_twists[L];
+    if (!twist) {
+        std::vector<uint64_t> tw(n);
+        const uint64_t w = root_of_unity(2 * n);
+        tw[0] = sg::inv(n);
+        for (uint64_t i = 1; i < n; i++) tw[i] = mul(tw[i - 1], w);
+        uint64_t *buf = c->alloc_n<uint64_t>(n);  // kept (as a live block) until the context is destroyed
+        SG_HIP(hipMemcpyAsync(buf, tw.data(), 8 * n, hipMemcpyHostToDevice, c->stream));
+        SG_HIP(hipStreamSynchronize(c->stream));
+        twist = buf;
+    }
     ProfScope ps(c, "odd_coset", 16.0 * n * ncols);
     const dim3 grid((uint32_t)blocks);
-    const uint64_t ninv = sg::inv(n);
     switch (L) {
 #define SG_CASE(LS)                                                                                                \
     case LS:                                                                                                       \
-        hipLaunchKernelGGL((odd_coset_tile<LS>), grid, dim3(NT), 0, c->stream, cols, out, ncols, n, n, c->rt_hi, \
-                           c->tw_f, c->tw_i, ninv);                                                                \
+        hipLaunchKernelGGL((odd_coset_tile<LS>), grid, dim3(NT), 0, c->stream, cols, out, ncols, n, n, twist,    \
+                           c->tw_f, c->tw_i);                                                                      \
         break;
         SG_CASE(1) SG_CASE(2) SG_CASE(3) SG_CASE(4) SG_CASE(5) SG_CASE(6)
         SG_CASE(7) SG_CASE(8) SG_CASE(9) SG_CASE(10) SG_CASE(11) SG_CASE(12)
