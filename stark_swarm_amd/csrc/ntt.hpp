@@ -54,6 +54,10 @@ void ntt_columns(Ctx *c, const uint64_t *in, uint64_t *out, uint64_t *scratch, u
 // w_{2n}^(2j+1) of <w_{2n}> (the even points w_{2n}^(2j) = w_n^j carry the
 // trace values themselves); cols and out are ncols x n, n = 2^L.
 void lde_odd_coset(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint64_t ncols);
+// twist[i] = w_{2n}^i / n for i < n = 2^L (L <= 12) on the device: the
+// odd-coset factor between the inverse and the forward transform; built once
+// per context and size
+const uint64_t *odd_twist_table(Ctx *c, uint32_t L);
 void lde_columns(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint32_t LB, uint64_t ncols,
                  uint64_t in_cs, uint64_t out_cs, uint64_t shift);
 

@@ -1,0 +1,180 @@
+// ntt_dev.hpp -- device building blocks of the Goldilocks NTT kernels for
+// gfx950: the 8K-element LDS tile, radix <= 16 register DFTs whose twiddles
+// are powers of 2 (w_32 = 2^78), and the Stockham steps that run S-point
+// sub-DFTs of a tile in place. Shared by ntt.hip and the kernels that fuse a
+// transform with their own loads (p256_air.hip).
+#pragma once
+#include "gl.hpp"
+#include "ntt.hpp"
+
+namespace sg {
+namespace {
+
+constexpr int TILE_LOG = 13;
+constexpr int TILE = 1 << TILE_LOG;
+constexpr int NT = 512;
+constexpr int EPT = TILE / NT;  // elements held per lane during a step
+constexpr int MAX_LR = 4;       // register DFTs of at most 2^MAX_LR points
+constexpr int PAD_SHIFT = 4;
+constexpr int LDS_ELEMS = TILE + (TILE >> PAD_SHIFT);
+
+__device__ __forceinline__ uint32_t padi(uint32_t e) { return e + (e >> PAD_SHIFT); }
+
+// x * 2^k mod p for a compile-time-foldable k in [0, 192): 2^96 = -1.
+// x may be lazily reduced (any u64); the result is canonical.
+__device__ __forceinline__ uint64_t mul_pow2(uint64_t x, int k) {
+    const bool negate = k >= 96;
+    const int s = k % 96;
+    uint64_t y;
+    if (s == 0) {
+        y = canon(x);
+    } else if (s < 32) {
+        // x * 2^s = lo + 2^64 * h, h < 2^32:  lo + h * (2^32 - 1)
+        const uint64_t lo = x << s;
+        const uint32_t h = (uint32_t)(x >> (64 - s));
+        const uint64_t t1 = ((uint64_t)(h ? h - 1 : 0) << 32) | (uint32_t)(0u - h);
+        uint64_t r = lo + t1;
+        if (r < t1) r += EPS;
+        y = canon(r);
+    } else if (s < 64) {
+        y = reduce128(x << s, x >> (64 - s));
+    } else {
+        // s in [64, 96): 2^s = -2^(s-96), so x * 2^s = -(x / 2^t), t = 96 - s.
+        // With x = a 2^t + b: x / 2^t = a - b 2^s and b 2^s = c (2^32 - 1),
+        // c = b << (s - 64) < 2^32; hence x * 2^s = c (2^32 - 1) - a.
+        const int t = 96 - s;
+        const uint64_t a = x >> t;
+        const uint32_t c = (uint32_t)(x << (s - 64));
+        const uint64_t ce = ((uint64_t)(c ? c - 1 : 0) << 32) | (uint32_t)(0u - c);
+        uint64_t r = ce - a;
+        if (ce < a) r -= EPS;  // + p (mod 2^64); ce - a + p > 0 since a < 2^63
+        y = canon(r);
+    }
+    return negate ? neg(y) : y;
+}
+
+// lazily reduced add / sub: a is any u64, b is canonical; results are any u64
+// congruent to a +- b (2^64 = 2^32 - 1 mod p)
+__device__ __forceinline__ uint64_t lz_add(uint64_t a, uint64_t b) {
+    uint64_t r;
+    const bool c = __builtin_add_overflow(a, b, &r);
+    return r + (uint64_t)(c ? 0xFFFFFFFFu : 0u);
+}
+__device__ __forceinline__ uint64_t lz_sub(uint64_t a, uint64_t b) {
+    uint64_t r;
+    const bool c = __builtin_sub_overflow(a, b, &r);
+    return r - (uint64_t)(c ? 0xFFFFFFFFu : 0u);
+}
+
+// w_32^e = 2^(78 e mod 192); inverse: 2^(-78 e mod 192)
+__device__ __forceinline__ int w32_exp(int e, bool inv) {
+    int k = (78 * e) % 192;
+    return inv ? (192 - k) % 192 : k;
+}
+
+template <int LR>
+__device__ __forceinline__ constexpr int brev_c(int x) {
+    int r = 0;
+    for (int i = 0; i < LR; i++) r |= ((x >> i) & 1) << (LR - 1 - i);
+    return r;
+}
+
+// in-register DFT of size R = 2^LR <= 32: v[k] <- sum_r v[r] w_R^(+-rk).
+// Values between butterflies are lazily reduced u64 (LDS holds them that way
+// too); every product and every store to global memory is canonical.
+template <int LR, bool INV>
+__device__ __forceinline__ void dft_reg(uint64_t *v) {
+    constexpr int R = 1 << LR;
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        const int j = brev_c<LR>(i);
+        if (i < j) {
+            uint64_t t = v[i];
+            v[i] = v[j];
+            v[j] = t;
+        }
+    }
+#pragma unroll
+    for (int lh = 0; lh < LR; lh++) {
+        const int half = 1 << lh;
+#pragma unroll
+        for (int base = 0; base < R; base += 2 * half) {
+#pragma unroll
+            for (int j = 0; j < half; j++) {
+                // w_{2 half}^j = w_32^(j * 32 / (2 half))
+                // t canonical (twiddled, or reduced once), u lazy
+                uint64_t t = v[base + j + half];
+                t = (j != 0) ? mul_pow2(t, w32_exp(j * (32 >> (lh + 1)), INV)) : canon(t);
+                const uint64_t u = v[base + j];
+                v[base + j] = lz_add(u, t);
+                v[base + j + half] = lz_sub(u, t);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t root_pow(const uint64_t *rt_lo, const uint64_t *rt_hi, uint32_t lg, uint64_t e,
+                                             bool inv) {
+    uint64_t e26 = (e & ((1ULL << lg) - 1)) << (26 - lg);
+    if (inv) e26 = (0 - e26) & ((1ULL << 26) - 1);
+    return mul(rt_lo[e26 & 8191], rt_hi[e26 >> 13]);
+}
+
+// w_{2^pre_log}^e for pre_log <= 13: rt_hi[j] = w_{2^26}^(8192 j) = w_{2^13}^j
+__device__ __forceinline__ uint64_t direct_root(const NttPass &p, uint32_t e) {
+    return p.rt_hi[(e & ((1u << p.pre_log) - 1)) << (13 - p.pre_log)];
+}
+
+// radix of the next Stockham step: ceil(remaining / steps_left), steps of <= MAX_LR bits
+constexpr int next_lr(int left) {
+    const int steps = (left + MAX_LR - 1) / MAX_LR;
+    return (left + steps - 1) / steps;
+}
+
+template <int LOG_S, int LOG_NS, bool INV>
+__device__ __forceinline__ void stockham_steps(uint64_t *lds, uint32_t tile, const uint64_t *tw) {
+    if constexpr (LOG_NS < LOG_S) {
+        constexpr int LR = next_lr(LOG_S - LOG_NS);
+        constexpr int R = 1 << LR;
+        constexpr int MAXG = EPT / R;
+        constexpr int SUB_LOG = LOG_S - LR;
+        constexpr uint32_t SUB = 1u << SUB_LOG;
+        constexpr uint32_t NS = 1u << LOG_NS;
+        const uint32_t groups = tile >> LR;
+        uint64_t v[MAXG][R];
+#pragma unroll
+        for (int u = 0; u < MAXG; u++) {
+            const uint32_t q = threadIdx.x + u * NT;
+            if (q < groups) {
+                const uint32_t g = q >> SUB_LOG, j = q & (SUB - 1);
+                const uint32_t base = (g << LOG_S) + j;
+#pragma unroll
+                for (int r = 0; r < R; r++) v[u][r] = lds[padi(base + r * SUB)];
+                if constexpr (LOG_NS > 0) {
+                    const uint32_t jm = j & (NS - 1);
+                    const uint64_t *t = tw + (NS << LR);
+#pragma unroll
+                    for (int r = 1; r < R; r++) v[u][r] = mul(v[u][r], t[jm * r]);
+                }
+                dft_reg<LR, INV>(v[u]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < MAXG; u++) {
+            const uint32_t q = threadIdx.x + u * NT;
+            if (q < groups) {
+                const uint32_t g = q >> SUB_LOG, j = q & (SUB - 1);
+                const uint32_t jm = j & (NS - 1);
+                const uint32_t base = (g << LOG_S) + ((j >> LOG_NS) << (LOG_NS + LR)) + jm;
+#pragma unroll
+                for (int r = 0; r < R; r++) lds[padi(base + r * NS)] = v[u][r];
+            }
+        }
+        __syncthreads();
+        stockham_steps<LOG_S, LOG_NS + LR, INV>(lds, tile, tw);
+    }
+}
+
+}  // namespace
+}  // namespace sg

@@ -31,6 +31,7 @@
 #include "gl.hpp"
 #include "keccak.hpp"
 #include "ntt.hpp"
+#include "ntt_dev.hpp"
 #include "p256_air.hpp"
 
 namespace sg {
@@ -669,6 +670,58 @@ __global__ __launch_bounds__(256) void expand_p256_k(const uint64_t *words, cons
     tr[g * P256_W * h + t] = x;
 }
 
+// expand_p256_k and the odd coset in one pass: a workgroup expands the cells
+// of 8192 / h whole columns of proof blockIdx.y (h = 2^LOG_H rows), writes
+// them to the trace and keeps them in LDS, then runs the odd-coset transform
+// of lde_odd_coset on them (inverse steps, twist[i] = w_{2h}^i / h, forward
+// steps) and writes the odd half of the blowup-2 LDE; the trace is not read
+// back from HBM.
+template <int LOG_H>
+__global__ __launch_bounds__(NT, 4) void expand_odd_p256_k(const uint64_t *words, const uint32_t *desc, uint32_t nw,
+                                                           uint64_t *tr, uint64_t *odd, const uint64_t *twist,
+                                                           const uint64_t *tw_f, const uint64_t *tw_i) {
+    __shared__ uint64_t lds[LDS_ELEMS];
+    constexpr uint32_t S = 1u << LOG_H;
+    constexpr uint32_t LOG_G = TILE_LOG - LOG_H;
+    const uint64_t k = blockIdx.y;
+    uint64_t blk = blockIdx.x;
+    if ((gridDim.x & 7) == 0) blk = (blk & 7) * (gridDim.x >> 3) + (blk >> 3);
+    const uint64_t col0 = blk << LOG_G;
+    const uint64_t *wk = words + k * S * nw;
+    uint64_t *trk = tr + k * P256_W * S, *oddk = odd + k * P256_W * S;
+#pragma unroll 2
+    for (uint32_t u = 0; u < EPT; u++) {
+        const uint32_t e = threadIdx.x + u * NT;
+        const uint64_t col = col0 + (e >> LOG_H), row = e & (S - 1);
+        uint64_t x = 0;
+        if (col < P256_W) {
+            const uint32_t d = desc[col];
+            const uint64_t v = wk[row * nw + (d >> 7)];
+            const uint32_t code = d & 127;
+            if (code < 64) x = (v >> code) & 1;
+            else if (code == D_VAL) x = v;
+            else x = code == D_POS ? (uint64_t)(v == 1) : (uint64_t)(v == ~0ULL);
+            trk[col * S + row] = x;
+        }
+        lds[padi(e)] = x;
+    }
+    __syncthreads();
+    stockham_steps<LOG_H, 0, true>(lds, TILE, tw_i);
+#pragma unroll 2
+    for (uint32_t u = 0; u < EPT; u++) {
+        const uint32_t e = threadIdx.x + u * NT;
+        lds[padi(e)] = mul(lds[padi(e)], twist[e & (S - 1)]);
+    }
+    __syncthreads();
+    stockham_steps<LOG_H, 0, false>(lds, TILE, tw_f);
+#pragma unroll 2
+    for (uint32_t u = 0; u < EPT; u++) {
+        const uint32_t e = threadIdx.x + u * NT;
+        const uint64_t col = col0 + (e >> LOG_H);
+        if (col < P256_W) oddk[col * S + (e & (S - 1))] = canon(lds[padi(e)]);
+    }
+}
+
 struct Sums {
     Acc pos{0, 0}, neg{0, 0}, rpos{0, 0}, rneg{0, 0};  // value = pos - neg - 2^26 (rpos - rneg)
     __device__ uint64_t value() const {
@@ -1132,9 +1185,10 @@ void build_p256_trace_host(const P256Chains &in, uint64_t h, uint64_t *tr, std::
 }
 
 void build_p256_traces_dev(Ctx *c, const std::vector<P256Chains> &in, uint64_t h, uint64_t *d_tr,
-                           std::vector<std::array<Be32, 6>> *rproj_out) {
+                           std::vector<std::array<Be32, 6>> *rproj_out, uint64_t *d_odd) {
     const size_t K = in.size();
     SG_REQUIRE(K >= 1 && K <= 65535, "p256 traces: 1 to 65535 proofs");
+    SG_REQUIRE(!d_odd || (h >= 2 && h <= 4096), "p256 traces: the fused odd coset needs 2 to 4096 rows");
     check_inputs(in, h);
     const Layout &L = layout();
     Scoped d_desc(c, 4ULL * P256_W);
@@ -1160,7 +1214,24 @@ void build_p256_traces_dev(Ctx *c, const std::vector<P256Chains> &in, uint64_t h
     if (rproj_out) *rproj_out = rp;
     Scoped d_words(c, 8 * K * per);
     SG_HIP(hipMemcpyAsync(d_words.p, words, 8 * K * per, hipMemcpyHostToDevice, c->stream));
-    {
+    if (d_odd) {
+        const uint32_t lh = ilog2(h);
+        const uint64_t *twist = odd_twist_table(c, lh);
+        const dim3 grid((uint32_t)((P256_W + (TILE >> lh) - 1) >> (TILE_LOG - lh)), (uint32_t)K);
+        ProfScope ps(c, "expand_odd_p256", 24.0 * P256_W * h * K);
+        switch (lh) {
+#define SG_CASE(LH)                                                                                                  \
+    case LH:                                                                                                         \
+        hipLaunchKernelGGL((expand_odd_p256_k<LH>), grid, dim3(NT), 0, c->stream, d_words.as<uint64_t>(),            \
+                           d_desc.as<uint32_t>(), (uint32_t)nw, d_tr, d_odd, twist, c->tw_f, c->tw_i);               \
+        break;
+            SG_CASE(1) SG_CASE(2) SG_CASE(3) SG_CASE(4) SG_CASE(5) SG_CASE(6)
+            SG_CASE(7) SG_CASE(8) SG_CASE(9) SG_CASE(10) SG_CASE(11) SG_CASE(12)
+#undef SG_CASE
+        default: throw Error(E_INTERNAL, "p256 traces: height out of range");
+        }
+        SG_HIP(hipGetLastError());
+    } else {
         ProfScope ps(c, "expand_p256", 8.0 * P256_W * h * K);
         hipLaunchKernelGGL(expand_p256_k, dim3((uint32_t)((h * P256_W + 255) / 256), (uint32_t)K), dim3(256), 0,
                            c->stream, d_words.as<uint64_t>(), d_desc.as<uint32_t>(), (uint32_t)nw, ilog2(h), d_tr);
@@ -1205,7 +1276,7 @@ __global__ __launch_bounds__(256) void add_into_k(uint64_t *a, const uint64_t *b
 }  // namespace p256
 
 void merge_p256_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint64_t n_trace, uint64_t blowup,
-                                 uint64_t *c_eval) {
+                                 uint64_t *c_eval, const uint64_t *d_odd) {
     SG_REQUIRE(is_pow2(n_trace) && n_trace >= 32 && is_pow2(blowup) && blowup >= 2,
                "merge_p256: n_trace must be a power of two >= 32 and blowup >= 2");
     SG_REQUIRE(K >= 1 && K <= 65535, "merge_p256: 1 to 65535 proofs");
@@ -1221,13 +1292,14 @@ void merge_p256_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint6
         SG_HIP(hipMemcpyAsync(d_gd.p, L.gd.data(), sizeof(GDesc) * ng, hipMemcpyHostToDevice, c->stream));
         {
             // the blowup-2 LDE as the traces (its even points) and the odd coset
-            Scoped odd(c, 8ULL * P256_W * n_trace * K), part(c, 8ULL * ng * n2 * K);
-            lde_odd_coset(c, d_tr, odd.as<uint64_t>(), ilog2(n_trace), (uint64_t)P256_W * K);
+            Scoped odd(c, d_odd ? 0 : 8ULL * P256_W * n_trace * K), part(c, 8ULL * ng * n2 * K);
+            if (!d_odd) lde_odd_coset(c, d_tr, odd.as<uint64_t>(), ilog2(n_trace), (uint64_t)P256_W * K);
+            const uint64_t *od = d_odd ? d_odd : odd.as<uint64_t>();
             {
                 ProfScope ps(c, "compose_p256", (8.0 * P256_W * n2 + 8.0 * ng * n2) * K);
                 hipLaunchKernelGGL(compose_p256_k, dim3((uint32_t)(n2 / 64), ng, K), dim3(64), 0, c->stream, d_tr,
-                                   odd.as<uint64_t>(), (uint64_t)P256_W, n2, (uint64_t)2, d_gd.as<GDesc>(), ng,
-                                   L.tail, part.as<uint64_t>());
+                                   od, (uint64_t)P256_W, n2, (uint64_t)2, d_gd.as<GDesc>(), ng, L.tail,
+                                   part.as<uint64_t>());
                 SG_HIP(hipGetLastError());
             }
             {

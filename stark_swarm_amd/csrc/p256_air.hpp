@@ -47,9 +47,11 @@ void build_p256_trace_host(const P256Chains &in, uint64_t h, uint64_t *tr, std::
 // The traces of K chain inputs written to d_tr (proof-major, P256_W x h
 // each): the host computes each row's limbs, quotients and carries, the
 // device expands them into cells. rproj_out (K entries) gets the r_proj
-// values used.
+// values used. With d_odd (same shape as d_tr, h <= 4096) one fused pass
+// also writes the odd half of each column's blowup-2 LDE there (what
+// lde_odd_coset computes from d_tr), without reading the cells back.
 void build_p256_traces_dev(Ctx *c, const std::vector<P256Chains> &in, uint64_t h, uint64_t *d_tr,
-                           std::vector<std::array<Be32, 6>> *rproj_out);
+                           std::vector<std::array<Be32, 6>> *rproj_out, uint64_t *d_odd = nullptr);
 // deep_ali_merge_ecdsa_double_multirow_streaming (crates/deep_ali/src/
 // lib.rs:1421-1540) with lambda_j = j + 1 for K proofs: lde of proof k at
 // lde + P256_W * k * n0, c_eval at c_eval + k * n0. Phi is evaluated at every
@@ -62,9 +64,10 @@ void merge_p256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace,
 // 2 n_trace points of <w_{2 n_trace}> (a blowup-2 LDE of all columns) and
 // interpolated from there; only the 180 columns of the row-gated boundary and
 // transition constraints are extended to all n points. Exact arithmetic
-// makes the result equal to merge_p256_batch on the full LDE.
+// makes the result equal to merge_p256_batch on the full LDE. d_odd, when
+// given, is the odd half of that blowup-2 LDE (build_p256_traces_dev's).
 void merge_p256_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint64_t n_trace, uint64_t blowup,
-                                 uint64_t *c_eval);
+                                 uint64_t *c_eval, const uint64_t *d_odd = nullptr);
 // The chain inputs of an ECDSA record: u1 = e / s and u2 = r / s mod n
 // (e = digest mod n), chain A from G with the MSB-first bits of u1, chain B
 // from Q with those of u2, 256 steps each, r_proj bound to the outputs.

@@ -538,14 +538,18 @@ std::vector<P256Proof> prove_p256_batch(Ctx *c, const std::vector<P256Chains> &i
         {
             std::vector<std::array<Be32, 6>> rp;
             Scoped d_tr(c, 8ULL * W * h * G);
+            // the odd half of the blowup-2 LDE, written by the trace expansion itself
+            const bool fused = h >= 32 && h <= 4096;
+            Scoped d_odd(c, fused ? 8ULL * W * h * G : 0);
             {
                 HostScope hs(c, "p256_trace");
                 const std::vector<P256Chains> part(in.begin() + k0, in.begin() + k0 + G);
-                build_p256_traces_dev(c, part, h, d_tr.as<uint64_t>(), &rp);
+                build_p256_traces_dev(c, part, h, d_tr.as<uint64_t>(), &rp, fused ? d_odd.as<uint64_t>() : nullptr);
             }
             HostScope hs(c, "p256_lde_merge");
             if (h >= 32) {
-                merge_p256_from_trace_batch(c, d_tr.as<uint64_t>(), G, h, SWARM_BLOWUP, d_c.as<uint64_t>());
+                merge_p256_from_trace_batch(c, d_tr.as<uint64_t>(), G, h, SWARM_BLOWUP, d_c.as<uint64_t>(),
+                                            fused ? d_odd.as<uint64_t>() : nullptr);
             } else {  // below 64 points of <w_{2 n_trace}>: compose on the full LDE
                 Scoped d_lde(c, 8ULL * W * n0 * G);
                 lde_columns(c, d_tr.as<uint64_t>(), d_lde.as<uint64_t>(), ilog2(h), ilog2(SWARM_BLOWUP), W * G, h,
