@@ -216,8 +216,9 @@ def main():
                          "and keep the fastest; 0: default splits")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: the prover core runs on a single MI355X and this build has no "
-              "multi-GPU path (see DESIGN.md, sharding). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: the prover core runs on a single MI355X and the library has no "
+              "multi-GPU path (independent proofs shard as one process per GPU). Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
     if args.calib_copy:
         calib_copy()
