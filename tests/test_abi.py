@@ -78,3 +78,13 @@ def test_shard_fs_binding_matches_oracle():
     import stark_swarm_amd as S
     a, b, c = b"\x01" * 32, b"\x02" * 32, b"\x03" * 32
     assert S.shard_fs_binding(a, b, 7, c) == O.shard_fs_binding(a, b, 7, c)
+
+
+def test_autotune_switches_without_a_device():
+    """the process-wide default and the plan report need no device: with
+    nothing tuned the report is empty, and a null context sets the default"""
+    import stark_swarm_amd as S
+    S.set_autotune_default(True)
+    S.set_autotune_default(False)
+    assert S.autotune_report() == {}
+    assert S.lib().stark_set_ntt_split(None, 10) != 0  # a split needs a context
