@@ -162,6 +162,34 @@ int ref_prove_p256_ecdsa(const uint8_t qx[32], const uint8_t qy[32], const uint8
                          const uint8_t digest[32], uint8_t **proof, size_t *proof_len, uint8_t root_f0[32],
                          uint8_t pi_hash[32]);
 
+/* ---- Ed25519 cofactored-verify AIR v16 and prove_zsk_ksk_binding_v2 (ed25519_verify_air.rs) ---- */
+typedef struct ref_ed_ctx ref_ed_ctx;
+/* public inputs of one signature (pk, sig = R || s, message); k_scalar = k (256 in production).
+ * NULL if R or A does not decompress. */
+ref_ed_ctx *ref_ed25519_ctx_new(const uint8_t pk[32], const uint8_t sig[64], const uint8_t *msg, size_t len,
+                                size_t k);
+void ref_ed25519_ctx_free(ref_ed_ctx *c);
+size_t ref_ed25519_width(const ref_ed_ctx *c);
+size_t ref_ed25519_height(const ref_ed_ctx *c);
+size_t ref_ed25519_n_constraints(const ref_ed_ctx *c);
+size_t ref_ed25519_v16_constraints(size_t k); /* verify_v16_per_row_constraints */
+void ref_ed25519_rows(const ref_ed_ctx *c, size_t out[11]);
+void ref_ed25519_columns(const ref_ed_ctx *c, size_t out[26]);
+void ref_ed25519_public(const ref_ed_ctx *c, uint8_t digest[64], uint8_t k_can[32], uint8_t *s_bits,
+                        uint8_t *k_bits);
+void ref_ed25519_points(const ref_ed_ctx *c, uint8_t out[6 * 128]);
+int ref_ed25519_fill(const ref_ed_ctx *c, uint64_t *tr /* width x height */);
+void ref_ed25519_constraints(const ref_ed_ctx *c, const uint64_t *cur, const uint64_t *nxt, size_t row,
+                             uint64_t *out);
+int ref_merge_ed25519(const ref_ed_ctx *c, const uint64_t *lde, size_t blowup, const uint64_t *coeffs,
+                      uint64_t *c_eval);
+int ref_ed25519_verify(const uint8_t pk[32], const uint8_t sig[64], const uint8_t *msg, size_t len);
+void ref_sha512(const uint8_t *msg, size_t len, uint8_t out[64]);
+int ref_prove_zsk_ksk_binding_v2(const uint8_t pk[32], const uint8_t sig[64], const uint8_t *msg, size_t len,
+                                 const uint8_t fs_binding[32], const uint8_t merkle_root[32], size_t k,
+                                 size_t blowup, uint8_t **proof, size_t *proof_len, uint8_t root_f0[32],
+                                 uint8_t pi_hash[32]);
+
 /* ---- prover-core microbench (coset LDE -> row Merkle -> STIR prove) ---- */
 /* trace: w x n_trace column-major; returns proof bytes of deep_fri_prove over
  * f0 = sum_c (c+1) * LDE_c with the row-tree root as public_inputs_hash, and

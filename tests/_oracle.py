@@ -129,6 +129,31 @@ def lib():
         L.ref_prove_p256_ecdsa.restype = ctypes.c_int
         L.ref_prove_p256_ecdsa.argtypes = [ctypes.c_char_p] * 5 + [ctypes.POINTER(u8p), ctypes.POINTER(sz),
                                                                     ctypes.c_char_p, ctypes.c_char_p]
+        vp = ctypes.c_void_p
+        L.ref_ed25519_ctx_new.restype = vp
+        L.ref_ed25519_ctx_new.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz, sz]
+        L.ref_ed25519_ctx_free.argtypes = [vp]
+        for f in ("width", "height", "n_constraints"):
+            getattr(L, "ref_ed25519_" + f).restype = sz
+            getattr(L, "ref_ed25519_" + f).argtypes = [vp]
+        L.ref_ed25519_v16_constraints.restype = sz
+        L.ref_ed25519_v16_constraints.argtypes = [sz]
+        L.ref_ed25519_rows.argtypes = [vp, ctypes.POINTER(sz)]
+        L.ref_ed25519_columns.argtypes = [vp, ctypes.POINTER(sz)]
+        L.ref_ed25519_public.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]
+        L.ref_ed25519_points.argtypes = [vp, ctypes.c_char_p]
+        L.ref_ed25519_fill.restype = ctypes.c_int
+        L.ref_ed25519_fill.argtypes = [vp, u64p]
+        L.ref_ed25519_constraints.argtypes = [vp, u64p, u64p, sz, u64p]
+        L.ref_merge_ed25519.restype = ctypes.c_int
+        L.ref_merge_ed25519.argtypes = [vp, u64p, sz, u64p, u64p]
+        L.ref_ed25519_verify.restype = ctypes.c_int
+        L.ref_ed25519_verify.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz]
+        L.ref_sha512.argtypes = [ctypes.c_char_p, sz, ctypes.c_char_p]
+        L.ref_prove_zsk_ksk_binding_v2.restype = ctypes.c_int
+        L.ref_prove_zsk_ksk_binding_v2.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz,
+                                                   ctypes.c_char_p, ctypes.c_char_p, sz, sz, ctypes.POINTER(u8p),
+                                                   ctypes.POINTER(sz), ctypes.c_char_p, ctypes.c_char_p]
         _lib = L
     return _lib
 
@@ -454,4 +479,100 @@ def prove_p256_ecdsa(qx, qy, r, s, digest):
     rc = lib().ref_prove_p256_ecdsa(_be32(qx), _be32(qy), _be32(r), _be32(s), bytes(digest), ctypes.byref(pp),
                                     ctypes.byref(pl), rf, pi)
     assert rc == 0, rc
+    return take_bytes(pp, pl), rf.raw, pi.raw
+
+
+ED_ROWS = ("k_hash", "reduce", "dec_r", "dec_a", "sb_first", "ka_first", "residual", "dbl1", "dbl2", "dbl3",
+           "result")
+ED_COLS = ("digest_bits", "reduce_r_bits", "dec_x", "acc_x", "cadd_out_x", "sblk_sb", "sblk_ka", "r_thread",
+           "res_x", "res_y", "res_z", "res_t", "th_sb_x", "th_ka_x", "th_r_x", "th_r_t", "sub_x_c", "sub_yz_c",
+           "dbl_in_x", "pa_r2_x3", "dec_sign", "dbl_sqa_carry_bits", "cadd_bit", "dec_x_bits", "reduce_q",
+           "reduce_carry_bits")
+
+
+class Ed25519:
+    """One signature's v16 AIR in the oracle (layout, trace, evaluator, merge)."""
+
+    def __init__(self, pk, sig, msg, k=256):
+        self.pk, self.sig, self.msg, self.k = bytes(pk), bytes(sig), bytes(msg), k
+        h = lib().ref_ed25519_ctx_new(self.pk, self.sig, self.msg, len(self.msg), k)
+        if not h:
+            raise ValueError("R or A does not decompress")
+        self.h = h
+        L = lib()
+        self.width, self.height = L.ref_ed25519_width(h), L.ref_ed25519_height(h)
+        self.n_constraints = L.ref_ed25519_n_constraints(h)
+        a = (sz * 11)()
+        L.ref_ed25519_rows(h, a)
+        self.rows = dict(zip(ED_ROWS, list(a)))
+        b = (sz * 26)()
+        L.ref_ed25519_columns(h, b)
+        self.cols = dict(zip(ED_COLS, list(b)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ref_ed25519_ctx_free(self.h)
+            self.h = None
+
+    def public(self):
+        dg, kc = ctypes.create_string_buffer(64), ctypes.create_string_buffer(32)
+        sb, kb = ctypes.create_string_buffer(self.k), ctypes.create_string_buffer(self.k)
+        lib().ref_ed25519_public(self.h, dg, kc, sb, kb)
+        return dg.raw, kc.raw, list(sb.raw), list(kb.raw)
+
+    def points(self):
+        """{R, A, sB, kA, residual_2, 8 residual_2}: (x, y, z, t) ints"""
+        buf = ctypes.create_string_buffer(6 * 128)
+        lib().ref_ed25519_points(self.h, buf)
+        names = ("R", "A", "sB", "kA", "res2", "res2x8")
+        out = {}
+        for i, n in enumerate(names):
+            out[n] = tuple(int.from_bytes(buf.raw[128 * i + 32 * j:128 * i + 32 * j + 32], "little") for j in range(4))
+        return out
+
+    def trace(self):
+        tr = np.zeros((self.width, self.height), dtype=np.uint64)
+        assert lib().ref_ed25519_fill(self.h, ptr(tr)) == 0
+        return tr
+
+    def constraints(self, cur, nxt, row):
+        cur = np.ascontiguousarray(cur, dtype=np.uint64)
+        nxt = np.ascontiguousarray(nxt, dtype=np.uint64)
+        out = np.zeros(self.n_constraints, dtype=np.uint64)
+        lib().ref_ed25519_constraints(self.h, ptr(cur), ptr(nxt), row, ptr(out))
+        return out
+
+    def merge(self, lde, blowup, coeffs=None):
+        lde = np.ascontiguousarray(lde, dtype=np.uint64)
+        out = np.zeros(self.height * blowup, dtype=np.uint64)
+        cp = None
+        if coeffs is not None:
+            coeffs = np.ascontiguousarray(coeffs, dtype=np.uint64)
+            cp = ptr(coeffs)
+        lib().ref_merge_ed25519(self.h, ptr(lde), blowup, cp, ptr(out))
+        return out
+
+
+def ed25519_v16_constraints(k):
+    return lib().ref_ed25519_v16_constraints(k)
+
+
+def ed25519_verify(pk, sig, msg):
+    return bool(lib().ref_ed25519_verify(bytes(pk), bytes(sig), bytes(msg), len(msg)))
+
+
+def sha512(msg):
+    out = ctypes.create_string_buffer(64)
+    lib().ref_sha512(bytes(msg), len(msg), out)
+    return out.raw
+
+
+def prove_zsk_ksk_binding_v2(pk, sig, msg, fs_binding, merkle_root, k=256, blowup=32):
+    """(proof, root_f0, pi_hash) of prove_zsk_ksk_binding_v2, or the negative code"""
+    pp, pl = u8p(), sz()
+    rf, pi = ctypes.create_string_buffer(32), ctypes.create_string_buffer(32)
+    rc = lib().ref_prove_zsk_ksk_binding_v2(bytes(pk), bytes(sig), bytes(msg), len(msg), bytes(fs_binding),
+                                           bytes(merkle_root), k, blowup, ctypes.byref(pp), ctypes.byref(pl), rf, pi)
+    if rc:
+        return rc
     return take_bytes(pp, pl), rf.raw, pi.raw
