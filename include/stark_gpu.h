@@ -326,6 +326,59 @@ int stark_prove_p256_chains_batch(stark_ctx *ctx, const stark_p256_chains *in, s
 int stark_prove_ecdsa_p256_batch(stark_ctx *ctx, const uint8_t *qx, const uint8_t *qy, const uint8_t *r,
                                  const uint8_t *s, const uint8_t *digest, size_t n_records, stark_p256_out *outs);
 
+/* ---- Ed25519 cofactored-verify AIR v16 (crates/deep_ali/src/ed25519_verify_air.rs) ---- */
+
+/* One ZskKskOutput of prove_zsk_ksk_binding_v2 (crates/swarm-dns/src/prover.rs:856-951). */
+typedef struct {
+    uint8_t pi_hash[32]; /* zsk_ksk_pi_hash_v2_stark (prover.rs:763-781) */
+    uint8_t root_f0[32];
+    size_t n_trace;
+    uint8_t *proof; /* ark-serialize compressed DeepFriProof<SexticExt>; stark_free() it */
+    size_t proof_len;
+} stark_zsk_out;
+
+/* verify_air_layout_v16 sizes (ed25519_verify_air.rs:4741-4818) for a signed
+ * message of msg_len bytes (the SHA-512 input is R || A || M) and k_scalar
+ * ladder bits (256 for Ed25519): trace width, height and
+ * verify_v16_per_row_constraints (:4737). */
+int stark_ed25519_shape(size_t msg_len, size_t k_scalar, size_t *width, size_t *height, size_t *n_constraints);
+
+/* prove_zsk_ksk_binding_v2 (crates/swarm-dns/src/prover.rs:856-951) for n
+ * signatures: signature i is pks[32 i..], sigs[64 i..] (R || s) over
+ * msgs[msg_off[i] .. msg_off[i] + msg_len[i]], bound to fs_bindings[32 i..]
+ * and merkle_roots[32 i..]. k_scalar 256 and blowup 32 with 54 queries are the
+ * reference's; a smaller k_scalar is its layout-test stub, another blowup /
+ * query count a parameter choice (the STIR schedule follows n0). Lockstep
+ * device batches per trace shape in HBM-sized waves; result i equals the
+ * proof of signature i alone. STARK_E_INVALID when a signature does not
+ * verify (the reference asserts). */
+int stark_prove_zsk_ksk_binding_v2_batch(stark_ctx *ctx, const uint8_t *pks, const uint8_t *sigs, const uint8_t *msgs,
+                                         const size_t *msg_off, const size_t *msg_len, const uint8_t *fs_bindings,
+                                         const uint8_t *merkle_roots, size_t n, size_t k_scalar, size_t blowup,
+                                         size_t queries, stark_zsk_out *outs);
+
+/* deep_ali_merge_ed25519_verify_streaming (crates/deep_ali/src/lib.rs:854-974)
+ * for one signature's layout: lde is width x (height * blowup), coeffs the
+ * n_constraints combination coefficients (any values). Phi is evaluated at
+ * every point; c_eval gets height * blowup values. */
+int stark_merge_ed25519(stark_ctx *ctx, const uint8_t pk[32], const uint8_t sig[64], const uint8_t *msg,
+                        size_t msg_len, size_t k_scalar, const uint64_t *lde, size_t blowup, const uint64_t *coeffs,
+                        size_t n_coeffs, uint64_t *c_eval);
+
+/* fill_verify_air_v16 (ed25519_verify_air.rs:4820-4865) into trace (width x
+ * height, column-major): on the device, copied back (stark_build_...) or on
+ * the host (stark_host_build_...), from the same witness words. */
+int stark_build_ed25519_trace(stark_ctx *ctx, const uint8_t pk[32], const uint8_t sig[64], const uint8_t *msg,
+                              size_t msg_len, size_t k_scalar, uint64_t *trace);
+int stark_host_build_ed25519_trace(const uint8_t pk[32], const uint8_t sig[64], const uint8_t *msg, size_t msg_len,
+                                   size_t k_scalar, uint64_t *trace);
+/* ed25519_verify::verify (crates/deep_ali/src/ed25519_verify.rs:54-118): 1 valid, 0 not */
+int stark_host_ed25519_verify(const uint8_t pk[32], const uint8_t sig[64], const uint8_t *msg, size_t msg_len);
+/* RFC 8032 section 5.1.6 signing from a 32-byte seed (synthetic workloads);
+ * no counterpart in the reference's prover path */
+void stark_host_ed25519_sign(const uint8_t seed[32], const uint8_t *msg, size_t msg_len, uint8_t pk[32],
+                             uint8_t sig[64]);
+
 /* prove_nsec3_completeness (crates/swarm-dns/src/prover.rs:348-446): records
  * = n x (owner_hash[32] || next_hash[32]), a closed chain (records[n-1]'s
  * next_hash equals records[0]'s owner_hash, else STARK_E_INVALID, where the

@@ -89,6 +89,16 @@ from .api import (  # noqa: F401
     prove_p256_chains_batch,
     prove_record_ecdsa_batch,
     ecdsa_record_pk_hash,
+    ED25519_K_SCALAR,
+    VerifyAirShapeV16,
+    ZskKskOutput,
+    verify_air_layout_v16,
+    ed25519_verify,
+    ed25519_sign,
+    fill_verify_air_v16,
+    deep_ali_merge_ed25519_verify_streaming,
+    prove_zsk_ksk_binding_v2,
+    prove_zsk_ksk_binding_v2_batch,
     deep_fri_prove,
     deep_fri_prove_batch,
     default_context,

@@ -74,6 +74,11 @@ class _Nsec3Out(ctypes.Structure):
                 ("record_count", sz), ("n_trace", sz), ("proof", u8p), ("proof_len", sz)]
 
 
+class _ZskOut(ctypes.Structure):
+    _fields_ = [("pi_hash", ctypes.c_uint8 * 32), ("root_f0", ctypes.c_uint8 * 32), ("n_trace", sz),
+                ("proof", u8p), ("proof_len", sz)]
+
+
 class _DsKskOut(ctypes.Structure):
     _fields_ = [("pi_hash", ctypes.c_uint8 * 32), ("asserted_digest", ctypes.c_uint8 * 32),
                 ("root_f0", ctypes.c_uint8 * 32), ("n_blocks", sz), ("n_trace", sz),
@@ -167,6 +172,19 @@ def lib():
                                                         ctypes.c_char_p, ctypes.c_char_p, sz,
                                                         ctypes.POINTER(_P256Out)]),
         "stark_host_build_ecdsa_p256_trace": (ctypes.c_int, [ctypes.POINTER(_P256Chains), sz, u64p]),
+        "stark_ed25519_shape": (ctypes.c_int, [sz, sz, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz)]),
+        "stark_prove_zsk_ksk_binding_v2_batch": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                                                ctypes.POINTER(sz), ctypes.POINTER(sz),
+                                                                ctypes.c_char_p, ctypes.c_char_p, sz, sz, sz, sz,
+                                                                ctypes.POINTER(_ZskOut)]),
+        "stark_merge_ed25519": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz, sz, u64p,
+                                               sz, u64p, sz, u64p]),
+        "stark_build_ed25519_trace": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz, sz,
+                                                     u64p]),
+        "stark_host_build_ed25519_trace": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz, sz,
+                                                          u64p]),
+        "stark_host_ed25519_verify": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz]),
+        "stark_host_ed25519_sign": (None, [ctypes.c_char_p, ctypes.c_char_p, sz, ctypes.c_char_p, ctypes.c_char_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -1099,3 +1117,121 @@ def prover_core(d_trace, w, n_trace, blowup, shift, params, d_lde, ctx=None):
     ctx._check(lib().stark_prover_core_dev(ctx.h, d_trace.ptr, w, n_trace, blowup, shift, ctypes.byref(p), d_lde.ptr,
                                            root, ctypes.byref(pp), ctypes.byref(pl)))
     return root.raw, _take(pp, pl.value)
+
+
+# ---------------------------------------------------------------- Ed25519 (ed25519_verify_air.rs v16)
+
+ED25519_K_SCALAR = 256  # prove_zsk_ksk_binding_v2's production k_scalar (prover.rs:835-840)
+
+
+@dataclass
+class VerifyAirShapeV16:
+    """Sizes of verify_air_layout_v16 (ed25519_verify_air.rs:4741-4818)."""
+    width: int
+    height: int
+    n_constraints: int
+
+
+def verify_air_layout_v16(msg_len_for_sha512, k_scalar=ED25519_K_SCALAR):
+    """Trace width / height and verify_v16_per_row_constraints for a SHA-512
+    input (R || A || M) of msg_len_for_sha512 bytes."""
+    assert msg_len_for_sha512 >= 64, "the SHA-512 input is R || A || M"
+    w, h, k = sz(), sz(), sz()
+    rc = lib().stark_ed25519_shape(msg_len_for_sha512 - 64, k_scalar, ctypes.byref(w), ctypes.byref(h), ctypes.byref(k))
+    if rc != 0:
+        raise StarkError(rc, "verify_air_layout_v16: k_scalar must be in [2, 256]")
+    return VerifyAirShapeV16(w.value, h.value, k.value)
+
+
+def ed25519_verify(pubkey, signature, message):
+    """ed25519_verify::verify (ed25519_verify.rs:54-118): cofactored, s < L."""
+    return bool(lib().stark_host_ed25519_verify(bytes(pubkey), bytes(signature), bytes(message), len(message)))
+
+
+def ed25519_sign(seed, message):
+    """RFC 8032 signing from a 32-byte seed: (public key, signature)."""
+    pk, sig = ctypes.create_string_buffer(32), ctypes.create_string_buffer(64)
+    lib().stark_host_ed25519_sign(bytes(seed), bytes(message), len(message), pk, sig)
+    return pk.raw, sig.raw
+
+
+def fill_verify_air_v16(pubkey, signature, message, k_scalar=ED25519_K_SCALAR, device=False, ctx=None):
+    """fill_verify_air_v16 (ed25519_verify_air.rs:4820-4865) for the inputs
+    prove_zsk_ksk_binding_v2 derives from (pubkey, signature, message): the
+    width x height trace, built on the host or (device=True) on the GPU."""
+    shape = verify_air_layout_v16(64 + len(message), k_scalar)
+    tr = np.zeros((shape.width, shape.height), dtype=np.uint64)
+    args = (bytes(pubkey), bytes(signature), bytes(message), len(message), k_scalar, tr.ctypes.data_as(u64p))
+    if device:
+        ctx = ctx or default_context()
+        ctx._check(lib().stark_build_ed25519_trace(ctx.h, *args))
+    else:
+        rc = lib().stark_host_build_ed25519_trace(*args)
+        if rc != 0:
+            raise StarkError(rc, "fill_verify_air_v16: R or A does not decompress")
+    return tr
+
+
+def deep_ali_merge_ed25519_verify_streaming(trace_evals_on_lde, combination_coeffs, pubkey, signature, message,
+                                            omega, n_trace, blowup, k_scalar=ED25519_K_SCALAR, ctx=None):
+    """deep_ali_merge_ed25519_verify_streaming (lib.rs:854-974); the layout
+    argument of the reference is given by the signature it was built for."""
+    shape = verify_air_layout_v16(64 + len(message), k_scalar)
+    n = n_trace * blowup
+    assert n_trace == shape.height
+    lde = np.asarray(trace_evals_on_lde, dtype=np.uint64)
+    assert lde.shape == (shape.width, n), "trace shape mismatch"
+    assert len(combination_coeffs) == shape.n_constraints, "need one combination coefficient per constraint"
+    del omega  # unused by the reference too (lib.rs:866)
+    ctx = ctx or default_context()
+    lde, lp = _u64(lde)
+    co, cp = _u64(np.asarray(combination_coeffs, dtype=np.uint64))
+    out = np.empty(n, dtype=np.uint64)
+    ctx._check(lib().stark_merge_ed25519(ctx.h, bytes(pubkey), bytes(signature), bytes(message), len(message),
+                                         k_scalar, lp, blowup, cp, shape.n_constraints, out.ctypes.data_as(u64p)))
+    return out, CompositionInfo(2 * n_trace, n_trace, n_trace / n, shape.n_constraints, 2, shape.width)
+
+
+@dataclass
+class ZskKskOutput:
+    """ZskKskOutput of prove_zsk_ksk_binding_v2 (prover.rs:856-951)."""
+    pi_hash: bytes
+    verified: bool
+    proof_blob: bytes
+    proof_bytes: int
+    root_f0: bytes
+    n_trace: int
+
+
+def prove_zsk_ksk_binding_v2_batch(requests, k_scalar=ED25519_K_SCALAR, blowup=BLOWUP, queries=NUM_QUERIES,
+                                   ctx=None):
+    """prove_zsk_ksk_binding_v2 for each (ksk_pubkey, signature, signed_data,
+    fs_binding_32, merkle_root_32) request in one batched call; raises
+    StarkError when a signature does not verify (the reference asserts)."""
+    ctx = ctx or default_context()
+    reqs = list(requests)
+    assert reqs
+    msgs = [bytes(r[2]) for r in reqs]
+    off, o = [], 0
+    for m in msgs:
+        off.append(o)
+        o += len(m)
+    n = len(reqs)
+    outs = (_ZskOut * n)()
+    ctx._check(lib().stark_prove_zsk_ksk_binding_v2_batch(
+        ctx.h, b"".join(bytes(r[0]) for r in reqs), b"".join(bytes(r[1]) for r in reqs), b"".join(msgs) or b"\0",
+        (sz * n)(*off), (sz * n)(*[len(m) for m in msgs]), b"".join(bytes(r[3]) for r in reqs),
+        b"".join(bytes(r[4]) for r in reqs), n, k_scalar, blowup, queries, outs))
+    res = []
+    for x in outs:
+        blob = _take(x.proof, x.proof_len)
+        res.append(ZskKskOutput(bytes(x.pi_hash), True, blob, len(blob), bytes(x.root_f0), x.n_trace))
+    return res
+
+
+def prove_zsk_ksk_binding_v2(ksk_pubkey, signature, signed_data, fs_binding_32, merkle_root_32,
+                             k_scalar=ED25519_K_SCALAR, ldt=None, ctx=None):
+    """prove_zsk_ksk_binding_v2 (prover.rs:856-951) for one signature (STIR)."""
+    assert ldt in (None, LdtMode.Stir), "the GPU prover implements the STIR mode the swarm uses"
+    return prove_zsk_ksk_binding_v2_batch([(ksk_pubkey, signature, signed_data, fs_binding_32, merkle_root_32)],
+                                          k_scalar, ctx=ctx)[0]

@@ -10,6 +10,7 @@
 
 #include "air.hpp"
 #include "ctx.hpp"
+#include "ed25519_air.hpp"
 #include "fri.hpp"
 #include "gl.hpp"
 #include "keccak.hpp"
@@ -805,6 +806,110 @@ int stark_host_build_ecdsa_p256_trace(stark_p256_chains *in, size_t n_trace, uin
 void stark_host_dns_leaf_hash(const stark_dns_record *r, const uint8_t salt[16], uint8_t out[32]) {
     sg::DnsRec d{r->domain, r->domain_len, r->record_type, r->ttl, r->rdata, r->rdata_len};
     sg::dns_leaf_hash(d, salt, out);
+}
+
+int stark_ed25519_shape(size_t msg_len, size_t k_scalar, size_t *width, size_t *height, size_t *n_constraints) {
+    try {
+        SG_REQUIRE(width && height && n_constraints, "ed25519_shape: null argument");
+        SG_REQUIRE(k_scalar >= 2 && k_scalar <= 256, "ed25519_shape: k_scalar in [2, 256]");
+        const sg::EdShape s = sg::ed25519_shape(msg_len, (uint32_t)k_scalar);
+        *width = s.width;
+        *height = s.height;
+        *n_constraints = s.n_constraints;
+        return STARK_OK;
+    } catch (const sg::Error &e) {
+        return e.code;
+    } catch (const std::exception &) {
+        return STARK_E_INTERNAL;
+    }
+}
+
+int stark_prove_zsk_ksk_binding_v2_batch(stark_ctx *ctx, const uint8_t *pks, const uint8_t *sigs, const uint8_t *msgs,
+                                         const size_t *msg_off, const size_t *msg_len, const uint8_t *fs_bindings,
+                                         const uint8_t *merkle_roots, size_t n, size_t k_scalar, size_t blowup,
+                                         size_t queries, stark_zsk_out *outs) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(pks && sigs && msg_off && msg_len && fs_bindings && merkle_roots && outs,
+                   "prove_zsk_ksk_binding_v2_batch: null argument");
+        SG_REQUIRE(n >= 1 && n <= 65535, "prove_zsk_ksk_binding_v2_batch: 1 to 65535 signatures");
+        SG_REQUIRE(k_scalar >= 2 && k_scalar <= 256, "prove_zsk_ksk_binding_v2_batch: k_scalar in [2, 256]");
+        for (size_t i = 0; i < n; i++) outs[i].proof = nullptr;
+        std::vector<sg::EdSig> v(n);
+        std::vector<std::array<uint8_t, 32>> fs(n), mr(n);
+        for (size_t i = 0; i < n; i++) {
+            SG_REQUIRE(msg_len[i] == 0 || msgs, "prove_zsk_ksk_binding_v2_batch: null message");
+            v[i] = sg::EdSig{pks + 32 * i, sigs + 64 * i, msg_len[i] ? msgs + msg_off[i] : nullptr, msg_len[i]};
+            memcpy(fs[i].data(), fs_bindings + 32 * i, 32);
+            memcpy(mr[i].data(), merkle_roots + 32 * i, 32);
+        }
+        std::vector<sg::ZskKsk> r =
+            sg::prove_zsk_ksk_binding_v2_batch(c, v, fs, mr, (uint32_t)k_scalar, blowup, (uint32_t)queries);
+        for (size_t i = 0; i < n; i++) {
+            memcpy(outs[i].pi_hash, r[i].pi_hash, 32);
+            memcpy(outs[i].root_f0, r[i].root_f0, 32);
+            outs[i].n_trace = r[i].n_trace;
+            outs[i].proof = copy_out(r[i].proof, &outs[i].proof_len);
+        }
+    });
+}
+
+int stark_merge_ed25519(stark_ctx *ctx, const uint8_t pk[32], const uint8_t sig[64], const uint8_t *msg,
+                        size_t msg_len, size_t k_scalar, const uint64_t *lde, size_t blowup, const uint64_t *coeffs,
+                        size_t n_coeffs, uint64_t *c_eval) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(pk && sig && lde && coeffs && c_eval && (msg || !msg_len), "merge_ed25519: null argument");
+        SG_REQUIRE(k_scalar >= 2 && k_scalar <= 256, "merge_ed25519: k_scalar in [2, 256]");
+        const sg::EdShape sh = sg::ed25519_shape(msg_len, (uint32_t)k_scalar);
+        SG_REQUIRE(n_coeffs == sh.n_constraints, "merge_ed25519: need one coefficient per constraint");
+        SG_REQUIRE(blowup >= 2 && blowup <= 64 && (blowup & (blowup - 1)) == 0, "merge_ed25519: blowup in {2, ..., 64}");
+        const size_t n0 = sh.height * blowup;
+        sg::Scoped dl(c, 8ULL * sh.width * n0), dc(c, 8 * n0), dk(c, 8 * n_coeffs);
+        h2d(c, dl.p, lde, 8ULL * sh.width * n0);
+        h2d(c, dk.p, coeffs, 8 * n_coeffs);
+        std::vector<sg::EdSig> v = {sg::EdSig{pk, sig, msg, msg_len}};
+        sg::merge_ed25519_batch(c, v, (uint32_t)k_scalar, dl.as<uint64_t>(), blowup, dk.as<uint64_t>(),
+                                dc.as<uint64_t>());
+        d2h(c, c_eval, dc.p, 8 * n0);
+    });
+}
+
+int stark_build_ed25519_trace(stark_ctx *ctx, const uint8_t pk[32], const uint8_t sig[64], const uint8_t *msg,
+                              size_t msg_len, size_t k_scalar, uint64_t *trace) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(pk && sig && trace && (msg || !msg_len), "build_ed25519_trace: null argument");
+        SG_REQUIRE(k_scalar >= 2 && k_scalar <= 256, "build_ed25519_trace: k_scalar in [2, 256]");
+        const sg::EdShape sh = sg::ed25519_shape(msg_len, (uint32_t)k_scalar);
+        sg::Scoped dt(c, 8ULL * sh.width * sh.height);
+        std::vector<sg::EdSig> v = {sg::EdSig{pk, sig, msg, msg_len}};
+        sg::build_ed25519_traces_dev(c, v, (uint32_t)k_scalar, dt.as<uint64_t>());
+        d2h(c, trace, dt.p, 8ULL * sh.width * sh.height);
+    });
+}
+
+int stark_host_build_ed25519_trace(const uint8_t pk[32], const uint8_t sig[64], const uint8_t *msg, size_t msg_len,
+                                   size_t k_scalar, uint64_t *trace) {
+    try {
+        SG_REQUIRE(pk && sig && trace && (msg || !msg_len), "ed25519 trace: null argument");
+        SG_REQUIRE(k_scalar >= 2 && k_scalar <= 256, "ed25519 trace: k_scalar in [2, 256]");
+        sg::build_ed25519_trace_host(sg::EdSig{pk, sig, msg, msg_len}, (uint32_t)k_scalar, trace);
+        return STARK_OK;
+    } catch (const sg::Error &e) {
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        return STARK_E_OOM;
+    } catch (const std::exception &) {
+        return STARK_E_INTERNAL;
+    }
+}
+
+int stark_host_ed25519_verify(const uint8_t pk[32], const uint8_t sig[64], const uint8_t *msg, size_t msg_len) {
+    if (!pk || !sig || (!msg && msg_len)) return 0;
+    return sg::ed25519_verify(sg::EdSig{pk, sig, msg, msg_len}) ? 1 : 0;
+}
+
+void stark_host_ed25519_sign(const uint8_t seed[32], const uint8_t *msg, size_t msg_len, uint8_t pk[32],
+                             uint8_t sig[64]) {
+    sg::ed25519_sign(seed, msg, msg_len, pk, sig);
 }
 
 }  // extern "C"

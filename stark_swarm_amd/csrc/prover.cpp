@@ -656,4 +656,93 @@ std::vector<uint8_t> prover_core_dev(Ctx *c, const uint64_t *d_trace, uint64_t w
     return out;
 }
 
+std::vector<ZskKsk> prove_zsk_ksk_binding_v2_batch(Ctx *c, const std::vector<EdSig> &sigs,
+                                                   const std::vector<std::array<uint8_t, 32>> &fs_bindings,
+                                                   const std::vector<std::array<uint8_t, 32>> &merkle_roots,
+                                                   uint32_t k, uint64_t blowup, uint32_t queries) {
+    const size_t K = sigs.size();
+    SG_REQUIRE(K >= 1 && K <= 65535, "prove_zsk_ksk_binding_v2: 1 to 65535 signatures");
+    SG_REQUIRE(fs_bindings.size() == K && merkle_roots.size() == K,
+               "prove_zsk_ksk_binding_v2: one fs binding and merkle root per signature");
+    SG_REQUIRE(is_pow2(blowup) && blowup >= 4 && blowup <= 64, "prove_zsk_ksk_binding_v2: blowup in {4, ..., 64}");
+    SG_REQUIRE(queries >= 1 && queries <= 512, "prove_zsk_ksk_binding_v2: 1 to 512 queries");
+    {
+        HostScope hs(c, "ed25519_verify");
+        std::vector<uint8_t> ok(K, 0);
+        parallel_for(K, [&](size_t i) { ok[i] = ed25519_verify(sigs[i]); });
+        for (size_t i = 0; i < K; i++)
+            SG_REQUIRE(ok[i], "prove_zsk_ksk_binding_v2: signature " + std::to_string(i) +
+                                  " does not verify (the prover must not commit to an invalid signature)");
+    }
+    // groups of one trace shape (the SHA-512 row count of R || A || M)
+    std::map<uint64_t, std::vector<size_t>> groups;
+    for (size_t i = 0; i < K; i++) groups[ed25519_shape(sigs[i].len, k).k_hash].push_back(i);
+    std::vector<ZskKsk> out(K);
+    for (auto &gr : groups) {
+        const EdShape sh = ed25519_shape(sigs[gr.second[0]].len, k);
+        const uint64_t h = sh.height, W = sh.width, n0 = h * blowup;
+        size_t free_b = 0, total_b = 0;
+        SG_HIP(hipMemGetInfo(&free_b, &total_b));
+        const uint64_t budget = (uint64_t)((free_b + c->cached_bytes) * 0.6);
+        // trace and its odd coset, gate-group sums, phi and FRI
+        const uint64_t per = 16ULL * W * h + 8ULL * 400 * 2 * h + 8ULL * 64 * n0;
+        SG_REQUIRE(per <= budget, "prove_zsk_ksk_binding_v2: one proof does not fit in device memory");
+        const size_t cap = (size_t)std::min<uint64_t>(1024, budget / per);
+        const std::vector<size_t> &idx = gr.second;
+        const size_t nw = (idx.size() + cap - 1) / cap, chunk = (idx.size() + nw - 1) / nw;
+        for (size_t k0 = 0; k0 < idx.size(); k0 += chunk) {
+            const uint32_t G = (uint32_t)std::min(chunk, idx.size() - k0);
+            std::vector<EdSig> part(G);
+            std::vector<std::array<uint8_t, 32>> pis(G);
+            for (uint32_t j = 0; j < G; j++) {
+                part[j] = sigs[idx[k0 + j]];
+                pis[j] = fs_bindings[idx[k0 + j]];
+            }
+            Scoped d_c(c, 8ULL * n0 * G);
+            {
+                Scoped d_tr(c, 8ULL * W * h * G);
+                {
+                    HostScope hs(c, "ed25519_trace");
+                    build_ed25519_traces_dev(c, part, k, d_tr.as<uint64_t>());
+                }
+                HostScope hs(c, "ed25519_merge");
+                merge_ed25519_from_trace_batch(c, part, k, d_tr.as<uint64_t>(), blowup, nullptr, d_c.as<uint64_t>());
+                if (c->profiling) SG_HIP(hipStreamSynchronize(c->stream));
+            }
+            HostScope hf(c, "ed25519_fri");
+            FriParams p;
+            p.schedule = stir_schedule(n0);
+            p.r = p.s0 = queries;
+            p.seed_z = SWARM_SEED_Z;
+            p.coeff_commit_final = true;
+            p.d_final = 1;
+            p.stir = true;
+            p.has_pi = true;
+            std::vector<std::vector<uint8_t>> proofs = deep_fri_prove_batch(c, d_c.as<uint64_t>(), G, n0, p, &pis, nullptr);
+            for (uint32_t j = 0; j < G; j++) {
+                const size_t i = idx[k0 + j];
+                const EdSig &s = sigs[i];
+                ZskKsk &o = out[i];
+                o.proof = std::move(proofs[j]);
+                o.n_trace = h;
+                memcpy(o.root_f0, o.proof.data(), 32);
+                // zsk_ksk_pi_hash_v2_stark (prover.rs:763-781)
+                uint8_t le[8];
+                for (int b = 0; b < 8; b++) le[b] = (uint8_t)((uint64_t)s.len >> (8 * b));
+                Sha3 hsh;
+                hsh.update((const uint8_t *)"ZSK-KSK-PIHASH-V2-ST", 20);
+                hsh.update(s.pk, 32);
+                hsh.update(s.sig, 64);
+                hsh.update(le, 8);
+                if (s.len) hsh.update(s.msg, s.len);
+                hsh.update(fs_bindings[i].data(), 32);
+                hsh.update(merkle_roots[i].data(), 32);
+                hsh.update(o.root_f0, 32);
+                hsh.final(o.pi_hash);
+            }
+        }
+    }
+    return out;
+}
+
 }  // namespace sg

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "ctx.hpp"
+#include "ed25519_air.hpp"
 #include "fri.hpp"
 #include "p256_air.hpp"
 #include "rsa_air.hpp"
@@ -121,5 +122,23 @@ std::vector<uint8_t> prove_outer_rollup(Ctx *c, const uint8_t *pi_hashes, size_t
                                         int phi_mode, uint8_t root_f0[32]);
 std::vector<uint8_t> prover_core_dev(Ctx *c, const uint64_t *d_trace, uint64_t w, uint64_t n_trace, uint64_t blowup,
                                      uint64_t shift, const FriParams &p, uint64_t *d_lde, uint8_t row_root[32]);
+
+// ZskKskOutput of prove_zsk_ksk_binding_v2 (crates/swarm-dns/src/prover.rs:856-951)
+struct ZskKsk {
+    uint8_t pi_hash[32], root_f0[32];
+    size_t n_trace = 0;
+    std::vector<uint8_t> proof;
+};
+// prove_zsk_ksk_binding_v2 for each signature: the v16 Ed25519 verify trace
+// (k_scalar rows per ladder) -> merge (lambda_j = j + 1) at the given blowup
+// (the reference uses 32) -> STIR with `queries` queries bound to
+// fs_bindings[i]; pi_hash = zsk_ksk_pi_hash_v2_stark (prover.rs:763-781).
+// Signatures are grouped by the SHA-512 row count of R || A || M and proved
+// in lockstep HBM-sized waves; result i equals the proof of signature i alone.
+// Throws E_INVALID when a signature does not verify (the reference asserts).
+std::vector<ZskKsk> prove_zsk_ksk_binding_v2_batch(Ctx *c, const std::vector<EdSig> &sigs,
+                                                   const std::vector<std::array<uint8_t, 32>> &fs_bindings,
+                                                   const std::vector<std::array<uint8_t, 32>> &merkle_roots,
+                                                   uint32_t k_scalar, uint64_t blowup, uint32_t queries);
 
 }  // namespace sg

@@ -35,6 +35,25 @@ uint64_t sha512_trace_height(uint64_t len);
 uint64_t build_sha512_trace(const uint8_t *msg, uint64_t len, uint64_t *tr, uint8_t digest[64], uint64_t height = 0);
 void build_sha512_traces_dev(Ctx *c, const std::vector<ByteSpan> &msgs, uint64_t h, uint64_t *d_tr,
                              std::vector<uint32_t> &n_blocks, std::vector<std::array<uint8_t, 64>> &digests);
+// the same rows written into a wider trace: rows h of trace k at
+// d_tr + k * pstride, column c at offset c * cstride
+void build_sha512_traces_into(Ctx *c, const std::vector<ByteSpan> &msgs, uint64_t h, uint64_t cstride,
+                              uint64_t pstride, uint64_t *d_tr, std::vector<uint32_t> &n_blocks,
+                              std::vector<std::array<uint8_t, 64>> &digests);
+// The SHA-512 slots (0..1525) of an AIR whose first 1510 columns are SHA-512
+// rows (wcols columns per proof): phi of those slots at every point of a
+// caller's LDE, zero on rows r with r + 1 >= row_limit; d_lam = the slots'
+// coefficients (nullptr: j + 1).
+void compose_sha512_rows(Ctx *c, const uint64_t *lde, uint64_t wcols, uint32_t K, uint64_t n, uint64_t blowup,
+                         const uint32_t *d_nb, uint64_t row_limit, const uint64_t *d_lam, uint64_t *phi);
+// ... and their five gate-group sums on a blowup-2 LDE (traces and odd coset,
+// n2 points): groups[(k * 5 + g) * n2 + i] (ungated, compression rows,
+// finalisation row, other rows, schedule rows), t1_lam = the weights of the
+// two T1 identities, whose round constants the caller subtracts
+void compose_sha512_groups(Ctx *c, const uint64_t *d_tr, const uint64_t *d_odd, uint64_t wcols, uint32_t K,
+                           uint64_t n2, const uint32_t *d_nb, const uint64_t *d_lam, uint64_t *groups,
+                           uint64_t *t1_lam);
+const uint64_t *sha512_round_constants();  // FIPS 180-4 K[0..80], host memory
 // deep_ali_merge_sha256 (lib.rs:453-545) for K proofs of one n_trace: lde of
 // proof k at lde + 756*k*n0, c_eval at c_eval + k*n0
 void merge_sha256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup,

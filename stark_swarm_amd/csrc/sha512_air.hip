@@ -183,22 +183,27 @@ std::array<uint64_t, 8> block_words(const uint8_t *blk, const uint64_t iv[8], Ro
 
 // Device expansion (as for SHA-256): lane = row, rows past the last block
 // repeat the last useful row.
+// rows h of trace k at tr + k * pstride, column c at offset c * cstride
 __global__ __launch_bounds__(256) void expand_sha512_k(const RowWords *words, const uint64_t *off,
-                                                       const uint32_t *useful, uint64_t h, uint64_t *tr) {
+                                                       const uint32_t *useful, uint64_t h, uint64_t cstride,
+                                                       uint64_t pstride, uint64_t *tr) {
     const uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (r >= h) return;
     const uint32_t k = blockIdx.y;
     const uint64_t src = off[k] + (r < useful[k] ? r : useful[k] - 1);
     const RowWords rw = words[src];
-    uint64_t *o = tr + (uint64_t)k * SHA512_W * h + r;
-    emit_row(rw, [&](uint32_t c, uint64_t v) { o[(uint64_t)c * h] = v; });
+    uint64_t *o = tr + (uint64_t)k * pstride + r;
+    emit_row(rw, [&](uint32_t c, uint64_t v) { o[(uint64_t)c * cstride] = v; });
 }
 
 // ---- constraints (sha512_air.rs:909-1282), one LDE point per lane ----
+// phi accumulator: slot w - 1 weighted by lam[w - 1], or by w without a table
 struct Phi {
     uint64_t acc = 0, w = 1;
+    const uint64_t *lam = nullptr;
+    __device__ __forceinline__ uint64_t weight() const { return lam ? lam[w - 1] : w; }
     __device__ __forceinline__ void put(uint64_t v) {
-        acc = add(acc, mul(w, v));
+        acc = add(acc, mul(weight(), v));
         w++;
     }
     __device__ __forceinline__ void skip(int k) { w += k; }
@@ -221,14 +226,22 @@ enum { G_ALL, G_COMP, G_FIN, G_NOTFIN, G_SCHED, NG };
 // limb identities.
 // GROUPS reads the blowup-2 LDE as its even points (lde: the traces) and
 // its odd points (odd), n / 2 rows each, as compose_sha256_k does.
+// wcols: columns per proof of the input (the SHA-512 cells are its first
+// 1510); row_limit: phi is zero on rows r with r + 1 >= row_limit (the SHA
+// rows of a larger AIR); lam: the slots' coefficients, or j + 1 when null.
 template <bool GROUPS>
 __global__ __launch_bounds__(AT) void compose_sha512_k(const uint64_t *lde, const uint64_t *odd, uint64_t n,
                                                        uint64_t blowup, const uint32_t *n_blocks, uint64_t *phi,
-                                                       uint64_t *t1_lam) {
+                                                       uint64_t *t1_lam, uint64_t wcols, uint64_t row_limit,
+                                                       const uint64_t *lam) {
     const uint64_t i = (uint64_t)blockIdx.x * AT + threadIdx.x;
     if (i >= n) return;
+    if (!GROUPS && i / blowup + 1 >= row_limit) {
+        phi[n * blockIdx.y + i] = 0;
+        return;
+    }
     const uint64_t rows = GROUPS ? n / 2 : n, at = GROUPS ? i >> 1 : i;
-    const uint64_t *L = (GROUPS && (i & 1) ? odd : lde) + (uint64_t)SHA512_W * rows * blockIdx.y;
+    const uint64_t *L = (GROUPS && (i & 1) ? odd : lde) + wcols * rows * blockIdx.y;
     const uint64_t ni = GROUPS ? (at + 1) & (rows - 1) : (i + blowup) & (n - 1);
     auto cur = [&](uint32_t c) { return L[(uint64_t)c * rows + at]; };
     auto nxt = [&](uint32_t c) { return L[(uint64_t)c * rows + ni]; };
@@ -238,6 +251,7 @@ __global__ __launch_bounds__(AT) void compose_sha512_k(const uint64_t *lde, cons
     const bool sched = GROUPS || (!past && br >= 15 && br <= 78);
     const uint64_t TWO32 = 1ULL << 32;
     Phi f, fc, ff, fn, fs;  // ungated; compression, finalisation, not finalisation, schedule (GROUPS mode)
+    f.lam = fc.lam = ff.lam = fn.lam = fs.lam = lam;
     Phi &gc = GROUPS ? fc : f, &gs = GROUPS ? fs : f;
     const uint32_t boff[8] = {BIT_A, BIT_B, BIT_C, BIT_E, BIT_F, BIT_G, BIT_WW14, BIT_WW1};
 #pragma unroll 1
@@ -307,11 +321,11 @@ __global__ __launch_bounds__(AT) void compose_sha512_k(const uint64_t *lde, cons
         f.put(boolc(cur(c0 + 2)));
         f.put(sub(sub(sub(cur(cw), cur(c0)), mul(2, cur(c0 + 1))), mul(4, cur(c0 + 2))));
     };
-    if (GROUPS && i == 0 && blockIdx.y == 0) t1_lam[0] = f.w;
+    if (GROUPS && i == 0 && blockIdx.y == 0) t1_lam[0] = f.weight();
     f.put(sub(add(cur(T1L_LO), mul(TWO32, cur(T1L_CW))),
               add(add(add(add(cur(A0 + 14), cur(S1_LO)), cur(CH_LO)), kt & 0xFFFFFFFFu), cur(W_LO))));
     carry3(T1L_C0, T1L_CW);
-    if (GROUPS && i == 0 && blockIdx.y == 0) t1_lam[1] = f.w;
+    if (GROUPS && i == 0 && blockIdx.y == 0) t1_lam[1] = f.weight();
     f.put(sub(add(cur(T1H_LO), mul(TWO32, cur(T1H_CW))),
               add(add(add(add(add(cur(A0 + 15), cur(S1_LO + 1)), cur(CH_LO + 1)), kt >> 32), cur(W_LO + 1)),
                   cur(T1L_CW))));
@@ -466,6 +480,12 @@ uint64_t build_sha512_trace(const uint8_t *msg, uint64_t len, uint64_t *tr, uint
 
 void build_sha512_traces_dev(Ctx *c, const std::vector<ByteSpan> &msgs, uint64_t h, uint64_t *d_tr,
                              std::vector<uint32_t> &n_blocks, std::vector<std::array<uint8_t, 64>> &digests) {
+    build_sha512_traces_into(c, msgs, h, h, (uint64_t)SHA512_W * h, d_tr, n_blocks, digests);
+}
+
+void build_sha512_traces_into(Ctx *c, const std::vector<ByteSpan> &msgs, uint64_t h, uint64_t cstride,
+                              uint64_t pstride, uint64_t *d_tr, std::vector<uint32_t> &n_blocks,
+                              std::vector<std::array<uint8_t, 64>> &digests) {
     const size_t K = msgs.size();
     std::vector<std::vector<sha5::RowWords>> rows(K);
     n_blocks.assign(K, 0);
@@ -490,7 +510,8 @@ void build_sha512_traces_dev(Ctx *c, const std::vector<ByteSpan> &msgs, uint64_t
     SG_HIP(hipMemcpyAsync(d_u.p, useful.data(), 4 * K, hipMemcpyHostToDevice, c->stream));
     ProfScope ps(c, "expand_sha512", 8.0 * SHA512_W * h * K);
     hipLaunchKernelGGL(sha5::expand_sha512_k, dim3((uint32_t)((h + 255) / 256), (uint32_t)K), dim3(256), 0, c->stream,
-                       d_w.as<sha5::RowWords>(), d_off.as<uint64_t>(), d_u.as<uint32_t>(), h, d_tr);
+                       d_w.as<sha5::RowWords>(), d_off.as<uint64_t>(), d_u.as<uint32_t>(), h, cstride, pstride,
+                       d_tr);
     SG_HIP(hipGetLastError());
     SG_HIP(hipStreamSynchronize(c->stream));
 }
@@ -513,7 +534,8 @@ void merge_sha512_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trac
         ProfScope ps(c, "compose_sha512", (8.0 * SHA512_W * n + 8.0 * n) * K);
         hipLaunchKernelGGL(sha5::compose_sha512_k<false>, dim3((uint32_t)((n + sha5::AT - 1) / sha5::AT), K),
                            dim3(sha5::AT), 0, c->stream, lde, (const uint64_t *)nullptr, n, blowup,
-                           d_nb.as<uint32_t>(), phi.as<uint64_t>(), (uint64_t *)nullptr);
+                           d_nb.as<uint32_t>(), phi.as<uint64_t>(), (uint64_t *)nullptr, (uint64_t)SHA512_W,
+                           ~0ULL, (const uint64_t *)nullptr);
         SG_HIP(hipGetLastError());
     }
     quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
@@ -541,7 +563,8 @@ void merge_sha512_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uin
             ProfScope ps(c, "compose_sha512", (8.0 * SHA512_W * n2 + 8.0 * sha5::NG * n2) * K);
             hipLaunchKernelGGL(sha5::compose_sha512_k<true>, dim3((uint32_t)((n2 + sha5::AT - 1) / sha5::AT), K),
                                dim3(sha5::AT), 0, c->stream, d_tr, odd.as<uint64_t>(), n2, (uint64_t)2,
-                               d_nb.as<uint32_t>(), groups.as<uint64_t>(), t1.as<uint64_t>());
+                               d_nb.as<uint32_t>(), groups.as<uint64_t>(), t1.as<uint64_t>(), (uint64_t)SHA512_W,
+                               ~0ULL, (const uint64_t *)nullptr);
             SG_HIP(hipGetLastError());
         }
         Scoped full(c, 8ULL * sha5::NG * n * K);
@@ -554,5 +577,26 @@ void merge_sha512_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uin
     }
     quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
 }
+
+void compose_sha512_rows(Ctx *c, const uint64_t *lde, uint64_t wcols, uint32_t K, uint64_t n, uint64_t blowup,
+                         const uint32_t *d_nb, uint64_t row_limit, const uint64_t *d_lam, uint64_t *phi) {
+    ProfScope ps(c, "compose_sha512", (8.0 * SHA512_W * n + 8.0 * n) * K);
+    hipLaunchKernelGGL(sha5::compose_sha512_k<false>, dim3((uint32_t)((n + sha5::AT - 1) / sha5::AT), K),
+                       dim3(sha5::AT), 0, c->stream, lde, (const uint64_t *)nullptr, n, blowup, d_nb, phi,
+                       (uint64_t *)nullptr, wcols, row_limit, d_lam);
+    SG_HIP(hipGetLastError());
+}
+
+void compose_sha512_groups(Ctx *c, const uint64_t *d_tr, const uint64_t *d_odd, uint64_t wcols, uint32_t K,
+                           uint64_t n2, const uint32_t *d_nb, const uint64_t *d_lam, uint64_t *groups,
+                           uint64_t *t1_lam) {
+    ProfScope ps(c, "compose_sha512", (8.0 * SHA512_W * n2 + 8.0 * sha5::NG * n2) * K);
+    hipLaunchKernelGGL(sha5::compose_sha512_k<true>, dim3((uint32_t)((n2 + sha5::AT - 1) / sha5::AT), K),
+                       dim3(sha5::AT), 0, c->stream, d_tr, d_odd, n2, (uint64_t)2, d_nb, groups, t1_lam, wcols,
+                       ~0ULL, d_lam);
+    SG_HIP(hipGetLastError());
+}
+
+const uint64_t *sha512_round_constants() { return sha5::KH; }
 
 }  // namespace sg
