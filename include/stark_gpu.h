@@ -424,6 +424,17 @@ int stark_synchronize(stark_ctx *ctx);
  * (ark-poly 0.4.2, used at trace_import.rs:272-279, lib.rs:400-410). */
 int stark_ntt_dev(stark_ctx *ctx, const uint64_t *d_in, uint64_t *d_out, size_t log_n, size_t ncols, int inverse);
 
+/* The allocation-free variant of stark_ntt_dev. It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops that forbid hipMalloc). The two-pass
+ * transform's scratch is one fixed 32 MiB array per device (held until the
+ * process ends) instead of an allocation, so log_n <= 24 with
+ * 2^log_n * ncols <= 2^22 (else STARK_E_INVALID); the call makes no
+ * hipMalloc / hipFree and times no launch plan. Calls that overlap on one
+ * device share the array and race. */
+int stark_ntt_dev_noalloc(stark_ctx *ctx, const uint64_t *d_in, uint64_t *d_out, size_t log_n, size_t ncols,
+                          int inverse);
+
 /* lde_trace_columns on device buffers; shift != 1 evaluates on the coset
  * shift * <w_n0> (coefficient i scaled by shift^i). */
 int stark_lde_columns_dev(stark_ctx *ctx, const uint64_t *d_cols, size_t w, size_t n_trace, size_t blowup,

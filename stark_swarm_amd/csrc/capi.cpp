@@ -568,6 +568,15 @@ int stark_ntt_dev(stark_ctx *ctx, const uint64_t *d_in, uint64_t *d_out, size_t 
     });
 }
 
+int stark_ntt_dev_noalloc(stark_ctx *ctx, const uint64_t *d_in, uint64_t *d_out, size_t log_n, size_t ncols,
+                          int inverse) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(d_in && d_out && ncols >= 1 && log_n <= 24, "ntt (no alloc): bad arguments");
+        sg::ntt_columns_noalloc(c, d_in, d_out, (uint32_t)log_n, ncols, inverse != 0);
+        SG_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
 int stark_lde_columns_dev(stark_ctx *ctx, const uint64_t *d_cols, size_t w, size_t n_trace, size_t blowup,
                           uint64_t shift, uint64_t *d_out) {
     return guard(ctx, [&](sg::Ctx *c) {

@@ -35,6 +35,7 @@
 // second pass) run a PLAIN instantiation with those paths compiled out:
 // fewer live registers than the general kernel at the same 128-VGPR budget.
 #include <algorithm>
+#include <map>
 #include <mutex>
 
 #include "gl.hpp"
@@ -754,6 +755,53 @@ void lde_odd_coset(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint
     default: throw Error(E_INTERNAL, "odd coset: size out of range");
     }
     SG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- allocation-free transforms
+
+// Scratch of the two-pass transforms for callers that cannot allocate device
+// memory (code under HIP graph capture, serving loops that forbid hipMalloc):
+// one fixed array per device, held until the process ends. Every call on a
+// device shares that device's array, so calls that overlap on one device race.
+constexpr uint64_t NOALLOC_SCRATCH_WORDS = 1ULL << 22;  // 32 MiB
+static __device__ __align__(256) uint64_t scratch[NOALLOC_SCRATCH_WORDS];
+
+static std::mutex scratch_mu;
+static std::map<int, uint64_t *> scratch_ptr;  // device -> address of its copy of `scratch`
+
+// the device address of `scratch` on device dev (current), looked up once per device
+static uint64_t *noalloc_scratch(int dev) {
+    std::lock_guard<std::mutex> g(scratch_mu);
+    auto it = scratch_ptr.find(dev);
+    if (it != scratch_ptr.end()) return it->second;
+    void *ptr = nullptr;
+    SG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    SG_REQUIRE(ptr != nullptr, "ntt (no alloc): scratch symbol has no device address");
+    scratch_ptr[dev] = (uint64_t *)ptr;
+    return (uint64_t *)ptr;
+}
+
+uint64_t ntt_noalloc_max_elems() { return NOALLOC_SCRATCH_WORDS; }
+
+void ntt_columns_noalloc(Ctx *c, const uint64_t *in, uint64_t *out, uint32_t L, uint64_t ncols, bool inv) {
+    SG_REQUIRE(L <= 24 && ncols >= 1, "ntt (no alloc): size above 2^24 or no columns");
+    SG_REQUIRE((ncols << L) <= NOALLOC_SCRATCH_WORDS,
+               "ntt (no alloc): n * ncols above the 2^22-element scratch of the allocation-free entry point");
+    uint64_t *scr = noalloc_scratch(c->device);
+    // no plan timing (it creates events) and no profiling events: the table's
+    // plan or the default split, the same transform either way
+    const bool tune = c->autotune, prof = c->profiling;
+    c->autotune = false;
+    c->profiling = false;
+    try {
+        ntt_columns(c, in, out, scr, L, ncols, 1ULL << L, 1ULL << L, inv, nullptr);
+    } catch (...) {
+        c->autotune = tune;
+        c->profiling = prof;
+        throw;
+    }
+    c->autotune = tune;
+    c->profiling = prof;
 }
 
 }  // namespace sg

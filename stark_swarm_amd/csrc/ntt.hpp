@@ -61,6 +61,13 @@ const uint64_t *odd_twist_table(Ctx *c, uint32_t L);
 void lde_columns(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint32_t LB, uint64_t ncols,
                  uint64_t in_cs, uint64_t out_cs, uint64_t shift);
 
+// ntt_columns over contiguous columns (n = 2^L apart) with the two-pass
+// scratch in a fixed per-device array instead of an allocation: calls no
+// hipMalloc / hipFree and creates no events. n * ncols must not exceed
+// ntt_noalloc_max_elems() (2^22); calls that overlap on one device race.
+void ntt_columns_noalloc(Ctx *c, const uint64_t *in, uint64_t *out, uint32_t L, uint64_t ncols, bool inv);
+uint64_t ntt_noalloc_max_elems();
+
 // Autotuned launch plans. A plan is the log2 size L1 of the first pass of a
 // two-pass transform of 2^L points (the second pass has 2^(L - L1)); for an
 // LDE of 2^L <= 4096 points L1 = L is the one-pass form with the cosets as
