@@ -204,12 +204,16 @@ def main():
     ap.add_argument("--dsksk", type=int, default=64, help="DS-KSK (SHA-256 AIR) proofs for the plain-vs-batched line (0 = skip)")
     ap.add_argument("--rsa", type=int, default=64, help="RSA-2048 record proofs for the plain-vs-batched line (0 = skip)")
     ap.add_argument("--p256", type=int, default=16, help="ECDSA-P256 record proofs for the plain-vs-batched line (0 = skip)")
-    ap.add_argument("--serve", default="200,600,2000", help="serving: Poisson arrival rates (requests/s), '' = skip")
-    ap.add_argument("--serve-requests", type=int, default=400, help="serving: requests per rate")
+    ap.add_argument("--ed25519", type=int, default=64,
+                    help="Ed25519 signatures for the C3 plain-vs-batched lines (blowup 8 with 90 queries, and the "
+                         "reference's blowup 32 with 54) (0 = skip)")
+    ap.add_argument("--serve", default="100,200,300,400",
+                    help="serving: Poisson arrival rates (requests/s), '' = skip")
+    ap.add_argument("--serve-requests", type=int, default=300, help="serving: requests per rate")
     ap.add_argument("--sha-batch", default="64,10,13",
                     help="batched uneven SHA-256/512 proofs: count,log2 min rows,log2 max rows ('' = skip); "
                          "the full workload is 256,10,17")
-    ap.add_argument("--only", choices=["shards", "ds_ksk", "rsa", "p256", "sha_batch", "serve"],
+    ap.add_argument("--only", choices=["shards", "ds_ksk", "rsa", "p256", "sha_batch", "serve", "ed25519"],
                     help="run only this plain-vs-batched line (for profiling it) and print its JSON")
     ap.add_argument("--shards-only", action="store_true",
                     help="run only the inner-shard plain-vs-batched measurement (for profiling that path)")
@@ -239,6 +243,8 @@ def main():
             line = rsa_batch(S, max(1, args.rsa))
         elif args.only == "p256":
             line = p256_batch(S, max(1, args.p256))
+        elif args.only == "ed25519":
+            line = {"ed25519": ed25519_batch(S, max(1, args.ed25519), 8, 90)}
         elif args.only == "sha_batch":
             cnt, lo, hi = (int(x) for x in args.sha_batch.split(","))
             line = c4_batch(S, cnt, lo, hi)
@@ -360,6 +366,10 @@ def main():
     if args.p256:
         log("ecdsa-p256 records: plain vs batched")
         res.update(p256_batch(S, args.p256))
+    if args.ed25519:
+        log("ed25519 (C3): plain vs batched, blowup 8 / 90 queries and blowup 32 / 54 queries")
+        res["ed25519"] = ed25519_batch(S, args.ed25519, 8, 90)
+        res["ed25519_reference_params"] = ed25519_batch(S, args.ed25519, 32, 54)
     if args.sha_batch:
         cnt, lo, hi = (int(x) for x in args.sha_batch.split(","))
         log("sha batch: %d uneven SHA-256/512 proofs, 2^%d..2^%d rows" % (cnt, lo, hi))
@@ -536,6 +546,41 @@ def p256_batch(S, nproofs):
                      "reference_lde_bytes_per_proof": lde_bytes, "bit_exact_vs_plain": exact}}
 
 
+def ed25519_batch(S, nproofs, blowup, queries):
+    """prove_zsk_ksk_binding_v2 (crates/swarm-dns/src/prover.rs:856-951) on
+    nproofs Ed25519 signatures from seeded keys over seeded random 32-256
+    byte messages: the v16 verify AIR (k_scalar 256, 41 519 columns, 70 606
+    constraints, 1024 or 2048 rows), one proof per call vs one batch call
+    (lockstep per trace shape), every batched proof equal to its single-call
+    proof."""
+    from stark_swarm_amd.zone import synthetic_ed25519_signatures
+    sigs = synthetic_ed25519_signatures(nproofs, 0x5EED0003)
+    reqs = [(pk, sig, msg, bytes([i & 0xFF]) * 32, bytes(32)) for i, (pk, sig, msg) in enumerate(sigs)]
+    S.prove_zsk_ksk_binding_v2_batch(reqs[:1], blowup=blowup, queries=queries)  # warm
+    lat, plain = [], []
+    t0 = time.perf_counter()
+    for r in reqs:
+        t = time.perf_counter()
+        plain.append(S.prove_zsk_ksk_binding_v2_batch([r], blowup=blowup, queries=queries)[0].proof_blob)
+        lat.append(time.perf_counter() - t)
+    plain_dt = time.perf_counter() - t0
+    S.prove_zsk_ksk_binding_v2_batch(reqs, blowup=blowup, queries=queries)  # warm: the batch's buffers, once
+    t0 = time.perf_counter()
+    outs = S.prove_zsk_ksk_binding_v2_batch(reqs, blowup=blowup, queries=queries)
+    bdt = time.perf_counter() - t0
+    exact = all(o.proof_blob == p for o, p in zip(outs, plain))
+    if not exact:
+        raise SystemExit("batched Ed25519 proofs differ from the single-call proofs")
+    lat.sort()
+    heights = sorted({o.n_trace for o in outs})
+    return {"proofs": nproofs, "k_scalar": 256, "blowup": blowup, "queries": queries, "trace_rows": heights,
+            "columns": 41519, "constraints": 70606, "message_bytes": [32, 256],
+            "plain_proofs_per_s": round(nproofs / plain_dt, 2), "batched_proofs_per_s": round(nproofs / bdt, 2),
+            "speedup": round(plain_dt / bdt, 3), "plain_latency_ms_p50": round(1e3 * lat[len(lat) // 2], 3),
+            "plain_latency_ms_p99": round(1e3 * lat[min(len(lat) - 1, (99 * len(lat)) // 100)], 3),
+            "reference_lde_bytes_per_proof": 8.0 * 41519 * 1024 * blowup, "bit_exact_vs_plain": exact}
+
+
 def c4_batch(S, nproofs, log_lo, log_hi):
     """Batched uneven SHA-256/512 proofs: nproofs alternating SHA-256 and
     SHA-512 AIR proofs (the prove_ds_ksk_binding pipeline: LDE x32, merge,
@@ -591,52 +636,61 @@ def c4_batch(S, nproofs, log_lo, log_hi):
 
 
 def serve(S, rates, per_rate, workers=4, max_batch=32):
-    """Serving: RRset proof requests arrive as a Poisson process at each rate
-    (requests/s) and go through the proving server (HBM-budget admission,
-    continuous batching over `workers` streams). Request mix: 40 % inner-shard
-    proofs of an RRset group of 8-64 synthetic zone records (HashRollup), 10 %
-    DS->KSK SHA-256 bindings of 44-132 byte DNSKEYs, 20 % RSA-2048 and 30 %
-    ECDSA-P256 signature records. Latency = completion - arrival. A sample of
-    results is checked byte-equal to the same request proven alone."""
-    import hashlib
+    """Serving: a synthetic signed zone of 10 000 RRsets (owner names of 2-4
+    labels of 3-12 lowercase letters; 60 % A / 25 % AAAA / 15 % TXT with
+    16-255 byte TXT rdata; signatures 50 % Ed25519 / 30 % ECDSA-P256 / 20 %
+    RSA-2048 from seeded keys). Each request is the proof of one RRset's
+    signature -- the per-record proof of the mixed-algorithm zone example
+    (Ed25519 verify AIR v16 at k_scalar 256, the P-256 ECDSA double-multirow
+    AIR, the RSA-2048 exponentiation AIR; blowup 32, 54 STIR queries, the
+    record's pk_hash_pre as public-input hash). Requests arrive as a Poisson
+    process at each rate (requests/s) and go through the proving server
+    (HBM-budget admission, continuous batching over `workers` streams).
+    Latency = completion - arrival. A sample of results is checked
+    byte-equal to the same request proven alone."""
     import threading
+
     from stark_swarm_amd.server import ProvingServer
-    from stark_swarm_amd.zone import build_synthetic_zone
+    from stark_swarm_amd.zone import ZoneSigner, synthetic_rrset_zone
     rng = np.random.default_rng(0x5EED0005)
-    zone = build_synthetic_zone(10000)
-    rsa_pool = rsa_records(S, 64, 0x5EED0007)
-    from stark_swarm_amd.zone import synthetic_ecdsa_records
-    p256_pool = synthetic_ecdsa_records(64, 0x5EED000A)
-    salt = b"swarm-test-zone1"
+    log("serve: zone of 10000 RRsets, seeded keys")
+    zone = synthetic_rrset_zone(10000, 0x5EED000C)
+    signer = ZoneSigner(0x5EED000D)
+    root0 = bytes(32)  # the record proof has no merkle root; it enters only the binding's own pi_hash
     out = []
     # the workers take the plans tuned so far but time none themselves:
     # tuning a new batch shape mid-stream would stall the requests behind it
     S.set_autotune_default(False)
+
+    def submit(srv, q):
+        if q[0] == "ed25519":
+            return srv.submit_ed25519(q[1], q[2], q[3], q[4], root0)
+        if q[0] == "p256":
+            return srv.submit_p256(q[1])
+        return srv.submit_rsa(q[1], q[2])
+
+    def alone(q):
+        if q[0] == "ed25519":
+            return S.prove_zsk_ksk_binding_v2(q[1], q[2], q[3], q[4], root0)
+        if q[0] == "p256":
+            return S.prove_record_ecdsa_batch([q[1]])[0]
+        return S.prove_rsa_exp_batch([q[1]], [q[2]])[0]
+
     with ProvingServer(workers=workers, hbm_budget=64 << 30, max_batch=max_batch) as srv:
-        # warm every worker's context once
-        for rid in [srv.submit_ds_ksk(b"w" * 44, b"\0" * 32, b"\0" * 32) for _ in range(workers)]:
-            srv.wait(rid)
-        for rid in [srv.submit_rsa(rsa_pool[0], b"\0" * 32) for _ in range(workers)]:
-            srv.wait(rid)
-        for rid in [srv.submit_p256(p256_pool[0]) for _ in range(workers)]:
-            srv.wait(rid)
+        # warm every worker's context once per kind and Ed25519 trace shape (SHA-512 blocks of R || A || M)
+        warm = {}
+        for r in zone[:3000]:
+            shape = (r.alg, (64 + len(r.message()) + 17 + 127) // 128 if r.alg == 15 else 0)
+            if shape not in warm:
+                warm[shape] = signer.request(r)
+        for q in warm.values():
+            for rid in [submit(srv, q) for _ in range(workers)]:
+                srv.wait(rid)
         for rate in rates:
             n = per_rate
-            reqs = []
-            for i in range(n):
-                fsb = rng.bytes(32)
-                u = rng.random()
-                if u < 0.4:
-                    k = int(rng.integers(8, 65))
-                    o = int(rng.integers(0, len(zone) - k))
-                    reqs.append(("shard", S.PackedShards([zone[o:o + k]]), fsb, (o, k)))
-                elif u < 0.6:
-                    reqs.append(("rsa", rsa_pool[int(rng.integers(0, len(rsa_pool)))], fsb, None))
-                elif u < 0.9:
-                    reqs.append(("p256", p256_pool[int(rng.integers(0, len(p256_pool)))], None, None))
-                else:
-                    key = rng.bytes(int(rng.integers(44, 133)))
-                    reqs.append(("ksk", key, fsb, hashlib.sha256(key).digest()))
+            picks = rng.integers(0, len(zone), size=n)
+            log("serve rate %d/s: signing %d RRsets" % (rate, n))
+            reqs = [signer.request(zone[int(i)]) for i in picks]
             gaps = rng.exponential(1.0 / rate, size=n)
             ids = [None] * n
             res = [None] * n
@@ -652,46 +706,41 @@ def serve(S, rates, per_rate, workers=4, max_batch=32):
                 t.start()
             t0 = time.perf_counter()
             due = t0
-            for j, (kind, x, fsb, aux) in enumerate(reqs):
+            for j, q in enumerate(reqs):
                 due += gaps[j]
                 d = due - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
-                if kind == "shard":
-                    ids[j] = srv.submit_inner_shard(salt, x, fsb)
-                elif kind == "rsa":
-                    ids[j] = srv.submit_rsa(x, fsb)
-                elif kind == "p256":
-                    ids[j] = srv.submit_p256(x)
-                else:
-                    ids[j] = srv.submit_ds_ksk(x, aux, fsb)
+                ids[j] = submit(srv, q)
             for t in waiters:
                 t.join()
             wall = time.perf_counter() - t0
-            lat = sorted(r.latency for r in res)
-            out.append({"rate_per_s": rate, "requests": n, "proofs_per_s": round(n / wall, 1),
-                        "latency_ms_p50": round(1e3 * lat[n // 2], 2),
-                        "latency_ms_p99": round(1e3 * lat[min(n - 1, (99 * n) // 100)], 2),
-                        "mean_batch": round(sum(r.batch_size for r in res) / n, 2)})
-            log("serve rate %d/s: %s" % (rate, out[-1]))
+
+            def pct(v, f):
+                v = sorted(v)
+                return round(1e3 * v[min(len(v) - 1, int(f * len(v)))], 2)
+            lat = [r.latency for r in res]
+            line = {"rate_per_s": rate, "requests": n, "proofs_per_s": round(n / wall, 1),
+                    "latency_ms_p50": pct(lat, 0.5), "latency_ms_p99": pct(lat, 0.99),
+                    "mean_batch": round(sum(r.batch_size for r in res) / n, 2)}
+            for kind in ("ed25519", "p256", "rsa"):
+                lk = [r.latency for q, r in zip(reqs, res) if q[0] == kind]
+                if lk:
+                    line[kind] = {"requests": len(lk), "latency_ms_p50": pct(lk, 0.5), "latency_ms_p99": pct(lk, 0.99)}
+            out.append(line)
+            log("serve rate %d/s: %s" % (rate, line))
             # byte-equality of a sample against the request proven alone
-            for j in range(0, n, max(1, n // 6)):
-                kind, x, fsb, aux = reqs[j]
-                if kind == "shard":
-                    alone = S.prove_inner_shard_batch(salt, x, [fsb])[0]
-                elif kind == "rsa":
-                    alone = S.prove_rsa_exp_batch([x], [fsb])[0]
-                elif kind == "p256":
-                    alone = S.prove_record_ecdsa_batch([x])[0]
-                else:
-                    alone = S.prove_ds_ksk_binding(x, aux, fsb)
-                if alone.proof_blob != res[j].proof_blob or alone.pi_hash != res[j].pi_hash:
-                    raise SystemExit("served proof %d differs from the proof made alone" % j)
+            for j in range(0, n, max(1, n // 9)):
+                a = alone(reqs[j])
+                if a.proof_blob != res[j].proof_blob or a.pi_hash != res[j].pi_hash:
+                    raise SystemExit("served proof %d (%s) differs from the proof made alone" % (j, reqs[j][0]))
         st = srv.stats()
     return {"serve": {"workers": workers, "max_batch": max_batch, "hbm_budget_GiB": 64,
                       "peak_admitted_GiB": round(st["peak_bytes_admitted"] / 2**30, 3),
-                      "mix": "40% inner-shard RRset groups (8-64 records), 10% DS-KSK SHA-256 (44-132 B keys), "
-                             "20% RSA-2048 and 30% ECDSA-P256 signature records",
+                      "zone": "10000 RRsets: 60% A / 25% AAAA / 15% TXT (16-255 B), owner names of 2-4 labels of "
+                              "3-12 lowercase letters",
+                      "mix": "one RRset signature proof per request: 50% Ed25519 (verify AIR v16, k_scalar 256), "
+                             "30% ECDSA-P256, 20% RSA-2048; blowup 32, 54 queries, seeded keys",
                       "rates": out, "sample_bit_exact_vs_alone": True}}
 
 

@@ -154,7 +154,10 @@ typedef struct stark_server stark_server;
 #define STARK_SERVE_INNER_SHARD 1 /* prove_inner_shard request */
 #define STARK_SERVE_RSA 2         /* RSA-2048 record (stark_prove_rsa_exp_batch with one record) */
 #define STARK_SERVE_P256 3        /* ECDSA-P256 record (stark_prove_ecdsa_p256_batch) */
+#define STARK_SERVE_ED25519 4     /* Ed25519 ZSK->KSK binding (stark_prove_zsk_ksk_binding_v2_batch, k_scalar 256) */
 #define STARK_SERVER_DRY_RUN 1    /* flag: schedule only, no device work (admission tests) */
+#define STARK_SERVER_NO_VALIDATE 2 /* flag: skip the signature checks at submit, so a bad request reaches a
+                                      batch (tests of the per-request fallback) */
 #define STARK_E_TIMEOUT (-6)
 
 typedef struct {
@@ -168,6 +171,11 @@ typedef struct {
     const uint8_t *fs_binding; /* 32 bytes; RSA: the public-inputs hash */
     const uint8_t *rsa_n, *rsa_s, *rsa_em; /* RSA: 256 bytes each, big-endian, s < n */
     const uint8_t *p256_qx, *p256_qy, *p256_r, *p256_s, *p256_digest; /* ECDSA-P256: 32 bytes each, big-endian */
+    const uint8_t *ed_pk, *ed_sig; /* Ed25519: 32-byte public key, 64-byte signature */
+    const uint8_t *ed_msg;         /* Ed25519: the signed bytes */
+    size_t ed_msg_len;
+    const uint8_t *ed_merkle_root; /* Ed25519: 32 bytes */
+    double deadline_s; /* > 0: the request completes with STARK_E_TIMEOUT unless admitted within this many seconds */
 } stark_serve_request;
 
 typedef struct {
@@ -183,6 +191,7 @@ typedef struct {
     uint8_t *proof;    /* stark_free() it */
     size_t proof_len;
     double t_submit, t_start, t_done; /* seconds since the server started; t_start = admission */
+    char error[192]; /* status != 0: what failed (this request's own error), NUL-terminated */
 } stark_serve_result;
 
 /* ---- context ---- */
@@ -404,7 +413,14 @@ int stark_prove_outer_rollup(stark_ctx *ctx, const uint8_t *pi_hashes, size_t n,
  * HBM budget, up to max_batch, and proves them as one lockstep batch while
  * other workers' batches run on their own streams. Each result equals the
  * single-call result for that request (stark_prove_ds_ksk_binding /
- * stark_prove_inner_shard). submit copies the request's inputs. */
+ * stark_prove_inner_shard). submit copies the request's inputs.
+ * Isolation: submit rejects (STARK_E_INVALID) a signature request whose
+ * signature does not verify; a batch that still fails is proved again one
+ * request at a time, so only the failing request's result carries a status
+ * and its own error text. A request with deadline_s > 0 that no worker
+ * admits within deadline_s completes with STARK_E_TIMEOUT and no proof.
+ * stark_server_last_error gives the calling thread's last submit or wait
+ * error (thread-local, so concurrent callers do not see each other's). */
 int stark_server_create(int device, int workers, size_t hbm_budget, size_t max_batch, int flags, stark_server **out);
 void stark_server_destroy(stark_server *s); /* finishes queued requests first */
 int stark_server_submit(stark_server *s, const stark_serve_request *req, uint64_t *id);

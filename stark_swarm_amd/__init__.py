@@ -99,6 +99,8 @@ from .api import (  # noqa: F401
     deep_ali_merge_ed25519_verify_streaming,
     prove_zsk_ksk_binding_v2,
     prove_zsk_ksk_binding_v2_batch,
+    ed25519_record_pk_hash,
+    record_pi_hash,
     deep_fri_prove,
     deep_fri_prove_batch,
     default_context,

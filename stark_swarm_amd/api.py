@@ -1192,6 +1192,25 @@ def deep_ali_merge_ed25519_verify_streaming(trace_evals_on_lde, combination_coef
     return out, CompositionInfo(2 * n_trace, n_trace, n_trace / n, shape.n_constraints, 2, shape.width)
 
 
+def ed25519_record_pk_hash(pubkey, signature, message):
+    """pk_hash_pre of prove_record_ed25519 (mixed_algorithm_zone_rollup.rs:
+    155-162): SHA3-256("ED25519-MIXEDZONE" || A || signature || message). That
+    record proof is prove_zsk_ksk_binding_v2's proof with this hash as the
+    FRI public-input hash (k_scalar 256, blowup 32, 54 queries)."""
+    import hashlib
+    h = hashlib.sha3_256(b"ED25519-MIXEDZONE")
+    for part in (pubkey, signature, message):
+        h.update(bytes(part))
+    return h.digest()
+
+
+def record_pi_hash(pk_hash_pre, root_f0):
+    """Per-record pi_hash of the mixed-algorithm zone
+    (mixed_algorithm_zone_rollup.rs:176-182): SHA3-256(pk_hash_pre || root_f0)."""
+    import hashlib
+    return hashlib.sha3_256(bytes(pk_hash_pre) + bytes(root_f0)).digest()
+
+
 @dataclass
 class ZskKskOutput:
     """ZskKskOutput of prove_zsk_ksk_binding_v2 (prover.rs:856-951)."""

@@ -2,6 +2,7 @@
 // Every entry point catches library errors and turns them into status codes.
 #include "../../include/stark_gpu.h"
 
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -28,8 +29,10 @@ struct stark_ctx {
 
 struct stark_server {
     sg::Server *s = nullptr;
-    std::string err;
 };
+
+// last submit / wait error of the calling thread (callers submit from many threads)
+static thread_local std::string server_err;
 
 namespace {
 
@@ -678,7 +681,8 @@ int stark_server_create(int device, int workers, size_t hbm_budget, size_t max_b
     stark_server *x = new (std::nothrow) stark_server();
     if (!x) return STARK_E_OOM;
     try {
-        x->s = new sg::Server(device, workers, hbm_budget, (uint32_t)max_batch, (flags & STARK_SERVER_DRY_RUN) != 0);
+        x->s = new sg::Server(device, workers, hbm_budget, (uint32_t)max_batch, (flags & STARK_SERVER_DRY_RUN) != 0,
+                              (flags & STARK_SERVER_NO_VALIDATE) == 0);
     } catch (const sg::Error &e) {
         delete x;
         return e.code;
@@ -701,6 +705,8 @@ int stark_server_submit(stark_server *s, const stark_serve_request *q, uint64_t 
     try {
         sg::ServeRequest r;
         r.kind = q->kind;
+        SG_REQUIRE(q->deadline_s >= 0, "server submit: negative deadline");
+        r.deadline_s = q->deadline_s;
         SG_REQUIRE(q->fs_binding || q->kind == STARK_SERVE_P256, "server submit: null fs_binding");
         if (q->fs_binding) memcpy(r.fs_binding.data(), q->fs_binding, 32);
         if (q->kind == STARK_SERVE_P256) {
@@ -711,6 +717,13 @@ int stark_server_submit(stark_server *s, const stark_serve_request *q, uint64_t 
             memcpy(r.p256.r.data(), q->p256_r, 32);
             memcpy(r.p256.s.data(), q->p256_s, 32);
             memcpy(r.p256.digest.data(), q->p256_digest, 32);
+        } else if (q->kind == STARK_SERVE_ED25519) {
+            SG_REQUIRE(q->ed_pk && q->ed_sig && (q->ed_msg || q->ed_msg_len == 0) && q->ed_merkle_root,
+                       "server submit: null Ed25519 input");
+            memcpy(r.ed_pk.data(), q->ed_pk, 32);
+            memcpy(r.ed_sig.data(), q->ed_sig, 64);
+            if (q->ed_msg_len) r.ed_msg.assign(q->ed_msg, q->ed_msg + q->ed_msg_len);
+            memcpy(r.ed_root.data(), q->ed_merkle_root, 32);
         } else if (q->kind == STARK_SERVE_DS_KSK) {
             SG_REQUIRE((q->dnskey || q->dnskey_len == 0) && q->parent_ds_hash, "server submit: null DS-KSK input");
             if (q->dnskey_len) r.key.assign(q->dnskey, q->dnskey + q->dnskey_len);
@@ -734,10 +747,10 @@ int stark_server_submit(stark_server *s, const stark_serve_request *q, uint64_t 
         *id = s->s->submit(std::move(r));
         return STARK_OK;
     } catch (const sg::Error &e) {
-        s->err = e.what();
+        server_err = e.what();
         return e.code;
     } catch (const std::exception &e) {
-        s->err = e.what();
+        server_err = e.what();
         return STARK_E_INTERNAL;
     }
 }
@@ -745,7 +758,10 @@ int stark_server_submit(stark_server *s, const stark_serve_request *q, uint64_t 
 int stark_server_wait(stark_server *s, uint64_t id, double timeout_s, stark_serve_result *out) {
     if (!s || !s->s || !out) return STARK_E_INVALID;
     sg::ServeResult r;
-    if (!s->s->wait(id, r, timeout_s)) return STARK_E_TIMEOUT;
+    if (!s->s->wait(id, r, timeout_s)) {
+        server_err = "server wait: result not ready in time";
+        return STARK_E_TIMEOUT;
+    }
     out->id = r.id;
     out->status = r.status;
     out->kind = r.kind;
@@ -759,7 +775,8 @@ int stark_server_wait(stark_server *s, uint64_t id, double timeout_s, stark_serv
     out->t_start = r.t_start;
     out->t_done = r.t_done;
     out->proof = copy_out(r.proof, &out->proof_len);
-    if (r.status) s->err = r.error;
+    snprintf(out->error, sizeof out->error, "%s", r.error.c_str());
+    if (r.status) server_err = r.error;
     return STARK_OK;
 }
 
@@ -771,7 +788,7 @@ int stark_server_stats(stark_server *s, size_t *completed, size_t *peak, size_t 
     return STARK_OK;
 }
 
-const char *stark_server_last_error(const stark_server *s) { return s ? s->err.c_str() : "null server"; }
+const char *stark_server_last_error(const stark_server *s) { return s ? server_err.c_str() : "null server"; }
 
 size_t stark_host_sha512_trace_height(size_t len) { return sg::sha512_trace_height(len); }
 

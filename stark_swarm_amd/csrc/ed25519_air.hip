@@ -166,28 +166,27 @@ static void ftobytes(const Fe &a, uint8_t b[32]) {
         for (int i = 0; i < 8; i++) b[8 * k + i] = (uint8_t)(a.v[k] >> (8 * i));
 }
 
-using Lb = std::array<int64_t, NL>;
+// 10 limbs as the trace holds them, with the field value they represent
+// (kept beside the limbs so no gadget re-derives it)
+struct Lb {
+    std::array<int64_t, NL> l{};
+    Fe v{{0, 0, 0, 0}};
+    int64_t &operator[](size_t i) { return l[i]; }
+    int64_t operator[](size_t i) const { return l[i]; }
+};
 // tight limbs of a canonical value (a frozen FieldElement, ed25519_field.rs:154-158)
 static Lb limbs(const Fe &a) {
-    Lb l;
+    Lb r;
+    r.v = a;
     for (int i = 0; i < NL; i++) {
         const int off = LOFF[i], lo = off / 64, sh = off % 64;
         uint64_t x = a.v[lo] >> sh;
         if (sh + LW[i] > 64 && lo + 1 < 4) x |= a.v[lo + 1] << (64 - sh);
-        l[i] = (int64_t)(x & ((1ULL << LW[i]) - 1));
-    }
-    return l;
-}
-// sum_i l_i 2^off_i mod p for possibly loose limbs
-static Fe of_limbs(const Lb &l) {
-    Fe r = fzero();
-    for (int i = 0; i < NL; i++) {
-        Fe s = fzero();
-        s.v[LOFF[i] / 64] = 1ULL << (LOFF[i] % 64);
-        r = fadd(r, fmul(fsmall(l[i]), s));
+        r.l[i] = (int64_t)(x & ((1ULL << LW[i]) - 1));
     }
     return r;
 }
+static const Fe &of_limbs(const Lb &l) { return l.v; }
 
 struct Pt {
     Fe X, Y, Z, T;
@@ -211,6 +210,7 @@ Consts::Consts() {
     const uint64_t e14[4] = {0xFFFFFFFFFFFFFFFBULL, ~0ULL, ~0ULL, 0x1FFFFFFFFFFFFFFFULL};  // (p - 1) / 4
     sqrtm1 = fpow(fsmall(2), e14);
     d_l = limbs(d);
+    d2_l.v = d2;
     for (int i = 0; i < NL; i++) d2_l[i] = 2 * d_l[i];
     uint8_t bc[32];
     bc[0] = 0x58;
@@ -768,8 +768,8 @@ struct Native {
     std::vector<uint64_t> cst;       // pins
 };
 
-static Lb one_l() { return Lb{1, 0, 0, 0, 0, 0, 0, 0, 0, 0}; }
-static Lb zero_l() { return Lb{}; }
+static Lb one_l() { return limbs(fone()); }
+static Lb zero_l() { return limbs(fzero()); }
 
 static void row_common(Em &e, const Layout &L) {
     if (!e.desc) return;

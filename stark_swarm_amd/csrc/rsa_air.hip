@@ -491,4 +491,11 @@ void rsa_pkcs1_em_sha256(const uint8_t digest[32], uint8_t em[256]) {
     memcpy(em + 256 - 32, digest, 32);
 }
 
+bool rsa_record_verifies(const RsaRecord &in) {
+    std::vector<rsa::RowWords> rows(32);
+    rsa::RecWords rec;
+    if (rsa::record_rows(in, 32, rec, rows.data()) != 0) return false;
+    return memcmp(rows[rsa::OUT_ROW].c, rec.em, sizeof rec.em) == 0;
+}
+
 }  // namespace sg

@@ -47,6 +47,9 @@ void merge_rsa_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint32_t R, uint64
 // Exact arithmetic makes the result equal to merge_rsa_batch. blowup >= 4.
 void merge_rsa_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint32_t R, uint64_t n_trace,
                                 uint64_t blowup, uint64_t *c_eval);
+// n > 0, s < n and s^65537 mod n == em: the record the exponentiation AIR
+// proves (its output-row constraints hold)
+bool rsa_record_verifies(const RsaRecord &rec);
 // emsa_pkcs1_v1_5_encode_sha256 (crates/deep_ali/src/rsa2048.rs) for k = 256
 void rsa_pkcs1_em_sha256(const uint8_t digest[32], uint8_t em[256]);
 

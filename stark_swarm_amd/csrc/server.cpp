@@ -13,6 +13,11 @@ namespace sg {
 static const uint64_t BLOWUP = 32;
 
 uint64_t Server::need_bytes(const ServeRequest &r) {
+    if (r.kind == SERVE_ED25519) {  // trace and odd coset, gate-group sums, phi, FRI
+        const EdShape sh = ed25519_shape(r.ed_msg.size(), 256);
+        const uint64_t h = sh.height, n0 = h * BLOWUP;
+        return 16ULL * sh.width * h + 8ULL * 400 * 2 * h + 8ULL * 64 * n0;
+    }
     if (r.kind == SERVE_RSA) {
         const uint64_t h = 32, n0 = h * BLOWUP;
         return 8ULL * RSA_W * 4 * h + 8ULL * 3 * 2 * h + 8ULL * 24 * n0;
@@ -33,9 +38,9 @@ uint64_t Server::need_bytes(const ServeRequest &r) {
     return 8ULL * 4 * (2 * nt + 2 * n0) + 8ULL * 16 * n0 + 2 * rec;
 }
 
-Server::Server(int device, int workers, uint64_t hbm_budget, uint32_t max_batch, bool dry_run)
+Server::Server(int device, int workers, uint64_t hbm_budget, uint32_t max_batch, bool dry_run, bool validate)
     : device_(device), budget_(hbm_budget), max_batch_(std::max<uint32_t>(1, max_batch)), dry_(dry_run),
-      t0_(std::chrono::steady_clock::now()) {
+      validate_(validate), t0_(std::chrono::steady_clock::now()) {
     SG_REQUIRE(workers >= 1 && workers <= 16, "server: 1 to 16 workers");
     SG_REQUIRE(hbm_budget > 0, "server: zero HBM budget");
     if (!dry_) {
@@ -67,8 +72,18 @@ double Server::now() const {
 }
 
 uint64_t Server::submit(ServeRequest &&r) {
-    SG_REQUIRE(r.kind == SERVE_DS_KSK || r.kind == SERVE_INNER_SHARD || r.kind == SERVE_RSA || r.kind == SERVE_P256,
+    SG_REQUIRE(r.kind == SERVE_DS_KSK || r.kind == SERVE_INNER_SHARD || r.kind == SERVE_RSA || r.kind == SERVE_P256 ||
+                   r.kind == SERVE_ED25519,
                "server: unknown request kind");
+    SG_REQUIRE(r.deadline_s >= 0, "server: negative deadline");
+    if (validate_) {  // the checks the provers make, before the request can join a batch
+        if (r.kind == SERVE_RSA)
+            SG_REQUIRE(rsa_record_verifies(r.rsa), "server: RSA record needs 0 < n, s < n and s^65537 mod n = em");
+        if (r.kind == SERVE_P256) (void)p256_ecdsa_chains(r.p256);  // throws E_INVALID for a bad key or signature
+        if (r.kind == SERVE_ED25519)
+            SG_REQUIRE(ed25519_verify(EdSig{r.ed_pk.data(), r.ed_sig.data(), r.ed_msg.data(), r.ed_msg.size()}),
+                       "server: Ed25519 signature does not verify");
+    }
     if (r.kind == SERVE_INNER_SHARD) {
         SG_REQUIRE(!r.domains.empty(), "server: an inner shard needs at least one record");
         SG_REQUIRE(r.rdata.size() == r.domains.size() && r.rtype.size() == r.domains.size() &&
@@ -136,6 +151,31 @@ void Server::worker(int w) {
                        (!queue_.empty() && (in_use_ + queue_.front().need <= budget_ || in_use_ == 0));
             });
             if (queue_.empty()) return;  // stop requested and nothing left
+            // requests whose deadline passed before admission complete without device work
+            const double tn = now();
+            std::vector<Item> expired;
+            for (auto q = queue_.begin(); q != queue_.end();) {
+                if (q->req.deadline_s > 0 && tn - q->t_submit > q->req.deadline_s) {
+                    expired.push_back(std::move(*q));
+                    q = queue_.erase(q);
+                } else {
+                    ++q;
+                }
+            }
+            for (Item &x : expired) {
+                ServeResult r;
+                r.id = x.id;
+                r.kind = x.req.kind;
+                r.status = E_TIMEOUT;
+                r.error = "server: deadline passed before admission";
+                r.t_submit = x.t_submit;
+                r.t_start = r.t_done = tn;
+                done_[r.id] = std::move(r);
+                completed_++;
+            }
+            if (!expired.empty()) cv_done_.notify_all();
+            if (queue_.empty()) continue;
+            if (!(in_use_ + queue_.front().need <= budget_ || in_use_ == 0)) continue;
             batch.push_back(std::move(queue_.front()));
             queue_.pop_front();
             taken = batch[0].need;
@@ -179,82 +219,117 @@ void Server::worker(int w) {
 }
 
 void Server::run_batch(Ctx *c, std::vector<Item> &batch, std::vector<ServeResult> &res) {
-    const size_t K = batch.size();
     if (dry_) {
         uint64_t need = 0;
         for (auto &b : batch) need += b.need;
         std::this_thread::sleep_for(std::chrono::microseconds(200 + need / (64ULL << 20)));
         return;
     }
+    int code = 0;
+    std::string msg;
     try {
-        if (batch[0].req.kind == SERVE_RSA) {
-            std::vector<RsaRecord> recs(K);
-            std::vector<std::array<uint8_t, 32>> pis(K);
-            for (size_t i = 0; i < K; i++) {
-                recs[i] = batch[i].req.rsa;
-                pis[i] = batch[i].req.fs_binding;
-            }
-            std::vector<RsaProof> out = prove_rsa_batch(c, recs, 1, 32, pis);
-            for (size_t i = 0; i < K; i++) {
-                memcpy(res[i].pi_hash, out[i].pi_hash, 32);
-                memcpy(res[i].root_f0, out[i].root_f0, 32);
-                res[i].n_trace = out[i].n_trace;
-                res[i].proof = std::move(out[i].proof);
-            }
-        } else if (batch[0].req.kind == SERVE_P256) {
-            std::vector<P256Record> recs(K);
-            for (size_t i = 0; i < K; i++) recs[i] = batch[i].req.p256;
-            std::vector<P256Proof> out = prove_ecdsa_p256_batch(c, recs);
-            for (size_t i = 0; i < K; i++) {
-                memcpy(res[i].pi_hash, out[i].pi_hash, 32);
-                memcpy(res[i].root_f0, out[i].root_f0, 32);
-                res[i].n_trace = out[i].n_trace;
-                res[i].proof = std::move(out[i].proof);
-            }
-        } else if (batch[0].req.kind == SERVE_DS_KSK) {
-            std::vector<ByteSpan> keys(K);
-            std::vector<std::array<uint8_t, 32>> par(K), fsb(K);
-            for (size_t i = 0; i < K; i++) {
-                keys[i] = {batch[i].req.key.data(), batch[i].req.key.size()};
-                par[i] = batch[i].req.parent;
-                fsb[i] = batch[i].req.fs_binding;
-            }
-            std::vector<DsKsk> out = prove_ds_ksk_binding_batch(c, keys, par, fsb, 1);
-            for (size_t i = 0; i < K; i++) {
-                memcpy(res[i].pi_hash, out[i].pi_hash, 32);
-                memcpy(res[i].root_f0, out[i].root_f0, 32);
-                memcpy(res[i].aux, out[i].asserted_digest, 32);
-                res[i].n_trace = out[i].n_trace;
-                res[i].proof = std::move(out[i].proof);
-            }
-        } else {
-            std::vector<std::vector<DnsRec>> shards(K);
-            std::vector<std::array<uint8_t, 32>> fsb(K);
-            for (size_t i = 0; i < K; i++) {
-                const ServeRequest &q = batch[i].req;
-                for (size_t r = 0; r < q.domains.size(); r++)
-                    shards[i].push_back({q.domains[r].data(), q.domains[r].size(), q.rtype[r], q.ttl[r],
-                                         q.rdata[r].data(), q.rdata[r].size()});
-                fsb[i] = q.fs_binding;
-            }
-            std::vector<InnerShard> out = prove_inner_shard_batch(c, batch[0].req.salt.data(), shards, fsb, 1);
-            for (size_t i = 0; i < K; i++) {
-                memcpy(res[i].pi_hash, out[i].pi_hash, 32);
-                memcpy(res[i].root_f0, out[i].root_f0, 32);
-                memcpy(res[i].aux, out[i].merkle_root, 32);
-                res[i].n_trace = out[i].n_trace;
-                res[i].proof = std::move(out[i].proof);
-            }
-        }
+        prove_batch(c, batch, res);
+        return;
     } catch (const Error &e) {
-        for (auto &r : res) {
-            r.status = e.code;
-            r.error = e.what();
-        }
+        code = e.code;
+        msg = e.what();
     } catch (const std::exception &e) {
-        for (auto &r : res) {
-            r.status = E_INTERNAL;
-            r.error = e.what();
+        code = E_INTERNAL;
+        msg = e.what();
+    }
+    if (batch.size() == 1) {
+        res[0] = ServeResult();
+        res[0].status = code;
+        res[0].error = msg;
+        return;
+    }
+    // one request of the batch failed it: prove each alone, so that only the
+    // failing request carries its error and the others get their proofs
+    for (size_t i = 0; i < batch.size(); i++) {
+        std::vector<Item> one;
+        one.push_back(std::move(batch[i]));
+        std::vector<ServeResult> r1(1);
+        run_batch(c, one, r1);
+        batch[i] = std::move(one[0]);
+        res[i] = std::move(r1[0]);
+    }
+}
+
+void Server::prove_batch(Ctx *c, std::vector<Item> &batch, std::vector<ServeResult> &res) {
+    const size_t K = batch.size();
+    if (batch[0].req.kind == SERVE_RSA) {
+        std::vector<RsaRecord> recs(K);
+        std::vector<std::array<uint8_t, 32>> pis(K);
+        for (size_t i = 0; i < K; i++) {
+            recs[i] = batch[i].req.rsa;
+            pis[i] = batch[i].req.fs_binding;
+        }
+        std::vector<RsaProof> out = prove_rsa_batch(c, recs, 1, 32, pis);
+        for (size_t i = 0; i < K; i++) {
+            memcpy(res[i].pi_hash, out[i].pi_hash, 32);
+            memcpy(res[i].root_f0, out[i].root_f0, 32);
+            res[i].n_trace = out[i].n_trace;
+            res[i].proof = std::move(out[i].proof);
+        }
+    } else if (batch[0].req.kind == SERVE_P256) {
+        std::vector<P256Record> recs(K);
+        for (size_t i = 0; i < K; i++) recs[i] = batch[i].req.p256;
+        std::vector<P256Proof> out = prove_ecdsa_p256_batch(c, recs);
+        for (size_t i = 0; i < K; i++) {
+            memcpy(res[i].pi_hash, out[i].pi_hash, 32);
+            memcpy(res[i].root_f0, out[i].root_f0, 32);
+            res[i].n_trace = out[i].n_trace;
+            res[i].proof = std::move(out[i].proof);
+        }
+    } else if (batch[0].req.kind == SERVE_ED25519) {
+        std::vector<EdSig> sigs(K);
+        std::vector<std::array<uint8_t, 32>> fsb(K), roots(K);
+        for (size_t i = 0; i < K; i++) {
+            const ServeRequest &q = batch[i].req;
+            sigs[i] = EdSig{q.ed_pk.data(), q.ed_sig.data(), q.ed_msg.data(), q.ed_msg.size()};
+            fsb[i] = q.fs_binding;
+            roots[i] = q.ed_root;
+        }
+        std::vector<ZskKsk> out = prove_zsk_ksk_binding_v2_batch(c, sigs, fsb, roots, 256, BLOWUP, 54);
+        for (size_t i = 0; i < K; i++) {
+            memcpy(res[i].pi_hash, out[i].pi_hash, 32);
+            memcpy(res[i].root_f0, out[i].root_f0, 32);
+            res[i].n_trace = out[i].n_trace;
+            res[i].proof = std::move(out[i].proof);
+        }
+    } else if (batch[0].req.kind == SERVE_DS_KSK) {
+        std::vector<ByteSpan> keys(K);
+        std::vector<std::array<uint8_t, 32>> par(K), fsb(K);
+        for (size_t i = 0; i < K; i++) {
+            keys[i] = {batch[i].req.key.data(), batch[i].req.key.size()};
+            par[i] = batch[i].req.parent;
+            fsb[i] = batch[i].req.fs_binding;
+        }
+        std::vector<DsKsk> out = prove_ds_ksk_binding_batch(c, keys, par, fsb, 1);
+        for (size_t i = 0; i < K; i++) {
+            memcpy(res[i].pi_hash, out[i].pi_hash, 32);
+            memcpy(res[i].root_f0, out[i].root_f0, 32);
+            memcpy(res[i].aux, out[i].asserted_digest, 32);
+            res[i].n_trace = out[i].n_trace;
+            res[i].proof = std::move(out[i].proof);
+        }
+    } else {
+        std::vector<std::vector<DnsRec>> shards(K);
+        std::vector<std::array<uint8_t, 32>> fsb(K);
+        for (size_t i = 0; i < K; i++) {
+            const ServeRequest &q = batch[i].req;
+            for (size_t r = 0; r < q.domains.size(); r++)
+                shards[i].push_back({q.domains[r].data(), q.domains[r].size(), q.rtype[r], q.ttl[r],
+                                     q.rdata[r].data(), q.rdata[r].size()});
+            fsb[i] = q.fs_binding;
+        }
+        std::vector<InnerShard> out = prove_inner_shard_batch(c, batch[0].req.salt.data(), shards, fsb, 1);
+        for (size_t i = 0; i < K; i++) {
+            memcpy(res[i].pi_hash, out[i].pi_hash, 32);
+            memcpy(res[i].root_f0, out[i].root_f0, 32);
+            memcpy(res[i].aux, out[i].merkle_root, 32);
+            res[i].n_trace = out[i].n_trace;
+            res[i].proof = std::move(out[i].proof);
         }
     }
 }

@@ -8,8 +8,14 @@
 // It stands in for the swarm worker's request loop
 // (crates/swarm-worker/src/main.rs:220-335, which calls prove_inner_shard
 // from tokio spawn_blocking threads) on the device side; besides inner
-// shards it proves DS->KSK bindings and RSA-2048 and ECDSA-P256 signature
-// records.
+// shards it proves DS->KSK bindings, RSA-2048 and ECDSA-P256 signature
+// records and Ed25519 ZSK->KSK bindings (prove_zsk_ksk_binding_v2).
+//
+// Isolation: submit checks that a signature request's signature verifies
+// (the provers would assert); a batch that still fails is proved again one
+// request at a time, so only the failing request carries the error. A
+// request may carry a deadline: one not admitted in time completes with
+// E_TIMEOUT without device work.
 #pragma once
 #include <stdint.h>
 
@@ -28,7 +34,8 @@
 
 namespace sg {
 
-enum ServeKind : int { SERVE_DS_KSK = 0, SERVE_INNER_SHARD = 1, SERVE_RSA = 2, SERVE_P256 = 3 };
+enum ServeKind : int { SERVE_DS_KSK = 0, SERVE_INNER_SHARD = 1, SERVE_RSA = 2, SERVE_P256 = 3, SERVE_ED25519 = 4 };
+constexpr int E_TIMEOUT = -6;
 
 struct ServeRequest {
     int kind = SERVE_DS_KSK;
@@ -41,7 +48,12 @@ struct ServeRequest {
     std::vector<uint32_t> ttl;
     RsaRecord rsa{};                    // RSA-2048: n, s, em (fs_binding = its public-input hash)
     P256Record p256{};                  // ECDSA-P256: qx, qy, r, s, digest
+    std::array<uint8_t, 32> ed_pk{};    // Ed25519: public key, signature, signed data, merkle root
+    std::array<uint8_t, 64> ed_sig{};
+    std::vector<uint8_t> ed_msg;
+    std::array<uint8_t, 32> ed_root{};
     std::array<uint8_t, 32> fs_binding{};
+    double deadline_s = 0;  // > 0: complete with E_TIMEOUT unless admitted within this many seconds
 };
 
 struct ServeResult {
@@ -61,7 +73,9 @@ public:
     // dry_run: schedule only (a worker sleeps in proportion to the request's
     // memory estimate and returns no proof) -- for testing admission on hosts
     // without a GPU.
-    Server(int device, int workers, uint64_t hbm_budget, uint32_t max_batch, bool dry_run);
+    // validate: check signatures at submit (off only to test the per-request
+    // fallback of a failing batch)
+    Server(int device, int workers, uint64_t hbm_budget, uint32_t max_batch, bool dry_run, bool validate = true);
     ~Server();
     uint64_t submit(ServeRequest &&r);
     // false on timeout; the result is removed from the server once returned
@@ -82,12 +96,13 @@ private:
     };
     void worker(int w);
     void run_batch(Ctx *c, std::vector<Item> &batch, std::vector<ServeResult> &res);
+    void prove_batch(Ctx *c, std::vector<Item> &batch, std::vector<ServeResult> &res);
     double now() const;
 
     const int device_;
     const uint64_t budget_;
     const uint32_t max_batch_;
-    const bool dry_;
+    const bool dry_, validate_;
     mutable std::mutex mu_;
     std::condition_variable cv_work_, cv_done_;
     std::deque<Item> queue_;

@@ -434,10 +434,14 @@ def test_server_results_equal_standalone(S):
     salt = b"swarm-test-zone1"
     reqs = []
     from stark_swarm_amd.zone import synthetic_ecdsa_records
+    from stark_swarm_amd.zone import synthetic_ed25519_signatures
     ecdsa = synthetic_ecdsa_records(4, 0x5EED0009)
-    for i in range(18):
+    eds = synthetic_ed25519_signatures(3, 0x5EED000A)
+    for i in range(21):
         fsb = hashlib.sha3_256(b"req%d" % i).digest()
-        if i % 3 == 0:
+        if i % 7 == 6:
+            reqs.append(("ed25519", eds[i // 7], fsb))
+        elif i % 3 == 0:
             reqs.append(("shard", zone[80 * i: 80 * i + 20 + 7 * i], fsb))
         elif i % 4 == 1:
             reqs.append(("rsa", S.RsaStackedRecord(*_rsa_rec(0x900 + i)), fsb))
@@ -457,6 +461,8 @@ def test_server_results_equal_standalone(S):
                     ids[j] = srv.submit_rsa(x, fsb)
                 elif kind == "p256":
                     ids[j] = srv.submit_p256(x)
+                elif kind == "ed25519":
+                    ids[j] = srv.submit_ed25519(*x, fsb, ROOT_ED)
                 else:
                     ids[j] = srv.submit_ds_ksk(x, hashlib.sha256(x).digest(), fsb)
         th = [threading.Thread(target=go, args=(range(t, len(reqs), 2),)) for t in range(2)]
@@ -476,10 +482,46 @@ def test_server_results_equal_standalone(S):
         elif kind == "p256":
             alone = S.prove_record_ecdsa_batch([x])[0]
             assert r.kind == 3 and r.n_trace == 256
+        elif kind == "ed25519":
+            alone = S.prove_zsk_ksk_binding_v2(*x, fsb, ROOT_ED)
+            assert r.kind == 4 and r.root_f0 == alone.root_f0
         else:
             alone = S.prove_ds_ksk_binding(x, hashlib.sha256(x).digest(), fsb)
             assert r.aux == alone.asserted_digest
         assert r.proof_blob == alone.proof_blob and r.pi_hash == alone.pi_hash
+
+
+ROOT_ED = bytes(range(100, 132))
+
+
+def test_server_bad_request_in_a_batch_fails_alone(S):
+    """With the submit-time signature checks off, a batch holding one Ed25519
+    request with a wrong signature fails; the server proves its requests
+    again one at a time, so only that request raises (with its own error)
+    and the others' proofs equal the single-call proofs."""
+    from stark_swarm_amd.server import ProvingServer
+    from stark_swarm_amd.zone import synthetic_ed25519_signatures
+    eds = synthetic_ed25519_signatures(5, 0x5EED000B)
+    bad = 3
+    pk, sig, msg = eds[bad]
+    eds[bad] = (pk, sig[:40] + bytes([sig[40] ^ 1]) + sig[41:], msg)
+    say("proving server: one bad Ed25519 request among 5, checks at submit off")
+    with ProvingServer(workers=1, hbm_budget=16 << 30, max_batch=8, validate=False) as srv:
+        ids = [srv.submit_ed25519(*e, bytes([i]) * 32, ROOT_ED) for i, e in enumerate(eds)]
+        out = []
+        for i in ids:
+            try:
+                out.append(srv.wait(i, timeout=300))
+            except S.StarkError as e:
+                out.append(e)
+    assert isinstance(out[bad], S.StarkError) and out[bad].code == -1
+    assert "verif" in str(out[bad]).lower()
+    for i, (e, r) in enumerate(zip(eds, out)):
+        if i == bad:
+            continue
+        assert not isinstance(r, Exception), r
+        assert r.proof_blob == S.prove_zsk_ksk_binding_v2(*e, bytes([i]) * 32, ROOT_ED).proof_blob
+    assert max(r.batch_size for i, r in enumerate(out) if i != bad) > 1, "the bad request must have shared a batch"
 
 
 # ---- RSA-2048 exponentiation AIR ----
