@@ -55,10 +55,23 @@ def splitmix_field(seed, n):
     return out[:n]
 
 
+CPU_SHARE = 16  # host cores one GPU's share of the box gives a job
+
+
+def cpu_threads():
+    """threads the CPU legs use: the host's cores, at most one GPU's share"""
+    return max(1, min(os.cpu_count() or 1, CPU_SHARE))
+
+
 def cpu_baseline(w_full, lg_full, blowup, schedule_full, s0):
-    """Time the C oracle (single thread) on a bounded sample of the pipeline and
-    scale to the full workload by rows x columns."""
+    """Time the C oracle on a bounded sample of the prover core in the
+    reference's own threading (one column at a time in the LDE -- ark-poly
+    without its parallel feature -- and the rayon loops on cpu_threads()
+    threads: Merkle leaves and levels at >= 256, FRI folds and coset
+    extraction at >= 2^12), and scale to the full workload by rows x columns
+    (which leaves out the LDE's extra log factor, so it favours the CPU)."""
     from tests import _oracle as O
+    T = cpu_threads()
     lg, w = 15, 128
     n = 1 << lg
     trace = splitmix_field(0x5EED0002, w * n).reshape(w, n)
@@ -67,18 +80,111 @@ def cpu_baseline(w_full, lg_full, blowup, schedule_full, s0):
     params = O.fri_params(sched, r=s0, s0=s0)
     rr = ctypes.create_string_buffer(32)
     pp, pl = O.u8p(), O.sz()
-    t = time.perf_counter()
-    rc = O.lib().ref_prover_core(O.ptr(trace), w, n, blowup, 7, params, rr, None, pp, pl)
-    dt = time.perf_counter() - t
+    with O.threads(T):
+        t = time.perf_counter()
+        rc = O.lib().ref_prover_core(O.ptr(trace), w, n, blowup, 7, params, rr, None, pp, pl)
+        dt = time.perf_counter() - t
     assert rc == 0
     O.take_bytes(pp, pl)
     scale = (w_full << lg_full) / (w << lg)
     full_s = dt * scale
-    return {"value": 1.0 / full_s, "unit": "proofs/s", "cores": 1, "kind": "port",
-            "sample": "oracle ref_prover_core (single thread) on 2^%d x %d, blowup %d, schedule %s, %d queries: "
-                      "%.2f s; scaled x%.0f by rows*columns to the 2^%d x %d workload" %
-                      (lg, w, blowup, sched, s0, dt, scale, lg_full, w_full),
-            "cpu": _cpu_model(), "small_proofs": cpu_small_proofs(O)}
+    with O.threads(T):
+        small = cpu_small_proofs(O)
+    return {"value": 1.0 / full_s, "unit": "proofs/s", "cores": T, "nproc": os.cpu_count(), "kind": "port",
+            "threading": "reference threading: LDE one column at a time (ark-poly 0.4.2 without 'parallel'); "
+                         "Merkle leaves / levels (>= 256, merkle lib.rs:345), FRI folds and coset extraction "
+                         "(>= 2^12, fri.rs:89) on %d threads" % T,
+            "sample": "oracle ref_prover_core on 2^%d x %d, blowup %d, schedule %s, %d queries: %.2f s on %d "
+                      "threads; scaled x%.0f by rows*columns to the 2^%d x %d workload" %
+                      (lg, w, blowup, sched, s0, dt, T, scale, lg_full, w_full),
+            "cpu": _cpu_model(), "small_proofs": small}
+
+
+def oracle_sha_proof(O, alg, msg, fsb, height):
+    """prove_ds_ksk_binding's pipeline for one C4 job with oracle pieces:
+    trace padded to `height` by repeating the last useful row -> LDE x32 ->
+    merge -> STIR proof with 54 queries bound to fsb"""
+    tr, nb, _ = (O.sha256_trace if alg == 256 else O.sha512_trace)(msg)
+    useful = 128 * nb
+    if height > tr.shape[1]:
+        tr = np.concatenate([tr, np.repeat(tr[:, useful - 1:useful], height - tr.shape[1], axis=1)], axis=1)
+    lde = O.lde_columns(tr, 32)
+    c = (O.merge_sha256 if alg == 256 else O.merge_sha512)(lde, height, 32, nb, 1)
+    del lde
+    lg = (32 * height).bit_length() - 1
+    sched = [8] * (lg // 3) + ([1 << (lg % 3)] if lg % 3 else [])
+    return O.deep_fri_prove(c, O.fri_params(sched, r=54, s0=54, pi_hash=fsb))
+
+
+def cpu_c4(jobs, budget_s=15.0):
+    """The C4 CPU leg: the oracle (reference threading, cpu_threads()) proves
+    the jobs in order of size, smallest first, until budget_s is spent; the
+    whole batch's time is the sampled time scaled by the LDE work
+    sum(columns x n0 x log2 n0) of all jobs over that of the sampled ones."""
+    from tests import _oracle as O
+    T = cpu_threads()
+
+    def work(a, h):
+        n0 = 32 * h
+        return (756 if a == 256 else 1510) * n0 * (n0.bit_length() - 1)
+    order = sorted(range(len(jobs)), key=lambda i: (jobs[i][2], jobs[i][0]))
+    done, spent, w_s = 0, 0.0, 0
+    with O.threads(T):
+        for i in order:
+            alg, msg, h, fsb = jobs[i]
+            t = time.perf_counter()
+            oracle_sha_proof(O, alg, msg, fsb, h)
+            spent += time.perf_counter() - t
+            w_s += work(alg, h)
+            done += 1
+            if spent >= budget_s:
+                break
+    w_all = sum(work(a, h) for a, _, h, _ in jobs)
+    est = spent * w_all / w_s
+    return {"proofs_per_s": round(len(jobs) / est, 4), "cores": T, "kind": "port",
+            "sample": "%d of the %d proofs (the smallest, 2^%d..2^%d rows) in %.1f s on %d threads; scaled by "
+                      "LDE work sum(columns x n0 x log2 n0) x%.1f" %
+                      (done, len(jobs), min(jobs[i][2] for i in order[:done]).bit_length() - 1,
+                       max(jobs[i][2] for i in order[:done]).bit_length() - 1, spent, T, w_all / w_s)}
+
+
+def cpu_ed25519(blowup, queries, cols=512, pts=16):
+    """Seconds the C oracle needs for one Ed25519 proof (k_scalar 256, a
+    2-block message) in the reference threading, from a bounded sample: the
+    trace fill in full, the LDE of `cols` of the 41 519 columns scaled to all
+    of them (the LDE runs one column at a time), the 70 606 constraints at
+    `pts` points scaled to every LDE point and divided over the threads (the
+    merge's into_par_iter), and one STIR proof over the LDE domain."""
+    from stark_swarm_amd.zone import synthetic_ed25519_signatures
+    from tests import _oracle as O
+    T = cpu_threads()
+    pk, sig, msg = synthetic_ed25519_signatures(1, 0x5EED0003)[0]
+    t = time.perf_counter()
+    e = O.Ed25519(pk, sig, msg, 256)
+    tr = e.trace()
+    fill = time.perf_counter() - t
+    h = tr.shape[1]
+    n0 = h * blowup
+    t = time.perf_counter()
+    O.lde_columns(tr[:cols], blowup)
+    lde_s = (time.perf_counter() - t) * tr.shape[0] / cols
+    cur, nxt = np.ascontiguousarray(tr[:, 0]), np.ascontiguousarray(tr[:, 1])
+    t = time.perf_counter()
+    for i in range(pts):
+        e.constraints(cur, nxt, i % (h - 1))
+    merge_s = (time.perf_counter() - t) * n0 / pts / T
+    lg = n0.bit_length() - 1
+    sched = [8] * (lg // 3) + ([1 << (lg % 3)] if lg % 3 else [])
+    c = splitmix_field(0x5EED000C, n0)
+    with O.threads(T):
+        t = time.perf_counter()
+        O.deep_fri_prove(c, O.fri_params(sched, r=queries, s0=queries, pi_hash=bytes(32)))
+        fri_s = time.perf_counter() - t
+    total = fill + lde_s + merge_s + fri_s
+    return {"proofs_per_s": round(1.0 / total, 5), "cores": T, "kind": "port",
+            "sample": "trace fill (%.2f s) + LDE of %d of %d columns x%.1f (%.1f s) + %d of %d constraint rows "
+                      "scaled to %d points over %d threads (%.1f s) + STIR over %d points (%.2f s)" %
+                      (fill, cols, tr.shape[0], tr.shape[0] / cols, lde_s, pts, n0, n0, T, merge_s, n0, fri_s)}
 
 
 def cpu_small_proofs(O):
@@ -102,11 +208,13 @@ def cpu_small_proofs(O):
     O.prove_inner_shard(b"swarm-test-zone1", leaves, bytes(32))
     out["inner_shard"] = round(1.0 / (time.perf_counter() - t), 3)
     out["p256"] = round(1.0 / cpu_p256_seconds(O), 5)
-    out["unit"] = ("proofs/s, one thread, one proof each (64-byte DS-KSK key, one RSA-2048 record, 1024-record "
-                   "shard; one ECDSA-P256 record estimated from a sample, see p256_sample)")
-    out["p256_sample"] = ("the full 256-row trace, the LDE of 2048 of the 151 334 columns (x73.9), the "
-                          "155 774 constraints at 64 of the 8192 points (x128) and one STIR proof over 8192 points, "
-                          "summed; the oracle's whole proof takes about 140 s on one core")
+    out["unit"] = ("proofs/s in the reference threading on %d threads, one proof each (64-byte DS-KSK key, one "
+                   "RSA-2048 record, 1024-record shard; one ECDSA-P256 record estimated from a sample, see "
+                   "p256_sample)" % O.lib().ref_get_threads())
+    out["p256_sample"] = ("the full 256-row trace, the LDE of 2048 of the 151 334 columns (x73.9, one column at a "
+                          "time), the 155 774 constraints at 64 of the 8192 points (x128, divided over the "
+                          "threads) and one STIR proof over 8192 points, summed; the oracle's whole proof takes "
+                          "about 140 s on one core")
     return out
 
 
@@ -130,7 +238,7 @@ def cpu_p256_seconds(O):
     t = time.perf_counter()
     for i in range(pts):
         O.p256_constraints(rows[0], rows[1], i % 255, 256)
-    merge_s = (time.perf_counter() - t) * 8192 / pts
+    merge_s = (time.perf_counter() - t) * 8192 / pts / O.lib().ref_get_threads()  # the merge's par_iter
     sched = [8, 8, 8, 8, 2]
     c = splitmix_field(0x5EED000C, 8192)
     t = time.perf_counter()
@@ -210,9 +318,8 @@ def main():
     ap.add_argument("--serve", default="100,200,300,400",
                     help="serving: Poisson arrival rates (requests/s), '' = skip")
     ap.add_argument("--serve-requests", type=int, default=300, help="serving: requests per rate")
-    ap.add_argument("--sha-batch", default="64,10,13",
-                    help="batched uneven SHA-256/512 proofs: count,log2 min rows,log2 max rows ('' = skip); "
-                         "the full workload is 256,10,17")
+    ap.add_argument("--sha-batch", default="256,10,17",
+                    help="batched uneven SHA-256/512 proofs: count,log2 min rows,log2 max rows ('' = skip)")
     ap.add_argument("--only", choices=["shards", "ds_ksk", "rsa", "p256", "sha_batch", "serve", "ed25519"],
                     help="run only this plain-vs-batched line (for profiling it) and print its JSON")
     ap.add_argument("--shards-only", action="store_true",
@@ -288,24 +395,37 @@ def main():
     stats = ctx.profile_stats()
     # the plans the warm-up tuned for this workload's transforms
     plans = {"%s 2^%d x%d, 2^%d cols" % (k[0], k[1], 1 << k[2], k[3]):
-             {"split": [best, k[1] - best] if best < k[1] else [best],
-              "ms": {str(s): round(v, 4) for s, v in sorted(ms.items())}}
+             {"split": [best[0], k[1] - best[0]] if best[0] < k[1] else [best[0]], "tile_log": best[1],
+              "ms": {"%d/tile%d" % p: round(v, 4) for p, v in sorted(ms.items())}}
              for k, (best, ms) in sorted(S.autotune_report().items())}
 
     ms_step = 1e3 * dt / args.steps
     # stage accounting
     gi = 1 << 30
-    lde_names = ["intt_passA", "intt_passB", "intt_small", "lde_passA", "lde_passB", "lde_small"]
-    lde_ms = sum(stats.get(k, {}).get("ms", 0.0) for k in lde_names) / args.steps
-    lde_alg = 8.0 * w * n + 8.0 * w * n0  # read the trace once, write the LDE once
     merkle_ms = sum(stats.get(k, {}).get("ms", 0.0) for k in ["merkle_leaves", "merkle_nodes"]) / args.steps
     host_stats = {k: v for k, v in stats.items() if k.startswith("host:")}
     stats = {k: v for k, v in stats.items() if not k.startswith("host:")}
-    dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
-    dname, d = dom
-    avg_ms = d["ms"] / d["launches"]
-    per_launch_bytes = d["bytes"] / d["launches"]
-    achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+    # The dominant stage is the coset LDE: pass A (coefficients -> the B
+    # pre-twiddled column DFTs, scratch) and pass B (row DFTs -> the LDE),
+    # one launch each per proof. Its algorithmic bytes are the coefficients
+    # read once and the LDE written once; the scratch round trip is traffic,
+    # not work.
+    def stage(names, alg_bytes):
+        ms = sum(stats.get(k, {}).get("ms", 0.0) for k in names) / args.steps
+        return {"kernels": names, "ms": round(ms, 3), "algorithmic_bytes": alg_bytes,
+                "GBps": round(alg_bytes / (ms * 1e-3) / 1e9, 1) if ms > 0 else None,
+                "frac": round(alg_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if ms > 0 else None}
+    stages = {"intt": stage(["intt_passA", "intt_passB", "intt_small"], 16.0 * w * n),
+              "lde": stage(["lde_passA", "lde_passB", "lde_small"], 8.0 * w * n + 8.0 * w * n0),
+              "merkle": stage(["merkle_leaves", "merkle_nodes"], 8.0 * w * n0 + 32.0 * n0)}
+    lde_launch_ms = stages["lde"]["ms"]
+    lde_alg = stages["lde"]["algorithmic_bytes"]
+    achieved = lde_alg / (lde_launch_ms * 1e-3) / 1e9 if lde_launch_ms > 0 else 0.0
+    full = (lg, w) == (20, 128)
+    traffic = None
+    if full:
+        ta, tb = profiled_traffic("lde_passA"), profiled_traffic("lde_passB")
+        traffic = ta + tb if ta is not None and tb is not None else None
     kernels = {k: {"ms_per_step": round(v["ms"] / args.steps, 3), "launches_per_step": v["launches"] / args.steps,
                    "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
                for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])}
@@ -326,23 +446,29 @@ def main():
                                "SHA3 row Merkle over 2^%d rows, STIR fold %s, %d queries" %
                                (lg, w, blowup, lg + 3, schedule, s0),
                    "rows": n, "cols": w, "blowup": blowup, "schedule": schedule, "queries": s0},
-        "lde_GBps": round(lde_alg / (lde_ms * 1e-3) / 1e9, 1) if lde_ms > 0 else None,
-        "lde_ms": round(lde_ms, 3),
+        "lde_GBps": stages["lde"]["GBps"],
+        "lde_ms": stages["lde"]["ms"],
         "merkle_ms": round(merkle_ms, 3),
         "proof_bytes": len(ref_blob),
-        "roofline": {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic(dname) if (lg, w) == (20, 128) else None,
-                     "traffic_source": "profiles/hbm_traffic__prover_core_2e20x128.md (FETCH_SIZE x2 + WRITE_SIZE x1, "
-                                       "calibrated on a 1 GiB copy)",
-                     "bytes_per_launch": per_launch_bytes, "avg_launch_ms": round(avg_ms, 4),
-                     "valu_issue_load": profiled_valu_load(dname) if (lg, w) == (20, 128) else None,
-                     "note": "integer-VALU bound (64-bit modular products on 32-bit lanes): the launch issues "
-                             "VALU instructions in valu_issue_load of the SIMD cycles, well before HBM limits"},
+        "roofline": {"bound": "valu", "kernel": "coset LDE stage: lde_passA + lde_passB (ntt_tile, one launch "
+                                                "each per proof)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": "profiles/hbm_traffic__prover_core_2e20x128.md: FETCH_SIZE x2 + WRITE_SIZE "
+                                       "x1 (calibrated on a 1 GiB copy) of lde_passA + lde_passB",
+                     "algorithmic_bytes_per_launch": lde_alg, "avg_launch_ms": round(lde_launch_ms, 4),
+                     "valu_issue_load": {k: profiled_valu_load(k) for k in ("lde_passA", "lde_passB")} if full
+                     else None,
+                     "note": "algorithmic bytes = the 2^20 x 128 coefficients read once + the 2^23 x 128 LDE written "
+                             "once, over the two passes' time; bound from the counters: both passes keep the SIMDs' "
+                             "integer VALU busy (64-bit modular products on 32-bit lanes), pass A at valu_issue_load "
+                             "~0.85, while their HBM traffic (the scratch round trip included) stays far below peak"},
+        "stages": stages,
         "kernels": kernels,
         "autotune": {"enabled": bool(args.autotune), "plans": plans,
                      "note": "split = log2 sizes of the two passes (one entry: one pass with the cosets as "
-                             "repetitions); ms = one timed run per candidate split in the warm-up"},
+                             "repetitions), tile_log = log2 of the LDS tile (13: 512 lanes, 14: 1024 lanes); "
+                             "ms = one timed run per candidate (split/tile) in the warm-up"},
     }
     if args.streams > 1:
         bres = batched(S, args, w, n, blowup, params, ref_blob, ms_step)
@@ -378,8 +504,16 @@ def main():
         log("serving: Poisson arrivals")
         res.update(serve(S, [int(r) for r in args.serve.split(",")], args.serve_requests))
     if not args.no_cpu:
-        log("cpu baseline: timing the oracle on a bounded sample")
+        log("cpu baseline: timing the oracle on a bounded sample (%d threads)" % cpu_threads())
         res["cpu_baseline"] = cpu_baseline(w, lg, blowup, schedule, s0)
+        if args.ed25519:
+            log("cpu: ed25519 proof estimate")
+            res["ed25519"]["cpu"] = cpu_ed25519(8, 90)
+            res["ed25519_reference_params"]["cpu"] = cpu_ed25519(32, 54)
+        if args.sha_batch:
+            log("cpu: sha batch sample")
+            a_, m_, h_, f_ = c4_jobs(*(int(x) for x in args.sha_batch.split(",")))
+            res["sha_batch"]["cpu"] = cpu_c4(list(zip(a_, m_, h_, f_)))
     print(json.dumps(res))
 
 
@@ -581,14 +715,10 @@ def ed25519_batch(S, nproofs, blowup, queries):
             "reference_lde_bytes_per_proof": 8.0 * 41519 * 1024 * blowup, "bit_exact_vs_plain": exact}
 
 
-def c4_batch(S, nproofs, log_lo, log_hi):
-    """Batched uneven SHA-256/512 proofs: nproofs alternating SHA-256 and
-    SHA-512 AIR proofs (the prove_ds_ksk_binding pipeline: LDE x32, merge,
-    STIR with 54 queries), messages of seeded uniform 64-4096 bytes, trace
-    heights log-uniform in [2^log_lo, 2^log_hi] raised to each message's
-    minimum (padding rows repeat the last useful row). One proof per call
-    (plain) vs one stark_prove_sha_batch call (lockstep per alg and height,
-    HBM-sized waves); every batched proof must equal its single-call proof."""
+def c4_jobs(nproofs, log_lo, log_hi):
+    """The C4 jobs: alternating SHA-256 / SHA-512, seeded 64-4096 byte
+    messages, heights 2^e for e uniform in [log_lo, log_hi] raised to each
+    message's minimum, seeded 32-byte bindings."""
     rng = np.random.default_rng(0x5EED0004)
     algs, msgs, hs = [], [], []
     for i in range(nproofs):
@@ -601,6 +731,18 @@ def c4_batch(S, nproofs, log_lo, log_hi):
         msgs.append(m)
         hs.append(h)
     fsb = [rng.bytes(32) for _ in range(nproofs)]
+    return algs, msgs, hs, fsb
+
+
+def c4_batch(S, nproofs, log_lo, log_hi):
+    """Batched uneven SHA-256/512 proofs: nproofs alternating SHA-256 and
+    SHA-512 AIR proofs (the prove_ds_ksk_binding pipeline: LDE x32, merge,
+    STIR with 54 queries), messages of seeded uniform 64-4096 bytes, trace
+    heights log-uniform in [2^log_lo, 2^log_hi] raised to each message's
+    minimum (padding rows repeat the last useful row). One proof per call
+    (plain) vs one stark_prove_sha_batch call (lockstep per alg and height,
+    HBM-sized waves); every batched proof must equal its single-call proof."""
+    algs, msgs, hs, fsb = c4_jobs(nproofs, log_lo, log_hi)
     lde_bytes = sum(8 * (756 if a == 256 else 1510) * 32 * h for a, h in zip(algs, hs))
     S.prove_sha_batch([256, 512], [b"w", b"w"], fsb[:2])  # warm
     # one untimed single proof per (alg, height): the first call of a shape

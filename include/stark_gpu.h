@@ -486,8 +486,14 @@ int stark_set_autotune(stark_ctx *ctx, int on);
 /* Pin the first pass of every two-pass transform that admits it to
  * 2^log_first_pass points (3..12; 0 = no pin); used to check each split. */
 int stark_set_ntt_split(stark_ctx *ctx, unsigned log_first_pass);
-/* The tuned plans, one line per shape: "<ntt|intt|lde> L LB log_cols -> L1 |
- * L1:ms ...". Returns the full length; writes at most cap - 1 bytes + NUL. */
+/* Pin the LDS tile of every tunable transform that admits it: 13 (2^13
+ * elements, 512 lanes), 14 (2^14 elements, 1024 lanes; passes of 2^8 points
+ * and more), 0 = no pin. A plan is a (split, tile) pair; every plan computes
+ * the same values. */
+int stark_set_ntt_tile(stark_ctx *ctx, unsigned tile_log);
+/* The tuned plans, one line per shape: "<ntt|intt|lde> L LB log_cols ->
+ * L1.tile | L1.tile:ms ...". Returns the full length; writes at most cap - 1
+ * bytes + NUL. */
 size_t stark_autotune_report(char *buf, size_t cap);
 
 /* ---- host-side helpers (no device work; used by CPU tests of the host logic) ---- */

@@ -1453,25 +1453,40 @@ void ref_ed25519_constraints(const ref_ed_ctx *C, const uint64_t *cur, const uin
 
 /* deep_ali_merge_ed25519_verify_streaming (lib.rs:854-974): every LDE point, nxt = (i + blowup) mod n,
  * trace_row = i / blowup, no phi quirk; coeffs NULL = lambda_j = j + 1. lde: width x (n_trace * blowup). */
-int ref_merge_ed25519(const ref_ed_ctx *C, const uint64_t *lde, size_t blowup, const uint64_t *coeffs,
-                      uint64_t *c_eval) {
-    const size_t h = C->height, n = h * blowup, w = C->width, nk = C->n_cons;
+typedef struct {
+    const ref_ed_ctx *C;
+    const uint64_t *lde, *coeffs;
+    size_t blowup;
+    uint64_t *c_eval;
+} ed_merge_job;
+static void ed_merge_range(void *ctx, size_t lo, size_t hi) {
+    const ed_merge_job *J = (const ed_merge_job *)ctx;
+    const ref_ed_ctx *C = J->C;
+    const size_t n = C->height * J->blowup, w = C->width, nk = C->n_cons;
     uint64_t *cur = (uint64_t *)malloc(8 * w), *nxt = (uint64_t *)malloc(8 * w), *cv = (uint64_t *)malloc(8 * nk);
-    for (size_t i = 0; i < n; i++) {
-        const size_t ni = (i + blowup) % n;
+    for (size_t i = lo; i < hi; i++) {
+        const size_t ni = (i + J->blowup) % n;
         for (size_t c = 0; c < w; c++) {
-            cur[c] = lde[c * n + i];
-            nxt[c] = lde[c * n + ni];
+            cur[c] = J->lde[c * n + i];
+            nxt[c] = J->lde[c * n + ni];
         }
-        ref_ed25519_constraints(C, cur, nxt, i / blowup, cv);
+        ref_ed25519_constraints(C, cur, nxt, i / J->blowup, cv);
         uint64_t acc = 0;
         for (size_t j = 0; j < nk; j++)
-            if (cv[j]) acc = gl_add(acc, gl_mul(coeffs ? coeffs[j] : (uint64_t)(j + 1), cv[j]));
-        c_eval[i] = acc;
+            if (cv[j]) acc = gl_add(acc, gl_mul(J->coeffs ? J->coeffs[j] : (uint64_t)(j + 1), cv[j]));
+        J->c_eval[i] = acc;
     }
     free(cur);
     free(nxt);
     free(cv);
+}
+
+int ref_merge_ed25519(const ref_ed_ctx *C, const uint64_t *lde, size_t blowup, const uint64_t *coeffs,
+                      uint64_t *c_eval) {
+    const size_t h = C->height, n = h * blowup;
+    ed_init(); /* the shared constants are set before any thread reads them */
+    ed_merge_job J = {C, lde, coeffs, blowup, c_eval};
+    ref_par_for(n, PAR_FRI, ed_merge_range, &J); /* into_par_iter at >= 2^12 points (lib.rs:884-920) */
     ref_quotient_from_phi(c_eval, n, h);
     return 0;
 }

@@ -127,6 +127,58 @@ static void mt_free(mtree *t) {
     for (size_t k = 0; k < t->nlev; k++) free(t->lv[k]);
     free(t->lv); free(t->len);
 }
+typedef struct {
+    mtree *t;
+    size_t level, a;
+} node_job;
+static void node_range(void *ctx, size_t lo, size_t hi) {
+    node_job *j = (node_job *)ctx;
+    for (size_t i = lo; i < hi; i++) {
+        sha3_ctx c;
+        uint8_t ds[32];
+        dslabel(ds, j->a, j->level + 1, i, j->t->label);
+        sha3_init(&c);
+        sha3_update(&c, ds, 32);
+        sha3_update(&c, j->t->lv[j->level] + 32 * j->a * i, 32 * j->a);
+        sha3_final(&c, j->t->lv[j->level + 1] + 32 * i);
+    }
+}
+
+/* leaf i of a tree = leaf_hash of vals + i * stride (nv values); leaves
+ * hashed in parallel at >= 256 (merkle lib.rs:397-409) */
+typedef struct {
+    mtree *t;
+    const uint64_t *vals;
+    size_t stride, nv;
+} leaf_job;
+static void leaf_range(void *ctx, size_t lo, size_t hi) {
+    leaf_job *j = (leaf_job *)ctx;
+    for (size_t i = lo; i < hi; i++) leaf_hash(j->t->arity, j->t->label, i, j->vals + i * j->stride, j->nv, j->t->lv[0] + 32 * i);
+}
+static void hash_leaves(mtree *t, size_t n, const uint64_t *vals, size_t stride, size_t nv) {
+    leaf_job j = {t, vals, stride, nv};
+    ref_par_for(n, PAR_MERKLE, leaf_range, &j);
+}
+/* f0 coset-packed leaves: leaf b = (f0[b + j nn])_{j < m} */
+typedef struct {
+    mtree *t;
+    const uint64_t *f0;
+    size_t nn, m;
+} packed_job;
+static void packed_range(void *ctx, size_t lo, size_t hi) {
+    packed_job *j = (packed_job *)ctx;
+    uint64_t *fib = (uint64_t *)malloc(8 * j->m);
+    for (size_t b = lo; b < hi; b++) {
+        for (size_t k = 0; k < j->m; k++) fib[k] = j->f0[b + k * j->nn];
+        leaf_hash(j->t->arity, j->t->label, b, fib, j->m, j->t->lv[0] + 32 * b);
+    }
+    free(fib);
+}
+static void hash_packed_leaves(mtree *t, const uint64_t *f0, size_t nn, size_t m) {
+    packed_job j = {t, f0, nn, m};
+    ref_par_for(nn, PAR_MERKLE, packed_range, &j);
+}
+
 /* finalize: levels padded by repeating the last node (merkle lib.rs:423-476) */
 static void mt_finalize(mtree *t, uint8_t root[32]) {
     size_t level = 0;
@@ -141,15 +193,8 @@ static void mt_finalize(mtree *t, uint8_t root[32]) {
         }
         size_t np = cur / a;
         t->lv[level + 1] = (uint8_t *)malloc(32 * (np + a));
-        for (size_t i = 0; i < np; i++) {
-            sha3_ctx c;
-            uint8_t ds[32];
-            dslabel(ds, a, level + 1, i, t->label);
-            sha3_init(&c);
-            sha3_update(&c, ds, 32);
-            sha3_update(&c, t->lv[level] + 32 * a * i, 32 * a);
-            sha3_final(&c, t->lv[level + 1] + 32 * i);
-        }
+        node_job nj = {t, level, a};
+        ref_par_for(np, PAR_MERKLE, node_range, &nj); /* par_chunks at >= 256 parents (lib.rs:443-449) */
         t->len[level + 1] = np;
         t->nlev = level + 2;
         level++;
@@ -191,7 +236,7 @@ static size_t pick_arity2(size_t n, size_t m) { size_t a = pick_arity(n, m); ret
 void ref_merkle_root(const uint64_t *leaf_vals, size_t n_leaves, size_t vpl, size_t arity, size_t depth, uint64_t label, uint8_t root[32]) {
     mtree t;
     mt_init(&t, arity, depth, label, n_leaves);
-    for (size_t i = 0; i < n_leaves; i++) leaf_hash(arity, label, i, leaf_vals + i * vpl, vpl, t.lv[0] + 32 * i);
+    hash_leaves(&t, n_leaves, leaf_vals, vpl, vpl);
     mt_finalize(&t, root);
     mt_free(&t);
 }
@@ -220,34 +265,59 @@ static void ood_coset_evals(const fp6 *f, size_t n, fp6 z, size_t m, fp6 *out) {
 
 /* per coset b: monomial coefficients of the degree-(m-1) interpolant of
  * f[b + j n/m] on the points omega^b zeta^j  (fri.rs:670-752) */
+typedef struct {
+    const fp6 *f;
+    size_t nn, m;
+    uint64_t minv, omega_inv;
+    const uint64_t *zi;
+    fp6 *tuples;
+} cc_job;
+static void cc_range(void *ctx, size_t lo, size_t hi) {
+    cc_job *J = (cc_job *)ctx;
+    const size_t m = J->m, nn = J->nn;
+    uint64_t obinv = gl_pow(J->omega_inv, lo); /* omega^{-b} */
+    for (size_t b = lo; b < hi; b++) {
+        uint64_t sh = J->minv; /* (1/m) * omega^{-bk} */
+        for (size_t k = 0; k < m; k++) {
+            fp6 acc = fp6_zero();
+            for (size_t j = 0; j < m; j++) acc = fp6_add(acc, fp6_scale(J->f[b + j * nn], J->zi[(j * k) % m]));
+            J->tuples[b * m + k] = fp6_scale(acc, sh);
+            sh = gl_mul(sh, obinv);
+        }
+        obinv = gl_mul(obinv, J->omega_inv);
+    }
+}
 static void coset_coeffs(const fp6 *f, size_t n, size_t m, fp6 *tuples /* (n/m) x m */) {
     size_t nn = n / m;
     uint64_t omega = gl_root_of_unity(n);
     uint64_t zeta_inv = gl_inv(gl_pow(omega, nn));
-    uint64_t minv = gl_inv(m), omega_inv = gl_inv(omega);
     uint64_t *zi = (uint64_t *)malloc(sizeof(uint64_t) * m);
     zi[0] = 1;
     for (size_t k = 1; k < m; k++) zi[k] = gl_mul(zi[k - 1], zeta_inv);
-    uint64_t obinv = 1; /* omega^{-b} */
-    for (size_t b = 0; b < nn; b++) {
-        uint64_t sh = minv; /* (1/m) * omega^{-bk} */
-        for (size_t k = 0; k < m; k++) {
-            fp6 acc = fp6_zero();
-            for (size_t j = 0; j < m; j++) acc = fp6_add(acc, fp6_scale(f[b + j * nn], zi[(j * k) % m]));
-            tuples[b * m + k] = fp6_scale(acc, sh);
-            sh = gl_mul(sh, obinv);
-        }
-        obinv = gl_mul(obinv, omega_inv);
-    }
+    cc_job J = {f, nn, m, gl_inv(m), gl_inv(omega), zi, tuples};
+    ref_par_for(nn, PAR_FRI, cc_range, &J); /* into_par_iter over cosets at >= 2^12 (fri.rs:700-703) */
     free(zi);
 }
 
-static void fold_tuples(const fp6 *tuples, size_t nn, size_t m, fp6 alpha, fp6 *out) {
-    for (size_t b = 0; b < nn; b++) {
+typedef struct {
+    const fp6 *tuples;
+    size_t m;
+    fp6 alpha, *out;
+} fold_job;
+static void fold_range(void *ctx, size_t lo, size_t hi) {
+    fold_job *J = (fold_job *)ctx;
+    for (size_t b = lo; b < hi; b++) {
         fp6 acc = fp6_zero(), ap = fp6_one();
-        for (size_t k = 0; k < m; k++) { acc = fp6_add(acc, fp6_mul(tuples[b * m + k], ap)); ap = fp6_mul(ap, alpha); }
-        out[b] = acc;
+        for (size_t k = 0; k < J->m; k++) {
+            acc = fp6_add(acc, fp6_mul(J->tuples[b * J->m + k], ap));
+            ap = fp6_mul(ap, J->alpha);
+        }
+        J->out[b] = acc;
     }
+}
+static void fold_tuples(const fp6 *tuples, size_t nn, size_t m, fp6 alpha, fp6 *out) {
+    fold_job J = {tuples, m, alpha, out};
+    ref_par_for(nn, PAR_FRI, fold_range, &J); /* par_iter_mut at >= 2^12 outputs (fri.rs:609-620) */
 }
 
 static void bind_statement(tscript *t, const ref_fri_params *p, size_t n0) {
@@ -318,13 +388,9 @@ static int fri_build(const uint64_t *f0, size_t n0, const ref_fri_params *p, fri
         size_t a = pick_arity2(nn0, 16), d = merkle_depth(nn0, a);
         mtree t;
         mt_init(&t, a, d, 0xFF, nn0);
-        uint64_t *fib = (uint64_t *)malloc(8 * m0);
-        for (size_t b = 0; b < nn0; b++) {
-            for (size_t j = 0; j < m0; j++) fib[j] = f0[b + j * nn0];
-            leaf_hash(a, 0xFF, b, fib, m0, t.lv[0] + 32 * b);
-        }
+        hash_packed_leaves(&t, f0, nn0, m0);
         mt_finalize(&t, s->root_f0);
-        mt_free(&t); free(fib);
+        mt_free(&t);
     }
     tr_absorb(&tr, s->root_f0, 32);
     fp6 z = challenge_ext(&tr, "z_fp3");
@@ -351,7 +417,7 @@ static int fri_build(const uint64_t *f0, size_t n0, const ref_fri_params *p, fri
             size_t a = pick_arity2(cur, m), d = merkle_depth(cur, a);
             mtree t;
             mt_init(&t, a, d, ell, cur);
-            for (size_t i = 0; i < cur; i++) leaf_hash(a, ell, i, s->f[ell][i].c, 6, t.lv[0] + 32 * i);
+            hash_leaves(&t, cur, s->f[ell][0].c, 6, 6);
             mt_finalize(&t, s->roots[ell]);
             mt_free(&t);
         }
@@ -362,7 +428,7 @@ static int fri_build(const uint64_t *f0, size_t n0, const ref_fri_params *p, fri
             size_t a = pick_arity2(nn, 16), d = merkle_depth(nn, a);
             mtree t;
             mt_init(&t, a, d, 0xFE, nn);
-            for (size_t b = 0; b < nn; b++) leaf_hash(a, 0xFE, b, (const uint64_t *)(tuples + b * m), 6 * m, t.lv[0] + 32 * b);
+            hash_leaves(&t, nn, (const uint64_t *)tuples, 6 * m, 6 * m);
             mt_finalize(&t, s->coeff_root);
             mt_free(&t);
             s->has_coeff = 1;
@@ -462,19 +528,14 @@ int ref_deep_fri_prove_raw(const uint64_t *f0, size_t n0, const ref_fri_params *
         size_t a = pick_arity2(nn0, 16), d = merkle_depth(nn0, a);
         uint8_t r[32];
         mt_init(&t0, a, d, 0xFF, nn0);
-        uint64_t *fib = (uint64_t *)malloc(8 * m0);
-        for (size_t bb = 0; bb < nn0; bb++) {
-            for (size_t j = 0; j < m0; j++) fib[j] = f0[bb + j * nn0];
-            leaf_hash(a, 0xFF, bb, fib, m0, t0.lv[0] + 32 * bb);
-        }
+        hash_packed_leaves(&t0, f0, nn0, m0);
         mt_finalize(&t0, r);
-        free(fib);
     }
     if (L >= 2) {
         size_t n1 = s.sz[1], a = pick_arity2(n1, p->schedule[1]), d = merkle_depth(n1, a);
         uint8_t r[32];
         mt_init(&t1, a, d, 1, n1);
-        for (size_t i = 0; i < n1; i++) leaf_hash(a, 1, i, s.f[1][i].c, 6, t1.lv[0] + 32 * i);
+        hash_leaves(&t1, n1, s.f[1][0].c, 6, 6);
         mt_finalize(&t1, r);
     }
     bb_u8(&b, 1);
@@ -517,6 +578,27 @@ int ref_deep_fri_prove(const uint64_t *f0, size_t n0, const ref_fri_params *p, u
 
 void ref_free(void *p) { free(p); }
 
+typedef struct {
+    mtree *t;
+    const uint64_t *lde;
+    uint64_t *f0;
+    size_t w, n0;
+} row_job;
+static void row_range(void *ctx, size_t lo, size_t hi) {
+    row_job *J = (row_job *)ctx;
+    uint64_t *row = (uint64_t *)malloc(8 * J->w);
+    for (size_t i = lo; i < hi; i++) {
+        uint64_t acc = 0;
+        for (size_t c = 0; c < J->w; c++) {
+            row[c] = J->lde[c * J->n0 + i];
+            acc = gl_add(acc, gl_mul(row[c], c + 1));
+        }
+        J->f0[i] = acc;
+        leaf_hash(2, 0, i, row, J->w, J->t->lv[0] + 32 * i);
+    }
+    free(row);
+}
+
 /* Prover-core microbench: coset LDE -> row tree (DsLabel arity 2, label 0,
  * leaf = the w values of one LDE row) -> f0 = sum_c (c+1) LDE_c ->
  * deep_fri_prove(f0) bound to the row root -> one row opening per STIR query
@@ -534,17 +616,9 @@ int ref_prover_core(const uint64_t *trace, size_t w, size_t n_trace, size_t blow
     if (depth == 0) depth = 1;
     mtree t;
     mt_init(&t, 2, depth, 0, n0);
-    uint64_t *row = (uint64_t *)malloc(8 * w);
     uint64_t *f0 = f0_out ? f0_out : (uint64_t *)malloc(8 * n0);
-    for (size_t i = 0; i < n0; i++) {
-        uint64_t acc = 0;
-        for (size_t c = 0; c < w; c++) {
-            row[c] = lde[c * n0 + i];
-            acc = gl_add(acc, gl_mul(row[c], c + 1));
-        }
-        f0[i] = acc;
-        leaf_hash(2, 0, i, row, w, t.lv[0] + 32 * i);
-    }
+    row_job rj = {&t, lde, f0, w, n0};
+    ref_par_for(n0, PAR_MERKLE, row_range, &rj); /* row leaves: push_leaves_parallel (lib.rs:390-409) */
     mt_finalize(&t, row_root);
     ref_fri_params q = *p;
     q.public_inputs_hash = row_root;
@@ -564,7 +638,7 @@ int ref_prover_core(const uint64_t *trace, size_t w, size_t n_trace, size_t blow
         *proof = b.p;
         *proof_len = b.len;
     }
-    free(pb); free(raw); free(row); free(lde);
+    free(pb); free(raw); free(lde);
     if (!f0_out) free(f0);
     mt_free(&t);
     return rc;

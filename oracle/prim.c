@@ -12,7 +12,44 @@
  * behind rand 0.8.5 StdRng, used at crates/deep_ali/src/fri.rs:1044-1051.
  */
 #include "refcpu_int.h"
+#include <pthread.h>
 #include <stdlib.h>
+
+/* Threads for the loops the reference runs with rayon (merkle lib.rs:345
+ * PARALLEL_THRESHOLD = 256 leaves / parents; fri.rs:89 PARALLEL_MIN_ELEMS =
+ * 2^12 fold outputs). 1 (the default) runs everything on the caller. */
+static int g_threads = 1;
+void ref_set_threads(int n) { g_threads = n < 1 ? 1 : n > 256 ? 256 : n; }
+int ref_get_threads(void) { return g_threads; }
+
+typedef struct {
+    void (*fn)(void *, size_t, size_t);
+    void *ctx;
+    size_t lo, hi;
+} par_job;
+static void *par_run(void *a) {
+    par_job *j = (par_job *)a;
+    j->fn(j->ctx, j->lo, j->hi);
+    return NULL;
+}
+void ref_par_for(size_t n, size_t threshold, void (*fn)(void *, size_t, size_t), void *ctx) {
+    int T = g_threads;
+    if (T <= 1 || n < threshold || n < 2) {
+        fn(ctx, 0, n);
+        return;
+    }
+    if ((size_t)T > n) T = (int)n;
+    pthread_t th[256];
+    par_job jobs[256];
+    for (int t = 0; t < T; t++) jobs[t] = (par_job){fn, ctx, n * (size_t)t / (size_t)T, n * (size_t)(t + 1) / (size_t)T};
+    int started[256] = {0};
+    for (int t = 1; t < T; t++) started[t] = pthread_create(&th[t], NULL, par_run, &jobs[t]) == 0;
+    par_run(&jobs[0]);
+    for (int t = 1; t < T; t++) {
+        if (started[t]) pthread_join(th[t], NULL);
+        else par_run(&jobs[t]); /* no thread: run its share here */
+    }
+}
 
 uint64_t gl_pow(uint64_t a, uint64_t e) {
     uint64_t r = 1;

@@ -24,6 +24,10 @@ extern "C" {
 uint64_t ref_gl_mul(uint64_t a, uint64_t b);
 uint64_t ref_gl_inv(uint64_t a);
 uint64_t ref_gl_root_of_unity(size_t n);
+/* threads for the loops the reference runs with rayon (1 = serial; the
+ * results are the same bytes for any count) */
+void ref_set_threads(int n);
+int ref_get_threads(void);
 void ref_fp6_mul(const uint64_t *a, const uint64_t *b, uint64_t *out);
 void ref_fp6_inv(const uint64_t *a, uint64_t *out);
 void ref_ntt(uint64_t *a, size_t n, int inverse);

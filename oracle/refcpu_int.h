@@ -48,6 +48,11 @@ static inline uint64_t gl_sub(uint64_t a, uint64_t b) {
 static inline uint64_t gl_neg(uint64_t a) { return a ? GL_P - a : 0; }
 static inline uint64_t gl_mul(uint64_t a, uint64_t b) { return gl_red128((u128)a * b); }
 static inline uint64_t gl_from_u64(uint64_t x) { return x >= GL_P ? x - GL_P : x; }
+/* fn(ctx, lo, hi) over [0, n) split across the threads of ref_set_threads
+ * when n >= threshold, else in one call on the caller */
+void ref_par_for(size_t n, size_t threshold, void (*fn)(void *, size_t, size_t), void *ctx);
+#define PAR_MERKLE 256  /* merkle lib.rs:345 */
+#define PAR_FRI 4096    /* fri.rs:89 */
 uint64_t gl_pow(uint64_t a, uint64_t e);
 uint64_t gl_inv(uint64_t a);
 /* omega_n = 7^((p-1)/n), n a power of two <= 2^32 (ark-poly group_gen). */

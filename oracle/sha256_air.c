@@ -318,26 +318,38 @@ void ref_sha256_constraints(const uint64_t *cur, const uint64_t *nxt, size_t row
 /* deep_ali_merge_sha256 (lib.rs:453-545): phi(i) = sum_j (j+1) C_j(i) over
  * the LDE (cur = point i, nxt = point i + blowup, row = i / blowup), then
  * the Z_H quotient (remainder dropped) evaluated back on the domain. */
+typedef struct {
+    const uint64_t *lde;
+    size_t n, blowup, n_blocks;
+    uint64_t *c_eval;
+} sha256_merge_job;
+static void sha256_merge_range(void *ctx, size_t lo, size_t hi) {
+    const sha256_merge_job *J = (const sha256_merge_job *)ctx;
+    const size_t n = J->n;
+    uint64_t *cur = (uint64_t *)malloc(8 * SHA_W), *nxt = (uint64_t *)malloc(8 * SHA_W);
+    uint64_t cv[SHA_K];
+    for (size_t i = lo; i < hi; i++) {
+        const size_t ni = (i + J->blowup) % n;
+        for (size_t c = 0; c < SHA_W; c++) {
+            cur[c] = J->lde[c * n + i];
+            nxt[c] = J->lde[c * n + ni];
+        }
+        ref_sha256_constraints(cur, nxt, i / J->blowup, J->n_blocks, cv);
+        uint64_t acc = 0;
+        for (size_t k = 0; k < SHA_K; k++) acc = gl_add(acc, gl_mul(k + 1, cv[k]));
+        J->c_eval[i] = acc;
+    }
+    free(cur);
+    free(nxt);
+}
+
 int ref_merge_sha256(const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_blocks, int phi_mode,
                      uint64_t *c_eval) {
     const size_t n = n_trace * blowup;
     const int compute = phi_mode == 0 || (phi_mode == 1 && n >= 4096);
     if (compute) {
-        uint64_t *cur = (uint64_t *)malloc(8 * SHA_W), *nxt = (uint64_t *)malloc(8 * SHA_W);
-        uint64_t cv[SHA_K];
-        for (size_t i = 0; i < n; i++) {
-            const size_t ni = (i + blowup) % n;
-            for (size_t c = 0; c < SHA_W; c++) {
-                cur[c] = lde[c * n + i];
-                nxt[c] = lde[c * n + ni];
-            }
-            ref_sha256_constraints(cur, nxt, i / blowup, n_blocks, cv);
-            uint64_t acc = 0;
-            for (size_t k = 0; k < SHA_K; k++) acc = gl_add(acc, gl_mul(k + 1, cv[k]));
-            c_eval[i] = acc;
-        }
-        free(cur);
-        free(nxt);
+        sha256_merge_job J = {lde, n, blowup, n_blocks, c_eval};
+        ref_par_for(n, PAR_FRI, sha256_merge_range, &J); /* par_iter_mut at >= 2^12 points (lib.rs:493-497) */
     } else {
         memset(c_eval, 0, 8 * n);
     }

@@ -352,26 +352,38 @@ void ref_sha512_constraints(const uint64_t *cur, const uint64_t *nxt, size_t row
 #undef PUT
 }
 
+typedef struct {
+    const uint64_t *lde;
+    size_t n, blowup, n_blocks;
+    uint64_t *c_eval;
+} sha512_merge_job;
+static void sha512_merge_range(void *ctx, size_t lo, size_t hi) {
+    const sha512_merge_job *J = (const sha512_merge_job *)ctx;
+    const size_t n = J->n;
+    uint64_t *cur = (uint64_t *)malloc(8 * W512), *nxt = (uint64_t *)malloc(8 * W512);
+    uint64_t cv[K512];
+    for (size_t i = lo; i < hi; i++) {
+        const size_t ni = (i + J->blowup) % n;
+        for (size_t c = 0; c < W512; c++) {
+            cur[c] = J->lde[c * n + i];
+            nxt[c] = J->lde[c * n + ni];
+        }
+        ref_sha512_constraints(cur, nxt, i / J->blowup, J->n_blocks, cv);
+        uint64_t acc = 0;
+        for (size_t k = 0; k < K512; k++) acc = gl_add(acc, gl_mul(k + 1, cv[k]));
+        J->c_eval[i] = acc;
+    }
+    free(cur);
+    free(nxt);
+}
+
 int ref_merge_sha512(const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_blocks, int phi_mode,
                      uint64_t *c_eval) {
     const size_t n = n_trace * blowup;
     const int compute = phi_mode == 0 || (phi_mode == 1 && n >= 4096);
     if (compute) {
-        uint64_t *cur = (uint64_t *)malloc(8 * W512), *nxt = (uint64_t *)malloc(8 * W512);
-        uint64_t cv[K512];
-        for (size_t i = 0; i < n; i++) {
-            const size_t ni = (i + blowup) % n;
-            for (size_t c = 0; c < W512; c++) {
-                cur[c] = lde[c * n + i];
-                nxt[c] = lde[c * n + ni];
-            }
-            ref_sha512_constraints(cur, nxt, i / blowup, n_blocks, cv);
-            uint64_t acc = 0;
-            for (size_t k = 0; k < K512; k++) acc = gl_add(acc, gl_mul(k + 1, cv[k]));
-            c_eval[i] = acc;
-        }
-        free(cur);
-        free(nxt);
+        sha512_merge_job J = {lde, n, blowup, n_blocks, c_eval};
+        ref_par_for(n, PAR_FRI, sha512_merge_range, &J); /* par_iter_mut at >= 2^12 points (lib.rs:601-605) */
     } else {
         memset(c_eval, 0, 8 * n);
     }

@@ -135,6 +135,7 @@ def lib():
         "stark_profile_names": (sz, [vp, ctypes.c_char_p, sz]),
         "stark_set_autotune": (ctypes.c_int, [vp, ctypes.c_int]),
         "stark_set_ntt_split": (ctypes.c_int, [vp, ctypes.c_uint]),
+        "stark_set_ntt_tile": (ctypes.c_int, [vp, ctypes.c_uint]),
         "stark_autotune_report": (sz, [ctypes.c_char_p, sz]),
         "stark_host_sha3_256": (None, [ctypes.c_char_p, sz, ctypes.c_char_p]),
         "stark_host_stdrng_first_u64": (ctypes.c_uint64, [ctypes.c_uint64]),
@@ -285,14 +286,21 @@ class Context:
 
     # launch-plan autotuner (the split of a transform's two passes)
     def autotune(self, on=True):
-        """Time the admissible pass splits on the first NTT / LDE of each
-        shape and keep the fastest (process-wide); off: default splits."""
+        """Time the admissible plans (pass split, LDS tile) on the first NTT /
+        LDE of each shape and keep the fastest (process-wide); off: default
+        plans."""
         self._check(lib().stark_set_autotune(self.h, int(on)))
 
     def ntt_split(self, log_first_pass):
         """Pin the first pass of every two-pass transform that admits it to
         2^log_first_pass points (0 = no pin)."""
         self._check(lib().stark_set_ntt_split(self.h, int(log_first_pass)))
+
+    def ntt_tile(self, tile_log):
+        """Pin the LDS tile of every tunable transform that admits it: 13
+        (2^13 elements on 512 lanes) or 14 (2^14 on 1024 lanes, passes of 2^8
+        points and more); 0 = no pin."""
+        self._check(lib().stark_set_ntt_tile(self.h, int(tile_log)))
 
 
 def set_autotune_default(on=True):
@@ -304,8 +312,8 @@ def set_autotune_default(on=True):
 
 
 def autotune_report():
-    """{(kind, log_points, log_blowup, log_cols): (chosen L1, {L1: ms})} of the
-    shapes tuned so far in this process."""
+    """{(kind, log_points, log_blowup, log_cols): ((L1, tile_log) chosen,
+    {(L1, tile_log): ms})} of the shapes tuned so far in this process."""
     n = lib().stark_autotune_report(None, 0)
     buf = ctypes.create_string_buffer(n + 1)
     lib().stark_autotune_report(buf, n + 1)
@@ -314,8 +322,11 @@ def autotune_report():
         head, times = line.split("|")
         shape, best = head.split("->")
         kind, L, LB, lc = shape.split()
-        ms = {int(a): float(b) for a, b in (t.split(":") for t in times.split())}
-        out[(kind, int(L), int(LB), int(lc))] = (int(best), ms)
+        def plan(x):
+            a, b = x.split(".")
+            return int(a), int(b)
+        ms = {plan(a): float(b) for a, b in (t.split(":") for t in times.split())}
+        out[(kind, int(L), int(LB), int(lc))] = (plan(best.strip()), ms)
     return out
 
 

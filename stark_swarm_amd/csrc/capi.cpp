@@ -655,6 +655,12 @@ int stark_set_ntt_split(stark_ctx *ctx, unsigned log_first_pass) {
     return STARK_OK;
 }
 
+int stark_set_ntt_tile(stark_ctx *ctx, unsigned tile_log) {
+    if (!ctx || !ctx->c || (tile_log != 0 && tile_log != 13 && tile_log != 14)) return STARK_E_INVALID;
+    ctx->c->force_tile = tile_log;
+    return STARK_OK;
+}
+
 size_t stark_autotune_report(char *buf, size_t cap) {
     const std::string all = sg::autotune_report();
     if (buf && cap) {

@@ -95,11 +95,12 @@ struct Ctx {
     // odd-coset twist tables w_{2n}^i / n (i < n) by log2 n, built once per context
     std::map<uint32_t, uint64_t *> odd_twists;
     // NTT / LDE launch plans: with autotune on, the first transform of each
-    // shape times the admissible pass splits and the process keeps the
-    // fastest; force_split (log2 of the first pass, 0 = off) pins one split
-    // for every two-pass transform it admits.
+    // shape times the admissible plans (pass split, tile) and the process
+    // keeps the fastest; force_split (log2 of the first pass, 0 = off) and
+    // force_tile (13 or 14, 0 = off) pin the split and the tile of every
+    // transform that admits them.
     bool autotune = false;
-    uint32_t force_split = 0;
+    uint32_t force_split = 0, force_tile = 0;
     void *upload_staging(size_t n);
     // copy n bytes of device memory to host memory after the stream's
     // earlier work; returns once dst holds them

@@ -67,6 +67,20 @@ SG_HD uint64_t mul(uint64_t a, uint64_t b) {
     return reduce128(lo, hi);
 }
 SG_HD uint64_t canon(uint64_t x) { return x >= P ? x - P : x; }
+// a * b mod p as any u64 congruent to it (reduce128 without its final
+// conditional subtraction), for consumers that accept lazy values
+SG_HD uint64_t mul_lazy(uint64_t a, uint64_t b) {
+    uint64_t lo, hi;
+    mul128(a, b, lo, hi);
+    const uint64_t hh = hi >> 32;
+    const uint32_t hl = (uint32_t)hi;
+    uint64_t t0 = lo - hh;
+    if (lo < hh) t0 -= EPS;
+    const uint64_t t1 = ((uint64_t)(hl ? hl - 1 : 0) << 32) | (uint32_t)(0u - hl);
+    uint64_t r = t0 + t1;
+    if (r < t1) r += EPS;
+    return r;
+}
 
 SG_HD uint64_t pow(uint64_t a, uint64_t e) {
     uint64_t r = 1;

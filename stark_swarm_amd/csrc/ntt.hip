@@ -56,11 +56,13 @@ namespace {
 // kept.
 constexpr int MODE_GENERAL = 0, MODE_PLAIN = 1;
 
-template <int LOG_S, bool INV, int MODE>
-// 512 lanes x 16 elements, at most 128 VGPRs: two blocks (4 waves per SIMD)
-// share a CU, which hides more of the 64-bit multiply latency than 256 x 32
-// at 2 waves per SIMD (about 5 % faster on the 2^20 x 128 LDE)
-__global__ __launch_bounds__(NT, 4) void ntt_tile(NttPass p) {
+template <int LOG_S, bool INV, int MODE, int TLOG>
+// 16 elements a lane, at most 128 VGPRs: 512 lanes with two blocks per CU or
+// 1024 lanes with one, 4 waves per SIMD either way, which hides more of the
+// 64-bit multiply latency than 256 x 32 at 2 waves per SIMD (about 5 %
+// faster on the 2^20 x 128 LDE)
+__global__ __launch_bounds__(TileCfg<TLOG>::LANES, 4) void ntt_tile(NttPass p) {
+    constexpr int LANES = TileCfg<TLOG>::LANES, PER_LANE = TileCfg<TLOG>::PER_LANE;
     if constexpr (MODE == MODE_PLAIN) {
         // no pre-twiddle, post twiddle or scaling: constants the compiler
         // folds, so those paths and their registers drop out
@@ -69,7 +71,7 @@ __global__ __launch_bounds__(NT, 4) void ntt_tile(NttPass p) {
         p.scale = 1;
         p.sp_lo = nullptr;
     }
-    __shared__ uint64_t lds[LDS_ELEMS];
+    __shared__ uint64_t lds[TileCfg<TLOG>::LDS];
     constexpr uint32_t S = 1u << LOG_S;
     const uint32_t log_g = p.log_g, G = 1u << log_g;
     const uint32_t tile = S << log_g;
@@ -91,11 +93,11 @@ __global__ __launch_bounds__(NT, 4) void ntt_tile(NttPass p) {
     const bool out_gfast = log_g > 0 && p.og < p.ok;
     // g-fast lane mapping: element u of a lane is (g = gl, i = il + u*di), so
     // the lane's twiddles form geometric sequences in u (no table gathers)
-    const bool inc_in = in_gfast && (1u << log_g) <= (uint32_t)NT;
-    const bool inc_out = out_gfast && (1u << log_g) <= (uint32_t)NT;
+    const bool inc_in = in_gfast && (1u << log_g) <= (uint32_t)LANES;
+    const bool inc_out = out_gfast && (1u << log_g) <= (uint32_t)LANES;
     const uint32_t gl = threadIdx.x & (G - 1);
     const uint32_t il = threadIdx.x >> log_g;
-    const uint32_t di = NT >> log_g;
+    const uint32_t di = LANES >> log_g;
     const bool gok = g0 + gl < p.ng;
 
     uint64_t *out0 = p.out + b_hi * p.ob + b_lo * p.ob2 + g0 * p.og + (uint64_t)rbase * p.out_rs;
@@ -121,7 +123,7 @@ __global__ __launch_bounds__(NT, 4) void ntt_tile(NttPass p) {
                 }
             }
 #pragma unroll 2
-            for (int u = 0; u < EPT; u++) {
+            for (int u = 0; u < PER_LANE; u++) {
                 const uint32_t i = il + u * di;
                 if (i < S) {
                     uint64_t x = gok ? in[(int64_t)gl * p.ig + (int64_t)i * p.ii] : 0;
@@ -133,8 +135,8 @@ __global__ __launch_bounds__(NT, 4) void ntt_tile(NttPass p) {
             }
         } else {
 #pragma unroll 2
-            for (int u = 0; u < EPT; u++) {
-                const uint32_t e = threadIdx.x + u * NT;
+            for (int u = 0; u < PER_LANE; u++) {
+                const uint32_t e = threadIdx.x + u * LANES;
                 if (e >= tile) continue;
                 uint32_t g, i;
                 if (in_gfast) {
@@ -157,7 +159,7 @@ __global__ __launch_bounds__(NT, 4) void ntt_tile(NttPass p) {
             }
         }
         __syncthreads();
-        stockham_steps<LOG_S, 0, INV>(lds, tile, p.tw);
+        stockham_steps<LOG_S, 0, INV, TLOG>(lds, tile, p.tw);
         uint64_t *out = out0 + (uint64_t)rep * p.out_rs;
         if (inc_out) {
             // post twiddle w^(+-g*k) and output scale * s^(g*sp_g + k*sp_k)
@@ -173,7 +175,7 @@ __global__ __launch_bounds__(NT, 4) void ntt_tile(NttPass p) {
             }
             const bool sc = p.sp_lo || p.scale != 1;
 #pragma unroll 2
-            for (int u = 0; u < EPT; u++) {
+            for (int u = 0; u < PER_LANE; u++) {
                 const uint32_t k = il + u * di;
                 if (k < S && gok) {
                     uint64_t x = lds[padi((gl << LOG_S) + k)];
@@ -187,8 +189,8 @@ __global__ __launch_bounds__(NT, 4) void ntt_tile(NttPass p) {
             }
         } else {
 #pragma unroll 2
-            for (int u = 0; u < EPT; u++) {
-                const uint32_t e = threadIdx.x + u * NT;
+            for (int u = 0; u < PER_LANE; u++) {
+                const uint32_t e = threadIdx.x + u * LANES;
                 if (e >= tile) continue;
                 uint32_t g, k;
                 if (out_gfast) {
@@ -305,10 +307,20 @@ __global__ __launch_bounds__(NT, 4) void odd_coset_tile(const uint64_t *in, uint
 }
 
 template <bool INV, int MODE>
-void launch_tile(uint32_t log_s, dim3 grid, hipStream_t st, const NttPass &p) {
+void launch_tile(uint32_t log_s, uint32_t tile_log, dim3 grid, hipStream_t st, const NttPass &p) {
+    if (tile_log == 14) {
+        switch (log_s) {
+#define SG_CASE(L) \
+    case L: hipLaunchKernelGGL((ntt_tile<L, INV, MODE, 14>), grid, dim3(TileCfg<14>::LANES), 0, st, p); break;
+            SG_CASE(8) SG_CASE(9) SG_CASE(10) SG_CASE(11) SG_CASE(12)
+#undef SG_CASE
+        default: throw Error(E_INVALID, "ntt pass: 16K tiles hold sub-DFTs of 256 to 4096 points");
+        }
+        return;
+    }
     switch (log_s) {
 #define SG_CASE(L) \
-    case L: hipLaunchKernelGGL((ntt_tile<L, INV, MODE>), grid, dim3(NT), 0, st, p); break;
+    case L: hipLaunchKernelGGL((ntt_tile<L, INV, MODE, 13>), grid, dim3(NT), 0, st, p); break;
         SG_CASE(0) SG_CASE(1) SG_CASE(2) SG_CASE(3) SG_CASE(4) SG_CASE(5) SG_CASE(6)
         SG_CASE(7) SG_CASE(8) SG_CASE(9) SG_CASE(10) SG_CASE(11) SG_CASE(12)
 #undef SG_CASE
@@ -320,8 +332,13 @@ void launch_tile(uint32_t log_s, dim3 grid, hipStream_t st, const NttPass &p) {
 
 void launch_pass(Ctx *c, NttPass p, const char *name) {
     SG_REQUIRE(p.log_s <= 12, "ntt pass: sub-DFT larger than 4096");
-    const uint32_t max_g = TILE_LOG - p.log_s;
-    if (p.log_g > max_g) p.log_g = max_g;
+    if (p.tile_log == 0) p.tile_log = TILE_LOG;
+    SG_REQUIRE(p.tile_log == TILE_LOG || (p.tile_log == 14 && p.log_s >= 8),
+               "ntt pass: tiles of 2^13, or of 2^14 for sub-DFTs of 256 points and more");
+    // as many sub-DFTs per tile as fit and exist
+    uint32_t max_g = p.tile_log - p.log_s;
+    while (max_g > 0 && (1ULL << max_g) > p.ng) max_g--;
+    p.log_g = max_g;
     if (p.nr == 0) p.nr = 1;
     if (p.nb2 == 0) p.nb2 = 1;
     SG_REQUIRE(p.pre_log <= 26 && p.post_log <= 26, "ntt pass: twiddle order above 2^26");
@@ -344,18 +361,12 @@ void launch_pass(Ctx *c, NttPass p, const char *name) {
     const dim3 grid((uint32_t)(tiles << rs_log));
     const bool plain = p.pre_log == 0 && p.post_log == 0 && p.scale == 1 && p.sp_lo == nullptr;
     if (p.inv)
-        launch_tile<true, MODE_GENERAL>(p.log_s, grid, c->stream, p);
+        launch_tile<true, MODE_GENERAL>(p.log_s, p.tile_log, grid, c->stream, p);
     else if (plain)
-        launch_tile<false, MODE_PLAIN>(p.log_s, grid, c->stream, p);
+        launch_tile<false, MODE_PLAIN>(p.log_s, p.tile_log, grid, c->stream, p);
     else
-        launch_tile<false, MODE_GENERAL>(p.log_s, grid, c->stream, p);
+        launch_tile<false, MODE_GENERAL>(p.log_s, p.tile_log, grid, c->stream, p);
     SG_HIP(hipGetLastError());
-}
-
-static uint32_t pick_log_g(uint64_t ng, uint32_t log_s) {
-    uint32_t lg = TILE_LOG - log_s;
-    while (lg > 0 && (1ULL << lg) > ng) lg--;
-    return lg;
 }
 
 // Power table s^e, e < 2^26, two levels of 2^13 entries, built on the host.
@@ -409,11 +420,16 @@ uint32_t plan_key(uint32_t kind, uint32_t L, uint32_t LB, uint64_t ncols) {
     return (kind << 24) | (L << 16) | (LB << 8) | lc;
 }
 
-// two-pass L1 window: within 2 of ceil(L/2), both passes of 2^3..2^12 points
+// two-pass L1 window: within 2 of ceil(L/2), both passes of 2^3..2^12
+// points, each with the 2^13 tile and, when both passes have 2^8 points or
+// more, the 2^14 tile
 void two_pass_window(uint32_t L, std::vector<uint32_t> &out) {
     const int mid = (int)(L + 1) / 2;
     const int lo = std::max({mid - 2, (int)L - 12, 3}), hi = std::min({mid + 2, 12, (int)L - 3});
-    for (int s = lo; s <= hi; s++) out.push_back((uint32_t)s);
+    for (int s = lo; s <= hi; s++) {
+        out.push_back(plan_code((uint32_t)s, 13));
+        if (s >= 8 && (int)L - s >= 8) out.push_back(plan_code((uint32_t)s, 14));
+    }
 }
 
 // The split for this call: the forced one if it is admissible, else the
@@ -421,10 +437,12 @@ void two_pass_window(uint32_t L, std::vector<uint32_t> &out) {
 // candidate timed now with `run`, else the default. Contexts with tuning
 // off still take plans other contexts have tuned, but never time any.
 template <class Run>
-uint32_t choose_split(Ctx *c, uint32_t kind, uint32_t L, uint32_t LB, uint64_t ncols, const PlanSet &ps, Run run) {
-    if (c->force_split) {
+uint32_t choose_plan(Ctx *c, uint32_t kind, uint32_t L, uint32_t LB, uint64_t ncols, const PlanSet &ps, Run run) {
+    if (c->force_split || c->force_tile) {
+        const uint32_t want = plan_code(c->force_split ? c->force_split : plan_split(ps.dflt),
+                                        c->force_tile ? c->force_tile : plan_tile(ps.dflt));
         for (uint32_t s : ps.cand)
-            if (s == c->force_split) return s;
+            if (plan_split(s) == plan_split(want) && plan_tile(s) == plan_tile(want)) return s;
         return ps.dflt;
     }
     if (ps.cand.size() < 2) return ps.dflt;
@@ -466,7 +484,7 @@ uint32_t choose_split(Ctx *c, uint32_t kind, uint32_t L, uint32_t LB, uint64_t n
 
 PlanSet ntt_plans(uint32_t L) {
     PlanSet ps;
-    ps.dflt = L <= 12 ? L : (L + 1) / 2;
+    ps.dflt = plan_code(L <= 12 ? L : (L + 1) / 2, 13);
     if (L > 12) two_pass_window(L, ps.cand);
     return ps;
 }
@@ -474,11 +492,14 @@ PlanSet ntt_plans(uint32_t L) {
 PlanSet lde_plans(uint32_t L, uint32_t LB) {
     PlanSet ps;
     if (L >= 4 && L <= 9 && L + LB <= (uint32_t)TILE_LOG) {
-        ps.dflt = L;  // all cosets of a column in one workgroup: nothing to choose
+        ps.dflt = plan_code(L, 13);  // all cosets of a column in one workgroup: nothing to choose
         return ps;
     }
-    ps.dflt = L <= 12 ? L : (L + 1) / 2;
-    if (L <= 12) ps.cand.push_back(L);
+    ps.dflt = plan_code(L <= 12 ? L : (L + 1) / 2, 13);
+    if (L <= 12) {
+        ps.cand.push_back(plan_code(L, 13));
+        if (L >= 8) ps.cand.push_back(plan_code(L, 14));
+    }
     two_pass_window(L, ps.cand);
     return ps;
 }
@@ -491,11 +512,11 @@ std::string autotune_report() {
     for (auto &kv : plan_table) {
         if (!kv.second.best) continue;
         const uint32_t k = kv.first;
-        snprintf(line, sizeof line, "%s %u %u %u -> %u |", kinds[(k >> 24) % 3], (k >> 16) & 255, (k >> 8) & 255,
-                 k & 255, kv.second.best);
+        snprintf(line, sizeof line, "%s %u %u %u -> %u.%u |", kinds[(k >> 24) % 3], (k >> 16) & 255,
+                 (k >> 8) & 255, k & 255, plan_split(kv.second.best), plan_tile(kv.second.best));
         out += line;
         for (auto &m : kv.second.ms) {
-            snprintf(line, sizeof line, " %u:%.4f", m.first, m.second);
+            snprintf(line, sizeof line, " %u.%u:%.4f", plan_split(m.first), plan_tile(m.first), m.second);
             out += line;
         }
         out += "\n";
@@ -507,9 +528,9 @@ namespace {
 
 // DFT of 2^L points per column as two strided passes of 2^L1 and 2^(L - L1)
 // points (four-step), through `scratch`
-void ntt_two_pass(Ctx *c, const uint64_t *in, uint64_t *out, uint64_t *scratch, uint32_t L, uint32_t L1,
+void ntt_two_pass(Ctx *c, const uint64_t *in, uint64_t *out, uint64_t *scratch, uint32_t L, uint32_t plan,
                   uint64_t ncols, uint64_t in_cs, uint64_t out_cs, bool inv, uint64_t scale, const PowTable *spow) {
-    const uint32_t L2 = L - L1;
+    const uint32_t L1 = plan_split(plan), L2 = L - L1;
     const uint64_t n = 1ULL << L, n1 = 1ULL << L1, n2 = 1ULL << L2;
     NttPass a = base_pass();
     a.in = in;
@@ -523,7 +544,7 @@ void ntt_two_pass(Ctx *c, const uint64_t *in, uint64_t *out, uint64_t *scratch, 
     a.og = 1;
     a.ok = n2;
     a.log_s = L1;
-    a.log_g = pick_log_g(n2, L1);
+    a.tile_log = plan_tile(plan);
     a.inv = inv;
     a.post_log = L;
     launch_pass(c, a, inv ? "intt_passA" : "ntt_passA");
@@ -539,7 +560,7 @@ void ntt_two_pass(Ctx *c, const uint64_t *in, uint64_t *out, uint64_t *scratch, 
     b.og = 1;
     b.ok = n1;
     b.log_s = L2;
-    b.log_g = pick_log_g(n1, L2);
+    b.tile_log = plan_tile(plan);
     b.inv = inv;
     b.scale = scale;
     if (spow) {
@@ -572,7 +593,6 @@ void ntt_columns(Ctx *c, const uint64_t *in, uint64_t *out, uint64_t *scratch, u
         p.og = out_cs;
         p.ok = 1;
         p.log_s = L;
-        p.log_g = pick_log_g(ncols, L);
         p.inv = inv;
         p.scale = scale;
         if (spow) {
@@ -585,15 +605,15 @@ void ntt_columns(Ctx *c, const uint64_t *in, uint64_t *out, uint64_t *scratch, u
     }
     PlanSet ps = ntt_plans(L);
     if (in == out) ps.cand.clear();
-    const uint32_t L1 = choose_split(c, inv ? 1 : 0, L, 0, ncols, ps, [&](uint32_t s) {
+    const uint32_t plan = choose_plan(c, inv ? 1 : 0, L, 0, ncols, ps, [&](uint32_t s) {
         ntt_two_pass(c, in, out, scratch, L, s, ncols, in_cs, out_cs, inv, scale, spow);
     });
-    ntt_two_pass(c, in, out, scratch, L, L1, ncols, in_cs, out_cs, inv, scale, spow);
+    ntt_two_pass(c, in, out, scratch, L, plan, ncols, in_cs, out_cs, inv, scale, spow);
 }
 
 namespace {
-void lde_forward(Ctx *c, const uint64_t *coef, uint64_t *out, uint32_t L, uint32_t LB, uint32_t L1, uint64_t ncols,
-                 uint64_t out_cs);
+void lde_forward(Ctx *c, const uint64_t *coef, uint64_t *out, uint32_t L, uint32_t LB, uint32_t plan,
+                 uint64_t ncols, uint64_t out_cs);
 }  // namespace
 
 // Coset LDE of `ncols` columns: coefficients c = iDFT(col) scaled by shift^j,
@@ -634,10 +654,10 @@ void lde_columns(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint32
     } else {
         const PlanSet ps = lde_plans(L, LB);
         const uint64_t *cf = coef.as<uint64_t>();
-        const uint32_t L1 = choose_split(c, 2, L, LB, ncols, ps, [&](uint32_t s) {
+        const uint32_t plan = choose_plan(c, 2, L, LB, ncols, ps, [&](uint32_t s) {
             lde_forward(c, cf, out, L, LB, s, ncols, out_cs);
         });
-        lde_forward(c, cf, out, L, LB, L1, ncols, out_cs);
+        lde_forward(c, cf, out, L, LB, plan, ncols, out_cs);
     }
 }
 
@@ -646,8 +666,9 @@ namespace {
 // The forward half of a coset LDE from its (shift-scaled) coefficients coef
 // (ncols x 2^L): L1 = L is one pass per column tile with the B cosets as
 // repetitions (L <= 12); otherwise two passes of 2^L1 and 2^(L - L1) points.
-void lde_forward(Ctx *c, const uint64_t *coef, uint64_t *out, uint32_t L, uint32_t LB, uint32_t L1, uint64_t ncols,
-                 uint64_t out_cs) {
+void lde_forward(Ctx *c, const uint64_t *coef, uint64_t *out, uint32_t L, uint32_t LB, uint32_t plan,
+                 uint64_t ncols, uint64_t out_cs) {
+    const uint32_t L1 = plan_split(plan), tl = plan_tile(plan);
     const uint64_t n = 1ULL << L, B = 1ULL << LB;
     const uint32_t LN = L + LB;
     if (L1 == L) {
@@ -662,7 +683,7 @@ void lde_forward(Ctx *c, const uint64_t *coef, uint64_t *out, uint32_t L, uint32
         p.og = out_cs;
         p.ok = B;
         p.log_s = L;
-        p.log_g = pick_log_g(ncols, L);
+        p.tile_log = tl;
         p.pre_log = LN;
         p.pre_i = 1;
         p.nr = (uint32_t)B;
@@ -688,7 +709,7 @@ void lde_forward(Ctx *c, const uint64_t *coef, uint64_t *out, uint32_t L, uint32
         a.og = 1;
         a.ok = B * n2;
         a.log_s = L1;
-        a.log_g = pick_log_g(n2, L1);
+        a.tile_log = tl;
         a.post_log = L;
         a.pre_log = LN;  // repetition r multiplies element j = i*n2 + g by w_{Bn}^(r j)
         a.pre_g = 1;
@@ -708,7 +729,7 @@ void lde_forward(Ctx *c, const uint64_t *coef, uint64_t *out, uint32_t L, uint32
         b.og = 1;
         b.ok = B * n1;
         b.log_s = L2;
-        b.log_g = pick_log_g(B * n1, L2);
+        b.tile_log = tl;
         launch_pass(c, b, "lde_passB");
     }
 }

@@ -26,6 +26,9 @@ struct NttPass {
     // blockIdx.x = tile * 2^rs_log + chunk, chunk c runs repetitions [c nr, (c+1) nr)
     uint32_t rs_log;
     uint32_t log_s, log_g;
+    // log2 of the LDS tile: 13 (512 lanes) or 14 (1024 lanes, log_s >= 8); 0 = 13.
+    // launch_pass sets log_g to the most sub-DFTs the tile holds.
+    uint32_t tile_log;
     uint32_t inv;
     // input pre-multiply by w_{2^pre_log}^(r * (g*pre_g + i*pre_i + b*pre_b)) (0 = off);
     // pre_direct (set by launch_pass when pre_log <= 13): each factor is one
@@ -69,22 +72,30 @@ void ntt_columns_noalloc(Ctx *c, const uint64_t *in, uint64_t *out, uint32_t L, 
 uint64_t ntt_noalloc_max_elems();
 
 // Autotuned launch plans. A plan is the log2 size L1 of the first pass of a
-// two-pass transform of 2^L points (the second pass has 2^(L - L1)); for an
-// LDE of 2^L <= 4096 points L1 = L is the one-pass form with the cosets as
-// repetitions. Two-pass candidates: L1 within 2 of ceil(L/2), both passes of
-// 2^3 to 2^12 points. The fastest split depends on the shape (sub-DFTs per
-// 8K tile, the length of coalesced runs, twiddle layers), so it is measured
-// rather than modelled: on the first call per (kind, L, LB, log2 columns)
-// with autotune on, each candidate runs once untimed and once between HIP
-// events on the context stream, and the process-wide table keeps the
-// fastest. Every candidate computes the same exact transform.
+// two-pass transform of 2^L points (the second pass has 2^(L - L1)) and the
+// LDS tile its passes use (2^13 elements on 512 lanes or 2^14 on 1024 lanes;
+// a 2^14 tile holds twice the sub-DFTs, so its global loads and stores run
+// twice as long); for an LDE of 2^L <= 4096 points L1 = L is the one-pass
+// form with the cosets as repetitions. Two-pass candidates: L1 within 2 of
+// ceil(L/2), both passes of 2^3 to 2^12 points, each with the 2^13 tile and,
+// when both passes have at least 2^8 points, the 2^14 tile. The fastest
+// plan depends on the shape (sub-DFTs per tile, the length of coalesced
+// runs, twiddle layers, workgroups per CU), so it is measured rather than
+// modelled: on the first call per (kind, L, LB, log2 columns) with autotune
+// on, each candidate runs once untimed and once between HIP events on the
+// context stream, and the process-wide table keeps the fastest. Every
+// candidate computes the same exact transform.
+// A plan is coded as L1 | tile_log << 8.
+constexpr uint32_t plan_code(uint32_t l1, uint32_t tile_log) { return l1 | tile_log << 8; }
+constexpr uint32_t plan_split(uint32_t code) { return code & 255; }
+constexpr uint32_t plan_tile(uint32_t code) { return code >> 8 ? code >> 8 : 13; }
 struct PlanSet {
-    std::vector<uint32_t> cand;  // admissible L1 (empty: nothing to choose)
-    uint32_t dflt;               // the untuned choice
+    std::vector<uint32_t> cand;  // admissible plans (empty: nothing to choose)
+    uint32_t dflt;               // the untuned plan
 };
 PlanSet ntt_plans(uint32_t L);
 PlanSet lde_plans(uint32_t L, uint32_t LB);
-// one "kind L LB log_cols -> L1 | L1:ms ..." line per tuned shape
+// one "kind L LB log_cols -> L1.tile | L1.tile:ms ..." line per tuned shape
 std::string autotune_report();
 
 }  // namespace sg

@@ -1,5 +1,5 @@
 // ntt_dev.hpp -- device building blocks of the Goldilocks NTT kernels for
-// gfx950: the 8K-element LDS tile, radix <= 16 register DFTs whose twiddles
+// gfx950: the 8K- or 16K-element LDS tile, radix <= 16 register DFTs whose twiddles
 // are powers of 2 (w_32 = 2^78), and the Stockham steps that run S-point
 // sub-DFTs of a tile in place. Shared by ntt.hip and the kernels that fuse a
 // transform with their own loads (p256_air.hip).
@@ -19,6 +19,17 @@ constexpr int PAD_SHIFT = 4;
 constexpr int LDS_ELEMS = TILE + (TILE >> PAD_SHIFT);
 
 __device__ __forceinline__ uint32_t padi(uint32_t e) { return e + (e >> PAD_SHIFT); }
+
+// A tile of 2^TLOG u64: 8K on 512 lanes (two workgroups per CU) or 16K on
+// 1024 lanes (one workgroup per CU, 136 KiB of LDS); 16 elements per lane
+// either way, so the register budget of the Stockham steps is the same.
+template <int TLOG>
+struct TileCfg {
+    static constexpr int SIZE = 1 << TLOG;
+    static constexpr int LANES = TLOG >= 14 ? 1024 : NT;
+    static constexpr int PER_LANE = SIZE / LANES;
+    static constexpr int LDS = SIZE + (SIZE >> PAD_SHIFT);
+};
 
 // x * 2^k mod p for a compile-time-foldable k in [0, 192): 2^96 = -1.
 // x may be lazily reduced (any u64); the result is canonical.
@@ -131,12 +142,13 @@ constexpr int next_lr(int left) {
     return (left + steps - 1) / steps;
 }
 
-template <int LOG_S, int LOG_NS, bool INV>
+template <int LOG_S, int LOG_NS, bool INV, int TLOG = TILE_LOG>
 __device__ __forceinline__ void stockham_steps(uint64_t *lds, uint32_t tile, const uint64_t *tw) {
     if constexpr (LOG_NS < LOG_S) {
+        constexpr int LANES = TileCfg<TLOG>::LANES;
         constexpr int LR = next_lr(LOG_S - LOG_NS);
         constexpr int R = 1 << LR;
-        constexpr int MAXG = EPT / R;
+        constexpr int MAXG = TileCfg<TLOG>::PER_LANE / R;
         constexpr int SUB_LOG = LOG_S - LR;
         constexpr uint32_t SUB = 1u << SUB_LOG;
         constexpr uint32_t NS = 1u << LOG_NS;
@@ -144,7 +156,7 @@ __device__ __forceinline__ void stockham_steps(uint64_t *lds, uint32_t tile, con
         uint64_t v[MAXG][R];
 #pragma unroll
         for (int u = 0; u < MAXG; u++) {
-            const uint32_t q = threadIdx.x + u * NT;
+            const uint32_t q = threadIdx.x + u * LANES;
             if (q < groups) {
                 const uint32_t g = q >> SUB_LOG, j = q & (SUB - 1);
                 const uint32_t base = (g << LOG_S) + j;
@@ -154,7 +166,7 @@ __device__ __forceinline__ void stockham_steps(uint64_t *lds, uint32_t tile, con
                     const uint32_t jm = j & (NS - 1);
                     const uint64_t *t = tw + (NS << LR);
 #pragma unroll
-                    for (int r = 1; r < R; r++) v[u][r] = mul(v[u][r], t[jm * r]);
+                    for (int r = 1; r < R; r++) v[u][r] = mul_lazy(v[u][r], t[jm * r]);  // dft_reg takes any u64
                 }
                 dft_reg<LR, INV>(v[u]);
             }
@@ -162,7 +174,7 @@ __device__ __forceinline__ void stockham_steps(uint64_t *lds, uint32_t tile, con
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < MAXG; u++) {
-            const uint32_t q = threadIdx.x + u * NT;
+            const uint32_t q = threadIdx.x + u * LANES;
             if (q < groups) {
                 const uint32_t g = q >> SUB_LOG, j = q & (SUB - 1);
                 const uint32_t jm = j & (NS - 1);
@@ -172,7 +184,7 @@ __device__ __forceinline__ void stockham_steps(uint64_t *lds, uint32_t tile, con
             }
         }
         __syncthreads();
-        stockham_steps<LOG_S, LOG_NS + LR, INV>(lds, tile, tw);
+        stockham_steps<LOG_S, LOG_NS + LR, INV, TLOG>(lds, tile, tw);
     }
 }
 

@@ -47,6 +47,9 @@ def lib():
         L.ref_gl_inv.argtypes = [ctypes.c_uint64]
         L.ref_gl_root_of_unity.restype = ctypes.c_uint64
         L.ref_gl_root_of_unity.argtypes = [sz]
+        L.ref_set_threads.restype = None
+        L.ref_set_threads.argtypes = [ctypes.c_int]
+        L.ref_get_threads.restype = ctypes.c_int
         L.ref_stdrng_first_u64.restype = ctypes.c_uint64
         L.ref_stdrng_first_u64.argtypes = [ctypes.c_uint64]
         L.ref_fp6_mul.argtypes = [u64p, u64p, u64p]
@@ -576,3 +579,18 @@ def prove_zsk_ksk_binding_v2(pk, sig, msg, fs_binding, merkle_root, k=256, blowu
     if rc:
         return rc
     return take_bytes(pp, pl), rf.raw, pi.raw
+
+
+class threads:
+    """with O.threads(n): run the oracle's rayon-site loops on n threads"""
+
+    def __init__(self, n):
+        self.n = n
+
+    def __enter__(self):
+        self.old = lib().ref_get_threads()
+        lib().ref_set_threads(self.n)
+        return self
+
+    def __exit__(self, *a):
+        lib().ref_set_threads(self.old)
