@@ -319,6 +319,10 @@ void ref_build_nsec3_trace(size_t n_trace, const uint64_t *chain, size_t n, uint
 /* deep_ali_merge_general (lib.rs:343-437) for AirType::Nsec3Chain:
  * C_k = nxt[k] - cur[4 + k] (eval_nsec3_chain_constraints,
  * air_workloads.rs:862-870), lambda_k = k + 1, same phi quirk */
+void ref_nsec3_constraints(const uint64_t cur[8], const uint64_t nxt[8], uint64_t out[4]) {
+    for (int k = 0; k < 4; k++) out[k] = gl_sub(nxt[k], cur[4 + k]);
+}
+
 int ref_merge_nsec3(const uint64_t *lde, size_t n_trace, size_t blowup, int phi_mode, uint64_t *c_eval) {
     const size_t n = n_trace * blowup;
     const int compute = phi_mode == 0 || (phi_mode == 1 && n >= 4096);
@@ -326,8 +330,13 @@ int ref_merge_nsec3(const uint64_t *lde, size_t n_trace, size_t blowup, int phi_
         uint64_t acc = 0;
         if (compute) {
             const size_t j = (i + blowup) % n;
-            for (int k = 0; k < 4; k++)
-                acc = gl_add(acc, gl_mul((uint64_t)(k + 1), gl_sub(lde[k * n + j], lde[(4 + k) * n + i])));
+            uint64_t cur[8], nxt[8], cv[4];
+            for (int c = 0; c < 8; c++) {
+                cur[c] = lde[c * n + i];
+                nxt[c] = lde[c * n + j];
+            }
+            ref_nsec3_constraints(cur, nxt, cv);
+            for (int k = 0; k < 4; k++) acc = gl_add(acc, gl_mul((uint64_t)(k + 1), cv[k]));
         }
         c_eval[i] = acc;
     }

@@ -117,6 +117,8 @@ int ref_merge_sha512(const uint64_t *lde, size_t n_trace, size_t blowup, size_t 
 /* ---- NSEC3 chain (AirType::Nsec3Chain) and prove_nsec3_completeness ---- */
 /* chain: n records x 8 raw u64 (owner limbs, next limbs) */
 void ref_build_nsec3_trace(size_t n_trace, const uint64_t *chain, size_t n, uint64_t *tr);
+/* eval_nsec3_chain_constraints (air_workloads.rs:862-870): C_k = nxt[k] - cur[4 + k] */
+void ref_nsec3_constraints(const uint64_t cur[8], const uint64_t nxt[8], uint64_t out[4]);
 int ref_merge_nsec3(const uint64_t *lde, size_t n_trace, size_t blowup, int phi_mode, uint64_t *c_eval);
 /* records: n x (owner_hash[32] || next_hash[32]); -2 if not a closed chain */
 int ref_prove_nsec3_completeness(const uint8_t *records, size_t n, const uint8_t salt[16], const uint8_t fs_binding[32],

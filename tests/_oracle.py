@@ -102,6 +102,8 @@ def lib():
         L.ref_merge_sha512.argtypes = [u64p, sz, sz, sz, ctypes.c_int, u64p]
         L.ref_build_nsec3_trace.argtypes = [sz, u64p, sz, u64p]
         L.ref_merge_nsec3.argtypes = [u64p, sz, sz, ctypes.c_int, u64p]
+        L.ref_nsec3_constraints.restype = None
+        L.ref_nsec3_constraints.argtypes = [u64p, u64p, u64p]
         L.ref_prove_nsec3_completeness.restype = ctypes.c_int
         L.ref_prove_nsec3_completeness.argtypes = [ctypes.c_char_p, sz, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
                                                    ctypes.POINTER(u8p), ctypes.POINTER(sz), ctypes.c_char_p,
@@ -375,6 +377,15 @@ def nsec3_trace(n_trace, chain):
     tr = np.zeros((8, n_trace), dtype=np.uint64)
     lib().ref_build_nsec3_trace(n_trace, ptr(ch), len(chain), ptr(tr))
     return tr
+
+
+def nsec3_constraints(cur, nxt):
+    """the 4 chain-link constraints of one row"""
+    c = np.ascontiguousarray(cur, dtype=np.uint64)
+    x = np.ascontiguousarray(nxt, dtype=np.uint64)
+    out = np.zeros(4, dtype=np.uint64)
+    lib().ref_nsec3_constraints(ptr(c), ptr(x), ptr(out))
+    return out
 
 
 def merge_nsec3(lde, n_trace, blowup, phi_mode=1):

@@ -47,6 +47,29 @@ def test_padding_repeats_last_next_and_closes():
     assert links_hold(tr)
 
 
+def test_planted_links_fire_with_their_values():
+    """C_k = nxt[k] - cur[4 + k] (air_workloads.rs:862-870): owner limb 2 of
+    row 5 plus 3 fires row 4's C_2 with +3; next limb 1 of row 5 plus 3 fires
+    row 5's C_1 with -3; every other row and slot vanishes"""
+    n = 16
+    owners = [[i, (31 * i) % 2**64, (17 * i) % 2**64, (7 * i) % 2**64] for i in range(n)]
+    tr = O.nsec3_trace(n, [(owners[i], owners[(i + 1) % n]) for i in range(n)])
+
+    def fired(t):
+        out = {}
+        for r in range(n):
+            c = O.nsec3_constraints(t[:, r], t[:, (r + 1) % n])
+            out.update({(r, k): int(c[k]) for k in range(4) if c[k]})
+        return out
+    assert fired(tr) == {}
+    t = tr.copy()
+    t[2, 5] = (int(t[2, 5]) + 3) % P
+    assert fired(t) == {(4, 2): 3}
+    t = tr.copy()
+    t[4 + 1, 5] = (int(t[4 + 1, 5]) + 3) % P
+    assert fired(t) == {(5, 1): P - 3}
+
+
 def test_limbs_reduced_into_the_field():
     big = [P, P + 5, 2**64 - 1, 3]
     tr = O.nsec3_trace(8, [(big, big)] * 8)

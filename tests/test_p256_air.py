@@ -175,6 +175,16 @@ def test_wrong_rproj_fires_only_boundary():
     assert list(nz[3]) == [2 * 77797 + 40 + 1]
 
 
+def row_fired(tr, n_trace, r):
+    """{slot: value} of the constraints that do not vanish on row r"""
+    cons = O.p256_constraints(tr[:, r], tr[:, (r + 1) % n_trace], r, n_trace)
+    return {int(i): int(cons[i]) for i in np.nonzero(cons)[0]}
+
+
+RADIX = 1 << 26
+P_LIMBS = [(P >> (26 * k)) & (RADIX - 1) for k in range(10)]  # P_LIMBS_TIGHT
+
+
 def plant(tr, col, row, value):
     t = tr.copy()
     t[col, row] = value
@@ -182,45 +192,73 @@ def plant(tr, col, row, value):
 
 
 def test_planted_mistakes_fire_documented_slots():
+    """Each planted cell fires the slots its gadget's formulas put on it, with
+    the values those formulas give (p256_field_air.rs: booleanity
+    b (1 - b), limb pack c - sum 2^b bit (205-224, 404-451), add chain
+    a + b + carry_in - c - 2^26 carry and carry[9] (425-451), mul position
+    identities with carries biased by 2^35 (1044-1120), select c - b -
+    sel (a - b) (1243-1272), freeze chain a - p - diff + net_in - 2^26 net_out
+    and mux c - diff - neg_9 (a - diff) (1449-1530); the row transition
+    nxt.acc - cur.select_c, p256_ecdsa_double_multirow_air.rs:274-289)."""
     pts, _, _ = k4_points()
     bits = [1, 1, 0, 1]
     tr, _ = two_pass(pts, bits, bits, 4)
     g = O.p256_gadgets(0)
-    c = O.p256_columns()
     row = 1
+    G = O.P
     # (1) a c bit of the first mul of chain A's double (slot base 0): its
-    # booleanity slot 26 i + b and its limb packing slot 260 + i
-    t = plant(tr, g[0][1][1] + 26 * 3 + 5, row, 2)
-    assert list(rows_nonzero(t, 4)[row]) == [26 * 3 + 5, 260 + 3]
+    # booleanity slot 26 i + b (2 (1 - 2)) and its limb packing slot 260 + i
+    col = g[0][1][1] + 26 * 3 + 5
+    old = int(tr[col, row])
+    t = plant(tr, col, row, 2)
+    assert row_fired(t, 4, row) == {26 * 3 + 5: G - 2, 260 + 3: (-(2 - old) * 32) % G}
     # (2) the last carry of the first add (step 5, after the 13 muls): chain
-    # identity k = 9 (slot 270 + 9) and the "carry[9] = 0" slot 289
+    # identity k = 9 (slot 270 + 9: -2^26) and the "carry[9] = 0" slot 289 (1)
     base = 13 * 1207
+    assert int(tr[g[13][1][4] + 9, row]) == 0
     t = plant(tr, g[13][1][4] + 9, row, 1)
-    assert list(rows_nonzero(t, 4)[row]) == [base + 279, base + 289]
+    assert row_fired(t, 4, row) == {base + 279: (-RADIX) % G, base + 289: 1}
     # (3) a carry bit of the last mul of the add gadget (index 55 + 13 in
-    # eval order): booleanity slot 540 + 19 + 36 k + b, and positions k, k + 1
+    # eval order) := 5: booleanity 5 (1 - 5) at 540 + 19 + 36 k + b, the signed
+    # carry at position k moves by (5 - bit) 2^b: -2^26 of it at position k,
+    # + it at position k + 1
     gi = 55 + 13
     base = 34318 + 13 * 1207
-    t = plant(tr, g[gi][1][4] + 36 * 7 + 3, row, 5)
-    assert list(rows_nonzero(t, 4)[row]) == [base + 540 + 7, base + 540 + 8, base + 559 + 36 * 7 + 3]
+    col = g[gi][1][4] + 36 * 7 + 3
+    d = (5 - int(tr[col, row])) << 3
+    t = plant(tr, col, row, 5)
+    assert row_fired(t, 4, row) == {base + 540 + 7: (-RADIX * d) % G, base + 540 + 8: d % G,
+                                    base + 559 + 36 * 7 + 3: (5 * (1 - 5)) % G}
     # (4) chain B's z select output, limb 0, at a row with a successor: its
-    # packing slot, its select identity, and the transition into the next row
+    # packing slot (+d), its select identity (+d), and the transition into
+    # the next row (nxt.acc - cur.c: -d)
     gb = O.p256_gadgets(1)
     base = 77797 + 77797 - 281
-    t = plant(tr, gb[-1][1][0], row, int(tr[gb[-1][1][0], row]) ^ 1)
-    assert list(rows_nonzero(t, 4)[row]) == [base + 260, base + 270, 2 * 77797 + 60 + 50]
+    col = gb[-1][1][0]
+    old = int(tr[col, row])
+    d = (old ^ 1) - old
+    t = plant(tr, col, row, old ^ 1)
+    assert row_fired(t, 4, row) == {base + 260: d % G, base + 270: d % G, 2 * 77797 + 60 + 50: (-d) % G}
     # (5) the neg carry of limb 9 in the first freeze after a sub of chain A's
-    # add gadget: chain identity k = 9, booleanity, exclusion, the mux (it
-    # selects the raw input) -- freeze slots after 14 muls, 20 adds, 9 subs
-    # and 20 freezes-after-adds
+    # add gadget -- freeze slots after 14 muls, 20 adds, 9 subs and 20
+    # freezes-after-adds: the chain identity k = 9 moves by 2^26 (neg_9' -
+    # neg_9) and each mux slot k by (neg_9 - neg_9')(a_k - diff_k), a_k read
+    # back from the valid chain identity
     fz = [i for i, (k, _) in enumerate(g) if k == 3 and i >= 55][20]
     base = 34318 + 14 * 1207 + 20 * 290 + 9 * 311 + 20 * 591
-    neg9 = g[fz][1][5] + 9
-    flipped = 1 - int(tr[neg9, row])
-    t = plant(tr, neg9, row, flipped)
-    got = list(rows_nonzero(t, 4)[row])
-    assert got[:1] == [base + 540 + 9]
-    assert all(base <= s < base + 591 for s in got)
+    cd = g[fz][1]
+    dl, pos, neg = cd[2], cd[4], cd[5]
+    net = [int(tr[pos + k, row]) - int(tr[neg + k, row]) for k in range(10)]
+    a = [P_LIMBS[k] + int(tr[dl + k, row]) - (net[k - 1] if k else 0) + RADIX * net[k] for k in range(10)]
+    n_old = int(tr[neg + 9, row])
+    dn = (1 - n_old) - n_old
+    t = plant(tr, neg + 9, row, 1 - n_old)
+    want = {base + 540 + 9: (RADIX * dn) % G}
+    for k in range(10):
+        v = -dn * (a[k] - int(tr[dl + k, row]))
+        if v % G:
+            want[base + 581 + k] = v % G
+    assert row_fired(t, 4, row) == want
 
 
 def test_ecdsa_bits_and_native_verify():

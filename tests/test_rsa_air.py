@@ -42,10 +42,15 @@ def limbs_value(col_vals):
 
 
 def nonzero_slots(tr, row, base=0, n_trace=None):
+    return set(fired(tr, row, base, n_trace))
+
+
+def fired(tr, row, base=0, n_trace=None):
+    """{slot: value} of the record's constraints that do not vanish on `row`"""
     h = tr.shape[1]
     n_trace = n_trace or h
     c = O.rsa_exp_constraints(tr[base:base + W, row], tr[base:base + W, (row + 1) % h], row, n_trace)
-    return set(int(i) for i in np.nonzero(c)[0])
+    return {int(i): int(c[i]) for i in np.nonzero(c)[0]}
 
 
 def test_sizes():
@@ -154,69 +159,95 @@ def good():
     return O.rsa_exp_trace([rand_record(0x5EED)])
 
 
+# Values: the reference's formulas (rsa2048_field_air.rs:345-415 -- bit
+# booleanity b (1 - b), limb pack limb - sum 2^b bit, position identity
+# p_ab - p_qn - c_k + carry_in - 2^26 carry_out with biased carries, carry
+# booleanity; rsa2048_exp_air.rs:210-272 -- phase booleanity, select
+# op - (phase (s - acc) + acc), transitions nxt - cur) at the planted cell,
+# every other cell valid.
+
 def test_planted_c_bit(good):
     tr = good.copy()
     i, b, r = 3, 5, 2
+    old = int(tr[C_CBITS + LB * i + b, r])
     tr[C_CBITS + LB * i + b, r] = 2
-    assert nonzero_slots(tr, r) == {K_CBOOL + LB * i + b, K_CPACK + i}
+    assert fired(tr, r) == {K_CBOOL + LB * i + b: P - 2, K_CPACK + i: (-(2 - old) * (1 << b)) % P}
 
 
 def test_planted_q_bit_flip(good):
     tr = good.copy()
     i, b, r = 7, 0, 5
+    old = int(tr[C_QBITS + LB * i + b, r])
     tr[C_QBITS + LB * i + b, r] ^= 1
-    assert nonzero_slots(tr, r) == {K_QPACK + i}
+    assert fired(tr, r) == {K_QPACK + i: (-((old ^ 1) - old)) % P}
 
 
 def test_planted_carry_bit(good):
     tr = good.copy()
     m, b, r = 10, 3, 1
+    old = int(tr[C_CARRY + 40 * m + b, r])
     tr[C_CARRY + 40 * m + b, r] = 2
-    assert nonzero_slots(tr, r) == {K_CARRY + 40 * m + b, K_POS + m, K_POS + m + 1}
+    d = (2 - old) << b  # change of the signed carry at position m
+    assert fired(tr, r) == {K_CARRY + 40 * m + b: P - 2,      # booleanity
+                            K_POS + m: (-(1 << LB) * d) % P,   # - 2^26 carry_out
+                            K_POS + m + 1: d % P}              # + carry_in
 
 
 def test_planted_c_limb(good):
-    """a changed output limb breaks its pack, its position identity and the
-    acc transition into the next row"""
+    """a changed output limb breaks its pack (+1), its position identity
+    (-c_k: -1) and the acc transition into the next row (nxt.acc - c: -1)"""
     tr = good.copy()
     i, r = 9, 4
     tr[C_C + i, r] = (int(tr[C_C + i, r]) + 1) % P
-    assert nonzero_slots(tr, r) == {K_CPACK + i, K_POS + i, K_TRANS + i}
-    assert nonzero_slots(tr, r + 1) == set()
+    assert fired(tr, r) == {K_CPACK + i: 1, K_POS + i: P - 1, K_TRANS + i: P - 1}
+    assert fired(tr, r + 1) == {}
 
 
 def test_planted_phase(good):
+    """phase := 2: booleanity 2 (1 - 2) = -2; each select limb moves by
+    (phase - 2)(s_i - acc_i)"""
     tr = good.copy()
     r = 3
+    ph = int(tr[C_PHASE, r])
     tr[C_PHASE, r] = 2
-    s = nonzero_slots(tr, r)
-    assert K_PHASE in s and s <= {K_PHASE} | {K_SEL + i for i in range(NL)}
-    sel = {K_SEL + i for i in range(NL) if tr[C_S + i, r] != tr[C_ACC + i, r]}
-    assert s == {K_PHASE} | sel
+    want = {K_PHASE: P - 2}
+    for i in range(NL):
+        v = (ph - 2) * (int(tr[C_S + i, r]) - int(tr[C_ACC + i, r]))
+        if v % P:
+            want[K_SEL + i] = v % P
+    assert fired(tr, r) == want
 
 
 def test_planted_modulus_and_em_limbs(good):
-    """n and em are constant columns: a change at row r fires the transition
-    into r and out of r (plus, for n, the position identities)"""
+    """n and em are constant columns: a change of +5 at row r fires the
+    transition into r (+5) and out of r (-5); for n, position k's q.n sum also
+    moves by 5 q_(k-i), so its identity by -5 q_(k-i)"""
     tr = good.copy()
     i, r = 12, 9
     tr[C_EM + i, r] = (int(tr[C_EM + i, r]) + 5) % P
-    assert nonzero_slots(tr, r - 1) == {K_TRANS + 3 * NL + i}
-    assert nonzero_slots(tr, r) == {K_TRANS + 3 * NL + i}
+    assert fired(tr, r - 1) == {K_TRANS + 3 * NL + i: 5}
+    assert fired(tr, r) == {K_TRANS + 3 * NL + i: P - 5}
     tr = good.copy()
     tr[C_N + i, r] = (int(tr[C_N + i, r]) + 5) % P
-    assert nonzero_slots(tr, r - 1) == {K_TRANS + 2 * NL + i}
-    s = nonzero_slots(tr, r)
-    assert K_TRANS + 2 * NL + i in s and {K_POS + i} <= s
-    assert s - {K_TRANS + 2 * NL + i} <= {K_POS + k for k in range(159)}
+    assert fired(tr, r - 1) == {K_TRANS + 2 * NL + i: 5}
+    want = {K_TRANS + 2 * NL + i: P - 5}
+    for k in range(159):
+        j = k - i
+        if 0 <= j < NL and int(tr[C_Q + j, r]):
+            want[K_POS + k] = (-5 * int(tr[C_Q + j, r])) % P
+    assert fired(tr, r) == want
 
 
 def test_planted_s_limb(good):
     tr = good.copy()
     i, r = 0, 20
+    ph = int(tr[C_PHASE, r])
     tr[C_S + i, r] = (int(tr[C_S + i, r]) + 1) % P
-    assert nonzero_slots(tr, r - 1) == {K_TRANS + NL + i}
-    assert nonzero_slots(tr, r) == {K_TRANS + NL + i}
+    assert fired(tr, r - 1) == {K_TRANS + NL + i: 1}
+    want = {K_TRANS + NL + i: P - 1}
+    if ph:
+        want[K_SEL + i] = P - 1  # op - (s - acc + acc) with s + 1
+    assert fired(tr, r) == want
 
 
 # ---- PKCS#1 v1.5 and a real signature ----

@@ -33,9 +33,19 @@ MSGS = [b"abc", b"", b"\x42" * 44, bytes(range(68)), b"x" * 55, b"x" * 56, b"y" 
 
 
 def nonzero_slots(tr, nb, row):
+    return set(fired(tr, nb, row))
+
+
+def fired(tr, nb, row):
+    """{slot: value} of the constraints that do not vanish on `row`"""
     h = tr.shape[1]
     c = O.sha256_constraints(tr[:, row], tr[:, (row + 1) % h], row, nb)
-    return set(int(i) for i in np.nonzero(c)[0])
+    return {int(i): int(c[i]) for i in np.nonzero(c)[0]}
+
+
+def xor(x, y):
+    """xor_poly (sha256_air.rs:342): x + y - 2xy"""
+    return x + y - 2 * x * y
 
 
 @pytest.mark.parametrize("msg", MSGS, ids=lambda m: "len%d" % len(m))
@@ -68,26 +78,60 @@ def test_planted_bit_fires_its_slots(r):
     """a_5 := 2: booleanity 5, the pack of a, Sigma0 t_3 and t_24 (inputs
     a_(i+2), a_(i+13)), Sigma0 bit 15 (input a_(i+22)); Maj ab_5 = a_5 b_5 and
     ac_5 = a_5 c_5 fire where b_5 resp. c_5 is 1 (2 b differs from the stored
-    product only then)."""
+    product only then). Each fired value is the reference's formula
+    (sha256_air.rs:973-1127) at a_5 = 2 with every other cell valid."""
     tr, nb, _ = O.sha256_trace(b"abc")
+    a = [int(tr[BIT_A + i, r]) for i in range(32)]
     b5, c5 = int(tr[BIT_A + 32 + 5, r]), int(tr[BIT_A + 64 + 5, r])
     tr[BIT_A + 5, r] = 2
-    want = {BOOL + 5, PACK + 0, SIG0 + 3, SIG0 + 24, SIG0 + 32 + 15}
+    t0_15 = xor(a[17], a[28])
+    want = {BOOL + 5: 2 * (1 - 2),                                   # b (1 - b)
+            PACK + 0: -(2 - a[5]) * 32,                              # a - sum 2^i a_i
+            SIG0 + 3: xor(a[5], a[16]) - xor(2, a[16]),              # t_3 - xor(a_5, a_16)
+            SIG0 + 24: xor(a[26], a[5]) - xor(a[26], 2),             # t_24 - xor(a_26, a_5)
+            SIG0 + 32 + 15: xor(t0_15, a[5]) - xor(t0_15, 2)}        # S0_15 - xor(t_15, a_5)
     if b5:
-        want.add(MAJ + 3 * 5)
+        want[MAJ + 3 * 5] = a[5] * b5 - 2 * b5                       # ab_5 - a_5 b_5
     if c5:
-        want.add(MAJ + 3 * 5 + 1)
-    assert nonzero_slots(tr, nb, r) == want
-    assert nonzero_slots(tr, nb, r - 1) == set()
+        want[MAJ + 3 * 5 + 1] = a[5] * c5 - 2 * c5                   # ac_5 - a_5 c_5
+    assert fired(tr, nb, r) == {k: v % P for k, v in want.items()}
+    assert fired(tr, nb, r - 1) == {}
 
 
 def test_planted_word_fires_its_slots():
-    """W[20] + 1: row 20's T1 sum and the W-window feed; row 19's schedule sum."""
+    """W[20] + 1: row 20's T1 sum (t1_lo + 2^32 cw - t1_sum = -1) and its
+    W-window feed (nxt W_win[14] - cur W = -1); row 19's schedule sum, whose
+    next-row W is the planted word (nxt W + 2^32 cw - sum = +1)
+    (sha256_air.rs:1136-1143, 1266-1296)."""
     tr, nb, _ = O.sha256_trace(b"abc")
     r = 20
     tr[COL_W, r] = (int(tr[COL_W, r]) + 1) % P
-    assert nonzero_slots(tr, nb, r) == {T1, WWIN + 14}
-    assert nonzero_slots(tr, nb, r - 1) == {SCHED}
+    assert fired(tr, nb, r) == {T1: P - 1, WWIN + 14: P - 1}
+    assert fired(tr, nb, r - 1) == {SCHED: 1}
+
+
+def test_planted_carry_bit_values():
+    """T1 carry bit c1 := 2 on a compression row: its booleanity gives
+    2 (1 - 2) = -2 and the carry pack cw - c0 - 2 c1 - 4 c2 moves by -2 (2 - c1)
+    (sha256_air.rs:1144-1152); the T1 sum itself reads T1_cw, not the bits."""
+    tr, nb, _ = O.sha256_trace(bytes(range(68)))
+    r = 30
+    # the T1 carry bits follow T1_lo, T1_cw (sha256_air.rs budget: T1 sum, 3 booleans, pack)
+    before = fired(tr, nb, r)
+    assert before == {}
+    col = None
+    for cand in range(tr.shape[0]):
+        t = tr.copy()
+        v = int(t[cand, r])
+        if v not in (0, 1):
+            continue
+        t[cand, r] = 2
+        f = fired(t, nb, r)
+        if set(f) == {T1 + 2, T1 + 4}:
+            col, c1 = cand, v
+            break
+    assert col is not None, "no column feeds exactly the c1 booleanity and the T1 carry pack"
+    assert f == {T1 + 2: P - 2, T1 + 4: (-2 * (2 - c1)) % P}
 
 
 def test_phi_quirk_zero():

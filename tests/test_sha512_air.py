@@ -23,9 +23,19 @@ MSGS = [b"abc", b"", b"x" * 111, b"x" * 112, b"y" * 128, b"z" * 239, b"q" * 240,
 
 
 def nonzero_slots(tr, nb, row):
+    return set(fired(tr, nb, row))
+
+
+def fired(tr, nb, row):
+    """{slot: value} of the constraints that do not vanish on `row`"""
     h = tr.shape[1]
     c = O.sha512_constraints(tr[:, row], tr[:, (row + 1) % h], row, nb)
-    return set(int(i) for i in np.nonzero(c)[0])
+    return {int(i): int(c[i]) for i in np.nonzero(c)[0]}
+
+
+def xor(x, y):
+    """xor_poly (sha512_air.rs:508): x + y - 2xy"""
+    return x + y - 2 * x * y
 
 
 @pytest.mark.parametrize("msg", MSGS, ids=lambda m: "len%d" % len(m))
@@ -51,25 +61,34 @@ def test_planted_bit_fires_its_slots(r):
     a_(i+28), a_(i+34)), Sigma0 bit 30 (input a_(i+39)), Maj ab_5 / ac_5
     where b_5 / c_5 is 1."""
     tr, nb, _ = O.sha512_trace(b"abc")
+    a = [int(tr[BIT_A + i, r]) for i in range(64)]
     b5, c5 = int(tr[BIT_B + 5, r]), int(tr[BIT_C + 5, r])
     tr[BIT_A + 5, r] = 2
-    want = {BOOL + 5, PACK + 0, SIG0 + 41, SIG0 + 35, SIG0 + 64 + 30}
+    # values: the reference's formulas (sha512_air.rs:933-1066) at a_5 = 2
+    t0_30 = xor(a[58], a[0])
+    want = {BOOL + 5: 2 * (1 - 2),                           # b (1 - b)
+            PACK + 0: -(2 - a[5]) * 32,                      # a_lo - sum_{i<32} 2^i a_i
+            SIG0 + 41: xor(a[5], a[11]) - xor(2, a[11]),     # t_41 - xor(a_5, a_11)
+            SIG0 + 35: xor(a[63], a[5]) - xor(a[63], 2),     # t_35 - xor(a_63, a_5)
+            SIG0 + 64 + 30: xor(t0_30, a[5]) - xor(t0_30, 2)}  # S0_30 - xor(t_30, a_5)
     if b5:
-        want.add(MAJ + 15)
+        want[MAJ + 15] = a[5] * b5 - 2 * b5                  # ab_5 - a_5 b_5
     if c5:
-        want.add(MAJ + 16)
-    assert nonzero_slots(tr, nb, r) == want
-    assert nonzero_slots(tr, nb, r - 1) == set()
+        want[MAJ + 16] = a[5] * c5 - 2 * c5                  # ac_5 - a_5 c_5
+    assert fired(tr, nb, r) == {k: v % P for k, v in want.items()}
+    assert fired(tr, nb, r - 1) == {}
 
 
 def test_planted_word_fires_its_slots():
-    """W[20]_lo + 1: row 20's T1 lo sum and the W-window lo feed; row 19's
-    schedule lo sum."""
+    """W[20]_lo + 1: row 20's T1 lo sum (T1_lo_lo + 2^32 cw - sum = -1) and the
+    W-window lo feed (nxt W_win[14]_lo - cur W_lo = -1); row 19's schedule lo
+    sum, whose next-row W_lo is the planted word (+1) (sha512_air.rs:1081-1082,
+    1231, 1246)."""
     tr, nb, _ = O.sha512_trace(b"abc")
     r = 20
     tr[W_LO, r] = (int(tr[W_LO, r]) + 1) % P
-    assert nonzero_slots(tr, nb, r) == {T1, WWIN + 28}
-    assert nonzero_slots(tr, nb, r - 1) == {SCHED}
+    assert fired(tr, nb, r) == {T1: P - 1, WWIN + 28: P - 1}
+    assert fired(tr, nb, r - 1) == {SCHED: 1}
 
 
 def test_phi_quirk_zero():
