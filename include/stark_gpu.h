@@ -208,8 +208,8 @@ int stark_lde_columns(stark_ctx *ctx, const uint64_t *cols, size_t w, size_t n_t
 
 /* deep_ali_merge_general (crates/deep_ali/src/lib.rs:343-437) for
  * air_id = STARK_AIR_HASH_ROLLUP (w 4, k 3) or STARK_AIR_NSEC3_CHAIN (w 8, k 4); coeffs are the k combination coefficients
- * (the reference passes lambda_j = j+1, prover.rs:100-102; only that choice
- * is accepted). lde is w x n0, c_eval receives n0 values. */
+ * (any values; the reference's provers pass lambda_j = j+1,
+ * prover.rs:100-102). lde is w x n0, c_eval receives n0 values. */
 int stark_merge(stark_ctx *ctx, int air_id, const uint64_t *lde, size_t w, size_t n_trace, size_t blowup,
                 const uint64_t *coeffs, size_t k, int phi_mode, uint64_t *c_eval);
 
@@ -249,8 +249,8 @@ int stark_deep_fri_prove_batch(stark_ctx *ctx, const uint64_t *f0s, size_t n_pro
                                uint8_t **proofs, size_t *proof_lens);
 
 /* deep_ali_merge_sha256 (crates/deep_ali/src/lib.rs:453-545): lde is the
- * 756 x n0 SHA-256 AIR LDE, coeffs the 766 combination coefficients (only
- * lambda_j = j+1, prover.rs:100-102, is accepted). */
+ * 756 x n0 SHA-256 AIR LDE, coeffs the 766 combination coefficients (any
+ * values; the provers use lambda_j = j+1, prover.rs:100-102). */
 int stark_merge_sha256(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_blocks,
                        const uint64_t *coeffs, size_t k, int phi_mode, uint64_t *c_eval);
 
@@ -272,9 +272,21 @@ int stark_prove_ds_ksk_binding_batch(stark_ctx *ctx, const uint8_t *dnskeys, con
                                      int ldt, int phi_mode, stark_ds_ksk_out *outs);
 
 /* deep_ali_merge_sha512 (crates/deep_ali/src/lib.rs:561-653): lde is the
- * 1510 x n0 SHA-512 AIR LDE, coeffs the 1526 coefficients (lambda_j = j+1). */
+ * 1510 x n0 SHA-512 AIR LDE, coeffs the 1526 coefficients (any values). */
 int stark_merge_sha512(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_blocks,
                        const uint64_t *coeffs, size_t k, int phi_mode, uint64_t *c_eval);
+/* deep_ali_merge_sha256 / _sha512 (alg 256 / 512) with the LDE (width x n0)
+ * and c_eval (n0) in device memory; coeffs in host memory. */
+int stark_merge_sha_dev(stark_ctx *ctx, int alg, const uint64_t *d_lde, size_t n_trace, size_t blowup,
+                        size_t n_blocks, const uint64_t *coeffs, size_t k, int phi_mode, uint64_t *d_c_eval);
+/* The same c_eval as stark_merge_sha_dev on lde_trace_columns(trace, blowup)
+ * (trace_import.rs:251-281), computed from the width x n_trace trace in
+ * device memory without that LDE: the provers' route (gate-group sums on a
+ * blowup-2 LDE, interpolated to n0 points). blowup >= 4. No counterpart in
+ * the reference, which always builds the LDE first. */
+int stark_merge_sha_from_trace_dev(stark_ctx *ctx, int alg, const uint64_t *d_trace, size_t n_trace, size_t blowup,
+                                   size_t n_blocks, const uint64_t *coeffs, size_t k, int phi_mode,
+                                   uint64_t *d_c_eval);
 
 /* SHA-256 (algs[i] = 256) or SHA-512 (512) AIR proofs over n messages with
  * the prove_ds_ksk_binding pipeline (trace -> LDE x32 -> merge -> STIR with
@@ -293,7 +305,8 @@ int stark_prove_sha_batch(stark_ctx *ctx, const int *algs, const uint8_t *msgs, 
  * 1827-1944; n_records = 1) and deep_ali_merge_rsa_stacked_streaming
  * (lib.rs:1949-2049; n_records stacked RSA-2048 chains): lde is the
  * (11041 n_records) x n0 LDE, coeffs the 11280 n_records coefficients
- * (lambda_j = j+1 only). The streaming merges always evaluate phi. */
+ * (any values; lambda_j = j+1 takes the regrouped fast path, others a
+ * term-by-term one). The streaming merges always evaluate phi. */
 int stark_merge_rsa_exp(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_records,
                         const uint64_t *coeffs, size_t k, uint64_t *c_eval);
 
@@ -312,7 +325,8 @@ int stark_prove_rsa_exp_batch(stark_ctx *ctx, const uint8_t *n_be, const uint8_t
 
 /* deep_ali_merge_ecdsa_double_multirow_streaming (crates/deep_ali/src/lib.rs:
  * 1421-1540): lde is the 151334 x n0 LDE of a double-chain trace, coeffs the
- * 155774 coefficients (lambda_j = j+1 only). Phi is evaluated at every point. */
+ * 155774 coefficients (any values; lambda_j = j+1 takes the regrouped fast
+ * path, others a term-by-term one). Phi is evaluated at every point. */
 int stark_merge_ecdsa_p256(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, size_t blowup, const uint64_t *coeffs,
                            size_t k, uint64_t *c_eval);
 

@@ -97,7 +97,14 @@ void ref_quotient_from_phi(uint64_t *phi, size_t n, size_t n_trace) {
 }
 
 int ref_merge_hash_rollup(const uint64_t *lde, size_t n_trace, size_t blowup, int phi_mode, uint64_t *c_eval) {
+    return ref_merge_hash_rollup_lam(lde, n_trace, blowup, phi_mode, NULL, c_eval);
+}
+
+/* the same merge with combination coefficients lam[0 .. 3) (NULL: k + 1) */
+int ref_merge_hash_rollup_lam(const uint64_t *lde, size_t n_trace, size_t blowup, int phi_mode, const uint64_t *lam,
+                              uint64_t *c_eval) {
     size_t n = n_trace * blowup;
+    const uint64_t l0 = lam ? lam[0] : 1, l1 = lam ? lam[1] : 2, l2 = lam ? lam[2] : 3;
     int compute = phi_mode == 0 || (phi_mode == 1 && n >= 4096);
     if (compute) {
         const uint64_t *idx = lde, *leaf = lde + n, *st = lde + 2 * n, *sq = lde + 3 * n;
@@ -106,7 +113,7 @@ int ref_merge_hash_rollup(const uint64_t *lde, size_t n_trace, size_t blowup, in
             uint64_t c0 = gl_sub(gl_sub(idx[j], idx[i]), 1);
             uint64_t c1 = gl_sub(sq[i], gl_mul(st[i], st[i]));
             uint64_t c2 = gl_sub(gl_sub(st[j], sq[i]), leaf[i]);
-            c_eval[i] = gl_add(gl_add(c0, gl_mul(2, c1)), gl_mul(3, c2));
+            c_eval[i] = gl_add(gl_add(gl_mul(l0, c0), gl_mul(l1, c1)), gl_mul(l2, c2));
         }
     } else {
         memset(c_eval, 0, 8 * n);
@@ -324,6 +331,12 @@ void ref_nsec3_constraints(const uint64_t cur[8], const uint64_t nxt[8], uint64_
 }
 
 int ref_merge_nsec3(const uint64_t *lde, size_t n_trace, size_t blowup, int phi_mode, uint64_t *c_eval) {
+    return ref_merge_nsec3_lam(lde, n_trace, blowup, phi_mode, NULL, c_eval);
+}
+
+/* the same merge with combination coefficients lam[0 .. 4) (NULL: k + 1) */
+int ref_merge_nsec3_lam(const uint64_t *lde, size_t n_trace, size_t blowup, int phi_mode, const uint64_t *lam,
+                        uint64_t *c_eval) {
     const size_t n = n_trace * blowup;
     const int compute = phi_mode == 0 || (phi_mode == 1 && n >= 4096);
     for (size_t i = 0; i < n; i++) {
@@ -336,7 +349,7 @@ int ref_merge_nsec3(const uint64_t *lde, size_t n_trace, size_t blowup, int phi_
                 nxt[c] = lde[c * n + j];
             }
             ref_nsec3_constraints(cur, nxt, cv);
-            for (int k = 0; k < 4; k++) acc = gl_add(acc, gl_mul((uint64_t)(k + 1), cv[k]));
+            for (int k = 0; k < 4; k++) acc = gl_add(acc, gl_mul(lam ? lam[k] : (uint64_t)(k + 1), cv[k]));
         }
         c_eval[i] = acc;
     }

@@ -45,6 +45,15 @@ void ref_build_hash_rollup_trace(size_t n_trace, const uint64_t *leaves, size_t 
 /* phi_mode: 0 = always compute phi (non-parallel build), 1 = rayon with >1 thread
  * (phi == 0 when n0 < 4096), 2 = rayon with 1 thread (phi always 0). */
 int ref_merge_hash_rollup(const uint64_t *lde, size_t n_trace, size_t blowup, int phi_mode, uint64_t *c_eval);
+/* the merges with caller coefficients (NULL: lambda_j = j + 1) */
+int ref_merge_hash_rollup_lam(const uint64_t *lde, size_t n_trace, size_t blowup, int phi_mode, const uint64_t *lam,
+                              uint64_t *c_eval);
+int ref_merge_nsec3_lam(const uint64_t *lde, size_t n_trace, size_t blowup, int phi_mode, const uint64_t *lam,
+                        uint64_t *c_eval);
+int ref_merge_sha256_lam(const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_blocks, int phi_mode,
+                         const uint64_t *lam, uint64_t *c_eval);
+int ref_merge_sha512_lam(const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_blocks, int phi_mode,
+                         const uint64_t *lam, uint64_t *c_eval);
 
 /* ---- Merkle (crates/merkle/src/lib.rs) ---- */
 /* root of a tree with constant arity, leaves of `vals_per_leaf` base values */

@@ -322,6 +322,7 @@ typedef struct {
     const uint64_t *lde;
     size_t n, blowup, n_blocks;
     uint64_t *c_eval;
+    const uint64_t *lam; /* NULL: lambda_k = k + 1 */
 } sha256_merge_job;
 static void sha256_merge_range(void *ctx, size_t lo, size_t hi) {
     const sha256_merge_job *J = (const sha256_merge_job *)ctx;
@@ -336,7 +337,7 @@ static void sha256_merge_range(void *ctx, size_t lo, size_t hi) {
         }
         ref_sha256_constraints(cur, nxt, i / J->blowup, J->n_blocks, cv);
         uint64_t acc = 0;
-        for (size_t k = 0; k < SHA_K; k++) acc = gl_add(acc, gl_mul(k + 1, cv[k]));
+        for (size_t k = 0; k < SHA_K; k++) acc = gl_add(acc, gl_mul(J->lam ? J->lam[k] : k + 1, cv[k]));
         J->c_eval[i] = acc;
     }
     free(cur);
@@ -345,10 +346,16 @@ static void sha256_merge_range(void *ctx, size_t lo, size_t hi) {
 
 int ref_merge_sha256(const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_blocks, int phi_mode,
                      uint64_t *c_eval) {
+    return ref_merge_sha256_lam(lde, n_trace, blowup, n_blocks, phi_mode, NULL, c_eval);
+}
+
+/* the same merge with combination coefficients lam[0 .. K) (NULL: k + 1) */
+int ref_merge_sha256_lam(const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_blocks, int phi_mode,
+                         const uint64_t *lam, uint64_t *c_eval) {
     const size_t n = n_trace * blowup;
     const int compute = phi_mode == 0 || (phi_mode == 1 && n >= 4096);
     if (compute) {
-        sha256_merge_job J = {lde, n, blowup, n_blocks, c_eval};
+        sha256_merge_job J = {lde, n, blowup, n_blocks, c_eval, lam};
         ref_par_for(n, PAR_FRI, sha256_merge_range, &J); /* par_iter_mut at >= 2^12 points (lib.rs:493-497) */
     } else {
         memset(c_eval, 0, 8 * n);

@@ -67,6 +67,8 @@ from .api import (  # noqa: F401
     build_sha256_trace_multi,
     deep_ali_merge_sha512,
     build_sha512_trace_multi,
+    merge_sha_dev,
+    merge_sha_from_trace_dev,
     prove_sha_batch,
     Nsec3Record,
     Nsec3Output,

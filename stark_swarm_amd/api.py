@@ -136,6 +136,9 @@ def lib():
         "stark_set_autotune": (ctypes.c_int, [vp, ctypes.c_int]),
         "stark_set_ntt_split": (ctypes.c_int, [vp, ctypes.c_uint]),
         "stark_set_ntt_tile": (ctypes.c_int, [vp, ctypes.c_uint]),
+        "stark_merge_sha_dev": (ctypes.c_int, [vp, ctypes.c_int, vp, sz, sz, sz, u64p, sz, ctypes.c_int, vp]),
+        "stark_merge_sha_from_trace_dev": (ctypes.c_int, [vp, ctypes.c_int, vp, sz, sz, sz, u64p, sz, ctypes.c_int,
+                                                          vp]),
         "stark_autotune_report": (sz, [ctypes.c_char_p, sz]),
         "stark_host_sha3_256": (None, [ctypes.c_char_p, sz, ctypes.c_char_p]),
         "stark_host_stdrng_first_u64": (ctypes.c_uint64, [ctypes.c_uint64]),
@@ -215,10 +218,11 @@ class DeviceBuffer:
         self.ctx._check(lib().stark_memcpy_h2d(self.ctx.h, self.ptr, arr.ctypes.data, arr.nbytes))
         return self
 
-    def download(self, shape, dtype=np.uint64):
+    def download(self, shape, dtype=np.uint64, offset=0):
+        """`shape` values from byte `offset` of the buffer"""
         out = np.empty(shape, dtype=dtype)
-        assert out.nbytes <= self.nbytes
-        self.ctx._check(lib().stark_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes))
+        assert offset >= 0 and offset + out.nbytes <= self.nbytes
+        self.ctx._check(lib().stark_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr + offset, out.nbytes))
         return out
 
     def free(self):
@@ -503,6 +507,26 @@ def deep_ali_merge_sha512(trace_evals_on_lde, combination_coeffs, omega, n_trace
     ctx._check(lib().stark_merge_sha512(ctx.h, lp, n_trace, blowup, n_blocks, cp, SHA512_CONSTRAINTS, phi_mode,
                                         out.ctypes.data_as(u64p)))
     return out, CompositionInfo(2 * n_trace, n_trace, n_trace / n, SHA512_CONSTRAINTS, 2, SHA512_WIDTH)
+
+
+def merge_sha_dev(alg, d_lde, n_trace, blowup, n_blocks, combination_coeffs, d_out, phi_mode=PHI_RAYON, ctx=None):
+    """deep_ali_merge_sha256 / _sha512 (alg 256 / 512) on device buffers
+    (stark_merge_sha_dev): d_lde the width x n0 LDE, d_out receives n0 values."""
+    ctx = ctx or default_context()
+    co, cp = _u64(np.asarray(combination_coeffs, dtype=np.uint64))
+    ctx._check(lib().stark_merge_sha_dev(ctx.h, int(alg), d_lde.ptr, n_trace, blowup, n_blocks, cp, len(co),
+                                         phi_mode, d_out.ptr))
+
+
+def merge_sha_from_trace_dev(alg, d_trace, n_trace, blowup, n_blocks, combination_coeffs, d_out, phi_mode=PHI_RAYON,
+                             ctx=None):
+    """The same c_eval as merge_sha_dev on the trace's LDE, computed from the
+    width x n_trace trace (device) without that LDE -- the provers' route
+    (stark_merge_sha_from_trace_dev). blowup >= 4."""
+    ctx = ctx or default_context()
+    co, cp = _u64(np.asarray(combination_coeffs, dtype=np.uint64))
+    ctx._check(lib().stark_merge_sha_from_trace_dev(ctx.h, int(alg), d_trace.ptr, n_trace, blowup, n_blocks, cp,
+                                                    len(co), phi_mode, d_out.ptr))
 
 
 @dataclass

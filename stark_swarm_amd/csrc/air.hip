@@ -22,8 +22,23 @@ namespace {
 constexpr int AT = 256;
 
 // grid.y = proof: lde of proof k at lde + 4 k n, phi at phi + k n
+// combination coefficients of an AIR with at most 4 constraints, passed by
+// value; on = 0: lambda_j = j + 1
+struct Lam4 {
+    uint64_t v[4];
+    uint32_t on;
+};
+Lam4 lam4(const uint64_t *lam, uint32_t k) {
+    Lam4 l{};
+    if (lam) {
+        l.on = 1;
+        for (uint32_t j = 0; j < k && j < 4; j++) l.v[j] = canon(lam[j]);
+    }
+    return l;
+}
+
 __global__ __launch_bounds__(AT) void compose_hash_rollup_k(const uint64_t *lde, uint64_t n, uint64_t blowup,
-                                                            uint64_t *phi) {
+                                                            uint64_t *phi, Lam4 lam) {
     const uint64_t i = (uint64_t)blockIdx.x * AT + threadIdx.x;
     if (i >= n) return;
     lde += 4 * n * blockIdx.y;
@@ -34,14 +49,16 @@ __global__ __launch_bounds__(AT) void compose_hash_rollup_k(const uint64_t *lde,
     const uint64_t c0 = sub(sub(idx[j], idx[i]), 1);
     const uint64_t c1 = sub(q, mul(s, s));
     const uint64_t c2 = sub(sub(st[j], q), leaf[i]);
-    // 1*c0 + 2*c1 + 3*c2
-    phi[i] = add(add(c0, add(c1, c1)), add(add(c2, c2), c2));
+    if (lam.on)
+        phi[i] = add(add(mul(lam.v[0], c0), mul(lam.v[1], c1)), mul(lam.v[2], c2));
+    else  // 1*c0 + 2*c1 + 3*c2
+        phi[i] = add(add(c0, add(c1, c1)), add(add(c2, c2), c2));
 }
 
 // AirType::Nsec3Chain (air_workloads.rs:789-870): 8 columns (owner limbs,
 // next limbs), C_k = nxt[k] - cur[4 + k], phi = sum (k + 1) C_k
 __global__ __launch_bounds__(AT) void compose_nsec3_k(const uint64_t *lde, uint64_t n, uint64_t blowup,
-                                                     uint64_t *phi) {
+                                                     uint64_t *phi, Lam4 lam) {
     const uint64_t i = (uint64_t)blockIdx.x * AT + threadIdx.x;
     if (i >= n) return;
     lde += 8 * n * blockIdx.y;
@@ -51,7 +68,10 @@ __global__ __launch_bounds__(AT) void compose_nsec3_k(const uint64_t *lde, uint6
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const uint64_t d = sub(lde[k * n + j], lde[(4 + k) * n + i]);
-        for (int t = 0; t <= k; t++) acc = add(acc, d);
+        if (lam.on)
+            acc = add(acc, mul(lam.v[k], d));
+        else
+            for (int t = 0; t <= k; t++) acc = add(acc, d);
     }
     phi[i] = acc;
 }
@@ -87,7 +107,7 @@ void fill_u64(Ctx *c, uint64_t *p, uint64_t n, uint64_t v) {
 bool phi_computed(int phi_mode, uint64_t n) { return phi_mode == 0 || (phi_mode == 1 && n >= 4096); }
 
 void merge_hash_rollup_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup,
-                             int phi_mode, uint64_t *c_eval) {
+                             int phi_mode, uint64_t *c_eval, const uint64_t *lam) {
     SG_REQUIRE(is_pow2(n_trace) && is_pow2(blowup) && blowup >= 2, "merge: sizes must be powers of two");
     SG_REQUIRE(K >= 1 && K <= 65535, "merge: batch size out of range");
     const uint64_t n = n_trace * blowup;
@@ -101,14 +121,14 @@ void merge_hash_rollup_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n
     {
         ProfScope ps(c, "compose_hash_rollup", (8.0 * 4 * n + 8.0 * n) * K);
         hipLaunchKernelGGL(compose_hash_rollup_k, dim3((uint32_t)((n + AT - 1) / AT), K), dim3(AT), 0, c->stream, lde,
-                           n, blowup, phi.as<uint64_t>());
+                           n, blowup, phi.as<uint64_t>(), lam4(lam, 3));
         SG_HIP(hipGetLastError());
     }
     quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
 }
 
 void merge_nsec3_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup, int phi_mode,
-                       uint64_t *c_eval) {
+                       uint64_t *c_eval, const uint64_t *lam) {
     SG_REQUIRE(is_pow2(n_trace) && is_pow2(blowup) && blowup >= 2, "merge: sizes must be powers of two");
     SG_REQUIRE(K >= 1 && K <= 65535, "merge: batch size out of range");
     const uint64_t n = n_trace * blowup;
@@ -121,7 +141,7 @@ void merge_nsec3_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace
     {
         ProfScope ps(c, "compose_nsec3", (8.0 * 8 * n + 8.0 * n) * K);
         hipLaunchKernelGGL(compose_nsec3_k, dim3((uint32_t)((n + AT - 1) / AT), K), dim3(AT), 0, c->stream, lde, n,
-                           blowup, phi.as<uint64_t>());
+                           blowup, phi.as<uint64_t>(), lam4(lam, 4));
         SG_HIP(hipGetLastError());
     }
     quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
@@ -157,8 +177,8 @@ void quotient_batch(Ctx *c, uint64_t *phi, uint32_t K, uint64_t n_trace, uint64_
 }
 
 void merge_hash_rollup_dev(Ctx *c, const uint64_t *lde, uint64_t n_trace, uint64_t blowup, int phi_mode,
-                           uint64_t *c_eval) {
-    merge_hash_rollup_batch(c, lde, 1, n_trace, blowup, phi_mode, c_eval);
+                           uint64_t *c_eval, const uint64_t *lam) {
+    merge_hash_rollup_batch(c, lde, 1, n_trace, blowup, phi_mode, c_eval, lam);
 }
 
 // HashRollup trace (air_workloads.rs:571-589): idx, leaf, state, state^2 with

@@ -135,6 +135,14 @@ int stark_lde_columns(stark_ctx *ctx, const uint64_t *cols, size_t w, size_t n_t
     });
 }
 
+// true when coeffs are the reference's own lambda_j = j + 1 (the kernels'
+// built-in weights); otherwise the caller's table goes to the device
+static bool reference_lambdas(const uint64_t *coeffs, size_t k) {
+    for (size_t j = 0; j < k; j++)
+        if (coeffs[j] != j + 1) return false;
+    return true;
+}
+
 int stark_merge(stark_ctx *ctx, int air_id, const uint64_t *lde, size_t w, size_t n_trace, size_t blowup,
                 const uint64_t *coeffs, size_t k, int phi_mode, uint64_t *c_eval) {
     return guard(ctx, [&](sg::Ctx *c) {
@@ -142,16 +150,15 @@ int stark_merge(stark_ctx *ctx, int air_id, const uint64_t *lde, size_t w, size_
             throw sg::Error(sg::E_UNSUPPORTED, "merge: only the HashRollup and Nsec3Chain AIRs");
         const size_t ww = air_id == STARK_AIR_HASH_ROLLUP ? 4 : 8, kk = air_id == STARK_AIR_HASH_ROLLUP ? 3 : 4;
         SG_REQUIRE(w == ww && k == kk && lde && coeffs && c_eval, "merge: wrong trace width or constraint count");
-        for (size_t j = 0; j < k; j++)
-            SG_REQUIRE(coeffs[j] == j + 1, "merge: only the reference combination coefficients (j + 1)");
         SG_REQUIRE(phi_mode >= 0 && phi_mode <= 2, "merge: bad phi_mode");
+        const uint64_t *lam = reference_lambdas(coeffs, k) ? nullptr : coeffs;
         const size_t n0 = n_trace * blowup;
         sg::Scoped dl(c, 8 * ww * n0), dc(c, 8 * n0);
         h2d(c, dl.p, lde, 8 * ww * n0);
         if (air_id == STARK_AIR_HASH_ROLLUP)
-            sg::merge_hash_rollup_dev(c, dl.as<uint64_t>(), n_trace, blowup, phi_mode, dc.as<uint64_t>());
+            sg::merge_hash_rollup_dev(c, dl.as<uint64_t>(), n_trace, blowup, phi_mode, dc.as<uint64_t>(), lam);
         else
-            sg::merge_nsec3_batch(c, dl.as<uint64_t>(), 1, n_trace, blowup, phi_mode, dc.as<uint64_t>());
+            sg::merge_nsec3_batch(c, dl.as<uint64_t>(), 1, n_trace, blowup, phi_mode, dc.as<uint64_t>(), lam);
         d2h(c, c_eval, dc.p, 8 * n0);
     });
 }
@@ -233,39 +240,70 @@ int stark_prove_inner_shard_batch(stark_ctx *ctx, const uint8_t salt[16], const 
     });
 }
 
-int stark_merge_sha256(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_blocks,
-                       const uint64_t *coeffs, size_t k, int phi_mode, uint64_t *c_eval) {
+// one SHA-256 / SHA-512 merge: lde (or, from_trace, the trace) and c_eval in
+// device memory; coeffs in host memory
+static void sha_merge_dev(sg::Ctx *c, int alg, bool from_trace, const uint64_t *d_in, size_t n_trace, size_t blowup,
+                          size_t n_blocks, const uint64_t *coeffs, size_t k, int phi_mode, uint64_t *d_c_eval) {
+    const char *what = alg == 256 ? "merge_sha256" : "merge_sha512";
+    const size_t K = alg == 256 ? sg::SHA256_K : sg::SHA512_K;
+    SG_REQUIRE(d_in && coeffs && d_c_eval, std::string(what) + ": null argument");
+    SG_REQUIRE(k == K, std::string(what) + ": need one combination coefficient per constraint");
+    SG_REQUIRE(phi_mode >= 0 && phi_mode <= 2, std::string(what) + ": bad phi_mode");
+    SG_REQUIRE(n_blocks >= 1 && n_blocks <= 0xFFFFFFFFu, std::string(what) + ": bad n_blocks");
+    const bool ref = reference_lambdas(coeffs, k);
+    sg::Scoped dk(c, ref ? 8 : 8 * k);
+    if (!ref) h2d(c, dk.p, coeffs, 8 * k);
+    const uint64_t *lam = ref ? nullptr : dk.as<uint64_t>();
+    const std::vector<uint32_t> nb = {(uint32_t)n_blocks};
+    if (alg == 256 && from_trace)
+        sg::merge_sha256_from_trace_batch(c, d_in, 1, n_trace, blowup, nb, phi_mode, d_c_eval, lam);
+    else if (alg == 256)
+        sg::merge_sha256_batch(c, d_in, 1, n_trace, blowup, nb, phi_mode, d_c_eval, lam);
+    else if (from_trace)
+        sg::merge_sha512_from_trace_batch(c, d_in, 1, n_trace, blowup, nb, phi_mode, d_c_eval, lam);
+    else
+        sg::merge_sha512_batch(c, d_in, 1, n_trace, blowup, nb, phi_mode, d_c_eval, lam);
+    SG_HIP(hipStreamSynchronize(c->stream));  // the coefficient table goes out of scope
+}
+
+static int sha_merge_host(stark_ctx *ctx, int alg, const uint64_t *lde, size_t n_trace, size_t blowup,
+                          size_t n_blocks, const uint64_t *coeffs, size_t k, int phi_mode, uint64_t *c_eval) {
     return guard(ctx, [&](sg::Ctx *c) {
-        SG_REQUIRE(lde && coeffs && c_eval, "merge_sha256: null argument");
-        SG_REQUIRE(k == sg::SHA256_K, "merge_sha256: need one combination coefficient per constraint (766)");
-        for (size_t j = 0; j < k; j++)
-            SG_REQUIRE(coeffs[j] == j + 1, "merge_sha256: only the reference combination coefficients (j + 1)");
-        SG_REQUIRE(phi_mode >= 0 && phi_mode <= 2, "merge_sha256: bad phi_mode");
-        SG_REQUIRE(n_blocks >= 1 && n_blocks <= 0xFFFFFFFFu, "merge_sha256: bad n_blocks");
-        const size_t n0 = n_trace * blowup;
-        sg::Scoped dl(c, 8ULL * sg::SHA256_W * n0), dc(c, 8 * n0);
-        h2d(c, dl.p, lde, 8ULL * sg::SHA256_W * n0);
-        sg::merge_sha256_batch(c, dl.as<uint64_t>(), 1, n_trace, blowup, {(uint32_t)n_blocks}, phi_mode,
-                               dc.as<uint64_t>());
+        SG_REQUIRE(lde && c_eval, "merge_sha: null argument");
+        const size_t W = alg == 256 ? sg::SHA256_W : sg::SHA512_W, n0 = n_trace * blowup;
+        sg::Scoped dl(c, 8ULL * W * n0), dc(c, 8 * n0);
+        h2d(c, dl.p, lde, 8ULL * W * n0);
+        sha_merge_dev(c, alg, false, dl.as<uint64_t>(), n_trace, blowup, n_blocks, coeffs, k, phi_mode,
+                      dc.as<uint64_t>());
         d2h(c, c_eval, dc.p, 8 * n0);
     });
 }
 
+int stark_merge_sha256(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_blocks,
+                       const uint64_t *coeffs, size_t k, int phi_mode, uint64_t *c_eval) {
+    return sha_merge_host(ctx, 256, lde, n_trace, blowup, n_blocks, coeffs, k, phi_mode, c_eval);
+}
+
 int stark_merge_sha512(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, size_t blowup, size_t n_blocks,
                        const uint64_t *coeffs, size_t k, int phi_mode, uint64_t *c_eval) {
+    return sha_merge_host(ctx, 512, lde, n_trace, blowup, n_blocks, coeffs, k, phi_mode, c_eval);
+}
+
+int stark_merge_sha_dev(stark_ctx *ctx, int alg, const uint64_t *d_lde, size_t n_trace, size_t blowup,
+                        size_t n_blocks, const uint64_t *coeffs, size_t k, int phi_mode, uint64_t *d_c_eval) {
     return guard(ctx, [&](sg::Ctx *c) {
-        SG_REQUIRE(lde && coeffs && c_eval, "merge_sha512: null argument");
-        SG_REQUIRE(k == sg::SHA512_K, "merge_sha512: need one combination coefficient per constraint (1526)");
-        for (size_t j = 0; j < k; j++)
-            SG_REQUIRE(coeffs[j] == j + 1, "merge_sha512: only the reference combination coefficients (j + 1)");
-        SG_REQUIRE(phi_mode >= 0 && phi_mode <= 2, "merge_sha512: bad phi_mode");
-        SG_REQUIRE(n_blocks >= 1 && n_blocks <= 0xFFFFFFFFu, "merge_sha512: bad n_blocks");
-        const size_t n0 = n_trace * blowup;
-        sg::Scoped dl(c, 8ULL * sg::SHA512_W * n0), dc(c, 8 * n0);
-        h2d(c, dl.p, lde, 8ULL * sg::SHA512_W * n0);
-        sg::merge_sha512_batch(c, dl.as<uint64_t>(), 1, n_trace, blowup, {(uint32_t)n_blocks}, phi_mode,
-                               dc.as<uint64_t>());
-        d2h(c, c_eval, dc.p, 8 * n0);
+        SG_REQUIRE(alg == 256 || alg == 512, "merge_sha_dev: alg 256 or 512");
+        sha_merge_dev(c, alg, false, d_lde, n_trace, blowup, n_blocks, coeffs, k, phi_mode, d_c_eval);
+    });
+}
+
+int stark_merge_sha_from_trace_dev(stark_ctx *ctx, int alg, const uint64_t *d_trace, size_t n_trace, size_t blowup,
+                                   size_t n_blocks, const uint64_t *coeffs, size_t k, int phi_mode,
+                                   uint64_t *d_c_eval) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(alg == 256 || alg == 512, "merge_sha_from_trace_dev: alg 256 or 512");
+        SG_REQUIRE(blowup >= 4, "merge_sha_from_trace_dev: blowup >= 4");
+        sha_merge_dev(c, alg, true, d_trace, n_trace, blowup, n_blocks, coeffs, k, phi_mode, d_c_eval);
     });
 }
 
@@ -315,13 +353,14 @@ int stark_merge_rsa_exp(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, siz
         SG_REQUIRE(lde && coeffs && c_eval, "merge_rsa_exp: null argument");
         SG_REQUIRE(n_records >= 1 && n_records <= sg::RSA_MAX_RECORDS, "merge_rsa_exp: 1 to 4096 records");
         SG_REQUIRE(k == sg::RSA_K * n_records, "merge_rsa_exp: need one coefficient per constraint (11280 per record)");
-        for (size_t j = 0; j < k; j++)
-            SG_REQUIRE(coeffs[j] == j + 1, "merge_rsa_exp: only the reference combination coefficients (j + 1)");
         SG_REQUIRE(n_trace <= (1u << 20) && blowup <= (1u << 20), "merge_rsa_exp: sizes out of range");
         const size_t n0 = n_trace * blowup, w = (size_t)sg::RSA_W * n_records;
-        sg::Scoped dl(c, 8ULL * w * n0), dc(c, 8 * n0);
+        const bool ref = reference_lambdas(coeffs, k);
+        sg::Scoped dl(c, 8ULL * w * n0), dc(c, 8 * n0), dk(c, ref ? 8 : 8 * k);
         h2d(c, dl.p, lde, 8ULL * w * n0);
-        sg::merge_rsa_batch(c, dl.as<uint64_t>(), 1, (uint32_t)n_records, n_trace, blowup, dc.as<uint64_t>());
+        if (!ref) h2d(c, dk.p, coeffs, 8 * k);
+        sg::merge_rsa_batch(c, dl.as<uint64_t>(), 1, (uint32_t)n_records, n_trace, blowup, dc.as<uint64_t>(),
+                            ref ? nullptr : dk.as<uint64_t>());
         d2h(c, c_eval, dc.p, 8 * n0);
     });
 }
@@ -353,13 +392,14 @@ int stark_merge_ecdsa_p256(stark_ctx *ctx, const uint64_t *lde, size_t n_trace, 
     return guard(ctx, [&](sg::Ctx *c) {
         SG_REQUIRE(lde && coeffs && c_eval, "merge_ecdsa_p256: null argument");
         SG_REQUIRE(k == sg::P256_K, "merge_ecdsa_p256: need one coefficient per constraint (155774)");
-        for (size_t j = 0; j < k; j++)
-            SG_REQUIRE(coeffs[j] == j + 1, "merge_ecdsa_p256: only the reference combination coefficients (j + 1)");
         SG_REQUIRE(n_trace <= (1u << 16) && blowup <= (1u << 10), "merge_ecdsa_p256: sizes out of range");
         const size_t n0 = n_trace * blowup;
-        sg::Scoped dl(c, 8ULL * sg::P256_W * n0), dc(c, 8 * n0);
+        const bool ref = reference_lambdas(coeffs, k);
+        sg::Scoped dl(c, 8ULL * sg::P256_W * n0), dc(c, 8 * n0), dk(c, ref ? 8 : 8 * k);
         h2d(c, dl.p, lde, 8ULL * sg::P256_W * n0);
-        sg::merge_p256_batch(c, dl.as<uint64_t>(), 1, n_trace, blowup, dc.as<uint64_t>());
+        if (!ref) h2d(c, dk.p, coeffs, 8 * k);
+        sg::merge_p256_batch(c, dl.as<uint64_t>(), 1, n_trace, blowup, dc.as<uint64_t>(),
+                             ref ? nullptr : dk.as<uint64_t>());
         d2h(c, c_eval, dc.p, 8 * n0);
     });
 }

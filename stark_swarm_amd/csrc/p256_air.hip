@@ -729,47 +729,74 @@ struct Sums {
     }
 };
 
+// a += lambda_slot x for the 1-based slot index: slot itself (GEN = false,
+// lambda_j = j + 1) or T[slot - 1] (GEN: the caller's coefficients)
+template <bool GEN>
+__device__ __forceinline__ void wmad(Acc &a, uint64_t x, uint32_t slot, const uint64_t *T) {
+    if constexpr (GEN)
+        acc_add(a, mul(x, T[slot - 1]));
+    else
+        acc_mad(a, x, slot);
+}
+
 // lambda (x - x^2) for one boolean-constrained cell
-__device__ __forceinline__ void boolean(Sums &S, uint64_t x, uint32_t lam) {
-    acc_mad(S.pos, x, lam);
-    acc_mad(S.neg, mul(x, x), lam);
+template <bool GEN>
+__device__ __forceinline__ void boolean(Sums &S, uint64_t x, uint32_t lam, const uint64_t *T) {
+    wmad<GEN>(S.pos, x, lam, T);
+    wmad<GEN>(S.neg, mul(x, x), lam, T);
+}
+
+// sum_k lambda_{s0 + k + 1} p_k over the 10 limbs of p (a constant term of
+// the sub and freeze chains)
+template <bool GEN>
+__device__ __forceinline__ uint64_t p_limbs_weighted(uint32_t s0, const uint64_t *T) {
+    if constexpr (GEN) {
+        uint64_t v = 0;
+#pragma unroll
+        for (int k = 0; k < NL; k++) v = add(v, mul((uint64_t)PL[k], T[s0 + k]));
+        return v;
+    } else {
+        return (uint64_t)(s0 + 1) * SP + SP1;
+    }
 }
 
 // range check of one element block at slot s: 260 booleanity slots, then
 // 10 limb packings limb_i - sum_b 2^b bit_ib
+template <bool GEN>
 __device__ __forceinline__ void range(Sums &S, const uint64_t *col, uint64_t n0, uint64_t pt, uint32_t lb,
-                                      uint32_t bb, uint32_t s) {
+                                      uint32_t bb, uint32_t s, const uint64_t *T) {
 #pragma unroll 1
     for (int i = 0; i < NL; i++) {
         Acc pk{0, 0};
 #pragma unroll 8
         for (int b = 0; b < LB; b++) {
             const uint64_t x = col[(uint64_t)(bb + LB * i + b) * n0 + pt];
-            boolean(S, x, s + LB * i + b + 1);
+            boolean<GEN>(S, x, s + LB * i + b + 1, T);
             acc_shl(pk, x, b);
         }
-        acc_mad(S.pos, col[(uint64_t)(lb + i) * n0 + pt], s + EC - NL + i + 1);
-        acc_mad(S.neg, acc_val(pk), s + EC - NL + i + 1);
+        wmad<GEN>(S.pos, col[(uint64_t)(lb + i) * n0 + pt], s + EC - NL + i + 1, T);
+        wmad<GEN>(S.neg, acc_val(pk), s + EC - NL + i + 1, T);
     }
 }
 
 // signed carries (pos_k, neg_k) of a sub / freeze chain whose identities sit
 // at slots s0 + k: net_m = pos_m - neg_m enters as carry-in of m + 1 and as
 // -2^26 carry-out of m; then booleanity of pos and neg, exclusion, at s1
+template <bool GEN>
 __device__ __forceinline__ void signed_chain(Sums &S, const uint64_t *col, uint64_t n0, uint64_t pt, uint32_t pos,
-                                             uint32_t neg, uint32_t s0, uint32_t s1) {
+                                             uint32_t neg, uint32_t s0, uint32_t s1, const uint64_t *T) {
 #pragma unroll 1
     for (int m = 0; m < NL; m++) {
         const uint64_t cp = col[(uint64_t)(pos + m) * n0 + pt], cn = col[(uint64_t)(neg + m) * n0 + pt];
         if (m + 1 < NL) {
-            acc_mad(S.pos, cp, s0 + m + 2);
-            acc_mad(S.neg, cn, s0 + m + 2);
+            wmad<GEN>(S.pos, cp, s0 + m + 2, T);
+            wmad<GEN>(S.neg, cn, s0 + m + 2, T);
         }
-        acc_mad(S.rpos, cp, s0 + m + 1);
-        acc_mad(S.rneg, cn, s0 + m + 1);
-        boolean(S, cp, s1 + m + 1);
-        boolean(S, cn, s1 + NL + m + 1);
-        acc_mad(S.pos, mul(cp, cn), s1 + 2 * NL + m + 1);
+        wmad<GEN>(S.rpos, cp, s0 + m + 1, T);
+        wmad<GEN>(S.rneg, cn, s0 + m + 1, T);
+        boolean<GEN>(S, cp, s1 + m + 1, T);
+        boolean<GEN>(S, cn, s1 + NL + m + 1, T);
+        wmad<GEN>(S.pos, mul(cp, cn), s1 + 2 * NL + m + 1, T);
     }
 }
 
@@ -779,9 +806,13 @@ __device__ __forceinline__ void signed_chain(Sums &S, const uint64_t *col, uint6
 // part[(k * gridDim.y + g) * n0 + pt]. With odd set (the gadget sums on a
 // blowup-2 LDE) the even points are rows pt / 2 of lde (the traces, n0 / 2
 // rows) and the odd points rows of odd.
+// GEN: slot j weighted by T[j] (the caller's combination coefficients)
+// instead of j + 1, with the position identities and the p-limb constants
+// summed term by term (their affine-in-slot regrouping needs lambda_j = j + 1).
+template <bool GEN>
 __global__ __launch_bounds__(64) void compose_p256_k(const uint64_t *lde, const uint64_t *odd, uint64_t wstride,
                                                     uint64_t n0, uint64_t blowup, const GDesc *gd, uint32_t ng,
-                                                    TailDesc td, uint64_t *part) {
+                                                    TailDesc td, uint64_t *part, const uint64_t *T) {
     const uint32_t g = blockIdx.y;
     const uint64_t k = blockIdx.z;
     const uint64_t pt = (uint64_t)blockIdx.x * 64 + threadIdx.x;
@@ -796,18 +827,18 @@ __global__ __launch_bounds__(64) void compose_p256_k(const uint64_t *lde, const 
             for (int j = 0; j < 6; j++)
                 for (int l = 0; l < NL; l++) {
                     const uint32_t lam = td.s + NL * j + l + 1;
-                    acc_mad(S.pos, AT(td.selc[j] + l), lam);
-                    acc_mad(S.neg, AT(td.rproj[j] + l), lam);
+                    wmad<GEN>(S.pos, AT(td.selc[j] + l), lam, T);
+                    wmad<GEN>(S.neg, AT(td.rproj[j] + l), lam, T);
                 }
         } else {
 #pragma unroll 1
             for (int j = 0; j < 6; j++)
                 for (int l = 0; l < NL; l++) {
                     const uint32_t lam = td.s + 6 * NL + NL * j + l + 1, lam2 = lam + 6 * NL;
-                    acc_mad(S.pos, col[(uint64_t)(td.acc[j] + l) * n0 + np], lam);
-                    acc_mad(S.neg, AT(td.selc[j] + l), lam);
-                    acc_mad(S.pos, col[(uint64_t)(td.rproj[j] + l) * n0 + np], lam2);
-                    acc_mad(S.neg, AT(td.rproj[j] + l), lam2);
+                    wmad<GEN>(S.pos, col[(uint64_t)(td.acc[j] + l) * n0 + np], lam, T);
+                    wmad<GEN>(S.neg, AT(td.selc[j] + l), lam, T);
+                    wmad<GEN>(S.pos, col[(uint64_t)(td.rproj[j] + l) * n0 + np], lam2, T);
+                    wmad<GEN>(S.neg, AT(td.rproj[j] + l), lam2, T);
                 }
         }
     } else {
@@ -815,22 +846,40 @@ __global__ __launch_bounds__(64) void compose_p256_k(const uint64_t *lde, const 
         const uint32_t s = d.s;
         switch (d.kind) {
         case GM: {  // eval_mul_gadget (p256_field_air.rs:1044-1123)
-            range(S, col, rows, at, d.c, d.cb, s);
-            range(S, col, rows, at, d.q, d.qb, s + EC);
+            range<GEN>(S, col, rows, at, d.c, d.cb, s, T);
+            range<GEN>(S, col, rows, at, d.q, d.qb, s + EC, T);
             const uint32_t L0 = s + 2 * EC + 1;  // lambda of position 0
-            uint64_t B = 0, B1 = 0;
-#pragma unroll
-            for (int j = 0; j < NL; j++) {
-                const uint64_t bj = AT(d.b + j);
-                B = add(B, bj);
-                B1 = add(B1, mul_small(bj, j));
-            }
+            if constexpr (GEN) {
+                // position k: lambda_k (sum_{i+j=k} (a_i b_j - q_i p_j) - c_k)
 #pragma unroll 1
-            for (int i = 0; i < NL; i++) {
-                acc_add(S.pos, mul(AT(d.a + i), add(mul_small(B, L0 + i), B1)));
-                // sum_j lambda_{i+j} q_i p_j = q_i ((L0 + i) SP + SP1), below 2^50
-                acc_add(S.neg, mul(AT(d.q + i), (uint64_t)(L0 + i) * SP + SP1));
-                acc_mad(S.neg, AT(d.c + i), L0 + i);
+                for (int kk = 0; kk < MSP; kk++) {
+                    uint64_t ab = 0, qp = 0;
+#pragma unroll
+                    for (int j = 0; j < NL; j++) {
+                        const int i = kk - j;
+                        if (i < 0 || i >= NL) continue;
+                        ab = add(ab, mul(AT(d.a + i), AT(d.b + j)));
+                        qp = add(qp, mul(AT(d.q + i), (uint64_t)PL[j]));  // j unrolled: PL[j] a constant
+                    }
+                    wmad<GEN>(S.pos, ab, L0 + kk, T);
+                    wmad<GEN>(S.neg, qp, L0 + kk, T);
+                    if (kk < NL) wmad<GEN>(S.neg, AT(d.c + kk), L0 + kk, T);
+                }
+            } else {
+                uint64_t B = 0, B1 = 0;
+#pragma unroll
+                for (int j = 0; j < NL; j++) {
+                    const uint64_t bj = AT(d.b + j);
+                    B = add(B, bj);
+                    B1 = add(B1, mul_small(bj, j));
+                }
+#pragma unroll 1
+                for (int i = 0; i < NL; i++) {
+                    acc_add(S.pos, mul(AT(d.a + i), add(mul_small(B, L0 + i), B1)));
+                    // sum_j lambda_{i+j} q_i p_j = q_i ((L0 + i) SP + SP1), below 2^50
+                    acc_add(S.neg, mul(AT(d.q + i), (uint64_t)(L0 + i) * SP + SP1));
+                    acc_mad(S.neg, AT(d.c + i), L0 + i);
+                }
             }
 #pragma unroll 1
             for (int m = 0; m < MCP; m++) {
@@ -838,84 +887,89 @@ __global__ __launch_bounds__(64) void compose_p256_k(const uint64_t *lde, const 
 #pragma unroll 8
                 for (int b = 0; b < MCB; b++) {
                     const uint64_t y = AT(d.cy + MCB * m + b);
-                    boolean(S, y, s + 2 * EC + MSP + MCB * m + b + 1);
+                    boolean<GEN>(S, y, s + 2 * EC + MSP + MCB * m + b + 1, T);
                     acc_shl(sc, y, b);
                 }
-                const uint64_t gamma = sub(L0 + m + 1, mul((uint64_t)(L0 + m), 1ULL << LB));
+                // the signed carry of position m: carry-in of m + 1, -2^26 carry-out of m
+                uint64_t gamma;
+                if constexpr (GEN)
+                    gamma = sub(T[L0 + m], mul(T[L0 + m - 1], 1ULL << LB));
+                else
+                    gamma = sub(L0 + m + 1, mul((uint64_t)(L0 + m), 1ULL << LB));
                 acc_add(S.pos, mul(sub(acc_val(sc), (uint64_t)MC_OFF), gamma));
             }
             break;
         }
         case GA: {  // eval_add_gadget (:404-455)
-            range(S, col, rows, at, d.c, d.cb, s);
+            range<GEN>(S, col, rows, at, d.c, d.cb, s, T);
 #pragma unroll 1
             for (int kk = 0; kk < NL; kk++) {
                 const uint32_t lam = s + EC + kk + 1;
                 const uint64_t cy = AT(d.pos + kk);
-                acc_mad(S.pos, AT(d.a + kk), lam);
-                acc_mad(S.pos, AT(d.b + kk), lam);
-                acc_mad(S.neg, AT(d.c + kk), lam);
+                wmad<GEN>(S.pos, AT(d.a + kk), lam, T);
+                wmad<GEN>(S.pos, AT(d.b + kk), lam, T);
+                wmad<GEN>(S.neg, AT(d.c + kk), lam, T);
                 if (kk + 1 < NL) {
-                    acc_mad(S.pos, cy, lam + 1);             // carry-in of kk + 1
-                    boolean(S, cy, s + EC + NL + kk + 1);     // booleanity, kk < 9
+                    wmad<GEN>(S.pos, cy, lam + 1, T);             // carry-in of kk + 1
+                    boolean<GEN>(S, cy, s + EC + NL + kk + 1, T);     // booleanity, kk < 9
                 } else {
-                    acc_mad(S.pos, cy, s + EC + 2 * NL);      // carry[9] = 0
+                    wmad<GEN>(S.pos, cy, s + EC + 2 * NL, T);      // carry[9] = 0
                 }
-                acc_mad(S.rpos, cy, lam);
+                wmad<GEN>(S.rpos, cy, lam, T);
             }
             break;
         }
         case GS: {  // eval_sub_gadget (:636-698)
-            range(S, col, rows, at, d.c, d.cb, s);
+            range<GEN>(S, col, rows, at, d.c, d.cb, s, T);
             const uint32_t s0 = s + EC;
-            acc_add(S.pos, (uint64_t)(s0 + 1) * SP + SP1);
+            acc_add(S.pos, p_limbs_weighted<GEN>(s0, T));
 #pragma unroll 1
             for (int kk = 0; kk < NL; kk++) {
-                acc_mad(S.pos, AT(d.a + kk), s0 + kk + 1);
-                acc_mad(S.neg, AT(d.b + kk), s0 + kk + 1);
-                acc_mad(S.neg, AT(d.c + kk), s0 + kk + 1);
+                wmad<GEN>(S.pos, AT(d.a + kk), s0 + kk + 1, T);
+                wmad<GEN>(S.neg, AT(d.b + kk), s0 + kk + 1, T);
+                wmad<GEN>(S.neg, AT(d.c + kk), s0 + kk + 1, T);
             }
-            signed_chain(S, col, rows, at, d.pos, d.neg, s0, s0 + NL);
+            signed_chain<GEN>(S, col, rows, at, d.pos, d.neg, s0, s0 + NL, T);
             const uint32_t lam = s + EC + 4 * NL + 1;
-            acc_mad(S.pos, AT(d.pos + NL - 1), lam);
-            acc_mad(S.pos, AT(d.neg + NL - 1), lam);
+            wmad<GEN>(S.pos, AT(d.pos + NL - 1), lam, T);
+            wmad<GEN>(S.pos, AT(d.neg + NL - 1), lam, T);
             break;
         }
         case GF: {  // eval_freeze_gadget (:1449-1531)
-            range(S, col, rows, at, d.q, d.qb, s);
-            range(S, col, rows, at, d.c, d.cb, s + EC);
+            range<GEN>(S, col, rows, at, d.q, d.qb, s, T);
+            range<GEN>(S, col, rows, at, d.c, d.cb, s + EC, T);
             const uint32_t s0 = s + 2 * EC;
-            acc_add(S.neg, (uint64_t)(s0 + 1) * SP + SP1);
+            acc_add(S.neg, p_limbs_weighted<GEN>(s0, T));
             Acc M{0, 0};
             const uint32_t sm = s0 + 4 * NL + 1;  // mux slots
 #pragma unroll 1
             for (int kk = 0; kk < NL; kk++) {
                 const uint64_t a = AT(d.a + kk), dk = AT(d.q + kk);
-                acc_mad(S.pos, a, s0 + kk + 1);
-                acc_mad(S.neg, dk, s0 + kk + 1);
-                acc_mad(S.pos, AT(d.c + kk), sm + kk + 1);
-                acc_mad(S.neg, dk, sm + kk + 1);
-                acc_mad(M, sub(a, dk), sm + kk + 1);
+                wmad<GEN>(S.pos, a, s0 + kk + 1, T);
+                wmad<GEN>(S.neg, dk, s0 + kk + 1, T);
+                wmad<GEN>(S.pos, AT(d.c + kk), sm + kk + 1, T);
+                wmad<GEN>(S.neg, dk, sm + kk + 1, T);
+                wmad<GEN>(M, sub(a, dk), sm + kk + 1, T);
             }
-            signed_chain(S, col, rows, at, d.pos, d.neg, s0, s0 + NL);
-            acc_mad(S.pos, AT(d.pos + NL - 1), s0 + 4 * NL + 1);
+            signed_chain<GEN>(S, col, rows, at, d.pos, d.neg, s0, s0 + NL, T);
+            wmad<GEN>(S.pos, AT(d.pos + NL - 1), s0 + 4 * NL + 1, T);
             acc_add(S.neg, mul(AT(d.neg + NL - 1), acc_val(M)));
             break;
         }
         default: {  // eval_select_gadget (:1243-1276)
-            range(S, col, rows, at, d.c, d.cb, s);
+            range<GEN>(S, col, rows, at, d.c, d.cb, s, T);
             Acc M{0, 0};
 #pragma unroll 1
             for (int kk = 0; kk < NL; kk++) {
                 const uint32_t lam = s + EC + kk + 1;
                 const uint64_t bk = AT(d.b + kk);
-                acc_mad(S.pos, AT(d.c + kk), lam);
-                acc_mad(S.neg, bk, lam);
-                acc_mad(M, sub(AT(d.a + kk), bk), lam);
+                wmad<GEN>(S.pos, AT(d.c + kk), lam, T);
+                wmad<GEN>(S.neg, bk, lam, T);
+                wmad<GEN>(M, sub(AT(d.a + kk), bk), lam, T);
             }
             const uint64_t sel = AT(d.sel);
             acc_add(S.neg, mul(sel, acc_val(M)));
-            boolean(S, sel, s + EC + NL + 1);
+            boolean<GEN>(S, sel, s + EC + NL + 1, T);
         }
         }
     }
@@ -1297,9 +1351,9 @@ void merge_p256_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint6
             const uint64_t *od = d_odd ? d_odd : odd.as<uint64_t>();
             {
                 ProfScope ps(c, "compose_p256", (8.0 * P256_W * n2 + 8.0 * ng * n2) * K);
-                hipLaunchKernelGGL(compose_p256_k, dim3((uint32_t)(n2 / 64), ng, K), dim3(64), 0, c->stream, d_tr,
-                                   od, (uint64_t)P256_W, n2, (uint64_t)2, d_gd.as<GDesc>(), ng, L.tail,
-                                   part.as<uint64_t>());
+                hipLaunchKernelGGL(compose_p256_k<false>, dim3((uint32_t)(n2 / 64), ng, K), dim3(64), 0, c->stream,
+                                   d_tr, od, (uint64_t)P256_W, n2, (uint64_t)2, d_gd.as<GDesc>(), ng, L.tail,
+                                   part.as<uint64_t>(), (const uint64_t *)nullptr);
                 SG_HIP(hipGetLastError());
             }
             {
@@ -1325,9 +1379,9 @@ void merge_p256_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint6
                     n_trace, n, 1);
         {
             ProfScope ps(c, "compose_p256_tail", (8.0 * NT * n + 8.0 * n) * K);
-            hipLaunchKernelGGL(compose_p256_k, dim3((uint32_t)(n / 64), 1, K), dim3(64), 0, c->stream,
+            hipLaunchKernelGGL(compose_p256_k<false>, dim3((uint32_t)(n / 64), 1, K), dim3(64), 0, c->stream,
                                tlde.as<uint64_t>(), (const uint64_t *)nullptr, (uint64_t)NT, n, blowup,
-                               (const GDesc *)nullptr, 0u, td, tphi.as<uint64_t>());
+                               (const GDesc *)nullptr, 0u, td, tphi.as<uint64_t>(), (const uint64_t *)nullptr);
             SG_HIP(hipGetLastError());
         }
         hipLaunchKernelGGL(add_into_k, dim3((uint32_t)((n * K + 255) / 256)), dim3(256), 0, c->stream,
@@ -1337,7 +1391,8 @@ void merge_p256_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint6
     quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
 }
 
-void merge_p256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup, uint64_t *c_eval) {
+void merge_p256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup, uint64_t *c_eval,
+                      const uint64_t *d_lam) {
     SG_REQUIRE(is_pow2(n_trace) && n_trace >= 2 && is_pow2(blowup) && blowup >= 2,
                "merge_p256: n_trace and blowup must be powers of two >= 2");
     SG_REQUIRE(K >= 1 && K <= 65535, "merge_p256: 1 to 65535 proofs");
@@ -1350,9 +1405,14 @@ void merge_p256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace,
     Scoped part(c, 8ULL * (ng + 1) * n * K), phi(c, 8 * n * K);
     {
         ProfScope ps(c, "compose_p256", (8.0 * P256_W * n + 8.0 * (ng + 1) * n) * K);
-        hipLaunchKernelGGL(compose_p256_k, dim3((uint32_t)(n / 64), ng + 1, K), dim3(64), 0, c->stream, lde,
-                           (const uint64_t *)nullptr, (uint64_t)P256_W, n, blowup, d_gd.as<GDesc>(), ng, L.tail,
-                           part.as<uint64_t>());
+        if (d_lam)
+            hipLaunchKernelGGL(compose_p256_k<true>, dim3((uint32_t)(n / 64), ng + 1, K), dim3(64), 0, c->stream,
+                               lde, (const uint64_t *)nullptr, (uint64_t)P256_W, n, blowup, d_gd.as<GDesc>(), ng,
+                               L.tail, part.as<uint64_t>(), d_lam);
+        else
+            hipLaunchKernelGGL(compose_p256_k<false>, dim3((uint32_t)(n / 64), ng + 1, K), dim3(64), 0, c->stream,
+                               lde, (const uint64_t *)nullptr, (uint64_t)P256_W, n, blowup, d_gd.as<GDesc>(), ng,
+                               L.tail, part.as<uint64_t>(), (const uint64_t *)nullptr);
         SG_HIP(hipGetLastError());
     }
     {

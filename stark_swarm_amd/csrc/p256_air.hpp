@@ -56,7 +56,10 @@ void build_p256_traces_dev(Ctx *c, const std::vector<P256Chains> &in, uint64_t h
 // lib.rs:1421-1540) with lambda_j = j + 1 for K proofs: lde of proof k at
 // lde + P256_W * k * n0, c_eval at c_eval + k * n0. Phi is evaluated at every
 // point (the streaming merge has no thread-count quirk).
-void merge_p256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup, uint64_t *c_eval);
+// d_lam: the 155 774 combination coefficients (device memory), or null for
+// lambda_j = j + 1
+void merge_p256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup, uint64_t *c_eval,
+                      const uint64_t *d_lam = nullptr);
 // The same c_eval from the K traces themselves (d_tr proof-major, P256_W x
 // n_trace each), without the full LDE: every gadget constraint has degree
 // <= 2 in trace polynomials of degree < n_trace, so their weighted sum is a

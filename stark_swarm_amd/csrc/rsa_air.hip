@@ -221,13 +221,26 @@ constexpr int PT = 64;  // points per workgroup
 // gate groups: ungated, the output row's EM check, the transitions
 enum { G_ALL, G_OUT, G_TRANS, NG };
 
+// a += lambda_slot x for the 1-based slot index: slot itself (GEN = false,
+// lambda_j = j + 1) or T[slot - 1] (GEN: the caller's coefficients)
+template <bool GEN>
+__device__ __forceinline__ void wmad(Acc &a, uint64_t x, uint32_t slot, const uint64_t *T) {
+    if constexpr (GEN)
+        acc_add(a, mul(x, T[slot - 1]));
+    else
+        acc_mad(a, x, slot);
+}
+
 // GROUPS = false: phi at every point. GROUPS = true: the three gate groups'
 // weighted sums with the gates ignored, phi[(k * NG + g) * n0 + pt], on the
 // n0 points of a blowup-2 LDE held as its even points (lde: the traces,
 // n0 / 2 rows) and its odd points (odd, the same layout).
-template <bool GROUPS>
+// GEN: slot j weighted by T[j] (the caller's combination coefficients)
+// instead of j + 1, the position identities summed term by term.
+template <bool GROUPS, bool GEN>
 __global__ __launch_bounds__(SL * PT) void compose_rsa_k(const uint64_t *lde, const uint64_t *odd, uint64_t n0,
-                                                        uint64_t blowup, uint32_t R, uint64_t *phi) {
+                                                        uint64_t blowup, uint32_t R, uint64_t *phi,
+                                                        const uint64_t *T) {
     __shared__ uint64_t red[SL][4][PT];
     const uint32_t lane = threadIdx.x & (PT - 1), s = threadIdx.x / PT;
     const uint64_t pt = (uint64_t)blockIdx.x * PT + lane;
@@ -281,37 +294,50 @@ __global__ __launch_bounds__(SL * PT) void compose_rsa_k(const uint64_t *lde, co
             const uint64_t a = AT(C_ACC + i), op = AT(C_OP2 + i), sv = AT(C_S + i);
             const uint64_t cc = AT(C_C + i), qq = AT(C_Q + i);
             // select: op - (ph (s - a) + a); the ph part is applied once below
-            acc_mad(pos, op, O + K_SEL + i);
-            acc_mad(neg, a, O + K_SEL + i);
-            acc_mad(sel, sub(sv, a), O + K_SEL + i);
+            wmad<GEN>(pos, op, O + K_SEL + i, T);
+            wmad<GEN>(neg, a, O + K_SEL + i, T);
+            wmad<GEN>(sel, sub(sv, a), O + K_SEL + i, T);
             // position identities, regrouped by the a and q limbs
-            acc_add(pos, mul(a, add(mul_small(B, L0 + i), B1)));
-            acc_add(neg, mul(qq, add(mul_small(N, L0 + i), N1)));
-            acc_mad(neg, cc, L0 + i);
+            if constexpr (GEN) {
+                // sum_j lambda_{L0 + i + j} op_j and n_j, term by term
+                uint64_t ab = 0, qn = 0;
+#pragma unroll 4
+                for (int j = 0; j < NL; j++) {
+                    const uint64_t lam = T[L0 + i + j - 1];
+                    ab = add(ab, mul(AT(C_OP2 + j), lam));
+                    qn = add(qn, mul(AT(C_N + j), lam));
+                }
+                acc_add(pos, mul(a, ab));
+                acc_add(neg, mul(qq, qn));
+            } else {
+                acc_add(pos, mul(a, add(mul_small(B, L0 + i), B1)));
+                acc_add(neg, mul(qq, add(mul_small(N, L0 + i), N1)));
+            }
+            wmad<GEN>(neg, cc, L0 + i, T);
             // range checks of c and q: booleanity and limb packing
             Acc pk{0, 0};
 #pragma unroll
             for (int b = 0; b < LB; b++) {
                 const uint64_t x = AT(C_CBITS + LB * i + b);
-                acc_mad(pos, x, O + K_CBOOL + LB * i + b);
-                acc_mad(neg, mul(x, x), O + K_CBOOL + LB * i + b);
+                wmad<GEN>(pos, x, O + K_CBOOL + LB * i + b, T);
+                wmad<GEN>(neg, mul(x, x), O + K_CBOOL + LB * i + b, T);
                 acc_shl(pk, x, b);
             }
-            acc_mad(pos, cc, O + K_CPACK + i);
-            acc_mad(neg, acc_val(pk), O + K_CPACK + i);
+            wmad<GEN>(pos, cc, O + K_CPACK + i, T);
+            wmad<GEN>(neg, acc_val(pk), O + K_CPACK + i, T);
             pk = Acc{0, 0};
 #pragma unroll
             for (int b = 0; b < LB; b++) {
                 const uint64_t x = AT(C_QBITS + LB * i + b);
-                acc_mad(pos, x, O + K_QBOOL + LB * i + b);
-                acc_mad(neg, mul(x, x), O + K_QBOOL + LB * i + b);
+                wmad<GEN>(pos, x, O + K_QBOOL + LB * i + b, T);
+                wmad<GEN>(neg, mul(x, x), O + K_QBOOL + LB * i + b, T);
                 acc_shl(pk, x, b);
             }
-            acc_mad(pos, qq, O + K_QPACK + i);
-            acc_mad(neg, acc_val(pk), O + K_QPACK + i);
+            wmad<GEN>(pos, qq, O + K_QPACK + i, T);
+            wmad<GEN>(neg, acc_val(pk), O + K_QPACK + i, T);
             if (out_row) {
-                acc_mad(ep, cc, O + K_EM + i);
-                acc_mad(en, AT(C_EM + i), O + K_EM + i);
+                wmad<GEN>(ep, cc, O + K_EM + i, T);
+                wmad<GEN>(en, AT(C_EM + i), O + K_EM + i, T);
             }
         }
         // transitions into the next row (a loop of its own keeps the bit
@@ -320,14 +346,14 @@ __global__ __launch_bounds__(SL * PT) void compose_rsa_k(const uint64_t *lde, co
 #pragma unroll 1
             for (int u = 0; u < NL / SL; u++) {
                 const uint32_t i = s + SL * u;
-                acc_mad(tp, NX(C_ACC + i), O + K_TRANS + i);
-                acc_mad(tn, AT(C_C + i), O + K_TRANS + i);
-                acc_mad(tp, NX(C_S + i), O + K_TRANS + NL + i);
-                acc_mad(tn, AT(C_S + i), O + K_TRANS + NL + i);
-                acc_mad(tp, NX(C_N + i), O + K_TRANS + 2 * NL + i);
-                acc_mad(tn, AT(C_N + i), O + K_TRANS + 2 * NL + i);
-                acc_mad(tp, NX(C_EM + i), O + K_TRANS + 3 * NL + i);
-                acc_mad(tn, AT(C_EM + i), O + K_TRANS + 3 * NL + i);
+                wmad<GEN>(tp, NX(C_ACC + i), O + K_TRANS + i, T);
+                wmad<GEN>(tn, AT(C_C + i), O + K_TRANS + i, T);
+                wmad<GEN>(tp, NX(C_S + i), O + K_TRANS + NL + i, T);
+                wmad<GEN>(tn, AT(C_S + i), O + K_TRANS + NL + i, T);
+                wmad<GEN>(tp, NX(C_N + i), O + K_TRANS + 2 * NL + i, T);
+                wmad<GEN>(tn, AT(C_N + i), O + K_TRANS + 2 * NL + i, T);
+                wmad<GEN>(tp, NX(C_EM + i), O + K_TRANS + 3 * NL + i, T);
+                wmad<GEN>(tn, AT(C_EM + i), O + K_TRANS + 3 * NL + i, T);
             }
         }
         // carries: booleanity, and sc_m (lambda_{m+1} - 2^26 lambda_m) from the
@@ -338,17 +364,21 @@ __global__ __launch_bounds__(SL * PT) void compose_rsa_k(const uint64_t *lde, co
 #pragma unroll
             for (int b = 0; b < CARRY_BITS; b++) {
                 const uint64_t y = AT(C_CARRY + CARRY_BITS * m + b);
-                acc_mad(pos, y, O + K_CARRY + CARRY_BITS * m + b);
-                acc_mad(neg, mul(y, y), O + K_CARRY + CARRY_BITS * m + b);
+                wmad<GEN>(pos, y, O + K_CARRY + CARRY_BITS * m + b, T);
+                wmad<GEN>(neg, mul(y, y), O + K_CARRY + CARRY_BITS * m + b, T);
                 acc_shl(sc, y, b);
             }
-            const uint64_t gamma = sub(L0 + m + 1, (uint64_t)(L0 + m) << LB);
+            uint64_t gamma;  // the signed carry of position m: carry-in of m + 1, -2^26 carry-out of m
+            if constexpr (GEN)
+                gamma = sub(T[L0 + m], mul(T[L0 + m - 1], 1ULL << LB));
+            else
+                gamma = sub(L0 + m + 1, (uint64_t)(L0 + m) << LB);
             acc_add(pos, mul(sub(acc_val(sc), CARRY_OFF), gamma));
         }
         const uint64_t ph = AT(C_PHASE);
         if (s == 0) {
-            acc_mad(pos, ph, O + K_PHASE);
-            acc_mad(neg, mul(ph, ph), O + K_PHASE);
+            wmad<GEN>(pos, ph, O + K_PHASE, T);
+            wmad<GEN>(neg, mul(ph, ph), O + K_PHASE, T);
         }
 #undef AT
 #undef NX
@@ -430,7 +460,7 @@ void build_rsa_traces_dev(Ctx *c, const std::vector<RsaRecord> &records, uint32_
 }
 
 void merge_rsa_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint32_t R, uint64_t n_trace, uint64_t blowup,
-                     uint64_t *c_eval) {
+                     uint64_t *c_eval, const uint64_t *d_lam) {
     SG_REQUIRE(is_pow2(n_trace) && n_trace >= 32 && is_pow2(blowup) && blowup >= 2,
                "merge_rsa: n_trace must be a power of two >= 32 and blowup >= 2");
     SG_REQUIRE(K >= 1 && K <= 65535, "merge_rsa: 1 to 65535 proofs");
@@ -440,8 +470,14 @@ void merge_rsa_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint32_t R, uint64
     Scoped phi(c, 8 * n * K);
     {
         ProfScope ps(c, "compose_rsa", (8.0 * RSA_W * n * R + 8.0 * n) * K);
-        hipLaunchKernelGGL(rsa::compose_rsa_k<false>, dim3((uint32_t)(n / rsa::PT), K), dim3(rsa::SL * rsa::PT), 0,
-                           c->stream, lde, (const uint64_t *)nullptr, n, blowup, R, phi.as<uint64_t>());
+        if (d_lam)
+            hipLaunchKernelGGL((rsa::compose_rsa_k<false, true>), dim3((uint32_t)(n / rsa::PT), K),
+                               dim3(rsa::SL * rsa::PT), 0, c->stream, lde, (const uint64_t *)nullptr, n, blowup, R,
+                               phi.as<uint64_t>(), d_lam);
+        else
+            hipLaunchKernelGGL((rsa::compose_rsa_k<false, false>), dim3((uint32_t)(n / rsa::PT), K),
+                               dim3(rsa::SL * rsa::PT), 0, c->stream, lde, (const uint64_t *)nullptr, n, blowup, R,
+                               phi.as<uint64_t>(), (const uint64_t *)nullptr);
         SG_HIP(hipGetLastError());
     }
     quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
@@ -463,8 +499,9 @@ void merge_rsa_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint32
             Scoped odd(c, 8ULL * RSA_W * R * n_trace * K);
             lde_odd_coset(c, d_tr, odd.as<uint64_t>(), ilog2(n_trace), (uint64_t)RSA_W * R * K);
             ProfScope ps(c, "compose_rsa", (8.0 * RSA_W * n2 * R + 8.0 * rsa::NG * n2) * K);
-            hipLaunchKernelGGL(rsa::compose_rsa_k<true>, dim3((uint32_t)(n2 / rsa::PT), K), dim3(rsa::SL * rsa::PT), 0,
-                               c->stream, d_tr, odd.as<uint64_t>(), n2, (uint64_t)2, R, groups.as<uint64_t>());
+            hipLaunchKernelGGL((rsa::compose_rsa_k<true, false>), dim3((uint32_t)(n2 / rsa::PT), K),
+                               dim3(rsa::SL * rsa::PT), 0, c->stream, d_tr, odd.as<uint64_t>(), n2, (uint64_t)2, R,
+                               groups.as<uint64_t>(), (const uint64_t *)nullptr);
             SG_HIP(hipGetLastError());
         }
         Scoped full(c, 8ULL * rsa::NG * n * K);

@@ -37,7 +37,8 @@ void build_rsa_traces_dev(Ctx *c, const std::vector<RsaRecord> &records, uint32_
 // lambda_j = j + 1, for K proofs: lde of proof k at lde + RSA_W*R*k*n0,
 // c_eval at c_eval + k*n0. The streaming merges always evaluate phi.
 void merge_rsa_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint32_t R, uint64_t n_trace, uint64_t blowup,
-                     uint64_t *c_eval);
+                     uint64_t *c_eval,
+                     const uint64_t *d_lam = nullptr);  // d_lam: 11280 R coefficients, or null (j + 1)
 // The same c_eval from the K traces themselves (d_tr: K x RSA_W R x
 // n_trace), without the blowup-x LDE: every constraint has degree <= 2 in the
 // cells of the current and next row, so each gate group's weighted sum

@@ -173,10 +173,13 @@ __global__ __launch_bounds__(256) void expand_sha256_k(const RowWords *words, co
 }
 
 // ---- constraints (sha256_air.rs:956-1294), one LDE point per lane ----
+// phi accumulator: slot w - 1 weighted by lam[w - 1], or by w = j + 1 without a table
 struct Phi {
-    uint64_t acc = 0, w = 1;  // w = j + 1 of the next constraint
+    uint64_t acc = 0, w = 1;
+    const uint64_t *lam = nullptr;
+    __device__ __forceinline__ uint64_t weight() const { return lam ? lam[w - 1] : w; }
     __device__ __forceinline__ void put(uint64_t v) {
-        acc = add(acc, mul(w, v));
+        acc = add(acc, mul(weight(), v));
         w++;
     }
     __device__ __forceinline__ void skip(int k) { w += k; }
@@ -201,11 +204,12 @@ enum { G_ALL, G_COMP, G_FIN, G_NOTFIN, G_SCHED, NG };
 // point, gates ignored and the row constant K[t] left out, written to
 // phi[(k * NG + g) * n + i], on the n points of a blowup-2 LDE held as its
 // even points (lde: the traces, n / 2 rows) and its odd points (odd, the
-// same layout); t1_lam gets the weight of the T1 identity.
+// same layout); t1_lam gets the weight of the T1 identity. lam: the 766
+// slots' combination coefficients, or j + 1 when null.
 template <bool GROUPS>
 __global__ __launch_bounds__(AT) void compose_sha256_k(const uint64_t *lde, const uint64_t *odd, uint64_t n,
                                                        uint64_t blowup, const uint32_t *n_blocks, uint64_t *phi,
-                                                       uint64_t *t1_lam) {
+                                                       uint64_t *t1_lam, const uint64_t *lam) {
     const uint64_t i = (uint64_t)blockIdx.x * AT + threadIdx.x;
     if (i >= n) return;
     // GROUPS: point i is row i / 2 of the traces (even i) or of the odd coset
@@ -220,6 +224,7 @@ __global__ __launch_bounds__(AT) void compose_sha256_k(const uint64_t *lde, cons
     const bool sched = GROUPS || (!past && br >= 15 && br <= 62);
     const uint64_t TWO32 = 1ULL << 32;
     Phi f, fc, ff, fn, fs;  // ungated; compression, finalisation, not finalisation, schedule (GROUPS mode)
+    f.lam = fc.lam = ff.lam = fn.lam = fs.lam = lam;
     Phi &gc = GROUPS ? fc : f, &gs = GROUPS ? fs : f;
     // booleanity of the 8 bit blocks
     const uint32_t boff[8] = {BIT_A, BIT_B, BIT_C, BIT_E, BIT_F, BIT_G, BIT_WW14, BIT_WW1};
@@ -282,7 +287,7 @@ __global__ __launch_bounds__(AT) void compose_sha256_k(const uint64_t *lde, cons
     // T1 = h + Sigma1 + Ch + K[t] + W[t] = T1_lo + 2^32 T1_cw; T2 = Sigma0 + Maj
     const uint64_t krow = GROUPS ? 0 : comp ? KD[br] : 0;
     const uint64_t t1s = add(add(add(add(cur(A_H), cur(S1_W)), cur(CH_W)), krow), cur(COL_W));
-    if (GROUPS && i == 0 && blockIdx.y == 0) *t1_lam = f.w;
+    if (GROUPS && i == 0 && blockIdx.y == 0) *t1_lam = f.weight();
     f.put(sub(add(cur(T1_LO), mul(TWO32, cur(T1_CW))), t1s));
     f.put(boolc(cur(T1_C0)));
     f.put(boolc(cur(T1_C1)));
@@ -454,7 +459,7 @@ void build_sha256_traces_dev(Ctx *c, const std::vector<ByteSpan> &msgs, uint64_t
 }
 
 void merge_sha256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup,
-                        const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval) {
+                        const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval, const uint64_t *d_lam) {
     SG_REQUIRE(is_pow2(n_trace) && n_trace >= 128 && is_pow2(blowup) && blowup >= 2,
                "merge_sha256: n_trace must be a power of two >= 128");
     SG_REQUIRE(K >= 1 && K <= 65535 && n_blocks.size() == K, "merge_sha256: one n_blocks per proof");
@@ -471,14 +476,15 @@ void merge_sha256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trac
         ProfScope ps(c, "compose_sha256", (8.0 * SHA256_W * n + 8.0 * n) * K);
         hipLaunchKernelGGL(sha::compose_sha256_k<false>, dim3((uint32_t)((n + sha::AT - 1) / sha::AT), K),
                            dim3(sha::AT), 0, c->stream, lde, (const uint64_t *)nullptr, n, blowup,
-                           d_nb.as<uint32_t>(), phi.as<uint64_t>(), (uint64_t *)nullptr);
+                           d_nb.as<uint32_t>(), phi.as<uint64_t>(), (uint64_t *)nullptr, d_lam);
         SG_HIP(hipGetLastError());
     }
     quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
 }
 
 void merge_sha256_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint64_t n_trace, uint64_t blowup,
-                                   const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval) {
+                                   const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval,
+                                   const uint64_t *d_lam) {
     SG_REQUIRE(is_pow2(n_trace) && n_trace >= 128 && is_pow2(blowup) && blowup >= 4,
                "merge_sha256: n_trace must be a power of two >= 128, blowup >= 4");
     SG_REQUIRE(K >= 1 && K <= 65535 && n_blocks.size() == K, "merge_sha256: one n_blocks per proof");
@@ -500,7 +506,7 @@ void merge_sha256_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uin
             ProfScope ps(c, "compose_sha256", (8.0 * SHA256_W * n2 + 8.0 * sha::NG * n2) * K);
             hipLaunchKernelGGL(sha::compose_sha256_k<true>, dim3((uint32_t)((n2 + sha::AT - 1) / sha::AT), K),
                                dim3(sha::AT), 0, c->stream, d_tr, odd.as<uint64_t>(), n2, (uint64_t)2,
-                               d_nb.as<uint32_t>(), groups.as<uint64_t>(), t1.as<uint64_t>());
+                               d_nb.as<uint32_t>(), groups.as<uint64_t>(), t1.as<uint64_t>(), d_lam);
             SG_HIP(hipGetLastError());
         }
         Scoped full(c, 8ULL * sha::NG * n * K);

@@ -55,12 +55,15 @@ void compose_sha512_groups(Ctx *c, const uint64_t *d_tr, const uint64_t *d_odd, 
                            uint64_t *t1_lam);
 const uint64_t *sha512_round_constants();  // FIPS 180-4 K[0..80], host memory
 // deep_ali_merge_sha256 (lib.rs:453-545) for K proofs of one n_trace: lde of
-// proof k at lde + 756*k*n0, c_eval at c_eval + k*n0
+// proof k at lde + 756*k*n0, c_eval at c_eval + k*n0; d_lam: the 766
+// combination coefficients (device memory), or null for lambda_j = j + 1
 void merge_sha256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup,
-                        const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval);
-// deep_ali_merge_sha512 (lib.rs:561-653), same batching
+                        const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval,
+                        const uint64_t *d_lam = nullptr);
+// deep_ali_merge_sha512 (lib.rs:561-653), same batching; d_lam: 1526 coefficients
 void merge_sha512_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup,
-                        const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval);
+                        const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval,
+                        const uint64_t *d_lam = nullptr);
 // The same c_eval from the K traces themselves (d_tr: K x W x n_trace),
 // without the blowup-x LDE. Every constraint has degree <= 2 in the cells of
 // the current and next row, and the next row's polynomial is T(w_n x), so
@@ -73,8 +76,10 @@ void merge_sha512_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trac
 // arithmetic makes the result equal to merge_sha256_batch / _sha512_batch
 // on the full LDE. blowup >= 4.
 void merge_sha256_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint64_t n_trace, uint64_t blowup,
-                                   const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval);
+                                   const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval,
+                                   const uint64_t *d_lam = nullptr);
 void merge_sha512_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint64_t n_trace, uint64_t blowup,
-                                   const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval);
+                                   const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval,
+                                   const uint64_t *d_lam = nullptr);
 
 }  // namespace sg

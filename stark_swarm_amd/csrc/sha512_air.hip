@@ -517,7 +517,7 @@ void build_sha512_traces_into(Ctx *c, const std::vector<ByteSpan> &msgs, uint64_
 }
 
 void merge_sha512_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trace, uint64_t blowup,
-                        const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval) {
+                        const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval, const uint64_t *d_lam) {
     SG_REQUIRE(is_pow2(n_trace) && n_trace >= 128 && is_pow2(blowup) && blowup >= 2,
                "merge_sha512: n_trace must be a power of two >= 128");
     SG_REQUIRE(K >= 1 && K <= 65535 && n_blocks.size() == K, "merge_sha512: one n_blocks per proof");
@@ -535,14 +535,15 @@ void merge_sha512_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trac
         hipLaunchKernelGGL(sha5::compose_sha512_k<false>, dim3((uint32_t)((n + sha5::AT - 1) / sha5::AT), K),
                            dim3(sha5::AT), 0, c->stream, lde, (const uint64_t *)nullptr, n, blowup,
                            d_nb.as<uint32_t>(), phi.as<uint64_t>(), (uint64_t *)nullptr, (uint64_t)SHA512_W,
-                           ~0ULL, (const uint64_t *)nullptr);
+                           ~0ULL, d_lam);
         SG_HIP(hipGetLastError());
     }
     quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
 }
 
 void merge_sha512_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint64_t n_trace, uint64_t blowup,
-                                   const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval) {
+                                   const std::vector<uint32_t> &n_blocks, int phi_mode, uint64_t *c_eval,
+                                   const uint64_t *d_lam) {
     SG_REQUIRE(is_pow2(n_trace) && n_trace >= 128 && is_pow2(blowup) && blowup >= 4,
                "merge_sha512: n_trace must be a power of two >= 128, blowup >= 4");
     SG_REQUIRE(K >= 1 && K <= 65535 && n_blocks.size() == K, "merge_sha512: one n_blocks per proof");
@@ -564,7 +565,7 @@ void merge_sha512_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uin
             hipLaunchKernelGGL(sha5::compose_sha512_k<true>, dim3((uint32_t)((n2 + sha5::AT - 1) / sha5::AT), K),
                                dim3(sha5::AT), 0, c->stream, d_tr, odd.as<uint64_t>(), n2, (uint64_t)2,
                                d_nb.as<uint32_t>(), groups.as<uint64_t>(), t1.as<uint64_t>(), (uint64_t)SHA512_W,
-                               ~0ULL, (const uint64_t *)nullptr);
+                               ~0ULL, d_lam);
             SG_HIP(hipGetLastError());
         }
         Scoped full(c, 8ULL * sha5::NG * n * K);

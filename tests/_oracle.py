@@ -62,6 +62,10 @@ def lib():
         L.ref_lde_columns.argtypes = [u64p, sz, sz, sz, ctypes.c_uint64, u64p]
         L.ref_build_hash_rollup_trace.argtypes = [sz, u64p, sz, u64p]
         L.ref_merge_hash_rollup.argtypes = [u64p, sz, sz, ctypes.c_int, u64p]
+        L.ref_merge_hash_rollup_lam.argtypes = [u64p, sz, sz, ctypes.c_int, u64p, u64p]
+        L.ref_merge_nsec3_lam.argtypes = [u64p, sz, sz, ctypes.c_int, u64p, u64p]
+        L.ref_merge_sha256_lam.argtypes = [u64p, sz, sz, sz, ctypes.c_int, u64p, u64p]
+        L.ref_merge_sha512_lam.argtypes = [u64p, sz, sz, sz, ctypes.c_int, u64p, u64p]
         L.ref_merkle_root.argtypes = [u64p, sz, sz, sz, sz, ctypes.c_uint64, ctypes.c_char_p]
         L.ref_deep_fri_prove.argtypes = [u64p, sz, ctypes.POINTER(FriParams), ctypes.POINTER(u8p),
                                          ctypes.POINTER(sz)]
@@ -197,10 +201,18 @@ def hash_rollup_trace(n_trace, leaves):
     return t
 
 
-def merge_hash_rollup(lde, n_trace, blowup, phi_mode=1):
+def _lam(coeffs):
+    if coeffs is None:
+        return None, None
+    c = np.ascontiguousarray(coeffs, dtype=np.uint64)
+    return c, ptr(c)
+
+
+def merge_hash_rollup(lde, n_trace, blowup, phi_mode=1, coeffs=None):
     lde = np.ascontiguousarray(lde, dtype=np.uint64)
     out = np.empty(n_trace * blowup, dtype=np.uint64)
-    lib().ref_merge_hash_rollup(ptr(lde), n_trace, blowup, phi_mode, ptr(out))
+    keep, cp = _lam(coeffs)
+    lib().ref_merge_hash_rollup_lam(ptr(lde), n_trace, blowup, phi_mode, cp, ptr(out))
     return out
 
 
@@ -271,10 +283,11 @@ def sha256_constraints(cur, nxt, row, n_blocks):
     return out
 
 
-def merge_sha256(lde, n_trace, blowup, n_blocks, phi_mode=1):
+def merge_sha256(lde, n_trace, blowup, n_blocks, phi_mode=1, coeffs=None):
     lde = np.ascontiguousarray(lde, dtype=np.uint64)
     out = np.zeros(n_trace * blowup, dtype=np.uint64)
-    lib().ref_merge_sha256(ptr(lde), n_trace, blowup, n_blocks, phi_mode, ptr(out))
+    keep, cp = _lam(coeffs)
+    lib().ref_merge_sha256_lam(ptr(lde), n_trace, blowup, n_blocks, phi_mode, cp, ptr(out))
     return out
 
 
@@ -310,10 +323,11 @@ def sha512_constraints(cur, nxt, row, n_blocks):
     return out
 
 
-def merge_sha512(lde, n_trace, blowup, n_blocks, phi_mode=1):
+def merge_sha512(lde, n_trace, blowup, n_blocks, phi_mode=1, coeffs=None):
     lde = np.ascontiguousarray(lde, dtype=np.uint64)
     out = np.zeros(n_trace * blowup, dtype=np.uint64)
-    lib().ref_merge_sha512(ptr(lde), n_trace, blowup, n_blocks, phi_mode, ptr(out))
+    keep, cp = _lam(coeffs)
+    lib().ref_merge_sha512_lam(ptr(lde), n_trace, blowup, n_blocks, phi_mode, cp, ptr(out))
     return out
 
 
@@ -388,10 +402,11 @@ def nsec3_constraints(cur, nxt):
     return out
 
 
-def merge_nsec3(lde, n_trace, blowup, phi_mode=1):
+def merge_nsec3(lde, n_trace, blowup, phi_mode=1, coeffs=None):
     lde = np.ascontiguousarray(lde, dtype=np.uint64)
     out = np.zeros(n_trace * blowup, dtype=np.uint64)
-    lib().ref_merge_nsec3(ptr(lde), n_trace, blowup, phi_mode, ptr(out))
+    keep, cp = _lam(coeffs)
+    lib().ref_merge_nsec3_lam(ptr(lde), n_trace, blowup, phi_mode, cp, ptr(out))
     return out
 
 
