@@ -180,6 +180,57 @@ def test_outer_rollup_golden_on_gpu(S):
     assert out.n0 == d["outer_n0"] and out.n_trace == d["outer_n_trace"]
 
 
+def _oracle_threads():
+    import os
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
+def test_prover_core_c2_rows_matches_oracle(S):
+    """The C2 shape on 8 of its 128 columns: 2^20 rows, blowup 8, STIR fold
+    [8, 8, 8, 8, 8, 4], 80 queries -- the whole proof byte for byte against
+    the oracle (run on the host's cores in the reference's threading)"""
+    import ctypes
+    ctx = S.default_context()
+    w, n, blowup, schedule, s0 = 8, 1 << 20, 8, [8, 8, 8, 8, 8, 4], 80
+    tr = rnd(520, (w, n))
+    params = S.DeepFriParams(schedule, s0, S.SEED_Z, True, 1, True, s0, None)
+    d_tr = ctx.alloc(tr.nbytes).upload(tr)
+    d_lde = ctx.alloc(tr.nbytes * blowup)
+    say("prover core 8 x 2^20, blowup 8 (GPU)")
+    root, blob = S.prover_core(d_tr, w, n, blowup, 7, params, d_lde)
+    rr = ctypes.create_string_buffer(32)
+    pp, pl = O.u8p(), O.sz()
+    say("prover core 8 x 2^20, blowup 8 (oracle, %d threads)" % _oracle_threads())
+    with O.threads(_oracle_threads()):
+        rc = O.lib().ref_prover_core(O.ptr(tr), w, n, blowup, 7, O.fri_params(schedule, r=s0, s0=s0), rr, None, pp, pl)
+    assert rc == 0
+    assert root == rr.raw
+    assert blob == O.take_bytes(pp, pl)
+
+
+def test_prover_core_c2_full_lde_columns(S):
+    """The full C2 shape (2^20 x 128, blowup 8, shift 7): three whole LDE
+    columns of the proof's device LDE against the oracle's coset LDE of
+    those columns, and the proof repeats byte for byte"""
+    ctx = S.default_context()
+    w, n, blowup, schedule, s0 = 128, 1 << 20, 8, [8, 8, 8, 8, 8, 4], 80
+    tr = rnd(521, (w, n))
+    params = S.DeepFriParams(schedule, s0, S.SEED_Z, True, 1, True, s0, None)
+    d_tr = ctx.alloc(tr.nbytes).upload(tr)
+    d_lde = ctx.alloc(tr.nbytes * blowup)
+    say("prover core 128 x 2^20, blowup 8 (GPU, twice)")
+    root, blob = S.prover_core(d_tr, w, n, blowup, 7, params, d_lde)
+    assert S.prover_core(d_tr, w, n, blowup, 7, params, d_lde) == (root, blob)
+    n0 = n * blowup
+    cols = [0, 77, 127]
+    say("oracle coset LDE of columns %s" % cols)
+    want = O.lde_columns(np.ascontiguousarray(tr[cols]), blowup, shift=7)
+    for i, c in enumerate(cols):
+        assert np.array_equal(d_lde.download((n0,), offset=8 * c * n0), want[i]), c
+    d_tr.free()
+    d_lde.free()
+
+
 def test_fri_ldt_unsupported_is_an_error(S):
     with pytest.raises(S.StarkError):
         S.prove_outer_rollup([b"\x00" * 32], b"\x01" * 32, S.LdtMode.Fri)
