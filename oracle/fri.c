@@ -241,6 +241,22 @@ void ref_merkle_root(const uint64_t *leaf_vals, size_t n_leaves, size_t vpl, siz
     mt_free(&t);
 }
 
+int ref_merkle_open(const uint64_t *leaf_vals, size_t n_leaves, size_t vpl, size_t arity, size_t depth, uint64_t label,
+                    size_t index, uint8_t **out, size_t *out_len) {
+    if (index >= n_leaves) return -1;
+    mtree t;
+    uint8_t root[32];
+    bbuf b = {0};
+    mt_init(&t, arity, depth, label, n_leaves);
+    hash_leaves(&t, n_leaves, leaf_vals, vpl, vpl);
+    mt_finalize(&t, root);
+    mt_open_ser(&t, index, &b);
+    mt_free(&t);
+    *out = b.p;
+    *out_len = b.len;
+    return 0;
+}
+
 /* ------------------------------ FRI pieces ------------------------------ */
 /* f(x) interpolated on <omega_n> (natural order), evaluated at m points
  * zeta^j * z, zeta = omega^(n/m)  (fri.rs:338-391) */
@@ -351,25 +367,77 @@ typedef struct {
     fp6 final_coeffs[64];
     size_t n_final_coeffs;
     uint64_t query_seed;
+    fp6 **q;            /* classic FRI: DEEP quotient layers q_l, l < L */
+    fp6 alpha_final;
 } fri_state;
 
 static void fs_free(fri_state *s) {
+    if (s->q)
+        for (size_t i = 0; i < s->L; i++) free(s->q[i]);
+    free(s->q);
     for (size_t i = 0; i <= s->L; i++) free(s->f[i]);
     free(s->f); free(s->sz); free(s->roots); free(s->fz); free(s->coset_evals); free(s->ce_off); free(s->coeff_tuples);
 }
 
+/* the layer's polynomial (6 component iFFTs of f on <omega_n>) at z, by Horner
+ * from the top coefficient (fri.rs:255-299) */
+static fp6 poly_eval_ext(const fp6 *f, size_t n, fp6 z) {
+    uint64_t *comp = (uint64_t *)malloc(sizeof(uint64_t) * n);
+    fp6 *coef = (fp6 *)malloc(sizeof(fp6) * n);
+    for (int c = 0; c < 6; c++) {
+        for (size_t i = 0; i < n; i++) comp[i] = f[i].c[c];
+        ref_ntt_inplace(comp, n, 1);
+        for (size_t i = 0; i < n; i++) coef[i].c[c] = comp[i];
+    }
+    fp6 acc = fp6_zero();
+    for (size_t k = n; k-- > 0;) acc = fp6_add(fp6_mul(acc, z), coef[k]);
+    free(comp); free(coef);
+    return acc;
+}
+
+/* DEEP quotient q_i = (f_i - f(z)) / (omega^i - z)  (fri.rs:301-333) */
+typedef struct {
+    const fp6 *f;
+    fp6 z, fz, *q;
+    uint64_t omega;
+} q_job;
+static void q_range(void *ctx, size_t lo, size_t hi) {
+    q_job *J = (q_job *)ctx;
+    uint64_t x = gl_pow(J->omega, lo);
+    for (size_t i = lo; i < hi; i++) {
+        fp6 den = fp6_sub(fp6_from_base(x), J->z);
+        J->q[i] = fp6_mul(fp6_sub(J->f[i], J->fz), fp6_inv(den));
+        x = gl_mul(x, J->omega);
+    }
+}
+
+/* classic-FRI leaf of layer ell at i: f_i, s_i, q_i (ext_leaf_fields,
+ * fri.rs:1356-1362); s_i = f_{ell+1}[i mod n/m], or 0 on the final
+ * coefficient-committed layer (fri.rs:773-789, :1600) */
+static void classic_leaves(const fri_state *s, size_t ell, size_t cur, size_t m, int final_cc, uint64_t *out) {
+    size_t nn = cur / m;
+    for (size_t i = 0; i < cur; i++) {
+        uint64_t *o = out + 18 * i;
+        memcpy(o, s->f[ell][i].c, 48);
+        if (final_cc) memset(o + 6, 0, 48);
+        else memcpy(o + 6, s->f[ell + 1][i % nn].c, 48);
+        memcpy(o + 12, s->q[ell][i].c, 48);
+    }
+}
+
 static int fri_build(const uint64_t *f0, size_t n0, const ref_fri_params *p, fri_state *s) {
     size_t L = p->n_layers;
-    if (!p->stir || L == 0) return -2; /* only the STIR path (production LdtMode::Stir) */
+    if (L == 0) return -2;
     memset(s, 0, sizeof *s);
     s->L = L; s->n0 = n0;
     int use_cc = p->coeff_commit_final && L > 0;
-    size_t normal = use_cc ? L - 1 : L;
+    const int stir = p->stir != 0;
     s->f = (fp6 **)calloc(L + 1, sizeof(fp6 *));
     s->sz = (size_t *)calloc(L + 1, sizeof(size_t));
     s->roots = (uint8_t(*)[32])calloc(L, 32);
     s->fz = (fp6 *)calloc(L, sizeof(fp6));
     s->ce_off = (size_t *)calloc(L + 1, sizeof(size_t));
+    if (!stir) s->q = (fp6 **)calloc(L, sizeof(fp6 *));
     size_t tot = 0, n = n0;
     for (size_t l = 0; l < L; l++) {
         if (p->schedule[l] < 2 || n % p->schedule[l]) return -1;
@@ -382,18 +450,27 @@ static int fri_build(const uint64_t *f0, size_t n0, const ref_fri_params *p, fri
     tr_init(&tr, "FRI/FS");
     bind_statement(&tr, p, n0);
 
-    /* coset-packed f0 tree (fri.rs:1420-1431) */
     size_t m0 = p->schedule[0], nn0 = n0 / m0;
-    {
+    if (stir) {
+        /* coset-packed f0 tree (fri.rs:1420-1431) */
         size_t a = pick_arity2(nn0, 16), d = merkle_depth(nn0, a);
         mtree t;
         mt_init(&t, a, d, 0xFF, nn0);
         hash_packed_leaves(&t, f0, nn0, m0);
         mt_finalize(&t, s->root_f0);
         mt_free(&t);
+    } else {
+        /* one value per leaf (fri.rs:1432-1440, f0_tree_config :1068-1072) */
+        size_t a = pick_arity2(n0, 16), d = merkle_depth(n0, a);
+        mtree t;
+        mt_init(&t, a, d, 0xFF, n0);
+        hash_leaves(&t, n0, f0, 1, 1);
+        mt_finalize(&t, s->root_f0);
+        mt_free(&t);
     }
     tr_absorb(&tr, s->root_f0, 32);
     fp6 z = challenge_ext(&tr, "z_fp3");
+    const fp6 z_ext = z;
     uint8_t seed_bytes[32];
     tr_challenge_bytes(&tr, "FRI/seed", 8, seed_bytes); /* trace_hash: unused by MerkleTreeChannel */
 
@@ -407,21 +484,43 @@ static int fri_build(const uint64_t *f0, size_t n0, const ref_fri_params *p, fri
         fp6 alpha = fp6_zero();
         if (!final_cc) alpha = challenge_ext(&tr, "alpha");
         fp6 *ce = s->coset_evals + s->ce_off[ell];
-        ood_coset_evals(s->f[ell], cur, z, m, ce);
-        s->fz[ell] = ce[0];
-        z = fp6_pow(z, m);
+        if (stir) {
+            ood_coset_evals(s->f[ell], cur, z, m, ce);
+            s->fz[ell] = ce[0];
+            z = fp6_pow(z, m);
+        } else {
+            /* compute_q_layer_ext_on_ext at the fixed z (fri.rs:1481-1484) */
+            s->fz[ell] = poly_eval_ext(s->f[ell], cur, z_ext);
+            s->q[ell] = (fp6 *)malloc(sizeof(fp6) * cur);
+            q_job qj = {s->f[ell], z_ext, s->fz[ell], s->q[ell], gl_root_of_unity(cur)};
+            ref_par_for(cur, PAR_FRI, q_range, &qj);
+        }
         fp6 *tuples = (fp6 *)malloc(sizeof(fp6) * cur);
         coset_coeffs(s->f[ell], cur, m, tuples);
-        /* layer tree: STIR leaf = 6 components of f (fri.rs:1364-1366,1506-1516) */
+        s->f[ell + 1] = (fp6 *)malloc(sizeof(fp6) * nn);
+        /* s_l (classic) is the fold repeated over the fibres, so the fold with
+         * this layer's alpha comes before the layer tree (fri.rs:1519-1524) */
+        if (!final_cc) fold_tuples(tuples, nn, m, alpha, s->f[ell + 1]);
         {
             size_t a = pick_arity2(cur, m), d = merkle_depth(cur, a);
             mtree t;
             mt_init(&t, a, d, ell, cur);
-            hash_leaves(&t, cur, s->f[ell][0].c, 6, 6);
+            if (stir) {
+                /* STIR leaf = 6 components of f (fri.rs:1364-1366,1506-1516) */
+                hash_leaves(&t, cur, s->f[ell][0].c, 6, 6);
+            } else {
+                uint64_t *lv = (uint64_t *)malloc(8 * 18 * cur);
+                classic_leaves(s, ell, cur, m, final_cc, lv);
+                hash_leaves(&t, cur, lv, 18, 18);
+                free(lv);
+            }
             mt_finalize(&t, s->roots[ell]);
             mt_free(&t);
         }
-        for (size_t j = 0; j < m; j++) tr_absorb_fp6(&tr, ce[j]);
+        if (stir)
+            for (size_t j = 0; j < m; j++) tr_absorb_fp6(&tr, ce[j]);
+        else
+            tr_absorb_fp6(&tr, s->fz[ell]);
         tr_absorb(&tr, s->roots[ell], 32);
         if (final_cc) {
             /* coefficient-tuple tree, label 0xFE (fri.rs:860-868,1649-1664) */
@@ -435,15 +534,14 @@ static int fri_build(const uint64_t *f0, size_t n0, const ref_fri_params *p, fri
             tr_absorb(&tr, s->coeff_root, 32);
             alpha = challenge_ext(&tr, "alpha");
             (void)challenge_ext(&tr, "beta_deg");
+            fold_tuples(tuples, nn, m, alpha, s->f[ell + 1]);
+            s->alpha_final = alpha;
         }
-        s->f[ell + 1] = (fp6 *)malloc(sizeof(fp6) * nn);
-        fold_tuples(tuples, nn, m, alpha, s->f[ell + 1]);
         s->sz[ell + 1] = nn;
         if (final_cc) { s->coeff_tuples = tuples; s->n_final_tuples = nn; s->m_final = m; }
         else free(tuples);
         cur = nn;
     }
-    (void)normal;
     /* final polynomial: first d_final coefficients of the last layer (fri.rs:1919-1937) */
     size_t nf = s->sz[L];
     {
@@ -481,6 +579,86 @@ int ref_fri_layers(const uint64_t *f0, size_t n0, const ref_fri_params *p, uint6
     return 0;
 }
 
+/* raw query index q (fri.rs:1044-1058): StdRng seeded by index_seed(seed, 0, q) */
+static uint64_t query_raw(uint64_t query_seed, size_t q, uint64_t npow2, size_t n0) {
+    uint64_t fl[3] = {query_seed, 0, gl_from_u64(q)};
+    uint64_t seed = tr_hash_fields_tagged("FRI/index", fl, 3);
+    return (ref_stdrng_first_u64(seed) & (npow2 - 1)) % n0;
+}
+
+/* Classic-FRI openings (fri_prove_queries, fri.rs:1724-1807, and the payloads
+ * of deep_fri_prove :2008-2032): the f0 tree and every layer tree are rebuilt
+ * as committed; r queries, each followed down the layers by i <- i mod n/m.
+ * Writes layer_proofs, f0_openings and queries of the DeepFriProof. */
+static void classic_queries(const fri_state *s, const uint64_t *f0, const ref_fri_params *p, uint64_t npow2,
+                            bbuf *b, uint64_t *raw_out) {
+    const size_t L = s->L, n0 = s->n0, r = p->r;
+    const int use_cc = p->coeff_commit_final && L > 0;
+    size_t *idx = (size_t *)malloc(sizeof(size_t) * (r ? r : 1) * (L + 1));
+    for (size_t q = 0; q < r; q++) {
+        size_t i = query_raw(s->query_seed, q, npow2, n0);
+        if (raw_out) raw_out[q] = i;
+        for (size_t ell = 0; ell < L; ell++) {
+            idx[q * (L + 1) + ell] = i;
+            i %= s->sz[ell] / p->schedule[ell];
+        }
+        idx[q * (L + 1) + L] = i;
+    }
+    mtree tf;
+    {
+        size_t a = pick_arity2(n0, 16), d = merkle_depth(n0, a);
+        uint8_t rt[32];
+        mt_init(&tf, a, d, 0xFF, n0);
+        hash_leaves(&tf, n0, f0, 1, 1);
+        mt_finalize(&tf, rt);
+    }
+    /* layer_proofs.layers: per layer, the r openings */
+    bb_u64(b, L);
+    for (size_t ell = 0; ell < L; ell++) {
+        size_t cur = s->sz[ell], m = p->schedule[ell];
+        int final_cc = use_cc && ell == L - 1;
+        size_t a = pick_arity2(cur, m), d = merkle_depth(cur, a);
+        uint64_t *lv = (uint64_t *)malloc(8 * 18 * cur);
+        uint8_t rt[32];
+        mtree t;
+        classic_leaves(s, ell, cur, m, final_cc, lv);
+        mt_init(&t, a, d, ell, cur);
+        hash_leaves(&t, cur, lv, 18, 18);
+        mt_finalize(&t, rt);
+        free(lv);
+        bb_u64(b, r);
+        for (size_t q = 0; q < r; q++) mt_open_ser(&t, idx[q * (L + 1) + ell], b);
+        mt_free(&t);
+    }
+    /* f0_openings */
+    bb_u64(b, r);
+    for (size_t q = 0; q < r; q++) mt_open_ser(&tf, idx[q * (L + 1)], b);
+    /* queries: FriQueryPayload { per_layer_refs, per_layer_payloads, f0_opening, final_index } */
+    bb_u64(b, r);
+    for (size_t q = 0; q < r; q++) {
+        bb_u64(b, L);
+        for (size_t ell = 0; ell < L; ell++) {
+            size_t i = idx[q * (L + 1) + ell], m = p->schedule[ell], nn = s->sz[ell] / m;
+            bb_u64(b, i);
+            bb_u64(b, i % m);
+            bb_u64(b, i % nn);
+            bb_u64(b, 0);
+        }
+        bb_u64(b, L);
+        for (size_t ell = 0; ell < L; ell++) {
+            size_t i = idx[q * (L + 1) + ell], nn = s->sz[ell] / p->schedule[ell];
+            int final_cc = use_cc && ell == L - 1;
+            bb_fp6(b, s->f[ell][i]);
+            bb_fp6(b, final_cc ? fp6_zero() : s->f[ell + 1][i % nn]);
+            bb_fp6(b, s->q[ell][i]);
+        }
+        mt_open_ser(&tf, idx[q * (L + 1)], b);
+        bb_u64(b, idx[q * (L + 1) + L]);
+    }
+    mt_free(&tf);
+    free(idx);
+}
+
 int ref_deep_fri_prove_raw(const uint64_t *f0, size_t n0, const ref_fri_params *p, uint8_t **out, size_t *out_len,
                            uint64_t *raw_out) {
     fri_state s;
@@ -491,9 +669,15 @@ int ref_deep_fri_prove_raw(const uint64_t *f0, size_t n0, const ref_fri_params *
     bb_put(&b, s.root_f0, 32);
     bb_u64(&b, L);
     for (size_t l = 0; l < L; l++) bb_put(&b, s.roots[l], 32);
-    bb_u64(&b, 0); /* layer_proofs.layers */
-    bb_u64(&b, 0); /* f0_openings */
-    bb_u64(&b, 0); /* queries */
+    uint64_t npow2 = 1;
+    while (npow2 < n0) npow2 <<= 1;
+    if (p->stir) {
+        bb_u64(&b, 0); /* layer_proofs.layers */
+        bb_u64(&b, 0); /* f0_openings */
+        bb_u64(&b, 0); /* queries */
+    } else {
+        classic_queries(&s, f0, p, npow2, &b, NULL); /* the raw indices are the STIR queries' */
+    }
     bb_u64(&b, L);
     for (size_t l = 0; l < L; l++) bb_fp6(&b, s.fz[l]);
     bb_u64(&b, s.n_final_coeffs);
@@ -512,6 +696,14 @@ int ref_deep_fri_prove_raw(const uint64_t *f0, size_t n0, const ref_fri_params *
     } else {
         bb_u8(&b, 0);
         bb_u8(&b, 0);
+    }
+    if (!p->stir) {
+        bb_u8(&b, 0); /* stir_coset_evals: None */
+        bb_u8(&b, 0); /* stir_proximity_queries: None */
+        fs_free(&s);
+        *out = b.p;
+        *out_len = b.len;
+        return 0;
     }
     bb_u8(&b, 1);
     bb_u64(&b, L);
@@ -540,8 +732,6 @@ int ref_deep_fri_prove_raw(const uint64_t *f0, size_t n0, const ref_fri_params *
     }
     bb_u8(&b, 1);
     bb_u64(&b, p->s0);
-    uint64_t npow2 = 1;
-    while (npow2 < n0) npow2 <<= 1;
     for (size_t q = 0; q < p->s0; q++) {
         uint64_t fl[3] = {s.query_seed, 0, gl_from_u64(q)};
         uint64_t seed = tr_hash_fields_tagged("FRI/index", fl, 3);

@@ -79,6 +79,19 @@ int ref_deep_fri_prove(const uint64_t *f0, size_t n0, const ref_fri_params *p, u
  * f_1 .. f_L (Fp6, concatenated) and the layer roots; sizes follow the schedule. */
 int ref_fri_layers(const uint64_t *f0, size_t n0, const ref_fri_params *p, uint64_t *layers_out, uint8_t *roots_out, uint8_t root_f0[32]);
 void ref_free(void *p);
+/* deep_fri_verify (fri.rs:2136-2710) over the serialized proof: 1 accept,
+ * 0 reject (also where the reference panics on a malformed proof), -1 when
+ * the bytes do not deserialize as DeepFriProof<SexticExt> */
+int ref_deep_fri_verify(const ref_fri_params *p, const uint8_t *proof, size_t len);
+/* MerkleTreeChannel::verify_opening (merkle lib.rs:754-799) for a tree of
+ * constant arity and `depth` levels; opening = serialized MerkleOpening.
+ * 1 / 0, -1 if the bytes do not deserialize */
+int ref_merkle_verify_opening(size_t arity, size_t depth, uint64_t label, const uint8_t root[32],
+                              const uint8_t *opening, size_t len);
+/* MerkleTreeChannel::open (merkle lib.rs:478-512) of leaf `index` of the
+ * tree of ref_merkle_root; *out (malloc'd) = serialized MerkleOpening */
+int ref_merkle_open(const uint64_t *leaf_vals, size_t n_leaves, size_t vals_per_leaf, size_t arity, size_t depth,
+                    uint64_t label, size_t index, uint8_t **out, size_t *out_len);
 
 /* ---- swarm-dns prover (crates/swarm-dns/src/prover.rs, dns.rs) ---- */
 /* Synthetic zone record i (swarm-ctrl main.rs:1331-1357), leaf hash h2 (dns.rs:72-89). */

@@ -163,6 +163,13 @@ def lib():
         L.ref_prove_zsk_ksk_binding_v2.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz,
                                                    ctypes.c_char_p, ctypes.c_char_p, sz, sz, ctypes.POINTER(u8p),
                                                    ctypes.POINTER(sz), ctypes.c_char_p, ctypes.c_char_p]
+        L.ref_deep_fri_verify.restype = ctypes.c_int
+        L.ref_deep_fri_verify.argtypes = [ctypes.POINTER(FriParams), ctypes.c_char_p, sz]
+        L.ref_merkle_verify_opening.restype = ctypes.c_int
+        L.ref_merkle_verify_opening.argtypes = [sz, sz, ctypes.c_uint64, ctypes.c_char_p, ctypes.c_char_p, sz]
+        L.ref_merkle_open.restype = ctypes.c_int
+        L.ref_merkle_open.argtypes = [u64p, sz, sz, sz, sz, ctypes.c_uint64, sz, ctypes.POINTER(u8p),
+                                      ctypes.POINTER(sz)]
         _lib = L
     return _lib
 
@@ -223,6 +230,54 @@ def deep_fri_prove(f0, params):
     if rc:
         raise ValueError("deep_fri_prove rc=%d" % rc)
     return take_bytes(pp, pl)
+
+
+def deep_fri_verify(params, proof):
+    """1 accept, 0 reject, -1 bytes that do not deserialize (fri.rs:2136-2710)"""
+    return lib().ref_deep_fri_verify(ctypes.byref(params), bytes(proof), len(proof))
+
+
+def merkle_root(vals, arity, label, depth=None):
+    vals = np.ascontiguousarray(vals, dtype=np.uint64)
+    n, vpl = vals.shape
+    if depth is None:
+        depth = merkle_depth(n, arity)
+    out = ctypes.create_string_buffer(32)
+    lib().ref_merkle_root(ptr(vals), n, vpl, arity, depth, label, out)
+    return out.raw
+
+
+def merkle_open(vals, arity, label, index, depth=None):
+    vals = np.ascontiguousarray(vals, dtype=np.uint64)
+    n, vpl = vals.shape
+    if depth is None:
+        depth = merkle_depth(n, arity)
+    pp, pl = u8p(), sz()
+    rc = lib().ref_merkle_open(ptr(vals), n, vpl, arity, depth, label, index, ctypes.byref(pp), ctypes.byref(pl))
+    if rc:
+        raise ValueError("merkle_open rc=%d" % rc)
+    return take_bytes(pp, pl)
+
+
+def merkle_verify_opening(arity, depth, label, root, opening):
+    return lib().ref_merkle_verify_opening(arity, depth, label, root, bytes(opening), len(opening))
+
+
+def merkle_depth(leaves, arity):
+    """fri.rs:950-959"""
+    d, cur = 1, leaves
+    while cur > arity:
+        cur = (cur + arity - 1) // arity
+        d += 1
+    return d
+
+
+def pick_arity(n, m):
+    """pick_arity_for_layer(n, m).max(2) (fri.rs:1094-1103)"""
+    for a in (128, 64, 32, 16, 8, 4):
+        if m >= a and n % a == 0:
+            return a
+    return 2
 
 
 def zone_leaf_hashes(first, count, salt):
