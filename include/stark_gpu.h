@@ -41,7 +41,7 @@ extern "C" {
 #define STARK_PHI_ONE_THREAD 2 /* parallel build, 1 thread: phi always 0 */
 
 #define STARK_LDT_STIR 0 /* LdtMode::Stir (crates/swarm-dns/src/prover.rs:37-41) */
-#define STARK_LDT_FRI 1  /* LdtMode::Fri: not implemented, returns STARK_E_UNSUPPORTED */
+#define STARK_LDT_FRI 1  /* LdtMode::Fri: classic DEEP-FRI with arity-2 folds (prover.rs:53-67) */
 
 /* AirType ids: the variant's position (1-based) in crates/deep_ali/src/air_workloads.rs:20-140 */
 #define STARK_AIR_HASH_ROLLUP 5 /* AirType::HashRollup (air_workloads.rs:83) */
@@ -224,6 +224,40 @@ int stark_deep_fri_prove(stark_ctx *ctx, const uint64_t *f0, size_t n0, const st
 int stark_merkle_commit(stark_ctx *ctx, const uint64_t *leaf_vals, size_t n_leaves, size_t vals_per_leaf,
                         size_t arity, uint64_t tree_label, uint8_t root[32]);
 
+/* MerkleTreeChannel::{new, push_leaves_parallel, finalize, open}
+ * (crates/merkle/src/lib.rs:365-512) with constant arity: builds the tree of
+ * stark_merkle_commit on the device and opens leaves indices[0..n_idx).
+ * *openings (library-allocated, stark_free() it) holds the serialized
+ * MerkleOpening of each index (leaf digest, sibling path, index; ark-serialize
+ * compressed) back to back, opening q at [offsets[q], offsets[q + 1]). */
+int stark_merkle_open(stark_ctx *ctx, const uint64_t *leaf_vals, size_t n_leaves, size_t vals_per_leaf, size_t arity,
+                      uint64_t tree_label, const size_t *indices, size_t n_idx, uint8_t root[32], uint8_t **openings,
+                      size_t *openings_len, size_t *offsets /* n_idx + 1 */);
+
+/* MerkleTreeChannel::verify_opening (crates/merkle/src/lib.rs:754-799) for a
+ * tree of `depth` levels of constant arity: 1 valid, 0 not, STARK_E_INVALID
+ * when the opening bytes do not deserialize. Host code, no context. */
+int stark_merkle_verify_opening(size_t arity, size_t depth, uint64_t tree_label, const uint8_t root[32],
+                                const uint8_t *opening, size_t len);
+/* compute_leaf_hash (crates/merkle/src/lib.rs:806-812) */
+void stark_host_merkle_leaf_hash(size_t arity, uint64_t tree_label, size_t index, const uint64_t *vals, size_t n,
+                                 uint8_t out[32]);
+
+/* deep_fri_verify::<SexticExt> (crates/deep_ali/src/fri.rs:2136-2710) on the
+ * ark-serialize compressed DeepFriProof bytes, STIR and classic FRI: 1
+ * accept, 0 reject, STARK_E_INVALID when the bytes do not deserialize (the
+ * reference's deserialize_with_mode error, swarm-verify/src/main.rs:243-245).
+ * A proof that would make the reference panic is rejected. Host code (the
+ * reference's verifier runs on the host too), no context. */
+int stark_deep_fri_verify(const stark_fri_params *params, const uint8_t *proof, size_t proof_len);
+/* n proofs under the same parameters on the library's host thread pool;
+ * public_inputs_hashes (32 bytes per proof, or NULL for
+ * params->public_inputs_hash) binds proof i to its own hash. results[i] as
+ * stark_deep_fri_verify. No single counterpart in the reference (a loop of
+ * deep_fri_verify calls, swarm-verify/src/main.rs:266). */
+int stark_deep_fri_verify_batch(const stark_fri_params *params, const uint8_t *const *proofs, const size_t *lens,
+                                size_t n, const uint8_t *public_inputs_hashes, int *results);
+
 /* prove_inner_shard (crates/swarm-dns/src/prover.rs:148-227). The local
  * self-verify of the reference is not run (proof bytes are byte-identical). */
 int stark_prove_inner_shard(stark_ctx *ctx, const uint8_t salt[16], const stark_dns_record *records, size_t n,
@@ -371,14 +405,14 @@ int stark_ed25519_shape(size_t msg_len, size_t k_scalar, size_t *width, size_t *
  * msgs[msg_off[i] .. msg_off[i] + msg_len[i]], bound to fs_bindings[32 i..]
  * and merkle_roots[32 i..]. k_scalar 256 and blowup 32 with 54 queries are the
  * reference's; a smaller k_scalar is its layout-test stub, another blowup /
- * query count a parameter choice (the STIR schedule follows n0). Lockstep
+ * query count a parameter choice (the fold schedule follows n0 and ldt). Lockstep
  * device batches per trace shape in HBM-sized waves; result i equals the
  * proof of signature i alone. STARK_E_INVALID when a signature does not
  * verify (the reference asserts). */
 int stark_prove_zsk_ksk_binding_v2_batch(stark_ctx *ctx, const uint8_t *pks, const uint8_t *sigs, const uint8_t *msgs,
                                          const size_t *msg_off, const size_t *msg_len, const uint8_t *fs_bindings,
                                          const uint8_t *merkle_roots, size_t n, size_t k_scalar, size_t blowup,
-                                         size_t queries, stark_zsk_out *outs);
+                                         size_t queries, int ldt, stark_zsk_out *outs);
 
 /* deep_ali_merge_ed25519_verify_streaming (crates/deep_ali/src/lib.rs:854-974)
  * for one signature's layout: lde is width x (height * blowup), coeffs the

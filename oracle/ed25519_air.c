@@ -1548,7 +1548,7 @@ int ref_prove_zsk_ksk_binding_v2(const uint8_t pk[32], const uint8_t sig[64], co
     p.seed_z = 0xDEEFBAADULL;
     p.coeff_commit_final = 1;
     p.d_final = 1;
-    p.stir = 1;
+    p.stir = ref_ldt_stir();
     p.public_inputs_hash = fs_binding;
     int rc = ref_deep_fri_prove(c, n0, &p, proof, proof_len);
     free(c);

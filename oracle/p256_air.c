@@ -1023,7 +1023,7 @@ int ref_prove_p256_ecdsa(const uint8_t qx[32], const uint8_t qy[32], const uint8
     size_t sched[24];
     ref_fri_params p;
     p.schedule = sched;
-    p.n_layers = ref_make_schedule(n0, sched);
+    p.n_layers = ref_stir_schedule(n0, sched) /* the examples run STIR only */;
     p.r = 54;
     p.s0 = 54;
     p.seed_z = 0xDEEFBAADULL;

@@ -228,10 +228,28 @@ void ref_shard_fs_binding(const uint8_t pk_hash[32], const uint8_t job_id[32], u
 #define SWARM_QUERIES 54
 #define SWARM_SEED_Z 0xDEEFBAADULL
 
+/* LdtMode of the swarm-dns provers below (prover.rs:37-67): 0 Stir, 1 Fri */
+static int g_ldt_fri = 0;
+void ref_set_ldt(int fri) { g_ldt_fri = fri != 0; }
+int ref_ldt_stir(void) { return !g_ldt_fri; }
+
+size_t ref_stir_schedule(size_t n0, size_t *s) {
+    int lg = 0;
+    while (((size_t)1 << lg) < n0) lg++;
+    size_t k = 0;
+    for (int i = 0; i < lg / 3; i++) s[k++] = 8;
+    if (lg % 3) s[k++] = (size_t)1 << (lg % 3);
+    return k;
+}
+
 size_t ref_make_schedule(size_t n0, size_t *s) {
     int lg = 0;
     while (((size_t)1 << lg) < n0) lg++;
     size_t k = 0;
+    if (g_ldt_fri) {
+        for (int i = 0; i < lg; i++) s[k++] = 2; /* arity-2 binary folds */
+        return k;
+    }
     for (int i = 0; i < lg / 3; i++) s[k++] = 8;
     if (lg % 3) s[k++] = (size_t)1 << (lg % 3);
     return k;
@@ -254,7 +272,7 @@ static int rollup_prove(const uint64_t *leaves, size_t n_active, const uint8_t b
     p.schedule = sched;
     p.n_layers = ref_make_schedule(n0, sched);
     p.r = SWARM_QUERIES; p.s0 = SWARM_QUERIES; p.seed_z = SWARM_SEED_Z;
-    p.coeff_commit_final = 1; p.d_final = 1; p.stir = 1;
+    p.coeff_commit_final = 1; p.d_final = 1; p.stir = ref_ldt_stir();
     p.public_inputs_hash = binding;
     int rc = ref_deep_fri_prove(c, n0, &p, proof, proof_len);
     if (rc == 0) memcpy(root_f0, *proof, 32);
@@ -388,7 +406,7 @@ int ref_prove_nsec3_completeness(const uint8_t *records, size_t n, const uint8_t
     p.schedule = sched;
     p.n_layers = ref_make_schedule(n0, sched);
     p.r = SWARM_QUERIES; p.s0 = SWARM_QUERIES; p.seed_z = SWARM_SEED_Z;
-    p.coeff_commit_final = 1; p.d_final = 1; p.stir = 1;
+    p.coeff_commit_final = 1; p.d_final = 1; p.stir = ref_ldt_stir();
     p.public_inputs_hash = fs_binding;
     int rc = ref_deep_fri_prove(ce, n0, &p, proof, proof_len);
     free(chain); free(tr); free(lde); free(ce);

@@ -109,8 +109,14 @@ int ref_prove_outer_rollup(const uint8_t *pi_hashes, size_t count, const uint8_t
 
 /* c <- FFT(pad(poly_div_zh(iFFT(c), n_trace))), remainder dropped (lib.rs:190-228, 421-429) */
 void ref_quotient_from_phi(uint64_t *phi, size_t n, size_t n_trace);
-/* make_schedule(n0, Stir): 8s then the 2^(log n0 mod 3) residual; returns the layer count */
+/* LdtMode of the swarm-dns provers (prove_inner_shard, prove_outer_rollup,
+ * prove_nsec3_completeness, prove_ds_ksk_binding, prove_zsk_ksk_binding_v2):
+ * 0 Stir (the default), 1 Fri (classic DEEP-FRI with arity-2 folds) */
+void ref_set_ldt(int fri);
+int ref_ldt_stir(void);
+/* make_schedule(n0, ldt): Stir = 8s then the 2^(log n0 mod 3) residual, Fri = log2(n0) 2s; returns the layer count */
 size_t ref_make_schedule(size_t n0, size_t *s);
+size_t ref_stir_schedule(size_t n0, size_t *s); /* make_schedule(n0, Stir) whatever ref_set_ldt says */
 
 /* ---- SHA-256 AIR and prove_ds_ksk_binding (sha256_air.rs, lib.rs:453-545, prover.rs:446-585) ---- */
 size_t ref_sha256_n_blocks(size_t len);

@@ -386,7 +386,7 @@ int ref_prove_ds_ksk_binding(const uint8_t *dnskey, size_t len, const uint8_t pa
     p.seed_z = 0xDEEFBAADULL;
     p.coeff_commit_final = 1;
     p.d_final = 1;
-    p.stir = 1;
+    p.stir = ref_ldt_stir();
     p.public_inputs_hash = fs_binding;
     int rc = ref_deep_fri_prove(c, n0, &p, proof, proof_len);
     free(c);

@@ -34,6 +34,9 @@ Rust functions they replace:
   prove_record_ecdsa_batch, EcdsaRecord
                            crates/swarm-dns/examples/mixed_algorithm_zone_rollup.rs:193-322
   DnsRecord                crates/swarm-dns/src/dns.rs:24-89
+  deep_fri_verify          crates/deep_ali/src/fri.rs:2136-2710 (host code)
+  MerkleChannelCfg, merkle_open, verify_opening, compute_leaf_hash
+                           crates/merkle/src/lib.rs:117-128,478-512,754-812
   shard_fs_binding         crates/swarm-proto/src/messages.rs:162-176
 
 There is no CPU fallback: importing works without a GPU, but every call that
@@ -60,6 +63,12 @@ from .api import (  # noqa: F401
     PHI_ONE_THREAD,
     PHI_SEQUENTIAL,
     build_params,
+    deep_fri_verify,
+    deep_fri_verify_batch,
+    MerkleChannelCfg,
+    merkle_open,
+    verify_opening,
+    compute_leaf_hash,
     set_autotune_default,
     autotune_report,
     deep_ali_merge_general,

@@ -180,7 +180,7 @@ def lib():
         "stark_prove_zsk_ksk_binding_v2_batch": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
                                                                 ctypes.POINTER(sz), ctypes.POINTER(sz),
                                                                 ctypes.c_char_p, ctypes.c_char_p, sz, sz, sz, sz,
-                                                                ctypes.POINTER(_ZskOut)]),
+                                                                ctypes.c_int, ctypes.POINTER(_ZskOut)]),
         "stark_merge_ed25519": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz, sz, u64p,
                                                sz, u64p, sz, u64p]),
         "stark_build_ed25519_trace": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz, sz,
@@ -189,6 +189,16 @@ def lib():
                                                           u64p]),
         "stark_host_ed25519_verify": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz]),
         "stark_host_ed25519_sign": (None, [ctypes.c_char_p, ctypes.c_char_p, sz, ctypes.c_char_p, ctypes.c_char_p]),
+        "stark_merkle_open": (ctypes.c_int, [vp, u64p, sz, sz, sz, ctypes.c_uint64, ctypes.POINTER(sz), sz,
+                                             ctypes.c_char_p, ctypes.POINTER(u8p), ctypes.POINTER(sz),
+                                             ctypes.POINTER(sz)]),
+        "stark_merkle_verify_opening": (ctypes.c_int, [sz, sz, ctypes.c_uint64, ctypes.c_char_p, ctypes.c_char_p,
+                                                       sz]),
+        "stark_host_merkle_leaf_hash": (None, [sz, ctypes.c_uint64, sz, u64p, sz, ctypes.c_char_p]),
+        "stark_deep_fri_verify": (ctypes.c_int, [ctypes.POINTER(_FriParams), ctypes.c_char_p, sz]),
+        "stark_deep_fri_verify_batch": (ctypes.c_int, [ctypes.POINTER(_FriParams), ctypes.POINTER(ctypes.c_char_p),
+                                                       ctypes.POINTER(sz), sz, ctypes.c_char_p,
+                                                       ctypes.POINTER(ctypes.c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -866,6 +876,92 @@ def merkle_commit(leaf_values, arity, tree_label, ctx=None):
     return root.raw
 
 
+def merkle_depth(leaves, arity):
+    """merkle_depth (fri.rs:950-959)."""
+    d, cur = 1, leaves
+    while cur > arity:
+        cur = (cur + arity - 1) // arity
+        d += 1
+    return d
+
+
+@dataclass
+class MerkleChannelCfg:
+    """MerkleChannelCfg (merkle lib.rs:117-128) with constant layer arity."""
+    layer_arities: List[int]
+    tree_label: int
+
+    @classmethod
+    def constant(cls, n_leaves, arity, tree_label):
+        return cls([arity] * merkle_depth(n_leaves, arity), tree_label)
+
+    def _arity(self):
+        assert self.layer_arities and all(a == self.layer_arities[0] for a in self.layer_arities), \
+            "this library builds constant-arity trees"
+        return self.layer_arities[0]
+
+
+def merkle_open(leaf_values, arity, tree_label, indices, ctx=None):
+    """MerkleTreeChannel push_leaves_parallel + finalize + open (merkle lib.rs:391-512): the root and, per
+    index, the serialized MerkleOpening (leaf digest, sibling path, index)."""
+    ctx = ctx or default_context()
+    v = np.ascontiguousarray(np.atleast_2d(leaf_values), dtype=np.uint64)
+    idx = (sz * max(1, len(indices)))(*indices)
+    root = ctypes.create_string_buffer(32)
+    pp, pl = u8p(), sz()
+    offs = (sz * (len(indices) + 1))()
+    ctx._check(lib().stark_merkle_open(ctx.h, v.ctypes.data_as(u64p), v.shape[0], v.shape[1], arity, tree_label,
+                                       idx, len(indices), root, ctypes.byref(pp), ctypes.byref(pl), offs))
+    blob = _take(pp, pl.value)
+    return root.raw, [blob[offs[q]:offs[q + 1]] for q in range(len(indices))]
+
+
+def verify_opening(cfg, root, opening):
+    """MerkleTreeChannel::verify_opening (merkle lib.rs:754-799) on a serialized MerkleOpening."""
+    rc = lib().stark_merkle_verify_opening(cfg._arity(), len(cfg.layer_arities), cfg.tree_label, bytes(root),
+                                           bytes(opening), len(opening))
+    if rc < 0:
+        raise StarkError(rc, "merkle opening does not deserialize")
+    return rc == 1
+
+
+def compute_leaf_hash(cfg, index, values):
+    """compute_leaf_hash (merkle lib.rs:806-812)."""
+    v = np.ascontiguousarray(values, dtype=np.uint64)
+    out = ctypes.create_string_buffer(32)
+    lib().stark_host_merkle_leaf_hash(cfg.layer_arities[0], cfg.tree_label, index, v.ctypes.data_as(u64p), v.size,
+                                      out)
+    return out.raw
+
+
+def deep_fri_verify(params, proof):
+    """deep_fri_verify::<SexticExt> (fri.rs:2136-2710) on the ark-serialize compressed proof: True / False.
+    Bytes that do not deserialize raise StarkError (the reference's deserialize_with_mode error)."""
+    p = params._c()
+    rc = lib().stark_deep_fri_verify(ctypes.byref(p), bytes(proof), len(proof))
+    if rc < 0:
+        raise StarkError(rc, "proof bytes do not deserialize as DeepFriProof<SexticExt>")
+    return rc == 1
+
+
+def deep_fri_verify_batch(params, proofs, public_inputs_hashes=None):
+    """deep_fri_verify over many proofs on the library's host threads; a list of True / False / None (None:
+    the bytes do not deserialize). public_inputs_hashes binds proof i to its own 32-byte hash."""
+    p = params._c()
+    n = len(proofs)
+    bufs = [bytes(x) for x in proofs]
+    arr = (ctypes.c_char_p * max(1, n))(*bufs)
+    lens = (sz * max(1, n))(*[len(x) for x in bufs])
+    res = (ctypes.c_int * max(1, n))()
+    pis = b"".join(public_inputs_hashes) if public_inputs_hashes is not None else None
+    if pis is not None:
+        assert len(pis) == 32 * n
+    rc = lib().stark_deep_fri_verify_batch(ctypes.byref(p), arr, lens, n, pis, res)
+    if rc < 0:
+        raise StarkError(rc, "deep_fri_verify_batch failed")
+    return [True if res[i] == 1 else (False if res[i] == 0 else None) for i in range(n)]
+
+
 # --------------------------------------------------------------------- swarm-dns
 class LdtMode(enum.Enum):
     Stir = _LDT_STIR
@@ -1258,7 +1354,7 @@ class ZskKskOutput:
 
 
 def prove_zsk_ksk_binding_v2_batch(requests, k_scalar=ED25519_K_SCALAR, blowup=BLOWUP, queries=NUM_QUERIES,
-                                   ctx=None):
+                                   ldt=LdtMode.Stir, ctx=None):
     """prove_zsk_ksk_binding_v2 for each (ksk_pubkey, signature, signed_data,
     fs_binding_32, merkle_root_32) request in one batched call; raises
     StarkError when a signature does not verify (the reference asserts)."""
@@ -1275,7 +1371,7 @@ def prove_zsk_ksk_binding_v2_batch(requests, k_scalar=ED25519_K_SCALAR, blowup=B
     ctx._check(lib().stark_prove_zsk_ksk_binding_v2_batch(
         ctx.h, b"".join(bytes(r[0]) for r in reqs), b"".join(bytes(r[1]) for r in reqs), b"".join(msgs) or b"\0",
         (sz * n)(*off), (sz * n)(*[len(m) for m in msgs]), b"".join(bytes(r[3]) for r in reqs),
-        b"".join(bytes(r[4]) for r in reqs), n, k_scalar, blowup, queries, outs))
+        b"".join(bytes(r[4]) for r in reqs), n, k_scalar, blowup, queries, ldt.value, outs))
     res = []
     for x in outs:
         blob = _take(x.proof, x.proof_len)
@@ -1284,8 +1380,7 @@ def prove_zsk_ksk_binding_v2_batch(requests, k_scalar=ED25519_K_SCALAR, blowup=B
 
 
 def prove_zsk_ksk_binding_v2(ksk_pubkey, signature, signed_data, fs_binding_32, merkle_root_32,
-                             k_scalar=ED25519_K_SCALAR, ldt=None, ctx=None):
-    """prove_zsk_ksk_binding_v2 (prover.rs:856-951) for one signature (STIR)."""
-    assert ldt in (None, LdtMode.Stir), "the GPU prover implements the STIR mode the swarm uses"
+                             k_scalar=ED25519_K_SCALAR, ldt=LdtMode.Stir, ctx=None):
+    """prove_zsk_ksk_binding_v2 (prover.rs:856-951) for one signature."""
     return prove_zsk_ksk_binding_v2_batch([(ksk_pubkey, signature, signed_data, fs_binding_32, merkle_root_32)],
-                                          k_scalar, ctx=ctx)[0]
+                                          k_scalar, ldt=ldt, ctx=ctx)[0]

@@ -21,6 +21,7 @@
 #include "rsa_air.hpp"
 #include "server.hpp"
 #include "sha256_air.hpp"
+#include "verify.hpp"
 
 struct stark_ctx {
     sg::Ctx *c = nullptr;
@@ -190,16 +191,91 @@ int stark_merkle_commit(stark_ctx *ctx, const uint64_t *leaf_vals, size_t n_leav
     });
 }
 
+int stark_merkle_open(stark_ctx *ctx, const uint64_t *leaf_vals, size_t n_leaves, size_t vals_per_leaf, size_t arity,
+                      uint64_t tree_label, const size_t *indices, size_t n_idx, uint8_t root[32], uint8_t **openings,
+                      size_t *openings_len, size_t *offsets) {
+    return guard(ctx, [&](sg::Ctx *c) {
+        SG_REQUIRE(leaf_vals && root && openings && openings_len && offsets && (indices || n_idx == 0) &&
+                       n_leaves >= 1 && arity >= 2 && arity <= 128 && vals_per_leaf >= 1,
+                   "merkle_open: bad arguments");
+        std::vector<std::vector<uint64_t>> idx(1);
+        for (size_t q = 0; q < n_idx; q++) {
+            SG_REQUIRE(indices[q] < n_leaves, "merkle_open: leaf index out of range");
+            idx[0].push_back(indices[q]);
+        }
+        sg::Scoped d(c, 8 * n_leaves * vals_per_leaf);
+        h2d(c, d.p, leaf_vals, 8 * n_leaves * vals_per_leaf);
+        sg::DevTree t;
+        sg::LeafSrc src{d.as<uint64_t>(), vals_per_leaf, 1, (uint32_t)vals_per_leaf, (uint32_t)vals_per_leaf};
+        sg::tree_build(c, t, src, n_leaves, (uint32_t)arity, sg::merkle_depth(n_leaves, (uint32_t)arity), tree_label);
+        memcpy(root, t.root, 32);
+        std::vector<std::vector<std::vector<uint8_t>>> ops;
+        if (n_idx) sg::tree_open_multi(c, t, idx, ops);
+        sg::tree_free(c, t);
+        std::vector<uint8_t> all;
+        offsets[0] = 0;
+        for (size_t q = 0; q < n_idx; q++) {
+            all.insert(all.end(), ops[0][q].begin(), ops[0][q].end());
+            offsets[q + 1] = all.size();
+        }
+        *openings = copy_out(all, openings_len);
+    });
+}
+
+int stark_merkle_verify_opening(size_t arity, size_t depth, uint64_t tree_label, const uint8_t root[32],
+                                const uint8_t *opening, size_t len) {
+    if (!root || !opening || arity < 2 || arity > 128 || depth > 64) return STARK_E_INVALID;
+    try {
+        return sg::merkle_verify_opening((uint32_t)arity, (uint32_t)depth, tree_label, root, opening, len);
+    } catch (...) {
+        return STARK_E_INTERNAL;
+    }
+}
+
+void stark_host_merkle_leaf_hash(size_t arity, uint64_t tree_label, size_t index, const uint64_t *vals, size_t n,
+                                 uint8_t out[32]) {
+    sg::host_leaf_hash((uint32_t)arity, tree_label, index, vals, n, out);
+}
+
+int stark_deep_fri_verify(const stark_fri_params *params, const uint8_t *proof, size_t proof_len) {
+    if (!proof) return STARK_E_INVALID;
+    try {
+        return sg::deep_fri_verify(to_params(params), proof, proof_len);
+    } catch (const sg::Error &e) {
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        return STARK_E_OOM;
+    } catch (...) {
+        return STARK_E_INTERNAL;
+    }
+}
+
+int stark_deep_fri_verify_batch(const stark_fri_params *params, const uint8_t *const *proofs, const size_t *lens,
+                                size_t n, const uint8_t *public_inputs_hashes, int *results) {
+    if ((n && (!proofs || !lens || !results))) return STARK_E_INVALID;
+    try {
+        const sg::FriParams p = to_params(params);
+        for (size_t i = 0; i < n; i++)
+            if (!proofs[i]) return STARK_E_INVALID;
+        sg::deep_fri_verify_batch(p, proofs, lens, n, public_inputs_hashes, results);
+        return STARK_OK;
+    } catch (const sg::Error &e) {
+        return e.code;
+    } catch (...) {
+        return STARK_E_INTERNAL;
+    }
+}
+
 int stark_prove_inner_shard(stark_ctx *ctx, const uint8_t salt[16], const stark_dns_record *records, size_t n,
                             const uint8_t fs_binding[32], int ldt, int phi_mode, stark_inner_shard_out *out) {
     return guard(ctx, [&](sg::Ctx *c) {
         SG_REQUIRE(salt && records && fs_binding && out, "prove_inner_shard: null argument");
-        if (ldt != STARK_LDT_STIR) throw sg::Error(sg::E_UNSUPPORTED, "only LdtMode::Stir is implemented");
+        SG_REQUIRE(ldt == STARK_LDT_STIR || ldt == STARK_LDT_FRI, "unknown LdtMode");
         std::vector<sg::DnsRec> recs(n);
         for (size_t i = 0; i < n; i++)
             recs[i] = {records[i].domain, records[i].domain_len, records[i].record_type, records[i].ttl,
                        records[i].rdata, records[i].rdata_len};
-        sg::InnerShard s = sg::prove_inner_shard(c, salt, recs, fs_binding, phi_mode);
+        sg::InnerShard s = sg::prove_inner_shard(c, salt, recs, fs_binding, phi_mode, ldt == STARK_LDT_STIR);
         memcpy(out->pi_hash, s.pi_hash, 32);
         memcpy(out->merkle_root, s.merkle_root, 32);
         memcpy(out->root_f0, s.root_f0, 32);
@@ -215,7 +291,7 @@ int stark_prove_inner_shard_batch(stark_ctx *ctx, const uint8_t salt[16], const 
     return guard(ctx, [&](sg::Ctx *c) {
         SG_REQUIRE(salt && shard_sizes && fs_bindings && outs && (records || n_shards == 0),
                    "prove_inner_shard_batch: null argument");
-        if (ldt != STARK_LDT_STIR) throw sg::Error(sg::E_UNSUPPORTED, "only LdtMode::Stir is implemented");
+        SG_REQUIRE(ldt == STARK_LDT_STIR || ldt == STARK_LDT_FRI, "unknown LdtMode");
         std::vector<std::vector<sg::DnsRec>> shards(n_shards);
         std::vector<std::array<uint8_t, 32>> fsb(n_shards);
         size_t off = 0;
@@ -227,7 +303,7 @@ int stark_prove_inner_shard_batch(stark_ctx *ctx, const uint8_t salt[16], const 
             off += shard_sizes[s];
             memcpy(fsb[s].data(), fs_bindings + 32 * s, 32);
         }
-        std::vector<sg::InnerShard> res = sg::prove_inner_shard_batch(c, salt, shards, fsb, phi_mode);
+        std::vector<sg::InnerShard> res = sg::prove_inner_shard_batch(c, salt, shards, fsb, phi_mode, ldt == STARK_LDT_STIR);
         for (size_t s = 0; s < n_shards; s++) outs[s].proof = nullptr;
         for (size_t s = 0; s < n_shards; s++) {
             memcpy(outs[s].pi_hash, res[s].pi_hash, 32);
@@ -480,7 +556,7 @@ int stark_prove_nsec3_completeness_batch(stark_ctx *ctx, const uint8_t *records,
                                          int ldt, int phi_mode, stark_nsec3_out *outs) {
     return guard(ctx, [&](sg::Ctx *c) {
         SG_REQUIRE(records && counts && salt && fs_bindings && outs, "prove_nsec3_completeness: null argument");
-        if (ldt != STARK_LDT_STIR) throw sg::Error(sg::E_UNSUPPORTED, "only LdtMode::Stir is implemented");
+        SG_REQUIRE(ldt == STARK_LDT_STIR || ldt == STARK_LDT_FRI, "unknown LdtMode");
         SG_REQUIRE(phi_mode >= 0 && phi_mode <= 2, "prove_nsec3_completeness: bad phi_mode");
         SG_REQUIRE(n_chains >= 1 && n_chains <= 65535, "prove_nsec3_completeness: 1 to 65535 chains");
         std::vector<sg::ByteSpan> ch(n_chains);
@@ -492,7 +568,7 @@ int stark_prove_nsec3_completeness_batch(stark_ctx *ctx, const uint8_t *records,
             memcpy(fsb[i].data(), fs_bindings + 32 * i, 32);
             outs[i].proof = nullptr;
         }
-        std::vector<sg::Nsec3Proof> r = sg::prove_nsec3_batch(c, ch, salt, fsb, phi_mode);
+        std::vector<sg::Nsec3Proof> r = sg::prove_nsec3_batch(c, ch, salt, fsb, phi_mode, ldt == STARK_LDT_STIR);
         for (size_t i = 0; i < n_chains; i++) fill_nsec3(r[i], &outs[i]);
     });
 }
@@ -510,12 +586,13 @@ int stark_prove_ds_ksk_binding(stark_ctx *ctx, const uint8_t *dnskey, size_t len
                                const uint8_t fs_binding[32], int ldt, int phi_mode, stark_ds_ksk_out *out) {
     return guard(ctx, [&](sg::Ctx *c) {
         SG_REQUIRE((dnskey || len == 0) && parent_ds_hash && fs_binding && out, "prove_ds_ksk_binding: null argument");
-        if (ldt != STARK_LDT_STIR) throw sg::Error(sg::E_UNSUPPORTED, "only LdtMode::Stir is implemented");
+        SG_REQUIRE(ldt == STARK_LDT_STIR || ldt == STARK_LDT_FRI, "unknown LdtMode");
         out->proof = nullptr;
         std::array<uint8_t, 32> par, fsb;
         memcpy(par.data(), parent_ds_hash, 32);
         memcpy(fsb.data(), fs_binding, 32);
-        std::vector<sg::DsKsk> r = sg::prove_ds_ksk_binding_batch(c, {{dnskey, len}}, {par}, {fsb}, phi_mode);
+        std::vector<sg::DsKsk> r = sg::prove_ds_ksk_binding_batch(c, {{dnskey, len}}, {par}, {fsb}, phi_mode,
+                                                                  ldt == STARK_LDT_STIR);
         fill_ds_ksk(r[0], out);
     });
 }
@@ -525,7 +602,7 @@ int stark_prove_ds_ksk_binding_batch(stark_ctx *ctx, const uint8_t *dnskeys, con
                                      int ldt, int phi_mode, stark_ds_ksk_out *outs) {
     return guard(ctx, [&](sg::Ctx *c) {
         SG_REQUIRE(offsets && lens && parent_ds_hashes && fs_bindings && outs, "prove_ds_ksk_binding_batch: null argument");
-        if (ldt != STARK_LDT_STIR) throw sg::Error(sg::E_UNSUPPORTED, "only LdtMode::Stir is implemented");
+        SG_REQUIRE(ldt == STARK_LDT_STIR || ldt == STARK_LDT_FRI, "unknown LdtMode");
         std::vector<sg::ByteSpan> keys(n_keys);
         std::vector<std::array<uint8_t, 32>> par(n_keys), fsb(n_keys);
         for (size_t i = 0; i < n_keys; i++) {
@@ -535,7 +612,7 @@ int stark_prove_ds_ksk_binding_batch(stark_ctx *ctx, const uint8_t *dnskeys, con
             memcpy(fsb[i].data(), fs_bindings + 32 * i, 32);
             outs[i].proof = nullptr;
         }
-        std::vector<sg::DsKsk> r = sg::prove_ds_ksk_binding_batch(c, keys, par, fsb, phi_mode);
+        std::vector<sg::DsKsk> r = sg::prove_ds_ksk_binding_batch(c, keys, par, fsb, phi_mode, ldt == STARK_LDT_STIR);
         for (size_t i = 0; i < n_keys; i++) fill_ds_ksk(r[i], &outs[i]);
     });
 }
@@ -566,8 +643,8 @@ int stark_prove_outer_rollup(stark_ctx *ctx, const uint8_t *pi_hashes, size_t n,
                              int phi_mode, uint8_t **proof, size_t *proof_len, uint8_t root_f0[32]) {
     return guard(ctx, [&](sg::Ctx *c) {
         SG_REQUIRE((pi_hashes || n == 0) && pk_hash && proof && proof_len && root_f0, "prove_outer_rollup: null argument");
-        if (ldt != STARK_LDT_STIR) throw sg::Error(sg::E_UNSUPPORTED, "only LdtMode::Stir is implemented");
-        std::vector<uint8_t> v = sg::prove_outer_rollup(c, pi_hashes, n, pk_hash, phi_mode, root_f0);
+        SG_REQUIRE(ldt == STARK_LDT_STIR || ldt == STARK_LDT_FRI, "unknown LdtMode");
+        std::vector<uint8_t> v = sg::prove_outer_rollup(c, pi_hashes, n, pk_hash, phi_mode, root_f0, ldt == STARK_LDT_STIR);
         *proof = copy_out(v, proof_len);
     });
 }
@@ -899,12 +976,13 @@ int stark_ed25519_shape(size_t msg_len, size_t k_scalar, size_t *width, size_t *
 int stark_prove_zsk_ksk_binding_v2_batch(stark_ctx *ctx, const uint8_t *pks, const uint8_t *sigs, const uint8_t *msgs,
                                          const size_t *msg_off, const size_t *msg_len, const uint8_t *fs_bindings,
                                          const uint8_t *merkle_roots, size_t n, size_t k_scalar, size_t blowup,
-                                         size_t queries, stark_zsk_out *outs) {
+                                         size_t queries, int ldt, stark_zsk_out *outs) {
     return guard(ctx, [&](sg::Ctx *c) {
         SG_REQUIRE(pks && sigs && msg_off && msg_len && fs_bindings && merkle_roots && outs,
                    "prove_zsk_ksk_binding_v2_batch: null argument");
         SG_REQUIRE(n >= 1 && n <= 65535, "prove_zsk_ksk_binding_v2_batch: 1 to 65535 signatures");
         SG_REQUIRE(k_scalar >= 2 && k_scalar <= 256, "prove_zsk_ksk_binding_v2_batch: k_scalar in [2, 256]");
+        SG_REQUIRE(ldt == STARK_LDT_STIR || ldt == STARK_LDT_FRI, "unknown LdtMode");
         for (size_t i = 0; i < n; i++) outs[i].proof = nullptr;
         std::vector<sg::EdSig> v(n);
         std::vector<std::array<uint8_t, 32>> fs(n), mr(n);
@@ -915,7 +993,8 @@ int stark_prove_zsk_ksk_binding_v2_batch(stark_ctx *ctx, const uint8_t *pks, con
             memcpy(mr[i].data(), merkle_roots + 32 * i, 32);
         }
         std::vector<sg::ZskKsk> r =
-            sg::prove_zsk_ksk_binding_v2_batch(c, v, fs, mr, (uint32_t)k_scalar, blowup, (uint32_t)queries);
+            sg::prove_zsk_ksk_binding_v2_batch(c, v, fs, mr, (uint32_t)k_scalar, blowup, (uint32_t)queries,
+                                               ldt == STARK_LDT_STIR);
         for (size_t i = 0; i < n; i++) {
             memcpy(outs[i].pi_hash, r[i].pi_hash, 32);
             memcpy(outs[i].root_f0, r[i].root_f0, 32);

@@ -347,6 +347,49 @@ void ood(Ctx *c, const uint64_t *coef, uint64_t n, int ncomp, uint64_t m, uint32
     }
 }
 
+// Classic-FRI layer leaves (fri.rs:255-333, 1356-1362): planes [K][18][n] =
+// f (6 components), s (6), q (6) per point. This kernel writes f and the DEEP
+// quotient q_i = (f_i - f(z)) / (w_n^i - z), with f(z) the first OOD coset
+// evaluation of the proof (ev: [K][m][6]) and z its table entry z^(2^0).
+__global__ __launch_bounds__(FT) void deep_quotient_k(const uint64_t *f, uint64_t n, int ncomp, const uint64_t *ev,
+                                                      uint32_t m, const uint64_t *ztab, uint32_t tstride,
+                                                      const uint64_t *rt_lo, const uint64_t *rt_hi, uint32_t logn,
+                                                      uint64_t *planes) {
+    const uint64_t i = (uint64_t)blockIdx.x * FT + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = blockIdx.y;
+    f += k * ncomp * n;
+    ev += k * m * 6;
+    ztab += k * tstride * 6;
+    planes += k * 18 * n;
+    fp6 fv, fz, z;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        fv.c[q] = q < ncomp ? f[q * n + i] : 0;
+        fz.c[q] = ev[q];
+        z.c[q] = ztab[q];
+    }
+    const uint64_t e26 = (i << (26 - logn)) & ((1ULL << 26) - 1);
+    const uint64_t x = mul(rt_lo[e26 & 8191], rt_hi[e26 >> 13]);
+    const fp6 qv = f6_mul(f6_sub(fv, fz), f6_inv(f6_sub(f6_base(x), z)));
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        planes[q * n + i] = fv.c[q];
+        planes[(12 + q) * n + i] = qv.c[q];
+    }
+}
+
+// s planes: s_i = f_{l+1}[i mod n/m] (the fold, repeated over the fibres,
+// fri.rs:773-789), or 0 (next == nullptr) on the coefficient-committed final layer
+__global__ __launch_bounds__(FT) void s_planes_k(const uint64_t *next, uint64_t n, uint64_t nn, uint64_t *planes) {
+    const uint64_t i = (uint64_t)blockIdx.x * FT + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = blockIdx.y;
+    planes += k * 18 * n + 6 * n;
+#pragma unroll
+    for (int q = 0; q < 6; q++) planes[q * n + i] = next ? next[k * 6 * nn + q * nn + (i & (nn - 1))] : 0;
+}
+
 struct Bytes {
     std::vector<uint8_t> v;
     void put(const void *d, size_t n) { v.insert(v.end(), (const uint8_t *)d, (const uint8_t *)d + n); }
@@ -372,6 +415,295 @@ void upload_powers(Ctx *c, const std::vector<fp6> &alpha, uint64_t m, uint64_t *
     SG_HIP(hipMemcpyAsync(d, pw.data(), 48 * pw.size(), hipMemcpyHostToDevice, c->stream));
 }
 
+
+// One Fiat-Shamir transcript per proof with the statement bound
+// (bind_statement_to_transcript, fri.rs:1105-1129)
+std::vector<Transcript> statement_transcripts(uint32_t K, uint64_t n0, const FriParams &p,
+                                              const std::vector<std::array<uint8_t, 32>> *pis) {
+    std::vector<Transcript> tr;
+    tr.reserve(K);
+    for (uint32_t k = 0; k < K; k++) {
+        tr.emplace_back("FRI/FS");
+        Transcript &t = tr.back();
+        if (pis || p.has_pi) {
+            t.absorb("CAIRO-PUBLIC-INPUTS-V1");
+            t.absorb(pis ? (*pis)[k].data() : p.pi, 32);
+        }
+        t.absorb("DEEP-FRI-STATEMENT-V2");
+        t.absorb_field(n0);
+        t.absorb_field(p.schedule.size());
+        for (uint64_t m : p.schedule) t.absorb_field(m);
+        t.absorb_field(canon(p.seed_z));
+        t.absorb_field(6);
+        t.absorb_field(p.coeff_commit_final ? 1 : 0);
+        t.absorb_field(p.stir ? 1 : 0);
+    }
+    return tr;
+}
+
+// first d_final coefficients of each proof's last layer (fri.rs:1919-1937),
+// absorbed, then the query seed (fri.rs:1939-1993)
+void final_poly_and_seed(Ctx *c, const uint64_t *f, uint64_t nf, uint32_t K, uint64_t d_final,
+                         std::vector<Transcript> &tr, std::vector<std::vector<fp6>> &final_coeffs,
+                         std::vector<uint64_t> &query_seed) {
+    std::vector<uint64_t> fin(6 * nf * K);
+    {
+        Scoped coef(c, 8 * 6 * nf * K), scr(c, 8 * 6 * nf * K);
+        const uint64_t *src = f;
+        if (nf > 1) {
+            ntt_columns(c, f, coef.as<uint64_t>(), scr.as<uint64_t>(), ilog2(nf), 6ULL * K, nf, nf, true, nullptr);
+            src = coef.as<uint64_t>();
+        }
+        c->d2h(fin.data(), src, 8 * fin.size());
+        SG_HIP(hipStreamSynchronize(c->stream));
+    }
+    const uint64_t dfin = d_final < nf ? d_final : nf;
+    final_coeffs.assign(K, std::vector<fp6>(dfin));
+    query_seed.assign(K, 0);
+    parallel_for(K, [&](size_t k) {
+        for (uint64_t i = 0; i < dfin; i++)
+            for (int q = 0; q < 6; q++) final_coeffs[k][i].c[q] = fin[(k * 6 + q) * nf + i];
+        for (auto &fc : final_coeffs[k])
+            for (int q = 0; q < 6; q++) tr[k].absorb_field(fc.c[q]);
+        query_seed[k] = tr[k].challenge_field("query_seed");
+    });
+}
+
+// Classic DEEP-FRI (LdtMode::Fri): per layer the DEEP quotient q at the one
+// out-of-domain point z, the fold s repeated over the fibres, a tree over the
+// 18-value leaves (f, s, q), and r queries followed down every layer
+// (fri.rs:1461-1600 with use_stir false, fri_prove_queries :1724-1807, the
+// payloads of deep_fri_prove :2008-2032). K proofs in lockstep, like the STIR
+// prover.
+std::vector<std::vector<uint8_t>> deep_fri_prove_classic(Ctx *c, const uint64_t *d_f0, uint32_t K, uint64_t n0,
+                                                         const FriParams &p,
+                                                         const std::vector<std::array<uint8_t, 32>> *pis) {
+    const size_t L = p.schedule.size();
+    const bool use_cc = p.coeff_commit_final;
+    std::vector<Transcript> tr = statement_transcripts(K, n0, p, pis);
+    // f0 tree: one value per leaf (fri.rs:1432-1440, f0_tree_config :1068-1072)
+    DevTree t0;
+    {
+        HostScope hs(c, "fri_f0_trees");
+        const uint32_t a = pick_arity(n0, 16);
+        LeafSrc src{d_f0, 1, n0, 1, 1, nullptr, n0};
+        tree_build(c, t0, src, n0, a, merkle_depth(n0, a), 0xFF, K);
+    }
+    // the one out-of-domain point z of every layer, as entry 0 of the OOD
+    // kernels' table of squares z^(2^i)
+    constexpr uint32_t tstride = 24;
+    std::vector<fp6> alpha(K);
+    std::vector<uint64_t> ztab((size_t)K * tstride * 6);
+    parallel_for(K, [&](size_t k) {
+        tr[k].absorb(t0.roots[k].data(), 32);
+        fp6 w;
+        tr[k].challenge_ext("z_fp3", w.c);
+        for (uint32_t i = 0; i < tstride; i++) {
+            memcpy(&ztab[((size_t)k * tstride + i) * 6], w.c, 48);
+            w = f6_mul(w, w);
+        }
+    });
+    Scoped d_ztab(c, 8 * ztab.size());
+    SG_HIP(hipMemcpyAsync(d_ztab.p, ztab.data(), 8 * ztab.size(), hipMemcpyHostToDevice, c->stream));
+    Scoped d_alpha(c, 48ULL * 16 * K);
+
+    std::vector<uint64_t *> planes(L, nullptr);  // [K][18][n_l] per layer, kept for the queries
+    std::vector<DevTree> trees(L);
+    std::vector<uint64_t> sz(L + 1);
+    std::vector<std::vector<uint64_t>> fz(L, std::vector<uint64_t>(6 * K));
+    std::vector<std::array<uint8_t, 32>> roots((size_t)L * K), coeff_roots(K);
+    std::vector<uint64_t> tuples_planes;
+    uint64_t n_final_tuples = 0, m_final = 0;
+    uint64_t cur = n0;
+    const uint64_t *f = d_f0;
+    int ncomp = 1;
+    uint64_t *owned = nullptr;
+    for (size_t ell = 0; ell < L; ell++) {
+        const uint64_t m = p.schedule[ell], nn = cur / m;
+        const bool final_cc = use_cc && ell == L - 1;
+        sz[ell] = cur;
+        if (!final_cc) parallel_for(K, [&](size_t k) { tr[k].challenge_ext("alpha", alpha[k].c); });
+        Scoped d_ev(c, 48ULL * m * K);
+        {
+            // f(z): the first of the OOD coset evaluations at z (fri.rs:255-299)
+            Scoped coef(c, 8ULL * K * ncomp * cur), scr(c, 8ULL * K * ncomp * cur);
+            ntt_columns(c, f, coef.as<uint64_t>(), scr.as<uint64_t>(), ilog2(cur), (uint64_t)K * ncomp, cur, cur, true,
+                        nullptr);
+            ood(c, coef.as<uint64_t>(), cur, ncomp, m, K, d_ztab.as<uint64_t>(), tstride, 0,
+                pow(root_of_unity(cur), nn), d_ev.as<uint64_t>());
+        }
+        planes[ell] = c->alloc_n<uint64_t>(18ULL * cur * K);
+        const dim3 grid((uint32_t)((cur + FT - 1) / FT), K);
+        {
+            ProfScope ps(c, "deep_quotient", (8.0 * ncomp * cur + 96.0 * cur) * K);
+            hipLaunchKernelGGL(deep_quotient_k, grid, dim3(FT), 0, c->stream, f, cur, ncomp, d_ev.as<uint64_t>(),
+                               (uint32_t)m, d_ztab.as<uint64_t>(), tstride, c->rt_lo, c->rt_hi, ilog2(cur),
+                               planes[ell]);
+            SG_HIP(hipGetLastError());
+        }
+        uint64_t *next = c->alloc_n<uint64_t>(6ULL * nn * K);
+        if (!final_cc) {
+            upload_powers(c, alpha, m, d_alpha.as<uint64_t>());
+            fold(c, f, cur, ncomp, m, K, d_alpha.as<uint64_t>(), next, nullptr);
+        }
+        hipLaunchKernelGGL(s_planes_k, grid, dim3(FT), 0, c->stream, final_cc ? nullptr : next, cur, nn, planes[ell]);
+        SG_HIP(hipGetLastError());
+        {
+            HostScope hs(c, "fri_layer_tree");
+            const uint32_t a = pick_arity(cur, m);
+            LeafSrc src{planes[ell], 1, cur, 18, 18, nullptr, 18 * cur};
+            tree_build(c, trees[ell], src, cur, a, merkle_depth(cur, a), ell, K, false);
+            std::vector<uint64_t> ev((size_t)K * m * 6);
+            c->d2h(ev.data(), d_ev.p, 8 * ev.size());
+            tree_read_roots(c, trees[ell]);
+            for (uint32_t k = 0; k < K; k++) memcpy(&fz[ell][6 * k], &ev[(size_t)k * m * 6], 48);
+        }
+        parallel_for(K, [&](size_t k) {
+            roots[ell * K + k] = trees[ell].roots[k];
+            for (int q = 0; q < 6; q++) tr[k].absorb_field(fz[ell][6 * k + q]);
+            tr[k].absorb(trees[ell].roots[k].data(), 32);
+        });
+        if (final_cc) {
+            // coefficient tuples and their tree (fri.rs:1649-1688), then this layer's alpha
+            Scoped tup(c, 8ULL * 6 * m * nn * K);
+            std::vector<fp6> zero(K, f6_zero());
+            upload_powers(c, zero, m, d_alpha.as<uint64_t>());
+            fold(c, f, cur, ncomp, m, K, d_alpha.as<uint64_t>(), next, tup.as<uint64_t>());
+            const uint32_t a = pick_arity(nn, 16);
+            DevTree tc;
+            LeafSrc src{tup.as<uint64_t>(), 1, nn, (uint32_t)(6 * m), (uint32_t)(6 * m), nullptr, 6 * m * nn};
+            tree_build(c, tc, src, nn, a, merkle_depth(nn, a), 0xFE, K, false);
+            tuples_planes.resize((size_t)6 * m * nn * K);
+            c->d2h(tuples_planes.data(), tup.p, 8 * tuples_planes.size());
+            tree_read_roots(c, tc);
+            n_final_tuples = nn;
+            m_final = m;
+            parallel_for(K, [&](size_t k) {
+                coeff_roots[k] = tc.roots[k];
+                tr[k].absorb(tc.roots[k].data(), 32);
+                tr[k].challenge_ext("alpha", alpha[k].c);
+                fp6 beta;
+                tr[k].challenge_ext("beta_deg", beta.c);
+            });
+            tree_free(c, tc);
+            upload_powers(c, alpha, m, d_alpha.as<uint64_t>());
+            fold(c, f, cur, ncomp, m, K, d_alpha.as<uint64_t>(), next, nullptr);
+        }
+        if (owned) c->release(owned);
+        owned = next;
+        f = next;
+        ncomp = 6;
+        cur = nn;
+    }
+    sz[L] = cur;
+    std::vector<std::vector<fp6>> final_coeffs;
+    std::vector<uint64_t> query_seed;
+    final_poly_and_seed(c, f, cur, K, p.d_final, tr, final_coeffs, query_seed);
+    if (owned) c->release(owned);
+
+    // query positions: i_0 = the raw index, i_{l+1} = i_l mod n_l / m_l
+    std::vector<std::vector<uint64_t>> raw;
+    fri_query_indices(c, query_seed, p.r, n0, raw);
+    const uint64_t r = p.r;
+    std::vector<std::vector<std::vector<uint64_t>>> pos(L + 1, std::vector<std::vector<uint64_t>>(K));
+    for (uint32_t k = 0; k < K; k++)
+        for (uint64_t q = 0; q < r; q++) {
+            uint64_t i = raw[k][q];
+            for (size_t ell = 0; ell <= L; ell++) {
+                pos[ell][k].push_back(i);
+                if (ell < L) i %= sz[ell] / p.schedule[ell];
+            }
+        }
+    // payloads (f, s, q) of every query and layer, gathered from the leaf planes
+    std::vector<std::vector<uint64_t>> pay(L);
+    {
+        HostScope hs(c, "fri_gather");
+        for (size_t ell = 0; ell < L; ell++) {
+            const uint64_t n = sz[ell];
+            std::vector<uint64_t> gidx;
+            gidx.reserve((size_t)K * r * 18);
+            for (uint32_t k = 0; k < K; k++)
+                for (uint64_t q = 0; q < r; q++)
+                    for (uint64_t j = 0; j < 18; j++) gidx.push_back(k * 18 * n + j * n + pos[ell][k][q]);
+            pay[ell].resize(gidx.size());
+            if (gidx.empty()) continue;
+            Scoped di(c, 8 * gidx.size()), dv(c, 8 * gidx.size());
+            SG_HIP(hipMemcpyAsync(di.p, gidx.data(), 8 * gidx.size(), hipMemcpyHostToDevice, c->stream));
+            gather_u64(c, planes[ell], di.as<uint64_t>(), gidx.size(), dv.as<uint64_t>());
+            c->d2h(pay[ell].data(), dv.p, 8 * gidx.size());
+            SG_HIP(hipStreamSynchronize(c->stream));
+        }
+    }
+    std::vector<std::vector<std::vector<std::vector<uint8_t>>>> lop(L);
+    std::vector<std::vector<std::vector<uint8_t>>> f0op;
+    {
+        HostScope ho(c, "fri_openings");
+        for (size_t ell = 0; ell < L; ell++) tree_open_multi(c, trees[ell], pos[ell], lop[ell]);
+        tree_open_multi(c, t0, pos[0], f0op);
+    }
+    for (size_t ell = 0; ell < L; ell++) {
+        tree_free(c, trees[ell]);
+        c->release(planes[ell]);
+    }
+    tree_free(c, t0);
+
+    HostScope hs(c, "fri_serialize");
+    std::vector<std::vector<uint8_t>> proofs(K);
+    const uint64_t omega0 = root_of_unity(n0);
+    parallel_for(K, [&](size_t k) {
+        Bytes b;
+        b.put(t0.roots[k].data(), 32);
+        b.u64(L);
+        for (size_t l = 0; l < L; l++) b.put(roots[l * K + k].data(), 32);
+        b.u64(L);  // layer_proofs.layers
+        for (size_t l = 0; l < L; l++) {
+            b.u64(r);
+            for (uint64_t q = 0; q < r; q++) b.put(lop[l][k][q].data(), lop[l][k][q].size());
+        }
+        b.u64(r);  // f0_openings
+        for (uint64_t q = 0; q < r; q++) b.put(f0op[k][q].data(), f0op[k][q].size());
+        b.u64(r);  // queries
+        for (uint64_t q = 0; q < r; q++) {
+            b.u64(L);
+            for (size_t l = 0; l < L; l++) {
+                const uint64_t i = pos[l][k][q], m = p.schedule[l];
+                b.u64(i);
+                b.u64(i % m);
+                b.u64(i % (sz[l] / m));
+                b.u64(0);
+            }
+            b.u64(L);
+            for (size_t l = 0; l < L; l++) b.put(&pay[l][((size_t)k * r + q) * 18], 8 * 18);
+            b.put(f0op[k][q].data(), f0op[k][q].size());
+            b.u64(pos[L][k][q]);
+        }
+        b.u64(L);
+        for (size_t l = 0; l < L; l++) b.f6(&fz[l][6 * k]);
+        b.u64(final_coeffs[k].size());
+        for (auto &v : final_coeffs[k]) b.f6(v.c);
+        b.u64(n0);
+        b.u64(omega0);
+        if (use_cc) {
+            const uint64_t nt = n_final_tuples, m = m_final;
+            const uint64_t *tp = &tuples_planes[(size_t)k * 6 * m * nt];
+            b.u8(1);
+            b.u64(nt);
+            for (uint64_t t = 0; t < nt; t++) {
+                b.u64(m);
+                for (uint64_t kq = 0; kq < 6 * m; kq++) b.u64(tp[kq * nt + t]);
+            }
+            b.u8(1);
+            b.put(coeff_roots[k].data(), 32);
+        } else {
+            b.u8(0);
+            b.u8(0);
+        }
+        b.u8(0);  // stir_coset_evals: None
+        b.u8(0);  // stir_proximity_queries: None
+        proofs[k] = std::move(b.v);
+    });
+    return proofs;
+}
 }  // namespace
 
 std::vector<uint64_t> stir_schedule(uint64_t n0) {
@@ -385,7 +717,7 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
                                                        const FriParams &p, const std::vector<std::array<uint8_t, 32>> *pis,
                                                        std::vector<std::vector<uint64_t>> *raw_out) {
     const size_t L = p.schedule.size();
-    if (!p.stir || L == 0) throw Error(E_UNSUPPORTED, "only the STIR low-degree test is implemented");
+    SG_REQUIRE(L > 0, "deep_fri_prove: empty fold schedule");
     SG_REQUIRE(K >= 1 && K <= 65535, "deep_fri_prove: batch size out of range");
     SG_REQUIRE(is_pow2(n0) && n0 >= 2 && n0 <= (1ULL << 24), "deep_fri_prove: n0 must be a power of two in [2, 2^24]");
     SG_REQUIRE(!pis || pis->size() == K, "deep_fri_prove: one public-inputs hash per proof");
@@ -396,26 +728,12 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
             n /= m;
         }
     }
-    const bool use_cc = p.coeff_commit_final;
-
-    std::vector<Transcript> tr;
-    tr.reserve(K);
-    for (uint32_t k = 0; k < K; k++) {
-        tr.emplace_back("FRI/FS");
-        Transcript &t = tr.back();
-        if (pis || p.has_pi) {
-            t.absorb("CAIRO-PUBLIC-INPUTS-V1");
-            t.absorb(pis ? (*pis)[k].data() : p.pi, 32);
-        }
-        t.absorb("DEEP-FRI-STATEMENT-V2");
-        t.absorb_field(n0);
-        t.absorb_field(L);
-        for (uint64_t m : p.schedule) t.absorb_field(m);
-        t.absorb_field(canon(p.seed_z));
-        t.absorb_field(6);
-        t.absorb_field(p.coeff_commit_final ? 1 : 0);
-        t.absorb_field(p.stir ? 1 : 0);
+    if (!p.stir) {
+        SG_REQUIRE(!raw_out, "deep_fri_prove: raw STIR query indices asked of a classic-FRI proof");
+        return deep_fri_prove_classic(c, d_f0, K, n0, p, pis);
     }
+    const bool use_cc = p.coeff_commit_final;
+    std::vector<Transcript> tr = statement_transcripts(K, n0, p, pis);
 
     // coset-packed f0 trees: leaf b = fibre f0[b + j n0/m0] (fri.rs:1420-1431)
     const uint64_t m0 = p.schedule[0], nn0 = n0 / m0;
@@ -538,29 +856,11 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
     }
 
     // final polynomial: first d_final coefficients of the last layer (fri.rs:1919-1937)
-    const uint64_t nf = cur;
-    std::vector<uint64_t> fin(6 * nf * K);
-    {
-        Scoped coef(c, 8 * 6 * nf * K), scr(c, 8 * 6 * nf * K);
-        const uint64_t *src = f;
-        if (nf > 1) {
-            ntt_columns(c, f, coef.as<uint64_t>(), scr.as<uint64_t>(), ilog2(nf), 6ULL * K, nf, nf, true, nullptr);
-            src = coef.as<uint64_t>();
-        }
-        c->d2h(fin.data(), src, 8 * fin.size());
-        SG_HIP(hipStreamSynchronize(c->stream));
-    }
-    const uint64_t dfin = p.d_final < nf ? p.d_final : nf;
-    std::vector<std::vector<fp6>> final_coeffs(K, std::vector<fp6>(dfin));
-    std::vector<uint64_t> query_seed(K);
+    std::vector<std::vector<fp6>> final_coeffs;
+    std::vector<uint64_t> query_seed;
+    final_poly_and_seed(c, f, cur, K, p.d_final, tr, final_coeffs, query_seed);
+    const uint64_t dfin = final_coeffs[0].size();
     std::vector<std::vector<uint64_t>> raw, base(K, std::vector<uint64_t>(p.s0));
-    parallel_for(K, [&](size_t k) {
-        for (uint64_t i = 0; i < dfin; i++)
-            for (int q = 0; q < 6; q++) final_coeffs[k][i].c[q] = fin[(k * 6 + q) * nf + i];
-        for (auto &fc : final_coeffs[k])
-            for (int q = 0; q < 6; q++) tr[k].absorb_field(fc.c[q]);
-        query_seed[k] = tr[k].challenge_field("query_seed");
-    });
     // STIR proximity query indices (fri.rs:1813-1893)
     fri_query_indices(c, query_seed, p.s0, n0, raw);
     for (uint32_t k = 0; k < K; k++)

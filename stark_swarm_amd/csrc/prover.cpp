@@ -98,8 +98,23 @@ void dns_merkle_root(const std::vector<uint8_t> &leaves, uint8_t root[32]) {
 
 static const uint64_t SWARM_BLOWUP = 32, SWARM_QUERIES = 54, SWARM_SEED_Z = 0xDEEFBAADULL;
 
+// build_params (prover.rs:87-98) over make_schedule (:53-67): arity-8 STIR
+// folds with a residual, or arity-2 classic folds (LdtMode::Fri); 54 queries
+// unless the caller asks for another count
+static FriParams swarm_fri_params(uint64_t n0, uint64_t queries, bool stir) {
+    FriParams p;
+    p.schedule = stir ? stir_schedule(n0) : std::vector<uint64_t>(ilog2(n0), 2);
+    p.r = p.s0 = queries;
+    p.seed_z = SWARM_SEED_Z;
+    p.coeff_commit_final = true;
+    p.d_final = 1;
+    p.stir = stir;
+    p.has_pi = true;
+    return p;
+}
+
 std::vector<uint8_t> rollup_prove(Ctx *c, const std::vector<uint64_t> &leaves, const uint8_t binding[32],
-                                  int phi_mode, uint8_t root_f0[32]) {
+                                  int phi_mode, uint8_t root_f0[32], bool stir) {
     uint64_t n_trace = 8;
     while (n_trace < leaves.size()) n_trace <<= 1;  // next_power_of_two().max(8)
     const uint64_t n0 = n_trace * SWARM_BLOWUP;
@@ -109,14 +124,7 @@ std::vector<uint8_t> rollup_prove(Ctx *c, const std::vector<uint64_t> &leaves, c
     SG_HIP(hipMemcpyAsync(d_tr.p, trace.data(), 8 * trace.size(), hipMemcpyHostToDevice, c->stream));
     lde_columns(c, d_tr.as<uint64_t>(), d_lde.as<uint64_t>(), ilog2(n_trace), ilog2(SWARM_BLOWUP), 4, n_trace, n0, 1);
     merge_hash_rollup_dev(c, d_lde.as<uint64_t>(), n_trace, SWARM_BLOWUP, phi_mode, d_c.as<uint64_t>());
-    FriParams p;
-    p.schedule = stir_schedule(n0);
-    p.r = p.s0 = SWARM_QUERIES;
-    p.seed_z = SWARM_SEED_Z;
-    p.coeff_commit_final = true;
-    p.d_final = 1;
-    p.stir = true;
-    p.has_pi = true;
+    FriParams p = swarm_fri_params(n0, SWARM_QUERIES, stir);
     memcpy(p.pi, binding, 32);
     std::vector<uint8_t> proof = deep_fri_prove_dev(c, d_c.as<uint64_t>(), n0, p, nullptr);
     memcpy(root_f0, proof.data(), 32);
@@ -133,16 +141,17 @@ static void pack_hash(const uint8_t *h, std::vector<uint64_t> &out) {
 
 // One proof at a time: the batch path with a batch of one.
 InnerShard prove_inner_shard(Ctx *c, const uint8_t salt[16], const std::vector<DnsRec> &recs,
-                             const uint8_t fs_binding[32], int phi_mode) {
+                             const uint8_t fs_binding[32], int phi_mode, bool stir) {
     std::array<uint8_t, 32> f;
     memcpy(f.data(), fs_binding, 32);
-    std::vector<InnerShard> v = prove_inner_shard_batch(c, salt, {recs}, {f}, phi_mode);
+    std::vector<InnerShard> v = prove_inner_shard_batch(c, salt, {recs}, {f}, phi_mode, stir);
     return std::move(v[0]);
 }
 
 std::vector<std::vector<uint8_t>> rollup_prove_batch(Ctx *c, const std::vector<std::vector<uint64_t>> &leaves,
                                                      const std::vector<std::array<uint8_t, 32>> &bindings,
-                                                     int phi_mode, std::vector<std::array<uint8_t, 32>> &root_f0) {
+                                                     int phi_mode, std::vector<std::array<uint8_t, 32>> &root_f0,
+                                                     bool stir) {
     const uint32_t K = (uint32_t)leaves.size();
     SG_REQUIRE(K >= 1 && bindings.size() == K, "rollup batch: one binding per proof");
     uint64_t n_trace = 8;
@@ -165,14 +174,7 @@ std::vector<std::vector<uint8_t>> rollup_prove_batch(Ctx *c, const std::vector<s
     lde_columns(c, d_tr.as<uint64_t>(), d_lde.as<uint64_t>(), ilog2(n_trace), ilog2(SWARM_BLOWUP), 4ULL * K, n_trace,
                 n0, 1);
     merge_hash_rollup_batch(c, d_lde.as<uint64_t>(), K, n_trace, SWARM_BLOWUP, phi_mode, d_c.as<uint64_t>());
-    FriParams p;
-    p.schedule = stir_schedule(n0);
-    p.r = p.s0 = SWARM_QUERIES;
-    p.seed_z = SWARM_SEED_Z;
-    p.coeff_commit_final = true;
-    p.d_final = 1;
-    p.stir = true;
-    p.has_pi = true;
+    FriParams p = swarm_fri_params(n0, SWARM_QUERIES, stir);
     HostScope h(c, "deep_fri_prove_batch");
     std::vector<std::vector<uint8_t>> proofs = deep_fri_prove_batch(c, d_c.as<uint64_t>(), K, n0, p, &bindings, nullptr);
     root_f0.resize(K);
@@ -183,7 +185,7 @@ std::vector<std::vector<uint8_t>> rollup_prove_batch(Ctx *c, const std::vector<s
 std::vector<InnerShard> prove_inner_shard_batch(Ctx *c, const uint8_t salt[16],
                                                 const std::vector<std::vector<DnsRec>> &shards,
                                                 const std::vector<std::array<uint8_t, 32>> &fs_bindings,
-                                                int phi_mode) {
+                                                int phi_mode, bool stir) {
     const size_t K = shards.size();
     SG_REQUIRE(fs_bindings.size() == K, "prove_inner_shard_batch: one fs_binding per shard");
     std::vector<InnerShard> out(K);
@@ -221,7 +223,7 @@ std::vector<InnerShard> prove_inner_shard_batch(Ctx *c, const uint8_t salt[16],
         }
         std::vector<std::array<uint8_t, 32>> rf;
         HostScope h(c, "rollup_prove_batch");
-        std::vector<std::vector<uint8_t>> proofs = rollup_prove_batch(c, lv, bd, phi_mode, rf);
+        std::vector<std::vector<uint8_t>> proofs = rollup_prove_batch(c, lv, bd, phi_mode, rf, stir);
         for (size_t j = 0; j < g.second.size(); j++) {
             InnerShard &o = out[g.second[j]];
             o.proof = std::move(proofs[j]);
@@ -262,7 +264,7 @@ static void ds_ksk_pi_hash(const ByteSpan &key, const uint8_t parent[32], const 
 
 std::vector<ShaProof> prove_sha_batch(Ctx *c, const std::vector<int> &algs, const std::vector<ByteSpan> &msgs,
                                       const std::vector<uint64_t> &heights,
-                                      const std::vector<std::array<uint8_t, 32>> &fsbs, int phi_mode) {
+                                      const std::vector<std::array<uint8_t, 32>> &fsbs, int phi_mode, bool stir) {
     const size_t K = msgs.size();
     SG_REQUIRE(algs.size() == K && heights.size() == K && fsbs.size() == K,
                "prove_sha: one alg, height and binding per message");
@@ -331,14 +333,7 @@ std::vector<ShaProof> prove_sha_batch(Ctx *c, const std::vector<int> &algs, cons
                     merge_sha512_from_trace_batch(c, d_tr.as<uint64_t>(), G, h, SWARM_BLOWUP, nb, phi_mode,
                                                   d_c.as<uint64_t>());
             }
-            FriParams p;
-            p.schedule = stir_schedule(n0);
-            p.r = p.s0 = SWARM_QUERIES;
-            p.seed_z = SWARM_SEED_Z;
-            p.coeff_commit_final = true;
-            p.d_final = 1;
-            p.stir = true;
-            p.has_pi = true;
+            FriParams p = swarm_fri_params(n0, SWARM_QUERIES, stir);
             std::vector<std::array<uint8_t, 32>> pis(G);
             for (uint32_t j = 0; j < G; j++) pis[j] = fsbs[ids[j]];
             std::vector<std::vector<uint8_t>> proofs = deep_fri_prove_batch(c, d_c.as<uint64_t>(), G, n0, p, &pis, nullptr);
@@ -354,11 +349,12 @@ std::vector<ShaProof> prove_sha_batch(Ctx *c, const std::vector<int> &algs, cons
 
 std::vector<DsKsk> prove_ds_ksk_binding_batch(Ctx *c, const std::vector<ByteSpan> &keys,
                                               const std::vector<std::array<uint8_t, 32>> &parents,
-                                              const std::vector<std::array<uint8_t, 32>> &fsbs, int phi_mode) {
+                                              const std::vector<std::array<uint8_t, 32>> &fsbs, int phi_mode,
+                                              bool stir) {
     const size_t K = keys.size();
     SG_REQUIRE(parents.size() == K && fsbs.size() == K, "prove_ds_ksk_binding: one parent hash and binding per key");
     std::vector<ShaProof> sp =
-        prove_sha_batch(c, std::vector<int>(K, 256), keys, std::vector<uint64_t>(K, 0), fsbs, phi_mode);
+        prove_sha_batch(c, std::vector<int>(K, 256), keys, std::vector<uint64_t>(K, 0), fsbs, phi_mode, stir);
     std::vector<DsKsk> out(K);
     for (size_t k = 0; k < K; k++) {
         DsKsk &o = out[k];
@@ -373,7 +369,8 @@ std::vector<DsKsk> prove_ds_ksk_binding_batch(Ctx *c, const std::vector<ByteSpan
 }
 
 std::vector<Nsec3Proof> prove_nsec3_batch(Ctx *c, const std::vector<ByteSpan> &chains, const uint8_t salt[16],
-                                          const std::vector<std::array<uint8_t, 32>> &fs_bindings, int phi_mode) {
+                                          const std::vector<std::array<uint8_t, 32>> &fs_bindings, int phi_mode,
+                                          bool stir) {
     const size_t K = chains.size();
     SG_REQUIRE(K >= 1 && fs_bindings.size() == K, "prove_nsec3: one fs_binding per chain");
     std::vector<Nsec3Proof> out(K);
@@ -425,14 +422,7 @@ std::vector<Nsec3Proof> prove_nsec3_batch(Ctx *c, const std::vector<ByteSpan> &c
             merge_nsec3_batch(c, d_lde.as<uint64_t>(), G, nt, SWARM_BLOWUP, phi_mode, d_c.as<uint64_t>());
             SG_HIP(hipStreamSynchronize(c->stream));  // the host trace goes out of scope
         }
-        FriParams p;
-        p.schedule = stir_schedule(n0);
-        p.r = p.s0 = SWARM_QUERIES;
-        p.seed_z = SWARM_SEED_Z;
-        p.coeff_commit_final = true;
-        p.d_final = 1;
-        p.stir = true;
-        p.has_pi = true;
+        FriParams p = swarm_fri_params(n0, SWARM_QUERIES, stir);
         std::vector<std::array<uint8_t, 32>> pis(G);
         for (uint32_t j = 0; j < G; j++) pis[j] = fs_bindings[g.second[j]];
         std::vector<std::vector<uint8_t>> proofs = deep_fri_prove_batch(c, d_c.as<uint64_t>(), G, n0, p, &pis, nullptr);
@@ -490,14 +480,7 @@ std::vector<RsaProof> prove_rsa_batch(Ctx *c, const std::vector<RsaRecord> &reco
             if (c->profiling) SG_HIP(hipStreamSynchronize(c->stream));
         }
         HostScope hf(c, "rsa_fri");
-        FriParams p;
-        p.schedule = stir_schedule(n0);
-        p.r = p.s0 = SWARM_QUERIES;
-        p.seed_z = SWARM_SEED_Z;
-        p.coeff_commit_final = true;
-        p.d_final = 1;
-        p.stir = true;
-        p.has_pi = true;
+        FriParams p = swarm_fri_params(n0, SWARM_QUERIES, true);
         std::vector<std::array<uint8_t, 32>> gp(pis.begin() + k0, pis.begin() + k0 + G);
         std::vector<std::vector<uint8_t>> proofs = deep_fri_prove_batch(c, d_c.as<uint64_t>(), G, n0, p, &gp, nullptr);
         for (uint32_t j = 0; j < G; j++) {
@@ -560,14 +543,7 @@ std::vector<P256Proof> prove_p256_batch(Ctx *c, const std::vector<P256Chains> &i
             for (uint32_t j = 0; j < G; j++) out[k0 + j].rproj = rp[j];
         }
         HostScope hf(c, "p256_fri");
-        FriParams p;
-        p.schedule = stir_schedule(n0);
-        p.r = p.s0 = SWARM_QUERIES;
-        p.seed_z = SWARM_SEED_Z;
-        p.coeff_commit_final = true;
-        p.d_final = 1;
-        p.stir = true;
-        p.has_pi = true;
+        FriParams p = swarm_fri_params(n0, SWARM_QUERIES, true);
         std::vector<std::array<uint8_t, 32>> gp(pis.begin() + k0, pis.begin() + k0 + G);
         std::vector<std::vector<uint8_t>> proofs = deep_fri_prove_batch(c, d_c.as<uint64_t>(), G, n0, p, &gp, nullptr);
         for (uint32_t j = 0; j < G; j++) {
@@ -607,10 +583,10 @@ std::vector<P256Proof> prove_ecdsa_p256_batch(Ctx *c, const std::vector<P256Reco
 }
 
 std::vector<uint8_t> prove_outer_rollup(Ctx *c, const uint8_t *pi_hashes, size_t n, const uint8_t pk_hash[32],
-                                        int phi_mode, uint8_t root_f0[32]) {
+                                        int phi_mode, uint8_t root_f0[32], bool stir) {
     std::vector<uint64_t> leaves;
     for (size_t i = 0; i < n; i++) pack_hash(pi_hashes + 32 * i, leaves);
-    return rollup_prove(c, leaves, pk_hash, phi_mode, root_f0);
+    return rollup_prove(c, leaves, pk_hash, phi_mode, root_f0, stir);
 }
 
 std::vector<uint8_t> prover_core_dev(Ctx *c, const uint64_t *d_trace, uint64_t w, uint64_t n_trace, uint64_t blowup,
@@ -659,7 +635,7 @@ std::vector<uint8_t> prover_core_dev(Ctx *c, const uint64_t *d_trace, uint64_t w
 std::vector<ZskKsk> prove_zsk_ksk_binding_v2_batch(Ctx *c, const std::vector<EdSig> &sigs,
                                                    const std::vector<std::array<uint8_t, 32>> &fs_bindings,
                                                    const std::vector<std::array<uint8_t, 32>> &merkle_roots,
-                                                   uint32_t k, uint64_t blowup, uint32_t queries) {
+                                                   uint32_t k, uint64_t blowup, uint32_t queries, bool stir) {
     const size_t K = sigs.size();
     SG_REQUIRE(K >= 1 && K <= 65535, "prove_zsk_ksk_binding_v2: 1 to 65535 signatures");
     SG_REQUIRE(fs_bindings.size() == K && merkle_roots.size() == K,
@@ -710,14 +686,7 @@ std::vector<ZskKsk> prove_zsk_ksk_binding_v2_batch(Ctx *c, const std::vector<EdS
                 if (c->profiling) SG_HIP(hipStreamSynchronize(c->stream));
             }
             HostScope hf(c, "ed25519_fri");
-            FriParams p;
-            p.schedule = stir_schedule(n0);
-            p.r = p.s0 = queries;
-            p.seed_z = SWARM_SEED_Z;
-            p.coeff_commit_final = true;
-            p.d_final = 1;
-            p.stir = true;
-            p.has_pi = true;
+            FriParams p = swarm_fri_params(n0, queries, stir);
             std::vector<std::vector<uint8_t>> proofs = deep_fri_prove_batch(c, d_c.as<uint64_t>(), G, n0, p, &pis, nullptr);
             for (uint32_t j = 0; j < G; j++) {
                 const size_t i = idx[k0 + j];

@@ -46,30 +46,31 @@ std::vector<uint8_t> dns_canonical(const DnsRec &r);
 size_t dns_canonical_to(const DnsRec &r, uint8_t *dst);  // returns the length written
 void dns_leaf_hash(const DnsRec &r, const uint8_t salt[16], uint8_t out[32]);
 void dns_merkle_root(const std::vector<uint8_t> &leaves, uint8_t root[32]);
+// stir = false: LdtMode::Fri, the classic arity-2 DEEP-FRI of make_schedule (prover.rs:53-67)
 std::vector<uint8_t> rollup_prove(Ctx *c, const std::vector<uint64_t> &leaves, const uint8_t binding[32],
-                                  int phi_mode, uint8_t root_f0[32]);
+                                  int phi_mode, uint8_t root_f0[32], bool stir = true);
 InnerShard prove_inner_shard(Ctx *c, const uint8_t salt[16], const std::vector<DnsRec> &recs,
-                             const uint8_t fs_binding[32], int phi_mode);
+                             const uint8_t fs_binding[32], int phi_mode, bool stir = true);
 // K shards in one lockstep device batch per trace size; host hashing on a thread pool
 std::vector<InnerShard> prove_inner_shard_batch(Ctx *c, const uint8_t salt[16],
                                                 const std::vector<std::vector<DnsRec>> &shards,
                                                 const std::vector<std::array<uint8_t, 32>> &fs_bindings,
-                                                int phi_mode);
+                                                int phi_mode, bool stir = true);
 std::vector<std::vector<uint8_t>> rollup_prove_batch(Ctx *c, const std::vector<std::vector<uint64_t>> &leaves,
                                                      const std::vector<std::array<uint8_t, 32>> &bindings,
-                                                     int phi_mode, std::vector<std::array<uint8_t, 32>> &root_f0);
+                                                     int phi_mode, std::vector<std::array<uint8_t, 32>> &root_f0, bool stir = true);
 // SHA-256 (algs[k] = 256) or SHA-512 (512) AIR proofs: trace of msgs[k] with
 // height heights[k] (0 = minimum) -> LDE x32 -> merge -> STIR (54 queries,
 // bound to fs_bindings[k]); lockstep batches per (alg, height) in HBM-sized
 // waves; result k equals the proof of input k alone.
 std::vector<ShaProof> prove_sha_batch(Ctx *c, const std::vector<int> &algs, const std::vector<ByteSpan> &msgs,
                                       const std::vector<uint64_t> &heights,
-                                      const std::vector<std::array<uint8_t, 32>> &fs_bindings, int phi_mode);
+                                      const std::vector<std::array<uint8_t, 32>> &fs_bindings, int phi_mode, bool stir = true);
 // prove_ds_ksk_binding (prover.rs:512-585) for K keys: lockstep device
 // batches per trace height; result k equals the single-key proof
 std::vector<DsKsk> prove_ds_ksk_binding_batch(Ctx *c, const std::vector<ByteSpan> &dnskeys,
                                               const std::vector<std::array<uint8_t, 32>> &parent_ds_hashes,
-                                              const std::vector<std::array<uint8_t, 32>> &fs_bindings, int phi_mode);
+                                              const std::vector<std::array<uint8_t, 32>> &fs_bindings, int phi_mode, bool stir = true);
 // prove_nsec3_completeness output (crates/swarm-dns/src/prover.rs:327-338)
 struct Nsec3Proof {
     uint8_t pi_hash[32], chain_root[32], root_f0[32];
@@ -81,7 +82,7 @@ struct Nsec3Proof {
 // lockstep device batches per trace height; result k equals the proof of
 // chain k alone. Throws E_INVALID for an empty or open chain.
 std::vector<Nsec3Proof> prove_nsec3_batch(Ctx *c, const std::vector<ByteSpan> &chains, const uint8_t salt[16],
-                                          const std::vector<std::array<uint8_t, 32>> &fs_bindings, int phi_mode);
+                                          const std::vector<std::array<uint8_t, 32>> &fs_bindings, int phi_mode, bool stir = true);
 
 // One RSA-2048 exponentiation proof (one record, or R stacked records).
 struct RsaProof {
@@ -119,7 +120,7 @@ std::vector<P256Proof> prove_p256_batch(Ctx *c, const std::vector<P256Chains> &i
 // signature does not verify.
 std::vector<P256Proof> prove_ecdsa_p256_batch(Ctx *c, const std::vector<P256Record> &recs);
 std::vector<uint8_t> prove_outer_rollup(Ctx *c, const uint8_t *pi_hashes, size_t n, const uint8_t pk_hash[32],
-                                        int phi_mode, uint8_t root_f0[32]);
+                                        int phi_mode, uint8_t root_f0[32], bool stir = true);
 std::vector<uint8_t> prover_core_dev(Ctx *c, const uint64_t *d_trace, uint64_t w, uint64_t n_trace, uint64_t blowup,
                                      uint64_t shift, const FriParams &p, uint64_t *d_lde, uint8_t row_root[32]);
 
@@ -139,6 +140,6 @@ struct ZskKsk {
 std::vector<ZskKsk> prove_zsk_ksk_binding_v2_batch(Ctx *c, const std::vector<EdSig> &sigs,
                                                    const std::vector<std::array<uint8_t, 32>> &fs_bindings,
                                                    const std::vector<std::array<uint8_t, 32>> &merkle_roots,
-                                                   uint32_t k_scalar, uint64_t blowup, uint32_t queries);
+                                                   uint32_t k_scalar, uint64_t blowup, uint32_t queries, bool stir = true);
 
 }  // namespace sg

@@ -163,6 +163,8 @@ def lib():
         L.ref_prove_zsk_ksk_binding_v2.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz,
                                                    ctypes.c_char_p, ctypes.c_char_p, sz, sz, ctypes.POINTER(u8p),
                                                    ctypes.POINTER(sz), ctypes.c_char_p, ctypes.c_char_p]
+        L.ref_set_ldt.argtypes = [ctypes.c_int]
+        L.ref_set_ldt.restype = None
         L.ref_deep_fri_verify.restype = ctypes.c_int
         L.ref_deep_fri_verify.argtypes = [ctypes.POINTER(FriParams), ctypes.c_char_p, sz]
         L.ref_merkle_verify_opening.restype = ctypes.c_int
@@ -230,6 +232,17 @@ def deep_fri_prove(f0, params):
     if rc:
         raise ValueError("deep_fri_prove rc=%d" % rc)
     return take_bytes(pp, pl)
+
+
+class ldt_fri:
+    """with O.ldt_fri(): the swarm-dns provers of the oracle run LdtMode::Fri
+    (classic DEEP-FRI, arity-2 folds, prover.rs:53-67)"""
+
+    def __enter__(self):
+        lib().ref_set_ldt(1)
+
+    def __exit__(self, *a):
+        lib().ref_set_ldt(0)
 
 
 def deep_fri_verify(params, proof):

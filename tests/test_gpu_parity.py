@@ -231,9 +231,15 @@ def test_prover_core_c2_full_lde_columns(S):
     d_lde.free()
 
 
-def test_fri_ldt_unsupported_is_an_error(S):
-    with pytest.raises(S.StarkError):
-        S.prove_outer_rollup([b"\x00" * 32], b"\x01" * 32, S.LdtMode.Fri)
+def test_unknown_ldt_mode_is_an_error(S):
+    # LdtMode has two variants (prover.rs:37-41); any other code is refused
+    import ctypes
+    ctx = S.default_context()
+    pp, pl = ctypes.POINTER(ctypes.c_uint8)(), ctypes.c_size_t()
+    rf = ctypes.create_string_buffer(32)
+    rc = S.lib().stark_prove_outer_rollup(ctx.h, b"\x00" * 32, 1, b"\x01" * 32, 7, 1, ctypes.byref(pp),
+                                          ctypes.byref(pl), rf)
+    assert rc == -1
 
 
 # ---------------------------------------------------------------- prover core

@@ -242,3 +242,94 @@ def test_golden_openings_verify():
         op1 = blob[first[(q, "l1")]:last[(q, "l1")]]
         assert O.merkle_verify_opening(a1, O.merkle_depth(nn, a1), 1, root1, op1) == 1
         assert O.merkle_verify_opening(a1, O.merkle_depth(nn, a1), 1, root_f0, op1) == 0
+
+
+# ---------------------------------------------------------------- the library's host verifier
+def lib_params(p):
+    """the same parameters as an oracle FriParams, for the library mirror"""
+    import stark_swarm_amd as S
+    sch = [p.schedule[i] for i in range(p.n_layers)]
+    return S.DeepFriParams(sch, p.r, p.seed_z, bool(p.coeff_commit_final), p.d_final, bool(p.stir), p.s0,
+                           p.public_inputs_hash)
+
+
+def lib_verdict(p, proof):
+    import stark_swarm_amd as S
+    try:
+        return 1 if S.deep_fri_verify(lib_params(p), proof) else 0
+    except S.StarkError:
+        return -1
+
+
+def corpus():
+    """(params, proof bytes) pairs: the golden proofs, tampered copies of them
+    (a few per field kind), wrong parameters and the oracle's classic and STIR
+    proofs"""
+    out = []
+    blob, p, n0 = golden_inner()
+    out.append((p, blob))
+    spans, _ = walk(blob)
+    seen = set()
+    for s, e, name in spans:
+        g = group_of(name)
+        if g in seen:
+            continue
+        seen.add(g)
+        out.append((p, flip(blob, s, e)))
+    out.append((p, blob[:-3]))
+    out.append((O.fri_params(stir_schedule(n0), pi_hash=p.public_inputs_hash, s0=53), blob))
+    blob2, p2 = golden_outer()
+    out.append((p2, blob2))
+    for n0, sch, cc, df, deg, stir in [(1 << 9, [2] * 9, True, 1, 1 << 9, False), (1 << 10, [8, 8, 4], False, 4,
+                                       512, False), (1 << 8, [4, 4, 4, 4], True, 1, 256, True)]:
+        f0 = low_degree_f0(n0, deg, n0)
+        pp = O.fri_params(sch, r=10, s0=10, coeff_commit_final=cc, d_final=df, stir=stir, pi_hash=bytes(32))
+        proof = O.deep_fri_prove(f0, pp)
+        out.append((pp, proof))
+        sp, _ = walk(proof)
+        seen = set()
+        for s, e, name in sp:
+            g = group_of(name)
+            if g not in seen:
+                seen.add(g)
+                out.append((pp, flip(proof, s, e)))
+    return out
+
+
+def test_library_verifier_equals_oracle():
+    c = corpus()
+    verdicts = [(O.deep_fri_verify(p, b), lib_verdict(p, b)) for p, b in c]
+    assert all(a == b for a, b in verdicts), [i for i, (a, b) in enumerate(verdicts) if a != b]
+    assert sum(1 for a, _ in verdicts if a == 1) >= 5 and sum(1 for a, _ in verdicts if a == 0) >= 20
+
+
+def test_library_verify_batch():
+    import stark_swarm_amd as S
+    blob, p, n0 = golden_inner()
+    d = ps()
+    fsbs = []
+    for w in d["shards"]:
+        fsbs.append(O.shard_fs_binding(bytes.fromhex(d["authority_pk_hash_hex"]), bytes.fromhex(d["job_id_hex"]),
+                                       w["shard_id"], bytes.fromhex(w["shard_nonce_hex"])))
+    bad = flip(blob, 40, 48)
+    lp = lib_params(p)
+    got = S.deep_fri_verify_batch(lp, [blob, blob, bad, blob[:50]], [fsbs[0], fsbs[1], fsbs[0], fsbs[0]])
+    assert got == [True, False, False, None]
+
+
+def test_library_merkle_verify_opening_equals_oracle():
+    import stark_swarm_amd as S
+    rng = np.random.default_rng(4)
+    for n, vpl, arity in [(1, 1, 2), (9, 3, 2), (64, 6, 8), (33, 18, 16)]:
+        vals = rng.integers(0, P, size=(n, vpl), dtype=np.uint64)
+        depth = O.merkle_depth(n, arity)
+        root = O.merkle_root(vals, arity, 3)
+        cfg = S.MerkleChannelCfg([arity] * depth, 3)
+        for idx in sorted({0, n - 1}):
+            op = O.merkle_open(vals, arity, 3, idx)
+            assert S.verify_opening(cfg, root, op) is True
+            assert S.compute_leaf_hash(cfg, idx, vals[idx]) == op[:32]
+            bad = bytearray(op)
+            bad[-1] ^= 1  # the index's top byte: the parent positions change (no path: nothing to check)
+            assert S.verify_opening(cfg, root, bytes(bad)) is (n == 1)
+            assert O.merkle_verify_opening(arity, depth, 3, root, bytes(bad)) == (n == 1)
