@@ -10,7 +10,7 @@
 //   pass A  Z[k1*n2+i2] = w_n^(i2*k1) * DFT_n1( x[i1*n2+i2] )     (columns)
 //   pass B  X[k1+n1*k2] = DFT_n2( Z[k1*n2+i2] )                    (rows)
 // Each pass is one launch of `ntt_tile<LOG_S, INV>`: a 512-lane workgroup owns
-// a tile of G sub-DFTs of size S = 2^LOG_S (S*G = 8192 u64, 68 KiB of padded
+// a tile of G sub-DFTs of size S = 2^LOG_S (S*G = 8192 u64, 70 KiB of padded
 // LDS, two workgroups per CU). The tile is loaded with its contiguous
 // dimension on consecutive lanes (coalesced runs of G u64 or whole rows), the
 // sub-DFT runs as Stockham steps of radix <= 16 held in VGPRs (16 elements
@@ -19,8 +19,9 @@
 //
 // Inside a register DFT every twiddle is a power of w_32 = 2^78 (mod p), so
 // it is a shift plus one 128->64 reduction instead of a full 64x64 product.
-// The LDS index is padded by one u64 per 16 to break the bank conflicts of
-// the strided Stockham writes.
+// The LDS index is padded by one u64 per 16 and skewed per sub-DFT (padi) to
+// break the bank conflicts of the strided Stockham writes and the g-fast
+// loads and stores.
 //
 // The LDE of a column with blowup B = 2^LB uses
 //   LDE[B*k + r] = DFT_n( c_j * w_{Bn}^(r*j) )[k],  r < B,
@@ -55,6 +56,11 @@ namespace {
 // read and write g-fast spilled 64 bytes and ran no faster, so it is not
 // kept.
 constexpr int MODE_GENERAL = 0, MODE_PLAIN = 1;
+// MODE_LDEA: the LDE's first pass, whose shape launch_pass checks: sub-DFTs
+// g-fast on both sides with one lane per (g, i) group, a pre-twiddle read
+// from the w_{2^13} table, a post twiddle, lazily reduced output and no
+// scaling. Every other path is compiled out.
+constexpr int MODE_LDEA = 2;
 
 template <int LOG_S, bool INV, int MODE, int TLOG>
 // 16 elements a lane, at most 128 VGPRs: 512 lanes with two blocks per CU or
@@ -70,6 +76,14 @@ __global__ __launch_bounds__(TileCfg<TLOG>::LANES, 4) void ntt_tile(NttPass p) {
         p.post_log = 0;
         p.scale = 1;
         p.sp_lo = nullptr;
+        p.lazy_out = 0;
+    }
+    if constexpr (MODE == MODE_LDEA) {
+        p.pre_direct = 1;
+        p.scale = 1;
+        p.sp_lo = nullptr;
+        p.lazy_out = 1;
+        p.inv = 0;
     }
     __shared__ uint64_t lds[TileCfg<TLOG>::LDS];
     constexpr uint32_t S = 1u << LOG_S;
@@ -93,8 +107,8 @@ __global__ __launch_bounds__(TileCfg<TLOG>::LANES, 4) void ntt_tile(NttPass p) {
     const bool out_gfast = log_g > 0 && p.og < p.ok;
     // g-fast lane mapping: element u of a lane is (g = gl, i = il + u*di), so
     // the lane's twiddles form geometric sequences in u (no table gathers)
-    const bool inc_in = in_gfast && (1u << log_g) <= (uint32_t)LANES;
-    const bool inc_out = out_gfast && (1u << log_g) <= (uint32_t)LANES;
+    const bool inc_in = MODE == MODE_LDEA || (in_gfast && (1u << log_g) <= (uint32_t)LANES);
+    const bool inc_out = MODE == MODE_LDEA || (out_gfast && (1u << log_g) <= (uint32_t)LANES);
     const uint32_t gl = threadIdx.x & (G - 1);
     const uint32_t il = threadIdx.x >> log_g;
     const uint32_t di = LANES >> log_g;
@@ -104,7 +118,7 @@ __global__ __launch_bounds__(TileCfg<TLOG>::LANES, 4) void ntt_tile(NttPass p) {
     for (uint32_t rep = 0; rep < p.nr; rep++) {
         // pre-twiddle w^(r * (g*pre_g + i*pre_i + b*pre_b)), r = r0 + rbase + rep
         const uint64_t rr = p.r0 + rbase + rep;
-        const bool pre = p.pre_log && rr;
+        const bool pre = (MODE == MODE_LDEA || p.pre_log) && rr;
         if (inc_in) {
             // pre factors along the lane's elements: a geometric sequence
             // (one product per step), or with pre_direct one table read per
@@ -127,10 +141,11 @@ __global__ __launch_bounds__(TileCfg<TLOG>::LANES, 4) void ntt_tile(NttPass p) {
                 const uint32_t i = il + u * di;
                 if (i < S) {
                     uint64_t x = gok ? in[(int64_t)gl * p.ig + (int64_t)i * p.ii] : 0;
-                    if (pre) x = mul(x, p.pre_direct ? direct_root(p, ed) : w);
-                    lds[padi((gl << LOG_S) + i)] = x;
+                    // the Stockham steps take lazily reduced values
+                    if (pre) x = mul_lazy(x, p.pre_direct ? direct_root(p, ed) : w);
+                    lds[padi<LOG_S>((gl << LOG_S) + i)] = x;
                 }
-                if (pre_inc) w = mul(w, st);
+                if (pre_inc) w = mul_lazy(w, st);
                 ed += es;
             }
         } else {
@@ -151,11 +166,11 @@ __global__ __launch_bounds__(TileCfg<TLOG>::LANES, 4) void ntt_tile(NttPass p) {
                     x = in[(int64_t)g * p.ig + (int64_t)i * p.ii];
                     if (pre) {
                         const uint64_t ee = (g0 + g) * p.pre_g + (uint64_t)i * p.pre_i + b * p.pre_b;
-                        x = mul(x, p.pre_direct ? direct_root(p, (uint32_t)(ee * rr))
-                                                : root_pow(p.rt_lo, p.rt_hi, p.pre_log, ee * rr, false));
+                        x = mul_lazy(x, p.pre_direct ? direct_root(p, (uint32_t)(ee * rr))
+                                                     : root_pow(p.rt_lo, p.rt_hi, p.pre_log, ee * rr, false));
                     }
                 }
-                lds[padi((g << LOG_S) + i)] = x;
+                lds[padi<LOG_S>((g << LOG_S) + i)] = x;
             }
         }
         __syncthreads();
@@ -165,8 +180,9 @@ __global__ __launch_bounds__(TileCfg<TLOG>::LANES, 4) void ntt_tile(NttPass p) {
             // post twiddle w^(+-g*k) and output scale * s^(g*sp_g + k*sp_k)
             uint64_t pw = 1, pst = 1, sw = p.scale, sst = 1;
             if (p.post_log) {
-                pw = root_pow(p.rt_lo, p.rt_hi, p.post_log, (g0 + gl) * il, p.inv);
-                pst = root_pow(p.rt_lo, p.rt_hi, p.post_log, (g0 + gl) * di, p.inv);
+                const uint64_t gg = g0 + gl;
+                pw = root_pow(p.rt_lo, p.rt_hi, p.post_log, gg * (il * p.post_mul + rr * p.post_rep), p.inv);
+                pst = root_pow(p.rt_lo, p.rt_hi, p.post_log, gg * di * p.post_mul, p.inv);
             }
             if (p.sp_lo) {
                 const uint64_t e0 = (g0 + gl) * p.sp_g + (uint64_t)il * p.sp_k, ed = (uint64_t)di * p.sp_k;
@@ -178,14 +194,19 @@ __global__ __launch_bounds__(TileCfg<TLOG>::LANES, 4) void ntt_tile(NttPass p) {
             for (int u = 0; u < PER_LANE; u++) {
                 const uint32_t k = il + u * di;
                 if (k < S && gok) {
-                    uint64_t x = lds[padi((gl << LOG_S) + k)];
-                    if (p.post_log) x = mul(x, pw);
-                    if (sc) x = mul(x, sw);
-                    if (!p.post_log && !sc) x = canon(x);
+                    uint64_t x = lds[padi<LOG_S>((gl << LOG_S) + k)];
+                    if (p.lazy_out) {
+                        if (p.post_log) x = mul_lazy(x, pw);
+                        if (sc) x = mul_lazy(x, sw);
+                    } else {
+                        if (p.post_log) x = mul(x, pw);
+                        if (sc) x = mul(x, sw);
+                        if (!p.post_log && !sc) x = canon(x);
+                    }
                     out[(int64_t)gl * p.og + (int64_t)k * p.ok] = x;
                 }
-                if (p.post_log) pw = mul(pw, pst);
-                if (p.sp_lo) sw = mul(sw, sst);
+                if (p.post_log) pw = mul_lazy(pw, pst);
+                if (p.sp_lo) sw = mul_lazy(sw, sst);
             }
         } else {
 #pragma unroll 2
@@ -201,14 +222,16 @@ __global__ __launch_bounds__(TileCfg<TLOG>::LANES, 4) void ntt_tile(NttPass p) {
                     g = e >> LOG_S;
                 }
                 if (g0 + g >= p.ng) continue;
-                uint64_t x = lds[padi((g << LOG_S) + k)];
-                if (p.post_log) x = mul(x, root_pow(p.rt_lo, p.rt_hi, p.post_log, (g0 + g) * k, p.inv));
+                uint64_t x = lds[padi<LOG_S>((g << LOG_S) + k)];
+                if (p.post_log)
+                    x = mul(x, root_pow(p.rt_lo, p.rt_hi, p.post_log, (g0 + g) * (k * p.post_mul + rr * p.post_rep),
+                                        p.inv));
                 if (p.scale != 1) x = mul(x, p.scale);
                 if (p.sp_lo) {
                     const uint64_t ee = (g0 + g) * p.sp_g + (uint64_t)k * p.sp_k;
                     x = mul(x, mul(p.sp_lo[ee & 8191], p.sp_hi[ee >> 13]));
                 }
-                if (!p.post_log && p.scale == 1 && !p.sp_lo) x = canon(x);
+                if (!p.lazy_out && !p.post_log && p.scale == 1 && !p.sp_lo) x = canon(x);
                 out[(int64_t)g * p.og + (int64_t)k * p.ok] = x;
             }
         }
@@ -252,7 +275,7 @@ __global__ __launch_bounds__(NT, 4) void lde_small_tile(const uint64_t *coef, ui
             if (i < S) {
                 uint64_t x = ok ? src[i] : 0;
                 if (r) x = mul(x, w);
-                lds[padi((gl << LOG_S) + i)] = x;
+                lds[padi<LOG_S>((gl << LOG_S) + i)] = x;
             }
             if (r) w = mul(w, st);
         }
@@ -264,7 +287,7 @@ __global__ __launch_bounds__(NT, 4) void lde_small_tile(const uint64_t *coef, ui
 #pragma unroll 2
     for (uint32_t u = 0; u < EPT; u++) {
         const uint32_t k = il + u * DI;
-        if (k < S) dst[(uint64_t)k << log_b] = canon(lds[padi((gl << LOG_S) + k)]);
+        if (k < S) dst[(uint64_t)k << log_b] = canon(lds[padi<LOG_S>((gl << LOG_S) + k)]);
     }
 }
 
@@ -287,14 +310,14 @@ __global__ __launch_bounds__(NT, 4) void odd_coset_tile(const uint64_t *in, uint
     for (uint32_t u = 0; u < EPT; u++) {
         const uint32_t e = threadIdx.x + u * NT;
         const uint64_t col = col0 + (e >> LOG_S);
-        lds[padi(e)] = col < ncols ? in[col * in_cs + (e & (S - 1))] : 0;
+        lds[padi<LOG_S>(e)] = col < ncols ? in[col * in_cs + (e & (S - 1))] : 0;
     }
     __syncthreads();
     stockham_steps<LOG_S, 0, true>(lds, TILE, tw_i);
 #pragma unroll 2
     for (uint32_t u = 0; u < EPT; u++) {
         const uint32_t e = threadIdx.x + u * NT;
-        lds[padi(e)] = mul(lds[padi(e)], twist[e & (S - 1)]);
+        lds[padi<LOG_S>(e)] = mul(lds[padi<LOG_S>(e)], twist[e & (S - 1)]);
     }
     __syncthreads();
     stockham_steps<LOG_S, 0, false>(lds, TILE, tw_f);
@@ -302,7 +325,7 @@ __global__ __launch_bounds__(NT, 4) void odd_coset_tile(const uint64_t *in, uint
     for (uint32_t u = 0; u < EPT; u++) {
         const uint32_t e = threadIdx.x + u * NT;
         const uint64_t col = col0 + (e >> LOG_S);
-        if (col < ncols) out[col * out_cs + (e & (S - 1))] = canon(lds[padi(e)]);
+        if (col < ncols) out[col * out_cs + (e & (S - 1))] = canon(lds[padi<LOG_S>(e)]);
     }
 }
 
@@ -360,10 +383,15 @@ void launch_pass(Ctx *c, NttPass p, const char *name) {
     p.nr >>= rs_log;
     const dim3 grid((uint32_t)(tiles << rs_log));
     const bool plain = p.pre_log == 0 && p.post_log == 0 && p.scale == 1 && p.sp_lo == nullptr;
+    const uint32_t lanes = p.tile_log == 14 ? 1024 : NT;
+    const bool ldea = !p.inv && p.pre_direct && p.post_log && p.lazy_out && p.scale == 1 && !p.sp_lo &&
+                      p.log_g > 0 && p.ig < p.ii && p.og < p.ok && (1u << p.log_g) <= lanes;
     if (p.inv)
         launch_tile<true, MODE_GENERAL>(p.log_s, p.tile_log, grid, c->stream, p);
     else if (plain)
         launch_tile<false, MODE_PLAIN>(p.log_s, p.tile_log, grid, c->stream, p);
+    else if (ldea)
+        launch_tile<false, MODE_LDEA>(p.log_s, p.tile_log, grid, c->stream, p);
     else
         launch_tile<false, MODE_GENERAL>(p.log_s, p.tile_log, grid, c->stream, p);
     SG_HIP(hipGetLastError());
@@ -400,6 +428,7 @@ static NttPass base_pass() {
     p.scale = 1;
     p.nr = 1;
     p.nb2 = 1;
+    p.post_mul = 1;
     return p;
 }
 
@@ -547,6 +576,7 @@ void ntt_two_pass(Ctx *c, const uint64_t *in, uint64_t *out, uint64_t *scratch, 
     a.tile_log = plan_tile(plan);
     a.inv = inv;
     a.post_log = L;
+    a.lazy_out = 1;  // pass B reads lazily reduced values
     launch_pass(c, a, inv ? "intt_passA" : "ntt_passA");
     NttPass b = base_pass();
     b.in = scratch;
@@ -710,10 +740,16 @@ void lde_forward(Ctx *c, const uint64_t *coef, uint64_t *out, uint32_t L, uint32
         a.ok = B * n2;
         a.log_s = L1;
         a.tile_log = tl;
-        a.post_log = L;
-        a.pre_log = LN;  // repetition r multiplies element j = i*n2 + g by w_{Bn}^(r j)
-        a.pre_g = 1;
-        a.pre_i = n2;
+        // repetition r multiplies element j = i*n2 + g by w_{Bn}^(r j) =
+        // w_{B n1}^(r i) w_{Bn}^(r g): the first factor before the sub-DFT
+        // over i (order B n1, one table read), the second, constant over i,
+        // after it together with the four-step twiddle w_n^(g k) = w_{Bn}^(B g k)
+        a.pre_log = L1 + LB;
+        a.pre_i = 1;
+        a.post_log = LN;
+        a.post_mul = B;
+        a.post_rep = 1;
+        a.lazy_out = 1;  // pass B reads lazily reduced values
         a.nr = (uint32_t)B;
         a.out_rs = n2;
         launch_pass(c, a, "lde_passA");

@@ -35,8 +35,13 @@ struct NttPass {
     // read of the w_{2^13} table instead of a product
     uint32_t pre_log, pre_direct;
     uint64_t pre_g, pre_i, pre_b;
-    // output multiply by w_{2^post_log}^(g*k), inverse root when inv (0 = off)
+    // output multiply by w_{2^post_log}^(g*(k*post_mul + r*post_rep)), inverse
+    // root when inv (0 = off); r = r0 + q is the repetition's multiplier
     uint32_t post_log;
+    uint64_t post_mul, post_rep;
+    // outputs may be any u64 congruent to the value (a scratch that another
+    // pass of these kernels reads) instead of canonical
+    uint32_t lazy_out;
     uint64_t scale;
     // output multiply by s^(g*sp_g + k*sp_k) from a two-level power table
     const uint64_t *sp_lo, *sp_hi;

@@ -16,26 +16,46 @@ constexpr int NT = 512;
 constexpr int EPT = TILE / NT;  // elements held per lane during a step
 constexpr int MAX_LR = 4;       // register DFTs of at most 2^MAX_LR points
 constexpr int PAD_SHIFT = 4;
-constexpr int LDS_ELEMS = TILE + (TILE >> PAD_SHIFT);
+// padded tile size: one u64 per 16, plus the sub-DFT skew of padi
+constexpr int LDS_ELEMS = TILE + (TILE >> PAD_SHIFT) + (TILE >> 5);
 
-__device__ __forceinline__ uint32_t padi(uint32_t e) { return e + (e >> PAD_SHIFT); }
+// LDS index of tile element e for sub-DFTs of 2^LOG_S points. One u64 of
+// padding per 16 breaks the bank conflicts of the strided Stockham accesses;
+// for LOG_S >= 6 each sub-DFT also starts 2 u64 after the previous one ends,
+// so a sub-DFT's stride is 2 mod 16 and the g-fast loads and stores (the 16
+// lanes of a quarter-wave on 8 sub-DFTs at 2 neighbouring offsets) land in
+// distinct banks instead of 8 to a bank.
+template <int LOG_S>
+__device__ __forceinline__ uint32_t padi(uint32_t e) {
+    if constexpr (LOG_S >= 6)
+        return e + (e >> PAD_SHIFT) + 2 * (e >> LOG_S);
+    else
+        return e + (e >> PAD_SHIFT);
+}
 
 // A tile of 2^TLOG u64: 8K on 512 lanes (two workgroups per CU) or 16K on
-// 1024 lanes (one workgroup per CU, 136 KiB of LDS); 16 elements per lane
+// 1024 lanes (one workgroup per CU, 140 KiB of LDS); 16 elements per lane
 // either way, so the register budget of the Stockham steps is the same.
 template <int TLOG>
 struct TileCfg {
     static constexpr int SIZE = 1 << TLOG;
     static constexpr int LANES = TLOG >= 14 ? 1024 : NT;
     static constexpr int PER_LANE = SIZE / LANES;
-    static constexpr int LDS = SIZE + (SIZE >> PAD_SHIFT);
+    static constexpr int LDS = SIZE + (SIZE >> PAD_SHIFT) + (SIZE >> 5);
 };
+
+// x * 2^s mod p for a compile-time-foldable s in [0, 96); x may be lazily
+// reduced (any u64), the result is canonical
+__device__ __forceinline__ uint64_t mul_pow2_pos(uint64_t x, int s);
 
 // x * 2^k mod p for a compile-time-foldable k in [0, 192): 2^96 = -1.
 // x may be lazily reduced (any u64); the result is canonical.
 __device__ __forceinline__ uint64_t mul_pow2(uint64_t x, int k) {
-    const bool negate = k >= 96;
-    const int s = k % 96;
+    const uint64_t y = mul_pow2_pos(x, k % 96);
+    return k >= 96 ? neg(y) : y;
+}
+
+__device__ __forceinline__ uint64_t mul_pow2_pos(uint64_t x, int s) {
     uint64_t y;
     if (s == 0) {
         y = canon(x);
@@ -61,11 +81,13 @@ __device__ __forceinline__ uint64_t mul_pow2(uint64_t x, int k) {
         if (ce < a) r -= EPS;  // + p (mod 2^64); ce - a + p > 0 since a < 2^63
         y = canon(r);
     }
-    return negate ? neg(y) : y;
+    return y;
 }
 
 // lazily reduced add / sub: a is any u64, b is canonical; results are any u64
-// congruent to a +- b (2^64 = 2^32 - 1 mod p)
+// congruent to a +- b (2^64 = 2^32 - 1 mod p). (32-bit carry chains measured
+// slower here: every step reads VCC, which serializes them and costs hazard
+// wait states.)
 __device__ __forceinline__ uint64_t lz_add(uint64_t a, uint64_t b) {
     uint64_t r;
     const bool c = __builtin_add_overflow(a, b, &r);
@@ -112,13 +134,20 @@ __device__ __forceinline__ void dft_reg(uint64_t *v) {
         for (int base = 0; base < R; base += 2 * half) {
 #pragma unroll
             for (int j = 0; j < half; j++) {
-                // w_{2 half}^j = w_32^(j * 32 / (2 half))
-                // t canonical (twiddled, or reduced once), u lazy
+                // w_{2 half}^j = w_32^(j * 32 / (2 half)) = +-2^s (2^96 = -1):
+                // t = 2^s x canonical (or x reduced once for j = 0), u lazy,
+                // and a negative twiddle swaps the sum and the difference
+                const int k = j != 0 ? w32_exp(j * (32 >> (lh + 1)), INV) : 0;
                 uint64_t t = v[base + j + half];
-                t = (j != 0) ? mul_pow2(t, w32_exp(j * (32 >> (lh + 1)), INV)) : canon(t);
+                t = (j != 0) ? mul_pow2_pos(t, k % 96) : canon(t);
                 const uint64_t u = v[base + j];
-                v[base + j] = lz_add(u, t);
-                v[base + j + half] = lz_sub(u, t);
+                if (k >= 96) {
+                    v[base + j] = lz_sub(u, t);
+                    v[base + j + half] = lz_add(u, t);
+                } else {
+                    v[base + j] = lz_add(u, t);
+                    v[base + j + half] = lz_sub(u, t);
+                }
             }
         }
     }
@@ -161,7 +190,7 @@ __device__ __forceinline__ void stockham_steps(uint64_t *lds, uint32_t tile, con
                 const uint32_t g = q >> SUB_LOG, j = q & (SUB - 1);
                 const uint32_t base = (g << LOG_S) + j;
 #pragma unroll
-                for (int r = 0; r < R; r++) v[u][r] = lds[padi(base + r * SUB)];
+                for (int r = 0; r < R; r++) v[u][r] = lds[padi<LOG_S>(base + r * SUB)];
                 if constexpr (LOG_NS > 0) {
                     const uint32_t jm = j & (NS - 1);
                     const uint64_t *t = tw + (NS << LR);
@@ -180,7 +209,7 @@ __device__ __forceinline__ void stockham_steps(uint64_t *lds, uint32_t tile, con
                 const uint32_t jm = j & (NS - 1);
                 const uint32_t base = (g << LOG_S) + ((j >> LOG_NS) << (LOG_NS + LR)) + jm;
 #pragma unroll
-                for (int r = 0; r < R; r++) lds[padi(base + r * NS)] = v[u][r];
+                for (int r = 0; r < R; r++) lds[padi<LOG_S>(base + r * NS)] = v[u][r];
             }
         }
         __syncthreads();

@@ -703,14 +703,14 @@ __global__ __launch_bounds__(NT, 4) void expand_odd_p256_k(const uint64_t *words
             else x = code == D_POS ? (uint64_t)(v == 1) : (uint64_t)(v == ~0ULL);
             trk[col * S + row] = x;
         }
-        lds[padi(e)] = x;
+        lds[padi<LOG_H>(e)] = x;
     }
     __syncthreads();
     stockham_steps<LOG_H, 0, true>(lds, TILE, tw_i);
 #pragma unroll 2
     for (uint32_t u = 0; u < EPT; u++) {
         const uint32_t e = threadIdx.x + u * NT;
-        lds[padi(e)] = mul(lds[padi(e)], twist[e & (S - 1)]);
+        lds[padi<LOG_H>(e)] = mul(lds[padi<LOG_H>(e)], twist[e & (S - 1)]);
     }
     __syncthreads();
     stockham_steps<LOG_H, 0, false>(lds, TILE, tw_f);
@@ -718,7 +718,7 @@ __global__ __launch_bounds__(NT, 4) void expand_odd_p256_k(const uint64_t *words
     for (uint32_t u = 0; u < EPT; u++) {
         const uint32_t e = threadIdx.x + u * NT;
         const uint64_t col = col0 + (e >> LOG_H);
-        if (col < P256_W) oddk[col * S + (e & (S - 1))] = canon(lds[padi(e)]);
+        if (col < P256_W) oddk[col * S + (e & (S - 1))] = canon(lds[padi<LOG_H>(e)]);
     }
 }
 
