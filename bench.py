@@ -460,9 +460,11 @@ def main():
                      "valu_issue_load": {k: profiled_valu_load(k) for k in ("lde_passA", "lde_passB")} if full
                      else None,
                      "note": "algorithmic bytes = the 2^20 x 128 coefficients read once + the 2^23 x 128 LDE written "
-                             "once, over the two passes' time; bound from the counters: both passes keep the SIMDs' "
-                             "integer VALU busy (64-bit modular products on 32-bit lanes), pass A at valu_issue_load "
-                             "~0.85, while their HBM traffic (the scratch round trip included) stays far below peak"},
+                             "once, over the two passes' time; bound from the counters: integer VALU work (64-bit "
+                             "modular products and lazily reduced butterflies on 32-bit lanes, 270 and 209 "
+                             "instructions per LDE element) issued at the valu_issue_load above, with pass B's "
+                             "remaining cycles in load / store latency; their HBM traffic (the scratch round trip "
+                             "included) stays far below peak"},
         "stages": stages,
         "kernels": kernels,
         "autotune": {"enabled": bool(args.autotune), "plans": plans,
