@@ -119,6 +119,12 @@ __global__ __launch_bounds__(TileCfg<TLOG>::LANES, 4) void ntt_tile(NttPass p) {
         // pre-twiddle w^(r * (g*pre_g + i*pre_i + b*pre_b)), r = r0 + rbase + rep
         const uint64_t rr = p.r0 + rbase + rep;
         const bool pre = (MODE == MODE_LDEA || p.pre_log) && rr;
+        // the lane index through an empty asm: the compiler cannot see that
+        // it is loop-invariant, so the per-element addresses of the fully
+        // unrolled load loops below are formed inside the loop rather than
+        // hoisted out of it as 16 live 64-bit values that spill to scratch
+        uint32_t tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
         if (inc_in) {
             // pre factors along the lane's elements: a geometric sequence
             // (one product per step), or with pre_direct one table read per
@@ -136,22 +142,41 @@ __global__ __launch_bounds__(TileCfg<TLOG>::LANES, 4) void ntt_tile(NttPass p) {
                     st = root_pow(p.rt_lo, p.rt_hi, p.pre_log, rr * di * p.pre_i, false);
                 }
             }
-#pragma unroll 2
+            // every load of the lane (and its table twiddle) is issued before
+            // the first is used, so the tile pays one memory latency
+            uint64_t xv[PER_LANE], wv[PER_LANE];
+            const uint32_t glr = tid & (G - 1), ilr = tid >> log_g;
+            const uint64_t *src = in + (int64_t)glr * p.ig + (int64_t)ilr * p.ii;
+            const int64_t sstep = (int64_t)di * p.ii;
+#pragma unroll
             for (int u = 0; u < PER_LANE; u++) {
-                const uint32_t i = il + u * di;
+                const uint32_t i = ilr + u * di;
+                xv[u] = (i < S && gok) ? src[u * sstep] : 0;
+                if (pre && p.pre_direct) wv[u] = direct_root(p, ed + (uint32_t)u * es);
+            }
+#pragma unroll
+            for (int u = 0; u < PER_LANE; u++) {
+                const uint32_t i = ilr + u * di;
                 if (i < S) {
-                    uint64_t x = gok ? in[(int64_t)gl * p.ig + (int64_t)i * p.ii] : 0;
+                    uint64_t x = xv[u];
                     // the Stockham steps take lazily reduced values
-                    if (pre) x = mul_lazy(x, p.pre_direct ? direct_root(p, ed) : w);
-                    lds[padi<LOG_S>((gl << LOG_S) + i)] = x;
+                    if (pre) x = mul_lazy(x, p.pre_direct ? wv[u] : w);
+                    lds[padi<LOG_S>((glr << LOG_S) + i)] = x;
                 }
                 if (pre_inc) w = mul_lazy(w, st);
-                ed += es;
             }
         } else {
-#pragma unroll 2
+            uint64_t xv[PER_LANE];
+#pragma unroll
             for (int u = 0; u < PER_LANE; u++) {
-                const uint32_t e = threadIdx.x + u * LANES;
+                const uint32_t e = tid + u * LANES;
+                const uint32_t g = in_gfast ? (e & (G - 1)) : (e >> LOG_S);
+                const uint32_t i = in_gfast ? (e >> log_g) : (e & (S - 1));
+                xv[u] = (e < tile && g0 + g < p.ng) ? in[(int64_t)g * p.ig + (int64_t)i * p.ii] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < PER_LANE; u++) {
+                const uint32_t e = tid + u * LANES;
                 if (e >= tile) continue;
                 uint32_t g, i;
                 if (in_gfast) {
@@ -161,14 +186,11 @@ __global__ __launch_bounds__(TileCfg<TLOG>::LANES, 4) void ntt_tile(NttPass p) {
                     i = e & (S - 1);
                     g = e >> LOG_S;
                 }
-                uint64_t x = 0;
-                if (g0 + g < p.ng) {
-                    x = in[(int64_t)g * p.ig + (int64_t)i * p.ii];
-                    if (pre) {
-                        const uint64_t ee = (g0 + g) * p.pre_g + (uint64_t)i * p.pre_i + b * p.pre_b;
-                        x = mul_lazy(x, p.pre_direct ? direct_root(p, (uint32_t)(ee * rr))
-                                                     : root_pow(p.rt_lo, p.rt_hi, p.pre_log, ee * rr, false));
-                    }
+                uint64_t x = xv[u];
+                if (pre && g0 + g < p.ng) {
+                    const uint64_t ee = (g0 + g) * p.pre_g + (uint64_t)i * p.pre_i + b * p.pre_b;
+                    x = mul_lazy(x, p.pre_direct ? direct_root(p, (uint32_t)(ee * rr))
+                                                 : root_pow(p.rt_lo, p.rt_hi, p.pre_log, ee * rr, false));
                 }
                 lds[padi<LOG_S>((g << LOG_S) + i)] = x;
             }

@@ -182,10 +182,14 @@ __device__ __forceinline__ void stockham_steps(uint64_t *lds, uint32_t tile, con
         constexpr uint32_t SUB = 1u << SUB_LOG;
         constexpr uint32_t NS = 1u << LOG_NS;
         const uint32_t groups = tile >> LR;
+        // the lane index through an empty asm, so a caller's loop around the
+        // steps cannot hoist their LDS addresses out as live (spilled) values
+        uint32_t lane = threadIdx.x;
+        asm volatile("" : "+v"(lane));
         uint64_t v[MAXG][R];
 #pragma unroll
         for (int u = 0; u < MAXG; u++) {
-            const uint32_t q = threadIdx.x + u * LANES;
+            const uint32_t q = lane + u * LANES;
             if (q < groups) {
                 const uint32_t g = q >> SUB_LOG, j = q & (SUB - 1);
                 const uint32_t base = (g << LOG_S) + j;
@@ -203,7 +207,7 @@ __device__ __forceinline__ void stockham_steps(uint64_t *lds, uint32_t tile, con
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < MAXG; u++) {
-            const uint32_t q = threadIdx.x + u * LANES;
+            const uint32_t q = lane + u * LANES;
             if (q < groups) {
                 const uint32_t g = q >> SUB_LOG, j = q & (SUB - 1);
                 const uint32_t jm = j & (NS - 1);
