@@ -166,33 +166,41 @@ __global__ __launch_bounds__(TileCfg<TLOG>::LANES, 4) void ntt_tile(NttPass p) {
                 if (pre_inc) w = mul_lazy(w, st);
             }
         } else {
-            uint64_t xv[PER_LANE];
+            // the loads of CH elements and their twiddle table reads are
+            // issued together, then used
+            constexpr int CH = 8;
 #pragma unroll
-            for (int u = 0; u < PER_LANE; u++) {
-                const uint32_t e = tid + u * LANES;
-                const uint32_t g = in_gfast ? (e & (G - 1)) : (e >> LOG_S);
-                const uint32_t i = in_gfast ? (e >> log_g) : (e & (S - 1));
-                xv[u] = (e < tile && g0 + g < p.ng) ? in[(int64_t)g * p.ig + (int64_t)i * p.ii] : 0;
-            }
+            for (int u0 = 0; u0 < PER_LANE; u0 += CH) {
+                uint64_t xv[CH], ta[CH], tb[CH];
 #pragma unroll
-            for (int u = 0; u < PER_LANE; u++) {
-                const uint32_t e = tid + u * LANES;
-                if (e >= tile) continue;
-                uint32_t g, i;
-                if (in_gfast) {
-                    g = e & (G - 1);
-                    i = e >> log_g;
-                } else {
-                    i = e & (S - 1);
-                    g = e >> LOG_S;
+                for (int c = 0; c < CH; c++) {
+                    const uint32_t e = tid + (u0 + c) * LANES;
+                    const uint32_t g = in_gfast ? (e & (G - 1)) : (e >> LOG_S);
+                    const uint32_t i = in_gfast ? (e >> log_g) : (e & (S - 1));
+                    const bool ok = e < tile && g0 + g < p.ng;
+                    xv[c] = ok ? in[(int64_t)g * p.ig + (int64_t)i * p.ii] : 0;
+                    ta[c] = tb[c] = 1;
+                    if (pre && ok) {
+                        const uint64_t ee = (g0 + g) * p.pre_g + (uint64_t)i * p.pre_i + b * p.pre_b;
+                        if (p.pre_direct) {
+                            ta[c] = direct_root(p, (uint32_t)(ee * rr));
+                        } else {
+                            const uint32_t e26 = root_index(p.pre_log, ee * rr, false);
+                            ta[c] = p.rt_lo[e26 & 8191];
+                            tb[c] = p.rt_hi[e26 >> 13];
+                        }
+                    }
                 }
-                uint64_t x = xv[u];
-                if (pre && g0 + g < p.ng) {
-                    const uint64_t ee = (g0 + g) * p.pre_g + (uint64_t)i * p.pre_i + b * p.pre_b;
-                    x = mul_lazy(x, p.pre_direct ? direct_root(p, (uint32_t)(ee * rr))
-                                                 : root_pow(p.rt_lo, p.rt_hi, p.pre_log, ee * rr, false));
+#pragma unroll
+                for (int c = 0; c < CH; c++) {
+                    const uint32_t e = tid + (u0 + c) * LANES;
+                    if (e >= tile) continue;
+                    const uint32_t g = in_gfast ? (e & (G - 1)) : (e >> LOG_S);
+                    const uint32_t i = in_gfast ? (e >> log_g) : (e & (S - 1));
+                    uint64_t x = xv[c];
+                    if (pre && g0 + g < p.ng) x = mul_lazy(x, p.pre_direct ? ta[c] : mul(ta[c], tb[c]));
+                    lds[padi<LOG_S>((g << LOG_S) + i)] = x;
                 }
-                lds[padi<LOG_S>((g << LOG_S) + i)] = x;
             }
         }
         __syncthreads();
@@ -231,30 +239,43 @@ __global__ __launch_bounds__(TileCfg<TLOG>::LANES, 4) void ntt_tile(NttPass p) {
                 if (p.sp_lo) sw = mul_lazy(sw, sst);
             }
         } else {
-#pragma unroll 2
-            for (int u = 0; u < PER_LANE; u++) {
-                const uint32_t e = threadIdx.x + u * LANES;
-                if (e >= tile) continue;
-                uint32_t g, k;
-                if (out_gfast) {
-                    g = e & (G - 1);
-                    k = e >> log_g;
-                } else {
-                    k = e & (S - 1);
-                    g = e >> LOG_S;
+            // the table reads of CH elements are issued together, then used
+            constexpr int CH = 4;
+#pragma unroll
+            for (int u0 = 0; u0 < PER_LANE; u0 += CH) {
+                uint64_t ra[CH], rb[CH], sa[CH], sb[CH];
+#pragma unroll
+                for (int c = 0; c < CH; c++) {
+                    const uint32_t e = tid + (u0 + c) * LANES;
+                    const uint32_t g = out_gfast ? (e & (G - 1)) : (e >> LOG_S);
+                    const uint32_t k = out_gfast ? (e >> log_g) : (e & (S - 1));
+                    const bool ok = e < tile && g0 + g < p.ng;
+                    ra[c] = rb[c] = sa[c] = sb[c] = 1;
+                    if (p.post_log && ok) {
+                        const uint32_t e26 = root_index(p.post_log, (g0 + g) * (k * p.post_mul + rr * p.post_rep),
+                                                        p.inv);
+                        ra[c] = p.rt_lo[e26 & 8191];
+                        rb[c] = p.rt_hi[e26 >> 13];
+                    }
+                    if (p.sp_lo && ok) {
+                        const uint64_t ee = (g0 + g) * p.sp_g + (uint64_t)k * p.sp_k;
+                        sa[c] = p.sp_lo[ee & 8191];
+                        sb[c] = p.sp_hi[ee >> 13];
+                    }
                 }
-                if (g0 + g >= p.ng) continue;
-                uint64_t x = lds[padi<LOG_S>((g << LOG_S) + k)];
-                if (p.post_log)
-                    x = mul(x, root_pow(p.rt_lo, p.rt_hi, p.post_log, (g0 + g) * (k * p.post_mul + rr * p.post_rep),
-                                        p.inv));
-                if (p.scale != 1) x = mul(x, p.scale);
-                if (p.sp_lo) {
-                    const uint64_t ee = (g0 + g) * p.sp_g + (uint64_t)k * p.sp_k;
-                    x = mul(x, mul(p.sp_lo[ee & 8191], p.sp_hi[ee >> 13]));
+#pragma unroll
+                for (int c = 0; c < CH; c++) {
+                    const uint32_t e = tid + (u0 + c) * LANES;
+                    const uint32_t g = out_gfast ? (e & (G - 1)) : (e >> LOG_S);
+                    const uint32_t k = out_gfast ? (e >> log_g) : (e & (S - 1));
+                    if (e >= tile || g0 + g >= p.ng) continue;
+                    uint64_t x = lds[padi<LOG_S>((g << LOG_S) + k)];
+                    if (p.post_log) x = mul(x, mul(ra[c], rb[c]));
+                    if (p.scale != 1) x = mul(x, p.scale);
+                    if (p.sp_lo) x = mul(x, mul(sa[c], sb[c]));
+                    if (!p.lazy_out && !p.post_log && p.scale == 1 && !p.sp_lo) x = canon(x);
+                    out[(int64_t)g * p.og + (int64_t)k * p.ok] = x;
                 }
-                if (!p.lazy_out && !p.post_log && p.scale == 1 && !p.sp_lo) x = canon(x);
-                out[(int64_t)g * p.og + (int64_t)k * p.ok] = x;
             }
         }
         __syncthreads();

@@ -153,10 +153,16 @@ __device__ __forceinline__ void dft_reg(uint64_t *v) {
     }
 }
 
-__device__ __forceinline__ uint64_t root_pow(const uint64_t *rt_lo, const uint64_t *rt_hi, uint32_t lg, uint64_t e,
-                                             bool inv) {
+// w_{2^lg}^(+-e) = rt_lo[i & 8191] * rt_hi[i >> 13] with i = root_index(lg, e, inv)
+// (rt_lo[j] = w_{2^26}^j, rt_hi[j] = w_{2^26}^(8192 j))
+__device__ __forceinline__ uint32_t root_index(uint32_t lg, uint64_t e, bool inv) {
     uint64_t e26 = (e & ((1ULL << lg) - 1)) << (26 - lg);
     if (inv) e26 = (0 - e26) & ((1ULL << 26) - 1);
+    return (uint32_t)e26;
+}
+__device__ __forceinline__ uint64_t root_pow(const uint64_t *rt_lo, const uint64_t *rt_hi, uint32_t lg, uint64_t e,
+                                             bool inv) {
+    const uint32_t e26 = root_index(lg, e, inv);
     return mul(rt_lo[e26 & 8191], rt_hi[e26 >> 13]);
 }
 
@@ -193,13 +199,20 @@ __device__ __forceinline__ void stockham_steps(uint64_t *lds, uint32_t tile, con
             if (q < groups) {
                 const uint32_t g = q >> SUB_LOG, j = q & (SUB - 1);
                 const uint32_t base = (g << LOG_S) + j;
-#pragma unroll
-                for (int r = 0; r < R; r++) v[u][r] = lds[padi<LOG_S>(base + r * SUB)];
+                // the group's twiddles (global memory, cache hits) are read
+                // before its LDS values so the two latencies overlap
+                uint64_t tv[R];
                 if constexpr (LOG_NS > 0) {
                     const uint32_t jm = j & (NS - 1);
                     const uint64_t *t = tw + (NS << LR);
 #pragma unroll
-                    for (int r = 1; r < R; r++) v[u][r] = mul_lazy(v[u][r], t[jm * r]);  // dft_reg takes any u64
+                    for (int r = 1; r < R; r++) tv[r] = t[jm * r];
+                }
+#pragma unroll
+                for (int r = 0; r < R; r++) v[u][r] = lds[padi<LOG_S>(base + r * SUB)];
+                if constexpr (LOG_NS > 0) {
+#pragma unroll
+                    for (int r = 1; r < R; r++) v[u][r] = mul_lazy(v[u][r], tv[r]);  // dft_reg takes any u64
                 }
                 dft_reg<LR, INV>(v[u]);
             }
