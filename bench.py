@@ -461,10 +461,10 @@ def main():
                      else None,
                      "note": "algorithmic bytes = the 2^20 x 128 coefficients read once + the 2^23 x 128 LDE written "
                              "once, over the two passes' time; bound from the counters: integer VALU work (64-bit "
-                             "modular products and lazily reduced butterflies on 32-bit lanes, 270 and 209 "
-                             "instructions per LDE element) issued at the valu_issue_load above, with pass B's "
-                             "remaining cycles in load / store latency; their HBM traffic (the scratch round trip "
-                             "included) stays far below peak"},
+                             "modular products and lazily reduced butterflies on 32-bit lanes, 274 and 218 "
+                             "instructions per LDE element) issued at the valu_issue_load above (about one "
+                             "instruction per 4 SIMD cycles: both passes are issue-bound); their HBM traffic (the "
+                             "scratch round trip included) stays far below peak"},
         "stages": stages,
         "kernels": kernels,
         "autotune": {"enabled": bool(args.autotune), "plans": plans,
