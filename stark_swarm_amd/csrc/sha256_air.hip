@@ -191,7 +191,14 @@ __device__ __forceinline__ uint64_t xorp(uint64_t x, uint64_t y) {
 }
 __device__ __forceinline__ uint64_t boolc(uint64_t b) { return mul(b, sub(1, b)); }
 
-constexpr int AT = 128;
+// SP waves per group of 64 points: wave `part` evaluates the constraint
+// sections s with s % SP == part for its 64 points (one per lane, so the
+// wave does not diverge) and skips the slots of the others, so every wave's
+// slot weights stay in step; the SP partial sums meet in LDS. One point's
+// ~770 constraints are a long serial chain for one lane; split, a small
+// proof's few hundred points still keep many CUs busy.
+constexpr int SP = 8;
+constexpr int AT = 64 * SP;
 
 // Gate groups of the constraints: ungated, compression rows, the
 // finalisation row, every other row, message-schedule rows.
@@ -210,8 +217,10 @@ template <bool GROUPS>
 __global__ __launch_bounds__(AT) void compose_sha256_k(const uint64_t *lde, const uint64_t *odd, uint64_t n,
                                                        uint64_t blowup, const uint32_t *n_blocks, uint64_t *phi,
                                                        uint64_t *t1_lam, const uint64_t *lam) {
-    const uint64_t i = (uint64_t)blockIdx.x * AT + threadIdx.x;
-    if (i >= n) return;
+    __shared__ uint64_t red[5][SP][64];
+    const uint32_t part = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const uint64_t i = (uint64_t)blockIdx.x * 64 + lane;  // n is a multiple of 64
+    auto mine = [&](int sec) { return sec % SP == (int)part; };
     // GROUPS: point i is row i / 2 of the traces (even i) or of the odd coset
     const uint64_t rows = GROUPS ? n / 2 : n, at = GROUPS ? i >> 1 : i;
     const uint64_t *L = (GROUPS && (i & 1) ? odd : lde) + (uint64_t)SHA256_W * rows * blockIdx.y;
@@ -229,9 +238,14 @@ __global__ __launch_bounds__(AT) void compose_sha256_k(const uint64_t *lde, cons
     // booleanity of the 8 bit blocks
     const uint32_t boff[8] = {BIT_A, BIT_B, BIT_C, BIT_E, BIT_F, BIT_G, BIT_WW14, BIT_WW1};
 #pragma unroll 1
-    for (int o = 0; o < 8; o++)
+    for (int o = 0; o < 8; o++) {  // sections 0-7
+        if (!mine(o)) {
+            f.skip(32);
+            continue;
+        }
 #pragma unroll 4
         for (int b = 0; b < 32; b++) f.put(boolc(cur(boff[o] + b)));
+    }
     auto pack = [&](uint32_t off) {
         uint64_t s = 0;
 #pragma unroll 8
@@ -240,7 +254,12 @@ __global__ __launch_bounds__(AT) void compose_sha256_k(const uint64_t *lde, cons
     };
     const uint32_t wcol[8] = {A_A, A_A + 1, A_A + 2, A_E, A_E + 1, A_E + 2, WW0 + 14, WW0 + 1};
 #pragma unroll 1
-    for (int o = 0; o < 8; o++) f.put(sub(cur(wcol[o]), pack(boff[o])));
+    for (int o = 0; o < 8; o++) {  // sections 8-15
+        if (mine(8 + o))
+            f.put(sub(cur(wcol[o]), pack(boff[o])));
+        else
+            f.skip(1);
+    }
     // Sigma0(a), Sigma1(e), sigma0(W_win[1]), sigma1(W_win[14])
     struct Sg {
         uint32_t src, t, b, w;
@@ -252,52 +271,83 @@ __global__ __launch_bounds__(AT) void compose_sha256_k(const uint64_t *lde, cons
                       {BIT_WW1, SS0_T, SS0_BIT, SS0_W, 7, 18, 3, true},
                       {BIT_WW14, SS1_T, SS1_BIT, SS1_W, 17, 19, 10, true}};
 #pragma unroll 1
-    for (int s = 0; s < 4; s++) {
+    for (int s = 0; s < 4; s++) {  // sections 16-23
         const Sg &g = sg[s];
+        if (mine(16 + 2 * s)) {
 #pragma unroll 2
-        for (int b = 0; b < 32; b++)
-            f.put(sub(cur(g.t + b), xorp(cur(g.src + (b + g.r1) % 32), cur(g.src + (b + g.r2) % 32))));
+            for (int b = 0; b < 32; b++)
+                f.put(sub(cur(g.t + b), xorp(cur(g.src + (b + g.r1) % 32), cur(g.src + (b + g.r2) % 32))));
+        } else {
+            f.skip(32);
+        }
+        if (mine(17 + 2 * s)) {
+#pragma unroll 2
+            for (int b = 0; b < 32; b++) {
+                const uint64_t third =
+                    g.shr ? (b + g.r3 < 32 ? cur(g.src + b + g.r3) : 0) : cur(g.src + (b + g.r3) % 32);
+                f.put(sub(cur(g.b + b), xorp(cur(g.t + b), third)));
+            }
+            f.put(sub(cur(g.w), pack(g.b)));
+        } else {
+            f.skip(33);
+        }
+    }
+    // Ch(e, f, g)_i = e f + g - e g (section 24)
+    if (mine(24)) {
 #pragma unroll 2
         for (int b = 0; b < 32; b++) {
-            const uint64_t third = g.shr ? (b + g.r3 < 32 ? cur(g.src + b + g.r3) : 0) : cur(g.src + (b + g.r3) % 32);
-            f.put(sub(cur(g.b + b), xorp(cur(g.t + b), third)));
+            const uint64_t e = cur(BIT_E + b), ff = cur(BIT_F + b), g = cur(BIT_G + b);
+            f.put(sub(cur(CH_BIT + b), sub(add(mul(e, ff), g), mul(e, g))));
         }
-        f.put(sub(cur(g.w), pack(g.b)));
+        f.put(sub(cur(CH_W), pack(CH_BIT)));
+    } else {
+        f.skip(33);
     }
-    // Ch(e, f, g)_i = e f + g - e g
+    // Maj(a, b, c) with the ab/ac/bc/u auxiliaries (sections 25-27)
+    if (mine(25)) {
 #pragma unroll 2
-    for (int b = 0; b < 32; b++) {
-        const uint64_t e = cur(BIT_E + b), ff = cur(BIT_F + b), g = cur(BIT_G + b);
-        f.put(sub(cur(CH_BIT + b), sub(add(mul(e, ff), g), mul(e, g))));
+        for (int b = 0; b < 32; b++) {
+            const uint64_t a = cur(BIT_A + b), bb = cur(BIT_B + b), c = cur(BIT_C + b);
+            f.put(sub(cur(MAJ_AB + b), mul(a, bb)));
+            f.put(sub(cur(MAJ_AC + b), mul(a, c)));
+            f.put(sub(cur(MAJ_BC + b), mul(bb, c)));
+        }
+    } else {
+        f.skip(96);
     }
-    f.put(sub(cur(CH_W), pack(CH_BIT)));
-    // Maj(a, b, c) with the ab/ac/bc/u auxiliaries
+    if (mine(26)) {
 #pragma unroll 2
-    for (int b = 0; b < 32; b++) {
-        const uint64_t a = cur(BIT_A + b), bb = cur(BIT_B + b), c = cur(BIT_C + b);
-        f.put(sub(cur(MAJ_AB + b), mul(a, bb)));
-        f.put(sub(cur(MAJ_AC + b), mul(a, c)));
-        f.put(sub(cur(MAJ_BC + b), mul(bb, c)));
+        for (int b = 0; b < 32; b++) f.put(sub(cur(MAJ_U + b), xorp(cur(MAJ_AB + b), cur(MAJ_AC + b))));
+    } else {
+        f.skip(32);
     }
+    if (mine(27)) {
 #pragma unroll 2
-    for (int b = 0; b < 32; b++) f.put(sub(cur(MAJ_U + b), xorp(cur(MAJ_AB + b), cur(MAJ_AC + b))));
-#pragma unroll 2
-    for (int b = 0; b < 32; b++) f.put(sub(cur(MAJ_BIT + b), xorp(cur(MAJ_U + b), cur(MAJ_BC + b))));
-    f.put(sub(cur(MAJ_W), pack(MAJ_BIT)));
+        for (int b = 0; b < 32; b++) f.put(sub(cur(MAJ_BIT + b), xorp(cur(MAJ_U + b), cur(MAJ_BC + b))));
+        f.put(sub(cur(MAJ_W), pack(MAJ_BIT)));
+    } else {
+        f.skip(33);
+    }
     // T1 = h + Sigma1 + Ch + K[t] + W[t] = T1_lo + 2^32 T1_cw; T2 = Sigma0 + Maj
-    const uint64_t krow = GROUPS ? 0 : comp ? KD[br] : 0;
-    const uint64_t t1s = add(add(add(add(cur(A_H), cur(S1_W)), cur(CH_W)), krow), cur(COL_W));
-    if (GROUPS && i == 0 && blockIdx.y == 0) *t1_lam = f.weight();
-    f.put(sub(add(cur(T1_LO), mul(TWO32, cur(T1_CW))), t1s));
-    f.put(boolc(cur(T1_C0)));
-    f.put(boolc(cur(T1_C1)));
-    f.put(boolc(cur(T1_C2)));
-    f.put(sub(sub(sub(cur(T1_CW), cur(T1_C0)), mul(2, cur(T1_C1))), mul(4, cur(T1_C2))));
-    f.put(sub(add(cur(T2_LO), mul(TWO32, cur(T2_CW))), add(cur(S0_W), cur(MAJ_W))));
-    f.put(boolc(cur(T2_C0)));
-    f.put(boolc(cur(T2_C1)));
-    f.put(sub(sub(cur(T2_CW), cur(T2_C0)), mul(2, cur(T2_C1))));
-    if (comp) {
+    // (section 28)
+    if (mine(28)) {
+        const uint64_t krow = GROUPS ? 0 : comp ? KD[br] : 0;
+        const uint64_t t1s = add(add(add(add(cur(A_H), cur(S1_W)), cur(CH_W)), krow), cur(COL_W));
+        if (GROUPS && i == 0 && blockIdx.y == 0) *t1_lam = f.weight();
+        f.put(sub(add(cur(T1_LO), mul(TWO32, cur(T1_CW))), t1s));
+        f.put(boolc(cur(T1_C0)));
+        f.put(boolc(cur(T1_C1)));
+        f.put(boolc(cur(T1_C2)));
+        f.put(sub(sub(sub(cur(T1_CW), cur(T1_C0)), mul(2, cur(T1_C1))), mul(4, cur(T1_C2))));
+        f.put(sub(add(cur(T2_LO), mul(TWO32, cur(T2_CW))), add(cur(S0_W), cur(MAJ_W))));
+        f.put(boolc(cur(T2_C0)));
+        f.put(boolc(cur(T2_C1)));
+        f.put(sub(sub(cur(T2_CW), cur(T2_C0)), mul(2, cur(T2_C1))));
+    } else {
+        f.skip(9);
+    }
+    // register shifts and new a / e (section 29)
+    if (comp && mine(29)) {
         gc.w = f.w;
         // register shifts b<-a, c<-b, d<-c, f<-e, g<-f, h<-g
         gc.put(sub(nxt(A_A + 1), cur(A_A)));
@@ -318,9 +368,12 @@ __global__ __launch_bounds__(AT) void compose_sha256_k(const uint64_t *lde, cons
     }
     // H-state: add-with-carry at finalisation, hold otherwise (GROUPS: both
     // forms at the same slot, to the finalisation and "not finalisation" sums)
+    // (section 30)
 #pragma unroll 1
     for (int k = 0; k < 8; k++) {
-        if (GROUPS) {
+        if (!mine(30)) {
+            f.skip(1);
+        } else if (GROUPS) {
             const uint64_t dl = sub(add(cur(H0 + k), cur(A_A + k)), nxt(H0 + k));
             ff.w = fn.w = f.w;
             ff.put(mul(dl, sub(dl, TWO32)));
@@ -333,11 +386,16 @@ __global__ __launch_bounds__(AT) void compose_sha256_k(const uint64_t *lde, cons
             f.put(sub(nxt(H0 + k), cur(H0 + k)));
         }
     }
-    // W window shift, then W_win[14]' = W
+    // W window shift, then W_win[14]' = W (section 31)
+    if (mine(31)) {
 #pragma unroll 2
-    for (int k = 0; k < 14; k++) f.put(sub(nxt(WW0 + k), cur(WW0 + k + 1)));
-    f.put(sub(nxt(WW0 + 14), cur(COL_W)));
-    if (sched) {
+        for (int k = 0; k < 14; k++) f.put(sub(nxt(WW0 + k), cur(WW0 + k + 1)));
+        f.put(sub(nxt(WW0 + 14), cur(COL_W)));
+    } else {
+        f.skip(15);
+    }
+    // message schedule (section 32, the last: no skip needed)
+    if (sched && mine(32)) {
         gs.w = f.w;
         const uint64_t sum = add(add(add(cur(WW0), cur(SS0_W)), cur(WW0 + 9)), cur(SS1_W));
         gs.put(sub(add(nxt(COL_W), mul(TWO32, cur(WS_CW))), sum));
@@ -345,6 +403,22 @@ __global__ __launch_bounds__(AT) void compose_sha256_k(const uint64_t *lde, cons
         gs.put(boolc(cur(WS_C1)));
         gs.put(sub(sub(cur(WS_CW), cur(WS_C0)), mul(2, cur(WS_C1))));
         f.w = gs.w;
+    }
+    // the SP partial sums of the point
+    red[0][part][lane] = f.acc;
+    red[1][part][lane] = fc.acc;
+    red[2][part][lane] = ff.acc;
+    red[3][part][lane] = fn.acc;
+    red[4][part][lane] = fs.acc;
+    __syncthreads();
+    if (part != 0) return;
+#pragma unroll
+    for (int q = 1; q < SP; q++) {
+        f.acc = add(f.acc, red[0][q][lane]);
+        fc.acc = add(fc.acc, red[1][q][lane]);
+        ff.acc = add(ff.acc, red[2][q][lane]);
+        fn.acc = add(fn.acc, red[3][q][lane]);
+        fs.acc = add(fs.acc, red[4][q][lane]);
     }
     if (GROUPS) {
         uint64_t *o = phi + (uint64_t)blockIdx.y * NG * n + i;
@@ -474,9 +548,9 @@ void merge_sha256_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trac
     SG_HIP(hipMemcpyAsync(d_nb.p, n_blocks.data(), 4 * K, hipMemcpyHostToDevice, c->stream));
     {
         ProfScope ps(c, "compose_sha256", (8.0 * SHA256_W * n + 8.0 * n) * K);
-        hipLaunchKernelGGL(sha::compose_sha256_k<false>, dim3((uint32_t)((n + sha::AT - 1) / sha::AT), K),
-                           dim3(sha::AT), 0, c->stream, lde, (const uint64_t *)nullptr, n, blowup,
-                           d_nb.as<uint32_t>(), phi.as<uint64_t>(), (uint64_t *)nullptr, d_lam);
+        hipLaunchKernelGGL(sha::compose_sha256_k<false>, dim3((uint32_t)(n / 64), K), dim3(sha::AT), 0,
+                           c->stream, lde, (const uint64_t *)nullptr, n, blowup, d_nb.as<uint32_t>(),
+                           phi.as<uint64_t>(), (uint64_t *)nullptr, d_lam);
         SG_HIP(hipGetLastError());
     }
     quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
@@ -504,7 +578,7 @@ void merge_sha256_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uin
             Scoped odd(c, 8ULL * SHA256_W * n_trace * K);
             lde_odd_coset(c, d_tr, odd.as<uint64_t>(), ilog2(n_trace), (uint64_t)SHA256_W * K);
             ProfScope ps(c, "compose_sha256", (8.0 * SHA256_W * n2 + 8.0 * sha::NG * n2) * K);
-            hipLaunchKernelGGL(sha::compose_sha256_k<true>, dim3((uint32_t)((n2 + sha::AT - 1) / sha::AT), K),
+            hipLaunchKernelGGL(sha::compose_sha256_k<true>, dim3((uint32_t)(n2 / 64), K),
                                dim3(sha::AT), 0, c->stream, d_tr, odd.as<uint64_t>(), n2, (uint64_t)2,
                                d_nb.as<uint32_t>(), groups.as<uint64_t>(), t1.as<uint64_t>(), d_lam);
             SG_HIP(hipGetLastError());

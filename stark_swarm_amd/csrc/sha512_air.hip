@@ -214,7 +214,13 @@ __device__ __forceinline__ uint64_t xorp(uint64_t x, uint64_t y) {
 }
 __device__ __forceinline__ uint64_t boolc(uint64_t b) { return mul(b, sub(1, b)); }
 
-constexpr int AT = 128;
+// SP waves per group of 64 points, one point per lane: wave `part`
+// evaluates the constraint sections s with s % SP == part and skips the
+// slots of the others (the slot weights stay in step); the partial sums meet
+// in LDS. One point's ~1 550 constraints split over SP waves keep many CUs
+// busy even for a small proof's few hundred points.
+constexpr int SP = 8;
+constexpr int AT = 64 * SP;
 
 // gate groups, as for SHA-256: ungated, compression rows, the finalisation
 // row, every other row, message-schedule rows
@@ -234,12 +240,12 @@ __global__ __launch_bounds__(AT) void compose_sha512_k(const uint64_t *lde, cons
                                                        uint64_t blowup, const uint32_t *n_blocks, uint64_t *phi,
                                                        uint64_t *t1_lam, uint64_t wcols, uint64_t row_limit,
                                                        const uint64_t *lam) {
-    const uint64_t i = (uint64_t)blockIdx.x * AT + threadIdx.x;
-    if (i >= n) return;
-    if (!GROUPS && i / blowup + 1 >= row_limit) {
-        phi[n * blockIdx.y + i] = 0;
-        return;
-    }
+    __shared__ uint64_t red[5][SP][64];
+    const uint32_t part = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const uint64_t i = (uint64_t)blockIdx.x * 64 + lane;  // n is a multiple of 64
+    // rows past row_limit get phi = 0: every section skipped
+    const bool dead = !GROUPS && i / blowup + 1 >= row_limit;
+    auto mine = [&](int sec) { return !dead && sec % SP == (int)part; };
     const uint64_t rows = GROUPS ? n / 2 : n, at = GROUPS ? i >> 1 : i;
     const uint64_t *L = (GROUPS && (i & 1) ? odd : lde) + wcols * rows * blockIdx.y;
     const uint64_t ni = GROUPS ? (at + 1) & (rows - 1) : (i + blowup) & (n - 1);
@@ -255,9 +261,14 @@ __global__ __launch_bounds__(AT) void compose_sha512_k(const uint64_t *lde, cons
     Phi &gc = GROUPS ? fc : f, &gs = GROUPS ? fs : f;
     const uint32_t boff[8] = {BIT_A, BIT_B, BIT_C, BIT_E, BIT_F, BIT_G, BIT_WW14, BIT_WW1};
 #pragma unroll 1
-    for (int o = 0; o < 8; o++)
+    for (int o = 0; o < 8; o++) {  // sections 0-7
+        if (!mine(o)) {
+            f.skip(64);
+            continue;
+        }
 #pragma unroll 4
         for (int b = 0; b < 64; b++) f.put(boolc(cur(boff[o] + b)));
+    }
     auto pack = [&](uint32_t off) {  // sum_{b<32} 2^b cell(off + b)
         uint64_t s = 0;
 #pragma unroll 8
@@ -266,9 +277,13 @@ __global__ __launch_bounds__(AT) void compose_sha512_k(const uint64_t *lde, cons
     };
     const uint32_t wlo[8] = {A0 + 0, A0 + 2, A0 + 4, A0 + 8, A0 + 10, A0 + 12, WW0 + 28, WW0 + 2};
 #pragma unroll 1
-    for (int o = 0; o < 8; o++) {
-        f.put(sub(cur(wlo[o]), pack(boff[o])));
-        f.put(sub(cur(wlo[o] + 1), pack(boff[o] + 32)));
+    for (int o = 0; o < 8; o++) {  // sections 8-15
+        if (mine(8 + o)) {
+            f.put(sub(cur(wlo[o]), pack(boff[o])));
+            f.put(sub(cur(wlo[o] + 1), pack(boff[o] + 32)));
+        } else {
+            f.skip(2);
+        }
     }
     struct Sg {
         uint32_t src, t, b, lo;
@@ -280,39 +295,64 @@ __global__ __launch_bounds__(AT) void compose_sha512_k(const uint64_t *lde, cons
                       {BIT_WW1, SS0_T, SS0_BIT, SS0_LO, 1, 8, 7, true},
                       {BIT_WW14, SS1_T, SS1_BIT, SS1_LO, 19, 61, 6, true}};
 #pragma unroll 1
-    for (int s = 0; s < 4; s++) {
+    for (int s = 0; s < 4; s++) {  // sections 16-23
         const Sg &g = sg[s];
+        if (mine(16 + 2 * s)) {
 #pragma unroll 2
-        for (int b = 0; b < 64; b++)
-            f.put(sub(cur(g.t + b), xorp(cur(g.src + (b + g.r1) % 64), cur(g.src + (b + g.r2) % 64))));
+            for (int b = 0; b < 64; b++)
+                f.put(sub(cur(g.t + b), xorp(cur(g.src + (b + g.r1) % 64), cur(g.src + (b + g.r2) % 64))));
+        } else {
+            f.skip(64);
+        }
+        if (mine(17 + 2 * s)) {
+#pragma unroll 2
+            for (int b = 0; b < 64; b++) {
+                const uint64_t third =
+                    g.shr ? (b + g.r3 < 64 ? cur(g.src + b + g.r3) : 0) : cur(g.src + (b + g.r3) % 64);
+                f.put(sub(cur(g.b + b), xorp(cur(g.t + b), third)));
+            }
+            f.put(sub(cur(g.lo), pack(g.b)));
+            f.put(sub(cur(g.lo + 1), pack(g.b + 32)));
+        } else {
+            f.skip(66);
+        }
+    }
+    if (mine(24)) {  // Ch (section 24)
 #pragma unroll 2
         for (int b = 0; b < 64; b++) {
-            const uint64_t third = g.shr ? (b + g.r3 < 64 ? cur(g.src + b + g.r3) : 0) : cur(g.src + (b + g.r3) % 64);
-            f.put(sub(cur(g.b + b), xorp(cur(g.t + b), third)));
+            const uint64_t e = cur(BIT_E + b), ff = cur(BIT_F + b), g = cur(BIT_G + b);
+            f.put(sub(cur(CH_BIT + b), sub(add(mul(e, ff), g), mul(e, g))));
         }
-        f.put(sub(cur(g.lo), pack(g.b)));
-        f.put(sub(cur(g.lo + 1), pack(g.b + 32)));
+        f.put(sub(cur(CH_LO), pack(CH_BIT)));
+        f.put(sub(cur(CH_LO + 1), pack(CH_BIT + 32)));
+    } else {
+        f.skip(66);
     }
+    if (mine(25)) {  // Maj auxiliaries (sections 25-27)
 #pragma unroll 2
-    for (int b = 0; b < 64; b++) {
-        const uint64_t e = cur(BIT_E + b), ff = cur(BIT_F + b), g = cur(BIT_G + b);
-        f.put(sub(cur(CH_BIT + b), sub(add(mul(e, ff), g), mul(e, g))));
+        for (int b = 0; b < 64; b++) {
+            const uint64_t a = cur(BIT_A + b), bb = cur(BIT_B + b), c = cur(BIT_C + b);
+            f.put(sub(cur(MAJ_AB + b), mul(a, bb)));
+            f.put(sub(cur(MAJ_AC + b), mul(a, c)));
+            f.put(sub(cur(MAJ_BC + b), mul(bb, c)));
+        }
+    } else {
+        f.skip(192);
     }
-    f.put(sub(cur(CH_LO), pack(CH_BIT)));
-    f.put(sub(cur(CH_LO + 1), pack(CH_BIT + 32)));
+    if (mine(26)) {
 #pragma unroll 2
-    for (int b = 0; b < 64; b++) {
-        const uint64_t a = cur(BIT_A + b), bb = cur(BIT_B + b), c = cur(BIT_C + b);
-        f.put(sub(cur(MAJ_AB + b), mul(a, bb)));
-        f.put(sub(cur(MAJ_AC + b), mul(a, c)));
-        f.put(sub(cur(MAJ_BC + b), mul(bb, c)));
+        for (int b = 0; b < 64; b++) f.put(sub(cur(MAJ_U + b), xorp(cur(MAJ_AB + b), cur(MAJ_AC + b))));
+    } else {
+        f.skip(64);
     }
+    if (mine(27)) {
 #pragma unroll 2
-    for (int b = 0; b < 64; b++) f.put(sub(cur(MAJ_U + b), xorp(cur(MAJ_AB + b), cur(MAJ_AC + b))));
-#pragma unroll 2
-    for (int b = 0; b < 64; b++) f.put(sub(cur(MAJ_BIT + b), xorp(cur(MAJ_U + b), cur(MAJ_BC + b))));
-    f.put(sub(cur(MAJ_LO), pack(MAJ_BIT)));
-    f.put(sub(cur(MAJ_LO + 1), pack(MAJ_BIT + 32)));
+        for (int b = 0; b < 64; b++) f.put(sub(cur(MAJ_BIT + b), xorp(cur(MAJ_U + b), cur(MAJ_BC + b))));
+        f.put(sub(cur(MAJ_LO), pack(MAJ_BIT)));
+        f.put(sub(cur(MAJ_LO + 1), pack(MAJ_BIT + 32)));
+    } else {
+        f.skip(66);
+    }
     // T1 per limb (carry-in from the lo limb), T2 per limb
     const uint64_t kt = GROUPS ? 0 : comp ? KD[br] : 0;
     auto carry3 = [&](uint32_t c0, uint32_t cw) {
@@ -321,22 +361,27 @@ __global__ __launch_bounds__(AT) void compose_sha512_k(const uint64_t *lde, cons
         f.put(boolc(cur(c0 + 2)));
         f.put(sub(sub(sub(cur(cw), cur(c0)), mul(2, cur(c0 + 1))), mul(4, cur(c0 + 2))));
     };
-    if (GROUPS && i == 0 && blockIdx.y == 0) t1_lam[0] = f.weight();
-    f.put(sub(add(cur(T1L_LO), mul(TWO32, cur(T1L_CW))),
-              add(add(add(add(cur(A0 + 14), cur(S1_LO)), cur(CH_LO)), kt & 0xFFFFFFFFu), cur(W_LO))));
-    carry3(T1L_C0, T1L_CW);
-    if (GROUPS && i == 0 && blockIdx.y == 0) t1_lam[1] = f.weight();
-    f.put(sub(add(cur(T1H_LO), mul(TWO32, cur(T1H_CW))),
-              add(add(add(add(add(cur(A0 + 15), cur(S1_LO + 1)), cur(CH_LO + 1)), kt >> 32), cur(W_LO + 1)),
-                  cur(T1L_CW))));
-    carry3(T1H_C0, T1H_CW);
-    f.put(sub(add(cur(T2L_LO), mul(TWO32, cur(T2L_CW))), add(cur(S0_LO), cur(MAJ_LO))));
-    f.put(boolc(cur(T2L_C0)));
-    f.put(sub(cur(T2L_CW), cur(T2L_C0)));
-    f.put(sub(add(cur(T2H_LO), mul(TWO32, cur(T2H_CW))), add(add(cur(S0_LO + 1), cur(MAJ_LO + 1)), cur(T2L_CW))));
-    f.put(boolc(cur(T2H_C0)));
-    f.put(sub(cur(T2H_CW), cur(T2H_C0)));
-    if (comp) {
+    if (mine(28)) {  // T1 / T2 limbs (section 28)
+        if (GROUPS && i == 0 && blockIdx.y == 0) t1_lam[0] = f.weight();
+        f.put(sub(add(cur(T1L_LO), mul(TWO32, cur(T1L_CW))),
+                  add(add(add(add(cur(A0 + 14), cur(S1_LO)), cur(CH_LO)), kt & 0xFFFFFFFFu), cur(W_LO))));
+        carry3(T1L_C0, T1L_CW);
+        if (GROUPS && i == 0 && blockIdx.y == 0) t1_lam[1] = f.weight();
+        f.put(sub(add(cur(T1H_LO), mul(TWO32, cur(T1H_CW))),
+                  add(add(add(add(add(cur(A0 + 15), cur(S1_LO + 1)), cur(CH_LO + 1)), kt >> 32), cur(W_LO + 1)),
+                      cur(T1L_CW))));
+        carry3(T1H_C0, T1H_CW);
+        f.put(sub(add(cur(T2L_LO), mul(TWO32, cur(T2L_CW))), add(cur(S0_LO), cur(MAJ_LO))));
+        f.put(boolc(cur(T2L_C0)));
+        f.put(sub(cur(T2L_CW), cur(T2L_C0)));
+        f.put(sub(add(cur(T2H_LO), mul(TWO32, cur(T2H_CW))),
+                  add(add(cur(S0_LO + 1), cur(MAJ_LO + 1)), cur(T2L_CW))));
+        f.put(boolc(cur(T2H_C0)));
+        f.put(sub(cur(T2H_CW), cur(T2H_C0)));
+    } else {
+        f.skip(16);
+    }
+    if (comp && mine(29)) {  // register shifts and new a / e (section 29)
         gc.w = f.w;
         // shifts b<-a, c<-b, d<-c, f<-e, g<-f, h<-g, both limbs
         const uint32_t dst[6] = {1, 2, 3, 5, 6, 7};
@@ -361,7 +406,11 @@ __global__ __launch_bounds__(AT) void compose_sha512_k(const uint64_t *lde, cons
     constexpr uint64_t INV32 = 0xFFFFFFFE00000002ull;  // 2^-32 mod p
     static_assert((unsigned __int128)INV32 * (1ull << 32) % P == 1, "2^-32");
 #pragma unroll 1
-    for (int k = 0; k < 8; k++) {
+    for (int k = 0; k < 8; k++) {  // section 30
+        if (!mine(30)) {
+            f.skip(2);
+            continue;
+        }
         if (GROUPS || fin) {
             const uint64_t dl = sub(add(cur(H0 + 2 * k), cur(A0 + 2 * k)), nxt(H0 + 2 * k));
             const uint64_t dh = sub(add(add(cur(H0 + 2 * k + 1), cur(A0 + 2 * k + 1)), mul(dl, INV32)),
@@ -379,14 +428,18 @@ __global__ __launch_bounds__(AT) void compose_sha512_k(const uint64_t *lde, cons
         }
         if (GROUPS) f.skip(2);
     }
+    if (mine(31)) {  // W window shift (section 31)
 #pragma unroll 2
-    for (int k = 0; k < 14; k++) {
-        f.put(sub(nxt(WW0 + 2 * k), cur(WW0 + 2 * k + 2)));
-        f.put(sub(nxt(WW0 + 2 * k + 1), cur(WW0 + 2 * k + 3)));
+        for (int k = 0; k < 14; k++) {
+            f.put(sub(nxt(WW0 + 2 * k), cur(WW0 + 2 * k + 2)));
+            f.put(sub(nxt(WW0 + 2 * k + 1), cur(WW0 + 2 * k + 3)));
+        }
+        f.put(sub(nxt(WW0 + 28), cur(W_LO)));
+        f.put(sub(nxt(WW0 + 29), cur(W_LO + 1)));
+    } else {
+        f.skip(30);
     }
-    f.put(sub(nxt(WW0 + 28), cur(W_LO)));
-    f.put(sub(nxt(WW0 + 29), cur(W_LO + 1)));
-    if (sched) {
+    if (sched && mine(32)) {  // message schedule (section 32, the last)
         gs.w = f.w;
         const uint64_t sl = add(add(add(cur(WW0), cur(SS0_LO)), cur(WW0 + 18)), cur(SS1_LO));
         gs.put(sub(add(nxt(W_LO), mul(TWO32, cur(WSL_CW))), sl));
@@ -399,6 +452,22 @@ __global__ __launch_bounds__(AT) void compose_sha512_k(const uint64_t *lde, cons
         gs.put(boolc(cur(WSH_C0 + 1)));
         gs.put(sub(sub(cur(WSH_CW), cur(WSH_C0)), mul(2, cur(WSH_C0 + 1))));
         f.w = gs.w;
+    }
+    // the SP partial sums of the point
+    red[0][part][lane] = f.acc;
+    red[1][part][lane] = fc.acc;
+    red[2][part][lane] = ff.acc;
+    red[3][part][lane] = fn.acc;
+    red[4][part][lane] = fs.acc;
+    __syncthreads();
+    if (part != 0) return;
+#pragma unroll
+    for (int q = 1; q < SP; q++) {
+        f.acc = add(f.acc, red[0][q][lane]);
+        fc.acc = add(fc.acc, red[1][q][lane]);
+        ff.acc = add(ff.acc, red[2][q][lane]);
+        fn.acc = add(fn.acc, red[3][q][lane]);
+        fs.acc = add(fs.acc, red[4][q][lane]);
     }
     if (GROUPS) {
         uint64_t *o = phi + (uint64_t)blockIdx.y * NG * n + i;
@@ -532,7 +601,7 @@ void merge_sha512_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint64_t n_trac
     SG_HIP(hipMemcpyAsync(d_nb.p, n_blocks.data(), 4 * K, hipMemcpyHostToDevice, c->stream));
     {
         ProfScope ps(c, "compose_sha512", (8.0 * SHA512_W * n + 8.0 * n) * K);
-        hipLaunchKernelGGL(sha5::compose_sha512_k<false>, dim3((uint32_t)((n + sha5::AT - 1) / sha5::AT), K),
+        hipLaunchKernelGGL(sha5::compose_sha512_k<false>, dim3((uint32_t)(n / 64), K),
                            dim3(sha5::AT), 0, c->stream, lde, (const uint64_t *)nullptr, n, blowup,
                            d_nb.as<uint32_t>(), phi.as<uint64_t>(), (uint64_t *)nullptr, (uint64_t)SHA512_W,
                            ~0ULL, d_lam);
@@ -562,7 +631,7 @@ void merge_sha512_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uin
             Scoped odd(c, 8ULL * SHA512_W * n_trace * K);
             lde_odd_coset(c, d_tr, odd.as<uint64_t>(), ilog2(n_trace), (uint64_t)SHA512_W * K);
             ProfScope ps(c, "compose_sha512", (8.0 * SHA512_W * n2 + 8.0 * sha5::NG * n2) * K);
-            hipLaunchKernelGGL(sha5::compose_sha512_k<true>, dim3((uint32_t)((n2 + sha5::AT - 1) / sha5::AT), K),
+            hipLaunchKernelGGL(sha5::compose_sha512_k<true>, dim3((uint32_t)(n2 / 64), K),
                                dim3(sha5::AT), 0, c->stream, d_tr, odd.as<uint64_t>(), n2, (uint64_t)2,
                                d_nb.as<uint32_t>(), groups.as<uint64_t>(), t1.as<uint64_t>(), (uint64_t)SHA512_W,
                                ~0ULL, d_lam);
@@ -581,8 +650,9 @@ void merge_sha512_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uin
 
 void compose_sha512_rows(Ctx *c, const uint64_t *lde, uint64_t wcols, uint32_t K, uint64_t n, uint64_t blowup,
                          const uint32_t *d_nb, uint64_t row_limit, const uint64_t *d_lam, uint64_t *phi) {
+    SG_REQUIRE(n % 64 == 0, "compose_sha512: point count not a multiple of 64");
     ProfScope ps(c, "compose_sha512", (8.0 * SHA512_W * n + 8.0 * n) * K);
-    hipLaunchKernelGGL(sha5::compose_sha512_k<false>, dim3((uint32_t)((n + sha5::AT - 1) / sha5::AT), K),
+    hipLaunchKernelGGL(sha5::compose_sha512_k<false>, dim3((uint32_t)(n / 64), K),
                        dim3(sha5::AT), 0, c->stream, lde, (const uint64_t *)nullptr, n, blowup, d_nb, phi,
                        (uint64_t *)nullptr, wcols, row_limit, d_lam);
     SG_HIP(hipGetLastError());
@@ -591,8 +661,9 @@ void compose_sha512_rows(Ctx *c, const uint64_t *lde, uint64_t wcols, uint32_t K
 void compose_sha512_groups(Ctx *c, const uint64_t *d_tr, const uint64_t *d_odd, uint64_t wcols, uint32_t K,
                            uint64_t n2, const uint32_t *d_nb, const uint64_t *d_lam, uint64_t *groups,
                            uint64_t *t1_lam) {
+    SG_REQUIRE(n2 % 64 == 0, "compose_sha512: point count not a multiple of 64");
     ProfScope ps(c, "compose_sha512", (8.0 * SHA512_W * n2 + 8.0 * sha5::NG * n2) * K);
-    hipLaunchKernelGGL(sha5::compose_sha512_k<true>, dim3((uint32_t)((n2 + sha5::AT - 1) / sha5::AT), K),
+    hipLaunchKernelGGL(sha5::compose_sha512_k<true>, dim3((uint32_t)(n2 / 64), K),
                        dim3(sha5::AT), 0, c->stream, d_tr, d_odd, n2, (uint64_t)2, d_nb, groups, t1_lam, wcols,
                        ~0ULL, d_lam);
     SG_HIP(hipGetLastError());
