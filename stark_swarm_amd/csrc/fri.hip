@@ -51,16 +51,19 @@ __global__ __launch_bounds__(FT) void ood_sums_k(const uint64_t *coef, uint64_t 
     constexpr int NW = FT / 64;
     __shared__ uint64_t zt[64 * 6];  // z^(t*FT), t < ch <= 64
     __shared__ uint64_t red[NW][M * 6];
+    __shared__ uint64_t tl[24 * 6];  // z^(2^i), i < 24: read once per block, not along each product chain
     const uint64_t pk = blockIdx.y;
     coef += pk * ncomp * n;
     T += pk * 6 * tstride;
+    const uint32_t tid = threadIdx.x;
+    if (tid < 24 * 6) tl[tid] = T[tid];
+    __syncthreads();
     auto tget = [&](int i) {
         fp6 r;
 #pragma unroll
-        for (int q = 0; q < 6; q++) r.c[q] = T[6 * i + q];
+        for (int q = 0; q < 6; q++) r.c[q] = tl[6 * i + q];
         return r;
     };
-    const uint32_t tid = threadIdx.x;
     if (tid < ch) {
         fp6 w = f6_one();
         for (int bit = 0; (1u << bit) <= tid; bit++)
@@ -235,26 +238,33 @@ __device__ __forceinline__ void f6_add_mul_upow(fp6 &acc, const fp6 &x) {
 // so each component costs a scalar times Fp6 per k (apow: alpha^k, k < M, per
 // proof) and only one component's M values are live at a time.
 // Optional coefficient-tuple planes: tuples[(k*6 + q) * nn + b] = y_{k,q}.
+// A block is 6 waves over 64 outputs b, wave q taking component q (one
+// output per lane, no divergence), so a small layer's per-output chain is a
+// sixth as long; the six Fp6 partial sums meet in LDS.
+constexpr int FOLD_T = 6 * 64;
 template <int LM>
-__global__ __launch_bounds__(FT) void coset_fold_k(const uint64_t *f, uint64_t n, int ncomp, const uint64_t *apow,
-                                                   uint64_t minv, const uint64_t *rt_lo, const uint64_t *rt_hi,
-                                                   uint32_t logn, uint64_t *fnext, uint64_t *tuples) {
+__global__ __launch_bounds__(FOLD_T) void coset_fold_k(const uint64_t *f, uint64_t n, int ncomp, const uint64_t *apow,
+                                                       uint64_t minv, const uint64_t *rt_lo, const uint64_t *rt_hi,
+                                                       uint32_t logn, uint64_t *fnext, uint64_t *tuples) {
     constexpr int M = 1 << LM;
+    __shared__ uint64_t red[6][6][64];
     const uint64_t nn = n >> LM;
-    const uint64_t b = (uint64_t)blockIdx.x * FT + threadIdx.x;
-    if (b >= nn) return;
+    const uint32_t wq = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const uint64_t b = (uint64_t)blockIdx.x * 64 + lane;
+    const bool live = b < nn;  // no early exit: every wave meets the barrier
     const uint64_t pk = blockIdx.y;
     f += pk * ncomp * n;
     fnext += pk * 6 * nn;
     apow += pk * 6 * M;
     if (tuples) tuples += pk * 6 * M * nn;
     // w^(-b) from the 2^26 root table
-    uint64_t e26 = (b << (26 - logn)) & ((1ULL << 26) - 1);
+    uint64_t e26 = ((live ? b : 0) << (26 - logn)) & ((1ULL << 26) - 1);
     e26 = (0 - e26) & ((1ULL << 26) - 1);
     const uint64_t wb = mul(rt_lo[e26 & 8191], rt_hi[e26 >> 13]);
     fp6 acc = f6_zero();
     auto component = [&](auto qc) {
         constexpr int q = decltype(qc)::value;
+        if (!live) return;
         if (q >= ncomp) {
             if (tuples)
 #pragma unroll
@@ -280,12 +290,22 @@ __global__ __launch_bounds__(FT) void coset_fold_k(const uint64_t *f, uint64_t n
         constexpr int upow[6] = {0, 2, 4, 1, 3, 5};
         f6_add_mul_upow<upow[q]>(acc, pq);
     };
-    component(std::integral_constant<int, 0>{});
-    component(std::integral_constant<int, 1>{});
-    component(std::integral_constant<int, 2>{});
-    component(std::integral_constant<int, 3>{});
-    component(std::integral_constant<int, 4>{});
-    component(std::integral_constant<int, 5>{});
+    switch (wq) {  // uniform per wave
+    case 0: component(std::integral_constant<int, 0>{}); break;
+    case 1: component(std::integral_constant<int, 1>{}); break;
+    case 2: component(std::integral_constant<int, 2>{}); break;
+    case 3: component(std::integral_constant<int, 3>{}); break;
+    case 4: component(std::integral_constant<int, 4>{}); break;
+    default: component(std::integral_constant<int, 5>{}); break;
+    }
+#pragma unroll
+    for (int q = 0; q < 6; q++) red[wq][q][lane] = acc.c[q];
+    __syncthreads();
+    if (wq != 0 || !live) return;
+#pragma unroll
+    for (int w = 1; w < 6; w++)
+#pragma unroll
+        for (int q = 0; q < 6; q++) acc.c[q] = add(acc.c[q], red[w][q][lane]);
 #pragma unroll
     for (int q = 0; q < 6; q++) fnext[q * nn + b] = acc.c[q];
 }
@@ -296,7 +316,7 @@ void launch_fold(Ctx *c, const uint64_t *f, uint64_t n, int ncomp, uint32_t K, c
     const uint64_t nn = n >> LM;
     const uint64_t minv = inv(1ULL << LM);
     ProfScope ps(c, "coset_fold", (8.0 * ncomp * n + 48.0 * nn + (tuples ? 48.0 * n : 0)) * K);
-    hipLaunchKernelGGL(coset_fold_k<LM>, dim3((uint32_t)((nn + FT - 1) / FT), K), dim3(FT), 0, c->stream, f, n, ncomp,
+    hipLaunchKernelGGL(coset_fold_k<LM>, dim3((uint32_t)((nn + 63) / 64), K), dim3(FOLD_T), 0, c->stream, f, n, ncomp,
                        d_apow, minv, c->rt_lo, c->rt_hi, ilog2(n), fnext, tuples);
     SG_HIP(hipGetLastError());
 }

@@ -148,6 +148,52 @@ __device__ __forceinline__ void keccak_f_dev(uint64_t *a) {
     for (int i = 0; i < 25; i++) a[i] = ((uint64_t)hi[i] << 32) | lo[i];
 }
 
+// ---- Keccak-f on two lanes: lane parity h holds the low (h = 0) or high
+// (h = 1) 32-bit half of every state word, so a permutation's instruction
+// chain is about two thirds as long as on one lane (small tree levels, whose
+// latency is the permutation chain). A 64-bit rotation needs the partner's
+// half, fetched by a DPP quad permutation [1, 0, 3, 2] (lane i reads lane
+// i ^ 1); with m the lane's half and o the partner's, rotl(x, n) is
+// alignbit(m, o, 32 - n) for n < 32 and alignbit(o, m, 64 - n) for n > 32 on
+// both lanes.
+__device__ __forceinline__ uint32_t kx_partner(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, true);
+}
+__device__ __forceinline__ uint32_t kx_rot(uint32_t m, int n) {
+    if (n == 0) return m;
+    const uint32_t o = kx_partner(m);
+    if (n == 32) return o;
+    return n < 32 ? __builtin_amdgcn_alignbit(m, o, 32 - n) : __builtin_amdgcn_alignbit(o, m, 64 - n);
+}
+__device__ __forceinline__ void keccak_round_half(uint32_t *a, uint32_t rc) {
+    uint32_t c[5];
+#pragma unroll
+    for (int x = 0; x < 5; x++)
+        c[x] = __builtin_amdgcn_bitop3_b32(__builtin_amdgcn_bitop3_b32(a[x], a[x + 5], a[x + 10], 0x96), a[x + 15],
+                                           a[x + 20], 0x96);
+    uint32_t r[5];
+#pragma unroll
+    for (int x = 0; x < 5; x++) r[x] = kx_rot(c[(x + 1) % 5], 1);
+    uint32_t b[25];
+#pragma unroll
+    for (int x = 0; x < 5; x++)
+#pragma unroll
+        for (int y = 0; y < 5; y++) {
+            const int src = x + 5 * y, dst = y + 5 * ((2 * x + 3 * y) % 5);
+            b[dst] = kx_rot(__builtin_amdgcn_bitop3_b32(a[src], c[(x + 4) % 5], r[x], 0x96), KROT_D[src]);
+        }
+#pragma unroll
+    for (int y = 0; y < 25; y += 5)
+#pragma unroll
+        for (int x = 0; x < 5; x++) a[y + x] = b[y + x] ^ (~b[y + (x + 1) % 5] & b[y + (x + 2) % 5]);
+    a[0] ^= rc;
+}
+// every lane of the wave must run this (the DPP reads the partner lane)
+__device__ __forceinline__ void keccak_f_half(uint32_t *a, uint32_t h) {
+#pragma unroll 2
+    for (int r = 0; r < 24; r++) keccak_round_half(a, (uint32_t)(KRC_D[r] >> (32 * h)));
+}
+
 void keccak_f1600_host(uint64_t s[25]) {
     for (int r = 0; r < 24; r++) keccak_round(s, KRC_H[r]);
 }
@@ -249,6 +295,8 @@ uint64_t stdrng_first_u64(uint64_t seed) {
 namespace {
 
 constexpr int KT = 256;
+// node levels of at most this many nodes (all trees of a batch) use two lanes per node
+constexpr uint64_t NODES2_MAX = 1u << 17;
 
 // absorb `total` message lanes, SHA3-256 padding, digest out: lanes t < 4
 // (the domain-separation header) come from head(t), the rest from lane(t),
@@ -346,6 +394,55 @@ __global__ __launch_bounds__(KT) void merkle_nodes_k(const uint64_t *child, uint
     o[1] = d[1];
     o[2] = d[2];
     o[3] = d[3];
+}
+
+// merkle_nodes_k on two lanes per node (keccak_f_half): lanes 2i and 2i + 1
+// of the grid hash node i. A pair past n_parent still runs the permutation
+// (its partner reads come from it) but loads and stores nothing.
+__global__ __launch_bounds__(KT) void merkle_nodes2_k(const uint64_t *child, uint64_t n_child, uint64_t n_parent,
+                                                      uint32_t arity, uint32_t level, uint64_t label, uint64_t *out) {
+    const uint64_t i = ((uint64_t)blockIdx.x * KT + threadIdx.x) >> 1;
+    const uint32_t h = threadIdx.x & 1;
+    const bool live = i < n_parent;
+    const uint64_t k = blockIdx.y;
+    const uint32_t *ch32 = (const uint32_t *)(child + 4 * k * n_child);
+    // 64-bit message lane t, this lane's half
+    auto word = [&](uint32_t t) -> uint32_t {
+        if (t < 4) {
+            const uint64_t v = t == 0 ? (uint64_t)arity : t == 1 ? (uint64_t)level : t == 2 ? i : label;
+            return (uint32_t)(v >> (32 * h));
+        }
+        if (!live) return 0;
+        const uint32_t j = (t - 4) >> 2, q = (t - 4) & 3;
+        uint64_t ci = i * arity + j;
+        if (ci >= n_child) ci = n_child - 1;
+        return ch32[2 * (4 * ci + q) + h];
+    };
+    uint32_t st[25];
+#pragma unroll
+    for (int j = 0; j < 25; j++) st[j] = 0;
+    const uint32_t total = 4 + 4 * arity;
+    uint32_t t0 = 0;
+    while (total - t0 >= 17) {
+#pragma unroll
+        for (int j = 0; j < 17; j++) st[j] ^= word(t0 + j);
+        keccak_f_half(st, h);
+        t0 += 17;
+    }
+    const uint32_t rem = total - t0;
+#pragma unroll
+    for (int j = 0; j < 17; j++) {
+        if ((uint32_t)j < rem) st[j] ^= word(t0 + j);
+        if ((uint32_t)j == rem) st[j] ^= h ? 0u : 0x06u;
+    }
+    st[16] ^= h ? 0x80000000u : 0u;
+    keccak_f_half(st, h);
+    if (!live) return;
+    uint32_t *o = (uint32_t *)(out + 4 * (k * n_parent + i));
+    o[h] = st[0];
+    o[2 + h] = st[1];
+    o[4 + h] = st[2];
+    o[6 + h] = st[3];
 }
 
 __global__ void gather_k(const uint64_t *src, const uint64_t *idx, uint64_t n, uint32_t width, uint64_t *out) {
@@ -647,9 +744,16 @@ void tree_build(Ctx *c, DevTree &t, const LeafSrc &src, uint64_t n_leaves, uint3
     for (size_t l = 1; l < t.real.size(); l++) {
         const uint64_t np = t.real[l];
         ProfScope ps(c, "merkle_nodes", (32.0 * t.real[l - 1] + 32.0 * np) * batch);
-        hipLaunchKernelGGL(merkle_nodes_k, dim3((uint32_t)((np + KT - 1) / KT), batch), dim3(KT), 0, c->stream,
-                           t.buf + 4 * t.off[l - 1], t.real[l - 1], np, arity, (uint32_t)l, label,
-                           t.buf + 4 * t.off[l]);
+        // levels too small to fill the chip one lane per node take two lanes
+        // per node: a shorter permutation chain for the same total work
+        if (np * batch <= NODES2_MAX)
+            hipLaunchKernelGGL(merkle_nodes2_k, dim3((uint32_t)((2 * np + KT - 1) / KT), batch), dim3(KT), 0,
+                               c->stream, t.buf + 4 * t.off[l - 1], t.real[l - 1], np, arity, (uint32_t)l, label,
+                               t.buf + 4 * t.off[l]);
+        else
+            hipLaunchKernelGGL(merkle_nodes_k, dim3((uint32_t)((np + KT - 1) / KT), batch), dim3(KT), 0, c->stream,
+                               t.buf + 4 * t.off[l - 1], t.real[l - 1], np, arity, (uint32_t)l, label,
+                               t.buf + 4 * t.off[l]);
         SG_HIP(hipGetLastError());
     }
     if (read_roots) tree_read_roots(c, t);
