@@ -312,11 +312,18 @@ __global__ __launch_bounds__(NT, 4) void lde_small_tile(const uint64_t *coef, ui
             st = root_pow(rt_lo, rt_hi, log_n0, (uint64_t)r * DI, false);
         }
         const uint64_t *src = coef + col * in_cs;
-#pragma unroll 2
+        // all of the lane's loads are in flight before the first is used
+        uint64_t xv[EPT];
+#pragma unroll
+        for (uint32_t u = 0; u < EPT; u++) {
+            const uint32_t i = il + u * DI;
+            xv[u] = (i < S && ok) ? src[i] : 0;
+        }
+#pragma unroll
         for (uint32_t u = 0; u < EPT; u++) {
             const uint32_t i = il + u * DI;
             if (i < S) {
-                uint64_t x = ok ? src[i] : 0;
+                uint64_t x = xv[u];
                 if (r) x = mul(x, w);
                 lds[padi<LOG_S>((gl << LOG_S) + i)] = x;
             }
@@ -349,18 +356,30 @@ __global__ __launch_bounds__(NT, 4) void odd_coset_tile(const uint64_t *in, uint
     uint64_t blk = blockIdx.x;
     if ((gridDim.x & 7) == 0) blk = (blk & 7) * (gridDim.x >> 3) + (blk >> 3);
     const uint64_t col0 = blk << LOG_G;
-#pragma unroll 2
-    for (uint32_t u = 0; u < EPT; u++) {
-        const uint32_t e = threadIdx.x + u * NT;
-        const uint64_t col = col0 + (e >> LOG_S);
-        lds[padi<LOG_S>(e)] = col < ncols ? in[col * in_cs + (e & (S - 1))] : 0;
+    {
+        // all of the lane's loads (and twist factors) are in flight before
+        // the first is used
+        uint64_t xv[EPT];
+#pragma unroll
+        for (uint32_t u = 0; u < EPT; u++) {
+            const uint32_t e = threadIdx.x + u * NT;
+            const uint64_t col = col0 + (e >> LOG_S);
+            xv[u] = col < ncols ? in[col * in_cs + (e & (S - 1))] : 0;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < EPT; u++) lds[padi<LOG_S>(threadIdx.x + u * NT)] = xv[u];
     }
     __syncthreads();
     stockham_steps<LOG_S, 0, true>(lds, TILE, tw_i);
-#pragma unroll 2
-    for (uint32_t u = 0; u < EPT; u++) {
-        const uint32_t e = threadIdx.x + u * NT;
-        lds[padi<LOG_S>(e)] = mul(lds[padi<LOG_S>(e)], twist[e & (S - 1)]);
+    {
+        uint64_t tv[EPT];
+#pragma unroll
+        for (uint32_t u = 0; u < EPT; u++) tv[u] = twist[(threadIdx.x + u * NT) & (S - 1)];
+#pragma unroll
+        for (uint32_t u = 0; u < EPT; u++) {
+            const uint32_t e = threadIdx.x + u * NT;
+            lds[padi<LOG_S>(e)] = mul(lds[padi<LOG_S>(e)], tv[u]);
+        }
     }
     __syncthreads();
     stockham_steps<LOG_S, 0, false>(lds, TILE, tw_f);
