@@ -237,12 +237,19 @@ __device__ __forceinline__ void wmad(Acc &a, uint64_t x, uint32_t slot, const ui
 // n0 / 2 rows) and its odd points (odd, the same layout).
 // GEN: slot j weighted by T[j] (the caller's combination coefficients)
 // instead of j + 1, the position identities summed term by term.
+// grid.z workgroups share a group of 64 points: wave s of workgroup z takes
+// the limbs and carry positions = z * SL + s (mod SL * gridDim.z), so a
+// small proof's few points still spread over many CUs; each workgroup writes
+// its partial sums to its own plane of phi (plane stride gridDim.y * n0, or
+// gridDim.y * NG * n0 with GROUPS) for sum_planes_k to add.
 template <bool GROUPS, bool GEN>
 __global__ __launch_bounds__(SL * PT) void compose_rsa_k(const uint64_t *lde, const uint64_t *odd, uint64_t n0,
                                                         uint64_t blowup, uint32_t R, uint64_t *phi,
                                                         const uint64_t *T) {
     __shared__ uint64_t red[SL][4][PT];
     const uint32_t lane = threadIdx.x & (PT - 1), s = threadIdx.x / PT;
+    const uint32_t sg = blockIdx.z * SL + s, ST = SL * gridDim.z;  // this wave's slice of ST
+    phi += (uint64_t)blockIdx.z * gridDim.y * (GROUPS ? NG : 1) * n0;
     const uint64_t pt = (uint64_t)blockIdx.x * PT + lane;
     // GROUPS: point pt is row pt / 2 of the traces (even pt) or of the odd coset
     const uint64_t rows = GROUPS ? n0 / 2 : n0, at = GROUPS ? pt >> 1 : pt;
@@ -289,8 +296,7 @@ __global__ __launch_bounds__(SL * PT) void compose_rsa_k(const uint64_t *lde, co
         Acc &ep = GROUPS ? epos : pos, &en = GROUPS ? eneg : neg, &tp = GROUPS ? tpos : pos, &tn = GROUPS ? tneg : neg;
         const uint32_t L0 = O + K_POS;
 #pragma unroll 1
-        for (int u = 0; u < NL / SL; u++) {
-            const uint32_t i = s + SL * u;
+        for (uint32_t i = sg; i < NL; i += ST) {
             const uint64_t a = AT(C_ACC + i), op = AT(C_OP2 + i), sv = AT(C_S + i);
             const uint64_t cc = AT(C_C + i), qq = AT(C_Q + i);
             // select: op - (ph (s - a) + a); the ph part is applied once below
@@ -344,8 +350,7 @@ __global__ __launch_bounds__(SL * PT) void compose_rsa_k(const uint64_t *lde, co
         // loops' registers free)
         if (trans) {
 #pragma unroll 1
-            for (int u = 0; u < NL / SL; u++) {
-                const uint32_t i = s + SL * u;
+            for (uint32_t i = sg; i < NL; i += ST) {
                 wmad<GEN>(tp, NX(C_ACC + i), O + K_TRANS + i, T);
                 wmad<GEN>(tn, AT(C_C + i), O + K_TRANS + i, T);
                 wmad<GEN>(tp, NX(C_S + i), O + K_TRANS + NL + i, T);
@@ -359,7 +364,7 @@ __global__ __launch_bounds__(SL * PT) void compose_rsa_k(const uint64_t *lde, co
         // carries: booleanity, and sc_m (lambda_{m+1} - 2^26 lambda_m) from the
         // position identities
 #pragma unroll 1
-        for (uint32_t m = s; m < CARRY_POS; m += SL) {
+        for (uint32_t m = sg; m < CARRY_POS; m += ST) {
             Acc sc{0, 0};
 #pragma unroll
             for (int b = 0; b < CARRY_BITS; b++) {
@@ -376,7 +381,7 @@ __global__ __launch_bounds__(SL * PT) void compose_rsa_k(const uint64_t *lde, co
             acc_add(pos, mul(sub(acc_val(sc), CARRY_OFF), gamma));
         }
         const uint64_t ph = AT(C_PHASE);
-        if (s == 0) {
+        if (sg == 0) {
             wmad<GEN>(pos, ph, O + K_PHASE, T);
             wmad<GEN>(neg, mul(ph, ph), O + K_PHASE, T);
         }
@@ -426,6 +431,19 @@ __global__ __launch_bounds__(256) void combine_rsa_k(const uint64_t *pg, uint64_
     phi[n * blockIdx.y + i] = v;
 }
 
+// dst[j] = sum of the nz planes src[z * len + j]
+__global__ __launch_bounds__(256) void sum_planes_k(const uint64_t *src, uint32_t nz, uint64_t len, uint64_t *dst) {
+    const uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= len) return;
+    uint64_t v = src[j];
+    for (uint32_t z = 1; z < nz; z++) v = add(v, src[(uint64_t)z * len + j]);
+    dst[j] = v;
+}
+
+// workgroups per group of 64 points (grid.z of compose_rsa_k): enough for a
+// small batch to reach many CUs, one when the batch fills the chip itself
+inline uint32_t slice_blocks(uint64_t groups) { return groups >= 512 ? 1 : groups >= 64 ? 2 : 10; }
+
 }  // namespace rsa
 
 void build_rsa_trace_host(const std::vector<RsaRecord> &records, uint64_t h, uint64_t *tr) {
@@ -470,15 +488,22 @@ void merge_rsa_batch(Ctx *c, const uint64_t *lde, uint32_t K, uint32_t R, uint64
     Scoped phi(c, 8 * n * K);
     {
         ProfScope ps(c, "compose_rsa", (8.0 * RSA_W * n * R + 8.0 * n) * K);
+        const uint32_t nz = rsa::slice_blocks(n / rsa::PT * K);
+        Scoped parts(c, nz > 1 ? 8 * n * K * nz : 8);
+        uint64_t *dst = nz > 1 ? parts.as<uint64_t>() : phi.as<uint64_t>();
+        const dim3 grid((uint32_t)(n / rsa::PT), K, nz);
         if (d_lam)
-            hipLaunchKernelGGL((rsa::compose_rsa_k<false, true>), dim3((uint32_t)(n / rsa::PT), K),
-                               dim3(rsa::SL * rsa::PT), 0, c->stream, lde, (const uint64_t *)nullptr, n, blowup, R,
-                               phi.as<uint64_t>(), d_lam);
+            hipLaunchKernelGGL((rsa::compose_rsa_k<false, true>), grid, dim3(rsa::SL * rsa::PT), 0, c->stream, lde,
+                               (const uint64_t *)nullptr, n, blowup, R, dst, d_lam);
         else
-            hipLaunchKernelGGL((rsa::compose_rsa_k<false, false>), dim3((uint32_t)(n / rsa::PT), K),
-                               dim3(rsa::SL * rsa::PT), 0, c->stream, lde, (const uint64_t *)nullptr, n, blowup, R,
-                               phi.as<uint64_t>(), (const uint64_t *)nullptr);
+            hipLaunchKernelGGL((rsa::compose_rsa_k<false, false>), grid, dim3(rsa::SL * rsa::PT), 0, c->stream, lde,
+                               (const uint64_t *)nullptr, n, blowup, R, dst, (const uint64_t *)nullptr);
         SG_HIP(hipGetLastError());
+        if (nz > 1) {
+            hipLaunchKernelGGL(rsa::sum_planes_k, dim3((uint32_t)((n * K + 255) / 256)), dim3(256), 0, c->stream,
+                               parts.as<uint64_t>(), nz, n * K, phi.as<uint64_t>());
+            SG_HIP(hipGetLastError());
+        }
     }
     quotient_batch(c, phi.as<uint64_t>(), K, n_trace, n, c_eval);
 }
@@ -499,10 +524,18 @@ void merge_rsa_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint32
             Scoped odd(c, 8ULL * RSA_W * R * n_trace * K);
             lde_odd_coset(c, d_tr, odd.as<uint64_t>(), ilog2(n_trace), (uint64_t)RSA_W * R * K);
             ProfScope ps(c, "compose_rsa", (8.0 * RSA_W * n2 * R + 8.0 * rsa::NG * n2) * K);
-            hipLaunchKernelGGL((rsa::compose_rsa_k<true, false>), dim3((uint32_t)(n2 / rsa::PT), K),
+            const uint32_t nz = rsa::slice_blocks(n2 / rsa::PT * K);
+            Scoped parts(c, nz > 1 ? 8ULL * rsa::NG * n2 * K * nz : 8);
+            hipLaunchKernelGGL((rsa::compose_rsa_k<true, false>), dim3((uint32_t)(n2 / rsa::PT), K, nz),
                                dim3(rsa::SL * rsa::PT), 0, c->stream, d_tr, odd.as<uint64_t>(), n2, (uint64_t)2, R,
-                               groups.as<uint64_t>(), (const uint64_t *)nullptr);
+                               nz > 1 ? parts.as<uint64_t>() : groups.as<uint64_t>(), (const uint64_t *)nullptr);
             SG_HIP(hipGetLastError());
+            if (nz > 1) {
+                const uint64_t len = (uint64_t)rsa::NG * n2 * K;
+                hipLaunchKernelGGL(rsa::sum_planes_k, dim3((uint32_t)((len + 255) / 256)), dim3(256), 0, c->stream,
+                                   parts.as<uint64_t>(), nz, len, groups.as<uint64_t>());
+                SG_HIP(hipGetLastError());
+            }
         }
         Scoped full(c, 8ULL * rsa::NG * n * K);
         lde_columns(c, groups.as<uint64_t>(), full.as<uint64_t>(), ilog2(n2), ilog2(blowup / 2),
