@@ -1847,7 +1847,7 @@ void build_ed25519_trace_host(const EdSig &s, uint32_t k, uint64_t *tr) {
     });
 }
 
-void build_ed25519_traces_dev(Ctx *c, const std::vector<EdSig> &sigs, uint32_t k, uint64_t *d_tr) {
+void build_ed25519_traces_dev(Ctx *c, const std::vector<EdSig> &sigs, uint32_t k, uint64_t *d_tr, EdConsts *consts) {
     const size_t G = sigs.size();
     SG_REQUIRE(G >= 1 && G <= 65535, "ed25519 traces: 1 to 65535 proofs");
     SG_REQUIRE(k >= 2 && k <= 256, "ed25519: k_scalar must be in [2, 256]");
@@ -1868,6 +1868,14 @@ void build_ed25519_traces_dev(Ctx *c, const std::vector<EdSig> &sigs, uint32_t k
     });
     for (size_t g = 0; g < G; g++)
         if (!err[g].empty()) throw Error(E_INVALID, "ed25519 traces: signature " + std::to_string(g) + ": " + err[g]);
+    if (consts) {
+        consts->cst.resize(G * L.nconst);
+        consts->nb.resize(G);
+        for (size_t g = 0; g < G; g++) {
+            std::copy(nat[g].cst.begin(), nat[g].cst.end(), consts->cst.begin() + g * L.nconst);
+            consts->nb[g] = (uint32_t)((sigs[g].len + 64 + 17 + 127) / 128);
+        }
+    }
     // rows in chunks of 64 across the batch
     const uint32_t chunks = (L.h + 63) / 64;
     std::vector<uint8_t> bad(G * chunks, 0);
@@ -1907,8 +1915,16 @@ void build_ed25519_traces_dev(Ctx *c, const std::vector<EdSig> &sigs, uint32_t k
 }
 
 // per-proof pins and SHA-512 block counts of a batch
-static void batch_consts(Ctx *c, const std::vector<EdSig> &sigs, const Layout &L, Scoped &d_cst, Scoped &d_nb) {
+static void batch_consts(Ctx *c, const std::vector<EdSig> &sigs, const Layout &L, Scoped &d_cst, Scoped &d_nb,
+                         const EdConsts *pre = nullptr) {
     const size_t G = sigs.size();
+    if (pre) {
+        SG_REQUIRE(pre->cst.size() == G * L.nconst && pre->nb.size() == G, "ed25519: pins of another batch");
+        SG_HIP(hipMemcpyAsync(d_cst.p, pre->cst.data(), 8 * pre->cst.size(), hipMemcpyHostToDevice, c->stream));
+        SG_HIP(hipMemcpyAsync(d_nb.p, pre->nb.data(), 4 * G, hipMemcpyHostToDevice, c->stream));
+        SG_HIP(hipStreamSynchronize(c->stream));
+        return;
+    }
     std::vector<uint64_t> cst(G * L.nconst);
     std::vector<uint32_t> nb(G);
     std::vector<std::string> err(G);
@@ -1977,7 +1993,8 @@ void merge_ed25519_batch(Ctx *c, const std::vector<EdSig> &sigs, uint32_t k, con
 }
 
 void merge_ed25519_from_trace_batch(Ctx *c, const std::vector<EdSig> &sigs, uint32_t k, const uint64_t *d_tr,
-                                    uint64_t blowup, const uint64_t *d_coeffs, uint64_t *c_eval) {
+                                    uint64_t blowup, const uint64_t *d_coeffs, uint64_t *c_eval,
+                                    const EdConsts *consts) {
     const size_t G = sigs.size();
     SG_REQUIRE(G >= 1 && G <= 65535, "merge_ed25519: 1 to 65535 proofs");
     SG_REQUIRE(k >= 2 && k <= 256, "ed25519: k_scalar must be in [2, 256]");
@@ -1989,7 +2006,7 @@ void merge_ed25519_from_trace_batch(Ctx *c, const std::vector<EdSig> &sigs, uint
     SG_REQUIRE(n <= (1ULL << 22), "merge_ed25519: domain above 2^22");
     const uint32_t nb = (uint32_t)L.blocks.size(), ng = (uint32_t)L.gates.size();
     Scoped d_cst(c, 8ULL * L.nconst * G), d_nb(c, 4 * G), d_blk(c, sizeof(Blk) * nb), t1(c, 16);
-    batch_consts(c, sigs, L, d_cst, d_nb);
+    batch_consts(c, sigs, L, d_cst, d_nb, consts);
     SG_HIP(hipMemcpyAsync(d_blk.p, L.blocks.data(), sizeof(Blk) * nb, hipMemcpyHostToDevice, c->stream));
     // gate groups: CSR of the partial-sum slots, and each group's rows
     std::vector<uint32_t> goff(ng + 1, 0), gidx, gates(2 * ng);

@@ -36,9 +36,17 @@ void ed25519_sign(const uint8_t seed[32], const uint8_t *msg, size_t len, uint8_
 // same words and expansion as the device path). Throws E_INVALID when R or A
 // does not decompress.
 void build_ed25519_trace_host(const EdSig &s, uint32_t k, uint64_t *tr);
+// Per-proof pins and SHA-512 block counts of a batch, from the native values
+// the trace builder computes (so the merge need not compute them again).
+struct EdConsts {
+    std::vector<uint64_t> cst;  // G x n_const pins
+    std::vector<uint32_t> nb;   // G SHA-512 block counts
+};
 // G traces of one shape (every message of the same SHA-512 block count class,
-// i.e. the same k_hash): d_tr is G x width x height.
-void build_ed25519_traces_dev(Ctx *c, const std::vector<EdSig> &sigs, uint32_t k, uint64_t *d_tr);
+// i.e. the same k_hash): d_tr is G x width x height. consts, if given,
+// receives the batch's pins for merge_ed25519_from_trace_batch.
+void build_ed25519_traces_dev(Ctx *c, const std::vector<EdSig> &sigs, uint32_t k, uint64_t *d_tr,
+                              EdConsts *consts = nullptr);
 
 // deep_ali_merge_ed25519_verify_streaming (lib.rs:854-974) on the callers'
 // LDEs (G x width x n, n = height * blowup), coefficients d_coeffs
@@ -51,7 +59,10 @@ void merge_ed25519_batch(Ctx *c, const std::vector<EdSig> &sigs, uint32_t k, con
 // sum is a polynomial of degree <= 2 height - 2, evaluated on a blowup-2 LDE
 // (the traces and their odd coset) with the gates ignored, interpolated to
 // all n points and selected there by each point's trace row. blowup >= 4.
+// consts: the trace builder's EdConsts of the same batch, or nullptr to
+// compute the pins here.
 void merge_ed25519_from_trace_batch(Ctx *c, const std::vector<EdSig> &sigs, uint32_t k, const uint64_t *d_tr,
-                                    uint64_t blowup, const uint64_t *d_coeffs, uint64_t *c_eval);
+                                    uint64_t blowup, const uint64_t *d_coeffs, uint64_t *c_eval,
+                                    const EdConsts *consts = nullptr);
 
 }  // namespace sg

@@ -677,12 +677,14 @@ std::vector<ZskKsk> prove_zsk_ksk_binding_v2_batch(Ctx *c, const std::vector<EdS
             Scoped d_c(c, 8ULL * n0 * G);
             {
                 Scoped d_tr(c, 8ULL * W * h * G);
+                EdConsts pins;
                 {
                     HostScope hs(c, "ed25519_trace");
-                    build_ed25519_traces_dev(c, part, k, d_tr.as<uint64_t>());
+                    build_ed25519_traces_dev(c, part, k, d_tr.as<uint64_t>(), &pins);
                 }
                 HostScope hs(c, "ed25519_merge");
-                merge_ed25519_from_trace_batch(c, part, k, d_tr.as<uint64_t>(), blowup, nullptr, d_c.as<uint64_t>());
+                merge_ed25519_from_trace_batch(c, part, k, d_tr.as<uint64_t>(), blowup, nullptr, d_c.as<uint64_t>(),
+                                               &pins);
                 if (c->profiling) SG_HIP(hipStreamSynchronize(c->stream));
             }
             HostScope hf(c, "ed25519_fri");
