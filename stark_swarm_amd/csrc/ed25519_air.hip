@@ -1488,7 +1488,7 @@ __global__ __launch_bounds__(64) void compose_ed_k(const uint64_t *lde, const ui
         sbase = sel == 1 ? B.s[1] : B.s[0];
     }
     uint32_t j = 0;
-    auto put = [&](uint64_t v) {
+    auto put = [&](uint64_t v) __attribute__((always_inline)) {
         if (GROUPS) {
             const uint32_t s0 = B.s[0] + j;
             a3_mad(acc[0], v, lam ? lam[s0] : (uint64_t)s0 + 1);
@@ -1502,15 +1502,40 @@ __global__ __launch_bounds__(64) void compose_ed_k(const uint64_t *lde, const ui
         }
         j++;
     };
-    auto elem = [&](uint32_t c, uint32_t cb) {
-#pragma unroll 5
-        for (int i = 0; i < EB; i++) put(boolc(CUR(cb + i)));
+    // The runs of cells below are read 16 at a time into registers before
+    // they are used, so a wave has 16 loads in flight instead of one (the
+    // slot bookkeeping of put between loads kept the compiler from batching
+    // them): put(boolc(cell)) for n consecutive columns from c0 ...
+    auto bools = [&](uint32_t c0, uint32_t n) __attribute__((always_inline)) {
+#pragma unroll 1
+        for (uint32_t i0 = 0; i0 < n; i0 += 16) {
+            uint64_t v[16];
+#pragma unroll
+            for (int r = 0; r < 16; r++) v[r] = i0 + r < n ? CUR(c0 + i0 + r) : 0;
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+                if (i0 + r < n) put(boolc(v[r]));
+        }
+    };
+    // ... and sum_b 2^b cell(c0 + b) for b < nb <= 64
+    auto packed = [&](uint32_t c0, uint32_t nb) __attribute__((always_inline)) -> uint64_t {
+        Acc sacc{0, 0};
+#pragma unroll 1
+        for (uint32_t b0 = 0; b0 < nb; b0 += 16) {
+            uint64_t v[16];
+#pragma unroll
+            for (int r = 0; r < 16; r++) v[r] = b0 + r < nb ? CUR(c0 + b0 + r) : 0;
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc_shl(sacc, v[r], (int)((b0 + r) & 63));  // a zero past nb adds nothing
+        }
+        return acc_val(sacc);
+    };
+    auto elem = [&](uint32_t c, uint32_t cb) __attribute__((always_inline)) {
+        bools(cb, EB);
         uint32_t off = 0;
 #pragma unroll 1
         for (int i = 0; i < NL; i++) {
-            Acc s{0, 0};
-            for (int b = 0; b < lw(i); b++) acc_shl(s, CUR(cb + off + b), b);
-            put(sub(CUR(c + i), acc_val(s)));
+            put(sub(CUR(c + i), packed(cb + off, (uint32_t)lw(i))));
             off += lw(i);
         }
     };
@@ -1524,12 +1549,7 @@ __global__ __launch_bounds__(64) void compose_ed_k(const uint64_t *lde, const ui
             b[i] = CUR(B.b + i);
         }
 #pragma unroll 1
-        for (int kk = 0; kk < NL; kk++) {
-            Acc s{0, 0};
-#pragma unroll 6
-            for (int bb = 0; bb < MCB; bb++) acc_shl(s, CUR(B.x + MCB * kk + bb), bb);
-            cy[kk] = sub(acc_val(s), 1ULL << (MCB - 1));
-        }
+        for (int kk = 0; kk < NL; kk++) cy[kk] = sub(packed(B.x + MCB * kk, MCB), 1ULL << (MCB - 1));
 #pragma unroll 1
         for (int kk = 0; kk < NL; kk++) {
             Acc t{0, 0};
@@ -1543,8 +1563,7 @@ __global__ __launch_bounds__(64) void compose_ed_k(const uint64_t *lde, const ui
             const uint64_t cin = kk == 0 ? mul_small(cy[NL - 1], 19) : cy[kk - 1];
             put(sub(sub(add(acc_val(t), cin), CUR(B.c + kk)), mul(1ULL << lw(kk), cy[kk])));
         }
-#pragma unroll 6
-        for (int i = 0; i < MCB * NL; i++) put(boolc(CUR(B.x + i)));
+        bools(B.x, MCB * NL);
         break;
     }
     case K_ADD: {  // eval_add_gadget (:431-484)
@@ -1604,8 +1623,7 @@ __global__ __launch_bounds__(64) void compose_ed_k(const uint64_t *lde, const ui
         }
         break;
     case K_BOOL:
-#pragma unroll 1
-        for (uint32_t i = 0; i < B.n; i++) put(boolc(CUR(B.a + i)));
+        bools(B.a, B.n);
         break;
     case K_TRANS4:  // eval_scalar_mult_transition (ed25519_scalar_mult_air.rs:208-237)
 #pragma unroll 1
@@ -1623,22 +1641,15 @@ __global__ __launch_bounds__(64) void compose_ed_k(const uint64_t *lde, const ui
         const uint32_t lc[3] = {B.a, B.c, B.x}, bc[3] = {B.b, B.cb, B.y}, nn[3] = {Q_L, R_L, R_L};
 #pragma unroll 1
         for (int g = 0; g < 3; g++) {
-#pragma unroll 4
-            for (uint32_t i = 0; i < nn[g] * LBITS; i++) put(boolc(CUR(bc[g] + i)));
+            bools(bc[g], nn[g] * LBITS);
 #pragma unroll 1
-            for (uint32_t i = 0; i < nn[g]; i++) {
-                Acc s{0, 0};
-                for (uint32_t b = 0; b < LBITS; b++) acc_shl(s, CUR(bc[g] + LBITS * i + b), (int)b);
-                put(sub(CUR(lc[g] + i), acc_val(s)));
-            }
+            for (uint32_t i = 0; i < nn[g]; i++) put(sub(CUR(lc[g] + i), packed(bc[g] + LBITS * i, LBITS)));
         }
         j += R_L;  // the r + slack = L - 1 slots are zero (:475-477)
         uint64_t prev = 0;
 #pragma unroll 1
         for (uint32_t kk = 0; kk < P_L; kk++) {
-            Acc cyv{0, 0};
-            for (uint32_t b = 0; b < CBITS; b++) acc_shl(cyv, CUR(B.z + CBITS * kk + b), (int)b);
-            const uint64_t cout = sub(acc_val(cyv), 1ULL << (CBITS - 1));
+            const uint64_t cout = sub(packed(B.z + CBITS * kk, CBITS), 1ULL << (CBITS - 1));
             Acc qs{0, 0};
             for (uint32_t i = 0; i < Q_L; i++)
                 if (kk >= i && kk - i < R_L) acc_mad(qs, CUR(B.a + i), L16[kk - i]);
@@ -1646,22 +1657,15 @@ __global__ __launch_bounds__(64) void compose_ed_k(const uint64_t *lde, const ui
             put(sub(add(sub(sub(in, acc_val(qs)), rk), prev), mul(1ULL << LBITS, cout)));
             prev = cout;
         }
-#pragma unroll 4
-        for (uint32_t i = 0; i < P_L * CBITS; i++) put(boolc(CUR(B.z + i)));
+        bools(B.z, P_L * CBITS);
         break;
     }
     case K_FMT: {  // the v1 digest-format block (ed25519_verify_air.rs:464-560)
-#pragma unroll 4
-        for (uint32_t i = 0; i < 512; i++) put(boolc(CUR(B.a + i)));
+        bools(B.a, 512);
 #pragma unroll 1
         for (uint32_t w = 0; w < 8; w++) {
-            Acc lo{0, 0}, hi{0, 0};
-            for (int i = 0; i < 32; i++) {
-                acc_shl(lo, CUR(B.a + 64 * w + i), i);
-                acc_shl(hi, CUR(B.a + 64 * w + 32 + i), i);
-            }
-            put(sub(CUR(B.b + 2 * w), acc_val(lo)));
-            put(sub(CUR(B.b + 2 * w + 1), acc_val(hi)));
+            put(sub(CUR(B.b + 2 * w), packed(B.a + 64 * w, 32)));
+            put(sub(CUR(B.b + 2 * w + 1), packed(B.a + 64 * w + 32, 32)));
         }
 #pragma unroll 1
         for (uint32_t w = 0; w < 8; w++)
