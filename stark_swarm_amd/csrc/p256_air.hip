@@ -566,24 +566,154 @@ bool row_step(const Step &S, Limbs *acc, const Limbs *base, int bit, const Limbs
     return true;
 }
 
-// The h rows of one chain (c = 0: A, 1: B) of one proof:
-// words[row * n_words + w] for the chain's words; out gets the chain's
-// accumulator after its last bit.
-bool chain_rows(const P256Chains &in, int c, uint64_t h, uint64_t *words, Limbs *out) {
+// ---- the point sequence of a chain on native F_p arithmetic, so that the
+// rows' witness words (the gadget quotients and carries, about 10 us a row)
+// can be filled in parallel once each row's accumulator is known. Montgomery
+// form on 4 x 64-bit words, R = 2^256; p = 2^256 - 2^224 + 2^192 + 2^96 - 1
+// is -1 mod 2^64, so -p^-1 mod 2^64 = 1. The programs are the gadgets' own
+// (DBL / ADD) and every result is the canonical residue, as in row_group.
+struct M4 {
+    uint64_t w[4];
+};
+constexpr uint64_t PM4[4] = {~0ULL, 0x00000000FFFFFFFFULL, 0, 0xFFFFFFFF00000001ULL};
+constexpr M4 R2M4 = {{0x3ULL, 0xfffffffbffffffffULL, 0xfffffffffffffffeULL, 0x4fffffffdULL}};  // R^2 mod p
+
+static bool m4_ge_p(const uint64_t *t) {
+    for (int i = 3; i >= 0; i--)
+        if (t[i] != PM4[i]) return t[i] > PM4[i];
+    return true;
+}
+static void m4_sub_p(uint64_t *t) {
+    unsigned __int128 br = 0;
+    for (int i = 0; i < 4; i++) {
+        const unsigned __int128 d = (unsigned __int128)t[i] - PM4[i] - br;
+        t[i] = (uint64_t)d;
+        br = (d >> 64) ? 1 : 0;
+    }
+}
+static M4 m4_add(const M4 &a, const M4 &b) {
+    M4 r;
+    unsigned __int128 c = 0;
+    for (int i = 0; i < 4; i++) {
+        c += (unsigned __int128)a.w[i] + b.w[i];
+        r.w[i] = (uint64_t)c;
+        c >>= 64;
+    }
+    if (c || m4_ge_p(r.w)) m4_sub_p(r.w);
+    return r;
+}
+static M4 m4_sub(const M4 &a, const M4 &b) {
+    M4 r;
+    unsigned __int128 br = 0;
+    for (int i = 0; i < 4; i++) {
+        const unsigned __int128 d = (unsigned __int128)a.w[i] - b.w[i] - br;
+        r.w[i] = (uint64_t)d;
+        br = (d >> 64) ? 1 : 0;
+    }
+    if (br) {  // + p
+        unsigned __int128 c = 0;
+        for (int i = 0; i < 4; i++) {
+            c += (unsigned __int128)r.w[i] + PM4[i];
+            r.w[i] = (uint64_t)c;
+            c >>= 64;
+        }
+    }
+    return r;
+}
+// a b R^-1 mod p (CIOS), a, b < p
+static M4 m4_mul(const M4 &a, const M4 &b) {
+    uint64_t t[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 4; i++) {
+        unsigned __int128 c = 0;
+        for (int j = 0; j < 4; j++) {
+            c += (unsigned __int128)a.w[j] * b.w[i] + t[j];
+            t[j] = (uint64_t)c;
+            c >>= 64;
+        }
+        c += t[4];
+        t[4] = (uint64_t)c;
+        t[5] = (uint64_t)(c >> 64);
+        const uint64_t m = t[0];
+        c = ((unsigned __int128)m * PM4[0] + t[0]) >> 64;
+        for (int j = 1; j < 4; j++) {
+            c += (unsigned __int128)m * PM4[j] + t[j];
+            t[j - 1] = (uint64_t)c;
+            c >>= 64;
+        }
+        c += t[4];
+        t[3] = (uint64_t)c;
+        t[4] = t[5] + (uint64_t)(c >> 64);
+    }
+    M4 r = {{t[0], t[1], t[2], t[3]}};
+    if (t[4] || m4_ge_p(r.w)) m4_sub_p(r.w);
+    return r;
+}
+static M4 m4_of(const Limbs &l) {
+    uint32_t w[9];
+    from_limbs(l, w);
+    M4 x;
+    for (int i = 0; i < 4; i++) x.w[i] = (uint64_t)w[2 * i] | (uint64_t)w[2 * i + 1] << 32;
+    return m4_mul(x, R2M4);
+}
+static Limbs limbs_of(const M4 &x) {
+    const M4 one = {{1, 0, 0, 0}};
+    const M4 v = m4_mul(x, one);
+    uint32_t w[8];
+    for (int i = 0; i < 4; i++) {
+        w[2 * i] = (uint32_t)v.w[i];
+        w[2 * i + 1] = (uint32_t)(v.w[i] >> 32);
+    }
+    return to_limbs(w, 8);
+}
+static void run_native(const Op *prog, int n_ops, M4 *reg) {
+    for (int k = 0; k < n_ops; k++) {
+        const Op o = prog[k];
+        reg[o.d] = o.op == GM ? m4_mul(reg[o.a], reg[o.b]) : o.op == GA ? m4_add(reg[o.a], reg[o.b])
+                                                                         : m4_sub(reg[o.a], reg[o.b]);
+    }
+}
+// acc_seq[r] = the accumulator entering row r, r <= h (inputs below p)
+static void chain_points(const P256Chains &in, int c, uint64_t h, std::array<Limbs, 3> *acc_seq) {
+    const M4 bm = m4_of(be_limbs(B_BE));
+    M4 acc[3], base[3];
+    for (int i = 0; i < 3; i++) {
+        acc[i] = m4_of(be_limbs(in.pts[6 * c + i].data()));
+        base[i] = m4_of(be_limbs(in.pts[6 * c + 3 + i].data()));
+    }
+    const std::vector<uint8_t> &bits = c ? in.b_bits : in.a_bits;
+    for (uint64_t r = 0;; r++) {
+        for (int i = 0; i < 3; i++) acc_seq[r][i] = limbs_of(acc[i]);
+        if (r == h) break;
+        M4 rd[16], ra[16];
+        for (int i = 0; i < 3; i++) rd[i] = acc[i];
+        rd[3] = bm;
+        run_native(DBL, 34, rd);
+        for (int i = 0; i < 3; i++) {
+            ra[i] = rd[8 + i];
+            ra[3 + i] = base[i];
+        }
+        ra[6] = bm;
+        run_native(ADD, 43, ra);
+        const int bit = r < bits.size() ? (bits[r] != 0) : 0;
+        for (int i = 0; i < 3; i++) acc[i] = bit ? ra[12 + i] : rd[8 + i];
+    }
+}
+// rows [r0, r1) of chain c from the known accumulators: row_step's own
+// accumulator out of each row must equal the next native one
+static bool chain_rows_range(const P256Chains &in, int c, uint64_t r0, uint64_t r1,
+                             const std::array<Limbs, 3> *acc_seq, uint64_t *words) {
     const Layout &L = layout();
     const uint64_t nw = L.n_words;
     const Limbs bconst = be_limbs(B_BE);
-    Limbs acc[3], base[3];
-    for (int i = 0; i < 3; i++) {
-        acc[i] = be_limbs(in.pts[6 * c + i].data());
-        base[i] = be_limbs(in.pts[6 * c + 3 + i].data());
-    }
+    Limbs base[3];
+    for (int i = 0; i < 3; i++) base[i] = be_limbs(in.pts[6 * c + 3 + i].data());
     const std::vector<uint8_t> &bits = c ? in.b_bits : in.a_bits;
-    for (uint64_t r = 0; r < h; r++) {
+    for (uint64_t r = r0; r < r1; r++) {
+        Limbs acc[3] = {acc_seq[r][0], acc_seq[r][1], acc_seq[r][2]};
         const int bit = r < bits.size() ? (bits[r] != 0) : 0;
         if (!row_step(L.ch[c], acc, base, bit, bconst, words + r * nw)) return false;
-        if (r + 1 == bits.size())
-            for (int i = 0; i < 3; i++) out[i] = acc[i];
+        for (int i = 0; i < 3; i++)
+            if (acc[i] != acc_seq[r + 1][i]) return false;
     }
     return true;
 }
@@ -614,13 +744,35 @@ void shared_rows(const P256Chains &in, uint64_t h, const Limbs (*out)[3], uint64
     }
 }
 
-// All h rows of one proof: every word of every row is written.
-void chain_words(const P256Chains &in, uint64_t h, uint64_t *words, std::array<Be32, 6> &rproj) {
-    Limbs out[2][3];
-    for (int c = 0; c < 2; c++)
-        if (!chain_rows(in, c, h, words, out[c]))
-            throw Error(E_INVALID, "p256 trace: a field product does not close (inputs must be below p)");
-    shared_rows(in, h, out, words, rproj);
+// All h rows of K proofs (words + k * h * n_words): the accumulators of every
+// row natively, chain by chain, then the rows' words in parallel chunks.
+void batch_words(const std::vector<P256Chains> &in, uint64_t h, uint64_t *words,
+                 std::vector<std::array<Be32, 6>> &rp) {
+    const size_t K = in.size();
+    const uint64_t per = h * layout().n_words;
+    std::vector<std::array<Limbs, 3>> seq(2 * K * (h + 1));
+    auto seq_of = [&](size_t t) { return seq.data() + t * (h + 1); };
+    parallel_for(2 * K, [&](size_t t) { chain_points(in[t / 2], (int)(t % 2), h, seq_of(t)); });
+    constexpr uint64_t RCH = 16;  // rows per task
+    const uint64_t chunks = (h + RCH - 1) / RCH;
+    std::vector<uint8_t> bad(2 * K * chunks, 0);
+    parallel_for(2 * K * chunks, [&](size_t u) {
+        const size_t t = u / chunks, k = t / 2;
+        const uint64_t r0 = (u % chunks) * RCH, r1 = std::min(h, r0 + RCH);
+        bad[u] = !chain_rows_range(in[k], (int)(t % 2), r0, r1, seq_of(t), words + k * per);
+    });
+    for (size_t u = 0; u < bad.size(); u++)
+        SG_REQUIRE(!bad[u], "p256 trace: a field product does not close (inputs must be below p)");
+    rp.resize(K);
+    parallel_for(K, [&](size_t k) {
+        Limbs out[2][3];
+        for (int c = 0; c < 2; c++) {
+            // each chain's output: the accumulator after its last bit
+            const std::vector<uint8_t> &bits = c ? in[k].b_bits : in[k].a_bits;
+            for (int i = 0; i < 3; i++) out[c][i] = seq_of(2 * k + c)[bits.size()][i];
+        }
+        shared_rows(in[k], h, out, words + k * per, rp[k]);
+    });
 }
 
 void check_inputs(const std::vector<P256Chains> &in, uint64_t h) {
@@ -1229,9 +1381,9 @@ void build_p256_trace_host(const P256Chains &in, uint64_t h, uint64_t *tr, std::
     check_inputs({in}, h);
     const Layout &L = layout();
     std::vector<uint64_t> words((size_t)h * L.n_words, 0);
-    std::array<Be32, 6> rp;
-    chain_words(in, h, words.data(), rp);
-    if (rproj_out) *rproj_out = rp;
+    std::vector<std::array<Be32, 6>> rp;
+    batch_words({in}, h, words.data(), rp);
+    if (rproj_out) *rproj_out = rp[0];
     parallel_for(16, [&](size_t part) {
         for (uint64_t col = part; col < P256_W; col += 16)
             for (uint64_t r = 0; r < h; r++) tr[col * h + r] = cell(L.desc[col], words.data() + r * L.n_words);
@@ -1250,21 +1402,8 @@ void build_p256_traces_dev(Ctx *c, const std::vector<P256Chains> &in, uint64_t h
     const uint64_t nw = L.n_words, per = h * nw;
     // every word is written below, straight into pinned memory
     uint64_t *words = (uint64_t *)c->upload_staging(8 * K * per);
-    std::vector<std::array<Be32, 6>> rp(K);
-    std::vector<std::array<std::array<Limbs, 3>, 2>> outs(K);
-    std::vector<int> bad(2 * K, 0);
-    parallel_for(2 * K, [&](size_t t) {
-        const size_t k = t / 2;
-        const int ch = (int)(t % 2);
-        bad[t] = !chain_rows(in[k], ch, h, words + k * per, outs[k][ch].data());
-    });
-    for (size_t t = 0; t < 2 * K; t++)
-        SG_REQUIRE(!bad[t], "p256 trace: a field product does not close (inputs must be below p)");
-    parallel_for(K, [&](size_t k) {
-        const Limbs out[2][3] = {{outs[k][0][0], outs[k][0][1], outs[k][0][2]},
-                                 {outs[k][1][0], outs[k][1][1], outs[k][1][2]}};
-        shared_rows(in[k], h, out, words + k * per, rp[k]);
-    });
+    std::vector<std::array<Be32, 6>> rp;
+    batch_words(in, h, words, rp);
     if (rproj_out) *rproj_out = rp;
     Scoped d_words(c, 8 * K * per);
     SG_HIP(hipMemcpyAsync(d_words.p, words, 8 * K * per, hipMemcpyHostToDevice, c->stream));
