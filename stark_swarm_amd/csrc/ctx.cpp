@@ -67,6 +67,27 @@ void Ctx::d2h(void *dst, const void *src, size_t n) {
     }
 }
 
+void Ctx::d2h2(void *dst1, const void *src1, size_t n1, void *dst2, const void *src2, size_t n2) {
+    constexpr size_t SMALL = 1ULL << 20;  // the pinned buffer's minimum size
+    if (n1 + n2 > SMALL) {
+        d2h(dst1, src1, n1);
+        d2h(dst2, src2, n2);
+        return;
+    }
+    if (pin_bytes < SMALL) {
+        if (pin) (void)hipHostFree(pin);
+        pin = nullptr;
+        pin_bytes = 0;
+        SG_HIP(hipHostMalloc(&pin, SMALL, hipHostMallocDefault));
+        pin_bytes = SMALL;
+    }
+    if (n1) SG_HIP(hipMemcpyAsync(pin, src1, n1, hipMemcpyDeviceToHost, stream));
+    if (n2) SG_HIP(hipMemcpyAsync((uint8_t *)pin + n1, src2, n2, hipMemcpyDeviceToHost, stream));
+    SG_HIP(hipStreamSynchronize(stream));
+    if (n1) memcpy(dst1, pin, n1);
+    if (n2) memcpy(dst2, (const uint8_t *)pin + n1, n2);
+}
+
 void *Ctx::upload_staging(size_t n) {
     if (up_bytes < n) {
         if (up) (void)hipHostFree(up);

@@ -105,6 +105,9 @@ struct Ctx {
     // copy n bytes of device memory to host memory after the stream's
     // earlier work; returns once dst holds them
     void d2h(void *dst, const void *src, size_t n);
+    // two such copies behind one wait (small reads of one stage, e.g. a
+    // layer's coset evaluations and its roots)
+    void d2h2(void *dst1, const void *src1, size_t n1, void *dst2, const void *src2, size_t n2);
 
     void *alloc(size_t bytes);
     void release(void *p);

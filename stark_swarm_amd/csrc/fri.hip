@@ -241,11 +241,17 @@ __device__ __forceinline__ void f6_add_mul_upow(fp6 &acc, const fp6 &x) {
 // A block is 6 waves over 64 outputs b, wave q taking component q (one
 // output per lane, no divergence), so a small layer's per-output chain is a
 // sixth as long; the six Fp6 partial sums meet in LDS.
+// apow null: the powers of the single proof come in pa (a kernel argument, so
+// a one-proof fold needs no upload and waits on no copy).
 constexpr int FOLD_T = 6 * 64;
+struct FoldPow {
+    uint64_t v[6 * 16];
+};
 template <int LM>
 __global__ __launch_bounds__(FOLD_T) void coset_fold_k(const uint64_t *f, uint64_t n, int ncomp, const uint64_t *apow,
-                                                       uint64_t minv, const uint64_t *rt_lo, const uint64_t *rt_hi,
-                                                       uint32_t logn, uint64_t *fnext, uint64_t *tuples) {
+                                                       FoldPow pa, uint64_t minv, const uint64_t *rt_lo,
+                                                       const uint64_t *rt_hi, uint32_t logn, uint64_t *fnext,
+                                                       uint64_t *tuples) {
     constexpr int M = 1 << LM;
     __shared__ uint64_t red[6][6][64];
     const uint64_t nn = n >> LM;
@@ -255,7 +261,7 @@ __global__ __launch_bounds__(FOLD_T) void coset_fold_k(const uint64_t *f, uint64
     const uint64_t pk = blockIdx.y;
     f += pk * ncomp * n;
     fnext += pk * 6 * nn;
-    apow += pk * 6 * M;
+    if (apow) apow += pk * 6 * M;
     if (tuples) tuples += pk * 6 * M * nn;
     // w^(-b) from the 2^26 root table
     uint64_t e26 = ((live ? b : 0) << (26 - logn)) & ((1ULL << 26) - 1);
@@ -283,7 +289,7 @@ __global__ __launch_bounds__(FOLD_T) void coset_fold_k(const uint64_t *f, uint64
             if (tuples) tuples[(uint64_t)(k * 6 + q) * nn + b] = y;
             fp6 ak;
 #pragma unroll
-            for (int i = 0; i < 6; i++) ak.c[i] = apow[6 * k + i];
+            for (int i = 0; i < 6; i++) ak.c[i] = apow ? apow[6 * k + i] : pa.v[6 * k + i];
             pq = f6_add(pq, f6_scale(ak, y));
             s = mul(s, wb);
         }
@@ -312,23 +318,39 @@ __global__ __launch_bounds__(FOLD_T) void coset_fold_k(const uint64_t *f, uint64
 
 template <int LM>
 void launch_fold(Ctx *c, const uint64_t *f, uint64_t n, int ncomp, uint32_t K, const uint64_t *d_apow,
-                 uint64_t *fnext, uint64_t *tuples) {
+                 const FoldPow &pa, uint64_t *fnext, uint64_t *tuples) {
     const uint64_t nn = n >> LM;
     const uint64_t minv = inv(1ULL << LM);
     ProfScope ps(c, "coset_fold", (8.0 * ncomp * n + 48.0 * nn + (tuples ? 48.0 * n : 0)) * K);
     hipLaunchKernelGGL(coset_fold_k<LM>, dim3((uint32_t)((nn + 63) / 64), K), dim3(FOLD_T), 0, c->stream, f, n, ncomp,
-                       d_apow, minv, c->rt_lo, c->rt_hi, ilog2(n), fnext, tuples);
+                       d_apow, pa, minv, c->rt_lo, c->rt_hi, ilog2(n), fnext, tuples);
     SG_HIP(hipGetLastError());
 }
 
-// d_apow: per proof alpha^k, k < m (m Fp6 values)
-void fold(Ctx *c, const uint64_t *f, uint64_t n, int ncomp, uint64_t m, uint32_t K, const uint64_t *d_apow,
-          uint64_t *fnext, uint64_t *tuples) {
+void upload_powers(Ctx *c, const std::vector<fp6> &alpha, uint64_t m, uint64_t *d);
+
+// alpha: one folding challenge per proof; its powers alpha^k (k < m) go to
+// the kernel as an argument for one proof, else through d_apow (K m Fp6)
+void fold(Ctx *c, const uint64_t *f, uint64_t n, int ncomp, uint64_t m, uint32_t K, const std::vector<fp6> &alpha,
+          uint64_t *d_apow, uint64_t *fnext, uint64_t *tuples) {
+    SG_REQUIRE(alpha.size() == K && m <= 16, "fold: one alpha per proof, arity at most 16");
+    FoldPow pa{};
+    const uint64_t *dp = nullptr;
+    if (K == 1) {
+        fp6 x = f6_one();
+        for (uint64_t j = 0; j < m; j++) {
+            memcpy(&pa.v[6 * j], x.c, 48);
+            x = f6_mul(x, alpha[0]);
+        }
+    } else {
+        upload_powers(c, alpha, m, d_apow);
+        dp = d_apow;
+    }
     switch (m) {
-    case 2: launch_fold<1>(c, f, n, ncomp, K, d_apow, fnext, tuples); break;
-    case 4: launch_fold<2>(c, f, n, ncomp, K, d_apow, fnext, tuples); break;
-    case 8: launch_fold<3>(c, f, n, ncomp, K, d_apow, fnext, tuples); break;
-    case 16: launch_fold<4>(c, f, n, ncomp, K, d_apow, fnext, tuples); break;
+    case 2: launch_fold<1>(c, f, n, ncomp, K, dp, pa, fnext, tuples); break;
+    case 4: launch_fold<2>(c, f, n, ncomp, K, dp, pa, fnext, tuples); break;
+    case 8: launch_fold<3>(c, f, n, ncomp, K, dp, pa, fnext, tuples); break;
+    case 16: launch_fold<4>(c, f, n, ncomp, K, dp, pa, fnext, tuples); break;
     default: throw Error(E_UNSUPPORTED, "fold arity must be 2, 4, 8 or 16");
     }
 }
@@ -563,8 +585,7 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_classic(Ctx *c, const uint64_t 
         }
         uint64_t *next = c->alloc_n<uint64_t>(6ULL * nn * K);
         if (!final_cc) {
-            upload_powers(c, alpha, m, d_alpha.as<uint64_t>());
-            fold(c, f, cur, ncomp, m, K, d_alpha.as<uint64_t>(), next, nullptr);
+            fold(c, f, cur, ncomp, m, K, alpha, d_alpha.as<uint64_t>(), next, nullptr);
         }
         hipLaunchKernelGGL(s_planes_k, grid, dim3(FT), 0, c->stream, final_cc ? nullptr : next, cur, nn, planes[ell]);
         SG_HIP(hipGetLastError());
@@ -574,8 +595,7 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_classic(Ctx *c, const uint64_t 
             LeafSrc src{planes[ell], 1, cur, 18, 18, nullptr, 18 * cur};
             tree_build(c, trees[ell], src, cur, a, merkle_depth(cur, a), ell, K, false);
             std::vector<uint64_t> ev((size_t)K * m * 6);
-            c->d2h(ev.data(), d_ev.p, 8 * ev.size());
-            tree_read_roots(c, trees[ell]);
+            tree_read_roots_with(c, trees[ell], ev.data(), d_ev.p, 8 * ev.size());
             for (uint32_t k = 0; k < K; k++) memcpy(&fz[ell][6 * k], &ev[(size_t)k * m * 6], 48);
         }
         parallel_for(K, [&](size_t k) {
@@ -587,15 +607,13 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_classic(Ctx *c, const uint64_t 
             // coefficient tuples and their tree (fri.rs:1649-1688), then this layer's alpha
             Scoped tup(c, 8ULL * 6 * m * nn * K);
             std::vector<fp6> zero(K, f6_zero());
-            upload_powers(c, zero, m, d_alpha.as<uint64_t>());
-            fold(c, f, cur, ncomp, m, K, d_alpha.as<uint64_t>(), next, tup.as<uint64_t>());
+            fold(c, f, cur, ncomp, m, K, zero, d_alpha.as<uint64_t>(), next, tup.as<uint64_t>());
             const uint32_t a = pick_arity(nn, 16);
             DevTree tc;
             LeafSrc src{tup.as<uint64_t>(), 1, nn, (uint32_t)(6 * m), (uint32_t)(6 * m), nullptr, 6 * m * nn};
             tree_build(c, tc, src, nn, a, merkle_depth(nn, a), 0xFE, K, false);
             tuples_planes.resize((size_t)6 * m * nn * K);
-            c->d2h(tuples_planes.data(), tup.p, 8 * tuples_planes.size());
-            tree_read_roots(c, tc);
+            tree_read_roots_with(c, tc, tuples_planes.data(), tup.p, 8 * tuples_planes.size());
             n_final_tuples = nn;
             m_final = m;
             parallel_for(K, [&](size_t k) {
@@ -606,8 +624,7 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_classic(Ctx *c, const uint64_t 
                 tr[k].challenge_ext("beta_deg", beta.c);
             });
             tree_free(c, tc);
-            upload_powers(c, alpha, m, d_alpha.as<uint64_t>());
-            fold(c, f, cur, ncomp, m, K, d_alpha.as<uint64_t>(), next, nullptr);
+            fold(c, f, cur, ncomp, m, K, alpha, d_alpha.as<uint64_t>(), next, nullptr);
         }
         if (owned) c->release(owned);
         owned = next;
@@ -822,8 +839,7 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
             const uint32_t a = pick_arity(cur, m);
             LeafSrc src{f, 1, cur, 6, (uint32_t)ncomp, nullptr, (uint64_t)ncomp * cur};
             tree_build(c, tl, src, cur, a, merkle_depth(cur, a), ell, K, false);
-            c->d2h(coset_evals[ell].data(), d_ev.p, 48ULL * m * K);
-            tree_read_roots(c, tl);
+            tree_read_roots_with(c, tl, coset_evals[ell].data(), d_ev.p, 48ULL * m * K);
         }
         parallel_for(K, [&](size_t k) {
             roots[ell * K + k] = tl.roots[k];
@@ -840,15 +856,13 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
         if (final_cc) {
             Scoped tup(c, 8ULL * 6 * m * nn * K);
             std::vector<fp6> zero(K, f6_zero());
-            upload_powers(c, zero, m, d_alpha.as<uint64_t>());
-            fold(c, f, cur, ncomp, m, K, d_alpha.as<uint64_t>(), next, tup.as<uint64_t>());
+            fold(c, f, cur, ncomp, m, K, zero, d_alpha.as<uint64_t>(), next, tup.as<uint64_t>());
             const uint32_t a = pick_arity(nn, 16);
             DevTree tc;
             LeafSrc src{tup.as<uint64_t>(), 1, nn, (uint32_t)(6 * m), (uint32_t)(6 * m), nullptr, 6 * m * nn};
             tree_build(c, tc, src, nn, a, merkle_depth(nn, a), 0xFE, K, false);
             tuples_planes.resize((size_t)6 * m * nn * K);
-            c->d2h(tuples_planes.data(), tup.p, 8 * tuples_planes.size());
-            tree_read_roots(c, tc);
+            tree_read_roots_with(c, tc, tuples_planes.data(), tup.p, 8 * tuples_planes.size());
             n_final_tuples = nn;
             m_final = m;
             parallel_for(K, [&](size_t k) {
@@ -862,8 +876,7 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
         }
         // fold with this layer's alpha (the final layer's is drawn after its coefficient commitment)
         HostScope hfold(c, "fri_fold");
-        upload_powers(c, alpha, m, d_alpha.as<uint64_t>());
-        fold(c, f, cur, ncomp, m, K, d_alpha.as<uint64_t>(), next, nullptr);
+        fold(c, f, cur, ncomp, m, K, alpha, d_alpha.as<uint64_t>(), next, nullptr);
         if (ell == 0) {
             layer1 = next;
         } else if (owned && owned != layer1) {

@@ -759,10 +759,12 @@ void tree_build(Ctx *c, DevTree &t, const LeafSrc &src, uint64_t n_leaves, uint3
     if (read_roots) tree_read_roots(c, t);
 }
 
-void tree_read_roots(Ctx *c, DevTree &t) {
+void tree_read_roots(Ctx *c, DevTree &t) { tree_read_roots_with(c, t, nullptr, nullptr, 0); }
+
+void tree_read_roots_with(Ctx *c, DevTree &t, void *dst, const void *src, size_t n) {
     // the top level holds one root per tree
     std::vector<uint64_t> r(4 * (size_t)t.batch);
-    c->d2h(r.data(), t.buf + 4 * t.off.back(), 32 * (size_t)t.batch);
+    c->d2h2(dst, src, n, r.data(), t.buf + 4 * t.off.back(), 32 * (size_t)t.batch);
     t.roots.resize(t.batch);
     for (uint32_t k = 0; k < t.batch; k++) memcpy(t.roots[k].data(), &r[4 * k], 32);
     memcpy(t.root, t.roots[0].data(), 32);

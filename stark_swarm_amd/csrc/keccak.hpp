@@ -84,6 +84,9 @@ struct LeafSrc {
 void tree_build(Ctx *c, DevTree &t, const LeafSrc &src, uint64_t n_leaves, uint32_t arity, uint32_t depth,
                 uint64_t label, uint32_t batch = 1, bool read_roots = true);
 void tree_read_roots(Ctx *c, DevTree &t);
+// the same, with n more bytes from device src copied to host dst behind the
+// same wait
+void tree_read_roots_with(Ctx *c, DevTree &t, void *dst, const void *src, size_t n);
 void tree_free(Ctx *c, DevTree &t);
 // Serialized MerkleOpenings (leaf[32], path Vec<Vec<[32]>>, index u64): idx[k]
 // lists the leaf indices to open in tree k; out[k][q] is one opening.
