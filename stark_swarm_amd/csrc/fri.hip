@@ -652,31 +652,30 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_classic(Ctx *c, const uint64_t 
             }
         }
     // payloads (f, s, q) of every query and layer, gathered from the leaf planes
+    // every layer's payload gather and every tree's opening digests behind one wait
     std::vector<std::vector<uint64_t>> pay(L);
-    {
-        HostScope hs(c, "fri_gather");
-        for (size_t ell = 0; ell < L; ell++) {
-            const uint64_t n = sz[ell];
-            std::vector<uint64_t> gidx;
-            gidx.reserve((size_t)K * r * 18);
-            for (uint32_t k = 0; k < K; k++)
-                for (uint64_t q = 0; q < r; q++)
-                    for (uint64_t j = 0; j < 18; j++) gidx.push_back(k * 18 * n + j * n + pos[ell][k][q]);
-            pay[ell].resize(gidx.size());
-            if (gidx.empty()) continue;
-            Scoped di(c, 8 * gidx.size()), dv(c, 8 * gidx.size());
-            SG_HIP(hipMemcpyAsync(di.p, gidx.data(), 8 * gidx.size(), hipMemcpyHostToDevice, c->stream));
-            gather_u64(c, planes[ell], di.as<uint64_t>(), gidx.size(), dv.as<uint64_t>());
-            c->d2h(pay[ell].data(), dv.p, 8 * gidx.size());
-            SG_HIP(hipStreamSynchronize(c->stream));
-        }
-    }
     std::vector<std::vector<std::vector<std::vector<uint8_t>>>> lop(L);
     std::vector<std::vector<std::vector<uint8_t>>> f0op;
     {
-        HostScope ho(c, "fri_openings");
-        for (size_t ell = 0; ell < L; ell++) tree_open_multi(c, trees[ell], pos[ell], lop[ell]);
-        tree_open_multi(c, t0, pos[0], f0op);
+        HostScope hs(c, "fri_gather_openings");
+        std::vector<GatherJob> jobs(2 * L + 1);
+        for (size_t ell = 0; ell < L; ell++) {
+            const uint64_t n = sz[ell];
+            GatherJob &g = jobs[ell];
+            g.src = planes[ell];
+            g.idx.reserve((size_t)K * r * 18);
+            for (uint32_t k = 0; k < K; k++)
+                for (uint64_t q = 0; q < r; q++)
+                    for (uint64_t j = 0; j < 18; j++) g.idx.push_back(k * 18 * n + j * n + pos[ell][k][q]);
+            jobs[L + ell] = tree_open_job(trees[ell], pos[ell]);
+        }
+        jobs[2 * L] = tree_open_job(t0, pos[0]);
+        gather_set(c, jobs);
+        for (size_t ell = 0; ell < L; ell++) {
+            pay[ell] = std::move(jobs[ell].out);
+            tree_open_finish(trees[ell], pos[ell], jobs[L + ell], lop[ell]);
+        }
+        tree_open_finish(t0, pos[0], jobs[2 * L], f0op);
     }
     for (size_t ell = 0; ell < L; ell++) {
         tree_free(c, trees[ell]);
@@ -911,21 +910,27 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_batch(Ctx *c, const uint64_t *d
         for (uint32_t k = 0; k < K; k++)
             for (uint64_t q = 0; q < p.s0; q++)
                 for (int q6 = 0; q6 < 6; q6++) gidx.push_back(k * 6 * n1 + q6 * n1 + base[k][q]);
+    // the payload gathers and both trees' opening digests behind one wait
     std::vector<uint64_t> gv(gidx.size());
-    if (!gidx.empty()) {
-        HostScope hs(c, "fri_gather");
-        Scoped di(c, 8 * gidx.size()), dv(c, 8 * gidx.size());
-        SG_HIP(hipMemcpyAsync(di.p, gidx.data(), 8 * gidx.size(), hipMemcpyHostToDevice, c->stream));
-        gather_u64(c, d_f0, di.as<uint64_t>(), nfib, dv.as<uint64_t>());
-        if (l2) gather_u64(c, layer1, di.as<uint64_t>() + nfib, gidx.size() - nfib, dv.as<uint64_t>() + nfib);
-        c->d2h(gv.data(), dv.p, 8 * gidx.size());
-        SG_HIP(hipStreamSynchronize(c->stream));
-    }
     std::vector<std::vector<std::vector<uint8_t>>> op0, op1;
     {
-        HostScope ho(c, "fri_openings");
-        tree_open_multi(c, t0, base, op0);
-        if (l2) tree_open_multi(c, t1, base, op1);
+        HostScope hs(c, "fri_gather_openings");
+        std::vector<GatherJob> jobs(l2 ? 4 : 2);
+        jobs[0].src = d_f0;
+        jobs[0].idx.assign(gidx.begin(), gidx.begin() + nfib);
+        jobs[1] = tree_open_job(t0, base);
+        if (l2) {
+            jobs[2].src = layer1;
+            jobs[2].idx.assign(gidx.begin() + nfib, gidx.end());
+            jobs[3] = tree_open_job(t1, base);
+        }
+        gather_set(c, jobs);
+        std::copy(jobs[0].out.begin(), jobs[0].out.end(), gv.begin());
+        tree_open_finish(t0, base, jobs[1], op0);
+        if (l2) {
+            std::copy(jobs[2].out.begin(), jobs[2].out.end(), gv.begin() + nfib);
+            tree_open_finish(t1, base, jobs[3], op1);
+        }
     }
     HostScope hs(c, "fri_serialize");
 

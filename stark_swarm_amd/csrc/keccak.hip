@@ -786,23 +786,58 @@ void gather_u64_w(Ctx *c, const uint64_t *src, const uint64_t *d_idx, uint64_t n
     SG_HIP(hipGetLastError());
 }
 
-void tree_open_multi(Ctx *c, const DevTree &t, const std::vector<std::vector<uint64_t>> &idx,
-                     std::vector<std::vector<std::vector<uint8_t>>> &out) {
+void gather_set(Ctx *c, std::vector<GatherJob> &jobs) {
+    size_t ni = 0, no = 0;
+    for (const GatherJob &j : jobs) {
+        ni += j.idx.size();
+        no += j.idx.size() * j.width;
+    }
+    if (!ni) {
+        for (GatherJob &j : jobs) j.out.clear();
+        return;
+    }
+    std::vector<uint64_t> all(ni);
+    size_t oi = 0;
+    for (const GatherJob &j : jobs) {
+        std::copy(j.idx.begin(), j.idx.end(), all.begin() + oi);
+        oi += j.idx.size();
+    }
+    Scoped d_idx(c, 8 * ni), d_out(c, 8 * no);
+    SG_HIP(hipMemcpyAsync(d_idx.p, all.data(), 8 * ni, hipMemcpyHostToDevice, c->stream));
+    oi = 0;
+    size_t oo = 0;
+    for (const GatherJob &j : jobs) {
+        gather_u64_w(c, j.src, d_idx.as<uint64_t>() + oi, j.idx.size(), j.width, d_out.as<uint64_t>() + oo);
+        oi += j.idx.size();
+        oo += j.idx.size() * j.width;
+    }
+    std::vector<uint64_t> h(no);
+    c->d2h(h.data(), d_out.p, 8 * no);
+    oo = 0;
+    for (GatherJob &j : jobs) {
+        j.out.assign(h.begin() + oo, h.begin() + oo + j.idx.size() * j.width);
+        oo += j.idx.size() * j.width;
+    }
+}
+
+// digests an opening needs, per query: the leaf, then every level's full
+// sibling group (a digest is a row of 4 u64 of t.buf)
+GatherJob tree_open_job(const DevTree &t, const std::vector<std::vector<uint64_t>> &idx) {
     const uint32_t a = t.arity;
     const size_t nlev = t.real.size(), K = idx.size();
     SG_REQUIRE(K <= t.batch, "merkle open: more index lists than trees");
-    // digest indices: per query, the leaf then every level's full group
     const size_t per = 1 + (nlev - 1) * a;
     std::vector<size_t> qoff(K + 1, 0);
     for (size_t k = 0; k < K; k++) {
         for (uint64_t q : idx[k]) SG_REQUIRE(q < t.real[0], "merkle open: leaf index out of range");
         qoff[k + 1] = qoff[k] + idx[k].size();
     }
-    out.assign(K, {});
-    if (!qoff[K]) return;
-    std::vector<uint64_t> want(qoff[K] * per);
+    GatherJob job;
+    job.src = t.buf;
+    job.width = 4;
+    job.idx.resize(qoff[K] * per);
     parallel_for(K, [&](size_t k) {
-        uint64_t *w = &want[qoff[k] * per];
+        uint64_t *w = &job.idx[qoff[k] * per];
         for (uint64_t q : idx[k]) {
             *w++ = t.off[0] + k * t.real[0] + q;
             uint64_t x = q;
@@ -817,14 +852,17 @@ void tree_open_multi(Ctx *c, const DevTree &t, const std::vector<std::vector<uin
             }
         }
     });
-    Scoped d_idx(c, 8 * want.size()), d_out(c, 32 * want.size());
-    SG_HIP(hipMemcpyAsync(d_idx.p, want.data(), 8 * want.size(), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(gather_k, dim3((uint32_t)((4 * want.size() + 255) / 256)), dim3(256), 0, c->stream, t.buf,
-                       d_idx.as<uint64_t>(), (uint64_t)want.size(), 4u, d_out.as<uint64_t>());
-    SG_HIP(hipGetLastError());
-    std::vector<uint64_t> h(4 * want.size());
-    c->d2h(h.data(), d_out.p, 32 * want.size());
-    SG_HIP(hipStreamSynchronize(c->stream));
+    return job;
+}
+
+void tree_open_finish(const DevTree &t, const std::vector<std::vector<uint64_t>> &idx, const GatherJob &job,
+                      std::vector<std::vector<std::vector<uint8_t>>> &out) {
+    const uint32_t a = t.arity;
+    const size_t nlev = t.real.size(), K = idx.size();
+    const size_t per = 1 + (nlev - 1) * a;
+    std::vector<size_t> qoff(K + 1, 0);
+    for (size_t k = 0; k < K; k++) qoff[k + 1] = qoff[k] + idx[k].size();
+    out.assign(K, {});
     // MerkleOpening: leaf[32] || LE64(levels) || per level (LE64(a-1) || a-1 siblings) || LE64(index)
     const size_t osz = 32 + 8 + (nlev - 1) * (8 + 32 * (size_t)(a - 1)) + 8;
     auto put64 = [](uint8_t *p, uint64_t x) {
@@ -832,7 +870,7 @@ void tree_open_multi(Ctx *c, const DevTree &t, const std::vector<std::vector<uin
     };
     parallel_for(K, [&](size_t k) {
         out[k].resize(idx[k].size());
-        const uint64_t *hw = &h[4 * qoff[k] * per];
+        const uint64_t *hw = &job.out[4 * qoff[k] * per];
         for (size_t j = 0; j < idx[k].size(); j++) {
             const uint64_t q = idx[k][j];
             std::vector<uint8_t> &o = out[k][j];
@@ -859,12 +897,32 @@ void tree_open_multi(Ctx *c, const DevTree &t, const std::vector<std::vector<uin
     });
 }
 
+void tree_open_multi(Ctx *c, const DevTree &t, const std::vector<std::vector<uint64_t>> &idx,
+                     std::vector<std::vector<std::vector<uint8_t>>> &out) {
+    std::vector<GatherJob> jobs(1);
+    jobs[0] = tree_open_job(t, idx);
+    gather_set(c, jobs);
+    tree_open_finish(t, idx, jobs[0], out);
+}
+
 void fri_query_indices(Ctx *c, const std::vector<uint64_t> &seeds, uint64_t s0, uint64_t n0,
                        std::vector<std::vector<uint64_t>> &raw) {
     const size_t K = seeds.size();
     raw.assign(K, std::vector<uint64_t>(s0));
     if (!K || !s0) return;
     SG_REQUIRE(K * s0 < (1ULL << 31), "query indices: too many queries");
+    uint64_t npow2 = 1;
+    while (npow2 < n0) npow2 <<= 1;
+    if (K * s0 <= 1024) {
+        // few queries: the host's SHA3 and ChaCha12 beat a launch and a wait
+        parallel_for(K, [&](size_t k) {
+            for (uint64_t q = 0; q < s0; q++) {
+                const uint64_t f[3] = {seeds[k], 0, q};
+                raw[k][q] = (stdrng_first_u64(hash_fields_tagged("FRI/index", f, 3)) & (npow2 - 1)) % n0;
+            }
+        });
+        return;
+    }
     // the byte stream of hash_fields_tagged("FRI/index", [seed, 0, q]) with seed = q = 0
     std::vector<uint8_t> b;
     auto app = [&](const char *str) { b.insert(b.end(), str, str + strlen(str)); };
@@ -889,8 +947,6 @@ void fri_query_indices(Ctx *c, const std::vector<uint64_t> &seeds, uint64_t s0, 
     memcpy(pad.data(), b.data(), b.size());
     pad[b.size()] ^= 0x06;
     pad[pad.size() - 1] ^= 0x80;
-    uint64_t npow2 = 1;
-    while (npow2 < n0) npow2 <<= 1;
     Scoped d_t(c, pad.size()), d_s(c, 8 * K), d_r(c, 8 * K * s0);
     SG_HIP(hipMemcpyAsync(d_t.p, pad.data(), pad.size(), hipMemcpyHostToDevice, c->stream));
     SG_HIP(hipMemcpyAsync(d_s.p, seeds.data(), 8 * K, hipMemcpyHostToDevice, c->stream));

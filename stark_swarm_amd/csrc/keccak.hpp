@@ -94,6 +94,21 @@ void tree_open_multi(Ctx *c, const DevTree &t, const std::vector<std::vector<uin
                      std::vector<std::vector<std::vector<uint8_t>>> &out);
 void tree_open_batch(Ctx *c, const DevTree &t, const std::vector<uint64_t> &idx, std::vector<std::vector<uint8_t>> &out);
 
+// Several gathers behind one index upload and one wait: job j reads `width`
+// u64 per index from src (device; row r at src + r * width) into out.
+struct GatherJob {
+    const uint64_t *src = nullptr;
+    uint32_t width = 1;
+    std::vector<uint64_t> idx;  // host row indices
+    std::vector<uint64_t> out;  // filled by gather_set: idx.size() * width
+};
+void gather_set(Ctx *c, std::vector<GatherJob> &jobs);
+// tree_open_multi in two halves around a gather_set: the digest gather of the
+// openings of leaves idx[k] of tree k, then the serialized openings from it
+GatherJob tree_open_job(const DevTree &t, const std::vector<std::vector<uint64_t>> &idx);
+void tree_open_finish(const DevTree &t, const std::vector<std::vector<uint64_t>> &idx, const GatherJob &job,
+                      std::vector<std::vector<std::vector<uint8_t>>> &out);
+
 // gather out[r] = src[idx[r]] for u64 elements (device index list)
 void gather_u64(Ctx *c, const uint64_t *src, const uint64_t *d_idx, uint64_t n, uint64_t *out);
 // gather rows of `width` u64: out[r*width + l] = src[idx[r]*width + l]

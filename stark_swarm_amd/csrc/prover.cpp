@@ -609,19 +609,18 @@ std::vector<uint8_t> prover_core_dev(Ctx *c, const uint64_t *d_trace, uint64_t w
     std::vector<uint64_t> raw;
     std::vector<uint8_t> out = deep_fri_prove_dev(c, d_f0.as<uint64_t>(), n0, p, &raw);
     // row openings at the raw query indices
-    std::vector<uint64_t> gi;
+    // the row values and the row-tree openings behind one wait
+    std::vector<GatherJob> jobs(2);
+    jobs[0].src = d_lde;
     for (uint64_t q : raw)
-        for (uint64_t col = 0; col < w; col++) gi.push_back(col * n0 + q);
-    std::vector<uint64_t> vals(gi.size());
-    if (!gi.empty()) {
-        Scoped di(c, 8 * gi.size()), dv(c, 8 * gi.size());
-        SG_HIP(hipMemcpyAsync(di.p, gi.data(), 8 * gi.size(), hipMemcpyHostToDevice, c->stream));
-        gather_u64(c, d_lde, di.as<uint64_t>(), gi.size(), dv.as<uint64_t>());
-        c->d2h(vals.data(), dv.p, 8 * gi.size());
-        SG_HIP(hipStreamSynchronize(c->stream));
-    }
-    std::vector<std::vector<uint8_t>> ops;
-    tree_open_batch(c, rows, raw, ops);
+        for (uint64_t col = 0; col < w; col++) jobs[0].idx.push_back(col * n0 + q);
+    const std::vector<std::vector<uint64_t>> ridx = {raw};
+    jobs[1] = tree_open_job(rows, ridx);
+    gather_set(c, jobs);
+    const std::vector<uint64_t> &vals = jobs[0].out;
+    std::vector<std::vector<std::vector<uint8_t>>> opk;
+    tree_open_finish(rows, ridx, jobs[1], opk);
+    const std::vector<std::vector<uint8_t>> &ops = opk[0];
     tree_free(c, rows);
     put_le(out, raw.size(), 8);
     for (size_t q = 0; q < raw.size(); q++) {
