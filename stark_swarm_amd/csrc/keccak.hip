@@ -659,17 +659,28 @@ void dns_commit_batch(Ctx *c, const RecordSource &src, const uint8_t salt[16],
         SG_HIP(hipGetLastError());
     }
     leaf_hashes.assign(K, {});
-    std::vector<uint8_t> all(32 * nrec);
-    c->d2h(all.data(), d_leaf.p, 32 * nrec);
-    // trees: shards of equal size go up level by level in one launch per level
     roots.assign(K, {});
+    // trees: shards of equal size go up level by level in one launch per
+    // level; every shard's root is gathered into d_roots, and the leaf hashes
+    // and the roots come back behind one wait at the end
+    Scoped d_roots(c, 32 * K), d_ridx(c, 8 * K);
+    std::vector<uint64_t> ridx;  // per group: the root rows j * n of its level-0 layout
+    std::vector<size_t> rorder;  // shard of each gathered root
     std::map<size_t, std::vector<size_t>> by_size;
     for (size_t k = 0; k < K; k++) by_size[src.counts[k]].push_back(k);
+    for (auto &g : by_size)
+        for (size_t j = 0; j < g.second.size(); j++) {
+            ridx.push_back(j * g.first);
+            rorder.push_back(g.second[j]);
+        }
+    SG_HIP(hipMemcpyAsync(d_ridx.p, ridx.data(), 8 * K, hipMemcpyHostToDevice, c->stream));
+    size_t r0 = 0;
     for (auto &g : by_size) {
         const size_t n = g.first, G = g.second.size();
         std::vector<uint64_t> idx;
         for (size_t k : g.second)
             for (size_t i = 0; i < n; i++) idx.push_back(first[k] + i);
+        // released at the end of the iteration: later work on the stream runs after this group's
         Scoped d_idx(c, 8 * idx.size()), a(c, 32 * idx.size()), b(c, 32 * idx.size());
         SG_HIP(hipMemcpyAsync(d_idx.p, idx.data(), 8 * idx.size(), hipMemcpyHostToDevice, c->stream));
         gather_u64_w(c, d_leaf.as<uint64_t>(), d_idx.as<uint64_t>(), idx.size(), 4, a.as<uint64_t>());
@@ -684,12 +695,12 @@ void dns_commit_batch(Ctx *c, const RecordSource &src, const uint8_t salt[16],
             std::swap(cur, nxt);
             len = np;
         }
-        std::vector<uint64_t> rt(4 * n * G);
-        c->d2h(rt.data(), cur, 32 * n * G);
-        SG_HIP(hipStreamSynchronize(c->stream));
-        for (size_t j = 0; j < G; j++) memcpy(roots[g.second[j]].data(), &rt[4 * n * j], 32);
+        gather_u64_w(c, cur, d_ridx.as<uint64_t>() + r0, G, 4, d_roots.as<uint64_t>() + 4 * r0);
+        r0 += G;
     }
-    SG_HIP(hipStreamSynchronize(c->stream));
+    std::vector<uint8_t> all(32 * nrec), rt(32 * K);
+    c->d2h2(all.data(), d_leaf.p, 32 * nrec, rt.data(), d_roots.p, 32 * K);
+    for (size_t j = 0; j < K; j++) memcpy(roots[rorder[j]].data(), &rt[32 * j], 32);
     for (size_t k = 0; k < K; k++)
         leaf_hashes[k].assign(all.begin() + 32 * first[k], all.begin() + 32 * first[k + 1]);
 }
