@@ -412,9 +412,9 @@ __global__ __launch_bounds__(KT) void merkle_nodes2_k(const uint64_t *child, uin
             const uint64_t v = t == 0 ? (uint64_t)arity : t == 1 ? (uint64_t)level : t == 2 ? i : label;
             return (uint32_t)(v >> (32 * h));
         }
-        if (!live) return 0;
+        // a dead pair reads node 0's children (in bounds) and hashes garbage
         const uint32_t j = (t - 4) >> 2, q = (t - 4) & 3;
-        uint64_t ci = i * arity + j;
+        uint64_t ci = (live ? i : 0) * arity + j;
         if (ci >= n_child) ci = n_child - 1;
         return ch32[2 * (4 * ci + q) + h];
     };

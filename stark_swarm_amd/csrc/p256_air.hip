@@ -841,28 +841,51 @@ __global__ __launch_bounds__(NT, 4) void expand_odd_p256_k(const uint64_t *words
     const uint64_t col0 = blk << LOG_G;
     const uint64_t *wk = words + k * S * nw;
     uint64_t *trk = tr + k * P256_W * S, *oddk = odd + k * P256_W * S;
-#pragma unroll 2
-    for (uint32_t u = 0; u < EPT; u++) {
-        const uint32_t e = threadIdx.x + u * NT;
-        const uint64_t col = col0 + (e >> LOG_H), row = e & (S - 1);
-        uint64_t x = 0;
-        if (col < P256_W) {
-            const uint32_t d = desc[col];
-            const uint64_t v = wk[row * nw + (d >> 7)];
-            const uint32_t code = d & 127;
-            if (code < 64) x = (v >> code) & 1;
-            else if (code == D_VAL) x = v;
-            else x = code == D_POS ? (uint64_t)(v == 1) : (uint64_t)(v == ~0ULL);
-            trk[col * S + row] = x;
+    {
+        // the lane's descriptor reads, then its word reads, each all in flight
+        // before the first is used (the word address needs the descriptor).
+        // Columns past the last read the last one's cells (in bounds, not
+        // stored), so no branch splits the reads: a conditional read each
+        // gets a wait for everything before it.
+        uint32_t dv[EPT];
+        uint64_t wv[EPT];
+#pragma unroll
+        for (uint32_t u = 0; u < EPT; u++) {
+            const uint64_t col = col0 + ((threadIdx.x + u * NT) >> LOG_H);
+            dv[u] = desc[col < P256_W ? col : P256_W - 1];
         }
-        lds[padi<LOG_H>(e)] = x;
+#pragma unroll
+        for (uint32_t u = 0; u < EPT; u++) {
+            const uint64_t row = (threadIdx.x + u * NT) & (S - 1);
+            wv[u] = wk[row * nw + (dv[u] >> 7)];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < EPT; u++) {
+            const uint32_t e = threadIdx.x + u * NT;
+            const uint64_t col = col0 + (e >> LOG_H), row = e & (S - 1);
+            uint64_t x = 0;
+            if (col < P256_W) {
+                const uint64_t v = wv[u];
+                const uint32_t code = dv[u] & 127;
+                if (code < 64) x = (v >> code) & 1;
+                else if (code == D_VAL) x = v;
+                else x = code == D_POS ? (uint64_t)(v == 1) : (uint64_t)(v == ~0ULL);
+                trk[col * S + row] = x;
+            }
+            lds[padi<LOG_H>(e)] = x;
+        }
     }
     __syncthreads();
     stockham_steps<LOG_H, 0, true>(lds, TILE, tw_i);
-#pragma unroll 2
-    for (uint32_t u = 0; u < EPT; u++) {
-        const uint32_t e = threadIdx.x + u * NT;
-        lds[padi<LOG_H>(e)] = mul(lds[padi<LOG_H>(e)], twist[e & (S - 1)]);
+    {
+        uint64_t tv[EPT];
+#pragma unroll
+        for (uint32_t u = 0; u < EPT; u++) tv[u] = twist[(threadIdx.x + u * NT) & (S - 1)];
+#pragma unroll
+        for (uint32_t u = 0; u < EPT; u++) {
+            const uint32_t e = threadIdx.x + u * NT;
+            lds[padi<LOG_H>(e)] = mul(lds[padi<LOG_H>(e)], tv[u]);
+        }
     }
     __syncthreads();
     stockham_steps<LOG_H, 0, false>(lds, TILE, tw_f);
