@@ -61,6 +61,10 @@ constexpr int MODE_GENERAL = 0, MODE_PLAIN = 1;
 // from the w_{2^13} table, a post twiddle, lazily reduced output and no
 // scaling. Every other path is compiled out.
 constexpr int MODE_LDEA = 2;
+// MODE_POST: the first pass of a two-pass NTT or iNTT: g-fast on both sides,
+// no pre-twiddle, a post twiddle (the iNTT's 1/n rides on it), lazily reduced
+// output and no shift powers.
+constexpr int MODE_POST = 3;
 
 template <int LOG_S, bool INV, int MODE, int TLOG>
 // 16 elements a lane, at most 128 VGPRs: 512 lanes with two blocks per CU or
@@ -85,6 +89,12 @@ __global__ __launch_bounds__(TileCfg<TLOG>::LANES, 4) void ntt_tile(NttPass p) {
         p.lazy_out = 1;
         p.inv = 0;
     }
+    if constexpr (MODE == MODE_POST) {
+        p.pre_log = 0;
+        p.sp_lo = nullptr;
+        p.lazy_out = 1;
+        p.inv = INV;
+    }
     __shared__ uint64_t lds[TileCfg<TLOG>::LDS];
     constexpr uint32_t S = 1u << LOG_S;
     const uint32_t log_g = p.log_g, G = 1u << log_g;
@@ -107,8 +117,8 @@ __global__ __launch_bounds__(TileCfg<TLOG>::LANES, 4) void ntt_tile(NttPass p) {
     const bool out_gfast = log_g > 0 && p.og < p.ok;
     // g-fast lane mapping: element u of a lane is (g = gl, i = il + u*di), so
     // the lane's twiddles form geometric sequences in u (no table gathers)
-    const bool inc_in = MODE == MODE_LDEA || (in_gfast && (1u << log_g) <= (uint32_t)LANES);
-    const bool inc_out = MODE == MODE_LDEA || (out_gfast && (1u << log_g) <= (uint32_t)LANES);
+    const bool inc_in = MODE == MODE_LDEA || MODE == MODE_POST || (in_gfast && (1u << log_g) <= (uint32_t)LANES);
+    const bool inc_out = MODE == MODE_LDEA || MODE == MODE_POST || (out_gfast && (1u << log_g) <= (uint32_t)LANES);
     const uint32_t gl = threadIdx.x & (G - 1);
     const uint32_t il = threadIdx.x >> log_g;
     const uint32_t di = LANES >> log_g;
@@ -207,19 +217,22 @@ __global__ __launch_bounds__(TileCfg<TLOG>::LANES, 4) void ntt_tile(NttPass p) {
         stockham_steps<LOG_S, 0, INV, TLOG>(lds, tile, p.tw);
         uint64_t *out = out0 + (uint64_t)rep * p.out_rs;
         if (inc_out) {
-            // post twiddle w^(+-g*k) and output scale * s^(g*sp_g + k*sp_k)
+            // post twiddle w^(+-g*k) and output scale * s^(g*sp_g + k*sp_k);
+            // with a post twiddle the scale is folded into its first factor
             uint64_t pw = 1, pst = 1, sw = p.scale, sst = 1;
             if (p.post_log) {
                 const uint64_t gg = g0 + gl;
                 pw = root_pow(p.rt_lo, p.rt_hi, p.post_log, gg * (il * p.post_mul + rr * p.post_rep), p.inv);
                 pst = root_pow(p.rt_lo, p.rt_hi, p.post_log, gg * di * p.post_mul, p.inv);
+                if (p.scale != 1) pw = mul(pw, p.scale);
+                sw = 1;
             }
             if (p.sp_lo) {
                 const uint64_t e0 = (g0 + gl) * p.sp_g + (uint64_t)il * p.sp_k, ed = (uint64_t)di * p.sp_k;
                 sw = mul(sw, mul(p.sp_lo[e0 & 8191], p.sp_hi[e0 >> 13]));
                 sst = mul(p.sp_lo[ed & 8191], p.sp_hi[ed >> 13]);
             }
-            const bool sc = p.sp_lo || p.scale != 1;
+            const bool sc = p.sp_lo || (p.scale != 1 && !p.post_log);
 #pragma unroll 2
             for (int u = 0; u < PER_LANE; u++) {
                 const uint32_t k = il + u * di;
@@ -448,8 +461,16 @@ void launch_pass(Ctx *c, NttPass p, const char *name) {
     const uint32_t lanes = p.tile_log == 14 ? 1024 : NT;
     const bool ldea = !p.inv && p.pre_direct && p.post_log && p.lazy_out && p.scale == 1 && !p.sp_lo &&
                       p.log_g > 0 && p.ig < p.ii && p.og < p.ok && (1u << p.log_g) <= lanes;
-    if (p.inv)
+    const bool post = !p.pre_log && p.post_log && p.lazy_out && !p.sp_lo && p.log_g > 0 && p.ig < p.ii &&
+                      p.og < p.ok && (1u << p.log_g) <= lanes;
+    if (p.inv && plain)
+        launch_tile<true, MODE_PLAIN>(p.log_s, p.tile_log, grid, c->stream, p);
+    else if (p.inv && post)
+        launch_tile<true, MODE_POST>(p.log_s, p.tile_log, grid, c->stream, p);
+    else if (p.inv)
         launch_tile<true, MODE_GENERAL>(p.log_s, p.tile_log, grid, c->stream, p);
+    else if (post)
+        launch_tile<false, MODE_POST>(p.log_s, p.tile_log, grid, c->stream, p);
     else if (plain)
         launch_tile<false, MODE_PLAIN>(p.log_s, p.tile_log, grid, c->stream, p);
     else if (ldea)
@@ -597,13 +618,13 @@ PlanSet lde_plans(uint32_t L, uint32_t LB) {
 
 std::string autotune_report() {
     std::lock_guard<std::mutex> g(plan_mu);
-    static const char *kinds[] = {"ntt", "intt", "lde"};
+    static const char *kinds[] = {"ntt", "intt", "lde", "odd"};
     std::string out;
     char line[96];
     for (auto &kv : plan_table) {
         if (!kv.second.best) continue;
         const uint32_t k = kv.first;
-        snprintf(line, sizeof line, "%s %u %u %u -> %u.%u |", kinds[(k >> 24) % 3], (k >> 16) & 255,
+        snprintf(line, sizeof line, "%s %u %u %u -> %u.%u |", kinds[(k >> 24) % 4], (k >> 16) & 255,
                  (k >> 8) & 255, k & 255, plan_split(kv.second.best), plan_tile(kv.second.best));
         out += line;
         for (auto &m : kv.second.ms) {
@@ -639,6 +660,9 @@ void ntt_two_pass(Ctx *c, const uint64_t *in, uint64_t *out, uint64_t *scratch, 
     a.inv = inv;
     a.post_log = L;
     a.lazy_out = 1;  // pass B reads lazily reduced values
+    // the iNTT's 1/n rides on pass A's post twiddle, so pass B without shift
+    // powers is a plain pass
+    a.scale = scale;
     launch_pass(c, a, inv ? "intt_passA" : "ntt_passA");
     NttPass b = base_pass();
     b.in = scratch;
@@ -654,7 +678,6 @@ void ntt_two_pass(Ctx *c, const uint64_t *in, uint64_t *out, uint64_t *scratch, 
     b.log_s = L2;
     b.tile_log = plan_tile(plan);
     b.inv = inv;
-    b.scale = scale;
     if (spow) {
         b.sp_lo = spow->lo;
         b.sp_hi = spow->hi;
@@ -832,6 +855,55 @@ void lde_forward(Ctx *c, const uint64_t *coef, uint64_t *out, uint32_t L, uint32
     }
 }
 
+// The odd coset's forward half above 2^12 points: coset r = 1 of the
+// blowup-2 LDE form, DFT_n(c_j w_{2n}^j), as two passes of 2^L1 and
+// 2^(L - L1) points. Element j = i n2 + g takes w_{2 n1}^i before pass A's
+// sub-DFT over i (one table read) and w_{2n}^g after it with the four-step
+// twiddle: w_n^(g k) w_{2n}^g = w_{2n}^(g (2k + 1)). The coefficients carry
+// 1/n and no shift powers, so the iNTT before this has no output product.
+void odd_forward(Ctx *c, const uint64_t *coef, uint64_t *out, uint32_t L, uint32_t plan, uint64_t ncols) {
+    const uint32_t L1 = plan_split(plan), tl = plan_tile(plan);
+    SG_REQUIRE(L1 < L && L - L1 <= 12 && L1 <= 12, "odd coset: pass split out of range");
+    const uint32_t L2 = L - L1;
+    const uint64_t n = 1ULL << L, n1 = 1ULL << L1, n2 = 1ULL << L2;
+    Scoped scr(c, 8 * n * ncols);
+    NttPass a = base_pass();
+    a.in = coef;
+    a.out = scr.as<uint64_t>();
+    a.nb = (uint32_t)ncols;
+    a.ib = n;
+    a.ob = n;
+    a.ng = n2;
+    a.ig = 1;
+    a.ii = n2;
+    a.og = 1;
+    a.ok = n2;
+    a.log_s = L1;
+    a.tile_log = tl;
+    a.r0 = 1;  // the one repetition, r = 1
+    a.pre_log = L1 + 1;
+    a.pre_i = 1;
+    a.post_log = L + 1;
+    a.post_mul = 2;
+    a.post_rep = 1;
+    a.lazy_out = 1;
+    launch_pass(c, a, "odd_passA");
+    NttPass b = base_pass();
+    b.in = scr.as<uint64_t>();
+    b.out = out;
+    b.nb = (uint32_t)ncols;
+    b.ib = n;
+    b.ob = n;
+    b.ng = n1;
+    b.ig = n2;
+    b.ii = 1;
+    b.og = 1;
+    b.ok = n1;
+    b.log_s = L2;
+    b.tile_log = tl;
+    launch_pass(c, b, "odd_passB");
+}
+
 }  // namespace
 
 const uint64_t *odd_twist_table(Ctx *c, uint32_t L) {
@@ -853,8 +925,21 @@ const uint64_t *odd_twist_table(Ctx *c, uint32_t L) {
 
 void lde_odd_coset(Ctx *c, const uint64_t *cols, uint64_t *out, uint32_t L, uint64_t ncols) {
     const uint64_t n = 1ULL << L;
-    if (L < 1 || L > 12) {
+    if (L < 1) {
         lde_columns(c, cols, out, L, 0, ncols, n, n, root_of_unity(2 * n));
+        return;
+    }
+    if (L > 12) {
+        SG_REQUIRE(L <= 24, "odd coset: trace above 2^24");
+        Scoped coef(c, 8 * n * ncols);
+        {
+            Scoped scr(c, 8 * n * ncols);
+            ntt_columns(c, cols, coef.as<uint64_t>(), scr.as<uint64_t>(), L, ncols, n, n, true, nullptr);
+        }
+        const uint64_t *cf = coef.as<uint64_t>();
+        const uint32_t plan = choose_plan(c, 3, L, 0, ncols, lde_plans(L, 0),
+                                          [&](uint32_t s) { odd_forward(c, cf, out, L, s, ncols); });
+        odd_forward(c, cf, out, L, plan, ncols);
         return;
     }
     const uint64_t blocks = (ncols + (TILE >> L) - 1) >> (TILE_LOG - L);
