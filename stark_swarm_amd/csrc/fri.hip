@@ -39,24 +39,29 @@ namespace {
 constexpr int FT = 256;
 
 // Residue sums S_rho = sum_{k = rho mod M} c_k z^k of proof blockIdx.y.
-// T: per proof, T[i] = z^(2^i) (24 entries). Block b covers [k0, k0 + FT*ch),
-// k0 = b*FT*ch; thread tid reads k = k0 + t*FT + tid (coalesced), all of
-// residue tid mod M, and keeps one accumulator A = sum_t c_k z^(t*FT). The
+// T: per proof, T[i] = z^(2^i) (24 entries). Block b covers [k0, k0 + OT*ch),
+// k0 = b*OT*ch; thread tid reads k = k0 + t*OT + tid (coalesced), all of
+// residue tid mod M, and keeps one accumulator A = sum_t c_k z^(t*OT). The
 // z^tid factors come in as a Horner tree: lanes at distance s combine as
-// v_l + z^s v_{l+s} for s = M..32 inside a wave, then across the FT/64 waves
+// v_l + z^s v_{l+s} for s = M..32 inside a wave, then across the OT/64 waves
 // with z^64; lane rho < M is left with z^-(k0+rho) S_rho(block).
+// One wave per block: the sums of a small layer are chains of Fp6 products
+// (the Horner tree, then z^(k0+rho) bit by bit), and the waves of a larger
+// block were measured to take as long as their chains one after another, so
+// more blocks of one wave each spread the chains over the chip.
+constexpr int OT = 64, LOG_OT = 6;
 template <int M>
-__global__ __launch_bounds__(FT) void ood_sums_k(const uint64_t *coef, uint64_t n, int ncomp, const uint64_t *T,
+__global__ __launch_bounds__(OT) void ood_sums_k(const uint64_t *coef, uint64_t n, int ncomp, const uint64_t *T,
                                                  uint32_t tstride, uint32_t ch, uint64_t *partials) {
-    constexpr int NW = FT / 64;
-    __shared__ uint64_t zt[64 * 6];  // z^(t*FT), t < ch <= 64
+    constexpr int NW = OT / 64;
+    __shared__ uint64_t zt[64 * 6];  // z^(t*OT), t < ch <= 64
     __shared__ uint64_t red[NW][M * 6];
     __shared__ uint64_t tl[24 * 6];  // z^(2^i), i < 24: read once per block, not along each product chain
     const uint64_t pk = blockIdx.y;
     coef += pk * ncomp * n;
     T += pk * 6 * tstride;
     const uint32_t tid = threadIdx.x;
-    if (tid < 24 * 6) tl[tid] = T[tid];
+    for (uint32_t i = tid; i < 24 * 6; i += OT) tl[i] = T[i];
     __syncthreads();
     auto tget = [&](int i) {
         fp6 r;
@@ -67,15 +72,25 @@ __global__ __launch_bounds__(FT) void ood_sums_k(const uint64_t *coef, uint64_t 
     if (tid < ch) {
         fp6 w = f6_one();
         for (int bit = 0; (1u << bit) <= tid; bit++)
-            if (tid >> bit & 1) w = f6_mul(w, tget(8 + bit));  // FT = 2^8
+            if (tid >> bit & 1) w = f6_mul(w, tget(LOG_OT + bit));
 #pragma unroll
         for (int q = 0; q < 6; q++) zt[6 * tid + q] = w.c[q];
     }
     __syncthreads();
-    const uint64_t k0 = (uint64_t)blockIdx.x * FT * ch;
+    const uint64_t k0 = (uint64_t)blockIdx.x * OT * ch;
     fp6 acc = f6_zero();
-    for (uint32_t t = 0; t < ch; t++) {
-        const uint64_t k = k0 + (uint64_t)t * FT + tid;
+    // the first element's factor is z^0 = 1: no product
+    if (k0 + tid < n) {
+        const uint64_t k = k0 + tid;
+        if (ncomp == 1) {
+            acc.c[0] = canon(coef[k]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 6; q++) acc.c[q] = canon(coef[q * n + k]);
+        }
+    }
+    for (uint32_t t = 1; t < ch; t++) {
+        const uint64_t k = k0 + (uint64_t)t * OT + tid;
         if (k >= n) break;
         fp6 zp;
 #pragma unroll
@@ -96,6 +111,48 @@ __global__ __launch_bounds__(FT) void ood_sums_k(const uint64_t *coef, uint64_t 
 #pragma unroll
         for (int q = 0; q < 6; q++) o.c[q] = __shfl_down(acc.c[q], s, 64);
         if ((lane & s) == 0) acc = f6_add(acc, f6_mul(o, tget(bit)));
+    }
+    if constexpr (NW == 1 && M <= 8) {
+        // Lane rho < M now holds z^-(k0+rho) S_rho(block); the scale factor
+        // z^(k0+rho) is applied lane-parallel. Group gi = lane / 8 takes
+        // residue gi; its lane q < 6 keeps slot q of the group's value and
+        // forms that slot of each Fp6 product (six base products, about a
+        // sixth of a whole product's latency), and the group regathers the
+        // six slots by shuffles. Fp6 = Fp[y]/(y^6 - 7); slots 0..5 hold the
+        // coefficients of y^0, y^2, y^4, y^1, y^3, y^5.
+        const uint32_t gi = lane >> 3, q = lane & 7, qs = q < 6 ? q : 5;
+        const uint32_t g = qs < 3 ? 2 * qs : 2 * (qs - 3) + 1;  // exponent of slot qs
+        const uint32_t src = gi < (uint32_t)M ? gi : 0;
+        fp6 a;
+        uint64_t mine = 0;
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            a.c[j] = __shfl(acc.c[j], (int)src, 64);
+            if ((uint32_t)j == qs) mine = a.c[j];
+        }
+        // k0 is a multiple of OT > rho: the bits of k0 + rho are those of k0 and rho
+        const uint64_t e = k0 + src, steps = k0 | (uint64_t)(M - 1);
+        for (int bit = 0; bit < 24; bit++) {
+            if (!(steps >> bit & 1)) continue;  // the same for the whole wave
+            uint64_t sn = 0, sw = 0;
+#pragma unroll
+            for (int ex = 0; ex < 6; ex++) {
+                const int sa = (ex & 1) ? 3 + (ex >> 1) : (ex >> 1);  // slot of y^ex
+                const uint32_t f = g >= (uint32_t)ex ? g - ex : g + 6 - ex;
+                const uint32_t fs = (f & 1) ? 3 + (f >> 1) : (f >> 1);
+                const uint64_t pr = mul(a.c[sa], tl[6 * bit + fs]);
+                // y^ex y^f = 7 y^g when ex + f = g + 6
+                const bool wrap = (uint32_t)ex > g;
+                sn = add(sn, wrap ? 0 : pr);
+                sw = add(sw, wrap ? pr : 0);
+            }
+            const uint64_t r = add(sn, mul7(sw));
+            if (e >> bit & 1) mine = r;
+#pragma unroll
+            for (int j = 0; j < 6; j++) a.c[j] = __shfl(mine, (int)((lane & ~7u) + j), 64);
+        }
+        if (gi < (uint32_t)M && q < 6) partials[(pk * gridDim.x + blockIdx.x) * M * 6 + gi * 6 + q] = mine;
+        return;
     }
     if (lane < (uint32_t)M) {
 #pragma unroll
@@ -139,8 +196,21 @@ __global__ __launch_bounds__(FIN_T) void ood_finish_k(const uint64_t *partials, 
     const uint32_t C = 6 * m, R = FIN_T / C, t = threadIdx.x;
     const uint32_t c = t % C, r = t / C;
     uint64_t s = 0;
-    if (r < R)
-        for (uint32_t b = r; b < nblocks; b += R) s = add(s, partials[(uint64_t)b * C + c]);
+    if (r < R) {
+        // eight partials in flight per step (clamped reads, masked sums)
+        constexpr uint32_t U = 8;
+        for (uint32_t b = r; b < nblocks; b += U * R) {
+            uint64_t x[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                const uint32_t bb = b + u * R;
+                x[u] = partials[(uint64_t)(bb < nblocks ? bb : nblocks - 1) * C + c];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++)
+                if (b + u * R < nblocks) s = add(s, x[u]);
+        }
+    }
     red[t] = s;
     __syncthreads();
     if (t < C) {
@@ -364,12 +434,12 @@ void launch_ood(Ctx *c, const uint64_t *coef, uint64_t n, int ncomp, uint32_t K,
     SG_REQUIRE(n <= (1ULL << 24) && toff + 24 <= tstride, "ood: table too short");
     // elements per thread: enough blocks to fill the chip, at most 64
     uint32_t ch = 64;
-    while (ch > 1 && (n / ((uint64_t)FT * ch)) * K < 2048) ch >>= 1;
-    const uint32_t nblocks = (uint32_t)((n + (uint64_t)FT * ch - 1) / ((uint64_t)FT * ch));
+    while (ch > 1 && (n / ((uint64_t)OT * ch)) * K < 8192) ch >>= 1;
+    const uint32_t nblocks = (uint32_t)((n + (uint64_t)OT * ch - 1) / ((uint64_t)OT * ch));
     Scoped d_part(c, 48ULL * M * nblocks * K);
     {
         ProfScope ps(c, "ood_sums", 8.0 * ncomp * n * K);
-        hipLaunchKernelGGL(ood_sums_k<M>, dim3(nblocks, K), dim3(FT), 0, c->stream, coef, n, ncomp, d_T + 6 * toff,
+        hipLaunchKernelGGL(ood_sums_k<M>, dim3(nblocks, K), dim3(OT), 0, c->stream, coef, n, ncomp, d_T + 6 * toff,
                            tstride, ch, d_part.as<uint64_t>());
         SG_HIP(hipGetLastError());
     }
