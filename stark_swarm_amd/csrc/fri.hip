@@ -352,17 +352,32 @@ __global__ __launch_bounds__(FOLD_T) void coset_fold_k(const uint64_t *f, uint64
         for (int j = 0; j < M; j++) v[j] = f[q * n + b + j * nn];
         idft_small<LM>(v);
         uint64_t s = minv;  // (1/m) w^(-bk)
-        fp6 pq = f6_zero();
+        // pq_i = sum_k alpha^k_i y_k: the M products of a slot are summed as
+        // 192-bit integers (lo, mid, top < 2^4) and reduced once, about a
+        // third of the instructions of a reduced product and add per term
+        uint64_t sl[6], sm[6], st[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) sl[i] = sm[i] = st[i] = 0;
 #pragma unroll
         for (int k = 0; k < M; k++) {
             const uint64_t y = mul(v[k], s);
             if (tuples) tuples[(uint64_t)(k * 6 + q) * nn + b] = y;
-            fp6 ak;
 #pragma unroll
-            for (int i = 0; i < 6; i++) ak.c[i] = apow ? apow[6 * k + i] : pa.v[6 * k + i];
-            pq = f6_add(pq, f6_scale(ak, y));
+            for (int i = 0; i < 6; i++) {
+                const uint64_t ak = apow ? apow[6 * k + i] : pa.v[6 * k + i];
+                uint64_t lo, hi;
+                mul128(ak, y, lo, hi);  // hi <= 2^64 - 2, so hi + carry does not wrap
+                sl[i] += lo;
+                const uint64_t h = hi + (sl[i] < lo);
+                sm[i] += h;
+                st[i] += sm[i] < h;
+            }
             s = mul(s, wb);
         }
+        // lo + mid 2^64 + top 2^128 with 2^128 = -2^32 (mod p), top < 16
+        fp6 pq;
+#pragma unroll
+        for (int i = 0; i < 6; i++) pq.c[i] = sub(reduce128(sl[i], sm[i]), st[i] << 32);
         constexpr int upow[6] = {0, 2, 4, 1, 3, 5};
         f6_add_mul_upow<upow[q]>(acc, pq);
     };
