@@ -174,14 +174,26 @@ __global__ __launch_bounds__(256) void expand_sha256_k(const RowWords *words, co
 
 // ---- constraints (sha256_air.rs:956-1294), one LDE point per lane ----
 // phi accumulator: slot w - 1 weighted by lam[w - 1], or by w = j + 1 without a table
+// The weighted terms are summed as a 192-bit integer (lo, mid, top; each
+// product is below 2^128, so top counts at most one carry per term) and
+// reduced once by value(): a 64x64 product and a three-word add per term
+// instead of a reduced product and a modular add.
 struct Phi {
-    uint64_t acc = 0, w = 1;
+    uint64_t lo = 0, mid = 0, w = 1;
+    uint32_t top = 0;
     const uint64_t *lam = nullptr;
     __device__ __forceinline__ uint64_t weight() const { return lam ? lam[w - 1] : w; }
     __device__ __forceinline__ void put(uint64_t v) {
-        acc = add(acc, mul(weight(), v));
+        uint64_t pl, ph;
+        mul128(weight(), v, pl, ph);  // ph <= 2^64 - 2: ph + carry does not wrap
+        lo += pl;
+        const uint64_t h = ph + (lo < pl);
+        mid += h;
+        top += mid < h;
         w++;
     }
+    // lo + mid 2^64 + top 2^128, with 2^128 = -2^32 (mod p)
+    __device__ __forceinline__ uint64_t value() const { return sub(reduce128(lo, mid), (uint64_t)top << 32); }
     __device__ __forceinline__ void skip(int k) { w += k; }
 };
 
@@ -405,30 +417,21 @@ __global__ __launch_bounds__(AT) void compose_sha256_k(const uint64_t *lde, cons
         f.w = gs.w;
     }
     // the SP partial sums of the point
-    red[0][part][lane] = f.acc;
-    red[1][part][lane] = fc.acc;
-    red[2][part][lane] = ff.acc;
-    red[3][part][lane] = fn.acc;
-    red[4][part][lane] = fs.acc;
+    uint64_t sv[NG] = {f.value(), fc.value(), ff.value(), fn.value(), fs.value()};  // G_ALL .. G_SCHED
+#pragma unroll
+    for (int g = 0; g < NG; g++) red[g][part][lane] = sv[g];
     __syncthreads();
     if (part != 0) return;
 #pragma unroll
-    for (int q = 1; q < SP; q++) {
-        f.acc = add(f.acc, red[0][q][lane]);
-        fc.acc = add(fc.acc, red[1][q][lane]);
-        ff.acc = add(ff.acc, red[2][q][lane]);
-        fn.acc = add(fn.acc, red[3][q][lane]);
-        fs.acc = add(fs.acc, red[4][q][lane]);
-    }
+    for (int q = 1; q < SP; q++)
+#pragma unroll
+        for (int g = 0; g < NG; g++) sv[g] = add(sv[g], red[g][q][lane]);
     if (GROUPS) {
         uint64_t *o = phi + (uint64_t)blockIdx.y * NG * n + i;
-        o[G_ALL * n] = f.acc;
-        o[G_COMP * n] = fc.acc;
-        o[G_FIN * n] = ff.acc;
-        o[G_NOTFIN * n] = fn.acc;
-        o[G_SCHED * n] = fs.acc;
+#pragma unroll
+        for (int g = 0; g < NG; g++) o[g * n] = sv[g];
     } else {
-        phi[n * blockIdx.y + i] = f.acc;
+        phi[n * blockIdx.y + i] = sv[0];
     }
 }
 
