@@ -67,6 +67,22 @@ void Ctx::d2h(void *dst, const void *src, size_t n) {
     }
 }
 
+const void *Ctx::d2h_pinned(const void *src, size_t n) {
+    if (pin_bytes < n) {
+        if (pin) (void)hipHostFree(pin);
+        pin = nullptr;
+        pin_bytes = 0;
+        const size_t sz = n < (1ULL << 20) ? (1ULL << 20) : n + n / 4;
+        SG_HIP(hipHostMalloc(&pin, sz, hipHostMallocDefault));
+        pin_bytes = sz;
+    }
+    if (n) {
+        SG_HIP(hipMemcpyAsync(pin, src, n, hipMemcpyDeviceToHost, stream));
+        SG_HIP(hipStreamSynchronize(stream));
+    }
+    return pin;
+}
+
 void Ctx::d2h2(void *dst1, const void *src1, size_t n1, void *dst2, const void *src2, size_t n2) {
     constexpr size_t SMALL = 1ULL << 20;  // the pinned buffer's minimum size
     if (n1 + n2 > SMALL) {

@@ -105,6 +105,9 @@ struct Ctx {
     // copy n bytes of device memory to host memory after the stream's
     // earlier work; returns once dst holds them
     void d2h(void *dst, const void *src, size_t n);
+    // n bytes from device src into the context's pinned buffer, behind one
+    // wait; the pointer stays valid until the next d2h* call on the context
+    const void *d2h_pinned(const void *src, size_t n);
     // two such copies behind one wait (small reads of one stage, e.g. a
     // layer's coset evaluations and its roots)
     void d2h2(void *dst1, const void *src1, size_t n1, void *dst2, const void *src2, size_t n2);

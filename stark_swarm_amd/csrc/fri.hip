@@ -738,7 +738,7 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_classic(Ctx *c, const uint64_t 
         }
     // payloads (f, s, q) of every query and layer, gathered from the leaf planes
     // every layer's payload gather and every tree's opening digests behind one wait
-    std::vector<std::vector<uint64_t>> pay(L);
+    std::vector<HostBuf> pay(L);
     std::vector<std::vector<std::vector<std::vector<uint8_t>>>> lop(L);
     std::vector<std::vector<std::vector<uint8_t>>> f0op;
     {
@@ -748,10 +748,12 @@ std::vector<std::vector<uint8_t>> deep_fri_prove_classic(Ctx *c, const uint64_t 
             const uint64_t n = sz[ell];
             GatherJob &g = jobs[ell];
             g.src = planes[ell];
-            g.idx.reserve((size_t)K * r * 18);
-            for (uint32_t k = 0; k < K; k++)
+            g.idx.resize((size_t)K * r * 18);
+            parallel_for(K, [&](size_t k) {
+                uint64_t *w = &g.idx[k * r * 18];
                 for (uint64_t q = 0; q < r; q++)
-                    for (uint64_t j = 0; j < 18; j++) g.idx.push_back(k * 18 * n + j * n + pos[ell][k][q]);
+                    for (uint64_t j = 0; j < 18; j++) *w++ = k * 18 * n + j * n + pos[ell][k][q];
+            });
             jobs[L + ell] = tree_open_job(trees[ell], pos[ell]);
         }
         jobs[2 * L] = tree_open_job(t0, pos[0]);

@@ -399,13 +399,11 @@ __global__ __launch_bounds__(KT) void merkle_nodes_k(const uint64_t *child, uint
 // merkle_nodes_k on two lanes per node (keccak_f_half): lanes 2i and 2i + 1
 // of the grid hash node i. A pair past n_parent still runs the permutation
 // (its partner reads come from it) but loads and stores nothing.
-__global__ __launch_bounds__(KT) void merkle_nodes2_k(const uint64_t *child, uint64_t n_child, uint64_t n_parent,
-                                                      uint32_t arity, uint32_t level, uint64_t label, uint64_t *out) {
-    const uint64_t i = ((uint64_t)blockIdx.x * KT + threadIdx.x) >> 1;
-    const uint32_t h = threadIdx.x & 1;
-    const bool live = i < n_parent;
-    const uint64_t k = blockIdx.y;
-    const uint32_t *ch32 = (const uint32_t *)(child + 4 * k * n_child);
+// One lane's half of a node digest: the node's four header lanes and its
+// arity children's digests (ch32: the child level of this tree as u32
+// halves, global or LDS) absorbed by the two-lane permutation.
+__device__ __forceinline__ void node2_hash(const uint32_t *ch32, uint64_t n_child, uint64_t i, bool live, uint32_t h,
+                                           uint32_t arity, uint32_t level, uint64_t label, uint32_t dig[4]) {
     // 64-bit message lane t, this lane's half
     auto word = [&](uint32_t t) -> uint32_t {
         if (t < 4) {
@@ -437,12 +435,61 @@ __global__ __launch_bounds__(KT) void merkle_nodes2_k(const uint64_t *child, uin
     }
     st[16] ^= h ? 0x80000000u : 0u;
     keccak_f_half(st, h);
+#pragma unroll
+    for (int j = 0; j < 4; j++) dig[j] = st[j];
+}
+
+__global__ __launch_bounds__(KT) void merkle_nodes2_k(const uint64_t *child, uint64_t n_child, uint64_t n_parent,
+                                                      uint32_t arity, uint32_t level, uint64_t label, uint64_t *out) {
+    const uint64_t i = ((uint64_t)blockIdx.x * KT + threadIdx.x) >> 1;
+    const uint32_t h = threadIdx.x & 1;
+    const bool live = i < n_parent;
+    const uint64_t k = blockIdx.y;
+    uint32_t d[4];
+    node2_hash((const uint32_t *)(child + 4 * k * n_child), n_child, i, live, h, arity, level, label, d);
     if (!live) return;
     uint32_t *o = (uint32_t *)(out + 4 * (k * n_parent + i));
-    o[h] = st[0];
-    o[2 + h] = st[1];
-    o[4 + h] = st[2];
-    o[6 + h] = st[3];
+    o[h] = d[0];
+    o[2 + h] = d[1];
+    o[4 + h] = d[2];
+    o[6 + h] = d[3];
+}
+
+// The top levels of a tree, each of at most TOP_NODES nodes, in one launch:
+// one workgroup per tree of the batch, two lanes per node, the levels one
+// after another behind a barrier. Each level is written to the tree buffer
+// and kept in LDS, where the next level reads it. Saves the launch gap
+// between dependent levels (several microseconds each) on small trees.
+constexpr uint32_t TOP_NODES = 128, TOP_MAX_LEVELS = 16;
+struct TopPlan {
+    uint32_t nlev;                      // levels computed: level0 .. level0 + nlev - 1
+    uint32_t level0;
+    uint64_t n[TOP_MAX_LEVELS + 1];     // n[s]: nodes of level level0 - 1 + s (one tree)
+    uint64_t off[TOP_MAX_LEVELS + 1];   // its digest offset in the tree buffer (all trees)
+};
+__global__ __launch_bounds__(2 * TOP_NODES) void merkle_top2_k(uint64_t *buf, TopPlan tp, uint32_t arity,
+                                                              uint64_t label) {
+    __shared__ uint32_t lev[2][8 * TOP_NODES];
+    const uint64_t k = blockIdx.y;
+    const uint32_t h = threadIdx.x & 1;
+    const uint64_t i = threadIdx.x >> 1;
+    for (uint32_t s = 0; s < tp.nlev; s++) {
+        const uint64_t n_child = tp.n[s], n_parent = tp.n[s + 1];
+        const bool live = i < n_parent;
+        const uint32_t *ch32 = s == 0 ? (const uint32_t *)(buf + 4 * (tp.off[0] + k * n_child)) : lev[(s - 1) & 1];
+        uint32_t d[4];
+        node2_hash(ch32, n_child, i, live, h, arity, tp.level0 + s, label, d);
+        if (live) {
+            uint32_t *o = (uint32_t *)(buf + 4 * (tp.off[s + 1] + k * n_parent + i));
+            uint32_t *l = lev[s & 1] + 8 * i;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                o[2 * j + h] = d[j];
+                l[2 * j + h] = d[j];
+            }
+        }
+        __syncthreads();
+    }
 }
 
 __global__ void gather_k(const uint64_t *src, const uint64_t *idx, uint64_t n, uint32_t width, uint64_t *out) {
@@ -752,7 +799,10 @@ void tree_build(Ctx *c, DevTree &t, const LeafSrc &src, uint64_t n_leaves, uint3
                            src, n_leaves, arity, label, t.buf);
         SG_HIP(hipGetLastError());
     }
-    for (size_t l = 1; l < t.real.size(); l++) {
+    // the first level from which every level has at most TOP_NODES nodes
+    size_t top = t.real.size();
+    while (top > 1 && t.real[top - 1] <= TOP_NODES) top--;
+    for (size_t l = 1; l < top; l++) {
         const uint64_t np = t.real[l];
         ProfScope ps(c, "merkle_nodes", (32.0 * t.real[l - 1] + 32.0 * np) * batch);
         // levels too small to fill the chip one lane per node take two lanes
@@ -765,6 +815,21 @@ void tree_build(Ctx *c, DevTree &t, const LeafSrc &src, uint64_t n_leaves, uint3
             hipLaunchKernelGGL(merkle_nodes_k, dim3((uint32_t)((np + KT - 1) / KT), batch), dim3(KT), 0, c->stream,
                                t.buf + 4 * t.off[l - 1], t.real[l - 1], np, arity, (uint32_t)l, label,
                                t.buf + 4 * t.off[l]);
+        SG_HIP(hipGetLastError());
+    }
+    if (top < t.real.size()) {
+        TopPlan tp{};
+        tp.nlev = (uint32_t)(t.real.size() - top);
+        SG_REQUIRE(tp.nlev <= TOP_MAX_LEVELS, "merkle: too many top levels");
+        tp.level0 = (uint32_t)top;
+        for (uint32_t s2 = 0; s2 <= tp.nlev; s2++) {
+            tp.n[s2] = t.real[top - 1 + s2];
+            tp.off[s2] = t.off[top - 1 + s2];
+        }
+        double bytes = 0;
+        for (size_t l = top; l < t.real.size(); l++) bytes += 32.0 * (t.real[l - 1] + t.real[l]) * batch;
+        ProfScope ps(c, "merkle_nodes", bytes);
+        hipLaunchKernelGGL(merkle_top2_k, dim3(1, batch), dim3(2 * TOP_NODES), 0, c->stream, t.buf, tp, arity, label);
         SG_HIP(hipGetLastError());
     }
     if (read_roots) tree_read_roots(c, t);
@@ -798,37 +863,48 @@ void gather_u64_w(Ctx *c, const uint64_t *src, const uint64_t *d_idx, uint64_t n
 }
 
 void gather_set(Ctx *c, std::vector<GatherJob> &jobs) {
-    size_t ni = 0, no = 0;
-    for (const GatherJob &j : jobs) {
-        ni += j.idx.size();
-        no += j.idx.size() * j.width;
+    const size_t J = jobs.size();
+    std::vector<size_t> ioff(J + 1, 0), ooff(J + 1, 0);
+    for (size_t j = 0; j < J; j++) {
+        ioff[j + 1] = ioff[j] + jobs[j].idx.size();
+        ooff[j + 1] = ooff[j] + jobs[j].idx.size() * jobs[j].width;
     }
+    const size_t ni = ioff[J], no = ooff[J];
     if (!ni) {
-        for (GatherJob &j : jobs) j.out.clear();
+        for (GatherJob &j : jobs) j.out.reset(0);
         return;
     }
-    std::vector<uint64_t> all(ni);
-    size_t oi = 0;
-    for (const GatherJob &j : jobs) {
-        std::copy(j.idx.begin(), j.idx.end(), all.begin() + oi);
-        oi += j.idx.size();
-    }
+    // pieces of at most 2^16 u64 (512 KiB), copied on the host's threads
+    struct Piece {
+        size_t job, off, n;
+    };
+    auto pieces = [&](const std::vector<size_t> &off) {
+        std::vector<Piece> v;
+        for (size_t j = 0; j < J; j++)
+            for (size_t a = off[j]; a < off[j + 1]; a += (size_t)1 << 16)
+                v.push_back({j, a - off[j], std::min(off[j + 1] - a, (size_t)1 << 16)});
+        return v;
+    };
+    // every index list straight into pinned memory, one upload
+    uint64_t *up = (uint64_t *)c->upload_staging(8 * ni);
+    const std::vector<Piece> pin_in = pieces(ioff);
+    parallel_for(pin_in.size(), [&](size_t i) {
+        const Piece &q = pin_in[i];
+        memcpy(up + ioff[q.job] + q.off, jobs[q.job].idx.data() + q.off, 8 * q.n);
+    });
     Scoped d_idx(c, 8 * ni), d_out(c, 8 * no);
-    SG_HIP(hipMemcpyAsync(d_idx.p, all.data(), 8 * ni, hipMemcpyHostToDevice, c->stream));
-    oi = 0;
-    size_t oo = 0;
-    for (const GatherJob &j : jobs) {
-        gather_u64_w(c, j.src, d_idx.as<uint64_t>() + oi, j.idx.size(), j.width, d_out.as<uint64_t>() + oo);
-        oi += j.idx.size();
-        oo += j.idx.size() * j.width;
-    }
-    std::vector<uint64_t> h(no);
-    c->d2h(h.data(), d_out.p, 8 * no);
-    oo = 0;
-    for (GatherJob &j : jobs) {
-        j.out.assign(h.begin() + oo, h.begin() + oo + j.idx.size() * j.width);
-        oo += j.idx.size() * j.width;
-    }
+    SG_HIP(hipMemcpyAsync(d_idx.p, up, 8 * ni, hipMemcpyHostToDevice, c->stream));
+    for (size_t j = 0; j < J; j++)
+        gather_u64_w(c, jobs[j].src, d_idx.as<uint64_t>() + ioff[j], jobs[j].idx.size(), jobs[j].width,
+                     d_out.as<uint64_t>() + ooff[j]);
+    // one download into pinned memory, copied out to the jobs in parallel
+    const uint64_t *h = (const uint64_t *)c->d2h_pinned(d_out.p, 8 * no);
+    for (GatherJob &j : jobs) j.out.reset(j.idx.size() * j.width);
+    const std::vector<Piece> pin_out = pieces(ooff);
+    parallel_for(pin_out.size(), [&](size_t i) {
+        const Piece &q = pin_out[i];
+        memcpy(jobs[q.job].out.p.get() + q.off, h + ooff[q.job] + q.off, 8 * q.n);
+    });
 }
 
 // digests an opening needs, per query: the leaf, then every level's full

@@ -6,6 +6,7 @@
 
 #include <array>
 #include <functional>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -94,13 +95,30 @@ void tree_open_multi(Ctx *c, const DevTree &t, const std::vector<std::vector<uin
                      std::vector<std::vector<std::vector<uint8_t>>> &out);
 void tree_open_batch(Ctx *c, const DevTree &t, const std::vector<uint64_t> &idx, std::vector<std::vector<uint8_t>> &out);
 
+// A host array of u64 that is not zero-filled when sized (gather_set writes
+// every element).
+struct HostBuf {
+    std::unique_ptr<uint64_t[]> p;
+    size_t n = 0;
+    void reset(size_t m) {
+        p.reset(m ? new uint64_t[m] : nullptr);
+        n = m;
+    }
+    uint64_t &operator[](size_t i) { return p[i]; }
+    const uint64_t &operator[](size_t i) const { return p[i]; }
+    const uint64_t *data() const { return p.get(); }
+    const uint64_t *begin() const { return p.get(); }
+    const uint64_t *end() const { return p.get() + n; }
+    size_t size() const { return n; }
+};
+
 // Several gathers behind one index upload and one wait: job j reads `width`
 // u64 per index from src (device; row r at src + r * width) into out.
 struct GatherJob {
     const uint64_t *src = nullptr;
     uint32_t width = 1;
     std::vector<uint64_t> idx;  // host row indices
-    std::vector<uint64_t> out;  // filled by gather_set: idx.size() * width
+    HostBuf out;                // filled by gather_set: idx.size() * width
 };
 void gather_set(Ctx *c, std::vector<GatherJob> &jobs);
 // tree_open_multi in two halves around a gather_set: the digest gather of the

@@ -617,7 +617,7 @@ std::vector<uint8_t> prover_core_dev(Ctx *c, const uint64_t *d_trace, uint64_t w
     const std::vector<std::vector<uint64_t>> ridx = {raw};
     jobs[1] = tree_open_job(rows, ridx);
     gather_set(c, jobs);
-    const std::vector<uint64_t> &vals = jobs[0].out;
+    const HostBuf &vals = jobs[0].out;
     std::vector<std::vector<std::vector<uint8_t>>> opk;
     tree_open_finish(rows, ridx, jobs[1], opk);
     const std::vector<std::vector<uint8_t>> &ops = opk[0];
