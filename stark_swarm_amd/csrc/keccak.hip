@@ -669,13 +669,17 @@ void dns_commit_batch(Ctx *c, const RecordSource &src, const uint8_t salt[16],
         blk_off[r] = total_blocks;
         total_blocks += nblk[r];
     }
-    std::vector<uint64_t> msg(17 * total_blocks, 0);
+    // the padded sponge blocks of every record, written (zeros included) on
+    // the host's threads straight into pinned memory for one upload
+    const size_t msg_words = 17 * total_blocks;
+    uint64_t *msg = (uint64_t *)c->upload_staging(8 * msg_words);
     {
         HostScope hm(c, "dns_messages");
         parallel_for(K, [&](size_t k) {
             for (size_t i = 0; i < src.counts[k]; i++) {
                 const size_t r = first[k] + i;
                 uint8_t *p = (uint8_t *)&msg[17 * blk_off[r]];
+                memset(p, 0, 136 * (size_t)nblk[r]);
                 memcpy(p, "DNS-LEAF-V1", 11);
                 memcpy(p + 11, salt, 16);
                 const size_t cl = src.write(k, i, p + 27);
@@ -693,13 +697,13 @@ void dns_commit_batch(Ctx *c, const RecordSource &src, const uint8_t salt[16],
         prefix2[4] &= 0xFFFF;  // bytes 32..33
     }
     // level sizes of one shard's tree; all trees of a batch share nothing but the launch
-    Scoped d_msg(c, 8 * msg.size()), d_off(c, 8 * nrec), d_nb(c, 4 * nrec), d_pre(c, 40), d_leaf(c, 32 * nrec);
-    SG_HIP(hipMemcpyAsync(d_msg.p, msg.data(), 8 * msg.size(), hipMemcpyHostToDevice, c->stream));
+    Scoped d_msg(c, 8 * msg_words), d_off(c, 8 * nrec), d_nb(c, 4 * nrec), d_pre(c, 40), d_leaf(c, 32 * nrec);
+    SG_HIP(hipMemcpyAsync(d_msg.p, msg, 8 * msg_words, hipMemcpyHostToDevice, c->stream));
     SG_HIP(hipMemcpyAsync(d_off.p, blk_off.data(), 8 * nrec, hipMemcpyHostToDevice, c->stream));
     SG_HIP(hipMemcpyAsync(d_nb.p, nblk.data(), 4 * nrec, hipMemcpyHostToDevice, c->stream));
     SG_HIP(hipMemcpyAsync(d_pre.p, prefix2, 40, hipMemcpyHostToDevice, c->stream));
     {
-        ProfScope ps(c, "dns_leaf", 8.0 * msg.size() + 32.0 * nrec);
+        ProfScope ps(c, "dns_leaf", 8.0 * msg_words + 32.0 * nrec);
         hipLaunchKernelGGL(dns_leaf_k, dim3((uint32_t)((nrec + KT - 1) / KT)), dim3(KT), 0, c->stream,
                            d_msg.as<uint64_t>(), d_off.as<uint64_t>(), d_nb.as<uint32_t>(), (uint64_t)nrec,
                            d_pre.as<uint64_t>(), d_leaf.as<uint64_t>());
