@@ -1387,13 +1387,26 @@ static Native native_of(const EdSig &s, const Layout &L) {
 // the words of every row of one proof, word-major (w[i * h + row]); false if
 // a mul gadget's carry chain did not close (a value outside the field's range)
 static bool proof_words(const EdSig &s, const Layout &L, const Native &N, uint64_t *w, uint32_t r0, uint32_t r1) {
+    // Word i of row r lives at w[i * h + r] (rows fastest, as the expansion
+    // kernel reads them). A row's words are emitted into a tile of 64 rows
+    // held in cache (the same layout with stride 64) and each word's run of
+    // rows is then copied out in one piece, instead of every row writing one
+    // u64 into each of its nw cache lines of the destination.
+    constexpr uint32_t TR = 64;
+    thread_local std::vector<uint64_t> tile;
+    tile.resize((size_t)L.nw * TR);
     bool bad = false;
-    for (uint32_t r = r0; r < r1; r++) {
-        const uint8_t rt = L.rtype[r];
-        Em e;
-        e.w = w + r;
-        e.ws = L.h;
-        emit_row(e, L, rt, r, N, &bad);
+    for (uint32_t rb = r0; rb < r1; rb += TR) {
+        const uint32_t nr = std::min(TR, r1 - rb);
+        uint32_t nw = 0;
+        for (uint32_t r = rb; r < rb + nr; r++) {
+            Em e;
+            e.w = tile.data() + (r - rb);
+            e.ws = TR;
+            emit_row(e, L, L.rtype[r], r, N, &bad);
+            nw = std::max(nw, e.nw);
+        }
+        for (uint32_t i = 0; i < nw; i++) memcpy(w + (size_t)i * L.h + rb, tile.data() + (size_t)i * TR, 8 * nr);
     }
     (void)s;
     return !bad;
@@ -1863,13 +1876,16 @@ void build_ed25519_traces_dev(Ctx *c, const std::vector<EdSig> &sigs, uint32_t k
     uint64_t *cw = words + G * per;
     std::vector<Native> nat(G);
     std::vector<std::string> err(G);
-    parallel_for(G, [&](size_t g) {
-        try {
-            nat[g] = native_of(sigs[g], L);
-        } catch (const std::exception &e) {
-            err[g] = e.what();
-        }
-    });
+    {
+        HostScope hs(c, "ed25519_native");
+        parallel_for(G, [&](size_t g) {
+            try {
+                nat[g] = native_of(sigs[g], L);
+            } catch (const std::exception &e) {
+                err[g] = e.what();
+            }
+        });
+    }
     for (size_t g = 0; g < G; g++)
         if (!err[g].empty()) throw Error(E_INVALID, "ed25519 traces: signature " + std::to_string(g) + ": " + err[g]);
     if (consts) {
@@ -1883,11 +1899,14 @@ void build_ed25519_traces_dev(Ctx *c, const std::vector<EdSig> &sigs, uint32_t k
     // rows in chunks of 64 across the batch
     const uint32_t chunks = (L.h + 63) / 64;
     std::vector<uint8_t> bad(G * chunks, 0);
-    parallel_for(G * chunks, [&](size_t t) {
-        const size_t g = t / chunks;
-        const uint32_t r0 = (uint32_t)(t % chunks) * 64, r1 = std::min(L.h, r0 + 64);
-        bad[t] = !proof_words(sigs[g], L, nat[g], words + g * per, r0, r1);
-    });
+    {
+        HostScope hs(c, "ed25519_words");
+        parallel_for(G * chunks, [&](size_t t) {
+            const size_t g = t / chunks;
+            const uint32_t r0 = (uint32_t)(t % chunks) * 64, r1 = std::min(L.h, r0 + 64);
+            bad[t] = !proof_words(sigs[g], L, nat[g], words + g * per, r0, r1);
+        });
+    }
     for (size_t t = 0; t < bad.size(); t++)
         SG_REQUIRE(!bad[t], "ed25519 trace: a field product does not close");
     for (size_t g = 0; g < G; g++) memcpy(cw + g * L.ncw, nat[g].cw, 8 * L.ncw);

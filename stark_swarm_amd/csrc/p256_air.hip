@@ -1426,7 +1426,10 @@ void build_p256_traces_dev(Ctx *c, const std::vector<P256Chains> &in, uint64_t h
     // every word is written below, straight into pinned memory
     uint64_t *words = (uint64_t *)c->upload_staging(8 * K * per);
     std::vector<std::array<Be32, 6>> rp;
-    batch_words(in, h, words, rp);
+    {
+        HostScope hs(c, "p256_words");
+        batch_words(in, h, words, rp);
+    }
     if (rproj_out) *rproj_out = rp;
     Scoped d_words(c, 8 * K * per);
     SG_HIP(hipMemcpyAsync(d_words.p, words, 8 * K * per, hipMemcpyHostToDevice, c->stream));
