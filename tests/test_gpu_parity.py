@@ -116,8 +116,11 @@ def test_merge_matches_oracle(S, lg, blowup, phi_mode):
 
 
 # ---------------------------------------------------------------- Merkle
+# 256 / 258 leaves of arity 2: the first node level has exactly 128 nodes (hashed
+# with the tree's top in one launch) or 129 (hashed alone, the top after it)
 @pytest.mark.parametrize("n,vpl,arity,label", [(1, 3, 2, 0), (7, 1, 2, 3), (64, 6, 8, 1), (100, 8, 16, 0xFF),
-                                               (1000, 13, 4, 7), (4096, 6, 8, 2), (33, 48, 16, 0xFE)])
+                                               (1000, 13, 4, 7), (4096, 6, 8, 2), (33, 48, 16, 0xFE),
+                                               (256, 2, 2, 5), (258, 3, 2, 9)])
 def test_merkle_matches_oracle(S, n, vpl, arity, label):
     vals = rnd(n + vpl, (n, vpl))
     import ctypes
