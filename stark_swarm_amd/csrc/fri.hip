@@ -45,10 +45,9 @@ constexpr int FT = 256;
 // z^tid factors come in as a Horner tree: lanes at distance s combine as
 // v_l + z^s v_{l+s} for s = M..32 inside a wave, then across the OT/64 waves
 // with z^64; lane rho < M is left with z^-(k0+rho) S_rho(block).
-// One wave per block: the sums of a small layer are chains of Fp6 products
-// (the Horner tree, then z^(k0+rho) bit by bit), and the waves of a larger
-// block were measured to take as long as their chains one after another, so
-// more blocks of one wave each spread the chains over the chip.
+// One wave per block: a small layer's block time is its chain of Fp6
+// products (the Horner tree, then the scale by z^(k0+rho), which runs
+// lane-parallel below), and one-wave blocks need no cross-wave Horner step.
 constexpr int OT = 64, LOG_OT = 6;
 template <int M>
 __global__ __launch_bounds__(OT) void ood_sums_k(const uint64_t *coef, uint64_t n, int ncomp, const uint64_t *T,
