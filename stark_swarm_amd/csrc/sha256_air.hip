@@ -522,10 +522,12 @@ void build_sha256_traces_dev(Ctx *c, const std::vector<ByteSpan> &msgs, uint64_t
         useful[k] = (uint32_t)rows[k].size();
         total += rows[k].size();
     }
-    std::vector<sha::RowWords> all(total);
-    parallel_for(K, [&](size_t k) { std::copy(rows[k].begin(), rows[k].end(), all.begin() + off[k]); });
+    // the row words of every proof copied on the host's threads straight into
+    // pinned memory (one DMA upload, no zero-filled staging vector)
+    sha::RowWords *all = (sha::RowWords *)c->upload_staging(sizeof(sha::RowWords) * total);
+    parallel_for(K, [&](size_t k) { std::copy(rows[k].begin(), rows[k].end(), all + off[k]); });
     Scoped d_w(c, sizeof(sha::RowWords) * total), d_off(c, 8 * K), d_u(c, 4 * K);
-    SG_HIP(hipMemcpyAsync(d_w.p, all.data(), sizeof(sha::RowWords) * total, hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipMemcpyAsync(d_w.p, all, sizeof(sha::RowWords) * total, hipMemcpyHostToDevice, c->stream));
     SG_HIP(hipMemcpyAsync(d_off.p, off.data(), 8 * K, hipMemcpyHostToDevice, c->stream));
     SG_HIP(hipMemcpyAsync(d_u.p, useful.data(), 4 * K, hipMemcpyHostToDevice, c->stream));
     ProfScope ps(c, "expand_sha256", 8.0 * SHA256_W * h * K);

@@ -574,10 +574,12 @@ void build_sha512_traces_into(Ctx *c, const std::vector<ByteSpan> &msgs, uint64_
         useful[k] = (uint32_t)rows[k].size();
         total += rows[k].size();
     }
-    std::vector<sha5::RowWords> all(total);
-    parallel_for(K, [&](size_t k) { std::copy(rows[k].begin(), rows[k].end(), all.begin() + off[k]); });
+    // the row words of every proof copied on the host's threads straight into
+    // pinned memory (one DMA upload, no zero-filled staging vector)
+    sha5::RowWords *all = (sha5::RowWords *)c->upload_staging(sizeof(sha5::RowWords) * total);
+    parallel_for(K, [&](size_t k) { std::copy(rows[k].begin(), rows[k].end(), all + off[k]); });
     Scoped d_w(c, sizeof(sha5::RowWords) * total), d_off(c, 8 * K), d_u(c, 4 * K);
-    SG_HIP(hipMemcpyAsync(d_w.p, all.data(), sizeof(sha5::RowWords) * total, hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipMemcpyAsync(d_w.p, all, sizeof(sha5::RowWords) * total, hipMemcpyHostToDevice, c->stream));
     SG_HIP(hipMemcpyAsync(d_off.p, off.data(), 8 * K, hipMemcpyHostToDevice, c->stream));
     SG_HIP(hipMemcpyAsync(d_u.p, useful.data(), 4 * K, hipMemcpyHostToDevice, c->stream));
     ProfScope ps(c, "expand_sha512", 8.0 * SHA512_W * h * K);
