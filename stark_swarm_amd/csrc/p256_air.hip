@@ -1329,7 +1329,7 @@ void p256_pk_hash_pre(const P256Record &rec, uint8_t out[32]) {
     h.final(out);
 }
 
-P256Chains p256_ecdsa_chains(const P256Record &rec) {
+P256Chains p256_ecdsa_chains(const P256Record &rec, bool verify) {
     uint32_t r[8], s[8], qx[8], qy[8], e[8], bc[8];
     be_to_words(rec.r.data(), r);
     be_to_words(rec.s.data(), s);
@@ -1362,20 +1362,22 @@ P256Chains p256_ecdsa_chains(const P256Record &rec) {
     invmod(s, N_W, w);
     mulmod(e, w, N_W, u1);
     mulmod(r, w, N_W, u2);
-    Pt G{}, Q{}, R;
-    be_to_words(GX_BE, G.x);
-    be_to_words(GY_BE, G.y);
-    G.z[0] = 1;
-    memcpy(Q.x, qx, 32);
-    memcpy(Q.y, qy, 32);
-    Q.z[0] = 1;
-    pt_mul2(G, u1, Q, u2, bc, R);
-    SG_REQUIRE(!zero8(R.z), "ecdsa: signature does not verify");
-    uint32_t zi[8], x[8];
-    invmod(R.z, P_W, zi);
-    mulmod(R.x, zi, P_W, x);
-    if (cmp8(x, N_W) >= 0) submod(x, N_W, P_W, x);  // x < p < 2n
-    SG_REQUIRE(cmp8(x, r) == 0, "ecdsa: signature does not verify");
+    if (verify) {
+        Pt G{}, Q{}, R;
+        be_to_words(GX_BE, G.x);
+        be_to_words(GY_BE, G.y);
+        G.z[0] = 1;
+        memcpy(Q.x, qx, 32);
+        memcpy(Q.y, qy, 32);
+        Q.z[0] = 1;
+        pt_mul2(G, u1, Q, u2, bc, R);
+        SG_REQUIRE(!zero8(R.z), "ecdsa: signature does not verify");
+        uint32_t zi[8], x[8];
+        invmod(R.z, P_W, zi);
+        mulmod(R.x, zi, P_W, x);
+        if (cmp8(x, N_W) >= 0) submod(x, N_W, P_W, x);  // x < p < 2n
+        SG_REQUIRE(cmp8(x, r) == 0, "ecdsa: signature does not verify");
+    }
     P256Chains c;
     Be32 one{};
     one[31] = 1;

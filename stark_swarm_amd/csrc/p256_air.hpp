@@ -76,7 +76,9 @@ void merge_p256_from_trace_batch(Ctx *c, const uint64_t *d_tr, uint32_t K, uint6
 // from Q with those of u2, 256 steps each, r_proj bound to the outputs.
 // Throws E_INVALID if the key is not a curve point below p or the signature
 // does not verify (the example asserts native_ecdsa_verify first).
-P256Chains p256_ecdsa_chains(const P256Record &rec);
+// verify = false skips the signature check itself (a double scalar
+// multiplication); the range and curve checks and u1, u2 always run
+P256Chains p256_ecdsa_chains(const P256Record &rec, bool verify = true);
 // SHA3-256("ECDSAP256-MIXEDZONE" || qx || qy || r || s || digest), the
 // record's public-input hash (mixed_algorithm_zone_rollup.rs:293-302)
 void p256_pk_hash_pre(const P256Record &rec, uint8_t out[32]);
