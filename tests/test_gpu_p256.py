@@ -140,6 +140,10 @@ def test_prove_ecdsa_rejects_bad_records(S):
         S.prove_record_ecdsa_batch([S.EcdsaRecord(qx, qy, r, (s + 1) % N, dg)])
     with pytest.raises(S.StarkError):
         S.prove_record_ecdsa_batch([S.EcdsaRecord(qx, (qy + 1) % FP, r, s, dg)])
+    # the signature checks run beside the proof pipeline: a batch with one
+    # record that does not verify still fails, naming that record
+    with pytest.raises(S.StarkError, match="record 1"):
+        S.prove_record_ecdsa_batch([S.EcdsaRecord(qx, qy, r, s, dg), S.EcdsaRecord(qx, qy, r, (s + 1) % N, dg)])
 
 
 def test_prove_p256_chains_low_degree_path(S):
