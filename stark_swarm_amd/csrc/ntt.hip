@@ -391,7 +391,7 @@ __global__ __launch_bounds__(NT, 4) void odd_coset_tile(const uint64_t *in, uint
 #pragma unroll
         for (uint32_t u = 0; u < EPT; u++) {
             const uint32_t e = threadIdx.x + u * NT;
-            lds[padi<LOG_S>(e)] = mul(lds[padi<LOG_S>(e)], tv[u]);
+            lds[padi<LOG_S>(e)] = mul_lazy(lds[padi<LOG_S>(e)], tv[u]);  // the forward steps take any u64
         }
     }
     __syncthreads();

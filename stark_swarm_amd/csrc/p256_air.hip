@@ -884,7 +884,7 @@ __global__ __launch_bounds__(NT, 4) void expand_odd_p256_k(const uint64_t *words
 #pragma unroll
         for (uint32_t u = 0; u < EPT; u++) {
             const uint32_t e = threadIdx.x + u * NT;
-            lds[padi<LOG_H>(e)] = mul(lds[padi<LOG_H>(e)], tv[u]);
+            lds[padi<LOG_H>(e)] = mul_lazy(lds[padi<LOG_H>(e)], tv[u]);  // the forward steps take any u64
         }
     }
     __syncthreads();
